@@ -1,0 +1,209 @@
+// common.hpp — shared host/device helpers of libbolap (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "bolap.h"
+
+// ---------------------------------------------------------------------------
+// Context.
+// ---------------------------------------------------------------------------
+struct bolap_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    int lds_per_cu = 160 * 1024;
+    int64_t hbm_bytes = 0;
+    char arch[32] = {0};
+
+    // Growable device scratch (partials, hash tables, sort buffers).  Carved
+    // per call by WsArena; grown only at a call boundary after a stream sync.
+    void* ws = nullptr;
+    size_t ws_size = 0;
+
+    // Small device area of self-resetting counters for last-workgroup
+    // reductions: zeroed once at creation, every kernel that increments one
+    // returns it to zero before exiting.
+    unsigned* counters = nullptr;
+    static constexpr int kNumCounters = 64;
+
+    // Device staging for small results that are copied back to the host.
+    void* result_dev = nullptr;
+    static constexpr size_t kResultBytes = size_t(1) << 20;
+
+    // Pinned host staging for results copied back at the end of a pipeline.
+    void* pinned = nullptr;
+    size_t pinned_size = 0;
+};
+
+namespace bolap {
+
+// Thread-local error text returned by bolap_last_error().
+void set_error(const char* fmt, ...);
+
+#define BOLAP_HIP_TRY(expr)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            ::bolap::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                               __FILE__, __LINE__);                                \
+            return BOLAP_ERR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+#define BOLAP_ARG_CHECK(cond, ...)                                                 \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            ::bolap::set_error(__VA_ARGS__);                                       \
+            return BOLAP_ERR_ARG;                                                  \
+        }                                                                          \
+    } while (0)
+
+#define BOLAP_TRY(expr)                                                            \
+    do {                                                                           \
+        int _s = (expr);                                                           \
+        if (_s != BOLAP_OK) return _s;                                             \
+    } while (0)
+
+// Ensures ctx->ws holds at least `bytes`.
+int ws_reserve(bolap_ctx* ctx, size_t bytes);
+// Ensures ctx->pinned holds at least `bytes`.
+int pinned_reserve(bolap_ctx* ctx, size_t bytes);
+// Checks the last kernel launch.
+int launch_check(const char* what);
+
+// Bump allocator over the context workspace.  Plan first (sizes only), then
+// reserve, then carve with the same sequence of sizes.
+struct WsArena {
+    size_t off = 0;
+    char* base = nullptr;
+    static size_t align(size_t v) { return (v + 255) & ~size_t(255); }
+    size_t plan(size_t bytes) {
+        const size_t at = off;
+        off += align(bytes);
+        return at;
+    }
+    template <class T>
+    T* at(size_t offset) const {
+        return reinterpret_cast<T*>(base + offset);
+    }
+};
+
+inline bool aligned(const void* p, size_t a) {
+    return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
+}
+
+inline int dtype_size(int32_t dt) {
+    switch (dt) {
+        case BOLAP_U8: return 1;
+        case BOLAP_I16: return 2;
+        case BOLAP_I32: return 4;
+        case BOLAP_I64: return 8;
+        default: return 0;
+    }
+}
+
+}  // namespace bolap
+
+// ---------------------------------------------------------------------------
+// Device helpers.
+// ---------------------------------------------------------------------------
+#ifdef __HIP_DEVICE_COMPILE__
+#define BOLAP_DEV __device__ __forceinline__
+#else
+#define BOLAP_DEV __device__ inline
+#endif
+
+namespace bolap {
+
+constexpr int kWave = 64;
+
+template <typename T, int R>
+struct alignas(sizeof(T) * R) VecT {
+    T v[R];
+};
+
+// One vector load of R consecutive elements starting at element unit*R.
+template <typename T, int R>
+BOLAP_DEV VecT<T, R> load_unit(const T* __restrict__ p, int64_t unit) {
+    return reinterpret_cast<const VecT<T, R>*>(p)[unit];
+}
+
+template <typename T>
+BOLAP_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Two's-complement 128-bit accumulator as (lo, hi).
+struct Acc128 {
+    uint64_t lo;
+    int64_t hi;
+};
+
+BOLAP_DEV void add128(Acc128& a, int64_t v) {
+    const uint64_t nlo = a.lo + static_cast<uint64_t>(v);
+    a.hi += (v >> 63) + (nlo < a.lo ? 1 : 0);
+    a.lo = nlo;
+}
+
+BOLAP_DEV void add128(Acc128& a, const Acc128& b) {
+    const uint64_t nlo = a.lo + b.lo;
+    a.hi += b.hi + (nlo < a.lo ? 1 : 0);
+    a.lo = nlo;
+}
+
+BOLAP_DEV Acc128 wave_sum128(Acc128 a) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        Acc128 o;
+        o.lo = __shfl_xor(a.lo, off, kWave);
+        o.hi = __shfl_xor(a.hi, off, kWave);
+        add128(a, o);
+    }
+    return a;
+}
+
+// Coherent (agent-scope) loads/stores for data exchanged between workgroups
+// that may sit on different XCDs (separate L2s).
+template <typename T>
+BOLAP_DEV T ld_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+BOLAP_DEV void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in every thread of the workgroup that finished last.  All
+// threads must call it after publishing their results with st_agent.
+BOLAP_DEV bool last_block_done(unsigned* counter) {
+    __shared__ bool s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicAdd(counter, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    const bool last = s_last;
+    if (last) __threadfence();
+    return last;
+}
+
+// 64-bit mixing hash for open-addressing tables (SplitMix64 output function).
+BOLAP_DEV uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// 32-bit key hash for LDS/HBM tables: multiplicative (Fibonacci) hashing.
+BOLAP_DEV uint32_t hash32(uint32_t k) { return k * 0x9E3779B1u; }
+
+}  // namespace bolap

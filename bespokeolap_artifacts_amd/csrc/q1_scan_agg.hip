@@ -1,0 +1,363 @@
+// q1_scan_agg.hip — fused filter-scan + low-cardinality group-by aggregate.
+//
+// Replaces the scan loop of run_q1 (bespoke_tpch/query_q1.cpp:143-228).  Per
+// row that passes ship_lo <= ship <= ship_hi it adds, into the accumulators
+// of group (a * n_b + b):
+//   quantity, price, disc_price, disc_price * (100 + tax) (128-bit), discount,
+//   and a count
+// with disc_price = price * (100 - disc) (declared-width layout) or the
+// truncated discounted_price column of the reference layout
+// (builder_impl.cpp:523-532).
+//
+// Design (HBM-bound streaming scan, no MFMA):
+//  * Each lane owns "units" of R consecutive rows, R = 16 B / widest column,
+//    so the widest column is read with one 128-bit load per unit and every
+//    other column with one narrower load of the same R rows; each load
+//    instruction of a wave covers a contiguous span (fully coalesced).
+//  * U units per lane are loaded before any is consumed (U*R rows of every
+//    column in flight per lane).
+//  * The predicate and the group select stay in registers: G group slots
+//    (compile-time) of per-lane accumulators updated with selects, so there
+//    are no atomics and no LDS traffic in the loop.
+//  * Wave shuffles + LDS reduce the block; blocks publish partials and the
+//    last block to finish (self-resetting counter) sums them in a fixed
+//    order.  Integer sums are exact, so results are bit-identical to the
+//    sequential reference.
+#include <cstring>
+
+#include "common.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kQ1Threads = 256;
+constexpr int kQ1Waves = kQ1Threads / kWave;
+constexpr int kFields = 8;  // qty, price, disc_price, charge lo, charge hi, disc, count, pad
+
+struct Q1K {
+    const void* ship;
+    const void* qty;
+    const void* price;
+    const void* disc_price;
+    const void* disc;
+    const void* tax;
+    const uint8_t* fa;
+    const uint8_t* fb;
+    int64_t rows;
+    int64_t lo, hi;
+    int32_t n_b;
+    int32_t n_groups;
+};
+
+template <int G>
+struct Q1Lane {
+    int64_t qty[G], price[G], dp[G], disc[G];
+    Acc128 ch[G];
+    uint32_t cnt[G];
+};
+
+// Column types of the two layouts.
+struct WideLayout {
+    using S = int32_t; using Q = int64_t; using P = int64_t; using DP = int64_t;
+    using D = int64_t; using T = int64_t;
+    static constexpr bool kHasDP = false;
+    static constexpr bool kTwoFlags = true;
+    static constexpr int kNativeR = 2;  // 16 B of an int64 column
+};
+struct RefLayout {
+    using S = int16_t; using Q = int16_t; using P = int32_t; using DP = int32_t;
+    using D = uint8_t; using T = uint8_t;
+    static constexpr bool kHasDP = true;
+    static constexpr bool kTwoFlags = false;
+    static constexpr int kNativeR = 4;  // 16 B of an int32 column
+};
+
+template <class L, int G>
+BOLAP_DEV void q1_row(Q1Lane<G>& acc, const Q1K& a, bool valid, int64_t s, int64_t q,
+                      int64_t p, int64_t dpc, int64_t d, int64_t t, uint32_t fa, uint32_t fb) {
+    const uint32_t g = L::kTwoFlags ? fa * uint32_t(a.n_b) + fb : fa;
+    // Codes outside the dictionaries (a >= n_a, b >= n_b) belong to no group.
+    const bool in_dict = (L::kTwoFlags ? fb < uint32_t(a.n_b) : true) &&
+                         g < uint32_t(a.n_groups);
+    const bool pass = valid && s >= a.lo && s <= a.hi && in_dict;
+    const int64_t dp = L::kHasDP ? dpc : int64_t(uint64_t(p) * uint64_t(100 - d));
+    const int64_t ch = int64_t(uint64_t(dp) * uint64_t(100 + t));
+    const int64_t ch_sign = ch >> 63;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        const bool m = pass && g == uint32_t(gi);
+        acc.qty[gi] += m ? q : 0;
+        acc.price[gi] += m ? p : 0;
+        acc.dp[gi] += m ? dp : 0;
+        acc.disc[gi] += m ? d : 0;
+        acc.cnt[gi] += m ? 1u : 0u;
+        const uint64_t add_lo = m ? uint64_t(ch) : 0;
+        const uint64_t nlo = acc.ch[gi].lo + add_lo;
+        acc.ch[gi].hi += (m ? ch_sign : 0) + (nlo < acc.ch[gi].lo ? 1 : 0);
+        acc.ch[gi].lo = nlo;
+    }
+}
+
+template <class L, int R, int U, int G>
+__global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials,
+                                                                  unsigned* counter,
+                                                                  bolap_q1_group* out) {
+    using S = typename L::S; using Q = typename L::Q; using P = typename L::P;
+    using DP = typename L::DP; using D = typename L::D; using T = typename L::T;
+    const S* __restrict__ ship = static_cast<const S*>(a.ship);
+    const Q* __restrict__ qty = static_cast<const Q*>(a.qty);
+    const P* __restrict__ price = static_cast<const P*>(a.price);
+    const DP* __restrict__ dpc = static_cast<const DP*>(a.disc_price);
+    const D* __restrict__ disc = static_cast<const D*>(a.disc);
+    const T* __restrict__ tax = static_cast<const T*>(a.tax);
+    const uint8_t* __restrict__ fa = a.fa;
+    const uint8_t* __restrict__ fb = a.fb;
+
+    Q1Lane<G> acc;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        acc.qty[g] = acc.price[g] = acc.dp[g] = acc.disc[g] = 0;
+        acc.ch[g].lo = 0;
+        acc.ch[g].hi = 0;
+        acc.cnt[g] = 0;
+    }
+
+    const int64_t n_units = a.rows / R;
+    const int64_t stride = int64_t(gridDim.x) * kQ1Threads * U;
+    for (int64_t base = int64_t(blockIdx.x) * kQ1Threads * U + threadIdx.x; base < n_units;
+         base += stride) {
+        VecT<S, R> vs[U];
+        VecT<Q, R> vq[U];
+        VecT<P, R> vp[U];
+        VecT<DP, R> vdp[U];
+        VecT<D, R> vd[U];
+        VecT<T, R> vt[U];
+        VecT<uint8_t, R> va[U], vb[U];
+        bool valid[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            int64_t u = base + int64_t(k) * kQ1Threads;
+            valid[k] = u < n_units;
+            u = valid[k] ? u : n_units - 1;  // in-bounds; contribution masked
+            vs[k] = load_unit<S, R>(ship, u);
+            vq[k] = load_unit<Q, R>(qty, u);
+            vp[k] = load_unit<P, R>(price, u);
+            if constexpr (L::kHasDP) vdp[k] = load_unit<DP, R>(dpc, u);
+            vd[k] = load_unit<D, R>(disc, u);
+            vt[k] = load_unit<T, R>(tax, u);
+            va[k] = load_unit<uint8_t, R>(fa, u);
+            if constexpr (L::kTwoFlags) vb[k] = load_unit<uint8_t, R>(fb, u);
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                q1_row<L, G>(acc, a, valid[k], vs[k].v[r], vq[k].v[r], vp[k].v[r],
+                             L::kHasDP ? int64_t(vdp[k].v[r]) : 0, vd[k].v[r], vt[k].v[r],
+                             va[k].v[r], L::kTwoFlags ? uint32_t(vb[k].v[r]) : 0u);
+            }
+        }
+    }
+    // Tail rows (rows % R) go to block 0.
+    if (blockIdx.x == 0) {
+        const int64_t row = n_units * R + threadIdx.x;
+        if (threadIdx.x < R && row < a.rows) {
+            q1_row<L, G>(acc, a, true, ship[row], qty[row], price[row],
+                         L::kHasDP ? int64_t(dpc[row]) : 0, disc[row], tax[row], fa[row],
+                         L::kTwoFlags ? uint32_t(fb[row]) : 0u);
+        }
+    }
+
+    // Block reduction: wave shuffles, then LDS across waves.
+    __shared__ int64_t s_part[kQ1Waves][G][kFields];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int64_t q = wave_sum(acc.qty[g]);
+        const int64_t p = wave_sum(acc.price[g]);
+        const int64_t dp = wave_sum(acc.dp[g]);
+        const int64_t d = wave_sum(acc.disc[g]);
+        const int64_t c = int64_t(wave_sum(uint64_t(acc.cnt[g])));
+        const Acc128 ch = wave_sum128(acc.ch[g]);
+        if (lane == 0) {
+            s_part[wave][g][0] = q;
+            s_part[wave][g][1] = p;
+            s_part[wave][g][2] = dp;
+            s_part[wave][g][3] = int64_t(ch.lo);
+            s_part[wave][g][4] = ch.hi;
+            s_part[wave][g][5] = d;
+            s_part[wave][g][6] = c;
+            s_part[wave][g][7] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+        const int g = threadIdx.x;
+        int64_t f[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+        Acc128 ch{0, 0};
+        for (int w = 0; w < kQ1Waves; ++w) {
+            for (int i = 0; i < kFields; ++i)
+                if (i != 3 && i != 4) f[i] += s_part[w][g][i];
+            add128(ch, Acc128{uint64_t(s_part[w][g][3]), s_part[w][g][4]});
+        }
+        f[3] = int64_t(ch.lo);
+        f[4] = ch.hi;
+        int64_t* dst = partials + (int64_t(blockIdx.x) * G + g) * kFields;
+        for (int i = 0; i < kFields; ++i) st_agent(dst + i, f[i]);
+    }
+
+    if (!last_block_done(counter)) return;
+
+    // Last block: sum all partials group by group (exact integer sums).
+    for (int g = 0; g < a.n_groups; ++g) {
+        int64_t v[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+        Acc128 c{0, 0};
+        for (int b = threadIdx.x; b < int(gridDim.x); b += kQ1Threads) {
+            const int64_t* src = partials + (int64_t(b) * G + g) * kFields;
+#pragma unroll
+            for (int i = 0; i < kFields; ++i)
+                if (i != 3 && i != 4) v[i] += ld_agent(src + i);
+            add128(c, Acc128{uint64_t(ld_agent(src + 3)), ld_agent(src + 4)});
+        }
+#pragma unroll
+        for (int i = 0; i < kFields; ++i)
+            if (i != 3 && i != 4) v[i] = wave_sum(v[i]);
+        c = wave_sum128(c);
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < kFields; ++i)
+                if (i != 3 && i != 4) s_part[wave][0][i] = v[i];
+            s_part[wave][0][3] = int64_t(c.lo);
+            s_part[wave][0][4] = c.hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+            Acc128 tc{0, 0};
+            for (int w = 0; w < kQ1Waves; ++w) {
+                for (int i = 0; i < kFields; ++i)
+                    if (i != 3 && i != 4) t[i] += s_part[w][0][i];
+                add128(tc, Acc128{uint64_t(s_part[w][0][3]), s_part[w][0][4]});
+            }
+            bolap_q1_group r;
+            r.sum_qty = t[0];
+            r.sum_base_price = t[1];
+            r.sum_disc_price = t[2];
+            r.sum_charge_lo = tc.lo;
+            r.sum_charge_hi = tc.hi;
+            r.sum_disc = t[5];
+            r.count = t[6];
+            r.reserved = 0;
+            out[g] = r;
+        }
+    }
+    if (threadIdx.x == 0) st_agent(counter, 0u);
+}
+
+// Units in flight per lane.
+constexpr int kQ1U = 4;
+
+template <class L, int R, int G>
+int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    const int64_t n_units = k.rows / R;
+    const int64_t tile = int64_t(kQ1Threads) * kQ1U;
+    int64_t blocks = (n_units + tile - 1) / tile;
+    const int64_t max_blocks = int64_t(ctx->num_cus) * 4;
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks < 1) blocks = 1;
+    WsArena plan;
+    const size_t o_part = plan.plan(size_t(blocks) * G * kFields * sizeof(int64_t));
+    BOLAP_TRY(ws_reserve(ctx, plan.off));
+    plan.base = static_cast<char*>(ctx->ws);
+    hipLaunchKernelGGL((q1_scan_agg_kernel<L, R, kQ1U, G>), dim3(unsigned(blocks)),
+                       dim3(kQ1Threads), 0, ctx->stream, k, plan.at<int64_t>(o_part),
+                       ctx->counters + 0, out_dev);
+    return launch_check("q1_scan_agg_kernel");
+}
+
+template <class L, int R>
+int launch_q1_r(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    const int g = k.n_groups;
+    if (g <= 2) return launch_q1_g<L, R, 2>(ctx, k, out_dev);
+    if (g <= 4) return launch_q1_g<L, R, 4>(ctx, k, out_dev);
+    if (g <= 6) return launch_q1_g<L, R, 6>(ctx, k, out_dev);
+    if (g <= 8) return launch_q1_g<L, R, 8>(ctx, k, out_dev);
+    return launch_q1_g<L, R, 16>(ctx, k, out_dev);
+}
+
+int q1_launch(bolap_ctx* ctx, const bolap_q1_args* a, bolap_q1_group* out_dev) {
+    BOLAP_ARG_CHECK(ctx && a && out_dev, "bolap_q1_scan_agg: null argument");
+    BOLAP_ARG_CHECK(a->rows >= 0, "bolap_q1_scan_agg: rows < 0");
+    BOLAP_ARG_CHECK(a->n_a >= 1 && a->n_b >= 1 && a->n_a * a->n_b <= BOLAP_Q1_MAX_GROUPS,
+                    "bolap_q1_scan_agg: n_a*n_b must be in 1..%d", BOLAP_Q1_MAX_GROUPS);
+    const bool wide = a->layout == BOLAP_Q1_LAYOUT_WIDE;
+    BOLAP_ARG_CHECK(wide || a->layout == BOLAP_Q1_LAYOUT_REF,
+                    "bolap_q1_scan_agg: unknown layout %d", a->layout);
+    Q1K k;
+    k.ship = a->ship;
+    k.qty = a->qty;
+    k.price = a->price;
+    k.disc_price = a->disc_price;
+    k.disc = a->disc;
+    k.tax = a->tax;
+    k.fa = a->flag_a;
+    k.fb = a->flag_b;
+    k.rows = a->rows;
+    k.lo = a->ship_lo;
+    k.hi = a->ship_hi;
+    k.n_b = a->n_b;
+    k.n_groups = a->n_a * a->n_b;
+    if (a->rows > 0) {
+        BOLAP_ARG_CHECK(a->ship && a->qty && a->price && a->disc && a->tax && a->flag_a,
+                        "bolap_q1_scan_agg: null column");
+        BOLAP_ARG_CHECK(!wide || a->flag_b, "bolap_q1_scan_agg: WIDE layout needs flag_b");
+        BOLAP_ARG_CHECK(wide || a->disc_price,
+                        "bolap_q1_scan_agg: REF layout needs disc_price");
+    }
+    if (wide) {
+        // 128-bit path needs every column aligned to its R=2 vector width.
+        const bool vec = aligned(k.ship, 8) && aligned(k.qty, 16) && aligned(k.price, 16) &&
+                         aligned(k.disc, 16) && aligned(k.tax, 16) && aligned(k.fa, 2) &&
+                         aligned(k.fb, 2);
+        return vec ? launch_q1_r<WideLayout, 2>(ctx, k, out_dev)
+                   : launch_q1_r<WideLayout, 1>(ctx, k, out_dev);
+    }
+    if (!k.fb) k.fb = k.fa;  // unused by the reference layout
+    const bool vec = aligned(k.ship, 8) && aligned(k.qty, 8) && aligned(k.price, 16) &&
+                     aligned(k.disc_price, 16) && aligned(k.disc, 4) && aligned(k.tax, 4) &&
+                     aligned(k.fa, 4);
+    return vec ? launch_q1_r<RefLayout, 4>(ctx, k, out_dev)
+               : launch_q1_r<RefLayout, 1>(ctx, k, out_dev);
+}
+
+}  // namespace
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" {
+
+int bolap_q1_scan_agg_async(bolap_ctx* ctx, const bolap_q1_args* args,
+                            bolap_q1_group* out_dev) {
+    return q1_launch(ctx, args, out_dev);
+}
+
+int bolap_q1_scan_agg(bolap_ctx* ctx, const bolap_q1_args* args, bolap_q1_group* out) {
+    BOLAP_ARG_CHECK(ctx && args && out, "bolap_q1_scan_agg: null argument");
+    BOLAP_ARG_CHECK(args->n_a >= 1 && args->n_b >= 1 &&
+                        args->n_a * args->n_b <= BOLAP_Q1_MAX_GROUPS,
+                    "bolap_q1_scan_agg: n_a*n_b must be in 1..%d", BOLAP_Q1_MAX_GROUPS);
+    const size_t bytes = sizeof(bolap_q1_group) * size_t(args->n_a * args->n_b);
+    BOLAP_TRY(pinned_reserve(ctx, bytes));
+    auto* dev_out = static_cast<bolap_q1_group*>(ctx->result_dev);
+    BOLAP_TRY(q1_launch(ctx, args, dev_out));
+    BOLAP_HIP_TRY(
+        hipMemcpyAsync(ctx->pinned, dev_out, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->pinned, bytes);
+    return BOLAP_OK;
+}
+
+}  // extern "C"

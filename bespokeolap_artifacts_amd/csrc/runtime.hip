@@ -1,0 +1,133 @@
+// runtime.hip — context, workspace and error plumbing of libbolap.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace bolap {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int ws_reserve(bolap_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->ws_size) return BOLAP_OK;
+    size_t want = ctx->ws_size ? ctx->ws_size : (size_t(64) << 20);
+    while (want < bytes) want *= 2;
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->ws) BOLAP_HIP_TRY(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_size = 0;
+    BOLAP_HIP_TRY(hipMalloc(&ctx->ws, want));
+    ctx->ws_size = want;
+    return BOLAP_OK;
+}
+
+int pinned_reserve(bolap_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->pinned_size) return BOLAP_OK;
+    size_t want = ctx->pinned_size ? ctx->pinned_size : (size_t(1) << 20);
+    while (want < bytes) want *= 2;
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned) BOLAP_HIP_TRY(hipHostFree(ctx->pinned));
+    ctx->pinned = nullptr;
+    ctx->pinned_size = 0;
+    BOLAP_HIP_TRY(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+    ctx->pinned_size = want;
+    return BOLAP_OK;
+}
+
+int launch_check(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return BOLAP_ERR_HIP;
+    }
+    return BOLAP_OK;
+}
+
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" {
+
+const char* bolap_last_error(void) { return g_err; }
+
+int bolap_abi_version(void) { return BOLAP_ABI_VERSION; }
+
+int bolap_ctx_create(int device, void* stream, bolap_ctx** out) {
+    BOLAP_ARG_CHECK(out != nullptr, "bolap_ctx_create: out is null");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("bolap_ctx_create: no HIP device visible");
+        return BOLAP_ERR_NO_DEVICE;
+    }
+    BOLAP_ARG_CHECK(device >= 0 && device < n, "bolap_ctx_create: device %d of %d", device, n);
+    BOLAP_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BOLAP_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("bolap_ctx_create: device %d is %s; this library is built for gfx950",
+                  device, prop.gcnArchName);
+        return BOLAP_ERR_UNSUPPORTED;
+    }
+    auto* ctx = new bolap_ctx();
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->lds_per_cu = static_cast<int>(prop.maxSharedMemoryPerMultiProcessor);
+    ctx->hbm_bytes = static_cast<int64_t>(prop.totalGlobalMem);
+    std::strncpy(ctx->arch, prop.gcnArchName, sizeof(ctx->arch) - 1);
+    hipError_t e = hipMalloc(&ctx->counters, sizeof(unsigned) * bolap_ctx::kNumCounters);
+    if (e == hipSuccess)
+        e = hipMemset(ctx->counters, 0, sizeof(unsigned) * bolap_ctx::kNumCounters);
+    if (e == hipSuccess) e = hipMalloc(&ctx->result_dev, bolap_ctx::kResultBytes);
+    if (e != hipSuccess) {
+        set_error("bolap_ctx_create: %s", hipGetErrorString(e));
+        if (ctx->counters) (void)hipFree(ctx->counters);
+        if (ctx->result_dev) (void)hipFree(ctx->result_dev);
+        delete ctx;
+        return BOLAP_ERR_HIP;
+    }
+    *out = ctx;
+    return BOLAP_OK;
+}
+
+int bolap_ctx_destroy(bolap_ctx* ctx) {
+    if (!ctx) return BOLAP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->result_dev) (void)hipFree(ctx->result_dev);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    delete ctx;
+    return BOLAP_OK;
+}
+
+int bolap_ctx_sync(bolap_ctx* ctx) {
+    BOLAP_ARG_CHECK(ctx != nullptr, "bolap_ctx_sync: null ctx");
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BOLAP_OK;
+}
+
+int bolap_ctx_device_info(bolap_ctx* ctx, bolap_device_info* out) {
+    BOLAP_ARG_CHECK(ctx && out, "bolap_ctx_device_info: null argument");
+    out->device = ctx->device;
+    out->compute_units = ctx->num_cus;
+    out->lds_bytes_per_cu = ctx->lds_per_cu;
+    out->wavefront = 64;
+    out->hbm_bytes = ctx->hbm_bytes;
+    std::memset(out->arch, 0, sizeof(out->arch));
+    std::strncpy(out->arch, ctx->arch, sizeof(out->arch) - 1);
+    return BOLAP_OK;
+}
+
+}  // extern "C"

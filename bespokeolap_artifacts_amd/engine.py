@@ -1,0 +1,183 @@
+"""Python handle on libbolap for tests and benchmarks.
+
+Device columns are torch tensors on a ROCm device (torch is used only to
+own HBM and to name the stream); every computation runs in libbolap's HIP
+kernels through the C ABI.  Results come back as plain Python values.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+from . import native as N
+
+_DTYPE_CODES = {}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dtype_code(t) -> int:
+    torch = _torch()
+    if not _DTYPE_CODES:
+        _DTYPE_CODES.update({torch.uint8: N.U8, torch.int16: N.I16, torch.int32: N.I32,
+                             torch.int64: N.I64})
+    return _DTYPE_CODES[t.dtype]
+
+
+def ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("libbolap needs device-resident columns")
+    if not t.is_contiguous():
+        raise ValueError("columns must be contiguous")
+    return t.data_ptr()
+
+
+def int128(lo: int, hi: int) -> int:
+    """Two's-complement 128-bit value from (lo u64, hi i64)."""
+    return (hi << 64) + (lo & ((1 << 64) - 1))
+
+
+@dataclass
+class Q1Group:
+    sum_qty: int
+    sum_base_price: int
+    sum_disc_price: int
+    sum_charge: int
+    sum_disc: int
+    count: int
+
+
+class Engine:
+    """One libbolap context on one device, issuing on torch's current stream."""
+
+    def __init__(self, device: int = 0, stream=None):
+        torch = _torch()
+        self.lib = N.load()
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+        self.stream = stream
+        h = C.c_void_p()
+        N.check(self.lib.bolap_ctx_create(device, C.c_void_p(stream.cuda_stream), C.byref(h)))
+        self.ctx = h
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.bolap_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        N.check(self.lib.bolap_ctx_sync(self.ctx))
+
+    def device_info(self) -> dict:
+        info = N.DeviceInfo()
+        N.check(self.lib.bolap_ctx_device_info(self.ctx, C.byref(info)))
+        return {"device": info.device, "compute_units": info.compute_units,
+                "lds_bytes_per_cu": info.lds_bytes_per_cu, "wavefront": info.wavefront,
+                "hbm_bytes": info.hbm_bytes, "arch": info.arch.decode()}
+
+    # ------------------------------------------------------------------ Q1
+    @staticmethod
+    def q1_args(cols: dict, ship_lo: int, ship_hi: int, n_a: int, n_b: int,
+                layout: int = N.Q1_LAYOUT_WIDE) -> N.Q1Args:
+        a = N.Q1Args()
+        a.layout = layout
+        a.rows = int(cols["ship"].numel())
+        a.ship = ptr(cols["ship"])
+        a.ship_lo, a.ship_hi = int(ship_lo), int(ship_hi)
+        a.qty = ptr(cols["qty"])
+        a.price = ptr(cols["price"])
+        a.disc_price = ptr(cols.get("disc_price"))
+        a.disc = ptr(cols["disc"])
+        a.tax = ptr(cols["tax"])
+        a.flag_a = ptr(cols["flag_a"])
+        a.flag_b = ptr(cols.get("flag_b"))
+        a.n_a, a.n_b = int(n_a), int(n_b)
+        return a
+
+    def q1_scan_agg(self, cols: dict, ship_lo: int, ship_hi: int, n_a: int, n_b: int,
+                    layout: int = N.Q1_LAYOUT_WIDE) -> list[Q1Group]:
+        a = self.q1_args(cols, ship_lo, ship_hi, n_a, n_b, layout)
+        out = (N.Q1Group * (n_a * n_b))()
+        N.check(self.lib.bolap_q1_scan_agg(self.ctx, C.byref(a), out))
+        return [Q1Group(g.sum_qty, g.sum_base_price, g.sum_disc_price,
+                        int128(g.sum_charge_lo, g.sum_charge_hi), g.sum_disc, g.count)
+                for g in out]
+
+    def q1_scan_agg_async(self, a: N.Q1Args, out_dev) -> None:
+        N.check(self.lib.bolap_q1_scan_agg_async(self.ctx, C.byref(a),
+                                                 C.c_void_p(out_dev.data_ptr())))
+
+    # ------------------------------------------------------- generators
+    def gen_lineitem(self, seed: int, rows: int) -> dict:
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        cols = {
+            "ship": torch.empty(rows, dtype=torch.int32, device=dev),
+            "qty": torch.empty(rows, dtype=torch.int64, device=dev),
+            "price": torch.empty(rows, dtype=torch.int64, device=dev),
+            "disc": torch.empty(rows, dtype=torch.int64, device=dev),
+            "tax": torch.empty(rows, dtype=torch.int64, device=dev),
+            "flag_a": torch.empty(rows, dtype=torch.uint8, device=dev),
+            "flag_b": torch.empty(rows, dtype=torch.uint8, device=dev),
+        }
+        g = N.GenLineitemCols(*(ptr(cols[k]) for k in
+                                ("ship", "qty", "price", "disc", "tax", "flag_a", "flag_b")))
+        N.check(self.lib.bolap_gen_lineitem(self.ctx, seed, rows, C.byref(g)))
+        return cols
+
+    def gen_dim(self, seed: int, rows: int) -> dict:
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        cols = {"key": torch.empty(rows, dtype=torch.int32, device=dev),
+                "cat": torch.empty(rows, dtype=torch.uint8, device=dev),
+                "cont": torch.empty(rows, dtype=torch.uint8, device=dev)}
+        N.check(self.lib.bolap_gen_dim(self.ctx, seed, rows, ptr(cols["key"]),
+                                       ptr(cols["cat"]), ptr(cols["cont"])))
+        return cols
+
+    def gen_fact(self, seed: int, rows: int, dim_rows: int) -> dict:
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        cols = {"fk": torch.empty(rows, dtype=torch.int32, device=dev),
+                "price": torch.empty(rows, dtype=torch.int64, device=dev),
+                "disc": torch.empty(rows, dtype=torch.int64, device=dev)}
+        N.check(self.lib.bolap_gen_fact(self.ctx, seed, rows, dim_rows, ptr(cols["fk"]),
+                                        ptr(cols["price"]), ptr(cols["disc"])))
+        return cols
+
+    def gen_chain(self, seed: int, cust_rows: int, order_rows: int) -> dict:
+        """Customers, orders and their clustered fact rows (1..7 per order)."""
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        t = {"cust_key": torch.empty(cust_rows, dtype=torch.int32, device=dev),
+             "cust_segment": torch.empty(cust_rows, dtype=torch.uint8, device=dev),
+             "order_key": torch.empty(order_rows, dtype=torch.int32, device=dev),
+             "order_day": torch.empty(order_rows, dtype=torch.int32, device=dev),
+             "order_cust": torch.empty(order_rows, dtype=torch.int32, device=dev),
+             "fact_offsets": torch.empty(order_rows + 1, dtype=torch.int64, device=dev)}
+        N.check(self.lib.bolap_gen_customers(self.ctx, seed, cust_rows, ptr(t["cust_key"]),
+                                             ptr(t["cust_segment"])))
+        N.check(self.lib.bolap_gen_orders(self.ctx, seed, order_rows, cust_rows,
+                                          ptr(t["order_key"]), ptr(t["order_day"]),
+                                          ptr(t["order_cust"]), ptr(t["fact_offsets"])))
+        fact_rows = int(t["fact_offsets"][-1].item())
+        t["fact_order"] = torch.empty(fact_rows, dtype=torch.int32, device=dev)
+        t["fact_price"] = torch.empty(fact_rows, dtype=torch.int64, device=dev)
+        t["fact_disc"] = torch.empty(fact_rows, dtype=torch.int64, device=dev)
+        N.check(self.lib.bolap_gen_order_facts(self.ctx, seed, order_rows,
+                                               ptr(t["fact_offsets"]), fact_rows,
+                                               ptr(t["fact_order"]), ptr(t["fact_price"]),
+                                               ptr(t["fact_disc"])))
+        return t

@@ -1,0 +1,153 @@
+"""ctypes view of the libbolap C ABI (include/bolap.h).
+
+This module only mirrors the header: structures, prototypes and a loader
+that fails loudly when the HIP library is missing.  There is no CPU
+fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libbolap.so"
+
+BOLAP_OK = 0
+BOLAP_ERR_ARG = 1
+BOLAP_ERR_HIP = 2
+BOLAP_ERR_UNSUPPORTED = 3
+BOLAP_ERR_CAPACITY = 4
+BOLAP_ERR_NO_DEVICE = 5
+
+U8, I16, I32, I64 = 0, 1, 2, 3
+PLACE_AUTO, PLACE_LDS, PLACE_HBM = 0, 1, 2
+OP_RANGE, OP_IN_SET = 0, 1
+Q1_LAYOUT_WIDE, Q1_LAYOUT_REF = 0, 1
+MAX_PREDS = 4
+Q1_MAX_GROUPS = 16
+
+vp = C.c_void_p
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("device", C.c_int32), ("compute_units", C.c_int32),
+                ("lds_bytes_per_cu", C.c_int32), ("wavefront", C.c_int32),
+                ("hbm_bytes", C.c_int64), ("arch", C.c_char * 32)]
+
+
+class Pred(C.Structure):
+    _fields_ = [("col", vp), ("dtype", C.c_int32), ("op", C.c_int32),
+                ("lo", C.c_int64), ("hi", C.c_int64),
+                ("inline_set", C.c_uint64 * 4), ("set", vp), ("set_bits", C.c_int64)]
+
+
+class Q1Args(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("rows", C.c_int64), ("ship", vp),
+                ("ship_lo", C.c_int64), ("ship_hi", C.c_int64),
+                ("qty", vp), ("price", vp), ("disc_price", vp), ("disc", vp), ("tax", vp),
+                ("flag_a", vp), ("flag_b", vp), ("n_a", C.c_int32), ("n_b", C.c_int32)]
+
+
+class Q1Group(C.Structure):
+    _fields_ = [("sum_qty", C.c_int64), ("sum_base_price", C.c_int64),
+                ("sum_disc_price", C.c_int64), ("sum_charge_lo", C.c_uint64),
+                ("sum_charge_hi", C.c_int64), ("sum_disc", C.c_int64),
+                ("count", C.c_int64), ("reserved", C.c_int64)]
+
+
+class JoinSumArgs(C.Structure):
+    _fields_ = [("dim_rows", C.c_int64), ("dim_key", vp), ("n_dim_preds", C.c_int32),
+                ("dim_preds", Pred * MAX_PREDS), ("fact_rows", C.c_int64),
+                ("fact_fk", vp), ("fact_price", vp), ("price_dtype", C.c_int32),
+                ("fact_disc", vp), ("disc_dtype", C.c_int32),
+                ("capacity_hint", C.c_int64), ("placement", C.c_int32)]
+
+
+class JoinSumResult(C.Structure):
+    _fields_ = [("revenue", C.c_int64), ("build_rows", C.c_int64), ("matches", C.c_int64),
+                ("capacity", C.c_int64), ("placement", C.c_int32), ("fallbacks", C.c_int32)]
+
+
+class ChainArgs(C.Structure):
+    _fields_ = [("cust_rows", C.c_int64), ("cust_key", vp), ("cust_segment", vp),
+                ("segment", C.c_int32), ("order_rows", C.c_int64), ("order_key", vp),
+                ("order_day", vp), ("order_cust", vp), ("day_lt", C.c_int32),
+                ("fact_rows", C.c_int64), ("fact_order", vp), ("fact_price", vp),
+                ("price_dtype", C.c_int32), ("fact_disc", vp), ("disc_dtype", C.c_int32),
+                ("fact_disc_price", vp), ("capacity_hint", C.c_int64)]
+
+
+class GroupRow(C.Structure):
+    _fields_ = [("key", C.c_int32), ("day", C.c_int32), ("revenue", C.c_int64)]
+
+
+class ChainStats(C.Structure):
+    _fields_ = [("cust_pass", C.c_int64), ("order_pass", C.c_int64),
+                ("fact_matches", C.c_int64), ("groups", C.c_int64), ("capacity", C.c_int64),
+                ("fallbacks", C.c_int32), ("sort_passes", C.c_int32)]
+
+
+class GenLineitemCols(C.Structure):
+    _fields_ = [("ship", vp), ("qty", vp), ("price", vp), ("disc", vp), ("tax", vp),
+                ("flag_a", vp), ("flag_b", vp)]
+
+
+P = C.POINTER
+i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
+
+# name -> (restype, argtypes); every entry must be exported by libbolap.so.
+PROTOTYPES = {
+    "bolap_abi_version": (C.c_int, []),
+    "bolap_last_error": (C.c_char_p, []),
+    "bolap_ctx_create": (C.c_int, [C.c_int, vp, P(vp)]),
+    "bolap_ctx_destroy": (C.c_int, [vp]),
+    "bolap_ctx_sync": (C.c_int, [vp]),
+    "bolap_ctx_device_info": (C.c_int, [vp, P(DeviceInfo)]),
+    "bolap_q1_scan_agg": (C.c_int, [vp, P(Q1Args), P(Q1Group)]),
+    "bolap_q1_scan_agg_async": (C.c_int, [vp, P(Q1Args), vp]),
+    "bolap_join_sum": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),
+    "bolap_chain_topk": (C.c_int, [vp, P(ChainArgs), i32, P(GroupRow), P(i32),
+                                   P(ChainStats)]),
+    "bolap_chain_groups_sorted": (C.c_int, [vp, P(ChainArgs), P(GroupRow), i64, P(i64),
+                                            P(ChainStats)]),
+    "bolap_radix_sort_u64": (C.c_int, [vp, vp, vp, i64, i32]),
+    "bolap_topk_u64": (C.c_int, [vp, vp, vp, i64, i32, vp, vp]),
+    "bolap_gen_lineitem": (C.c_int, [vp, u64, i64, P(GenLineitemCols)]),
+    "bolap_gen_dim": (C.c_int, [vp, u64, i64, vp, vp, vp]),
+    "bolap_gen_fact": (C.c_int, [vp, u64, i64, i64, vp, vp, vp]),
+    "bolap_gen_customers": (C.c_int, [vp, u64, i64, vp, vp]),
+    "bolap_gen_orders": (C.c_int, [vp, u64, i64, i64, vp, vp, vp, vp]),
+    "bolap_gen_order_facts": (C.c_int, [vp, u64, i64, vp, i64, vp, vp, vp]),
+}
+
+_lib = None
+
+
+class BolapError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"bolap status {status}: {message}")
+        self.status = status
+
+
+def load() -> C.CDLL:
+    """Loads libbolap.so, raising when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(python -m bespokeolap_artifacts_amd.build); there is no CPU fallback")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != BOLAP_OK:
+        msg = load().bolap_last_error()
+        raise BolapError(status, msg.decode() if msg else "")

@@ -1,0 +1,291 @@
+/*
+ * bolap.h — C ABI of the MI355X (gfx950) execution layer for the bespoke
+ * per-query OLAP pipelines of DataManagementLab/BespokeOLAP_Artifacts.
+ *
+ * The reference runs each query as a single-threaded C++ loop nest behind
+ *   std::vector<QNResultRow> run_qN(const Database&, const QNArgs&)
+ * (e.g. bespoke_tpch/query_q1.hpp:22, query_q3.hpp:17, query_q19.hpp:14).
+ * This header is the seam a re-implemented run_qN body calls instead of
+ * its scalar loops: plain pointers and sizes, no C++ or torch types.
+ *
+ * Conventions
+ *  - Every column pointer is a DEVICE pointer (HBM-resident column, as the
+ *    device mirror of the reference Database holds it).  Columns read with
+ *    128-bit loads must be 16-byte aligned; unaligned columns take a
+ *    slower scalar path, results are identical.
+ *  - Functions return 0 (BOLAP_OK) or a bolap_status; bolap_last_error()
+ *    gives a thread-local message.  The C++ seam rethrows it as
+ *    std::runtime_error, which is how the reference reports errors.
+ *  - Capacity overflow of an estimate-sized hash table is NOT an error: the
+ *    pipeline re-runs with exact sizing and reports it in `fallbacks`.
+ *  - All work is issued on the context's stream.  Functions that return
+ *    host results synchronise that stream before returning.
+ */
+#ifndef BOLAP_H
+#define BOLAP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BOLAP_ABI_VERSION 1
+
+typedef enum {
+    BOLAP_OK = 0,
+    BOLAP_ERR_ARG = 1,          /* bad argument (null pointer, size, dtype) */
+    BOLAP_ERR_HIP = 2,          /* HIP runtime error */
+    BOLAP_ERR_UNSUPPORTED = 3,  /* valid request this build does not cover */
+    BOLAP_ERR_CAPACITY = 4,     /* a fixed maximum was exceeded */
+    BOLAP_ERR_NO_DEVICE = 5
+} bolap_status;
+
+typedef enum {
+    BOLAP_U8 = 0,
+    BOLAP_I16 = 1,
+    BOLAP_I32 = 2,
+    BOLAP_I64 = 3
+} bolap_dtype;
+
+typedef enum {
+    BOLAP_PLACE_AUTO = 0,
+    BOLAP_PLACE_LDS = 1,   /* hash table staged per workgroup in LDS */
+    BOLAP_PLACE_HBM = 2    /* open-addressing table in HBM, probed through L2 */
+} bolap_placement;
+
+/* ------------------------------------------------------------------------ */
+/* Context: one per (host thread, device, stream).                            */
+/* ------------------------------------------------------------------------ */
+typedef struct bolap_ctx bolap_ctx;
+
+typedef struct {
+    int32_t device;
+    int32_t compute_units;
+    int32_t lds_bytes_per_cu;
+    int32_t wavefront;
+    int64_t hbm_bytes;
+    char arch[32];
+} bolap_device_info;
+
+/* `stream` is a hipStream_t (NULL = the null stream). */
+int bolap_ctx_create(int device, void* stream, bolap_ctx** out);
+int bolap_ctx_destroy(bolap_ctx* ctx);
+int bolap_ctx_sync(bolap_ctx* ctx);
+int bolap_ctx_device_info(bolap_ctx* ctx, bolap_device_info* out);
+const char* bolap_last_error(void);
+int bolap_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Predicates (dimension-side filters).                                       */
+/* A row passes when every predicate holds (conjunction).                     */
+/* ------------------------------------------------------------------------ */
+typedef enum {
+    BOLAP_OP_RANGE = 0,   /* lo <= v <= hi  (equality: lo == hi)            */
+    BOLAP_OP_IN_SET = 1   /* bit v of the set is 1; v outside the set fails */
+} bolap_op;
+
+#define BOLAP_MAX_PREDS 4
+
+typedef struct {
+    const void* col;          /* device column */
+    int32_t dtype;            /* bolap_dtype */
+    int32_t op;               /* bolap_op */
+    int64_t lo, hi;           /* BOLAP_OP_RANGE bounds, inclusive */
+    uint64_t inline_set[4];   /* IN_SET over values 0..255 when `set` is NULL */
+    const uint32_t* set;      /* device bitmap for IN_SET over 0..set_bits-1 */
+    int64_t set_bits;
+} bolap_pred;
+
+/* ------------------------------------------------------------------------ */
+/* Q1 shape: fused filter-scan + low-cardinality group-by aggregate.          */
+/* Replaces the scan loop of run_q1 (bespoke_tpch/query_q1.cpp:104-228) and  */
+/* its Q1Agg accumulators (query_q1.cpp:27-34).                               */
+/* ------------------------------------------------------------------------ */
+typedef enum {
+    /* declared-width layout: ship i32, qty/price/disc/tax i64, flag_a and
+       flag_b u8 dictionary codes; disc_price = price * (100 - disc) exact. */
+    BOLAP_Q1_LAYOUT_WIDE = 0,
+    /* reference Database layout (bespoke_tpch/builder_impl.hpp:39-66):
+       shipdate i16, quantity i16 (x100), extendedprice i32,
+       discounted_price i32 (truncated at build, builder_impl.cpp:523-532),
+       discount u8, tax u8, returnflag_linestatus u8 group code.          */
+    BOLAP_Q1_LAYOUT_REF = 1
+} bolap_q1_layout;
+
+typedef struct {
+    int32_t layout;           /* bolap_q1_layout */
+    int64_t rows;
+    const void* ship;         /* filter column */
+    int64_t ship_lo, ship_hi; /* row passes when ship_lo <= ship <= ship_hi */
+    const void* qty;
+    const void* price;
+    const void* disc_price;   /* REF layout only */
+    const void* disc;
+    const void* tax;
+    const uint8_t* flag_a;    /* WIDE: flag_a code; REF: combined group code */
+    const uint8_t* flag_b;    /* WIDE only */
+    int32_t n_a, n_b;         /* dictionary sizes; group = a * n_b + b   */
+} bolap_q1_args;
+
+/* Exact integer accumulators of one group (Q1Agg, query_q1.cpp:27-34).
+   sum_charge is a two's-complement 128-bit integer (lo, hi). */
+typedef struct {
+    int64_t sum_qty;
+    int64_t sum_base_price;
+    int64_t sum_disc_price;
+    uint64_t sum_charge_lo;
+    int64_t sum_charge_hi;
+    int64_t sum_disc;
+    int64_t count;
+    int64_t reserved;
+} bolap_q1_group;
+
+#define BOLAP_Q1_MAX_GROUPS 16
+
+/* Host result: `out` holds n_a*n_b groups, indexed a*n_b+b. */
+int bolap_q1_scan_agg(bolap_ctx* ctx, const bolap_q1_args* args, bolap_q1_group* out);
+/* Stream-ordered variant: `out_dev` is a device array of n_a*n_b groups.
+   No host synchronisation. */
+int bolap_q1_scan_agg_async(bolap_ctx* ctx, const bolap_q1_args* args,
+                            bolap_q1_group* out_dev);
+
+/* ------------------------------------------------------------------------ */
+/* Q19/Q17/Q14 shape: filtered dimension -> hash build -> fact probe ->       */
+/* scalar SUM(price * (100 - disc)).                                          */
+/* Replaces run_q19's part scan + lineitem probe                              */
+/* (bespoke_tpch/query_q19.cpp:210-345) with a hash join on the key.          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t dim_rows;
+    const int32_t* dim_key;          /* unique keys */
+    int32_t n_dim_preds;
+    bolap_pred dim_preds[BOLAP_MAX_PREDS];
+    int64_t fact_rows;
+    const int32_t* fact_fk;
+    const void* fact_price;          /* i64 (WIDE) or i32 (reference extendedprice) */
+    int32_t price_dtype;
+    const void* fact_disc;           /* i64 (WIDE) or u8 (reference discount) */
+    int32_t disc_dtype;
+    /* Sizing.  capacity_hint == 0: exact path (count pass, host reads the
+       count, then build).  > 0: expected build rows from an estimator; the
+       table is sized from it without a count pass and an overflow flag
+       triggers the exact path. */
+    int64_t capacity_hint;
+    int32_t placement;               /* bolap_placement */
+} bolap_join_sum_args;
+
+typedef struct {
+    int64_t revenue;        /* SUM(price * (100 - disc)), exact int64 */
+    int64_t build_rows;     /* dimension rows that passed the filter */
+    int64_t matches;        /* fact rows whose key found a build row */
+    int64_t capacity;       /* hash slots of the table that produced revenue */
+    int32_t placement;      /* BOLAP_PLACE_LDS or BOLAP_PLACE_HBM */
+    int32_t fallbacks;      /* 1 when an estimate-sized table overflowed */
+} bolap_join_sum_result;
+
+int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* args,
+                   bolap_join_sum_result* out);
+
+/* ------------------------------------------------------------------------ */
+/* Q3 shape: customer filter -> orders filter + join -> fact join ->          */
+/* GROUP BY order key SUM(price * (100 - disc)) -> ORDER BY revenue DESC,     */
+/* day ASC, key ASC.  Replaces run_q3 (bespoke_tpch/query_q3.cpp:153-365);    */
+/* the key tiebreak refines the reference's unspecified std::sort tie order  */
+/* (query_q3.cpp:348-356).                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t cust_rows;
+    const int32_t* cust_key;
+    const uint8_t* cust_segment;
+    int32_t segment;                 /* customer passes when segment == this */
+    int64_t order_rows;
+    const int32_t* order_key;        /* unique, >= 0 */
+    const int32_t* order_day;
+    const int32_t* order_cust;
+    int32_t day_lt;                  /* order passes when day < day_lt */
+    int64_t fact_rows;
+    const int32_t* fact_order;
+    const void* fact_price;          /* i64 or i32 */
+    int32_t price_dtype;
+    const void* fact_disc;           /* i64 or u8; ignored when disc_price set */
+    int32_t disc_dtype;
+    const int32_t* fact_disc_price;  /* optional: reference discounted_price
+                                        (builder_impl.cpp:523-532); when set,
+                                        revenue sums it instead */
+    int64_t capacity_hint;           /* 0 = exact sizing; >0 = estimated groups */
+} bolap_chain_args;
+
+typedef struct {
+    int32_t key;
+    int32_t day;
+    int64_t revenue;
+} bolap_group_row;
+
+typedef struct {
+    int64_t cust_pass;
+    int64_t order_pass;      /* orders that passed day + customer filter */
+    int64_t fact_matches;    /* fact rows joined to a passing order */
+    int64_t groups;          /* passing orders with >= 1 fact row */
+    int64_t capacity;        /* group hash slots */
+    int32_t fallbacks;
+    int32_t sort_passes;     /* radix passes run by the ordering step */
+} bolap_chain_stats;
+
+/* Top-k rows in order; *n_out = min(k, groups). */
+int bolap_chain_topk(bolap_ctx* ctx, const bolap_chain_args* args, int32_t k,
+                     bolap_group_row* out, int32_t* n_out, bolap_chain_stats* stats);
+/* All groups in order (the reference returns every group, query_q3.cpp:348).
+   `out` must hold `max_rows`; *n_out = groups (error when > max_rows). */
+int bolap_chain_groups_sorted(bolap_ctx* ctx, const bolap_chain_args* args,
+                              bolap_group_row* out, int64_t max_rows, int64_t* n_out,
+                              bolap_chain_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* LSD radix sort of 64-bit keys with 32-bit payloads (ascending, stable).    */
+/* Replaces the 16-bit-digit counting sorts of bespoke_tpch/query_q10.cpp:    */
+/* 134-166 and 318-373.  keys/vals are device arrays, sorted in place.        */
+/* key_bits: only the low key_bits bits take part (passes = ceil(bits/8)).    */
+/* ------------------------------------------------------------------------ */
+int bolap_radix_sort_u64(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n,
+                         int32_t key_bits);
+/* k smallest keys (ascending) with their payloads, by LDS-histogram radix
+   select; keys_out/vals_out are device arrays of k entries. */
+int bolap_topk_u64(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64_t n,
+                   int32_t k, uint64_t* keys_out, uint32_t* vals_out);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded synthetic tables (counter-based SplitMix64, order independent).     */
+/* Every output is a caller-allocated device array.                           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t* ship;
+    int64_t* qty;
+    int64_t* price;
+    int64_t* disc;
+    int64_t* tax;
+    uint8_t* flag_a;
+    uint8_t* flag_b;
+} bolap_gen_lineitem_cols;
+
+int bolap_gen_lineitem(bolap_ctx* ctx, uint64_t seed, int64_t rows,
+                       const bolap_gen_lineitem_cols* cols);
+int bolap_gen_dim(bolap_ctx* ctx, uint64_t seed, int64_t rows, int32_t* key,
+                  uint8_t* cat, uint8_t* cont);
+int bolap_gen_fact(bolap_ctx* ctx, uint64_t seed, int64_t rows, int64_t dim_rows,
+                   int32_t* fk, int64_t* price, int64_t* disc);
+int bolap_gen_customers(bolap_ctx* ctx, uint64_t seed, int64_t rows, int32_t* key,
+                        uint8_t* segment);
+/* Orders plus per-order fact row counts (1..7); fact_offsets gets the
+   exclusive prefix sum (rows+1 entries, last = fact rows). */
+int bolap_gen_orders(bolap_ctx* ctx, uint64_t seed, int64_t rows, int64_t cust_rows,
+                     int32_t* key, int32_t* day, int32_t* cust, int64_t* fact_offsets);
+int bolap_gen_order_facts(bolap_ctx* ctx, uint64_t seed, int64_t order_rows,
+                          const int64_t* fact_offsets, int64_t fact_rows,
+                          int32_t* fk_order, int64_t* price, int64_t* disc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BOLAP_H */

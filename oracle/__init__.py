@@ -1,0 +1,242 @@
+"""ctypes wrapper of the C oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _DIR / "liboracle.so"
+_lib = None
+
+vp = C.c_void_p
+i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
+
+
+class Q1Group(C.Structure):
+    _fields_ = [("sum_qty", i64), ("sum_base_price", i64), ("sum_disc_price", i64),
+                ("sum_charge_lo", u64), ("sum_charge_hi", i64), ("sum_disc", i64),
+                ("count", i64), ("reserved", i64)]
+
+
+class Pred(C.Structure):
+    _fields_ = [("col", vp), ("dtype", i32), ("op", i32), ("lo", i64), ("hi", i64),
+                ("inline_set", u64 * 4), ("set", vp), ("set_bits", i64)]
+
+
+class JoinSumOut(C.Structure):
+    _fields_ = [("revenue", i64), ("build_rows", i64), ("matches", i64)]
+
+
+class GroupRow(C.Structure):
+    _fields_ = [("key", i32), ("day", i32), ("revenue", i64)]
+
+
+class ChainStats(C.Structure):
+    _fields_ = [("cust_pass", i64), ("order_pass", i64), ("fact_matches", i64),
+                ("groups", i64)]
+
+
+GROUP_DTYPE = np.dtype([("key", "<i4"), ("day", "<i4"), ("revenue", "<i8")])
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        L.og_mix64.restype = u64
+        L.og_mix64.argtypes = [u64]
+        L.og_hash.restype = u64
+        L.og_hash.argtypes = [u64, C.c_uint32, u64]
+        L.og_uniform.restype = i64
+        L.og_uniform.argtypes = [u64, i64, i64]
+        L.og_lineitem.argtypes = [u64, i64, i64] + [vp] * 7
+        L.og_dim.argtypes = [u64, i64, vp, vp, vp]
+        L.og_fact.argtypes = [u64, i64, i64, i64, vp, vp, vp]
+        L.og_customers.argtypes = [u64, i64, vp, vp]
+        L.og_orders.restype = i64
+        L.og_orders.argtypes = [u64, i64, i64, vp, vp, vp, vp]
+        L.og_order_facts.argtypes = [u64, i64, vp, vp, vp, vp]
+        L.oracle_q1_wide.argtypes = [i64, vp, i64, i64, vp, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.oracle_q1_ref.argtypes = [i64, vp, i64, i64, vp, vp, vp, vp, vp, vp, i32, vp]
+        L.oracle_pred_row.restype = C.c_int
+        L.oracle_pred_row.argtypes = [C.POINTER(Pred), i64]
+        L.oracle_join_sum.argtypes = [i64, vp, i32, C.POINTER(Pred), i64, vp, vp, i32, vp,
+                                      i32, C.POINTER(JoinSumOut)]
+        L.oracle_chain_groups.restype = i64
+        L.oracle_chain_groups.argtypes = [i64, vp, vp, i32, i64, vp, vp, vp, i32, i64, vp,
+                                          vp, i32, vp, i32, vp, vp, i64,
+                                          C.POINTER(ChainStats)]
+        L.oracle_sort_groups.argtypes = [vp, i64]
+        L.oracle_radix_sort_u64.argtypes = [vp, vp, i64, i32]
+        L.oracle_topk_u64.argtypes = [vp, vp, i64, i32, vp, vp]
+        L.oracle_set_mutation.argtypes = [i32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+_NP_CODES = {np.dtype(np.uint8): 0, np.dtype(np.int16): 1, np.dtype(np.int32): 2,
+             np.dtype(np.int64): 3}
+
+
+def dtype_code(a: np.ndarray) -> int:
+    return _NP_CODES[a.dtype]
+
+
+def set_mutation(m: int) -> None:
+    lib().oracle_set_mutation(m)
+
+
+# ----------------------------------------------------------------- generators
+def gen_lineitem(seed: int, rows: int, row0: int = 0) -> dict:
+    c = {"ship": np.empty(rows, np.int32), "qty": np.empty(rows, np.int64),
+         "price": np.empty(rows, np.int64), "disc": np.empty(rows, np.int64),
+         "tax": np.empty(rows, np.int64), "flag_a": np.empty(rows, np.uint8),
+         "flag_b": np.empty(rows, np.uint8)}
+    lib().og_lineitem(seed, rows, row0, *(_p(c[k]) for k in
+                                          ("ship", "qty", "price", "disc", "tax", "flag_a",
+                                           "flag_b")))
+    return c
+
+
+def gen_dim(seed: int, rows: int) -> dict:
+    c = {"key": np.empty(rows, np.int32), "cat": np.empty(rows, np.uint8),
+         "cont": np.empty(rows, np.uint8)}
+    lib().og_dim(seed, rows, _p(c["key"]), _p(c["cat"]), _p(c["cont"]))
+    return c
+
+
+def gen_fact(seed: int, rows: int, dim_rows: int, row0: int = 0) -> dict:
+    c = {"fk": np.empty(rows, np.int32), "price": np.empty(rows, np.int64),
+         "disc": np.empty(rows, np.int64)}
+    lib().og_fact(seed, rows, row0, dim_rows, _p(c["fk"]), _p(c["price"]), _p(c["disc"]))
+    return c
+
+
+def gen_chain(seed: int, cust_rows: int, order_rows: int) -> dict:
+    t = {"cust_key": np.empty(cust_rows, np.int32),
+         "cust_segment": np.empty(cust_rows, np.uint8),
+         "order_key": np.empty(order_rows, np.int32),
+         "order_day": np.empty(order_rows, np.int32),
+         "order_cust": np.empty(order_rows, np.int32),
+         "fact_offsets": np.empty(order_rows + 1, np.int64)}
+    L = lib()
+    L.og_customers(seed, cust_rows, _p(t["cust_key"]), _p(t["cust_segment"]))
+    n = L.og_orders(seed, order_rows, cust_rows, _p(t["order_key"]), _p(t["order_day"]),
+                    _p(t["order_cust"]), _p(t["fact_offsets"]))
+    t["fact_order"] = np.empty(n, np.int32)
+    t["fact_price"] = np.empty(n, np.int64)
+    t["fact_disc"] = np.empty(n, np.int64)
+    L.og_order_facts(seed, order_rows, _p(t["fact_offsets"]), _p(t["fact_order"]),
+                     _p(t["fact_price"]), _p(t["fact_disc"]))
+    return t
+
+
+# ------------------------------------------------------------------------ Q1
+def _int128(lo: int, hi: int) -> int:
+    return (hi << 64) + (lo & ((1 << 64) - 1))
+
+
+def _groups(out) -> list[dict]:
+    return [{"sum_qty": g.sum_qty, "sum_base_price": g.sum_base_price,
+             "sum_disc_price": g.sum_disc_price,
+             "sum_charge": _int128(g.sum_charge_lo, g.sum_charge_hi),
+             "sum_disc": g.sum_disc, "count": g.count} for g in out]
+
+
+def q1_wide(c: dict, ship_lo: int, ship_hi: int, n_a: int, n_b: int) -> list[dict]:
+    out = (Q1Group * (n_a * n_b))()
+    lib().oracle_q1_wide(len(c["ship"]), _p(c["ship"]), ship_lo, ship_hi, _p(c["qty"]),
+                         _p(c["price"]), _p(c["disc"]), _p(c["tax"]), _p(c["flag_a"]),
+                         _p(c["flag_b"]), n_a, n_b, out)
+    return _groups(out)
+
+
+def q1_ref(c: dict, ship_lo: int, ship_hi: int, n_groups: int) -> list[dict]:
+    out = (Q1Group * n_groups)()
+    lib().oracle_q1_ref(len(c["ship"]), _p(c["ship"]), ship_lo, ship_hi, _p(c["qty"]),
+                        _p(c["price"]), _p(c["disc_price"]), _p(c["disc"]), _p(c["tax"]),
+                        _p(c["code"]), n_groups, out)
+    return _groups(out)
+
+
+# --------------------------------------------------------------- predicates
+def make_pred(col: np.ndarray, op: int, lo: int = 0, hi: int = 0, values=None,
+              set_bitmap: np.ndarray | None = None) -> Pred:
+    p = Pred()
+    p.col = _p(col)
+    p.dtype = dtype_code(col)
+    p.op = op
+    p.lo, p.hi = lo, hi
+    if values is not None:
+        words = [0, 0, 0, 0]
+        for v in values:
+            words[v >> 6] |= 1 << (v & 63)
+        for i in range(4):
+            p.inline_set[i] = words[i]
+    if set_bitmap is not None:
+        p.set = _p(set_bitmap)
+        p.set_bits = len(set_bitmap) * 32
+    return p
+
+
+def join_sum(dim_key, preds: list, fact_fk, fact_price, fact_disc) -> dict:
+    arr = (Pred * max(1, len(preds)))(*preds)
+    out = JoinSumOut()
+    lib().oracle_join_sum(len(dim_key), _p(dim_key), len(preds), arr, len(fact_fk),
+                          _p(fact_fk), _p(fact_price), dtype_code(fact_price),
+                          _p(fact_disc), dtype_code(fact_disc), C.byref(out))
+    return {"revenue": out.revenue, "build_rows": out.build_rows, "matches": out.matches}
+
+
+def chain_groups(t: dict, segment: int, day_lt: int, disc_price=None):
+    n_orders = len(t["order_key"])
+    out = np.empty(max(1, n_orders), GROUP_DTYPE)
+    st = ChainStats()
+    n = lib().oracle_chain_groups(
+        len(t["cust_key"]), _p(t["cust_key"]), _p(t["cust_segment"]), segment, n_orders,
+        _p(t["order_key"]), _p(t["order_day"]), _p(t["order_cust"]), day_lt,
+        len(t["fact_order"]), _p(t["fact_order"]), _p(t["fact_price"]),
+        dtype_code(t["fact_price"]), _p(t["fact_disc"]), dtype_code(t["fact_disc"]),
+        _p(disc_price), _p(out), len(out), C.byref(st))
+    stats = {"cust_pass": st.cust_pass, "order_pass": st.order_pass,
+             "fact_matches": st.fact_matches, "groups": st.groups}
+    return out[:n].copy(), stats
+
+
+def sort_groups(rows: np.ndarray) -> np.ndarray:
+    r = np.ascontiguousarray(rows.copy())
+    lib().oracle_sort_groups(_p(r), len(r))
+    return r
+
+
+def radix_sort_u64(keys: np.ndarray, vals: np.ndarray, key_bits: int = 64):
+    k = np.ascontiguousarray(keys.astype(np.uint64))
+    v = np.ascontiguousarray(vals.astype(np.uint32))
+    lib().oracle_radix_sort_u64(_p(k), _p(v), len(k), key_bits)
+    return k, v
+
+
+def topk_u64(keys: np.ndarray, vals: np.ndarray, k: int):
+    m = min(k, len(keys))
+    ko = np.empty(max(1, m), np.uint64)
+    vo = np.empty(max(1, m), np.uint32)
+    kk = np.ascontiguousarray(keys.astype(np.uint64))
+    vv = np.ascontiguousarray(vals.astype(np.uint32))
+    lib().oracle_topk_u64(_p(kk), _p(vv), len(kk), k, _p(ko), _p(vo))
+    return ko[:m], vo[:m]

@@ -1,0 +1,112 @@
+/*
+ * oracle.h — CPU restatement of the reference pipelines.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (libbolap) never links,
+ * loads or calls it.
+ *
+ * Every function is single-threaded plain C over host arrays and cites the
+ * reference file:line it restates (paths relative to the reference root).
+ * Integer accumulation uses unsigned 64/128-bit arithmetic so overflow
+ * wraps exactly like the two's-complement sums of the GPU path.
+ */
+#ifndef BOLAP_ORACLE_H
+#define BOLAP_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- seeded generators (CPU mirror of the device generators) ----------- */
+uint64_t og_mix64(uint64_t z);
+uint64_t og_hash(uint64_t seed, uint32_t stream, uint64_t row);
+int64_t og_uniform(uint64_t h, int64_t lo, int64_t hi);
+
+void og_lineitem(uint64_t seed, int64_t rows, int64_t row0, int32_t* ship, int64_t* qty,
+                 int64_t* price, int64_t* disc, int64_t* tax, uint8_t* flag_a,
+                 uint8_t* flag_b);
+void og_dim(uint64_t seed, int64_t rows, int32_t* key, uint8_t* cat, uint8_t* cont);
+void og_fact(uint64_t seed, int64_t rows, int64_t row0, int64_t dim_rows, int32_t* fk,
+             int64_t* price, int64_t* disc);
+void og_customers(uint64_t seed, int64_t rows, int32_t* key, uint8_t* segment);
+int64_t og_orders(uint64_t seed, int64_t rows, int64_t cust_rows, int32_t* key,
+                  int32_t* day, int32_t* cust, int64_t* fact_offsets);
+void og_order_facts(uint64_t seed, int64_t order_rows, const int64_t* fact_offsets,
+                    int32_t* fk_order, int64_t* price, int64_t* disc);
+
+/* --- Q1: filter-scan + low-cardinality group-by ------------------------ */
+typedef struct {
+    int64_t sum_qty, sum_base_price, sum_disc_price;
+    uint64_t sum_charge_lo;
+    int64_t sum_charge_hi;
+    int64_t sum_disc, count, reserved;
+} oracle_q1_group;
+
+void oracle_q1_wide(int64_t rows, const int32_t* ship, int64_t ship_lo, int64_t ship_hi,
+                    const int64_t* qty, const int64_t* price, const int64_t* disc,
+                    const int64_t* tax, const uint8_t* flag_a, const uint8_t* flag_b,
+                    int32_t n_a, int32_t n_b, oracle_q1_group* out);
+void oracle_q1_ref(int64_t rows, const int16_t* ship, int64_t ship_lo, int64_t ship_hi,
+                   const int16_t* qty, const int32_t* price, const int32_t* disc_price,
+                   const uint8_t* disc, const uint8_t* tax, const uint8_t* code,
+                   int32_t n_groups, oracle_q1_group* out);
+
+/* --- predicates ---------------------------------------------------------- */
+typedef struct {
+    const void* col;
+    int32_t dtype;   /* 0 u8, 1 i16, 2 i32, 3 i64 */
+    int32_t op;      /* 0 range, 1 in-set */
+    int64_t lo, hi;
+    uint64_t inline_set[4];
+    const uint32_t* set;
+    int64_t set_bits;
+} oracle_pred;
+
+int oracle_pred_row(const oracle_pred* p, int64_t row);
+
+/* --- join + scalar sum (Q19 shape) -------------------------------------- */
+typedef struct {
+    int64_t revenue, build_rows, matches;
+} oracle_join_sum_out;
+
+void oracle_join_sum(int64_t dim_rows, const int32_t* dim_key, int32_t n_preds,
+                     const oracle_pred* preds, int64_t fact_rows, const int32_t* fact_fk,
+                     const void* fact_price, int32_t price_dtype, const void* fact_disc,
+                     int32_t disc_dtype, oracle_join_sum_out* out);
+
+/* --- chain join + group-by (Q3 shape) ------------------------------------ */
+typedef struct {
+    int32_t key;
+    int32_t day;
+    int64_t revenue;
+} oracle_group_row;
+
+typedef struct {
+    int64_t cust_pass, order_pass, fact_matches, groups;
+} oracle_chain_stats;
+
+int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
+                            const uint8_t* cust_segment, int32_t segment, int64_t order_rows,
+                            const int32_t* order_key, const int32_t* order_day,
+                            const int32_t* order_cust, int32_t day_lt, int64_t fact_rows,
+                            const int32_t* fact_order, const void* fact_price,
+                            int32_t price_dtype, const void* fact_disc, int32_t disc_dtype,
+                            const int32_t* fact_disc_price, oracle_group_row* out,
+                            int64_t max_out, oracle_chain_stats* stats);
+void oracle_sort_groups(oracle_group_row* rows, int64_t n);
+
+/* --- radix sort / top-k --------------------------------------------------- */
+void oracle_radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int32_t key_bits);
+void oracle_topk_u64(const uint64_t* keys, const uint32_t* vals, int64_t n, int32_t k,
+                     uint64_t* keys_out, uint32_t* vals_out);
+
+/* --- planted mistakes, for tests that show the checks have teeth -------- */
+void oracle_set_mutation(int32_t m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

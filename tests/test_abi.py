@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports exactly what include/bolap.h declares.
+
+CPU-only: no call here touches a GPU.
+"""
+import ctypes as C
+import pathlib
+import re
+
+import pytest
+
+from bespokeolap_artifacts_amd import native
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "bolap.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(bolap_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_is_built():
+    assert native.LIB_PATH.exists(), "run __graft_entry__.build() first"
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(str(native.LIB_PATH))
+    names = header_functions()
+    assert len(names) >= 15
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_prototypes_mirror_the_header():
+    assert set(native.PROTOTYPES) == header_functions()
+    lib = native.load()
+    assert lib.bolap_abi_version() == 1
+    assert lib.bolap_last_error() == b""
+
+
+def test_struct_sizes_match_c_layout():
+    # Sizes fixed by the C declarations (natural alignment, x86-64 / amdgcn).
+    assert C.sizeof(native.Pred) == 80
+    assert C.sizeof(native.Q1Group) == 64
+    assert C.sizeof(native.GroupRow) == 16
+    assert C.sizeof(native.JoinSumArgs) == 8 + 8 + 8 + 80 * 4 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8
+
+
+def test_loader_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(native, "_lib", None)
+    monkeypatch.setattr(native, "LIB_PATH", tmp_path / "libbolap.so")
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        native.load()
