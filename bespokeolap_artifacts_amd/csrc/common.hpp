@@ -6,8 +6,15 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "bolap.h"
+
+struct bolap_trace_rec {
+    char name[40];
+    hipEvent_t start = nullptr, stop = nullptr;  // null for counts
+    int64_t count = 0;
+};
 
 // ---------------------------------------------------------------------------
 // Context.
@@ -38,6 +45,11 @@ struct bolap_ctx {
     // Pinned host staging for results copied back at the end of a pipeline.
     void* pinned = nullptr;
     size_t pinned_size = 0;
+
+    // Stage tracing (bolap_ctx_set_trace).
+    bool trace = false;
+    std::vector<bolap_trace_rec> trace_recs;
+    std::vector<hipEvent_t> event_pool;
 };
 
 namespace bolap {
@@ -75,6 +87,16 @@ int ws_reserve(bolap_ctx* ctx, size_t bytes);
 int pinned_reserve(bolap_ctx* ctx, size_t bytes);
 // Checks the last kernel launch.
 int launch_check(const char* what);
+
+// Records a pair of events around a stage when tracing is on.
+struct TraceScope {
+    bolap_ctx* ctx;
+    int idx = -1;
+    TraceScope(bolap_ctx* c, const char* name);
+    ~TraceScope();
+};
+// Records a COUNT entry when tracing is on.
+void trace_count(bolap_ctx* ctx, const char* name, int64_t n);
 
 // Bump allocator over the context workspace.  Plan first (sizes only), then
 // reserve, then carve with the same sequence of sizes.
@@ -133,12 +155,38 @@ BOLAP_DEV VecT<T, R> load_unit(const T* __restrict__ p, int64_t unit) {
     return reinterpret_cast<const VecT<T, R>*>(p)[unit];
 }
 
-template <typename T>
-BOLAP_DEV T wave_sum(T v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+// ---- wave reductions with DPP (VALU only, no LDS round trips) -------------
+// Inclusive prefix within each 16-lane row by row_shr 1, 2, 4, 8 (lanes whose
+// source falls outside the row read 0); lanes 15, 31, 47 and 63 then hold the
+// four row totals, which readlane moves to scalar registers.  Every lane of
+// the wave must be active.
+template <int CTRL>
+BOLAP_DEV uint32_t dpp_u32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xf, 0xf, true));
 }
+template <int CTRL>
+BOLAP_DEV uint64_t dpp_u64(uint64_t v) {
+    const uint32_t lo = dpp_u32<CTRL>(uint32_t(v));
+    const uint32_t hi = dpp_u32<CTRL>(uint32_t(v >> 32));
+    return (uint64_t(hi) << 32) | lo;
+}
+BOLAP_DEV uint64_t readlane_u64(uint64_t v, int lane) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), lane));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), lane));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Sum over the 64 lanes (mod 2^64); the result is wave-uniform.
+BOLAP_DEV uint64_t wave_sum_u64(uint64_t x) {
+    x += dpp_u64<0x111>(x);
+    x += dpp_u64<0x112>(x);
+    x += dpp_u64<0x114>(x);
+    x += dpp_u64<0x118>(x);
+    return readlane_u64(x, 15) + readlane_u64(x, 31) + readlane_u64(x, 47) +
+           readlane_u64(x, 63);
+}
+BOLAP_DEV int64_t wave_sum(int64_t v) { return int64_t(wave_sum_u64(uint64_t(v))); }
+BOLAP_DEV uint64_t wave_sum(uint64_t v) { return wave_sum_u64(v); }
 
 // Two's-complement 128-bit accumulator as (lo, hi).
 struct Acc128 {
@@ -158,15 +206,23 @@ BOLAP_DEV void add128(Acc128& a, const Acc128& b) {
     a.lo = nlo;
 }
 
+template <int CTRL>
+BOLAP_DEV void dpp_step128(Acc128& a) {
+    Acc128 o;
+    o.lo = dpp_u64<CTRL>(a.lo);
+    o.hi = int64_t(dpp_u64<CTRL>(uint64_t(a.hi)));
+    add128(a, o);
+}
+
 BOLAP_DEV Acc128 wave_sum128(Acc128 a) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        Acc128 o;
-        o.lo = __shfl_xor(a.lo, off, kWave);
-        o.hi = __shfl_xor(a.hi, off, kWave);
-        add128(a, o);
-    }
-    return a;
+    dpp_step128<0x111>(a);
+    dpp_step128<0x112>(a);
+    dpp_step128<0x114>(a);
+    dpp_step128<0x118>(a);
+    Acc128 r{readlane_u64(a.lo, 15), int64_t(readlane_u64(uint64_t(a.hi), 15))};
+    for (int lane = 31; lane < 64; lane += 16)
+        add128(r, Acc128{readlane_u64(a.lo, lane), int64_t(readlane_u64(uint64_t(a.hi), lane))});
+    return r;
 }
 
 // Coherent (agent-scope) loads/stores for data exchanged between workgroups

@@ -1,0 +1,448 @@
+// join_sum.hip — filtered dimension -> hash build -> fact probe -> SUM.
+//
+// Replaces the part scan + lineitem probe of run_q19
+// (bespoke_tpch/query_q19.cpp:210-345) and run_q17's bitset semi-join
+// (query_q17.cpp:161-262): the dimension filter's surviving keys form the
+// build side, every fact row probes it, and matching rows add
+// price * (100 - disc) into one exact int64.
+//
+// Kernels (all on the context stream):
+//   count  (exact path only) predicate pass over the dimension, one atomic
+//          per wave; the host reads the count to size the table.
+//   init   fills the table image with EMPTY, clears bitmap and result slot.
+//   build  predicate pass + open-addressing insert (atomicCAS, linear
+//          probing) of surviving keys into an HBM table image, plus a
+//          one-hash bitmap prefilter; an overflow flag is raised when the
+//          distinct keys exceed half the slots (estimate-sized tables).
+//   probe  streams the fact key column with 128-bit loads; per key one
+//          bitmap test, and only on a bitmap hit a linear probe of the
+//          table; bitmap and table are staged in LDS when they fit (one
+//          copy per workgroup) and read through L2 otherwise.  Matches
+//          gather price/disc and accumulate per lane; DPP wave sums, LDS,
+//          per-block partials and a last-block merge give the result.
+#include <cstring>
+
+#include "preds.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / kWave;
+constexpr int32_t kEmpty = INT32_MIN;
+constexpr uint32_t kSlotMul = 0x9E3779B1u;   // Fibonacci hashing
+constexpr uint32_t kBitMul = 0x85EBCA77u;    // independent multiplier for the bitmap
+constexpr int kLdsBudget = 96 * 1024;
+
+// Result slot layout (int64): revenue, matches, build rows, distinct keys,
+// flags (bit 0 overflow, bit 1 INT32_MIN key present), counted rows.
+enum { kRev = 0, kMatch = 1, kBuild = 2, kDistinct = 3, kFlags = 4, kCount = 5, kResN = 8 };
+
+struct Table {
+    int32_t* slots;
+    uint32_t* bits;
+    int32_t cap_log2;
+    int32_t bb_log2;
+    int64_t max_fill;
+};
+
+BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
+    return (uint32_t(key) * kSlotMul) >> (32 - cap_log2);
+}
+BOLAP_DEV uint32_t bit_of(int32_t key, int bb_log2) {
+    return (uint32_t(key) * kBitMul) >> (32 - bb_log2);
+}
+
+__global__ __launch_bounds__(256) void zero_res_k(int64_t* res) {
+    if (threadIdx.x < kResN) res[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void dim_count_k(PredSet ps, int64_t rows, int64_t* res) {
+    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
+         base += int64_t(gridDim.x) * kThreads) {
+        const int64_t row = base + threadIdx.x;
+        const bool pass = row < rows && eval_preds(ps, row);
+        const uint64_t m = __ballot(pass);
+        if ((threadIdx.x & (kWave - 1)) == 0 && m)
+            atomicAdd(reinterpret_cast<unsigned long long*>(res + kCount),
+                      static_cast<unsigned long long>(__popcll(m)));
+    }
+}
+
+__global__ __launch_bounds__(256) void table_init_k(Table t, int64_t* res) {
+    const int64_t cap = int64_t(1) << t.cap_log2;
+    const int64_t words = (int64_t(1) << t.bb_log2) / 32;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cap;
+         i += int64_t(gridDim.x) * 256)
+        t.slots[i] = kEmpty;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
+         i += int64_t(gridDim.x) * 256)
+        t.bits[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kResN && threadIdx.x != kCount) res[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows,
+                                                        const int32_t* __restrict__ keys,
+                                                        Table t, int64_t* res) {
+    const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
+    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
+         base += int64_t(gridDim.x) * kThreads) {
+        const int64_t row = base + threadIdx.x;
+        const bool pass = row < rows && eval_preds(ps, row);
+        const uint64_t m = __ballot(pass);
+        if ((threadIdx.x & (kWave - 1)) == 0 && m)
+            atomicAdd(reinterpret_cast<unsigned long long*>(res + kBuild),
+                      static_cast<unsigned long long>(__popcll(m)));
+        if (!pass) continue;
+        const int32_t key = keys[row];
+        if (key == kEmpty) {
+            atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
+            continue;
+        }
+        const uint32_t b = bit_of(key, t.bb_log2);
+        atomicOr(t.bits + (b >> 5), 1u << (b & 31));
+        uint32_t s = slot_of(key, t.cap_log2);
+        bool placed = false;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const int32_t prev = atomicCAS(t.slots + s, kEmpty, key);
+            if (prev == kEmpty) {
+                const unsigned long long d =
+                    atomicAdd(reinterpret_cast<unsigned long long*>(res + kDistinct), 1ull);
+                if (int64_t(d) + 1 > t.max_fill)
+                    atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 1ull);
+                placed = true;
+                break;
+            }
+            if (prev == key) {
+                placed = true;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+        if (!placed) atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 1ull);
+    }
+}
+
+template <typename TP, typename TD>
+BOLAP_DEV void probe_key(int32_t key, bool valid, int64_t row, bool has_min,
+                         const uint32_t* B, const int32_t* T, int bb_log2, int cap_log2,
+                         uint32_t mask, const TP* __restrict__ price,
+                         const TD* __restrict__ disc, int64_t& rev, uint32_t& cnt) {
+    bool found = false;
+    if (key == kEmpty) {
+        found = valid && has_min;
+    } else {
+        const uint32_t b = bit_of(key, bb_log2);
+        if (valid && ((B[b >> 5] >> (b & 31)) & 1u)) {
+            uint32_t s = slot_of(key, cap_log2);
+            for (uint32_t probe = 0; probe <= mask; ++probe) {
+                const int32_t t = T[s];
+                if (t == key) {
+                    found = true;
+                    break;
+                }
+                if (t == kEmpty) break;
+                s = (s + 1) & mask;
+            }
+        }
+    }
+    if (found) {
+        rev += int64_t(uint64_t(int64_t(price[row])) * uint64_t(100 - int64_t(disc[row])));
+        cnt += 1;
+    }
+}
+
+constexpr int kProbeU = 4;  // 4-key units in flight per lane
+
+template <bool kLds, typename TP, typename TD>
+__global__ __launch_bounds__(kThreads) void probe_sum_k(
+    const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
+    int64_t rows, Table t, int64_t* res, int64_t* partials, unsigned* counter) {
+    extern __shared__ uint32_t lds[];
+    const int64_t words = (int64_t(1) << t.bb_log2) / 32;
+    const int64_t cap = int64_t(1) << t.cap_log2;
+    const uint32_t* B = t.bits;
+    const int32_t* T = t.slots;
+    if constexpr (kLds) {
+        for (int64_t i = threadIdx.x; i < words; i += kThreads) lds[i] = t.bits[i];
+        int32_t* ts = reinterpret_cast<int32_t*>(lds + words);
+        for (int64_t i = threadIdx.x; i < cap; i += kThreads) ts[i] = t.slots[i];
+        __syncthreads();
+        B = lds;
+        T = ts;
+    }
+    const bool has_min = (res[kFlags] & 2) != 0;
+    const uint32_t mask = uint32_t(cap - 1);
+    int64_t rev = 0;
+    uint32_t cnt = 0;
+    const int64_t n_units = rows / 4;
+    const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
+    for (int64_t base = int64_t(blockIdx.x) * kThreads * kProbeU + threadIdx.x; base < n_units;
+         base += stride) {
+        VecT<int32_t, 4> v[kProbeU];
+        bool ok[kProbeU];
+#pragma unroll
+        for (int k = 0; k < kProbeU; ++k) {
+            int64_t u = base + int64_t(k) * kThreads;
+            ok[k] = u < n_units;
+            u = ok[k] ? u : n_units - 1;
+            v[k] = load_unit<int32_t, 4>(fk, u);
+        }
+#pragma unroll
+        for (int k = 0; k < kProbeU; ++k) {
+            const int64_t u = base + int64_t(k) * kThreads;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                probe_key<TP, TD>(v[k].v[r], ok[k], u * 4 + r, has_min, B, T, t.bb_log2,
+                                  t.cap_log2, mask, price, disc, rev, cnt);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        const int64_t row = n_units * 4 + threadIdx.x;
+        if (row < rows)
+            probe_key<TP, TD>(fk[row], true, row, has_min, B, T, t.bb_log2, t.cap_log2, mask,
+                              price, disc, rev, cnt);
+    }
+
+    __shared__ int64_t s_rev[kWaves];
+    __shared__ int64_t s_cnt[kWaves];
+    const int64_t wr = wave_sum(rev);
+    const int64_t wc = int64_t(wave_sum(uint64_t(cnt)));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_rev[wave] = wr;
+        s_cnt[wave] = wc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t a = 0, c = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            a += s_rev[w];
+            c += s_cnt[w];
+        }
+        partials[2 * int64_t(blockIdx.x)] = a;
+        partials[2 * int64_t(blockIdx.x) + 1] = c;
+    }
+    if (!last_block_done(counter)) return;
+    uint64_t a = 0, c = 0;
+    for (int b = threadIdx.x; b < int(gridDim.x); b += kThreads) {
+        a += uint64_t(partials[2 * int64_t(b)]);
+        c += uint64_t(partials[2 * int64_t(b) + 1]);
+    }
+    a = wave_sum(a);
+    c = wave_sum(c);
+    __syncthreads();
+    if (lane == 0) {
+        s_rev[wave] = int64_t(a);
+        s_cnt[wave] = int64_t(c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ta = 0, tc = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            ta += uint64_t(s_rev[w]);
+            tc += uint64_t(s_cnt[w]);
+        }
+        res[kRev] = int64_t(ta);
+        res[kMatch] = int64_t(tc);
+        st_agent(counter, 0u);
+    }
+}
+
+int ceil_log2(int64_t v) {
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+struct Sizing {
+    int cap_log2, bb_log2;
+    int placement;
+    size_t lds_bytes;
+};
+
+// Table geometry for `n` expected build keys: LDS tables run at load <= 1/4
+// with a 64-bit-per-key bitmap; HBM tables at load <= 1/2 with 16 bits per
+// key.  Either overflows (and the caller re-runs exactly) past `cap / 2`.
+int size_table(int64_t n, int placement, Sizing& z) {
+    if (n < 1) n = 1;
+    const int lds_cap = ceil_log2(n * 4 < 64 ? 64 : n * 4);
+    int lds_bb = ceil_log2(n * 64);
+    if (lds_bb < 10) lds_bb = 10;
+    if (lds_bb > 18) lds_bb = 18;
+    const size_t lds_bytes = (size_t(1) << lds_cap) * 4 + (size_t(1) << lds_bb) / 8;
+    const bool fits = lds_cap <= 30 && lds_bytes <= size_t(kLdsBudget);
+    if (placement == BOLAP_PLACE_LDS && !fits) {
+        set_error("bolap_join_sum: %lld keys do not fit the LDS budget", (long long)n);
+        return BOLAP_ERR_CAPACITY;
+    }
+    if (fits && placement != BOLAP_PLACE_HBM) {
+        z = Sizing{lds_cap, lds_bb, BOLAP_PLACE_LDS, lds_bytes};
+        return BOLAP_OK;
+    }
+    int cap = ceil_log2(n * 2 < 64 ? 64 : n * 2);
+    int bb = ceil_log2(n * 16);
+    if (bb < 10) bb = 10;
+    if (bb > 31) bb = 31;
+    if (cap > 31) {
+        set_error("bolap_join_sum: %lld keys exceed the table limit", (long long)n);
+        return BOLAP_ERR_CAPACITY;
+    }
+    z = Sizing{cap, bb, BOLAP_PLACE_HBM, 0};
+    return BOLAP_OK;
+}
+
+template <bool kLds, typename TP, typename TD>
+int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+                 int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+    auto kern = probe_sum_k<kLds, TP, TD>;
+    if (kLds) {
+        BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          int(lds)));
+    }
+    int per_cu = 0;
+    BOLAP_HIP_TRY(
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, kLds ? lds : 0));
+    int64_t blocks = int64_t(ctx->num_cus) * (per_cu > 0 ? per_cu : 1);
+    const int64_t need = (a->fact_rows / 4 + int64_t(kThreads) * kProbeU - 1) /
+                         (int64_t(kThreads) * kProbeU);
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    if (blocks > max_blocks) blocks = max_blocks;
+    TraceScope ts(ctx, "join_probe");
+    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), kLds ? lds : 0,
+                       ctx->stream, a->fact_fk, static_cast<const TP*>(a->fact_price),
+                       static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
+                       ws.at<int64_t>(o_part), ctx->counters + 1);
+    return launch_check("probe_sum_k");
+}
+
+template <bool kLds>
+int dispatch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+    const bool p64 = a->price_dtype == BOLAP_I64;
+    const bool d64 = a->disc_dtype == BOLAP_I64;
+    if (p64 && d64) return launch_probe<kLds, int64_t, int64_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+    if (p64) return launch_probe<kLds, int64_t, uint8_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+    if (d64) return launch_probe<kLds, int32_t, int64_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+    return launch_probe<kLds, int32_t, uint8_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+}
+
+inline unsigned small_grid(bolap_ctx* ctx, int64_t rows, int threads) {
+    int64_t b = (rows + threads - 1) / threads;
+    const int64_t cap = int64_t(ctx->num_cus) * 8;
+    if (b > cap) b = cap;
+    return unsigned(b < 1 ? 1 : b);
+}
+
+// One attempt with a given expected build size; exact == sized from a count.
+int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
+                 int64_t n_expect, int64_t* host_res, Sizing& z) {
+    BOLAP_TRY(size_table(n_expect, a->placement, z));
+    const int64_t cap = int64_t(1) << z.cap_log2;
+    const int64_t words = (int64_t(1) << z.bb_log2) / 32;
+    const int64_t max_blocks = int64_t(ctx->num_cus) * 8;
+    WsArena ws;
+    const size_t o_slots = ws.plan(size_t(cap) * 4);
+    const size_t o_bits = ws.plan(size_t(words) * 4);
+    const size_t o_part = ws.plan(size_t(max_blocks) * 2 * sizeof(int64_t));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+    Table t{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits), z.cap_log2, z.bb_log2, cap / 2};
+    int64_t init_blocks = (cap + 255) / 256;
+    if (init_blocks > 4096) init_blocks = 4096;
+    {
+        TraceScope ts(ctx, "join_init");
+        hipLaunchKernelGGL(table_init_k, dim3(unsigned(init_blocks)), dim3(256), 0, ctx->stream,
+                           t, res);
+    }
+    BOLAP_TRY(launch_check("table_init_k"));
+    if (a->dim_rows > 0) {
+        TraceScope ts(ctx, "join_build");
+        hipLaunchKernelGGL(dim_build_k, dim3(small_grid(ctx, a->dim_rows, kThreads)),
+                           dim3(kThreads), 0, ctx->stream, ps, a->dim_rows, a->dim_key, t, res);
+        BOLAP_TRY(launch_check("dim_build_k"));
+    }
+    if (z.placement == BOLAP_PLACE_LDS)
+        BOLAP_TRY(dispatch_probe<true>(ctx, a, t, z.lds_bytes, res, ws, o_part, max_blocks));
+    else
+        BOLAP_TRY(dispatch_probe<false>(ctx, a, t, 0, res, ws, o_part, max_blocks));
+    BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, res, kResN * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host_res, ctx->pinned, kResN * sizeof(int64_t));
+    return BOLAP_OK;
+}
+
+}  // namespace
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
+                              bolap_join_sum_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_join_sum: null argument");
+    BOLAP_ARG_CHECK(a->dim_rows >= 0 && a->fact_rows >= 0, "bolap_join_sum: negative rows");
+    BOLAP_ARG_CHECK(a->dim_rows == 0 || a->dim_key, "bolap_join_sum: null dim_key");
+    BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_fk && a->fact_price && a->fact_disc),
+                    "bolap_join_sum: null fact column");
+    BOLAP_ARG_CHECK(a->price_dtype == BOLAP_I64 || a->price_dtype == BOLAP_I32,
+                    "bolap_join_sum: price dtype must be i64 or i32");
+    BOLAP_ARG_CHECK(a->disc_dtype == BOLAP_I64 || a->disc_dtype == BOLAP_U8,
+                    "bolap_join_sum: disc dtype must be i64 or u8");
+    BOLAP_ARG_CHECK(a->placement >= BOLAP_PLACE_AUTO && a->placement <= BOLAP_PLACE_HBM,
+                    "bolap_join_sum: placement %d", a->placement);
+    BOLAP_ARG_CHECK(aligned(a->fact_fk, 16) || a->fact_rows == 0,
+                    "bolap_join_sum: fact_fk must be 16-byte aligned");
+    PredSet ps;
+    BOLAP_TRY(make_predset(ps, a->n_dim_preds, a->dim_preds, a->dim_rows, "bolap_join_sum"));
+
+    int64_t r[kResN];
+    Sizing z{};
+    int fallbacks = 0;
+    bool done = false;
+    if (a->capacity_hint > 0) {
+        BOLAP_TRY(join_attempt(ctx, a, ps, a->capacity_hint, r, z));
+        if (r[kFlags] & 1)
+            fallbacks = 1;  // estimate too small: exact path below
+        else
+            done = true;
+    }
+    if (!done) {
+        // Exact sizing: count the surviving dimension rows first.
+        int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+        hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, res);
+        BOLAP_TRY(launch_check("zero_res_k"));
+        if (a->dim_rows > 0) {
+            TraceScope ts(ctx, "join_count");
+            hipLaunchKernelGGL(dim_count_k, dim3(small_grid(ctx, a->dim_rows, kThreads)),
+                               dim3(kThreads), 0, ctx->stream, ps, a->dim_rows, res);
+            BOLAP_TRY(launch_check("dim_count_k"));
+        }
+        int64_t count = 0;
+        BOLAP_TRY(pinned_reserve(ctx, sizeof(int64_t)));
+        BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, res + kCount, sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(&count, ctx->pinned, sizeof(int64_t));
+        BOLAP_TRY(join_attempt(ctx, a, ps, count, r, z));
+        if (r[kFlags] & 1) {
+            set_error("bolap_join_sum: exact-sized table overflowed");
+            return BOLAP_ERR_CAPACITY;
+        }
+    }
+    trace_count(ctx, "join_build_rows", r[kBuild]);
+    trace_count(ctx, "join_probe_rows_in", a->fact_rows);
+    trace_count(ctx, "join_rows_emitted", r[kMatch]);
+    out->revenue = r[kRev];
+    out->matches = r[kMatch];
+    out->build_rows = r[kBuild];
+    out->capacity = int64_t(1) << z.cap_log2;
+    out->placement = z.placement;
+    out->fallbacks = fallbacks;
+    return BOLAP_OK;
+}

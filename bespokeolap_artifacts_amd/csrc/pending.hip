@@ -5,10 +5,6 @@ using namespace bolap;
 
 extern "C" {
 
-int bolap_join_sum(bolap_ctx*, const bolap_join_sum_args*, bolap_join_sum_result*) {
-    set_error("bolap_join_sum: not built yet");
-    return BOLAP_ERR_UNSUPPORTED;
-}
 int bolap_chain_topk(bolap_ctx*, const bolap_chain_args*, int32_t, bolap_group_row*, int32_t*,
                      bolap_chain_stats*) {
     set_error("bolap_chain_topk: not built yet");

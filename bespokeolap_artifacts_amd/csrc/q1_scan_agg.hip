@@ -30,7 +30,7 @@
 namespace bolap {
 namespace {
 
-constexpr int kQ1Threads = 256;
+constexpr int kQ1Threads = 512;
 constexpr int kQ1Waves = kQ1Threads / kWave;
 constexpr int kFields = 8;  // qty, price, disc_price, charge lo, charge hi, disc, count, pad
 
@@ -85,16 +85,20 @@ BOLAP_DEV void q1_row(Q1Lane<G>& acc, const Q1K& a, bool valid, int64_t s, int64
     const int64_t ch_sign = ch >> 63;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const bool m = pass && g == uint32_t(gi);
-        acc.qty[gi] += m ? q : 0;
-        acc.price[gi] += m ? p : 0;
-        acc.dp[gi] += m ? dp : 0;
-        acc.disc[gi] += m ? d : 0;
-        acc.cnt[gi] += m ? 1u : 0u;
-        const uint64_t add_lo = m ? uint64_t(ch) : 0;
-        const uint64_t nlo = acc.ch[gi].lo + add_lo;
-        acc.ch[gi].hi += (m ? ch_sign : 0) + (nlo < acc.ch[gi].lo ? 1 : 0);
-        acc.ch[gi].lo = nlo;
+        // A real branch per group (the empty volatile asm keeps the compiler
+        // from flattening it into selects): the body runs once under the mask
+        // of the lanes in group gi and is skipped when no lane is.
+        if (pass && g == uint32_t(gi)) {
+            asm volatile("");
+            acc.qty[gi] += q;
+            acc.price[gi] += p;
+            acc.dp[gi] += dp;
+            acc.disc[gi] += d;
+            acc.cnt[gi] += 1u;
+            const uint64_t nlo = acc.ch[gi].lo + uint64_t(ch);
+            acc.ch[gi].hi += ch_sign + (nlo < acc.ch[gi].lo ? 1 : 0);
+            acc.ch[gi].lo = nlo;
+        }
     }
 }
 
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         }
     }
 
-    // Block reduction: wave shuffles, then LDS across waves.
+    // Block reduction: DPP within each wave, then LDS across waves.
     __shared__ int64_t s_part[kQ1Waves][G][kFields];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -192,55 +196,60 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         }
     }
     __syncthreads();
-    if (threadIdx.x < G) {
-        const int g = threadIdx.x;
-        int64_t f[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
-        Acc128 ch{0, 0};
-        for (int w = 0; w < kQ1Waves; ++w) {
-            for (int i = 0; i < kFields; ++i)
-                if (i != 3 && i != 4) f[i] += s_part[w][g][i];
-            add128(ch, Acc128{uint64_t(s_part[w][g][3]), s_part[w][g][4]});
+    // Block partial: row of G*kFields int64 per block.
+    constexpr int kP = G * kFields;
+    if (threadIdx.x < kP) {
+        const int g = threadIdx.x / kFields;
+        const int f = threadIdx.x % kFields;
+        int64_t v = 0;
+        if (f == 3) {
+            Acc128 c{0, 0};
+            for (int w = 0; w < kQ1Waves; ++w)
+                add128(c, Acc128{uint64_t(s_part[w][g][3]), s_part[w][g][4]});
+            v = int64_t(c.lo);
+            partials[int64_t(blockIdx.x) * kP + threadIdx.x + 1] = c.hi;
+        } else if (f != 4) {
+            for (int w = 0; w < kQ1Waves; ++w) v += s_part[w][g][f];
         }
-        f[3] = int64_t(ch.lo);
-        f[4] = ch.hi;
-        int64_t* dst = partials + (int64_t(blockIdx.x) * G + g) * kFields;
-        for (int i = 0; i < kFields; ++i) st_agent(dst + i, f[i]);
+        if (f != 4) partials[int64_t(blockIdx.x) * kP + threadIdx.x] = v;
     }
 
     if (!last_block_done(counter)) return;
 
-    // Last block: sum all partials group by group (exact integer sums).
-    for (int g = 0; g < a.n_groups; ++g) {
-        int64_t v[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
-        Acc128 c{0, 0};
-        for (int b = threadIdx.x; b < int(gridDim.x); b += kQ1Threads) {
-            const int64_t* src = partials + (int64_t(b) * G + g) * kFields;
-#pragma unroll
-            for (int i = 0; i < kFields; ++i)
-                if (i != 3 && i != 4) v[i] += ld_agent(src + i);
-            add128(c, Acc128{uint64_t(ld_agent(src + 3)), ld_agent(src + 4)});
-        }
-#pragma unroll
-        for (int i = 0; i < kFields; ++i)
-            if (i != 3 && i != 4) v[i] = wave_sum(v[i]);
-        c = wave_sum128(c);
-        __syncthreads();
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < kFields; ++i)
-                if (i != 3 && i != 4) s_part[wave][0][i] = v[i];
-            s_part[wave][0][3] = int64_t(c.lo);
-            s_part[wave][0][4] = c.hi;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
-            Acc128 tc{0, 0};
-            for (int w = 0; w < kQ1Waves; ++w) {
-                for (int i = 0; i < kFields; ++i)
-                    if (i != 3 && i != 4) t[i] += s_part[w][0][i];
-                add128(tc, Acc128{uint64_t(s_part[w][0][3]), s_part[w][0][4]});
+    // Last block: thread (row0, col) sums column col over rows row0,
+    // row0 + RS, ...; the 128-bit charge column pairs (3, 4) are summed by
+    // the col % 8 == 3 thread.  Plain loads after the acquire fence.
+    constexpr int RS = kQ1Threads / kP;
+    __shared__ int64_t s_red[RS][kP];
+    const int col = threadIdx.x % kP;
+    const int row0 = threadIdx.x / kP;
+    if (row0 < RS) {
+        const int f = col % kFields;
+        if (f == 3) {
+            Acc128 c{0, 0};
+            for (int b = row0; b < int(gridDim.x); b += RS) {
+                const int64_t* src = partials + int64_t(b) * kP + col;
+                add128(c, Acc128{uint64_t(src[0]), src[1]});
             }
+            s_red[row0][col] = int64_t(c.lo);
+            s_red[row0][col + 1] = c.hi;
+        } else if (f != 4) {
+            uint64_t v = 0;
+            for (int b = row0; b < int(gridDim.x); b += RS) v += uint64_t(partials[int64_t(b) * kP + col]);
+            s_red[row0][col] = int64_t(v);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+        const int g = threadIdx.x;
+        int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+        Acc128 tc{0, 0};
+        for (int r = 0; r < RS; ++r) {
+            for (int i = 0; i < kFields; ++i)
+                if (i != 3 && i != 4) t[i] += s_red[r][g * kFields + i];
+            add128(tc, Acc128{uint64_t(s_red[r][g * kFields + 3]), s_red[r][g * kFields + 4]});
+        }
+        if (g < a.n_groups) {
             bolap_q1_group r;
             r.sum_qty = t[0];
             r.sum_base_price = t[1];
@@ -264,13 +273,17 @@ int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     const int64_t n_units = k.rows / R;
     const int64_t tile = int64_t(kQ1Threads) * kQ1U;
     int64_t blocks = (n_units + tile - 1) / tile;
-    const int64_t max_blocks = int64_t(ctx->num_cus) * 4;
+    int per_cu = 0;
+    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, q1_scan_agg_kernel<L, R, kQ1U, G>, kQ1Threads, 0));
+    const int64_t max_blocks = int64_t(ctx->num_cus) * (per_cu > 0 ? per_cu : 1);
     if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
     WsArena plan;
     const size_t o_part = plan.plan(size_t(blocks) * G * kFields * sizeof(int64_t));
     BOLAP_TRY(ws_reserve(ctx, plan.off));
     plan.base = static_cast<char*>(ctx->ws);
+    TraceScope ts(ctx, "q1_scan");
     hipLaunchKernelGGL((q1_scan_agg_kernel<L, R, kQ1U, G>), dim3(unsigned(blocks)),
                        dim3(kQ1Threads), 0, ctx->stream, k, plan.at<int64_t>(o_part),
                        ctx->counters + 0, out_dev);

@@ -51,6 +51,41 @@ int launch_check(const char* what) {
     return BOLAP_OK;
 }
 
+static hipEvent_t take_event(bolap_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+TraceScope::TraceScope(bolap_ctx* c, const char* name) : ctx(c) {
+    if (!ctx || !ctx->trace) return;
+    bolap_trace_rec r;
+    std::snprintf(r.name, sizeof(r.name), "%s", name);
+    r.start = take_event(ctx);
+    r.stop = take_event(ctx);
+    if (!r.start || !r.stop) return;
+    (void)hipEventRecord(r.start, ctx->stream);
+    idx = int(ctx->trace_recs.size());
+    ctx->trace_recs.push_back(r);
+}
+
+TraceScope::~TraceScope() {
+    if (idx >= 0) (void)hipEventRecord(ctx->trace_recs[size_t(idx)].stop, ctx->stream);
+}
+
+void trace_count(bolap_ctx* ctx, const char* name, int64_t n) {
+    if (!ctx || !ctx->trace) return;
+    bolap_trace_rec r;
+    std::snprintf(r.name, sizeof(r.name), "%s", name);
+    r.count = n;
+    ctx->trace_recs.push_back(r);
+}
+
 }  // namespace bolap
 
 using namespace bolap;
@@ -108,7 +143,45 @@ int bolap_ctx_destroy(bolap_ctx* ctx) {
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->result_dev) (void)hipFree(ctx->result_dev);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto& r : ctx->trace_recs) {
+        if (r.start) (void)hipEventDestroy(r.start);
+        if (r.stop) (void)hipEventDestroy(r.stop);
+    }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     delete ctx;
+    return BOLAP_OK;
+}
+
+int bolap_ctx_set_trace(bolap_ctx* ctx, int enabled) {
+    BOLAP_ARG_CHECK(ctx != nullptr, "bolap_ctx_set_trace: null ctx");
+    ctx->trace = enabled != 0;
+    return BOLAP_OK;
+}
+
+int bolap_ctx_read_trace(bolap_ctx* ctx, bolap_trace_entry* out, int32_t max_entries,
+                         int32_t* n_out) {
+    BOLAP_ARG_CHECK(ctx && n_out && (out || max_entries == 0),
+                    "bolap_ctx_read_trace: null argument");
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int32_t n = 0;
+    for (auto& r : ctx->trace_recs) {
+        if (n < max_entries) {
+            bolap_trace_entry& e = out[n];
+            std::memset(&e, 0, sizeof(e));
+            std::snprintf(e.name, sizeof(e.name), "%s", r.name);
+            e.count = r.count;
+            e.ms = -1.0;
+            if (r.start) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) e.ms = ms;
+            }
+        }
+        ++n;
+        if (r.start) ctx->event_pool.push_back(r.start);
+        if (r.stop) ctx->event_pool.push_back(r.stop);
+    }
+    ctx->trace_recs.clear();
+    *n_out = n;
     return BOLAP_OK;
 }
 

@@ -87,6 +87,17 @@ class Engine:
                 "lds_bytes_per_cu": info.lds_bytes_per_cu, "wavefront": info.wavefront,
                 "hbm_bytes": info.hbm_bytes, "arch": info.arch.decode()}
 
+    def set_trace(self, on: bool) -> None:
+        N.check(self.lib.bolap_ctx_set_trace(self.ctx, int(bool(on))))
+
+    def read_trace(self, max_entries: int = 4096) -> list[tuple[str, float, int]]:
+        """(name, ms, count) per traced stage or count, in issue order."""
+        buf = (N.TraceEntry * max_entries)()
+        n = C.c_int32(0)
+        N.check(self.lib.bolap_ctx_read_trace(self.ctx, buf, max_entries, C.byref(n)))
+        return [(buf[i].name.decode(), buf[i].ms, buf[i].count)
+                for i in range(min(n.value, max_entries))]
+
     # ------------------------------------------------------------------ Q1
     @staticmethod
     def q1_args(cols: dict, ship_lo: int, ship_hi: int, n_a: int, n_b: int,
@@ -118,6 +129,56 @@ class Engine:
     def q1_scan_agg_async(self, a: N.Q1Args, out_dev) -> None:
         N.check(self.lib.bolap_q1_scan_agg_async(self.ctx, C.byref(a),
                                                  C.c_void_p(out_dev.data_ptr())))
+
+    # ------------------------------------------------------------ join + sum
+    @staticmethod
+    def pred(col, op: int = N.OP_RANGE, lo: int = 0, hi: int = 0, values=None,
+             set_bitmap=None) -> N.Pred:
+        """Predicate on a device column: lo <= v <= hi, or v in a set given as
+        inline values (0..255) or as a device uint32 bitmap."""
+        p = N.Pred()
+        p.col = ptr(col)
+        p.dtype = dtype_code(col)
+        p.op = op
+        p.lo, p.hi = int(lo), int(hi)
+        if values is not None:
+            words = [0, 0, 0, 0]
+            for v in values:
+                words[v >> 6] |= 1 << (v & 63)
+            for i in range(4):
+                p.inline_set[i] = words[i]
+        if set_bitmap is not None:
+            p.set = ptr(set_bitmap)
+            p.set_bits = int(set_bitmap.numel()) * 32
+        return p
+
+    def join_sum_args(self, dim_key, preds: list, fact_fk, fact_price, fact_disc,
+                      capacity_hint: int = 0, placement: int = N.PLACE_AUTO) -> N.JoinSumArgs:
+        a = N.JoinSumArgs()
+        a.dim_rows = int(dim_key.numel())
+        a.dim_key = ptr(dim_key)
+        a.n_dim_preds = len(preds)
+        for i, p in enumerate(preds):
+            a.dim_preds[i] = p
+        a.fact_rows = int(fact_fk.numel())
+        a.fact_fk = ptr(fact_fk)
+        a.fact_price = ptr(fact_price)
+        a.price_dtype = dtype_code(fact_price)
+        a.fact_disc = ptr(fact_disc)
+        a.disc_dtype = dtype_code(fact_disc)
+        a.capacity_hint = int(capacity_hint)
+        a.placement = int(placement)
+        return a
+
+    def join_sum(self, *args, **kw) -> dict:
+        a = self.join_sum_args(*args, **kw)
+        return self.join_sum_run(a)
+
+    def join_sum_run(self, a: N.JoinSumArgs) -> dict:
+        r = N.JoinSumResult()
+        N.check(self.lib.bolap_join_sum(self.ctx, C.byref(a), C.byref(r)))
+        return {"revenue": r.revenue, "build_rows": r.build_rows, "matches": r.matches,
+                "capacity": r.capacity, "placement": r.placement, "fallbacks": r.fallbacks}
 
     # ------------------------------------------------------- generators
     def gen_lineitem(self, seed: int, rows: int) -> dict:

@@ -35,6 +35,10 @@ class DeviceInfo(C.Structure):
                 ("hbm_bytes", C.c_int64), ("arch", C.c_char * 32)]
 
 
+class TraceEntry(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("ms", C.c_double), ("count", C.c_int64)]
+
+
 class Pred(C.Structure):
     _fields_ = [("col", vp), ("dtype", C.c_int32), ("op", C.c_int32),
                 ("lo", C.c_int64), ("hi", C.c_int64),
@@ -103,6 +107,8 @@ PROTOTYPES = {
     "bolap_ctx_destroy": (C.c_int, [vp]),
     "bolap_ctx_sync": (C.c_int, [vp]),
     "bolap_ctx_device_info": (C.c_int, [vp, P(DeviceInfo)]),
+    "bolap_ctx_set_trace": (C.c_int, [vp, C.c_int]),
+    "bolap_ctx_read_trace": (C.c_int, [vp, P(TraceEntry), i32, P(i32)]),
     "bolap_q1_scan_agg": (C.c_int, [vp, P(Q1Args), P(Q1Group)]),
     "bolap_q1_scan_agg_async": (C.c_int, [vp, P(Q1Args), vp]),
     "bolap_join_sum": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),

@@ -77,6 +77,22 @@ int bolap_ctx_device_info(bolap_ctx* ctx, bolap_device_info* out);
 const char* bolap_last_error(void);
 int bolap_abi_version(void);
 
+/* Stage tracing: the counterpart of the reference's -DTRACE output
+   (bespoke_tpch/trace_utils.hpp:18-53, "PROFILE name ns" / "COUNT name n").
+   When enabled, every pipeline stage records a pair of HIP events on the
+   context stream and every pipeline records its row counts.  Reading the
+   trace synchronises the stream, returns the entries in issue order and
+   clears them. */
+typedef struct {
+    char name[40];
+    double ms;        /* GPU time between the stage's events; -1 for counts */
+    int64_t count;    /* row count for COUNT entries; 0 for timings */
+} bolap_trace_entry;
+
+int bolap_ctx_set_trace(bolap_ctx* ctx, int enabled);
+int bolap_ctx_read_trace(bolap_ctx* ctx, bolap_trace_entry* out, int32_t max_entries,
+                         int32_t* n_out);
+
 /* ------------------------------------------------------------------------ */
 /* Predicates (dimension-side filters).                                       */
 /* A row passes when every predicate holds (conjunction).                     */

@@ -227,31 +227,36 @@ int oracle_pred_row(const oracle_pred* p, int64_t row) {
 /* passing keys in a byte mask over the key domain                            */
 /* (bespoke_tpch/query_q19.cpp:210-250, query_q17.cpp:161-177), then every    */
 /* fact row whose key is marked adds price * (100 - disc) into one int64      */
-/* (query_q19.cpp:271-345).  Keys are unique and non-negative, as the         */
-/* reference's key-indexed masks assume.                                      */
+/* (query_q19.cpp:271-345).  A key repeated among passing dimension rows     */
+/* still matches each fact row once (the mask semantics).                     */
 /* ------------------------------------------------------------------------ */
 void oracle_join_sum(int64_t dim_rows, const int32_t* dim_key, int32_t n_preds,
                      const oracle_pred* preds, int64_t fact_rows, const int32_t* fact_fk,
                      const void* fact_price, int32_t price_dtype, const void* fact_disc,
                      int32_t disc_dtype, oracle_join_sum_out* out) {
-    int32_t max_key = 0;
-    for (int64_t i = 0; i < dim_rows; ++i)
-        if (dim_key[i] > max_key) max_key = dim_key[i];
-    uint8_t* mask = (uint8_t*)calloc((size_t)max_key + 1, 1);
+    /* The reference indexes its mask by the key itself; an offset by the
+       smallest key extends that to negative keys. */
+    int64_t min_key = 0, max_key = -1;
+    for (int64_t i = 0; i < dim_rows; ++i) {
+        if (i == 0 || dim_key[i] < min_key) min_key = dim_key[i];
+        if (i == 0 || dim_key[i] > max_key) max_key = dim_key[i];
+    }
+    const int64_t span = max_key - min_key + 1;
+    uint8_t* mask = (uint8_t*)calloc((size_t)(span > 0 ? span : 1), 1);
     int64_t build = 0;
     for (int64_t i = 0; i < dim_rows; ++i) {
         int pass = 1;
         for (int32_t p = 0; p < n_preds && pass; ++p) pass = oracle_pred_row(&preds[p], i);
-        if (pass && dim_key[i] >= 0) {
-            mask[dim_key[i]] = 1;
+        if (pass) {
+            mask[dim_key[i] - min_key] = 1;
             ++build;
         }
     }
     uint64_t revenue = 0;
     int64_t matches = 0;
     for (int64_t i = 0; i < fact_rows; ++i) {
-        const int32_t k = fact_fk[i];
-        if (k < 0 || k > max_key || !mask[k]) continue;
+        const int64_t k = fact_fk[i];
+        if (k < min_key || k > max_key || !mask[k - min_key]) continue;
         const int64_t p = load_col(fact_price, price_dtype, i);
         const int64_t d = load_col(fact_disc, disc_dtype, i);
         const int64_t factor = (g_mutation == 2) ? d : (100 - d);
@@ -272,6 +277,7 @@ void oracle_join_sum(int64_t dim_rows, const int32_t* dim_key, int32_t n_preds,
 /* orders' contiguous fact ranges; for fact tables in any order this uses the */
 /* dense per-key array of run_q18 (query_q18.cpp:221-255).  Groups come out   */
 /* in order-row order; orders without fact rows are dropped (:244-246).       */
+/* Order keys are unique; customer keys may repeat (set semantics).          */
 /* ------------------------------------------------------------------------ */
 int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
                             const uint8_t* cust_segment, int32_t segment, int64_t order_rows,
@@ -281,39 +287,44 @@ int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
                             int32_t price_dtype, const void* fact_disc, int32_t disc_dtype,
                             const int32_t* fact_disc_price, oracle_group_row* out,
                             int64_t max_out, oracle_chain_stats* stats) {
-    int32_t max_cust = 0;
-    for (int64_t i = 0; i < cust_rows; ++i)
-        if (cust_key[i] > max_cust) max_cust = cust_key[i];
-    uint8_t* in_seg = (uint8_t*)calloc((size_t)max_cust + 1, 1);
+    /* Key-indexed arrays as in the reference, offset by the smallest key. */
+    int64_t cmin = 0, cmax = -1;
+    for (int64_t i = 0; i < cust_rows; ++i) {
+        if (i == 0 || cust_key[i] < cmin) cmin = cust_key[i];
+        if (i == 0 || cust_key[i] > cmax) cmax = cust_key[i];
+    }
+    uint8_t* in_seg = (uint8_t*)calloc((size_t)(cmax >= cmin ? cmax - cmin + 1 : 1), 1);
     int64_t cust_pass = 0;
     for (int64_t i = 0; i < cust_rows; ++i) {
-        if (cust_segment[i] == segment && cust_key[i] >= 0) {
-            in_seg[cust_key[i]] = 1;
+        if (cust_segment[i] == segment) {
+            in_seg[cust_key[i] - cmin] = 1;
             ++cust_pass;
         }
     }
-    int32_t max_order = 0;
-    for (int64_t i = 0; i < order_rows; ++i)
-        if (order_key[i] > max_order) max_order = order_key[i];
-    int32_t* row_of = (int32_t*)malloc(sizeof(int32_t) * ((size_t)max_order + 1));
-    for (int64_t k = 0; k <= max_order; ++k) row_of[k] = -1;
+    int64_t omin = 0, omax = -1;
+    for (int64_t i = 0; i < order_rows; ++i) {
+        if (i == 0 || order_key[i] < omin) omin = order_key[i];
+        if (i == 0 || order_key[i] > omax) omax = order_key[i];
+    }
+    const int64_t ospan = omax >= omin ? omax - omin + 1 : 1;
+    int32_t* row_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)ospan);
+    for (int64_t k = 0; k < ospan; ++k) row_of[k] = -1;
     int64_t order_pass = 0;
     const int32_t day_limit = (g_mutation == 3) ? day_lt + 1 : day_lt;
     for (int64_t o = 0; o < order_rows; ++o) {
-        const int32_t c = order_cust[o];
+        const int64_t c = order_cust[o];
         if (order_day[o] >= day_limit) continue;
-        if (c < 0 || c > max_cust || !in_seg[c]) continue;
-        if (order_key[o] < 0) continue;
-        row_of[order_key[o]] = (int32_t)o;
+        if (c < cmin || c > cmax || !in_seg[c - cmin]) continue;
+        row_of[order_key[o] - omin] = (int32_t)o;
         ++order_pass;
     }
     uint64_t* rev = (uint64_t*)calloc((size_t)order_rows + 1, sizeof(uint64_t));
     int32_t* cnt = (int32_t*)calloc((size_t)order_rows + 1, sizeof(int32_t));
     int64_t matches = 0;
     for (int64_t i = 0; i < fact_rows; ++i) {
-        const int32_t k = fact_order[i];
-        if (k < 0 || k > max_order) continue;
-        const int32_t o = row_of[k];
+        const int64_t k = fact_order[i];
+        if (k < omin || k > omax) continue;
+        const int32_t o = row_of[k - omin];
         if (o < 0) continue;
         int64_t v;
         if (fact_disc_price) {

@@ -1,0 +1,137 @@
+"""Filtered hash join + scalar SUM on the GPU vs the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle
+from bespokeolap_artifacts_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+SEED = 42
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def run_both(engine, d, f, cat, conts, **kw):
+    preds = [engine.pred(d["cat"], N.OP_RANGE, cat[0], cat[1]),
+             engine.pred(d["cont"], N.OP_IN_SET, values=conts)]
+    got = engine.join_sum(d["key"], preds, f["fk"], f["price"], f["disc"], **kw)
+    hd = {k: host(v) for k, v in d.items()}
+    opreds = [oracle.make_pred(hd["cat"], 0, cat[0], cat[1]),
+              oracle.make_pred(hd["cont"], 1, values=conts)]
+    want = oracle.join_sum(hd["key"], opreds, host(f["fk"]), host(f["price"]), host(f["disc"]))
+    return got, want
+
+
+def same(got, want):
+    return all(got[k] == want[k] for k in ("revenue", "build_rows", "matches"))
+
+
+@pytest.mark.parametrize("dim_rows,fact_rows", [(1, 1), (10, 7), (5000, 100_003),
+                                                (200_000, 3_000_001)])
+def test_join_sum_parity(engine, dim_rows, fact_rows):
+    d = engine.gen_dim(SEED, dim_rows)
+    f = engine.gen_fact(SEED, fact_rows, dim_rows)
+    for cat, conts in (((7, 7), [3, 17, 29]), ((0, 149), list(range(40))), ((0, 10), [5])):
+        got, want = run_both(engine, d, f, cat, conts)
+        assert same(got, want), (cat, got, want)
+        assert got["fallbacks"] == 0
+
+
+def test_join_sum_full_c2_size(engine):
+    d = engine.gen_dim(SEED, 2_000_000)
+    f = engine.gen_fact(SEED, 60_000_000, 2_000_000)
+    got, want = run_both(engine, d, f, (7, 7), [3, 17, 29])
+    assert same(got, want)
+    assert got["placement"] == N.PLACE_LDS
+    assert 700 < want["build_rows"] < 1300
+
+
+@pytest.mark.parametrize("placement", [N.PLACE_LDS, N.PLACE_HBM])
+def test_join_sum_placements_agree(engine, placement):
+    d = engine.gen_dim(SEED, 100_000)
+    f = engine.gen_fact(SEED, 1_000_000, 100_000)
+    got, want = run_both(engine, d, f, (3, 40), [1, 2, 3, 4, 5], placement=placement)
+    assert same(got, want) and got["placement"] == placement
+
+
+def test_join_sum_large_build_goes_to_hbm(engine):
+    d = engine.gen_dim(SEED, 2_000_000)
+    f = engine.gen_fact(SEED, 5_000_000, 2_000_000)
+    got, want = run_both(engine, d, f, (0, 149), list(range(40)))
+    assert same(got, want) and got["placement"] == N.PLACE_HBM
+    assert want["build_rows"] == 2_000_000
+
+
+@pytest.mark.parametrize("hint", [1, 10, 100, 1000, 10_000, 10 ** 6])
+def test_join_sum_capacity_hints_never_change_results(engine, hint):
+    d = engine.gen_dim(SEED, 400_000)
+    f = engine.gen_fact(SEED, 2_000_000, 400_000)
+    got, want = run_both(engine, d, f, (7, 30), [3, 17, 29], capacity_hint=hint)
+    assert same(got, want)
+    true_build = want["build_rows"]
+    # Under-estimates beyond the table's slack fall back to exact sizing.
+    if hint * 2 < true_build:
+        assert got["fallbacks"] == 1
+    if hint >= true_build:
+        assert got["fallbacks"] == 0
+
+
+def test_join_sum_narrow_columns_and_bitmap_predicate(engine):
+    import torch
+    d = engine.gen_dim(SEED, 50_000)
+    f = engine.gen_fact(SEED, 500_000, 50_000)
+    bitmap = np.zeros(8, np.uint32)
+    for v in (3, 17, 29, 200):
+        bitmap[v >> 5] |= np.uint32(1 << (v & 31))
+    dbm = torch.from_numpy(bitmap.view(np.int32)).cuda()
+    price32 = f["price"].to(torch.int32)
+    disc8 = f["disc"].to(torch.uint8)
+    preds = [engine.pred(d["cat"], N.OP_RANGE, 0, 74),
+             engine.pred(d["cont"], N.OP_IN_SET, set_bitmap=dbm)]
+    got = engine.join_sum(d["key"], preds, f["fk"], price32, disc8)
+    hd = {k: host(v) for k, v in d.items()}
+    opreds = [oracle.make_pred(hd["cat"], 0, 0, 74),
+              oracle.make_pred(hd["cont"], 1, set_bitmap=bitmap)]
+    want = oracle.join_sum(hd["key"], opreds, host(f["fk"]), host(price32), host(disc8))
+    assert same(got, want)
+
+
+def test_join_sum_negative_min_and_duplicate_keys(engine):
+    import torch
+    rng = np.random.default_rng(11)
+    keys = rng.integers(-5000, 5000, 20_000).astype(np.int32)
+    keys[:5] = np.iinfo(np.int32).min  # the table's empty marker is a real key here
+    cat = rng.integers(0, 4, 20_000).astype(np.uint8)
+    fk = rng.integers(-6000, 6000, 300_001).astype(np.int32)
+    fk[::1000] = np.iinfo(np.int32).min
+    price = rng.integers(1, 10 ** 6, 300_001).astype(np.int64)
+    disc = rng.integers(0, 11, 300_001).astype(np.int64)
+    dk, dc = torch.from_numpy(keys).cuda(), torch.from_numpy(cat).cuda()
+    ff, fp, fd = (torch.from_numpy(x).cuda() for x in (fk, price, disc))
+    for lo, hi in ((0, 0), (0, 3), (2, 3)):
+        got = engine.join_sum(dk, [engine.pred(dc, N.OP_RANGE, lo, hi)], ff, fp, fd)
+        want = oracle.join_sum(keys, [oracle.make_pred(cat, 0, lo, hi)], fk, price, disc)
+        assert same(got, want), (lo, hi)
+
+
+def test_join_sum_empty_inputs(engine):
+    import torch
+    d = engine.gen_dim(SEED, 1000)
+    f = engine.gen_fact(SEED, 1000, 1000)
+    e32 = torch.empty(0, dtype=torch.int32, device="cuda")
+    e64 = torch.empty(0, dtype=torch.int64, device="cuda")
+    got = engine.join_sum(d["key"], [engine.pred(d["cat"], N.OP_RANGE, 0, 149)], e32, e64, e64)
+    assert got["revenue"] == 0 and got["matches"] == 0 and got["build_rows"] == 1000
+    got = engine.join_sum(d["key"], [engine.pred(d["cat"], N.OP_RANGE, 200, 300)],
+                          f["fk"], f["price"], f["disc"])
+    assert got["revenue"] == 0 and got["build_rows"] == 0
+
+
+def test_join_sum_rejects_misaligned_fact_keys(engine):
+    d = engine.gen_dim(SEED, 100)
+    f = engine.gen_fact(SEED, 101, 100)
+    with pytest.raises(N.BolapError):
+        engine.join_sum(d["key"], [], f["fk"][1:], f["price"][1:], f["disc"][1:])
