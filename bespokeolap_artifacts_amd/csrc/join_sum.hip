@@ -12,12 +12,14 @@
 //   init   fills the table image with EMPTY, clears bitmap and result slot.
 //   build  predicate pass + open-addressing insert (atomicCAS, linear
 //          probing) of surviving keys into an HBM table image, plus a
-//          one-hash bitmap prefilter; an overflow flag is raised when the
+//          blocked Bloom filter (2 bits in one 32-bit word per key); an
+//          overflow flag is raised when the
 //          distinct keys exceed half the slots (estimate-sized tables).
-//   probe  streams the fact key column with 128-bit loads; per key one
-//          bitmap test, and only on a bitmap hit a linear probe of the
-//          table; bitmap and table are staged in LDS when they fit (one
-//          copy per workgroup) and read through L2 otherwise.  Matches
+//   probe  streams the fact key column with 128-bit loads; for the 16 keys
+//          a lane holds it first issues all 16 Bloom-word reads, forms a
+//          hit mask, and probes the table only for Bloom hits; filter and
+//          table are staged in LDS when they fit (one copy per workgroup)
+//          and read through L2 otherwise.  Matches
 //          gather price/disc and accumulate per lane; DPP wave sums, LDS,
 //          per-block partials and a last-block merge give the result.
 #include <cstring>
@@ -49,8 +51,15 @@ struct Table {
 BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
     return (uint32_t(key) * kSlotMul) >> (32 - cap_log2);
 }
-BOLAP_DEV uint32_t bit_of(int32_t key, int bb_log2) {
-    return (uint32_t(key) * kBitMul) >> (32 - bb_log2);
+// Blocked Bloom filter: one 32-bit word per key (chosen by the top bits of
+// a second multiplicative hash) with two bits set from middle bits of the
+// slot hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
+BOLAP_DEV uint32_t bloom_word(int32_t key, int bb_log2) {
+    return (uint32_t(key) * kBitMul) >> (32 - (bb_log2 - 5));
+}
+BOLAP_DEV uint32_t bloom_mask(int32_t key) {
+    const uint32_t h = uint32_t(key) * kSlotMul;
+    return (1u << ((h >> 7) & 31)) | (1u << ((h >> 12) & 31));
 }
 
 __global__ __launch_bounds__(256) void zero_res_k(int64_t* res) {
@@ -99,8 +108,7 @@ __global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows
             atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
             continue;
         }
-        const uint32_t b = bit_of(key, t.bb_log2);
-        atomicOr(t.bits + (b >> 5), 1u << (b & 31));
+        atomicOr(t.bits + bloom_word(key, t.bb_log2), bloom_mask(key));
         uint32_t s = slot_of(key, t.cap_log2);
         bool placed = false;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -123,33 +131,23 @@ __global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows
     }
 }
 
+// Exact membership test of a key that passed the Bloom filter.
+BOLAP_DEV bool table_has(int32_t key, const int32_t* T, int cap_log2, uint32_t mask) {
+    uint32_t s = slot_of(key, cap_log2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t t = T[s];
+        if (t == key) return true;
+        if (t == kEmpty) return false;
+        s = (s + 1) & mask;
+    }
+    return false;
+}
+
 template <typename TP, typename TD>
-BOLAP_DEV void probe_key(int32_t key, bool valid, int64_t row, bool has_min,
-                         const uint32_t* B, const int32_t* T, int bb_log2, int cap_log2,
-                         uint32_t mask, const TP* __restrict__ price,
-                         const TD* __restrict__ disc, int64_t& rev, uint32_t& cnt) {
-    bool found = false;
-    if (key == kEmpty) {
-        found = valid && has_min;
-    } else {
-        const uint32_t b = bit_of(key, bb_log2);
-        if (valid && ((B[b >> 5] >> (b & 31)) & 1u)) {
-            uint32_t s = slot_of(key, cap_log2);
-            for (uint32_t probe = 0; probe <= mask; ++probe) {
-                const int32_t t = T[s];
-                if (t == key) {
-                    found = true;
-                    break;
-                }
-                if (t == kEmpty) break;
-                s = (s + 1) & mask;
-            }
-        }
-    }
-    if (found) {
-        rev += int64_t(uint64_t(int64_t(price[row])) * uint64_t(100 - int64_t(disc[row])));
-        cnt += 1;
-    }
+BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __restrict__ disc,
+                         int64_t& rev, uint32_t& cnt) {
+    rev += int64_t(uint64_t(int64_t(price[row])) * uint64_t(100 - int64_t(disc[row])));
+    cnt += 1;
 }
 
 constexpr int kProbeU = 4;  // 4-key units in flight per lane
@@ -188,20 +186,43 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
             u = ok[k] ? u : n_units - 1;
             v[k] = load_unit<int32_t, 4>(fk, u);
         }
+        // Phase 1: all Bloom words first (independent LDS reads), then a
+        // per-lane hit mask over the kProbeU * 4 keys.
+        constexpr int kKeys = kProbeU * 4;
+        uint32_t wv[kKeys];
 #pragma unroll
-        for (int k = 0; k < kProbeU; ++k) {
-            const int64_t u = base + int64_t(k) * kThreads;
+        for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(v[i / 4].v[i % 4], t.bb_log2)];
+        uint32_t hits = 0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                probe_key<TP, TD>(v[k].v[r], ok[k], u * 4 + r, has_min, B, T, t.bb_log2,
-                                  t.cap_log2, mask, price, disc, rev, cnt);
+        for (int i = 0; i < kKeys; ++i) {
+            const int32_t key = v[i / 4].v[i % 4];
+            const uint32_t m = bloom_mask(key);
+            const bool h = ok[i / 4] && (key == kEmpty ? has_min : (wv[i] & m) == m);
+            hits |= (h ? 1u : 0u) << i;
+        }
+        // Phase 2: exact probes only for Bloom hits (rare, divergent).
+        if (hits) {
+#pragma unroll
+            for (int i = 0; i < kKeys; ++i) {
+                if (hits & (1u << i)) {
+                    const int32_t key = v[i / 4].v[i % 4];
+                    const int64_t row = (base + int64_t(i / 4) * kThreads) * 4 + (i % 4);
+                    if (key == kEmpty || table_has(key, T, t.cap_log2, mask))
+                        add_match<TP, TD>(row, price, disc, rev, cnt);
+                }
+            }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
         const int64_t row = n_units * 4 + threadIdx.x;
-        if (row < rows)
-            probe_key<TP, TD>(fk[row], true, row, has_min, B, T, t.bb_log2, t.cap_log2, mask,
-                              price, disc, rev, cnt);
+        if (row < rows) {
+            const int32_t key = fk[row];
+            const uint32_t m = bloom_mask(key);
+            const bool h = key == kEmpty ? has_min
+                                         : ((B[bloom_word(key, t.bb_log2)] & m) == m &&
+                                            table_has(key, T, t.cap_log2, mask));
+            if (h) add_match<TP, TD>(row, price, disc, rev, cnt);
+        }
     }
 
     __shared__ int64_t s_rev[kWaves];
@@ -263,12 +284,12 @@ struct Sizing {
 };
 
 // Table geometry for `n` expected build keys: LDS tables run at load <= 1/4
-// with a 64-bit-per-key bitmap; HBM tables at load <= 1/2 with 16 bits per
-// key.  Either overflows (and the caller re-runs exactly) past `cap / 2`.
+// with a 128-bit-per-key Bloom filter; HBM tables at load <= 1/2 with 16 bits
+// per key.  Either overflows (and the caller re-runs exactly) past `cap / 2`.
 int size_table(int64_t n, int placement, Sizing& z) {
     if (n < 1) n = 1;
     const int lds_cap = ceil_log2(n * 4 < 64 ? 64 : n * 4);
-    int lds_bb = ceil_log2(n * 64);
+    int lds_bb = ceil_log2(n * 128);
     if (lds_bb < 10) lds_bb = 10;
     if (lds_bb > 18) lds_bb = 18;
     const size_t lds_bytes = (size_t(1) << lds_cap) * 4 + (size_t(1) << lds_bb) / 8;
