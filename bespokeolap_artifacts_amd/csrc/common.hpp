@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,13 @@ struct bolap_ctx {
     // Pinned host staging for results copied back at the end of a pipeline.
     void* pinned = nullptr;
     size_t pinned_size = 0;
+
+    // Tuning knobs (bolap_tune_set).
+    std::map<std::string, int64_t> knobs;
+    int64_t knob(const char* name, int64_t dflt = 0) const {
+        auto it = knobs.find(name);
+        return (it == knobs.end() || it->second == 0) ? dflt : it->second;
+    }
 
     // Stage tracing (bolap_ctx_set_trace).
     bool trace = false;

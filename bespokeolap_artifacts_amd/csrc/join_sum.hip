@@ -150,9 +150,7 @@ BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __
     cnt += 1;
 }
 
-constexpr int kProbeU = 4;  // 4-key units in flight per lane
-
-template <bool kLds, typename TP, typename TD>
+template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
 __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
     int64_t rows, Table t, int64_t* res, int64_t* partials, unsigned* counter) {
@@ -175,29 +173,31 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
     uint32_t cnt = 0;
     const int64_t n_units = rows / 4;
     const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
-    for (int64_t base = int64_t(blockIdx.x) * kThreads * kProbeU + threadIdx.x; base < n_units;
-         base += stride) {
+    struct Tile {
         VecT<int32_t, 4> v[kProbeU];
-        bool ok[kProbeU];
+    };
+    auto load_tile = [&](Tile& x, int64_t at) {
 #pragma unroll
         for (int k = 0; k < kProbeU; ++k) {
-            int64_t u = base + int64_t(k) * kThreads;
-            ok[k] = u < n_units;
-            u = ok[k] ? u : n_units - 1;
-            v[k] = load_unit<int32_t, 4>(fk, u);
+            int64_t u = at + int64_t(k) * kThreads;
+            u = u < n_units ? u : n_units - 1;  // in bounds; masked when consumed
+            x.v[k] = load_unit<int32_t, 4>(fk, u);
         }
+    };
+    auto use_tile = [&](const Tile& x, int64_t at) {
         // Phase 1: all Bloom words first (independent LDS reads), then a
         // per-lane hit mask over the kProbeU * 4 keys.
         constexpr int kKeys = kProbeU * 4;
         uint32_t wv[kKeys];
 #pragma unroll
-        for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(v[i / 4].v[i % 4], t.bb_log2)];
+        for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(x.v[i / 4].v[i % 4], t.bb_log2)];
         uint32_t hits = 0;
 #pragma unroll
         for (int i = 0; i < kKeys; ++i) {
-            const int32_t key = v[i / 4].v[i % 4];
+            const int32_t key = x.v[i / 4].v[i % 4];
             const uint32_t m = bloom_mask(key);
-            const bool h = ok[i / 4] && (key == kEmpty ? has_min : (wv[i] & m) == m);
+            const bool ok = at + int64_t(i / 4) * kThreads < n_units;
+            const bool h = ok && (key == kEmpty ? has_min : (wv[i] & m) == m);
             hits |= (h ? 1u : 0u) << i;
         }
         // Phase 2: exact probes only for Bloom hits (rare, divergent).
@@ -205,12 +205,35 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
 #pragma unroll
             for (int i = 0; i < kKeys; ++i) {
                 if (hits & (1u << i)) {
-                    const int32_t key = v[i / 4].v[i % 4];
-                    const int64_t row = (base + int64_t(i / 4) * kThreads) * 4 + (i % 4);
+                    const int32_t key = x.v[i / 4].v[i % 4];
+                    const int64_t row = (at + int64_t(i / 4) * kThreads) * 4 + (i % 4);
                     if (key == kEmpty || table_has(key, T, t.cap_log2, mask))
                         add_match<TP, TD>(row, price, disc, rev, cnt);
                 }
             }
+        }
+    };
+    int64_t base = int64_t(blockIdx.x) * kThreads * kProbeU + threadIdx.x;
+    if constexpr (!kPipe) {
+        for (; base < n_units; base += stride) {
+            Tile A;
+            load_tile(A, base);
+            use_tile(A, base);
+        }
+    } else if (n_units > 0 && base < n_units) {
+        // Two register buffers: the next tile's keys are in flight while
+        // the current tile is probed.
+        Tile A, Bt;
+        load_tile(A, base);
+        for (;;) {
+            load_tile(Bt, base + stride);
+            use_tile(A, base);
+            base += stride;
+            if (base >= n_units) break;
+            load_tile(A, base + stride);
+            use_tile(Bt, base);
+            base += stride;
+            if (base >= n_units) break;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
@@ -314,10 +337,10 @@ int size_table(int64_t n, int placement, Sizing& z) {
     return BOLAP_OK;
 }
 
-template <bool kLds, typename TP, typename TD>
-int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
-                 int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
-    auto kern = probe_sum_k<kLds, TP, TD>;
+template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
+int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+    auto kern = probe_sum_k<kLds, TP, TD, kProbeU, kPipe>;
     if (kLds) {
         BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -326,7 +349,7 @@ int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, s
     int per_cu = 0;
     BOLAP_HIP_TRY(
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, kLds ? lds : 0));
-    int64_t blocks = int64_t(ctx->num_cus) * (per_cu > 0 ? per_cu : 1);
+    int64_t blocks = int64_t(ctx->num_cus) * ctx->knob("join.blocks_per_cu", per_cu > 0 ? per_cu : 1);
     const int64_t need = (a->fact_rows / 4 + int64_t(kThreads) * kProbeU - 1) /
                          (int64_t(kThreads) * kProbeU);
     if (blocks > need) blocks = need;
@@ -338,6 +361,19 @@ int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, s
                        static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
                        ws.at<int64_t>(o_part), ctx->counters + 1);
     return launch_check("probe_sum_k");
+}
+
+// Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,
+// 3 pipelined U=2, 4 pipelined U=4 (4-key units per lane per tile).
+template <bool kLds, typename TP, typename TD>
+int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+                 int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+    switch (ctx->knob("join.probe_variant", 4)) {
+        case 1: return launch_probe_v<kLds, TP, TD, 4, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+        case 2: return launch_probe_v<kLds, TP, TD, 8, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+        case 3: return launch_probe_v<kLds, TP, TD, 2, true>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+        default: return launch_probe_v<kLds, TP, TD, 4, true>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+    }
 }
 
 template <bool kLds>

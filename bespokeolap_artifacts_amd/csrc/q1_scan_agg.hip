@@ -14,8 +14,8 @@
 //    so the widest column is read with one 128-bit load per unit and every
 //    other column with one narrower load of the same R rows; each load
 //    instruction of a wave covers a contiguous span (fully coalesced).
-//  * U units per lane are loaded before any is consumed (U*R rows of every
-//    column in flight per lane).
+//  * Tiles of U units per lane are double-buffered in registers: the next
+//    tile's loads are in flight while the current one is aggregated.
 //  * The predicate and the group select stay in registers: G group slots
 //    (compile-time) of per-lane accumulators updated with selects, so there
 //    are no atomics and no LDS traffic in the loop.
@@ -30,8 +30,6 @@
 namespace bolap {
 namespace {
 
-constexpr int kQ1Threads = 512;
-constexpr int kQ1Waves = kQ1Threads / kWave;
 constexpr int kFields = 8;  // qty, price, disc_price, charge lo, charge hi, disc, count, pad
 
 struct Q1K {
@@ -102,10 +100,11 @@ BOLAP_DEV void q1_row(Q1Lane<G>& acc, const Q1K& a, bool valid, int64_t s, int64
     }
 }
 
-template <class L, int R, int U, int G>
+template <class L, int R, int U, int G, bool kPipe, int kQ1Threads>
 __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials,
                                                                   unsigned* counter,
                                                                   bolap_q1_group* out) {
+    constexpr int kQ1Waves = kQ1Threads / kWave;
     using S = typename L::S; using Q = typename L::Q; using P = typename L::P;
     using DP = typename L::DP; using D = typename L::D; using T = typename L::T;
     const S* __restrict__ ship = static_cast<const S*>(a.ship);
@@ -126,40 +125,66 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         acc.cnt[g] = 0;
     }
 
+    // Software-pipelined loop over tiles of U units per lane: the loads of
+    // tile i+1 are issued before tile i is consumed (two register buffers),
+    // so every wave keeps a tile in flight while it computes.
     const int64_t n_units = a.rows / R;
     const int64_t stride = int64_t(gridDim.x) * kQ1Threads * U;
-    for (int64_t base = int64_t(blockIdx.x) * kQ1Threads * U + threadIdx.x; base < n_units;
-         base += stride) {
-        VecT<S, R> vs[U];
-        VecT<Q, R> vq[U];
-        VecT<P, R> vp[U];
-        VecT<DP, R> vdp[U];
-        VecT<D, R> vd[U];
-        VecT<T, R> vt[U];
-        VecT<uint8_t, R> va[U], vb[U];
-        bool valid[U];
+    struct Tile {
+        VecT<S, R> s[U];
+        VecT<Q, R> q[U];
+        VecT<P, R> p[U];
+        VecT<DP, R> dp[U];
+        VecT<D, R> d[U];
+        VecT<T, R> t[U];
+        VecT<uint8_t, R> fa[U], fb[U];
+    };
+    auto load_tile = [&](Tile& x, int64_t at) {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-            int64_t u = base + int64_t(k) * kQ1Threads;
-            valid[k] = u < n_units;
-            u = valid[k] ? u : n_units - 1;  // in-bounds; contribution masked
-            vs[k] = load_unit<S, R>(ship, u);
-            vq[k] = load_unit<Q, R>(qty, u);
-            vp[k] = load_unit<P, R>(price, u);
-            if constexpr (L::kHasDP) vdp[k] = load_unit<DP, R>(dpc, u);
-            vd[k] = load_unit<D, R>(disc, u);
-            vt[k] = load_unit<T, R>(tax, u);
-            va[k] = load_unit<uint8_t, R>(fa, u);
-            if constexpr (L::kTwoFlags) vb[k] = load_unit<uint8_t, R>(fb, u);
+            int64_t u = at + int64_t(k) * kQ1Threads;
+            u = u < n_units ? u : n_units - 1;  // in bounds; masked when consumed
+            x.s[k] = load_unit<S, R>(ship, u);
+            x.q[k] = load_unit<Q, R>(qty, u);
+            x.p[k] = load_unit<P, R>(price, u);
+            if constexpr (L::kHasDP) x.dp[k] = load_unit<DP, R>(dpc, u);
+            x.d[k] = load_unit<D, R>(disc, u);
+            x.t[k] = load_unit<T, R>(tax, u);
+            x.fa[k] = load_unit<uint8_t, R>(fa, u);
+            if constexpr (L::kTwoFlags) x.fb[k] = load_unit<uint8_t, R>(fb, u);
         }
+        asm volatile("" ::: "memory");  // keep the loads ahead of the compute
+    };
+    auto use_tile = [&](const Tile& x, int64_t at) {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
+            const bool valid = at + int64_t(k) * kQ1Threads < n_units;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                q1_row<L, G>(acc, a, valid[k], vs[k].v[r], vq[k].v[r], vp[k].v[r],
-                             L::kHasDP ? int64_t(vdp[k].v[r]) : 0, vd[k].v[r], vt[k].v[r],
-                             va[k].v[r], L::kTwoFlags ? uint32_t(vb[k].v[r]) : 0u);
-            }
+            for (int r = 0; r < R; ++r)
+                q1_row<L, G>(acc, a, valid, x.s[k].v[r], x.q[k].v[r], x.p[k].v[r],
+                             L::kHasDP ? int64_t(x.dp[k].v[r]) : 0, x.d[k].v[r], x.t[k].v[r],
+                             x.fa[k].v[r], L::kTwoFlags ? uint32_t(x.fb[k].v[r]) : 0u);
+        }
+    };
+    int64_t base = int64_t(blockIdx.x) * kQ1Threads * U + threadIdx.x;
+    if constexpr (!kPipe) {
+        for (; base < n_units; base += stride) {
+            Tile A;
+            load_tile(A, base);
+            use_tile(A, base);
+        }
+    } else if (n_units > 0 && base < n_units) {
+        Tile A, B;
+        load_tile(A, base);
+        for (;;) {
+            load_tile(B, base + stride);
+            use_tile(A, base);
+            base += stride;
+            if (base >= n_units) break;
+            load_tile(A, base + stride);
+            use_tile(B, base);
+            base += stride;
+            if (base >= n_units) break;
         }
     }
     // Tail rows (rows % R) go to block 0.
@@ -265,18 +290,21 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
     if (threadIdx.x == 0) st_agent(counter, 0u);
 }
 
-// Units in flight per lane.
-constexpr int kQ1U = 4;
+// Kernel variants (knob "q1.variant"): 1..3 plain loop with U = 1, 2, 4
+// units per lane; 4..5 software-pipelined loop with U = 1, 2.  All use
+// 256-thread workgroups; "q1.blocks_per_cu" caps the persistent grid.
+constexpr int kQ1Threads = 256;
 
-template <class L, int R, int G>
-int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+template <class L, int R, int G, int U, bool kPipe>
+int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    auto kern = q1_scan_agg_kernel<L, R, U, G, kPipe, kQ1Threads>;
     const int64_t n_units = k.rows / R;
-    const int64_t tile = int64_t(kQ1Threads) * kQ1U;
+    const int64_t tile = int64_t(kQ1Threads) * U;
     int64_t blocks = (n_units + tile - 1) / tile;
     int per_cu = 0;
-    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, q1_scan_agg_kernel<L, R, kQ1U, G>, kQ1Threads, 0));
-    const int64_t max_blocks = int64_t(ctx->num_cus) * (per_cu > 0 ? per_cu : 1);
+    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kQ1Threads, 0));
+    const int64_t want = ctx->knob("q1.blocks_per_cu", per_cu > 0 ? per_cu : 1);
+    const int64_t max_blocks = int64_t(ctx->num_cus) * want;
     if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
     WsArena plan;
@@ -284,10 +312,21 @@ int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     BOLAP_TRY(ws_reserve(ctx, plan.off));
     plan.base = static_cast<char*>(ctx->ws);
     TraceScope ts(ctx, "q1_scan");
-    hipLaunchKernelGGL((q1_scan_agg_kernel<L, R, kQ1U, G>), dim3(unsigned(blocks)),
-                       dim3(kQ1Threads), 0, ctx->stream, k, plan.at<int64_t>(o_part),
-                       ctx->counters + 0, out_dev);
+    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kQ1Threads), 0, ctx->stream, k,
+                       plan.at<int64_t>(o_part), ctx->counters + 0, out_dev);
     return launch_check("q1_scan_agg_kernel");
+}
+
+template <class L, int R, int G>
+int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    if (R == 1) return launch_q1_v<L, R, G, 2, false>(ctx, k, out_dev);  // unaligned columns
+    switch (ctx->knob("q1.variant", 5)) {
+        case 1: return launch_q1_v<L, R, G, 1, false>(ctx, k, out_dev);
+        case 2: return launch_q1_v<L, R, G, 2, false>(ctx, k, out_dev);
+        case 3: return launch_q1_v<L, R, G, 4, false>(ctx, k, out_dev);
+        case 4: return launch_q1_v<L, R, G, 1, true>(ctx, k, out_dev);
+        default: return launch_q1_v<L, R, G, 2, true>(ctx, k, out_dev);
+    }
 }
 
 template <class L, int R>

@@ -185,6 +185,19 @@ int bolap_ctx_read_trace(bolap_ctx* ctx, bolap_trace_entry* out, int32_t max_ent
     return BOLAP_OK;
 }
 
+int bolap_tune_set(bolap_ctx* ctx, const char* knob, int64_t value) {
+    BOLAP_ARG_CHECK(ctx && knob && knob[0], "bolap_tune_set: null argument");
+    ctx->knobs[knob] = value;
+    return BOLAP_OK;
+}
+
+int bolap_tune_get(bolap_ctx* ctx, const char* knob, int64_t* value) {
+    BOLAP_ARG_CHECK(ctx && knob && value, "bolap_tune_get: null argument");
+    auto it = ctx->knobs.find(knob);
+    *value = it == ctx->knobs.end() ? 0 : it->second;
+    return BOLAP_OK;
+}
+
 int bolap_ctx_sync(bolap_ctx* ctx) {
     BOLAP_ARG_CHECK(ctx != nullptr, "bolap_ctx_sync: null ctx");
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));

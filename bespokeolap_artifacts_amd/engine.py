@@ -87,6 +87,14 @@ class Engine:
                 "lds_bytes_per_cu": info.lds_bytes_per_cu, "wavefront": info.wavefront,
                 "hbm_bytes": info.hbm_bytes, "arch": info.arch.decode()}
 
+    def tune_set(self, knob: str, value: int) -> None:
+        N.check(self.lib.bolap_tune_set(self.ctx, knob.encode(), int(value)))
+
+    def tune_get(self, knob: str) -> int:
+        v = C.c_int64(0)
+        N.check(self.lib.bolap_tune_get(self.ctx, knob.encode(), C.byref(v)))
+        return v.value
+
     def set_trace(self, on: bool) -> None:
         N.check(self.lib.bolap_ctx_set_trace(self.ctx, int(bool(on))))
 

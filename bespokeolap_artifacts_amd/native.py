@@ -109,6 +109,8 @@ PROTOTYPES = {
     "bolap_ctx_device_info": (C.c_int, [vp, P(DeviceInfo)]),
     "bolap_ctx_set_trace": (C.c_int, [vp, C.c_int]),
     "bolap_ctx_read_trace": (C.c_int, [vp, P(TraceEntry), i32, P(i32)]),
+    "bolap_tune_set": (C.c_int, [vp, C.c_char_p, i64]),
+    "bolap_tune_get": (C.c_int, [vp, C.c_char_p, P(i64)]),
     "bolap_q1_scan_agg": (C.c_int, [vp, P(Q1Args), P(Q1Group)]),
     "bolap_q1_scan_agg_async": (C.c_int, [vp, P(Q1Args), vp]),
     "bolap_join_sum": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),

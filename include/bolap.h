@@ -90,6 +90,13 @@ typedef struct {
 } bolap_trace_entry;
 
 int bolap_ctx_set_trace(bolap_ctx* ctx, int enabled);
+
+/* Tuning knobs (kernel variant, tile size, workgroups per CU) read by the
+   pipelines at launch; 0 or unset = the library's default choice.  The
+   autotuner writes them; no reference counterpart (the reference hard-codes
+   its constants, e.g. the 16-bit radix of bespoke_tpch/query_q10.cpp:139). */
+int bolap_tune_set(bolap_ctx* ctx, const char* knob, int64_t value);
+int bolap_tune_get(bolap_ctx* ctx, const char* knob, int64_t* value);
 int bolap_ctx_read_trace(bolap_ctx* ctx, bolap_trace_entry* out, int32_t max_entries,
                          int32_t* n_out);
 
