@@ -20,8 +20,8 @@
 //          hit mask, and probes the table only for Bloom hits; filter and
 //          table are staged in LDS when they fit (one copy per workgroup)
 //          and read through L2 otherwise.  Matches
-//          gather price/disc and accumulate per lane; DPP wave sums, LDS,
-//          per-block partials and a last-block merge give the result.
+//          gather price/disc and accumulate per lane; DPP wave sums, LDS
+//          and per-block partials, summed by a one-workgroup merge kernel.
 #include <cstring>
 
 #include "preds.hpp"
@@ -153,7 +153,7 @@ BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __
 template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
 __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
-    int64_t rows, Table t, int64_t* res, int64_t* partials, unsigned* counter) {
+    int64_t rows, Table t, const int64_t* res, int64_t* partials) {
     extern __shared__ uint32_t lds[];
     const int64_t words = (int64_t(1) << t.bb_log2) / 32;
     const int64_t cap = int64_t(1) << t.cap_log2;
@@ -268,29 +268,33 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         partials[2 * int64_t(blockIdx.x)] = a;
         partials[2 * int64_t(blockIdx.x) + 1] = c;
     }
-    if (!last_block_done(counter)) return;
+}
+
+// Sums the per-block (revenue, matches) partials (one workgroup), after the
+// probe kernel on the same stream.
+__global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ partials,
+                                                    int nblocks, int64_t* res) {
+    __shared__ uint64_t s_a[256 / kWave], s_c[256 / kWave];
     uint64_t a = 0, c = 0;
-    for (int b = threadIdx.x; b < int(gridDim.x); b += kThreads) {
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
         a += uint64_t(partials[2 * int64_t(b)]);
         c += uint64_t(partials[2 * int64_t(b) + 1]);
     }
     a = wave_sum(a);
     c = wave_sum(c);
-    __syncthreads();
-    if (lane == 0) {
-        s_rev[wave] = int64_t(a);
-        s_cnt[wave] = int64_t(c);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_a[threadIdx.x / kWave] = a;
+        s_c[threadIdx.x / kWave] = c;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t ta = 0, tc = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            ta += uint64_t(s_rev[w]);
-            tc += uint64_t(s_cnt[w]);
+        for (int w = 0; w < 256 / kWave; ++w) {
+            ta += s_a[w];
+            tc += s_c[w];
         }
         res[kRev] = int64_t(ta);
         res[kMatch] = int64_t(tc);
-        st_agent(counter, 0u);
     }
 }
 
@@ -355,12 +359,18 @@ int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
     if (blocks > max_blocks) blocks = max_blocks;
-    TraceScope ts(ctx, "join_probe");
-    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), kLds ? lds : 0,
-                       ctx->stream, a->fact_fk, static_cast<const TP*>(a->fact_price),
-                       static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
-                       ws.at<int64_t>(o_part), ctx->counters + 1);
-    return launch_check("probe_sum_k");
+    {
+        TraceScope ts(ctx, "join_probe");
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), kLds ? lds : 0,
+                           ctx->stream, a->fact_fk, static_cast<const TP*>(a->fact_price),
+                           static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
+                           ws.at<int64_t>(o_part));
+    }
+    BOLAP_TRY(launch_check("probe_sum_k"));
+    TraceScope ts(ctx, "join_merge");
+    hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, ws.at<int64_t>(o_part),
+                       int(blocks), res);
+    return launch_check("join_merge_k");
 }
 
 // Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,

@@ -19,10 +19,9 @@
 //  * The predicate and the group select stay in registers: G group slots
 //    (compile-time) of per-lane accumulators updated with selects, so there
 //    are no atomics and no LDS traffic in the loop.
-//  * Wave shuffles + LDS reduce the block; blocks publish partials and the
-//    last block to finish (self-resetting counter) sums them in a fixed
-//    order.  Integer sums are exact, so results are bit-identical to the
-//    sequential reference.
+//  * DPP wave sums + LDS reduce the block into one partial row; a one-
+//    workgroup merge kernel sums the rows.  Integer sums are exact, so
+//    results are bit-identical to the sequential reference.
 #include <cstring>
 
 #include "common.hpp"
@@ -101,9 +100,7 @@ BOLAP_DEV void q1_row(Q1Lane<G>& acc, const Q1K& a, bool valid, int64_t s, int64
 }
 
 template <class L, int R, int U, int G, bool kPipe, int kQ1Threads>
-__global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials,
-                                                                  unsigned* counter,
-                                                                  bolap_q1_group* out) {
+__global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials) {
     constexpr int kQ1Waves = kQ1Threads / kWave;
     using S = typename L::S; using Q = typename L::Q; using P = typename L::P;
     using DP = typename L::DP; using D = typename L::D; using T = typename L::T;
@@ -239,12 +236,18 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         if (f != 4) partials[int64_t(blockIdx.x) * kP + threadIdx.x] = v;
     }
 
-    if (!last_block_done(counter)) return;
+}
 
-    // Last block: thread (row0, col) sums column col over rows row0,
-    // row0 + RS, ...; the 128-bit charge column pairs (3, 4) are summed by
-    // the col % 8 == 3 thread.  Plain loads after the acquire fence.
-    constexpr int RS = kQ1Threads / kP;
+// Merge of the per-block partial rows (one workgroup).  Thread (row0, col)
+// sums column col over rows row0, row0 + RS, ...; the 128-bit charge column
+// pairs (3, 4) are summed by the col % 8 == 3 thread.  Launched after the
+// scan on the same stream, so the partials are visible without fences.
+template <int G>
+__global__ __launch_bounds__(256) void q1_merge_k(const int64_t* __restrict__ partials,
+                                                  int nblocks, int n_groups,
+                                                  bolap_q1_group* out) {
+    constexpr int kP = G * kFields;
+    constexpr int RS = 256 / kP;
     __shared__ int64_t s_red[RS][kP];
     const int col = threadIdx.x % kP;
     const int row0 = threadIdx.x / kP;
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         const int f = col % kFields;
         if (f == 3) {
             Acc128 c{0, 0};
-            for (int b = row0; b < int(gridDim.x); b += RS) {
+            for (int b = row0; b < nblocks; b += RS) {
                 const int64_t* src = partials + int64_t(b) * kP + col;
                 add128(c, Acc128{uint64_t(src[0]), src[1]});
             }
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
             s_red[row0][col + 1] = c.hi;
         } else if (f != 4) {
             uint64_t v = 0;
-            for (int b = row0; b < int(gridDim.x); b += RS) v += uint64_t(partials[int64_t(b) * kP + col]);
+            for (int b = row0; b < nblocks; b += RS) v += uint64_t(partials[int64_t(b) * kP + col]);
             s_red[row0][col] = int64_t(v);
         }
     }
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
                 if (i != 3 && i != 4) t[i] += s_red[r][g * kFields + i];
             add128(tc, Acc128{uint64_t(s_red[r][g * kFields + 3]), s_red[r][g * kFields + 4]});
         }
-        if (g < a.n_groups) {
+        if (g < n_groups) {
             bolap_q1_group r;
             r.sum_qty = t[0];
             r.sum_base_price = t[1];
@@ -287,8 +290,8 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
             out[g] = r;
         }
     }
-    if (threadIdx.x == 0) st_agent(counter, 0u);
 }
+
 
 // Kernel variants (knob "q1.variant"): 1..3 plain loop with U = 1, 2, 4
 // units per lane; 4..5 software-pipelined loop with U = 1, 2.  All use
@@ -311,10 +314,16 @@ int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     const size_t o_part = plan.plan(size_t(blocks) * G * kFields * sizeof(int64_t));
     BOLAP_TRY(ws_reserve(ctx, plan.off));
     plan.base = static_cast<char*>(ctx->ws);
-    TraceScope ts(ctx, "q1_scan");
-    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kQ1Threads), 0, ctx->stream, k,
-                       plan.at<int64_t>(o_part), ctx->counters + 0, out_dev);
-    return launch_check("q1_scan_agg_kernel");
+    {
+        TraceScope ts(ctx, "q1_scan");
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kQ1Threads), 0, ctx->stream, k,
+                           plan.at<int64_t>(o_part));
+    }
+    BOLAP_TRY(launch_check("q1_scan_agg_kernel"));
+    TraceScope ts(ctx, "q1_merge");
+    hipLaunchKernelGGL(q1_merge_k<G>, dim3(1), dim3(256), 0, ctx->stream,
+                       plan.at<int64_t>(o_part), int(blocks), k.n_groups, out_dev);
+    return launch_check("q1_merge_k");
 }
 
 template <class L, int R, int G>
