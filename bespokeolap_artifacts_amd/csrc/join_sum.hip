@@ -277,8 +277,9 @@ __global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ 
     __shared__ uint64_t s_a[256 / kWave], s_c[256 / kWave];
     uint64_t a = 0, c = 0;
     for (int b = threadIdx.x; b < nblocks; b += 256) {
-        a += uint64_t(partials[2 * int64_t(b)]);
-        c += uint64_t(partials[2 * int64_t(b) + 1]);
+        const VecT<int64_t, 2> v = reinterpret_cast<const VecT<int64_t, 2>*>(partials)[b];
+        a += uint64_t(v.v[0]);
+        c += uint64_t(v.v[1]);
     }
     a = wave_sum(a);
     c = wave_sum(c);

@@ -238,60 +238,85 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
 
 }
 
-// Merge of the per-block partial rows (one workgroup).  Thread (row0, col)
-// sums column col over rows row0, row0 + RS, ...; the 128-bit charge column
-// pairs (3, 4) are summed by the col % 8 == 3 thread.  Launched after the
-// scan on the same stream, so the partials are visible without fences.
+// Merge of the per-block partial rows (one workgroup).  Thread t owns rows
+// t, t + 256, ... and loads each row whole with independent 16-byte loads,
+// accumulating every (group, field) in registers (the charge pair in 128
+// bits); DPP wave sums and LDS then combine the threads.  Launched after
+// the scan on the same stream, so the partials are visible without fences.
 template <int G>
 __global__ __launch_bounds__(256) void q1_merge_k(const int64_t* __restrict__ partials,
                                                   int nblocks, int n_groups,
                                                   bolap_q1_group* out) {
     constexpr int kP = G * kFields;
-    constexpr int RS = 256 / kP;
-    __shared__ int64_t s_red[RS][kP];
-    const int col = threadIdx.x % kP;
-    const int row0 = threadIdx.x / kP;
-    if (row0 < RS) {
-        const int f = col % kFields;
-        if (f == 3) {
-            Acc128 c{0, 0};
-            for (int b = row0; b < nblocks; b += RS) {
-                const int64_t* src = partials + int64_t(b) * kP + col;
-                add128(c, Acc128{uint64_t(src[0]), src[1]});
-            }
-            s_red[row0][col] = int64_t(c.lo);
-            s_red[row0][col + 1] = c.hi;
-        } else if (f != 4) {
-            uint64_t v = 0;
-            for (int b = row0; b < nblocks; b += RS) v += uint64_t(partials[int64_t(b) * kP + col]);
-            s_red[row0][col] = int64_t(v);
+    constexpr int kWaves256 = 256 / kWave;
+    int64_t acc[G][kFields];
+    Acc128 ch[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int i = 0; i < kFields; ++i) acc[g][i] = 0;
+        ch[g] = Acc128{0, 0};
+    }
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        const VecT<int64_t, 2>* row = reinterpret_cast<const VecT<int64_t, 2>*>(partials) +
+                                      int64_t(b) * (kP / 2);
+        VecT<int64_t, 2> v[kP / 2];
+#pragma unroll
+        for (int j = 0; j < kP / 2; ++j) v[j] = row[j];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int64_t* x = &v[g * kFields / 2].v[0];
+            acc[g][0] += x[0];
+            acc[g][1] += x[1];
+            acc[g][2] += x[2];
+            add128(ch[g], Acc128{uint64_t(x[3]), x[4]});
+            acc[g][5] += x[5];
+            acc[g][6] += x[6];
+        }
+    }
+    __shared__ int64_t s_w[kWaves256][G][kFields];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int64_t q = wave_sum(acc[g][0]);
+        const int64_t p = wave_sum(acc[g][1]);
+        const int64_t dp = wave_sum(acc[g][2]);
+        const Acc128 c = wave_sum128(ch[g]);
+        const int64_t d = wave_sum(acc[g][5]);
+        const int64_t n = wave_sum(acc[g][6]);
+        if (lane == 0) {
+            s_w[wave][g][0] = q;
+            s_w[wave][g][1] = p;
+            s_w[wave][g][2] = dp;
+            s_w[wave][g][3] = int64_t(c.lo);
+            s_w[wave][g][4] = c.hi;
+            s_w[wave][g][5] = d;
+            s_w[wave][g][6] = n;
         }
     }
     __syncthreads();
-    if (threadIdx.x < G) {
+    if (threadIdx.x < G && int(threadIdx.x) < n_groups) {
         const int g = threadIdx.x;
         int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
         Acc128 tc{0, 0};
-        for (int r = 0; r < RS; ++r) {
+        for (int w = 0; w < kWaves256; ++w) {
             for (int i = 0; i < kFields; ++i)
-                if (i != 3 && i != 4) t[i] += s_red[r][g * kFields + i];
-            add128(tc, Acc128{uint64_t(s_red[r][g * kFields + 3]), s_red[r][g * kFields + 4]});
+                if (i != 3 && i != 4 && i != 7) t[i] += s_w[w][g][i];
+            add128(tc, Acc128{uint64_t(s_w[w][g][3]), s_w[w][g][4]});
         }
-        if (g < n_groups) {
-            bolap_q1_group r;
-            r.sum_qty = t[0];
-            r.sum_base_price = t[1];
-            r.sum_disc_price = t[2];
-            r.sum_charge_lo = tc.lo;
-            r.sum_charge_hi = tc.hi;
-            r.sum_disc = t[5];
-            r.count = t[6];
-            r.reserved = 0;
-            out[g] = r;
-        }
+        bolap_q1_group r;
+        r.sum_qty = t[0];
+        r.sum_base_price = t[1];
+        r.sum_disc_price = t[2];
+        r.sum_charge_lo = tc.lo;
+        r.sum_charge_hi = tc.hi;
+        r.sum_disc = t[5];
+        r.count = t[6];
+        r.reserved = 0;
+        out[g] = r;
     }
 }
-
 
 // Kernel variants (knob "q1.variant"): 1..3 plain loop with U = 1, 2, 4
 // units per lane; 4..5 software-pipelined loop with U = 1, 2.  All use
