@@ -244,20 +244,26 @@ BOLAP_DEV void st_agent(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Returns true in every thread of the workgroup that finished last.  All
-// threads must call it after publishing their results with st_agent.
+// Returns true in every thread of the workgroup that finished last.  Every
+// workgroup publishes with a release RMW on the counter (write-back of its
+// own stores, no cache invalidation); only the last one runs the acquire
+// fence.  All threads must call it after writing their results.  The last
+// workgroup must return the counter to zero (reset_counter) when done.
 BOLAP_DEV bool last_block_done(unsigned* counter) {
     __shared__ bool s_last;
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned t = atomicAdd(counter, 1u);
+        const unsigned t =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (t == gridDim.x - 1);
     }
     __syncthreads();
     const bool last = s_last;
-    if (last) __threadfence();
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return last;
+}
+BOLAP_DEV void reset_counter(unsigned* counter) {
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // 64-bit mixing hash for open-addressing tables (SplitMix64 output function).

@@ -150,10 +150,45 @@ BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __
     cnt += 1;
 }
 
-template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
+// Sums the per-block (revenue, matches) partials with one workgroup of NT
+// threads into res[kRev], res[kMatch].
+template <int NT>
+BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks, int64_t* res) {
+    __shared__ uint64_t s_a[NT / kWave], s_c[NT / kWave];
+    uint64_t a = 0, c = 0;
+    for (int b = threadIdx.x; b < nblocks; b += NT) {
+        const VecT<int64_t, 2> v = reinterpret_cast<const VecT<int64_t, 2>*>(partials)[b];
+        a += uint64_t(v.v[0]);
+        c += uint64_t(v.v[1]);
+    }
+    a = wave_sum(a);
+    c = wave_sum(c);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_a[threadIdx.x / kWave] = a;
+        s_c[threadIdx.x / kWave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ta = 0, tc = 0;
+        for (int w = 0; w < NT / kWave; ++w) {
+            ta += s_a[w];
+            tc += s_c[w];
+        }
+        res[kRev] = int64_t(ta);
+        res[kMatch] = int64_t(tc);
+    }
+}
+
+// The same merge as a separate kernel after the probe on the same stream.
+__global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ partials,
+                                                    int nblocks, int64_t* res) {
+    join_merge_rows<256>(partials, nblocks, res);
+}
+
+template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock>
 __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
-    int64_t rows, Table t, const int64_t* res, int64_t* partials) {
+    int64_t rows, Table t, int64_t* res, int64_t* partials, unsigned* counter) {
     extern __shared__ uint32_t lds[];
     const int64_t words = (int64_t(1) << t.bb_log2) / 32;
     const int64_t cap = int64_t(1) << t.cap_log2;
@@ -268,34 +303,10 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         partials[2 * int64_t(blockIdx.x)] = a;
         partials[2 * int64_t(blockIdx.x) + 1] = c;
     }
-}
-
-// Sums the per-block (revenue, matches) partials (one workgroup), after the
-// probe kernel on the same stream.
-__global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ partials,
-                                                    int nblocks, int64_t* res) {
-    __shared__ uint64_t s_a[256 / kWave], s_c[256 / kWave];
-    uint64_t a = 0, c = 0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) {
-        const VecT<int64_t, 2> v = reinterpret_cast<const VecT<int64_t, 2>*>(partials)[b];
-        a += uint64_t(v.v[0]);
-        c += uint64_t(v.v[1]);
-    }
-    a = wave_sum(a);
-    c = wave_sum(c);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        s_a[threadIdx.x / kWave] = a;
-        s_c[threadIdx.x / kWave] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t ta = 0, tc = 0;
-        for (int w = 0; w < 256 / kWave; ++w) {
-            ta += s_a[w];
-            tc += s_c[w];
-        }
-        res[kRev] = int64_t(ta);
-        res[kMatch] = int64_t(tc);
+    if constexpr (kLastBlock) {
+        if (!last_block_done(counter)) return;
+        join_merge_rows<kThreads>(partials, int(gridDim.x), res);
+        if (threadIdx.x == 0) reset_counter(counter);
     }
 }
 
@@ -342,10 +353,10 @@ int size_table(int64_t n, int placement, Sizing& z) {
     return BOLAP_OK;
 }
 
-template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
-int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
+int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
                    int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
-    auto kern = probe_sum_k<kLds, TP, TD, kProbeU, kPipe>;
+    auto kern = probe_sum_k<kLds, TP, TD, kProbeU, kPipe, kLast>;
     if (kLds) {
         BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -365,13 +376,25 @@ int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
         hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), kLds ? lds : 0,
                            ctx->stream, a->fact_fk, static_cast<const TP*>(a->fact_price),
                            static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
-                           ws.at<int64_t>(o_part));
+                           ws.at<int64_t>(o_part), ctx->counters + 1);
     }
     BOLAP_TRY(launch_check("probe_sum_k"));
+    if (kLast) return BOLAP_OK;
     TraceScope ts(ctx, "join_merge");
     hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, ws.at<int64_t>(o_part),
                        int(blocks), res);
     return launch_check("join_merge_k");
+}
+
+// Finish (knob "join.finish"): 1 = last workgroup merges, 2 = merge kernel.
+template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
+int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
+                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+    if (ctx->knob("join.finish", 1) == 2)
+        return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, false>(ctx, a, t, lds, res, ws, o_part,
+                                                                   max_blocks);
+    return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, true>(ctx, a, t, lds, res, ws, o_part,
+                                                              max_blocks);
 }
 
 // Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,

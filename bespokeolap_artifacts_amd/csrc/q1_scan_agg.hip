@@ -99,8 +99,96 @@ BOLAP_DEV void q1_row(Q1Lane<G>& acc, const Q1K& a, bool valid, int64_t s, int64
     }
 }
 
-template <class L, int R, int U, int G, bool kPipe, int kQ1Threads>
-__global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials) {
+// Merge of the per-block partial rows (one workgroup).  Thread t owns rows
+// t, t + 256, ... and loads each row whole with independent 16-byte loads,
+// accumulating every (group, field) in registers (the charge pair in 128
+// bits); DPP wave sums and LDS then combine the threads.  Launched after
+// the scan on the same stream, so the partials are visible without fences.
+template <int G, int NT>
+BOLAP_DEV void q1_merge_rows(const int64_t* __restrict__ partials, int nblocks, int n_groups,
+                             bolap_q1_group* out) {
+    constexpr int kP = G * kFields;
+    constexpr int kWaves256 = NT / kWave;
+    int64_t acc[G][kFields];
+    Acc128 ch[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int i = 0; i < kFields; ++i) acc[g][i] = 0;
+        ch[g] = Acc128{0, 0};
+    }
+    for (int b = threadIdx.x; b < nblocks; b += NT) {
+        const VecT<int64_t, 2>* row = reinterpret_cast<const VecT<int64_t, 2>*>(partials) +
+                                      int64_t(b) * (kP / 2);
+        VecT<int64_t, 2> v[kP / 2];
+#pragma unroll
+        for (int j = 0; j < kP / 2; ++j) v[j] = row[j];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int64_t* x = &v[g * kFields / 2].v[0];
+            acc[g][0] += x[0];
+            acc[g][1] += x[1];
+            acc[g][2] += x[2];
+            add128(ch[g], Acc128{uint64_t(x[3]), x[4]});
+            acc[g][5] += x[5];
+            acc[g][6] += x[6];
+        }
+    }
+    __shared__ int64_t s_w[kWaves256][G][kFields];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int64_t q = wave_sum(acc[g][0]);
+        const int64_t p = wave_sum(acc[g][1]);
+        const int64_t dp = wave_sum(acc[g][2]);
+        const Acc128 c = wave_sum128(ch[g]);
+        const int64_t d = wave_sum(acc[g][5]);
+        const int64_t n = wave_sum(acc[g][6]);
+        if (lane == 0) {
+            s_w[wave][g][0] = q;
+            s_w[wave][g][1] = p;
+            s_w[wave][g][2] = dp;
+            s_w[wave][g][3] = int64_t(c.lo);
+            s_w[wave][g][4] = c.hi;
+            s_w[wave][g][5] = d;
+            s_w[wave][g][6] = n;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < G && int(threadIdx.x) < n_groups) {
+        const int g = threadIdx.x;
+        int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+        Acc128 tc{0, 0};
+        for (int w = 0; w < kWaves256; ++w) {
+            for (int i = 0; i < kFields; ++i)
+                if (i != 3 && i != 4 && i != 7) t[i] += s_w[w][g][i];
+            add128(tc, Acc128{uint64_t(s_w[w][g][3]), s_w[w][g][4]});
+        }
+        bolap_q1_group r;
+        r.sum_qty = t[0];
+        r.sum_base_price = t[1];
+        r.sum_disc_price = t[2];
+        r.sum_charge_lo = tc.lo;
+        r.sum_charge_hi = tc.hi;
+        r.sum_disc = t[5];
+        r.count = t[6];
+        r.reserved = 0;
+        out[g] = r;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void q1_merge_k(const int64_t* __restrict__ partials,
+                                                  int nblocks, int n_groups,
+                                                  bolap_q1_group* out) {
+    q1_merge_rows<G, 256>(partials, nblocks, n_groups, out);
+}
+
+template <class L, int R, int U, int G, bool kPipe, int kQ1Threads, bool kLastBlock>
+__global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t* partials,
+                                                                  unsigned* counter,
+                                                                  bolap_q1_group* out) {
     constexpr int kQ1Waves = kQ1Threads / kWave;
     using S = typename L::S; using Q = typename L::Q; using P = typename L::P;
     using DP = typename L::DP; using D = typename L::D; using T = typename L::T;
@@ -235,87 +323,12 @@ __global__ __launch_bounds__(kQ1Threads) void q1_scan_agg_kernel(Q1K a, int64_t*
         }
         if (f != 4) partials[int64_t(blockIdx.x) * kP + threadIdx.x] = v;
     }
+    if constexpr (kLastBlock) {
+        if (!last_block_done(counter)) return;
+        q1_merge_rows<G, kQ1Threads>(partials, int(gridDim.x), a.n_groups, out);
+        if (threadIdx.x == 0) reset_counter(counter);
+    }
 
-}
-
-// Merge of the per-block partial rows (one workgroup).  Thread t owns rows
-// t, t + 256, ... and loads each row whole with independent 16-byte loads,
-// accumulating every (group, field) in registers (the charge pair in 128
-// bits); DPP wave sums and LDS then combine the threads.  Launched after
-// the scan on the same stream, so the partials are visible without fences.
-template <int G>
-__global__ __launch_bounds__(256) void q1_merge_k(const int64_t* __restrict__ partials,
-                                                  int nblocks, int n_groups,
-                                                  bolap_q1_group* out) {
-    constexpr int kP = G * kFields;
-    constexpr int kWaves256 = 256 / kWave;
-    int64_t acc[G][kFields];
-    Acc128 ch[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-#pragma unroll
-        for (int i = 0; i < kFields; ++i) acc[g][i] = 0;
-        ch[g] = Acc128{0, 0};
-    }
-    for (int b = threadIdx.x; b < nblocks; b += 256) {
-        const VecT<int64_t, 2>* row = reinterpret_cast<const VecT<int64_t, 2>*>(partials) +
-                                      int64_t(b) * (kP / 2);
-        VecT<int64_t, 2> v[kP / 2];
-#pragma unroll
-        for (int j = 0; j < kP / 2; ++j) v[j] = row[j];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int64_t* x = &v[g * kFields / 2].v[0];
-            acc[g][0] += x[0];
-            acc[g][1] += x[1];
-            acc[g][2] += x[2];
-            add128(ch[g], Acc128{uint64_t(x[3]), x[4]});
-            acc[g][5] += x[5];
-            acc[g][6] += x[6];
-        }
-    }
-    __shared__ int64_t s_w[kWaves256][G][kFields];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const int64_t q = wave_sum(acc[g][0]);
-        const int64_t p = wave_sum(acc[g][1]);
-        const int64_t dp = wave_sum(acc[g][2]);
-        const Acc128 c = wave_sum128(ch[g]);
-        const int64_t d = wave_sum(acc[g][5]);
-        const int64_t n = wave_sum(acc[g][6]);
-        if (lane == 0) {
-            s_w[wave][g][0] = q;
-            s_w[wave][g][1] = p;
-            s_w[wave][g][2] = dp;
-            s_w[wave][g][3] = int64_t(c.lo);
-            s_w[wave][g][4] = c.hi;
-            s_w[wave][g][5] = d;
-            s_w[wave][g][6] = n;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < G && int(threadIdx.x) < n_groups) {
-        const int g = threadIdx.x;
-        int64_t t[kFields] = {0, 0, 0, 0, 0, 0, 0, 0};
-        Acc128 tc{0, 0};
-        for (int w = 0; w < kWaves256; ++w) {
-            for (int i = 0; i < kFields; ++i)
-                if (i != 3 && i != 4 && i != 7) t[i] += s_w[w][g][i];
-            add128(tc, Acc128{uint64_t(s_w[w][g][3]), s_w[w][g][4]});
-        }
-        bolap_q1_group r;
-        r.sum_qty = t[0];
-        r.sum_base_price = t[1];
-        r.sum_disc_price = t[2];
-        r.sum_charge_lo = tc.lo;
-        r.sum_charge_hi = tc.hi;
-        r.sum_disc = t[5];
-        r.count = t[6];
-        r.reserved = 0;
-        out[g] = r;
-    }
 }
 
 // Kernel variants (knob "q1.variant"): 1..3 plain loop with U = 1, 2, 4
@@ -323,9 +336,9 @@ __global__ __launch_bounds__(256) void q1_merge_k(const int64_t* __restrict__ pa
 // 256-thread workgroups; "q1.blocks_per_cu" caps the persistent grid.
 constexpr int kQ1Threads = 256;
 
-template <class L, int R, int G, int U, bool kPipe>
-int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
-    auto kern = q1_scan_agg_kernel<L, R, U, G, kPipe, kQ1Threads>;
+template <class L, int R, int G, int U, bool kPipe, bool kLast>
+int launch_q1_f(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    auto kern = q1_scan_agg_kernel<L, R, U, G, kPipe, kQ1Threads, kLast>;
     const int64_t n_units = k.rows / R;
     const int64_t tile = int64_t(kQ1Threads) * U;
     int64_t blocks = (n_units + tile - 1) / tile;
@@ -342,13 +355,22 @@ int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     {
         TraceScope ts(ctx, "q1_scan");
         hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kQ1Threads), 0, ctx->stream, k,
-                           plan.at<int64_t>(o_part));
+                           plan.at<int64_t>(o_part), ctx->counters + 0, out_dev);
     }
     BOLAP_TRY(launch_check("q1_scan_agg_kernel"));
+    if (kLast) return BOLAP_OK;
     TraceScope ts(ctx, "q1_merge");
     hipLaunchKernelGGL(q1_merge_k<G>, dim3(1), dim3(256), 0, ctx->stream,
                        plan.at<int64_t>(o_part), int(blocks), k.n_groups, out_dev);
     return launch_check("q1_merge_k");
+}
+
+// Finish (knob "q1.finish"): 1 = the last workgroup merges the partials
+// (release/acquire on a counter), 2 = a separate one-workgroup merge kernel.
+template <class L, int R, int G, int U, bool kPipe>
+int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
+    if (ctx->knob("q1.finish", 1) == 2) return launch_q1_f<L, R, G, U, kPipe, false>(ctx, k, out_dev);
+    return launch_q1_f<L, R, G, U, kPipe, true>(ctx, k, out_dev);
 }
 
 template <class L, int R, int G>
