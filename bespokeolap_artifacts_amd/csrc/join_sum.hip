@@ -54,12 +54,20 @@ BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
 // Blocked Bloom filter: one 32-bit word per key (chosen by the top bits of
 // a second multiplicative hash) with two bits set from middle bits of the
 // slot hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
-BOLAP_DEV uint32_t bloom_word(int32_t key, int bb_log2) {
-    return (uint32_t(key) * kBitMul) >> (32 - (bb_log2 - 5));
+BOLAP_DEV uint32_t bloom_hash(int32_t key) { return uint32_t(key) * kBitMul; }
+BOLAP_DEV uint32_t bloom_word(uint32_t h, int bb_log2) { return h >> (32 - (bb_log2 - 5)); }
+// Two bit positions from the ten hash bits just below the word index (the
+// top bb_log2 - 5 bits pick the word; 10 <= bb_log2 <= 22 keeps the three
+// fields disjoint and inside the word).
+BOLAP_DEV bool bloom_test(uint32_t w, uint32_t h, int bb_log2) {
+    const int sh = 32 - bb_log2;
+    const uint32_t b1 = (h >> sh) & 31, b2 = (h >> (sh - 5)) & 31;
+    return ((w >> b1) & (w >> b2) & 1u) != 0;
 }
-BOLAP_DEV uint32_t bloom_mask(int32_t key) {
-    const uint32_t h = uint32_t(key) * kSlotMul;
-    return (1u << ((h >> 7) & 31)) | (1u << ((h >> 12) & 31));
+BOLAP_DEV uint32_t bloom_bits(uint32_t h, int bb_log2) {
+    const int sh = 32 - bb_log2;
+    const uint32_t b1 = (h >> sh) & 31, b2 = (h >> (sh - 5)) & 31;
+    return (1u << b1) | (1u << b2);
 }
 
 __global__ __launch_bounds__(256) void zero_res_k(int64_t* res) {
@@ -104,11 +112,12 @@ __global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows
                       static_cast<unsigned long long>(__popcll(m)));
         if (!pass) continue;
         const int32_t key = keys[row];
-        if (key == kEmpty) {
+        const uint32_t h = bloom_hash(key);
+        atomicOr(t.bits + bloom_word(h, t.bb_log2), bloom_bits(h, t.bb_log2));
+        if (key == kEmpty) {  // the table's empty marker: recorded as a flag
             atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
             continue;
         }
-        atomicOr(t.bits + bloom_word(key, t.bb_log2), bloom_mask(key));
         uint32_t s = slot_of(key, t.cap_log2);
         bool placed = false;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -223,17 +232,21 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         // Phase 1: all Bloom words first (independent LDS reads), then a
         // per-lane hit mask over the kProbeU * 4 keys.
         constexpr int kKeys = kProbeU * 4;
-        uint32_t wv[kKeys];
-#pragma unroll
-        for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(x.v[i / 4].v[i % 4], t.bb_log2)];
-        uint32_t hits = 0;
+        uint32_t hv[kKeys], wv[kKeys];
 #pragma unroll
         for (int i = 0; i < kKeys; ++i) {
-            const int32_t key = x.v[i / 4].v[i % 4];
-            const uint32_t m = bloom_mask(key);
-            const bool ok = at + int64_t(i / 4) * kThreads < n_units;
-            const bool h = ok && (key == kEmpty ? has_min : (wv[i] & m) == m);
-            hits |= (h ? 1u : 0u) << i;
+            hv[i] = bloom_hash(x.v[i / 4].v[i % 4]);
+            wv[i] = B[bloom_word(hv[i], t.bb_log2)];
+        }
+        uint32_t hits = 0;
+#pragma unroll
+        for (int i = 0; i < kKeys; ++i)
+            hits |= (bloom_test(wv[i], hv[i], t.bb_log2) ? 1u : 0u) << i;
+        // Units past the end were clamped to the last unit: drop them.
+        if (at + int64_t(kProbeU - 1) * kThreads >= n_units) {
+#pragma unroll
+            for (int k = 0; k < kProbeU; ++k)
+                if (at + int64_t(k) * kThreads >= n_units) hits &= ~(0xFu << (4 * k));
         }
         // Phase 2: exact probes only for Bloom hits (rare, divergent).
         if (hits) {
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                 if (hits & (1u << i)) {
                     const int32_t key = x.v[i / 4].v[i % 4];
                     const int64_t row = (at + int64_t(i / 4) * kThreads) * 4 + (i % 4);
-                    if (key == kEmpty || table_has(key, T, t.cap_log2, mask))
+                    if (key == kEmpty ? has_min : table_has(key, T, t.cap_log2, mask))
                         add_match<TP, TD>(row, price, disc, rev, cnt);
                 }
             }
@@ -275,10 +288,7 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         const int64_t row = n_units * 4 + threadIdx.x;
         if (row < rows) {
             const int32_t key = fk[row];
-            const uint32_t m = bloom_mask(key);
-            const bool h = key == kEmpty ? has_min
-                                         : ((B[bloom_word(key, t.bb_log2)] & m) == m &&
-                                            table_has(key, T, t.cap_log2, mask));
+            const bool h = key == kEmpty ? has_min : table_has(key, T, t.cap_log2, mask);
             if (h) add_match<TP, TD>(row, price, disc, rev, cnt);
         }
     }
@@ -330,7 +340,7 @@ int size_table(int64_t n, int placement, Sizing& z) {
     const int lds_cap = ceil_log2(n * 4 < 64 ? 64 : n * 4);
     int lds_bb = ceil_log2(n * 128);
     if (lds_bb < 10) lds_bb = 10;
-    if (lds_bb > 18) lds_bb = 18;
+    if (lds_bb > 18) lds_bb = 18;  // 32 KB of LDS
     const size_t lds_bytes = (size_t(1) << lds_cap) * 4 + (size_t(1) << lds_bb) / 8;
     const bool fits = lds_cap <= 30 && lds_bytes <= size_t(kLdsBudget);
     if (placement == BOLAP_PLACE_LDS && !fits) {
@@ -344,7 +354,7 @@ int size_table(int64_t n, int placement, Sizing& z) {
     int cap = ceil_log2(n * 2 < 64 ? 64 : n * 2);
     int bb = ceil_log2(n * 16);
     if (bb < 10) bb = 10;
-    if (bb > 31) bb = 31;
+    if (bb > 22) bb = 22;  // 512 KB filter, L2-resident
     if (cap > 31) {
         set_error("bolap_join_sum: %lld keys exceed the table limit", (long long)n);
         return BOLAP_ERR_CAPACITY;
@@ -365,7 +375,9 @@ int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
     int per_cu = 0;
     BOLAP_HIP_TRY(
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, kLds ? lds : 0));
-    int64_t blocks = int64_t(ctx->num_cus) * ctx->knob("join.blocks_per_cu", per_cu > 0 ? per_cu : 1);
+    int64_t want = ctx->knob("join.blocks_per_cu", 2);
+    if (per_cu > 0 && want > per_cu) want = per_cu;
+    int64_t blocks = int64_t(ctx->num_cus) * want;
     const int64_t need = (a->fact_rows / 4 + int64_t(kThreads) * kProbeU - 1) /
                          (int64_t(kThreads) * kProbeU);
     if (blocks > need) blocks = need;
@@ -390,11 +402,11 @@ int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
 template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
 int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
                    int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
-    if (ctx->knob("join.finish", 1) == 2)
-        return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, false>(ctx, a, t, lds, res, ws, o_part,
-                                                                   max_blocks);
-    return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, true>(ctx, a, t, lds, res, ws, o_part,
-                                                              max_blocks);
+    if (ctx->knob("join.finish", 2) == 1)
+        return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, true>(ctx, a, t, lds, res, ws, o_part,
+                                                                  max_blocks);
+    return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, false>(ctx, a, t, lds, res, ws, o_part,
+                                                               max_blocks);
 }
 
 // Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,
@@ -402,7 +414,7 @@ int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
 template <bool kLds, typename TP, typename TD>
 int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
                  int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
-    switch (ctx->knob("join.probe_variant", 4)) {
+    switch (ctx->knob("join.probe_variant", 3)) {
         case 1: return launch_probe_v<kLds, TP, TD, 4, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
         case 2: return launch_probe_v<kLds, TP, TD, 8, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
         case 3: return launch_probe_v<kLds, TP, TD, 2, true>(ctx, a, t, lds, res, ws, o_part, max_blocks);

@@ -344,7 +344,9 @@ int launch_q1_f(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     int64_t blocks = (n_units + tile - 1) / tile;
     int per_cu = 0;
     BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kQ1Threads, 0));
-    const int64_t want = ctx->knob("q1.blocks_per_cu", per_cu > 0 ? per_cu : 1);
+    // One workgroup per CU measured fastest (fewer partial rows to merge).
+    int64_t want = ctx->knob("q1.blocks_per_cu", 1);
+    if (per_cu > 0 && want > per_cu) want = per_cu;
     const int64_t max_blocks = int64_t(ctx->num_cus) * want;
     if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
@@ -366,17 +368,18 @@ int launch_q1_f(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
 }
 
 // Finish (knob "q1.finish"): 1 = the last workgroup merges the partials
-// (release/acquire on a counter), 2 = a separate one-workgroup merge kernel.
+// (release/acquire on a counter), 2 (default) = a separate one-workgroup
+// merge kernel; both measured ~10 us at 256 workgroups on MI355X.
 template <class L, int R, int G, int U, bool kPipe>
 int launch_q1_v(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
-    if (ctx->knob("q1.finish", 1) == 2) return launch_q1_f<L, R, G, U, kPipe, false>(ctx, k, out_dev);
-    return launch_q1_f<L, R, G, U, kPipe, true>(ctx, k, out_dev);
+    if (ctx->knob("q1.finish", 2) == 1) return launch_q1_f<L, R, G, U, kPipe, true>(ctx, k, out_dev);
+    return launch_q1_f<L, R, G, U, kPipe, false>(ctx, k, out_dev);
 }
 
 template <class L, int R, int G>
 int launch_q1_g(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     if (R == 1) return launch_q1_v<L, R, G, 2, false>(ctx, k, out_dev);  // unaligned columns
-    switch (ctx->knob("q1.variant", 5)) {
+    switch (ctx->knob("q1.variant", 4)) {
         case 1: return launch_q1_v<L, R, G, 1, false>(ctx, k, out_dev);
         case 2: return launch_q1_v<L, R, G, 2, false>(ctx, k, out_dev);
         case 3: return launch_q1_v<L, R, G, 4, false>(ctx, k, out_dev);
