@@ -1,0 +1,697 @@
+// chain_join.hip — customer filter -> orders filter + join -> fact join ->
+// GROUP BY order key SUM(price * (100 - disc)) -> ORDER BY revenue DESC,
+// day ASC, key ASC (top-k or all groups).
+//
+// Replaces run_q3 (bespoke_tpch/query_q3.cpp:153-365): the customer
+// segment mask (:182-205), the date-filtered orders (:207-210, :306-318), the
+// per-order revenue over the order's fact rows (:219-260, which the reference
+// reads through precomputed contiguous ranges) and the final std::sort
+// (:348-356).  Here the fact table may be in any order:
+//   stats    min/max of the customer and order keys (key ranges for bitmaps)
+//   cust     bitmap over the customer key range, bit = segment matches
+//   count    (exact sizing only) orders passing day < limit and the bitmap
+//   init     group table (open addressing, EMPTY keys) + side arrays cleared
+//   build    passing orders inserted (key -> slot, day stored), an exact
+//            bitmap over the order key range marks them for the fact probe
+//   probe    fact keys streamed with 128-bit loads; bitmap test; runs of the
+//            same key inside a lane are merged before the slot lookup; one
+//            64-bit atomic per run adds the revenue, a plain byte store marks
+//            the group as hit
+//   compact  hit groups -> (revenue, day, key) rows (wave-aggregated appends)
+//   order    rows packed into one u64 key (revenue desc, day, key) when the
+//            value ranges fit 64 bits, then radix select (top-k) or a full
+//            LSD radix sort; otherwise three stable radix passes.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "prims.hpp"
+#include "radix.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kT = 256;
+constexpr int32_t kEmpty = INT32_MIN;
+constexpr uint32_t kSlotMul = 0x9E3779B1u;
+
+// Device statistics slot (int64).
+enum {
+    sCMin = 0, sCMax, sOMin, sOMax, sCustPass, sOrderPass, sCount, sDMin, sDMax, sMatch,
+    sGroups, sFlags, sRMin, sRMax, sN = 16
+};
+
+struct RowT {
+    int64_t rev;
+    int32_t day;
+    int32_t key;
+};
+
+__global__ void stats_init_k(int64_t* st) {
+    if (threadIdx.x < sN) {
+        int64_t v = 0;
+        if (threadIdx.x == sCMin || threadIdx.x == sOMin || threadIdx.x == sDMin ||
+            threadIdx.x == sRMin)
+            v = INT64_MAX;
+        if (threadIdx.x == sCMax || threadIdx.x == sOMax || threadIdx.x == sDMax ||
+            threadIdx.x == sRMax)
+            v = INT64_MIN;
+        st[threadIdx.x] = v;
+    }
+}
+
+BOLAP_DEV int64_t wave_min(int64_t v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+BOLAP_DEV int64_t wave_max(int64_t v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block min/max folded into st[imin], st[imax] with one atomic pair per wave.
+BOLAP_DEV void fold_minmax(int64_t lo, int64_t hi, int64_t* st, int imin, int imax) {
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (lo != INT64_MAX) atomicMin(reinterpret_cast<long long*>(st + imin), (long long)lo);
+        if (hi != INT64_MIN) atomicMax(reinterpret_cast<long long*>(st + imax), (long long)hi);
+    }
+}
+
+BOLAP_DEV void count_ballot(bool p, int64_t* st, int idx) {
+    const uint64_t m = __ballot(p);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m)
+        atomicAdd(reinterpret_cast<unsigned long long*>(st + idx),
+                  (unsigned long long)__popcll(m));
+}
+
+__global__ __launch_bounds__(kT) void key_minmax_k(const int32_t* __restrict__ ck, int64_t nc,
+                                                   const int32_t* __restrict__ ok, int64_t no,
+                                                   int64_t* st) {
+    int64_t cl = INT64_MAX, ch = INT64_MIN, ol = INT64_MAX, oh = INT64_MIN;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < nc; i += stride) {
+        const int64_t v = ck[i];
+        cl = v < cl ? v : cl;
+        ch = v > ch ? v : ch;
+    }
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < no; i += stride) {
+        const int64_t v = ok[i];
+        ol = v < ol ? v : ol;
+        oh = v > oh ? v : oh;
+    }
+    fold_minmax(cl, ch, st, sCMin, sCMax);
+    fold_minmax(ol, oh, st, sOMin, sOMax);
+}
+
+__global__ __launch_bounds__(kT) void fill_u32_k(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT)
+        p[i] = v;
+}
+
+struct Ranges {
+    int64_t cmin, cmax, omin, omax;
+};
+
+BOLAP_DEV bool test_bit(const uint32_t* __restrict__ bits, int64_t rel, int64_t span) {
+    if (rel < 0 || rel >= span) return false;
+    return (bits[rel >> 5] >> (rel & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ key,
+                                                  const uint8_t* __restrict__ seg, int64_t n,
+                                                  int32_t segment, Ranges r,
+                                                  uint32_t* __restrict__ bits, int64_t* st) {
+    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
+        const int64_t i = base + threadIdx.x;
+        const bool pass = i < n && int32_t(seg[i]) == segment;
+        count_ballot(pass, st, sCustPass);
+        if (pass) {
+            const int64_t rel = int64_t(key[i]) - r.cmin;
+            atomicOr(bits + (rel >> 5), 1u << (rel & 31));
+        }
+    }
+}
+
+BOLAP_DEV bool order_pass(int64_t i, int64_t n, const int32_t* __restrict__ day,
+                          const int32_t* __restrict__ cust, int32_t day_lt,
+                          const uint32_t* __restrict__ cbits, int64_t cspan, const Ranges& r) {
+    if (i >= n) return false;
+    if (day[i] >= day_lt) return false;
+    return test_bit(cbits, int64_t(cust[i]) - r.cmin, cspan);
+}
+
+__global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ day,
+                                                    const int32_t* __restrict__ cust, int64_t n,
+                                                    int32_t day_lt,
+                                                    const uint32_t* __restrict__ cbits,
+                                                    int64_t cspan, Ranges r, int64_t* st) {
+    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
+        const int64_t i = base + threadIdx.x;
+        count_ballot(order_pass(i, n, day, cust, day_lt, cbits, cspan, r), st, sCount);
+    }
+}
+
+struct GTable {
+    int32_t* keys;   // cap slots, kEmpty = free
+    int32_t* days;
+    uint64_t* rev;   // revenue per slot (two's complement)
+    uint8_t* hit;    // 1 when the group got at least one fact row
+    uint32_t* obits; // exact bitmap over the order key range
+    int32_t cap_log2;
+    int64_t max_fill;
+};
+
+BOLAP_DEV uint32_t gslot(int32_t key, int cap_log2) {
+    return (uint32_t(key) * kSlotMul) >> (32 - cap_log2);
+}
+
+__global__ __launch_bounds__(kT) void gtable_init_k(GTable t, int64_t owords) {
+    const int64_t cap = int64_t(1) << t.cap_log2;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < cap; i += stride) {
+        t.keys[i] = kEmpty;
+        t.rev[i] = 0;
+        t.hit[i] = 0;
+    }
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < owords; i += stride)
+        t.obits[i] = 0;
+}
+
+__global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ okey,
+                                                    const int32_t* __restrict__ day,
+                                                    const int32_t* __restrict__ cust, int64_t n,
+                                                    int32_t day_lt,
+                                                    const uint32_t* __restrict__ cbits,
+                                                    int64_t cspan, Ranges r, GTable t,
+                                                    int64_t* st) {
+    const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
+    int64_t dl = INT64_MAX, dh = INT64_MIN;
+    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
+        const int64_t i = base + threadIdx.x;
+        const bool pass = order_pass(i, n, day, cust, day_lt, cbits, cspan, r);
+        count_ballot(pass, st, sOrderPass);
+        if (!pass) continue;
+        const int32_t key = okey[i];
+        const int32_t d = day[i];
+        dl = d < dl ? d : dl;
+        dh = d > dh ? d : dh;
+        const int64_t rel = int64_t(key) - r.omin;
+        atomicOr(t.obits + (rel >> 5), 1u << (rel & 31));
+        uint32_t s = gslot(key, t.cap_log2);
+        bool placed = false;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const int32_t prev = atomicCAS(t.keys + s, kEmpty, key);
+            if (prev == kEmpty || prev == key) {
+                t.days[s] = d;
+                if (prev == kEmpty) {
+                    const unsigned long long c = atomicAdd(
+                        reinterpret_cast<unsigned long long*>(st + sGroups), 1ull);
+                    if (int64_t(c) + 1 > t.max_fill)
+                        atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
+                }
+                placed = true;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+        if (!placed) atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
+    }
+    fold_minmax(dl, dh, st, sDMin, sDMax);
+}
+
+// Slot of a key known to be in the table.
+BOLAP_DEV uint32_t find_slot(int32_t key, const int32_t* __restrict__ keys, int cap_log2) {
+    const uint32_t mask = (uint32_t(1) << cap_log2) - 1;
+    uint32_t s = gslot(key, cap_log2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        if (keys[s] == key) return s;
+        s = (s + 1) & mask;
+    }
+    return 0xFFFFFFFFu;
+}
+
+template <typename TP, typename TD>
+__global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ fk,
+                                                   const TP* __restrict__ price,
+                                                   const TD* __restrict__ disc,
+                                                   const int32_t* __restrict__ dprice,
+                                                   int64_t n, Ranges r, GTable t,
+                                                   int64_t* st) {
+    const int64_t ospan = r.omax - r.omin + 1;
+    const int64_t n_units = n / 4;
+    uint64_t matched = 0;
+    auto row_value = [&](int64_t row) -> int64_t {
+        if (dprice) return int64_t(dprice[row]);
+        return int64_t(uint64_t(int64_t(price[row])) * uint64_t(100 - int64_t(disc[row])));
+    };
+    // Processes rows [row0, row0 + cnt) whose keys are in kv, merging runs of
+    // equal keys before one atomic per run.
+    auto run_rows = [&](const int32_t* kv, int64_t row0, int cnt) {
+        int32_t cur_key = 0;
+        uint32_t cur_slot = 0xFFFFFFFFu;
+        uint64_t acc = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const int32_t key = kv[j];
+            if (!test_bit(t.obits, int64_t(key) - r.omin, ospan)) continue;
+            ++matched;
+            const uint64_t v = uint64_t(row_value(row0 + j));
+            if (cur_slot != 0xFFFFFFFFu && key == cur_key) {
+                acc += v;
+                continue;
+            }
+            if (cur_slot != 0xFFFFFFFFu) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(t.rev + cur_slot),
+                          (unsigned long long)acc);
+                t.hit[cur_slot] = 1;
+            }
+            cur_key = key;
+            cur_slot = find_slot(key, t.keys, t.cap_log2);
+            acc = v;
+        }
+        if (cur_slot != 0xFFFFFFFFu) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(t.rev + cur_slot),
+                      (unsigned long long)acc);
+            t.hit[cur_slot] = 1;
+        }
+    };
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < n_units;
+         u += int64_t(gridDim.x) * kT) {
+        const VecT<int32_t, 4> v = load_unit<int32_t, 4>(fk, u);
+        run_rows(v.v, u * 4, 4);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int32_t kv[4];
+        const int rem = int(n - n_units * 4);
+        for (int j = 0; j < rem; ++j) kv[j] = fk[n_units * 4 + j];
+        run_rows(kv, n_units * 4, rem);
+    }
+    const uint64_t wm = wave_sum(matched);
+    if ((threadIdx.x & (kWave - 1)) == 0 && wm)
+        atomicAdd(reinterpret_cast<unsigned long long*>(st + sMatch), (unsigned long long)wm);
+}
+
+__global__ __launch_bounds__(kT) void compact_groups_k(GTable t, RowT* __restrict__ out,
+                                                       unsigned long long* __restrict__ n_out,
+                                                       int64_t* st) {
+    const int64_t cap = int64_t(1) << t.cap_log2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    int64_t rl = INT64_MAX, rh = INT64_MIN;
+    for (int64_t base = int64_t(blockIdx.x) * kT; base < cap; base += int64_t(gridDim.x) * kT) {
+        const int64_t i = base + threadIdx.x;
+        const bool h = i < cap && t.hit[i];
+        const uint64_t m = __ballot(h);
+        unsigned long long at = 0;
+        if (lane == 0 && m) at = atomicAdd(n_out, (unsigned long long)__popcll(m));
+        at = __shfl(at, 0, kWave);
+        if (h) {
+            const int64_t rv = int64_t(t.rev[i]);
+            rl = rv < rl ? rv : rl;
+            rh = rv > rh ? rv : rh;
+            out[at + __popcll(m & lt_mask)] = RowT{rv, t.days[i], t.keys[i]};
+        }
+    }
+    fold_minmax(rl, rh, st, sRMin, sRMax);
+}
+
+int bits_for(uint64_t span) {  // bits to hold values 0..span
+    int b = 0;
+    while (b < 64 && (span >> b) != 0) ++b;
+    return b;
+}
+
+struct Pack {
+    int64_t rmax, dmin, kmin;
+    int kb, db;
+};
+
+// Ascending order of the packed key = revenue desc, day asc, key asc.
+__global__ __launch_bounds__(kT) void pack_rows_k(const RowT* __restrict__ rows, int64_t n,
+                                                  Pack p, uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ idx) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const RowT x = rows[i];
+        const int hs = p.db + p.kb;  // fields wider than 63 bits hold only zeros
+        const uint64_t rf = uint64_t(p.rmax) - uint64_t(x.rev);
+        const uint64_t df = uint64_t(int64_t(x.day) - p.dmin);
+        keys[i] = (hs >= 64 ? 0 : rf << hs) | (p.kb >= 64 ? 0 : df << p.kb) |
+                  uint64_t(int64_t(x.key) - p.kmin);
+        idx[i] = uint32_t(i);
+    }
+}
+
+// Field keys for the fallback ordering (one stable pass per field).
+__global__ __launch_bounds__(kT) void field_keys_k(const RowT* __restrict__ rows,
+                                                   const uint32_t* __restrict__ idx, int64_t n,
+                                                   int field, uint64_t* __restrict__ keys) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const RowT x = rows[idx[i]];
+        uint64_t k;
+        if (field == 0) k = uint64_t(uint32_t(x.key) ^ 0x80000000u);
+        else if (field == 1) k = uint64_t(uint32_t(x.day) ^ 0x80000000u);
+        else k = ~(uint64_t(x.rev) ^ 0x8000000000000000ull);  // descending
+        keys[i] = k;
+    }
+}
+
+__global__ __launch_bounds__(kT) void iota_k(uint32_t* idx, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        idx[i] = uint32_t(i);
+}
+
+__global__ __launch_bounds__(kT) void gather_rows_k(const RowT* __restrict__ rows,
+                                                    const uint32_t* __restrict__ idx, int64_t n,
+                                                    bolap_group_row* __restrict__ out) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const RowT x = rows[idx[i]];
+        out[i] = bolap_group_row{x.key, x.day, x.rev};
+    }
+}
+
+inline unsigned grid_for(bolap_ctx* ctx, int64_t n, int per_cu = 4) {
+    int64_t b = (n + kT - 1) / kT;
+    const int64_t cap = int64_t(ctx->num_cus) * per_cu;
+    if (b > cap) b = cap;
+    return unsigned(b < 1 ? 1 : b);
+}
+
+int ceil_log2(int64_t v) {
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+struct ChainRun {
+    int64_t st[sN];
+    int64_t n_groups;
+    int cap_log2;
+};
+
+int read_stats(bolap_ctx* ctx, int64_t* dst) {
+    BOLAP_TRY(pinned_reserve(ctx, sN * 8));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->result_dev, sN * 8, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(dst, ctx->pinned, sN * 8);
+    return BOLAP_OK;
+}
+
+// Runs the join + group-by; the hit groups end up as RowT rows at workspace
+// offset *o_rows and the statistics in `run`.
+int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bool exact,
+                 WsArena& ws, size_t* o_rows_out, ChainRun& run) {
+    int64_t* st = static_cast<int64_t*>(ctx->result_dev);
+    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(64), 0, ctx->stream, st);
+    {
+        TraceScope ts(ctx, "chain_stats");
+        hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
+                           ctx->stream, a->cust_key, a->cust_rows, a->order_key, a->order_rows,
+                           st);
+    }
+    BOLAP_TRY(launch_check("key_minmax_k"));
+    BOLAP_TRY(read_stats(ctx, run.st));
+    Ranges r{run.st[sCMin], run.st[sCMax], run.st[sOMin], run.st[sOMax]};
+    const int64_t cspan = a->cust_rows > 0 ? r.cmax - r.cmin + 1 : 0;
+    const int64_t ospan = a->order_rows > 0 ? r.omax - r.omin + 1 : 0;
+    const int64_t cwords = (cspan + 31) / 32 + 1;
+    const int64_t owords = (ospan + 31) / 32 + 1;
+
+    // Customer bitmap first; the group table is planned once its size is known.
+    const size_t o_cb = ws.plan(size_t(cwords) * 4);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    hipLaunchKernelGGL(fill_u32_k, dim3(grid_for(ctx, cwords)), dim3(kT), 0, ctx->stream,
+                       ws.at<uint32_t>(o_cb), cwords, 0u);
+    if (a->cust_rows > 0) {
+        TraceScope ts(ctx, "chain_cust");
+        hipLaunchKernelGGL(cust_bits_k, dim3(grid_for(ctx, a->cust_rows)), dim3(kT), 0,
+                           ctx->stream, a->cust_key, a->cust_segment, a->cust_rows, a->segment,
+                           r, ws.at<uint32_t>(o_cb), st);
+    }
+    BOLAP_TRY(launch_check("cust_bits_k"));
+    if (exact) {
+        if (a->order_rows > 0) {
+            TraceScope ts(ctx, "chain_count");
+            hipLaunchKernelGGL(order_count_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
+                               ctx->stream, a->order_day, a->order_cust, a->order_rows,
+                               a->day_lt, ws.at<uint32_t>(o_cb), cspan, r, st);
+        }
+        BOLAP_TRY(launch_check("order_count_k"));
+        BOLAP_TRY(read_stats(ctx, run.st));
+        n_expect = run.st[sCount];
+    }
+    const int cap_log2 = ceil_log2(n_expect * 2 < 1024 ? 1024 : n_expect * 2);
+    BOLAP_ARG_CHECK(cap_log2 <= 31, "bolap_chain: %lld groups exceed the table limit",
+                    (long long)n_expect);
+    const int64_t cap = int64_t(1) << cap_log2;
+    const size_t keep = ws.off;
+    const size_t o_keys = ws.plan(size_t(cap) * 4);
+    const size_t o_days = ws.plan(size_t(cap) * 4);
+    const size_t o_rev = ws.plan(size_t(cap) * 8);
+    const size_t o_hit = ws.plan(size_t(cap));
+    const size_t o_ob = ws.plan(size_t(owords) * 4);
+    const size_t o_rows = ws.plan(size_t(cap) * sizeof(RowT));
+    const size_t o_cnt = ws.plan(64);
+    BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
+    ws.base = static_cast<char*>(ctx->ws);
+    const uint32_t* cbits = ws.at<uint32_t>(o_cb);
+    GTable t{ws.at<int32_t>(o_keys), ws.at<int32_t>(o_days), ws.at<uint64_t>(o_rev),
+             ws.at<uint8_t>(o_hit), ws.at<uint32_t>(o_ob), cap_log2, cap / 2};
+    {
+        TraceScope ts(ctx, "chain_init");
+        hipLaunchKernelGGL(gtable_init_k, dim3(grid_for(ctx, cap, 8)), dim3(kT), 0, ctx->stream,
+                           t, owords);
+    }
+    BOLAP_TRY(launch_check("gtable_init_k"));
+    if (a->order_rows > 0) {
+        TraceScope ts(ctx, "chain_build");
+        hipLaunchKernelGGL(order_build_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
+                           ctx->stream, a->order_key, a->order_day, a->order_cust, a->order_rows,
+                           a->day_lt, cbits, cspan, r, t, st);
+    }
+    BOLAP_TRY(launch_check("order_build_k"));
+    if (a->fact_rows > 0 && ospan > 0) {
+        TraceScope ts(ctx, "chain_probe");
+        const unsigned g = grid_for(ctx, a->fact_rows / 4 + 1, 8);
+        const bool p64 = a->price_dtype == BOLAP_I64, d64 = a->disc_dtype == BOLAP_I64;
+        const int32_t* dp = a->fact_disc_price;
+        if (p64 && d64)
+            hipLaunchKernelGGL((fact_probe_k<int64_t, int64_t>), dim3(g), dim3(kT), 0, ctx->stream,
+                               a->fact_order, static_cast<const int64_t*>(a->fact_price),
+                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+        else if (p64)
+            hipLaunchKernelGGL((fact_probe_k<int64_t, uint8_t>), dim3(g), dim3(kT), 0, ctx->stream,
+                               a->fact_order, static_cast<const int64_t*>(a->fact_price),
+                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+        else if (d64)
+            hipLaunchKernelGGL((fact_probe_k<int32_t, int64_t>), dim3(g), dim3(kT), 0, ctx->stream,
+                               a->fact_order, static_cast<const int32_t*>(a->fact_price),
+                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+        else
+            hipLaunchKernelGGL((fact_probe_k<int32_t, uint8_t>), dim3(g), dim3(kT), 0, ctx->stream,
+                               a->fact_order, static_cast<const int32_t*>(a->fact_price),
+                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+    }
+    BOLAP_TRY(launch_check("fact_probe_k"));
+    auto* n_rows = ws.at<unsigned long long>(o_cnt);
+    hipLaunchKernelGGL(fill_u32_k, dim3(1), dim3(kT), 0, ctx->stream,
+                       reinterpret_cast<uint32_t*>(n_rows), 2, 0u);
+    {
+        TraceScope ts(ctx, "chain_compact");
+        hipLaunchKernelGGL(compact_groups_k, dim3(grid_for(ctx, cap, 8)), dim3(kT), 0,
+                           ctx->stream, t, ws.at<RowT>(o_rows), n_rows, st);
+    }
+    BOLAP_TRY(launch_check("compact_groups_k"));
+    BOLAP_TRY(read_stats(ctx, run.st));
+    BOLAP_TRY(pinned_reserve(ctx, 8));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(&run.n_groups, ctx->pinned, 8);
+    run.cap_log2 = cap_log2;
+    *o_rows_out = o_rows;
+    return BOLAP_OK;
+}
+
+// Orders the n rows at workspace offset o_rows and writes the first `limit`
+// as bolap_group_row at workspace offset *o_out.
+int order_rows(bolap_ctx* ctx, WsArena& ws, size_t o_rows, int64_t n, int64_t limit,
+               const ChainRun& run, size_t* o_out, int* passes) {
+    *passes = 0;
+    const int64_t m = std::min(limit, n);
+    const size_t keep = ws.off;
+    const size_t o_k = ws.plan(size_t(n > 0 ? n : 1) * 8);
+    const size_t o_i = ws.plan(size_t(n > 0 ? n : 1) * 4);
+    const size_t o_ko = ws.plan(size_t(m > 0 ? m : 1) * 8);
+    const size_t o_io = ws.plan(size_t(m > 0 ? m : 1) * 4);
+    *o_out = ws.plan(size_t(m > 0 ? m : 1) * sizeof(bolap_group_row));
+    const size_t o_scr =
+        ws.plan(std::max(radix_sort_scratch_bytes(ctx, n), topk_scratch_bytes(n)));
+    BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
+    ws.base = static_cast<char*>(ctx->ws);
+    if (m <= 0) return BOLAP_OK;
+    const RowT* rows = ws.at<RowT>(o_rows);
+    uint64_t* keys = ws.at<uint64_t>(o_k);
+    uint32_t* idx = ws.at<uint32_t>(o_i);
+    const int64_t rmin = run.st[sRMin], rmax = run.st[sRMax];
+    const int64_t dmin = run.st[sDMin], dmax = run.st[sDMax];
+    const int64_t kmin = run.st[sOMin], kmax = run.st[sOMax];
+    const int rb = bits_for(uint64_t(rmax) - uint64_t(rmin));
+    const int db = bits_for(uint64_t(dmax) - uint64_t(dmin));
+    const int kb = bits_for(uint64_t(kmax) - uint64_t(kmin));
+    // Revenue spans beyond int64 (wrapped sums) take the fallback.
+    const bool rev_ok = rmax >= rmin && (rmax - rmin) >= 0;
+    TraceScope ts(ctx, "chain_order");
+    const unsigned g = grid_for(ctx, n);
+    const uint32_t* sel = nullptr;
+    if (rev_ok && rb + db + kb <= 64) {
+        Pack p{rmax, dmin, kmin, kb, db};
+        hipLaunchKernelGGL(pack_rows_k, dim3(g), dim3(kT), 0, ctx->stream, rows, n, p, keys, idx);
+        BOLAP_TRY(launch_check("pack_rows_k"));
+        if (m < n && m <= 4096) {
+            BOLAP_TRY(topk_pairs(ctx, keys, idx, n, int32_t(m), ws.at<uint64_t>(o_ko),
+                                 ws.at<uint32_t>(o_io), ws.at<void>(o_scr)));
+            sel = ws.at<uint32_t>(o_io);
+        } else {
+            BOLAP_TRY(radix_sort_pairs(ctx, keys, idx, n, rb + db + kb, ws.at<void>(o_scr),
+                                       passes));
+            sel = idx;
+        }
+    } else {
+        // Three stable passes: key, then day, then revenue descending.
+        hipLaunchKernelGGL(iota_k, dim3(g), dim3(kT), 0, ctx->stream, idx, n);
+        int total = 0;
+        for (int field = 0; field < 3; ++field) {
+            hipLaunchKernelGGL(field_keys_k, dim3(g), dim3(kT), 0, ctx->stream, rows, idx, n,
+                               field, keys);
+            BOLAP_TRY(launch_check("field_keys_k"));
+            int p = 0;
+            BOLAP_TRY(radix_sort_pairs(ctx, keys, idx, n, field == 2 ? 64 : 32,
+                                       ws.at<void>(o_scr), &p));
+            total += p;
+        }
+        *passes = total;
+        sel = idx;
+    }
+    hipLaunchKernelGGL(gather_rows_k, dim3(grid_for(ctx, m)), dim3(kT), 0, ctx->stream, rows, sel,
+                       m, ws.at<bolap_group_row>(*o_out));
+    return launch_check("gather_rows_k");
+}
+
+int chain_validate(const bolap_chain_args* a) {
+    BOLAP_ARG_CHECK(a, "bolap_chain: null args");
+    BOLAP_ARG_CHECK(a->cust_rows >= 0 && a->order_rows >= 0 && a->fact_rows >= 0,
+                    "bolap_chain: negative rows");
+    BOLAP_ARG_CHECK(a->order_rows < (int64_t(1) << 31), "bolap_chain: order_rows >= 2^31");
+    BOLAP_ARG_CHECK(a->cust_rows == 0 || (a->cust_key && a->cust_segment),
+                    "bolap_chain: null customer column");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || (a->order_key && a->order_day && a->order_cust),
+                    "bolap_chain: null order column");
+    BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_order && (a->fact_disc_price ||
+                                                            (a->fact_price && a->fact_disc))),
+                    "bolap_chain: null fact column");
+    BOLAP_ARG_CHECK(a->fact_rows == 0 || aligned(a->fact_order, 16),
+                    "bolap_chain: fact_order must be 16-byte aligned");
+    BOLAP_ARG_CHECK(a->fact_disc_price || a->price_dtype == BOLAP_I64 ||
+                        a->price_dtype == BOLAP_I32,
+                    "bolap_chain: price dtype must be i64 or i32");
+    BOLAP_ARG_CHECK(a->fact_disc_price || a->disc_dtype == BOLAP_I64 || a->disc_dtype == BOLAP_U8,
+                    "bolap_chain: disc dtype must be i64 or u8");
+    return BOLAP_OK;
+}
+
+// Full pipeline: group-by (estimate-sized with exact fallback), ordering,
+// and the first `limit` rows (all when limit < 0) copied to `host_out`.
+int chain_run(bolap_ctx* ctx, const bolap_chain_args* a, int64_t limit,
+              bolap_group_row* host_out, int64_t max_rows, int64_t* n_out,
+              bolap_chain_stats* stats) {
+    BOLAP_TRY(chain_validate(a));
+    ChainRun run{};
+    WsArena ws;
+    size_t o_rows = 0;
+    int fallbacks = 0;
+    const bool exact = a->capacity_hint <= 0;
+    BOLAP_TRY(chain_groups(ctx, a, a->capacity_hint, exact, ws, &o_rows, run));
+    if (!exact && (run.st[sFlags] & 1)) {
+        fallbacks = 1;  // the estimate was too small: exact sizing
+        ws = WsArena{};
+        BOLAP_TRY(chain_groups(ctx, a, 0, true, ws, &o_rows, run));
+    }
+    if (run.st[sFlags] & 1) {
+        set_error("bolap_chain: exact-sized group table overflowed");
+        return BOLAP_ERR_CAPACITY;
+    }
+    const int64_t n = run.n_groups;
+    const int64_t m = limit < 0 ? n : std::min<int64_t>(limit, n);
+    if (m > max_rows) {
+        set_error("bolap_chain: %lld rows exceed max_rows %lld", (long long)m,
+                  (long long)max_rows);
+        return BOLAP_ERR_CAPACITY;
+    }
+    size_t o_out = 0;
+    int passes = 0;
+    BOLAP_TRY(order_rows(ctx, ws, o_rows, n, m, run, &o_out, &passes));
+    if (m > 0) {
+        const size_t bytes = size_t(m) * sizeof(bolap_group_row);
+        BOLAP_TRY(pinned_reserve(ctx, bytes));
+        BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ws.at<bolap_group_row>(o_out), bytes,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(host_out, ctx->pinned, bytes);
+    }
+    *n_out = m;
+    trace_count(ctx, "chain_customer_rows_emitted", run.st[sCustPass]);
+    trace_count(ctx, "chain_orders_rows_emitted", run.st[sOrderPass]);
+    trace_count(ctx, "chain_join_rows_emitted", run.st[sMatch]);
+    trace_count(ctx, "chain_agg_groups_created", n);
+    if (stats) {
+        stats->cust_pass = run.st[sCustPass];
+        stats->order_pass = run.st[sOrderPass];
+        stats->fact_matches = run.st[sMatch];
+        stats->groups = n;
+        stats->capacity = int64_t(1) << run.cap_log2;
+        stats->fallbacks = fallbacks;
+        stats->sort_passes = passes;
+    }
+    return BOLAP_OK;
+}
+
+}  // namespace
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" {
+
+int bolap_chain_topk(bolap_ctx* ctx, const bolap_chain_args* args, int32_t k,
+                     bolap_group_row* out, int32_t* n_out, bolap_chain_stats* stats) {
+    BOLAP_ARG_CHECK(ctx && args && n_out && (out || k == 0) && k >= 0,
+                    "bolap_chain_topk: bad arguments");
+    int64_t n = 0;
+    BOLAP_TRY(chain_run(ctx, args, k, out, k, &n, stats));
+    *n_out = int32_t(n);
+    return BOLAP_OK;
+}
+
+int bolap_chain_groups_sorted(bolap_ctx* ctx, const bolap_chain_args* args,
+                              bolap_group_row* out, int64_t max_rows, int64_t* n_out,
+                              bolap_chain_stats* stats) {
+    BOLAP_ARG_CHECK(ctx && args && n_out && max_rows >= 0,
+                    "bolap_chain_groups_sorted: bad arguments");
+    return chain_run(ctx, args, -1, out, max_rows, n_out, stats);
+}
+
+}  // extern "C"

@@ -89,8 +89,10 @@ void set_error(const char* fmt, ...);
         if (_s != BOLAP_OK) return _s;                                             \
     } while (0)
 
-// Ensures ctx->ws holds at least `bytes`.
-int ws_reserve(bolap_ctx* ctx, size_t bytes);
+// Ensures ctx->ws holds at least `bytes`.  When it has to grow, the first
+// `keep` bytes are copied into the new buffer (regions planned earlier in
+// the same call stay valid as offsets; raw pointers must be re-derived).
+int ws_reserve(bolap_ctx* ctx, size_t bytes, size_t keep = 0);
 // Ensures ctx->pinned holds at least `bytes`.
 int pinned_reserve(bolap_ctx* ctx, size_t bytes);
 // Checks the last kernel launch.

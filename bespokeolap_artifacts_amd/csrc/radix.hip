@@ -1,0 +1,425 @@
+// radix.hip — stable LSD radix sort of (u64 key, u32 value) pairs and an
+// LDS-histogram radix select for top-k.
+//
+// Replaces the 16-bit-digit counting sorts of the reference
+// (bespoke_tpch/query_q10.cpp:134-166 radix_sort_rows_by_orderkey and
+// :318-373 radix_sort_groups, where descending order comes from sorting ~key)
+// with a device-wide sort:
+//   per pass (8-bit digit): histogram kernel (LDS counters per workgroup over
+//   a contiguous chunk) -> exclusive scan of the (digit, workgroup) counts ->
+//   scatter kernel that ranks keys stably inside each 256-key tile with wave
+//   ballots (peer masks over the 8 digit bits) and per-wave LDS counts.
+// Top-k: rounds of 12-bit LDS histograms over the remaining candidates find
+// the digit bucket holding the k-th key; keys below it are certain, keys in
+// it stay candidates; once at most kSelMax remain, one workgroup sorts them by
+// (key, input index) with a bitonic sort in LDS.
+#include <algorithm>
+#include <cstring>
+
+#include "prims.hpp"
+#include "radix.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kRT = 256;        // threads per workgroup
+constexpr int kBins = 256;      // 8-bit digits
+constexpr int kTileK = kRT;     // keys per tile in the scatter kernel
+
+BOLAP_DEV uint32_t digit_of(uint64_t key, uint64_t kmask, int shift) {
+    return uint32_t(((key & kmask) >> shift) & 0xFF);
+}
+
+__global__ __launch_bounds__(kRT) void rs_hist_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                 int64_t chunk, uint64_t kmask, int shift,
+                                                 int nblocks, int64_t* __restrict__ hist) {
+    __shared__ uint32_t h[kBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t b0 = int64_t(blockIdx.x) * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kRT)
+        atomicAdd(&h[digit_of(keys[i], kmask, shift)], 1u);
+    __syncthreads();
+    hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__ kin,
+                                                    const uint32_t* __restrict__ vin, int64_t n,
+                                                    int64_t chunk, uint64_t kmask, int shift,
+                                                    int nblocks,
+                                                    const int64_t* __restrict__ offsets,
+                                                    uint64_t* __restrict__ kout,
+                                                    uint32_t* __restrict__ vout) {
+    constexpr int kW = kRT / kWave;
+    __shared__ int64_t run[kBins];
+    __shared__ uint32_t wcnt[kW][kBins];
+    run[threadIdx.x] = offsets[int64_t(threadIdx.x) * nblocks + blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    const int64_t b0 = int64_t(blockIdx.x) * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t base = b0; base < b1; base += kTileK) {
+        for (int w = 0; w < kW; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        const bool valid = i < b1;
+        const uint64_t key = valid ? kin[i] : 0;
+        const uint32_t val = valid ? vin[i] : 0;
+        const uint32_t d = digit_of(key, kmask, shift);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t wrank = uint32_t(__popcll(peers & lt_mask));
+        if (valid && wrank == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            int64_t pos = run[d] + wrank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int w = 0; w < kW; ++w) tot += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- top-k
+constexpr int kSelBits = 12;
+constexpr int kSelBins = 1 << kSelBits;
+constexpr int kHistBlocks = 64;
+constexpr int kSelMax = 8192;   // candidates sorted by one workgroup in LDS
+
+struct KeyIdx {
+    uint64_t key;
+    uint64_t idx;  // input index (stable tie order)
+};
+
+// The selection digit of a candidate: kSelBits bits of the key (field 0)
+// or of the input index (field 1, used once every key bit is fixed).
+BOLAP_DEV uint32_t sel_digit(const KeyIdx& e, int field, int shift) {
+    return uint32_t((field ? e.idx : e.key) >> shift) & (kSelBins - 1);
+}
+
+// Histogram of one digit among the candidates.  `cand` may be null (then
+// the candidates are the whole input `keys`, tagged by position).
+__global__ __launch_bounds__(kRT) void sel_hist_k(const uint64_t* __restrict__ keys,
+                                                  const KeyIdx* __restrict__ cand, int64_t n,
+                                                  int field, int shift,
+                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kSelBins];
+    for (int b = threadIdx.x; b < kSelBins; b += kRT) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = int64_t(blockIdx.x) * kRT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kRT) {
+        const KeyIdx e = cand ? cand[i] : KeyIdx{keys[i], uint64_t(i)};
+        atomicAdd(&h[sel_digit(e, field, shift)], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSelBins; b += kRT)
+        hist[int64_t(blockIdx.x) * kSelBins + b] = h[b];
+}
+
+// Sums the per-workgroup histograms and finds the bucket b* holding the
+// need-th smallest candidate: info = {b*, count below b*, count in b*}.
+__global__ __launch_bounds__(1024) void sel_pick_k(const uint32_t* __restrict__ hist,
+                                                   int nblocks, int64_t need,
+                                                   int64_t* __restrict__ info) {
+    __shared__ uint32_t tot[kSelBins];
+    for (int b = threadIdx.x; b < kSelBins; b += 1024) {
+        uint32_t s = 0;
+        for (int r = 0; r < nblocks; ++r) s += hist[int64_t(r) * kSelBins + b];
+        tot[b] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        int b = 0;
+        for (; b < kSelBins; ++b) {
+            if (run + tot[b] >= need) break;
+            run += tot[b];
+        }
+        if (b == kSelBins) b = kSelBins - 1;
+        info[0] = b;
+        info[1] = run;
+        info[2] = tot[b];
+    }
+}
+
+// Appends candidates below b* to `sure` and those in b* to `next`.
+__global__ __launch_bounds__(kRT) void sel_split_k(const uint64_t* __restrict__ keys,
+                                                   const KeyIdx* __restrict__ cand, int64_t n,
+                                                   int field, int shift,
+                                                   const int64_t* __restrict__ info,
+                                                   KeyIdx* __restrict__ sure,
+                                                   unsigned long long* __restrict__ n_sure,
+                                                   KeyIdx* __restrict__ next,
+                                                   unsigned long long* __restrict__ n_next) {
+    const uint32_t bstar = uint32_t(info[0]);
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    for (int64_t base = int64_t(blockIdx.x) * kRT; base < n; base += int64_t(gridDim.x) * kRT) {
+        const int64_t i = base + threadIdx.x;
+        KeyIdx e{0, 0};
+        if (i < n) e = cand ? cand[i] : KeyIdx{keys[i], uint64_t(i)};
+        const uint32_t d = sel_digit(e, field, shift);
+        const bool lo = i < n && d < bstar;
+        const bool eq = i < n && d == bstar;
+        // wave-aggregated appends: one atomic per wave and target
+        const uint64_t mlo = __ballot(lo), meq = __ballot(eq);
+        unsigned long long blo = 0, beq = 0;
+        if (lane == 0) {
+            if (mlo) blo = atomicAdd(n_sure, (unsigned long long)__popcll(mlo));
+            if (meq) beq = atomicAdd(n_next, (unsigned long long)__popcll(meq));
+        }
+        blo = __shfl(blo, 0, kWave);
+        beq = __shfl(beq, 0, kWave);
+        if (lo) sure[blo + __popcll(mlo & lt_mask)] = e;
+        if (eq) next[beq + __popcll(meq & lt_mask)] = e;
+    }
+}
+
+// One workgroup: bitonic sort of m <= kSelMax (key, idx) pairs in LDS,
+// writes the first k keys and their values (vals[idx]).
+__global__ __launch_bounds__(1024) void sel_sort_k(const KeyIdx* __restrict__ a, int64_t m,
+                                                   int32_t k, const uint32_t* __restrict__ vals,
+                                                   uint64_t* __restrict__ kout,
+                                                   uint32_t* __restrict__ vout) {
+    extern __shared__ KeyIdx s[];
+    int p = 1;
+    while (p < m) p <<= 1;
+    for (int i = threadIdx.x; i < p; i += 1024)
+        s[i] = i < m ? a[i] : KeyIdx{~uint64_t(0), ~uint64_t(0)};
+    __syncthreads();
+    for (int size = 2; size <= p; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < p / 2; t += 1024) {
+                const int lo = 2 * stride * (t / stride) + (t % stride);
+                const int hi = lo + stride;
+                const bool up = ((lo & size) == 0);
+                const KeyIdx x = s[lo], y = s[hi];
+                const bool gt = x.key > y.key || (x.key == y.key && x.idx > y.idx);
+                if (gt == up) {
+                    s[lo] = y;
+                    s[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < k && i < m; i += 1024) {
+        kout[i] = s[i].key;
+        if (vout) vout[i] = vals ? vals[s[i].idx] : uint32_t(s[i].idx);
+    }
+}
+
+__global__ __launch_bounds__(kRT) void sel_tag_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                 KeyIdx* __restrict__ dst) {
+    for (int64_t i = int64_t(blockIdx.x) * kRT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kRT)
+        dst[i] = KeyIdx{keys[i], uint64_t(i)};
+}
+
+__global__ __launch_bounds__(kRT) void sel_concat_k(const KeyIdx* __restrict__ src, int64_t n,
+                                                    KeyIdx* __restrict__ dst) {
+    for (int64_t i = int64_t(blockIdx.x) * kRT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kRT)
+        dst[i] = src[i];
+}
+
+__global__ void zero_u64_k(unsigned long long* p, int n) {
+    if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+size_t radix_sort_scratch_bytes(bolap_ctx* ctx, int64_t n) {
+    (void)ctx;
+    const int64_t nb = 1024;
+    WsArena w;
+    w.plan(size_t(n) * 8);
+    w.plan(size_t(n) * 4);
+    w.plan(size_t(kBins) * nb * 8);
+    w.plan(scan_scratch_bytes(int64_t(kBins) * nb));
+    return w.off;
+}
+
+int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits,
+                     void* scratch, int* passes_out) {
+    if (passes_out) *passes_out = 0;
+    if (n < 2 || key_bits <= 0) return BOLAP_OK;
+    if (key_bits > 64) key_bits = 64;
+    // Contiguous chunks per workgroup keep the scatter stable.
+    int64_t nblocks = (n + 8 * kTileK - 1) / (8 * kTileK);
+    if (nblocks > 1024) nblocks = 1024;
+    if (nblocks < 1) nblocks = 1;
+    int64_t chunk = (n + nblocks - 1) / nblocks;
+    chunk = (chunk + kTileK - 1) / kTileK * kTileK;
+    nblocks = (n + chunk - 1) / chunk;
+    WsArena w;
+    const size_t o_k = w.plan(size_t(n) * 8);
+    const size_t o_v = w.plan(size_t(n) * 4);
+    const size_t o_h = w.plan(size_t(kBins) * 1024 * 8);
+    const size_t o_s = w.plan(scan_scratch_bytes(int64_t(kBins) * 1024));
+    w.base = static_cast<char*>(scratch);
+    uint64_t* tk = w.at<uint64_t>(o_k);
+    uint32_t* tv = w.at<uint32_t>(o_v);
+    int64_t* hist = w.at<int64_t>(o_h);
+    const uint64_t kmask = key_bits >= 64 ? ~uint64_t(0) : ((uint64_t(1) << key_bits) - 1);
+    uint64_t* sk = keys;
+    uint32_t* sv = vals;
+    uint64_t* dk = tk;
+    uint32_t* dv = tv;
+    int passes = 0;
+    for (int shift = 0; shift < key_bits; shift += 8, ++passes) {
+        hipLaunchKernelGGL(rs_hist_k, dim3(unsigned(nblocks)), dim3(kRT), 0, ctx->stream, sk, n,
+                           chunk, kmask, shift, int(nblocks), hist);
+        BOLAP_TRY(launch_check("rs_hist_k"));
+        BOLAP_TRY(exclusive_scan_i64(ctx, hist, int64_t(kBins) * nblocks, w.at<void>(o_s)));
+        hipLaunchKernelGGL(rs_scatter_k, dim3(unsigned(nblocks)), dim3(kRT), 0, ctx->stream, sk,
+                           sv, n, chunk, kmask, shift, int(nblocks), hist, dk, dv);
+        BOLAP_TRY(launch_check("rs_scatter_k"));
+        std::swap(sk, dk);
+        std::swap(sv, dv);
+    }
+    if (sk != keys) {
+        BOLAP_HIP_TRY(hipMemcpyAsync(keys, sk, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        BOLAP_HIP_TRY(hipMemcpyAsync(vals, sv, size_t(n) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (passes_out) *passes_out = passes;
+    return BOLAP_OK;
+}
+
+size_t topk_scratch_bytes(int64_t n) {
+    WsArena w;
+    w.plan(size_t(kHistBlocks) * kSelBins * 4);
+    w.plan(64);
+    w.plan(64);
+    w.plan(size_t(n) * sizeof(KeyIdx));  // sure
+    w.plan(size_t(n) * sizeof(KeyIdx));  // cand A
+    w.plan(size_t(n) * sizeof(KeyIdx));  // cand B
+    return w.off;
+}
+
+int topk_pairs(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64_t n, int32_t k,
+               uint64_t* kout, uint32_t* vout, void* scratch) {
+    if (n <= 0 || k <= 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(k <= kSelMax, "topk: k = %d exceeds %d", k, kSelMax);
+    WsArena w;
+    const size_t o_h = w.plan(size_t(kHistBlocks) * kSelBins * 4);
+    const size_t o_info = w.plan(64);
+    const size_t o_cnt = w.plan(64);
+    const size_t o_sure = w.plan(size_t(n) * sizeof(KeyIdx));
+    const size_t o_ca = w.plan(size_t(n) * sizeof(KeyIdx));
+    const size_t o_cb = w.plan(size_t(n) * sizeof(KeyIdx));
+    w.base = static_cast<char*>(scratch);
+    auto* hist = w.at<uint32_t>(o_h);
+    auto* info = w.at<int64_t>(o_info);
+    auto* cnt = w.at<unsigned long long>(o_cnt);  // [0] sure, [1] next
+    KeyIdx* sure = w.at<KeyIdx>(o_sure);
+    KeyIdx* ca = w.at<KeyIdx>(o_ca);
+    KeyIdx* cb = w.at<KeyIdx>(o_cb);
+    BOLAP_TRY(pinned_reserve(ctx, 64));
+    auto* host = static_cast<int64_t*>(ctx->pinned);
+
+    const KeyIdx* cand = nullptr;  // null: the whole input
+    int64_t m = n;                 // candidates still undecided
+    int64_t n_sure = 0;            // candidates certainly in the top k
+    int64_t need = k;              // how many of the undecided are in
+    // Digit schedule: key bits from the top (the last digit overlaps bits
+    // already fixed, which is harmless), then index bits for ties.
+    const int sched_field[8] = {0, 0, 0, 0, 0, 0, 1, 1};
+    const int sched_shift[8] = {52, 40, 28, 16, 4, 0, 20, 8};
+    hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, ctx->stream, cnt, 2);
+    for (int r = 0; r < 9 && n_sure + m > kSelMax && m > need; ++r) {
+        const int field = r < 8 ? sched_field[r] : 1;
+        const int sh = r < 8 ? sched_shift[r] : 0;
+        int64_t hb = (m + kRT - 1) / kRT;
+        if (hb > kHistBlocks) hb = kHistBlocks;
+        hipLaunchKernelGGL(sel_hist_k, dim3(unsigned(hb)), dim3(kRT), 0, ctx->stream, keys, cand,
+                           m, field, sh, hist);
+        hipLaunchKernelGGL(sel_pick_k, dim3(1), dim3(1024), 0, ctx->stream, hist, int(hb), need,
+                           info);
+        KeyIdx* next = (cand == ca) ? cb : ca;
+        hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 1, 1);
+        int64_t sb = (m + kRT - 1) / kRT;
+        if (sb > int64_t(ctx->num_cus) * 4) sb = int64_t(ctx->num_cus) * 4;
+        hipLaunchKernelGGL(sel_split_k, dim3(unsigned(sb)), dim3(kRT), 0, ctx->stream, keys, cand,
+                           m, field, sh, info, sure, cnt, next, cnt + 1);
+        BOLAP_TRY(launch_check("topk rounds"));
+        BOLAP_HIP_TRY(hipMemcpyAsync(host, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
+        BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        n_sure = host[0];
+        m = host[1];
+        need = k - n_sure;
+        cand = next;
+    }
+    // Final candidates: the certain ones followed by the undecided ones
+    // (all of them are needed, or few enough to sort).
+    const KeyIdx* fin = sure;
+    int64_t nf = n_sure + m;
+    if (cand == nullptr) {
+        hipLaunchKernelGGL(sel_tag_k, dim3(unsigned((n + kRT - 1) / kRT)), dim3(kRT), 0,
+                           ctx->stream, keys, n, ca);
+        BOLAP_TRY(launch_check("sel_tag_k"));
+        fin = ca;
+        nf = n;
+    } else if (m > 0) {
+        hipLaunchKernelGGL(sel_concat_k, dim3(unsigned((m + kRT - 1) / kRT)), dim3(kRT), 0,
+                           ctx->stream, cand, m, sure + n_sure);
+        BOLAP_TRY(launch_check("sel_concat_k"));
+    }
+    if (nf > kSelMax) {
+        set_error("topk: %lld candidates left after every digit", (long long)nf);
+        return BOLAP_ERR_CAPACITY;
+    }
+    int p = 1;
+    while (p < nf) p <<= 1;
+    const size_t lds = size_t(p) * sizeof(KeyIdx);
+    BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(sel_sort_k),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(size_t(kSelMax) * sizeof(KeyIdx))));
+    hipLaunchKernelGGL(sel_sort_k, dim3(1), dim3(1024), lds, ctx->stream, fin, nf, k, vals, kout,
+                       vout);
+    return launch_check("sel_sort_k");
+}
+
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" {
+
+int bolap_radix_sort_u64(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n,
+                         int32_t key_bits) {
+    BOLAP_ARG_CHECK(ctx && n >= 0, "bolap_radix_sort_u64: bad arguments");
+    BOLAP_ARG_CHECK(n < (int64_t(1) << 32), "bolap_radix_sort_u64: n >= 2^32");
+    if (n < 2) return BOLAP_OK;
+    BOLAP_ARG_CHECK(keys && vals, "bolap_radix_sort_u64: null array");
+    BOLAP_ARG_CHECK(key_bits >= 0 && key_bits <= 64, "bolap_radix_sort_u64: key_bits %d",
+                    key_bits);
+    BOLAP_TRY(ws_reserve(ctx, radix_sort_scratch_bytes(ctx, n)));
+    TraceScope ts(ctx, "radix_sort");
+    return radix_sort_pairs(ctx, keys, vals, n, key_bits, ctx->ws, nullptr);
+}
+
+int bolap_topk_u64(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64_t n,
+                   int32_t k, uint64_t* keys_out, uint32_t* vals_out) {
+    BOLAP_ARG_CHECK(ctx && n >= 0 && k >= 0, "bolap_topk_u64: bad arguments");
+    BOLAP_ARG_CHECK(n < (int64_t(1) << 32), "bolap_topk_u64: n >= 2^32");
+    if (n == 0 || k == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(keys && keys_out, "bolap_topk_u64: null array");
+    BOLAP_TRY(ws_reserve(ctx, topk_scratch_bytes(n)));
+    TraceScope ts(ctx, "topk");
+    return topk_pairs(ctx, keys, vals, n, k, keys_out, vals_out, ctx->ws);
+}
+
+}  // extern "C"

@@ -16,15 +16,19 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
-int ws_reserve(bolap_ctx* ctx, size_t bytes) {
+int ws_reserve(bolap_ctx* ctx, size_t bytes, size_t keep) {
     if (bytes <= ctx->ws_size) return BOLAP_OK;
     size_t want = ctx->ws_size ? ctx->ws_size : (size_t(64) << 20);
     while (want < bytes) want *= 2;
+    void* fresh = nullptr;
+    BOLAP_HIP_TRY(hipMalloc(&fresh, want));
+    if (ctx->ws && keep > 0) {
+        const size_t n = keep < ctx->ws_size ? keep : ctx->ws_size;
+        BOLAP_HIP_TRY(hipMemcpyAsync(fresh, ctx->ws, n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->ws) BOLAP_HIP_TRY(hipFree(ctx->ws));
-    ctx->ws = nullptr;
-    ctx->ws_size = 0;
-    BOLAP_HIP_TRY(hipMalloc(&ctx->ws, want));
+    ctx->ws = fresh;
     ctx->ws_size = want;
     return BOLAP_OK;
 }

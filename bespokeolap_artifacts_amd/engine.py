@@ -188,6 +188,68 @@ class Engine:
         return {"revenue": r.revenue, "build_rows": r.build_rows, "matches": r.matches,
                 "capacity": r.capacity, "placement": r.placement, "fallbacks": r.fallbacks}
 
+    # ------------------------------------------------- chain join + group-by
+    def chain_args(self, t: dict, segment: int, day_lt: int, capacity_hint: int = 0,
+                   disc_price=None) -> N.ChainArgs:
+        a = N.ChainArgs()
+        a.cust_rows = int(t["cust_key"].numel())
+        a.cust_key, a.cust_segment = ptr(t["cust_key"]), ptr(t["cust_segment"])
+        a.segment = int(segment)
+        a.order_rows = int(t["order_key"].numel())
+        a.order_key, a.order_day = ptr(t["order_key"]), ptr(t["order_day"])
+        a.order_cust = ptr(t["order_cust"])
+        a.day_lt = int(day_lt)
+        a.fact_rows = int(t["fact_order"].numel())
+        a.fact_order = ptr(t["fact_order"])
+        a.fact_price = ptr(t.get("fact_price"))
+        a.price_dtype = dtype_code(t["fact_price"]) if t.get("fact_price") is not None else N.I64
+        a.fact_disc = ptr(t.get("fact_disc"))
+        a.disc_dtype = dtype_code(t["fact_disc"]) if t.get("fact_disc") is not None else N.I64
+        a.fact_disc_price = ptr(disc_price)
+        a.capacity_hint = int(capacity_hint)
+        return a
+
+    @staticmethod
+    def _chain_stats(st: N.ChainStats) -> dict:
+        return {"cust_pass": st.cust_pass, "order_pass": st.order_pass,
+                "fact_matches": st.fact_matches, "groups": st.groups, "capacity": st.capacity,
+                "fallbacks": st.fallbacks, "sort_passes": st.sort_passes}
+
+    def chain_topk(self, a: N.ChainArgs, k: int):
+        out = (N.GroupRow * max(1, k))()
+        n = C.c_int32(0)
+        st = N.ChainStats()
+        N.check(self.lib.bolap_chain_topk(self.ctx, C.byref(a), k, out, C.byref(n),
+                                          C.byref(st)))
+        return [(out[i].key, out[i].day, out[i].revenue) for i in range(n.value)], \
+            self._chain_stats(st)
+
+    def chain_groups_sorted(self, a: N.ChainArgs, max_rows: int):
+        out = (N.GroupRow * max(1, max_rows))()
+        n = C.c_int64(0)
+        st = N.ChainStats()
+        N.check(self.lib.bolap_chain_groups_sorted(self.ctx, C.byref(a), out, max_rows,
+                                                   C.byref(n), C.byref(st)))
+        import numpy as np
+        arr = np.frombuffer(out, dtype=np.dtype([("key", "<i4"), ("day", "<i4"),
+                                                 ("revenue", "<i8")]), count=n.value).copy()
+        return arr, self._chain_stats(st)
+
+    # ------------------------------------------------------------ radix sort
+    def radix_sort(self, keys, vals, key_bits: int = 64) -> None:
+        """Sorts device uint64 keys (int64 tensor) with int32 payloads in place."""
+        N.check(self.lib.bolap_radix_sort_u64(self.ctx, ptr(keys), ptr(vals), keys.numel(),
+                                              key_bits))
+
+    def topk(self, keys, vals, k: int):
+        torch = _torch()
+        ko = torch.empty(max(1, k), dtype=torch.int64, device=keys.device)
+        vo = torch.empty(max(1, k), dtype=torch.int32, device=keys.device)
+        N.check(self.lib.bolap_topk_u64(self.ctx, ptr(keys), ptr(vals), keys.numel(), k,
+                                        ptr(ko), ptr(vo)))
+        m = min(k, keys.numel())
+        return ko[:m], vo[:m]
+
     # ------------------------------------------------------- generators
     def gen_lineitem(self, seed: int, rows: int) -> dict:
         torch = _torch()

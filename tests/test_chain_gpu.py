@@ -1,0 +1,112 @@
+"""Chain join + high-cardinality group-by + ordering on the GPU vs the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+SEED = 42
+
+
+def host(t):
+    return {k: v.cpu().numpy() for k, v in t.items()}
+
+
+def want_sorted(t, seg, day_lt, disc_price=None):
+    rows, stats = oracle.chain_groups(host(t), seg, day_lt,
+                                      None if disc_price is None else disc_price.cpu().numpy())
+    return oracle.sort_groups(rows), stats
+
+
+def as_array(rows):
+    a = np.zeros(len(rows), oracle.GROUP_DTYPE)
+    for i, (k, d, r) in enumerate(rows):
+        a[i] = (k, d, r)
+    return a
+
+
+def check_stats(got, st):
+    for k in ("cust_pass", "order_pass", "fact_matches", "groups"):
+        assert got[k] == st[k], k
+
+
+@pytest.mark.parametrize("cust,orders", [(10, 40), (1500, 15_000), (150_000, 1_500_000)])
+def test_chain_topk_and_all_groups(engine, cust, orders):
+    t = engine.gen_chain(SEED, cust, orders)
+    want, st = want_sorted(t, 1, 1155)
+    a = engine.chain_args(t, 1, 1155)
+    top, gs = engine.chain_topk(a, 10)
+    assert np.array_equal(as_array(top), want[:10])
+    check_stats(gs, st)
+    allg, gs2 = engine.chain_groups_sorted(a, len(want) + 5)
+    assert np.array_equal(allg, want)
+    check_stats(gs2, st)
+
+
+def test_chain_full_c3_size(engine):
+    t = engine.gen_chain(SEED, 1_500_000, 15_000_000)
+    want, st = want_sorted(t, 1, 1155)
+    top, gs = engine.chain_topk(engine.chain_args(t, 1, 1155), 10)
+    assert np.array_equal(as_array(top), want[:10])
+    check_stats(gs, st)
+    assert 1_300_000 < st["groups"] < 1_600_000
+
+
+@pytest.mark.parametrize("hint", [1, 1000, 100_000, 10_000_000])
+def test_chain_capacity_hints_never_change_results(engine, hint):
+    t = engine.gen_chain(SEED, 20_000, 200_000)
+    want, st = want_sorted(t, 3, 1800)
+    top, gs = engine.chain_topk(engine.chain_args(t, 3, 1800, capacity_hint=hint), 25)
+    assert np.array_equal(as_array(top), want[:25])
+    assert gs["fallbacks"] == (1 if hint * 2 < st["order_pass"] else gs["fallbacks"])
+    if hint >= st["order_pass"]:
+        assert gs["fallbacks"] == 0
+
+
+def test_chain_shuffled_fact_narrow_types_and_disc_price(engine):
+    import torch
+    t = engine.gen_chain(SEED, 5000, 60_000)
+    perm = torch.randperm(t["fact_order"].numel(), generator=torch.Generator().manual_seed(3))
+    perm = perm.cuda()
+    for k in ("fact_order", "fact_price", "fact_disc"):
+        t[k] = t[k][perm].contiguous()
+    want, _ = want_sorted(t, 2, 900)
+    top, _ = engine.chain_topk(engine.chain_args(t, 2, 900), 50)
+    assert np.array_equal(as_array(top), want[:50])
+    t32 = dict(t)
+    t32["fact_price"] = t["fact_price"].to(torch.int32)
+    t32["fact_disc"] = t["fact_disc"].to(torch.uint8)
+    allg, _ = engine.chain_groups_sorted(engine.chain_args(t32, 2, 900), 100_000)
+    assert np.array_equal(allg, want)
+    dp = ((t["fact_price"] * (100 - t["fact_disc"])) // 100).to(torch.int32)
+    want_dp, _ = want_sorted(t, 2, 900, disc_price=dp)
+    allg, _ = engine.chain_groups_sorted(engine.chain_args(t, 2, 900, disc_price=dp), 100_000)
+    assert np.array_equal(allg, want_dp)
+
+
+def test_chain_negative_keys_and_wide_ranges_fall_back(engine):
+    import torch
+    t = engine.gen_chain(SEED, 3000, 30_000)
+    # Negative, widely spread order keys (unique) and huge revenues: the
+    # packed ordering does not fit, the three-pass sort takes over.
+    okey = t["order_key"].to(torch.int64) * 70_001 - 1_000_000_000
+    okey = okey.to(torch.int32)
+    remap = {int(a): int(b) for a, b in zip(t["order_key"].cpu().tolist(), okey.cpu().tolist())}
+    fo = torch.tensor([remap[int(x)] for x in t["fact_order"].cpu().tolist()],
+                      dtype=torch.int32).cuda()
+    t2 = dict(t)
+    t2["order_key"], t2["fact_order"] = okey.contiguous(), fo
+    t2["fact_price"] = t["fact_price"] * (1 << 30)
+    want, st = want_sorted(t2, 0, 2000)
+    allg, gs = engine.chain_groups_sorted(engine.chain_args(t2, 0, 2000), 100_000)
+    assert np.array_equal(allg, want)
+    assert gs["sort_passes"] > 8
+
+
+def test_chain_empty_results(engine):
+    t = engine.gen_chain(SEED, 1000, 5000)
+    top, gs = engine.chain_topk(engine.chain_args(t, 9, 1155), 10)  # no such segment
+    assert top == [] and gs["groups"] == 0
+    top, gs = engine.chain_topk(engine.chain_args(t, 1, 0), 10)  # no order before day 0
+    assert top == [] and gs["order_pass"] == 0
