@@ -90,7 +90,7 @@ def test_chain_negative_keys_and_wide_ranges_fall_back(engine):
     t = engine.gen_chain(SEED, 3000, 30_000)
     # Negative, widely spread order keys (unique) and huge revenues: the
     # packed ordering does not fit, the three-pass sort takes over.
-    okey = t["order_key"].to(torch.int64) * 70_001 - 1_000_000_000
+    okey = t["order_key"].to(torch.int64) * 7_001 - 100_000_000
     okey = okey.to(torch.int32)
     remap = {int(a): int(b) for a, b in zip(t["order_key"].cpu().tolist(), okey.cpu().tolist())}
     fo = torch.tensor([remap[int(x)] for x in t["fact_order"].cpu().tolist()],
