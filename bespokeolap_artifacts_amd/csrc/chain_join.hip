@@ -87,12 +87,6 @@ BOLAP_DEV void fold_minmax(int64_t lo, int64_t hi, int64_t* st, int imin, int im
     }
 }
 
-BOLAP_DEV void count_ballot(bool p, int64_t* st, int idx) {
-    const uint64_t m = __ballot(p);
-    if ((threadIdx.x & (kWave - 1)) == 0 && m)
-        atomicAdd(reinterpret_cast<unsigned long long*>(st + idx),
-                  (unsigned long long)__popcll(m));
-}
 
 __global__ __launch_bounds__(kT) void key_minmax_k(const int32_t* __restrict__ ck, int64_t nc,
                                                    const int32_t* __restrict__ ok, int64_t no,
@@ -131,15 +125,17 @@ __global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ ke
                                                   const uint8_t* __restrict__ seg, int64_t n,
                                                   int32_t segment, Ranges r,
                                                   uint32_t* __restrict__ bits, int64_t* st) {
+    WaveCount wc;
     for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
         const int64_t i = base + threadIdx.x;
         const bool pass = i < n && int32_t(seg[i]) == segment;
-        count_ballot(pass, st, sCustPass);
+        wc.add(pass);
         if (pass) {
             const int64_t rel = int64_t(key[i]) - r.cmin;
             atomicOr(bits + (rel >> 5), 1u << (rel & 31));
         }
     }
+    wc.flush(st + sCustPass);
 }
 
 BOLAP_DEV bool order_pass(int64_t i, int64_t n, const int32_t* __restrict__ day,
@@ -155,10 +151,10 @@ __global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ 
                                                     int32_t day_lt,
                                                     const uint32_t* __restrict__ cbits,
                                                     int64_t cspan, Ranges r, int64_t* st) {
-    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
-        const int64_t i = base + threadIdx.x;
-        count_ballot(order_pass(i, n, day, cust, day_lt, cbits, cspan, r), st, sCount);
-    }
+    WaveCount wc;
+    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT)
+        wc.add(order_pass(base + threadIdx.x, n, day, cust, day_lt, cbits, cspan, r));
+    wc.flush(st + sCount);
 }
 
 struct GTable {
@@ -168,7 +164,7 @@ struct GTable {
     uint8_t* hit;    // 1 when the group got at least one fact row
     uint32_t* obits; // exact bitmap over the order key range
     int32_t cap_log2;
-    int64_t max_fill;
+    uint32_t max_probe;  // insert gives up (overflow) after this many slots
 };
 
 BOLAP_DEV uint32_t gslot(int32_t key, int cap_log2) {
@@ -196,11 +192,14 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
                                                     int64_t* st) {
     const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
     int64_t dl = INT64_MAX, dh = INT64_MIN;
+    WaveCount passed;
+    uint64_t inserted = 0;
+    bool failed = false;
     for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
         const int64_t i = base + threadIdx.x;
         const bool pass = order_pass(i, n, day, cust, day_lt, cbits, cspan, r);
-        count_ballot(pass, st, sOrderPass);
-        if (!pass) continue;
+        passed.add(pass);
+        if (!pass || failed) continue;
         const int32_t key = okey[i];
         const int32_t d = day[i];
         dl = d < dl ? d : dl;
@@ -209,23 +208,24 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
         atomicOr(t.obits + (rel >> 5), 1u << (rel & 31));
         uint32_t s = gslot(key, t.cap_log2);
         bool placed = false;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
+        for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
             const int32_t prev = atomicCAS(t.keys + s, kEmpty, key);
             if (prev == kEmpty || prev == key) {
                 t.days[s] = d;
-                if (prev == kEmpty) {
-                    const unsigned long long c = atomicAdd(
-                        reinterpret_cast<unsigned long long*>(st + sGroups), 1ull);
-                    if (int64_t(c) + 1 > t.max_fill)
-                        atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
-                }
+                inserted += prev == kEmpty;
                 placed = true;
                 break;
             }
             s = (s + 1) & mask;
         }
-        if (!placed) atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
+        failed = !placed;  // table too small: stop inserting, report overflow
     }
+    passed.flush(st + sOrderPass);
+    const uint64_t wins = wave_sum(inserted);
+    if ((threadIdx.x & (kWave - 1)) == 0 && wins)
+        atomicAdd(reinterpret_cast<unsigned long long*>(st + sGroups), (unsigned long long)wins);
+    if (__ballot(failed) && (threadIdx.x & (kWave - 1)) == 0)
+        atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
     fold_minmax(dl, dh, st, sDMin, sDMax);
 }
 
@@ -300,25 +300,43 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
         atomicAdd(reinterpret_cast<unsigned long long*>(st + sMatch), (unsigned long long)wm);
 }
 
+constexpr int kCompactR = 8;  // slots per lane per iteration
+
 __global__ __launch_bounds__(kT) void compact_groups_k(GTable t, RowT* __restrict__ out,
                                                        unsigned long long* __restrict__ n_out,
                                                        int64_t* st) {
-    const int64_t cap = int64_t(1) << t.cap_log2;
+    const int64_t cap = int64_t(1) << t.cap_log2;  // >= 1024, a multiple of kCompactR
     const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
     int64_t rl = INT64_MAX, rh = INT64_MIN;
-    for (int64_t base = int64_t(blockIdx.x) * kT; base < cap; base += int64_t(gridDim.x) * kT) {
-        const int64_t i = base + threadIdx.x;
-        const bool h = i < cap && t.hit[i];
-        const uint64_t m = __ballot(h);
+    for (int64_t base = int64_t(blockIdx.x) * kT * kCompactR; base < cap;
+         base += int64_t(gridDim.x) * kT * kCompactR) {
+        const int64_t i0 = base + int64_t(threadIdx.x) * kCompactR;
+        uint32_t hits = 0;
+        if (i0 < cap) {
+            const VecT<uint8_t, kCompactR> h = load_unit<uint8_t, kCompactR>(t.hit, i0 / kCompactR);
+#pragma unroll
+            for (int j = 0; j < kCompactR; ++j) hits |= (h.v[j] ? 1u : 0u) << j;
+        }
+        // Exclusive prefix of the lanes' hit counts across the wave.
+        const uint32_t c = uint32_t(__popc(hits));
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
         unsigned long long at = 0;
-        if (lane == 0 && m) at = atomicAdd(n_out, (unsigned long long)__popcll(m));
-        at = __shfl(at, 0, kWave);
-        if (h) {
+        if (lane == 0 && total) at = atomicAdd(n_out, (unsigned long long)total);
+        at = __shfl(at, 0, kWave) + (incl - c);
+        while (hits) {
+            const int j = __ffs(hits) - 1;
+            hits &= hits - 1;
+            const int64_t i = i0 + j;
             const int64_t rv = int64_t(t.rev[i]);
             rl = rv < rl ? rv : rl;
             rh = rv > rh ? rv : rh;
-            out[at + __popcll(m & lt_mask)] = RowT{rv, t.days[i], t.keys[i]};
+            out[at++] = RowT{rv, t.days[i], t.keys[i]};
         }
     }
     fold_minmax(rl, rh, st, sRMin, sRMax);
@@ -469,8 +487,11 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
     BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
     ws.base = static_cast<char*>(ctx->ws);
     const uint32_t* cbits = ws.at<uint32_t>(o_cb);
+    // Exact tables (cap >= 2 x count) always place every key; estimate-sized
+    // ones give up after 128 slots and fall back to exact sizing.
+    const uint32_t max_probe = exact ? uint32_t(cap) : 128u;
     GTable t{ws.at<int32_t>(o_keys), ws.at<int32_t>(o_days), ws.at<uint64_t>(o_rev),
-             ws.at<uint8_t>(o_hit), ws.at<uint32_t>(o_ob), cap_log2, cap / 2};
+             ws.at<uint8_t>(o_hit), ws.at<uint32_t>(o_ob), cap_log2, max_probe};
     {
         TraceScope ts(ctx, "chain_init");
         hipLaunchKernelGGL(gtable_init_k, dim3(grid_for(ctx, cap, 8)), dim3(kT), 0, ctx->stream,
@@ -512,11 +533,12 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
                        reinterpret_cast<uint32_t*>(n_rows), 2, 0u);
     {
         TraceScope ts(ctx, "chain_compact");
-        hipLaunchKernelGGL(compact_groups_k, dim3(grid_for(ctx, cap, 8)), dim3(kT), 0,
+        hipLaunchKernelGGL(compact_groups_k, dim3(grid_for(ctx, cap / kCompactR, 8)), dim3(kT), 0,
                            ctx->stream, t, ws.at<RowT>(o_rows), n_rows, st);
     }
     BOLAP_TRY(launch_check("compact_groups_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
+    if (run.st[sGroups] > cap / 2) run.st[sFlags] |= 1;  // load factor above 1/2
     BOLAP_TRY(pinned_reserve(ctx, 8));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));

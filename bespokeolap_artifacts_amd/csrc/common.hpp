@@ -268,6 +268,18 @@ BOLAP_DEV void reset_counter(unsigned* counter) {
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per-wave row counter: ballots accumulate in a wave-uniform register and
+// the wave adds its total to a global counter once, at the end of the kernel
+// (one atomic per wave instead of one per loop iteration).
+struct WaveCount {
+    uint64_t n = 0;
+    BOLAP_DEV void add(bool p) { n += uint64_t(__popcll(__ballot(p))); }
+    BOLAP_DEV void flush(int64_t* dst) const {
+        if ((threadIdx.x & (kWave - 1)) == 0 && n)
+            atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)n);
+    }
+};
+
 // 64-bit mixing hash for open-addressing tables (SplitMix64 output function).
 BOLAP_DEV uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;

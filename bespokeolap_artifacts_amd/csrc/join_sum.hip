@@ -45,7 +45,7 @@ struct Table {
     uint32_t* bits;
     int32_t cap_log2;
     int32_t bb_log2;
-    int64_t max_fill;
+    uint32_t max_probe;  // insert gives up (overflow) after this many slots
 };
 
 BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
@@ -75,15 +75,13 @@ __global__ __launch_bounds__(256) void zero_res_k(int64_t* res) {
 }
 
 __global__ __launch_bounds__(kThreads) void dim_count_k(PredSet ps, int64_t rows, int64_t* res) {
+    WaveCount wc;
     for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
          base += int64_t(gridDim.x) * kThreads) {
         const int64_t row = base + threadIdx.x;
-        const bool pass = row < rows && eval_preds(ps, row);
-        const uint64_t m = __ballot(pass);
-        if ((threadIdx.x & (kWave - 1)) == 0 && m)
-            atomicAdd(reinterpret_cast<unsigned long long*>(res + kCount),
-                      static_cast<unsigned long long>(__popcll(m)));
+        wc.add(row < rows && eval_preds(ps, row));
     }
+    wc.flush(res + kCount);
 }
 
 __global__ __launch_bounds__(256) void table_init_k(Table t, int64_t* res) {
@@ -102,42 +100,42 @@ __global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows
                                                         const int32_t* __restrict__ keys,
                                                         Table t, int64_t* res) {
     const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
+    WaveCount passed;
+    uint64_t inserted = 0;
+    bool failed = false, has_min = false;
     for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
          base += int64_t(gridDim.x) * kThreads) {
         const int64_t row = base + threadIdx.x;
         const bool pass = row < rows && eval_preds(ps, row);
-        const uint64_t m = __ballot(pass);
-        if ((threadIdx.x & (kWave - 1)) == 0 && m)
-            atomicAdd(reinterpret_cast<unsigned long long*>(res + kBuild),
-                      static_cast<unsigned long long>(__popcll(m)));
-        if (!pass) continue;
+        passed.add(pass);
+        if (!pass || failed) continue;
         const int32_t key = keys[row];
         const uint32_t h = bloom_hash(key);
         atomicOr(t.bits + bloom_word(h, t.bb_log2), bloom_bits(h, t.bb_log2));
         if (key == kEmpty) {  // the table's empty marker: recorded as a flag
-            atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
+            has_min = true;
             continue;
         }
         uint32_t s = slot_of(key, t.cap_log2);
         bool placed = false;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
+        for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
             const int32_t prev = atomicCAS(t.slots + s, kEmpty, key);
-            if (prev == kEmpty) {
-                const unsigned long long d =
-                    atomicAdd(reinterpret_cast<unsigned long long*>(res + kDistinct), 1ull);
-                if (int64_t(d) + 1 > t.max_fill)
-                    atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 1ull);
-                placed = true;
-                break;
-            }
-            if (prev == key) {
+            if (prev == kEmpty || prev == key) {
+                inserted += prev == kEmpty;
                 placed = true;
                 break;
             }
             s = (s + 1) & mask;
         }
-        if (!placed) atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 1ull);
+        failed = !placed;  // estimate-sized table too small: overflow
     }
+    passed.flush(res + kBuild);
+    const uint64_t wins = wave_sum(inserted);
+    const bool lead = (threadIdx.x & (kWave - 1)) == 0;
+    if (lead && wins)
+        atomicAdd(reinterpret_cast<unsigned long long*>(res + kDistinct), (unsigned long long)wins);
+    const uint64_t flags = (__ballot(failed) ? 1ull : 0ull) | (__ballot(has_min) ? 2ull : 0ull);
+    if (lead && flags) atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), flags);
 }
 
 // Exact membership test of a key that passed the Bloom filter.
@@ -442,7 +440,7 @@ inline unsigned small_grid(bolap_ctx* ctx, int64_t rows, int threads) {
 
 // One attempt with a given expected build size; exact == sized from a count.
 int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
-                 int64_t n_expect, int64_t* host_res, Sizing& z) {
+                 int64_t n_expect, bool exact, int64_t* host_res, Sizing& z) {
     BOLAP_TRY(size_table(n_expect, a->placement, z));
     const int64_t cap = int64_t(1) << z.cap_log2;
     const int64_t words = (int64_t(1) << z.bb_log2) / 32;
@@ -454,7 +452,10 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     int64_t* res = static_cast<int64_t*>(ctx->result_dev);
-    Table t{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits), z.cap_log2, z.bb_log2, cap / 2};
+    // Exact tables (load <= 1/2) always place every key; estimate-sized ones
+    // give up after 128 slots and fall back to exact sizing.
+    Table t{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits), z.cap_log2, z.bb_log2,
+            exact ? uint32_t(cap) : 128u};
     int64_t init_blocks = (cap + 255) / 256;
     if (init_blocks > 4096) init_blocks = 4096;
     {
@@ -478,6 +479,7 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
                                  hipMemcpyDeviceToHost, ctx->stream));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::memcpy(host_res, ctx->pinned, kResN * sizeof(int64_t));
+    if (host_res[kDistinct] > cap / 2) host_res[kFlags] |= 1;  // load factor above 1/2
     return BOLAP_OK;
 }
 
@@ -509,7 +511,7 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
     int fallbacks = 0;
     bool done = false;
     if (a->capacity_hint > 0) {
-        BOLAP_TRY(join_attempt(ctx, a, ps, a->capacity_hint, r, z));
+        BOLAP_TRY(join_attempt(ctx, a, ps, a->capacity_hint, false, r, z));
         if (r[kFlags] & 1)
             fallbacks = 1;  // estimate too small: exact path below
         else
@@ -532,7 +534,7 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
                                      hipMemcpyDeviceToHost, ctx->stream));
         BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
         std::memcpy(&count, ctx->pinned, sizeof(int64_t));
-        BOLAP_TRY(join_attempt(ctx, a, ps, count, r, z));
+        BOLAP_TRY(join_attempt(ctx, a, ps, count, true, r, z));
         if (r[kFlags] & 1) {
             set_error("bolap_join_sum: exact-sized table overflowed");
             return BOLAP_ERR_CAPACITY;
