@@ -88,21 +88,34 @@ BOLAP_DEV void fold_minmax(int64_t lo, int64_t hi, int64_t* st, int imin, int im
 }
 
 
+// Min/max of an int32 column: four keys per lane per 16-byte load, two
+// loads in flight per iteration.
+BOLAP_DEV void col_minmax(const int32_t* __restrict__ k, int64_t n, int64_t& lo, int64_t& hi) {
+    const int64_t units = n / 4;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += 2 * stride) {
+        const VecT<int32_t, 4> a = load_unit<int32_t, 4>(k, u);
+        const int64_t u2 = u + stride < units ? u + stride : u;
+        const VecT<int32_t, 4> b = load_unit<int32_t, 4>(k, u2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            lo = min(lo, int64_t(min(a.v[j], b.v[j])));
+            hi = max(hi, int64_t(max(a.v[j], b.v[j])));
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - units * 4) {
+        const int64_t v = k[units * 4 + threadIdx.x];
+        lo = min(lo, v);
+        hi = max(hi, v);
+    }
+}
+
 __global__ __launch_bounds__(kT) void key_minmax_k(const int32_t* __restrict__ ck, int64_t nc,
                                                    const int32_t* __restrict__ ok, int64_t no,
                                                    int64_t* st) {
     int64_t cl = INT64_MAX, ch = INT64_MIN, ol = INT64_MAX, oh = INT64_MIN;
-    const int64_t stride = int64_t(gridDim.x) * kT;
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < nc; i += stride) {
-        const int64_t v = ck[i];
-        cl = v < cl ? v : cl;
-        ch = v > ch ? v : ch;
-    }
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < no; i += stride) {
-        const int64_t v = ok[i];
-        ol = v < ol ? v : ol;
-        oh = v > oh ? v : oh;
-    }
+    col_minmax(ck, nc, cl, ch);
+    col_minmax(ok, no, ol, oh);
     fold_minmax(cl, ch, st, sCMin, sCMax);
     fold_minmax(ol, oh, st, sOMin, sOMax);
 }
@@ -126,24 +139,76 @@ __global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ ke
                                                   int32_t segment, Ranges r,
                                                   uint32_t* __restrict__ bits, int64_t* st) {
     WaveCount wc;
-    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
-        const int64_t i = base + threadIdx.x;
-        const bool pass = i < n && int32_t(seg[i]) == segment;
-        wc.add(pass);
-        if (pass) {
-            const int64_t rel = int64_t(key[i]) - r.cmin;
-            atomicOr(bits + (rel >> 5), 1u << (rel & 31));
+    const int64_t units = (n + 3) / 4;
+    const int64_t full = n / 4;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
+         u += int64_t(gridDim.x) * kT) {
+        int32_t kv[4];
+        uint8_t sv[4];
+        int cnt = 4;
+        if (u < full) {
+            const VecT<int32_t, 4> k4 = load_unit<int32_t, 4>(key, u);
+            const VecT<uint8_t, 4> s4 = load_unit<uint8_t, 4>(seg, u);
+            for (int j = 0; j < 4; ++j) {
+                kv[j] = k4.v[j];
+                sv[j] = s4.v[j];
+            }
+        } else {
+            cnt = int(n - u * 4);
+            for (int j = 0; j < 4; ++j) {
+                kv[j] = j < cnt ? key[u * 4 + j] : 0;
+                sv[j] = j < cnt ? seg[u * 4 + j] : 0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool pass = j < cnt && int32_t(sv[j]) == segment;
+            wc.add(pass);
+            if (pass) {
+                const int64_t rel = int64_t(kv[j]) - r.cmin;
+                atomicOr(bits + (rel >> 5), 1u << (rel & 31));
+            }
         }
     }
     wc.flush(st + sCustPass);
 }
 
-BOLAP_DEV bool order_pass(int64_t i, int64_t n, const int32_t* __restrict__ day,
-                          const int32_t* __restrict__ cust, int32_t day_lt,
-                          const uint32_t* __restrict__ cbits, int64_t cspan, const Ranges& r) {
-    if (i >= n) return false;
-    if (day[i] >= day_lt) return false;
-    return test_bit(cbits, int64_t(cust[i]) - r.cmin, cspan);
+// Four order rows per lane: day and customer columns read with 16-byte loads.
+struct Orders4 {
+    int32_t day[4], cust[4], key[4];
+    int cnt;
+};
+
+BOLAP_DEV Orders4 load_orders4(const int32_t* __restrict__ okey, const int32_t* __restrict__ day,
+                               const int32_t* __restrict__ cust, int64_t n, int64_t u,
+                               bool with_key) {
+    Orders4 o;
+    if ((u + 1) * 4 <= n) {
+        const VecT<int32_t, 4> d = load_unit<int32_t, 4>(day, u);
+        const VecT<int32_t, 4> c = load_unit<int32_t, 4>(cust, u);
+        VecT<int32_t, 4> k{};
+        if (with_key) k = load_unit<int32_t, 4>(okey, u);
+        for (int j = 0; j < 4; ++j) {
+            o.day[j] = d.v[j];
+            o.cust[j] = c.v[j];
+            o.key[j] = k.v[j];
+        }
+        o.cnt = 4;
+    } else {
+        o.cnt = int(n - u * 4);
+        for (int j = 0; j < 4; ++j) {
+            const bool in = j < o.cnt;
+            o.day[j] = in ? day[u * 4 + j] : 0;
+            o.cust[j] = in ? cust[u * 4 + j] : 0;
+            o.key[j] = (in && with_key) ? okey[u * 4 + j] : 0;
+        }
+    }
+    return o;
+}
+
+BOLAP_DEV bool order_pass_v(int32_t day, int32_t cust, int32_t day_lt,
+                            const uint32_t* __restrict__ cbits, int64_t cspan, const Ranges& r) {
+    return day < day_lt && test_bit(cbits, int64_t(cust) - r.cmin, cspan);
 }
 
 __global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ day,
@@ -152,8 +217,14 @@ __global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ 
                                                     const uint32_t* __restrict__ cbits,
                                                     int64_t cspan, Ranges r, int64_t* st) {
     WaveCount wc;
-    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT)
-        wc.add(order_pass(base + threadIdx.x, n, day, cust, day_lt, cbits, cspan, r));
+    const int64_t units = (n + 3) / 4;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
+         u += int64_t(gridDim.x) * kT) {
+        const Orders4 o = load_orders4(nullptr, day, cust, n, u, false);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            wc.add(j < o.cnt && order_pass_v(o.day[j], o.cust[j], day_lt, cbits, cspan, r));
+    }
     wc.flush(st + sCount);
 }
 
@@ -195,30 +266,36 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
     WaveCount passed;
     uint64_t inserted = 0;
     bool failed = false;
-    for (int64_t base = int64_t(blockIdx.x) * kT; base < n; base += int64_t(gridDim.x) * kT) {
-        const int64_t i = base + threadIdx.x;
-        const bool pass = order_pass(i, n, day, cust, day_lt, cbits, cspan, r);
-        passed.add(pass);
-        if (!pass || failed) continue;
-        const int32_t key = okey[i];
-        const int32_t d = day[i];
-        dl = d < dl ? d : dl;
-        dh = d > dh ? d : dh;
-        const int64_t rel = int64_t(key) - r.omin;
-        atomicOr(t.obits + (rel >> 5), 1u << (rel & 31));
-        uint32_t s = gslot(key, t.cap_log2);
-        bool placed = false;
-        for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
-            const int32_t prev = atomicCAS(t.keys + s, kEmpty, key);
-            if (prev == kEmpty || prev == key) {
-                t.days[s] = d;
-                inserted += prev == kEmpty;
-                placed = true;
-                break;
+    const int64_t units = (n + 3) / 4;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
+         u += int64_t(gridDim.x) * kT) {
+        const Orders4 o = load_orders4(okey, day, cust, n, u, true);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool pass =
+                j < o.cnt && order_pass_v(o.day[j], o.cust[j], day_lt, cbits, cspan, r);
+            passed.add(pass);
+            if (!pass || failed) continue;
+            const int32_t key = o.key[j];
+            const int32_t d = o.day[j];
+            dl = d < dl ? d : dl;
+            dh = d > dh ? d : dh;
+            const int64_t rel = int64_t(key) - r.omin;
+            atomicOr(t.obits + (rel >> 5), 1u << (rel & 31));
+            uint32_t s = gslot(key, t.cap_log2);
+            bool placed = false;
+            for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
+                const int32_t prev = atomicCAS(t.keys + s, kEmpty, key);
+                if (prev == kEmpty || prev == key) {
+                    t.days[s] = d;
+                    inserted += prev == kEmpty;
+                    placed = true;
+                    break;
+                }
+                s = (s + 1) & mask;
             }
-            s = (s + 1) & mask;
+            failed = !placed;  // table too small: stop inserting, report overflow
         }
-        failed = !placed;  // table too small: stop inserting, report overflow
     }
     passed.flush(st + sOrderPass);
     const uint64_t wins = wave_sum(inserted);
@@ -284,10 +361,15 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
             t.hit[cur_slot] = 1;
         }
     };
-    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < n_units;
-         u += int64_t(gridDim.x) * kT) {
-        const VecT<int32_t, 4> v = load_unit<int32_t, 4>(fk, u);
-        run_rows(v.v, u * 4, 4);
+    // Two 4-key units per lane per iteration (both loads in flight), then the
+    // units' runs in row order.
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < n_units; u += 2 * stride) {
+        const int64_t u2 = u + stride;
+        const VecT<int32_t, 4> v1 = load_unit<int32_t, 4>(fk, u);
+        const VecT<int32_t, 4> v2 = load_unit<int32_t, 4>(fk, u2 < n_units ? u2 : u);
+        run_rows(v1.v, u * 4, 4);
+        if (u2 < n_units) run_rows(v2.v, u2 * 4, 4);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         int32_t kv[4];
@@ -300,35 +382,68 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
         atomicAdd(reinterpret_cast<unsigned long long*>(st + sMatch), (unsigned long long)wm);
 }
 
-constexpr int kCompactR = 8;  // slots per lane per iteration
+constexpr int kCompactR = 8;    // slots per lane per tile
+constexpr int kCT = 1024;       // compaction workgroup
+constexpr int kCTile = kCT * kCompactR;
 
-__global__ __launch_bounds__(kT) void compact_groups_k(GTable t, RowT* __restrict__ out,
-                                                       unsigned long long* __restrict__ n_out,
-                                                       int64_t* st) {
-    const int64_t cap = int64_t(1) << t.cap_log2;  // >= 1024, a multiple of kCompactR
-    const int lane = threadIdx.x & (kWave - 1);
-    int64_t rl = INT64_MAX, rh = INT64_MIN;
-    for (int64_t base = int64_t(blockIdx.x) * kT * kCompactR; base < cap;
-         base += int64_t(gridDim.x) * kT * kCompactR) {
-        const int64_t i0 = base + int64_t(threadIdx.x) * kCompactR;
-        uint32_t hits = 0;
-        if (i0 < cap) {
-            const VecT<uint8_t, kCompactR> h = load_unit<uint8_t, kCompactR>(t.hit, i0 / kCompactR);
+BOLAP_DEV uint32_t hit_mask8(const uint8_t* __restrict__ hit, int64_t i0, int64_t end) {
+    uint32_t m = 0;
+    if (i0 + kCompactR <= end) {
+        const VecT<uint8_t, kCompactR> h = load_unit<uint8_t, kCompactR>(hit, i0 / kCompactR);
 #pragma unroll
-            for (int j = 0; j < kCompactR; ++j) hits |= (h.v[j] ? 1u : 0u) << j;
-        }
-        // Exclusive prefix of the lanes' hit counts across the wave.
-        const uint32_t c = uint32_t(__popc(hits));
+        for (int j = 0; j < kCompactR; ++j) m |= (h.v[j] ? 1u : 0u) << j;
+    }
+    return m;
+}
+
+// Each workgroup compacts one contiguous chunk of slots (a multiple of
+// kCTile): pass 1 counts the hit slots, one atomic reserves the output
+// range, pass 2 writes the rows in slot order.
+__global__ __launch_bounds__(kCT) void compact_groups_k(GTable t, int64_t chunk,
+                                                        RowT* __restrict__ out,
+                                                        unsigned long long* __restrict__ n_out,
+                                                        int64_t* st) {
+    constexpr int kW = kCT / kWave;
+    __shared__ uint32_t s_w[kW];
+    __shared__ unsigned long long s_base;
+    const int64_t cap = int64_t(1) << t.cap_log2;
+    const int64_t b0 = int64_t(blockIdx.x) * chunk;
+    const int64_t b1 = b0 + chunk < cap ? b0 + chunk : cap;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    uint32_t mine = 0;
+    for (int64_t tb = b0; tb < b1; tb += kCTile)
+        mine += __popc(hit_mask8(t.hit, tb + int64_t(threadIdx.x) * kCompactR, b1));
+    const uint32_t wsum = uint32_t(wave_sum(uint64_t(mine)));
+    if (lane == 0) s_w[wave] = wsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kW; ++w) tot += s_w[w];
+        s_base = tot ? atomicAdd(n_out, (unsigned long long)tot) : 0;
+    }
+    __syncthreads();
+    unsigned long long base = s_base;
+    int64_t rl = INT64_MAX, rh = INT64_MIN;
+    for (int64_t tb = b0; tb < b1; tb += kCTile) {
+        const int64_t i0 = tb + int64_t(threadIdx.x) * kCompactR;
+        uint32_t hits = hit_mask8(t.hit, i0, b1);
+        const uint32_t c = __popc(hits);
         uint32_t incl = c;
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
             const uint32_t y = __shfl_up(incl, off, kWave);
             if (lane >= off) incl += y;
         }
-        const uint32_t total = __shfl(incl, kWave - 1, kWave);
-        unsigned long long at = 0;
-        if (lane == 0 && total) at = atomicAdd(n_out, (unsigned long long)total);
-        at = __shfl(at, 0, kWave) + (incl - c);
+        __syncthreads();
+        if (lane == kWave - 1) s_w[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, tile = 0;
+        for (int w = 0; w < kW; ++w) {
+            before += w < wave ? s_w[w] : 0;
+            tile += s_w[w];
+        }
+        unsigned long long at = base + before + (incl - c);
         while (hits) {
             const int j = __ffs(hits) - 1;
             hits &= hits - 1;
@@ -338,6 +453,7 @@ __global__ __launch_bounds__(kT) void compact_groups_k(GTable t, RowT* __restric
             rh = rv > rh ? rv : rh;
             out[at++] = RowT{rv, t.days[i], t.keys[i]};
         }
+        base += tile;
     }
     fold_minmax(rl, rh, st, sRMin, sRMax);
 }
@@ -400,7 +516,7 @@ __global__ __launch_bounds__(kT) void gather_rows_k(const RowT* __restrict__ row
     }
 }
 
-inline unsigned grid_for(bolap_ctx* ctx, int64_t n, int per_cu = 4) {
+inline unsigned grid_for(bolap_ctx* ctx, int64_t n, int per_cu = 8) {
     int64_t b = (n + kT - 1) / kT;
     const int64_t cap = int64_t(ctx->num_cus) * per_cu;
     if (b > cap) b = cap;
@@ -533,8 +649,12 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
                        reinterpret_cast<uint32_t*>(n_rows), 2, 0u);
     {
         TraceScope ts(ctx, "chain_compact");
-        hipLaunchKernelGGL(compact_groups_k, dim3(grid_for(ctx, cap / kCompactR, 8)), dim3(kT), 0,
-                           ctx->stream, t, ws.at<RowT>(o_rows), n_rows, st);
+        int64_t cblocks = int64_t(ctx->num_cus);
+        int64_t chunk = (cap + cblocks - 1) / cblocks;
+        chunk = (chunk + kCTile - 1) / kCTile * kCTile;
+        cblocks = (cap + chunk - 1) / chunk;
+        hipLaunchKernelGGL(compact_groups_k, dim3(unsigned(cblocks)), dim3(kCT), 0, ctx->stream,
+                           t, chunk, ws.at<RowT>(o_rows), n_rows, st);
     }
     BOLAP_TRY(launch_check("compact_groups_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
@@ -627,6 +747,11 @@ int chain_validate(const bolap_chain_args* a) {
                     "bolap_chain: null fact column");
     BOLAP_ARG_CHECK(a->fact_rows == 0 || aligned(a->fact_order, 16),
                     "bolap_chain: fact_order must be 16-byte aligned");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || (aligned(a->order_key, 16) && aligned(a->order_day, 16) &&
+                                           aligned(a->order_cust, 16)),
+                    "bolap_chain: order columns must be 16-byte aligned");
+    BOLAP_ARG_CHECK(a->cust_rows == 0 || (aligned(a->cust_key, 16) && aligned(a->cust_segment, 4)),
+                    "bolap_chain: customer columns must be 16/4-byte aligned");
     BOLAP_ARG_CHECK(a->fact_disc_price || a->price_dtype == BOLAP_I64 ||
                         a->price_dtype == BOLAP_I32,
                     "bolap_chain: price dtype must be i64 or i32");
