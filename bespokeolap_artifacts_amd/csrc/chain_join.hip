@@ -35,11 +35,45 @@ constexpr int kT = 256;
 constexpr int32_t kEmpty = INT32_MIN;
 constexpr uint32_t kSlotMul = 0x9E3779B1u;
 
-// Device statistics slot (int64).
+// Device statistics (int64), each with kRep replicas: workgroup b folds
+// into replica b % kRep, so the workgroups of one XCD share a replica and no
+// address sees more than (#workgroups / kRep) atomics.  read_stats folds the
+// replicas (sum, min or max).
 enum {
     sCMin = 0, sCMax, sOMin, sOMax, sCustPass, sOrderPass, sCount, sDMin, sDMax, sMatch,
     sGroups, sFlags, sRMin, sRMax, sN = 16
 };
+constexpr int kRep = 8;
+
+BOLAP_DEV bool is_min_stat(int i) { return i == sCMin || i == sOMin || i == sDMin || i == sRMin; }
+BOLAP_DEV bool is_max_stat(int i) { return i == sCMax || i == sOMax || i == sDMax || i == sRMax; }
+
+// Workgroup-wide fold of a wave-uniform value into statistic `idx`
+// (op 0 add, 1 min, 2 max).  Every thread of the workgroup must call it.
+template <int NT>
+BOLAP_DEV void block_fold(int64_t* st, int idx, int64_t wave_value, int op) {
+    __shared__ int64_t s_v[NT / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) s_v[threadIdx.x / kWave] = wave_value;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t v = s_v[0];
+        for (int w = 1; w < NT / kWave; ++w) {
+            const int64_t x = s_v[w];
+            v = op == 0 ? int64_t(uint64_t(v) + uint64_t(x)) : op == 1 ? (x < v ? x : v)
+                                                                       : (x > v ? x : v);
+        }
+        long long* dst = reinterpret_cast<long long*>(st + idx * kRep + (blockIdx.x % kRep));
+        if (op == 0) {
+            if (v) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
+        } else if (op == 1) {
+            if (v != INT64_MAX) atomicMin(dst, (long long)v);
+        } else if (v != INT64_MIN) {
+            atomicMax(dst, (long long)v);
+        }
+    }
+    __syncthreads();
+}
 
 struct RowT {
     int64_t rev;
@@ -48,15 +82,9 @@ struct RowT {
 };
 
 __global__ void stats_init_k(int64_t* st) {
-    if (threadIdx.x < sN) {
-        int64_t v = 0;
-        if (threadIdx.x == sCMin || threadIdx.x == sOMin || threadIdx.x == sDMin ||
-            threadIdx.x == sRMin)
-            v = INT64_MAX;
-        if (threadIdx.x == sCMax || threadIdx.x == sOMax || threadIdx.x == sDMax ||
-            threadIdx.x == sRMax)
-            v = INT64_MIN;
-        st[threadIdx.x] = v;
+    if (threadIdx.x < sN * kRep) {
+        const int i = threadIdx.x / kRep;
+        st[threadIdx.x] = is_min_stat(i) ? INT64_MAX : is_max_stat(i) ? INT64_MIN : 0;
     }
 }
 
@@ -77,14 +105,11 @@ BOLAP_DEV int64_t wave_max(int64_t v) {
     return v;
 }
 
-// Block min/max folded into st[imin], st[imax] with one atomic pair per wave.
+// Workgroup min/max folded into statistics imin / imax.
+template <int NT>
 BOLAP_DEV void fold_minmax(int64_t lo, int64_t hi, int64_t* st, int imin, int imax) {
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        if (lo != INT64_MAX) atomicMin(reinterpret_cast<long long*>(st + imin), (long long)lo);
-        if (hi != INT64_MIN) atomicMax(reinterpret_cast<long long*>(st + imax), (long long)hi);
-    }
+    block_fold<NT>(st, imin, wave_min(lo), 1);
+    block_fold<NT>(st, imax, wave_max(hi), 2);
 }
 
 
@@ -116,8 +141,8 @@ __global__ __launch_bounds__(kT) void key_minmax_k(const int32_t* __restrict__ c
     int64_t cl = INT64_MAX, ch = INT64_MIN, ol = INT64_MAX, oh = INT64_MIN;
     col_minmax(ck, nc, cl, ch);
     col_minmax(ok, no, ol, oh);
-    fold_minmax(cl, ch, st, sCMin, sCMax);
-    fold_minmax(ol, oh, st, sOMin, sOMax);
+    fold_minmax<kT>(cl, ch, st, sCMin, sCMax);
+    fold_minmax<kT>(ol, oh, st, sOMin, sOMax);
 }
 
 __global__ __launch_bounds__(kT) void fill_u32_k(uint32_t* p, int64_t n, uint32_t v) {
@@ -170,7 +195,7 @@ __global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ ke
             }
         }
     }
-    wc.flush(st + sCustPass);
+    block_fold<kT>(st, sCustPass, int64_t(wc.n), 0);
 }
 
 // Four order rows per lane: day and customer columns read with 16-byte loads.
@@ -225,7 +250,7 @@ __global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ 
         for (int j = 0; j < 4; ++j)
             wc.add(j < o.cnt && order_pass_v(o.day[j], o.cust[j], day_lt, cbits, cspan, r));
     }
-    wc.flush(st + sCount);
+    block_fold<kT>(st, sCount, int64_t(wc.n), 0);
 }
 
 struct GTable {
@@ -297,13 +322,10 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
             failed = !placed;  // table too small: stop inserting, report overflow
         }
     }
-    passed.flush(st + sOrderPass);
-    const uint64_t wins = wave_sum(inserted);
-    if ((threadIdx.x & (kWave - 1)) == 0 && wins)
-        atomicAdd(reinterpret_cast<unsigned long long*>(st + sGroups), (unsigned long long)wins);
-    if (__ballot(failed) && (threadIdx.x & (kWave - 1)) == 0)
-        atomicOr(reinterpret_cast<unsigned long long*>(st + sFlags), 1ull);
-    fold_minmax(dl, dh, st, sDMin, sDMax);
+    block_fold<kT>(st, sOrderPass, int64_t(passed.n), 0);
+    block_fold<kT>(st, sGroups, int64_t(wave_sum(inserted)), 0);
+    block_fold<kT>(st, sFlags, __ballot(failed) ? 1 : 0, 0);
+    fold_minmax<kT>(dl, dh, st, sDMin, sDMax);
 }
 
 // Slot of a key known to be in the table.
@@ -377,9 +399,7 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
         for (int j = 0; j < rem; ++j) kv[j] = fk[n_units * 4 + j];
         run_rows(kv, n_units * 4, rem);
     }
-    const uint64_t wm = wave_sum(matched);
-    if ((threadIdx.x & (kWave - 1)) == 0 && wm)
-        atomicAdd(reinterpret_cast<unsigned long long*>(st + sMatch), (unsigned long long)wm);
+    block_fold<kT>(st, sMatch, int64_t(wave_sum(matched)), 0);
 }
 
 constexpr int kCompactR = 8;    // slots per lane per tile
@@ -455,7 +475,7 @@ __global__ __launch_bounds__(kCT) void compact_groups_k(GTable t, int64_t chunk,
         }
         base += tile;
     }
-    fold_minmax(rl, rh, st, sRMin, sRMax);
+    fold_minmax<kCT>(rl, rh, st, sRMin, sRMax);
 }
 
 int bits_for(uint64_t span) {  // bits to hold values 0..span
@@ -535,12 +555,27 @@ struct ChainRun {
     int cap_log2;
 };
 
+bool host_is_min(int i) { return i == sCMin || i == sOMin || i == sDMin || i == sRMin; }
+bool host_is_max(int i) { return i == sCMax || i == sOMax || i == sDMax || i == sRMax; }
+
+// Copies the statistics back and folds the replicas.
 int read_stats(bolap_ctx* ctx, int64_t* dst) {
-    BOLAP_TRY(pinned_reserve(ctx, sN * 8));
-    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->result_dev, sN * 8, hipMemcpyDeviceToHost,
+    const size_t bytes = sN * kRep * 8;
+    BOLAP_TRY(pinned_reserve(ctx, bytes));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->result_dev, bytes, hipMemcpyDeviceToHost,
                                  ctx->stream));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::memcpy(dst, ctx->pinned, sN * 8);
+    const int64_t* rep = static_cast<const int64_t*>(ctx->pinned);
+    for (int i = 0; i < sN; ++i) {
+        int64_t v = rep[i * kRep];
+        for (int r = 1; r < kRep; ++r) {
+            const int64_t x = rep[i * kRep + r];
+            if (host_is_min(i)) v = x < v ? x : v;
+            else if (host_is_max(i)) v = x > v ? x : v;
+            else v = int64_t(uint64_t(v) + uint64_t(x));
+        }
+        dst[i] = v;
+    }
     return BOLAP_OK;
 }
 
@@ -549,7 +584,7 @@ int read_stats(bolap_ctx* ctx, int64_t* dst) {
 int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bool exact,
                  WsArena& ws, size_t* o_rows_out, ChainRun& run) {
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
-    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(64), 0, ctx->stream, st);
+    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
     {
         TraceScope ts(ctx, "chain_stats");
         hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
@@ -772,12 +807,12 @@ int chain_run(bolap_ctx* ctx, const bolap_chain_args* a, int64_t limit,
     int fallbacks = 0;
     const bool exact = a->capacity_hint <= 0;
     BOLAP_TRY(chain_groups(ctx, a, a->capacity_hint, exact, ws, &o_rows, run));
-    if (!exact && (run.st[sFlags] & 1)) {
+    if (!exact && (run.st[sFlags] != 0)) {
         fallbacks = 1;  // the estimate was too small: exact sizing
         ws = WsArena{};
         BOLAP_TRY(chain_groups(ctx, a, 0, true, ws, &o_rows, run));
     }
-    if (run.st[sFlags] & 1) {
+    if (run.st[sFlags] != 0) {
         set_error("bolap_chain: exact-sized group table overflowed");
         return BOLAP_ERR_CAPACITY;
     }
