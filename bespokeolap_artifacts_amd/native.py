@@ -120,6 +120,14 @@ PROTOTYPES = {
                                             P(ChainStats)]),
     "bolap_radix_sort_u64": (C.c_int, [vp, vp, vp, i64, i32]),
     "bolap_topk_u64": (C.c_int, [vp, vp, vp, i64, i32, vp, vp]),
+    "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),
+    "bolap_mlp_destroy": (C.c_int, [vp]),
+    "bolap_mlp_num_params": (C.c_int, []),
+    "bolap_mlp_get_params": (C.c_int, [vp, P(C.c_float)]),
+    "bolap_mlp_set_params": (C.c_int, [vp, P(C.c_float)]),
+    "bolap_mlp_predict": (C.c_int, [vp, vp, i64, vp]),
+    "bolap_mlp_train_step": (C.c_int, [vp, vp, vp, i64, C.c_float, P(C.c_float)]),
+    "bolap_mlp_gradients": (C.c_int, [vp, vp, vp, i64, P(C.c_float), P(C.c_float)]),
     "bolap_gen_lineitem": (C.c_int, [vp, u64, i64, P(GenLineitemCols)]),
     "bolap_gen_dim": (C.c_int, [vp, u64, i64, vp, vp, vp]),
     "bolap_gen_fact": (C.c_int, [vp, u64, i64, i64, vp, vp, vp]),

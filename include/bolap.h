@@ -279,6 +279,35 @@ int bolap_topk_u64(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, i
                    int32_t k, uint64_t* keys_out, uint32_t* vals_out);
 
 /* ------------------------------------------------------------------------ */
+/* Learned cardinality estimator: MLP 256-128-64-3 (ReLU), BF16 MFMA GEMMs    */
+/* with FP32 accumulation, trained with Adam on the device.  No reference     */
+/* counterpart: the reference sizes tables from a hard-coded selectivity      */
+/* (bespoke_tpch/query_q20.cpp:16, 405-417) and grows them on overflow        */
+/* (:158-160).  Outputs: log selectivity, log join size, log group count.     */
+/* Inputs/outputs are device fp32 row-major arrays (n x 256, n x 3).          */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_MLP_IN 256
+#define BOLAP_MLP_OUT 3
+typedef struct bolap_mlp bolap_mlp;
+
+int bolap_mlp_create(bolap_ctx* ctx, uint64_t seed, bolap_mlp** out);
+int bolap_mlp_destroy(bolap_mlp* mlp);
+/* Parameter count and host copies, ordered W1t[128][256], b1[128],
+   W2t[64][128], b2[64], W3t[3][64], b3[3] (fp32 master weights). */
+int bolap_mlp_num_params(void);
+int bolap_mlp_get_params(bolap_mlp* mlp, float* host_params);
+int bolap_mlp_set_params(bolap_mlp* mlp, const float* host_params);
+int bolap_mlp_predict(bolap_mlp* mlp, const float* x_dev, int64_t n, float* y_dev);
+/* One Adam step on the mean squared error of a batch; *loss_out (optional,
+   host) receives the batch loss before the step. */
+int bolap_mlp_train_step(bolap_mlp* mlp, const float* x_dev, const float* t_dev, int64_t n,
+                         float lr, float* loss_out);
+/* Gradients of the batch loss at the current weights (host, parameter order),
+   no update. */
+int bolap_mlp_gradients(bolap_mlp* mlp, const float* x_dev, const float* t_dev, int64_t n,
+                        float* grads_host, float* loss_out);
+
+/* ------------------------------------------------------------------------ */
 /* Seeded synthetic tables (counter-based SplitMix64, order independent).     */
 /* Every output is a caller-allocated device array.                           */
 /* ------------------------------------------------------------------------ */
