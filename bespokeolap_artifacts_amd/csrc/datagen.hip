@@ -119,6 +119,55 @@ __global__ __launch_bounds__(kGenThreads) void gen_order_facts_k(
     }
 }
 
+
+__global__ __launch_bounds__(kGenThreads) void gen_iota_k(int64_t rows, int32_t first,
+                                                           int32_t* out) {
+    for (int64_t r = int64_t(blockIdx.x) * kGenThreads + threadIdx.x; r < rows;
+         r += int64_t(gridDim.x) * kGenThreads)
+        out[r] = int32_t(int64_t(first) + r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kGenThreads) void gen_uniform_k(uint64_t seed, uint32_t stream,
+                                                              int64_t rows, int32_t lo,
+                                                              int32_t hi, T* out) {
+    for (int64_t r = int64_t(blockIdx.x) * kGenThreads + threadIdx.x; r < rows;
+         r += int64_t(gridDim.x) * kGenThreads)
+        out[r] = T(gen_uniform(gen_hash(seed, stream, uint64_t(r)), lo, hi));
+}
+
+// Zipf-distributed keys: the top 32 hash bits pick a point u of [0, 2^32);
+// its top 16 bits index the inverse-CDF table and the low 16 bits
+// interpolate linearly between neighbouring entries (integer arithmetic, so
+// the host restatement is bit-identical).
+__global__ __launch_bounds__(kGenThreads) void gen_zipf_k(uint64_t seed, uint32_t stream,
+                                                           int64_t rows,
+                                                           const int32_t* __restrict__ q,
+                                                           int64_t n_keys, uint64_t mul,
+                                                           int32_t* out) {
+    for (int64_t r = int64_t(blockIdx.x) * kGenThreads + threadIdx.x; r < rows;
+         r += int64_t(gridDim.x) * kGenThreads) {
+        const uint32_t u = uint32_t(gen_hash(seed, stream, uint64_t(r)) >> 32);
+        const uint32_t i = u >> 16, f = u & 0xFFFFu;
+        const int64_t a = q[i], b = q[i + 1];
+        const int64_t rank = a + (((b - a) * int64_t(f)) >> 16);
+        out[r] = int32_t(uint64_t(rank - 1) * mul % uint64_t(n_keys)) + 1;
+    }
+}
+
+__global__ __launch_bounds__(kGenThreads) void gen_price_disc_k(uint64_t seed, uint32_t stream,
+                                                                 int64_t rows, int64_t* price,
+                                                                 int64_t* disc) {
+    for (int64_t r = int64_t(blockIdx.x) * kGenThreads + threadIdx.x; r < rows;
+         r += int64_t(gridDim.x) * kGenThreads) {
+        const uint64_t u = uint64_t(r);
+        const int64_t q = gen_uniform(gen_hash(seed, stream, u), 1, 50);
+        const int64_t unit = gen_uniform(gen_hash(seed, stream + 1, u), 90000, 200000);
+        price[r] = q * unit / 100;
+        disc[r] = gen_uniform(gen_hash(seed, stream + 2, u), 0, 10);
+    }
+}
+
 }  // namespace
 }  // namespace bolap
 
@@ -199,6 +248,62 @@ int bolap_gen_order_facts(bolap_ctx* ctx, uint64_t seed, int64_t order_rows,
                        ctx->stream, seed, order_rows, fact_offsets, fact_rows, fk_order, price,
                        disc);
     return launch_check("gen_order_facts");
+}
+
+int bolap_gen_iota_i32(bolap_ctx* ctx, int64_t rows, int32_t first, int32_t* out) {
+    BOLAP_ARG_CHECK(ctx && rows >= 0, "bolap_gen_iota_i32: bad arguments");
+    if (rows == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(out && int64_t(first) + rows - 1 <= INT32_MAX,
+                    "bolap_gen_iota_i32: null column or int32 overflow");
+    hipLaunchKernelGGL(gen_iota_k, dim3(gen_blocks(rows)), dim3(kGenThreads), 0, ctx->stream,
+                       rows, first, out);
+    return launch_check("gen_iota");
+}
+
+int bolap_gen_uniform_i32(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                          int32_t lo, int32_t hi, int32_t* out) {
+    BOLAP_ARG_CHECK(ctx && rows >= 0 && lo <= hi, "bolap_gen_uniform_i32: bad arguments");
+    if (rows == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(out, "bolap_gen_uniform_i32: null column");
+    hipLaunchKernelGGL(gen_uniform_k<int32_t>, dim3(gen_blocks(rows)), dim3(kGenThreads), 0,
+                       ctx->stream, seed, stream, rows, lo, hi, out);
+    return launch_check("gen_uniform_i32");
+}
+
+int bolap_gen_uniform_u8(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                         int32_t lo, int32_t hi, uint8_t* out) {
+    BOLAP_ARG_CHECK(ctx && rows >= 0 && 0 <= lo && lo <= hi && hi <= 255,
+                    "bolap_gen_uniform_u8: bad arguments");
+    if (rows == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(out, "bolap_gen_uniform_u8: null column");
+    hipLaunchKernelGGL(gen_uniform_k<uint8_t>, dim3(gen_blocks(rows)), dim3(kGenThreads), 0,
+                       ctx->stream, seed, stream, rows, lo, hi, out);
+    return launch_check("gen_uniform_u8");
+}
+
+int bolap_gen_zipf_i32(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                       const int32_t* quantiles_dev, int64_t n_keys, uint64_t mul,
+                       int32_t* out) {
+    BOLAP_ARG_CHECK(ctx && rows >= 0 && n_keys >= 1 && n_keys <= INT32_MAX && mul >= 1,
+                    "bolap_gen_zipf_i32: bad arguments");
+    if (rows == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(out && quantiles_dev, "bolap_gen_zipf_i32: null array");
+    BOLAP_ARG_CHECK(std::gcd(mul % uint64_t(n_keys), uint64_t(n_keys)) == 1 || n_keys == 1,
+                    "bolap_gen_zipf_i32: mul %llu is not coprime to %lld",
+                    (unsigned long long)mul, (long long)n_keys);
+    hipLaunchKernelGGL(gen_zipf_k, dim3(gen_blocks(rows)), dim3(kGenThreads), 0, ctx->stream,
+                       seed, stream, rows, quantiles_dev, n_keys, mul, out);
+    return launch_check("gen_zipf_i32");
+}
+
+int bolap_gen_price_disc(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                         int64_t* price, int64_t* disc) {
+    BOLAP_ARG_CHECK(ctx && rows >= 0, "bolap_gen_price_disc: bad arguments");
+    if (rows == 0) return BOLAP_OK;
+    BOLAP_ARG_CHECK(price && disc, "bolap_gen_price_disc: null column");
+    hipLaunchKernelGGL(gen_price_disc_k, dim3(gen_blocks(rows)), dim3(kGenThreads), 0,
+                       ctx->stream, seed, stream, rows, price, disc);
+    return launch_check("gen_price_disc");
 }
 
 }  // extern "C"

@@ -250,6 +250,52 @@ class Engine:
         m = min(k, keys.numel())
         return ko[:m], vo[:m]
 
+    # ------------------------------------------------- star join + group-by
+    def star_groupby(self, a: N.StarArgs, want_rows: bool = False, max_rows: int = 0):
+        """Runs bolap_star_groupby; returns (stats dict, keys, revenues) with
+        device tensors of the groups in key order when want_rows."""
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        keys = rev = None
+        if want_rows:
+            cap = max_rows or (int(a.group_max) - int(a.group_min) + 1)
+            keys = torch.empty(max(1, cap), dtype=torch.int32, device=dev)
+            rev = torch.empty(max(1, cap), dtype=torch.int64, device=dev)
+        n = C.c_int64(0)
+        st = N.StarStats()
+        N.check(self.lib.bolap_star_groupby(self.ctx, C.byref(a), ptr(keys), ptr(rev),
+                                            keys.numel() if want_rows else 0, C.byref(n),
+                                            C.byref(st)))
+        k = int(a.n_dims)
+        stats = {"dim_pass": list(st.dim_pass)[:k], "dim_placement": list(st.dim_placement)[:k],
+                 "group_placement": st.group_placement, "matches": st.matches,
+                 "groups": st.groups, "group_capacity": st.group_capacity,
+                 "revenue": st.revenue, "fallbacks": st.fallbacks,
+                 "count_passes": st.count_passes}
+        if want_rows:
+            return stats, keys[:n.value], rev[:n.value]
+        return stats, None, None
+
+    def hll(self, col):
+        """HyperLogLog registers (4096, uint8 numpy) of a device int32 column."""
+        import numpy as np
+        regs = np.zeros(N.HLL_REGS, np.uint8)
+        N.check(self.lib.bolap_hll_i32(self.ctx, ptr(col), int(col.numel()),
+                                       regs.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return regs
+
+    def sample_counts(self, args: list, seed: int):
+        """Sample counts (n_plans x 16 int64 numpy) of a batch of star plans."""
+        import numpy as np
+        n = len(args)
+        out = np.zeros((max(1, n), N.SAMPLE_COUNTS), np.int64)
+        if n == 0:
+            return out[:0]
+        arr = (N.StarArgs * n)(*args)
+        N.check(self.lib.bolap_star_sample_counts(self.ctx, arr, n, seed,
+                                                  out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
     # ------------------------------------------------------- generators
     def gen_lineitem(self, seed: int, rows: int) -> dict:
         torch = _torch()

@@ -45,6 +45,34 @@ class Pred(C.Structure):
                 ("inline_set", C.c_uint64 * 4), ("set", vp), ("set_bits", C.c_int64)]
 
 
+
+STAR_MAX_DIMS = 3
+STAR_LDS_KEYS = 1024
+HLL_REGS = 4096
+SAMPLE_ROWS = 4096
+SAMPLE_COUNTS = 16
+
+
+class StarDim(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("key", vp), ("key_min", C.c_int32),
+                ("key_max", C.c_int32), ("n_preds", C.c_int32), ("preds", Pred * MAX_PREDS),
+                ("fact_fk", vp), ("pass_hint", C.c_int64)]
+
+
+class StarArgs(C.Structure):
+    _fields_ = [("n_dims", C.c_int32), ("dims", StarDim * STAR_MAX_DIMS),
+                ("fact_rows", C.c_int64), ("group_key", vp), ("group_min", C.c_int32),
+                ("group_max", C.c_int32), ("fact_price", vp), ("price_dtype", C.c_int32),
+                ("fact_disc", vp), ("disc_dtype", C.c_int32), ("group_hint", C.c_int64)]
+
+
+class StarStats(C.Structure):
+    _fields_ = [("dim_pass", C.c_int64 * STAR_MAX_DIMS),
+                ("dim_placement", C.c_int32 * STAR_MAX_DIMS),
+                ("group_placement", C.c_int32), ("matches", C.c_int64), ("groups", C.c_int64),
+                ("group_capacity", C.c_int64), ("revenue", C.c_int64),
+                ("fallbacks", C.c_int32), ("count_passes", C.c_int32)]
+
 class Q1Args(C.Structure):
     _fields_ = [("layout", C.c_int32), ("rows", C.c_int64), ("ship", vp),
                 ("ship_lo", C.c_int64), ("ship_hi", C.c_int64),
@@ -120,6 +148,15 @@ PROTOTYPES = {
                                             P(ChainStats)]),
     "bolap_radix_sort_u64": (C.c_int, [vp, vp, vp, i64, i32]),
     "bolap_topk_u64": (C.c_int, [vp, vp, vp, i64, i32, vp, vp]),
+    "bolap_star_groupby": (C.c_int, [vp, P(StarArgs), vp, vp, i64, P(C.c_int64),
+                                     P(StarStats)]),
+    "bolap_hll_i32": (C.c_int, [vp, vp, i64, P(C.c_uint8)]),
+    "bolap_star_sample_counts": (C.c_int, [vp, P(StarArgs), C.c_int32, u64, P(C.c_int64)]),
+    "bolap_gen_iota_i32": (C.c_int, [vp, i64, C.c_int32, vp]),
+    "bolap_gen_uniform_i32": (C.c_int, [vp, u64, C.c_uint32, i64, C.c_int32, C.c_int32, vp]),
+    "bolap_gen_uniform_u8": (C.c_int, [vp, u64, C.c_uint32, i64, C.c_int32, C.c_int32, vp]),
+    "bolap_gen_zipf_i32": (C.c_int, [vp, u64, C.c_uint32, i64, vp, i64, u64, vp]),
+    "bolap_gen_price_disc": (C.c_int, [vp, u64, C.c_uint32, i64, vp, vp]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),
     "bolap_mlp_destroy": (C.c_int, [vp]),
     "bolap_mlp_num_params": (C.c_int, []),

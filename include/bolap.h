@@ -266,6 +266,83 @@ int bolap_chain_groups_sorted(bolap_ctx* ctx, const bolap_chain_args* args,
                               bolap_chain_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* Star join + hash group-by: a fact table semi-joined with 1..3 filtered      */
+/* dimensions, grouped on one fact column, SUM(price * (100 - disc)) per      */
+/* group.  Shape of the CEB star queries (bespoke_ceb/query_q1a.cpp:241-621:  */
+/* dimension masks, then a fact scan) and of the estimate-sized group maps    */
+/* (bespoke_tpch/query_q20.cpp:132-223, 405-417; bespoke_ceb/query_q11b.cpp:  */
+/* 273-362).  Sizing: with every hint 0 the pipeline counts first (each       */
+/* dimension filter, then the join rows) and sizes from the counts; with      */
+/* hints it skips the counting passes.  A hint that proves too small (a       */
+/* table over half full, or an insert past the probe bound) reruns the whole  */
+/* query with exact sizing and reports fallbacks = 1; results never change.   */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_STAR_MAX_DIMS 3
+#define BOLAP_STAR_LDS_KEYS 1024   /* largest key set staged in LDS per dimension */
+
+typedef struct {
+    int64_t rows;
+    const int32_t* key;          /* unique keys within [key_min, key_max] */
+    int32_t key_min, key_max;    /* key range from the table's statistics */
+    int32_t n_preds;
+    bolap_pred preds[BOLAP_MAX_PREDS];
+    const int32_t* fact_fk;      /* fact column joined to key */
+    int64_t pass_hint;           /* estimated passing rows; 0 = count exactly */
+} bolap_star_dim;
+
+typedef struct {
+    int32_t n_dims;              /* 1..BOLAP_STAR_MAX_DIMS */
+    bolap_star_dim dims[BOLAP_STAR_MAX_DIMS];
+    int64_t fact_rows;
+    const int32_t* group_key;    /* fact column grouped on */
+    int32_t group_min, group_max;/* its range from the table's statistics */
+    const void* fact_price;      /* BOLAP_I64 or BOLAP_I32 */
+    int32_t price_dtype;
+    const void* fact_disc;       /* BOLAP_I64 or BOLAP_U8 */
+    int32_t disc_dtype;
+    int64_t group_hint;          /* estimated group count; 0 = count join rows first */
+} bolap_star_args;
+
+typedef struct {
+    int64_t dim_pass[BOLAP_STAR_MAX_DIMS];
+    int32_t dim_placement[BOLAP_STAR_MAX_DIMS];  /* LDS (hash set) or HBM (bitmap) */
+    int32_t group_placement;     /* LDS (per-workgroup pre-aggregation) or HBM */
+    int64_t matches;             /* join rows */
+    int64_t groups;
+    int64_t group_capacity;
+    int64_t revenue;             /* SUM over every group */
+    int32_t fallbacks;
+    int32_t count_passes;        /* counting passes that ran (0 when fully hinted) */
+} bolap_star_stats;
+
+/* Groups in ascending key order into keys_out / rev_out (device arrays of
+   max_rows; both may be null to get the statistics only). */
+int bolap_star_groupby(bolap_ctx* ctx, const bolap_star_args* args, int32_t* keys_out,
+                       int64_t* rev_out, int64_t max_rows, int64_t* n_out,
+                       bolap_star_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Table statistics for the estimator (computed once per column, at load).   */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_HLL_REGS 4096        /* HyperLogLog, p = 12 */
+#define BOLAP_SAMPLE_ROWS 4096
+#define BOLAP_SAMPLE_COUNTS 16
+
+/* HyperLogLog registers of an int32 column (host array of 4096 bytes):
+   h = mix64(value), register h >> 52, rank = leading zeros of (h << 12) + 1. */
+int bolap_hll_i32(bolap_ctx* ctx, const int32_t* col, int64_t n, uint8_t* regs_host);
+
+/* Sample counts of n_plans star plans on 4096 sampled rows (row i of a table
+   of n rows is ((mix-hash(seed, stream, i) >> 32) * n) >> 32).  Per plan,
+   BOLAP_SAMPLE_COUNTS int64 (host): [0..2] sampled dimension rows passing
+   each dimension filter; [3..5] sampled fact rows whose key passes each
+   dimension; [6] sampled fact rows passing every dimension; [7] distinct
+   group keys among those; [8] of them seen once; [9] seen twice.  Needs
+   dimension keys equal to key_min + row. */
+int bolap_star_sample_counts(bolap_ctx* ctx, const bolap_star_args* plans, int32_t n_plans,
+                             uint64_t seed, int64_t* counts_host);
+
+/* ------------------------------------------------------------------------ */
 /* LSD radix sort of 64-bit keys with 32-bit payloads (ascending, stable).    */
 /* Replaces the 16-bit-digit counting sorts of bespoke_tpch/query_q10.cpp:    */
 /* 134-166 and 318-373.  keys/vals are device arrays, sorted in place.        */
@@ -336,6 +413,23 @@ int bolap_gen_orders(bolap_ctx* ctx, uint64_t seed, int64_t rows, int64_t cust_r
 int bolap_gen_order_facts(bolap_ctx* ctx, uint64_t seed, int64_t order_rows,
                           const int64_t* fact_offsets, int64_t fact_rows,
                           int32_t* fk_order, int64_t* price, int64_t* disc);
+
+/* Generic columns for the star schema.  Stream ids select independent
+   sequences.  zipf: ranks drawn through a 65,537-entry inverse-CDF table
+   (quantiles[i] = rank at cumulative probability i / 65536, 1-based,
+   non-decreasing; linear interpolation between entries), then scattered over
+   1..n_keys by key = (rank - 1) * mul % n_keys + 1 (mul coprime to n_keys). */
+int bolap_gen_iota_i32(bolap_ctx* ctx, int64_t rows, int32_t first, int32_t* out);
+int bolap_gen_uniform_i32(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                          int32_t lo, int32_t hi, int32_t* out);
+int bolap_gen_uniform_u8(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                         int32_t lo, int32_t hi, uint8_t* out);
+int bolap_gen_zipf_i32(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                       const int32_t* quantiles_dev, int64_t n_keys, uint64_t mul,
+                       int32_t* out);
+/* price = U{1..50} * U{90000..200000} / 100, disc = U{0..10} (streams s, s+1, s+2). */
+int bolap_gen_price_disc(bolap_ctx* ctx, uint64_t seed, uint32_t stream, int64_t rows,
+                         int64_t* price, int64_t* disc);
 
 #ifdef __cplusplus
 }

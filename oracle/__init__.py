@@ -240,3 +240,130 @@ def topk_u64(keys: np.ndarray, vals: np.ndarray, k: int):
     vv = np.ascontiguousarray(vals.astype(np.uint32))
     lib().oracle_topk_u64(_p(kk), _p(vv), len(kk), k, _p(ko), _p(vo))
     return ko[:m], vo[:m]
+
+
+# ------------------------------------------------------ star schema (star.c)
+class StarDim(C.Structure):
+    _fields_ = [("rows", i64), ("key", vp), ("key_min", i32), ("key_max", i32),
+                ("n_preds", i32), ("preds", C.POINTER(Pred)), ("fact_fk", vp)]
+
+
+class StarStats(C.Structure):
+    _fields_ = [("dim_pass", i64 * 3), ("matches", i64), ("groups", i64), ("revenue", i64)]
+
+
+def _star_protos(L) -> None:
+    if getattr(L, "_star_ready", False):
+        return
+    L.og_uniform_i32.argtypes = [u64, C.c_uint32, i64, i32, i32, vp]
+    L.og_uniform_u8.argtypes = [u64, C.c_uint32, i64, i32, i32, vp]
+    L.og_zipf_i32.argtypes = [u64, C.c_uint32, i64, vp, i64, u64, vp]
+    L.og_price_disc.argtypes = [u64, C.c_uint32, i64, vp, vp]
+    L.oracle_hll_i32.argtypes = [vp, i64, vp]
+    L.oracle_star_groupby.restype = i64
+    L.oracle_star_groupby.argtypes = [i32, C.POINTER(StarDim), i64, vp, i32, i32, vp, i32, vp,
+                                      i32, vp, vp, i64, C.POINTER(StarStats)]
+    L.oracle_star_sample_counts.argtypes = [i32, C.POINTER(StarDim), i64, vp, u64, vp]
+    L._star_ready = True
+
+
+def _slib():
+    L = lib()
+    _star_protos(L)
+    return L
+
+
+def zipf_i32(seed: int, stream: int, rows: int, quantiles: np.ndarray, n_keys: int,
+             mul: int) -> np.ndarray:
+    q = np.ascontiguousarray(quantiles, np.int32)
+    out = np.empty(rows, np.int32)
+    _slib().og_zipf_i32(seed, stream, rows, _p(q), n_keys, mul, _p(out))
+    return out
+
+
+def uniform_i32(seed: int, stream: int, rows: int, lo: int, hi: int) -> np.ndarray:
+    out = np.empty(rows, np.int32)
+    _slib().og_uniform_i32(seed, stream, rows, lo, hi, _p(out))
+    return out
+
+
+def uniform_u8(seed: int, stream: int, rows: int, lo: int, hi: int) -> np.ndarray:
+    out = np.empty(rows, np.uint8)
+    _slib().og_uniform_u8(seed, stream, rows, lo, hi, _p(out))
+    return out
+
+
+def hll_i32(col: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(col, np.int32)
+    regs = np.zeros(4096, np.uint8)
+    _slib().oracle_hll_i32(_p(c), len(c), _p(regs))
+    return regs
+
+
+def gen_star(seed: int, fact_rows: int, dim_rows, fk_s, n_groups: int, group_s: float) -> dict:
+    """Host copy of the device star schema (same generators, same streams)."""
+    from bespokeolap_artifacts_amd import workload as W
+    dims = []
+    for j, rows in enumerate(dim_rows):
+        dims.append({"key": np.arange(1, rows + 1, dtype=np.int32),
+                     "cat": uniform_u8(seed, W.dim_stream(j, 1), rows, 0, W.CAT_MAX),
+                     "val": uniform_i32(seed, W.dim_stream(j, 2), rows, 0, W.VAL_MAX)})
+    fact = {}
+    for j, rows in enumerate(dim_rows):
+        fact[f"fk{j}"] = zipf_i32(seed, W.FK_STREAM + j, fact_rows,
+                                  W.zipf_quantiles(rows, fk_s[j]), rows, W.coprime_mul(rows))
+    fact["grp"] = zipf_i32(seed, W.GRP_STREAM, fact_rows, W.zipf_quantiles(n_groups, group_s),
+                           n_groups, W.coprime_mul(n_groups))
+    price = np.empty(fact_rows, np.int64)
+    disc = np.empty(fact_rows, np.int64)
+    _slib().og_price_disc(seed, W.PRICE_STREAM, fact_rows, _p(price), _p(disc))
+    fact["price"], fact["disc"] = price, disc
+    return {"dims": dims, "fact": fact, "dim_rows": list(dim_rows), "n_groups": n_groups}
+
+
+def _bind_star(t: dict, plan: dict):
+    keep = []
+    arr = (StarDim * max(1, len(plan["dims"])))()
+    for i, d in enumerate(plan["dims"]):
+        j = d["dim"]
+        dt = t["dims"][j]
+        preds = [make_pred(dt[p["col"]], p["op"], p.get("lo", 0), p.get("hi", 0),
+                           values=p.get("values")) for p in d["preds"]]
+        pa = (Pred * max(1, len(preds)))(*preds)
+        keep.append(pa)
+        arr[i].rows = len(dt["key"])
+        arr[i].key = _p(dt["key"])
+        arr[i].key_min, arr[i].key_max = 1, t["dim_rows"][j]
+        arr[i].n_preds = len(preds)
+        arr[i].preds = C.cast(pa, C.POINTER(Pred))
+        arr[i].fact_fk = _p(t["fact"][f"fk{j}"])
+    return arr, keep
+
+
+def _group_range(t: dict, col: str):
+    return (1, t["n_groups"]) if col == "grp" else (1, t["dim_rows"][int(col[2:])])
+
+
+def star_groupby(t: dict, plan: dict):
+    """(stats, keys, revenues) of a star plan over host tables, groups in key order."""
+    arr, keep = _bind_star(t, plan)
+    gmin, gmax = _group_range(t, plan["group"])
+    gk = t["fact"][plan["group"]]
+    cap = gmax - gmin + 1
+    keys = np.empty(cap, np.int32)
+    rev = np.empty(cap, np.int64)
+    st = StarStats()
+    n = _slib().oracle_star_groupby(len(plan["dims"]), arr, len(gk), _p(gk), gmin, gmax,
+                                    _p(t["fact"]["price"]), 3, _p(t["fact"]["disc"]), 3,
+                                    _p(keys), _p(rev), cap, C.byref(st))
+    stats = {"dim_pass": list(st.dim_pass)[:len(plan["dims"])], "matches": st.matches,
+             "groups": st.groups, "revenue": st.revenue}
+    return stats, keys[:n].copy(), rev[:n].copy()
+
+
+def star_sample_counts(t: dict, plan: dict, seed: int) -> np.ndarray:
+    arr, keep = _bind_star(t, plan)
+    gk = t["fact"][plan["group"]]
+    out = np.zeros(16, np.int64)
+    _slib().oracle_star_sample_counts(len(plan["dims"]), arr, len(gk), _p(gk), seed, _p(out))
+    return out

@@ -21,6 +21,7 @@ typedef unsigned __int128 u128;
 
 static int32_t g_mutation = 0;
 void oracle_set_mutation(int32_t m) { g_mutation = m; }
+int32_t oracle_mutation(void) { return g_mutation; }
 
 /* ------------------------------------------------------------------------ */
 /* Seeded generators.                                                         */

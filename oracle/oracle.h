@@ -103,8 +103,42 @@ void oracle_radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int32_t ke
 void oracle_topk_u64(const uint64_t* keys, const uint32_t* vals, int64_t n, int32_t k,
                      uint64_t* keys_out, uint32_t* vals_out);
 
+/* --- star join + group-by and estimator statistics (star.c) ------------- */
+void og_uniform_i32(uint64_t seed, uint32_t stream, int64_t rows, int32_t lo, int32_t hi,
+                    int32_t* out);
+void og_uniform_u8(uint64_t seed, uint32_t stream, int64_t rows, int32_t lo, int32_t hi,
+                   uint8_t* out);
+void og_zipf_i32(uint64_t seed, uint32_t stream, int64_t rows, const int32_t* quantiles,
+                 int64_t n_keys, uint64_t mul, int32_t* out);
+void og_price_disc(uint64_t seed, uint32_t stream, int64_t rows, int64_t* price,
+                   int64_t* disc);
+void oracle_hll_i32(const int32_t* col, int64_t n, uint8_t* regs);
+
+typedef struct {
+    int64_t rows;
+    const int32_t* key;
+    int32_t key_min, key_max;
+    int32_t n_preds;
+    const oracle_pred* preds;
+    const int32_t* fact_fk;
+} oracle_star_dim;
+
+typedef struct {
+    int64_t dim_pass[3];
+    int64_t matches, groups, revenue;
+} oracle_star_stats;
+
+int64_t oracle_star_groupby(int32_t n_dims, const oracle_star_dim* dims, int64_t fact_rows,
+                            const int32_t* group_key, int32_t group_min, int32_t group_max,
+                            const void* price, int32_t price_dtype, const void* disc,
+                            int32_t disc_dtype, int32_t* keys_out, int64_t* rev_out,
+                            int64_t max_out, oracle_star_stats* stats);
+void oracle_star_sample_counts(int32_t n_dims, const oracle_star_dim* dims, int64_t fact_rows,
+                               const int32_t* group_key, uint64_t seed, int64_t* counts);
+
 /* --- planted mistakes, for tests that show the checks have teeth -------- */
 void oracle_set_mutation(int32_t m);
+int32_t oracle_mutation(void);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,44 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(native.JoinSumArgs) == 8 + 8 + 8 + 80 * 4 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8
 
 
+STRUCTS = {  # ctypes mirror -> C type name in bolap.h
+    "DeviceInfo": "bolap_device_info", "TraceEntry": "bolap_trace_entry", "Pred": "bolap_pred",
+    "Q1Args": "bolap_q1_args", "Q1Group": "bolap_q1_group",
+    "JoinSumArgs": "bolap_join_sum_args", "JoinSumResult": "bolap_join_sum_result",
+    "ChainArgs": "bolap_chain_args", "GroupRow": "bolap_group_row",
+    "ChainStats": "bolap_chain_stats", "GenLineitemCols": "bolap_gen_lineitem_cols",
+    "StarDim": "bolap_star_dim", "StarArgs": "bolap_star_args", "StarStats": "bolap_star_stats",
+}
+
+
+def test_every_struct_matches_the_compiled_layout(tmp_path):
+    """Compiles the header with gcc and compares sizeof/offsetof of every
+    field with the ctypes mirror."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bolap.h"', 'int main(void){']
+    expect = []
+    for py, cname in STRUCTS.items():
+        cls = getattr(native, py)
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(cls, fname).offset)
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == expect
+
+
 def test_loader_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(native, "_lib", None)
     monkeypatch.setattr(native, "LIB_PATH", tmp_path / "libbolap.so")
