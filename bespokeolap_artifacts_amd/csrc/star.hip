@@ -15,8 +15,10 @@
 //     BOLAP_STAR_LDS_KEYS keys pass, or
 //   * an exact bitmap over [key_min, key_max] in HBM (probed through L2).
 // The group table is open addressing in HBM (u32 key - group_min + 1, u64
-// revenue); when few groups are expected each workgroup pre-aggregates into
-// a 4096-slot LDS table first and spills to HBM only on an LDS miss.
+// revenue); each workgroup pre-aggregates into an LDS table first and spills
+// to HBM only on an LDS miss.  Groups come out in key order through a
+// bitmap over the key range (popcount prefix sums give each group its rank)
+// or, for key ranges above 2^26, through the LSD radix sort.
 //
 // Sizing: the plain ("exact") path counts each dimension filter, then counts
 // the join rows with a probe pass, and sizes from those counts.  The hinted
@@ -39,10 +41,13 @@ constexpr int kT = 256;
 constexpr int kMaxDims = BOLAP_STAR_MAX_DIMS;
 constexpr int kSetLog2 = 11;
 constexpr int kSetCap = 1 << kSetLog2;  // 2 x BOLAP_STAR_LDS_KEYS
-constexpr int kGlLog2 = 12;
-constexpr int kGlCap = 1 << kGlLog2;    // LDS pre-aggregation slots
-constexpr int64_t kGlMaxGroups = 1024;  // estimate that selects LDS pre-aggregation
-constexpr int kGlProbe = 32;
+// LDS pre-aggregation table per workgroup: 4096 slots when every group is
+// expected to fit (<= 1024 groups), else 2048 slots that catch the hot keys
+// (Zipf-skewed group keys would otherwise serialise on a few HBM atomics).
+constexpr int kGlLog2Small = 12;
+constexpr int kGlLog2Large = 11;
+constexpr int64_t kGlMaxGroups = 1024;
+constexpr int kGlProbe = 8;
 constexpr uint32_t kMul = 0x9E3779B1u;
 constexpr int kPlaceSet = BOLAP_PLACE_LDS, kPlaceBits = BOLAP_PLACE_HBM;
 
@@ -201,18 +206,19 @@ __global__ __launch_bounds__(kT) void star_probe_k(StarK s, GTab g,
                                                    int32_t gmin, int64_t gspan,
                                                    const TP* __restrict__ price,
                                                    const TD* __restrict__ disc, int64_t rows,
-                                                   int64_t* st) {
+                                                   int gl_log2, int64_t* st) {
     extern __shared__ uint32_t lds[];
     const int set_words = s.n_lds * kSetCap;
+    const int gl_cap = 1 << gl_log2;
     uint32_t* lkeys = lds + set_words;
-    uint64_t* lrev = reinterpret_cast<uint64_t*>(lkeys + kGlCap);
+    uint64_t* lrev = reinterpret_cast<uint64_t*>(lkeys + gl_cap);
     for (int j = 0; j < s.n; ++j) {
         const DimK& d = s.d[j];
         if (d.place != kPlaceSet) continue;
         for (int i = threadIdx.x; i < kSetCap; i += kT) lds[d.lds_off + i] = d.set[i];
     }
     if (kMode == 2) {
-        for (int i = threadIdx.x; i < kGlCap; i += kT) {
+        for (int i = threadIdx.x; i < gl_cap; i += kT) {
             lkeys[i] = 0u;
             lrev[i] = 0u;
         }
@@ -253,15 +259,16 @@ __global__ __launch_bounds__(kT) void star_probe_k(StarK s, GTab g,
             const uint64_t rv = uint64_t(ld(price, r) * (100 - ld(disc, r)));
             bool done = false;
             if (kMode == 2) {
-                uint32_t sl = slot_of(key1, kGlLog2);
+                uint32_t sl = slot_of(key1, gl_log2);
                 for (int probe = 0; probe < kGlProbe && !done; ++probe) {
-                    const uint32_t prev = atomicCAS(lkeys + sl, 0u, key1);
-                    if (prev == 0u || prev == key1) {
+                    uint32_t x = lkeys[sl];  // keys only go 0 -> key
+                    if (x == 0u) x = atomicCAS(lkeys + sl, 0u, key1);
+                    if (x == 0u || x == key1) {
                         atomicAdd(reinterpret_cast<unsigned long long*>(lrev + sl),
                                   (unsigned long long)rv);
                         done = true;
                     }
-                    sl = (sl + 1) & (kGlCap - 1);
+                    sl = (sl + 1) & uint32_t(gl_cap - 1);
                 }
             }
             if (!done && !global_add(g, key1, rv, fresh)) flags |= kFlagOverflow;
@@ -269,7 +276,7 @@ __global__ __launch_bounds__(kT) void star_probe_k(StarK s, GTab g,
     }
     if (kMode == 2) {
         __syncthreads();
-        for (int i = threadIdx.x; i < kGlCap; i += kT) {
+        for (int i = threadIdx.x; i < gl_cap; i += kT) {
             const uint32_t key1 = lkeys[i];
             if (key1 && !global_add(g, key1, lrev[i], fresh)) flags |= kFlagOverflow;
         }
@@ -289,7 +296,7 @@ __global__ __launch_bounds__(kCT) void star_compact_k(GTab g, int64_t chunk,
                                                       uint64_t* __restrict__ okeys,
                                                       uint32_t* __restrict__ oslot,
                                                       unsigned long long* __restrict__ n_out,
-                                                      int64_t* st) {
+                                                      uint32_t* __restrict__ bits, int64_t* st) {
     constexpr int kW = kCT / kWave;
     __shared__ uint32_t s_w[kW];
     __shared__ unsigned long long s_base;
@@ -332,6 +339,7 @@ __global__ __launch_bounds__(kCT) void star_compact_k(GTab g, int64_t chunk,
             const unsigned long long at = base + before + before_lane;
             okeys[at] = uint64_t(key1 - 1);
             oslot[at] = uint32_t(i);
+            if (bits) atomicOr(bits + ((key1 - 1) >> 5), 1u << ((key1 - 1) & 31));
         }
         base += tile;
     }
@@ -347,6 +355,32 @@ __global__ __launch_bounds__(kT) void star_gather_k(const uint64_t* __restrict__
          i += int64_t(gridDim.x) * kT) {
         if (kout) kout[i] = int32_t(int64_t(gmin) + int64_t(keys[i]));
         if (rout) rout[i] = int64_t(rev[slots[i]]);
+    }
+}
+
+__global__ __launch_bounds__(kT) void word_pop_k(const uint32_t* __restrict__ bits, int64_t w,
+                                                 int64_t* __restrict__ cnt) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < w;
+         i += int64_t(gridDim.x) * kT)
+        cnt[i] = __popc(bits[i]);
+}
+
+// Group i goes to its rank among the present keys: the exclusive prefix of
+// the word popcounts plus the set bits below it in its word.
+__global__ __launch_bounds__(kT) void rank_scatter_k(const uint64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ slots,
+                                                     const uint64_t* __restrict__ rev, int64_t n,
+                                                     const uint32_t* __restrict__ bits,
+                                                     const int64_t* __restrict__ prefix,
+                                                     int32_t gmin, int32_t* __restrict__ kout,
+                                                     int64_t* __restrict__ rout) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const uint64_t v = keys[i];
+        const int64_t w = int64_t(v >> 5);
+        const int64_t pos = prefix[w] + __popc(bits[w] & ((1u << (v & 31)) - 1u));
+        if (kout) kout[pos] = int32_t(int64_t(gmin) + int64_t(v));
+        if (rout) rout[pos] = int64_t(rev[slots[i]]);
     }
 }
 
@@ -388,12 +422,16 @@ int validate(const bolap_star_args* a) {
     BOLAP_ARG_CHECK(a->disc_dtype == BOLAP_I64 || a->disc_dtype == BOLAP_U8,
                     "bolap_star: disc dtype %d", a->disc_dtype);
     BOLAP_ARG_CHECK(a->group_hint >= 0, "bolap_star: group_hint < 0");
+    BOLAP_ARG_CHECK(int64_t(a->group_max) - a->group_min < int64_t(0xFFFFFFFF),
+                    "bolap_star: the group key range must hold fewer than 2^32 - 1 keys");
     BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->group_key && a->fact_price && a->fact_disc),
                     "bolap_star: null fact column");
     for (int j = 0; j < a->n_dims; ++j) {
         const bolap_star_dim& d = a->dims[j];
         BOLAP_ARG_CHECK(d.rows >= 0 && d.key_min <= d.key_max && d.pass_hint >= 0,
                         "bolap_star: dimension %d: bad rows, key range or hint", j);
+        BOLAP_ARG_CHECK(int64_t(d.key_max) - d.key_min < (int64_t(1) << 31),
+                        "bolap_star: dimension %d: key range above 2^31 keys", j);
         BOLAP_ARG_CHECK(d.rows == 0 || d.key, "bolap_star: dimension %d has no key column", j);
         BOLAP_ARG_CHECK(a->fact_rows == 0 || d.fact_fk,
                         "bolap_star: dimension %d has no fact key column", j);
@@ -409,12 +447,16 @@ struct StarRun {
     int32_t group_place = kPlaceBits;
     int32_t count_passes = 0;
     size_t o_keys = 0, o_slots = 0, o_rev = 0;
+    size_t o_bits = 0, o_cnt = 0;
+    int64_t bit_words = 0;     // > 0: key-order output through the bitmap
     bool overflow = false;
 };
 
+constexpr int64_t kMaxBitmapSpan = int64_t(1) << 26;
+
 template <int kMode>
 void launch_probe(bolap_ctx* ctx, const bolap_star_args* a, const StarK& k, const GTab& g,
-                  unsigned grid, size_t lds, int64_t* st) {
+                  unsigned grid, size_t lds, int gl_log2, int64_t* st) {
     const int64_t gspan = int64_t(a->group_max) - a->group_min + 1;
     const bool p64 = a->price_dtype == BOLAP_I64, d64 = a->disc_dtype == BOLAP_I64;
     // More than 64 KB of dynamic LDS needs the per-kernel limit raised.
@@ -425,7 +467,7 @@ void launch_probe(bolap_ctx* ctx, const bolap_star_args* a, const StarK& k, cons
         hipLaunchKernelGGL((star_probe_k<TP, TD, kMode>), dim3(grid), dim3(kT), lds,          \
                            ctx->stream, k, g, a->group_key, a->group_min, gspan,              \
                            static_cast<const TP*>(a->fact_price),                             \
-                           static_cast<const TD*>(a->fact_disc), a->fact_rows, st);           \
+                           static_cast<const TD*>(a->fact_disc), a->fact_rows, gl_log2, st);  \
     } while (0)
     if (p64 && d64) BOLAP_STAR_LAUNCH(int64_t, int64_t);
     else if (p64) BOLAP_STAR_LAUNCH(int64_t, uint8_t);
@@ -445,7 +487,8 @@ unsigned probe_grid(bolap_ctx* ctx, int64_t rows, size_t lds) {
     return grid_for(ctx, rows / 4 + 1, per_cu);
 }
 
-int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, StarRun& run) {
+int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, bool want_rows, WsArena& ws,
+             StarRun& run) {
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
     hipLaunchKernelGGL(st_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
     BOLAP_TRY(launch_check("st_init_k"));
@@ -522,7 +565,8 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
     if (exact || est_g == 0) {
         if (a->fact_rows > 0) {
             TraceScope ts(ctx, "star_count");
-            launch_probe<0>(ctx, a, k, g, probe_grid(ctx, a->fact_rows, set_lds), set_lds, st);
+            launch_probe<0>(ctx, a, k, g, probe_grid(ctx, a->fact_rows, set_lds), set_lds, 0,
+                            st);
             BOLAP_TRY(launch_check("star_probe_k(count)"));
         }
         BOLAP_TRY(read_stats(ctx, run.st));
@@ -536,7 +580,8 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
                     (long long)est_g);
     const int64_t cap = int64_t(1) << cap_log2;
     run.cap = cap;
-    const bool gl = est_g <= kGlMaxGroups && ctx->knob("star.group_lds", 1) == 1;
+    const bool gl = ctx->knob("star.group_lds", 1) == 1;
+    const int gl_log2 = est_g <= kGlMaxGroups ? kGlLog2Small : kGlLog2Large;
     run.group_place = gl ? kPlaceSet : kPlaceBits;
     const size_t keep = ws.off;
     const size_t o_gk = ws.plan(size_t(cap) * 4);
@@ -545,6 +590,11 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
     run.o_slots = ws.plan(size_t(cap) * 4);
     run.o_rev = o_gr;
     const size_t o_cnt = ws.plan(64);
+    if (want_rows && gspan <= kMaxBitmapSpan) {
+        run.bit_words = (gspan + 31) / 32;
+        run.o_bits = ws.plan(size_t(run.bit_words) * 4);
+        run.o_cnt = ws.plan(size_t(run.bit_words) * 8);
+    }
     BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
     ws.base = static_cast<char*>(ctx->ws);
     for (int j = 0; j < a->n_dims; ++j) k.d[j].set = ws.at<uint32_t>(o_set[j]);
@@ -566,14 +616,17 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
                            cap);
         hipLaunchKernelGGL(fill32_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<uint32_t>(o_cnt),
                            int64_t(16));
+        if (run.bit_words)
+            hipLaunchKernelGGL(fill32_k, dim3(grid_for(ctx, run.bit_words)), dim3(kT), 0,
+                               ctx->stream, ws.at<uint32_t>(run.o_bits), run.bit_words);
     }
     BOLAP_TRY(launch_check("star_init"));
     if (a->fact_rows > 0) {
         TraceScope ts(ctx, "star_probe");
-        const size_t lds = set_lds + (gl ? size_t(kGlCap) * 12 : 0);
+        const size_t lds = set_lds + (gl ? (size_t(12) << gl_log2) : 0);
         const unsigned grid = probe_grid(ctx, a->fact_rows, lds);
-        if (gl) launch_probe<2>(ctx, a, k, g, grid, lds, st);
-        else launch_probe<1>(ctx, a, k, g, grid, lds, st);
+        if (gl) launch_probe<2>(ctx, a, k, g, grid, lds, gl_log2, st);
+        else launch_probe<1>(ctx, a, k, g, grid, lds, 0, st);
     }
     BOLAP_TRY(launch_check("star_probe_k"));
     auto* n_rows = ws.at<unsigned long long>(o_cnt);
@@ -585,7 +638,7 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
         cblocks = (cap + chunk - 1) / chunk;
         hipLaunchKernelGGL(star_compact_k, dim3(unsigned(cblocks)), dim3(kCT), 0, ctx->stream, g,
                            chunk, ws.at<uint64_t>(run.o_keys), ws.at<uint32_t>(run.o_slots),
-                           n_rows, st);
+                           n_rows, run.bit_words ? ws.at<uint32_t>(run.o_bits) : nullptr, st);
     }
     BOLAP_TRY(launch_check("star_compact_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
@@ -599,10 +652,7 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, WsArena& ws, 
     }
     BOLAP_ARG_CHECK(run.st[sRange] == 0,
                     "bolap_star: a key lies outside the [min, max] range passed in the args");
-    BOLAP_TRY(pinned_reserve(ctx, 8));
-    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::memcpy(&run.n_groups, ctx->pinned, 8);
+    run.n_groups = run.st[sGroups];  // each group is created by exactly one CAS
     bool over = run.st[sOver] != 0 || run.st[sGroups] > cap / 2;
     for (int j = 0; j < a->n_dims; ++j)
         if (k.d[j].place == kPlaceSet && run.st[sSet0 + j] > BOLAP_STAR_LDS_KEYS) over = true;
@@ -617,6 +667,22 @@ int star_emit(bolap_ctx* ctx, const bolap_star_args* a, WsArena& ws, StarRun& ru
     BOLAP_ARG_CHECK(n <= max_rows, "bolap_star_groupby: %lld groups exceed max_rows %lld",
                     (long long)n, (long long)max_rows);
     if (n == 0) return BOLAP_OK;
+    if (run.bit_words) {
+        TraceScope ts(ctx, "star_order");
+        const size_t keep = ws.off;
+        const size_t o_scr = ws.plan(scan_scratch_bytes(run.bit_words));
+        BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
+        ws.base = static_cast<char*>(ctx->ws);
+        int64_t* cnt = ws.at<int64_t>(run.o_cnt);
+        hipLaunchKernelGGL(word_pop_k, dim3(grid_for(ctx, run.bit_words)), dim3(kT), 0,
+                           ctx->stream, ws.at<uint32_t>(run.o_bits), run.bit_words, cnt);
+        BOLAP_TRY(exclusive_scan_i64(ctx, cnt, run.bit_words, ws.at<void>(o_scr)));
+        hipLaunchKernelGGL(rank_scatter_k, dim3(grid_for(ctx, n)), dim3(kT), 0, ctx->stream,
+                           ws.at<uint64_t>(run.o_keys), ws.at<uint32_t>(run.o_slots),
+                           ws.at<uint64_t>(run.o_rev), n, ws.at<uint32_t>(run.o_bits), cnt,
+                           a->group_min, keys_out, rev_out);
+        return launch_check("rank_scatter_k");
+    }
     const int64_t gspan = int64_t(a->group_max) - a->group_min + 1;
     const int bits = std::max(1, ceil_log2(gspan));
     const size_t keep = ws.off;
@@ -650,7 +716,8 @@ extern "C" int bolap_star_groupby(bolap_ctx* ctx, const bolap_star_args* a, int3
     for (int j = 0; j < a->n_dims; ++j) exact = exact && a->dims[j].pass_hint == 0;
     StarRun run;
     WsArena ws;
-    BOLAP_TRY(star_run(ctx, a, exact, ws, run));
+    const bool want_rows = keys_out || rev_out;
+    BOLAP_TRY(star_run(ctx, a, exact, want_rows, ws, run));
     int32_t fallbacks = 0;
     if (run.overflow) {
         BOLAP_ARG_CHECK(!exact, "bolap_star_groupby: exact sizing overflowed");
@@ -658,11 +725,11 @@ extern "C" int bolap_star_groupby(bolap_ctx* ctx, const bolap_star_args* a, int3
         const int32_t passes = run.count_passes;
         ws = WsArena();
         run = StarRun();
-        BOLAP_TRY(star_run(ctx, a, true, ws, run));
+        BOLAP_TRY(star_run(ctx, a, true, want_rows, ws, run));
         run.count_passes += passes;
         BOLAP_ARG_CHECK(!run.overflow, "bolap_star_groupby: exact sizing overflowed");
     }
-    if (keys_out || rev_out) BOLAP_TRY(star_emit(ctx, a, ws, run, keys_out, rev_out, max_rows));
+    if (want_rows) BOLAP_TRY(star_emit(ctx, a, ws, run, keys_out, rev_out, max_rows));
     if (n_out) *n_out = run.n_groups;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));

@@ -73,8 +73,8 @@ def test_exact_plans_match_oracle(engine, small):
         assert st["fallbacks"] == 0 and st["count_passes"] == 2
         seen.update((p, ) for p in st["dim_placement"])
         seen.add(("g", st["group_placement"]))
-    # both dimension placements and both group placements occurred
-    assert {(N.PLACE_LDS,), (N.PLACE_HBM,), ("g", N.PLACE_LDS), ("g", N.PLACE_HBM)} <= seen
+    # both dimension placements occurred; groups pre-aggregate in LDS by default
+    assert {(N.PLACE_LDS,), (N.PLACE_HBM,), ("g", N.PLACE_LDS)} <= seen
 
 
 def test_hinted_plans_skip_counting_and_match(engine, small):
@@ -111,6 +111,21 @@ def test_group_placement_knob_gives_same_result(engine, small):
         assert st2["group_placement"] == N.PLACE_HBM
     finally:
         engine.tune_set("star.group_lds", 0)
+
+
+def test_wide_group_range_orders_through_radix_sort(engine, small):
+    """A group key range above 2^26 keys takes the radix-sort ordering path;
+    the groups must equal the bitmap-ordered ones."""
+    s, t = small
+    import torch
+    for plan in plans(8, seed=6):
+        a = W.bind(s, plan)
+        st1, k1, r1 = engine.star_groupby(a, want_rows=True)
+        b = W.bind(s, plan)
+        b.group_min, b.group_max = -(1 << 30), 1 << 30
+        st2, k2, r2 = engine.star_groupby(b, want_rows=True, max_rows=max(1, st1["groups"]))
+        assert torch.equal(k1, k2) and torch.equal(r1, r2)
+        assert st1["groups"] == st2["groups"] and st1["revenue"] == st2["revenue"]
 
 
 def test_edge_cases(engine):
