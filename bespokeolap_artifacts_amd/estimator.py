@@ -375,14 +375,22 @@ def would_fall_back(rec: Record, dims: list[int], group_hint: int) -> bool:
 
 def tune_margins(recs: list[Record], preds: list[dict], target: float = 0.02) -> dict:
     """Smallest margins (from a fixed ladder) whose fallback rate on the
-    training plans is at most `target`."""
-    ladder = [1.25, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0]
+    training plans is at most `target`: the dimension margin first (with the
+    largest group margin), then the group margin."""
+    ladder = [1.25, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0, 16.0]
+
+    def rate(margin):
+        fb = sum(would_fall_back(r, *sizing_hints(r, p, margin)) for r, p in zip(recs, preds))
+        return fb / max(1, len(recs))
+
     best = {"dim": ladder[-1], "group": ladder[-1]}
     for m in ladder:
-        margin = {"dim": m, "group": m}
-        fb = sum(would_fall_back(r, *sizing_hints(r, p, margin)) for r, p in zip(recs, preds))
-        if fb / max(1, len(recs)) <= target:
-            best = margin
+        if rate({"dim": m, "group": ladder[-1]}) <= target / 2:
+            best["dim"] = m
+            break
+    for m in ladder:
+        if rate({"dim": best["dim"], "group": m}) <= target:
+            best["group"] = m
             break
     return best
 
