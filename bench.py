@@ -30,12 +30,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workloads", default="c2,c1,c3,c4,copy")
+    ap.add_argument("--workloads", default="c2,c1,c3,c4,c5,copy")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c4-train", type=int, default=20_000, help="C4 training plans")
     ap.add_argument("--c4-eval", type=int, default=500, help="C4 held-out plans")
+    ap.add_argument("--c5-queries", type=int, default=512, help="C5 batch size")
     return ap.parse_args()
 
 
@@ -188,6 +189,52 @@ def bench_c4(eng, torch, args):
     return res
 
 
+# ------------------------------------------------------------ C5 batch
+def bench_c5(eng, torch, args):
+    """512 parameterised CEB-style queries (8 templates, 1-5 joins) over six
+    tables of 36M/15M/4.5M/2.5M/130k/100 rows: one at a time vs one batch
+    (shared scans, hipGraph replay, autotuned stage configuration)."""
+    from bespokeolap_artifacts_amd import ceb as CB
+    s = CB.CebSchema(eng, seed=SEED)
+    qs = CB.random_queries(s, args.c5_queries, seed=SEED)
+    bound = [CB.bind(s, q) for q in qs]
+    for b in bound[:8]:
+        CB.run_one(eng, s, b)
+    lat = []
+    single = []
+    t0 = time.perf_counter()
+    for b in bound:
+        t1 = time.perf_counter()
+        single.append(CB.run_one(eng, s, b))
+        lat.append(time.perf_counter() - t1)
+    t_single = time.perf_counter() - t0
+    tune_ms = CB.autotune(eng, s, bound)
+    batch = CB.run_batch(eng, s, bound)
+    reps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch = CB.run_batch(eng, s, bound)
+    t_batch = (time.perf_counter() - t0) / reps
+    n = len(bound)
+    lat_ms = np.array(lat) * 1e3
+    by_template = {}
+    for q, l in zip(qs, lat_ms):
+        by_template.setdefault(q["template"], []).append(l)
+    return {"workload": f"C5 {n} CEB-style queries, 8 templates, 1-5 joins, "
+                        "cast_info 36M / movie_info 15M / movie_keyword 4.5M / title 2.5M",
+            "queries": n, "ms_per_step": t_batch * 1e3,
+            "batched_queries_per_s": n / t_batch, "single_queries_per_s": n / t_single,
+            "batched_over_single": t_single / t_batch,
+            "batched_latency_ms": {"p50": t_batch * 1e3, "p99": t_batch * 1e3},
+            "single_latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
+                                  "p99": float(np.percentile(lat_ms, 99))},
+            "single_ms_by_template": {str(k): float(np.mean(v)) for k, v in
+                                      sorted(by_template.items())},
+            "autotuned": {"blocks_per_cu": eng.tune_get("ceb.blocks_per_cu"),
+                          "qchunk": eng.tune_get("ceb.qchunk"), "batch_ms": tune_ms},
+            "bit_identical": batch == single}
+
+
 # ------------------------------------------------------------- copy ceiling
 def bench_copy(eng, torch, args):
     """Plain device copy of 1 GiB: the measured bandwidth ceiling, and the
@@ -294,7 +341,7 @@ def main():
     for w in args.workloads.split(","):
         log("workload", w)
         results[w] = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
-                      "copy": bench_copy}[w](eng, torch, args)
+                      "c5": bench_c5, "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))
     # The headline is the first workload with a bandwidth figure (C2 by default).
     head = next((results[w] for w in results if "GBps" in results[w]),

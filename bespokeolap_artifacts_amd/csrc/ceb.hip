@@ -1,0 +1,698 @@
+// ceb.hip — CEB-style star queries over movie ids: COUNT(*) as the sum over
+// qualifying movies of the product of per-table match counts, and
+// MIN(production_year), one query at a time or as a batch.
+//
+// Reference: bespoke_ceb/query_q1a.cpp builds per-dimension masks (:69-152),
+// filters title into candidate movies (:241-306), counts matching rows per
+// movie in dense arrays per fact table (:346-467, :505-596) and sums the
+// per-movie products (:602-621); query_q10a.cpp:334 ff. keeps MIN aggregates.
+//
+// Device stages (the same kernels serve one query and a batch of n):
+//   1. title: per query, a bitmap over movies passing year / kind / title-
+//      string predicates; one wave evaluates 64 movies for every query of
+//      the batch, so the title columns are read once.
+//   2. rank: per query, the exclusive prefix of the bitmap's word popcounts,
+//      so qualifying movie m gets a dense index (its rank) and the per-table
+//      count arrays hold only qualifying movies.
+//   3. layout: per query and table, the offset of its count array in one
+//      arena (an exclusive scan over the batch).
+//   4. fact: one persistent kernel per fact table reads the table once and,
+//      for every query that uses the table, tests each row (title bitmap,
+//      row predicates, dimension bitmaps) and counts it.  Rows are sorted by
+//      movie, so a wave-wide segmented sum leaves one atomic per movie run.
+//   5. final: per query, the 128-bit sum of count products and the minimum
+//      year over movies with a match in every table; partials, then a merge.
+// A batch is captured once into a hipGraph and replayed; the fact stage's
+// workgroups per CU and query chunk come from the autotuner.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kRankT = 1024;
+constexpr int kMaxBatch = 8192;
+constexpr int kTables = 3;  // 0 cast_info, 1 movie_info, 2 movie_keyword
+
+struct QK {
+    int32_t tables, agg, year_lo, year_hi;
+    uint32_t kind_mask, role_mask;
+    const uint32_t* title_bits;
+    int64_t title_bits_n;
+    const uint32_t* info_bits;
+    int64_t info_bits_n;
+    const uint32_t* type_bits;
+    int64_t type_bits_n;
+    const uint32_t* kw_bits;
+    int64_t kw_bits_n;
+};
+
+struct QState {
+    uint64_t n_tf;              // movies passing the title predicates
+    uint64_t off[kTables];      // count-array offsets in the arena (u32 units)
+};
+
+struct Part {
+    uint64_t lo;
+    int64_t hi;
+    int64_t min_year;
+    int64_t joined;
+};
+
+struct DbK {
+    int64_t n_title, words;
+    const int32_t *year, *kind, *code;
+    int64_t rows[kTables];
+    const int32_t* movie[kTables];
+    const int32_t* a1[kTables];  // role / info_type / keyword
+    const int32_t* a2;           // movie_info.info
+};
+
+BOLAP_DEV bool bit_of(const uint32_t* bits, int64_t n, int64_t v) {
+    if (!bits) return true;
+    if (v < 0 || v >= n) return false;
+    return (bits[v >> 5] >> (v & 31)) & 1u;
+}
+
+// 1. Title bitmaps: wave g covers movies [64 g, 64 g + 64) = words 2g, 2g+1.
+__global__ __launch_bounds__(kT) void ceb_title_k(DbK db, const QK* __restrict__ qs, int32_t nq,
+                                                  uint32_t* __restrict__ tf) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = int64_t(gridDim.x) * (kT / kWave);
+    for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave;
+         g * kWave < db.n_title; g += wstride) {
+        const int64_t m = g * kWave + lane;
+        const bool in = m < db.n_title;
+        const int32_t y = in ? db.year[m] : 0;
+        const int32_t k = in ? db.kind[m] : 0;
+        const int32_t c = in ? db.code[m] : 0;
+        for (int q = 0; q < nq; ++q) {
+            const QK& Q = qs[q];
+            const bool p = in && y >= Q.year_lo && y <= Q.year_hi && uint32_t(k) < 32u &&
+                           ((Q.kind_mask >> k) & 1u) && bit_of(Q.title_bits, Q.title_bits_n, c);
+            const uint64_t b = __ballot(p);
+            if (lane == 0) {
+                uint32_t* row = tf + int64_t(q) * db.words;
+                row[2 * g] = uint32_t(b);
+                if (2 * g + 1 < db.words) row[2 * g + 1] = uint32_t(b >> 32);
+            }
+        }
+    }
+}
+
+// 2. Rank prefix per query (one workgroup per query, contiguous chunks per
+// thread, a workgroup scan of the chunk totals).
+__global__ __launch_bounds__(kRankT) void ceb_rank_k(const uint32_t* __restrict__ tf, int64_t words,
+                                                     uint32_t* __restrict__ rank,
+                                                     QState* __restrict__ st) {
+    __shared__ uint32_t s_w[kRankT / kWave];
+    const int q = blockIdx.x;
+    const uint32_t* row = tf + int64_t(q) * words;
+    uint32_t* rr = rank + int64_t(q) * words;
+    const int64_t per = (words + kRankT - 1) / kRankT;
+    const int64_t b0 = int64_t(threadIdx.x) * per;
+    const int64_t b1 = b0 + per < words ? b0 + per : words;
+    uint32_t mine = 0;
+    for (int64_t i = b0; i < b1; ++i) mine += __popc(row[i]);
+    // workgroup exclusive scan of `mine`
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    uint32_t incl = mine;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += y;
+    }
+    if (lane == kWave - 1) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < kRankT / kWave; ++w) {
+        before += w < wave ? s_w[w] : 0u;
+        total += s_w[w];
+    }
+    uint32_t run = before + incl - mine;
+    for (int64_t i = b0; i < b1; ++i) {
+        rr[i] = run;
+        run += __popc(row[i]);
+    }
+    if (threadIdx.x == 0) st[q].n_tf = total;
+}
+
+// 3. Count-array offsets: exclusive scan over the batch of n_tf x (tables).
+__global__ __launch_bounds__(kRankT) void ceb_layout_k(const QK* __restrict__ qs, int32_t nq,
+                                                       QState* __restrict__ st,
+                                                       unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long s_w[kRankT / kWave];
+    __shared__ unsigned long long s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (int base = 0; base < nq; base += kRankT) {
+        const int q = base + threadIdx.x;
+        unsigned long long need = 0;
+        if (q < nq) need = st[q].n_tf * unsigned(__popc(uint32_t(qs[q].tables) & 7u));
+        unsigned long long incl = need;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned long long y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        if (lane == kWave - 1) s_w[wave] = incl;
+        __syncthreads();
+        unsigned long long before = s_carry, tile = 0;
+        for (int w = 0; w < kRankT / kWave; ++w) {
+            before += w < wave ? s_w[w] : 0ull;
+            tile += s_w[w];
+        }
+        if (q < nq) {
+            unsigned long long off = before + incl - need;
+            for (int t = 0; t < kTables; ++t) {
+                if ((qs[q].tables >> t) & 1) {
+                    st[q].off[t] = off;
+                    off += st[q].n_tf;
+                } else {
+                    st[q].off[t] = ~0ull;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += tile;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = s_carry;
+}
+
+// Zeroes the used part of the arena (its length is on the device).
+__global__ __launch_bounds__(kT) void ceb_zero_k(uint32_t* __restrict__ arena,
+                                                 const unsigned long long* __restrict__ total) {
+    const int64_t n = int64_t(*total);
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        arena[i] = 0u;
+}
+
+template <int TB>
+BOLAP_DEV bool row_pred(const QK& Q, int32_t x, int32_t z) {
+    if (TB == 0) return uint32_t(x) < 32u && ((Q.role_mask >> x) & 1u);
+    if (TB == 1) return bit_of(Q.type_bits, Q.type_bits_n, x) && bit_of(Q.info_bits, Q.info_bits_n, z);
+    return bit_of(Q.kw_bits, Q.kw_bits_n, x);
+}
+
+// 4. One fact table for every query in qlist.  Each wave takes 256
+// consecutive rows as four chunks of 64 (one row per lane per chunk, so each
+// load is one coalesced 256-byte access); within a chunk the movie ids are
+// non-decreasing across lanes, so a segmented inclusive sum keyed on the
+// movie leaves the run's total in its last lane, which adds it to the
+// query's count array at the movie's rank.
+template <int TB>
+__global__ __launch_bounds__(kT) void ceb_fact_k(DbK db, const QK* __restrict__ qs,
+                                                 const int32_t* __restrict__ qlist, int32_t nql,
+                                                 int32_t qchunk, const uint32_t* __restrict__ tf,
+                                                 const uint32_t* __restrict__ rank,
+                                                 const QState* __restrict__ st,
+                                                 uint32_t* __restrict__ arena) {
+    constexpr int U = 4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t rows = db.rows[TB];
+    const int32_t* __restrict__ movie = db.movie[TB];
+    const int32_t* __restrict__ a1 = db.a1[TB];
+    const int32_t* __restrict__ a2 = db.a2;
+    const int64_t chunks = (rows + kWave * U - 1) / (kWave * U);
+    const int64_t wstride = int64_t(gridDim.x) * (kT / kWave);
+    for (int c0 = 0; c0 < nql; c0 += qchunk) {
+        const int c1 = c0 + qchunk < nql ? c0 + qchunk : nql;
+        for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave; g < chunks;
+             g += wstride) {
+            int32_t m[U], x[U], z[U];
+            bool in[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int64_t r = (g * U + j) * kWave + lane;
+                in[j] = r < rows;
+                m[j] = in[j] ? movie[r] - 1 : -1;
+                x[j] = in[j] ? a1[r] : 0;
+                z[j] = (TB == 1 && in[j]) ? a2[r] : 0;
+                if (m[j] < 0 || m[j] >= db.n_title) in[j] = false;
+            }
+            for (int qi = c0; qi < c1; ++qi) {
+                const int q = qlist[qi];
+                const QK& Q = qs[q];
+                const uint32_t* row = tf + int64_t(q) * db.words;
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    uint32_t word = 0;
+                    bool ok = false;
+                    if (in[j]) {
+                        word = row[m[j] >> 5];
+                        ok = ((word >> (m[j] & 31)) & 1u) && row_pred<TB>(Q, x[j], z[j]);
+                    }
+                    if (__ballot(ok) == 0) continue;  // wave-uniform
+                    uint32_t v = ok ? 1u : 0u;
+                    const int32_t key = in[j] ? m[j] : -1;
+#pragma unroll
+                    for (int off = 1; off < kWave; off <<= 1) {
+                        const uint32_t vo = __shfl_up(v, off, kWave);
+                        const int32_t ko = __shfl_up(key, off, kWave);
+                        if (lane >= off && ko == key) v += vo;
+                    }
+                    const int32_t next = __shfl_down(key, 1, kWave);
+                    const bool tail = lane == kWave - 1 || next != key;
+                    if (tail && v > 0 && key >= 0) {
+                        const int64_t w = key >> 5;
+                        const uint32_t r = rank[int64_t(q) * db.words + w] +
+                                           __popc(word & ((1u << (key & 31)) - 1u));
+                        atomicAdd(arena + st[q].off[TB] + r, v);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// 5. Per (query, slice): sum of count products and min year over movies
+// with a match in every table of the query.
+__global__ __launch_bounds__(kT) void ceb_final_k(DbK db, const QK* __restrict__ qs,
+                                                  const uint32_t* __restrict__ tf,
+                                                  const uint32_t* __restrict__ rank,
+                                                  const QState* __restrict__ st,
+                                                  const uint32_t* __restrict__ arena,
+                                                  int32_t slices, Part* __restrict__ parts) {
+    __shared__ uint64_t s_lo[kT / kWave];
+    __shared__ int64_t s_hi[kT / kWave], s_min[kT / kWave], s_j[kT / kWave];
+    const int q = blockIdx.y;
+    const int b = blockIdx.x;
+    const QK& Q = qs[q];
+    const QState S = st[q];
+    const int64_t per = (db.words + slices - 1) / slices;
+    const int64_t w0 = int64_t(b) * per;
+    const int64_t w1 = w0 + per < db.words ? w0 + per : db.words;
+    const uint32_t* row = tf + int64_t(q) * db.words;
+    const uint32_t* rk = rank + int64_t(q) * db.words;
+    Acc128 acc{0, 0};
+    int64_t mn = INT64_MAX, joined = 0;
+    for (int64_t w = w0 + threadIdx.x; w < w1; w += kT) {
+        uint32_t bits = row[w];
+        uint32_t r = rk[w];
+        while (bits) {
+            const int bpos = __ffs(bits) - 1;
+            bits &= bits - 1;
+            uint64_t lo = 1, hi = 0;
+            bool all = true;
+            for (int t = 0; t < kTables; ++t) {
+                if (!((Q.tables >> t) & 1)) continue;
+                const uint32_t c = arena[S.off[t] + r];
+                all = all && c != 0;
+                // (hi, lo) *= c, exact while the product stays below 2^128
+                const uint64_t nlo = lo * uint64_t(c);
+                hi = hi * uint64_t(c) + __umul64hi(lo, uint64_t(c));
+                lo = nlo;
+            }
+            ++r;
+            if (!all) continue;
+            ++joined;
+            const uint64_t nlo = acc.lo + lo;
+            acc.hi += int64_t(hi) + (nlo < acc.lo ? 1 : 0);
+            acc.lo = nlo;
+            const int64_t y = db.year[w * 32 + bpos];
+            mn = y < mn ? y : mn;
+        }
+    }
+    const Acc128 ws = wave_sum128(acc);
+    int64_t wm = mn;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(wm, off, kWave);
+        wm = o < wm ? o : wm;
+    }
+    const int64_t wj = int64_t(wave_sum(uint64_t(joined)));
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_lo[wave] = ws.lo;
+        s_hi[wave] = ws.hi;
+        s_min[wave] = wm;
+        s_j[wave] = wj;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc128 t{0, 0};
+        int64_t m2 = INT64_MAX, j2 = 0;
+        for (int w = 0; w < kT / kWave; ++w) {
+            add128(t, Acc128{s_lo[w], s_hi[w]});
+            m2 = s_min[w] < m2 ? s_min[w] : m2;
+            j2 += s_j[w];
+        }
+        parts[int64_t(q) * slices + b] = Part{t.lo, t.hi, m2, j2};
+    }
+}
+
+__global__ __launch_bounds__(kWave) void ceb_merge_k(const Part* __restrict__ parts, int32_t slices,
+                                                     const QState* __restrict__ st,
+                                                     bolap_ceb_result* __restrict__ out) {
+    const int q = blockIdx.x;
+    Acc128 acc{0, 0};
+    int64_t mn = INT64_MAX, joined = 0;
+    for (int b = threadIdx.x; b < slices; b += kWave) {
+        const Part p = parts[int64_t(q) * slices + b];
+        add128(acc, Acc128{p.lo, p.hi});
+        mn = p.min_year < mn ? p.min_year : mn;
+        joined += p.joined;
+    }
+    const Acc128 t = wave_sum128(acc);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(mn, off, kWave);
+        mn = o < mn ? o : mn;
+    }
+    const int64_t j = int64_t(wave_sum(uint64_t(joined)));
+    if (threadIdx.x == 0)
+        out[q] = bolap_ceb_result{t.lo, uint64_t(t.hi), mn, int64_t(st[q].n_tf), j};
+}
+
+// ------------------------------------------------------------------ host
+struct Layout {
+    int32_t nq = 0, slices = 0;
+    int64_t words = 0;
+    size_t o_qs, o_ql, o_st, o_tf, o_rank, o_total, o_parts, o_res, o_arena, bytes;
+    int64_t arena_words = 0;
+};
+
+int check_db(const bolap_ceb_db* d) {
+    BOLAP_ARG_CHECK(d->title_rows > 0 && d->title_rows < (int64_t(1) << 31),
+                    "bolap_ceb: title_rows %lld", (long long)d->title_rows);
+    BOLAP_ARG_CHECK(d->t_year && d->t_kind && d->t_code, "bolap_ceb: null title column");
+    BOLAP_ARG_CHECK(d->ci_rows >= 0 && d->mi_rows >= 0 && d->mk_rows >= 0,
+                    "bolap_ceb: negative row count");
+    BOLAP_ARG_CHECK(d->ci_rows == 0 || (d->ci_movie && d->ci_role), "bolap_ceb: cast_info");
+    BOLAP_ARG_CHECK(d->mi_rows == 0 || (d->mi_movie && d->mi_type && d->mi_info),
+                    "bolap_ceb: movie_info");
+    BOLAP_ARG_CHECK(d->mk_rows == 0 || (d->mk_movie && d->mk_keyword),
+                    "bolap_ceb: movie_keyword");
+    return BOLAP_OK;
+}
+
+int check_query(const bolap_ceb_query* q, int i) {
+    BOLAP_ARG_CHECK((q->tables & 7) != 0 && (q->tables & ~7) == 0,
+                    "bolap_ceb: query %d: tables 0x%x", i, q->tables);
+    BOLAP_ARG_CHECK(q->agg == BOLAP_CEB_COUNT || q->agg == BOLAP_CEB_MIN_YEAR,
+                    "bolap_ceb: query %d: agg %d", i, q->agg);
+    BOLAP_ARG_CHECK((!q->title_bits || q->title_bits_n >= 0) && (!q->info_bits || q->info_bits_n >= 0) &&
+                        (!q->type_bits || q->type_bits_n >= 0) && (!q->kw_bits || q->kw_bits_n >= 0),
+                    "bolap_ceb: query %d: negative bitmap size", i);
+    return BOLAP_OK;
+}
+
+DbK make_db(const bolap_ceb_db* d) {
+    DbK k;
+    std::memset(&k, 0, sizeof(k));
+    k.n_title = d->title_rows;
+    k.words = (d->title_rows + 31) / 32;
+    k.year = d->t_year;
+    k.kind = d->t_kind;
+    k.code = d->t_code;
+    k.rows[0] = d->ci_rows;
+    k.movie[0] = d->ci_movie;
+    k.a1[0] = d->ci_role;
+    k.rows[1] = d->mi_rows;
+    k.movie[1] = d->mi_movie;
+    k.a1[1] = d->mi_type;
+    k.a2 = d->mi_info;
+    k.rows[2] = d->mk_rows;
+    k.movie[2] = d->mk_movie;
+    k.a1[2] = d->mk_keyword;
+    return k;
+}
+
+QK make_q(const bolap_ceb_query& q) {
+    QK k;
+    k.tables = q.tables;
+    k.agg = q.agg;
+    k.year_lo = q.year_lo;
+    k.year_hi = q.year_hi;
+    k.kind_mask = q.kind_mask;
+    k.role_mask = q.role_mask;
+    k.title_bits = q.title_bits;
+    k.title_bits_n = q.title_bits_n;
+    k.info_bits = q.info_bits;
+    k.info_bits_n = q.info_bits_n;
+    k.type_bits = q.type_bits;
+    k.type_bits_n = q.type_bits_n;
+    k.kw_bits = q.kw_bits;
+    k.kw_bits_n = q.kw_bits_n;
+    return k;
+}
+
+Layout plan_layout(bolap_ctx* ctx, const DbK& db, const bolap_ceb_query* qs, int32_t nq) {
+    Layout L;
+    L.nq = nq;
+    L.words = db.words;
+    L.slices = int32_t(std::max<int64_t>(1, std::min<int64_t>(64, (db.words + 4095) / 4096)));
+    (void)ctx;
+    int64_t arena = 0;
+    for (int i = 0; i < nq; ++i) arena += int64_t(__builtin_popcount(qs[i].tables & 7)) * db.n_title;
+    L.arena_words = arena;
+    WsArena a;
+    L.o_qs = a.plan(size_t(nq) * sizeof(QK));
+    L.o_ql = a.plan(size_t(kTables) * nq * 4);
+    L.o_st = a.plan(size_t(nq) * sizeof(QState));
+    L.o_tf = a.plan(size_t(nq) * db.words * 4);
+    L.o_rank = a.plan(size_t(nq) * db.words * 4);
+    L.o_total = a.plan(64);
+    L.o_parts = a.plan(size_t(nq) * L.slices * sizeof(Part));
+    L.o_res = a.plan(size_t(nq) * sizeof(bolap_ceb_result));
+    L.o_arena = a.plan(size_t(arena) * 4);
+    L.bytes = a.off;
+    return L;
+}
+
+int ceb_reserve(bolap_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->ceb_size) return BOLAP_OK;
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->ceb_exec) {
+        (void)hipGraphExecDestroy(ctx->ceb_exec);
+        ctx->ceb_exec = nullptr;
+    }
+    if (ctx->ceb_graph) {
+        (void)hipGraphDestroy(ctx->ceb_graph);
+        ctx->ceb_graph = nullptr;
+    }
+    ctx->ceb_key.clear();
+    if (ctx->ceb_buf) BOLAP_HIP_TRY(hipFree(ctx->ceb_buf));
+    ctx->ceb_buf = nullptr;
+    ctx->ceb_size = 0;
+    const size_t want = bytes + bytes / 8;
+    BOLAP_HIP_TRY(hipMalloc(&ctx->ceb_buf, want));
+    ctx->ceb_size = want;
+    return BOLAP_OK;
+}
+
+unsigned grid_for(bolap_ctx* ctx, int64_t n, int per_cu) {
+    int64_t b = (n + kT - 1) / kT;
+    const int64_t cap = int64_t(ctx->num_cus) * per_cu;
+    if (b > cap) b = cap;
+    return unsigned(b < 1 ? 1 : b);
+}
+
+// Enqueues every stage of a planned batch on the context stream.
+int enqueue(bolap_ctx* ctx, const DbK& db, const Layout& L, const int32_t* n_per_table) {
+    char* base = static_cast<char*>(ctx->ceb_buf);
+    auto* qs = reinterpret_cast<const QK*>(base + L.o_qs);
+    auto* ql = reinterpret_cast<const int32_t*>(base + L.o_ql);
+    auto* st = reinterpret_cast<QState*>(base + L.o_st);
+    auto* tf = reinterpret_cast<uint32_t*>(base + L.o_tf);
+    auto* rank = reinterpret_cast<uint32_t*>(base + L.o_rank);
+    auto* total = reinterpret_cast<unsigned long long*>(base + L.o_total);
+    auto* parts = reinterpret_cast<Part*>(base + L.o_parts);
+    auto* res = reinterpret_cast<bolap_ceb_result*>(base + L.o_res);
+    auto* arena = reinterpret_cast<uint32_t*>(base + L.o_arena);
+    const int per_cu = int(ctx->knob("ceb.blocks_per_cu", 4));
+    const int qchunk = int(std::max<int64_t>(1, ctx->knob("ceb.qchunk", L.nq)));
+    hipLaunchKernelGGL(ceb_title_k, dim3(grid_for(ctx, (db.n_title + 63) / 64 * 64, 8)),
+                       dim3(kT), 0, ctx->stream, db, qs, L.nq, tf);
+    hipLaunchKernelGGL(ceb_rank_k, dim3(unsigned(L.nq)), dim3(kRankT), 0, ctx->stream, tf,
+                       L.words, rank, st);
+    hipLaunchKernelGGL(ceb_layout_k, dim3(1), dim3(kRankT), 0, ctx->stream, qs, L.nq, st, total);
+    hipLaunchKernelGGL(ceb_zero_k, dim3(unsigned(ctx->num_cus * 8)), dim3(kT), 0, ctx->stream,
+                       arena, total);
+    BOLAP_TRY(launch_check("ceb title/rank/layout"));
+    for (int t = 0; t < kTables; ++t) {
+        const int32_t nql = n_per_table[t];
+        if (nql == 0 || db.rows[t] == 0) continue;
+        const unsigned grid = grid_for(ctx, (db.rows[t] + 3) / 4, per_cu);
+        const int32_t* list = ql + t * L.nq;
+        if (t == 0)
+            hipLaunchKernelGGL(ceb_fact_k<0>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
+                               nql, qchunk, tf, rank, st, arena);
+        else if (t == 1)
+            hipLaunchKernelGGL(ceb_fact_k<1>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
+                               nql, qchunk, tf, rank, st, arena);
+        else
+            hipLaunchKernelGGL(ceb_fact_k<2>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
+                               nql, qchunk, tf, rank, st, arena);
+        BOLAP_TRY(launch_check("ceb_fact_k"));
+    }
+    hipLaunchKernelGGL(ceb_final_k, dim3(unsigned(L.slices), unsigned(L.nq)), dim3(kT), 0,
+                       ctx->stream, db, qs, tf, rank, st, arena, L.slices, parts);
+    hipLaunchKernelGGL(ceb_merge_k, dim3(unsigned(L.nq)), dim3(kWave), 0, ctx->stream, parts,
+                       L.slices, st, res);
+    return launch_check("ceb final");
+}
+
+// Uploads the queries and their per-table lists; returns the list lengths.
+int upload(bolap_ctx* ctx, const Layout& L, const bolap_ceb_query* qs, int32_t* n_per_table) {
+    std::vector<QK> hq(static_cast<size_t>(L.nq));
+    std::vector<int32_t> lists(static_cast<size_t>(kTables) * L.nq, 0);
+    for (int t = 0; t < kTables; ++t) n_per_table[t] = 0;
+    for (int i = 0; i < L.nq; ++i) {
+        hq[size_t(i)] = make_q(qs[i]);
+        for (int t = 0; t < kTables; ++t)
+            if ((qs[i].tables >> t) & 1) lists[size_t(t) * L.nq + n_per_table[t]++] = i;
+    }
+    char* base = static_cast<char*>(ctx->ceb_buf);
+    const size_t qb = hq.size() * sizeof(QK), lb = lists.size() * 4;
+    BOLAP_TRY(pinned_reserve(ctx, qb + lb));
+    std::memcpy(ctx->pinned, hq.data(), qb);
+    std::memcpy(static_cast<char*>(ctx->pinned) + qb, lists.data(), lb);
+    BOLAP_HIP_TRY(hipMemcpyAsync(base + L.o_qs, ctx->pinned, qb, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    BOLAP_HIP_TRY(hipMemcpyAsync(base + L.o_ql, static_cast<char*>(ctx->pinned) + qb, lb,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    return BOLAP_OK;
+}
+
+int read_results(bolap_ctx* ctx, const Layout& L, bolap_ceb_result* out) {
+    const size_t rb = size_t(L.nq) * sizeof(bolap_ceb_result);
+    BOLAP_TRY(pinned_reserve(ctx, rb));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, static_cast<char*>(ctx->ceb_buf) + L.o_res, rb,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->pinned, rb);
+    return BOLAP_OK;
+}
+
+std::string graph_key(bolap_ctx* ctx, const bolap_ceb_db* d, const Layout& L,
+                      const int32_t* npt) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%p|%lld|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%lld|"
+                  "%d|%d|%d|%d|%lld|%lld",
+                  ctx->ceb_buf, (long long)d->title_rows, (const void*)d->t_year,
+                  (const void*)d->t_kind, (const void*)d->t_code, (const void*)d->ci_movie,
+                  (const void*)d->ci_role, (const void*)d->mi_movie, (const void*)d->mi_type,
+                  (const void*)d->mi_info, (const void*)d->mk_movie,
+                  (const void*)d->mk_keyword, (const void*)nullptr, (long long)d->ci_rows,
+                  (long long)d->mi_rows, (long long)d->mk_rows, L.nq, npt[0], npt[1], npt[2],
+                  (long long)ctx->knob("ceb.blocks_per_cu", 4),
+                  (long long)ctx->knob("ceb.qchunk", L.nq));
+    return buf;
+}
+
+int run_batch(bolap_ctx* ctx, const bolap_ceb_db* d, const bolap_ceb_query* qs, int32_t n,
+              bolap_ceb_result* out, bool use_graph) {
+    BOLAP_TRY(check_db(d));
+    BOLAP_ARG_CHECK(n >= 1 && n <= kMaxBatch, "bolap_ceb: batch of %d queries (1..%d)", n,
+                    kMaxBatch);
+    for (int i = 0; i < n; ++i) BOLAP_TRY(check_query(&qs[i], i));
+    const DbK db = make_db(d);
+    const Layout L = plan_layout(ctx, db, qs, n);
+    BOLAP_TRY(ceb_reserve(ctx, L.bytes));
+    int32_t npt[kTables];
+    BOLAP_TRY(upload(ctx, L, qs, npt));
+    if (!use_graph) {
+        BOLAP_TRY(enqueue(ctx, db, L, npt));
+        return read_results(ctx, L, out);
+    }
+    const std::string key = graph_key(ctx, d, L, npt);
+    if (key != ctx->ceb_key || !ctx->ceb_exec) {
+        if (ctx->ceb_exec) (void)hipGraphExecDestroy(ctx->ceb_exec);
+        if (ctx->ceb_graph) (void)hipGraphDestroy(ctx->ceb_graph);
+        ctx->ceb_exec = nullptr;
+        ctx->ceb_graph = nullptr;
+        ctx->ceb_key.clear();
+        // Record on a private stream (the context stream may be the null
+        // stream, which cannot capture); the graph then runs on ctx->stream.
+        if (!ctx->capture_stream)
+            BOLAP_HIP_TRY(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        hipStream_t run_stream = ctx->stream;
+        BOLAP_HIP_TRY(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+        ctx->stream = ctx->capture_stream;
+        const int s = enqueue(ctx, db, L, npt);
+        ctx->stream = run_stream;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->capture_stream, &g);
+        BOLAP_TRY(s);
+        BOLAP_HIP_TRY(e);
+        ctx->ceb_graph = g;
+        BOLAP_HIP_TRY(hipGraphInstantiate(&ctx->ceb_exec, g, nullptr, nullptr, 0));
+        ctx->ceb_key = key;
+    }
+    BOLAP_HIP_TRY(hipGraphLaunch(ctx->ceb_exec, ctx->stream));
+    return read_results(ctx, L, out);
+}
+
+}  // namespace
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" {
+
+int bolap_ceb_run(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* q,
+                  bolap_ceb_result* out) {
+    BOLAP_ARG_CHECK(ctx && db && q && out, "bolap_ceb_run: null argument");
+    TraceScope ts(ctx, "ceb_query");
+    return run_batch(ctx, db, q, 1, out, false);
+}
+
+int bolap_ceb_run_batch(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* qs,
+                        int32_t n, bolap_ceb_result* out) {
+    BOLAP_ARG_CHECK(ctx && db && qs && out, "bolap_ceb_run_batch: null argument");
+    TraceScope ts(ctx, "ceb_batch");
+    return run_batch(ctx, db, qs, n, out, ctx->knob("ceb.graph", 1) == 1);
+}
+
+int bolap_ceb_autotune(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* qs,
+                       int32_t n, double* best_ms) {
+    BOLAP_ARG_CHECK(ctx && db && qs && n >= 1, "bolap_ceb_autotune: bad arguments");
+    std::vector<bolap_ceb_result> out(static_cast<size_t>(n));
+    const int64_t per_cu[] = {2, 4, 8, 16};
+    std::vector<int64_t> chunks = {16, 64, int64_t(n)};
+    double best = 1e30;
+    int64_t bp = 4, bc = n;
+    hipEvent_t e0, e1;
+    BOLAP_HIP_TRY(hipEventCreate(&e0));
+    BOLAP_HIP_TRY(hipEventCreate(&e1));
+    int status = BOLAP_OK;
+    for (int64_t p : per_cu) {
+        for (int64_t c : chunks) {
+            if (c > n && c != int64_t(n)) continue;
+            ctx->knobs["ceb.blocks_per_cu"] = p;
+            ctx->knobs["ceb.qchunk"] = c;
+            status = run_batch(ctx, db, qs, n, out.data(), true);  // capture + warm
+            if (status != BOLAP_OK) break;
+            float ms = 0.f;
+            (void)hipEventRecord(e0, ctx->stream);
+            status = run_batch(ctx, db, qs, n, out.data(), true);
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (status != BOLAP_OK) break;
+            if (ms < best) {
+                best = ms;
+                bp = p;
+                bc = c;
+            }
+        }
+        if (status != BOLAP_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->knobs["ceb.blocks_per_cu"] = bp;
+    ctx->knobs["ceb.qchunk"] = bc;
+    if (best_ms) *best_ms = best;
+    return status;
+}
+
+}  // extern "C"

@@ -54,6 +54,16 @@ struct bolap_ctx {
         return (it == knobs.end() || it->second == 0) ? dflt : it->second;
     }
 
+    // Batched CEB workload: its own device buffer (per-query bitmaps and
+    // count arrays can reach gigabytes, sized exactly per batch) and the
+    // instantiated hipGraph of the last batch shape.
+    void* ceb_buf = nullptr;
+    size_t ceb_size = 0;
+    hipGraph_t ceb_graph = nullptr;
+    hipGraphExec_t ceb_exec = nullptr;
+    std::string ceb_key;
+    hipStream_t capture_stream = nullptr;  // capture cannot run on the null stream
+
     // Stage tracing (bolap_ctx_set_trace).
     bool trace = false;
     std::vector<bolap_trace_rec> trace_recs;

@@ -144,6 +144,10 @@ int bolap_ctx_destroy(bolap_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->ceb_exec) (void)hipGraphExecDestroy(ctx->ceb_exec);
+    if (ctx->ceb_graph) (void)hipGraphDestroy(ctx->ceb_graph);
+    if (ctx->ceb_buf) (void)hipFree(ctx->ceb_buf);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->result_dev) (void)hipFree(ctx->result_dev);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

@@ -73,6 +73,31 @@ class StarStats(C.Structure):
                 ("group_capacity", C.c_int64), ("revenue", C.c_int64),
                 ("fallbacks", C.c_int32), ("count_passes", C.c_int32)]
 
+
+CEB_CI, CEB_MI, CEB_MK = 1, 2, 4
+CEB_COUNT, CEB_MIN_YEAR = 0, 1
+
+
+class CebDb(C.Structure):
+    _fields_ = [("title_rows", C.c_int64), ("t_year", vp), ("t_kind", vp), ("t_code", vp),
+                ("ci_rows", C.c_int64), ("ci_movie", vp), ("ci_role", vp),
+                ("mi_rows", C.c_int64), ("mi_movie", vp), ("mi_type", vp), ("mi_info", vp),
+                ("mk_rows", C.c_int64), ("mk_movie", vp), ("mk_keyword", vp)]
+
+
+class CebQuery(C.Structure):
+    _fields_ = [("tables", C.c_int32), ("agg", C.c_int32), ("year_lo", C.c_int32),
+                ("year_hi", C.c_int32), ("kind_mask", C.c_uint32), ("role_mask", C.c_uint32),
+                ("title_bits", vp), ("title_bits_n", C.c_int64),
+                ("info_bits", vp), ("info_bits_n", C.c_int64),
+                ("type_bits", vp), ("type_bits_n", C.c_int64),
+                ("kw_bits", vp), ("kw_bits_n", C.c_int64)]
+
+
+class CebResult(C.Structure):
+    _fields_ = [("count_lo", C.c_uint64), ("count_hi", C.c_uint64), ("min_year", C.c_int64),
+                ("title_pass", C.c_int64), ("joined_movies", C.c_int64)]
+
 class Q1Args(C.Structure):
     _fields_ = [("layout", C.c_int32), ("rows", C.c_int64), ("ship", vp),
                 ("ship_lo", C.c_int64), ("ship_hi", C.c_int64),
@@ -157,6 +182,10 @@ PROTOTYPES = {
     "bolap_gen_uniform_u8": (C.c_int, [vp, u64, C.c_uint32, i64, C.c_int32, C.c_int32, vp]),
     "bolap_gen_zipf_i32": (C.c_int, [vp, u64, C.c_uint32, i64, vp, i64, u64, vp]),
     "bolap_gen_price_disc": (C.c_int, [vp, u64, C.c_uint32, i64, vp, vp]),
+    "bolap_ceb_run": (C.c_int, [vp, P(CebDb), P(CebQuery), P(CebResult)]),
+    "bolap_ceb_run_batch": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(CebResult)]),
+    "bolap_ceb_autotune": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(C.c_double)]),
+    "bolap_ilike_mask": (C.c_int, [C.c_char_p, P(C.c_int64), i64, C.c_char_p, P(C.c_uint8)]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),
     "bolap_mlp_destroy": (C.c_int, [vp]),
     "bolap_mlp_num_params": (C.c_int, []),

@@ -322,6 +322,85 @@ int bolap_star_groupby(bolap_ctx* ctx, const bolap_star_args* args, int32_t* key
                        bolap_star_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* CEB-style star queries over movie ids (batched parameterised workload).   */
+/* Restates the factorised COUNT of bespoke_ceb/query_q1a.cpp:69-152          */
+/* (dimension masks), :241-596 (per-movie count arrays) and :602-621          */
+/* (COUNT(*) = sum over qualifying movies of the product of the per-table     */
+/* counts), with MIN(production_year) as in query_q10a.cpp:334 ff.  Fact      */
+/* tables must be sorted by movie_id (the reference builder's order,          */
+/* bespoke_ceb/builder_impl.cpp:432-452); title ids are 1..title_rows.        */
+/* String predicates arrive as bitmaps over dictionary codes, computed on     */
+/* the host (bolap_ilike_mask).                                               */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_CEB_CI 1             /* cast_info takes part */
+#define BOLAP_CEB_MI 2             /* movie_info (and info_type) */
+#define BOLAP_CEB_MK 4             /* movie_keyword (and keyword) */
+#define BOLAP_CEB_COUNT 0
+#define BOLAP_CEB_MIN_YEAR 1
+
+typedef struct {
+    int64_t title_rows;
+    const int32_t* t_year;       /* title.production_year */
+    const int32_t* t_kind;       /* title.kind_id */
+    const int32_t* t_code;       /* title.title dictionary code */
+    int64_t ci_rows;
+    const int32_t* ci_movie;     /* cast_info.movie_id (sorted) */
+    const int32_t* ci_role;      /* cast_info.role_id */
+    int64_t mi_rows;
+    const int32_t* mi_movie;     /* movie_info.movie_id (sorted) */
+    const int32_t* mi_type;      /* movie_info.info_type_id */
+    const int32_t* mi_info;      /* movie_info.info dictionary code */
+    int64_t mk_rows;
+    const int32_t* mk_movie;     /* movie_keyword.movie_id (sorted) */
+    const int32_t* mk_keyword;   /* movie_keyword.keyword_id */
+} bolap_ceb_db;
+
+typedef struct {
+    int32_t tables;              /* BOLAP_CEB_CI | BOLAP_CEB_MI | BOLAP_CEB_MK, at least one */
+    int32_t agg;                 /* BOLAP_CEB_COUNT or BOLAP_CEB_MIN_YEAR */
+    int32_t year_lo, year_hi;    /* title.production_year in [lo, hi] */
+    uint32_t kind_mask;          /* bit k: title.kind_id == k passes (k < 32) */
+    uint32_t role_mask;          /* bit r: cast_info.role_id == r passes (r < 32) */
+    const uint32_t* title_bits;  /* device bitmap over title codes, or null (no predicate) */
+    int64_t title_bits_n;
+    const uint32_t* info_bits;   /* device bitmap over movie_info.info codes, or null */
+    int64_t info_bits_n;
+    const uint32_t* type_bits;   /* device bitmap over info_type ids, or null */
+    int64_t type_bits_n;
+    const uint32_t* kw_bits;     /* device bitmap over keyword ids, or null */
+    int64_t kw_bits_n;
+} bolap_ceb_query;
+
+typedef struct {
+    uint64_t count_lo;           /* COUNT(*) as an unsigned 128-bit integer */
+    uint64_t count_hi;
+    int64_t min_year;            /* MIN(production_year); INT64_MAX when no row joins */
+    int64_t title_pass;          /* movies passing the title predicates */
+    int64_t joined_movies;       /* of those, movies with a match in every table */
+} bolap_ceb_result;
+
+/* One query, executed alone (the plain path). */
+int bolap_ceb_run(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* q,
+                  bolap_ceb_result* out);
+/* A batch of queries through persistent stage kernels that share each table
+   scan among the queries reading it, replayed from a captured hipGraph.
+   out receives n results. */
+int bolap_ceb_run_batch(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* qs,
+                        int32_t n, bolap_ceb_result* out);
+/* Times the batch under each candidate stage configuration and keeps the
+   fastest in the context's knobs (ceb.blocks_per_cu, ceb.qchunk); best_ms
+   (optional) receives its batch time. */
+int bolap_ceb_autotune(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* qs,
+                       int32_t n, double* best_ms);
+
+/* Host-side ILIKE over a string dictionary: mask[i] = 1 when string i
+   (chars[offsets[i] .. offsets[i+1])) matches pattern, '%' matching any
+   run of characters and '_' any one character, ASCII case-insensitive
+   (bespoke_ceb/query_q10a.cpp:43-209). */
+int bolap_ilike_mask(const char* chars, const int64_t* offsets, int64_t n,
+                     const char* pattern, uint8_t* mask);
+
+/* ------------------------------------------------------------------------ */
 /* Table statistics for the estimator (computed once per column, at load).   */
 /* ------------------------------------------------------------------------ */
 #define BOLAP_HLL_REGS 4096        /* HyperLogLog, p = 12 */

@@ -367,3 +367,77 @@ def star_sample_counts(t: dict, plan: dict, seed: int) -> np.ndarray:
     out = np.zeros(16, np.int64)
     _slib().oracle_star_sample_counts(len(plan["dims"]), arr, len(gk), _p(gk), seed, _p(out))
     return out
+
+
+# ------------------------------------------------------- CEB queries (ceb.c)
+class CebDb(C.Structure):
+    _fields_ = [("title_rows", i64), ("t_year", vp), ("t_kind", vp), ("t_code", vp),
+                ("ci_rows", i64), ("ci_movie", vp), ("ci_role", vp),
+                ("mi_rows", i64), ("mi_movie", vp), ("mi_type", vp), ("mi_info", vp),
+                ("mk_rows", i64), ("mk_movie", vp), ("mk_keyword", vp)]
+
+
+class CebQuery(C.Structure):
+    _fields_ = [("tables", i32), ("year_lo", i32), ("year_hi", i32), ("kind_mask", C.c_uint32),
+                ("role_mask", C.c_uint32), ("title_mask", vp), ("title_n", i64),
+                ("info_mask", vp), ("info_n", i64), ("type_mask", vp), ("type_n", i64),
+                ("kw_mask", vp), ("kw_n", i64)]
+
+
+def gen_ceb(seed: int, scale: float) -> dict:
+    """Host copy of the device CEB tables (same generators and streams; the
+    fact tables' movie ids sorted ascending)."""
+    from bespokeolap_artifacts_amd import ceb as CB
+    from bespokeolap_artifacts_amd import workload as W
+    n = {k: max(1, int(v * scale)) if k != "info_type" else v for k, v in CB.SIZES.items()}
+    nt = n["title"]
+
+    def zipf_sorted(rows, stream, keys):
+        return np.sort(zipf_i32(seed, stream, rows, W.zipf_quantiles(keys, CB.ZIPF_S), keys,
+                                W.coprime_mul(keys)))
+
+    t = {"n": n,
+         "title": {"year": uniform_i32(seed, CB.S_T_YEAR, nt, CB.YEAR_LO, CB.YEAR_HI),
+                   "kind": uniform_i32(seed, CB.S_T_KIND, nt, 1, 7),
+                   "code": uniform_i32(seed, CB.S_T_CODE, nt, 0, CB.DICT_SIZES["title"] - 1)},
+         "ci": {"movie": zipf_sorted(n["cast_info"], CB.S_CI_MOVIE, nt),
+                "role": uniform_i32(seed, CB.S_CI_ROLE, n["cast_info"], 1, 12)},
+         "mi": {"movie": zipf_sorted(n["movie_info"], CB.S_MI_MOVIE, nt),
+                "type": uniform_i32(seed, CB.S_MI_TYPE, n["movie_info"], 1, n["info_type"]),
+                "info": uniform_i32(seed, CB.S_MI_INFO, n["movie_info"], 0,
+                                    CB.DICT_SIZES["info"] - 1)},
+         "mk": {"movie": zipf_sorted(n["movie_keyword"], CB.S_MK_MOVIE, nt),
+                "keyword": zipf_i32(seed, CB.S_MK_KW, n["movie_keyword"],
+                                    W.zipf_quantiles(n["keyword"], CB.ZIPF_S), n["keyword"],
+                                    W.coprime_mul(n["keyword"]))}}
+    return t
+
+
+def ceb_run(t: dict, q: dict, masks: dict) -> dict:
+    """One query over host tables.  masks: optional uint8 arrays 'title' (over
+    title codes), 'info' (over info codes), 'info_type' and 'keyword' (over
+    ids, index = id)."""
+    L = lib()
+    if not getattr(L, "_ceb_ready", False):
+        L.oracle_ceb_run.argtypes = [C.POINTER(CebDb), C.POINTER(CebQuery), vp]
+        L._ceb_ready = True
+    db = CebDb(len(t["title"]["year"]), _p(t["title"]["year"]), _p(t["title"]["kind"]),
+               _p(t["title"]["code"]), len(t["ci"]["movie"]), _p(t["ci"]["movie"]),
+               _p(t["ci"]["role"]), len(t["mi"]["movie"]), _p(t["mi"]["movie"]),
+               _p(t["mi"]["type"]), _p(t["mi"]["info"]), len(t["mk"]["movie"]),
+               _p(t["mk"]["movie"]), _p(t["mk"]["keyword"]))
+    cq = CebQuery(q["tables"], q["year_lo"], q["year_hi"], q["kind_mask"], q["role_mask"])
+    keep = []
+    for field, key in (("title", "title"), ("info", "info"), ("type", "info_type"),
+                       ("kw", "keyword")):
+        m = masks.get(key)
+        if m is None:
+            continue
+        m = np.ascontiguousarray(m, np.uint8)
+        keep.append(m)
+        setattr(cq, f"{field}_mask", _p(m))
+        setattr(cq, f"{field}_n", len(m))
+    out = np.zeros(5, np.uint64)
+    L.oracle_ceb_run(C.byref(db), C.byref(cq), _p(out))
+    return {"count": (int(out[1]) << 64) | int(out[0]), "min_year": int(out[2].view(np.int64)),
+            "title_pass": int(out[3]), "joined_movies": int(out[4])}

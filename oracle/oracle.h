@@ -136,6 +136,30 @@ int64_t oracle_star_groupby(int32_t n_dims, const oracle_star_dim* dims, int64_t
 void oracle_star_sample_counts(int32_t n_dims, const oracle_star_dim* dims, int64_t fact_rows,
                                const int32_t* group_key, uint64_t seed, int64_t* counts);
 
+/* --- CEB star queries (ceb.c) --------------------------------------------- */
+typedef struct {
+    int64_t title_rows;
+    const int32_t *t_year, *t_kind, *t_code;
+    int64_t ci_rows;
+    const int32_t *ci_movie, *ci_role;
+    int64_t mi_rows;
+    const int32_t *mi_movie, *mi_type, *mi_info;
+    int64_t mk_rows;
+    const int32_t *mk_movie, *mk_keyword;
+} oracle_ceb_db;
+
+typedef struct {
+    int32_t tables, year_lo, year_hi;
+    uint32_t kind_mask, role_mask;
+    const uint8_t* title_mask; int64_t title_n;  /* masks over codes / ids, or null */
+    const uint8_t* info_mask; int64_t info_n;
+    const uint8_t* type_mask; int64_t type_n;
+    const uint8_t* kw_mask; int64_t kw_n;
+} oracle_ceb_query;
+
+/* out: count_lo, count_hi, min_year, title_pass, joined_movies */
+void oracle_ceb_run(const oracle_ceb_db* db, const oracle_ceb_query* q, uint64_t* out);
+
 /* --- planted mistakes, for tests that show the checks have teeth -------- */
 void oracle_set_mutation(int32_t m);
 int32_t oracle_mutation(void);

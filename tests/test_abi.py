@@ -53,6 +53,7 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "ChainArgs": "bolap_chain_args", "GroupRow": "bolap_group_row",
     "ChainStats": "bolap_chain_stats", "GenLineitemCols": "bolap_gen_lineitem_cols",
     "StarDim": "bolap_star_dim", "StarArgs": "bolap_star_args", "StarStats": "bolap_star_stats",
+    "CebDb": "bolap_ceb_db", "CebQuery": "bolap_ceb_query", "CebResult": "bolap_ceb_result",
 }
 
 

@@ -1,0 +1,100 @@
+"""CEB star queries on the GPU: one at a time and batched (hipGraph replay,
+autotuned), bit-exact against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bespokeolap_artifacts_amd import ceb as CB
+from bespokeolap_artifacts_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(engine):
+    return CB.CebSchema(engine, seed=3, scale=0.01), O.gen_ceb(3, 0.01)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def oracle_masks(schema, q):
+    m = {}
+    for key in ("title", "info", "info_type", "keyword"):
+        if key in q:
+            mask = schema.bits(key, q[key])[2]
+            if key in ("info_type", "keyword"):  # over ids: id = code + 1
+                mask = np.concatenate([[0], mask]).astype(np.uint8)
+            m[key] = mask
+    return m
+
+
+def test_tables_match_oracle(small):
+    s, t = small
+    for c in ("year", "kind", "code"):
+        np.testing.assert_array_equal(host(s.title[c]), t["title"][c])
+    for name, cols in (("ci", ("movie", "role")), ("mi", ("movie", "type", "info")),
+                       ("mk", ("movie", "keyword"))):
+        for c in cols:
+            np.testing.assert_array_equal(host(getattr(s, name)[c]), t[name][c])
+
+
+def test_single_queries_match_oracle(engine, small):
+    s, t = small
+    qs = CB.random_queries(s, 48, seed=5)
+    for q in qs:
+        got = CB.run_one(engine, s, CB.bind(s, q))
+        assert got == O.ceb_run(t, q, oracle_masks(s, q)), q
+
+
+def test_batch_equals_single_and_replays(engine, small):
+    s, t = small
+    qs = CB.random_queries(s, 64, seed=6)
+    bound = [CB.bind(s, q) for q in qs]
+    single = [CB.run_one(engine, s, b) for b in bound]
+    for rep in range(3):  # graph replays, with eager work in between
+        assert CB.run_batch(engine, s, bound) == single
+        CB.run_one(engine, s, bound[rep])
+    engine.tune_set("ceb.graph", 2)
+    try:
+        assert CB.run_batch(engine, s, bound) == single
+    finally:
+        engine.tune_set("ceb.graph", 0)
+
+
+def test_autotune_keeps_results(engine, small):
+    s, t = small
+    qs = [CB.bind(s, q) for q in CB.random_queries(s, 40, seed=7)]
+    before = CB.run_batch(engine, s, qs)
+    ms = CB.autotune(engine, s, qs)
+    assert ms > 0
+    assert engine.tune_get("ceb.blocks_per_cu") in (2, 4, 8, 16)
+    assert CB.run_batch(engine, s, qs) == before
+
+
+def test_empty_and_min_edge_cases(engine, small):
+    s, t = small
+    q = {"tables": N.CEB_CI, "agg": N.CEB_MIN_YEAR, "year_lo": 2030, "year_hi": 2040,
+         "kind_mask": 0xFE, "role_mask": 0x1FFE}
+    r = CB.run_one(engine, s, CB.bind(s, q))
+    assert r == {"count": 0, "min_year": 2**63 - 1, "title_pass": 0, "joined_movies": 0}
+    # a role mask that admits nothing: movies pass the title but join nothing
+    q2 = dict(q, year_lo=1880, year_hi=2025, role_mask=1 << 20)
+    r2 = CB.run_one(engine, s, CB.bind(s, q2))
+    assert r2["title_pass"] > 0 and r2["count"] == 0 and r2["joined_movies"] == 0
+    assert r2 == O.ceb_run(t, q2, {})
+    with pytest.raises(N.BolapError):
+        CB.run_one(engine, s, CB.bind(s, dict(q, tables=0)))
+
+
+def test_full_size_batch_matches_single_and_oracle(engine):
+    s = CB.CebSchema(engine, seed=42, scale=1.0)
+    qs = CB.random_queries(s, 512, seed=42)
+    bound = [CB.bind(s, q) for q in qs]
+    batch = CB.run_batch(engine, s, bound)
+    for i in range(0, 512, 8):
+        assert CB.run_one(engine, s, bound[i]) == batch[i]
+    t = O.gen_ceb(42, 1.0)
+    for i in (0, 1, 2, 3, 5, 8, 13, 21):
+        assert batch[i] == O.ceb_run(t, qs[i], oracle_masks(s, qs[i])), qs[i]
