@@ -64,6 +64,10 @@ struct bolap_ctx {
     std::string ceb_key;
     hipStream_t capture_stream = nullptr;  // capture cannot run on the null stream
 
+    // Device-visible address of `pinned` (kernels write small results there
+    // directly, with system-scope stores).
+    void* pinned_dev = nullptr;
+
     // Stage tracing (bolap_ctx_set_trace).
     bool trace = false;
     std::vector<bolap_trace_rec> trace_recs;

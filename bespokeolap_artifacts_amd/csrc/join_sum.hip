@@ -6,22 +6,31 @@
 // build side, every fact row probes it, and matching rows add
 // price * (100 - disc) into one exact int64.
 //
-// Kernels (all on the context stream):
-//   count  (exact path only) predicate pass over the dimension, one atomic
-//          per wave; the host reads the count to size the table.
-//   init   fills the table image with EMPTY, clears bitmap and result slot.
-//   build  predicate pass + open-addressing insert (atomicCAS, linear
-//          probing) of surviving keys into an HBM table image, plus a
-//          blocked Bloom filter (2 bits in one 32-bit word per key); an
-//          overflow flag is raised when the
-//          distinct keys exceed half the slots (estimate-sized tables).
-//   probe  streams the fact key column with 128-bit loads; for the 16 keys
-//          a lane holds it first issues all 16 Bloom-word reads, forms a
-//          hit mask, and probes the table only for Bloom hits; filter and
-//          table are staged in LDS when they fit (one copy per workgroup)
-//          and read through L2 otherwise.  Matches
-//          gather price/disc and accumulate per lane; DPP wave sums, LDS
-//          and per-block partials, summed by a one-workgroup merge kernel.
+// Kernels (all on the context stream; nothing returns to the host before
+// the result):
+//   compact  predicate pass over the dimension (R rows per lane, one vector
+//            load per predicate column), surviving keys compacted with wave
+//            ballots into a key list of 64 segments; workgroup b appends to
+//            segment b % 64 with one atomic per wave that has a survivor, so
+//            no counter sees more than a 64th of the appends.  The sum of
+//            the segment lengths is the exact build size.
+//   hbm_init / hbm_build  only when the build side is too large for LDS:
+//            an open-addressing table (linear probing, load <= 1/2) and a
+//            blocked Bloom filter (two bits in one 32-bit word per key) in
+//            HBM, sized on the device from the list length.  Both kernels
+//            exit at once when the build side goes to LDS.
+//   probe    every workgroup reads the list length, sizes its table from it
+//            and, when it fits, builds its own copy in LDS (load <= 1/4,
+//            128 filter bits per key) from the L2-resident key list; then it
+//            streams the fact key column with 128-bit loads.  For the keys
+//            a lane holds it first issues all Bloom-word reads, forms a hit
+//            mask, and probes the table only for Bloom hits.  Matches gather
+//            price/disc and accumulate per lane; DPP wave sums, LDS and
+//            per-block partials, summed by a one-workgroup merge kernel
+//            that also writes the result row into pinned host memory.
+// An estimate (capacity_hint) fixes the table geometry at launch instead;
+// a build side larger than half its slots raises the overflow flag and the
+// host reruns with device sizing.
 #include <cstring>
 
 #include "preds.hpp"
@@ -31,29 +40,82 @@ namespace {
 
 constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / kWave;
+constexpr int kCompactThreads = 256;
 constexpr int32_t kEmpty = INT32_MIN;
 constexpr uint32_t kSlotMul = 0x9E3779B1u;   // Fibonacci hashing
 constexpr uint32_t kBitMul = 0x85EBCA77u;    // independent multiplier for the bitmap
 constexpr int kLdsBudget = 96 * 1024;
+// Device-sized LDS tables: up to 2048 keys in 64 KB (placement AUTO, two
+// workgroups per CU), up to 4096 keys in 96 KB when LDS is forced.
+constexpr int64_t kAutoLdsKeys = 2048;
+constexpr int64_t kForcedLdsKeys = 4096;
 
-// Result slot layout (int64): revenue, matches, build rows, distinct keys,
-// flags (bit 0 overflow, bit 1 INT32_MIN key present), counted rows.
-enum { kRev = 0, kMatch = 1, kBuild = 2, kDistinct = 3, kFlags = 4, kCount = 5, kResN = 8 };
+// Result slot layout (int64): revenue, matches, build rows, flags
+// (bit 1: an INT32_MIN key passed the filter).
+enum { kRev = 0, kMatch = 1, kBuild = 2, kFlags = 4, kResN = 8 };
+
+// Segmented key list: segment r holds keys[r * seg_cap .. + cnt[r * kSegStride]).
+constexpr int kSegs = 64;
+constexpr int kSegStride = 16;  // counters 64 bytes apart
+struct KList {
+    int32_t* keys;
+    uint32_t* cnt;
+    int64_t seg_cap;
+};
 
 struct Table {
     int32_t* slots;
     uint32_t* bits;
-    int32_t cap_log2;
-    int32_t bb_log2;
-    uint32_t max_probe;  // insert gives up (overflow) after this many slots
 };
+
+// How the probe side sizes its table: from the device count (fixed == 0)
+// or from an estimate fixed at launch (fixed == 1).
+struct Geo {
+    int32_t fixed;
+    int32_t placement;      // bolap_placement
+    int32_t cap_log2;       // fixed only
+    int32_t bb_log2;        // fixed only
+    int64_t lds_keys_max;   // device sizing: largest build that goes to LDS
+};
+
+// Table geometry for n build keys.  flag: 0 ok, 1 estimate too small
+// (n > slots / 2), 4 forced LDS placement cannot hold n keys.
+struct Shape {
+    bool lds;
+    int32_t cap_log2, bb_log2;
+    int32_t flag;
+};
+
+__host__ __device__ inline int clog2(int64_t v) {
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+__host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+__host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
+    if (g.fixed) {
+        Shape s{g.placement == BOLAP_PLACE_LDS, g.cap_log2, g.bb_log2, 0};
+        if (n > (int64_t(1) << g.cap_log2) / 2) s.flag = 1;
+        return s;
+    }
+    const int64_t m = n < 1 ? 1 : n;
+    if (g.placement != BOLAP_PLACE_HBM && n <= g.lds_keys_max)
+        return Shape{true, clog2(m * 4 < 64 ? 64 : m * 4), clampi(clog2(m * 128), 10, 18), 0};
+    if (g.placement == BOLAP_PLACE_LDS) return Shape{true, 0, 0, 4};
+    return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), clampi(clog2(m * 16), 10, 22), 0};
+}
+
+__host__ __device__ inline size_t lds_bytes_of(int cap_log2, int bb_log2) {
+    return (size_t(1) << cap_log2) * 4 + (size_t(1) << bb_log2) / 8;
+}
 
 BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
     return (uint32_t(key) * kSlotMul) >> (32 - cap_log2);
 }
 // Blocked Bloom filter: one 32-bit word per key (chosen by the top bits of
 // a second multiplicative hash) with two bits set from middle bits of the
-// slot hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
+// same hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
 BOLAP_DEV uint32_t bloom_hash(int32_t key) { return uint32_t(key) * kBitMul; }
 BOLAP_DEV uint32_t bloom_word(uint32_t h, int bb_log2) { return h >> (32 - (bb_log2 - 5)); }
 // Two bit positions from the ten hash bits just below the word index (the
@@ -70,72 +132,16 @@ BOLAP_DEV uint32_t bloom_bits(uint32_t h, int bb_log2) {
     return (1u << b1) | (1u << b2);
 }
 
-__global__ __launch_bounds__(256) void zero_res_k(int64_t* res) {
-    if (threadIdx.x < kResN) res[threadIdx.x] = 0;
-}
-
-__global__ __launch_bounds__(kThreads) void dim_count_k(PredSet ps, int64_t rows, int64_t* res) {
-    WaveCount wc;
-    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
-         base += int64_t(gridDim.x) * kThreads) {
-        const int64_t row = base + threadIdx.x;
-        wc.add(row < rows && eval_preds(ps, row));
+// Inserts a key (not the empty marker) with linear probing; the table is
+// sized so that every key finds a slot.
+BOLAP_DEV void table_insert(int32_t* T, int32_t key, int cap_log2) {
+    const uint32_t mask = (uint32_t(1) << cap_log2) - 1;
+    uint32_t s = slot_of(key, cap_log2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t prev = atomicCAS(T + s, kEmpty, key);
+        if (prev == kEmpty || prev == key) return;
+        s = (s + 1) & mask;
     }
-    wc.flush(res + kCount);
-}
-
-__global__ __launch_bounds__(256) void table_init_k(Table t, int64_t* res) {
-    const int64_t cap = int64_t(1) << t.cap_log2;
-    const int64_t words = (int64_t(1) << t.bb_log2) / 32;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cap;
-         i += int64_t(gridDim.x) * 256)
-        t.slots[i] = kEmpty;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
-         i += int64_t(gridDim.x) * 256)
-        t.bits[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < kResN && threadIdx.x != kCount) res[threadIdx.x] = 0;
-}
-
-__global__ __launch_bounds__(kThreads) void dim_build_k(PredSet ps, int64_t rows,
-                                                        const int32_t* __restrict__ keys,
-                                                        Table t, int64_t* res) {
-    const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
-    WaveCount passed;
-    uint64_t inserted = 0;
-    bool failed = false, has_min = false;
-    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < rows;
-         base += int64_t(gridDim.x) * kThreads) {
-        const int64_t row = base + threadIdx.x;
-        const bool pass = row < rows && eval_preds(ps, row);
-        passed.add(pass);
-        if (!pass || failed) continue;
-        const int32_t key = keys[row];
-        const uint32_t h = bloom_hash(key);
-        atomicOr(t.bits + bloom_word(h, t.bb_log2), bloom_bits(h, t.bb_log2));
-        if (key == kEmpty) {  // the table's empty marker: recorded as a flag
-            has_min = true;
-            continue;
-        }
-        uint32_t s = slot_of(key, t.cap_log2);
-        bool placed = false;
-        for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
-            const int32_t prev = atomicCAS(t.slots + s, kEmpty, key);
-            if (prev == kEmpty || prev == key) {
-                inserted += prev == kEmpty;
-                placed = true;
-                break;
-            }
-            s = (s + 1) & mask;
-        }
-        failed = !placed;  // estimate-sized table too small: overflow
-    }
-    passed.flush(res + kBuild);
-    const uint64_t wins = wave_sum(inserted);
-    const bool lead = (threadIdx.x & (kWave - 1)) == 0;
-    if (lead && wins)
-        atomicAdd(reinterpret_cast<unsigned long long*>(res + kDistinct), (unsigned long long)wins);
-    const uint64_t flags = (__ballot(failed) ? 1ull : 0ull) | (__ballot(has_min) ? 2ull : 0ull);
-    if (lead && flags) atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), flags);
 }
 
 // Exact membership test of a key that passed the Bloom filter.
@@ -150,6 +156,142 @@ BOLAP_DEV bool table_has(int32_t key, const int32_t* T, int cap_log2, uint32_t m
     return false;
 }
 
+__global__ __launch_bounds__(256) void zero_res_k(int64_t* res, uint32_t* seg_cnt) {
+    if (threadIdx.x < kResN) res[threadIdx.x] = 0;
+    if (threadIdx.x < kSegs) seg_cnt[threadIdx.x * kSegStride] = 0;
+}
+
+// Inclusive prefix sums of the segment lengths into `ends` (LDS), computed
+// by wave 0; returns the build size to every thread of the workgroup.
+BOLAP_DEV int64_t klist_prefix(const KList& kl, uint32_t* ends) {
+    __shared__ int64_t s_total;
+    if (threadIdx.x < kWave) {
+        uint32_t c = kl.cnt[threadIdx.x * kSegStride];
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = __shfl_up(c, d, kWave);
+            if (int(threadIdx.x) >= d) c += o;
+        }
+        ends[threadIdx.x] = c;
+        if (threadIdx.x == kWave - 1) s_total = int64_t(c);
+    }
+    __syncthreads();
+    return s_total;
+}
+
+// Key i of the segmented list (0 <= i < build size), found by a binary
+// search over the segment end offsets.
+BOLAP_DEV int32_t klist_key(const KList& kl, const uint32_t* ends, int64_t i) {
+    int lo = 0, hi = kSegs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (int64_t(ends[mid]) > i) hi = mid; else lo = mid + 1;
+    }
+    const int64_t start = lo ? int64_t(ends[lo - 1]) : 0;
+    return kl.keys[int64_t(lo) * kl.seg_cap + (i - start)];
+}
+
+// Build size: the sum of the segment lengths, computed by wave 0 (all 64
+// lanes) and returned to every thread of the workgroup; `seg` (LDS, may be
+// null) receives the lengths.
+BOLAP_DEV int64_t klist_size(const KList& kl, uint32_t* seg) {
+    __shared__ int64_t s_n;
+    if (threadIdx.x < kWave) {
+        const uint32_t c = kl.cnt[threadIdx.x * kSegStride];
+        if (seg) seg[threadIdx.x] = c;
+        const uint64_t n = wave_sum(uint64_t(c));
+        if (threadIdx.x == 0) s_n = int64_t(n);
+    }
+    __syncthreads();
+    return s_n;
+}
+
+// Filter + compaction of the dimension keys.  Lane `u` evaluates rows
+// u*R .. u*R+R-1; for each of the R row slots one ballot gives the wave's
+// survivors, so a lane's offset is the survivors of lower lanes in earlier
+// slots plus those of lower lanes in its own slot.  Workgroup b appends to
+// segment b % kSegs (the tail rows past the last whole unit go to segment 0).
+template <int R>
+__global__ __launch_bounds__(kCompactThreads) void dim_compact_k(PredSet ps, int64_t rows,
+                                                                 const int32_t* __restrict__ keys,
+                                                                 KList kl, int64_t* res) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t units = rows / R;
+    const int seg = int(blockIdx.x % kSegs);
+    int32_t* __restrict__ out = kl.keys + int64_t(seg) * kl.seg_cap;
+    uint32_t* cnt = kl.cnt + seg * kSegStride;
+    bool has_min = false;
+    for (int64_t base = int64_t(blockIdx.x) * kCompactThreads; base < units;
+         base += int64_t(gridDim.x) * kCompactThreads) {
+        const int64_t u = base + threadIdx.x;
+        const uint32_t m = u < units ? eval_preds_unit<R>(ps, u) : 0u;
+        uint64_t bal[R];
+        uint32_t total = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            bal[r] = __ballot((m >> r) & 1u);
+            total += uint32_t(__popcll(bal[r]));
+        }
+        if (total == 0) continue;  // wave-uniform
+        VecT<int32_t, R> k{};
+        if (m) k = load_unit<int32_t, R>(keys, u);
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(cnt, total);
+        at = uint32_t(__builtin_amdgcn_readfirstlane(int(at)));
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if ((m >> r) & 1u) {
+                const int32_t key = k.v[r];
+                out[at + uint32_t(__popcll(bal[r] & lt))] = key;
+                has_min |= key == kEmpty;
+            }
+            at += uint32_t(__popcll(bal[r]));
+        }
+    }
+    // Rows past the last whole unit (fewer than R).
+    if (blockIdx.x == 0 && int64_t(threadIdx.x) < rows - units * R) {
+        const int64_t row = units * R + threadIdx.x;
+        if (eval_preds(ps, row)) {
+            const int32_t key = keys[row];
+            out[atomicAdd(cnt, 1u)] = key;
+            has_min |= key == kEmpty;
+        }
+    }
+    if (__ballot(has_min) && lane == 0)
+        atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
+}
+
+// HBM table for a build side that does not go to LDS.
+__global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl) {
+    const Shape s = shape_of(klist_size(kl, nullptr), g);
+    if (s.flag || s.lds) return;
+    const int64_t cap = int64_t(1) << s.cap_log2;
+    const int64_t words = (int64_t(1) << s.bb_log2) / 32;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cap;
+         i += int64_t(gridDim.x) * 256)
+        t.slots[i] = kEmpty;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
+         i += int64_t(gridDim.x) * 256)
+        t.bits[i] = 0;
+}
+
+// Workgroup b inserts segment b % kSegs (gridDim.x is a multiple of kSegs).
+__global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl) {
+    __shared__ uint32_t seg[kSegs];
+    const Shape s = shape_of(klist_size(kl, seg), g);
+    if (s.flag || s.lds) return;
+    const int r = int(blockIdx.x % kSegs);
+    const int64_t per = gridDim.x / kSegs;
+    const int32_t* __restrict__ keys = kl.keys + int64_t(r) * kl.seg_cap;
+    for (int64_t i = int64_t(blockIdx.x / kSegs) * 256 + threadIdx.x; i < seg[r]; i += per * 256) {
+        const int32_t key = keys[i];
+        const uint32_t h = bloom_hash(key);
+        atomicOr(t.bits + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+        if (key != kEmpty) table_insert(t.slots, key, s.cap_log2);
+    }
+}
+
 template <typename TP, typename TD>
 BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __restrict__ disc,
                          int64_t& rev, uint32_t& cnt) {
@@ -158,10 +300,14 @@ BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __
 }
 
 // Sums the per-block (revenue, matches) partials with one workgroup of NT
-// threads into res[kRev], res[kMatch].
+// threads into res[kRev], res[kMatch], records the build size in
+// res[kBuild], and writes the whole result slot to `host` (pinned host
+// memory) with system-scope stores.
 template <int NT>
-BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks, int64_t* res) {
+BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks, KList kl,
+                               int64_t* res, int64_t* host) {
     __shared__ uint64_t s_a[NT / kWave], s_c[NT / kWave];
+    const int64_t n = klist_size(kl, nullptr);
     uint64_t a = 0, c = 0;
     for (int b = threadIdx.x; b < nblocks; b += NT) {
         const VecT<int64_t, 2> v = reinterpret_cast<const VecT<int64_t, 2>*>(partials)[b];
@@ -183,36 +329,31 @@ BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks
         }
         res[kRev] = int64_t(ta);
         res[kMatch] = int64_t(tc);
+        res[kBuild] = n;
+        for (int i = 0; i < kResN; ++i) {
+            const int64_t v = i == kRev ? int64_t(ta) : i == kMatch ? int64_t(tc)
+                            : i == kBuild ? n : res[i];
+            __hip_atomic_store(host + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
 // The same merge as a separate kernel after the probe on the same stream.
 __global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ partials,
-                                                    int nblocks, int64_t* res) {
-    join_merge_rows<256>(partials, nblocks, res);
+                                                    int nblocks, KList kl, int64_t* res,
+                                                    int64_t* host) {
+    join_merge_rows<256>(partials, nblocks, kl, res, host);
 }
 
-template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock>
-__global__ __launch_bounds__(kThreads) void probe_sum_k(
-    const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
-    int64_t rows, Table t, int64_t* res, int64_t* partials, unsigned* counter) {
-    extern __shared__ uint32_t lds[];
-    const int64_t words = (int64_t(1) << t.bb_log2) / 32;
-    const int64_t cap = int64_t(1) << t.cap_log2;
-    const uint32_t* B = t.bits;
-    const int32_t* T = t.slots;
-    if constexpr (kLds) {
-        for (int64_t i = threadIdx.x; i < words; i += kThreads) lds[i] = t.bits[i];
-        int32_t* ts = reinterpret_cast<int32_t*>(lds + words);
-        for (int64_t i = threadIdx.x; i < cap; i += kThreads) ts[i] = t.slots[i];
-        __syncthreads();
-        B = lds;
-        T = ts;
-    }
-    const bool has_min = (res[kFlags] & 2) != 0;
-    const uint32_t mask = uint32_t(cap - 1);
-    int64_t rev = 0;
-    uint32_t cnt = 0;
+// The fact-side loop of one workgroup over the table (B, T), in LDS or in
+// HBM depending on where the pointers come from (each call site is inlined
+// with its own address space).
+template <typename TP, typename TD, int kProbeU, bool kPipe>
+BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, const TP* __restrict__ price,
+                          const TD* __restrict__ disc, int64_t rows, const uint32_t* B,
+                          const int32_t* T, int cap_log2, int bb_log2, bool has_min,
+                          int64_t& rev, uint32_t& cnt) {
+    const uint32_t mask = (uint32_t(1) << cap_log2) - 1;
     const int64_t n_units = rows / 4;
     const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
     struct Tile {
@@ -227,19 +368,19 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         }
     };
     auto use_tile = [&](const Tile& x, int64_t at) {
-        // Phase 1: all Bloom words first (independent LDS reads), then a
+        // Phase 1: all Bloom words first (independent reads), then a
         // per-lane hit mask over the kProbeU * 4 keys.
         constexpr int kKeys = kProbeU * 4;
         uint32_t hv[kKeys], wv[kKeys];
 #pragma unroll
         for (int i = 0; i < kKeys; ++i) {
             hv[i] = bloom_hash(x.v[i / 4].v[i % 4]);
-            wv[i] = B[bloom_word(hv[i], t.bb_log2)];
+            wv[i] = B[bloom_word(hv[i], bb_log2)];
         }
         uint32_t hits = 0;
 #pragma unroll
         for (int i = 0; i < kKeys; ++i)
-            hits |= (bloom_test(wv[i], hv[i], t.bb_log2) ? 1u : 0u) << i;
+            hits |= (bloom_test(wv[i], hv[i], bb_log2) ? 1u : 0u) << i;
         // Units past the end were clamped to the last unit: drop them.
         if (at + int64_t(kProbeU - 1) * kThreads >= n_units) {
 #pragma unroll
@@ -253,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                 if (hits & (1u << i)) {
                     const int32_t key = x.v[i / 4].v[i % 4];
                     const int64_t row = (at + int64_t(i / 4) * kThreads) * 4 + (i % 4);
-                    if (key == kEmpty ? has_min : table_has(key, T, t.cap_log2, mask))
+                    if (key == kEmpty ? has_min : table_has(key, T, cap_log2, mask))
                         add_match<TP, TD>(row, price, disc, rev, cnt);
                 }
             }
@@ -286,8 +427,48 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         const int64_t row = n_units * 4 + threadIdx.x;
         if (row < rows) {
             const int32_t key = fk[row];
-            const bool h = key == kEmpty ? has_min : table_has(key, T, t.cap_log2, mask);
+            const bool h = key == kEmpty ? has_min : table_has(key, T, cap_log2, mask);
             if (h) add_match<TP, TD>(row, price, disc, rev, cnt);
+        }
+    }
+}
+
+template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock>
+__global__ __launch_bounds__(kThreads) void probe_sum_k(
+    const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
+    int64_t rows, Geo g, Table hbm, KList kl, int64_t* res, int64_t* partials,
+    unsigned* counter, int64_t* host) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t ends[kSegs];
+    const int64_t n = klist_prefix(kl, ends);
+    const Shape s = shape_of(n, g);
+    const bool has_min = (res[kFlags] & 2) != 0;
+    int64_t rev = 0;
+    uint32_t cnt = 0;
+    // No build keys (or a table that cannot hold them): nothing matches.
+    if (s.flag == 0 && n > 0) {
+        if (s.lds) {
+            const int64_t words = (int64_t(1) << s.bb_log2) / 32;
+            const int64_t cap = int64_t(1) << s.cap_log2;
+            uint32_t* Bl = lds;
+            int32_t* Tl = reinterpret_cast<int32_t*>(lds + words);
+            for (int64_t i = threadIdx.x; i < words; i += kThreads) Bl[i] = 0;
+            for (int64_t i = threadIdx.x; i < cap; i += kThreads) Tl[i] = kEmpty;
+            __syncthreads();
+            // One key per thread per round (independent loads), not one
+            // segment after another.
+            for (int64_t i = threadIdx.x; i < n; i += kThreads) {
+                const int32_t key = klist_key(kl, ends, i);
+                const uint32_t h = bloom_hash(key);
+                atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+                if (key != kEmpty) table_insert(Tl, key, s.cap_log2);
+            }
+            __syncthreads();
+            probe_rows<TP, TD, kProbeU, kPipe>(fk, price, disc, rows, Bl, Tl, s.cap_log2,
+                                               s.bb_log2, has_min, rev, cnt);
+        } else {
+            probe_rows<TP, TD, kProbeU, kPipe>(fk, price, disc, rows, hbm.bits, hbm.slots,
+                                               s.cap_log2, s.bb_log2, has_min, rev, cnt);
         }
     }
 
@@ -313,66 +494,66 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
     }
     if constexpr (kLastBlock) {
         if (!last_block_done(counter)) return;
-        join_merge_rows<kThreads>(partials, int(gridDim.x), res);
+        join_merge_rows<kThreads>(partials, int(gridDim.x), kl, res, host);
         if (threadIdx.x == 0) reset_counter(counter);
     }
 }
 
-int ceil_log2(int64_t v) {
-    int l = 0;
-    while ((int64_t(1) << l) < v) ++l;
-    return l;
-}
-
-struct Sizing {
-    int cap_log2, bb_log2;
-    int placement;
-    size_t lds_bytes;
-};
-
-// Table geometry for `n` expected build keys: LDS tables run at load <= 1/4
-// with a 128-bit-per-key Bloom filter; HBM tables at load <= 1/2 with 16 bits
-// per key.  Either overflows (and the caller re-runs exactly) past `cap / 2`.
-int size_table(int64_t n, int placement, Sizing& z) {
+// Table geometry fixed from an expected build size (capacity_hint): LDS
+// tables at load <= 1/4 with a 128-bit-per-key Bloom filter; HBM tables at
+// load <= 1/2 with 16 bits per key.  Either overflows past `cap / 2`.
+int size_from_hint(int64_t n, int placement, Geo& g, size_t& lds_bytes) {
     if (n < 1) n = 1;
-    const int lds_cap = ceil_log2(n * 4 < 64 ? 64 : n * 4);
-    int lds_bb = ceil_log2(n * 128);
-    if (lds_bb < 10) lds_bb = 10;
-    if (lds_bb > 18) lds_bb = 18;  // 32 KB of LDS
-    const size_t lds_bytes = (size_t(1) << lds_cap) * 4 + (size_t(1) << lds_bb) / 8;
-    const bool fits = lds_cap <= 30 && lds_bytes <= size_t(kLdsBudget);
+    const int lds_cap = clog2(n * 4 < 64 ? 64 : n * 4);
+    const int lds_bb = clampi(clog2(n * 128), 10, 18);  // at most 32 KB of filter
+    const size_t lb = lds_bytes_of(lds_cap, lds_bb);
+    const bool fits = lds_cap <= 30 && lb <= size_t(kLdsBudget);
     if (placement == BOLAP_PLACE_LDS && !fits) {
         set_error("bolap_join_sum: %lld keys do not fit the LDS budget", (long long)n);
         return BOLAP_ERR_CAPACITY;
     }
     if (fits && placement != BOLAP_PLACE_HBM) {
-        z = Sizing{lds_cap, lds_bb, BOLAP_PLACE_LDS, lds_bytes};
+        g = Geo{1, BOLAP_PLACE_LDS, lds_cap, lds_bb, 0};
+        lds_bytes = lb;
         return BOLAP_OK;
     }
-    int cap = ceil_log2(n * 2 < 64 ? 64 : n * 2);
-    int bb = ceil_log2(n * 16);
-    if (bb < 10) bb = 10;
-    if (bb > 22) bb = 22;  // 512 KB filter, L2-resident
+    const int cap = clog2(n * 2 < 64 ? 64 : n * 2);
     if (cap > 31) {
         set_error("bolap_join_sum: %lld keys exceed the table limit", (long long)n);
         return BOLAP_ERR_CAPACITY;
     }
-    z = Sizing{cap, bb, BOLAP_PLACE_HBM, 0};
+    g = Geo{1, BOLAP_PLACE_HBM, cap, clampi(clog2(n * 16), 10, 22), 0};
+    lds_bytes = 0;
     return BOLAP_OK;
 }
 
-template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
-int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
-                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
-    auto kern = probe_sum_k<kLds, TP, TD, kProbeU, kPipe, kLast>;
-    if (kLds) {
+// Everything one run of the pipeline launches with (device pointers carved
+// from the workspace, grid sizes, table geometry).
+struct Plan {
+    Geo g;
+    size_t lds;              // dynamic LDS of the probe kernel
+    Table t;                 // HBM table (unused when the build goes to LDS)
+    KList kl;
+    int64_t* res;
+    int64_t* partials;
+    int64_t* host;           // device address of the pinned result row
+    int64_t max_blocks;
+    int64_t hbm_cap;
+    bool may_hbm;
+    int compact_r;           // rows per lane of the compaction (4 or 1)
+    unsigned compact_grid;
+};
+
+template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
+int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
+    auto kern = probe_sum_k<TP, TD, kProbeU, kPipe, kLast>;
+    if (p.lds > 0) {
         BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          int(lds)));
+                                          int(p.lds)));
     }
     int per_cu = 0;
-    BOLAP_HIP_TRY(
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, kLds ? lds : 0));
+    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, p.lds));
     int64_t want = ctx->knob("join.blocks_per_cu", 2);
     if (per_cu > 0 && want > per_cu) want = per_cu;
     int64_t blocks = int64_t(ctx->num_cus) * want;
@@ -380,106 +561,137 @@ int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t,
                          (int64_t(kThreads) * kProbeU);
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks > p.max_blocks) blocks = p.max_blocks;
     {
         TraceScope ts(ctx, "join_probe");
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), kLds ? lds : 0,
-                           ctx->stream, a->fact_fk, static_cast<const TP*>(a->fact_price),
-                           static_cast<const TD*>(a->fact_disc), a->fact_rows, t, res,
-                           ws.at<int64_t>(o_part), ctx->counters + 1);
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), p.lds, ctx->stream,
+                           a->fact_fk, static_cast<const TP*>(a->fact_price),
+                           static_cast<const TD*>(a->fact_disc), a->fact_rows, p.g, p.t, p.kl,
+                           p.res, p.partials, ctx->counters + 1, p.host);
     }
     BOLAP_TRY(launch_check("probe_sum_k"));
     if (kLast) return BOLAP_OK;
     TraceScope ts(ctx, "join_merge");
-    hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, ws.at<int64_t>(o_part),
-                       int(blocks), res);
+    hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, p.partials, int(blocks),
+                       p.kl, p.res, p.host);
     return launch_check("join_merge_k");
 }
 
 // Finish (knob "join.finish"): 1 = last workgroup merges, 2 = merge kernel.
-template <bool kLds, typename TP, typename TD, int kProbeU, bool kPipe>
-int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
-                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+template <typename TP, typename TD, int kProbeU, bool kPipe>
+int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     if (ctx->knob("join.finish", 2) == 1)
-        return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, true>(ctx, a, t, lds, res, ws, o_part,
-                                                                  max_blocks);
-    return launch_probe_f<kLds, TP, TD, kProbeU, kPipe, false>(ctx, a, t, lds, res, ws, o_part,
-                                                               max_blocks);
+        return launch_probe_f<TP, TD, kProbeU, kPipe, true>(ctx, a, p);
+    return launch_probe_f<TP, TD, kProbeU, kPipe, false>(ctx, a, p);
 }
 
 // Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,
 // 3 pipelined U=2, 4 pipelined U=4 (4-key units per lane per tile).
-template <bool kLds, typename TP, typename TD>
-int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
-                 int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+template <typename TP, typename TD>
+int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     switch (ctx->knob("join.probe_variant", 3)) {
-        case 1: return launch_probe_v<kLds, TP, TD, 4, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-        case 2: return launch_probe_v<kLds, TP, TD, 8, false>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-        case 3: return launch_probe_v<kLds, TP, TD, 2, true>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-        default: return launch_probe_v<kLds, TP, TD, 4, true>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+        case 1: return launch_probe_v<TP, TD, 4, false>(ctx, a, p);
+        case 2: return launch_probe_v<TP, TD, 8, false>(ctx, a, p);
+        case 3: return launch_probe_v<TP, TD, 2, true>(ctx, a, p);
+        default: return launch_probe_v<TP, TD, 4, true>(ctx, a, p);
     }
 }
 
-template <bool kLds>
-int dispatch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Table& t, size_t lds,
-                   int64_t* res, WsArena& ws, size_t o_part, int64_t max_blocks) {
+int dispatch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     const bool p64 = a->price_dtype == BOLAP_I64;
     const bool d64 = a->disc_dtype == BOLAP_I64;
-    if (p64 && d64) return launch_probe<kLds, int64_t, int64_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-    if (p64) return launch_probe<kLds, int64_t, uint8_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-    if (d64) return launch_probe<kLds, int32_t, int64_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
-    return launch_probe<kLds, int32_t, uint8_t>(ctx, a, t, lds, res, ws, o_part, max_blocks);
+    if (p64 && d64) return launch_probe<int64_t, int64_t>(ctx, a, p);
+    if (p64) return launch_probe<int64_t, uint8_t>(ctx, a, p);
+    if (d64) return launch_probe<int32_t, int64_t>(ctx, a, p);
+    return launch_probe<int32_t, uint8_t>(ctx, a, p);
 }
 
-inline unsigned small_grid(bolap_ctx* ctx, int64_t rows, int threads) {
-    int64_t b = (rows + threads - 1) / threads;
-    const int64_t cap = int64_t(ctx->num_cus) * 8;
+inline unsigned grid_for(bolap_ctx* ctx, int64_t items, int threads, int per_cu) {
+    int64_t b = (items + threads - 1) / threads;
+    const int64_t cap = int64_t(ctx->num_cus) * per_cu;
     if (b > cap) b = cap;
     return unsigned(b < 1 ? 1 : b);
 }
 
-// One attempt with a given expected build size; exact == sized from a count.
-int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
-                 int64_t n_expect, bool exact, int64_t* host_res, Sizing& z) {
-    BOLAP_TRY(size_table(n_expect, a->placement, z));
-    const int64_t cap = int64_t(1) << z.cap_log2;
-    const int64_t words = (int64_t(1) << z.bb_log2) / 32;
-    const int64_t max_blocks = int64_t(ctx->num_cus) * 8;
-    WsArena ws;
-    const size_t o_slots = ws.plan(size_t(cap) * 4);
-    const size_t o_bits = ws.plan(size_t(words) * 4);
-    const size_t o_part = ws.plan(size_t(max_blocks) * 2 * sizeof(int64_t));
-    BOLAP_TRY(ws_reserve(ctx, ws.off));
-    ws.base = static_cast<char*>(ctx->ws);
-    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
-    // Exact tables (load <= 1/2) always place every key; estimate-sized ones
-    // give up after 128 slots and fall back to exact sizing.
-    Table t{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits), z.cap_log2, z.bb_log2,
-            exact ? uint32_t(cap) : 128u};
-    int64_t init_blocks = (cap + 255) / 256;
-    if (init_blocks > 4096) init_blocks = 4096;
-    {
-        TraceScope ts(ctx, "join_init");
-        hipLaunchKernelGGL(table_init_k, dim3(unsigned(init_blocks)), dim3(256), 0, ctx->stream,
-                           t, res);
-    }
-    BOLAP_TRY(launch_check("table_init_k"));
+// The launches of one run.
+int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
+                 const Plan& p) {
+    hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, p.res, p.kl.cnt);
+    BOLAP_TRY(launch_check("zero_res_k"));
     if (a->dim_rows > 0) {
-        TraceScope ts(ctx, "join_build");
-        hipLaunchKernelGGL(dim_build_k, dim3(small_grid(ctx, a->dim_rows, kThreads)),
-                           dim3(kThreads), 0, ctx->stream, ps, a->dim_rows, a->dim_key, t, res);
-        BOLAP_TRY(launch_check("dim_build_k"));
+        TraceScope ts(ctx, "join_compact");
+        if (p.compact_r == 4)
+            hipLaunchKernelGGL(dim_compact_k<4>, dim3(p.compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->dim_rows, a->dim_key, p.kl, p.res);
+        else
+            hipLaunchKernelGGL(dim_compact_k<1>, dim3(p.compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->dim_rows, a->dim_key, p.kl, p.res);
+        BOLAP_TRY(launch_check("dim_compact_k"));
     }
-    if (z.placement == BOLAP_PLACE_LDS)
-        BOLAP_TRY(dispatch_probe<true>(ctx, a, t, z.lds_bytes, res, ws, o_part, max_blocks));
-    else
-        BOLAP_TRY(dispatch_probe<false>(ctx, a, t, 0, res, ws, o_part, max_blocks));
+    if (p.may_hbm && a->dim_rows > 0) {
+        // Small grids (grid-stride loops): both exit at once when the build
+        // side goes to LDS, which is the common case.
+        TraceScope ts(ctx, "join_hbm_build");
+        hipLaunchKernelGGL(hbm_init_k, dim3(grid_for(ctx, p.hbm_cap, 256, 1)), dim3(256), 0,
+                           ctx->stream, p.g, p.t, p.kl);
+        BOLAP_TRY(launch_check("hbm_init_k"));
+        hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, p.g, p.t,
+                           p.kl);
+        BOLAP_TRY(launch_check("hbm_build_k"));
+    }
+    return dispatch_probe(ctx, a, p);
+}
+
+// One run of the pipeline with table geometry `g`; host_res receives the
+// result slot.
+int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps, const Geo& g,
+                 size_t lds, int64_t* host_res) {
+    Plan p{};
+    p.g = g;
+    p.lds = lds;
+    // HBM table: the largest the build side can need under `g`.
+    p.may_hbm = g.fixed ? g.placement == BOLAP_PLACE_HBM : g.placement != BOLAP_PLACE_LDS;
+    int cap_log2 = 0, bb_log2 = 0;
+    if (p.may_hbm) {
+        const Shape worst = g.fixed ? Shape{false, g.cap_log2, g.bb_log2, 0}
+                                    : shape_of(a->dim_rows, Geo{0, BOLAP_PLACE_HBM, 0, 0, 0});
+        cap_log2 = worst.cap_log2;
+        bb_log2 = worst.bb_log2;
+    }
+    p.hbm_cap = p.may_hbm ? int64_t(1) << cap_log2 : 0;
+    const int64_t words = p.may_hbm ? (int64_t(1) << bb_log2) / 32 : 0;
+    p.max_blocks = int64_t(ctx->num_cus) * 8;
+    // Compaction geometry and the segment capacity it implies: workgroup b
+    // covers at most ceil(units / (grid * threads)) * threads units of R rows,
+    // segment r takes the workgroups b = r (mod kSegs), and segment 0 also
+    // takes the fewer than R tail rows.
+    p.compact_r = (aligned(a->dim_key, 16) && preds_aligned(ps, 4)) ? 4 : 1;
+    const int64_t units = a->dim_rows / p.compact_r;
+    p.compact_grid = grid_for(ctx, units, kCompactThreads, 16);
+    const int64_t per_block =
+        (units + int64_t(p.compact_grid) * kCompactThreads - 1) /
+        (int64_t(p.compact_grid) * kCompactThreads) * kCompactThreads * p.compact_r;
+    const int64_t blocks_per_seg = (int64_t(p.compact_grid) + kSegs - 1) / kSegs;
+    const int64_t seg_cap = blocks_per_seg * per_block + p.compact_r;
+
+    WsArena ws;
+    const size_t o_cnt = ws.plan(size_t(kSegs) * kSegStride * 4);
+    const size_t o_keys = ws.plan(size_t(seg_cap) * kSegs * 4);
+    const size_t o_slots = ws.plan(size_t(p.hbm_cap) * 4);
+    const size_t o_bits = ws.plan(size_t(words) * 4);
+    const size_t o_part = ws.plan(size_t(p.max_blocks) * 2 * sizeof(int64_t));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
     BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
-    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, res, kResN * sizeof(int64_t),
-                                 hipMemcpyDeviceToHost, ctx->stream));
+    ws.base = static_cast<char*>(ctx->ws);
+    p.res = static_cast<int64_t*>(ctx->result_dev);
+    p.kl = KList{ws.at<int32_t>(o_keys), ws.at<uint32_t>(o_cnt), seg_cap};
+    p.t = Table{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits)};
+    p.partials = ws.at<int64_t>(o_part);
+    p.host = static_cast<int64_t*>(ctx->pinned_dev);
+
+    BOLAP_TRY(enqueue_join(ctx, a, ps, p));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::memcpy(host_res, ctx->pinned, kResN * sizeof(int64_t));
-    if (host_res[kDistinct] > cap / 2) host_res[kFlags] |= 1;  // load factor above 1/2
     return BOLAP_OK;
 }
 
@@ -492,6 +704,8 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
                               bolap_join_sum_result* out) {
     BOLAP_ARG_CHECK(ctx && a && out, "bolap_join_sum: null argument");
     BOLAP_ARG_CHECK(a->dim_rows >= 0 && a->fact_rows >= 0, "bolap_join_sum: negative rows");
+    BOLAP_ARG_CHECK(a->dim_rows <= (int64_t(1) << 30), "bolap_join_sum: %lld dimension rows "
+                    "exceed 2^30", (long long)a->dim_rows);
     BOLAP_ARG_CHECK(a->dim_rows == 0 || a->dim_key, "bolap_join_sum: null dim_key");
     BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_fk && a->fact_price && a->fact_disc),
                     "bolap_join_sum: null fact column");
@@ -507,47 +721,43 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
     BOLAP_TRY(make_predset(ps, a->n_dim_preds, a->dim_preds, a->dim_rows, "bolap_join_sum"));
 
     int64_t r[kResN];
-    Sizing z{};
+    Geo g{};
     int fallbacks = 0;
     bool done = false;
     if (a->capacity_hint > 0) {
-        BOLAP_TRY(join_attempt(ctx, a, ps, a->capacity_hint, false, r, z));
-        if (r[kFlags] & 1)
-            fallbacks = 1;  // estimate too small: exact path below
+        size_t lds = 0;
+        BOLAP_TRY(size_from_hint(a->capacity_hint, a->placement, g, lds));
+        BOLAP_TRY(join_attempt(ctx, a, ps, g, lds, r));
+        if (shape_of(r[kBuild], g).flag & 1)
+            fallbacks = 1;  // estimate too small: device-sized run below
         else
             done = true;
     }
     if (!done) {
-        // Exact sizing: count the surviving dimension rows first.
-        int64_t* res = static_cast<int64_t*>(ctx->result_dev);
-        hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, res);
-        BOLAP_TRY(launch_check("zero_res_k"));
-        if (a->dim_rows > 0) {
-            TraceScope ts(ctx, "join_count");
-            hipLaunchKernelGGL(dim_count_k, dim3(small_grid(ctx, a->dim_rows, kThreads)),
-                               dim3(kThreads), 0, ctx->stream, ps, a->dim_rows, res);
-            BOLAP_TRY(launch_check("dim_count_k"));
+        // Exact sizing on the device from the compacted build size.
+        const int64_t lds_keys =
+            a->placement == BOLAP_PLACE_LDS ? kForcedLdsKeys : kAutoLdsKeys;
+        g = Geo{0, a->placement, 0, 0, lds_keys};
+        size_t lds = 0;
+        if (a->placement != BOLAP_PLACE_HBM) {
+            const Shape big = shape_of(lds_keys, g);
+            lds = lds_bytes_of(big.cap_log2, big.bb_log2);
         }
-        int64_t count = 0;
-        BOLAP_TRY(pinned_reserve(ctx, sizeof(int64_t)));
-        BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, res + kCount, sizeof(int64_t),
-                                     hipMemcpyDeviceToHost, ctx->stream));
-        BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        std::memcpy(&count, ctx->pinned, sizeof(int64_t));
-        BOLAP_TRY(join_attempt(ctx, a, ps, count, true, r, z));
-        if (r[kFlags] & 1) {
-            set_error("bolap_join_sum: exact-sized table overflowed");
+        BOLAP_TRY(join_attempt(ctx, a, ps, g, lds, r));
+        if (shape_of(r[kBuild], g).flag & 4) {
+            set_error("bolap_join_sum: %lld keys do not fit the LDS budget", (long long)r[kBuild]);
             return BOLAP_ERR_CAPACITY;
         }
     }
+    const Shape s = shape_of(r[kBuild], g);
     trace_count(ctx, "join_build_rows", r[kBuild]);
     trace_count(ctx, "join_probe_rows_in", a->fact_rows);
     trace_count(ctx, "join_rows_emitted", r[kMatch]);
     out->revenue = r[kRev];
     out->matches = r[kMatch];
     out->build_rows = r[kBuild];
-    out->capacity = int64_t(1) << z.cap_log2;
-    out->placement = z.placement;
+    out->capacity = int64_t(1) << s.cap_log2;
+    out->placement = s.lds ? BOLAP_PLACE_LDS : BOLAP_PLACE_HBM;
     out->fallbacks = fallbacks;
     return BOLAP_OK;
 }

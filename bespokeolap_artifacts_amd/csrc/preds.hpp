@@ -55,8 +55,7 @@ BOLAP_DEV int64_t load_any(const void* col, int32_t dtype, int64_t row) {
     }
 }
 
-BOLAP_DEV bool eval_pred(const PredK& p, int64_t row) {
-    const int64_t v = load_any(p.col, p.dtype, row);
+BOLAP_DEV bool pred_holds(const PredK& p, int64_t v) {
     if (p.op == BOLAP_OP_RANGE) return v >= p.lo && v <= p.hi;
     if (p.set) {
         if (v < 0 || v >= p.set_bits) return false;
@@ -66,6 +65,48 @@ BOLAP_DEV bool eval_pred(const PredK& p, int64_t row) {
     const int w = int(v >> 6);
     const uint64_t word = w == 0 ? p.inl[0] : w == 1 ? p.inl[1] : w == 2 ? p.inl[2] : p.inl[3];
     return (word >> (v & 63)) & 1u;
+}
+
+// R consecutive rows of one column (rows unit*R .. unit*R+R-1) with one
+// vector load; the column must be aligned to R elements.
+template <int R, typename T>
+BOLAP_DEV void load_rows(const void* col, int64_t unit, int64_t (&v)[R]) {
+    const VecT<T, R> x = load_unit<T, R>(static_cast<const T*>(col), unit);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = int64_t(x.v[r]);
+}
+
+// Conjunction over R consecutive rows: bit r of the result is row unit*R+r.
+// Every predicate is evaluated for every row (no short circuit), so the
+// loads of all predicate columns are issued together.
+template <int R>
+BOLAP_DEV uint32_t eval_preds_unit(const PredSet& ps, int64_t unit) {
+    uint32_t m = (1u << R) - 1u;
+    for (int i = 0; i < ps.n; ++i) {
+        const PredK& p = ps.p[i];
+        int64_t v[R];
+        switch (p.dtype) {  // uniform across the wave
+            case BOLAP_U8: load_rows<R, uint8_t>(p.col, unit, v); break;
+            case BOLAP_I16: load_rows<R, int16_t>(p.col, unit, v); break;
+            case BOLAP_I32: load_rows<R, int32_t>(p.col, unit, v); break;
+            default: load_rows<R, int64_t>(p.col, unit, v); break;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (!pred_holds(p, v[r])) m &= ~(1u << r);
+    }
+    return m;
+}
+
+// True when every predicate column is aligned for eval_preds_unit<R>.
+inline bool preds_aligned(const PredSet& ps, int R) {
+    for (int i = 0; i < ps.n; ++i)
+        if (!aligned(ps.p[i].col, size_t(R) * size_t(dtype_size(ps.p[i].dtype)))) return false;
+    return true;
+}
+
+BOLAP_DEV bool eval_pred(const PredK& p, int64_t row) {
+    return pred_holds(p, load_any(p.col, p.dtype, row));
 }
 
 BOLAP_DEV bool eval_preds(const PredSet& ps, int64_t row) {

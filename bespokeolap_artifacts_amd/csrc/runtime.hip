@@ -41,8 +41,10 @@ int pinned_reserve(bolap_ctx* ctx, size_t bytes) {
     if (ctx->pinned) BOLAP_HIP_TRY(hipHostFree(ctx->pinned));
     ctx->pinned = nullptr;
     ctx->pinned_size = 0;
+    ctx->pinned_dev = nullptr;
     BOLAP_HIP_TRY(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
     ctx->pinned_size = want;
+    BOLAP_HIP_TRY(hipHostGetDevicePointer(&ctx->pinned_dev, ctx->pinned, 0));
     return BOLAP_OK;
 }
 

@@ -135,3 +135,33 @@ def test_join_sum_rejects_misaligned_fact_keys(engine):
     f = engine.gen_fact(SEED, 101, 100)
     with pytest.raises(N.BolapError):
         engine.join_sum(d["key"], [], f["fk"][1:], f["price"][1:], f["disc"][1:])
+
+
+def test_join_sum_repeated_calls_recompute(engine):
+    """Repeated calls with the same arguments reuse the workspace and the
+    pinned result row; each call re-reads the columns, so changing their
+    contents in place changes the result exactly as the oracle says."""
+    import torch
+    d = engine.gen_dim(SEED, 100_000)
+    f = engine.gen_fact(SEED, 1_000_003, 100_000)
+    preds = [engine.pred(d["cat"], N.OP_RANGE, 3, 40),
+             engine.pred(d["cont"], N.OP_IN_SET, values=[1, 2, 3, 4, 5])]
+    hd = {k: host(v) for k, v in d.items()}
+    opreds = [oracle.make_pred(hd["cat"], 0, 3, 40),
+              oracle.make_pred(hd["cont"], 1, values=[1, 2, 3, 4, 5])]
+    for hint in (0, 5000):
+        a = engine.join_sum_args(d["key"], preds, f["fk"], f["price"], f["disc"],
+                                 capacity_hint=hint)
+        want = oracle.join_sum(hd["key"], opreds, host(f["fk"]), host(f["price"]), host(f["disc"]))
+        for _ in range(4):
+            got = engine.join_sum_run(a)
+            assert same(got, want), (hint, got, want)
+            torch.cuda.synchronize()
+        f["price"].mul_(3).add_(1)
+        torch.cuda.synchronize()
+        want = oracle.join_sum(hd["key"], opreds, host(f["fk"]), host(f["price"]), host(f["disc"]))
+        for _ in range(3):
+            got = engine.join_sum_run(a)
+            assert same(got, want), (hint, got, want)
+            f["disc"].add_(0)  # eager work on the same stream between replays
+            torch.cuda.synchronize()
