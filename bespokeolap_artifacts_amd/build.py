@@ -72,7 +72,8 @@ def build_oracle() -> pathlib.Path:
     cc = shutil.which("gcc") or "cc"
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     cmd = [cc, "-O3", "-march=x86-64-v2", "-std=c11", "-fPIC", "-shared",
-           str(ORACLE_DIR / "oracle.c"), str(ORACLE_DIR / "star.c"), str(ORACLE_DIR / "ceb.c"), "-o", str(tmp)]
+           str(ORACLE_DIR / "oracle.c"), str(ORACLE_DIR / "star.c"), str(ORACLE_DIR / "ceb.c"),
+           str(ORACLE_DIR / "tpch.c"), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")

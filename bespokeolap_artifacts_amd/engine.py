@@ -296,6 +296,124 @@ class Engine:
                                                   out.ctypes.data_as(C.POINTER(C.c_int64))))
         return out
 
+    # --------------------------------------------------------------- Q6
+    @staticmethod
+    def q6_params(date: str, discount: str, quantity: str) -> dict:
+        """run_q6's arguments from the query strings (host-side C++)."""
+        p = N.Q6Params()
+        lib = N.load()
+        N.check(lib.bolap_q6_parse(date.encode(), discount.encode(), quantity.encode(),
+                                   C.byref(p)))
+        return {k: getattr(p, k) for k, _ in N.Q6Params._fields_}
+
+    @staticmethod
+    def q6_args(cols: dict, p: dict, zones=None) -> N.Q6Args:
+        a = N.Q6Args()
+        a.rows = int(cols["ship"].numel())
+        a.ship, a.disc, a.qty, a.price = (ptr(cols[k]) for k in ("ship", "disc", "qty", "price"))
+        a.ship_lo, a.ship_hi = p["ship_lo"], p["ship_hi"]
+        a.disc_lo, a.disc_hi = p["disc_lo"], p["disc_hi"]
+        a.qty_lt = p["qty_lt"]
+        a.zones = ptr(zones) if zones is not None else None
+        return a
+
+    def q6_run(self, a: N.Q6Args) -> dict:
+        r = N.Q6Result()
+        N.check(self.lib.bolap_q6(self.ctx, C.byref(a), C.byref(r)))
+        return {"revenue": r.revenue, "rows_emitted": r.rows_emitted,
+                "rows_scanned": r.rows_scanned}
+
+    def q6(self, cols: dict, p: dict, zones=None) -> dict:
+        return self.q6_run(self.q6_args(cols, p, zones))
+
+    def q6_zones(self, cols: dict):
+        """Zone map (uint8 tensor of 16-byte zones) of the Q6 columns."""
+        torch = _torch()
+        rows = int(cols["ship"].numel())
+        nz = (rows + N.ZONE_ROWS - 1) // N.ZONE_ROWS
+        z = torch.empty(max(1, nz) * 16, dtype=torch.uint8, device=f"cuda:{self.device}")
+        N.check(self.lib.bolap_q6_zones(self.ctx, rows, ptr(cols["ship"]), ptr(cols["disc"]),
+                                        ptr(cols["qty"]), ptr(z)))
+        return z
+
+    def q6_cluster(self, cols: dict, with_perm: bool = False) -> dict:
+        """Copies of the Q6 columns ordered by the shard key."""
+        torch = _torch()
+        rows = int(cols["ship"].numel())
+        out = {k: torch.empty_like(cols[k]) for k in ("ship", "disc", "qty", "price")}
+        perm = torch.empty(max(1, rows), dtype=torch.int32, device=f"cuda:{self.device}") \
+            if with_perm else None
+        N.check(self.lib.bolap_q6_cluster(self.ctx, rows, *(ptr(cols[k]) for k in
+                                                            ("ship", "disc", "qty", "price")),
+                                          *(ptr(out[k]) for k in ("ship", "disc", "qty", "price")),
+                                          ptr(perm) if perm is not None else None))
+        if perm is not None:
+            out["perm"] = perm[:rows]
+        return out
+
+    # -------------------------------------------------------------- Q18
+    @staticmethod
+    def q18_args(t: dict, threshold: int, orderkey_sorted: bool = True) -> N.Q18Args:
+        a = N.Q18Args()
+        a.li_rows = int(t["l_orderkey"].numel())
+        a.l_orderkey, a.l_quantity = ptr(t["l_orderkey"]), ptr(t["l_qty"])
+        a.orderkey_sorted = int(orderkey_sorted)
+        a.order_rows = int(t["o_orderkey"].numel())
+        a.o_orderkey, a.o_custkey = ptr(t["o_orderkey"]), ptr(t["o_custkey"])
+        a.o_orderdate, a.o_totalprice = ptr(t["o_orderdate"]), ptr(t["o_totalprice"])
+        a.cust_rows = int(t["c_custkey"].numel())
+        a.c_custkey = ptr(t["c_custkey"])
+        a.quantity_threshold = int(threshold)
+        return a
+
+    def q18_run(self, a: N.Q18Args, max_rows: int = 1 << 16):
+        """Result rows as a structured numpy array (oracle Q18_DTYPE layout)
+        and the stats dict."""
+        import numpy as np
+        rows = (N.Q18Row * max(1, max_rows))()
+        st = N.Q18Stats()
+        N.check(self.lib.bolap_q18(self.ctx, C.byref(a), rows, max_rows, C.byref(st)))
+        dt = np.dtype([("custkey", "<i4"), ("orderkey", "<i4"), ("orderdate", "<i4"),
+                       ("totalprice", "<i4"), ("sum_qty", "<i8"), ("cust_row", "<i4"),
+                       ("order_row", "<i4")])
+        arr = np.frombuffer(bytes(rows), dtype=dt)[:st.rows].copy()
+        return arr, {"rows": st.rows, "dense_path": st.dense_path,
+                     "sorted_pass": st.sorted_pass, "fallbacks": st.fallbacks}
+
+    def q18(self, t: dict, threshold: int, orderkey_sorted: bool = True, max_rows: int = 1 << 16):
+        return self.q18_run(self.q18_args(t, threshold, orderkey_sorted), max_rows)
+
+    # ---------------------------------------------------- build derivations
+    def derive_lineitem(self, price, disc, rf, ls, n_ls: int, orderkey=None):
+        torch = _torch()
+        rows = int(price.numel())
+        dp = torch.empty(rows, dtype=torch.int32, device=price.device)
+        code = torch.empty(rows, dtype=torch.uint8, device=price.device)
+        s = C.c_int32()
+        N.check(self.lib.bolap_derive_lineitem(self.ctx, rows, ptr(price), ptr(disc), ptr(rf),
+                                               ptr(ls), n_ls, ptr(orderkey), ptr(dp), ptr(code),
+                                               C.byref(s)))
+        return dp, code, bool(s.value)
+
+    def date_order(self, date):
+        torch = _torch()
+        rows = int(date.numel())
+        out = torch.empty(max(1, rows), dtype=torch.int32, device=date.device)
+        N.check(self.lib.bolap_date_order(self.ctx, rows, ptr(date), ptr(out)))
+        return out[:rows]
+
+    def order_ranges(self, o_orderkey, l_orderkey, map_size: int):
+        torch = _torch()
+        o_rows = int(o_orderkey.numel())
+        dev = o_orderkey.device
+        m = torch.empty(max(1, map_size), dtype=torch.int32, device=dev)
+        start = torch.empty(max(1, o_rows), dtype=torch.int32, device=dev)
+        end = torch.empty(max(1, o_rows), dtype=torch.int32, device=dev)
+        N.check(self.lib.bolap_order_ranges(self.ctx, o_rows, ptr(o_orderkey), map_size, ptr(m),
+                                            int(l_orderkey.numel()), ptr(l_orderkey),
+                                            ptr(start), ptr(end)))
+        return m[:map_size], start[:o_rows], end[:o_rows]
+
     # ------------------------------------------------------- generators
     def gen_lineitem(self, seed: int, rows: int) -> dict:
         torch = _torch()
@@ -333,6 +451,36 @@ class Engine:
         N.check(self.lib.bolap_gen_fact(self.ctx, seed, rows, dim_rows, ptr(cols["fk"]),
                                         ptr(cols["price"]), ptr(cols["disc"])))
         return cols
+
+    def gen_uniform_i32(self, seed: int, stream: int, rows: int, lo: int, hi: int):
+        torch = _torch()
+        out = torch.empty(max(1, rows), dtype=torch.int32, device=f"cuda:{self.device}")
+        N.check(self.lib.bolap_gen_uniform_i32(self.ctx, seed, stream, rows, lo, hi, ptr(out)))
+        return out[:rows]
+
+    def gen_q6_lineitem(self, seed: int, rows: int) -> dict:
+        """Q6 columns in the reference layout (builder_impl.hpp:39-66): ship
+        i16 day offset, discount u8 hundredths, quantity i16 x 100, price i32
+        cents, from the seeded lineitem generator."""
+        torch = _torch()
+        w = self.gen_lineitem(seed, rows)
+        return {"ship": w["ship"].to(torch.int16), "disc": w["disc"].to(torch.uint8),
+                "qty": (w["qty"] * 100).to(torch.int16), "price": w["price"].to(torch.int32)}
+
+    def gen_q18_tables(self, seed: int, cust_rows: int, order_rows: int) -> dict:
+        """Customers, orders and lineitem for Q18 in the reference layout:
+        lineitem sorted by order key with 1..7 rows per order (the chain
+        generator), quantity U{1..50} x 100 (i16), order date i16, total price
+        U{1000.00..500000.00} in cents (i32); keys 1..N."""
+        torch = _torch()
+        t = self.gen_chain(seed, cust_rows, order_rows)
+        n = int(t["fact_order"].numel())
+        qty = self.gen_uniform_i32(seed, 0x700, n, 1, 50)
+        price = self.gen_uniform_i32(seed, 0x701, order_rows, 100_000, 50_000_000)
+        return {"l_orderkey": t["fact_order"], "l_qty": (qty * 100).to(torch.int16),
+                "o_orderkey": t["order_key"], "o_custkey": t["order_cust"],
+                "o_orderdate": t["order_day"].to(torch.int16), "o_totalprice": price,
+                "c_custkey": t["cust_key"]}
 
     def gen_chain(self, seed: int, cust_rows: int, order_rows: int) -> dict:
         """Customers, orders and their clustered fact rows (1..7 per order)."""

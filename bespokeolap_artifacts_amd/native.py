@@ -149,6 +149,51 @@ class GenLineitemCols(C.Structure):
                 ("flag_a", vp), ("flag_b", vp)]
 
 
+ZONE_ROWS = 2048
+
+
+class Zone(C.Structure):
+    _fields_ = [("ship_min", C.c_int16), ("ship_max", C.c_int16), ("qty_min", C.c_int16),
+                ("qty_max", C.c_int16), ("disc_min", C.c_uint8), ("disc_max", C.c_uint8),
+                ("reserved", C.c_uint8 * 6)]
+
+
+class Q6Args(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("ship", vp), ("disc", vp), ("qty", vp), ("price", vp),
+                ("ship_lo", C.c_int32), ("ship_hi", C.c_int32), ("disc_lo", C.c_int32),
+                ("disc_hi", C.c_int32), ("qty_lt", C.c_int32), ("zones", vp)]
+
+
+class Q6Result(C.Structure):
+    _fields_ = [("revenue", C.c_int64), ("rows_emitted", C.c_int64),
+                ("rows_scanned", C.c_int64)]
+
+
+class Q6Params(C.Structure):
+    _fields_ = [("ship_lo", C.c_int32), ("ship_hi", C.c_int32), ("disc_lo", C.c_int32),
+                ("disc_hi", C.c_int32), ("qty_lt", C.c_int32), ("month_lo", C.c_int32),
+                ("month_hi", C.c_int32)]
+
+
+class Q18Args(C.Structure):
+    _fields_ = [("li_rows", C.c_int64), ("l_orderkey", vp), ("l_quantity", vp),
+                ("orderkey_sorted", C.c_int32), ("order_rows", C.c_int64), ("o_orderkey", vp),
+                ("o_custkey", vp), ("o_orderdate", vp), ("o_totalprice", vp),
+                ("cust_rows", C.c_int64), ("c_custkey", vp),
+                ("quantity_threshold", C.c_int32)]
+
+
+class Q18Row(C.Structure):
+    _fields_ = [("custkey", C.c_int32), ("orderkey", C.c_int32), ("orderdate", C.c_int32),
+                ("totalprice", C.c_int32), ("sum_qty", C.c_int64), ("cust_row", C.c_int32),
+                ("order_row", C.c_int32)]
+
+
+class Q18Stats(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("dense_path", C.c_int32), ("sorted_pass", C.c_int32),
+                ("fallbacks", C.c_int32), ("reserved", C.c_int32)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -194,6 +239,14 @@ PROTOTYPES = {
     "bolap_mlp_predict": (C.c_int, [vp, vp, i64, vp]),
     "bolap_mlp_train_step": (C.c_int, [vp, vp, vp, i64, C.c_float, P(C.c_float)]),
     "bolap_mlp_gradients": (C.c_int, [vp, vp, vp, i64, P(C.c_float), P(C.c_float)]),
+    "bolap_q6_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P(Q6Params)]),
+    "bolap_q6": (C.c_int, [vp, P(Q6Args), P(Q6Result)]),
+    "bolap_q6_zones": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+    "bolap_q6_cluster": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bolap_q18": (C.c_int, [vp, P(Q18Args), P(Q18Row), i64, P(Q18Stats)]),
+    "bolap_derive_lineitem": (C.c_int, [vp, i64, vp, vp, vp, vp, i32, vp, vp, vp, P(i32)]),
+    "bolap_date_order": (C.c_int, [vp, i64, vp, vp]),
+    "bolap_order_ranges": (C.c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp]),
     "bolap_gen_lineitem": (C.c_int, [vp, u64, i64, P(GenLineitemCols)]),
     "bolap_gen_dim": (C.c_int, [vp, u64, i64, vp, vp, vp]),
     "bolap_gen_fact": (C.c_int, [vp, u64, i64, i64, vp, vp, vp]),

@@ -464,6 +464,133 @@ int bolap_mlp_gradients(bolap_mlp* mlp, const float* x_dev, const float* t_dev, 
                         float* grads_host, float* loss_out);
 
 /* ------------------------------------------------------------------------ */
+/* Q6 shape: three-predicate range scan + SUM(extendedprice * discount) over  */
+/* the reference layout.  Replaces run_q6's scan (bespoke_tpch/query_q6.cpp:  */
+/* 155-397): the flat loop (:197-223), or, with a zone map, the shard-pruned  */
+/* loops (:224-379) over columns clustered by the shard key.                  */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_ZONE_ROWS 2048
+
+/* min/max of the Q6 predicate columns over BOLAP_ZONE_ROWS consecutive rows
+   (the last zone may be shorter); 16 bytes. */
+typedef struct {
+    int16_t ship_min, ship_max;
+    int16_t qty_min, qty_max;
+    uint8_t disc_min, disc_max;
+    uint8_t reserved[6];
+} bolap_zone;
+
+typedef struct {
+    int64_t rows;
+    const int16_t* ship;       /* shipdate, days since 1992-01-01 */
+    const uint8_t* disc;       /* discount in hundredths (8-byte aligned) */
+    const int16_t* qty;        /* quantity x 100 */
+    const int32_t* price;      /* extendedprice in cents */
+    int32_t ship_lo, ship_hi;  /* a row passes when ship_lo <= ship < ship_hi, */
+    int32_t disc_lo, disc_hi;  /* disc_lo <= disc <= disc_hi                    */
+    int32_t qty_lt;            /* and qty < qty_lt                              */
+    const bolap_zone* zones;   /* null: flat scan; else ceil(rows / ZONE_ROWS)
+                                  zones of these columns (bolap_q6_zones) */
+} bolap_q6_args;
+
+typedef struct {
+    int64_t revenue;       /* SUM(extendedprice * discount), exact int64 */
+    int64_t rows_emitted;  /* rows that passed */
+    int64_t rows_scanned;  /* rows of the blocks that were read */
+} bolap_q6_result;
+
+/* Q6 arguments from the query's strings as run_q6 derives them
+   (query_q6.cpp:171-190, 92-95): [DATE, DATE + 1 year) as day offsets with
+   the day clamped to the month; DISCOUNT +- 0.01 in hundredths; QUANTITY
+   x 100; month_lo/hi = the shard month-index window (year * 12 + month - 1).
+   Host only. */
+typedef struct {
+    int32_t ship_lo, ship_hi, disc_lo, disc_hi, qty_lt, month_lo, month_hi;
+} bolap_q6_params;
+
+int bolap_q6_parse(const char* date, const char* discount, const char* quantity,
+                   bolap_q6_params* out);
+int bolap_q6(bolap_ctx* ctx, const bolap_q6_args* args, bolap_q6_result* out);
+/* Zone map of (possibly clustered) columns: zones_out holds
+   ceil(rows / BOLAP_ZONE_ROWS) entries. */
+int bolap_q6_zones(bolap_ctx* ctx, int64_t rows, const int16_t* ship, const uint8_t* disc,
+                   const int16_t* qty, bolap_zone* zones_out);
+/* Copies of the Q6 columns ordered by the reference's shard key (ship month,
+   discount, quantity / 100; builder_impl.cpp:855-911), stable within a key;
+   perm_out (optional) receives the source row of each output row. */
+int bolap_q6_cluster(bolap_ctx* ctx, int64_t rows, const int16_t* ship, const uint8_t* disc,
+                     const int16_t* qty, const int32_t* price, int16_t* ship_out,
+                     uint8_t* disc_out, int16_t* qty_out, int32_t* price_out,
+                     uint32_t* perm_out);
+
+/* ------------------------------------------------------------------------ */
+/* Q18 shape: SUM(quantity) per order key into a dense array, orders above a  */
+/* threshold, joined to orders and customers, ordered by totalprice desc,     */
+/* orderdate asc, orderkey asc.  Replaces run_q18 (bespoke_tpch/query_q18.cpp:*/
+/* 183-455): the dense-key path when orders/customers carry keys 1..N (first  */
+/* and last key checked, :194-199), else the max-key-sized path.  An order    */
+/* qualifies when its total exceeds the threshold; the reference's dense path */
+/* tests running sums, which is the same for non-negative quantities.         */
+/* Remaining ties (duplicate order keys) are broken by order row.             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t li_rows;
+    const int32_t* l_orderkey;
+    const int16_t* l_quantity;      /* x 100 */
+    int32_t orderkey_sorted;        /* lineitem sorted by order key: run sums
+                                       without atomics (checked on the device;
+                                       out of order reruns with atomics) */
+    int64_t order_rows;
+    const int32_t* o_orderkey;
+    const int32_t* o_custkey;
+    const int16_t* o_orderdate;
+    const int32_t* o_totalprice;    /* cents */
+    int64_t cust_rows;
+    const int32_t* c_custkey;
+    int32_t quantity_threshold;     /* x 100 */
+} bolap_q18_args;
+
+typedef struct {
+    int32_t custkey, orderkey, orderdate, totalprice;
+    int64_t sum_qty;
+    int32_t cust_row;   /* customer row (for c_name on the host) */
+    int32_t order_row;
+} bolap_q18_row;
+
+typedef struct {
+    int64_t rows;         /* result rows */
+    int32_t dense_path;   /* 1: dense order keys */
+    int32_t sorted_pass;  /* 1: run sums without atomics */
+    int32_t fallbacks;    /* 1: sorted hint was wrong, rerun with atomics */
+    int32_t reserved;
+} bolap_q18_stats;
+
+/* rows_out (host) holds max_rows rows; more result rows is an error
+   (stats->rows says how many). */
+int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* args, bolap_q18_row* rows_out,
+              int64_t max_rows, bolap_q18_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Build-time derivations of the reference Database layout on the device.    */
+/* ------------------------------------------------------------------------ */
+/* discounted_price = extendedprice * (100 - discount) / 100 truncated, and
+   returnflag_linestatus = rf * n_ls + ls (builder_impl.cpp:523-532, 552-567);
+   *orderkey_sorted = keys non-decreasing (:569-577; 1 when orderkey is null). */
+int bolap_derive_lineitem(bolap_ctx* ctx, int64_t rows, const int32_t* price,
+                          const uint8_t* disc, const uint8_t* rf, const uint8_t* ls,
+                          int32_t n_ls, const int32_t* orderkey, int32_t* disc_price_out,
+                          uint8_t* code_out, int32_t* orderkey_sorted);
+/* order_out[k] = row placed k-th by a stable sort on the date
+   (build_date_sorted_indices, builder_impl.cpp:374-405). */
+int bolap_date_order(bolap_ctx* ctx, int64_t rows, const int16_t* date, uint32_t* order_out);
+/* map_out[key] = order row (-1 absent, last row wins; map_size = max key + 1)
+   and each order's lineitem rows [start, end), {0, 0} when it has none
+   (builder_impl.cpp:610-630, 929-991). */
+int bolap_order_ranges(bolap_ctx* ctx, int64_t o_rows, const int32_t* o_orderkey,
+                       int64_t map_size, int32_t* map_out, int64_t li_rows,
+                       const int32_t* l_orderkey, uint32_t* start_out, uint32_t* end_out);
+
+/* ------------------------------------------------------------------------ */
 /* Seeded synthetic tables (counter-based SplitMix64, order independent).     */
 /* Every output is a caller-allocated device array.                           */
 /* ------------------------------------------------------------------------ */

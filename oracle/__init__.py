@@ -441,3 +441,120 @@ def ceb_run(t: dict, q: dict, masks: dict) -> dict:
     L.oracle_ceb_run(C.byref(db), C.byref(cq), _p(out))
     return {"count": (int(out[1]) << 64) | int(out[0]), "min_year": int(out[2].view(np.int64)),
             "title_pass": int(out[3]), "joined_movies": int(out[4])}
+
+
+# --------------------------------------------- TPC-H Q6 / Q18, build (tpch.c)
+class Q6Out(C.Structure):
+    _fields_ = [("revenue", i64), ("rows_emitted", i64), ("rows_scanned", i64)]
+
+
+class Q18Row(C.Structure):
+    _fields_ = [("custkey", i32), ("orderkey", i32), ("orderdate", i32), ("totalprice", i32),
+                ("sum_qty", i64), ("cust_row", i32), ("order_row", i32)]
+
+
+Q18_DTYPE = np.dtype([("custkey", "<i4"), ("orderkey", "<i4"), ("orderdate", "<i4"),
+                      ("totalprice", "<i4"), ("sum_qty", "<i8"), ("cust_row", "<i4"),
+                      ("order_row", "<i4")])
+
+
+def _tpch_lib():
+    L = lib()
+    if not getattr(L, "_tpch_ready", False):
+        L.oracle_days_from_civil.restype = i64
+        L.oracle_days_from_civil.argtypes = [i32, i32, i32]
+        L.oracle_civil_from_days.argtypes = [i32, vp, vp, vp]
+        L.oracle_q6_flat.argtypes = [i64, vp, vp, vp, vp, i32, i32, i32, i32, i32,
+                                     C.POINTER(Q6Out)]
+        L.oracle_q6_sharded.argtypes = [i64, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32,
+                                        C.POINTER(Q6Out)]
+        L.oracle_q18.restype = i64
+        L.oracle_q18.argtypes = [i64, vp, vp, i64, vp, vp, vp, vp, i64, vp, i32, vp, i64, vp]
+        L.oracle_derive_lineitem.restype = i32
+        L.oracle_derive_lineitem.argtypes = [i64, vp, vp, vp, vp, i32, vp, vp, vp]
+        L.oracle_date_order.argtypes = [i64, vp, vp]
+        L.oracle_order_ranges.argtypes = [i64, vp, i64, vp, i64, vp, i32, vp, vp]
+        L._tpch_ready = True
+    return L
+
+
+def days_from_civil(y: int, m: int, d: int) -> int:
+    return int(_tpch_lib().oracle_days_from_civil(y, m, d))
+
+
+def civil_from_days(z: int) -> tuple[int, int, int]:
+    y, m, d = C.c_int32(), C.c_int32(), C.c_int32()
+    _tpch_lib().oracle_civil_from_days(z, C.byref(y), C.byref(m), C.byref(d))
+    return y.value, m.value, d.value
+
+
+def _q6_cols(c: dict):
+    cols = [np.ascontiguousarray(c[k], dt) for k, dt in
+            (("ship", np.int16), ("disc", np.uint8), ("qty", np.int16), ("price", np.int32))]
+    return cols, [_p(x) for x in cols]
+
+
+def q6_flat(c: dict, p: dict) -> dict:
+    cols, ptrs = _q6_cols(c)
+    out = Q6Out()
+    _tpch_lib().oracle_q6_flat(len(cols[0]), *ptrs, p["ship_lo"], p["ship_hi"], p["disc_lo"],
+                               p["disc_hi"], p["qty_lt"], C.byref(out))
+    return {"revenue": out.revenue, "rows_emitted": out.rows_emitted,
+            "rows_scanned": out.rows_scanned}
+
+
+def q6_sharded(c: dict, p: dict) -> dict:
+    cols, ptrs = _q6_cols(c)
+    out = Q6Out()
+    _tpch_lib().oracle_q6_sharded(len(cols[0]), *ptrs, p["ship_lo"], p["ship_hi"],
+                                  p["disc_lo"], p["disc_hi"], p["qty_lt"], p["month_lo"],
+                                  p["month_hi"], C.byref(out))
+    return {"revenue": out.revenue, "rows_emitted": out.rows_emitted,
+            "rows_scanned": out.rows_scanned}
+
+
+def q18(t: dict, threshold: int, max_out: int = 1 << 20):
+    """Rows (structured array, Q18_DTYPE) in output order and the dense flag."""
+    a = {k: np.ascontiguousarray(t[k], dt) for k, dt in
+         (("l_orderkey", np.int32), ("l_qty", np.int16), ("o_orderkey", np.int32),
+          ("o_custkey", np.int32), ("o_orderdate", np.int16), ("o_totalprice", np.int32),
+          ("c_custkey", np.int32))}
+    out = np.zeros(max(1, max_out), Q18_DTYPE)
+    dense = C.c_int32()
+    n = _tpch_lib().oracle_q18(len(a["l_orderkey"]), _p(a["l_orderkey"]), _p(a["l_qty"]),
+                               len(a["o_orderkey"]), _p(a["o_orderkey"]), _p(a["o_custkey"]),
+                               _p(a["o_orderdate"]), _p(a["o_totalprice"]),
+                               len(a["c_custkey"]), _p(a["c_custkey"]), threshold, _p(out),
+                               max_out, C.byref(dense))
+    assert n <= max_out, (n, max_out)
+    return out[:n], bool(dense.value)
+
+
+def derive_lineitem(price, disc, rf, ls, n_ls: int, orderkey):
+    price = np.ascontiguousarray(price, np.int32)
+    disc, rf, ls = (np.ascontiguousarray(x, np.uint8) for x in (disc, rf, ls))
+    orderkey = np.ascontiguousarray(orderkey, np.int32)
+    dp = np.empty(len(price), np.int32)
+    code = np.empty(len(price), np.uint8)
+    s = _tpch_lib().oracle_derive_lineitem(len(price), _p(price), _p(disc), _p(rf), _p(ls),
+                                           n_ls, _p(orderkey), _p(dp), _p(code))
+    return dp, code, bool(s)
+
+
+def date_order(date) -> np.ndarray:
+    date = np.ascontiguousarray(date, np.int16)
+    out = np.empty(max(1, len(date)), np.uint32)
+    _tpch_lib().oracle_date_order(len(date), _p(date), _p(out))
+    return out[:len(date)]
+
+
+def order_ranges(o_orderkey, l_orderkey, li_sorted: bool):
+    ok = np.ascontiguousarray(o_orderkey, np.int32)
+    lk = np.ascontiguousarray(l_orderkey, np.int32)
+    map_size = int(ok.max()) + 1 if len(ok) and ok.max() >= 0 else 0
+    m = np.empty(max(1, map_size), np.int32)
+    start = np.empty(max(1, len(ok)), np.uint32)
+    end = np.empty(max(1, len(ok)), np.uint32)
+    _tpch_lib().oracle_order_ranges(len(ok), _p(ok), map_size, _p(m), len(lk), _p(lk),
+                                    int(li_sorted), _p(start), _p(end))
+    return m[:map_size], start[:len(ok)], end[:len(ok)]

@@ -160,6 +160,42 @@ typedef struct {
 /* out: count_lo, count_hi, min_year, title_pass, joined_movies */
 void oracle_ceb_run(const oracle_ceb_db* db, const oracle_ceb_query* q, uint64_t* out);
 
+/* --- TPC-H Q6 / Q18 and build-time derivations (tpch.c) ----------------- */
+int64_t oracle_days_from_civil(int32_t y, int32_t m, int32_t d);
+void oracle_civil_from_days(int32_t z, int32_t* y, int32_t* m, int32_t* d);
+
+typedef struct {
+    int64_t revenue, rows_emitted, rows_scanned;
+} oracle_q6_out;
+
+void oracle_q6_flat(int64_t rows, const int16_t* ship, const uint8_t* disc, const int16_t* qty,
+                    const int32_t* price, int32_t ship_lo, int32_t ship_hi, int32_t disc_lo,
+                    int32_t disc_hi, int32_t qty_lt, oracle_q6_out* out);
+void oracle_q6_sharded(int64_t rows, const int16_t* ship, const uint8_t* disc,
+                       const int16_t* qty, const int32_t* price, int32_t ship_lo,
+                       int32_t ship_hi, int32_t disc_lo, int32_t disc_hi, int32_t qty_lt,
+                       int32_t month_lo, int32_t month_hi, oracle_q6_out* out);
+
+typedef struct {
+    int32_t custkey, orderkey, orderdate, totalprice;
+    int64_t sum_qty;
+    int32_t cust_row, order_row;
+} oracle_q18_row;
+
+int64_t oracle_q18(int64_t li_rows, const int32_t* l_orderkey, const int16_t* l_qty,
+                   int64_t o_rows, const int32_t* o_orderkey, const int32_t* o_custkey,
+                   const int16_t* o_orderdate, const int32_t* o_totalprice, int64_t c_rows,
+                   const int32_t* c_custkey, int32_t threshold, oracle_q18_row* out,
+                   int64_t max_out, int32_t* dense_path);
+
+int32_t oracle_derive_lineitem(int64_t rows, const int32_t* price, const uint8_t* disc,
+                               const uint8_t* rf, const uint8_t* ls, int32_t n_ls,
+                               const int32_t* orderkey, int32_t* disc_price, uint8_t* code);
+void oracle_date_order(int64_t rows, const int16_t* date, uint32_t* order);
+void oracle_order_ranges(int64_t o_rows, const int32_t* o_orderkey, int64_t map_size,
+                         int32_t* map, int64_t li_rows, const int32_t* l_orderkey,
+                         int32_t li_sorted, uint32_t* start, uint32_t* end);
+
 /* --- planted mistakes, for tests that show the checks have teeth -------- */
 void oracle_set_mutation(int32_t m);
 int32_t oracle_mutation(void);

@@ -1,0 +1,255 @@
+"""CPU checks of the Q6 / Q18 / build restatements in oracle/tpch.c and of the
+host-side Q6 argument parser in libbolap.
+
+Each oracle function is compared with an independent formulation (numpy
+masks, pandas group-by/merge, Python datetime, numpy's stable argsort), and
+planted mistakes in the oracle must be caught by those comparisons.
+"""
+import datetime
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import oracle
+
+EPOCH = datetime.date(1970, 1, 1)
+BASE = datetime.date(1992, 1, 1)
+
+
+def offset(d: datetime.date) -> int:
+    return (d - BASE).days
+
+
+# ------------------------------------------------------------------ dates
+def test_civil_dates_match_python_datetime():
+    for d in (datetime.date(1992, 1, 1), datetime.date(1996, 2, 29), datetime.date(2000, 3, 1),
+              datetime.date(1899, 12, 31), datetime.date(2100, 2, 28), datetime.date(1970, 1, 1)):
+        z = (d - EPOCH).days
+        assert oracle.days_from_civil(d.year, d.month, d.day) == z
+        assert oracle.civil_from_days(z) == (d.year, d.month, d.day)
+    rng = np.random.default_rng(3)
+    for z in rng.integers(-200_000, 200_000, 2000):
+        d = EPOCH + datetime.timedelta(days=int(z))
+        assert oracle.civil_from_days(int(z)) == (d.year, d.month, d.day)
+
+
+def add_year_clamped(d: datetime.date) -> datetime.date:
+    try:
+        return d.replace(year=d.year + 1)
+    except ValueError:  # Feb 29 -> Feb 28
+        return d.replace(year=d.year + 1, day=28)
+
+
+@pytest.mark.parametrize("date", ["1994-01-01", "1996-02-29", "1997-12-31", "1992-03-15",
+                                  "1998-01-01"])
+@pytest.mark.parametrize("disc,qty", [("0.06", "24"), ("0.02", "25"), ("0.00", "24.5"),
+                                      ("0.10", "1")])
+def test_q6_parse_follows_the_reference(date, disc, qty):
+    from bespokeolap_artifacts_amd.engine import Engine
+    p = Engine.q6_params(date, disc, qty)
+    d = datetime.date.fromisoformat(date)
+    e = add_year_clamped(d)
+    assert p["ship_lo"] == offset(d) and p["ship_hi"] == offset(e)
+    raw = round(float(disc) * 100)
+    assert (p["disc_lo"], p["disc_hi"]) == (raw - 1, raw + 1)
+    assert p["qty_lt"] == round(float(qty) * 100)
+    assert p["month_lo"] == d.year * 12 + d.month - 1
+    want_hi = e.year * 12 + e.month - 1 - (1 if e.day == 1 else 0)
+    assert p["month_hi"] == max(want_hi, p["month_lo"])
+
+
+def test_q6_parse_rejects_malformed_dates():
+    from bespokeolap_artifacts_amd import native as N
+    from bespokeolap_artifacts_amd.engine import Engine
+    for bad in ("1994-1-01", "19940101", "1994-13-01", "abcd-ef-gh"):
+        with pytest.raises(N.BolapError):
+            Engine.q6_params(bad, "0.06", "24")
+
+
+# --------------------------------------------------------------------- Q6
+def q6_table(seed: int, rows: int, frac_qty: bool = False) -> dict:
+    rng = np.random.default_rng(seed)
+    qty = rng.integers(1, 5001, rows) if frac_qty else rng.integers(1, 51, rows) * 100
+    if frac_qty:
+        qty[::97] = 2450  # exactly at the fractional limit below
+    return {"ship": rng.integers(0, 2526, rows).astype(np.int16),
+            "disc": rng.integers(0, 11, rows).astype(np.uint8),
+            "qty": qty.astype(np.int16),
+            "price": rng.integers(90_000, 100_000 * 50, rows).astype(np.int32)}
+
+
+def q6_numpy(c: dict, p: dict) -> dict:
+    s = c["ship"].astype(np.int64)
+    d = c["disc"].astype(np.int64)
+    m = (s >= p["ship_lo"]) & (s < p["ship_hi"]) & (d >= p["disc_lo"]) & (d <= p["disc_hi"]) & \
+        (c["qty"].astype(np.int64) < p["qty_lt"])
+    return {"revenue": int((c["price"].astype(np.int64)[m] * d[m]).sum()),
+            "rows_emitted": int(m.sum())}
+
+
+def params(date, disc, qty):
+    from bespokeolap_artifacts_amd.engine import Engine
+    return Engine.q6_params(date, disc, qty)
+
+
+CASES = [("1994-01-01", "0.06", "24"), ("1996-02-29", "0.00", "24.5"),
+         ("1992-01-01", "0.10", "50"), ("1998-06-15", "0.05", "1")]
+
+
+@pytest.mark.parametrize("rows", [0, 1, 1000, 200_003])
+@pytest.mark.parametrize("case", CASES)
+def test_q6_flat_and_sharded_match_numpy(rows, case):
+    c = q6_table(rows + 7, rows, frac_qty=case[2] == "24.5")
+    p = params(*case)
+    want = q6_numpy(c, p)
+    flat = oracle.q6_flat(c, p)
+    assert {k: flat[k] for k in want} == want
+    sh = oracle.q6_sharded(c, p)
+    assert {k: sh[k] for k in want} == want
+    assert sh["rows_scanned"] <= rows
+
+
+def test_q6_sharding_prunes():
+    c = q6_table(1, 300_000)
+    sh = oracle.q6_sharded(c, params("1994-01-01", "0.06", "24"))
+    # one year of ~7, 3 of 11 discounts, 23 of 50 quantities
+    assert sh["rows_scanned"] < 0.04 * 300_000
+
+
+def test_q6_planted_sharding_mistake_is_caught():
+    c = q6_table(5, 100_000, frac_qty=True)
+    p = params("1996-02-29", "0.00", "24.5")
+    want = q6_numpy(c, p)
+    oracle.set_mutation(9)
+    try:
+        got = oracle.q6_sharded(c, p)
+    finally:
+        oracle.set_mutation(0)
+    assert got["revenue"] != want["revenue"]
+
+
+# -------------------------------------------------------------------- Q18
+def q18_tables(seed: int, n_orders: int, n_cust: int, dense: bool, sorted_li: bool,
+               dup_cust: bool = False) -> dict:
+    rng = np.random.default_rng(seed)
+    okeys = np.arange(1, n_orders + 1, dtype=np.int32)
+    per = rng.integers(1, 8, n_orders)
+    lkeys = np.repeat(okeys, per)
+    lqty = (rng.integers(1, 51, len(lkeys)) * 100).astype(np.int16)
+    if not sorted_li:
+        perm = rng.permutation(len(lkeys))
+        lkeys, lqty = lkeys[perm], lqty[perm]
+    ocust = rng.integers(1, n_cust + 3, n_orders).astype(np.int32)  # some have no customer
+    odate = rng.integers(0, 2406, n_orders).astype(np.int16)
+    oprice = rng.integers(1000, 5000, n_orders).astype(np.int32)  # many ties
+    if not dense:
+        order = np.argsort(odate, kind="stable")
+        okeys, ocust, odate, oprice = okeys[order], ocust[order], odate[order], oprice[order]
+    ckeys = np.arange(1, n_cust + 1, dtype=np.int32)
+    if dup_cust:
+        ckeys = rng.permutation(ckeys)
+        ckeys[::5] = ckeys[1::5][: len(ckeys[::5])] if n_cust > 5 else ckeys[::5]
+    return {"l_orderkey": lkeys.astype(np.int32), "l_qty": lqty, "o_orderkey": okeys,
+            "o_custkey": ocust, "o_orderdate": odate, "o_totalprice": oprice,
+            "c_custkey": ckeys}
+
+
+def q18_pandas(t: dict, threshold: int) -> pd.DataFrame:
+    li = pd.DataFrame({"k": t["l_orderkey"], "q": t["l_qty"].astype(np.int64)})
+    sums = li.groupby("k")["q"].sum()
+    sums = sums[sums > threshold]
+    o = pd.DataFrame({"orderkey": t["o_orderkey"], "custkey": t["o_custkey"],
+                      "orderdate": t["o_orderdate"].astype(np.int32),
+                      "totalprice": t["o_totalprice"], "order_row": np.arange(len(t["o_orderkey"]))})
+    o = o[o["orderkey"].isin(sums.index)].copy()
+    o["sum_qty"] = o["orderkey"].map(sums).astype(np.int64)
+    c = pd.DataFrame({"custkey": t["c_custkey"], "cust_row": np.arange(len(t["c_custkey"]))})
+    c = c.groupby("custkey")["cust_row"].max().reset_index()  # last row wins
+    m = o.merge(c, on="custkey", how="inner")
+    m = m.sort_values(["totalprice", "orderdate", "orderkey", "order_row"],
+                      ascending=[False, True, True, True], kind="stable")
+    return m[["custkey", "orderkey", "orderdate", "totalprice", "sum_qty", "cust_row",
+              "order_row"]].reset_index(drop=True)
+
+
+def as_frame(rows: np.ndarray) -> pd.DataFrame:
+    return pd.DataFrame({k: rows[k].astype(np.int64) for k in rows.dtype.names})
+
+
+@pytest.mark.parametrize("dense", [True, False])
+@pytest.mark.parametrize("sorted_li", [True, False])
+@pytest.mark.parametrize("threshold", [-1, 15_000, 20_000, 40_000])
+def test_q18_matches_pandas(dense, sorted_li, threshold):
+    t = q18_tables(11, 3000, 400, dense, sorted_li)
+    rows, is_dense = oracle.q18(t, threshold)
+    assert is_dense == dense
+    want = q18_pandas(t, threshold)
+    pd.testing.assert_frame_equal(as_frame(rows), want.astype(np.int64))
+
+
+def test_q18_duplicate_customer_keys_last_row_wins():
+    t = q18_tables(4, 2000, 300, dense=False, sorted_li=True, dup_cust=True)
+    rows, _ = oracle.q18(t, 10_000)
+    pd.testing.assert_frame_equal(as_frame(rows), q18_pandas(t, 10_000).astype(np.int64))
+
+
+@pytest.mark.parametrize("m", [10, 11])
+def test_q18_planted_mistakes_are_caught(m):
+    t = q18_tables(2, 3000, 400, dense=True, sorted_li=True)
+    # a threshold some order sum equals exactly
+    li = pd.Series(t["l_qty"].astype(np.int64)).groupby(t["l_orderkey"]).sum()
+    threshold = int(li.iloc[7])
+    want = q18_pandas(t, threshold)
+    oracle.set_mutation(m)
+    try:
+        rows, _ = oracle.q18(t, threshold)
+    finally:
+        oracle.set_mutation(0)
+    assert not as_frame(rows).equals(want.astype(np.int64))
+
+
+# ------------------------------------------------------------------ build
+def test_derive_lineitem_matches_numpy():
+    rng = np.random.default_rng(9)
+    n = 50_000
+    price = rng.integers(-2_000_000, 10_000_000, n).astype(np.int32)
+    disc = rng.integers(0, 11, n).astype(np.uint8)
+    rf = rng.integers(0, 3, n).astype(np.uint8)
+    ls = rng.integers(0, 2, n).astype(np.uint8)
+    keys = np.sort(rng.integers(0, 10_000, n)).astype(np.int32)
+    dp, code, s = oracle.derive_lineitem(price, disc, rf, ls, 2, keys)
+    prod = price.astype(np.int64) * (100 - disc.astype(np.int64))
+    want = np.sign(prod) * (np.abs(prod) // 100)  # C division truncates toward zero
+    assert np.array_equal(dp, want.astype(np.int32))
+    assert np.array_equal(code, (rf.astype(np.int32) * 2 + ls).astype(np.uint8))
+    assert s
+    keys[100], keys[101] = keys[101] + 1, keys[100]
+    assert not oracle.derive_lineitem(price, disc, rf, ls, 2, keys)[2]
+
+
+def test_date_order_is_a_stable_sort():
+    rng = np.random.default_rng(1)
+    d = rng.integers(-50, 2500, 100_001).astype(np.int16)
+    assert np.array_equal(oracle.date_order(d), np.argsort(d, kind="stable").astype(np.uint32))
+
+
+@pytest.mark.parametrize("sorted_li", [True, False])
+def test_order_ranges_match_python(sorted_li):
+    rng = np.random.default_rng(5)
+    ok = rng.permutation(np.arange(0, 3000, dtype=np.int32) * 3)[:2500]
+    lk = np.sort(rng.choice(ok, 9000)).astype(np.int32)
+    if not sorted_li:
+        lk = rng.permutation(lk)
+    m, start, end = oracle.order_ranges(ok, lk, sorted_li)
+    first, last = {}, {}
+    for i, k in enumerate(lk.tolist()):
+        first.setdefault(k, i)
+        last[k] = i
+    for r, k in enumerate(ok.tolist()):
+        assert m[k] == r
+        if k in first:
+            assert (start[r], end[r]) == (first[k], last[k] + 1)
+        else:
+            assert (start[r], end[r]) == (0, 0)
+    assert (m >= 0).sum() == len(ok)
