@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workloads", default="c2,c1,c3,c4,c5,copy")
+    ap.add_argument("--workloads", default="c2,c1,c3,q6,q18,c4,c5,copy")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -176,6 +176,87 @@ def bench_c2(eng, torch, args):
                                "kind": "port", "ms": med * 1e3,
                                "sample": f"full 60,000,000-row fact + 2,000,000-row dimension, "
                                          f"median of {reps} runs"}
+    return res
+
+
+# ------------------------------------------------------------------- Q6
+def bench_q6(eng, torch, args):
+    """TPC-H Q6 over 60,000,000 lineitem rows in the reference layout (ship
+    i16, discount u8, quantity i16 x 100, price i32: 9 B/row): the flat scan,
+    and the zone-map scan over the same rows clustered by the shard key."""
+    rows = 60_000_000
+    c = eng.gen_q6_lineitem(SEED, rows)
+    p = eng.q6_params("1994-01-01", "0.06", "24")
+    a_flat = eng.q6_args(c, p)
+    first = eng.q6_run(a_flat)
+    ms = time_steps(torch, lambda: eng.q6_run(a_flat), args.steps, args.warmup)
+    cl = eng.q6_cluster(c)
+    zones = eng.q6_zones(cl)
+    a_zone = eng.q6_args(cl, p, zones)
+    zres = eng.q6_run(a_zone)
+    ms_zone = time_steps(torch, lambda: eng.q6_run(a_zone), args.steps, args.warmup)
+    eng.set_trace(True)
+    for _ in range(args.steps):
+        eng.q6_run(a_flat)
+        eng.q6_run(a_zone)
+    torch.cuda.synchronize()
+    tr = eng.read_trace()
+    eng.set_trace(False)
+    alg = rows * 9
+    k_ms = stage_ms(tr, "q6_flat")
+    res = {"workload": "Q6 filter-scan, 60,000,000 lineitem rows (reference layout, 9 B/row)",
+           "ms_per_step": ms, "GBps": alg / (ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+           "rows_per_s": rows / (ms * 1e-3), "kernel": "q6_flat_k",
+           "kernel_ms": k_ms, "kernel_GBps": alg / (k_ms * 1e-3) / 1e9 if k_ms else None,
+           "zoned_ms_per_step": ms_zone, "zoned_kernel_ms": stage_ms(tr, "q6_zoned"),
+           "zoned_rows_scanned": zres["rows_scanned"], "rows_emitted": first["rows_emitted"],
+           "consistent": zres["revenue"] == first["revenue"]}
+    if not args.no_cpu:
+        import oracle
+        h = {k: v.cpu().numpy() for k, v in c.items()}
+        want = oracle.q6_flat(h, p)
+        res["parity"] = want == first
+        med, reps = cpu_time(lambda: oracle.q6_flat(h, p), args.cpu_seconds)
+        res["cpu_baseline"] = {"value": alg / med / 1e9, "unit": "GB/s", "cores": 1,
+                               "kind": "port", "ms": med * 1e3,
+                               "sample": f"full table, flat loop, median of {reps} runs"}
+    return res
+
+
+# ------------------------------------------------------------------ Q18
+def bench_q18(eng, torch, args):
+    """TPC-H Q18: SUM(quantity) per order over 60M lineitem rows sorted by
+    order key (1-7 per order), 15M orders, 1.5M customers, threshold 300."""
+    t = eng.gen_q18_tables(SEED, 1_500_000, 15_000_000)
+    n = int(t["l_orderkey"].numel())
+    a = eng.q18_args(t, 30_000)
+    first, st = eng.q18_run(a)
+    ms = time_steps(torch, lambda: eng.q18_run(a), args.steps, args.warmup)
+    eng.set_trace(True)
+    for _ in range(args.steps):
+        eng.q18_run(a)
+    torch.cuda.synchronize()
+    tr = eng.read_trace()
+    eng.set_trace(False)
+    alg = n * 6
+    k_ms = stage_ms(tr, "q18_sum")
+    res = {"workload": f"Q18 group-by + threshold, {n:,} lineitem rows, 15,000,000 orders, "
+                       "1,500,000 customers",
+           "ms_per_step": ms, "GBps": alg / (ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+           "rows_per_s": n / (ms * 1e-3), "kernel": "q18_sum_sorted_k", "sum_stage_ms": k_ms,
+           "sum_stage_GBps": alg / (k_ms * 1e-3) / 1e9 if k_ms else None,
+           "stages_ms": {k: stage_ms(tr, k) for k in ("q18_bounds", "q18_sum", "q18_select",
+                                                      "q18_sort")},
+           "result_rows": int(st["rows"]), "sorted_pass": st["sorted_pass"]}
+    if not args.no_cpu:
+        import oracle
+        h = {k: v.cpu().numpy() for k, v in t.items()}
+        want, _ = oracle.q18(h, 30_000)
+        res["parity"] = [tuple(r) for r in want.tolist()] == [tuple(r) for r in first.tolist()]
+        med, reps = cpu_time(lambda: oracle.q18(h, 30_000), args.cpu_seconds)
+        res["cpu_baseline"] = {"value": alg / med / 1e9, "unit": "GB/s", "cores": 1,
+                               "kind": "port", "ms": med * 1e3,
+                               "sample": f"full tables, median of {reps} runs"}
     return res
 
 
@@ -348,8 +429,9 @@ def main():
     results = {}
     for w in args.workloads.split(","):
         log("workload", w)
-        results[w] = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
-                      "c5": bench_c5, "copy": bench_copy}[w](eng, torch, args)
+        results[w] = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "q6": bench_q6,
+                      "q18": bench_q18, "c4": bench_c4, "c5": bench_c5,
+                      "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))
     # The headline is the first workload with a bandwidth figure (C2 by default).
     head = next((results[w] for w in results if "GBps" in results[w]),

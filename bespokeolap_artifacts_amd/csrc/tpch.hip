@@ -375,19 +375,27 @@ __global__ __launch_bounds__(kT) void q18_init_k(int64_t* res) {
 }
 
 // Largest key of up to three int32 columns (res[0..2], -1 when empty or all
-// negative), one atomic per workgroup and column.
+// negative), one atomic per workgroup and column.  Each thread keeps four
+// independent coalesced loads in flight per round.
+BOLAP_DEV int32_t col_max(const int32_t* __restrict__ v, int64_t n) {
+    const int64_t step = int64_t(gridDim.x) * kT;
+    int32_t m = -1;
+    int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+    for (; i + 3 * step < n; i += 4 * step) {
+        const int32_t a = v[i], b = v[i + step], c = v[i + 2 * step], d = v[i + 3 * step];
+        const int32_t x = a > b ? a : b, y = c > d ? c : d;
+        const int32_t z = x > y ? x : y;
+        m = z > m ? z : m;
+    }
+    for (; i < n; i += step) m = v[i] > m ? v[i] : m;
+    return m;
+}
+
 __global__ __launch_bounds__(kT) void q18_max_k(const int32_t* __restrict__ l, int64_t nl,
                                                 const int32_t* __restrict__ o, int64_t no,
                                                 const int32_t* __restrict__ c, int64_t nc,
                                                 int64_t* res) {
-    int32_t m[3] = {-1, -1, -1};
-    const int64_t step = int64_t(gridDim.x) * kT;
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < nl; i += step)
-        m[0] = l[i] > m[0] ? l[i] : m[0];
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < no; i += step)
-        m[1] = o[i] > m[1] ? o[i] : m[1];
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < nc; i += step)
-        m[2] = c[i] > m[2] ? c[i] : m[2];
+    int32_t m[3] = {col_max(l, nl), col_max(o, no), col_max(c, nc)};
     __shared__ int32_t s[3][kT / kWave];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -427,7 +435,35 @@ __global__ __launch_bounds__(kT) void cust_map_k(const int32_t* __restrict__ c, 
 }
 
 // Sorted lineitem: the lane holding a run's first row sums the run and
-// stores it.  Any pair of adjacent rows out of order raises the flag.
+// stores it.  Each lane loads its own 8 rows and the next 8 (the next lane's
+// rows, an L2 hit), so a run that crosses into the next unit is finished in
+// registers; only a run longer than that reads on, row by row.  Any pair of
+// adjacent rows out of order raises the flag.
+BOLAP_DEV int q18_unit(const int32_t* __restrict__ key, const int16_t* __restrict__ qty,
+                       int64_t n, int64_t u, int32_t (&k)[8], int32_t (&q)[8]) {
+    const int64_t r0 = u * 8;
+    if (r0 + 8 <= n) {
+        const VecT<int32_t, 4> k0 = load_unit<int32_t, 4>(key, 2 * u);
+        const VecT<int32_t, 4> k1 = load_unit<int32_t, 4>(key, 2 * u + 1);
+        const VecT<int16_t, 8> qv = load_unit<int16_t, 8>(qty, u);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            k[j] = k0.v[j];
+            k[j + 4] = k1.v[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] = qv.v[j];
+        return 8;
+    }
+    const int m = r0 < n ? int(n - r0) : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        k[j] = j < m ? key[r0 + j] : 0;
+        q[j] = j < m ? qty[r0 + j] : 0;
+    }
+    return m;
+}
+
 __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict__ key,
                                                        const int16_t* __restrict__ qty,
                                                        int64_t n, int32_t* sum, int64_t size,
@@ -437,28 +473,18 @@ __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict
     for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
          u += int64_t(gridDim.x) * kT) {
         const int64_t r0 = u * 8;
-        int32_t k[8];
-        int32_t q[8];
-        int m = 8;
-        if (r0 + 8 <= n) {
-            const VecT<int32_t, 4> k0 = load_unit<int32_t, 4>(key, 2 * u);
-            const VecT<int32_t, 4> k1 = load_unit<int32_t, 4>(key, 2 * u + 1);
-            const VecT<int16_t, 8> qv = load_unit<int16_t, 8>(qty, u);
+        int32_t k[16], q[16];
+        int32_t ka[8], qa[8], kb[8], qb[8];
+        const int m = q18_unit(key, qty, n, u, ka, qa);
+        const int mb = q18_unit(key, qty, n, u + 1, kb, qb);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                k[j] = k0.v[j];
-                k[j + 4] = k1.v[j];
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) q[j] = qv.v[j];
-        } else {
-            m = int(n - r0);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                k[j] = j < m ? key[r0 + j] : 0;
-                q[j] = j < m ? qty[r0 + j] : 0;
-            }
+        for (int j = 0; j < 8; ++j) {
+            k[j] = ka[j];
+            q[j] = qa[j];
+            k[j + 8] = kb[j];
+            q[j + 8] = qb[j];
         }
+        const int mm = 8 + mb;  // valid rows in k[0..mm) when m == 8
         const bool has_prev = r0 > 0;
         const int32_t prev = has_prev ? key[r0 - 1] : 0;
         if (has_prev && k[0] < prev) bad = true;
@@ -470,11 +496,17 @@ __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict
             if (j >= m) break;
             const bool start = j == 0 ? (!has_prev || k[0] != prev) : k[j] != k[j - 1];
             if (!start) continue;
-            uint32_t s = uint32_t(q[j]);
-            int t = j + 1;
-            while (t < m && k[t] == k[j]) s += uint32_t(q[t++]);
-            if (t == 8) {  // the run may continue past this unit
-                int64_t r = r0 + 8;
+            uint32_t s = 0;
+            int t = j;
+            const int lim = m == 8 ? mm : m;
+#pragma unroll
+            for (int x = 0; x < 16; ++x)
+                if (x >= j && x < lim && t == x && k[x] == k[j]) {
+                    s += uint32_t(q[x]);
+                    ++t;
+                }
+            if (t == 16) {  // the run continues past the lookahead unit
+                int64_t r = r0 + 16;
                 while (r < n && key[r] == k[j]) s += uint32_t(int32_t(qty[r++]));
             }
             if (k[j] >= 0 && k[j] < size) sum[k[j]] = int32_t(s);
@@ -616,6 +648,34 @@ __global__ __launch_bounds__(kT) void q18_keys_k(const bolap_q18_row* __restrict
             const uint32_t d = uint32_t(r.orderdate) ^ 0x80000000u;
             keys[i] = (uint64_t(p) << 32) | d;
         }
+    }
+}
+
+// One workgroup orders up to kRankMax rows: each row's place is the number of
+// rows before it in (price desc, date asc, key asc, order row asc), a strict
+// total order because order rows are unique.
+constexpr int kRankT = 1024;
+constexpr int kRankMax = 2048;
+
+BOLAP_DEV bool q18_before(const int4& a, const int4& b) {
+    if (a.x != b.x) return a.x > b.x;  // totalprice desc
+    if (a.y != b.y) return a.y < b.y;  // orderdate asc
+    if (a.z != b.z) return a.z < b.z;  // orderkey asc
+    return a.w < b.w;                  // order row asc
+}
+
+__global__ __launch_bounds__(kRankT) void q18_rank_sort_k(const bolap_q18_row* __restrict__ rows,
+                                                          int n, bolap_q18_row* out) {
+    __shared__ int4 s[kRankMax];
+    for (int i = threadIdx.x; i < n; i += kRankT)
+        s[i] = make_int4(rows[i].totalprice, rows[i].orderdate, rows[i].orderkey,
+                         rows[i].order_row);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kRankT) {
+        const int4 me = s[i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) rank += q18_before(s[j], me) ? 1 : 0;
+        out[rank] = rows[i];
     }
 }
 
@@ -871,29 +931,45 @@ int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* a, bolap_q18_row* rows_out,
     BOLAP_ARG_CHECK(a->order_rows < (int64_t(1) << 31), "bolap_q18: too many orders");
     std::memset(st, 0, sizeof(*st));
     int64_t* res = static_cast<int64_t*>(ctx->result_dev);
-    // Key bounds and the reference's density checks (first and last key).
-    int64_t h[kQ18ResN + 4];
+    // The reference's density checks read the first and last keys only
+    // (query_q18.cpp:194-199); the key maxima are needed only off the dense
+    // path.
+    int64_t h[kQ18ResN];
+    int32_t ends[4] = {0, 0, 0, 0};
     {
         TraceScope ts(ctx, "q18_bounds");
-        hipLaunchKernelGGL(q18_init_k, dim3(1), dim3(kT), 0, ctx->stream, res);
-        BOLAP_TRY(launch_check("q18_init_k"));
-        const int64_t most = a->li_rows > a->order_rows ? a->li_rows : a->order_rows;
-        hipLaunchKernelGGL(q18_max_k, dim3(grid_of(ctx, most, 4)), dim3(kT), 0, ctx->stream,
-                           a->l_orderkey, a->li_rows, a->o_orderkey, a->order_rows, a->c_custkey,
-                           a->cust_rows, res);
-        BOLAP_TRY(launch_check("q18_max_k"));
+        BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int32_t)));
+        int32_t* pe = static_cast<int32_t*>(ctx->pinned);
+        if (a->order_rows > 0) {
+            BOLAP_HIP_TRY(hipMemcpyAsync(pe, a->o_orderkey, 4, hipMemcpyDeviceToHost, ctx->stream));
+            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 1, a->o_orderkey + a->order_rows - 1, 4,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (a->cust_rows > 0) {
+            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 2, a->c_custkey, 4, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 3, a->c_custkey + a->cust_rows - 1, 4,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        }
+        BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(ends, pe, sizeof(ends));
     }
-    BOLAP_TRY(read_result(ctx, res, kQ18ResN, h));
-    int32_t ends[4] = {0, 0, 0, 0};
-    if (a->order_rows > 0) {
-        BOLAP_HIP_TRY(hipMemcpy(&ends[0], a->o_orderkey, 4, hipMemcpyDeviceToHost));
-        BOLAP_HIP_TRY(hipMemcpy(&ends[1], a->o_orderkey + a->order_rows - 1, 4,
-                                hipMemcpyDeviceToHost));
-    }
-    if (a->cust_rows > 0) {
-        BOLAP_HIP_TRY(hipMemcpy(&ends[2], a->c_custkey, 4, hipMemcpyDeviceToHost));
-        BOLAP_HIP_TRY(hipMemcpy(&ends[3], a->c_custkey + a->cust_rows - 1, 4,
-                                hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(q18_init_k, dim3(1), dim3(kT), 0, ctx->stream, res);
+    BOLAP_TRY(launch_check("q18_init_k"));
+    {
+        const bool dense_o = a->order_rows > 0 && ends[0] == 1 && ends[1] == a->order_rows;
+        const bool dense_c = a->cust_rows > 0 && ends[2] == 1 && ends[3] == a->cust_rows;
+        h[kQ18MaxL] = h[kQ18MaxO] = h[kQ18MaxC] = -1;
+        if (!dense_o || !dense_c) {
+            TraceScope ts(ctx, "q18_bounds");
+            const int64_t most = a->li_rows > a->order_rows ? a->li_rows : a->order_rows;
+            hipLaunchKernelGGL(q18_max_k, dim3(grid_of(ctx, most / 4, 4)), dim3(kT), 0,
+                               ctx->stream, dense_o ? nullptr : a->l_orderkey,
+                               dense_o ? 0 : a->li_rows, dense_o ? nullptr : a->o_orderkey,
+                               dense_o ? 0 : a->order_rows, a->c_custkey, a->cust_rows, res);
+            BOLAP_TRY(launch_check("q18_max_k"));
+            BOLAP_TRY(read_result(ctx, res, kQ18ResN, h));
+        }
     }
     const bool dense_orders = a->order_rows > 0 && ends[0] == 1 && ends[1] == a->order_rows;
     const bool dense_cust = a->cust_rows > 0 && ends[2] == 1 && ends[3] == a->cust_rows;
@@ -985,7 +1061,12 @@ int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* a, bolap_q18_row* rows_out,
     uint64_t* keys = ws.at<uint64_t>(o_keys);
     uint32_t* vals = ws.at<uint32_t>(o_vals);
     bolap_q18_row* sorted = ws.at<bolap_q18_row>(o_sorted);
-    {
+    if (n <= kRankMax) {
+        TraceScope ts(ctx, "q18_sort");
+        hipLaunchKernelGGL(q18_rank_sort_k, dim3(1), dim3(kRankT), 0, ctx->stream, rows_dev,
+                           int(n), sorted);
+        BOLAP_TRY(launch_check("q18_rank_sort_k"));
+    } else {
         TraceScope ts(ctx, "q18_sort");
         const unsigned g = grid_of(ctx, n, 4);
         hipLaunchKernelGGL(q18_keys_k, dim3(g), dim3(kT), 0, ctx->stream, rows_dev, n, 1, keys,
