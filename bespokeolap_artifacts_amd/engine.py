@@ -134,6 +134,54 @@ class Engine:
                         int128(g.sum_charge_lo, g.sum_charge_hi), g.sum_disc, g.count)
                 for g in out]
 
+    @staticmethod
+    def q1_finalize(groups: list, rf_dict: list[str], ls_dict: list[str],
+                    layout: int = N.Q1_LAYOUT_WIDE) -> list[dict]:
+        """run_q1's result rows from the exact group accumulators (host C++);
+        groups are indexed rf * len(ls_dict) + ls."""
+        n_rf, n_ls = len(rf_dict), len(ls_dict)
+        gs = (N.Q1Group * (n_rf * n_ls))()
+        for i, g in enumerate(groups):
+            charge = g.sum_charge & ((1 << 128) - 1)
+            lo, hi = charge & ((1 << 64) - 1), charge >> 64
+            gs[i] = N.Q1Group(g.sum_qty, g.sum_base_price, g.sum_disc_price, lo,
+                              hi - (1 << 64) if hi >= (1 << 63) else hi, g.sum_disc, g.count, 0)
+        rf = (C.c_char_p * n_rf)(*(x.encode() for x in rf_dict))
+        ls = (C.c_char_p * n_ls)(*(x.encode() for x in ls_dict))
+        rows = (N.Q1Row * (n_rf * n_ls))()
+        n = C.c_int32()
+        N.check(N.load().bolap_q1_finalize(gs, layout, n_rf, n_ls, rf, ls, rows, C.byref(n)))
+        out = []
+        for r in rows[:n.value]:
+            d = {k: getattr(r, k) for k, _ in N.Q1Row._fields_}
+            d["returnflag"], d["linestatus"] = d["returnflag"].decode(), d["linestatus"].decode()
+            out.append(d)
+        return out
+
+    @staticmethod
+    def q1_write_csv(path: str, rows: list[dict]) -> None:
+        arr = (N.Q1Row * max(1, len(rows)))()
+        for i, r in enumerate(rows):
+            arr[i] = N.Q1Row(r["returnflag"].encode(), r["linestatus"].encode(), r["sum_qty"],
+                             r["sum_base_price"], r["sum_disc_price"], r["sum_charge"],
+                             r["avg_qty"], r["avg_price"], r["avg_disc"], r["count_order"])
+        N.check(N.load().bolap_q1_write_csv(str(path).encode(), arr, len(rows)))
+
+    @staticmethod
+    def q6_write_csv(path: str, result: dict) -> None:
+        r = N.Q6Result(result["revenue"], result["rows_emitted"], result["rows_scanned"])
+        N.check(N.load().bolap_q6_write_csv(str(path).encode(), C.byref(r)))
+
+    @staticmethod
+    def q18_write_csv(path: str, rows, c_name: list[str]) -> None:
+        """rows: the structured array returned by q18_run."""
+        n = len(rows)
+        arr = (N.Q18Row * max(1, n))()
+        for i, r in enumerate(rows):
+            arr[i] = N.Q18Row(*(int(r[k]) for k, _ in N.Q18Row._fields_))
+        names = (C.c_char_p * max(1, len(c_name)))(*(x.encode() for x in c_name))
+        N.check(N.load().bolap_q18_write_csv(str(path).encode(), arr, n, names, len(c_name)))
+
     def q1_scan_agg_async(self, a: N.Q1Args, out_dev) -> None:
         N.check(self.lib.bolap_q1_scan_agg_async(self.ctx, C.byref(a),
                                                  C.c_void_p(out_dev.data_ptr())))

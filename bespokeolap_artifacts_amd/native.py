@@ -112,6 +112,14 @@ class Q1Group(C.Structure):
                 ("count", C.c_int64), ("reserved", C.c_int64)]
 
 
+class Q1Row(C.Structure):
+    _fields_ = [("returnflag", C.c_char * 8), ("linestatus", C.c_char * 8),
+                ("sum_qty", C.c_double), ("sum_base_price", C.c_double),
+                ("sum_disc_price", C.c_double), ("sum_charge", C.c_double),
+                ("avg_qty", C.c_double), ("avg_price", C.c_double), ("avg_disc", C.c_double),
+                ("count_order", C.c_int64)]
+
+
 class JoinSumArgs(C.Structure):
     _fields_ = [("dim_rows", C.c_int64), ("dim_key", vp), ("n_dim_preds", C.c_int32),
                 ("dim_preds", Pred * MAX_PREDS), ("fact_rows", C.c_int64),
@@ -239,6 +247,11 @@ PROTOTYPES = {
     "bolap_mlp_predict": (C.c_int, [vp, vp, i64, vp]),
     "bolap_mlp_train_step": (C.c_int, [vp, vp, vp, i64, C.c_float, P(C.c_float)]),
     "bolap_mlp_gradients": (C.c_int, [vp, vp, vp, i64, P(C.c_float), P(C.c_float)]),
+    "bolap_q1_finalize": (C.c_int, [P(Q1Group), i32, i32, i32, P(C.c_char_p), P(C.c_char_p),
+                                    P(Q1Row), P(i32)]),
+    "bolap_q1_write_csv": (C.c_int, [C.c_char_p, P(Q1Row), i32]),
+    "bolap_q6_write_csv": (C.c_int, [C.c_char_p, P(Q6Result)]),
+    "bolap_q18_write_csv": (C.c_int, [C.c_char_p, P(Q18Row), i64, P(C.c_char_p), i64]),
     "bolap_q6_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P(Q6Params)]),
     "bolap_q6": (C.c_int, [vp, P(Q6Args), P(Q6Result)]),
     "bolap_q6_zones": (C.c_int, [vp, i64, vp, vp, vp, vp]),

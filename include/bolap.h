@@ -174,6 +174,28 @@ int bolap_q1_scan_agg(bolap_ctx* ctx, const bolap_q1_args* args, bolap_q1_group*
 int bolap_q1_scan_agg_async(bolap_ctx* ctx, const bolap_q1_args* args,
                             bolap_q1_group* out_dev);
 
+/* Q1 result rows as run_q1 finalises them (query_q1.cpp:234-275): sums
+   divided by their fixed-point scale, averages over the count, empty groups
+   dropped, rows ordered by (returnflag, linestatus) strings.  groups holds
+   n_rf * n_ls accumulators indexed rf * n_ls + ls; rf_dict / ls_dict are the
+   dictionary strings by code.  Scales: REF layout quantity x100, prices and
+   discounted_price in cents, charge in cents x hundredths; WIDE layout
+   quantity in units, disc_price in cents x hundredths, charge x another 100.
+   rows_out holds n_rf * n_ls rows.  Host only. */
+typedef struct {
+    char returnflag[8];
+    char linestatus[8];
+    double sum_qty, sum_base_price, sum_disc_price, sum_charge;
+    double avg_qty, avg_price, avg_disc;
+    int64_t count_order;
+} bolap_q1_row;
+
+int bolap_q1_finalize(const bolap_q1_group* groups, int32_t layout, int32_t n_rf,
+                      int32_t n_ls, const char* const* rf_dict, const char* const* ls_dict,
+                      bolap_q1_row* rows_out, int32_t* n_out);
+/* write_q1_csv (query_q1.cpp:283-298): fixed notation, 2 decimals, avg_disc 4. */
+int bolap_q1_write_csv(const char* path, const bolap_q1_row* rows, int32_t n);
+
 /* ------------------------------------------------------------------------ */
 /* Q19/Q17/Q14 shape: filtered dimension -> hash build -> fact probe ->       */
 /* scalar SUM(price * (100 - disc)).                                          */
@@ -569,6 +591,13 @@ typedef struct {
    (stats->rows says how many). */
 int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* args, bolap_q18_row* rows_out,
               int64_t max_rows, bolap_q18_stats* stats);
+
+/* write_q6_csv (query_q6.cpp:399-408) and write_q18_csv (query_q18.cpp:465-484:
+   c_name quoted with \" escapes, ISO dates, 2 decimals); c_name[cust_row]
+   gives each row's customer name.  Host only. */
+int bolap_q6_write_csv(const char* path, const bolap_q6_result* result);
+int bolap_q18_write_csv(const char* path, const bolap_q18_row* rows, int64_t n,
+                        const char* const* c_name, int64_t n_names);
 
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */

