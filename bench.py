@@ -396,12 +396,14 @@ def bench_c3(eng, torch, args):
 
 # HBM bytes per launch (read + written) of each workload's dominant kernel,
 # measured by two rocprofv3 runs (FETCH_SIZE, then WRITE_SIZE; summaries under
-# profiles/).  A 1 GiB device copy reads FETCH_SIZE = 524,308 KiB and
+# profiles/).  A 1 GiB device copy reads FETCH_SIZE = 524,306 KiB and
 # WRITE_SIZE = 1,048,576 KiB, so FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0.
 MEASURED_TRAFFIC = {
-    "probe_sum_k": 248.6e6 + 0.0e6,
+    "probe_sum_k": 248.3e6 + 0.0e6,
     "q1_scan_agg_kernel": 228.3e6 + 0.1e6,
     "fact_probe_k": 823.9e6 + 131.7e6,
+    "q6_flat_k": 540.6e6 + 0.0e6,
+    "q18_sum_sorted_k": 371.6e6 + 71.4e6,
 }
 
 
