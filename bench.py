@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workloads", default="c2,c1,c3,q6,q18,c4,c5,copy")
+    ap.add_argument("--workloads", default="c2,c1,c3,q6,q14,q17,q18,c4,c5,copy")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -220,6 +220,57 @@ def bench_q6(eng, torch, args):
         res["cpu_baseline"] = {"value": alg / med / 1e9, "unit": "GB/s", "cores": 1,
                                "kind": "port", "ms": med * 1e3,
                                "sample": f"full table, flat loop, median of {reps} runs"}
+    return res
+
+
+# ------------------------------------------------------------ Q14 / Q17
+def bench_q14_q17(eng, torch, args, which):
+    """Q14 (one-month promo revenue) and Q17 (small-quantity revenue) over
+    60,000,000 lineitem rows joined to 2,000,000 parts (reference layout)."""
+    from bespokeolap_artifacts_amd import native as N
+    t = eng.gen_part_lineitem(SEED, 2_000_000, 60_000_000)
+    rows = 60_000_000
+    if which == "q14":
+        lo, hi = eng.q14_window("1995-09-01")
+        step = lambda: eng.q14(t, lo, hi)  # noqa: E731
+        first = step()
+        ms = time_steps(torch, step, args.steps, args.warmup)
+        alg = rows * 2 + first["rows_emitted"] * 8
+        res = {"workload": "Q14 promo revenue, 60,000,000 lineitem rows, 2,000,000 parts, "
+                           "one-month window",
+               "ms_per_step": ms, "GBps": alg / (ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+               "rows_per_s": rows / (ms * 1e-3), "rows_emitted": first["rows_emitted"],
+               "kernel": "q14_scan_k"}
+    else:
+        preds = [eng.pred(t["p_brand"], N.OP_RANGE, 3, 3),
+                 eng.pred(t["p_container"], N.OP_RANGE, 7, 7)]
+        a = eng.q17_args(t, preds)
+        step = lambda: eng.q17_run(a)  # noqa: E731
+        first = step()
+        ms = time_steps(torch, step, args.steps, args.warmup)
+        alg = rows * 4 + first["matches"] * 6
+        res = {"workload": "Q17 small-quantity revenue, 60,000,000 lineitem rows, "
+                           "2,000,000 parts (brand and container equality)",
+               "ms_per_step": ms, "GBps": alg / (ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+               "probes_per_s": rows / (ms * 1e-3), "parts": first["parts"],
+               "matches": first["matches"], "kernel": "q17_probe_k"}
+    if not args.no_cpu:
+        import oracle
+        h = {k: v.cpu().numpy() for k, v in t.items()}
+        part = {"partkey": h["p_partkey"], "promo": h["p_promo"]}
+        li = {"ship": h["l_ship"], "partkey": h["l_partkey"], "disc_price": h["l_disc_price"],
+              "qty": h["l_qty"], "price": h["l_price"]}
+        if which == "q14":
+            run = lambda: oracle.q14(li, part, lo, hi)  # noqa: E731
+        else:
+            op = [oracle.make_pred(h["p_brand"], 0, 3, 3),
+                  oracle.make_pred(h["p_container"], 0, 7, 7)]
+            run = lambda: oracle.q17(h["p_partkey"], op, li)  # noqa: E731
+        res["parity"] = run() == first
+        med, reps = cpu_time(run, args.cpu_seconds)
+        res["cpu_baseline"] = {"value": alg / med / 1e9, "unit": "GB/s", "cores": 1,
+                               "kind": "port", "ms": med * 1e3,
+                               "sample": f"full tables, median of {reps} runs"}
     return res
 
 
@@ -432,6 +483,8 @@ def main():
     for w in args.workloads.split(","):
         log("workload", w)
         results[w] = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "q6": bench_q6,
+                      "q14": lambda e, t, a: bench_q14_q17(e, t, a, "q14"),
+                      "q17": lambda e, t, a: bench_q14_q17(e, t, a, "q17"),
                       "q18": bench_q18, "c4": bench_c4, "c5": bench_c5,
                       "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))

@@ -262,25 +262,34 @@ __global__ __launch_bounds__(kCompactThreads) void dim_compact_k(PredSet ps, int
         atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
 }
 
-// HBM table for a build side that does not go to LDS.
-__global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl) {
+// HBM table for a build side that does not go to LDS.  With `force` the
+// table is built in HBM whatever its geometry (a canonical copy that every
+// probe workgroup copies into LDS, so slot numbers agree across workgroups),
+// and the per-slot aggregates agg_sum/agg_cnt (when given) are cleared.
+__global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl, int force,
+                                                  uint64_t* agg_sum, uint32_t* agg_cnt) {
     const Shape s = shape_of(klist_size(kl, nullptr), g);
-    if (s.flag || s.lds) return;
+    if (s.flag || (s.lds && !force)) return;
     const int64_t cap = int64_t(1) << s.cap_log2;
     const int64_t words = (int64_t(1) << s.bb_log2) / 32;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cap;
-         i += int64_t(gridDim.x) * 256)
+         i += int64_t(gridDim.x) * 256) {
         t.slots[i] = kEmpty;
+        if (agg_sum) {
+            agg_sum[i] = 0;
+            agg_cnt[i] = 0;
+        }
+    }
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
          i += int64_t(gridDim.x) * 256)
         t.bits[i] = 0;
 }
 
 // Workgroup b inserts segment b % kSegs (gridDim.x is a multiple of kSegs).
-__global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl) {
+__global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl, int force) {
     __shared__ uint32_t seg[kSegs];
     const Shape s = shape_of(klist_size(kl, seg), g);
-    if (s.flag || s.lds) return;
+    if (s.flag || (s.lds && !force)) return;
     const int r = int(blockIdx.x % kSegs);
     const int64_t per = gridDim.x / kSegs;
     const int32_t* __restrict__ keys = kl.keys + int64_t(r) * kl.seg_cap;
@@ -345,15 +354,14 @@ __global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ 
     join_merge_rows<256>(partials, nblocks, kl, res, host);
 }
 
-// The fact-side loop of one workgroup over the table (B, T), in LDS or in
-// HBM depending on where the pointers come from (each call site is inlined
-// with its own address space).
-template <typename TP, typename TD, int kProbeU, bool kPipe>
-BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, const TP* __restrict__ price,
-                          const TD* __restrict__ disc, int64_t rows, const uint32_t* B,
-                          const int32_t* T, int cap_log2, int bb_log2, bool has_min,
-                          int64_t& rev, uint32_t& cnt) {
-    const uint32_t mask = (uint32_t(1) << cap_log2) - 1;
+// The fact-side loop of one workgroup over the Bloom filter B: every key
+// that passes the filter is handed to on_cand(row, key), which does the
+// exact lookup (the filter and the table live in LDS or in HBM depending on
+// where the pointers come from; each call site is inlined with its own
+// address space).
+template <int kProbeU, bool kPipe, class OnCand>
+BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const uint32_t* B,
+                          int bb_log2, OnCand&& on_cand) {
     const int64_t n_units = rows / 4;
     const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
     struct Tile {
@@ -394,8 +402,7 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, const TP* __restrict__
                 if (hits & (1u << i)) {
                     const int32_t key = x.v[i / 4].v[i % 4];
                     const int64_t row = (at + int64_t(i / 4) * kThreads) * 4 + (i % 4);
-                    if (key == kEmpty ? has_min : table_has(key, T, cap_log2, mask))
-                        add_match<TP, TD>(row, price, disc, rev, cnt);
+                    on_cand(row, key);
                 }
             }
         }
@@ -425,11 +432,7 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, const TP* __restrict__
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
         const int64_t row = n_units * 4 + threadIdx.x;
-        if (row < rows) {
-            const int32_t key = fk[row];
-            const bool h = key == kEmpty ? has_min : table_has(key, T, cap_log2, mask);
-            if (h) add_match<TP, TD>(row, price, disc, rev, cnt);
-        }
+        if (row < rows) on_cand(row, fk[row]);
     }
 }
 
@@ -464,11 +467,19 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                 if (key != kEmpty) table_insert(Tl, key, s.cap_log2);
             }
             __syncthreads();
-            probe_rows<TP, TD, kProbeU, kPipe>(fk, price, disc, rows, Bl, Tl, s.cap_log2,
-                                               s.bb_log2, has_min, rev, cnt);
+            const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
+            probe_rows<kProbeU, kPipe>(fk, rows, Bl, s.bb_log2, [&](int64_t row, int32_t key) {
+                if (key == kEmpty ? has_min : table_has(key, Tl, s.cap_log2, mask))
+                    add_match<TP, TD>(row, price, disc, rev, cnt);
+            });
         } else {
-            probe_rows<TP, TD, kProbeU, kPipe>(fk, price, disc, rows, hbm.bits, hbm.slots,
-                                               s.cap_log2, s.bb_log2, has_min, rev, cnt);
+            const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
+            const int32_t* Th = hbm.slots;
+            probe_rows<kProbeU, kPipe>(fk, rows, hbm.bits, s.bb_log2,
+                                       [&](int64_t row, int32_t key) {
+                if (key == kEmpty ? has_min : table_has(key, Th, s.cap_log2, mask))
+                    add_match<TP, TD>(row, price, disc, rev, cnt);
+            });
         }
     }
 
@@ -496,6 +507,119 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
         if (!last_block_done(counter)) return;
         join_merge_rows<kThreads>(partials, int(gridDim.x), kl, res, host);
         if (threadIdx.x == 0) reset_counter(counter);
+    }
+}
+
+// Exact slot of a key, or -1.
+BOLAP_DEV int32_t table_find(int32_t key, const int32_t* T, int cap_log2, uint32_t mask) {
+    uint32_t s = slot_of(key, cap_log2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t t = T[s];
+        if (t == key) return int32_t(s);
+        if (t == kEmpty) return -1;
+        s = (s + 1) & mask;
+    }
+    return -1;
+}
+
+// Q17 (bespoke_tpch/query_q17.cpp:119-309) over the canonical table: pass 1
+// adds quantity and 1 into the part's slot for every lineitem row of a
+// passing part; pass 2 sums extendedprice over the rows with
+// quantity * 5 * cnt < sum (0.2 * average, :288-291).  Both passes copy the
+// HBM table into LDS when it has the LDS geometry.  Part keys are never the
+// empty marker (negative part keys are filtered out before the build).
+template <int kPass>
+__global__ __launch_bounds__(kThreads) void q17_probe_k(
+    const int32_t* __restrict__ pk, const int16_t* __restrict__ qty,
+    const int32_t* __restrict__ price, int64_t rows, Geo g, Table hbm, KList kl,
+    uint64_t* agg_sum, uint32_t* agg_cnt, int64_t* partials) {
+    extern __shared__ uint32_t lds[];
+    const int64_t n = klist_size(kl, nullptr);
+    const Shape s = shape_of(n, g);
+    uint64_t acc = 0;
+    uint32_t cnt = 0;
+    if (s.flag == 0 && n > 0) {
+        const int64_t words = (int64_t(1) << s.bb_log2) / 32;
+        const int64_t cap = int64_t(1) << s.cap_log2;
+        const uint32_t mask = uint32_t(cap - 1);
+        auto on_cand = [&](const int32_t* T, int64_t row, int32_t key) {
+            const int32_t slot = key == kEmpty ? -1 : table_find(key, T, s.cap_log2, mask);
+            if (slot < 0) return;
+            const int64_t q = int64_t(qty[row]);
+            if (kPass == 1) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(agg_sum + slot),
+                          (unsigned long long)q);
+                atomicAdd(agg_cnt + slot, 1u);
+                cnt += 1;
+            } else {
+                const int64_t c = int64_t(agg_cnt[slot]);
+                if (q * 5 * c < int64_t(agg_sum[slot])) {
+                    acc += uint64_t(int64_t(price[row]));
+                    cnt += 1;
+                }
+            }
+        };
+        if (s.lds) {
+            uint32_t* Bl = lds;
+            int32_t* Tl = reinterpret_cast<int32_t*>(lds + words);
+            for (int64_t i = threadIdx.x; i < words; i += kThreads) Bl[i] = hbm.bits[i];
+            for (int64_t i = threadIdx.x; i < cap; i += kThreads) Tl[i] = hbm.slots[i];
+            __syncthreads();
+            probe_rows<2, true>(pk, rows, Bl, s.bb_log2,
+                                [&](int64_t row, int32_t key) { on_cand(Tl, row, key); });
+        } else {
+            const int32_t* Th = hbm.slots;
+            probe_rows<2, true>(pk, rows, hbm.bits, s.bb_log2,
+                                [&](int64_t row, int32_t key) { on_cand(Th, row, key); });
+        }
+    }
+    __shared__ uint64_t s_a[kWaves], s_c[kWaves];
+    const uint64_t wa = wave_sum_u64(acc), wc = wave_sum_u64(uint64_t(cnt));
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_a[threadIdx.x / kWave] = wa;
+        s_c[threadIdx.x / kWave] = wc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, c = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            a += s_a[w];
+            c += s_c[w];
+        }
+        partials[2 * int64_t(blockIdx.x)] = int64_t(a);
+        partials[2 * int64_t(blockIdx.x) + 1] = int64_t(c);
+    }
+}
+
+// Sums the two passes' partials: host row = (SUM(extendedprice), rows
+// emitted, part rows, matches).
+__global__ __launch_bounds__(256) void q17_merge_k(const int64_t* __restrict__ p1,
+                                                   const int64_t* __restrict__ p2, int nblocks,
+                                                   KList kl, int64_t* host) {
+    const int64_t n = klist_size(kl, nullptr);
+    uint64_t m = 0, sum = 0, rows = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        m += uint64_t(p1[2 * b + 1]);
+        sum += uint64_t(p2[2 * b]);
+        rows += uint64_t(p2[2 * b + 1]);
+    }
+    __shared__ uint64_t sh[3][4];
+    m = wave_sum_u64(m);
+    sum = wave_sum_u64(sum);
+    rows = wave_sum_u64(rows);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sh[0][threadIdx.x / kWave] = sum;
+        sh[1][threadIdx.x / kWave] = rows;
+        sh[2][threadIdx.x / kWave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int64_t v = n;
+        if (threadIdx.x != 2) {
+            const int f = threadIdx.x == 3 ? 2 : int(threadIdx.x);
+            v = int64_t(sh[f][0] + sh[f][1] + sh[f][2] + sh[f][3]);
+        }
+        __hip_atomic_store(host + threadIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -633,10 +757,10 @@ int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
         // side goes to LDS, which is the common case.
         TraceScope ts(ctx, "join_hbm_build");
         hipLaunchKernelGGL(hbm_init_k, dim3(grid_for(ctx, p.hbm_cap, 256, 1)), dim3(256), 0,
-                           ctx->stream, p.g, p.t, p.kl);
+                           ctx->stream, p.g, p.t, p.kl, 0, nullptr, nullptr);
         BOLAP_TRY(launch_check("hbm_init_k"));
         hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, p.g, p.t,
-                           p.kl);
+                           p.kl, 0);
         BOLAP_TRY(launch_check("hbm_build_k"));
     }
     return dispatch_probe(ctx, a, p);
@@ -759,5 +883,114 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
     out->capacity = int64_t(1) << s.cap_log2;
     out->placement = s.lds ? BOLAP_PLACE_LDS : BOLAP_PLACE_HBM;
     out->fallbacks = fallbacks;
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_q17: null argument");
+    BOLAP_ARG_CHECK(a->part_rows >= 0 && a->li_rows >= 0, "bolap_q17: negative rows");
+    BOLAP_ARG_CHECK(a->part_rows <= (int64_t(1) << 30), "bolap_q17: too many parts");
+    BOLAP_ARG_CHECK(a->part_rows == 0 || a->p_partkey, "bolap_q17: null p_partkey");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (a->l_partkey && a->l_quantity && a->l_price),
+                    "bolap_q17: null lineitem column");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || aligned(a->l_partkey, 16),
+                    "bolap_q17: l_partkey must be 16-byte aligned");
+    BOLAP_ARG_CHECK(a->n_part_preds >= 0 && a->n_part_preds < BOLAP_MAX_PREDS,
+                    "bolap_q17: %d part predicates (max %d)", a->n_part_preds,
+                    BOLAP_MAX_PREDS - 1);
+    // The reference's part scan skips negative part keys (query_q17.cpp:168-171).
+    bolap_pred preds[BOLAP_MAX_PREDS];
+    for (int i = 0; i < a->n_part_preds; ++i) preds[i] = a->part_preds[i];
+    bolap_pred& nonneg = preds[a->n_part_preds];
+    std::memset(&nonneg, 0, sizeof(nonneg));
+    nonneg.col = a->p_partkey;
+    nonneg.dtype = BOLAP_I32;
+    nonneg.op = BOLAP_OP_RANGE;
+    nonneg.lo = 0;
+    nonneg.hi = INT32_MAX;
+    PredSet ps;
+    BOLAP_TRY(make_predset(ps, a->n_part_preds + 1, preds, a->part_rows, "bolap_q17"));
+
+    const Geo g{0, BOLAP_PLACE_AUTO, 0, 0, kAutoLdsKeys};
+    const Shape worst = shape_of(a->part_rows, Geo{0, BOLAP_PLACE_HBM, 0, 0, 0});
+    const Shape lds_max = shape_of(kAutoLdsKeys, g);
+    const size_t lds = lds_bytes_of(lds_max.cap_log2, lds_max.bb_log2);
+    int cap_log2 = worst.cap_log2 > lds_max.cap_log2 ? worst.cap_log2 : lds_max.cap_log2;
+    int bb_log2 = worst.bb_log2 > lds_max.bb_log2 ? worst.bb_log2 : lds_max.bb_log2;
+    const int64_t cap = int64_t(1) << cap_log2;
+    const int64_t words = (int64_t(1) << bb_log2) / 32;
+    const int64_t max_blocks = int64_t(ctx->num_cus) * 8;
+    const int compact_r = (aligned(a->p_partkey, 16) && preds_aligned(ps, 4)) ? 4 : 1;
+    const int64_t units = a->part_rows / compact_r;
+    const unsigned compact_grid = grid_for(ctx, units, kCompactThreads, 16);
+    const int64_t per_block = (units + int64_t(compact_grid) * kCompactThreads - 1) /
+                              (int64_t(compact_grid) * kCompactThreads) * kCompactThreads *
+                              compact_r;
+    const int64_t seg_cap = (int64_t(compact_grid) + kSegs - 1) / kSegs * per_block + compact_r;
+    WsArena ws;
+    const size_t o_cnt = ws.plan(size_t(kSegs) * kSegStride * 4);
+    const size_t o_keys = ws.plan(size_t(seg_cap) * kSegs * 4);
+    const size_t o_slots = ws.plan(size_t(cap) * 4);
+    const size_t o_bits = ws.plan(size_t(words) * 4);
+    const size_t o_sum = ws.plan(size_t(cap) * 8);
+    const size_t o_acnt = ws.plan(size_t(cap) * 4);
+    const size_t o_part = ws.plan(size_t(max_blocks) * 2 * sizeof(int64_t) * 2);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
+    ws.base = static_cast<char*>(ctx->ws);
+    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+    const KList kl{ws.at<int32_t>(o_keys), ws.at<uint32_t>(o_cnt), seg_cap};
+    const Table t{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits)};
+    uint64_t* agg_sum = ws.at<uint64_t>(o_sum);
+    uint32_t* agg_cnt = ws.at<uint32_t>(o_acnt);
+    int64_t* part1 = ws.at<int64_t>(o_part);
+    int64_t* part2 = part1 + 2 * max_blocks;
+
+    hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, res, kl.cnt);
+    BOLAP_TRY(launch_check("zero_res_k"));
+    if (a->part_rows > 0) {
+        TraceScope ts(ctx, "q17_part_scan");
+        if (compact_r == 4)
+            hipLaunchKernelGGL(dim_compact_k<4>, dim3(compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->part_rows, a->p_partkey, kl, res);
+        else
+            hipLaunchKernelGGL(dim_compact_k<1>, dim3(compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->part_rows, a->p_partkey, kl, res);
+        BOLAP_TRY(launch_check("dim_compact_k"));
+    }
+    {
+        TraceScope ts(ctx, "q17_build");
+        hipLaunchKernelGGL(hbm_init_k, dim3(grid_for(ctx, cap, 256, 1)), dim3(256), 0,
+                           ctx->stream, g, t, kl, 1, agg_sum, agg_cnt);
+        BOLAP_TRY(launch_check("hbm_init_k"));
+        hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, g, t, kl, 1);
+        BOLAP_TRY(launch_check("hbm_build_k"));
+    }
+    int64_t blocks = int64_t(ctx->num_cus) * 2;
+    const int64_t need = (a->li_rows / 4 + int64_t(kThreads) * 2 - 1) / (int64_t(kThreads) * 2);
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    for (int pass = 1; pass <= 2; ++pass) {
+        TraceScope ts(ctx, pass == 1 ? "q17_sum_scan" : "q17_filter_scan");
+        auto kern = pass == 1 ? q17_probe_k<1> : q17_probe_k<2>;
+        BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), lds, ctx->stream,
+                           a->l_partkey, a->l_quantity, a->l_price, a->li_rows, g, t, kl,
+                           agg_sum, agg_cnt, pass == 1 ? part1 : part2);
+        BOLAP_TRY(launch_check("q17_probe_k"));
+    }
+    hipLaunchKernelGGL(q17_merge_k, dim3(1), dim3(256), 0, ctx->stream, part1, part2,
+                       int(blocks), kl, static_cast<int64_t*>(ctx->pinned_dev));
+    BOLAP_TRY(launch_check("q17_merge_k"));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t* h = static_cast<const int64_t*>(ctx->pinned);
+    out->sum_extendedprice = h[0];
+    out->rows_emitted = h[1];
+    out->parts = h[2];
+    out->matches = h[3];
+    trace_count(ctx, "q17_part_rows_emitted", out->parts);
+    trace_count(ctx, "q17_lineitem_sum_rows_emitted", out->matches);
+    trace_count(ctx, "q17_lineitem_filter_rows_emitted", out->rows_emitted);
     return BOLAP_OK;
 }

@@ -5,7 +5,8 @@
 // exact integer accumulators into doubles (sums / scale, averages / count),
 // drops empty groups and orders rows by (returnflag, linestatus) strings;
 // write_q1_csv (:283-298) prints fixed notation with two decimals (four for
-// avg_disc).  Q6: write_q6_csv (query_q6.cpp:399-408).  Q18: write_q18_csv
+// avg_disc).  Q6: write_q6_csv (query_q6.cpp:399-408).  Q14: write_q14_csv
+// (query_q14.cpp:367-375).  Q17: write_q17_csv (query_q17.cpp:311-322).  Q18: write_q18_csv
 // (query_q18.cpp:465-484) with its date format (:41-52) and field quoting
 // (:65-79).
 #include <algorithm>
@@ -148,6 +149,26 @@ int bolap_q6_write_csv(const char* path, const bolap_q6_result* result) {
     out << "revenue\n";
     out.setf(std::ios::fixed);
     out << std::setprecision(2) << double(result->revenue) / (100.0 * 100.0) << "\n";
+    return out.good() ? BOLAP_OK : BOLAP_ERR_ARG;
+}
+
+int bolap_q14_write_csv(const char* path, const bolap_q14_result* result) {
+    BOLAP_ARG_CHECK(path && result, "bolap_q14_write_csv: bad arguments");
+    std::ofstream out(path);
+    BOLAP_ARG_CHECK(out.good(), "bolap_q14_write_csv: cannot open %s", path);
+    out << "promo_revenue\n";
+    out.setf(std::ios::fixed);
+    out << std::setprecision(2) << result->promo_revenue << "\n";
+    return out.good() ? BOLAP_OK : BOLAP_ERR_ARG;
+}
+
+int bolap_q17_write_csv(const char* path, const bolap_q17_result* result) {
+    BOLAP_ARG_CHECK(path && result, "bolap_q17_write_csv: bad arguments");
+    std::ofstream out(path);
+    BOLAP_ARG_CHECK(out.good(), "bolap_q17_write_csv: cannot open %s", path);
+    out << "avg_yearly\n";
+    out.setf(std::ios::fixed);
+    out << std::setprecision(2) << double(result->sum_extendedprice) / (7.0 * 100.0) << "\n";
     return out.good() ? BOLAP_OK : BOLAP_ERR_ARG;
 }
 

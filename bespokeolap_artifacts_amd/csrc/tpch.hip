@@ -366,6 +366,122 @@ __global__ __launch_bounds__(kT) void gather_q6_k(const uint32_t* __restrict__ p
     }
 }
 
+// ----------------------------------------------------------------- Q14 ---
+// partkey -> promo flag (query_q14.cpp:202-223): the largest part row of a
+// key wins (the reference's sequential assignment lets the last row win),
+// packed as (row + 1) << 1 | promo under an atomic maximum, then narrowed to
+// int8 (-1 absent).
+__global__ __launch_bounds__(kT) void q14_part_k(const int32_t* __restrict__ pk,
+                                                 const uint8_t* __restrict__ promo, int64_t n,
+                                                 int32_t* packed, int64_t size) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = pk[i];
+        if (k >= 0 && k < size) atomicMax(packed + k, int32_t(((i + 1) << 1) | (promo[i] ? 1 : 0)));
+    }
+}
+
+__global__ __launch_bounds__(kT) void q14_narrow_k(const int32_t* __restrict__ packed,
+                                                   int64_t size, int8_t* flag) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < size;
+         i += int64_t(gridDim.x) * kT)
+        flag[i] = packed[i] == 0 ? int8_t(-1) : int8_t(packed[i] & 1);
+}
+
+struct Q14K {
+    const int16_t* ship;
+    const int32_t* pk;
+    const int32_t* dp;
+    int64_t rows;
+    int32_t ship_lo, ship_hi;
+    const int8_t* flag;
+    int64_t size;
+};
+
+BOLAP_DEV void q14_row(const Q14K& a, int64_t i, uint64_t& promo, uint64_t& total,
+                       uint64_t& cnt) {
+    const int32_t k = a.pk[i];
+    if (k < 0 || k >= a.size) return;
+    const int8_t f = a.flag[k];
+    if (f < 0) return;
+    const int64_t v = a.dp[i];
+    total += uint64_t(v);
+    promo += f ? uint64_t(v) : 0ull;
+    cnt += 1;
+}
+
+// Block sums of three u64 values -> partials[3 * b].
+BOLAP_DEV void block_out3(uint64_t x, uint64_t y, uint64_t z, int64_t* partials) {
+    __shared__ uint64_t s[3][kT / kWave];
+    x = wave_sum_u64(x);
+    y = wave_sum_u64(y);
+    z = wave_sum_u64(z);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s[0][threadIdx.x / kWave] = x;
+        s[1][threadIdx.x / kWave] = y;
+        s[2][threadIdx.x / kWave] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+        for (int w = 0; w < kT / kWave; ++w) t += s[threadIdx.x][w];
+        partials[3 * int64_t(blockIdx.x) + threadIdx.x] = int64_t(t);
+    }
+}
+
+// Lineitem pass: 8 rows per lane; the part key and discounted price of a
+// unit are loaded only when one of its ship dates is in the window; zones
+// (when given) skip blocks of 2048 rows outside the window.
+template <bool kZoned>
+__global__ __launch_bounds__(kT) void q14_scan_k(Q14K a, const bolap_zone* __restrict__ zones,
+                                                 int64_t* partials) {
+    uint64_t promo = 0, total = 0, cnt = 0;
+    const int64_t units = a.rows / 8;
+    const int64_t n_zones = (a.rows + kZoneRows - 1) / kZoneRows;
+    const int64_t groups = kZoned ? n_zones : (units + kT - 1) / kT;
+    for (int64_t z = blockIdx.x; z < groups; z += gridDim.x) {
+        bool full = false;
+        if (kZoned) {
+            const bolap_zone zn = zones[z];
+            if (zn.ship_max < a.ship_lo || zn.ship_min >= a.ship_hi) continue;
+            full = zn.ship_min >= a.ship_lo && zn.ship_max < a.ship_hi;
+        }
+        const int64_t u = z * kT + threadIdx.x;  // zone z = units z*256 .. z*256+255
+        if (u >= units) continue;
+        uint32_t m = 0xffu;
+        if (!full) {
+            const VecT<int16_t, 8> sv = load_unit<int16_t, 8>(a.ship, u);
+            m = 0;
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                m |= (sv.v[r] >= a.ship_lo && sv.v[r] < a.ship_hi ? 1u : 0u) << r;
+        }
+        if (!m) continue;
+        const VecT<int32_t, 4> k0 = load_unit<int32_t, 4>(a.pk, 2 * u);
+        const VecT<int32_t, 4> k1 = load_unit<int32_t, 4>(a.pk, 2 * u + 1);
+        const VecT<int32_t, 4> d0 = load_unit<int32_t, 4>(a.dp, 2 * u);
+        const VecT<int32_t, 4> d1 = load_unit<int32_t, 4>(a.dp, 2 * u + 1);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (!((m >> r) & 1u)) continue;
+            const int32_t k = r < 4 ? k0.v[r] : k1.v[r - 4];
+            if (k < 0 || k >= a.size) continue;
+            const int8_t f = a.flag[k];
+            if (f < 0) continue;
+            const int64_t v = r < 4 ? d0.v[r] : d1.v[r - 4];
+            total += uint64_t(v);
+            promo += f ? uint64_t(v) : 0ull;
+            cnt += 1;
+        }
+    }
+    // Rows past the last whole unit.
+    if (blockIdx.x == 0 && int64_t(threadIdx.x) < a.rows - units * 8) {
+        const int64_t i = units * 8 + threadIdx.x;
+        if (a.ship[i] >= a.ship_lo && a.ship[i] < a.ship_hi) q14_row(a, i, promo, total, cnt);
+    }
+    block_out3(promo, total, cnt, partials);
+}
+
 // ----------------------------------------------------------------- Q18 ---
 enum { kQ18MaxL = 0, kQ18MaxO = 1, kQ18MaxC = 2, kQ18Unsorted = 3, kQ18Count = 4,
        kQ18ResN = 8 };
@@ -822,6 +938,98 @@ int bolap_q6_parse(const char* date, const char* discount, const char* quantity,
     int32_t mhi = ey * 12 + (m - 1);
     if (ed == 1) mhi -= 1;
     out->month_hi = mhi < out->month_lo ? out->month_lo : mhi;
+    return BOLAP_OK;
+}
+
+int bolap_q14_parse(const char* date, int32_t* ship_lo, int32_t* ship_hi) {
+    BOLAP_ARG_CHECK(ship_lo && ship_hi, "bolap_q14_parse: null output");
+    int y, m, d;
+    BOLAP_ARG_CHECK(parse_ymd(date, y, m, d), "bolap_q14_parse: bad DATE '%s'", date ? date : "");
+    // query_q14.cpp:54-74: one month later, the day clamped to that month.
+    int mi = m;  // month index of DATE + 1 month, 0-based
+    int ey = y + mi / 12;
+    mi %= 12;
+    const int em = mi + 1;
+    const int ed = d > days_in_month(ey, em) ? days_in_month(ey, em) : d;
+    *ship_lo = int32_t(int16_t(days_from_civil(y, m, d) - kBaseDays));
+    *ship_hi = int32_t(int16_t(days_from_civil(ey, em, ed) - kBaseDays));
+    return BOLAP_OK;
+}
+
+int bolap_q14(bolap_ctx* ctx, const bolap_q14_args* a, bolap_q14_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_q14: null argument");
+    BOLAP_ARG_CHECK(a->li_rows >= 0 && a->part_rows >= 0 && a->part_rows < (int64_t(1) << 30),
+                    "bolap_q14: bad row counts");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (a->l_ship && a->l_partkey && a->l_disc_price),
+                    "bolap_q14: null lineitem column");
+    BOLAP_ARG_CHECK(a->part_rows == 0 || (a->p_partkey && a->p_promo),
+                    "bolap_q14: null part column");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (aligned(a->l_ship, 16) && aligned(a->l_partkey, 16) &&
+                                        aligned(a->l_disc_price, 16)),
+                    "bolap_q14: lineitem columns must be 16-byte aligned");
+    std::memset(out, 0, sizeof(*out));
+    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+    int64_t h[kQ18ResN];
+    {
+        TraceScope ts(ctx, "q14_part_scan");
+        hipLaunchKernelGGL(q18_init_k, dim3(1), dim3(kT), 0, ctx->stream, res);
+        BOLAP_TRY(launch_check("q18_init_k"));
+        hipLaunchKernelGGL(q18_max_k, dim3(grid_of(ctx, a->part_rows / 4, 4)), dim3(kT), 0,
+                           ctx->stream, nullptr, 0, nullptr, 0, a->p_partkey, a->part_rows, res);
+        BOLAP_TRY(launch_check("q18_max_k"));
+        BOLAP_TRY(read_result(ctx, res, kQ18ResN, h));
+    }
+    if (h[kQ18MaxC] < 0) return BOLAP_OK;  // query_q14.cpp:193-200
+    const int64_t size = h[kQ18MaxC] + 1;
+    const int64_t units = a->li_rows / 8;
+    const int64_t n_zones = (a->li_rows + kZoneRows - 1) / kZoneRows;
+    const int64_t groups = a->zones ? n_zones : (units + kT - 1) / kT;
+    int64_t grid = groups < int64_t(ctx->num_cus) * 8 ? groups : int64_t(ctx->num_cus) * 8;
+    if (grid < 1) grid = 1;
+    WsArena ws;
+    const size_t o_packed = ws.plan(size_t(size) * 4);
+    const size_t o_flag = ws.plan(size_t(size));
+    const size_t o_part = ws.plan(size_t(grid) * 3 * sizeof(int64_t));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int64_t)));
+    ws.base = static_cast<char*>(ctx->ws);
+    int32_t* packed = ws.at<int32_t>(o_packed);
+    int8_t* flag = ws.at<int8_t>(o_flag);
+    {
+        TraceScope ts(ctx, "q14_part_map");
+        hipLaunchKernelGGL(fill_i32_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
+                           packed, size, 0);
+        BOLAP_TRY(launch_check("fill_i32_k"));
+        hipLaunchKernelGGL(q14_part_k, dim3(grid_of(ctx, a->part_rows, 4)), dim3(kT), 0,
+                           ctx->stream, a->p_partkey, a->p_promo, a->part_rows, packed, size);
+        BOLAP_TRY(launch_check("q14_part_k"));
+        hipLaunchKernelGGL(q14_narrow_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
+                           packed, size, flag);
+        BOLAP_TRY(launch_check("q14_narrow_k"));
+    }
+    Q14K k{a->l_ship, a->l_partkey, a->l_disc_price, a->li_rows, a->ship_lo, a->ship_hi, flag,
+           size};
+    {
+        TraceScope ts(ctx, "q14_lineitem_scan");
+        if (a->zones)
+            hipLaunchKernelGGL(q14_scan_k<true>, dim3(unsigned(grid)), dim3(kT), 0, ctx->stream,
+                               k, a->zones, ws.at<int64_t>(o_part));
+        else
+            hipLaunchKernelGGL(q14_scan_k<false>, dim3(unsigned(grid)), dim3(kT), 0, ctx->stream,
+                               k, nullptr, ws.at<int64_t>(o_part));
+        BOLAP_TRY(launch_check("q14_scan_k"));
+    }
+    hipLaunchKernelGGL(q6_merge_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<int64_t>(o_part),
+                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev));
+    BOLAP_TRY(launch_check("q6_merge_k"));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t* hp = static_cast<const int64_t*>(ctx->pinned);
+    out->promo_sum = hp[0];
+    out->total_sum = hp[1];
+    out->rows_emitted = hp[2];
+    out->promo_revenue =
+        out->total_sum > 0 ? 100.0 * double(out->promo_sum) / double(out->total_sum) : 0.0;
+    trace_count(ctx, "q14_lineitem_scan_rows_emitted", out->rows_emitted);
     return BOLAP_OK;
 }
 

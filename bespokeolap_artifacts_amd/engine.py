@@ -399,6 +399,69 @@ class Engine:
             out["perm"] = perm[:rows]
         return out
 
+    # ---------------------------------------------------------- Q14 / Q17
+    @staticmethod
+    def q14_window(date: str) -> tuple[int, int]:
+        lo, hi = C.c_int32(), C.c_int32()
+        N.check(N.load().bolap_q14_parse(date.encode(), C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def q14(self, t: dict, ship_lo: int, ship_hi: int, zones=None) -> dict:
+        a = N.Q14Args()
+        a.li_rows = int(t["l_ship"].numel())
+        a.l_ship, a.l_partkey = ptr(t["l_ship"]), ptr(t["l_partkey"])
+        a.l_disc_price = ptr(t["l_disc_price"])
+        a.ship_lo, a.ship_hi = ship_lo, ship_hi
+        a.zones = ptr(zones) if zones is not None else None
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_promo = ptr(t["p_partkey"]), ptr(t["p_promo"])
+        r = N.Q14Result()
+        N.check(self.lib.bolap_q14(self.ctx, C.byref(a), C.byref(r)))
+        return {"promo_sum": r.promo_sum, "total_sum": r.total_sum,
+                "rows_emitted": r.rows_emitted, "promo_revenue": r.promo_revenue}
+
+    def q17_args(self, t: dict, preds: list) -> N.Q17Args:
+        a = N.Q17Args()
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey = ptr(t["p_partkey"])
+        a.n_part_preds = len(preds)
+        for i, p in enumerate(preds):
+            a.part_preds[i] = p
+        a.li_rows = int(t["l_partkey"].numel())
+        a.l_partkey, a.l_quantity, a.l_price = (ptr(t[k]) for k in
+                                                ("l_partkey", "l_qty", "l_price"))
+        return a
+
+    def q17_run(self, a: N.Q17Args) -> dict:
+        r = N.Q17Result()
+        N.check(self.lib.bolap_q17(self.ctx, C.byref(a), C.byref(r)))
+        return {"sum_extendedprice": r.sum_extendedprice, "parts": r.parts,
+                "matches": r.matches, "rows_emitted": r.rows_emitted}
+
+    def q17(self, t: dict, preds: list) -> dict:
+        return self.q17_run(self.q17_args(t, preds))
+
+    def gen_part_lineitem(self, seed: int, parts: int, rows: int) -> dict:
+        """Part (keys 1..parts; brand U{0..24}, container U{0..39}, type
+        U{0..149} with the first 25 types PROMO) and lineitem rows in the
+        reference layout (ship i16, partkey U{1..parts}, quantity x100 i16,
+        extendedprice i32, truncated discounted_price i32)."""
+        torch = _torch()
+        li = self.gen_q6_lineitem(seed, rows)
+        dev = f"cuda:{self.device}"
+        pk = torch.empty(max(1, parts), dtype=torch.int32, device=dev)[:parts]
+        N.check(self.lib.bolap_gen_iota_i32(self.ctx, parts, 1, ptr(pk) if parts else None))
+        disc = li["disc"].to(torch.int64)
+        dp = (li["price"].to(torch.int64) * (100 - disc)).div(100, rounding_mode="trunc")
+        return {"p_partkey": pk,
+                "p_brand": self.gen_uniform_i32(seed, 0x800, parts, 0, 24).to(torch.uint8),
+                "p_container": self.gen_uniform_i32(seed, 0x801, parts, 0, 39).to(torch.uint8),
+                "p_promo": (self.gen_uniform_i32(seed, 0x802, parts, 0, 149) < 25).to(torch.uint8),
+                "l_ship": li["ship"], "l_qty": li["qty"], "l_price": li["price"],
+                "l_disc": li["disc"],
+                "l_partkey": self.gen_uniform_i32(seed, 0x803, rows, 1, max(1, parts)),
+                "l_disc_price": dp.to(torch.int32)}
+
     # -------------------------------------------------------------- Q18
     @staticmethod
     def q18_args(t: dict, threshold: int, orderkey_sorted: bool = True) -> N.Q18Args:

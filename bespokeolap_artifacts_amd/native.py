@@ -202,6 +202,28 @@ class Q18Stats(C.Structure):
                 ("fallbacks", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Q14Args(C.Structure):
+    _fields_ = [("li_rows", C.c_int64), ("l_ship", vp), ("l_partkey", vp),
+                ("l_disc_price", vp), ("ship_lo", C.c_int32), ("ship_hi", C.c_int32),
+                ("zones", vp), ("part_rows", C.c_int64), ("p_partkey", vp), ("p_promo", vp)]
+
+
+class Q14Result(C.Structure):
+    _fields_ = [("promo_sum", C.c_int64), ("total_sum", C.c_int64),
+                ("rows_emitted", C.c_int64), ("promo_revenue", C.c_double)]
+
+
+class Q17Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_partkey", vp), ("n_part_preds", C.c_int32),
+                ("part_preds", Pred * MAX_PREDS), ("li_rows", C.c_int64), ("l_partkey", vp),
+                ("l_quantity", vp), ("l_price", vp)]
+
+
+class Q17Result(C.Structure):
+    _fields_ = [("sum_extendedprice", C.c_int64), ("parts", C.c_int64),
+                ("matches", C.c_int64), ("rows_emitted", C.c_int64)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -252,6 +274,11 @@ PROTOTYPES = {
     "bolap_q1_write_csv": (C.c_int, [C.c_char_p, P(Q1Row), i32]),
     "bolap_q6_write_csv": (C.c_int, [C.c_char_p, P(Q6Result)]),
     "bolap_q18_write_csv": (C.c_int, [C.c_char_p, P(Q18Row), i64, P(C.c_char_p), i64]),
+    "bolap_q14_parse": (C.c_int, [C.c_char_p, P(i32), P(i32)]),
+    "bolap_q14": (C.c_int, [vp, P(Q14Args), P(Q14Result)]),
+    "bolap_q14_write_csv": (C.c_int, [C.c_char_p, P(Q14Result)]),
+    "bolap_q17": (C.c_int, [vp, P(Q17Args), P(Q17Result)]),
+    "bolap_q17_write_csv": (C.c_int, [C.c_char_p, P(Q17Result)]),
     "bolap_q6_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P(Q6Params)]),
     "bolap_q6": (C.c_int, [vp, P(Q6Args), P(Q6Result)]),
     "bolap_q6_zones": (C.c_int, [vp, i64, vp, vp, vp, vp]),

@@ -233,6 +233,32 @@ typedef struct {
 int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* args,
                    bolap_join_sum_result* out);
 
+/* Q17: parts passing the predicates (brand / container codes) -> per-part
+   SUM(quantity) and COUNT over their lineitem rows -> SUM(extendedprice) of
+   the rows with quantity * 5 * count < SUM(quantity), i.e. below 0.2 times
+   the part's average.  Replaces run_q17 (bespoke_tpch/query_q17.cpp:
+   119-309); parts with negative keys are skipped as there (:168-171).  At
+   most BOLAP_MAX_PREDS - 1 predicates. */
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    int32_t n_part_preds;
+    bolap_pred part_preds[BOLAP_MAX_PREDS];
+    int64_t li_rows;
+    const int32_t* l_partkey;
+    const int16_t* l_quantity;       /* x 100 */
+    const int32_t* l_price;          /* extendedprice, cents */
+} bolap_q17_args;
+
+typedef struct {
+    int64_t sum_extendedprice;  /* exact; avg_yearly = this / 7 / 100 */
+    int64_t parts;              /* part rows that passed */
+    int64_t matches;            /* lineitem rows of passing parts */
+    int64_t rows_emitted;       /* rows below the threshold */
+} bolap_q17_result;
+
+int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* args, bolap_q17_result* out);
+
 /* ------------------------------------------------------------------------ */
 /* Q3 shape: customer filter -> orders filter + join -> fact join ->          */
 /* GROUP BY order key SUM(price * (100 - disc)) -> ORDER BY revenue DESC,     */
@@ -598,6 +624,39 @@ int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* args, bolap_q18_row* rows_ou
 int bolap_q6_write_csv(const char* path, const bolap_q6_result* result);
 int bolap_q18_write_csv(const char* path, const bolap_q18_row* rows, int64_t n,
                         const char* const* c_name, int64_t n_names);
+
+/* ------------------------------------------------------------------------ */
+/* Q14 shape: lineitem rows shipped in a one-month window joined to part    */
+/* through a dense partkey -> promo-flag array, SUM(discounted_price) in     */
+/* total and for promo parts.  Replaces run_q14 (bespoke_tpch/query_q14.cpp: */
+/* 176-365); the flag array is built per call over 0..max part key with the  */
+/* last part row of a key winning, as there.                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t li_rows;
+    const int16_t* l_ship;
+    const int32_t* l_partkey;
+    const int32_t* l_disc_price;   /* truncated discounted_price (cents) */
+    int32_t ship_lo, ship_hi;      /* ship_lo <= ship < ship_hi */
+    const bolap_zone* zones;       /* optional zone map of these rows (bolap_q6_zones) */
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const uint8_t* p_promo;        /* 1 when the part's type starts with "PROMO" */
+} bolap_q14_args;
+
+typedef struct {
+    int64_t promo_sum, total_sum, rows_emitted;
+    double promo_revenue;          /* 100 * promo / total, 0 when total <= 0 */
+} bolap_q14_result;
+
+/* [DATE, DATE + 1 month) as day offsets, the day clamped to the month
+   (query_q14.cpp:46-74).  Host only. */
+int bolap_q14_parse(const char* date, int32_t* ship_lo, int32_t* ship_hi);
+int bolap_q14(bolap_ctx* ctx, const bolap_q14_args* args, bolap_q14_result* out);
+/* write_q14_csv (query_q14.cpp:367-375), write_q17_csv (query_q17.cpp:311-322).
+   Host only. */
+int bolap_q14_write_csv(const char* path, const bolap_q14_result* result);
+int bolap_q17_write_csv(const char* path, const bolap_q17_result* result);
 
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */

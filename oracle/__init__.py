@@ -458,6 +458,16 @@ Q18_DTYPE = np.dtype([("custkey", "<i4"), ("orderkey", "<i4"), ("orderdate", "<i
                       ("order_row", "<i4")])
 
 
+class Q14Out(C.Structure):
+    _fields_ = [("promo_sum", i64), ("total_sum", i64), ("rows_emitted", i64),
+                ("promo_revenue", C.c_double)]
+
+
+class Q17Out(C.Structure):
+    _fields_ = [("sum_extendedprice", i64), ("parts", i64), ("matches", i64),
+                ("rows_emitted", i64)]
+
+
 def _tpch_lib():
     L = lib()
     if not getattr(L, "_tpch_ready", False):
@@ -474,6 +484,9 @@ def _tpch_lib():
         L.oracle_derive_lineitem.argtypes = [i64, vp, vp, vp, vp, i32, vp, vp, vp]
         L.oracle_date_order.argtypes = [i64, vp, vp]
         L.oracle_order_ranges.argtypes = [i64, vp, i64, vp, i64, vp, i32, vp, vp]
+        L.oracle_q14.argtypes = [i64, vp, vp, vp, i32, i32, i64, vp, vp, C.POINTER(Q14Out)]
+        L.oracle_q17.argtypes = [i64, vp, i32, C.POINTER(Pred), i64, vp, vp, vp,
+                                 C.POINTER(Q17Out)]
         L._tpch_ready = True
     return L
 
@@ -558,3 +571,27 @@ def order_ranges(o_orderkey, l_orderkey, li_sorted: bool):
     _tpch_lib().oracle_order_ranges(len(ok), _p(ok), map_size, _p(m), len(lk), _p(lk),
                                     int(li_sorted), _p(start), _p(end))
     return m[:map_size], start[:len(ok)], end[:len(ok)]
+
+
+def q14(li: dict, part: dict, ship_lo: int, ship_hi: int) -> dict:
+    a = [np.ascontiguousarray(li[k], dt) for k, dt in
+         (("ship", np.int16), ("partkey", np.int32), ("disc_price", np.int32))]
+    pk = np.ascontiguousarray(part["partkey"], np.int32)
+    pr = np.ascontiguousarray(part["promo"], np.uint8)
+    out = Q14Out()
+    _tpch_lib().oracle_q14(len(a[0]), *[_p(x) for x in a], ship_lo, ship_hi, len(pk), _p(pk),
+                           _p(pr), C.byref(out))
+    return {"promo_sum": out.promo_sum, "total_sum": out.total_sum,
+            "rows_emitted": out.rows_emitted, "promo_revenue": out.promo_revenue}
+
+
+def q17(part_key, preds: list, li: dict) -> dict:
+    pk = np.ascontiguousarray(part_key, np.int32)
+    a = [np.ascontiguousarray(li[k], dt) for k, dt in
+         (("partkey", np.int32), ("qty", np.int16), ("price", np.int32))]
+    arr = (Pred * max(1, len(preds)))(*preds)
+    out = Q17Out()
+    _tpch_lib().oracle_q17(len(pk), _p(pk), len(preds), arr, len(a[0]), *[_p(x) for x in a],
+                           C.byref(out))
+    return {"sum_extendedprice": out.sum_extendedprice, "parts": out.parts,
+            "matches": out.matches, "rows_emitted": out.rows_emitted}

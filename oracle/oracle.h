@@ -196,6 +196,23 @@ void oracle_order_ranges(int64_t o_rows, const int32_t* o_orderkey, int64_t map_
                          int32_t* map, int64_t li_rows, const int32_t* l_orderkey,
                          int32_t li_sorted, uint32_t* start, uint32_t* end);
 
+typedef struct {
+    int64_t promo_sum, total_sum, rows_emitted;
+    double promo_revenue;
+} oracle_q14_out;
+
+void oracle_q14(int64_t li_rows, const int16_t* ship, const int32_t* partkey,
+                const int32_t* disc_price, int32_t ship_lo, int32_t ship_hi, int64_t p_rows,
+                const int32_t* p_partkey, const uint8_t* p_promo, oracle_q14_out* out);
+
+typedef struct {
+    int64_t sum_extendedprice, parts, matches, rows_emitted;
+} oracle_q17_out;
+
+void oracle_q17(int64_t p_rows, const int32_t* p_partkey, int32_t n_preds,
+                const oracle_pred* preds, int64_t li_rows, const int32_t* l_partkey,
+                const int16_t* l_qty, const int32_t* l_price, oracle_q17_out* out);
+
 /* --- planted mistakes, for tests that show the checks have teeth -------- */
 void oracle_set_mutation(int32_t m);
 int32_t oracle_mutation(void);

@@ -377,3 +377,102 @@ void oracle_order_ranges(int64_t o_rows, const int32_t* o_orderkey, int64_t map_
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Q14: bespoke_tpch/query_q14.cpp:176-365.  partkey -> promo flag over      */
+/* 0..max part key (-1 absent; the last part row with a key wins), then every */
+/* lineitem row with ship_lo <= ship < ship_hi whose part is known adds its   */
+/* discounted_price to the total and, for promo parts, to the promo sum.      */
+/* promo_revenue = 100 * promo / total (0 when total <= 0).                   */
+/* ------------------------------------------------------------------------ */
+void oracle_q14(int64_t li_rows, const int16_t* ship, const int32_t* partkey,
+                const int32_t* disc_price, int32_t ship_lo, int32_t ship_hi, int64_t p_rows,
+                const int32_t* p_partkey, const uint8_t* p_promo, oracle_q14_out* out) {
+    memset(out, 0, sizeof(*out));
+    const int32_t mp = max_i32(p_partkey, p_rows);
+    if (mp < 0) return;
+    const int64_t size = (int64_t)mp + 1;
+    int8_t* flag = (int8_t*)malloc((size_t)size);
+    memset(flag, -1, (size_t)size);
+    for (int64_t r = 0; r < p_rows; ++r)
+        if (p_partkey[r] >= 0) flag[p_partkey[r]] = (int8_t)(p_promo[r] ? 1 : 0);
+    int64_t promo = 0, total = 0, emitted = 0;
+    for (int64_t i = 0; i < li_rows; ++i) {
+        /* planted mistake 12: an inclusive end date */
+        if (ship[i] < ship_lo || (oracle_mutation() == 12 ? ship[i] > ship_hi : ship[i] >= ship_hi))
+            continue;
+        const int32_t k = partkey[i];
+        if (k < 0 || k >= size) continue;
+        const int8_t f = flag[k];
+        if (f < 0) continue;
+        total += disc_price[i];
+        if (f) promo += disc_price[i];
+        emitted += 1;
+    }
+    free(flag);
+    out->promo_sum = promo;
+    out->total_sum = total;
+    out->rows_emitted = emitted;
+    out->promo_revenue = total > 0 ? 100.0 * (double)promo / (double)total : 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q17: bespoke_tpch/query_q17.cpp:119-309.  Parts passing the predicates     */
+/* set bits over 0..max part key; every lineitem row of such a part adds its  */
+/* quantity to sum_qty[part] and 1 to cnt[part] and is kept; a kept row adds  */
+/* its extendedprice when quantity * 5 * cnt < sum_qty (0.2 * average,       */
+/* :288-291).                                                                 */
+/* ------------------------------------------------------------------------ */
+void oracle_q17(int64_t p_rows, const int32_t* p_partkey, int32_t n_preds,
+                const oracle_pred* preds, int64_t li_rows, const int32_t* l_partkey,
+                const int16_t* l_qty, const int32_t* l_price, oracle_q17_out* out) {
+    memset(out, 0, sizeof(*out));
+    const int32_t mp = max_i32(p_partkey, p_rows);
+    if (mp < 0) return;
+    const int64_t size = (int64_t)mp + 1;
+    uint8_t* match = (uint8_t*)calloc((size_t)size, 1);
+    int64_t parts = 0;
+    for (int64_t r = 0; r < p_rows; ++r) {
+        const int32_t k = p_partkey[r];
+        if (k < 0 || k > mp) continue;
+        int ok = 1;
+        for (int32_t j = 0; j < n_preds; ++j) ok = ok && oracle_pred_row(&preds[j], r);
+        if (ok) {
+            match[k] = 1;
+            parts += 1;
+        }
+    }
+    out->parts = parts;
+    if (parts == 0) {
+        free(match);
+        return;
+    }
+    int64_t* sum = (int64_t*)calloc((size_t)size, sizeof(int64_t));
+    int32_t* cnt = (int32_t*)calloc((size_t)size, sizeof(int32_t));
+    int64_t* kept = (int64_t*)malloc(sizeof(int64_t) * (size_t)(li_rows > 0 ? li_rows : 1));
+    int64_t nk = 0;
+    for (int64_t i = 0; i < li_rows; ++i) {
+        const uint32_t k = (uint32_t)l_partkey[i];
+        if (k >= (uint64_t)size || !match[k]) continue;
+        sum[k] += l_qty[i];
+        cnt[k] += 1;
+        kept[nk++] = i;
+    }
+    int64_t ext = 0, emitted = 0;
+    for (int64_t j = 0; j < nk; ++j) {
+        const int64_t i = kept[j];
+        const int32_t k = l_partkey[i];
+        const int64_t lhs = (int64_t)l_qty[i] * 5 * cnt[k];
+        /* planted mistake 13: quantity <= 0.2 * average */
+        if (oracle_mutation() == 13 ? lhs > sum[k] : lhs >= sum[k]) continue;
+        ext += l_price[i];
+        emitted += 1;
+    }
+    free(match);
+    free(sum);
+    free(cnt);
+    free(kept);
+    out->matches = nk;
+    out->rows_emitted = emitted;
+    out->sum_extendedprice = ext;
+}

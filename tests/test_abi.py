@@ -56,7 +56,9 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "CebDb": "bolap_ceb_db", "CebQuery": "bolap_ceb_query", "CebResult": "bolap_ceb_result",
     "Zone": "bolap_zone", "Q6Args": "bolap_q6_args", "Q6Result": "bolap_q6_result",
     "Q6Params": "bolap_q6_params", "Q18Args": "bolap_q18_args", "Q18Row": "bolap_q18_row",
-    "Q18Stats": "bolap_q18_stats", "Q1Row": "bolap_q1_row",
+    "Q18Stats": "bolap_q18_stats", "Q1Row": "bolap_q1_row", "Q14Args": "bolap_q14_args",
+    "Q14Result": "bolap_q14_result", "Q17Args": "bolap_q17_args",
+    "Q17Result": "bolap_q17_result",
 }
 
 

@@ -112,3 +112,18 @@ def test_q18_csv(tmp_path):
         lines.append(f'"{name}",{r["custkey"]},{r["orderkey"]},{d.isoformat()},'
                      f'{r["totalprice"] / 100.0:.2f},{r["sum_qty"] / 100.0:.2f}\n')
     assert p.read_text() == "".join(lines)
+
+
+def test_q14_q17_csv(tmp_path):
+    from bespokeolap_artifacts_amd import native as N
+    lib = N.load()
+    for promo, total in ((0, 0), (123, 1000), (987654321, 1234567890)):
+        r = N.Q14Result(promo, total, 1, 100.0 * promo / total if total > 0 else 0.0)
+        p = tmp_path / "q14.csv"
+        N.check(lib.bolap_q14_write_csv(str(p).encode(), r))
+        assert p.read_text() == f"promo_revenue\n{r.promo_revenue:.2f}\n"
+    for s in (0, 7, 123456789012, -35):
+        r = N.Q17Result(s, 0, 0, 0)
+        p = tmp_path / "q17.csv"
+        N.check(lib.bolap_q17_write_csv(str(p).encode(), r))
+        assert p.read_text() == f"avg_yearly\n{s / 700.0:.2f}\n"

@@ -201,3 +201,88 @@ def test_order_ranges_match_oracle(engine):
     hm, hs, he = oracle.order_ranges(ok.cpu().numpy(), lk2.cpu().numpy(), False)
     assert np.array_equal(start.cpu().numpy().astype(np.uint32), hs)
     assert np.array_equal(end.cpu().numpy().astype(np.uint32), he)
+
+
+# ------------------------------------------------------------- Q14 / Q17
+def host_part_li(t: dict):
+    h = host(t)
+    part = {"partkey": h["p_partkey"], "promo": h["p_promo"], "brand": h["p_brand"],
+            "container": h["p_container"]}
+    li = {"ship": h["l_ship"], "partkey": h["l_partkey"], "disc_price": h["l_disc_price"],
+          "qty": h["l_qty"], "price": h["l_price"]}
+    return part, li
+
+
+@pytest.mark.parametrize("parts,rows", [(100, 1000), (20_000, 600_003), (2_000_000, 60_000_000)])
+def test_q14_matches_oracle(engine, parts, rows):
+    t = engine.gen_part_lineitem(SEED, parts, rows)
+    part, li = host_part_li(t)
+    for date in ("1995-09-01", "1992-01-01", "1998-11-30"):
+        lo, hi = engine.q14_window(date)
+        got = engine.q14(t, lo, hi)
+        assert got == oracle.q14(li, part, lo, hi), date
+
+
+def test_q14_over_zones_of_clustered_rows(engine):
+    import torch
+    t = engine.gen_part_lineitem(SEED, 200_000, 6_000_000)
+    cl = engine.q6_cluster({"ship": t["l_ship"], "disc": t["l_disc"], "qty": t["l_qty"],
+                            "price": t["l_price"]}, with_perm=True)
+    perm = cl["perm"].long()
+    t2 = dict(t)
+    for k in ("l_ship", "l_partkey", "l_disc_price", "l_qty", "l_price", "l_disc"):
+        t2[k] = t[k][perm].contiguous()
+    zones = engine.q6_zones({"ship": t2["l_ship"], "disc": t2["l_disc"], "qty": t2["l_qty"]})
+    part, li = host_part_li(t)
+    lo, hi = engine.q14_window("1995-09-01")
+    got = engine.q14(t2, lo, hi, zones)
+    assert got == oracle.q14(li, part, lo, hi)
+    torch.cuda.synchronize()
+
+
+def test_q14_duplicate_and_negative_part_keys(engine):
+    import torch
+    t = engine.gen_part_lineitem(SEED, 50_000, 1_000_000)
+    pk = t["p_partkey"].clone()
+    pk[::9] = pk[1::9][: pk[::9].numel()]
+    pk[3] = -7
+    t["p_partkey"] = pk
+    part, li = host_part_li(t)
+    lo, hi = engine.q14_window("1995-09-01")
+    assert engine.q14(t, lo, hi) == oracle.q14(li, part, lo, hi)
+    t["p_partkey"] = torch.full_like(pk, -1)
+    assert engine.q14(t, lo, hi)["total_sum"] == 0
+
+
+@pytest.mark.parametrize("parts,rows", [(100, 1000), (20_000, 600_003), (2_000_000, 60_000_000)])
+def test_q17_matches_oracle(engine, parts, rows):
+    t = engine.gen_part_lineitem(SEED, parts, rows)
+    part, li = host_part_li(t)
+    for brand, container in ((3, 7), (24, 39), (0, 0)):
+        preds = [engine.pred(t["p_brand"], N.OP_RANGE, brand, brand),
+                 engine.pred(t["p_container"], N.OP_RANGE, container, container)]
+        got = engine.q17(t, preds)
+        opreds = [oracle.make_pred(part["brand"], 0, brand, brand),
+                  oracle.make_pred(part["container"], 0, container, container)]
+        assert got == oracle.q17(part["partkey"], opreds, li), (brand, container)
+
+
+def test_q17_many_parts_take_the_hbm_table(engine):
+    t = engine.gen_part_lineitem(SEED, 200_000, 3_000_000)
+    part, li = host_part_li(t)
+    preds = [engine.pred(t["p_brand"], N.OP_RANGE, 0, 12)]  # about half of the parts
+    got = engine.q17(t, preds)
+    want = oracle.q17(part["partkey"], [oracle.make_pred(part["brand"], 0, 0, 12)], li)
+    assert got == want and got["parts"] > 4096
+
+
+def test_q17_duplicate_and_negative_part_keys(engine):
+    t = engine.gen_part_lineitem(SEED, 50_000, 1_000_000)
+    pk = t["p_partkey"].clone()
+    pk[::9] = pk[1::9][: pk[::9].numel()]
+    pk[3] = -7
+    t["p_partkey"] = pk
+    part, li = host_part_li(t)
+    preds = [engine.pred(t["p_brand"], N.OP_RANGE, 0, 3)]
+    got = engine.q17(t, preds)
+    assert got == oracle.q17(part["partkey"], [oracle.make_pred(part["brand"], 0, 0, 3)], li)

@@ -1,5 +1,5 @@
-"""CPU checks of the Q6 / Q18 / build restatements in oracle/tpch.c and of the
-host-side Q6 argument parser in libbolap.
+"""CPU checks of the Q6 / Q14 / Q17 / Q18 / build restatements in oracle/tpch.c
+and of the host-side Q6 / Q14 argument parsers in libbolap.
 
 Each oracle function is compared with an independent formulation (numpy
 masks, pandas group-by/merge, Python datetime, numpy's stable argsort), and
@@ -253,3 +253,105 @@ def test_order_ranges_match_python(sorted_li):
         else:
             assert (start[r], end[r]) == (0, 0)
     assert (m >= 0).sum() == len(ok)
+
+
+# ------------------------------------------------------------- Q14 / Q17
+def add_month_clamped(d: datetime.date) -> datetime.date:
+    y, m = (d.year + 1, 1) if d.month == 12 else (d.year, d.month + 1)
+    for day in (d.day, 30, 29, 28):
+        try:
+            return datetime.date(y, m, day)
+        except ValueError:
+            continue
+
+
+@pytest.mark.parametrize("date", ["1995-09-01", "1996-01-31", "1997-12-15", "1996-02-29",
+                                  "1995-01-30"])
+def test_q14_window_follows_the_reference(date):
+    from bespokeolap_artifacts_amd.engine import Engine
+    d = datetime.date.fromisoformat(date)
+    assert Engine.q14_window(date) == (offset(d), offset(add_month_clamped(d)))
+
+
+def part_tables(seed: int, parts: int, rows: int, dup: bool = False):
+    rng = np.random.default_rng(seed)
+    pk = np.arange(1, parts + 1, dtype=np.int32)
+    if dup:
+        pk[::9] = pk[1::9][: len(pk[::9])] if parts > 9 else pk[::9]
+        pk[3] = -7
+    part = {"partkey": pk, "brand": rng.integers(0, 25, parts).astype(np.uint8),
+            "container": rng.integers(0, 40, parts).astype(np.uint8),
+            "promo": (rng.integers(0, 150, parts) < 25).astype(np.uint8)}
+    li = {"ship": rng.integers(0, 2526, rows).astype(np.int16),
+          "partkey": rng.integers(-3, parts + 5, rows).astype(np.int32),
+          "qty": (rng.integers(1, 51, rows) * 100).astype(np.int16),
+          "price": rng.integers(90_000, 5_000_000, rows).astype(np.int32)}
+    li["disc_price"] = rng.integers(80_000, 5_000_000, rows).astype(np.int32)
+    return part, li
+
+
+def q14_pandas(part, li, lo, hi):
+    p = pd.DataFrame({"k": part["partkey"], "promo": part["promo"].astype(np.int64),
+                      "row": np.arange(len(part["partkey"]))})
+    p = p[p["k"] >= 0].sort_values("row").drop_duplicates("k", keep="last")
+    m = pd.DataFrame({"k": li["partkey"], "s": li["ship"].astype(np.int64),
+                      "v": li["disc_price"].astype(np.int64)})
+    m = m[(m["s"] >= lo) & (m["s"] < hi)].merge(p, on="k")
+    total, promo = int(m["v"].sum()), int((m["v"] * m["promo"]).sum())
+    return {"promo_sum": promo, "total_sum": total, "rows_emitted": len(m),
+            "promo_revenue": 100.0 * promo / total if total > 0 else 0.0}
+
+
+@pytest.mark.parametrize("dup", [False, True])
+def test_q14_matches_pandas(dup):
+    part, li = part_tables(3, 5000, 200_000, dup)
+    for date in ("1995-09-01", "1993-02-01", "1998-12-01"):
+        from bespokeolap_artifacts_amd.engine import Engine
+        lo, hi = Engine.q14_window(date)
+        assert oracle.q14(li, part, lo, hi) == q14_pandas(part, li, lo, hi)
+
+
+def q17_pandas(part, li, brand, container):
+    p = pd.DataFrame({"k": part["partkey"], "b": part["brand"], "c": part["container"]})
+    p = p[(p["k"] >= 0) & (p["b"] == brand) & (p["c"] == container)]
+    keys = set(p["k"].tolist())
+    m = pd.DataFrame({"k": li["partkey"], "q": li["qty"].astype(np.int64),
+                      "p": li["price"].astype(np.int64)})
+    m = m[m["k"].isin(keys)]
+    g = m.groupby("k")["q"].agg(["sum", "count"])
+    m = m.join(g, on="k")
+    keep = m["q"] * 5 * m["count"] < m["sum"]
+    return {"sum_extendedprice": int(m.loc[keep, "p"].sum()), "parts": len(p),
+            "matches": len(m), "rows_emitted": int(keep.sum())}
+
+
+@pytest.mark.parametrize("dup", [False, True])
+def test_q17_matches_pandas(dup):
+    part, li = part_tables(4, 20_000, 300_000, dup)
+    for brand, container in ((3, 7), (12, 0), (24, 39)):
+        preds = [oracle.make_pred(part["brand"], 0, brand, brand),
+                 oracle.make_pred(part["container"], 0, container, container)]
+        assert oracle.q17(part["partkey"], preds, li) == q17_pandas(part, li, brand, container)
+
+
+@pytest.mark.parametrize("m", [12, 13])
+def test_q14_q17_planted_mistakes_are_caught(m):
+    part, li = part_tables(5, 2000, 200_000)
+    from bespokeolap_artifacts_amd.engine import Engine
+    lo, hi = Engine.q14_window("1995-09-01")
+    li["ship"][::50] = hi  # rows exactly on the exclusive end
+    preds = [oracle.make_pred(part["brand"], 0, 3, 3),
+             oracle.make_pred(part["container"], 0, 7, 7)]
+    # a part whose average puts one row exactly on the threshold:
+    # quantities 100, 600, 600, 600, 600 -> 100 * 5 * 5 == 2500 == sum
+    k = int(part["partkey"][(part["brand"] == 3) & (part["container"] == 7)][0])
+    li["partkey"][:5] = k
+    li["qty"][:5] = [100, 600, 600, 600, 600]
+    li["partkey"][5:][li["partkey"][5:] == k] = k + 1
+    want14, want17 = q14_pandas(part, li, lo, hi), q17_pandas(part, li, 3, 7)
+    oracle.set_mutation(m)
+    try:
+        got14, got17 = oracle.q14(li, part, lo, hi), oracle.q17(part["partkey"], preds, li)
+    finally:
+        oracle.set_mutation(0)
+    assert got14 != want14 if m == 12 else got17 != want17
