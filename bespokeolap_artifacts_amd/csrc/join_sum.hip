@@ -20,7 +20,7 @@
 //            HBM, sized on the device from the list length.  Both kernels
 //            exit at once when the build side goes to LDS.
 //   probe    every workgroup reads the list length, sizes its table from it
-//            and, when it fits, builds its own copy in LDS (load <= 1/4,
+//            and, when it fits, builds its own copy in LDS (load <= 1/2,
 //            128 filter bits per key) from the L2-resident key list; then it
 //            streams the fact key column with 128-bit loads.  For the keys
 //            a lane holds it first issues all Bloom-word reads, forms a hit
@@ -45,10 +45,14 @@ constexpr int32_t kEmpty = INT32_MIN;
 constexpr uint32_t kSlotMul = 0x9E3779B1u;   // Fibonacci hashing
 constexpr uint32_t kBitMul = 0x85EBCA77u;    // independent multiplier for the bitmap
 constexpr int kLdsBudget = 96 * 1024;
-// Device-sized LDS tables: up to 2048 keys in 64 KB (placement AUTO, two
-// workgroups per CU), up to 4096 keys in 96 KB when LDS is forced.
+// Device-sized LDS tables: 2 slots and 2^lds_bits_log2 filter bits per key
+// (knob "join.lds_bloom_bits", 7 by default); up to 2048 keys with placement
+// AUTO (48 KB at 128 bits per key), up to 4096 keys when LDS is forced.
 constexpr int64_t kAutoLdsKeys = 2048;
 constexpr int64_t kForcedLdsKeys = 4096;
+// Matching fact rows a probe workgroup collects before gathering their
+// payload (beyond it, matches are gathered in the loop).
+constexpr uint32_t kDeferMax = 2048;
 
 // Result slot layout (int64): revenue, matches, build rows, flags
 // (bit 1: an INT32_MIN key passed the filter).
@@ -76,6 +80,7 @@ struct Geo {
     int32_t cap_log2;       // fixed only
     int32_t bb_log2;        // fixed only
     int64_t lds_keys_max;   // device sizing: largest build that goes to LDS
+    int32_t lds_bits_log2;  // device sizing: log2 filter bits per key in LDS
 };
 
 // Table geometry for n build keys.  flag: 0 ok, 1 estimate too small
@@ -101,7 +106,8 @@ __host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
     }
     const int64_t m = n < 1 ? 1 : n;
     if (g.placement != BOLAP_PLACE_HBM && n <= g.lds_keys_max)
-        return Shape{true, clog2(m * 4 < 64 ? 64 : m * 4), clampi(clog2(m * 128), 10, 18), 0};
+        return Shape{true, clog2(m * 2 < 64 ? 64 : m * 2),
+                     clampi(clog2(m << g.lds_bits_log2), 10, 18), 0};
     if (g.placement == BOLAP_PLACE_LDS) return Shape{true, 0, 0, 4};
     return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), clampi(clog2(m * 16), 10, 22), 0};
 }
@@ -116,6 +122,8 @@ BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
 // Blocked Bloom filter: one 32-bit word per key (chosen by the top bits of
 // a second multiplicative hash) with two bits set from middle bits of the
 // same hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
+// The false-positive rate must stay far below 1/512: a wave probes 512 keys
+// per tile, and one hit sends the whole wave down the exact-lookup path.
 BOLAP_DEV uint32_t bloom_hash(int32_t key) { return uint32_t(key) * kBitMul; }
 BOLAP_DEV uint32_t bloom_word(uint32_t h, int bb_log2) { return h >> (32 - (bb_log2 - 5)); }
 // Two bit positions from the ten hash bits just below the word index (the
@@ -161,6 +169,20 @@ __global__ __launch_bounds__(256) void zero_res_k(int64_t* res, uint32_t* seg_cn
     if (threadIdx.x < kSegs) seg_cnt[threadIdx.x * kSegStride] = 0;
 }
 
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight (a full __syncthreads waits for them).
+BOLAP_DEV void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// U units of 4 fact keys held by a lane.
+template <int U>
+struct KeyTile {
+    VecT<int32_t, 4> v[U];
+};
+
 // Inclusive prefix sums of the segment lengths into `ends` (LDS), computed
 // by wave 0; returns the build size to every thread of the workgroup.
 BOLAP_DEV int64_t klist_prefix(const KList& kl, uint32_t* ends) {
@@ -175,7 +197,7 @@ BOLAP_DEV int64_t klist_prefix(const KList& kl, uint32_t* ends) {
         ends[threadIdx.x] = c;
         if (threadIdx.x == kWave - 1) s_total = int64_t(c);
     }
-    __syncthreads();
+    lds_barrier();
     return s_total;
 }
 
@@ -359,14 +381,32 @@ __global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ 
 // exact lookup (the filter and the table live in LDS or in HBM depending on
 // where the pointers come from; each call site is inlined with its own
 // address space).
+// The lane's first tile (units base0, base0 + kThreads, ...), clamped into
+// bounds; loaded before the workgroup builds its table so that the HBM
+// latency of the first keys overlaps the build.
+template <int kProbeU>
+BOLAP_DEV KeyTile<kProbeU> first_tile(const int32_t* __restrict__ fk, int64_t rows) {
+    const int64_t n_units = rows / 4;
+    const int64_t base = int64_t(blockIdx.x) * kThreads * kProbeU + threadIdx.x;
+    KeyTile<kProbeU> t{};
+    if (n_units > 0) {
+#pragma unroll
+        for (int k = 0; k < kProbeU; ++k) {
+            int64_t u = base + int64_t(k) * kThreads;
+            u = u < n_units ? u : n_units - 1;
+            t.v[k] = load_unit<int32_t, 4>(fk, u);
+        }
+    }
+    return t;
+}
+
 template <int kProbeU, bool kPipe, class OnCand>
 BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const uint32_t* B,
-                          int bb_log2, OnCand&& on_cand) {
+                          int bb_log2, OnCand&& on_cand,
+                          const KeyTile<kProbeU>* first = nullptr) {
     const int64_t n_units = rows / 4;
     const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
-    struct Tile {
-        VecT<int32_t, 4> v[kProbeU];
-    };
+    using Tile = KeyTile<kProbeU>;
     auto load_tile = [&](Tile& x, int64_t at) {
 #pragma unroll
         for (int k = 0; k < kProbeU; ++k) {
@@ -409,16 +449,46 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
     };
     int64_t base = int64_t(blockIdx.x) * kThreads * kProbeU + threadIdx.x;
     if constexpr (!kPipe) {
-        for (; base < n_units; base += stride) {
+        // Whole tiles without clamping or masking, then the remaining units
+        // one at a time.
+        for (; base + int64_t(kProbeU - 1) * kThreads < n_units; base += stride) {
             Tile A;
-            load_tile(A, base);
-            use_tile(A, base);
+#pragma unroll
+            for (int k = 0; k < kProbeU; ++k)
+                A.v[k] = load_unit<int32_t, 4>(fk, base + int64_t(k) * kThreads);
+            constexpr int kKeys = kProbeU * 4;
+            uint32_t hv[kKeys], wv[kKeys];
+#pragma unroll
+            for (int i = 0; i < kKeys; ++i) {
+                hv[i] = bloom_hash(A.v[i / 4].v[i % 4]);
+                wv[i] = B[bloom_word(hv[i], bb_log2)];
+            }
+            uint32_t hits = 0;
+#pragma unroll
+            for (int i = 0; i < kKeys; ++i)
+                hits |= (bloom_test(wv[i], hv[i], bb_log2) ? 1u : 0u) << i;
+            if (hits) {
+#pragma unroll
+                for (int i = 0; i < kKeys; ++i)
+                    if (hits & (1u << i))
+                        on_cand((base + int64_t(i / 4) * kThreads) * 4 + (i % 4),
+                                A.v[i / 4].v[i % 4]);
+            }
+        }
+        for (; base < n_units; base += kThreads) {
+            const VecT<int32_t, 4> v = load_unit<int32_t, 4>(fk, base);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t h = bloom_hash(v.v[i]);
+                if (bloom_test(B[bloom_word(h, bb_log2)], h, bb_log2)) on_cand(base * 4 + i, v.v[i]);
+            }
         }
     } else if (n_units > 0 && base < n_units) {
         // Two register buffers: the next tile's keys are in flight while
         // the current tile is probed.
         Tile A, Bt;
-        load_tile(A, base);
+        if (first) A = *first;
+        else load_tile(A, base);
         for (;;) {
             load_tile(Bt, base + stride);
             use_tile(A, base);
@@ -436,18 +506,35 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
     }
 }
 
-template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock>
+template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock, bool kDefer>
 __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
     int64_t rows, Geo g, Table hbm, KList kl, int64_t* res, int64_t* partials,
     unsigned* counter, int64_t* host) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t ends[kSegs];
+    const KeyTile<kProbeU> first = first_tile<kProbeU>(fk, rows);
+    const bool has_min = (res[kFlags] & 2) != 0;
     const int64_t n = klist_prefix(kl, ends);
     const Shape s = shape_of(n, g);
-    const bool has_min = (res[kFlags] & 2) != 0;
     int64_t rev = 0;
     uint32_t cnt = 0;
+    // Matching rows are appended to an LDS list (an LDS atomic) and gathered
+    // after the loop; past kDeferMax (or beyond 2^32 rows) a lane gathers at
+    // once.
+    __shared__ uint32_t s_nm;
+    __shared__ uint32_t s_rows[kDeferMax];
+    const bool can_defer = kDefer && rows <= int64_t(UINT32_MAX);
+    auto defer = [&](int64_t row) {
+        if (can_defer) {
+            const uint32_t at = atomicAdd(&s_nm, 1u);
+            if (at < kDeferMax) {
+                s_rows[at] = uint32_t(row);
+                return;
+            }
+        }
+        add_match<TP, TD>(row, price, disc, rev, cnt);
+    };
     // No build keys (or a table that cannot hold them): nothing matches.
     if (s.flag == 0 && n > 0) {
         if (s.lds) {
@@ -457,7 +544,7 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
             int32_t* Tl = reinterpret_cast<int32_t*>(lds + words);
             for (int64_t i = threadIdx.x; i < words; i += kThreads) Bl[i] = 0;
             for (int64_t i = threadIdx.x; i < cap; i += kThreads) Tl[i] = kEmpty;
-            __syncthreads();
+            lds_barrier();
             // One key per thread per round (independent loads), not one
             // segment after another.
             for (int64_t i = threadIdx.x; i < n; i += kThreads) {
@@ -466,21 +553,31 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                 atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
                 if (key != kEmpty) table_insert(Tl, key, s.cap_log2);
             }
-            __syncthreads();
+            if (threadIdx.x == 0) s_nm = 0;
+            lds_barrier();
             const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
-            probe_rows<kProbeU, kPipe>(fk, rows, Bl, s.bb_log2, [&](int64_t row, int32_t key) {
+            probe_rows<kProbeU, kPipe>(fk, rows, Bl, s.bb_log2,
+                                       [&](int64_t row, int32_t key) {
                 if (key == kEmpty ? has_min : table_has(key, Tl, s.cap_log2, mask))
-                    add_match<TP, TD>(row, price, disc, rev, cnt);
-            });
+                    defer(row);
+            }, &first);
         } else {
+            if (threadIdx.x == 0) s_nm = 0;
+            lds_barrier();
             const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
             const int32_t* Th = hbm.slots;
             probe_rows<kProbeU, kPipe>(fk, rows, hbm.bits, s.bb_log2,
                                        [&](int64_t row, int32_t key) {
                 if (key == kEmpty ? has_min : table_has(key, Th, s.cap_log2, mask))
-                    add_match<TP, TD>(row, price, disc, rev, cnt);
-            });
+                    defer(row);
+            }, &first);
         }
+        // The deferred matches: their price/disc gathers are issued together
+        // here instead of stalling the streaming loop.
+        __syncthreads();
+        const uint32_t nm = s_nm < kDeferMax ? s_nm : kDeferMax;
+        for (uint32_t i = threadIdx.x; i < nm; i += kThreads)
+            add_match<TP, TD>(s_rows[i], price, disc, rev, cnt);
     }
 
     __shared__ int64_t s_rev[kWaves];
@@ -566,11 +663,11 @@ __global__ __launch_bounds__(kThreads) void q17_probe_k(
             for (int64_t i = threadIdx.x; i < cap; i += kThreads) Tl[i] = hbm.slots[i];
             __syncthreads();
             probe_rows<2, true>(pk, rows, Bl, s.bb_log2,
-                                [&](int64_t row, int32_t key) { on_cand(Tl, row, key); });
+                                      [&](int64_t row, int32_t key) { on_cand(Tl, row, key); });
         } else {
             const int32_t* Th = hbm.slots;
             probe_rows<2, true>(pk, rows, hbm.bits, s.bb_log2,
-                                [&](int64_t row, int32_t key) { on_cand(Th, row, key); });
+                                       [&](int64_t row, int32_t key) { on_cand(Th, row, key); });
         }
     }
     __shared__ uint64_t s_a[kWaves], s_c[kWaves];
@@ -624,11 +721,12 @@ __global__ __launch_bounds__(256) void q17_merge_k(const int64_t* __restrict__ p
 }
 
 // Table geometry fixed from an expected build size (capacity_hint): LDS
-// tables at load <= 1/4 with a 128-bit-per-key Bloom filter; HBM tables at
-// load <= 1/2 with 16 bits per key.  Either overflows past `cap / 2`.
+// tables at load <= 1/2 with a 128-bit-per-key Bloom filter; HBM
+// tables at load <= 1/2 with 16 bits per key.  Either overflows past
+// `cap / 2`.
 int size_from_hint(int64_t n, int placement, Geo& g, size_t& lds_bytes) {
     if (n < 1) n = 1;
-    const int lds_cap = clog2(n * 4 < 64 ? 64 : n * 4);
+    const int lds_cap = clog2(n * 2 < 64 ? 64 : n * 2);
     const int lds_bb = clampi(clog2(n * 128), 10, 18);  // at most 32 KB of filter
     const size_t lb = lds_bytes_of(lds_cap, lds_bb);
     const bool fits = lds_cap <= 30 && lb <= size_t(kLdsBudget);
@@ -651,6 +749,17 @@ int size_from_hint(int64_t n, int placement, Geo& g, size_t& lds_bytes) {
     return BOLAP_OK;
 }
 
+// Rows per lane of the compaction: 4 when the key and predicate columns are
+// aligned for vector loads, else 1.  Knob "join.compact_rows" = 16 gives one
+// 16-byte load of a u8 column per lane; measured slower on 2.5M-row
+// dimensions (fewer waves in flight), so 4 is the default.
+int compact_rows(bolap_ctx* ctx, const int32_t* keys, const PredSet& ps) {
+    const int64_t cap = ctx->knob("join.compact_rows", 4);
+    if (cap >= 16 && aligned(keys, 64) && preds_aligned(ps, 16)) return 16;
+    if (cap >= 4 && aligned(keys, 16) && preds_aligned(ps, 4)) return 4;
+    return 1;
+}
+
 // Everything one run of the pipeline launches with (device pointers carved
 // from the workspace, grid sizes, table geometry).
 struct Plan {
@@ -664,13 +773,13 @@ struct Plan {
     int64_t max_blocks;
     int64_t hbm_cap;
     bool may_hbm;
-    int compact_r;           // rows per lane of the compaction (4 or 1)
+    int compact_r;           // rows per lane of the compaction (16, 4 or 1)
     unsigned compact_grid;
 };
 
-template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
-int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
-    auto kern = probe_sum_k<TP, TD, kProbeU, kPipe, kLast>;
+template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast, bool kDefer>
+int launch_probe_d(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
+    auto kern = probe_sum_k<TP, TD, kProbeU, kPipe, kLast, kDefer>;
     if (p.lds > 0) {
         BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -699,6 +808,14 @@ int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) 
     hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, p.partials, int(blocks),
                        p.kl, p.res, p.host);
     return launch_check("join_merge_k");
+}
+
+// Deferred payload gathers (knob "join.defer"): 1 = on, 2 = off.
+template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
+int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
+    if (ctx->knob("join.defer", 1) == 2)
+        return launch_probe_d<TP, TD, kProbeU, kPipe, kLast, false>(ctx, a, p);
+    return launch_probe_d<TP, TD, kProbeU, kPipe, kLast, true>(ctx, a, p);
 }
 
 // Finish (knob "join.finish"): 1 = last workgroup merges, 2 = merge kernel.
@@ -744,7 +861,10 @@ int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     BOLAP_TRY(launch_check("zero_res_k"));
     if (a->dim_rows > 0) {
         TraceScope ts(ctx, "join_compact");
-        if (p.compact_r == 4)
+        if (p.compact_r == 16)
+            hipLaunchKernelGGL(dim_compact_k<16>, dim3(p.compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->dim_rows, a->dim_key, p.kl, p.res);
+        else if (p.compact_r == 4)
             hipLaunchKernelGGL(dim_compact_k<4>, dim3(p.compact_grid), dim3(kCompactThreads), 0,
                                ctx->stream, ps, a->dim_rows, a->dim_key, p.kl, p.res);
         else
@@ -789,7 +909,7 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     // covers at most ceil(units / (grid * threads)) * threads units of R rows,
     // segment r takes the workgroups b = r (mod kSegs), and segment 0 also
     // takes the fewer than R tail rows.
-    p.compact_r = (aligned(a->dim_key, 16) && preds_aligned(ps, 4)) ? 4 : 1;
+    p.compact_r = compact_rows(ctx, a->dim_key, ps);
     const int64_t units = a->dim_rows / p.compact_r;
     p.compact_grid = grid_for(ctx, units, kCompactThreads, 16);
     const int64_t per_block =
@@ -861,7 +981,8 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
         // Exact sizing on the device from the compacted build size.
         const int64_t lds_keys =
             a->placement == BOLAP_PLACE_LDS ? kForcedLdsKeys : kAutoLdsKeys;
-        g = Geo{0, a->placement, 0, 0, lds_keys};
+        g = Geo{0, a->placement, 0, 0, lds_keys,
+                int32_t(clampi(int(ctx->knob("join.lds_bloom_bits", 7)), 5, 8))};
         size_t lds = 0;
         if (a->placement != BOLAP_PLACE_HBM) {
             const Shape big = shape_of(lds_keys, g);
@@ -911,7 +1032,7 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     PredSet ps;
     BOLAP_TRY(make_predset(ps, a->n_part_preds + 1, preds, a->part_rows, "bolap_q17"));
 
-    const Geo g{0, BOLAP_PLACE_AUTO, 0, 0, kAutoLdsKeys};
+    const Geo g{0, BOLAP_PLACE_AUTO, 0, 0, kAutoLdsKeys, 7};
     const Shape worst = shape_of(a->part_rows, Geo{0, BOLAP_PLACE_HBM, 0, 0, 0});
     const Shape lds_max = shape_of(kAutoLdsKeys, g);
     const size_t lds = lds_bytes_of(lds_max.cap_log2, lds_max.bb_log2);
@@ -920,7 +1041,7 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     const int64_t cap = int64_t(1) << cap_log2;
     const int64_t words = (int64_t(1) << bb_log2) / 32;
     const int64_t max_blocks = int64_t(ctx->num_cus) * 8;
-    const int compact_r = (aligned(a->p_partkey, 16) && preds_aligned(ps, 4)) ? 4 : 1;
+    const int compact_r = compact_rows(ctx, a->p_partkey, ps);
     const int64_t units = a->part_rows / compact_r;
     const unsigned compact_grid = grid_for(ctx, units, kCompactThreads, 16);
     const int64_t per_block = (units + int64_t(compact_grid) * kCompactThreads - 1) /
@@ -950,7 +1071,10 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     BOLAP_TRY(launch_check("zero_res_k"));
     if (a->part_rows > 0) {
         TraceScope ts(ctx, "q17_part_scan");
-        if (compact_r == 4)
+        if (compact_r == 16)
+            hipLaunchKernelGGL(dim_compact_k<16>, dim3(compact_grid), dim3(kCompactThreads), 0,
+                               ctx->stream, ps, a->part_rows, a->p_partkey, kl, res);
+        else if (compact_r == 4)
             hipLaunchKernelGGL(dim_compact_k<4>, dim3(compact_grid), dim3(kCompactThreads), 0,
                                ctx->stream, ps, a->part_rows, a->p_partkey, kl, res);
         else
