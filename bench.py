@@ -232,15 +232,24 @@ def bench_q14_q17(eng, torch, args, which):
     rows = 60_000_000
     if which == "q14":
         lo, hi = eng.q14_window("1995-09-01")
-        step = lambda: eng.q14(t, lo, hi)  # noqa: E731
+        a = eng.q14_args(t, lo, hi)
+        step = lambda: eng.q14_run(a)  # noqa: E731
         first = step()
         ms = time_steps(torch, step, args.steps, args.warmup)
+        eng.set_trace(True)
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        tr = eng.read_trace()
+        eng.set_trace(False)
         alg = rows * 2 + first["rows_emitted"] * 8
         res = {"workload": "Q14 promo revenue, 60,000,000 lineitem rows, 2,000,000 parts, "
                            "one-month window",
                "ms_per_step": ms, "GBps": alg / (ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
                "rows_per_s": rows / (ms * 1e-3), "rows_emitted": first["rows_emitted"],
-               "kernel": "q14_scan_k"}
+               "kernel": "q14_flat_k",
+               "stages_ms": {k: stage_ms(tr, k) for k in ("q14_part_map", "q14_lineitem_scan",
+                                                          "q14_part_scan")}}
     else:
         preds = [eng.pred(t["p_brand"], N.OP_RANGE, 3, 3),
                  eng.pred(t["p_container"], N.OP_RANGE, 7, 7)]

@@ -50,9 +50,9 @@ constexpr int kLdsBudget = 96 * 1024;
 // AUTO (48 KB at 128 bits per key), up to 4096 keys when LDS is forced.
 constexpr int64_t kAutoLdsKeys = 2048;
 constexpr int64_t kForcedLdsKeys = 4096;
-// Matching fact rows a probe workgroup collects before gathering their
-// payload (beyond it, matches are gathered in the loop).
-constexpr uint32_t kDeferMax = 2048;
+// Bloom hits a probe workgroup collects in LDS before checking them
+// (beyond it, hits are checked in the loop).
+constexpr uint32_t kCandMax = 2048;
 
 // Result slot layout (int64): revenue, matches, build rows, flags
 // (bit 1: an INT32_MIN key passed the filter).
@@ -98,6 +98,12 @@ __host__ __device__ inline int clog2(int64_t v) {
 }
 __host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 
+// Filter bits of an HBM table: 128 per key up to 2^24 bits (2 MB, so that
+// the filter stays resident in one XCD's L2 while the fact side streams).
+__host__ __device__ inline int hbm_bloom_log2(int64_t m) {
+    return clampi(clog2(m * 128), 10, 24);
+}
+
 __host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
     if (g.fixed) {
         Shape s{g.placement == BOLAP_PLACE_LDS, g.cap_log2, g.bb_log2, 0};
@@ -109,7 +115,7 @@ __host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
         return Shape{true, clog2(m * 2 < 64 ? 64 : m * 2),
                      clampi(clog2(m << g.lds_bits_log2), 10, 18), 0};
     if (g.placement == BOLAP_PLACE_LDS) return Shape{true, 0, 0, 4};
-    return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), clampi(clog2(m * 16), 10, 22), 0};
+    return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), hbm_bloom_log2(m), 0};
 }
 
 __host__ __device__ inline size_t lds_bytes_of(int cap_log2, int bb_log2) {
@@ -127,7 +133,7 @@ BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
 BOLAP_DEV uint32_t bloom_hash(int32_t key) { return uint32_t(key) * kBitMul; }
 BOLAP_DEV uint32_t bloom_word(uint32_t h, int bb_log2) { return h >> (32 - (bb_log2 - 5)); }
 // Two bit positions from the ten hash bits just below the word index (the
-// top bb_log2 - 5 bits pick the word; 10 <= bb_log2 <= 22 keeps the three
+// top bb_log2 - 5 bits pick the word; 10 <= bb_log2 <= 24 keeps the three
 // fields disjoint and inside the word).
 BOLAP_DEV bool bloom_test(uint32_t w, uint32_t h, int bb_log2) {
     const int sh = 32 - bb_log2;
@@ -284,34 +290,34 @@ __global__ __launch_bounds__(kCompactThreads) void dim_compact_k(PredSet ps, int
         atomicOr(reinterpret_cast<unsigned long long*>(res + kFlags), 2ull);
 }
 
-// HBM table for a build side that does not go to LDS.  With `force` the
-// table is built in HBM whatever its geometry (a canonical copy that every
-// probe workgroup copies into LDS, so slot numbers agree across workgroups),
-// and the per-slot aggregates agg_sum/agg_cnt (when given) are cleared.
-__global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl, int force,
+// HBM table for a build side that does not go to LDS.  The per-slot
+// aggregates agg_sum/agg_cnt (when given) are cleared for any geometry:
+// callers that aggregate per build key index them by HBM slot, or by the
+// key's list position when the table goes to LDS.
+__global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl,
                                                   uint64_t* agg_sum, uint32_t* agg_cnt) {
     const Shape s = shape_of(klist_size(kl, nullptr), g);
-    if (s.flag || (s.lds && !force)) return;
+    if (s.flag || (s.lds && !agg_sum)) return;
     const int64_t cap = int64_t(1) << s.cap_log2;
     const int64_t words = (int64_t(1) << s.bb_log2) / 32;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cap;
          i += int64_t(gridDim.x) * 256) {
-        t.slots[i] = kEmpty;
+        if (!s.lds) t.slots[i] = kEmpty;
         if (agg_sum) {
             agg_sum[i] = 0;
             agg_cnt[i] = 0;
         }
     }
+    if (s.lds) return;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
          i += int64_t(gridDim.x) * 256)
         t.bits[i] = 0;
 }
 
-// Workgroup b inserts segment b % kSegs (gridDim.x is a multiple of kSegs).
-__global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl, int force) {
+__global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl) {
     __shared__ uint32_t seg[kSegs];
     const Shape s = shape_of(klist_size(kl, seg), g);
-    if (s.flag || (s.lds && !force)) return;
+    if (s.flag || s.lds) return;
     const int r = int(blockIdx.x % kSegs);
     const int64_t per = gridDim.x / kSegs;
     const int32_t* __restrict__ keys = kl.keys + int64_t(r) * kl.seg_cap;
@@ -400,10 +406,53 @@ BOLAP_DEV KeyTile<kProbeU> first_tile(const int32_t* __restrict__ fk, int64_t ro
     return t;
 }
 
-template <int kProbeU, bool kPipe, class OnCand>
+// Bloom hits collected by a probe loop (kCollect): (row, key) entries in
+// LDS, appended with one LDS atomic per wave; entries past `cap` go to the
+// loop's inline on_cand instead.  The caller zeroes *n before the loop
+// (ordered by a barrier) and checks the entries after it.
+struct CandBuf {
+    uint2* e;
+    uint32_t* n;
+    uint32_t cap;
+};
+
+// Appends the lane's hit keys (bit i of `hits` for key i of the tile) to
+// cb: a ballot per key slot gives each lane its offset (mbcnt), so the
+// wave's only divergent work is the predicated LDS stores.
+template <int kKeys, class Row, class OnCand>
+BOLAP_DEV void collect_hits(const CandBuf& cb, const int32_t* keys, uint32_t hits, Row&& row_of,
+                            OnCand&& on_cand) {
+    uint64_t m[kKeys];
+    uint32_t pre[kKeys];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < kKeys; ++i) {
+        m[i] = __ballot((hits >> i) & 1u);
+        pre[i] = tot;
+        tot += uint32_t(__popcll(m[i]));
+    }
+    const uint64_t act = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(act)) - 1;
+    uint32_t base = 0;
+    if (int(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(cb.n, tot);
+    base = __shfl(base, leader, kWave);
+#pragma unroll
+    for (int i = 0; i < kKeys; ++i) {
+        if ((hits >> i) & 1u) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                uint32_t(m[i] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[i]), 0u));
+            const uint32_t at = base + pre[i] + below;
+            const int64_t row = row_of(i);
+            if (at < cb.cap) cb.e[at] = make_uint2(uint32_t(row), uint32_t(keys[i]));
+            else on_cand(row, keys[i]);
+        }
+    }
+}
+
+template <int kProbeU, int kPipe, bool kCollect = false, class OnCand>
 BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const uint32_t* B,
                           int bb_log2, OnCand&& on_cand,
-                          const KeyTile<kProbeU>* first = nullptr) {
+                          const KeyTile<kProbeU>* first = nullptr, CandBuf cb = {}) {
     const int64_t n_units = rows / 4;
     const int64_t stride = int64_t(gridDim.x) * kThreads * kProbeU;
     using Tile = KeyTile<kProbeU>;
@@ -435,8 +484,18 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
             for (int k = 0; k < kProbeU; ++k)
                 if (at + int64_t(k) * kThreads >= n_units) hits &= ~(0xFu << (4 * k));
         }
-        // Phase 2: exact probes only for Bloom hits (rare, divergent).
-        if (hits) {
+        // Phase 2: Bloom hits (rare) are collected for an exact check after
+        // the loop, or checked at once.
+        if constexpr (kCollect) {
+            if (__ballot(hits != 0)) {
+                int32_t keys[kKeys];
+#pragma unroll
+                for (int i = 0; i < kKeys; ++i) keys[i] = x.v[i / 4].v[i % 4];
+                collect_hits<kKeys>(cb, keys, hits, [&](int i) {
+                    return (at + int64_t(i / 4) * kThreads) * 4 + (i % 4);
+                }, on_cand);
+            }
+        } else if (hits) {
 #pragma unroll
             for (int i = 0; i < kKeys; ++i) {
                 if (hits & (1u << i)) {
@@ -484,20 +543,43 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
             }
         }
     } else if (n_units > 0 && base < n_units) {
-        // Two register buffers: the next tile's keys are in flight while
-        // the current tile is probed.
-        Tile A, Bt;
-        if (first) A = *first;
-        else load_tile(A, base);
-        for (;;) {
+        if constexpr (kPipe == 2) {
+            // Two register buffers: the next tile's keys are in flight while
+            // the current tile is probed.
+            Tile A, Bt;
+            if (first) A = *first;
+            else load_tile(A, base);
+            for (;;) {
+                load_tile(Bt, base + stride);
+                use_tile(A, base);
+                base += stride;
+                if (base >= n_units) break;
+                load_tile(A, base + stride);
+                use_tile(Bt, base);
+                base += stride;
+                if (base >= n_units) break;
+            }
+        } else {
+            // Three register buffers: two tiles in flight while one is
+            // probed, so a wave's exact-lookup work overlaps more latency.
+            Tile A, Bt, C;
+            if (first) A = *first;
+            else load_tile(A, base);
             load_tile(Bt, base + stride);
-            use_tile(A, base);
-            base += stride;
-            if (base >= n_units) break;
-            load_tile(A, base + stride);
-            use_tile(Bt, base);
-            base += stride;
-            if (base >= n_units) break;
+            for (;;) {
+                load_tile(C, base + 2 * stride);
+                use_tile(A, base);
+                base += stride;
+                if (base >= n_units) break;
+                load_tile(A, base + 2 * stride);
+                use_tile(Bt, base);
+                base += stride;
+                if (base >= n_units) break;
+                load_tile(Bt, base + 2 * stride);
+                use_tile(C, base);
+                base += stride;
+                if (base >= n_units) break;
+            }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
@@ -506,7 +588,7 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
     }
 }
 
-template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLastBlock, bool kDefer>
+template <typename TP, typename TD, int kProbeU, int kPipe, bool kLastBlock, bool kCollect>
 __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
     int64_t rows, Geo g, Table hbm, KList kl, int64_t* res, int64_t* partials,
@@ -519,24 +601,17 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
     const Shape s = shape_of(n, g);
     int64_t rev = 0;
     uint32_t cnt = 0;
-    // Matching rows are appended to an LDS list (an LDS atomic) and gathered
-    // after the loop; past kDeferMax (or beyond 2^32 rows) a lane gathers at
-    // once.
-    __shared__ uint32_t s_nm;
-    __shared__ uint32_t s_rows[kDeferMax];
-    const bool can_defer = kDefer && rows <= int64_t(UINT32_MAX);
-    auto defer = [&](int64_t row) {
-        if (can_defer) {
-            const uint32_t at = atomicAdd(&s_nm, 1u);
-            if (at < kDeferMax) {
-                s_rows[at] = uint32_t(row);
-                return;
-            }
-        }
-        add_match<TP, TD>(row, price, disc, rev, cnt);
-    };
+    // Bloom hits are collected in LDS (kCollect, rows below 2^32) and
+    // checked and gathered after the streaming loop; past kCandMax, or
+    // without collection, a lane checks and gathers at once.
+    __shared__ uint32_t s_cn;
+    __shared__ uint2 s_cand[kCandMax];
+    const bool collect = kCollect && rows <= int64_t(UINT32_MAX);
+    const CandBuf cb{s_cand, &s_cn, collect ? kCandMax : 0u};
     // No build keys (or a table that cannot hold them): nothing matches.
     if (s.flag == 0 && n > 0) {
+        const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
+        const int32_t* T = hbm.slots;
         if (s.lds) {
             const int64_t words = (int64_t(1) << s.bb_log2) / 32;
             const int64_t cap = int64_t(1) << s.cap_log2;
@@ -553,31 +628,35 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                 atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
                 if (key != kEmpty) table_insert(Tl, key, s.cap_log2);
             }
-            if (threadIdx.x == 0) s_nm = 0;
+            if (threadIdx.x == 0) s_cn = 0;
             lds_barrier();
-            const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
-            probe_rows<kProbeU, kPipe>(fk, rows, Bl, s.bb_log2,
-                                       [&](int64_t row, int32_t key) {
+            T = Tl;
+            probe_rows<kProbeU, kPipe, kCollect>(fk, rows, Bl, s.bb_log2,
+                                                 [&](int64_t row, int32_t key) {
                 if (key == kEmpty ? has_min : table_has(key, Tl, s.cap_log2, mask))
-                    defer(row);
-            }, &first);
+                    add_match<TP, TD>(row, price, disc, rev, cnt);
+            }, &first, cb);
         } else {
-            if (threadIdx.x == 0) s_nm = 0;
+            if (threadIdx.x == 0) s_cn = 0;
             lds_barrier();
-            const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
-            const int32_t* Th = hbm.slots;
-            probe_rows<kProbeU, kPipe>(fk, rows, hbm.bits, s.bb_log2,
-                                       [&](int64_t row, int32_t key) {
-                if (key == kEmpty ? has_min : table_has(key, Th, s.cap_log2, mask))
-                    defer(row);
-            }, &first);
+            probe_rows<kProbeU, kPipe, kCollect>(fk, rows, hbm.bits, s.bb_log2,
+                                                 [&](int64_t row, int32_t key) {
+                if (key == kEmpty ? has_min : table_has(key, T, s.cap_log2, mask))
+                    add_match<TP, TD>(row, price, disc, rev, cnt);
+            }, &first, cb);
         }
-        // The deferred matches: their price/disc gathers are issued together
-        // here instead of stalling the streaming loop.
-        __syncthreads();
-        const uint32_t nm = s_nm < kDeferMax ? s_nm : kDeferMax;
-        for (uint32_t i = threadIdx.x; i < nm; i += kThreads)
-            add_match<TP, TD>(s_rows[i], price, disc, rev, cnt);
+        if constexpr (kCollect) {
+            // The collected candidates: exact checks and payload gathers of
+            // all threads together instead of stalling the streaming loop.
+            __syncthreads();
+            const uint32_t nc = s_cn < cb.cap ? s_cn : cb.cap;
+            for (uint32_t i = threadIdx.x; i < nc; i += kThreads) {
+                const uint2 e = s_cand[i];
+                const int32_t key = int32_t(e.y);
+                if (key == kEmpty ? has_min : table_has(key, T, s.cap_log2, mask))
+                    add_match<TP, TD>(e.x, price, disc, rev, cnt);
+            }
+        }
     }
 
     __shared__ int64_t s_rev[kWaves];
@@ -619,55 +698,148 @@ BOLAP_DEV int32_t table_find(int32_t key, const int32_t* T, int cap_log2, uint32
     return -1;
 }
 
-// Q17 (bespoke_tpch/query_q17.cpp:119-309) over the canonical table: pass 1
-// adds quantity and 1 into the part's slot for every lineitem row of a
-// passing part; pass 2 sums extendedprice over the rows with
-// quantity * 5 * cnt < sum (0.2 * average, :288-291).  Both passes copy the
-// HBM table into LDS when it has the LDS geometry.  Part keys are never the
+// Q17 (bespoke_tpch/query_q17.cpp:119-309): pass 1 adds quantity and 1
+// into the part's aggregate for every lineitem row of a passing part; pass
+// 2 sums extendedprice over the rows with quantity * 5 * cnt < sum (0.2 *
+// average, :288-291).  The aggregates are indexed by the key's position in
+// the key list when the table goes to LDS (every workgroup builds its own
+// table there; a key listed twice, a duplicated part row, takes its
+// smallest position, which does not depend on insertion order) and by the
+// HBM slot otherwise.  Pass 1 also records each match as (row, index) in
+// the workgroup's own region of `mlist` (list_cap entries) and its count in
+// mcnt[block]; pass 2, launched with the same grid, walks that list instead
+// of the fact keys, and re-scans the workgroup's rows (the same grid-stride
+// share as pass 1) only when its list overflowed.  Part keys are never the
 // empty marker (negative part keys are filtered out before the build).
+constexpr uint32_t kListOverflow = 0xffffffffu;
+constexpr int kQ17Pipe = 2;          // register buffers of the fact-key loop
+constexpr uint32_t kQ17CandMax = 1024;  // Bloom hits collected per workgroup
+
+// Slot holding `key` after inserting it (the table must have room).
+BOLAP_DEV uint32_t table_insert_at(int32_t* T, int32_t key, int cap_log2) {
+    const uint32_t mask = (uint32_t(1) << cap_log2) - 1;
+    uint32_t s = slot_of(key, cap_log2);
+    for (uint32_t probe = 0; probe < mask; ++probe) {
+        const int32_t prev = atomicCAS(T + s, kEmpty, key);
+        if (prev == kEmpty || prev == key) return s;
+        s = (s + 1) & mask;
+    }
+    return s;
+}
+
+// Dynamic LDS of the Q17 passes: filter, key slots and aggregate indices.
+__host__ __device__ inline size_t q17_lds_bytes(int cap_log2, int bb_log2) {
+    return lds_bytes_of(cap_log2, bb_log2) + (size_t(1) << cap_log2) * 4;
+}
+
 template <int kPass>
 __global__ __launch_bounds__(kThreads) void q17_probe_k(
     const int32_t* __restrict__ pk, const int16_t* __restrict__ qty,
     const int32_t* __restrict__ price, int64_t rows, Geo g, Table hbm, KList kl,
-    uint64_t* agg_sum, uint32_t* agg_cnt, int64_t* partials) {
+    uint64_t* agg_sum, uint32_t* agg_cnt, uint2* mlist, uint32_t* mcnt, int64_t list_cap,
+    int64_t* partials) {
     extern __shared__ uint32_t lds[];
-    const int64_t n = klist_size(kl, nullptr);
+    __shared__ uint32_t ends[kSegs];
+    __shared__ uint32_t s_n;
+    KeyTile<2> first{};
+    if (kPass == 1) first = first_tile<2>(pk, rows);
+    const int64_t n = klist_prefix(kl, ends);
     const Shape s = shape_of(n, g);
     uint64_t acc = 0;
     uint32_t cnt = 0;
-    if (s.flag == 0 && n > 0) {
-        const int64_t words = (int64_t(1) << s.bb_log2) / 32;
-        const int64_t cap = int64_t(1) << s.cap_log2;
-        const uint32_t mask = uint32_t(cap - 1);
-        auto on_cand = [&](const int32_t* T, int64_t row, int32_t key) {
-            const int32_t slot = key == kEmpty ? -1 : table_find(key, T, s.cap_log2, mask);
-            if (slot < 0) return;
-            const int64_t q = int64_t(qty[row]);
+    uint2* my = mlist + int64_t(blockIdx.x) * list_cap;
+    bool scan = s.flag == 0 && n > 0;
+    if (kPass == 1) {
+        if (threadIdx.x == 0) s_n = 0;
+    } else if (scan) {
+        const uint32_t m = mcnt[blockIdx.x];
+        if (m != kListOverflow) {
+            scan = false;
+            for (uint32_t i = threadIdx.x; i < m; i += kThreads) {
+                const uint2 e = my[i];
+                const int64_t q = int64_t(qty[e.x]);
+                const int64_t c = int64_t(agg_cnt[e.y]);
+                if (q * 5 * c < int64_t(agg_sum[e.y])) {
+                    acc += uint64_t(int64_t(price[e.x]));
+                    cnt += 1;
+                }
+            }
+        }
+    }
+    if (scan) {
+        const uint32_t mask = (uint32_t(1) << s.cap_log2) - 1;
+        auto on_match = [&](int64_t row, uint32_t at) {
             if (kPass == 1) {
-                atomicAdd(reinterpret_cast<unsigned long long*>(agg_sum + slot),
-                          (unsigned long long)q);
-                atomicAdd(agg_cnt + slot, 1u);
+                atomicAdd(reinterpret_cast<unsigned long long*>(agg_sum + at),
+                          (unsigned long long)int64_t(qty[row]));
+                atomicAdd(agg_cnt + at, 1u);
                 cnt += 1;
+                const uint32_t i = atomicAdd(&s_n, 1u);
+                if (int64_t(i) < list_cap) my[i] = make_uint2(uint32_t(row), at);
             } else {
-                const int64_t c = int64_t(agg_cnt[slot]);
-                if (q * 5 * c < int64_t(agg_sum[slot])) {
+                const int64_t q = int64_t(qty[row]);
+                const int64_t c = int64_t(agg_cnt[at]);
+                if (q * 5 * c < int64_t(agg_sum[at])) {
                     acc += uint64_t(int64_t(price[row]));
                     cnt += 1;
                 }
             }
         };
+        // Aggregate index of a key, or -1: the key's list position (LDS
+        // table) or its HBM slot.  The empty marker is never a build key (no
+        // part key is negative).
+        const int32_t* T = hbm.slots;
+        const uint32_t* Ix = nullptr;
+        const uint32_t* Bf = hbm.bits;
         if (s.lds) {
+            const int64_t words = (int64_t(1) << s.bb_log2) / 32;
+            const int64_t cap = int64_t(1) << s.cap_log2;
             uint32_t* Bl = lds;
             int32_t* Tl = reinterpret_cast<int32_t*>(lds + words);
-            for (int64_t i = threadIdx.x; i < words; i += kThreads) Bl[i] = hbm.bits[i];
-            for (int64_t i = threadIdx.x; i < cap; i += kThreads) Tl[i] = hbm.slots[i];
+            uint32_t* Il = reinterpret_cast<uint32_t*>(Tl + cap);
+            for (int64_t i = threadIdx.x; i < words; i += kThreads) Bl[i] = 0;
+            for (int64_t i = threadIdx.x; i < cap; i += kThreads) {
+                Tl[i] = kEmpty;
+                Il[i] = 0xffffffffu;
+            }
+            lds_barrier();
+            for (int64_t i = threadIdx.x; i < n; i += kThreads) {
+                const int32_t key = klist_key(kl, ends, i);
+                const uint32_t h = bloom_hash(key);
+                atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+                atomicMin(Il + table_insert_at(Tl, key, s.cap_log2), uint32_t(i));
+            }
+            T = Tl;
+            Ix = Il;
+            Bf = Bl;
+        }
+        auto find_at = [&](int32_t key) -> int64_t {
+            if (key == kEmpty) return -1;
+            const int32_t slot = table_find(key, T, s.cap_log2, mask);
+            if (slot < 0) return -1;
+            return Ix ? int64_t(Ix[slot]) : int64_t(slot);
+        };
+        auto on_cand = [&](int64_t row, int32_t key) {
+            const int64_t at = find_at(key);
+            if (at >= 0) on_match(row, uint32_t(at));
+        };
+        if constexpr (kPass == 1) {
+            // Bloom hits collected in LDS, checked after the loop.
+            __shared__ uint32_t s_cn;
+            __shared__ uint2 s_cand[kQ17CandMax];
+            const CandBuf cb{s_cand, &s_cn, rows <= int64_t(UINT32_MAX) ? kQ17CandMax : 0u};
+            if (threadIdx.x == 0) s_cn = 0;
+            lds_barrier();
+            probe_rows<2, kQ17Pipe, true>(pk, rows, Bf, s.bb_log2, on_cand, &first, cb);
             __syncthreads();
-            probe_rows<2, true>(pk, rows, Bl, s.bb_log2,
-                                      [&](int64_t row, int32_t key) { on_cand(Tl, row, key); });
+            const uint32_t nc = s_cn < cb.cap ? s_cn : cb.cap;
+            for (uint32_t i = threadIdx.x; i < nc; i += kThreads) {
+                const uint2 e = s_cand[i];
+                on_cand(e.x, int32_t(e.y));
+            }
         } else {
-            const int32_t* Th = hbm.slots;
-            probe_rows<2, true>(pk, rows, hbm.bits, s.bb_log2,
-                                       [&](int64_t row, int32_t key) { on_cand(Th, row, key); });
+            lds_barrier();
+            probe_rows<2, kQ17Pipe>(pk, rows, Bf, s.bb_log2, on_cand);
         }
     }
     __shared__ uint64_t s_a[kWaves], s_c[kWaves];
@@ -685,6 +857,7 @@ __global__ __launch_bounds__(kThreads) void q17_probe_k(
         }
         partials[2 * int64_t(blockIdx.x)] = int64_t(a);
         partials[2 * int64_t(blockIdx.x) + 1] = int64_t(c);
+        if (kPass == 1) mcnt[blockIdx.x] = int64_t(s_n) <= list_cap ? s_n : kListOverflow;
     }
 }
 
@@ -744,7 +917,7 @@ int size_from_hint(int64_t n, int placement, Geo& g, size_t& lds_bytes) {
         set_error("bolap_join_sum: %lld keys exceed the table limit", (long long)n);
         return BOLAP_ERR_CAPACITY;
     }
-    g = Geo{1, BOLAP_PLACE_HBM, cap, clampi(clog2(n * 16), 10, 22), 0};
+    g = Geo{1, BOLAP_PLACE_HBM, cap, hbm_bloom_log2(n), 0};
     lds_bytes = 0;
     return BOLAP_OK;
 }
@@ -777,7 +950,7 @@ struct Plan {
     unsigned compact_grid;
 };
 
-template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast, bool kDefer>
+template <typename TP, typename TD, int kProbeU, int kPipe, bool kLast, bool kDefer>
 int launch_probe_d(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     auto kern = probe_sum_k<TP, TD, kProbeU, kPipe, kLast, kDefer>;
     if (p.lds > 0) {
@@ -810,8 +983,9 @@ int launch_probe_d(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) 
     return launch_check("join_merge_k");
 }
 
-// Deferred payload gathers (knob "join.defer"): 1 = on, 2 = off.
-template <typename TP, typename TD, int kProbeU, bool kPipe, bool kLast>
+// Collected Bloom hits (knob "join.defer"): 1 = on, 2 = off (checked and
+// gathered in the loop).
+template <typename TP, typename TD, int kProbeU, int kPipe, bool kLast>
 int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     if (ctx->knob("join.defer", 1) == 2)
         return launch_probe_d<TP, TD, kProbeU, kPipe, kLast, false>(ctx, a, p);
@@ -819,7 +993,7 @@ int launch_probe_f(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) 
 }
 
 // Finish (knob "join.finish"): 1 = last workgroup merges, 2 = merge kernel.
-template <typename TP, typename TD, int kProbeU, bool kPipe>
+template <typename TP, typename TD, int kProbeU, int kPipe>
 int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     if (ctx->knob("join.finish", 2) == 1)
         return launch_probe_f<TP, TD, kProbeU, kPipe, true>(ctx, a, p);
@@ -827,14 +1001,16 @@ int launch_probe_v(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) 
 }
 
 // Probe variants (knob "join.probe_variant"): 1 plain U=4, 2 plain U=8,
-// 3 pipelined U=2, 4 pipelined U=4 (4-key units per lane per tile).
+// 3 double-buffered U=2, 4 double-buffered U=4, 5 triple-buffered U=2
+// (4-key units per lane per tile).
 template <typename TP, typename TD>
 int launch_probe(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     switch (ctx->knob("join.probe_variant", 3)) {
-        case 1: return launch_probe_v<TP, TD, 4, false>(ctx, a, p);
-        case 2: return launch_probe_v<TP, TD, 8, false>(ctx, a, p);
-        case 3: return launch_probe_v<TP, TD, 2, true>(ctx, a, p);
-        default: return launch_probe_v<TP, TD, 4, true>(ctx, a, p);
+        case 1: return launch_probe_v<TP, TD, 4, 0>(ctx, a, p);
+        case 2: return launch_probe_v<TP, TD, 8, 0>(ctx, a, p);
+        case 3: return launch_probe_v<TP, TD, 2, 2>(ctx, a, p);
+        case 5: return launch_probe_v<TP, TD, 2, 3>(ctx, a, p);
+        default: return launch_probe_v<TP, TD, 4, 2>(ctx, a, p);
     }
 }
 
@@ -877,10 +1053,10 @@ int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
         // side goes to LDS, which is the common case.
         TraceScope ts(ctx, "join_hbm_build");
         hipLaunchKernelGGL(hbm_init_k, dim3(grid_for(ctx, p.hbm_cap, 256, 1)), dim3(256), 0,
-                           ctx->stream, p.g, p.t, p.kl, 0, nullptr, nullptr);
+                           ctx->stream, p.g, p.t, p.kl, nullptr, nullptr);
         BOLAP_TRY(launch_check("hbm_init_k"));
         hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, p.g, p.t,
-                           p.kl, 0);
+                           p.kl);
         BOLAP_TRY(launch_check("hbm_build_k"));
     }
     return dispatch_probe(ctx, a, p);
@@ -1035,7 +1211,7 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     const Geo g{0, BOLAP_PLACE_AUTO, 0, 0, kAutoLdsKeys, 7};
     const Shape worst = shape_of(a->part_rows, Geo{0, BOLAP_PLACE_HBM, 0, 0, 0});
     const Shape lds_max = shape_of(kAutoLdsKeys, g);
-    const size_t lds = lds_bytes_of(lds_max.cap_log2, lds_max.bb_log2);
+    const size_t lds = q17_lds_bytes(lds_max.cap_log2, lds_max.bb_log2);
     int cap_log2 = worst.cap_log2 > lds_max.cap_log2 ? worst.cap_log2 : lds_max.cap_log2;
     int bb_log2 = worst.bb_log2 > lds_max.bb_log2 ? worst.bb_log2 : lds_max.bb_log2;
     const int64_t cap = int64_t(1) << cap_log2;
@@ -1056,6 +1232,19 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     const size_t o_sum = ws.plan(size_t(cap) * 8);
     const size_t o_acnt = ws.plan(size_t(cap) * 4);
     const size_t o_part = ws.plan(size_t(max_blocks) * 2 * sizeof(int64_t) * 2);
+    int64_t blocks = int64_t(ctx->num_cus) * 2;
+    const int64_t need = (a->li_rows / 4 + int64_t(kThreads) * 2 - 1) / (int64_t(kThreads) * 2);
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    // Match list: a 32nd of each workgroup's rows (at least 256 entries);
+    // row indices are 32-bit, so larger tables always re-scan.
+    const int64_t rows_per_block = (a->li_rows + blocks - 1) / blocks;
+    int64_t list_cap = rows_per_block / 32 > 256 ? rows_per_block / 32 : 256;
+    const int64_t forced_cap = ctx->knob("q17.list_cap", 0);
+    if (forced_cap > 0) list_cap = forced_cap;
+    if (a->li_rows >= (int64_t(1) << 32)) list_cap = 0;
+    const size_t o_list = ws.plan(size_t(blocks) * size_t(list_cap) * sizeof(uint2));
+    const size_t o_mcnt = ws.plan(size_t(blocks) * sizeof(uint32_t));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
     ws.base = static_cast<char*>(ctx->ws);
@@ -1085,15 +1274,13 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     {
         TraceScope ts(ctx, "q17_build");
         hipLaunchKernelGGL(hbm_init_k, dim3(grid_for(ctx, cap, 256, 1)), dim3(256), 0,
-                           ctx->stream, g, t, kl, 1, agg_sum, agg_cnt);
+                           ctx->stream, g, t, kl, agg_sum, agg_cnt);
         BOLAP_TRY(launch_check("hbm_init_k"));
-        hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, g, t, kl, 1);
+        hipLaunchKernelGGL(hbm_build_k, dim3(kSegs * 4), dim3(256), 0, ctx->stream, g, t, kl);
         BOLAP_TRY(launch_check("hbm_build_k"));
     }
-    int64_t blocks = int64_t(ctx->num_cus) * 2;
-    const int64_t need = (a->li_rows / 4 + int64_t(kThreads) * 2 - 1) / (int64_t(kThreads) * 2);
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
+    uint2* mlist = ws.at<uint2>(o_list);
+    uint32_t* mcnt = ws.at<uint32_t>(o_mcnt);
     for (int pass = 1; pass <= 2; ++pass) {
         TraceScope ts(ctx, pass == 1 ? "q17_sum_scan" : "q17_filter_scan");
         auto kern = pass == 1 ? q17_probe_k<1> : q17_probe_k<2>;
@@ -1101,7 +1288,8 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
         hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), lds, ctx->stream,
                            a->l_partkey, a->l_quantity, a->l_price, a->li_rows, g, t, kl,
-                           agg_sum, agg_cnt, pass == 1 ? part1 : part2);
+                           agg_sum, agg_cnt, mlist, mcnt, list_cap,
+                           pass == 1 ? part1 : part2);
         BOLAP_TRY(launch_check("q17_probe_k"));
     }
     hipLaunchKernelGGL(q17_merge_k, dim3(1), dim3(256), 0, ctx->stream, part1, part2,

@@ -235,8 +235,11 @@ __global__ __launch_bounds__(kT) void q6_zoned_k(Q6K a, const bolap_zone* __rest
 }
 
 // Sums the block partials into res[0..2] and the pinned host row.
+// Sums three u64 fields of the per-workgroup partials into res[0..2] and
+// the pinned host row; `flag` (optional) is copied to host[3].
 __global__ __launch_bounds__(kT) void q6_merge_k(const int64_t* __restrict__ partials, int nb,
-                                                  int64_t* res, int64_t* host) {
+                                                  int64_t* res, int64_t* host,
+                                                  const int32_t* flag = nullptr) {
     uint64_t r = 0, c = 0, s = 0;
     for (int b = threadIdx.x; b < nb; b += kT) {
         r += uint64_t(partials[3 * b]);
@@ -260,6 +263,8 @@ __global__ __launch_bounds__(kT) void q6_merge_k(const int64_t* __restrict__ par
         __hip_atomic_store(host + threadIdx.x, int64_t(t), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    if (threadIdx.x == 3 && flag)
+        __hip_atomic_store(host + 3, int64_t(*flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // One zone per workgroup: min/max of ship, quantity and discount.
@@ -370,14 +375,26 @@ __global__ __launch_bounds__(kT) void gather_q6_k(const uint32_t* __restrict__ p
 // partkey -> promo flag (query_q14.cpp:202-223): the largest part row of a
 // key wins (the reference's sequential assignment lets the last row win),
 // packed as (row + 1) << 1 | promo under an atomic maximum, then narrowed to
-// int8 (-1 absent).
+// int8 (-1 absent).  The array covers keys 0..size-1; a larger key sets
+// *ovf so that the host reruns with the exact size.
+__global__ __launch_bounds__(kT) void q14_clear_k(int32_t* packed, int64_t size, int32_t* ovf) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ovf = 0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < size;
+         i += int64_t(gridDim.x) * kT)
+        packed[i] = 0;
+}
+
 __global__ __launch_bounds__(kT) void q14_part_k(const int32_t* __restrict__ pk,
                                                  const uint8_t* __restrict__ promo, int64_t n,
-                                                 int32_t* packed, int64_t size) {
+                                                 int32_t* packed, int64_t size, int32_t* ovf) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT) {
         const int32_t k = pk[i];
-        if (k >= 0 && k < size) atomicMax(packed + k, int32_t(((i + 1) << 1) | (promo[i] ? 1 : 0)));
+        if (k < 0) continue;
+        if (k < size)
+            atomicMax(packed + k, int32_t(((i + 1) << 1) | (promo[i] ? 1 : 0)));
+        else
+            *ovf = 1;
     }
 }
 
@@ -429,23 +446,19 @@ BOLAP_DEV void block_out3(uint64_t x, uint64_t y, uint64_t z, int64_t* partials)
     }
 }
 
-// Lineitem pass: 8 rows per lane; the part key and discounted price of a
-// unit are loaded only when one of its ship dates is in the window; zones
-// (when given) skip blocks of 2048 rows outside the window.
-template <bool kZoned>
-__global__ __launch_bounds__(kT) void q14_scan_k(Q14K a, const bolap_zone* __restrict__ zones,
-                                                 int64_t* partials) {
+// Lineitem pass over a zone map: one zone (2048 rows = 256 units of 8) per
+// workgroup iteration; zones outside the window are skipped, the ship test
+// is skipped for zones wholly inside it, and the part key and discounted
+// price of a unit are loaded only when one of its rows is in the window.
+__global__ __launch_bounds__(kT) void q14_zoned_k(Q14K a, const bolap_zone* __restrict__ zones,
+                                                  int64_t* partials) {
     uint64_t promo = 0, total = 0, cnt = 0;
     const int64_t units = a.rows / 8;
     const int64_t n_zones = (a.rows + kZoneRows - 1) / kZoneRows;
-    const int64_t groups = kZoned ? n_zones : (units + kT - 1) / kT;
-    for (int64_t z = blockIdx.x; z < groups; z += gridDim.x) {
-        bool full = false;
-        if (kZoned) {
-            const bolap_zone zn = zones[z];
-            if (zn.ship_max < a.ship_lo || zn.ship_min >= a.ship_hi) continue;
-            full = zn.ship_min >= a.ship_lo && zn.ship_max < a.ship_hi;
-        }
+    for (int64_t z = blockIdx.x; z < n_zones; z += gridDim.x) {
+        const bolap_zone zn = zones[z];
+        if (zn.ship_max < a.ship_lo || zn.ship_min >= a.ship_hi) continue;
+        const bool full = zn.ship_min >= a.ship_lo && zn.ship_max < a.ship_hi;
         const int64_t u = z * kT + threadIdx.x;  // zone z = units z*256 .. z*256+255
         if (u >= units) continue;
         uint32_t m = 0xffu;
@@ -472,6 +485,79 @@ __global__ __launch_bounds__(kT) void q14_scan_k(Q14K a, const bolap_zone* __res
             total += uint64_t(v);
             promo += f ? uint64_t(v) : 0ull;
             cnt += 1;
+        }
+    }
+    // Rows past the last whole unit.
+    if (blockIdx.x == 0 && int64_t(threadIdx.x) < a.rows - units * 8) {
+        const int64_t i = units * 8 + threadIdx.x;
+        if (a.ship[i] >= a.ship_lo && a.ship[i] < a.ship_hi) q14_row(a, i, promo, total, cnt);
+    }
+    block_out3(promo, total, cnt, partials);
+}
+
+// Flat lineitem pass, kU units of 8 rows per lane in flight: all ship
+// loads, then the part-key and price loads of the units with a row in the
+// window, then one flag gather per row in the window (the flag array is
+// small enough to stay in L2), so the three dependent reads of a unit
+// overlap across kU units.
+template <int kU>
+__global__ __launch_bounds__(kT) void q14_flat_k(Q14K a, int64_t* partials) {
+    uint64_t promo = 0, total = 0, cnt = 0;
+    const int64_t units = a.rows / 8;
+    const int64_t stride = int64_t(gridDim.x) * kT * kU;
+    for (int64_t base = int64_t(blockIdx.x) * kT * kU + threadIdx.x; base < units;
+         base += stride) {
+        uint32_t m[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int64_t u = base + int64_t(k) * kT;
+            m[k] = 0;
+            if (u < units) {
+                const VecT<int16_t, 8> sv = load_unit<int16_t, 8>(a.ship, u);
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    m[k] |= (sv.v[r] >= a.ship_lo && sv.v[r] < a.ship_hi ? 1u : 0u) << r;
+            }
+        }
+        int32_t key[kU][8], dp[kU][8];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int64_t u = base + int64_t(k) * kT;
+            if (m[k]) {
+                const VecT<int32_t, 4> k0 = load_unit<int32_t, 4>(a.pk, 2 * u);
+                const VecT<int32_t, 4> k1 = load_unit<int32_t, 4>(a.pk, 2 * u + 1);
+                const VecT<int32_t, 4> d0 = load_unit<int32_t, 4>(a.dp, 2 * u);
+                const VecT<int32_t, 4> d1 = load_unit<int32_t, 4>(a.dp, 2 * u + 1);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    key[k][r] = k0.v[r];
+                    key[k][r + 4] = k1.v[r];
+                    dp[k][r] = d0.v[r];
+                    dp[k][r + 4] = d1.v[r];
+                }
+            }
+        }
+        int8_t f[kU][8];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                f[k][r] = -1;
+                if (((m[k] >> r) & 1u) && key[k][r] >= 0 && key[k][r] < a.size)
+                    f[k][r] = a.flag[key[k][r]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            if (!m[k]) continue;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                if (!((m[k] >> r) & 1u) || f[k][r] < 0) continue;
+                const int64_t v = dp[k][r];
+                total += uint64_t(v);
+                promo += f[k][r] ? uint64_t(v) : 0ull;
+                cnt += 1;
+            }
         }
     }
     // Rows past the last whole unit.
@@ -956,6 +1042,70 @@ int bolap_q14_parse(const char* date, int32_t* ship_lo, int32_t* ship_hi) {
     return BOLAP_OK;
 }
 
+namespace {
+
+constexpr int kQ14U = 4;  // units of 8 rows per lane in flight (flat pass)
+
+// One Q14 pass with a partkey array of `size` entries; *overflow is set
+// when a part key is >= size (the result is then not valid).
+int q14_pass(bolap_ctx* ctx, const bolap_q14_args* a, int64_t size, bolap_q14_result* out,
+             bool* overflow) {
+    const int64_t units = a->li_rows / 8;
+    const int64_t n_zones = (a->li_rows + kZoneRows - 1) / kZoneRows;
+    const int64_t groups = a->zones ? n_zones : (units + kT * kQ14U - 1) / (kT * kQ14U);
+    int64_t grid = groups < int64_t(ctx->num_cus) * 8 ? groups : int64_t(ctx->num_cus) * 8;
+    if (grid < 1) grid = 1;
+    WsArena ws;
+    const size_t o_packed = ws.plan(size_t(size) * 4);
+    const size_t o_flag = ws.plan(size_t(size));
+    const size_t o_part = ws.plan(size_t(grid) * 3 * sizeof(int64_t));
+    const size_t o_ovf = ws.plan(sizeof(int32_t));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int64_t)));
+    ws.base = static_cast<char*>(ctx->ws);
+    int32_t* packed = ws.at<int32_t>(o_packed);
+    int8_t* flag = ws.at<int8_t>(o_flag);
+    int32_t* ovf = ws.at<int32_t>(o_ovf);
+    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+    {
+        TraceScope ts(ctx, "q14_part_map");
+        hipLaunchKernelGGL(q14_clear_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
+                           packed, size, ovf);
+        BOLAP_TRY(launch_check("q14_clear_k"));
+        hipLaunchKernelGGL(q14_part_k, dim3(grid_of(ctx, a->part_rows, 4)), dim3(kT), 0,
+                           ctx->stream, a->p_partkey, a->p_promo, a->part_rows, packed, size,
+                           ovf);
+        BOLAP_TRY(launch_check("q14_part_k"));
+        hipLaunchKernelGGL(q14_narrow_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
+                           packed, size, flag);
+        BOLAP_TRY(launch_check("q14_narrow_k"));
+    }
+    Q14K k{a->l_ship, a->l_partkey, a->l_disc_price, a->li_rows, a->ship_lo, a->ship_hi, flag,
+           size};
+    {
+        TraceScope ts(ctx, "q14_lineitem_scan");
+        if (a->zones)
+            hipLaunchKernelGGL(q14_zoned_k, dim3(unsigned(grid)), dim3(kT), 0, ctx->stream,
+                               k, a->zones, ws.at<int64_t>(o_part));
+        else
+            hipLaunchKernelGGL(q14_flat_k<kQ14U>, dim3(unsigned(grid)), dim3(kT), 0,
+                               ctx->stream, k, ws.at<int64_t>(o_part));
+        BOLAP_TRY(launch_check("q14_scan_k"));
+    }
+    hipLaunchKernelGGL(q6_merge_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<int64_t>(o_part),
+                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev), ovf);
+    BOLAP_TRY(launch_check("q6_merge_k"));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t* hp = static_cast<const int64_t*>(ctx->pinned);
+    *overflow = hp[3] != 0;
+    out->promo_sum = hp[0];
+    out->total_sum = hp[1];
+    out->rows_emitted = hp[2];
+    return BOLAP_OK;
+}
+
+}  // namespace
+
 int bolap_q14(bolap_ctx* ctx, const bolap_q14_args* a, bolap_q14_result* out) {
     BOLAP_ARG_CHECK(ctx && a && out, "bolap_q14: null argument");
     BOLAP_ARG_CHECK(a->li_rows >= 0 && a->part_rows >= 0 && a->part_rows < (int64_t(1) << 30),
@@ -968,65 +1118,36 @@ int bolap_q14(bolap_ctx* ctx, const bolap_q14_args* a, bolap_q14_result* out) {
                                         aligned(a->l_disc_price, 16)),
                     "bolap_q14: lineitem columns must be 16-byte aligned");
     std::memset(out, 0, sizeof(*out));
-    int64_t* res = static_cast<int64_t*>(ctx->result_dev);
-    int64_t h[kQ18ResN];
-    {
-        TraceScope ts(ctx, "q14_part_scan");
-        hipLaunchKernelGGL(q18_init_k, dim3(1), dim3(kT), 0, ctx->stream, res);
-        BOLAP_TRY(launch_check("q18_init_k"));
-        hipLaunchKernelGGL(q18_max_k, dim3(grid_of(ctx, a->part_rows / 4, 4)), dim3(kT), 0,
-                           ctx->stream, nullptr, 0, nullptr, 0, a->p_partkey, a->part_rows, res);
-        BOLAP_TRY(launch_check("q18_max_k"));
-        BOLAP_TRY(read_result(ctx, res, kQ18ResN, h));
+    // The reference sizes its partkey array by the largest key
+    // (query_q14.cpp:193-223).  Part keys are 1..part_rows in TPC-H, so the
+    // first pass covers keys below part_rows * 1.5 + 1024 without a host
+    // round trip; a larger key reruns with the exact size.  Keys between the
+    // largest part key and the array size hold no part, so both sizes give
+    // the same sums.
+    int64_t size = a->part_rows + a->part_rows / 2 + 1024;
+    const int64_t forced = ctx->knob("q14.array_size", 0);
+    if (forced > 0) size = forced;
+    bool overflow = false;
+    BOLAP_TRY(q14_pass(ctx, a, size, out, &overflow));
+    if (overflow) {
+        int64_t* res = static_cast<int64_t*>(ctx->result_dev);
+        int64_t h[kQ18ResN];
+        {
+            TraceScope ts(ctx, "q14_part_scan");
+            hipLaunchKernelGGL(q18_init_k, dim3(1), dim3(kT), 0, ctx->stream, res);
+            BOLAP_TRY(launch_check("q18_init_k"));
+            hipLaunchKernelGGL(q18_max_k, dim3(grid_of(ctx, a->part_rows / 4, 4)), dim3(kT), 0,
+                               ctx->stream, nullptr, 0, nullptr, 0, a->p_partkey, a->part_rows,
+                               res);
+            BOLAP_TRY(launch_check("q18_max_k"));
+            BOLAP_TRY(read_result(ctx, res, kQ18ResN, h));
+        }
+        BOLAP_ARG_CHECK(h[kQ18MaxC] >= size, "bolap_q14: part key maximum %lld below %lld",
+                        (long long)h[kQ18MaxC], (long long)size);
+        BOLAP_TRY(q14_pass(ctx, a, h[kQ18MaxC] + 1, out, &overflow));
+        BOLAP_ARG_CHECK(!overflow, "bolap_q14: overflow at the exact size");
+        trace_count(ctx, "q14_fallbacks", 1);
     }
-    if (h[kQ18MaxC] < 0) return BOLAP_OK;  // query_q14.cpp:193-200
-    const int64_t size = h[kQ18MaxC] + 1;
-    const int64_t units = a->li_rows / 8;
-    const int64_t n_zones = (a->li_rows + kZoneRows - 1) / kZoneRows;
-    const int64_t groups = a->zones ? n_zones : (units + kT - 1) / kT;
-    int64_t grid = groups < int64_t(ctx->num_cus) * 8 ? groups : int64_t(ctx->num_cus) * 8;
-    if (grid < 1) grid = 1;
-    WsArena ws;
-    const size_t o_packed = ws.plan(size_t(size) * 4);
-    const size_t o_flag = ws.plan(size_t(size));
-    const size_t o_part = ws.plan(size_t(grid) * 3 * sizeof(int64_t));
-    BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int64_t)));
-    ws.base = static_cast<char*>(ctx->ws);
-    int32_t* packed = ws.at<int32_t>(o_packed);
-    int8_t* flag = ws.at<int8_t>(o_flag);
-    {
-        TraceScope ts(ctx, "q14_part_map");
-        hipLaunchKernelGGL(fill_i32_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
-                           packed, size, 0);
-        BOLAP_TRY(launch_check("fill_i32_k"));
-        hipLaunchKernelGGL(q14_part_k, dim3(grid_of(ctx, a->part_rows, 4)), dim3(kT), 0,
-                           ctx->stream, a->p_partkey, a->p_promo, a->part_rows, packed, size);
-        BOLAP_TRY(launch_check("q14_part_k"));
-        hipLaunchKernelGGL(q14_narrow_k, dim3(grid_of(ctx, size, 4)), dim3(kT), 0, ctx->stream,
-                           packed, size, flag);
-        BOLAP_TRY(launch_check("q14_narrow_k"));
-    }
-    Q14K k{a->l_ship, a->l_partkey, a->l_disc_price, a->li_rows, a->ship_lo, a->ship_hi, flag,
-           size};
-    {
-        TraceScope ts(ctx, "q14_lineitem_scan");
-        if (a->zones)
-            hipLaunchKernelGGL(q14_scan_k<true>, dim3(unsigned(grid)), dim3(kT), 0, ctx->stream,
-                               k, a->zones, ws.at<int64_t>(o_part));
-        else
-            hipLaunchKernelGGL(q14_scan_k<false>, dim3(unsigned(grid)), dim3(kT), 0, ctx->stream,
-                               k, nullptr, ws.at<int64_t>(o_part));
-        BOLAP_TRY(launch_check("q14_scan_k"));
-    }
-    hipLaunchKernelGGL(q6_merge_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<int64_t>(o_part),
-                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev));
-    BOLAP_TRY(launch_check("q6_merge_k"));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const int64_t* hp = static_cast<const int64_t*>(ctx->pinned);
-    out->promo_sum = hp[0];
-    out->total_sum = hp[1];
-    out->rows_emitted = hp[2];
     out->promo_revenue =
         out->total_sum > 0 ? 100.0 * double(out->promo_sum) / double(out->total_sum) : 0.0;
     trace_count(ctx, "q14_lineitem_scan_rows_emitted", out->rows_emitted);

@@ -407,6 +407,9 @@ class Engine:
         return lo.value, hi.value
 
     def q14(self, t: dict, ship_lo: int, ship_hi: int, zones=None) -> dict:
+        return self.q14_run(self.q14_args(t, ship_lo, ship_hi, zones))
+
+    def q14_args(self, t: dict, ship_lo: int, ship_hi: int, zones=None) -> N.Q14Args:
         a = N.Q14Args()
         a.li_rows = int(t["l_ship"].numel())
         a.l_ship, a.l_partkey = ptr(t["l_ship"]), ptr(t["l_partkey"])
@@ -415,6 +418,9 @@ class Engine:
         a.zones = ptr(zones) if zones is not None else None
         a.part_rows = int(t["p_partkey"].numel())
         a.p_partkey, a.p_promo = ptr(t["p_partkey"]), ptr(t["p_promo"])
+        return a
+
+    def q14_run(self, a: N.Q14Args) -> dict:
         r = N.Q14Result()
         N.check(self.lib.bolap_q14(self.ctx, C.byref(a), C.byref(r)))
         return {"promo_sum": r.promo_sum, "total_sum": r.total_sum,

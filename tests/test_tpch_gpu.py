@@ -254,6 +254,31 @@ def test_q14_duplicate_and_negative_part_keys(engine):
     assert engine.q14(t, lo, hi)["total_sum"] == 0
 
 
+def test_q14_sparse_part_keys_rerun_at_the_exact_size(engine):
+    # Keys far above the dense-key guess (1.5 x parts + 1024) take the exact
+    # rerun; lineitem keys are mapped the same way so rows still match.
+    t = dict(engine.gen_part_lineitem(SEED, 20_000, 600_003))
+    t["p_partkey"] = t["p_partkey"] * 7 + 100_000
+    t["l_partkey"] = t["l_partkey"] * 7 + 100_000
+    part, li = host_part_li(t)
+    lo, hi = engine.q14_window("1995-09-01")
+    got = engine.q14(t, lo, hi)
+    assert got == oracle.q14(li, part, lo, hi) and got["rows_emitted"] > 0
+
+
+@pytest.mark.parametrize("size", [1, 1000, 20_001, 1 << 20])
+def test_q14_result_independent_of_array_size(engine, size):
+    t = engine.gen_part_lineitem(SEED, 20_000, 600_003)
+    part, li = host_part_li(t)
+    lo, hi = engine.q14_window("1995-09-01")
+    engine.tune_set("q14.array_size", size)
+    try:
+        got = engine.q14(t, lo, hi)
+    finally:
+        engine.tune_set("q14.array_size", 0)
+    assert got == oracle.q14(li, part, lo, hi)
+
+
 @pytest.mark.parametrize("parts,rows", [(100, 1000), (20_000, 600_003), (2_000_000, 60_000_000)])
 def test_q17_matches_oracle(engine, parts, rows):
     t = engine.gen_part_lineitem(SEED, parts, rows)
@@ -265,6 +290,42 @@ def test_q17_matches_oracle(engine, parts, rows):
         opreds = [oracle.make_pred(part["brand"], 0, brand, brand),
                   oracle.make_pred(part["container"], 0, container, container)]
         assert got == oracle.q17(part["partkey"], opreds, li), (brand, container)
+
+
+@pytest.mark.parametrize("cap", [1, 3, 64])
+def test_q17_match_list_overflow_rescans(engine, cap):
+    # Workgroups whose match list overflows re-scan their own rows in pass 2;
+    # with cap 1 nearly all do, with 64 only some.
+    t = engine.gen_part_lineitem(SEED, 20_000, 600_003)
+    part, li = host_part_li(t)
+    preds = [engine.pred(t["p_brand"], N.OP_RANGE, 0, 2)]
+    engine.tune_set("q17.list_cap", cap)
+    try:
+        got = engine.q17(t, preds)
+    finally:
+        engine.tune_set("q17.list_cap", 0)
+    want = oracle.q17(part["partkey"], [oracle.make_pred(part["brand"], 0, 0, 2)], li)
+    assert got == want and got["matches"] > 0
+
+
+@pytest.mark.parametrize("brand_hi", [2, 12])  # LDS table / HBM table
+def test_q17_duplicate_negative_and_min_keys(engine, brand_hi):
+    import torch
+    t = dict(engine.gen_part_lineitem(SEED, 20_000, 600_003))
+    pk = t["p_partkey"].clone()
+    idx = torch.arange(0, pk.numel() - 1, 7, device=pk.device)
+    pk[idx] = pk[idx + 1]          # duplicated part rows
+    pk[5] = -3
+    t["p_partkey"] = pk
+    lk = t["l_partkey"].clone()
+    lk[::1000] = -2 ** 31          # the table's empty marker as a fact key
+    lk[1::1000] = -3
+    t["l_partkey"] = lk
+    part, li = host_part_li(t)
+    preds = [engine.pred(t["p_brand"], N.OP_RANGE, 0, brand_hi)]
+    got = engine.q17(t, preds)
+    want = oracle.q17(part["partkey"], [oracle.make_pred(part["brand"], 0, 0, brand_hi)], li)
+    assert got == want and got["matches"] > 0
 
 
 def test_q17_many_parts_take_the_hbm_table(engine):
