@@ -38,7 +38,7 @@
 namespace bolap {
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kCompactThreads = 256;
 constexpr int32_t kEmpty = INT32_MIN;
@@ -50,6 +50,9 @@ constexpr int kLdsBudget = 96 * 1024;
 // AUTO (48 KB at 128 bits per key), up to 4096 keys when LDS is forced.
 constexpr int64_t kAutoLdsKeys = 2048;
 constexpr int64_t kForcedLdsKeys = 4096;
+// Largest device-sized build (placement AUTO) whose filter stays in LDS
+// with the table in HBM (32 bits per key at the limit).
+constexpr int64_t kHybridKeys = 8192;
 // Bloom hits a probe workgroup collects in LDS before checking them
 // (beyond it, hits are checked in the loop).
 constexpr uint32_t kCandMax = 2048;
@@ -89,6 +92,7 @@ struct Shape {
     bool lds;
     int32_t cap_log2, bb_log2;
     int32_t flag;
+    bool lds_filter;  // table in HBM, filter built in LDS by every probe workgroup
 };
 
 __host__ __device__ inline int clog2(int64_t v) {
@@ -115,6 +119,12 @@ __host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
         return Shape{true, clog2(m * 2 < 64 ? 64 : m * 2),
                      clampi(clog2(m << g.lds_bits_log2), 10, 18), 0};
     if (g.placement == BOLAP_PLACE_LDS) return Shape{true, 0, 0, 4};
+    // Mid-size builds keep the filter in LDS (32 KB) and only the table in
+    // HBM: the filter test per fact key stays an LDS read, and only the
+    // collected candidates probe the HBM table.
+    if (g.placement == BOLAP_PLACE_AUTO && n <= kHybridKeys)
+        return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), clampi(clog2(m << 7), 10, 18), 0,
+                     true};
     return Shape{false, clog2(m * 2 < 64 ? 64 : m * 2), hbm_bloom_log2(m), 0};
 }
 
@@ -125,25 +135,27 @@ __host__ __device__ inline size_t lds_bytes_of(int cap_log2, int bb_log2) {
 BOLAP_DEV uint32_t slot_of(int32_t key, int cap_log2) {
     return (uint32_t(key) * kSlotMul) >> (32 - cap_log2);
 }
-// Blocked Bloom filter: one 32-bit word per key (chosen by the top bits of
-// a second multiplicative hash) with two bits set from middle bits of the
-// same hash.  bb_log2 counts bits; the word index has bb_log2 - 5 bits.
-// The false-positive rate must stay far below 1/512: a wave probes 512 keys
-// per tile, and one hit sends the whole wave down the exact-lookup path.
+// Blocked Bloom filter: one 32-bit word per key, chosen by the top
+// bb_log2 - 5 bits of a multiplicative hash, with two bits set at positions
+// taken from the ten hash bits just below (10 <= bb_log2 <= 27 keeps the
+// fields disjoint and inside the word).  For lambda keys per word the
+// false-positive rate is about (lambda + lambda^2) / 256.  The test costs
+// a handful of VALU operations per key, which bounds the probe loop as
+// much as the HBM stream does: a three-bit variant (second multiply) cut
+// false positives fourfold but made the loop slower.
 BOLAP_DEV uint32_t bloom_hash(int32_t key) { return uint32_t(key) * kBitMul; }
-BOLAP_DEV uint32_t bloom_word(uint32_t h, int bb_log2) { return h >> (32 - (bb_log2 - 5)); }
-// Two bit positions from the ten hash bits just below the word index (the
-// top bb_log2 - 5 bits pick the word; 10 <= bb_log2 <= 24 keeps the three
-// fields disjoint and inside the word).
-BOLAP_DEV bool bloom_test(uint32_t w, uint32_t h, int bb_log2) {
-    const int sh = 32 - bb_log2;
-    const uint32_t b1 = (h >> sh) & 31, b2 = (h >> (sh - 5)) & 31;
-    return ((w >> b1) & (w >> b2) & 1u) != 0;
+BOLAP_DEV uint32_t bloom_word(int32_t key, int bb_log2) {
+    return bloom_hash(key) >> (32 - (bb_log2 - 5));
 }
-BOLAP_DEV uint32_t bloom_bits(uint32_t h, int bb_log2) {
+BOLAP_DEV uint32_t bloom_mask(int32_t key, int bb_log2) {
+    const uint32_t h = bloom_hash(key);
     const int sh = 32 - bb_log2;
-    const uint32_t b1 = (h >> sh) & 31, b2 = (h >> (sh - 5)) & 31;
-    return (1u << b1) | (1u << b2);
+    return (1u << ((h >> sh) & 31)) | (1u << ((h >> (sh - 5)) & 31));
+}
+BOLAP_DEV bool bloom_test(uint32_t w, int32_t key, int bb_log2) {
+    const uint32_t h = bloom_hash(key);
+    const int sh = 32 - bb_log2;
+    return ((w >> ((h >> sh) & 31)) & (w >> ((h >> (sh - 5)) & 31)) & 1u) != 0;
 }
 
 // Inserts a key (not the empty marker) with linear probing; the table is
@@ -308,7 +320,7 @@ __global__ __launch_bounds__(256) void hbm_init_k(Geo g, Table t, KList kl,
             agg_cnt[i] = 0;
         }
     }
-    if (s.lds) return;
+    if (s.lds || s.lds_filter) return;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < words;
          i += int64_t(gridDim.x) * 256)
         t.bits[i] = 0;
@@ -323,8 +335,8 @@ __global__ __launch_bounds__(256) void hbm_build_k(Geo g, Table t, KList kl) {
     const int32_t* __restrict__ keys = kl.keys + int64_t(r) * kl.seg_cap;
     for (int64_t i = int64_t(blockIdx.x / kSegs) * 256 + threadIdx.x; i < seg[r]; i += per * 256) {
         const int32_t key = keys[i];
-        const uint32_t h = bloom_hash(key);
-        atomicOr(t.bits + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+        if (!s.lds_filter)
+            atomicOr(t.bits + bloom_word(key, s.bb_log2), bloom_mask(key, s.bb_log2));
         if (key != kEmpty) table_insert(t.slots, key, s.cap_log2);
     }
 }
@@ -406,6 +418,19 @@ BOLAP_DEV KeyTile<kProbeU> first_tile(const int32_t* __restrict__ fk, int64_t ro
     return t;
 }
 
+// Fills an LDS filter of 2^bb_log2 bits with the n keys of the list (one
+// key per thread per round); the caller orders it with a barrier.
+BOLAP_DEV void build_lds_filter(uint32_t* Bl, int bb_log2, const KList& kl, const uint32_t* ends,
+                                int64_t n) {
+    const int64_t words = (int64_t(1) << bb_log2) / 32;
+    for (int64_t i = threadIdx.x; i < words; i += blockDim.x) Bl[i] = 0;
+    lds_barrier();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t key = klist_key(kl, ends, i);
+        atomicOr(Bl + bloom_word(key, bb_log2), bloom_mask(key, bb_log2));
+    }
+}
+
 // Bloom hits collected by a probe loop (kCollect): (row, key) entries in
 // LDS, appended with one LDS atomic per wave; entries past `cap` go to the
 // loop's inline on_cand instead.  The caller zeroes *n before the loop
@@ -468,16 +493,13 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
         // Phase 1: all Bloom words first (independent reads), then a
         // per-lane hit mask over the kProbeU * 4 keys.
         constexpr int kKeys = kProbeU * 4;
-        uint32_t hv[kKeys], wv[kKeys];
+        uint32_t wv[kKeys];
 #pragma unroll
-        for (int i = 0; i < kKeys; ++i) {
-            hv[i] = bloom_hash(x.v[i / 4].v[i % 4]);
-            wv[i] = B[bloom_word(hv[i], bb_log2)];
-        }
+        for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(x.v[i / 4].v[i % 4], bb_log2)];
         uint32_t hits = 0;
 #pragma unroll
         for (int i = 0; i < kKeys; ++i)
-            hits |= (bloom_test(wv[i], hv[i], bb_log2) ? 1u : 0u) << i;
+            hits |= (bloom_test(wv[i], x.v[i / 4].v[i % 4], bb_log2) ? 1u : 0u) << i;
         // Units past the end were clamped to the last unit: drop them.
         if (at + int64_t(kProbeU - 1) * kThreads >= n_units) {
 #pragma unroll
@@ -516,16 +538,13 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
             for (int k = 0; k < kProbeU; ++k)
                 A.v[k] = load_unit<int32_t, 4>(fk, base + int64_t(k) * kThreads);
             constexpr int kKeys = kProbeU * 4;
-            uint32_t hv[kKeys], wv[kKeys];
+            uint32_t wv[kKeys];
 #pragma unroll
-            for (int i = 0; i < kKeys; ++i) {
-                hv[i] = bloom_hash(A.v[i / 4].v[i % 4]);
-                wv[i] = B[bloom_word(hv[i], bb_log2)];
-            }
+            for (int i = 0; i < kKeys; ++i) wv[i] = B[bloom_word(A.v[i / 4].v[i % 4], bb_log2)];
             uint32_t hits = 0;
 #pragma unroll
             for (int i = 0; i < kKeys; ++i)
-                hits |= (bloom_test(wv[i], hv[i], bb_log2) ? 1u : 0u) << i;
+                hits |= (bloom_test(wv[i], A.v[i / 4].v[i % 4], bb_log2) ? 1u : 0u) << i;
             if (hits) {
 #pragma unroll
                 for (int i = 0; i < kKeys; ++i)
@@ -538,8 +557,8 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
             const VecT<int32_t, 4> v = load_unit<int32_t, 4>(fk, base);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint32_t h = bloom_hash(v.v[i]);
-                if (bloom_test(B[bloom_word(h, bb_log2)], h, bb_log2)) on_cand(base * 4 + i, v.v[i]);
+                if (bloom_test(B[bloom_word(v.v[i], bb_log2)], v.v[i], bb_log2))
+                    on_cand(base * 4 + i, v.v[i]);
             }
         }
     } else if (n_units > 0 && base < n_units) {
@@ -589,7 +608,7 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
 }
 
 template <typename TP, typename TD, int kProbeU, int kPipe, bool kLastBlock, bool kCollect>
-__global__ __launch_bounds__(kThreads) void probe_sum_k(
+__global__ __launch_bounds__(kThreads, 8) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
     int64_t rows, Geo g, Table hbm, KList kl, int64_t* res, int64_t* partials,
     unsigned* counter, int64_t* host) {
@@ -624,8 +643,7 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
             // segment after another.
             for (int64_t i = threadIdx.x; i < n; i += kThreads) {
                 const int32_t key = klist_key(kl, ends, i);
-                const uint32_t h = bloom_hash(key);
-                atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+                atomicOr(Bl + bloom_word(key, s.bb_log2), bloom_mask(key, s.bb_log2));
                 if (key != kEmpty) table_insert(Tl, key, s.cap_log2);
             }
             if (threadIdx.x == 0) s_cn = 0;
@@ -637,9 +655,14 @@ __global__ __launch_bounds__(kThreads) void probe_sum_k(
                     add_match<TP, TD>(row, price, disc, rev, cnt);
             }, &first, cb);
         } else {
+            const uint32_t* Bf = hbm.bits;
+            if (s.lds_filter) {
+                build_lds_filter(lds, s.bb_log2, kl, ends, n);
+                Bf = lds;
+            }
             if (threadIdx.x == 0) s_cn = 0;
             lds_barrier();
-            probe_rows<kProbeU, kPipe, kCollect>(fk, rows, hbm.bits, s.bb_log2,
+            probe_rows<kProbeU, kPipe, kCollect>(fk, rows, Bf, s.bb_log2,
                                                  [&](int64_t row, int32_t key) {
                 if (key == kEmpty ? has_min : table_has(key, T, s.cap_log2, mask))
                     add_match<TP, TD>(row, price, disc, rev, cnt);
@@ -733,7 +756,7 @@ __host__ __device__ inline size_t q17_lds_bytes(int cap_log2, int bb_log2) {
 }
 
 template <int kPass>
-__global__ __launch_bounds__(kThreads) void q17_probe_k(
+__global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
     const int32_t* __restrict__ pk, const int16_t* __restrict__ qty,
     const int32_t* __restrict__ price, int64_t rows, Geo g, Table hbm, KList kl,
     uint64_t* agg_sum, uint32_t* agg_cnt, uint2* mlist, uint32_t* mcnt, int64_t list_cap,
@@ -805,13 +828,15 @@ __global__ __launch_bounds__(kThreads) void q17_probe_k(
             lds_barrier();
             for (int64_t i = threadIdx.x; i < n; i += kThreads) {
                 const int32_t key = klist_key(kl, ends, i);
-                const uint32_t h = bloom_hash(key);
-                atomicOr(Bl + bloom_word(h, s.bb_log2), bloom_bits(h, s.bb_log2));
+                atomicOr(Bl + bloom_word(key, s.bb_log2), bloom_mask(key, s.bb_log2));
                 atomicMin(Il + table_insert_at(Tl, key, s.cap_log2), uint32_t(i));
             }
             T = Tl;
             Ix = Il;
             Bf = Bl;
+        } else if (s.lds_filter) {
+            build_lds_filter(lds, s.bb_log2, kl, ends, n);
+            Bf = lds;
         }
         auto find_at = [&](int32_t key) -> int64_t {
             if (key == kEmpty) return -1;
@@ -1163,6 +1188,12 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
         if (a->placement != BOLAP_PLACE_HBM) {
             const Shape big = shape_of(lds_keys, g);
             lds = lds_bytes_of(big.cap_log2, big.bb_log2);
+        }
+        if (a->placement == BOLAP_PLACE_AUTO) {
+            // Room for the LDS filter of a mid-size build as well.
+            const Shape mid = shape_of(kHybridKeys, g);
+            const size_t fb = (size_t(1) << mid.bb_log2) / 8;
+            if (lds < fb) lds = fb;
         }
         BOLAP_TRY(join_attempt(ctx, a, ps, g, lds, r));
         if (shape_of(r[kBuild], g).flag & 4) {

@@ -57,6 +57,18 @@ def test_join_sum_placements_agree(engine, placement):
     assert same(got, want) and got["placement"] == placement
 
 
+@pytest.mark.parametrize("n_conts", [7, 9, 23, 26])
+def test_join_sum_mid_size_builds(engine, n_conts):
+    # About 2,400 / 3,000 / 7,800 / 8,800 build keys: past the LDS table
+    # limit (2,048) the table goes to HBM with the filter in LDS up to 8,192
+    # keys, then both go to HBM.
+    d = engine.gen_dim(SEED, 2_000_000)
+    f = engine.gen_fact(SEED, 3_000_001, 2_000_000)
+    got, want = run_both(engine, d, f, (7, 7), list(range(n_conts)))
+    assert same(got, want) and want["build_rows"] > 2048
+    assert got["placement"] == N.PLACE_HBM
+
+
 def test_join_sum_large_build_goes_to_hbm(engine):
     d = engine.gen_dim(SEED, 2_000_000)
     f = engine.gen_fact(SEED, 5_000_000, 2_000_000)

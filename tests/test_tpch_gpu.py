@@ -308,7 +308,7 @@ def test_q17_match_list_overflow_rescans(engine, cap):
     assert got == want and got["matches"] > 0
 
 
-@pytest.mark.parametrize("brand_hi", [2, 12])  # LDS table / HBM table
+@pytest.mark.parametrize("brand_hi", [0, 2, 12])  # LDS table / LDS filter + HBM table / HBM
 def test_q17_duplicate_negative_and_min_keys(engine, brand_hi):
     import torch
     t = dict(engine.gen_part_lineitem(SEED, 20_000, 600_003))
