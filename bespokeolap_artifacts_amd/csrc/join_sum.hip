@@ -561,43 +561,56 @@ BOLAP_DEV void probe_rows(const int32_t* __restrict__ fk, int64_t rows, const ui
                     on_cand(base * 4 + i, v.v[i]);
             }
         }
-    } else if (n_units > 0 && base < n_units) {
+    } else {
+        // Wave-uniform trip count (from the wave's first lane; a lane past
+        // the end loads a clamped unit whose hits are masked), so the loop
+        // is not structured around a divergent exit, which would copy the
+        // loop-carried tile registers at the latch and wait for their loads.
+        const int64_t wave_base = int64_t(blockIdx.x) * kThreads * kProbeU +
+                                  int64_t(__builtin_amdgcn_readfirstlane(threadIdx.x & ~(kWave - 1)));
+        const int64_t trips = n_units > wave_base ? (n_units - wave_base + stride - 1) / stride : 0;
         if constexpr (kPipe == 2) {
-            // Two register buffers: the next tile's keys are in flight while
-            // the current tile is probed.
+            // The scheduling barriers keep the next tile's loads issued
+            // before the wait for the current tile (otherwise the compiler
+            // hoists the current tile's hashing above them and waits for
+            // everything first, halving the loads in flight).
             Tile A, Bt;
-            if (first) A = *first;
-            else load_tile(A, base);
-            for (;;) {
+            if (trips > 0) {
+                if (first) A = *first;
+                else load_tile(A, base);
+            }
+            for (int64_t t = 0; t < trips; t += 2) {
                 load_tile(Bt, base + stride);
+                __builtin_amdgcn_sched_barrier(0);
                 use_tile(A, base);
-                base += stride;
-                if (base >= n_units) break;
-                load_tile(A, base + stride);
-                use_tile(Bt, base);
-                base += stride;
-                if (base >= n_units) break;
+                if (t + 1 >= trips) break;
+                load_tile(A, base + 2 * stride);
+                __builtin_amdgcn_sched_barrier(0);
+                use_tile(Bt, base + stride);
+                base += 2 * stride;
             }
         } else {
             // Three register buffers: two tiles in flight while one is
-            // probed, so a wave's exact-lookup work overlaps more latency.
+            // probed.
             Tile A, Bt, C;
-            if (first) A = *first;
-            else load_tile(A, base);
-            load_tile(Bt, base + stride);
-            for (;;) {
+            if (trips > 0) {
+                if (first) A = *first;
+                else load_tile(A, base);
+                load_tile(Bt, base + stride);
+            }
+            for (int64_t t = 0; t < trips; t += 3) {
                 load_tile(C, base + 2 * stride);
+                __builtin_amdgcn_sched_barrier(0);
                 use_tile(A, base);
-                base += stride;
-                if (base >= n_units) break;
-                load_tile(A, base + 2 * stride);
-                use_tile(Bt, base);
-                base += stride;
-                if (base >= n_units) break;
-                load_tile(Bt, base + 2 * stride);
-                use_tile(C, base);
-                base += stride;
-                if (base >= n_units) break;
+                if (t + 1 >= trips) break;
+                load_tile(A, base + 3 * stride);
+                __builtin_amdgcn_sched_barrier(0);
+                use_tile(Bt, base + stride);
+                if (t + 2 >= trips) break;
+                load_tile(Bt, base + 4 * stride);
+                __builtin_amdgcn_sched_barrier(0);
+                use_tile(C, base + 2 * stride);
+                base += 3 * stride;
             }
         }
     }
@@ -655,18 +668,19 @@ __global__ __launch_bounds__(kThreads, 8) void probe_sum_k(
                     add_match<TP, TD>(row, price, disc, rev, cnt);
             }, &first, cb);
         } else {
-            const uint32_t* Bf = hbm.bits;
-            if (s.lds_filter) {
-                build_lds_filter(lds, s.bb_log2, kl, ends, n);
-                Bf = lds;
-            }
-            if (threadIdx.x == 0) s_cn = 0;
-            lds_barrier();
-            probe_rows<kProbeU, kPipe, kCollect>(fk, rows, Bf, s.bb_log2,
-                                                 [&](int64_t row, int32_t key) {
+            auto check = [&](int64_t row, int32_t key) {
                 if (key == kEmpty ? has_min : table_has(key, T, s.cap_log2, mask))
                     add_match<TP, TD>(row, price, disc, rev, cnt);
-            }, &first, cb);
+            };
+            if (s.lds_filter) build_lds_filter(lds, s.bb_log2, kl, ends, n);
+            if (threadIdx.x == 0) s_cn = 0;
+            lds_barrier();
+            // Separate calls keep the LDS filter's reads LDS reads.
+            if (s.lds_filter)
+                probe_rows<kProbeU, kPipe, kCollect>(fk, rows, lds, s.bb_log2, check, &first, cb);
+            else
+                probe_rows<kProbeU, kPipe, kCollect>(fk, rows, hbm.bits, s.bb_log2, check, &first,
+                                                     cb);
         }
         if constexpr (kCollect) {
             // The collected candidates: exact checks and payload gathers of
@@ -813,7 +827,6 @@ __global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
         // part key is negative).
         const int32_t* T = hbm.slots;
         const uint32_t* Ix = nullptr;
-        const uint32_t* Bf = hbm.bits;
         if (s.lds) {
             const int64_t words = (int64_t(1) << s.bb_log2) / 32;
             const int64_t cap = int64_t(1) << s.cap_log2;
@@ -833,11 +846,11 @@ __global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
             }
             T = Tl;
             Ix = Il;
-            Bf = Bl;
         } else if (s.lds_filter) {
             build_lds_filter(lds, s.bb_log2, kl, ends, n);
-            Bf = lds;
         }
+        // The filter sits at the start of LDS for both LDS geometries.
+        const bool lds_bf = s.lds || s.lds_filter;
         auto find_at = [&](int32_t key) -> int64_t {
             if (key == kEmpty) return -1;
             const int32_t slot = table_find(key, T, s.cap_log2, mask);
@@ -855,7 +868,11 @@ __global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
             const CandBuf cb{s_cand, &s_cn, rows <= int64_t(UINT32_MAX) ? kQ17CandMax : 0u};
             if (threadIdx.x == 0) s_cn = 0;
             lds_barrier();
-            probe_rows<2, kQ17Pipe, true>(pk, rows, Bf, s.bb_log2, on_cand, &first, cb);
+            // Separate calls keep an LDS filter's reads LDS reads.
+            if (lds_bf)
+                probe_rows<2, kQ17Pipe, true>(pk, rows, lds, s.bb_log2, on_cand, &first, cb);
+            else
+                probe_rows<2, kQ17Pipe, true>(pk, rows, hbm.bits, s.bb_log2, on_cand, &first, cb);
             __syncthreads();
             const uint32_t nc = s_cn < cb.cap ? s_cn : cb.cap;
             for (uint32_t i = threadIdx.x; i < nc; i += kThreads) {
@@ -864,7 +881,10 @@ __global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
             }
         } else {
             lds_barrier();
-            probe_rows<2, kQ17Pipe>(pk, rows, Bf, s.bb_log2, on_cand);
+            if (lds_bf)
+                probe_rows<2, kQ17Pipe>(pk, rows, lds, s.bb_log2, on_cand);
+            else
+                probe_rows<2, kQ17Pipe>(pk, rows, hbm.bits, s.bb_log2, on_cand);
         }
     }
     __shared__ uint64_t s_a[kWaves], s_c[kWaves];
