@@ -455,6 +455,25 @@ def bench_c3(eng, torch, args):
            "stages_ms": {k: float(np.mean(v)) for k, v in stages.items()},
            "groups": st["groups"], "fact_matches": st["fact_matches"],
            "kernel": "fact_probe_k", "consistent": r_exact == first and r_hint == first}
+    # The reference's own layout: each order's fact rows as a contiguous range
+    # (lineitem_ranges, made on the device at build time, outside the timing).
+    _, start, end = eng.order_ranges(t["order_key"], t["fact_order"], order_rows + 1)
+    a_rng = eng.chain_args(t, 1, 1155, ranges=(start, end))
+    ms_rng, tr_r, r_rng = timed(a_rng)
+    _, st_r = eng.chain_topk(a_rng, 10)
+    k_ms = stage_ms(tr_r, "chain_ranges")
+    # orders' day and customer for every row; range, key and the price and
+    # discount of the fact rows for the passing orders
+    k_bytes = order_rows * 8 + st_r["order_pass"] * 12 + st_r["fact_matches"] * 16
+    stages_r = {}
+    for n, ms, _ in tr_r:
+        if ms >= 0:
+            stages_r.setdefault(n, []).append(ms)
+    res["by_ranges"] = {"ms_per_step": ms_rng, "kernel": "range_groups_k", "kernel_ms": k_ms,
+                        "kernel_bytes": k_bytes,
+                        "kernel_GBps": k_bytes / (k_ms * 1e-3) / 1e9 if k_ms else None,
+                        "stages_ms": {k: float(np.mean(v)) for k, v in stages_r.items()},
+                        "consistent": r_rng == first and st_r["groups"] == st["groups"]}
     if not args.no_cpu:
         import oracle
         h = {k: v.cpu().numpy() for k, v in t.items()}

@@ -361,6 +361,175 @@ BOLAP_DEV uint32_t hit_mask8(const uint8_t* __restrict__ hit, int64_t i0, int64_
     return m;
 }
 
+// Orders whose fact rows are given as contiguous ranges (the reference's
+// lineitem_ranges, query_q3.cpp:219-260): no group table, two passes.
+//   filter  workgroup b walks orders [b * chunk, (b + 1) * chunk), sixteen
+//           per lane per tile (four 16-byte loads per column), and appends
+//           the rows of passing orders to an LDS list that is flushed to
+//           `pass_idx` with one atomic per flush (when it could not take
+//           another tile, and at the end);
+//   sum     the passing orders, one per lane with every lane busy: range,
+//           key and day gathers, then the range's fact rows (eight
+//           independent loads per round); orders with rows append their
+//           group row through an LDS buffer the same way.
+constexpr int kRangeBuf = 2048;  // staged group rows; a sum tile adds at most kT * 4
+constexpr int kFilterPer = 16;   // orders per lane per filter tile
+constexpr int kPassBuf = kT * kFilterPer;  // orders per filter tile
+constexpr int kPassCap = 2 * kPassBuf;     // staged passing orders (32 KB)
+
+struct RangeArgs {
+    const int32_t* okey;
+    const int32_t* day;
+    const int32_t* cust;
+    const uint32_t* start;
+    const uint32_t* end;
+    int64_t n;
+    int32_t day_lt;
+    int64_t chunk;
+};
+
+// Appends `cnt` entries staged in LDS to `out` (one atomic), then empties
+// the stage.  Every thread of the workgroup calls it with the same cnt.
+template <typename T>
+BOLAP_DEV void flush_stage(T* stage, uint32_t* s_n, unsigned long long* s_base, uint32_t cnt,
+                           T* __restrict__ out, unsigned long long* __restrict__ n_out) {
+    if (threadIdx.x == 0) *s_base = cnt ? atomicAdd(n_out, (unsigned long long)cnt) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += kT) out[*s_base + i] = stage[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_n = 0;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kT) void range_filter_k(RangeArgs a, const uint32_t* __restrict__ cbits,
+                                                     int64_t cspan, Ranges r,
+                                                     uint32_t* __restrict__ pass_idx,
+                                                     unsigned long long* __restrict__ n_pass,
+                                                     int64_t* st) {
+    __shared__ uint32_t s_idx[kPassCap];
+    __shared__ uint32_t s_n;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    WaveCount passed;
+    const int64_t b0 = int64_t(blockIdx.x) * a.chunk;
+    const int64_t b1 = b0 + a.chunk < a.n ? b0 + a.chunk : a.n;
+    for (int64_t t0 = b0; t0 < b1; t0 += int64_t(kPassBuf)) {
+        // Lane l takes units t0/4 + l + q * kT, q < 4 (coalesced per q).
+        Orders4 o[kFilterPer / 4];
+#pragma unroll
+        for (int q = 0; q < kFilterPer / 4; ++q) {
+            const int64_t i0 = t0 + (int64_t(threadIdx.x) + int64_t(q) * kT) * 4;
+            if (i0 < b1) o[q] = load_orders4(a.okey, a.day, a.cust, b1, i0 / 4, false);
+            else o[q].cnt = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kFilterPer / 4; ++q) {
+            const int64_t i0 = t0 + (int64_t(threadIdx.x) + int64_t(q) * kT) * 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool pass = j < o[q].cnt && order_pass_v(o[q].day[j], o[q].cust[j],
+                                                               a.day_lt, cbits, cspan, r);
+                passed.add(pass);
+                if (pass) s_idx[atomicAdd(&s_n, 1u)] = uint32_t(i0 + j);
+            }
+        }
+        __syncthreads();
+        const uint32_t cnt = s_n;
+        __syncthreads();
+        if (cnt > uint32_t(kPassCap - kPassBuf))
+            flush_stage(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
+    }
+    __syncthreads();
+    const uint32_t cnt = s_n;
+    __syncthreads();
+    flush_stage(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
+    block_fold<kT>(st, sOrderPass, int64_t(passed.n), 0);
+}
+
+template <typename TP, typename TD>
+__global__ __launch_bounds__(kT) void range_groups_k(RangeArgs a, const TP* __restrict__ price,
+                                                     const TD* __restrict__ disc,
+                                                     const int32_t* __restrict__ dprice,
+                                                     const uint32_t* __restrict__ pass_idx,
+                                                     const unsigned long long* __restrict__ n_pass,
+                                                     RowT* __restrict__ out,
+                                                     unsigned long long* __restrict__ n_out,
+                                                     int64_t* st) {
+    __shared__ RowT s_rows[kRangeBuf];
+    __shared__ uint32_t s_n;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    int64_t dl = INT64_MAX, dh = INT64_MIN, kl = INT64_MAX, kh = INT64_MIN;
+    int64_t rl = INT64_MAX, rh = INT64_MIN;
+    uint64_t matched = 0;
+    const int64_t np = int64_t(*n_pass);
+    // Contiguous share of the list per workgroup, whole tiles of kT * 4.
+    int64_t chunk = (np + gridDim.x - 1) / gridDim.x;
+    chunk = (chunk + kT * 4 - 1) / (kT * 4) * (kT * 4);
+    const int64_t b0 = int64_t(blockIdx.x) * chunk;
+    const int64_t b1 = b0 + chunk < np ? b0 + chunk : np;
+    for (int64_t t0 = b0; t0 < b1; t0 += int64_t(kT) * 4) {
+        uint32_t oi[4], s[4], e[4];
+        int32_t key[4], day[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t p = t0 + threadIdx.x + int64_t(q) * kT;
+            oi[q] = p < b1 ? pass_idx[p] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s[q] = e[q] = 0;
+            if (oi[q] != 0xffffffffu) {
+                s[q] = a.start[oi[q]];
+                e[q] = a.end[oi[q]];
+                key[q] = a.okey[oi[q]];
+                day[q] = a.day[oi[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (e[q] <= s[q]) continue;
+            uint64_t acc = 0;
+            for (uint32_t r0 = s[q]; r0 < e[q]; r0 += 8) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t row = r0 + k;
+                    if (row < e[q]) {
+                        acc += dprice ? uint64_t(int64_t(dprice[row]))
+                                      : uint64_t(int64_t(price[row])) *
+                                            uint64_t(100 - int64_t(disc[row]));
+                    }
+                }
+            }
+            matched += e[q] - s[q];
+            const int64_t rev = int64_t(acc);
+            rl = rev < rl ? rev : rl;
+            rh = rev > rh ? rev : rh;
+            dl = day[q] < dl ? day[q] : dl;
+            dh = day[q] > dh ? day[q] : dh;
+            kl = key[q] < kl ? key[q] : kl;
+            kh = key[q] > kh ? key[q] : kh;
+            s_rows[atomicAdd(&s_n, 1u)] = RowT{rev, day[q], key[q]};
+        }
+        // Every thread reads the fill level before any appends again, so
+        // that all of them take the same decision.
+        __syncthreads();
+        const uint32_t cnt = s_n;
+        __syncthreads();
+        if (cnt > uint32_t(kRangeBuf - kT * 4)) flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+    }
+    __syncthreads();
+    const uint32_t cnt = s_n;
+    __syncthreads();
+    flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+    block_fold<kT>(st, sMatch, int64_t(wave_sum(matched)), 0);
+    fold_minmax<kT>(dl, dh, st, sDMin, sDMax);
+    fold_minmax<kT>(kl, kh, st, sOMin, sOMax);
+    fold_minmax<kT>(rl, rh, st, sRMin, sRMax);
+}
+
 // Each workgroup compacts one contiguous chunk of slots (a multiple of
 // kCTile): pass 1 counts the hit slots, one atomic reserves the output
 // range, pass 2 writes the rows in slot order.
@@ -648,6 +817,99 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
     return BOLAP_OK;
 }
 
+// The group-by when each order's fact rows are a given contiguous range:
+// customer bitmap, then one pass over the orders that sums the ranges of
+// the passing ones.  Rows at workspace offset *o_rows_out.
+int chain_groups_ranges(bolap_ctx* ctx, const bolap_chain_args* a, WsArena& ws,
+                        size_t* o_rows_out, ChainRun& run) {
+    int64_t* st = static_cast<int64_t*>(ctx->result_dev);
+    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+    {
+        TraceScope ts(ctx, "chain_stats");
+        hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->cust_rows / 4 + 1)), dim3(kT), 0,
+                           ctx->stream, a->cust_key, a->cust_rows, nullptr, int64_t(0), st);
+    }
+    BOLAP_TRY(launch_check("key_minmax_k"));
+    int64_t cst[sN];
+    BOLAP_TRY(read_stats(ctx, cst));
+    Ranges r{cst[sCMin], cst[sCMax], 0, 0};
+    const int64_t cspan = a->cust_rows > 0 ? r.cmax - r.cmin + 1 : 0;
+    const int64_t cwords = (cspan + 31) / 32 + 1;
+    // Rows and passing orders: at most one per order.  Filter chunks are
+    // whole tiles of kPassBuf orders.
+    const int64_t blocks_max = int64_t(ctx->num_cus) * 4;
+    int64_t chunk = (a->order_rows + blocks_max - 1) / blocks_max;
+    chunk = (chunk + kPassBuf - 1) / kPassBuf * kPassBuf;
+    const int64_t blocks = a->order_rows > 0 ? (a->order_rows + chunk - 1) / chunk : 0;
+    const size_t o_cb = ws.plan(size_t(cwords) * 4);
+    const size_t o_rows = ws.plan(size_t(a->order_rows > 0 ? a->order_rows : 1) * sizeof(RowT));
+    const size_t o_pass = ws.plan(size_t(a->order_rows > 0 ? a->order_rows : 1) * 4);
+    const size_t o_cnt = ws.plan(64);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+    hipLaunchKernelGGL(fill_u32_k, dim3(grid_for(ctx, cwords)), dim3(kT), 0, ctx->stream,
+                       ws.at<uint32_t>(o_cb), cwords, 0u);
+    auto* n_rows = ws.at<unsigned long long>(o_cnt);
+    auto* n_pass = n_rows + 1;
+    hipLaunchKernelGGL(fill_u32_k, dim3(1), dim3(kT), 0, ctx->stream,
+                       reinterpret_cast<uint32_t*>(n_rows), 4, 0u);
+    if (a->cust_rows > 0) {
+        TraceScope ts(ctx, "chain_cust");
+        hipLaunchKernelGGL(cust_bits_k, dim3(grid_for(ctx, a->cust_rows)), dim3(kT), 0,
+                           ctx->stream, a->cust_key, a->cust_segment, a->cust_rows, a->segment,
+                           r, ws.at<uint32_t>(o_cb), st);
+    }
+    BOLAP_TRY(launch_check("cust_bits_k"));
+    if (blocks > 0) {
+        const RangeArgs ra{a->order_key, a->order_day, a->order_cust, a->order_fact_start,
+                           a->order_fact_end, a->order_rows, a->day_lt, chunk};
+        const uint32_t* cb = ws.at<uint32_t>(o_cb);
+        uint32_t* pass = ws.at<uint32_t>(o_pass);
+        {
+            TraceScope ts(ctx, "chain_range_filter");
+            hipLaunchKernelGGL(range_filter_k, dim3(unsigned(blocks)), dim3(kT), 0, ctx->stream,
+                               ra, cb, cspan, r, pass, n_pass, st);
+            BOLAP_TRY(launch_check("range_filter_k"));
+        }
+        TraceScope ts(ctx, "chain_ranges");
+        RowT* rows = ws.at<RowT>(o_rows);
+        const bool p64 = a->price_dtype == BOLAP_I64, d64 = a->disc_dtype == BOLAP_I64;
+        const int32_t* dp = a->fact_disc_price;
+        const dim3 g = dim3(unsigned(blocks_max)), b = dim3(kT);
+        if (p64 && d64)
+            hipLaunchKernelGGL((range_groups_k<int64_t, int64_t>), g, b, 0, ctx->stream, ra,
+                               static_cast<const int64_t*>(a->fact_price),
+                               static_cast<const int64_t*>(a->fact_disc), dp, pass, n_pass, rows,
+                               n_rows, st);
+        else if (p64)
+            hipLaunchKernelGGL((range_groups_k<int64_t, uint8_t>), g, b, 0, ctx->stream, ra,
+                               static_cast<const int64_t*>(a->fact_price),
+                               static_cast<const uint8_t*>(a->fact_disc), dp, pass, n_pass, rows,
+                               n_rows, st);
+        else if (d64)
+            hipLaunchKernelGGL((range_groups_k<int32_t, int64_t>), g, b, 0, ctx->stream, ra,
+                               static_cast<const int32_t*>(a->fact_price),
+                               static_cast<const int64_t*>(a->fact_disc), dp, pass, n_pass, rows,
+                               n_rows, st);
+        else
+            hipLaunchKernelGGL((range_groups_k<int32_t, uint8_t>), g, b, 0, ctx->stream, ra,
+                               static_cast<const int32_t*>(a->fact_price),
+                               static_cast<const uint8_t*>(a->fact_disc), dp, pass, n_pass, rows,
+                               n_rows, st);
+    }
+    BOLAP_TRY(launch_check("range_groups_k"));
+    BOLAP_TRY(read_stats(ctx, run.st));
+    BOLAP_TRY(pinned_reserve(ctx, 8));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(&run.n_groups, ctx->pinned, 8);
+    run.st[sGroups] = run.n_groups;
+    run.cap_log2 = -1;
+    *o_rows_out = o_rows;
+    return BOLAP_OK;
+}
+
 // Orders the n rows at workspace offset o_rows and writes the first `limit`
 // as bolap_group_row at workspace offset *o_out.
 int order_rows(bolap_ctx* ctx, WsArena& ws, size_t o_rows, int64_t n, int64_t limit,
@@ -722,11 +984,15 @@ int chain_validate(const bolap_chain_args* a) {
                     "bolap_chain: null customer column");
     BOLAP_ARG_CHECK(a->order_rows == 0 || (a->order_key && a->order_day && a->order_cust),
                     "bolap_chain: null order column");
-    BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_order && (a->fact_disc_price ||
-                                                            (a->fact_price && a->fact_disc))),
+    BOLAP_ARG_CHECK(a->fact_rows == 0 || ((a->fact_order || a->order_fact_start) &&
+                                          (a->fact_disc_price || (a->fact_price && a->fact_disc))),
                     "bolap_chain: null fact column");
-    BOLAP_ARG_CHECK(a->fact_rows == 0 || aligned(a->fact_order, 16),
+    BOLAP_ARG_CHECK((a->order_fact_start == nullptr) == (a->order_fact_end == nullptr),
+                    "bolap_chain: order_fact_start and order_fact_end go together");
+    BOLAP_ARG_CHECK(a->order_fact_start || a->fact_rows == 0 || aligned(a->fact_order, 16),
                     "bolap_chain: fact_order must be 16-byte aligned");
+    BOLAP_ARG_CHECK(!a->order_fact_start || a->fact_rows < (int64_t(1) << 32),
+                    "bolap_chain: fact ranges are 32-bit row numbers");
     BOLAP_ARG_CHECK(a->order_rows == 0 || (aligned(a->order_key, 16) && aligned(a->order_day, 16) &&
                                            aligned(a->order_cust, 16)),
                     "bolap_chain: order columns must be 16-byte aligned");
@@ -750,9 +1016,13 @@ int chain_run(bolap_ctx* ctx, const bolap_chain_args* a, int64_t limit,
     WsArena ws;
     size_t o_rows = 0;
     int fallbacks = 0;
+    const bool ranged = a->order_fact_start != nullptr;
     const bool exact = a->capacity_hint <= 0;
-    BOLAP_TRY(chain_groups(ctx, a, a->capacity_hint, exact, ws, &o_rows, run));
-    if (!exact && (run.st[sFlags] != 0)) {
+    if (ranged)
+        BOLAP_TRY(chain_groups_ranges(ctx, a, ws, &o_rows, run));
+    else
+        BOLAP_TRY(chain_groups(ctx, a, a->capacity_hint, exact, ws, &o_rows, run));
+    if (!ranged && !exact && (run.st[sFlags] != 0)) {
         fallbacks = 1;  // the estimate was too small: exact sizing
         ws = WsArena{};
         BOLAP_TRY(chain_groups(ctx, a, 0, true, ws, &o_rows, run));
@@ -789,7 +1059,7 @@ int chain_run(bolap_ctx* ctx, const bolap_chain_args* a, int64_t limit,
         stats->order_pass = run.st[sOrderPass];
         stats->fact_matches = run.st[sMatch];
         stats->groups = n;
-        stats->capacity = int64_t(1) << run.cap_log2;
+        stats->capacity = run.cap_log2 >= 0 ? int64_t(1) << run.cap_log2 : 0;
         stats->fallbacks = fallbacks;
         stats->sort_passes = passes;
     }

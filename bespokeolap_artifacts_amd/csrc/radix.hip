@@ -95,7 +95,6 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
 // ---------------------------------------------------------------- top-k
 constexpr int kSelBits = 12;
 constexpr int kSelBins = 1 << kSelBits;
-constexpr int kHistBlocks = 64;
 constexpr int kSelMax = 8192;   // candidates sorted by one workgroup in LDS
 
 struct KeyIdx {
@@ -109,8 +108,10 @@ BOLAP_DEV uint32_t sel_digit(const KeyIdx& e, int field, int shift) {
     return uint32_t((field ? e.idx : e.key) >> shift) & (kSelBins - 1);
 }
 
-// Histogram of one digit among the candidates.  `cand` may be null (then
-// the candidates are the whole input `keys`, tagged by position).
+// Histogram of one digit among the candidates, added into the global
+// histogram `hist` (zero on entry) with one atomic per non-empty LDS bin.
+// `cand` may be null (then the candidates are the whole input `keys`,
+// tagged by position).
 __global__ __launch_bounds__(kRT) void sel_hist_k(const uint64_t* __restrict__ keys,
                                                   const KeyIdx* __restrict__ cand, int64_t n,
                                                   int field, int shift,
@@ -125,19 +126,18 @@ __global__ __launch_bounds__(kRT) void sel_hist_k(const uint64_t* __restrict__ k
     }
     __syncthreads();
     for (int b = threadIdx.x; b < kSelBins; b += kRT)
-        hist[int64_t(blockIdx.x) * kSelBins + b] = h[b];
+        if (h[b]) atomicAdd(hist + b, h[b]);
 }
 
-// Sums the per-workgroup histograms and finds the bucket b* holding the
-// need-th smallest candidate: info = {b*, count below b*, count in b*}.
-__global__ __launch_bounds__(1024) void sel_pick_k(const uint32_t* __restrict__ hist,
-                                                   int nblocks, int64_t need,
+// Finds the bucket b* of the global histogram holding the need-th smallest
+// candidate: info = {b*, count below b*, count in b*}; then clears the
+// histogram for the next digit.
+__global__ __launch_bounds__(1024) void sel_pick_k(uint32_t* __restrict__ hist, int64_t need,
                                                    int64_t* __restrict__ info) {
     __shared__ uint32_t tot[kSelBins];
     for (int b = threadIdx.x; b < kSelBins; b += 1024) {
-        uint32_t s = 0;
-        for (int r = 0; r < nblocks; ++r) s += hist[int64_t(r) * kSelBins + b];
-        tot[b] = s;
+        tot[b] = hist[b];
+        hist[b] = 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -236,7 +236,9 @@ __global__ __launch_bounds__(kRT) void sel_concat_k(const KeyIdx* __restrict__ s
 }
 
 __global__ void zero_u64_k(unsigned long long* p, int n) {
-    if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n;
+         i += int(gridDim.x * blockDim.x))
+        p[i] = 0;
 }
 
 }  // namespace
@@ -300,7 +302,7 @@ int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
 
 size_t topk_scratch_bytes(int64_t n) {
     WsArena w;
-    w.plan(size_t(kHistBlocks) * kSelBins * 4);
+    w.plan(size_t(kSelBins) * 4);
     w.plan(64);
     w.plan(64);
     w.plan(size_t(n) * sizeof(KeyIdx));  // sure
@@ -314,7 +316,7 @@ int topk_pairs(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64
     if (n <= 0 || k <= 0) return BOLAP_OK;
     BOLAP_ARG_CHECK(k <= kSelMax, "topk: k = %d exceeds %d", k, kSelMax);
     WsArena w;
-    const size_t o_h = w.plan(size_t(kHistBlocks) * kSelBins * 4);
+    const size_t o_h = w.plan(size_t(kSelBins) * 4);
     const size_t o_info = w.plan(64);
     const size_t o_cnt = w.plan(64);
     const size_t o_sure = w.plan(size_t(n) * sizeof(KeyIdx));
@@ -339,15 +341,17 @@ int topk_pairs(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64
     const int sched_field[8] = {0, 0, 0, 0, 0, 0, 1, 1};
     const int sched_shift[8] = {52, 40, 28, 16, 4, 0, 20, 8};
     hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, ctx->stream, cnt, 2);
+    hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(1024), 0, ctx->stream,
+                       reinterpret_cast<unsigned long long*>(hist), kSelBins / 2);
     for (int r = 0; r < 9 && n_sure + m > kSelMax && m > need; ++r) {
         const int field = r < 8 ? sched_field[r] : 1;
         const int sh = r < 8 ? sched_shift[r] : 0;
-        int64_t hb = (m + kRT - 1) / kRT;
-        if (hb > kHistBlocks) hb = kHistBlocks;
+        int64_t hb = (m + kRT * 8 - 1) / (kRT * 8);
+        if (hb > int64_t(ctx->num_cus) * 2) hb = int64_t(ctx->num_cus) * 2;
+        if (hb < 1) hb = 1;
         hipLaunchKernelGGL(sel_hist_k, dim3(unsigned(hb)), dim3(kRT), 0, ctx->stream, keys, cand,
                            m, field, sh, hist);
-        hipLaunchKernelGGL(sel_pick_k, dim3(1), dim3(1024), 0, ctx->stream, hist, int(hb), need,
-                           info);
+        hipLaunchKernelGGL(sel_pick_k, dim3(1), dim3(1024), 0, ctx->stream, hist, need, info);
         KeyIdx* next = (cand == ca) ? cb : ca;
         hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 1, 1);
         int64_t sb = (m + kRT - 1) / kRT;

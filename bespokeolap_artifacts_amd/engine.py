@@ -238,7 +238,9 @@ class Engine:
 
     # ------------------------------------------------- chain join + group-by
     def chain_args(self, t: dict, segment: int, day_lt: int, capacity_hint: int = 0,
-                   disc_price=None) -> N.ChainArgs:
+                   disc_price=None, ranges=None) -> N.ChainArgs:
+        """`ranges` = (start, end) device uint32/int32 arrays per order row
+        (order_ranges): the fact side is then read by ranges."""
         a = N.ChainArgs()
         a.cust_rows = int(t["cust_key"].numel())
         a.cust_key, a.cust_segment = ptr(t["cust_key"]), ptr(t["cust_segment"])
@@ -255,6 +257,8 @@ class Engine:
         a.disc_dtype = dtype_code(t["fact_disc"]) if t.get("fact_disc") is not None else N.I64
         a.fact_disc_price = ptr(disc_price)
         a.capacity_hint = int(capacity_hint)
+        if ranges is not None:
+            a.order_fact_start, a.order_fact_end = ptr(ranges[0]), ptr(ranges[1])
         return a
 
     @staticmethod

@@ -139,7 +139,8 @@ class ChainArgs(C.Structure):
                 ("order_day", vp), ("order_cust", vp), ("day_lt", C.c_int32),
                 ("fact_rows", C.c_int64), ("fact_order", vp), ("fact_price", vp),
                 ("price_dtype", C.c_int32), ("fact_disc", vp), ("disc_dtype", C.c_int32),
-                ("fact_disc_price", vp), ("capacity_hint", C.c_int64)]
+                ("fact_disc_price", vp), ("capacity_hint", C.c_int64),
+                ("order_fact_start", vp), ("order_fact_end", vp)]
 
 
 class GroupRow(C.Structure):

@@ -286,6 +286,14 @@ typedef struct {
                                         (builder_impl.cpp:523-532); when set,
                                         revenue sums it instead */
     int64_t capacity_hint;           /* 0 = exact sizing; >0 = estimated groups */
+    /* Optional, together: order row i's fact rows are exactly
+       [order_fact_start[i], order_fact_end[i]) (the reference's
+       lineitem_ranges, builder_impl.cpp:929-991, as bolap_order_ranges makes
+       them; query_q3.cpp:219-260 reads the fact side through them).  The
+       fact side is then read by ranges: fact_order and capacity_hint are
+       unused and no group table is built. */
+    const uint32_t* order_fact_start;
+    const uint32_t* order_fact_end;
 } bolap_chain_args;
 
 typedef struct {

@@ -110,3 +110,93 @@ def test_chain_empty_results(engine):
     assert top == [] and gs["groups"] == 0
     top, gs = engine.chain_topk(engine.chain_args(t, 1, 0), 10)  # no order before day 0
     assert top == [] and gs["order_pass"] == 0
+
+
+# ---------------------------------------------------------------- by ranges
+def offsets_ranges(t):
+    import torch
+    off = t["fact_offsets"]
+    return off[:-1].to(torch.int32).contiguous(), off[1:].to(torch.int32).contiguous()
+
+
+@pytest.mark.parametrize("cust,orders", [(10, 40), (1500, 15_000), (150_000, 1_500_000),
+                                         (1_500_000, 15_000_000)])
+def test_chain_by_ranges_matches_oracle(engine, cust, orders):
+    t = engine.gen_chain(SEED, cust, orders)
+    want, st = want_sorted(t, 1, 1155)
+    _, start, end = engine.order_ranges(t["order_key"], t["fact_order"],
+                                        int(t["order_key"].max().item()) + 1)
+    a = engine.chain_args(t, 1, 1155, ranges=(start, end))
+    top, gs = engine.chain_topk(a, 10)
+    assert np.array_equal(as_array(top), want[:10])
+    check_stats(gs, st)
+    if orders <= 1_500_000:
+        allg, gs2 = engine.chain_groups_sorted(a, len(want) + 5)
+        assert np.array_equal(allg, want)
+        check_stats(gs2, st)
+
+
+def test_chain_by_ranges_shuffled_orders_and_orders_without_facts(engine):
+    import torch
+    t = engine.gen_chain(SEED, 5000, 60_000)
+    # Drop the fact rows of every tenth order, then shuffle the order rows.
+    keep = (t["fact_order"] % 10) != 3
+    t2 = dict(t)
+    for k in ("fact_order", "fact_price", "fact_disc"):
+        t2[k] = t[k][keep].contiguous()
+    perm = torch.randperm(t["order_key"].numel(), generator=torch.Generator().manual_seed(5))
+    perm = perm.cuda()
+    for k in ("order_key", "order_day", "order_cust"):
+        t2[k] = t[k][perm].contiguous()
+    want, st = want_sorted(t2, 2, 1500)
+    _, start, end = engine.order_ranges(t2["order_key"], t2["fact_order"],
+                                        int(t2["order_key"].max().item()) + 1)
+    assert int((end == start).sum().item()) > 0
+    allg, gs = engine.chain_groups_sorted(engine.chain_args(t2, 2, 1500, ranges=(start, end)),
+                                          100_000)
+    assert np.array_equal(allg, want)
+    check_stats(gs, st)
+    assert gs["order_pass"] > gs["groups"]
+
+
+def test_chain_by_ranges_narrow_types_and_disc_price(engine):
+    import torch
+    t = engine.gen_chain(SEED, 5000, 60_000)
+    rng = offsets_ranges(t)
+    t32 = dict(t)
+    t32["fact_price"] = t["fact_price"].to(torch.int32)
+    t32["fact_disc"] = t["fact_disc"].to(torch.uint8)
+    want, _ = want_sorted(t, 2, 900)
+    allg, _ = engine.chain_groups_sorted(engine.chain_args(t32, 2, 900, ranges=rng), 100_000)
+    assert np.array_equal(allg, want)
+    dp = ((t["fact_price"] * (100 - t["fact_disc"])) // 100).to(torch.int32)
+    want_dp, _ = want_sorted(t, 2, 900, disc_price=dp)
+    allg, _ = engine.chain_groups_sorted(engine.chain_args(t, 2, 900, disc_price=dp, ranges=rng),
+                                         100_000)
+    assert np.array_equal(allg, want_dp)
+
+
+def test_chain_by_ranges_negative_keys_and_wide_values(engine):
+    import torch
+    t = engine.gen_chain(SEED, 3000, 30_000)
+    okey = (t["order_key"].to(torch.int64) * 7_001 - 100_000_000).to(torch.int32)
+    remap = {int(a): int(b) for a, b in zip(t["order_key"].cpu().tolist(), okey.cpu().tolist())}
+    fo = torch.tensor([remap[int(x)] for x in t["fact_order"].cpu().tolist()],
+                      dtype=torch.int32).cuda()
+    t2 = dict(t)
+    t2["order_key"], t2["fact_order"] = okey.contiguous(), fo
+    t2["fact_price"] = t["fact_price"] * (1 << 30)
+    want, _ = want_sorted(t2, 0, 2000)
+    allg, gs = engine.chain_groups_sorted(engine.chain_args(t2, 0, 2000,
+                                                            ranges=offsets_ranges(t2)), 100_000)
+    assert np.array_equal(allg, want)
+    assert gs["sort_passes"] > 8
+
+
+def test_chain_by_ranges_empty_results(engine):
+    t = engine.gen_chain(SEED, 1000, 5000)
+    rng = offsets_ranges(t)
+    top, gs = engine.chain_topk(engine.chain_args(t, 9, 1155, ranges=rng), 10)
+    assert top == [] and gs["groups"] == 0
+    top, gs = engine.chain_topk(engine.chain_args(t, 1, 0, ranges=rng), 10)
+    assert top == [] and gs["order_pass"] == 0
