@@ -637,10 +637,10 @@ __global__ __launch_bounds__(kT) void cust_map_k(const int32_t* __restrict__ c, 
 }
 
 // Sorted lineitem: the lane holding a run's first row sums the run and
-// stores it.  Each lane loads its own 8 rows and the next 8 (the next lane's
-// rows, an L2 hit), so a run that crosses into the next unit is finished in
-// registers; only a run longer than that reads on, row by row.  Any pair of
-// adjacent rows out of order raises the flag.
+// stores it.  Each lane loads its own 8 rows and takes the next 8 from the
+// next lane (the wave's last lane loads them), so a run that crosses into
+// the next unit is finished in registers; only a run longer than that reads
+// on, row by row.  Any pair of adjacent rows out of order raises the flag.
 BOLAP_DEV int q18_unit(const int32_t* __restrict__ key, const int16_t* __restrict__ qty,
                        int64_t n, int64_t u, int32_t (&k)[8], int32_t (&q)[8]) {
     const int64_t r0 = u * 8;
@@ -666,53 +666,128 @@ BOLAP_DEV int q18_unit(const int32_t* __restrict__ key, const int16_t* __restric
     return m;
 }
 
+// One wave tile of q18_sum_sorted_k: lane `lane` holds unit u's rows.
+struct Q18Tile {
+    int32_t k[8], q[8];
+    int m;
+    int32_t prev;  // the previous row's key (first lane: loaded)
+};
+
+BOLAP_DEV void q18_load_tile(const int32_t* __restrict__ key, const int16_t* __restrict__ qty,
+                             int64_t n, int64_t units, int64_t u, int lane, Q18Tile& t) {
+    if (u < units) {
+        t.m = q18_unit(key, qty, n, u, t.k, t.q);
+    } else {
+        t.m = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t.k[j] = t.q[j] = 0;
+    }
+    t.prev = 0;
+    if (lane == 0 && u < units && u > 0) t.prev = key[u * 8 - 1];
+}
+
+// Sums the runs that start in the lane's rows of tile t and stores them
+// through the wave's LDS rows.
+BOLAP_DEV void q18_tile_runs(const int32_t* __restrict__ key, const int16_t* __restrict__ qty,
+                             int64_t n, int64_t units, int64_t u, int lane, const Q18Tile& t,
+                             int2* s_out, int32_t* sum, int64_t size, bool& bad) {
+    const int64_t r0 = u * 8;
+    const int m = t.m;
+    int32_t k[16], q[16];
+    const int64_t rb = r0 + 8;
+    const int mb = rb >= n ? 0 : (n - rb >= 8 ? 8 : int(n - rb));
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        k[j] = t.k[j];
+        q[j] = t.q[j];
+        k[j + 8] = __shfl_down(t.k[j], 1, kWave);
+        q[j + 8] = __shfl_down(t.q[j], 1, kWave);
+    }
+    const int32_t up = __shfl_up(t.k[7], 1, kWave);
+    if (lane == kWave - 1 && mb > 0) {
+        int32_t kb[8], qb[8];
+        (void)q18_unit(key, qty, n, u + 1, kb, qb);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            k[j + 8] = kb[j];
+            q[j + 8] = qb[j];
+        }
+    }
+    const bool has_prev = r0 > 0 && m > 0;
+    const int32_t prev = lane == 0 ? t.prev : up;
+    if (has_prev && k[0] < prev) bad = true;
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (j < m && k[j] < k[j - 1]) bad = true;
+    // One backward pass over the 16 rows: srun[x] = sum of the run of
+    // equal keys from x to its end inside the window, reach bit x = that
+    // run reaches row 15 of a full window (it may continue past it).
+    const int lim = m == 8 ? 8 + mb : m;
+    uint32_t srun[16];
+    uint32_t reach = 0;
+#pragma unroll
+    for (int x = 15; x >= 0; --x) {
+        const bool v = x < lim;
+        const bool cont = x + 1 < lim && k[x + 1] == k[x];
+        srun[x] = v ? uint32_t(q[x]) + (cont ? srun[x + 1 < 16 ? x + 1 : 15] : 0u) : 0u;
+        const bool r = v && (x == 15 || (cont && ((reach >> (x + 1)) & 1u)));
+        reach |= (r ? 1u : 0u) << x;
+    }
+    // Run starts among the lane's 8 rows: their sums, then the wave's rows
+    // go through LDS so that the stores are issued with full waves.
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const bool start = j < m && (j == 0 ? (!has_prev || k[0] != prev) : k[j] != k[j - 1]);
+        uint32_t sj = srun[j];
+        if (start && ((reach >> j) & 1u)) {  // the run continues past the window
+            int64_t r = r0 + 16;
+            while (r < n && key[r] == k[j]) sj += uint32_t(int32_t(qty[r++]));
+        }
+        const bool keep = start && k[j] >= 0 && k[j] < size;
+        const uint64_t mj = __ballot(keep);
+        if (keep) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                uint32_t(mj >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mj), 0u));
+            s_out[tot + below] = make_int2(k[j], int32_t(sj));
+        }
+        tot += uint32_t(__popcll(mj));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    for (uint32_t i = uint32_t(lane); i < tot; i += kWave) {
+        const int2 e = s_out[i];
+        sum[e.x] = e.y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Two wave tiles per trip (both loaded before either is summed), the trip
+// count wave-uniform: lanes past the end take part with no rows, so the
+// lane shuffles always see the whole wave.
 __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict__ key,
                                                        const int16_t* __restrict__ qty,
                                                        int64_t n, int32_t* sum, int64_t size,
                                                        int64_t* res) {
+    constexpr int kW = kT / kWave;
+    __shared__ int2 s_out[kW][kWave * 8];  // a wave's (key, sum) rows before the stores
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int wave = int(threadIdx.x) / kWave;
     const int64_t units = (n + 7) / 8;
+    const int64_t stride = int64_t(gridDim.x) * kT;
     bool bad = false;
-    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
-         u += int64_t(gridDim.x) * kT) {
-        const int64_t r0 = u * 8;
-        int32_t k[16], q[16];
-        int32_t ka[8], qa[8], kb[8], qb[8];
-        const int m = q18_unit(key, qty, n, u, ka, qa);
-        const int mb = q18_unit(key, qty, n, u + 1, kb, qb);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            k[j] = ka[j];
-            q[j] = qa[j];
-            k[j + 8] = kb[j];
-            q[j + 8] = qb[j];
-        }
-        const int mm = 8 + mb;  // valid rows in k[0..mm) when m == 8
-        const bool has_prev = r0 > 0;
-        const int32_t prev = has_prev ? key[r0 - 1] : 0;
-        if (has_prev && k[0] < prev) bad = true;
-#pragma unroll
-        for (int j = 1; j < 8; ++j)
-            if (j < m && k[j] < k[j - 1]) bad = true;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (j >= m) break;
-            const bool start = j == 0 ? (!has_prev || k[0] != prev) : k[j] != k[j - 1];
-            if (!start) continue;
-            uint32_t s = 0;
-            int t = j;
-            const int lim = m == 8 ? mm : m;
-#pragma unroll
-            for (int x = 0; x < 16; ++x)
-                if (x >= j && x < lim && t == x && k[x] == k[j]) {
-                    s += uint32_t(q[x]);
-                    ++t;
-                }
-            if (t == 16) {  // the run continues past the lookahead unit
-                int64_t r = r0 + 16;
-                while (r < n && key[r] == k[j]) s += uint32_t(int32_t(qty[r++]));
-            }
-            if (k[j] >= 0 && k[j] < size) sum[k[j]] = int32_t(s);
-        }
+    for (int64_t wb = int64_t(blockIdx.x) * kT + (threadIdx.x & ~(kWave - 1)); wb < units;
+         wb += 2 * stride) {
+        const int64_t ua = wb + lane, ub = wb + stride + lane;
+        Q18Tile a, b;
+        q18_load_tile(key, qty, n, units, ua, lane, a);
+        q18_load_tile(key, qty, n, units, ub, lane, b);
+        q18_tile_runs(key, qty, n, units, ua, lane, a, s_out[wave], sum, size, bad);
+        if (wb + stride < units)
+            q18_tile_runs(key, qty, n, units, ub, lane, b, s_out[wave], sum, size, bad);
     }
     if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0)
         atomicOr(reinterpret_cast<unsigned long long*>(res + kQ18Unsorted), 1ull);
@@ -853,10 +928,12 @@ __global__ __launch_bounds__(kT) void q18_keys_k(const bolap_q18_row* __restrict
     }
 }
 
-// One workgroup orders up to kRankMax rows: each row's place is the number of
+// Up to kRankMax rows are ordered by rank: a row's place is the number of
 // rows before it in (price desc, date asc, key asc, order row asc), a strict
-// total order because order rows are unique.
-constexpr int kRankT = 1024;
+// total order because order rows are unique.  Workgroup b ranks rows
+// b * 64 .. b * 64 + 63 (one per lane); its four waves split the rows they
+// are compared with, so the n^2 comparisons spread over n / 64 CUs.
+constexpr int kRankT = 256;
 constexpr int kRankMax = 2048;
 
 BOLAP_DEV bool q18_before(const int4& a, const int4& b) {
@@ -868,16 +945,38 @@ BOLAP_DEV bool q18_before(const int4& a, const int4& b) {
 
 __global__ __launch_bounds__(kRankT) void q18_rank_sort_k(const bolap_q18_row* __restrict__ rows,
                                                           int n, bolap_q18_row* out) {
+    constexpr int kW = kRankT / kWave;
     __shared__ int4 s[kRankMax];
+    __shared__ int s_rank[kW][kWave];
     for (int i = threadIdx.x; i < n; i += kRankT)
         s[i] = make_int4(rows[i].totalprice, rows[i].orderdate, rows[i].orderkey,
                          rows[i].order_row);
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kRankT) {
-        const int4 me = s[i];
-        int rank = 0;
-        for (int j = 0; j < n; ++j) rank += q18_before(s[j], me) ? 1 : 0;
-        out[rank] = rows[i];
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int wave = int(threadIdx.x) / kWave;
+    const int i = int(blockIdx.x) * kWave + lane;
+    const int4 me = s[i < n ? i : 0];
+    const int per = (n + kW - 1) / kW;
+    const int j0 = wave * per;
+    const int j1 = j0 + per < n ? j0 + per : n;
+    int rank = 0;
+    int j = j0;
+    // Eight broadcast LDS reads in flight, then their comparisons.
+    for (; j + 8 <= j1; j += 8) {
+        int4 o[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) o[x] = s[j + x];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) rank += q18_before(o[x], me) ? 1 : 0;
+    }
+    for (; j < j1; ++j) rank += q18_before(s[j], me) ? 1 : 0;
+    s_rank[wave][lane] = rank;
+    __syncthreads();
+    if (wave == 0 && i < n) {
+        int r = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) r += s_rank[w][lane];
+        out[r] = rows[i];
     }
 }
 
@@ -1392,7 +1491,8 @@ int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* a, bolap_q18_row* rows_out,
     bolap_q18_row* sorted = ws.at<bolap_q18_row>(o_sorted);
     if (n <= kRankMax) {
         TraceScope ts(ctx, "q18_sort");
-        hipLaunchKernelGGL(q18_rank_sort_k, dim3(1), dim3(kRankT), 0, ctx->stream, rows_dev,
+        hipLaunchKernelGGL(q18_rank_sort_k, dim3(unsigned((n + kWave - 1) / kWave)),
+                           dim3(kRankT), 0, ctx->stream, rows_dev,
                            int(n), sorted);
         BOLAP_TRY(launch_check("q18_rank_sort_k"));
     } else {

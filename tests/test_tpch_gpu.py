@@ -97,6 +97,19 @@ def test_q18_dense_keys_sorted_lineitem(engine, cust, orders, threshold):
     assert st["sorted_pass"] == 1 and st["fallbacks"] == 0
 
 
+def test_q18_long_runs_cross_the_lookahead(engine):
+    # Eight orders' rows merged under one key: runs of about 32 rows, past
+    # the 16-row window a lane sums in registers.
+    import torch
+    t = engine.gen_q18_tables(SEED, 5000, 100_000)
+    keys, inv = torch.unique_consecutive(t["l_orderkey"], return_inverse=True)
+    first = torch.arange(keys.numel(), device=keys.device) // 8 * 8
+    t["l_orderkey"] = keys[first][inv].contiguous()
+    for thr in (50_000, 100_000, 140_000):
+        got, st = run_q18(engine, t, thr)
+        assert st["sorted_pass"] == 1 and len(got) > 0
+
+
 def test_q18_date_sorted_orders_take_the_max_key_path(engine):
     import torch
     t = engine.gen_q18_tables(SEED, 20_000, 300_000)
