@@ -666,11 +666,14 @@ BOLAP_DEV int q18_unit(const int32_t* __restrict__ key, const int16_t* __restric
     return m;
 }
 
-// One wave tile of q18_sum_sorted_k: lane `lane` holds unit u's rows.
+// One wave tile of q18_sum_sorted_k: lane `lane` holds unit u's rows; the
+// last lane also holds the next unit (the next wave's first), the first
+// lane the previous row's key, all loaded together.
 struct Q18Tile {
     int32_t k[8], q[8];
+    int32_t nk[8], nq[8];
     int m;
-    int32_t prev;  // the previous row's key (first lane: loaded)
+    int32_t prev;
 };
 
 BOLAP_DEV void q18_load_tile(const int32_t* __restrict__ key, const int16_t* __restrict__ qty,
@@ -682,6 +685,9 @@ BOLAP_DEV void q18_load_tile(const int32_t* __restrict__ key, const int16_t* __r
 #pragma unroll
         for (int j = 0; j < 8; ++j) t.k[j] = t.q[j] = 0;
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t.nk[j] = t.nq[j] = 0;
+    if (lane == kWave - 1 && u + 1 < units) (void)q18_unit(key, qty, n, u + 1, t.nk, t.nq);
     t.prev = 0;
     if (lane == 0 && u < units && u > 0) t.prev = key[u * 8 - 1];
 }
@@ -704,13 +710,11 @@ BOLAP_DEV void q18_tile_runs(const int32_t* __restrict__ key, const int16_t* __r
         q[j + 8] = __shfl_down(t.q[j], 1, kWave);
     }
     const int32_t up = __shfl_up(t.k[7], 1, kWave);
-    if (lane == kWave - 1 && mb > 0) {
-        int32_t kb[8], qb[8];
-        (void)q18_unit(key, qty, n, u + 1, kb, qb);
+    if (lane == kWave - 1) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            k[j + 8] = kb[j];
-            q[j + 8] = qb[j];
+            k[j + 8] = t.nk[j];
+            q[j + 8] = t.nq[j];
         }
     }
     const bool has_prev = r0 > 0 && m > 0;
@@ -765,9 +769,8 @@ BOLAP_DEV void q18_tile_runs(const int32_t* __restrict__ key, const int16_t* __r
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// Two wave tiles per trip (both loaded before either is summed), the trip
-// count wave-uniform: lanes past the end take part with no rows, so the
-// lane shuffles always see the whole wave.
+// Wave-uniform trip count: lanes past the end take part with no rows, so
+// the lane shuffles always see the whole wave.
 __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict__ key,
                                                        const int16_t* __restrict__ qty,
                                                        int64_t n, int32_t* sum, int64_t size,
@@ -780,14 +783,11 @@ __global__ __launch_bounds__(kT) void q18_sum_sorted_k(const int32_t* __restrict
     const int64_t stride = int64_t(gridDim.x) * kT;
     bool bad = false;
     for (int64_t wb = int64_t(blockIdx.x) * kT + (threadIdx.x & ~(kWave - 1)); wb < units;
-         wb += 2 * stride) {
-        const int64_t ua = wb + lane, ub = wb + stride + lane;
-        Q18Tile a, b;
-        q18_load_tile(key, qty, n, units, ua, lane, a);
-        q18_load_tile(key, qty, n, units, ub, lane, b);
-        q18_tile_runs(key, qty, n, units, ua, lane, a, s_out[wave], sum, size, bad);
-        if (wb + stride < units)
-            q18_tile_runs(key, qty, n, units, ub, lane, b, s_out[wave], sum, size, bad);
+         wb += stride) {
+        const int64_t u = wb + lane;
+        Q18Tile t;
+        q18_load_tile(key, qty, n, units, u, lane, t);
+        q18_tile_runs(key, qty, n, units, u, lane, t, s_out[wave], sum, size, bad);
     }
     if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0)
         atomicOr(reinterpret_cast<unsigned long long*>(res + kQ18Unsorted), 1ull);
