@@ -499,7 +499,8 @@ MEASURED_TRAFFIC = {
     "q1_scan_agg_kernel": 228.3e6 + 0.1e6,
     "fact_probe_k": 823.9e6 + 131.7e6,
     "q6_flat_k": 540.6e6 + 0.0e6,
-    "q18_sum_sorted_k": 371.6e6 + 71.5e6,
+    "q18_sum_sorted_k": 378.9e6 + 107.7e6,
+    "range_groups_k": 749.9e6 + 23.3e6,
     "q14_flat_k": 327.5e6 + 0.1e6,
     "q17_probe_k<1>": 248.1e6 + 4.3e6,
 }
