@@ -39,6 +39,13 @@ struct bolap_ctx {
     unsigned* counters = nullptr;
     static constexpr int kNumCounters = 64;
 
+    // Join state kept between calls: the key list's segment counters and
+    // result slots, zeroed at creation and returned to zero by the join's
+    // final kernel; join_state_clean says the last join call got that far.
+    void* join_state = nullptr;
+    static constexpr size_t kJoinStateBytes = 8192;
+    bool join_state_clean = true;
+
     // Device staging for small results that are copied back to the host.
     void* result_dev = nullptr;
     static constexpr size_t kResultBytes = size_t(1) << 20;
@@ -67,6 +74,9 @@ struct bolap_ctx {
     // Device-visible address of `pinned` (kernels write small results there
     // directly, with system-scope stores).
     void* pinned_dev = nullptr;
+    // Sequence number of the last result row a kernel was asked to publish
+    // in pinned memory (wait_pinned_seq).
+    int64_t pinned_seq = 0;
 
     // Stage tracing (bolap_ctx_set_trace).
     bool trace = false;
@@ -109,6 +119,11 @@ void set_error(const char* fmt, ...);
 int ws_reserve(bolap_ctx* ctx, size_t bytes, size_t keep = 0);
 // Ensures ctx->pinned holds at least `bytes`.
 int pinned_reserve(bolap_ctx* ctx, size_t bytes);
+// Waits until the pinned int64 at `slot` holds `seq` (written last, with a
+// system-scope release store, by the stream's final kernel): a host spin
+// instead of a stream synchronisation, with a synchronisation after a
+// second as the fallback.  Later work on the stream stays ordered after it.
+int wait_pinned_seq(bolap_ctx* ctx, size_t slot, int64_t seq);
 // Checks the last kernel launch.
 int launch_check(const char* what);
 

@@ -354,7 +354,7 @@ BOLAP_DEV void add_match(int64_t row, const TP* __restrict__ price, const TD* __
 // memory) with system-scope stores.
 template <int NT>
 BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks, KList kl,
-                               int64_t* res, int64_t* host) {
+                               int64_t* res, int64_t* host, int64_t seq) {
     __shared__ uint64_t s_a[NT / kWave], s_c[NT / kWave];
     const int64_t n = klist_size(kl, nullptr);
     uint64_t a = 0, c = 0;
@@ -384,14 +384,19 @@ BOLAP_DEV void join_merge_rows(const int64_t* __restrict__ partials, int nblocks
                             : i == kBuild ? n : res[i];
             __hip_atomic_store(host + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
+        // Published last: the host waits for this value.
+        __hip_atomic_store(host + kResN, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        // The next call starts from zeroed result slots and counters.
+        for (int i = 0; i < kResN; ++i) res[i] = 0;
     }
+    if (threadIdx.x < kSegs) kl.cnt[threadIdx.x * kSegStride] = 0;
 }
 
 // The same merge as a separate kernel after the probe on the same stream.
 __global__ __launch_bounds__(256) void join_merge_k(const int64_t* __restrict__ partials,
                                                     int nblocks, KList kl, int64_t* res,
-                                                    int64_t* host) {
-    join_merge_rows<256>(partials, nblocks, kl, res, host);
+                                                    int64_t* host, int64_t seq) {
+    join_merge_rows<256>(partials, nblocks, kl, res, host, seq);
 }
 
 // The fact-side loop of one workgroup over the Bloom filter B: every key
@@ -624,7 +629,7 @@ template <typename TP, typename TD, int kProbeU, int kPipe, bool kLastBlock, boo
 __global__ __launch_bounds__(kThreads, 8) void probe_sum_k(
     const int32_t* __restrict__ fk, const TP* __restrict__ price, const TD* __restrict__ disc,
     int64_t rows, Geo g, Table hbm, KList kl, int64_t* res, int64_t* partials,
-    unsigned* counter, int64_t* host) {
+    unsigned* counter, int64_t* host, int64_t seq) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t ends[kSegs];
     const KeyTile<kProbeU> first = first_tile<kProbeU>(fk, rows);
@@ -718,7 +723,7 @@ __global__ __launch_bounds__(kThreads, 8) void probe_sum_k(
     }
     if constexpr (kLastBlock) {
         if (!last_block_done(counter)) return;
-        join_merge_rows<kThreads>(partials, int(gridDim.x), kl, res, host);
+        join_merge_rows<kThreads>(partials, int(gridDim.x), kl, res, host, seq);
         if (threadIdx.x == 0) reset_counter(counter);
     }
 }
@@ -988,6 +993,8 @@ struct Plan {
     int64_t* res;
     int64_t* partials;
     int64_t* host;           // device address of the pinned result row
+    int64_t seq;             // published after the row (host slot kResN)
+    bool state_clean;        // counters and result slots are already zero
     int64_t max_blocks;
     int64_t hbm_cap;
     bool may_hbm;
@@ -1018,13 +1025,13 @@ int launch_probe_d(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) 
         hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), p.lds, ctx->stream,
                            a->fact_fk, static_cast<const TP*>(a->fact_price),
                            static_cast<const TD*>(a->fact_disc), a->fact_rows, p.g, p.t, p.kl,
-                           p.res, p.partials, ctx->counters + 1, p.host);
+                           p.res, p.partials, ctx->counters + 1, p.host, p.seq);
     }
     BOLAP_TRY(launch_check("probe_sum_k"));
     if (kLast) return BOLAP_OK;
     TraceScope ts(ctx, "join_merge");
     hipLaunchKernelGGL(join_merge_k, dim3(1), dim3(256), 0, ctx->stream, p.partials, int(blocks),
-                       p.kl, p.res, p.host);
+                       p.kl, p.res, p.host, p.seq);
     return launch_check("join_merge_k");
 }
 
@@ -1078,8 +1085,10 @@ inline unsigned grid_for(bolap_ctx* ctx, int64_t items, int threads, int per_cu)
 // The launches of one run.
 int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
                  const Plan& p) {
-    hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, p.res, p.kl.cnt);
-    BOLAP_TRY(launch_check("zero_res_k"));
+    if (!p.state_clean) {
+        hipLaunchKernelGGL(zero_res_k, dim3(1), dim3(256), 0, ctx->stream, p.res, p.kl.cnt);
+        BOLAP_TRY(launch_check("zero_res_k"));
+    }
     if (a->dim_rows > 0) {
         TraceScope ts(ctx, "join_compact");
         if (p.compact_r == 16)
@@ -1140,22 +1149,32 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     const int64_t seg_cap = blocks_per_seg * per_block + p.compact_r;
 
     WsArena ws;
-    const size_t o_cnt = ws.plan(size_t(kSegs) * kSegStride * 4);
     const size_t o_keys = ws.plan(size_t(seg_cap) * kSegs * 4);
     const size_t o_slots = ws.plan(size_t(p.hbm_cap) * 4);
     const size_t o_bits = ws.plan(size_t(words) * 4);
     const size_t o_part = ws.plan(size_t(p.max_blocks) * 2 * sizeof(int64_t));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
+    BOLAP_TRY(pinned_reserve(ctx, (kResN + 1) * sizeof(int64_t)));
     ws.base = static_cast<char*>(ctx->ws);
-    p.res = static_cast<int64_t*>(ctx->result_dev);
-    p.kl = KList{ws.at<int32_t>(o_keys), ws.at<uint32_t>(o_cnt), seg_cap};
+    // Segment counters (64 bytes apart) and result slots in the context's
+    // join state, which the merge leaves zeroed for the next call.
+    char* js = static_cast<char*>(ctx->join_state);
+    static_assert(size_t(kSegs) * kSegStride * 4 + kResN * 8 <= bolap_ctx::kJoinStateBytes,
+                  "join state too small");
+    p.res = reinterpret_cast<int64_t*>(js + size_t(kSegs) * kSegStride * 4);
+    p.kl = KList{ws.at<int32_t>(o_keys), reinterpret_cast<uint32_t*>(js), seg_cap};
     p.t = Table{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits)};
     p.partials = ws.at<int64_t>(o_part);
     p.host = static_cast<int64_t*>(ctx->pinned_dev);
+    p.seq = ++ctx->pinned_seq;
+    p.state_clean = ctx->join_state_clean;
+    ctx->join_state_clean = false;  // until this call's merge is known to have run
 
     BOLAP_TRY(enqueue_join(ctx, a, ps, p));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // The merge publishes the row and then p.seq; spinning on it returns
+    // sooner than a stream synchronisation.
+    BOLAP_TRY(wait_pinned_seq(ctx, kResN, p.seq));
+    ctx->join_state_clean = true;
     std::memcpy(host_res, ctx->pinned, kResN * sizeof(int64_t));
     return BOLAP_OK;
 }

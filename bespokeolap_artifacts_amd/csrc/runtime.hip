@@ -1,4 +1,5 @@
 // runtime.hip — context, workspace and error plumbing of libbolap.
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -42,10 +43,26 @@ int pinned_reserve(bolap_ctx* ctx, size_t bytes) {
     ctx->pinned = nullptr;
     ctx->pinned_size = 0;
     ctx->pinned_dev = nullptr;
-    BOLAP_HIP_TRY(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+    BOLAP_HIP_TRY(hipHostMalloc(&ctx->pinned, want, hipHostMallocMapped | hipHostMallocCoherent));
     ctx->pinned_size = want;
     BOLAP_HIP_TRY(hipHostGetDevicePointer(&ctx->pinned_dev, ctx->pinned, 0));
     return BOLAP_OK;
+}
+
+int wait_pinned_seq(bolap_ctx* ctx, size_t slot, int64_t seq) {
+    const int64_t* flag = static_cast<const int64_t*>(ctx->pinned) + slot;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 0;; ++spin) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return BOLAP_OK;
+        __builtin_ia32_pause();
+        if ((spin & 1023u) == 1023u &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1))
+            break;
+    }
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return BOLAP_OK;
+    set_error("result row %lld was not published", (long long)seq);
+    return BOLAP_ERR_HIP;
 }
 
 int launch_check(const char* what) {
@@ -130,10 +147,13 @@ int bolap_ctx_create(int device, void* stream, bolap_ctx** out) {
     if (e == hipSuccess)
         e = hipMemset(ctx->counters, 0, sizeof(unsigned) * bolap_ctx::kNumCounters);
     if (e == hipSuccess) e = hipMalloc(&ctx->result_dev, bolap_ctx::kResultBytes);
+    if (e == hipSuccess) e = hipMalloc(&ctx->join_state, bolap_ctx::kJoinStateBytes);
+    if (e == hipSuccess) e = hipMemset(ctx->join_state, 0, bolap_ctx::kJoinStateBytes);
     if (e != hipSuccess) {
         set_error("bolap_ctx_create: %s", hipGetErrorString(e));
         if (ctx->counters) (void)hipFree(ctx->counters);
         if (ctx->result_dev) (void)hipFree(ctx->result_dev);
+        if (ctx->join_state) (void)hipFree(ctx->join_state);
         delete ctx;
         return BOLAP_ERR_HIP;
     }
@@ -152,6 +172,7 @@ int bolap_ctx_destroy(bolap_ctx* ctx) {
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->result_dev) (void)hipFree(ctx->result_dev);
+    if (ctx->join_state) (void)hipFree(ctx->join_state);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (auto& r : ctx->trace_recs) {
         if (r.start) (void)hipEventDestroy(r.start);
