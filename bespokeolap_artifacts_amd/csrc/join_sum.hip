@@ -915,7 +915,7 @@ __global__ __launch_bounds__(kThreads, 8) void q17_probe_k(
 // emitted, part rows, matches).
 __global__ __launch_bounds__(256) void q17_merge_k(const int64_t* __restrict__ p1,
                                                    const int64_t* __restrict__ p2, int nblocks,
-                                                   KList kl, int64_t* host) {
+                                                   KList kl, int64_t* host, int64_t seq) {
     const int64_t n = klist_size(kl, nullptr);
     uint64_t m = 0, sum = 0, rows = 0;
     for (int b = threadIdx.x; b < nblocks; b += 256) {
@@ -933,13 +933,18 @@ __global__ __launch_bounds__(256) void q17_merge_k(const int64_t* __restrict__ p
         sh[2][threadIdx.x / kWave] = m;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
-        int64_t v = n;
-        if (threadIdx.x != 2) {
-            const int f = threadIdx.x == 3 ? 2 : int(threadIdx.x);
-            v = int64_t(sh[f][0] + sh[f][1] + sh[f][2] + sh[f][3]);
+    if (threadIdx.x == 0) {
+        // One thread writes the row, then the sequence number (host slot 4)
+        // with release semantics: the host waits for that value.
+        for (int i = 0; i < 4; ++i) {
+            int64_t v = n;
+            if (i != 2) {
+                const int f = i == 3 ? 2 : i;
+                v = int64_t(sh[f][0] + sh[f][1] + sh[f][2] + sh[f][3]);
+            }
+            __hip_atomic_store(host + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
-        __hip_atomic_store(host + threadIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(host + 4, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1316,7 +1321,7 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     const size_t o_list = ws.plan(size_t(blocks) * size_t(list_cap) * sizeof(uint2));
     const size_t o_mcnt = ws.plan(size_t(blocks) * sizeof(uint32_t));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));
+    BOLAP_TRY(pinned_reserve(ctx, kResN * sizeof(int64_t)));  // row + sequence (slot 4)
     ws.base = static_cast<char*>(ctx->ws);
     int64_t* res = static_cast<int64_t*>(ctx->result_dev);
     const KList kl{ws.at<int32_t>(o_keys), ws.at<uint32_t>(o_cnt), seg_cap};
@@ -1362,10 +1367,11 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
                            pass == 1 ? part1 : part2);
         BOLAP_TRY(launch_check("q17_probe_k"));
     }
+    const int64_t seq = ++ctx->pinned_seq;
     hipLaunchKernelGGL(q17_merge_k, dim3(1), dim3(256), 0, ctx->stream, part1, part2,
-                       int(blocks), kl, static_cast<int64_t*>(ctx->pinned_dev));
+                       int(blocks), kl, static_cast<int64_t*>(ctx->pinned_dev), seq);
     BOLAP_TRY(launch_check("q17_merge_k"));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    BOLAP_TRY(wait_pinned_seq(ctx, 4, seq));
     const int64_t* h = static_cast<const int64_t*>(ctx->pinned);
     out->sum_extendedprice = h[0];
     out->rows_emitted = h[1];

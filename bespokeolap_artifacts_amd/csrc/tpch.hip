@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kT) void q6_zoned_k(Q6K a, const bolap_zone* __rest
 // Sums three u64 fields of the per-workgroup partials into res[0..2] and
 // the pinned host row; `flag` (optional) is copied to host[3].
 __global__ __launch_bounds__(kT) void q6_merge_k(const int64_t* __restrict__ partials, int nb,
-                                                  int64_t* res, int64_t* host,
+                                                  int64_t* res, int64_t* host, int64_t seq,
                                                   const int32_t* flag = nullptr) {
     uint64_t r = 0, c = 0, s = 0;
     for (int b = threadIdx.x; b < nb; b += kT) {
@@ -256,15 +256,20 @@ __global__ __launch_bounds__(kT) void q6_merge_k(const int64_t* __restrict__ par
         sh[2][threadIdx.x / kWave] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-        uint64_t t = 0;
-        for (int w = 0; w < kT / kWave; ++w) t += sh[threadIdx.x][w];
-        res[threadIdx.x] = int64_t(t);
-        __hip_atomic_store(host + threadIdx.x, int64_t(t), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0) {
+        // One thread writes every field, then the sequence number (host slot
+        // 4) with release semantics: the host waits for that value.
+        for (int f = 0; f < 3; ++f) {
+            uint64_t t = 0;
+            for (int w = 0; w < kT / kWave; ++w) t += sh[f][w];
+            res[f] = int64_t(t);
+            __hip_atomic_store(host + f, int64_t(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        if (flag)
+            __hip_atomic_store(host + 3, int64_t(*flag), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(host + 4, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    if (threadIdx.x == 3 && flag)
-        __hip_atomic_store(host + 3, int64_t(*flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // One zone per workgroup: min/max of ship, quantity and discount.
@@ -1160,7 +1165,7 @@ int q14_pass(bolap_ctx* ctx, const bolap_q14_args* a, int64_t size, bolap_q14_re
     const size_t o_part = ws.plan(size_t(grid) * 3 * sizeof(int64_t));
     const size_t o_ovf = ws.plan(sizeof(int32_t));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int64_t)));
+    BOLAP_TRY(pinned_reserve(ctx, 5 * sizeof(int64_t)));
     ws.base = static_cast<char*>(ctx->ws);
     int32_t* packed = ws.at<int32_t>(o_packed);
     int8_t* flag = ws.at<int8_t>(o_flag);
@@ -1191,10 +1196,11 @@ int q14_pass(bolap_ctx* ctx, const bolap_q14_args* a, int64_t size, bolap_q14_re
                                ctx->stream, k, ws.at<int64_t>(o_part));
         BOLAP_TRY(launch_check("q14_scan_k"));
     }
+    const int64_t seq = ++ctx->pinned_seq;
     hipLaunchKernelGGL(q6_merge_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<int64_t>(o_part),
-                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev), ovf);
+                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev), seq, ovf);
     BOLAP_TRY(launch_check("q6_merge_k"));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    BOLAP_TRY(wait_pinned_seq(ctx, 4, seq));
     const int64_t* hp = static_cast<const int64_t*>(ctx->pinned);
     *overflow = hp[3] != 0;
     out->promo_sum = hp[0];
@@ -1274,7 +1280,7 @@ int bolap_q6(bolap_ctx* ctx, const bolap_q6_args* a, bolap_q6_result* out) {
     WsArena ws;
     const size_t o_part = ws.plan(size_t(grid) * 3 * sizeof(int64_t));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int64_t)));
+    BOLAP_TRY(pinned_reserve(ctx, 5 * sizeof(int64_t)));
     ws.base = static_cast<char*>(ctx->ws);
     int64_t* res = static_cast<int64_t*>(ctx->result_dev);
     {
@@ -1287,10 +1293,11 @@ int bolap_q6(bolap_ctx* ctx, const bolap_q6_args* a, bolap_q6_result* out) {
                                ws.at<int64_t>(o_part));
         BOLAP_TRY(launch_check("q6 scan"));
     }
+    const int64_t seq = ++ctx->pinned_seq;
     hipLaunchKernelGGL(q6_merge_k, dim3(1), dim3(kT), 0, ctx->stream, ws.at<int64_t>(o_part),
-                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev));
+                       int(grid), res, static_cast<int64_t*>(ctx->pinned_dev), seq);
     BOLAP_TRY(launch_check("q6_merge_k"));
-    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    BOLAP_TRY(wait_pinned_seq(ctx, 4, seq));
     const int64_t* h = static_cast<const int64_t*>(ctx->pinned);
     out->revenue = h[0];
     out->rows_emitted = h[1];
