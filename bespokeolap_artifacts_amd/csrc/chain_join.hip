@@ -198,14 +198,20 @@ __global__ __launch_bounds__(kT) void order_count_k(const int32_t* __restrict__ 
     block_fold<kT>(st, sCount, int64_t(wc.n), 0);
 }
 
+// Group table.  Hashed: open addressing over `cap` slots.  Direct (dense
+// order keys, span <= 8 x orders): slot = key - kbase, no key array, no
+// probing; only the hit bytes are cleared per call, days and revenue are
+// written for each passing order by the build.
 struct GTable {
-    int32_t* keys;   // cap slots, kEmpty = free
+    int32_t* keys;   // cap slots, kEmpty = free (hashed only)
     int32_t* days;
     uint64_t* rev;   // revenue per slot (two's complement)
     uint8_t* hit;    // 1 when the group got at least one fact row
     uint32_t* obits; // exact bitmap over the order key range
     int32_t cap_log2;
     uint32_t max_probe;  // insert gives up (overflow) after this many slots
+    int32_t direct;
+    int64_t kbase;       // direct: key of slot 0
 };
 
 BOLAP_DEV uint32_t gslot(int32_t key, int cap_log2) {
@@ -216,8 +222,10 @@ __global__ __launch_bounds__(kT) void gtable_init_k(GTable t, int64_t owords) {
     const int64_t cap = int64_t(1) << t.cap_log2;
     const int64_t stride = int64_t(gridDim.x) * kT;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < cap; i += stride) {
-        t.keys[i] = kEmpty;
-        t.rev[i] = 0;
+        if (!t.direct) {
+            t.keys[i] = kEmpty;
+            t.rev[i] = 0;
+        }
         t.hit[i] = 0;
     }
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < owords; i += stride)
@@ -252,6 +260,13 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
             dh = d > dh ? d : dh;
             const int64_t rel = int64_t(key) - r.omin;
             atomicOr(t.obits + (rel >> 5), 1u << (rel & 31));
+            if (t.direct) {
+                const int64_t slot = int64_t(key) - t.kbase;
+                t.days[slot] = d;
+                t.rev[slot] = 0;
+                ++inserted;
+                continue;
+            }
             uint32_t s = gslot(key, t.cap_log2);
             bool placed = false;
             for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
@@ -319,7 +334,7 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
                 t.hit[cur_slot] = 1;
             }
             cur_key = key;
-            cur_slot = find_slot(key, t.keys, t.cap_log2);
+            cur_slot = t.direct ? uint32_t(int64_t(key) - t.kbase) : find_slot(key, t.keys, t.cap_log2);
             acc = v;
         }
         if (cur_slot != 0xFFFFFFFFu) {
@@ -585,7 +600,7 @@ __global__ __launch_bounds__(kCT) void compact_groups_k(GTable t, int64_t chunk,
             const int64_t rv = int64_t(t.rev[i]);
             rl = rv < rl ? rv : rl;
             rh = rv > rh ? rv : rh;
-            out[at++] = RowT{rv, t.days[i], t.keys[i]};
+            out[at++] = RowT{rv, t.days[i], t.direct ? int32_t(t.kbase + i) : t.keys[i]};
         }
         base += tile;
     }
@@ -726,7 +741,11 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
                            r, ws.at<uint32_t>(o_cb), st);
     }
     BOLAP_TRY(launch_check("cust_bits_k"));
-    if (exact) {
+    // Dense order keys: a directly addressed table over the key span, no
+    // sizing count, no probing.
+    const bool direct = ospan > 0 && ospan <= (int64_t(1) << 27) &&
+                        ospan <= 8 * a->order_rows + 1024 && ctx->knob("chain.direct", 1) == 1;
+    if (exact && !direct) {
         if (a->order_rows > 0) {
             TraceScope ts(ctx, "chain_count");
             hipLaunchKernelGGL(order_count_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
@@ -737,17 +756,19 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
         BOLAP_TRY(read_stats(ctx, run.st));
         n_expect = run.st[sCount];
     }
-    const int cap_log2 = ceil_log2(n_expect * 2 < 1024 ? 1024 : n_expect * 2);
+    const int cap_log2 = direct ? ceil_log2(ospan < 1024 ? 1024 : ospan)
+                                : ceil_log2(n_expect * 2 < 1024 ? 1024 : n_expect * 2);
     BOLAP_ARG_CHECK(cap_log2 <= 31, "bolap_chain: %lld groups exceed the table limit",
                     (long long)n_expect);
     const int64_t cap = int64_t(1) << cap_log2;
+    const int64_t rows_cap = direct ? (a->order_rows > 0 ? a->order_rows : 1) : cap;
     const size_t keep = ws.off;
-    const size_t o_keys = ws.plan(size_t(cap) * 4);
+    const size_t o_keys = ws.plan(direct ? 64 : size_t(cap) * 4);
     const size_t o_days = ws.plan(size_t(cap) * 4);
     const size_t o_rev = ws.plan(size_t(cap) * 8);
     const size_t o_hit = ws.plan(size_t(cap));
     const size_t o_ob = ws.plan(size_t(owords) * 4);
-    const size_t o_rows = ws.plan(size_t(cap) * sizeof(RowT));
+    const size_t o_rows = ws.plan(size_t(rows_cap) * sizeof(RowT));
     const size_t o_cnt = ws.plan(64);
     BOLAP_TRY(ws_reserve(ctx, ws.off, keep));
     ws.base = static_cast<char*>(ctx->ws);
@@ -756,7 +777,8 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
     // ones give up after 128 slots and fall back to exact sizing.
     const uint32_t max_probe = exact ? uint32_t(cap) : 128u;
     GTable t{ws.at<int32_t>(o_keys), ws.at<int32_t>(o_days), ws.at<uint64_t>(o_rev),
-             ws.at<uint8_t>(o_hit), ws.at<uint32_t>(o_ob), cap_log2, max_probe};
+             ws.at<uint8_t>(o_hit), ws.at<uint32_t>(o_ob), cap_log2, max_probe,
+             direct ? 1 : 0, r.omin};
     {
         TraceScope ts(ctx, "chain_init");
         hipLaunchKernelGGL(gtable_init_k, dim3(grid_for(ctx, cap, 8)), dim3(kT), 0, ctx->stream,
@@ -807,7 +829,7 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
     }
     BOLAP_TRY(launch_check("compact_groups_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
-    if (run.st[sGroups] > cap / 2) run.st[sFlags] |= 1;  // load factor above 1/2
+    if (!direct && run.st[sGroups] > cap / 2) run.st[sFlags] |= 1;  // load factor above 1/2
     BOLAP_TRY(pinned_reserve(ctx, 8));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));

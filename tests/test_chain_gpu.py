@@ -55,13 +55,39 @@ def test_chain_full_c3_size(engine):
 
 @pytest.mark.parametrize("hint", [1, 1000, 100_000, 10_000_000])
 def test_chain_capacity_hints_never_change_results(engine, hint):
+    # The hashed group table (forced; dense keys would take the directly
+    # addressed one, which ignores hints).
     t = engine.gen_chain(SEED, 20_000, 200_000)
     want, st = want_sorted(t, 3, 1800)
-    top, gs = engine.chain_topk(engine.chain_args(t, 3, 1800, capacity_hint=hint), 25)
+    engine.tune_set("chain.direct", 2)
+    try:
+        top, gs = engine.chain_topk(engine.chain_args(t, 3, 1800, capacity_hint=hint), 25)
+    finally:
+        engine.tune_set("chain.direct", 0)
     assert np.array_equal(as_array(top), want[:25])
     assert gs["fallbacks"] == (1 if hint * 2 < st["order_pass"] else gs["fallbacks"])
     if hint >= st["order_pass"]:
         assert gs["fallbacks"] == 0
+    top, gs = engine.chain_topk(engine.chain_args(t, 3, 1800, capacity_hint=hint), 25)
+    assert np.array_equal(as_array(top), want[:25]) and gs["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("mode", [1, 2])  # directly addressed, hashed
+def test_chain_group_table_modes_agree(engine, mode):
+    import torch
+    t = engine.gen_chain(SEED, 5000, 60_000)
+    perm = torch.randperm(t["fact_order"].numel(), generator=torch.Generator().manual_seed(9))
+    perm = perm.cuda()
+    for k in ("fact_order", "fact_price", "fact_disc"):
+        t[k] = t[k][perm].contiguous()
+    want, st = want_sorted(t, 4, 1300)
+    engine.tune_set("chain.direct", mode)
+    try:
+        allg, gs = engine.chain_groups_sorted(engine.chain_args(t, 4, 1300), 100_000)
+    finally:
+        engine.tune_set("chain.direct", 0)
+    assert np.array_equal(allg, want)
+    check_stats(gs, st)
 
 
 def test_chain_shuffled_fact_narrow_types_and_disc_price(engine):
