@@ -509,7 +509,9 @@ MEASURED_TRAFFIC = {
 def roofline(head):
     """Roofline of the workload's dominant kernel (algorithmic bytes per
     launch / measured kernel time)."""
-    gbps = head.get("probe_kernel_GBps") or head.get("kernel_GBps") or head["GBps"]
+    gbps = head.get("probe_kernel_GBps") or head.get("kernel_GBps") or head.get("GBps")
+    if gbps is None:  # a run of workloads without a bandwidth figure (C4, C5)
+        return None
     return {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": gbps / HBM_PEAK_GBPS, "traffic": MEASURED_TRAFFIC.get(head.get("kernel")),
             "kernel": head.get("kernel")}

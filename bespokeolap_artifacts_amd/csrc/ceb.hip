@@ -16,10 +16,11 @@
 //      count arrays hold only qualifying movies.
 //   3. layout: per query and table, the offset of its count array in one
 //      arena (an exclusive scan over the batch).
-//   4. fact: one persistent kernel per fact table reads the table once and,
-//      for every query that uses the table, tests each row (title bitmap,
-//      row predicates, dimension bitmaps) and counts it.  Rows are sorted by
-//      movie, so a wave-wide segmented sum leaves one atomic per movie run.
+//   4. fact: one persistent kernel per fact table reads the table once and
+//      tests each row against the queries that use the table and that its
+//      movie passes (a movie-major copy of the title bits lists them), then
+//      the row predicates and dimension bitmaps, and counts it.  Rows are
+//      sorted by movie, so one popcount per movie run leaves one atomic.
 //   5. final: per query, the 128-bit sum of count products and the minimum
 //      year over movies with a match in every table; partials, then a merge.
 // A batch is captured once into a hipGraph and replayed; the fact stage's
@@ -80,66 +81,123 @@ BOLAP_DEV bool bit_of(const uint32_t* bits, int64_t n, int64_t v) {
 }
 
 // 1. Title bitmaps: wave g covers movies [64 g, 64 g + 64) = words 2g, 2g+1.
+// Two layouts of the same bits: query-major `tf` (one bitmap over movies per
+// query, for ranks and the final stage) and movie-major `qm` (word w of movie
+// m = which of queries 32 w .. 32 w + 31 the movie passes, stored at
+// qm[w * n_title + m]), which lets the fact stage visit only the queries a
+// row's movie passes.
+// The predicates of up to kStage queries are staged in LDS per workgroup, so
+// the per-query loop reads broadcast LDS words instead of scalar loads.
+constexpr int kStage = 512;  // a multiple of 32: a staged chunk holds whole query words
+
+struct TitleQ {
+    int32_t lo, hi;
+    uint32_t kinds, pad;
+    const uint32_t* bits;
+    int64_t n;
+};
+
 __global__ __launch_bounds__(kT) void ceb_title_k(DbK db, const QK* __restrict__ qs, int32_t nq,
-                                                  uint32_t* __restrict__ tf) {
+                                                  uint32_t* __restrict__ tf,
+                                                  uint32_t* __restrict__ qm) {
+    __shared__ TitleQ s_q[kStage];
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = int64_t(gridDim.x) * (kT / kWave);
-    for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave;
-         g * kWave < db.n_title; g += wstride) {
-        const int64_t m = g * kWave + lane;
-        const bool in = m < db.n_title;
-        const int32_t y = in ? db.year[m] : 0;
-        const int32_t k = in ? db.kind[m] : 0;
-        const int32_t c = in ? db.code[m] : 0;
-        for (int q = 0; q < nq; ++q) {
-            const QK& Q = qs[q];
-            const bool p = in && y >= Q.year_lo && y <= Q.year_hi && uint32_t(k) < 32u &&
-                           ((Q.kind_mask >> k) & 1u) && bit_of(Q.title_bits, Q.title_bits_n, c);
-            const uint64_t b = __ballot(p);
-            if (lane == 0) {
-                uint32_t* row = tf + int64_t(q) * db.words;
-                row[2 * g] = uint32_t(b);
-                if (2 * g + 1 < db.words) row[2 * g + 1] = uint32_t(b >> 32);
+    for (int c0 = 0; c0 < nq; c0 += kStage) {
+        const int cn = nq - c0 < kStage ? nq - c0 : kStage;
+        __syncthreads();  // the previous chunk's readers are done
+        for (int i = threadIdx.x; i < cn; i += kT) {
+            const QK& Q = qs[c0 + i];
+            s_q[i] = TitleQ{Q.year_lo, Q.year_hi, Q.kind_mask, 0u, Q.title_bits, Q.title_bits_n};
+        }
+        __syncthreads();
+        for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave;
+             g * kWave < db.n_title; g += wstride) {
+            const int64_t m = g * kWave + lane;
+            const bool in = m < db.n_title;
+            const int32_t y = in ? db.year[m] : 0;
+            const int32_t k = in ? db.kind[m] : 0;
+            const int32_t c = in ? db.code[m] : 0;
+            uint32_t acc = 0;
+            for (int i = 0; i < cn; ++i) {
+                const int q = c0 + i;
+                const TitleQ Q = s_q[i];
+                const bool p = in && y >= Q.lo && y <= Q.hi && uint32_t(k) < 32u &&
+                               ((Q.kinds >> k) & 1u) && bit_of(Q.bits, Q.n, c);
+                acc |= uint32_t(p) << (q & 31);
+                if ((q & 31) == 31 || q == nq - 1) {
+                    if (in) qm[int64_t(q >> 5) * db.n_title + m] = acc;
+                    acc = 0;
+                }
+                const uint64_t bl = __ballot(p);
+                if (lane == 0) {
+                    uint32_t* row = tf + int64_t(q) * db.words;
+                    row[2 * g] = uint32_t(bl);
+                    if (2 * g + 1 < db.words) row[2 * g + 1] = uint32_t(bl >> 32);
+                }
             }
         }
     }
 }
 
-// 2. Rank prefix per query (one workgroup per query, contiguous chunks per
-// thread, a workgroup scan of the chunk totals).
+// 2. Rank prefix per query: one workgroup per query walks its bitmap in
+// tiles of 4 words per thread (16-byte coalesced loads and stores), a
+// workgroup scan of the per-thread popcounts per tile, and a running carry.
 __global__ __launch_bounds__(kRankT) void ceb_rank_k(const uint32_t* __restrict__ tf, int64_t words,
                                                      uint32_t* __restrict__ rank,
                                                      QState* __restrict__ st) {
-    __shared__ uint32_t s_w[kRankT / kWave];
+    constexpr int kW = kRankT / kWave;
+    __shared__ uint32_t s_w[kW];
     const int q = blockIdx.x;
     const uint32_t* row = tf + int64_t(q) * words;
     uint32_t* rr = rank + int64_t(q) * words;
-    const int64_t per = (words + kRankT - 1) / kRankT;
-    const int64_t b0 = int64_t(threadIdx.x) * per;
-    const int64_t b1 = b0 + per < words ? b0 + per : words;
-    uint32_t mine = 0;
-    for (int64_t i = b0; i < b1; ++i) mine += __popc(row[i]);
-    // workgroup exclusive scan of `mine`
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    uint32_t incl = mine;
+    // rows start at q * words: 16-byte accesses only when that stays aligned
+    const bool vec = (words & 3) == 0;
+    uint32_t carry = 0;
+    for (int64_t t0 = 0; t0 < words; t0 += 4 * kRankT) {
+        const int64_t i = t0 + 4 * int64_t(threadIdx.x);
+        uint32_t v[4] = {0u, 0u, 0u, 0u};
+        if (vec && i + 3 < words) {
+            const uint4 u = *reinterpret_cast<const uint4*>(row + i);
+            v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+        } else {
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += y;
+            for (int k = 0; k < 4; ++k) v[k] = i + k < words ? row[i + k] : 0u;
+        }
+        const uint32_t mine = __popc(v[0]) + __popc(v[1]) + __popc(v[2]) + __popc(v[3]);
+        uint32_t incl = mine;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        if (lane == kWave - 1) s_w[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, tile = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) {
+            before += w < wave ? s_w[w] : 0u;
+            tile += s_w[w];
+        }
+        __syncthreads();  // s_w is rewritten by the next tile
+        uint32_t run = carry + before + incl - mine;
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[k] = run;
+            run += __popc(v[k]);
+        }
+        if (vec && i + 3 < words) {
+            *reinterpret_cast<uint4*>(rr + i) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i + k < words) rr[i + k] = o[k];
+        }
+        carry += tile;
     }
-    if (lane == kWave - 1) s_w[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int w = 0; w < kRankT / kWave; ++w) {
-        before += w < wave ? s_w[w] : 0u;
-        total += s_w[w];
-    }
-    uint32_t run = before + incl - mine;
-    for (int64_t i = b0; i < b1; ++i) {
-        rr[i] = run;
-        run += __popc(row[i]);
-    }
-    if (threadIdx.x == 0) st[q].n_tf = total;
+    if (threadIdx.x == 0) st[q].n_tf = carry;
 }
 
 // 3. Count-array offsets: exclusive scan over the batch of n_tf x (tables).
@@ -195,86 +253,206 @@ __global__ __launch_bounds__(kT) void ceb_zero_k(uint32_t* __restrict__ arena,
         arena[i] = 0u;
 }
 
+// A query's row predicate on fact table TB and its count-array offset, as the
+// fact stage stages them in LDS: cast_info keeps the role mask, movie_info the
+// info_type and info bitmaps, movie_keyword the keyword bitmap.
 template <int TB>
-BOLAP_DEV bool row_pred(const QK& Q, int32_t x, int32_t z) {
-    if (TB == 0) return uint32_t(x) < 32u && ((Q.role_mask >> x) & 1u);
-    if (TB == 1) return bit_of(Q.type_bits, Q.type_bits_n, x) && bit_of(Q.info_bits, Q.info_bits_n, z);
-    return bit_of(Q.kw_bits, Q.kw_bits_n, x);
-}
+struct FactQ;
+template <>
+struct FactQ<0> {
+    uint64_t off;
+    uint32_t role, pad;
+    BOLAP_DEV void set(const QK& Q, uint64_t o) { off = o; role = Q.role_mask; pad = 0; }
+    BOLAP_DEV bool pass(int32_t x, int32_t) const { return uint32_t(x) < 32u && ((role >> x) & 1u); }
+};
+template <>
+struct FactQ<1> {
+    uint64_t off;
+    const uint32_t *b1, *b2;
+    int64_t n1, n2;
+    BOLAP_DEV void set(const QK& Q, uint64_t o) {
+        off = o;
+        b1 = Q.type_bits;
+        n1 = Q.type_bits_n;
+        b2 = Q.info_bits;
+        n2 = Q.info_bits_n;
+    }
+    BOLAP_DEV bool pass(int32_t x, int32_t z) const { return bit_of(b1, n1, x) && bit_of(b2, n2, z); }
+};
+template <>
+struct FactQ<2> {
+    uint64_t off;
+    const uint32_t* b1;
+    int64_t n1;
+    BOLAP_DEV void set(const QK& Q, uint64_t o) {
+        off = o;
+        b1 = Q.kw_bits;
+        n1 = Q.kw_bits_n;
+    }
+    BOLAP_DEV bool pass(int32_t x, int32_t) const { return bit_of(b1, n1, x); }
+};
 
-// 4. One fact table for every query in qlist.  Each wave takes 256
-// consecutive rows as four chunks of 64 (one row per lane per chunk, so each
-// load is one coalesced 256-byte access); within a chunk the movie ids are
-// non-decreasing across lanes, so a segmented inclusive sum keyed on the
-// movie leaves the run's total in its last lane, which adds it to the
-// query's count array at the movie's rank.
+// 4. One fact table for the queries that use it (bit q of use[q >> 5]).
+// Each wave owns a contiguous range of rows and walks it in groups of 256
+// rows, as four chunks of 64 (one row per lane per chunk, so each load is one
+// coalesced 256-byte access).  For each 32-query word the wave loads its
+// rows' movie-major title words and visits only the queries some row of the
+// group passes, instead of testing every row against every query.
+// Within a chunk the movie ids are non-decreasing across lanes; the last lane
+// of each run of one movie holds the run's lanes as a mask, so a run's matches
+// are one popcount of the query's ballot.  A run that reaches the chunk's last
+// lane and continues in the next chunk of the wave's range hands its count on
+// (a scalar within a group, a per-wave LDS slot per query between groups), so
+// a movie with many rows costs one atomic per wave range and query rather
+// than one per chunk.  The lane that ends a run adds its count to the query's
+// count array at the movie's rank.
 template <int TB>
 __global__ __launch_bounds__(kT) void ceb_fact_k(DbK db, const QK* __restrict__ qs,
-                                                 const int32_t* __restrict__ qlist, int32_t nql,
-                                                 int32_t qchunk, const uint32_t* __restrict__ tf,
+                                                 const uint32_t* __restrict__ use, int32_t qwords,
+                                                 int32_t wchunk, const uint32_t* __restrict__ qm,
+                                                 const uint32_t* __restrict__ tf,
                                                  const uint32_t* __restrict__ rank,
                                                  const QState* __restrict__ st,
                                                  uint32_t* __restrict__ arena) {
     constexpr int U = 4;
+    constexpr int64_t kGroup = int64_t(U) * kWave;
+    __shared__ FactQ<TB> s_q[kStage];
+    __shared__ uint32_t s_carry[kT / kWave][kStage];
     const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
     const int64_t rows = db.rows[TB];
     const int32_t* __restrict__ movie = db.movie[TB];
     const int32_t* __restrict__ a1 = db.a1[TB];
     const int32_t* __restrict__ a2 = db.a2;
-    const int64_t chunks = (rows + kWave * U - 1) / (kWave * U);
-    const int64_t wstride = int64_t(gridDim.x) * (kT / kWave);
-    for (int c0 = 0; c0 < nql; c0 += qchunk) {
-        const int c1 = c0 + qchunk < nql ? c0 + qchunk : nql;
-        for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave; g < chunks;
-             g += wstride) {
+    const int64_t groups = (rows + kGroup - 1) / kGroup;
+    const int64_t nwaves = int64_t(gridDim.x) * (kT / kWave);
+    const int64_t per = (groups + nwaves - 1) / nwaves;
+    const int64_t g0 = (int64_t(blockIdx.x) * (kT / kWave) + wv) * per;
+    const int64_t g1 = g0 + per < groups ? g0 + per : groups;
+    // lanes 0 .. lane
+    const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
+    if (wchunk > kStage / 32) wchunk = kStage / 32;
+    for (int c0 = 0; c0 < qwords; c0 += wchunk) {
+        const int c1 = c0 + wchunk < qwords ? c0 + wchunk : qwords;
+        const int q0 = c0 * 32;
+        __syncthreads();  // the previous chunk's readers are done
+        for (int i = threadIdx.x; i < (c1 - c0) * 32; i += kT) {
+            const int q = q0 + i;
+            if ((use[q >> 5] >> (q & 31)) & 1u) s_q[i].set(qs[q], st[q].off[TB]);
+        }
+        for (int i = lane; i < kStage; i += kWave) s_carry[wv][i] = 0u;
+        __syncthreads();
+        bool cin0 = false;  // the first run of the group continues the previous group's last
+        for (int64_t g = g0; g < g1; ++g) {
             int32_t m[U], x[U], z[U];
-            bool in[U];
+            uint64_t run[U];  // the run's lanes, held by its last lane (else 0)
 #pragma unroll
             for (int j = 0; j < U; ++j) {
                 const int64_t r = (g * U + j) * kWave + lane;
-                in[j] = r < rows;
-                m[j] = in[j] ? movie[r] - 1 : -1;
-                x[j] = in[j] ? a1[r] : 0;
-                z[j] = (TB == 1 && in[j]) ? a2[r] : 0;
-                if (m[j] < 0 || m[j] >= db.n_title) in[j] = false;
+                const bool in = r < rows;
+                m[j] = in ? movie[r] - 1 : -1;
+                x[j] = in ? a1[r] : 0;
+                z[j] = (TB == 1 && in) ? a2[r] : 0;
+                if (m[j] < 0 || m[j] >= db.n_title) m[j] = -1;
             }
-            for (int qi = c0; qi < c1; ++qi) {
-                const int q = qlist[qi];
-                const QK& Q = qs[q];
-                const uint32_t* row = tf + int64_t(q) * db.words;
+            // first movie of the next group in this wave's range (-1: none)
+            int32_t mnext = -1;
+            if (g + 1 < g1) {
+                mnext = movie[(g + 1) * kGroup] - 1;
+                if (mnext < 0 || mnext >= db.n_title) mnext = -1;
+            }
+            bool cin[U], cout[U], single[U];
+            uint64_t lastrun[U];
+            int firsttail[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int32_t prev = __shfl_up(m[j], 1, kWave);
+                const int32_t next = __shfl_down(m[j], 1, kWave);
+                const uint64_t heads = __ballot(lane == 0 || prev != m[j]);
+                const bool tail = m[j] >= 0 && (lane == kWave - 1 || next != m[j]);
+                const int h = 63 - __clzll(heads & upto);  // lane 0 is always a head
+                run[j] = tail ? (upto & ~((1ull << h) - 1ull)) : 0ull;
+                const uint64_t tails = __ballot(tail);
+                firsttail[j] = tails ? __builtin_ctzll(tails) : kWave;
+                lastrun[j] = readlane_u64(run[j], kWave - 1);
+                // every lane holds the same (valid) movie
+                single[j] = heads == 1ull && tails == (1ull << (kWave - 1));
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int32_t mlast = __builtin_amdgcn_readlane(m[j], kWave - 1);
+                const int32_t mfollow =
+                    j + 1 < U ? __builtin_amdgcn_readlane(m[j < U - 1 ? j + 1 : j], 0) : mnext;
+                cout[j] = mlast >= 0 && mfollow == mlast;
+                cin[j] = j == 0 ? cin0 : cout[j - 1];
+            }
+            for (int w = c0; w < c1; ++w) {
+                const uint32_t uw = use[w];
+                if (uw == 0) continue;
+                uint32_t M[U], any = 0;
 #pragma unroll
                 for (int j = 0; j < U; ++j) {
-                    uint32_t word = 0;
-                    bool ok = false;
-                    if (in[j]) {
-                        word = row[m[j] >> 5];
-                        ok = ((word >> (m[j] & 31)) & 1u) && row_pred<TB>(Q, x[j], z[j]);
-                    }
-                    if (__ballot(ok) == 0) continue;  // wave-uniform
-                    uint32_t v = ok ? 1u : 0u;
-                    const int32_t key = in[j] ? m[j] : -1;
+                    M[j] = m[j] >= 0 ? qm[int64_t(w) * db.n_title + m[j]] : 0u;
+                    any |= M[j];
+                }
+                uint32_t Ms[U];  // the chunk's movie's word when the chunk is one movie
 #pragma unroll
-                    for (int off = 1; off < kWave; off <<= 1) {
-                        const uint32_t vo = __shfl_up(v, off, kWave);
-                        const int32_t ko = __shfl_up(key, off, kWave);
-                        if (lane >= off && ko == key) v += vo;
+                for (int j = 0; j < U; ++j) Ms[j] = __builtin_amdgcn_readfirstlane(M[j]);
+                uint32_t todo = wave_or(any) & uw;
+                while (todo) {
+                    const int b = __builtin_ctz(todo);
+                    todo &= todo - 1u;
+                    const int q = w * 32 + b;
+                    const FactQ<TB> F = s_q[q - q0];
+                    uint32_t carry = cin[0] ? s_carry[wv][q - q0] : 0u;
+#pragma unroll
+                    for (int j = 0; j < U; ++j) {
+                        const uint32_t cj = cin[j] ? carry : 0u;
+                        if (single[j]) {  // wave-uniform: one movie, one run
+                            uint32_t c = cj;
+                            if ((Ms[j] >> b) & 1u) c += uint32_t(__popcll(__ballot(F.pass(x[j], z[j]))));
+                            carry = cout[j] ? c : 0u;
+                            if (!cout[j] && c && lane == 0) {
+                                const int64_t wi = int64_t(q) * db.words + (m[j] >> 5);
+                                const uint32_t r =
+                                    rank[wi] + __popc(tf[wi] & ((1u << (m[j] & 31)) - 1u));
+                                atomicAdd(arena + F.off + r, c);
+                            }
+                            continue;
+                        }
+                        const bool ok = ((M[j] >> b) & 1u) && F.pass(x[j], z[j]);
+                        const uint64_t bal = __ballot(ok);
+                        uint32_t v = uint32_t(__popcll(bal & run[j]));
+                        if (lane == firsttail[j]) v += cj;
+                        if (cout[j]) {
+                            carry = uint32_t(__popcll(bal & lastrun[j])) +
+                                    (firsttail[j] == kWave - 1 ? cj : 0u);
+                            if (lane == kWave - 1) v = 0;
+                        } else {
+                            carry = 0u;
+                        }
+                        if (v) {
+                            const int64_t wi = int64_t(q) * db.words + (m[j] >> 5);
+                            const uint32_t r =
+                                rank[wi] + __popc(tf[wi] & ((1u << (m[j] & 31)) - 1u));
+                            atomicAdd(arena + F.off + r, v);
+                        }
                     }
-                    const int32_t next = __shfl_down(key, 1, kWave);
-                    const bool tail = lane == kWave - 1 || next != key;
-                    if (tail && v > 0 && key >= 0) {
-                        const int64_t w = key >> 5;
-                        const uint32_t r = rank[int64_t(q) * db.words + w] +
-                                           __popc(word & ((1u << (key & 31)) - 1u));
-                        atomicAdd(arena + st[q].off[TB] + r, v);
-                    }
+                    if (lane == 0) s_carry[wv][q - q0] = cout[U - 1] ? carry : 0u;
                 }
             }
+            cin0 = cout[U - 1];
         }
     }
 }
 
-// 5. Per (query, slice): sum of count products and min year over movies
-// with a match in every table of the query.
+// 5. Per (query, slice of bitmap words): sum of count products and min year
+// over movies with a match in every table of the query.  A wave takes 64
+// words (2048 movies) at a time; a wave scan of their popcounts numbers the
+// passing movies, whose ranks are consecutive from the tile's first rank, so
+// lane k of a round takes the tile's k-th passing movie and the count loads
+// of a round are coalesced.  The lane finds its movie (needed for the year)
+// by a binary search of the scan across lanes and a select within the word.
 __global__ __launch_bounds__(kT) void ceb_final_k(DbK db, const QK* __restrict__ qs,
                                                   const uint32_t* __restrict__ tf,
                                                   const uint32_t* __restrict__ rank,
@@ -285,7 +463,8 @@ __global__ __launch_bounds__(kT) void ceb_final_k(DbK db, const QK* __restrict__
     __shared__ int64_t s_hi[kT / kWave], s_min[kT / kWave], s_j[kT / kWave];
     const int q = blockIdx.y;
     const int b = blockIdx.x;
-    const QK& Q = qs[q];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int tables = qs[q].tables;
     const QState S = st[q];
     const int64_t per = (db.words + slices - 1) / slices;
     const int64_t w0 = int64_t(b) * per;
@@ -294,30 +473,61 @@ __global__ __launch_bounds__(kT) void ceb_final_k(DbK db, const QK* __restrict__
     const uint32_t* rk = rank + int64_t(q) * db.words;
     Acc128 acc{0, 0};
     int64_t mn = INT64_MAX, joined = 0;
-    for (int64_t w = w0 + threadIdx.x; w < w1; w += kT) {
-        uint32_t bits = row[w];
-        uint32_t r = rk[w];
-        while (bits) {
-            const int bpos = __ffs(bits) - 1;
-            bits &= bits - 1;
+    for (int64_t t = w0 + int64_t(wave) * kWave; t < w1; t += int64_t(kT)) {
+        const uint32_t word = t + lane < w1 ? row[t + lane] : 0u;
+        const uint32_t cnt = __popc(word);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t excl = incl - cnt;
+        const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), kWave - 1));
+        if (total == 0) continue;
+        const uint32_t rbase = rk[t];
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+            const uint32_t k = k0 + uint32_t(lane);
+            const bool valid = k < total;
+            const uint32_t r = rbase + k;
             uint64_t lo = 1, hi = 0;
-            bool all = true;
-            for (int t = 0; t < kTables; ++t) {
-                if (!((Q.tables >> t) & 1)) continue;
-                const uint32_t c = arena[S.off[t] + r];
+            bool all = valid;
+            for (int tb = 0; tb < kTables; ++tb) {
+                if (!((tables >> tb) & 1)) continue;
+                const uint32_t c = valid ? arena[S.off[tb] + r] : 0u;
                 all = all && c != 0;
                 // (hi, lo) *= c, exact while the product stays below 2^128
                 const uint64_t nlo = lo * uint64_t(c);
                 hi = hi * uint64_t(c) + __umul64hi(lo, uint64_t(c));
                 lo = nlo;
             }
-            ++r;
+            // largest lane s with excl[s] <= k: the word holding passing movie k
+            int src = 0;
+#pragma unroll
+            for (int step = kWave / 2; step > 0; step >>= 1) {
+                const uint32_t e = uint32_t(__shfl(int(excl), src + step, kWave));
+                if (e <= k) src += step;
+            }
+            const uint32_t wd = uint32_t(__shfl(int(word), src, kWave));
+            uint32_t jj = k - uint32_t(__shfl(int(excl), src, kWave));
             if (!all) continue;
+            // position of the jj-th set bit of wd
+            uint32_t v = wd;
+            int pos = 0;
+#pragma unroll
+            for (int sh = 16; sh > 0; sh >>= 1) {
+                const uint32_t low = __popc(v & ((1u << sh) - 1u));
+                if (jj >= low) {
+                    jj -= low;
+                    v >>= sh;
+                    pos += sh;
+                }
+            }
             ++joined;
             const uint64_t nlo = acc.lo + lo;
             acc.hi += int64_t(hi) + (nlo < acc.lo ? 1 : 0);
             acc.lo = nlo;
-            const int64_t y = db.year[w * 32 + bpos];
+            const int64_t y = db.year[(t + src) * 32 + pos];
             mn = y < mn ? y : mn;
         }
     }
@@ -329,7 +539,6 @@ __global__ __launch_bounds__(kT) void ceb_final_k(DbK db, const QK* __restrict__
         wm = o < wm ? o : wm;
     }
     const int64_t wj = int64_t(wave_sum(uint64_t(joined)));
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     if (lane == 0) {
         s_lo[wave] = ws.lo;
         s_hi[wave] = ws.hi;
@@ -374,9 +583,9 @@ __global__ __launch_bounds__(kWave) void ceb_merge_k(const Part* __restrict__ pa
 
 // ------------------------------------------------------------------ host
 struct Layout {
-    int32_t nq = 0, slices = 0;
+    int32_t nq = 0, slices = 0, qwords = 0;
     int64_t words = 0;
-    size_t o_qs, o_ql, o_st, o_tf, o_rank, o_total, o_parts, o_res, o_arena, bytes;
+    size_t o_qs, o_use, o_st, o_tf, o_qm, o_rank, o_total, o_parts, o_res, o_arena, bytes;
     int64_t arena_words = 0;
 };
 
@@ -449,16 +658,18 @@ Layout plan_layout(bolap_ctx* ctx, const DbK& db, const bolap_ceb_query* qs, int
     Layout L;
     L.nq = nq;
     L.words = db.words;
-    L.slices = int32_t(std::max<int64_t>(1, std::min<int64_t>(64, (db.words + 4095) / 4096)));
+    L.qwords = (nq + 31) / 32;
+    L.slices = int32_t(std::max<int64_t>(1, std::min<int64_t>(64, (db.words + 2047) / 2048)));
     (void)ctx;
     int64_t arena = 0;
     for (int i = 0; i < nq; ++i) arena += int64_t(__builtin_popcount(qs[i].tables & 7)) * db.n_title;
     L.arena_words = arena;
     WsArena a;
     L.o_qs = a.plan(size_t(nq) * sizeof(QK));
-    L.o_ql = a.plan(size_t(kTables) * nq * 4);
+    L.o_use = a.plan(size_t(kTables) * L.qwords * 4);
     L.o_st = a.plan(size_t(nq) * sizeof(QState));
     L.o_tf = a.plan(size_t(nq) * db.words * 4);
+    L.o_qm = a.plan(size_t(L.qwords) * size_t(db.n_title) * 4);
     L.o_rank = a.plan(size_t(nq) * db.words * 4);
     L.o_total = a.plan(64);
     L.o_parts = a.plan(size_t(nq) * L.slices * sizeof(Part));
@@ -500,18 +711,20 @@ unsigned grid_for(bolap_ctx* ctx, int64_t n, int per_cu) {
 int enqueue(bolap_ctx* ctx, const DbK& db, const Layout& L, const int32_t* n_per_table) {
     char* base = static_cast<char*>(ctx->ceb_buf);
     auto* qs = reinterpret_cast<const QK*>(base + L.o_qs);
-    auto* ql = reinterpret_cast<const int32_t*>(base + L.o_ql);
+    auto* use = reinterpret_cast<const uint32_t*>(base + L.o_use);
     auto* st = reinterpret_cast<QState*>(base + L.o_st);
     auto* tf = reinterpret_cast<uint32_t*>(base + L.o_tf);
+    auto* qm = reinterpret_cast<uint32_t*>(base + L.o_qm);
     auto* rank = reinterpret_cast<uint32_t*>(base + L.o_rank);
     auto* total = reinterpret_cast<unsigned long long*>(base + L.o_total);
     auto* parts = reinterpret_cast<Part*>(base + L.o_parts);
     auto* res = reinterpret_cast<bolap_ceb_result*>(base + L.o_res);
     auto* arena = reinterpret_cast<uint32_t*>(base + L.o_arena);
     const int per_cu = int(ctx->knob("ceb.blocks_per_cu", 4));
-    const int qchunk = int(std::max<int64_t>(1, ctx->knob("ceb.qchunk", L.nq)));
+    // queries per pass over a fact table, in whole 32-query words
+    const int wchunk = int(std::max<int64_t>(1, (ctx->knob("ceb.qchunk", L.nq) + 31) / 32));
     hipLaunchKernelGGL(ceb_title_k, dim3(grid_for(ctx, (db.n_title + 63) / 64 * 64, 8)),
-                       dim3(kT), 0, ctx->stream, db, qs, L.nq, tf);
+                       dim3(kT), 0, ctx->stream, db, qs, L.nq, tf, qm);
     hipLaunchKernelGGL(ceb_rank_k, dim3(unsigned(L.nq)), dim3(kRankT), 0, ctx->stream, tf,
                        L.words, rank, st);
     hipLaunchKernelGGL(ceb_layout_k, dim3(1), dim3(kRankT), 0, ctx->stream, qs, L.nq, st, total);
@@ -522,16 +735,16 @@ int enqueue(bolap_ctx* ctx, const DbK& db, const Layout& L, const int32_t* n_per
         const int32_t nql = n_per_table[t];
         if (nql == 0 || db.rows[t] == 0) continue;
         const unsigned grid = grid_for(ctx, (db.rows[t] + 3) / 4, per_cu);
-        const int32_t* list = ql + t * L.nq;
+        const uint32_t* tuse = use + size_t(t) * L.qwords;
         if (t == 0)
-            hipLaunchKernelGGL(ceb_fact_k<0>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
-                               nql, qchunk, tf, rank, st, arena);
+            hipLaunchKernelGGL(ceb_fact_k<0>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, tuse,
+                               L.qwords, wchunk, qm, tf, rank, st, arena);
         else if (t == 1)
-            hipLaunchKernelGGL(ceb_fact_k<1>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
-                               nql, qchunk, tf, rank, st, arena);
+            hipLaunchKernelGGL(ceb_fact_k<1>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, tuse,
+                               L.qwords, wchunk, qm, tf, rank, st, arena);
         else
-            hipLaunchKernelGGL(ceb_fact_k<2>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, list,
-                               nql, qchunk, tf, rank, st, arena);
+            hipLaunchKernelGGL(ceb_fact_k<2>, dim3(grid), dim3(kT), 0, ctx->stream, db, qs, tuse,
+                               L.qwords, wchunk, qm, tf, rank, st, arena);
         BOLAP_TRY(launch_check("ceb_fact_k"));
     }
     hipLaunchKernelGGL(ceb_final_k, dim3(unsigned(L.slices), unsigned(L.nq)), dim3(kT), 0,
@@ -541,15 +754,19 @@ int enqueue(bolap_ctx* ctx, const DbK& db, const Layout& L, const int32_t* n_per
     return launch_check("ceb final");
 }
 
-// Uploads the queries and their per-table lists; returns the list lengths.
+// Uploads the queries and, per table, a bitmap over the batch of the queries
+// that use it; returns how many queries use each table.
 int upload(bolap_ctx* ctx, const Layout& L, const bolap_ceb_query* qs, int32_t* n_per_table) {
     std::vector<QK> hq(static_cast<size_t>(L.nq));
-    std::vector<int32_t> lists(static_cast<size_t>(kTables) * L.nq, 0);
+    std::vector<uint32_t> lists(static_cast<size_t>(kTables) * L.qwords, 0u);
     for (int t = 0; t < kTables; ++t) n_per_table[t] = 0;
     for (int i = 0; i < L.nq; ++i) {
         hq[size_t(i)] = make_q(qs[i]);
         for (int t = 0; t < kTables; ++t)
-            if ((qs[i].tables >> t) & 1) lists[size_t(t) * L.nq + n_per_table[t]++] = i;
+            if ((qs[i].tables >> t) & 1) {
+                lists[size_t(t) * L.qwords + size_t(i >> 5)] |= 1u << (i & 31);
+                ++n_per_table[t];
+            }
     }
     char* base = static_cast<char*>(ctx->ceb_buf);
     const size_t qb = hq.size() * sizeof(QK), lb = lists.size() * 4;
@@ -558,7 +775,7 @@ int upload(bolap_ctx* ctx, const Layout& L, const bolap_ceb_query* qs, int32_t* 
     std::memcpy(static_cast<char*>(ctx->pinned) + qb, lists.data(), lb);
     BOLAP_HIP_TRY(hipMemcpyAsync(base + L.o_qs, ctx->pinned, qb, hipMemcpyHostToDevice,
                                  ctx->stream));
-    BOLAP_HIP_TRY(hipMemcpyAsync(base + L.o_ql, static_cast<char*>(ctx->pinned) + qb, lb,
+    BOLAP_HIP_TRY(hipMemcpyAsync(base + L.o_use, static_cast<char*>(ctx->pinned) + qb, lb,
                                  hipMemcpyHostToDevice, ctx->stream));
     return BOLAP_OK;
 }
@@ -658,7 +875,7 @@ int bolap_ceb_autotune(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_q
     BOLAP_ARG_CHECK(ctx && db && qs && n >= 1, "bolap_ceb_autotune: bad arguments");
     std::vector<bolap_ceb_result> out(static_cast<size_t>(n));
     const int64_t per_cu[] = {2, 4, 8, 16};
-    std::vector<int64_t> chunks = {16, 64, int64_t(n)};
+    std::vector<int64_t> chunks = {64, 256, int64_t(n)};
     double best = 1e30;
     int64_t bp = 4, bc = n;
     hipEvent_t e0, e1;

@@ -224,6 +224,17 @@ BOLAP_DEV uint64_t wave_sum_u64(uint64_t x) {
     return readlane_u64(x, 15) + readlane_u64(x, 31) + readlane_u64(x, 47) +
            readlane_u64(x, 63);
 }
+// Bitwise OR over the 64 lanes; the result is wave-uniform.
+BOLAP_DEV uint32_t wave_or(uint32_t x) {
+    x |= dpp_u32<0x111>(x);
+    x |= dpp_u32<0x112>(x);
+    x |= dpp_u32<0x114>(x);
+    x |= dpp_u32<0x118>(x);
+    return uint32_t(__builtin_amdgcn_readlane(int(x), 15)) |
+           uint32_t(__builtin_amdgcn_readlane(int(x), 31)) |
+           uint32_t(__builtin_amdgcn_readlane(int(x), 47)) |
+           uint32_t(__builtin_amdgcn_readlane(int(x), 63));
+}
 BOLAP_DEV int64_t wave_sum(int64_t v) { return int64_t(wave_sum_u64(uint64_t(v))); }
 BOLAP_DEV uint64_t wave_sum(uint64_t v) { return wave_sum_u64(v); }
 

@@ -98,3 +98,28 @@ def test_full_size_batch_matches_single_and_oracle(engine):
     t = O.gen_ceb(42, 1.0)
     for i in (0, 1, 2, 3, 5, 8, 13, 21):
         assert batch[i] == O.ceb_run(t, qs[i], oracle_masks(s, qs[i])), qs[i]
+
+
+@pytest.mark.parametrize("qchunk", [32, 64, 0])
+def test_batch_across_query_words_and_chunks(engine, small, qchunk):
+    """A batch whose 32-query words use disjoint tables (a whole word skips a
+    table), a ragged last word, and fact passes over 1, 2 or all words at a
+    time: every result equals the one-query run."""
+    s, t = small
+    qs = CB.random_queries(s, 75, seed=11)
+    for i, q in enumerate(qs):  # word 0: movie_keyword only; word 1: no movie_keyword
+        if i < 32:
+            q.update(tables=N.CEB_MK, keyword=q.get("keyword") or "%a%")
+        elif i < 64:
+            q["tables"] = q["tables"] & ~N.CEB_MK or N.CEB_CI
+            q.pop("keyword", None)
+    bound = [CB.bind(s, q) for q in qs]
+    single = [CB.run_one(engine, s, b) for b in bound]
+    old = engine.tune_get("ceb.qchunk")
+    engine.tune_set("ceb.qchunk", qchunk or len(qs))
+    try:
+        assert CB.run_batch(engine, s, bound) == single
+    finally:
+        engine.tune_set("ceb.qchunk", old)
+    for i in (0, 31, 32, 63, 64, 74):
+        assert single[i] == O.ceb_run(t, qs[i], oracle_masks(s, qs[i])), qs[i]
