@@ -328,6 +328,28 @@ class Engine:
             return stats, keys[:n.value], rev[:n.value]
         return stats, None, None
 
+    def group_minmax(self, pair_group, pair_item, item_offset, item_count, item_has,
+                     item_value, range_value, groups: int, keys: int, dense_max: int = 0,
+                     max_rows: int | None = None):
+        """Runs bolap_group_minmax over device int32 columns (item_has: device
+        uint8 or None); returns (stats dict, rows as a device int32 tensor of
+        shape (n, 5): group, key, has, min, max in (group, key) order)."""
+        torch = _torch()
+        a = N.MinmaxArgs(int(pair_group.numel()), ptr(pair_group), ptr(pair_item),
+                         int(item_offset.numel()), ptr(item_offset), ptr(item_count),
+                         ptr(item_has), ptr(item_value), int(range_value.numel()),
+                         ptr(range_value), int(groups), int(keys), int(dense_max))
+        st = N.MinmaxStats()
+        N.check(self.lib.bolap_group_minmax(self.ctx, C.byref(a), None, 0, None, C.byref(st)))
+        cap = st.groups if max_rows is None else max_rows
+        out = torch.empty((max(1, cap), 5), dtype=torch.int32, device=f"cuda:{self.device}")
+        n = C.c_int64(0)
+        N.check(self.lib.bolap_group_minmax(self.ctx, C.byref(a), ptr(out), cap, C.byref(n),
+                                            C.byref(st)))
+        stats = {"expanded_rows": st.expanded_rows, "groups": st.groups,
+                 "placement": st.placement, "slots": st.slots}
+        return stats, out[:n.value]
+
     def hll(self, col):
         """HyperLogLog registers (4096, uint8 numpy) of a device int32 column."""
         import numpy as np

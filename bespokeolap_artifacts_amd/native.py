@@ -98,6 +98,27 @@ class CebResult(C.Structure):
     _fields_ = [("count_lo", C.c_uint64), ("count_hi", C.c_uint64), ("min_year", C.c_int64),
                 ("title_pass", C.c_int64), ("joined_movies", C.c_int64)]
 
+MINMAX_DENSE, MINMAX_HASH = 0, 1
+
+
+class MinmaxArgs(C.Structure):
+    _fields_ = [("pairs", C.c_int64), ("pair_group", vp), ("pair_item", vp),
+                ("items", C.c_int64), ("item_offset", vp), ("item_count", vp),
+                ("item_has", vp), ("item_value", vp), ("values", C.c_int64),
+                ("range_value", vp), ("groups", C.c_int32), ("keys", C.c_int32),
+                ("dense_max_slots", C.c_int64)]
+
+
+class MinmaxRow(C.Structure):
+    _fields_ = [("group", C.c_int32), ("key", C.c_int32), ("has", C.c_int32),
+                ("min", C.c_int32), ("max", C.c_int32)]
+
+
+class MinmaxStats(C.Structure):
+    _fields_ = [("expanded_rows", C.c_int64), ("groups", C.c_int64),
+                ("placement", C.c_int32), ("pad", C.c_int32), ("slots", C.c_int64)]
+
+
 class Q1Args(C.Structure):
     _fields_ = [("layout", C.c_int32), ("rows", C.c_int64), ("ship", vp),
                 ("ship_lo", C.c_int64), ("ship_hi", C.c_int64),
@@ -260,6 +281,8 @@ PROTOTYPES = {
     "bolap_gen_price_disc": (C.c_int, [vp, u64, C.c_uint32, i64, vp, vp]),
     "bolap_ceb_run": (C.c_int, [vp, P(CebDb), P(CebQuery), P(CebResult)]),
     "bolap_ceb_run_batch": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(CebResult)]),
+    "bolap_group_minmax": (C.c_int, [vp, P(MinmaxArgs), vp, i64, P(C.c_int64),
+                                     P(MinmaxStats)]),
     "bolap_ceb_autotune": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(C.c_double)]),
     "bolap_ilike_mask": (C.c_int, [C.c_char_p, P(C.c_int64), i64, C.c_char_p, P(C.c_uint8)]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),

@@ -457,6 +457,59 @@ int bolap_ilike_mask(const char* chars, const int64_t* offsets, int64_t n,
                      const char* pattern, uint8_t* mask);
 
 /* ------------------------------------------------------------------------ */
+/* Grouped MIN/MAX of a nullable per-item value over join rows that expand  */
+/* through per-item value ranges (CSR).  Restates the result aggregation of  */
+/* bespoke_ceb/query_q10a.cpp:838-1107: each (person, movie) join row whose  */
+/* person has a name group (pair_group >= 0; -1 drops the row, :1078-1080)    */
+/* expands into its movie's info values (item_offset / item_count into        */
+/* range_value, :1081-1090); every (group, info) pair is a group holding      */
+/* {has, min, max} of the movie years, created as {0, 0, 0} by a movie        */
+/* without a year (:880-900).  Placement as the reference: a dense matrix     */
+/* when groups x keys <= dense_max_slots (0: 5,000,000, :843-856), else a     */
+/* hashed table.  Rows come out in (group, key) order (the reference's order  */
+/* is first-seen or hash order, :1094-1120).  All pointers are device         */
+/* pointers; a group, item, range or key outside its bounds is an error.     */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_MINMAX_DENSE 0
+#define BOLAP_MINMAX_HASH 1
+
+typedef struct {
+    int64_t pairs;               /* join rows */
+    const int32_t* pair_group;   /* group of the row's person, [0, groups) or -1 */
+    const int32_t* pair_item;    /* the row's movie: index into the item arrays */
+    int64_t items;               /* length of the item arrays */
+    const int32_t* item_offset;  /* first range value of the item */
+    const int32_t* item_count;   /* number of range values of the item */
+    const uint8_t* item_has;     /* 1 when the item's value is present (null: all present) */
+    const int32_t* item_value;   /* the aggregated value (production year) */
+    int64_t values;              /* length of range_value */
+    const int32_t* range_value;  /* second group key (dense info id), [0, keys) */
+    int32_t groups;              /* first key domain (name groups) */
+    int32_t keys;                /* second key domain (info ids) */
+    int64_t dense_max_slots;     /* 0: 5,000,000 */
+} bolap_minmax_args;
+
+typedef struct {
+    int32_t group, key;          /* the group */
+    int32_t has;                 /* 1 when some row of the group had a value */
+    int32_t min, max;            /* 0, 0 when has == 0 */
+} bolap_minmax_row;
+
+typedef struct {
+    int64_t expanded_rows;       /* join rows x range lengths (rows aggregated) */
+    int64_t groups;
+    int32_t placement;           /* BOLAP_MINMAX_DENSE / BOLAP_MINMAX_HASH */
+    int32_t pad;
+    int64_t slots;               /* dense matrix slots (0 when hashed) */
+} bolap_minmax_stats;
+
+/* Rows into rows_out (device array of max_rows; null for the statistics
+   only).  Synchronous: rows_out is complete when the call returns. */
+int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* args,
+                       bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
+                       bolap_minmax_stats* stats);
+
+/* ------------------------------------------------------------------------ */
 /* Table statistics for the estimator (computed once per column, at load).   */
 /* ------------------------------------------------------------------------ */
 #define BOLAP_HLL_REGS 4096        /* HyperLogLog, p = 12 */

@@ -595,3 +595,32 @@ def q17(part_key, preds: list, li: dict) -> dict:
                            C.byref(out))
     return {"sum_extendedprice": out.sum_extendedprice, "parts": out.parts,
             "matches": out.matches, "rows_emitted": out.rows_emitted}
+
+
+# ------------------------------------------------ grouped MIN/MAX (ceb.c)
+MINMAX_DTYPE = np.dtype([("group", "<i4"), ("key", "<i4"), ("has", "<i4"), ("min", "<i4"),
+                         ("max", "<i4")])
+
+
+def group_minmax(pair_group, pair_item, item_offset, item_count, item_has, item_value,
+                 range_value, groups: int, keys: int, dense_max: int = 0) -> np.ndarray:
+    """Grouped MIN/MAX rows in (group, key) order (item_has may be None)."""
+    L = lib()
+    if not getattr(L, "_mm_ready", False):
+        L.oracle_group_minmax.restype = i64
+        L.oracle_group_minmax.argtypes = [i64, vp, vp, i64, vp, vp, vp, vp, i64, vp, i32, i32,
+                                          i64, vp, i64]
+        L._mm_ready = True
+    c32 = lambda a: np.ascontiguousarray(a, np.int32)
+    pg, pi, off, cnt = c32(pair_group), c32(pair_item), c32(item_offset), c32(item_count)
+    val, rv = c32(item_value), c32(range_value)
+    hv = None if item_has is None else np.ascontiguousarray(item_has, np.uint8)
+    cap = max(1, min(int(cnt[pi[pg >= 0]].astype(np.int64).sum()) if len(pg) else 0,
+                     groups * keys))
+    out = np.zeros(cap, MINMAX_DTYPE)
+    n = L.oracle_group_minmax(len(pg), _p(pg), _p(pi), len(off), _p(off), _p(cnt),
+                              None if hv is None else _p(hv), _p(val), len(rv), _p(rv),
+                              groups, keys, dense_max, _p(out), cap)
+    if n < 0:
+        raise ValueError("oracle_group_minmax: out of bounds" if n == -1 else "short output")
+    return out[:n]

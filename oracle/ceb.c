@@ -81,3 +81,118 @@ void oracle_ceb_run(const oracle_ceb_db* db, const oracle_ceb_query* q, uint64_t
     for (int t = 0; t < 3; ++t) free(cnt[t]);
     free(tmask);
 }
+
+/* ---- grouped MIN/MAX (bespoke_ceb/query_q10a.cpp:838-1107) -------------- */
+
+typedef struct {
+    uint64_t key;
+    int32_t has, val;
+} mm_row;
+
+static int mm_cmp(const void* a, const void* b) {
+    const uint64_t x = ((const mm_row*)a)->key, y = ((const mm_row*)b)->key;
+    return x < y ? -1 : x > y;
+}
+
+/* One aggregate slot, updated as query_q10a.cpp:880-900: created {has, v, v}
+   or {0, 0, 0} by its first row; a later row with a value sets or widens it. */
+static void mm_fold(int32_t* used, int32_t* has, int32_t* mn, int32_t* mx, int32_t h, int32_t v) {
+    if (oracle_mutation() == 14 && !h) { h = 1; v = 0; }  /* planted: null year taken as 0 */
+    if (!*used) {
+        *used = 1;
+        *has = h;
+        *mn = h ? v : 0;
+        *mx = h ? v : 0;
+    } else if (h) {
+        if (*has) {
+            if (v < *mn) *mn = v;
+            if (v > *mx) *mx = v;
+        } else {
+            *has = 1;
+            *mn = v;
+            *mx = v;
+        }
+    }
+}
+
+int64_t oracle_group_minmax(int64_t pairs, const int32_t* pg, const int32_t* pi, int64_t items,
+                            const int32_t* off, const int32_t* cnt, const uint8_t* hasv,
+                            const int32_t* val, int64_t values, const int32_t* rv,
+                            int32_t groups, int32_t keys, int64_t dense_max, int32_t* out,
+                            int64_t max_rows) {
+    if (dense_max <= 0) dense_max = 5000000;  /* query_q10a.cpp:843 */
+    const int64_t slots = (int64_t)groups * keys;
+    /* bounds (the reference indexes without checks) */
+    int64_t expanded = 0;
+    for (int64_t p = 0; p < pairs; ++p) {
+        const int32_t g = pg[p];
+        if (g < 0) continue;                 /* no name group: dropped (:1078-1080) */
+        const int32_t it = pi[p];
+        if (g >= groups || it < 0 || it >= items) return -1;
+        if (off[it] < 0 || cnt[it] < 0 || (int64_t)off[it] + cnt[it] > values) return -1;
+        for (int64_t i = off[it]; i < (int64_t)off[it] + cnt[it]; ++i)
+            if (rv[i] < 0 || rv[i] >= keys) return -1;
+        expanded += cnt[it];
+    }
+    int64_t n = 0;
+    if (slots <= dense_max) {                /* dense matrix (:847-853) */
+        int32_t* used = (int32_t*)calloc((size_t)slots, 4);
+        int32_t* has = (int32_t*)calloc((size_t)slots, 4);
+        int32_t* mn = (int32_t*)calloc((size_t)slots, 4);
+        int32_t* mx = (int32_t*)calloc((size_t)slots, 4);
+        for (int64_t p = 0; p < pairs; ++p) {
+            const int32_t g = pg[p];
+            if (g < 0) continue;
+            const int32_t it = pi[p];
+            const int32_t h = hasv ? (hasv[it] != 0) : 1;
+            for (int64_t i = off[it]; i < (int64_t)off[it] + cnt[it]; ++i) {
+                const int64_t s = (int64_t)g * keys + rv[i];
+                mm_fold(used + s, has + s, mn + s, mx + s, h, val[it]);
+            }
+        }
+        for (int64_t s = 0; s < slots; ++s) {
+            if (!used[s]) continue;
+            if (n >= max_rows) { n = -2; break; }
+            int32_t* o = out + 5 * n++;
+            o[0] = (int32_t)(s / keys);
+            o[1] = (int32_t)(s % keys);
+            o[2] = has[s];
+            o[3] = mn[s];
+            o[4] = mx[s];
+        }
+        free(used); free(has); free(mn); free(mx);
+        return n;
+    }
+    /* hashed in the reference (:905-921); here the expanded rows sorted by key */
+    mm_row* rows = (mm_row*)malloc((size_t)(expanded > 0 ? expanded : 1) * sizeof(mm_row));
+    int64_t m = 0;
+    for (int64_t p = 0; p < pairs; ++p) {
+        const int32_t g = pg[p];
+        if (g < 0) continue;
+        const int32_t it = pi[p];
+        const int32_t h = hasv ? (hasv[it] != 0) : 1;
+        for (int64_t i = off[it]; i < (int64_t)off[it] + cnt[it]; ++i) {
+            rows[m].key = ((uint64_t)(uint32_t)g << 32) | (uint32_t)rv[i];
+            rows[m].has = h;
+            rows[m].val = val[it];
+            ++m;
+        }
+    }
+    qsort(rows, (size_t)m, sizeof(mm_row), mm_cmp);
+    for (int64_t i = 0; i < m;) {
+        int32_t used = 0, has = 0, mn = 0, mx = 0;
+        int64_t j = i;
+        for (; j < m && rows[j].key == rows[i].key; ++j)
+            mm_fold(&used, &has, &mn, &mx, rows[j].has, rows[j].val);
+        if (n >= max_rows) { n = -2; break; }
+        int32_t* o = out + 5 * n++;
+        o[0] = (int32_t)(rows[i].key >> 32);
+        o[1] = (int32_t)(uint32_t)rows[i].key;
+        o[2] = has;
+        o[3] = mn;
+        o[4] = mx;
+        i = j;
+    }
+    free(rows);
+    return n;
+}

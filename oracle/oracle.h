@@ -160,6 +160,17 @@ typedef struct {
 /* out: count_lo, count_hi, min_year, title_pass, joined_movies */
 void oracle_ceb_run(const oracle_ceb_db* db, const oracle_ceb_query* q, uint64_t* out);
 
+/* Grouped MIN/MAX over join rows expanded through per-item ranges
+   (bespoke_ceb/query_q10a.cpp:838-1107).  out: 5 int32 per row (group, key,
+   has, min, max) in (group, key) order.  Returns the row count, -1 when a
+   group, item, range or key is out of bounds, -2 when max_rows is short. */
+int64_t oracle_group_minmax(int64_t pairs, const int32_t* pair_group, const int32_t* pair_item,
+                            int64_t items, const int32_t* item_offset,
+                            const int32_t* item_count, const uint8_t* item_has,
+                            const int32_t* item_value, int64_t values,
+                            const int32_t* range_value, int32_t groups, int32_t keys,
+                            int64_t dense_max, int32_t* out, int64_t max_rows);
+
 /* --- TPC-H Q6 / Q18 and build-time derivations (tpch.c) ----------------- */
 int64_t oracle_days_from_civil(int32_t y, int32_t m, int32_t d);
 void oracle_civil_from_days(int32_t z, int32_t* y, int32_t* m, int32_t* d);
