@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workloads", default="c2,c1,c3,q6,q14,q17,q18,c4,c5,copy")
+    ap.add_argument("--workloads", default="c2,c1,c3,q6,q14,q17,q18,c4,c5,minmax,sort,copy")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -337,6 +337,98 @@ def bench_q18(eng, torch, args):
     return res
 
 
+# ------------------------------------------- CEB q10a grouped MIN/MAX (a13)
+def bench_minmax(eng, torch, args):
+    """Grouped MIN/MAX(year) over 4,000,000 (person, movie) join rows expanded
+    through 2,500,000 movies' info ranges (0-8 values each), nullable years
+    (20% null), 10% of join rows without a name group; 200,000 name groups x
+    2,000 infos (400 M slots: hashed) and 2,000 x 2,000 (4 M slots: dense)."""
+    import ctypes as C
+    from bespokeolap_artifacts_amd import native as N
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(SEED)
+    pairs, items, keys = 4_000_000, 2_500_000, 2000
+    cnt = torch.randint(0, 9, (items,), device=dev, dtype=torch.int32, generator=g)
+    off = (torch.cumsum(cnt.to(torch.int64), 0) - cnt).to(torch.int32)
+    values = int(cnt.sum())
+    rv = torch.randint(0, keys, (values,), device=dev, dtype=torch.int32, generator=g)
+    has = (torch.rand(items, device=dev, generator=g) >= 0.2).to(torch.uint8)
+    year = torch.randint(1880, 2026, (items,), device=dev, dtype=torch.int32, generator=g)
+    item = torch.randint(0, items, (pairs,), device=dev, dtype=torch.int32, generator=g)
+    drop = torch.rand(pairs, device=dev, generator=g) < 0.1
+    out = {}
+    for label, groups in (("hashed", 200_000), ("dense", 2000)):
+        grp = torch.randint(0, groups, (pairs,), device=dev, dtype=torch.int32, generator=g)
+        grp[drop] = -1
+        st, rows = eng.group_minmax(grp, item, off, cnt, has, year, rv, groups, keys)
+        cap = int(st["groups"])
+        buf = torch.empty((cap, 5), dtype=torch.int32, device=dev)
+        a = N.MinmaxArgs(pairs, grp.data_ptr(), item.data_ptr(), items, off.data_ptr(),
+                         cnt.data_ptr(), has.data_ptr(), year.data_ptr(), values,
+                         rv.data_ptr(), groups, keys, 0)
+        n = C.c_int64(0)
+
+        def step():
+            N.check(eng.lib.bolap_group_minmax(eng.ctx, C.byref(a), buf.data_ptr(), cap,
+                                               C.byref(n), None))
+        ms = time_steps(torch, step, args.steps, args.warmup)
+        exp = int(st["expanded_rows"])
+        # join rows (group, item) + per join row its item's offset, count, has,
+        # year + the expanded range values
+        alg = pairs * 8 + pairs * 13 + exp * 4
+        r = {"ms_per_step": ms, "expanded_rows": exp, "groups": cap,
+             "placement": "dense" if st["placement"] == N.MINMAX_DENSE else "hashed",
+             "expanded_rows_per_s": exp / (ms * 1e-3), "algorithmic_bytes": alg,
+             "GBps_alg": alg / (ms * 1e-3) / 1e9}
+        if not args.no_cpu:
+            import oracle
+            h = [x.cpu().numpy() for x in (grp, item, off, cnt, has, year, rv)]
+            want = oracle.group_minmax(*h, groups, keys)
+            gotr = rows.cpu().numpy()
+            r["parity"] = bool(len(want) == len(gotr) and all(
+                np.array_equal(want[f], gotr[:, i])
+                for i, f in enumerate(("group", "key", "has", "min", "max"))))
+            med, reps = cpu_time(lambda: oracle.group_minmax(*h, groups, keys),
+                                 args.cpu_seconds)
+            r["cpu_baseline"] = {"value": exp / med, "unit": "expanded rows/s", "cores": 1,
+                                 "kind": "port", "ms": med * 1e3,
+                                 "sample": f"full input, median of {reps} runs"}
+        out[label] = r
+    return {"workload": "CEB q10a grouped MIN/MAX(year), 4,000,000 join rows, 2,500,000 "
+                        "movies' info ranges", "ms_per_step": out["hashed"]["ms_per_step"],
+            "variants": out}
+
+
+# ------------------------------------------- CEB build sort order (a10)
+def bench_sort(eng, torch, args):
+    """cast_info's build order: 36,000,000 rows sorted by (movie_id, role_id),
+    then both columns gathered into that order."""
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(SEED)
+    n = 36_000_000
+    movie = torch.randint(1, 2_500_001, (n,), device=dev, dtype=torch.int32, generator=g)
+    role = torch.randint(1, 13, (n,), device=dev, dtype=torch.int32, generator=g)
+
+    def step():
+        order = eng.sort_order(movie, role)
+        eng.gather_rows(movie, order)
+        eng.gather_rows(role, order)
+    ms = time_steps(torch, step, args.steps, args.warmup)
+    res = {"workload": "CEB build: cast_info 36,000,000 rows sorted by (movie_id, role_id) "
+                       "+ gather of both columns", "ms_per_step": ms,
+           "rows_per_s": n / (ms * 1e-3)}
+    if not args.no_cpu:
+        import oracle
+        m = 2_000_000
+        hm, hr = movie[:m].cpu().numpy(), role[:m].cpu().numpy()
+        med, reps = cpu_time(lambda: oracle.sort_order_i32(hm, hr), args.cpu_seconds)
+        res["cpu_baseline"] = {"value": m / med, "unit": "rows/s", "cores": 1, "kind": "port",
+                               "ms": med * 1e3,
+                               "sample": f"first 2,000,000 rows (order only), median of "
+                                         f"{reps} runs"}
+    return res
+
+
 # ------------------------------------------------------------ C4 estimator
 def bench_c4(eng, torch, args):
     """Learned-sized vs exact-sized star plans: 20,000 training plans over 16
@@ -536,6 +628,7 @@ def main():
                       "q14": lambda e, t, a: bench_q14_q17(e, t, a, "q14"),
                       "q17": lambda e, t, a: bench_q14_q17(e, t, a, "q17"),
                       "q18": bench_q18, "c4": bench_c4, "c5": bench_c5,
+                      "minmax": bench_minmax, "sort": bench_sort,
                       "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))
     # The headline is the first workload with a bandwidth figure (C2 by default).

@@ -271,3 +271,44 @@ def autotune(engine, schema: CebSchema, qs: list) -> float:
     N.check(engine.lib.bolap_ceb_autotune(engine.ctx, C.byref(schema.db), arr, len(qs),
                                           C.byref(best)))
     return best.value
+
+
+class StringPool:
+    """Host string pool of libbolap (bolap_string_pool_*): strings -> int32 ids in
+    first-appearance order, -1 for nulls, as the reference builder's StringPool
+    (bespoke_ceb/builder_impl.hpp:14-89) assigns them to every string column."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.h = C.c_void_p()
+        N.check(lib.bolap_string_pool_create(C.byref(self.h)))
+
+    def intern(self, strings, valid=None) -> np.ndarray:
+        enc = [s.encode() if isinstance(s, str) else bytes(s) for s in strings]
+        chars = np.frombuffer(b"".join(enc) or b"\0", np.uint8)
+        offs = np.zeros(len(enc) + 1, np.int64)
+        offs[1:] = np.cumsum([len(e) for e in enc])
+        ids = np.zeros(max(1, len(enc)), np.int32)
+        v = None if valid is None else np.ascontiguousarray(valid, np.uint8)
+        N.check(self.lib.bolap_string_pool_intern(self.h, chars.ctypes.data, offs.ctypes.data,
+                                                  len(enc),
+                                                  None if v is None else v.ctypes.data,
+                                                  ids.ctypes.data))
+        return ids[:len(enc)]
+
+    def export(self, lower: bool = False) -> tuple[bytes, np.ndarray]:
+        n, c = C.c_int64(), C.c_int64()
+        N.check(self.lib.bolap_string_pool_size(self.h, C.byref(n), C.byref(c)))
+        chars = np.zeros(max(1, c.value), np.uint8)
+        offs = np.zeros(n.value + 1, np.int64)
+        N.check(self.lib.bolap_string_pool_export(self.h, int(lower), chars.ctypes.data,
+                                                  offs.ctypes.data))
+        return chars[:c.value].tobytes(), offs
+
+    def close(self):
+        if self.h:
+            self.lib.bolap_string_pool_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()

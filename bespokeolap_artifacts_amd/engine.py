@@ -350,6 +350,26 @@ class Engine:
                  "placement": st.placement, "slots": st.slots}
         return stats, out[:n.value]
 
+    def sort_order(self, primary, secondary=None):
+        """Device uint32 row order sorting by (primary, secondary) ascending."""
+        torch = _torch()
+        order = torch.empty(max(1, primary.numel()), dtype=torch.int32,
+                            device=f"cuda:{self.device}")
+        N.check(self.lib.bolap_sort_order_i32(self.ctx, ptr(primary), ptr(secondary),
+                                              int(primary.numel()), ptr(order)))
+        return order[:primary.numel()]
+
+    def gather_rows(self, src, order):
+        """src[order] on the device through bolap_gather_rows."""
+        torch = _torch()
+        dst = torch.empty((max(1, order.numel()),) + tuple(src.shape[1:]), dtype=src.dtype,
+                          device=src.device)
+        N.check(self.lib.bolap_gather_rows(self.ctx, ptr(src), int(src.shape[0]),
+                                           int(src.element_size() * (src[0].numel() if
+                                                                     src.dim() > 1 else 1)),
+                                           ptr(order), int(order.numel()), ptr(dst)))
+        return dst[:order.numel()]
+
     def hll(self, col):
         """HyperLogLog registers (4096, uint8 numpy) of a device int32 column."""
         import numpy as np

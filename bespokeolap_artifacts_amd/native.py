@@ -283,6 +283,13 @@ PROTOTYPES = {
     "bolap_ceb_run_batch": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(CebResult)]),
     "bolap_group_minmax": (C.c_int, [vp, P(MinmaxArgs), vp, i64, P(C.c_int64),
                                      P(MinmaxStats)]),
+    "bolap_string_pool_create": (C.c_int, [P(vp)]),
+    "bolap_string_pool_destroy": (None, [vp]),
+    "bolap_string_pool_intern": (C.c_int, [vp, vp, vp, i64, vp, vp]),
+    "bolap_string_pool_size": (C.c_int, [vp, P(C.c_int64), P(C.c_int64)]),
+    "bolap_string_pool_export": (C.c_int, [vp, C.c_int32, vp, vp]),
+    "bolap_sort_order_i32": (C.c_int, [vp, vp, vp, i64, vp]),
+    "bolap_gather_rows": (C.c_int, [vp, vp, i64, C.c_int32, vp, i64, vp]),
     "bolap_ceb_autotune": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(C.c_double)]),
     "bolap_ilike_mask": (C.c_int, [C.c_char_p, P(C.c_int64), i64, C.c_char_p, P(C.c_uint8)]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),

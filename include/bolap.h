@@ -510,6 +510,41 @@ int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* args,
                        bolap_minmax_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* CEB build-time data (bespoke_ceb/builder_impl.hpp:14-89, builder_impl.cpp  */
+/* :364-466): one string pool mapping every string column to int32 ids, and  */
+/* per-table sort orders.                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct bolap_string_pool bolap_string_pool;
+
+/* Replaces StringPool (builder_impl.hpp:14-89).  Host memory only. */
+int bolap_string_pool_create(bolap_string_pool** out);
+void bolap_string_pool_destroy(bolap_string_pool* pool);
+/* ids_out[i] = id of string i (chars[offsets[i] .. offsets[i+1])), a new id
+   (the pool's size) for a string not seen before; -1 where valid[i] == 0
+   (valid may be null).  StringPool::intern, builder_impl.hpp:26-39, with ids
+   in first-appearance order. */
+int bolap_string_pool_intern(bolap_string_pool* pool, const char* chars, const int64_t* offsets,
+                             int64_t n, const uint8_t* valid, int32_t* ids_out);
+int bolap_string_pool_size(const bolap_string_pool* pool, int64_t* n_strings, int64_t* n_chars);
+/* The dictionary in id order (chars of n_chars bytes, offsets of n_strings +
+   1), ASCII lower-cased when lower != 0 (StringPool::finalize and
+   get_lower, builder_impl.hpp:41-82). */
+int bolap_string_pool_export(const bolap_string_pool* pool, int32_t lower, char* chars,
+                             int64_t* offsets);
+
+/* order_out[i] = the input row at position i of the table sorted ascending by
+   (primary, secondary) as signed int32 (secondary may be null); ties keep
+   input order.  Replaces sort_table_by_i32_column(s), builder_impl.cpp
+   :265-340 (std::sort, ties unspecified).  Device pointers; synchronous. */
+int bolap_sort_order_i32(bolap_ctx* ctx, const int32_t* primary, const int32_t* secondary,
+                         int64_t rows, uint32_t* order_out);
+/* dst[i] = src[order[i]] for elements of elem_bytes (1, 2, 4 or 8): applies a
+   sort order to one column (reorder_vector, builder_impl.cpp:255-263).  An
+   order entry >= src_rows is an error.  Device pointers; synchronous. */
+int bolap_gather_rows(bolap_ctx* ctx, const void* src, int64_t src_rows, int32_t elem_bytes,
+                      const uint32_t* order, int64_t rows, void* dst);
+
+/* ------------------------------------------------------------------------ */
 /* Table statistics for the estimator (computed once per column, at load).   */
 /* ------------------------------------------------------------------------ */
 #define BOLAP_HLL_REGS 4096        /* HyperLogLog, p = 12 */

@@ -624,3 +624,17 @@ def group_minmax(pair_group, pair_item, item_offset, item_count, item_has, item_
     if n < 0:
         raise ValueError("oracle_group_minmax: out of bounds" if n == -1 else "short output")
     return out[:n]
+
+
+def sort_order_i32(primary, secondary=None) -> np.ndarray:
+    """Row order sorting by (primary, secondary) ascending, ties in input order."""
+    L = lib()
+    if not getattr(L, "_so_ready", False):
+        L.oracle_sort_order_i32.restype = None
+        L.oracle_sort_order_i32.argtypes = [vp, vp, i64, vp]
+        L._so_ready = True
+    p = np.ascontiguousarray(primary, np.int32)
+    s = None if secondary is None else np.ascontiguousarray(secondary, np.int32)
+    out = np.zeros(len(p), np.uint32)
+    L.oracle_sort_order_i32(_p(p), None if s is None else _p(s), len(p), _p(out))
+    return out

@@ -196,3 +196,23 @@ int64_t oracle_group_minmax(int64_t pairs, const int32_t* pg, const int32_t* pi,
     free(rows);
     return n;
 }
+
+/* ---- table sort order (bespoke_ceb/builder_impl.cpp:265-340) ------------ */
+
+static const int32_t* so_p;
+static const int32_t* so_s;
+
+static int so_cmp(const void* a, const void* b) {
+    const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    if (so_p[x] != so_p[y]) return so_p[x] < so_p[y] ? -1 : 1;
+    if (so_s && so_s[x] != so_s[y]) return so_s[x] < so_s[y] ? -1 : 1;
+    return x < y ? -1 : x > y;  /* input order on ties */
+}
+
+void oracle_sort_order_i32(const int32_t* primary, const int32_t* secondary, int64_t rows,
+                           uint32_t* order) {
+    for (int64_t i = 0; i < rows; ++i) order[i] = (uint32_t)i;
+    so_p = primary;
+    so_s = secondary;
+    qsort(order, (size_t)rows, 4, so_cmp);
+}

@@ -171,6 +171,12 @@ int64_t oracle_group_minmax(int64_t pairs, const int32_t* pair_group, const int3
                             const int32_t* range_value, int32_t groups, int32_t keys,
                             int64_t dense_max, int32_t* out, int64_t max_rows);
 
+/* Row order of a table sorted ascending by (primary, secondary) (secondary
+   may be null), ties in input order: sort_table_by_i32_column(s),
+   bespoke_ceb/builder_impl.cpp:265-340 (std::sort leaves ties open). */
+void oracle_sort_order_i32(const int32_t* primary, const int32_t* secondary, int64_t rows,
+                           uint32_t* order);
+
 /* --- TPC-H Q6 / Q18 and build-time derivations (tpch.c) ----------------- */
 int64_t oracle_days_from_civil(int32_t y, int32_t m, int32_t d);
 void oracle_civil_from_days(int32_t z, int32_t* y, int32_t* m, int32_t* d);
