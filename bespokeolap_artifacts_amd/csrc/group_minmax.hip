@@ -15,13 +15,16 @@
 //   count   one lane per join row: validates its group, item and range and
 //           sums the expanded rows (sizes the hashed table exactly: it can
 //           never hold more groups than expanded rows).
-//   dense   slots = group * keys + key: int32 min / max under atomics, a
-//           used byte and a has byte; a bitmap of used slots, a popcount
-//           scan and a rank scatter give the rows in slot order.
-//   hashed  open addressing on the 64-bit key (group << 32 | key) with CAS
-//           insertion and linear probing, the same per-slot fields; occupied
-//           slots are compacted (one atomic per wave) and LSD radix sorted by
-//           key.
+//   dense   slots = group * keys + key: int32 min / max side by side under
+//           atomics and a used byte; a bitmap of used slots, a popcount scan
+//           and a rank scatter give the rows in slot order.
+//   hashed  open addressing on the 64-bit slot number group * keys + key,
+//           16-byte slots {key, min, max} so that an expanded row touches one
+//           cache line (CAS insertion, linear probing); occupied slots are
+//           compacted (one atomic per wave) and LSD radix sorted on the
+//           slot number's bits.
+// A group has a value exactly when min <= max: both start at the empty
+// extremes (INT32_MAX, INT32_MIN) and every folded value lands between them.
 #include <algorithm>
 #include <cstring>
 
@@ -106,31 +109,34 @@ __global__ __launch_bounds__(kT) void mm_count_k(Args a, int64_t* __restrict__ s
     }
 }
 
-__global__ __launch_bounds__(kT) void mm_init_k(int32_t* __restrict__ mn, int32_t* __restrict__ mx,
-                                                uint8_t* __restrict__ used,
-                                                uint8_t* __restrict__ has,
-                                                uint64_t* __restrict__ keys, int64_t n) {
+struct HSlot {  // hashed table slot
+    uint64_t key;
+    int32_t mn, mx;
+};
+
+__global__ __launch_bounds__(kT) void mm_init_dense_k(int2* __restrict__ mm,
+                                                      uint8_t* __restrict__ used, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT) {
-        mn[i] = INT32_MAX;
-        mx[i] = INT32_MIN;
-        has[i] = 0;
-        if (used) used[i] = 0;
-        if (keys) keys[i] = kEmpty;
+        mm[i] = make_int2(INT32_MAX, INT32_MIN);
+        used[i] = 0;
     }
 }
 
-BOLAP_DEV void fold(int32_t* mn, int32_t* mx, uint8_t* hs, int64_t s, bool has, int32_t v) {
-    if (!has) return;
-    hs[s] = 1;
-    atomicMin(mn + s, v);
-    atomicMax(mx + s, v);
+__global__ __launch_bounds__(kT) void mm_init_hash_k(HSlot* __restrict__ t, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        t[i] = HSlot{kEmpty, INT32_MAX, INT32_MIN};
 }
 
-__global__ __launch_bounds__(kT) void mm_dense_k(Args a, int32_t* __restrict__ mn,
-                                                 int32_t* __restrict__ mx,
+BOLAP_DEV void fold(int32_t* mn, int32_t* mx, bool has, int32_t v) {
+    if (!has) return;
+    atomicMin(mn, v);
+    atomicMax(mx, v);
+}
+
+__global__ __launch_bounds__(kT) void mm_dense_k(Args a, int2* __restrict__ mm,
                                                  uint8_t* __restrict__ used,
-                                                 uint8_t* __restrict__ hs,
                                                  int64_t* __restrict__ st) {
     uint64_t errs = 0;
     for (int64_t p = int64_t(blockIdx.x) * kT + threadIdx.x; p < a.pairs;
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(kT) void mm_dense_k(Args a, int32_t* __restrict__ m
             }
             const int64_t s = base + k;
             used[s] = 1;
-            fold(mn, mx, hs, s, has, v);
+            fold(&mm[s].x, &mm[s].y, has, v);
         }
     }
     errs = wave_sum(errs);
@@ -182,25 +188,21 @@ __global__ __launch_bounds__(kT) void mm_bits_k(const uint8_t* __restrict__ used
 __global__ __launch_bounds__(kT) void mm_dense_emit_k(const uint32_t* __restrict__ bits,
                                                       const int64_t* __restrict__ cnt,
                                                       int64_t slots, int32_t keys,
-                                                      const int32_t* __restrict__ mn,
-                                                      const int32_t* __restrict__ mx,
-                                                      const uint8_t* __restrict__ hs,
+                                                      const int2* __restrict__ mm,
                                                       bolap_minmax_row* __restrict__ out) {
     for (int64_t s = int64_t(blockIdx.x) * kT + threadIdx.x; s < slots;
          s += int64_t(gridDim.x) * kT) {
         const uint32_t b = bits[s >> 5];
         if (!((b >> (s & 31)) & 1u)) continue;
         const int64_t pos = cnt[s >> 5] + __popc(b & ((1u << (s & 31)) - 1u));
-        const bool h = hs[s] != 0;
+        const int2 v = mm[s];
+        const bool h = v.x <= v.y;
         out[pos] = bolap_minmax_row{int32_t(s / keys), int32_t(s % keys), h ? 1 : 0,
-                                    h ? mn[s] : 0, h ? mx[s] : 0};
+                                    h ? v.x : 0, h ? v.y : 0};
     }
 }
 
-__global__ __launch_bounds__(kT) void mm_hash_k(Args a, uint64_t* __restrict__ keys,
-                                                uint64_t mask, int32_t* __restrict__ mn,
-                                                int32_t* __restrict__ mx,
-                                                uint8_t* __restrict__ hs,
+__global__ __launch_bounds__(kT) void mm_hash_k(Args a, HSlot* __restrict__ t, uint64_t mask,
                                                 int64_t* __restrict__ st) {
     uint64_t errs = 0;
     for (int64_t p = int64_t(blockIdx.x) * kT + threadIdx.x; p < a.pairs;
@@ -215,21 +217,21 @@ __global__ __launch_bounds__(kT) void mm_hash_k(Args a, uint64_t* __restrict__ k
                 ++errs;
                 continue;
             }
-            const uint64_t key = (uint64_t(uint32_t(g)) << 32) | uint32_t(k);
+            const uint64_t key = uint64_t(g) * uint64_t(a.keys) + uint64_t(k);
             // the table holds at most half as many keys as slots: a free
             // slot is always reached within mask + 1 probes
             uint64_t s = mix64(key) & mask;
             for (uint64_t step = 0; step <= mask; ++step) {
-                uint64_t cur = keys[s];
+                uint64_t cur = t[s].key;
                 if (cur == kEmpty) {
-                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(keys + s),
+                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(&t[s].key),
                                     (unsigned long long)kEmpty, (unsigned long long)key);
                     if (cur == kEmpty) cur = key;
                 }
                 if (cur == key) break;
                 s = (s + 1) & mask;
             }
-            fold(mn, mx, hs, int64_t(s), has, v);
+            fold(&t[s].mn, &t[s].mx, has, v);
         }
     }
     errs = wave_sum(errs);
@@ -237,47 +239,72 @@ __global__ __launch_bounds__(kT) void mm_hash_k(Args a, uint64_t* __restrict__ k
         atomicAdd(reinterpret_cast<unsigned long long*>(st + mErrors), errs);
 }
 
-// Occupied slots -> (key, slot) pairs, one atomic per wave.
-__global__ __launch_bounds__(kT) void mm_compact_k(const uint64_t* __restrict__ keys, int64_t cap,
+// Occupied slots -> (key, slot) pairs in two passes over per-workgroup
+// chunks of the table: occupied counts per chunk, a device scan, then each
+// workgroup writes its chunk's pairs from its offset (no shared counter).
+__global__ __launch_bounds__(kT) void mm_occ_count_k(const HSlot* __restrict__ t, int64_t cap,
+                                                     int64_t chunk, int64_t* __restrict__ cnt,
+                                                     int64_t* __restrict__ st) {
+    __shared__ uint64_t s_w[kT / kWave];
+    const int64_t c0 = int64_t(blockIdx.x) * chunk;
+    const int64_t c1 = c0 + chunk < cap ? c0 + chunk : cap;
+    uint64_t n = 0;
+    for (int64_t s = c0 + threadIdx.x; s < c1; s += kT) n += t[s].key != kEmpty ? 1u : 0u;
+    n = wave_sum(n);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t tot = 0;
+        for (int w = 0; w < kT / kWave; ++w) tot += s_w[w];
+        cnt[blockIdx.x] = int64_t(tot);
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(st + mGroups), tot);
+    }
+}
+
+__global__ __launch_bounds__(kT) void mm_compact_k(const HSlot* __restrict__ t, int64_t cap,
+                                                   int64_t chunk, const int64_t* __restrict__ off,
                                                    uint64_t* __restrict__ ck,
-                                                   uint32_t* __restrict__ cs,
-                                                   int64_t* __restrict__ st) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t stride = int64_t(gridDim.x) * kT;
-    // trip count uniform across the wave: the ballot needs every lane
-    for (int64_t b0 = int64_t(blockIdx.x) * kT + (threadIdx.x & ~(kWave - 1)); b0 < cap;
-         b0 += stride) {
-        const int64_t s = b0 + lane;
-        const uint64_t k = s < cap ? keys[s] : kEmpty;
+                                                   uint32_t* __restrict__ cs) {
+    __shared__ uint32_t s_w[kT / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t c0 = int64_t(blockIdx.x) * chunk;
+    const int64_t c1 = c0 + chunk < cap ? c0 + chunk : cap;
+    uint64_t base = uint64_t(off[blockIdx.x]);
+    // the trip count is the same for every thread of the workgroup
+    for (int64_t b0 = c0; b0 < c1; b0 += kT) {
+        const int64_t s = b0 + threadIdx.x;
+        const uint64_t k = s < c1 ? t[s].key : kEmpty;
         const bool occ = k != kEmpty;
         const uint64_t bal = __ballot(occ);
-        if (!bal) continue;
-        unsigned long long base = 0;
-        if (lane == 0)
-            base = atomicAdd(reinterpret_cast<unsigned long long*>(st + mGroups),
-                             (unsigned long long)__popcll(bal));
-        base = readlane_u64(base, 0);
+        if (lane == 0) s_w[wave] = uint32_t(__popcll(bal));
+        __syncthreads();
+        uint32_t before = 0, tile = 0;
+#pragma unroll
+        for (int w = 0; w < kT / kWave; ++w) {
+            before += w < wave ? s_w[w] : 0u;
+            tile += s_w[w];
+        }
+        __syncthreads();  // s_w is rewritten by the next tile
         if (occ) {
-            const uint64_t at = base + __popcll(bal & ((1ull << lane) - 1ull));
+            const uint64_t at = base + before + __popcll(bal & ((1ull << lane) - 1ull));
             ck[at] = k;
             cs[at] = uint32_t(s);
         }
+        base += tile;
     }
 }
 
 __global__ __launch_bounds__(kT) void mm_hash_emit_k(const uint64_t* __restrict__ ck,
                                                      const uint32_t* __restrict__ cs, int64_t n,
-                                                     const int32_t* __restrict__ mn,
-                                                     const int32_t* __restrict__ mx,
-                                                     const uint8_t* __restrict__ hs,
+                                                     int32_t keys, const HSlot* __restrict__ t,
                                                      bolap_minmax_row* __restrict__ out) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT) {
-        const uint32_t s = cs[i];
+        const HSlot v = t[cs[i]];
         const uint64_t k = ck[i];
-        const bool h = hs[s] != 0;
-        out[i] = bolap_minmax_row{int32_t(k >> 32), int32_t(uint32_t(k)), h ? 1 : 0,
-                                  h ? mn[s] : 0, h ? mx[s] : 0};
+        const bool h = v.mn <= v.mx;
+        out[i] = bolap_minmax_row{int32_t(k / uint64_t(keys)), int32_t(k % uint64_t(keys)),
+                                  h ? 1 : 0, h ? v.mn : 0, h ? v.mx : 0};
     }
 }
 
@@ -344,22 +371,19 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
     WsArena ws;
     if (dense) {
         const int64_t words = (slots + 31) / 32;
-        const size_t o_mn = ws.plan(size_t(slots) * 4), o_mx = ws.plan(size_t(slots) * 4);
-        const size_t o_used = ws.plan(size_t(slots)), o_has = ws.plan(size_t(slots));
+        const size_t o_mm = ws.plan(size_t(slots) * 8), o_used = ws.plan(size_t(slots));
         const size_t o_bits = ws.plan(size_t(words) * 4), o_cnt = ws.plan(size_t(words) * 8);
         const size_t o_scr = ws.plan(scan_scratch_bytes(words));
         BOLAP_TRY(ws_reserve(ctx, ws.off));
         ws.base = static_cast<char*>(ctx->ws);
-        int32_t* mn = ws.at<int32_t>(o_mn);
-        int32_t* mx = ws.at<int32_t>(o_mx);
+        int2* mm = ws.at<int2>(o_mm);
         uint8_t* used = ws.at<uint8_t>(o_used);
-        uint8_t* hs = ws.at<uint8_t>(o_has);
         {
             TraceScope ts(ctx, "minmax_dense");
-            hipLaunchKernelGGL(mm_init_k, dim3(grid_of(ctx, slots)), dim3(kT), 0, ctx->stream, mn,
-                               mx, used, hs, static_cast<uint64_t*>(nullptr), slots);
+            hipLaunchKernelGGL(mm_init_dense_k, dim3(grid_of(ctx, slots)), dim3(kT), 0,
+                               ctx->stream, mm, used, slots);
             hipLaunchKernelGGL(mm_dense_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
-                               k, mn, mx, used, hs, st);
+                               k, mm, used, st);
             hipLaunchKernelGGL(mm_bits_k, dim3(grid_of(ctx, words)), dim3(kT), 0, ctx->stream,
                                used, slots, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), st);
             BOLAP_TRY(launch_check("mm_dense_k"));
@@ -377,7 +401,7 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
             BOLAP_TRY(exclusive_scan_i64(ctx, ws.at<int64_t>(o_cnt), words, ws.at<void>(o_scr)));
             hipLaunchKernelGGL(mm_dense_emit_k, dim3(grid_of(ctx, slots)), dim3(kT), 0,
                                ctx->stream, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), slots,
-                               a->keys, mn, mx, hs, rows_out);
+                               a->keys, mm, rows_out);
             BOLAP_TRY(launch_check("mm_dense_emit_k"));
         }
     } else {
@@ -387,24 +411,27 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
         BOLAP_ARG_CHECK(lg <= 32, "bolap_group_minmax: %lld expanded rows need a table beyond "
                         "2^32 slots", (long long)expanded);
         const int64_t cap = int64_t(1) << lg;
-        const size_t o_keys = ws.plan(size_t(cap) * 8), o_mn = ws.plan(size_t(cap) * 4);
-        const size_t o_mx = ws.plan(size_t(cap) * 4), o_has = ws.plan(size_t(cap));
+        const size_t o_t = ws.plan(size_t(cap) * sizeof(HSlot));
         const size_t o_ck = ws.plan(size_t(bound) * 8), o_cs = ws.plan(size_t(bound) * 4);
+        const int64_t nb = int64_t(grid_of(ctx, cap));
+        const int64_t chunk = (cap + nb - 1) / nb;
+        const size_t o_occ = ws.plan(size_t(nb) * 8), o_oscr = ws.plan(scan_scratch_bytes(nb));
         const size_t o_scr = ws.plan(radix_sort_scratch_bytes(ctx, bound));
         BOLAP_TRY(ws_reserve(ctx, ws.off));
         ws.base = static_cast<char*>(ctx->ws);
-        uint64_t* keys = ws.at<uint64_t>(o_keys);
-        int32_t* mn = ws.at<int32_t>(o_mn);
-        int32_t* mx = ws.at<int32_t>(o_mx);
-        uint8_t* hs = ws.at<uint8_t>(o_has);
+        HSlot* t = ws.at<HSlot>(o_t);
         {
             TraceScope ts(ctx, "minmax_hash");
-            hipLaunchKernelGGL(mm_init_k, dim3(grid_of(ctx, cap)), dim3(kT), 0, ctx->stream, mn,
-                               mx, static_cast<uint8_t*>(nullptr), hs, keys, cap);
+            hipLaunchKernelGGL(mm_init_hash_k, dim3(grid_of(ctx, cap)), dim3(kT), 0, ctx->stream,
+                               t, cap);
             hipLaunchKernelGGL(mm_hash_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
-                               k, keys, uint64_t(cap - 1), mn, mx, hs, st);
-            hipLaunchKernelGGL(mm_compact_k, dim3(grid_of(ctx, cap)), dim3(kT), 0, ctx->stream,
-                               keys, cap, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs), st);
+                               k, t, uint64_t(cap - 1), st);
+            int64_t* occ = ws.at<int64_t>(o_occ);
+            hipLaunchKernelGGL(mm_occ_count_k, dim3(unsigned(nb)), dim3(kT), 0, ctx->stream, t,
+                               cap, chunk, occ, st);
+            BOLAP_TRY(exclusive_scan_i64(ctx, occ, nb, ws.at<void>(o_oscr)));
+            hipLaunchKernelGGL(mm_compact_k, dim3(unsigned(nb)), dim3(kT), 0, ctx->stream, t, cap,
+                               chunk, occ, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs));
             BOLAP_TRY(launch_check("mm_hash_k"));
         }
         BOLAP_TRY(read_st(ctx, h));
@@ -419,11 +446,11 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
             TraceScope ts(ctx, "minmax_emit");
             int passes = 0;
             BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs),
-                                       groups_n, 32 + std::max(1, log2_ceil(a->groups)),
+                                       groups_n, std::max(1, log2_ceil(slots)),
                                        ws.at<void>(o_scr), &passes));
             hipLaunchKernelGGL(mm_hash_emit_k, dim3(grid_of(ctx, groups_n)), dim3(kT), 0,
                                ctx->stream, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs),
-                               groups_n, mn, mx, hs, rows_out);
+                               groups_n, a->keys, t, rows_out);
             BOLAP_TRY(launch_check("mm_hash_emit_k"));
         }
     }
