@@ -39,7 +39,7 @@ constexpr uint32_t kSlotMul = 0x9E3779B1u;
 // Statistics slots (replicated, see fold.hpp); read_stats folds them.
 enum {
     sCMin = 0, sCMax, sOMin, sOMax, sCustPass, sOrderPass, sCount, sDMin, sDMax, sMatch,
-    sGroups, sFlags, sRMin, sRMax, sN = 16
+    sGroups, sFlags, sRMin, sRMax, sBad, sN = 16
 };
 
 BOLAP_DEV bool is_min_stat(int i) { return i == sCMin || i == sOMin || i == sDMin || i == sRMin; }
@@ -109,6 +109,8 @@ __global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ ke
                                                   int32_t segment, Ranges r,
                                                   uint32_t* __restrict__ bits, int64_t* st) {
     WaveCount wc;
+    uint64_t bad = 0;  // keys outside [cmin, cmax] (only with caller-given ranges)
+    const int64_t cspan = r.cmax - r.cmin + 1;
     const int64_t units = (n + 3) / 4;
     const int64_t full = n / 4;
     for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
@@ -134,13 +136,18 @@ __global__ __launch_bounds__(kT) void cust_bits_k(const int32_t* __restrict__ ke
         for (int j = 0; j < 4; ++j) {
             const bool pass = j < cnt && int32_t(sv[j]) == segment;
             wc.add(pass);
+            if (j < cnt && (int64_t(kv[j]) < r.cmin || int64_t(kv[j]) > r.cmax)) {
+                ++bad;
+                continue;
+            }
             if (pass) {
                 const int64_t rel = int64_t(kv[j]) - r.cmin;
-                atomicOr(bits + (rel >> 5), 1u << (rel & 31));
+                if (rel < cspan) atomicOr(bits + (rel >> 5), 1u << (rel & 31));
             }
         }
     }
     block_fold<kT>(st, sCustPass, int64_t(wc.n), 0);
+    block_fold<kT>(st, sBad, int64_t(wave_sum(bad)), 0);
 }
 
 // Four order rows per lane: day and customer columns read with 16-byte loads.
@@ -242,7 +249,7 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
     const uint32_t mask = (uint32_t(1) << t.cap_log2) - 1;
     int64_t dl = INT64_MAX, dh = INT64_MIN;
     WaveCount passed;
-    uint64_t inserted = 0;
+    uint64_t inserted = 0, bad = 0;
     bool failed = false;
     const int64_t units = (n + 3) / 4;
     for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
@@ -253,6 +260,10 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
             const bool pass =
                 j < o.cnt && order_pass_v(o.day[j], o.cust[j], day_lt, cbits, cspan, r);
             passed.add(pass);
+            if (j < o.cnt && (int64_t(o.key[j]) < r.omin || int64_t(o.key[j]) > r.omax)) {
+                ++bad;  // only with caller-given ranges
+                continue;
+            }
             if (!pass || failed) continue;
             const int32_t key = o.key[j];
             const int32_t d = o.day[j];
@@ -284,6 +295,7 @@ __global__ __launch_bounds__(kT) void order_build_k(const int32_t* __restrict__ 
     }
     block_fold<kT>(st, sOrderPass, int64_t(passed.n), 0);
     block_fold<kT>(st, sGroups, int64_t(wave_sum(inserted)), 0);
+    block_fold<kT>(st, sBad, int64_t(wave_sum(bad)), 0);
     block_fold<kT>(st, sFlags, __ballot(failed) ? 1 : 0, 0);
     fold_minmax<kT>(dl, dh, st, sDMin, sDMax);
 }
@@ -714,15 +726,20 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
                  WsArena& ws, size_t* o_rows_out, ChainRun& run) {
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
     hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
-    {
-        TraceScope ts(ctx, "chain_stats");
-        hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
-                           ctx->stream, a->cust_key, a->cust_rows, a->order_key, a->order_rows,
-                           st);
+    Ranges r;
+    if (a->key_ranges_known) {
+        r = Ranges{a->cust_key_min, a->cust_key_max, a->order_key_min, a->order_key_max};
+    } else {
+        {
+            TraceScope ts(ctx, "chain_stats");
+            hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->order_rows)), dim3(kT), 0,
+                               ctx->stream, a->cust_key, a->cust_rows, a->order_key,
+                               a->order_rows, st);
+        }
+        BOLAP_TRY(launch_check("key_minmax_k"));
+        BOLAP_TRY(read_stats(ctx, run.st));
+        r = Ranges{run.st[sCMin], run.st[sCMax], run.st[sOMin], run.st[sOMax]};
     }
-    BOLAP_TRY(launch_check("key_minmax_k"));
-    BOLAP_TRY(read_stats(ctx, run.st));
-    Ranges r{run.st[sCMin], run.st[sCMax], run.st[sOMin], run.st[sOMax]};
     const int64_t cspan = a->cust_rows > 0 ? r.cmax - r.cmin + 1 : 0;
     const int64_t ospan = a->order_rows > 0 ? r.omax - r.omin + 1 : 0;
     const int64_t cwords = (cspan + 31) / 32 + 1;
@@ -829,6 +846,15 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
     }
     BOLAP_TRY(launch_check("compact_groups_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
+    BOLAP_ARG_CHECK(run.st[sBad] == 0,
+                    "bolap_chain: %lld customer or order keys outside the key ranges passed in "
+                    "the args", (long long)run.st[sBad]);
+    if (a->key_ranges_known) {  // no min/max pass ran: the ordering packs keys by these
+        run.st[sCMin] = r.cmin;
+        run.st[sCMax] = r.cmax;
+        run.st[sOMin] = r.omin;
+        run.st[sOMax] = r.omax;
+    }
     if (!direct && run.st[sGroups] > cap / 2) run.st[sFlags] |= 1;  // load factor above 1/2
     BOLAP_TRY(pinned_reserve(ctx, 8));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -845,16 +871,19 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
 int chain_groups_ranges(bolap_ctx* ctx, const bolap_chain_args* a, WsArena& ws,
                         size_t* o_rows_out, ChainRun& run) {
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
-    hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
-    {
-        TraceScope ts(ctx, "chain_stats");
-        hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->cust_rows / 4 + 1)), dim3(kT), 0,
-                           ctx->stream, a->cust_key, a->cust_rows, nullptr, int64_t(0), st);
+    Ranges r{a->cust_key_min, a->cust_key_max, 0, 0};
+    if (!a->key_ranges_known) {
+        hipLaunchKernelGGL(stats_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+        {
+            TraceScope ts(ctx, "chain_stats");
+            hipLaunchKernelGGL(key_minmax_k, dim3(grid_for(ctx, a->cust_rows / 4 + 1)), dim3(kT),
+                               0, ctx->stream, a->cust_key, a->cust_rows, nullptr, int64_t(0), st);
+        }
+        BOLAP_TRY(launch_check("key_minmax_k"));
+        int64_t cst[sN];
+        BOLAP_TRY(read_stats(ctx, cst));
+        r = Ranges{cst[sCMin], cst[sCMax], 0, 0};
     }
-    BOLAP_TRY(launch_check("key_minmax_k"));
-    int64_t cst[sN];
-    BOLAP_TRY(read_stats(ctx, cst));
-    Ranges r{cst[sCMin], cst[sCMax], 0, 0};
     const int64_t cspan = a->cust_rows > 0 ? r.cmax - r.cmin + 1 : 0;
     const int64_t cwords = (cspan + 31) / 32 + 1;
     // Rows and passing orders: at most one per order.  Filter chunks are
@@ -922,6 +951,9 @@ int chain_groups_ranges(bolap_ctx* ctx, const bolap_chain_args* a, WsArena& ws,
     }
     BOLAP_TRY(launch_check("range_groups_k"));
     BOLAP_TRY(read_stats(ctx, run.st));
+    BOLAP_ARG_CHECK(run.st[sBad] == 0,
+                    "bolap_chain: %lld customer keys outside the key range passed in the args",
+                    (long long)run.st[sBad]);
     BOLAP_TRY(pinned_reserve(ctx, 8));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, n_rows, 8, hipMemcpyDeviceToHost, ctx->stream));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1011,6 +1043,9 @@ int chain_validate(const bolap_chain_args* a) {
                     "bolap_chain: null fact column");
     BOLAP_ARG_CHECK((a->order_fact_start == nullptr) == (a->order_fact_end == nullptr),
                     "bolap_chain: order_fact_start and order_fact_end go together");
+    BOLAP_ARG_CHECK(!a->key_ranges_known || (a->cust_key_min <= a->cust_key_max &&
+                                             a->order_key_min <= a->order_key_max),
+                    "bolap_chain: key ranges with min > max");
     BOLAP_ARG_CHECK(a->order_fact_start || a->fact_rows == 0 || aligned(a->fact_order, 16),
                     "bolap_chain: fact_order must be 16-byte aligned");
     BOLAP_ARG_CHECK(!a->order_fact_start || a->fact_rows < (int64_t(1) << 32),

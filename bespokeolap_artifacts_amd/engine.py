@@ -238,9 +238,11 @@ class Engine:
 
     # ------------------------------------------------- chain join + group-by
     def chain_args(self, t: dict, segment: int, day_lt: int, capacity_hint: int = 0,
-                   disc_price=None, ranges=None) -> N.ChainArgs:
+                   disc_price=None, ranges=None, key_ranges=None) -> N.ChainArgs:
         """`ranges` = (start, end) device uint32/int32 arrays per order row
-        (order_ranges): the fact side is then read by ranges."""
+        (order_ranges): the fact side is then read by ranges.  `key_ranges` =
+        (cust_min, cust_max, order_min, order_max) from the tables' statistics:
+        the call then skips its own min/max pass."""
         a = N.ChainArgs()
         a.cust_rows = int(t["cust_key"].numel())
         a.cust_key, a.cust_segment = ptr(t["cust_key"]), ptr(t["cust_segment"])
@@ -259,6 +261,10 @@ class Engine:
         a.capacity_hint = int(capacity_hint)
         if ranges is not None:
             a.order_fact_start, a.order_fact_end = ptr(ranges[0]), ptr(ranges[1])
+        if key_ranges is not None:
+            a.key_ranges_known = 1
+            (a.cust_key_min, a.cust_key_max, a.order_key_min,
+             a.order_key_max) = (int(v) for v in key_ranges)
         return a
 
     @staticmethod

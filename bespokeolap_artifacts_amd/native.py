@@ -161,7 +161,10 @@ class ChainArgs(C.Structure):
                 ("fact_rows", C.c_int64), ("fact_order", vp), ("fact_price", vp),
                 ("price_dtype", C.c_int32), ("fact_disc", vp), ("disc_dtype", C.c_int32),
                 ("fact_disc_price", vp), ("capacity_hint", C.c_int64),
-                ("order_fact_start", vp), ("order_fact_end", vp)]
+                ("order_fact_start", vp), ("order_fact_end", vp),
+                ("key_ranges_known", C.c_int32), ("cust_key_min", C.c_int32),
+                ("cust_key_max", C.c_int32), ("order_key_min", C.c_int32),
+                ("order_key_max", C.c_int32)]
 
 
 class GroupRow(C.Structure):

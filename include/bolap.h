@@ -294,6 +294,14 @@ typedef struct {
        unused and no group table is built. */
     const uint32_t* order_fact_start;
     const uint32_t* order_fact_end;
+    /* Optional key ranges from the tables' statistics (a built database keeps
+       per-column i32_min / i32_max, bespoke_ceb/builder_impl.hpp:100-101):
+       with key_ranges_known = 1 the call skips its own min/max pass and the
+       host round trip after it.  A customer or order key outside its range
+       is an error (BOLAP_ERR_ARG), never a wrong result. */
+    int32_t key_ranges_known;
+    int32_t cust_key_min, cust_key_max;
+    int32_t order_key_min, order_key_max;
 } bolap_chain_args;
 
 typedef struct {

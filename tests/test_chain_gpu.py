@@ -226,3 +226,59 @@ def test_chain_by_ranges_empty_results(engine):
     assert top == [] and gs["groups"] == 0
     top, gs = engine.chain_topk(engine.chain_args(t, 1, 0, ranges=rng), 10)
     assert top == [] and gs["order_pass"] == 0
+
+
+@pytest.mark.parametrize("mode", [1, 2])  # directly addressed, hashed group table
+@pytest.mark.parametrize("widen", [0, 1000])
+def test_chain_known_key_ranges_match_computed(engine, mode, widen):
+    """Key ranges from the caller (the tables' statistics) instead of the
+    call's own min/max pass: same rows on both group-table paths and by
+    ranges, also for ranges wider than the keys, with order rows shuffled."""
+    import torch
+    t = engine.gen_chain(SEED, 20_000, 200_000)
+    perm = torch.randperm(t["order_key"].numel(), generator=torch.Generator().manual_seed(9))
+    perm = perm.cuda()
+    t2 = dict(t)
+    for k in ("order_key", "order_day", "order_cust"):
+        t2[k] = t[k][perm].contiguous()
+    want, st = want_sorted(t2, 3, 1200)
+    kr = (int(t2["cust_key"].min()) - widen, int(t2["cust_key"].max()) + widen,
+          int(t2["order_key"].min()) - widen, int(t2["order_key"].max()) + widen)
+    engine.tune_set("chain.direct", mode)
+    try:
+        allg, gs = engine.chain_groups_sorted(engine.chain_args(t2, 3, 1200, key_ranges=kr),
+                                              len(want) + 5)
+    finally:
+        engine.tune_set("chain.direct", 0)
+    assert np.array_equal(allg, want)
+    check_stats(gs, st)
+    _, start, end = engine.order_ranges(t2["order_key"], t2["fact_order"],
+                                        int(t2["order_key"].max().item()) + 1)
+    allr, gr = engine.chain_groups_sorted(
+        engine.chain_args(t2, 3, 1200, ranges=(start, end), key_ranges=kr), len(want) + 5)
+    assert np.array_equal(allr, want)
+    check_stats(gr, st)
+
+
+@pytest.mark.parametrize("which", ["cust", "order"])
+def test_chain_key_outside_given_range_is_an_error(engine, which):
+    from bespokeolap_artifacts_amd import native as N
+    t = engine.gen_chain(SEED, 2000, 20_000)
+    kr = [int(t["cust_key"].min()), int(t["cust_key"].max()),
+          int(t["order_key"].min()), int(t["order_key"].max())]
+    if which == "cust":
+        kr[1] -= 1
+    else:
+        kr[3] -= 1
+    with pytest.raises(N.BolapError):
+        engine.chain_groups_sorted(engine.chain_args(t, 1, 1155, key_ranges=kr), 50_000)
+    if which == "cust":
+        _, start, end = engine.order_ranges(t["order_key"], t["fact_order"],
+                                            int(t["order_key"].max().item()) + 1)
+        with pytest.raises(N.BolapError):
+            engine.chain_groups_sorted(
+                engine.chain_args(t, 1, 1155, ranges=(start, end), key_ranges=kr), 50_000)
+    # the context stays usable
+    want, _ = want_sorted(t, 1, 1155)
+    allg, _ = engine.chain_groups_sorted(engine.chain_args(t, 1, 1155), len(want) + 5)
+    assert np.array_equal(allg, want)
