@@ -97,12 +97,19 @@ struct TitleQ {
     int64_t n;
 };
 
+// A wave takes 256 movies as four 64-movie chunks, so one LDS read of a
+// query's predicate serves four chunks and the query's eight bitmap words
+// for those movies go out as one eight-lane store.
+constexpr int kTitleU = 4;
+static_assert(kTitleU == 4, "the bitmap-word store below selects among four chunks");
+
 __global__ __launch_bounds__(kT) void ceb_title_k(DbK db, const QK* __restrict__ qs, int32_t nq,
                                                   uint32_t* __restrict__ tf,
                                                   uint32_t* __restrict__ qm) {
     __shared__ TitleQ s_q[kStage];
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = int64_t(gridDim.x) * (kT / kWave);
+    constexpr int64_t kSpan = int64_t(kWave) * kTitleU;  // movies per wave step
     for (int c0 = 0; c0 < nq; c0 += kStage) {
         const int cn = nq - c0 < kStage ? nq - c0 : kStage;
         __syncthreads();  // the previous chunk's readers are done
@@ -112,28 +119,46 @@ __global__ __launch_bounds__(kT) void ceb_title_k(DbK db, const QK* __restrict__
         }
         __syncthreads();
         for (int64_t g = int64_t(blockIdx.x) * (kT / kWave) + threadIdx.x / kWave;
-             g * kWave < db.n_title; g += wstride) {
-            const int64_t m = g * kWave + lane;
-            const bool in = m < db.n_title;
-            const int32_t y = in ? db.year[m] : 0;
-            const int32_t k = in ? db.kind[m] : 0;
-            const int32_t c = in ? db.code[m] : 0;
-            uint32_t acc = 0;
+             g * kSpan < db.n_title; g += wstride) {
+            int32_t y[kTitleU], k[kTitleU], c[kTitleU];
+            bool in[kTitleU];
+            uint32_t acc[kTitleU];
+#pragma unroll
+            for (int j = 0; j < kTitleU; ++j) {
+                const int64_t m = g * kSpan + j * kWave + lane;
+                in[j] = m < db.n_title;
+                y[j] = in[j] ? db.year[m] : 0;
+                k[j] = in[j] ? db.kind[m] : 0;
+                c[j] = in[j] ? db.code[m] : 0;
+                acc[j] = 0;
+            }
+            // lane l < 8 writes bitmap word 8 g + l: chunk l / 2, half l % 2
+            const int64_t w_out = g * (2 * kTitleU) + lane;
+            const bool writer = lane < 2 * kTitleU && w_out < db.words;
             for (int i = 0; i < cn; ++i) {
                 const int q = c0 + i;
                 const TitleQ Q = s_q[i];
-                const bool p = in && y >= Q.lo && y <= Q.hi && uint32_t(k) < 32u &&
-                               ((Q.kinds >> k) & 1u) && bit_of(Q.bits, Q.n, c);
-                acc |= uint32_t(p) << (q & 31);
-                if ((q & 31) == 31 || q == nq - 1) {
-                    if (in) qm[int64_t(q >> 5) * db.n_title + m] = acc;
-                    acc = 0;
+                uint64_t bl[kTitleU];
+#pragma unroll
+                for (int j = 0; j < kTitleU; ++j) {
+                    const bool p = in[j] && y[j] >= Q.lo && y[j] <= Q.hi &&
+                                   uint32_t(k[j]) < 32u && ((Q.kinds >> k[j]) & 1u) &&
+                                   bit_of(Q.bits, Q.n, c[j]);
+                    acc[j] |= uint32_t(p) << (q & 31);
+                    bl[j] = __ballot(p);
                 }
-                const uint64_t bl = __ballot(p);
-                if (lane == 0) {
-                    uint32_t* row = tf + int64_t(q) * db.words;
-                    row[2 * g] = uint32_t(bl);
-                    if (2 * g + 1 < db.words) row[2 * g + 1] = uint32_t(bl >> 32);
+                if ((q & 31) == 31 || q == nq - 1) {
+#pragma unroll
+                    for (int j = 0; j < kTitleU; ++j) {
+                        if (in[j]) qm[int64_t(q >> 5) * db.n_title + g * kSpan + j * kWave + lane] =
+                            acc[j];
+                        acc[j] = 0;
+                    }
+                }
+                if (writer) {
+                    const int j = lane >> 1;
+                    const uint64_t b = j == 0 ? bl[0] : j == 1 ? bl[1] : j == 2 ? bl[2] : bl[3];
+                    tf[int64_t(q) * db.words + w_out] = (lane & 1) ? uint32_t(b >> 32) : uint32_t(b);
                 }
             }
         }
@@ -723,7 +748,7 @@ int enqueue(bolap_ctx* ctx, const DbK& db, const Layout& L, const int32_t* n_per
     const int per_cu = int(ctx->knob("ceb.blocks_per_cu", 4));
     // queries per pass over a fact table, in whole 32-query words
     const int wchunk = int(std::max<int64_t>(1, (ctx->knob("ceb.qchunk", L.nq) + 31) / 32));
-    hipLaunchKernelGGL(ceb_title_k, dim3(grid_for(ctx, (db.n_title + 63) / 64 * 64, 8)),
+    hipLaunchKernelGGL(ceb_title_k, dim3(grid_for(ctx, (db.n_title + 3) / 4, 8)),
                        dim3(kT), 0, ctx->stream, db, qs, L.nq, tf, qm);
     hipLaunchKernelGGL(ceb_rank_k, dim3(unsigned(L.nq)), dim3(kRankT), 0, ctx->stream, tf,
                        L.words, rank, st);
