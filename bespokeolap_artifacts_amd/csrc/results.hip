@@ -172,6 +172,22 @@ int bolap_q17_write_csv(const char* path, const bolap_q17_result* result) {
     return out.good() ? BOLAP_OK : BOLAP_ERR_ARG;
 }
 
+int bolap_q3_write_csv(const char* path, const bolap_group_row* rows, int64_t n,
+                       const int32_t* shippriority) {
+    BOLAP_ARG_CHECK(path && (rows || n == 0) && n >= 0, "bolap_q3_write_csv: bad arguments");
+    std::ofstream out(path);
+    BOLAP_ARG_CHECK(out.good(), "bolap_q3_write_csv: cannot open %s", path);
+    out << "l_orderkey,revenue,o_orderdate,o_shippriority\n";
+    out.setf(std::ios::fixed);
+    out << std::setprecision(2);
+    for (int64_t i = 0; i < n; ++i) {
+        const bolap_group_row& r = rows[i];
+        out << r.key << ',' << double(r.revenue) / 100.0 << ',' << date_string(r.day) << ','
+            << (shippriority ? shippriority[i] : 0) << "\n";
+    }
+    return out.good() ? BOLAP_OK : BOLAP_ERR_ARG;
+}
+
 int bolap_q18_write_csv(const char* path, const bolap_q18_row* rows, int64_t n,
                         const char* const* c_name, int64_t n_names) {
     BOLAP_ARG_CHECK(path && (rows || n == 0) && n >= 0 && (c_name || n_names == 0),

@@ -182,6 +182,18 @@ class Engine:
         names = (C.c_char_p * max(1, len(c_name)))(*(x.encode() for x in c_name))
         N.check(N.load().bolap_q18_write_csv(str(path).encode(), arr, n, names, len(c_name)))
 
+    @staticmethod
+    def q3_write_csv(path: str, rows, shippriority=None) -> None:
+        """rows: (key, day, revenue) tuples or a GROUP_DTYPE array, in output order."""
+        n = len(rows)
+        arr = (N.GroupRow * max(1, n))()
+        for i, r in enumerate(rows):
+            arr[i] = N.GroupRow(int(r[0]), int(r[1]), int(r[2]))
+        sp = None
+        if shippriority is not None:
+            sp = (C.c_int32 * max(1, n))(*(int(x) for x in shippriority))
+        N.check(N.load().bolap_q3_write_csv(str(path).encode(), arr, n, sp))
+
     def q1_scan_agg_async(self, a: N.Q1Args, out_dev) -> None:
         N.check(self.lib.bolap_q1_scan_agg_async(self.ctx, C.byref(a),
                                                  C.c_void_p(out_dev.data_ptr())))

@@ -284,6 +284,7 @@ PROTOTYPES = {
     "bolap_gen_price_disc": (C.c_int, [vp, u64, C.c_uint32, i64, vp, vp]),
     "bolap_ceb_run": (C.c_int, [vp, P(CebDb), P(CebQuery), P(CebResult)]),
     "bolap_ceb_run_batch": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(CebResult)]),
+    "bolap_q3_write_csv": (C.c_int, [C.c_char_p, P(GroupRow), i64, vp]),
     "bolap_group_minmax": (C.c_int, [vp, P(MinmaxArgs), vp, i64, P(C.c_int64),
                                      P(MinmaxStats)]),
     "bolap_string_pool_create": (C.c_int, [P(vp)]),

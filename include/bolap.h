@@ -722,6 +722,12 @@ typedef struct {
 int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* args, bolap_q18_row* rows_out,
               int64_t max_rows, bolap_q18_stats* stats);
 
+/* write_q3_csv (query_q3.cpp:367-380): rows as bolap_chain_topk /
+   bolap_chain_groups_sorted return them (revenue in cents of the summed
+   discounted_price, day = order date offset from 1992-01-01), 2 decimals,
+   ISO dates; shippriority[i] per row (null: 0, the TPC-H value).  Host only. */
+int bolap_q3_write_csv(const char* path, const bolap_group_row* rows, int64_t n,
+                       const int32_t* shippriority);
 /* write_q6_csv (query_q6.cpp:399-408) and write_q18_csv (query_q18.cpp:465-484:
    c_name quoted with \" escapes, ISO dates, 2 decimals); c_name[cust_row]
    gives each row's customer name.  Host only. */

@@ -127,3 +127,19 @@ def test_q14_q17_csv(tmp_path):
         p = tmp_path / "q17.csv"
         N.check(lib.bolap_q17_write_csv(str(p).encode(), r))
         assert p.read_text() == f"avg_yearly\n{s / 700.0:.2f}\n"
+
+
+def test_q3_csv(tmp_path):
+    """write_q3_csv (bespoke_tpch/query_q3.cpp:367-380): key, revenue / 100 with
+    two decimals, ISO order date, shippriority."""
+    rows = [(172484, 699, 45567367), (5, 0, 1), (2**31 - 1, 2405, 123456789012),
+            (7, 30, 0)]
+    p = tmp_path / "q3.csv"
+    Engine.q3_write_csv(str(p), rows)
+    base = datetime.date(1992, 1, 1)
+    want = ["l_orderkey,revenue,o_orderdate,o_shippriority\n"]
+    for k, d, r in rows:
+        want.append(f"{k},{r / 100.0:.2f},{(base + datetime.timedelta(days=d)).isoformat()},0\n")
+    assert p.read_text() == "".join(want)
+    Engine.q3_write_csv(str(p), rows, shippriority=[0, 1, 2, 3])
+    assert [ln.rsplit(",", 1)[1] for ln in p.read_text().splitlines()[1:]] == ["0", "1", "2", "3"]
