@@ -13,7 +13,11 @@
 // thread interns the tables in turn (its concurrent interning leaves the ids
 // open).  The sort order is an LSD radix sort on the device of the packed
 // (primary, secondary) key with the row index as payload: ascending by the
-// signed keys, ties in input order (std::sort leaves them open).
+// signed keys, ties in input order (std::sort leaves them open).  The key is
+// packed as offsets from the columns' minima, (p - pmin) << sbits | (s -
+// smin), so the sort runs only as many 8-bit passes as the key spans need
+// (cast_info's (movie_id, role_id): 26 bits, four passes instead of eight).
+#include <algorithm>
 #include <cctype>
 #include <cstring>
 #include <string>
@@ -43,13 +47,56 @@ unsigned grid_of(bolap_ctx* ctx, int64_t n) {
 // Signed int32 -> order-preserving unsigned.
 BOLAP_DEV uint32_t bias(int32_t v) { return uint32_t(v) ^ 0x80000000u; }
 
+// Per-workgroup minima and maxima of both columns folded with one atomic
+// per workgroup and value (a grid of at most a few hundred workgroups).
+__global__ __launch_bounds__(kT) void so_range_k(const int32_t* __restrict__ p,
+                                                 const int32_t* __restrict__ s, int64_t n,
+                                                 unsigned* __restrict__ mm) {
+    __shared__ unsigned s_v[4][kT / kWave];
+    unsigned v[4] = {~0u, 0u, ~0u, 0u};  // biased: min p, max p, min s, max s
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const unsigned a = bias(p[i]);
+        v[0] = a < v[0] ? a : v[0];
+        v[1] = a > v[1] ? a : v[1];
+        if (s) {
+            const unsigned b = bias(s[i]);
+            v[2] = b < v[2] ? b : v[2];
+            v[3] = b > v[3] ? b : v[3];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const unsigned o = unsigned(__shfl_xor(int(v[k]), off, kWave));
+            v[k] = (k & 1) ? (o > v[k] ? o : v[k]) : (o < v[k] ? o : v[k]);
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) s_v[k][threadIdx.x / kWave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        unsigned r = s_v[k][0];
+        for (int w = 1; w < kT / kWave; ++w)
+            r = (k & 1) ? (s_v[k][w] > r ? s_v[k][w] : r) : (s_v[k][w] < r ? s_v[k][w] : r);
+        if (k & 1) atomicMax(mm + k, r);
+        else atomicMin(mm + k, r);
+    }
+}
+
+__global__ void so_range_init_k(unsigned* mm) {
+    if (threadIdx.x < 4) mm[threadIdx.x] = (threadIdx.x & 1) ? 0u : ~0u;
+}
+
 __global__ __launch_bounds__(kT) void so_keys_k(const int32_t* __restrict__ p,
                                                 const int32_t* __restrict__ s, int64_t n,
+                                                unsigned pmin, unsigned smin, int sbits,
                                                 uint64_t* __restrict__ keys,
                                                 uint32_t* __restrict__ rows) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT) {
-        keys[i] = s ? (uint64_t(bias(p[i])) << 32) | bias(s[i]) : uint64_t(bias(p[i]));
+        const uint64_t hi = uint64_t(bias(p[i]) - pmin);
+        keys[i] = s ? (hi << sbits) | uint64_t(bias(s[i]) - smin) : hi;
         rows[i] = uint32_t(i);
     }
 }
@@ -150,11 +197,26 @@ int bolap_sort_order_i32(bolap_ctx* ctx, const int32_t* primary, const int32_t* 
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     uint64_t* keys = ws.at<uint64_t>(o_keys);
+    unsigned* mm = static_cast<unsigned*>(ctx->result_dev);
+    hipLaunchKernelGGL(so_range_init_k, dim3(1), dim3(kWave), 0, ctx->stream, mm);
+    hipLaunchKernelGGL(so_range_k, dim3(std::min<unsigned>(grid_of(ctx, rows), unsigned(ctx->num_cus))),
+                       dim3(kT), 0, ctx->stream, primary, secondary, rows, mm);
+    BOLAP_TRY(launch_check("so_range_k"));
+    unsigned h[4];
+    BOLAP_HIP_TRY(hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    auto bits_of = [](uint64_t span) {  // bits to hold 0 .. span
+        int b = 0;
+        while (b < 64 && (span >> b) != 0) ++b;
+        return b;
+    };
+    const int pbits = bits_of(uint64_t(h[1] - h[0]));
+    const int sbits = secondary ? bits_of(uint64_t(h[3] - h[2])) : 0;
     hipLaunchKernelGGL(so_keys_k, dim3(grid_of(ctx, rows)), dim3(kT), 0, ctx->stream, primary,
-                       secondary, rows, keys, order_out);
+                       secondary, rows, h[0], h[2], sbits, keys, order_out);
     BOLAP_TRY(launch_check("so_keys_k"));
     int passes = 0;
-    BOLAP_TRY(radix_sort_pairs(ctx, keys, order_out, rows, secondary ? 64 : 32,
+    BOLAP_TRY(radix_sort_pairs(ctx, keys, order_out, rows, std::max(1, pbits + sbits),
                                ws.at<void>(o_scr), &passes));
     BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return BOLAP_OK;
