@@ -123,3 +123,17 @@ def test_batch_across_query_words_and_chunks(engine, small, qchunk):
         engine.tune_set("ceb.qchunk", old)
     for i in (0, 31, 32, 63, 64, 74):
         assert single[i] == O.ceb_run(t, qs[i], oracle_masks(s, qs[i])), qs[i]
+
+
+def test_batch_beyond_one_staged_chunk(engine, small):
+    """600 queries: more than the 512 whose predicates one workgroup stages in
+    LDS, so the title and fact stages walk two query chunks (and a ragged last
+    32-query word)."""
+    s, t = small
+    qs = CB.random_queries(s, 600, seed=13)
+    bound = [CB.bind(s, q) for q in qs]
+    batch = CB.run_batch(engine, s, bound)
+    for i in list(range(0, 600, 37)) + [511, 512, 513, 575, 576, 599]:
+        assert CB.run_one(engine, s, bound[i]) == batch[i], i
+    for i in (0, 512, 599):
+        assert batch[i] == O.ceb_run(t, qs[i], oracle_masks(s, qs[i])), qs[i]
