@@ -487,7 +487,7 @@ def bench_c5(eng, torch, args):
                                       sorted(by_template.items())},
             "autotuned": {"blocks_per_cu": eng.tune_get("ceb.blocks_per_cu"),
                           "qchunk": eng.tune_get("ceb.qchunk"), "batch_ms": tune_ms},
-            "bit_identical": batch == single}
+            "bit_identical": batch == single, "kernel": "ceb_fact_k"}
 
 
 # ------------------------------------------------------------- copy ceiling
@@ -605,6 +605,18 @@ MEASURED_TRAFFIC = {
 }
 
 
+# L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)) of each join workload's
+# dominant kernel, from a rocprofv3 counter pass (summary under profiles/).
+MEASURED_L2_HIT = {
+    "probe_sum_k": 0.039,
+    "fact_probe_k": 0.455,
+    "range_groups_k": 0.557,
+    "q14_flat_k": 0.106,
+    "q17_probe_k<1>": 0.043,
+    "ceb_fact_k": 0.179,
+}
+
+
 def roofline(head):
     """Roofline of the workload's dominant kernel (algorithmic bytes per
     launch / measured kernel time)."""
@@ -638,6 +650,12 @@ def main():
                       "minmax": bench_minmax, "sort": bench_sort,
                       "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))
+    for r in results.values():
+        if r.get("kernel") in MEASURED_L2_HIT:
+            r["l2_hit_rate"] = MEASURED_L2_HIT[r["kernel"]]
+        br = r.get("by_ranges")
+        if isinstance(br, dict) and br.get("kernel") in MEASURED_L2_HIT:
+            br["l2_hit_rate"] = MEASURED_L2_HIT[br["kernel"]]
     # The headline is the first workload with a bandwidth figure (C2 by default).
     head = next((results[w] for w in results if "GBps" in results[w]),
                 next(iter(results.values())))
