@@ -577,6 +577,16 @@ __global__ __launch_bounds__(kT) void q14_flat_k(Q14K a, int64_t* partials) {
 enum { kQ18MaxL = 0, kQ18MaxO = 1, kQ18MaxC = 2, kQ18Unsorted = 3, kQ18Count = 4,
        kQ18ResN = 8 };
 
+// First and last order and customer keys (the density checks' inputs) into
+// pinned host memory in one launch; zeros for an empty table.
+__global__ void q18_ends_k(const int32_t* __restrict__ okey, int64_t n_o,
+                           const int32_t* __restrict__ ckey, int64_t n_c, int32_t* out) {
+    if (threadIdx.x == 0) out[0] = n_o > 0 ? okey[0] : 0;
+    if (threadIdx.x == 1) out[1] = n_o > 0 ? okey[n_o - 1] : 0;
+    if (threadIdx.x == 2) out[2] = n_c > 0 ? ckey[0] : 0;
+    if (threadIdx.x == 3) out[3] = n_c > 0 ? ckey[n_c - 1] : 0;
+}
+
 __global__ __launch_bounds__(kT) void q18_init_k(int64_t* res) {
     if (threadIdx.x < kQ18ResN) res[threadIdx.x] = threadIdx.x < 3 ? -1 : 0;
 }
@@ -1375,17 +1385,9 @@ int bolap_q18(bolap_ctx* ctx, const bolap_q18_args* a, bolap_q18_row* rows_out,
         TraceScope ts(ctx, "q18_bounds");
         BOLAP_TRY(pinned_reserve(ctx, 4 * sizeof(int32_t)));
         int32_t* pe = static_cast<int32_t*>(ctx->pinned);
-        if (a->order_rows > 0) {
-            BOLAP_HIP_TRY(hipMemcpyAsync(pe, a->o_orderkey, 4, hipMemcpyDeviceToHost, ctx->stream));
-            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 1, a->o_orderkey + a->order_rows - 1, 4,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-        }
-        if (a->cust_rows > 0) {
-            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 2, a->c_custkey, 4, hipMemcpyDeviceToHost,
-                                         ctx->stream));
-            BOLAP_HIP_TRY(hipMemcpyAsync(pe + 3, a->c_custkey + a->cust_rows - 1, 4,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-        }
+        hipLaunchKernelGGL(q18_ends_k, dim3(1), dim3(kWave), 0, ctx->stream, a->o_orderkey,
+                           a->order_rows, a->c_custkey, a->cust_rows, pe);
+        BOLAP_TRY(launch_check("q18_ends_k"));
         BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
         std::memcpy(ends, pe, sizeof(ends));
     }
