@@ -67,6 +67,48 @@ def build_lib(jobs: int | None = None) -> pathlib.Path:
     return LIB
 
 
+DRIVER_SRC = PKG / "driver" / "tpch_query.cpp"
+DRIVER = PKG / "bolap_tpch"
+
+
+def _arrow_dir() -> pathlib.Path | None:
+    """pyarrow's bundled Arrow C++ (headers and libarrow/libparquet), if present."""
+    try:
+        import pyarrow
+    except ImportError:
+        return None
+    d = pathlib.Path(pyarrow.__file__).resolve().parent
+    if (d / "include" / "parquet" / "arrow" / "reader.h").exists() and \
+            list(d.glob("libparquet.so.*")):
+        return d
+    return None
+
+
+def build_driver() -> pathlib.Path | None:
+    """The TPC-H load/build/query driver (host C++ over libbolap and Arrow);
+    skipped when Arrow C++ is not installed."""
+    arrow = _arrow_dir()
+    if arrow is None:
+        print("bolap_tpch skipped: no Arrow C++ (pyarrow) in this environment")
+        return None
+    libs = sorted(arrow.glob("libarrow.so.*")) + sorted(arrow.glob("libparquet.so.*"))
+    arrow_so = next(x.name for x in libs if x.name.startswith("libarrow.so.") and
+                    x.name.count(".") == 2)
+    parquet_so = next(x.name for x in libs if x.name.startswith("libparquet.so.") and
+                      x.name.count(".") == 2)
+    tmp = DRIVER.with_suffix(".tmp")
+    cmd = [HIPCC, "-x", "c++", "-std=c++20", "-O2", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", f"-I{ROOT / 'include'}", f"-I{arrow / 'include'}",
+           str(DRIVER_SRC), "-o", str(tmp), f"-L{PKG}", "-lbolap", "-Wl,-rpath,$ORIGIN",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", f"-L{arrow}",
+           f"-l:{arrow_so}", f"-l:{parquet_so}", f"-Wl,-rpath,{arrow}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"bolap_tpch build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_oracle() -> pathlib.Path:
     """Compiles the C oracle (test infrastructure) into oracle/liboracle.so."""
     cc = shutil.which("gcc") or "cc"
@@ -84,6 +126,7 @@ def build_oracle() -> pathlib.Path:
 def build_all() -> None:
     build_oracle()
     build_lib()
+    build_driver()
 
 
 if __name__ == "__main__":

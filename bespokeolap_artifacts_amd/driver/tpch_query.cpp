@@ -1,0 +1,320 @@
+// tpch_query.cpp — the reference's load() / build() / query() sequence for the
+// lineitem queries Q1 and Q6, with the kernels of libbolap doing the scans.
+//
+//   bolap_tpch <dir/>  <  requests
+//
+// load   reads <dir>lineitem.parquet with Arrow (bespoke_tpch/loader_impl.cpp:9-22;
+//        the path is a directory prefix, :12).
+// build  converts the columns Q1 and Q6 read into the reference Database
+//        layout (bespoke_tpch/builder_impl.cpp:447-577): decimals through
+//        FormatValue + stod + llround(x * scale) (:287-338), dates as int16 day
+//        offsets from 1992-01-01 (:226-257), returnflag / linestatus as
+//        first-appearance dictionary codes (:204-224); uploads them and derives
+//        discounted_price and the returnflag_linestatus code on the device
+//        (bolap_derive_lineitem).
+// query  reads one request per line until an empty line (query_impl.cpp:142-157),
+//        `1 DELTA` or `6 DATE DISCOUNT QUANTITY` with std::quoted tokens
+//        (args_parser.hpp:82-216), runs them in order, writes resultN.csv and
+//        prints "N | Execution ms: T" (query_impl.cpp:161-169); other query ids
+//        are reported on stderr and skipped.
+#include <arrow/api.h>
+#include <arrow/io/file.h>
+#include <parquet/arrow/reader.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bolap.h"
+
+namespace {
+
+constexpr int32_t kBaseDays = 8035;  // 1992-01-01 as days since 1970-01-01
+
+void check(int status) {
+    if (status != BOLAP_OK) throw std::runtime_error(bolap_last_error());
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+std::shared_ptr<arrow::Table> read_parquet(const std::string& path) {
+    auto file = arrow::io::ReadableFile::Open(path);
+    if (!file.ok()) throw std::runtime_error(path + ": " + file.status().ToString());
+    auto reader = parquet::arrow::OpenFile(*file, arrow::default_memory_pool());
+    if (!reader.ok()) throw std::runtime_error(path + ": " + reader.status().ToString());
+    auto table = (*reader)->ReadTable();
+    if (!table.ok()) throw std::runtime_error(path + ": " + table.status().ToString());
+    return *table;
+}
+
+std::shared_ptr<arrow::ChunkedArray> column(const arrow::Table& t, const char* name) {
+    auto c = t.GetColumnByName(name);
+    if (!c) throw std::runtime_error(std::string("lineitem has no column ") + name);
+    return c;
+}
+
+// Fixed-point integers as the reference builds them: decimals through their
+// text and a double, llround(value * scale); doubles and integers scaled the
+// same way; nulls as 0.
+std::vector<int64_t> scaled(const arrow::ChunkedArray& col, int64_t scale) {
+    std::vector<int64_t> out;
+    out.reserve(size_t(col.length()));
+    for (const auto& chunk : col.chunks()) {
+        const int64_t n = chunk->length();
+        switch (chunk->type_id()) {
+            case arrow::Type::DECIMAL128: {
+                auto a = std::static_pointer_cast<arrow::Decimal128Array>(chunk);
+                for (int64_t i = 0; i < n; ++i)
+                    out.push_back(a->IsNull(i) ? 0
+                                               : std::llround(std::stod(a->FormatValue(i)) *
+                                                              double(scale)));
+                break;
+            }
+            case arrow::Type::DOUBLE: {
+                auto a = std::static_pointer_cast<arrow::DoubleArray>(chunk);
+                for (int64_t i = 0; i < n; ++i)
+                    out.push_back(a->IsNull(i) ? 0 : std::llround(a->Value(i) * double(scale)));
+                break;
+            }
+            case arrow::Type::INT64: {
+                auto a = std::static_pointer_cast<arrow::Int64Array>(chunk);
+                for (int64_t i = 0; i < n; ++i) out.push_back(a->IsNull(i) ? 0 : a->Value(i) * scale);
+                break;
+            }
+            case arrow::Type::INT32: {
+                auto a = std::static_pointer_cast<arrow::Int32Array>(chunk);
+                for (int64_t i = 0; i < n; ++i)
+                    out.push_back(a->IsNull(i) ? 0 : int64_t(a->Value(i)) * scale);
+                break;
+            }
+            default:
+                throw std::runtime_error("unsupported numeric column type " +
+                                         chunk->type()->ToString());
+        }
+    }
+    return out;
+}
+
+std::vector<int16_t> date_offsets(const arrow::ChunkedArray& col) {
+    std::vector<int16_t> out;
+    out.reserve(size_t(col.length()));
+    for (const auto& chunk : col.chunks()) {
+        if (chunk->type_id() != arrow::Type::DATE32)
+            throw std::runtime_error("unsupported date type " + chunk->type()->ToString());
+        auto a = std::static_pointer_cast<arrow::Date32Array>(chunk);
+        for (int64_t i = 0; i < a->length(); ++i)
+            out.push_back(a->IsNull(i) ? int16_t(0) : int16_t(a->Value(i) - kBaseDays));
+    }
+    return out;
+}
+
+struct Dict {
+    std::vector<uint8_t> codes;
+    std::vector<std::string> values;
+};
+
+Dict dictionary(const arrow::ChunkedArray& col) {
+    Dict d;
+    std::unordered_map<std::string, uint8_t> index;
+    for (const auto& chunk : col.chunks()) {
+        for (int64_t i = 0; i < chunk->length(); ++i) {
+            std::string v;
+            if (chunk->type_id() == arrow::Type::STRING)
+                v = std::static_pointer_cast<arrow::StringArray>(chunk)->GetString(i);
+            else if (chunk->type_id() == arrow::Type::LARGE_STRING)
+                v = std::static_pointer_cast<arrow::LargeStringArray>(chunk)->GetString(i);
+            else
+                throw std::runtime_error("unsupported string type " + chunk->type()->ToString());
+            auto it = index.find(v);
+            if (it == index.end()) {
+                if (d.values.size() >= 255) throw std::runtime_error("dictionary beyond 255 values");
+                it = index.emplace(v, uint8_t(d.values.size())).first;
+                d.values.push_back(v);
+            }
+            d.codes.push_back(it->second);
+        }
+    }
+    return d;
+}
+
+template <typename T>
+T* upload(const std::vector<T>& v) {
+    T* p = nullptr;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, v.size()) * sizeof(T)),
+              "hipMalloc");
+    if (!v.empty())
+        hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+    return p;
+}
+
+// The device mirror of the lineitem columns Q1 and Q6 read.
+struct Lineitem {
+    int64_t rows = 0;
+    int16_t *ship = nullptr, *qty = nullptr;
+    int32_t *price = nullptr, *disc_price = nullptr;
+    uint8_t *disc = nullptr, *tax = nullptr, *code = nullptr;
+    std::vector<std::string> rf, ls;
+};
+
+Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
+    Lineitem li;
+    li.rows = t.num_rows();
+    const auto qty = scaled(*column(t, "l_quantity"), 100);
+    const auto price = scaled(*column(t, "l_extendedprice"), 100);
+    const auto disc = scaled(*column(t, "l_discount"), 100);
+    const auto tax = scaled(*column(t, "l_tax"), 100);
+    std::vector<int16_t> h_qty(qty.begin(), qty.end());
+    std::vector<int32_t> h_price(price.begin(), price.end());
+    std::vector<uint8_t> h_disc(disc.begin(), disc.end()), h_tax(tax.begin(), tax.end());
+    Dict rf = dictionary(*column(t, "l_returnflag"));
+    Dict ls = dictionary(*column(t, "l_linestatus"));
+    if (rf.values.size() * std::max<size_t>(1, ls.values.size()) > BOLAP_Q1_MAX_GROUPS)
+        throw std::runtime_error("more returnflag x linestatus groups than Q1 supports");
+    li.ship = upload(date_offsets(*column(t, "l_shipdate")));
+    li.qty = upload(h_qty);
+    li.price = upload(h_price);
+    li.disc = upload(h_disc);
+    li.tax = upload(h_tax);
+    uint8_t* rf_dev = upload(rf.codes);
+    uint8_t* ls_dev = upload(ls.codes);
+    li.disc_price = upload(std::vector<int32_t>(size_t(li.rows)));
+    li.code = upload(std::vector<uint8_t>(size_t(li.rows)));
+    int32_t sorted = 0;
+    check(bolap_derive_lineitem(ctx, li.rows, li.price, li.disc, rf_dev, ls_dev,
+                                int32_t(ls.values.size()), nullptr, li.disc_price, li.code,
+                                &sorted));
+    hip_check(hipDeviceSynchronize(), "derive");
+    hip_check(hipFree(rf_dev), "hipFree");
+    hip_check(hipFree(ls_dev), "hipFree");
+    li.rf = rf.values;
+    li.ls = ls.values;
+    return li;
+}
+
+int64_t days_from_civil(int64_t y, unsigned m, unsigned d) {  // Hinnant's algorithm
+    y -= m <= 2;
+    const int64_t era = (y >= 0 ? y : y - 399) / 400;
+    const unsigned yoe = unsigned(y - era * 400);
+    const unsigned doy = (153 * (m + (m > 2 ? -3 : 9)) + 2) / 5 + d - 1;
+    const unsigned doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+    return era * 146097 + int64_t(doe) - 719468;
+}
+
+void run_q1(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const std::string& csv) {
+    std::istringstream iss(line);
+    std::string qid, delta;
+    if (!(iss >> qid) || !(iss >> std::quoted(delta)))
+        throw std::runtime_error("Q1: failed to parse DELTA");
+    const int32_t cutoff =
+        int32_t(days_from_civil(1998, 12, 1) - std::stoi(delta)) - kBaseDays;  // query_q1.cpp:111-115
+    bolap_q1_args a{};
+    a.layout = BOLAP_Q1_LAYOUT_REF;
+    a.rows = li.rows;
+    a.ship = li.ship;
+    a.ship_lo = INT16_MIN;
+    a.ship_hi = int16_t(cutoff);
+    a.qty = li.qty;
+    a.price = li.price;
+    a.disc_price = li.disc_price;
+    a.disc = li.disc;
+    a.tax = li.tax;
+    a.flag_a = li.code;
+    a.n_a = int32_t(li.rf.size());
+    a.n_b = int32_t(li.ls.size());
+    std::vector<bolap_q1_group> g(size_t(a.n_a * a.n_b));
+    check(bolap_q1_scan_agg(ctx, &a, g.data()));
+    std::vector<const char*> rf, ls;
+    for (const auto& s : li.rf) rf.push_back(s.c_str());
+    for (const auto& s : li.ls) ls.push_back(s.c_str());
+    std::vector<bolap_q1_row> rows(g.size());
+    int32_t n = 0;
+    check(bolap_q1_finalize(g.data(), BOLAP_Q1_LAYOUT_REF, a.n_a, a.n_b, rf.data(), ls.data(),
+                            rows.data(), &n));
+    check(bolap_q1_write_csv(csv.c_str(), rows.data(), n));
+}
+
+void run_q6(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const std::string& csv) {
+    std::istringstream iss(line);
+    std::string qid, date, discount, quantity;
+    if (!(iss >> qid) || !(iss >> std::quoted(date)) || !(iss >> std::quoted(discount)) ||
+        !(iss >> std::quoted(quantity)))
+        throw std::runtime_error("Q6: failed to parse DATE DISCOUNT QUANTITY");
+    bolap_q6_params p;
+    check(bolap_q6_parse(date.c_str(), discount.c_str(), quantity.c_str(), &p));
+    bolap_q6_args a{};
+    a.rows = li.rows;
+    a.ship = li.ship;
+    a.disc = li.disc;
+    a.qty = li.qty;
+    a.price = li.price;
+    a.ship_lo = p.ship_lo;
+    a.ship_hi = p.ship_hi;
+    a.disc_lo = p.disc_lo;
+    a.disc_hi = p.disc_hi;
+    a.qty_lt = p.qty_lt;
+    a.zones = nullptr;
+    bolap_q6_result r;
+    check(bolap_q6(ctx, &a, &r));
+    check(bolap_q6_write_csv(csv.c_str(), &r));
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 2) {
+        std::cerr << "usage: bolap_tpch <data directory/> < requests\n";
+        return 2;
+    }
+    try {
+        bolap_ctx* ctx = nullptr;
+        check(bolap_ctx_create(0, nullptr, &ctx));
+        const std::string dir = argv[1];
+        const auto table = read_parquet(dir + "lineitem.parquet");  // load
+        const Lineitem li = build(ctx, *table);                       // build
+        std::vector<std::pair<std::string, std::string>> requests;   // query
+        std::string line;
+        while (std::getline(std::cin, line)) {
+            if (line.empty()) break;
+            std::istringstream iss(line);
+            std::string id;
+            if (!(iss >> id)) continue;
+            requests.emplace_back(id, line);
+        }
+        for (size_t i = 0; i < requests.size(); ++i) {
+            const auto& [id, text] = requests[i];
+            const std::string csv = "result" + std::to_string(i + 1) + ".csv";
+            const auto start = std::chrono::steady_clock::now();
+            if (id == "1") {
+                run_q1(ctx, li, text, csv);
+            } else if (id == "6") {
+                run_q6(ctx, li, text, csv);
+            } else {
+                std::cerr << "query " << id << " is not served by this driver\n";
+                continue;
+            }
+            const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                                std::chrono::steady_clock::now() - start)
+                                .count();
+            std::cout << (i + 1) << " | Execution ms: " << ms << "\n";
+        }
+        check(bolap_ctx_destroy(ctx));
+    } catch (const std::exception& e) {
+        std::cerr << "bolap_tpch: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}
