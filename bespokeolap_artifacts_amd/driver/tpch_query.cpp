@@ -1,28 +1,32 @@
-// tpch_query.cpp — the reference's load() / build() / query() sequence for the
-// lineitem queries Q1 and Q6, with the kernels of libbolap doing the scans.
+// tpch_query.cpp — the reference's load() / build() / query() sequence for
+// TPC-H Q1, Q6, Q14, Q17 and Q18, with the kernels of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
 //
-// load   reads <dir>lineitem.parquet with Arrow (bespoke_tpch/loader_impl.cpp:9-22;
-//        the path is a directory prefix, :12).
-// build  converts the columns Q1 and Q6 read into the reference Database
-//        layout (bespoke_tpch/builder_impl.cpp:447-577): decimals through
+// load   reads <dir>lineitem.parquet and, when present, part, orders and
+//        customer .parquet with Arrow (bespoke_tpch/loader_impl.cpp:9-22; the
+//        path is a directory prefix, :12).
+// build  converts the columns these queries read into the reference Database
+//        layout (bespoke_tpch/builder_impl.cpp:447-999): decimals through
 //        FormatValue + stod + llround(x * scale) (:287-338), dates as int16 day
 //        offsets from 1992-01-01 (:226-257), returnflag / linestatus as
-//        first-appearance dictionary codes (:204-224); uploads them and derives
-//        discounted_price and the returnflag_linestatus code on the device
-//        (bolap_derive_lineitem).
-// query  reads one request per line until an empty line (query_impl.cpp:142-157),
-//        `1 DELTA` or `6 DATE DISCOUNT QUANTITY` with std::quoted tokens
-//        (args_parser.hpp:82-216), runs them in order, writes resultN.csv and
-//        prints "N | Execution ms: T" (query_impl.cpp:161-169); other query ids
-//        are reported on stderr and skipped.
+//        first-appearance dictionary codes (:204-224), part's promo flag from
+//        its type strings (query_q14.cpp:96-100, 200-208); uploads them and
+//        derives discounted_price and the returnflag_linestatus code on the
+//        device (bolap_derive_lineitem).
+// query  reads one request per line until an empty line (query_impl.cpp:142-157):
+//        `1 DELTA`, `6 DATE DISCOUNT QUANTITY`, `14 DATE`, `17 BRAND CONTAINER`,
+//        `18 QUANTITY` with std::quoted tokens (args_parser.hpp), runs them in
+//        order, writes resultN.csv and prints "N | Execution ms: T"
+//        (query_impl.cpp:161-169); other query ids are reported on stderr and
+//        skipped.
 #include <arrow/api.h>
 #include <arrow/io/file.h>
 #include <parquet/arrow/reader.h>
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -62,7 +66,7 @@ std::shared_ptr<arrow::Table> read_parquet(const std::string& path) {
 
 std::shared_ptr<arrow::ChunkedArray> column(const arrow::Table& t, const char* name) {
     auto c = t.GetColumnByName(name);
-    if (!c) throw std::runtime_error(std::string("lineitem has no column ") + name);
+    if (!c) throw std::runtime_error(std::string("no column ") + name);
     return c;
 }
 
@@ -108,6 +112,33 @@ std::vector<int64_t> scaled(const arrow::ChunkedArray& col, int64_t scale) {
     return out;
 }
 
+std::vector<int32_t> int32s(const arrow::ChunkedArray& col) {
+    const auto v = scaled(col, 1);
+    std::vector<int32_t> out(v.size());
+    for (size_t i = 0; i < v.size(); ++i) out[i] = int32_t(v[i]);
+    return out;
+}
+
+std::vector<std::string> strings(const arrow::ChunkedArray& col) {
+    std::vector<std::string> out;
+    out.reserve(size_t(col.length()));
+    for (const auto& chunk : col.chunks()) {
+        for (int64_t i = 0; i < chunk->length(); ++i) {
+            if (chunk->IsNull(i)) {
+                out.emplace_back();
+            } else if (chunk->type_id() == arrow::Type::STRING) {
+                out.push_back(std::static_pointer_cast<arrow::StringArray>(chunk)->GetString(i));
+            } else if (chunk->type_id() == arrow::Type::LARGE_STRING) {
+                out.push_back(
+                    std::static_pointer_cast<arrow::LargeStringArray>(chunk)->GetString(i));
+            } else {
+                throw std::runtime_error("unsupported string type " + chunk->type()->ToString());
+            }
+        }
+    }
+    return out;
+}
+
 std::vector<int16_t> date_offsets(const arrow::ChunkedArray& col) {
     std::vector<int16_t> out;
     out.reserve(size_t(col.length()));
@@ -129,23 +160,14 @@ struct Dict {
 Dict dictionary(const arrow::ChunkedArray& col) {
     Dict d;
     std::unordered_map<std::string, uint8_t> index;
-    for (const auto& chunk : col.chunks()) {
-        for (int64_t i = 0; i < chunk->length(); ++i) {
-            std::string v;
-            if (chunk->type_id() == arrow::Type::STRING)
-                v = std::static_pointer_cast<arrow::StringArray>(chunk)->GetString(i);
-            else if (chunk->type_id() == arrow::Type::LARGE_STRING)
-                v = std::static_pointer_cast<arrow::LargeStringArray>(chunk)->GetString(i);
-            else
-                throw std::runtime_error("unsupported string type " + chunk->type()->ToString());
-            auto it = index.find(v);
-            if (it == index.end()) {
-                if (d.values.size() >= 255) throw std::runtime_error("dictionary beyond 255 values");
-                it = index.emplace(v, uint8_t(d.values.size())).first;
-                d.values.push_back(v);
-            }
-            d.codes.push_back(it->second);
+    for (const auto& v : strings(col)) {
+        auto it = index.find(v);
+        if (it == index.end()) {
+            if (d.values.size() >= 256) throw std::runtime_error("dictionary beyond 256 values");
+            it = index.emplace(v, uint8_t(d.values.size())).first;
+            d.values.push_back(v);
         }
+        d.codes.push_back(it->second);
     }
     return d;
 }
@@ -161,13 +183,33 @@ T* upload(const std::vector<T>& v) {
     return p;
 }
 
-// The device mirror of the lineitem columns Q1 and Q6 read.
+// The device mirror of the columns the served queries read.
 struct Lineitem {
     int64_t rows = 0;
     int16_t *ship = nullptr, *qty = nullptr;
-    int32_t *price = nullptr, *disc_price = nullptr;
+    int32_t *price = nullptr, *disc_price = nullptr, *orderkey = nullptr, *partkey = nullptr;
     uint8_t *disc = nullptr, *tax = nullptr, *code = nullptr;
+    int32_t orderkey_sorted = 0;
     std::vector<std::string> rf, ls;
+};
+
+struct Part {
+    int64_t rows = -1;  // -1: no part table
+    int32_t* partkey = nullptr;
+    uint8_t *brand = nullptr, *container = nullptr, *promo = nullptr;
+    std::vector<std::string> brands, containers;
+};
+
+struct Orders {
+    int64_t rows = -1;
+    int32_t *orderkey = nullptr, *custkey = nullptr, *totalprice = nullptr;
+    int16_t* orderdate = nullptr;
+};
+
+struct Customer {
+    int64_t rows = -1;
+    int32_t* custkey = nullptr;
+    std::vector<std::string> name;
 };
 
 Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
@@ -189,20 +231,62 @@ Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
     li.price = upload(h_price);
     li.disc = upload(h_disc);
     li.tax = upload(h_tax);
+    li.orderkey = upload(int32s(*column(t, "l_orderkey")));
+    li.partkey = upload(int32s(*column(t, "l_partkey")));
     uint8_t* rf_dev = upload(rf.codes);
     uint8_t* ls_dev = upload(ls.codes);
     li.disc_price = upload(std::vector<int32_t>(size_t(li.rows)));
     li.code = upload(std::vector<uint8_t>(size_t(li.rows)));
-    int32_t sorted = 0;
     check(bolap_derive_lineitem(ctx, li.rows, li.price, li.disc, rf_dev, ls_dev,
-                                int32_t(ls.values.size()), nullptr, li.disc_price, li.code,
-                                &sorted));
+                                int32_t(ls.values.size()), li.orderkey, li.disc_price, li.code,
+                                &li.orderkey_sorted));
     hip_check(hipDeviceSynchronize(), "derive");
     hip_check(hipFree(rf_dev), "hipFree");
     hip_check(hipFree(ls_dev), "hipFree");
     li.rf = rf.values;
     li.ls = ls.values;
     return li;
+}
+
+Part build_part(const arrow::Table& t) {
+    Part p;
+    p.rows = t.num_rows();
+    p.partkey = upload(int32s(*column(t, "p_partkey")));
+    Dict brand = dictionary(*column(t, "p_brand"));
+    Dict container = dictionary(*column(t, "p_container"));
+    std::vector<uint8_t> promo;
+    for (const auto& type : strings(*column(t, "p_type")))
+        promo.push_back(type.compare(0, 5, "PROMO") == 0 ? 1 : 0);  // query_q14.cpp:96-100
+    p.brand = upload(brand.codes);
+    p.container = upload(container.codes);
+    p.promo = upload(promo);
+    p.brands = brand.values;
+    p.containers = container.values;
+    return p;
+}
+
+Orders build_orders(const arrow::Table& t) {
+    Orders o;
+    o.rows = t.num_rows();
+    o.orderkey = upload(int32s(*column(t, "o_orderkey")));
+    o.custkey = upload(int32s(*column(t, "o_custkey")));
+    o.orderdate = upload(date_offsets(*column(t, "o_orderdate")));
+    const auto total = scaled(*column(t, "o_totalprice"), 100);
+    o.totalprice = upload(std::vector<int32_t>(total.begin(), total.end()));
+    return o;
+}
+
+Customer build_customer(const arrow::Table& t) {
+    Customer c;
+    c.rows = t.num_rows();
+    c.custkey = upload(int32s(*column(t, "c_custkey")));
+    c.name = strings(*column(t, "c_name"));
+    return c;
+}
+
+bool exists(const std::string& path) {
+    auto f = arrow::io::ReadableFile::Open(path);
+    return f.ok();
 }
 
 int64_t days_from_civil(int64_t y, unsigned m, unsigned d) {  // Hinnant's algorithm
@@ -272,6 +356,101 @@ void run_q6(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const s
     check(bolap_q6_write_csv(csv.c_str(), &r));
 }
 
+std::vector<std::string> quoted_args(const std::string& line, int n, const char* what) {
+    std::istringstream iss(line);
+    std::string qid;
+    std::vector<std::string> v(static_cast<size_t>(n));
+    bool ok = bool(iss >> qid);
+    for (int i = 0; ok && i < n; ++i) ok = bool(iss >> std::quoted(v[size_t(i)]));
+    if (!ok) throw std::runtime_error(std::string(what) + ": failed to parse its arguments");
+    return v;
+}
+
+void run_q14(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line,
+             const std::string& csv) {
+    if (part.rows < 0) throw std::runtime_error("Q14 needs part.parquet");
+    const auto v = quoted_args(line, 1, "Q14");
+    int32_t lo = 0, hi = 0;
+    check(bolap_q14_parse(v[0].c_str(), &lo, &hi));
+    bolap_q14_args a{};
+    a.li_rows = li.rows;
+    a.l_ship = li.ship;
+    a.l_partkey = li.partkey;
+    a.l_disc_price = li.disc_price;
+    a.ship_lo = lo;
+    a.ship_hi = hi;
+    a.zones = nullptr;
+    a.part_rows = part.rows;
+    a.p_partkey = part.partkey;
+    a.p_promo = part.promo;
+    bolap_q14_result r;
+    check(bolap_q14(ctx, &a, &r));
+    check(bolap_q14_write_csv(csv.c_str(), &r));
+}
+
+void run_q17(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line,
+             const std::string& csv) {
+    if (part.rows < 0) throw std::runtime_error("Q17 needs part.parquet");
+    const auto v = quoted_args(line, 2, "Q17");
+    bolap_q17_result r{};
+    // a null argument or a string outside the dictionary gives the zero row
+    // (query_q17.cpp:129-160)
+    const auto b = std::find(part.brands.begin(), part.brands.end(), v[0]);
+    const auto c = std::find(part.containers.begin(), part.containers.end(), v[1]);
+    if (v[0] != "<<NULL>>" && v[1] != "<<NULL>>" && b != part.brands.end() &&
+        c != part.containers.end()) {
+        bolap_q17_args a{};
+        a.part_rows = part.rows;
+        a.p_partkey = part.partkey;
+        a.n_part_preds = 2;
+        const int64_t bc = b - part.brands.begin(), cc = c - part.containers.begin();
+        a.part_preds[0] = bolap_pred{part.brand, BOLAP_U8, BOLAP_OP_RANGE, bc, bc, {}, nullptr, 0};
+        a.part_preds[1] =
+            bolap_pred{part.container, BOLAP_U8, BOLAP_OP_RANGE, cc, cc, {}, nullptr, 0};
+        a.li_rows = li.rows;
+        a.l_partkey = li.partkey;
+        a.l_quantity = li.qty;
+        a.l_price = li.price;
+        check(bolap_q17(ctx, &a, &r));
+    }
+    check(bolap_q17_write_csv(csv.c_str(), &r));
+}
+
+void run_q18(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+             const std::string& line, const std::string& csv) {
+    if (o.rows < 0 || c.rows < 0) throw std::runtime_error("Q18 needs orders and customer");
+    const auto v = quoted_args(line, 1, "Q18");
+    std::vector<bolap_q18_row> rows;
+    if (v[0] != "<<NULL>>") {  // query_q18.cpp:194-200: no rows
+        bolap_q18_args a{};
+        a.li_rows = li.rows;
+        a.l_orderkey = li.orderkey;
+        a.l_quantity = li.qty;
+        a.orderkey_sorted = li.orderkey_sorted;
+        a.order_rows = o.rows;
+        a.o_orderkey = o.orderkey;
+        a.o_custkey = o.custkey;
+        a.o_orderdate = o.orderdate;
+        a.o_totalprice = o.totalprice;
+        a.cust_rows = c.rows;
+        a.c_custkey = c.custkey;
+        a.quantity_threshold = int32_t(std::llround(std::stod(v[0]) * 100.0));  // :44-57
+        bolap_q18_stats st;
+        rows.resize(4096);
+        int s = bolap_q18(ctx, &a, rows.data(), int64_t(rows.size()), &st);
+        if (s != BOLAP_OK && st.rows > int64_t(rows.size())) {
+            rows.resize(size_t(st.rows));
+            s = bolap_q18(ctx, &a, rows.data(), int64_t(rows.size()), &st);
+        }
+        check(s);
+        rows.resize(size_t(st.rows));
+    }
+    std::vector<const char*> names;
+    for (const auto& n : c.name) names.push_back(n.c_str());
+    check(bolap_q18_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), names.data(),
+                              int64_t(names.size())));
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -283,8 +462,16 @@ int main(int argc, char** argv) {
         bolap_ctx* ctx = nullptr;
         check(bolap_ctx_create(0, nullptr, &ctx));
         const std::string dir = argv[1];
-        const auto table = read_parquet(dir + "lineitem.parquet");  // load
-        const Lineitem li = build(ctx, *table);                       // build
+        // load + build: lineitem, and the other tables when their files exist
+        const Lineitem li = build(ctx, *read_parquet(dir + "lineitem.parquet"));
+        Part part;
+        Orders orders;
+        Customer customer;
+        if (exists(dir + "part.parquet")) part = build_part(*read_parquet(dir + "part.parquet"));
+        if (exists(dir + "orders.parquet"))
+            orders = build_orders(*read_parquet(dir + "orders.parquet"));
+        if (exists(dir + "customer.parquet"))
+            customer = build_customer(*read_parquet(dir + "customer.parquet"));
         std::vector<std::pair<std::string, std::string>> requests;   // query
         std::string line;
         while (std::getline(std::cin, line)) {
@@ -302,6 +489,12 @@ int main(int argc, char** argv) {
                 run_q1(ctx, li, text, csv);
             } else if (id == "6") {
                 run_q6(ctx, li, text, csv);
+            } else if (id == "14") {
+                run_q14(ctx, li, part, text, csv);
+            } else if (id == "17") {
+                run_q17(ctx, li, part, text, csv);
+            } else if (id == "18") {
+                run_q18(ctx, li, orders, customer, text, csv);
             } else {
                 std::cerr << "query " << id << " is not served by this driver\n";
                 continue;

@@ -21,10 +21,20 @@ pytestmark = pytest.mark.gpu
 BASE = datetime.date(1992, 1, 1)
 
 
-def lineitem(seed: int, n: int):
-    """Columns in the reference's integer units plus the Parquet table."""
+BRANDS = [f"Brand#{i}{j}" for i in range(1, 6) for j in range(1, 6)]
+CONTAINERS = [f"{a} {b}" for a in ("SM", "MED", "LG", "JUMBO", "WRAP")
+              for b in ("CASE", "BOX", "BAG", "JAR", "PKG", "PACK", "CAN", "DRUM")]
+
+
+def tables(seed: int, orders: int, parts: int = 2000, custs: int = 300):
+    """A small TPC-H-shaped database (orders 1..n with 1-7 lineitems each, in
+    order-key order) as Parquet tables, and its columns in the reference's
+    integer units."""
     pa = pytest.importorskip("pyarrow")
     rng = np.random.default_rng(seed)
+    per = rng.integers(1, 8, orders) if orders > 1 else np.array([1])
+    okey = np.repeat(np.arange(1, orders + 1), per)
+    n = len(okey)
     qty = rng.integers(1, 51, n)                                  # units
     price = qty * rng.integers(90_000, 200_001, n) // 100         # cents
     disc = rng.integers(0, 11, n)                                 # hundredths
@@ -34,22 +44,46 @@ def lineitem(seed: int, n: int):
     late = ship > 1260
     rf = np.where(late, "N", np.where(rng.random(n) < 0.5, "R", "A"))
     ls = np.where(late & (rng.random(n) < 0.97), "O", "F")
+    pkey = rng.integers(1, parts + 1, n)
     D = decimal.Decimal
-    t = pa.table({
-        "l_orderkey": pa.array(np.arange(1, n + 1, dtype=np.int32)),
+    dec = lambda cents: pa.array([D(int(c)).scaleb(-2) for c in cents], pa.decimal128(15, 2))
+    day = lambda offs: pa.array([BASE + datetime.timedelta(days=int(s)) for s in offs],
+                                pa.date32())
+    li = pa.table({
+        "l_orderkey": pa.array(okey.astype(np.int32)), "l_partkey": pa.array(pkey.astype(np.int32)),
         "l_quantity": pa.array([D(int(q)) for q in qty], pa.decimal128(15, 2)),
-        "l_extendedprice": pa.array([D(int(p)).scaleb(-2) for p in price],
-                                    pa.decimal128(15, 2)),
-        "l_discount": pa.array([D(int(d)).scaleb(-2) for d in disc], pa.decimal128(15, 2)),
-        "l_tax": pa.array([D(int(x)).scaleb(-2) for x in tax], pa.decimal128(15, 2)),
-        "l_returnflag": pa.array(rf.tolist()),
-        "l_linestatus": pa.array(ls.tolist()),
-        "l_shipdate": pa.array([BASE + datetime.timedelta(days=int(s)) for s in ship],
-                               pa.date32()),
-    })
+        "l_extendedprice": dec(price), "l_discount": dec(disc), "l_tax": dec(tax),
+        "l_returnflag": pa.array(rf.tolist()), "l_linestatus": pa.array(ls.tolist()),
+        "l_shipdate": day(ship)})
+    brand = rng.integers(0, len(BRANDS), parts)
+    cont = rng.integers(0, len(CONTAINERS), parts)
+    promo = rng.random(parts) < 0.25
+    ptype = [("PROMO " if p else "STANDARD ") + "ANODIZED TIN" for p in promo]
+    part = pa.table({"p_partkey": pa.array(np.arange(1, parts + 1, dtype=np.int32)),
+                     "p_brand": pa.array([BRANDS[i] for i in brand]),
+                     "p_container": pa.array([CONTAINERS[i] for i in cont]),
+                     "p_type": pa.array(ptype)})
+    ocust = rng.integers(1, custs + 1, orders)
+    oday = rng.integers(0, 2406, orders)
+    ototal = rng.integers(100_000, 50_000_000, orders)            # cents
+    ords = pa.table({"o_orderkey": pa.array(np.arange(1, orders + 1, dtype=np.int32)),
+                     "o_custkey": pa.array(ocust.astype(np.int32)),
+                     "o_orderdate": day(oday), "o_totalprice": dec(ototal)})
+    names = [f"Customer#{i:09d}" for i in range(1, custs + 1)]
+    names[0], names[1] = 'He said "hi"', "plain,comma"
+    cust = pa.table({"c_custkey": pa.array(np.arange(1, custs + 1, dtype=np.int32)),
+                     "c_name": pa.array(names)})
     cols = {"qty": qty * 100, "price": price, "disc": disc, "tax": tax, "ship": ship,
-            "rf": rf, "ls": ls}
-    return t, cols
+            "rf": rf, "ls": ls, "okey": okey, "pkey": pkey, "brand": brand, "cont": cont,
+            "promo": promo, "ocust": ocust, "oday": oday, "ototal": ototal, "names": names}
+    return {"lineitem": li, "part": part, "orders": ords, "customer": cust}, cols
+
+
+def write(tabs, d: pathlib.Path):
+    pq = pytest.importorskip("pyarrow.parquet")
+    d.mkdir()
+    for name, t in tabs.items():
+        pq.write_table(t, d / f"{name}.parquet")
 
 
 def first_appearance(values):
@@ -80,31 +114,93 @@ def want_q6(c, date: str, discount: str, quantity: str) -> str:
     return f"revenue\n{rev / 10000.0:.2f}\n"
 
 
-@pytest.mark.parametrize("n", [1, 100_003])
-def test_driver_answers_q1_and_q6_like_the_reference(tmp_path, n):
+def want_q14(c, date: str) -> str:
+    lo, hi = Engine.q14_window(date)
+    m = (c["ship"] >= lo) & (c["ship"] < hi)
+    dp = c["price"] * (100 - c["disc"]) // 100
+    total = int(dp[m].sum())
+    promo = int(dp[m & c["promo"][c["pkey"] - 1]].sum())
+    return f"promo_revenue\n{(100.0 * promo / total if total > 0 else 0.0):.2f}\n"
+
+
+def want_q17(c, brand: str, container: str) -> str:
+    s = 0
+    if brand in BRANDS and container in CONTAINERS:
+        ok = (c["brand"] == BRANDS.index(brand)) & (c["cont"] == CONTAINERS.index(container))
+        for p in np.nonzero(ok)[0] + 1:
+            rows = c["pkey"] == p
+            q = c["qty"][rows]
+            below = q * 5 * len(q) < q.sum()
+            s += int(c["price"][rows][below].sum())
+    return f"avg_yearly\n{s / 700.0:.2f}\n"
+
+
+def want_q18(c, quantity: str) -> str:
+    out = ["c_name,c_custkey,o_orderkey,o_orderdate,o_totalprice,sum(l_quantity)\n"]
+    if quantity == "<<NULL>>":
+        return "".join(out)
+    thr = round(float(quantity) * 100)
+    sums = np.bincount(c["okey"], weights=c["qty"]).astype(np.int64)
+    rows = []
+    for k in np.nonzero(sums > thr)[0]:
+        o = k - 1
+        rows.append((-int(c["ototal"][o]), int(c["oday"][o]), int(k)))
+    for neg_total, day, k in sorted(rows):
+        cust = int(c["ocust"][k - 1])
+        name = c["names"][cust - 1].replace('"', '\\"')
+        d = (BASE + datetime.timedelta(days=day)).isoformat()
+        out.append(f'"{name}",{cust},{k},{d},{-neg_total / 100.0:.2f},'
+                   f'{int(sums[k]) / 100.0:.2f}\n')
+    return "".join(out)
+
+
+def run_driver(data: pathlib.Path, run: pathlib.Path, requests):
+    run.mkdir()
+    return subprocess.run([str(B.DRIVER), str(data) + "/"], input="\n".join(requests) + "\n\n",
+                          capture_output=True, text=True, cwd=run, timeout=120)
+
+
+@pytest.mark.parametrize("orders", [1, 25_001])
+def test_driver_answers_like_the_reference(tmp_path, orders):
     if not B.DRIVER.exists():
         pytest.skip("bolap_tpch not built (no Arrow C++)")
-    pq = pytest.importorskip("pyarrow.parquet")
-    table, cols = lineitem(11 + n, n)
-    data = tmp_path / "data"
-    data.mkdir()
-    pq.write_table(table, data / "lineitem.parquet")
+    tabs, cols = tables(11 + orders, orders)
+    write(tabs, tmp_path / "data")
+    b0, c0 = BRANDS[int(cols["brand"][0])], CONTAINERS[int(cols["cont"][0])]
     requests = ['1 90', '6 "1994-01-01" "0.06" "24"', '9 "GREEN"', '1 "60"',
-                '6 1995-01-01 0.05 25']
-    run = tmp_path / "run"
-    run.mkdir()
-    r = subprocess.run([str(B.DRIVER), str(data) + "/"], input="\n".join(requests) + "\n\n",
-                       capture_output=True, text=True, cwd=run, timeout=120)
+                '6 1995-01-01 0.05 25', '14 "1995-09-01"', f'17 "{b0}" "{c0}"',
+                '17 "Brand#99" "SM BOX"', '18 250', '18 "<<NULL>>"', '14 1996-02-29']
+    r = run_driver(tmp_path / "data", tmp_path / "run", requests)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
-    assert [ln.split(" | ")[0] for ln in lines] == ["1", "2", "4", "5"]
+    served = [str(i + 1) for i, q in enumerate(requests) if not q.startswith("9 ")]
+    assert [ln.split(" | ")[0] for ln in lines] == served
     assert all(ln.split(" | ")[1].startswith("Execution ms: ") for ln in lines)
     assert "query 9 is not served" in r.stderr
-    assert (run / "result1.csv").read_text() == want_q1(cols, 90)
-    assert (run / "result2.csv").read_text() == want_q6(cols, "1994-01-01", "0.06", "24")
+    run = tmp_path / "run"
     assert not (run / "result3.csv").exists()
-    assert (run / "result4.csv").read_text() == want_q1(cols, 60)
-    assert (run / "result5.csv").read_text() == want_q6(cols, "1995-01-01", "0.05", "25")
+    want = [want_q1(cols, 90), want_q6(cols, "1994-01-01", "0.06", "24"), None,
+            want_q1(cols, 60), want_q6(cols, "1995-01-01", "0.05", "25"),
+            want_q14(cols, "1995-09-01"), want_q17(cols, b0, c0),
+            want_q17(cols, "Brand#99", "SM BOX"), want_q18(cols, "250"),
+            want_q18(cols, "<<NULL>>"), want_q14(cols, "1996-02-29")]
+    for i, w in enumerate(want):
+        if w is not None:
+            assert (run / f"result{i + 1}.csv").read_text() == w, requests[i]
+    if orders > 1:
+        assert want[8].count("\n") > 2  # some orders pass the Q18 threshold
+
+
+def test_driver_lineitem_only_serves_q1_q6_and_rejects_q14(tmp_path):
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    tabs, cols = tables(5, 3000)
+    write({"lineitem": tabs["lineitem"]}, tmp_path / "data")
+    r = run_driver(tmp_path / "data", tmp_path / "ok", ["1 90", "6 1994-01-01 0.06 24"])
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "ok" / "result1.csv").read_text() == want_q1(cols, 90)
+    r = run_driver(tmp_path / "data", tmp_path / "bad", ["14 1995-09-01"])
+    assert r.returncode == 1 and "part.parquet" in r.stderr
 
 
 def test_driver_reports_a_missing_file(tmp_path):
