@@ -51,19 +51,44 @@ BOLAP_DEV uint32_t bias(int32_t v) { return uint32_t(v) ^ 0x80000000u; }
 // per workgroup and value (a grid of at most a few hundred workgroups).
 __global__ __launch_bounds__(kT) void so_range_k(const int32_t* __restrict__ p,
                                                  const int32_t* __restrict__ s, int64_t n,
-                                                 unsigned* __restrict__ mm) {
+                                                 bool vec, unsigned* __restrict__ mm) {
     __shared__ unsigned s_v[4][kT / kWave];
     unsigned v[4] = {~0u, 0u, ~0u, 0u};  // biased: min p, max p, min s, max s
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kT) {
-        const unsigned a = bias(p[i]);
+    auto fold_p = [&](int32_t x) {
+        const unsigned a = bias(x);
         v[0] = a < v[0] ? a : v[0];
         v[1] = a > v[1] ? a : v[1];
+    };
+    auto fold_s = [&](int32_t x) {
+        const unsigned b = bias(x);
+        v[2] = b < v[2] ? b : v[2];
+        v[3] = b > v[3] ? b : v[3];
+    };
+    // four rows per lane per 16-byte load (vec: both columns 16-byte
+    // aligned), two units in flight; the tail (or unaligned columns) row by row
+    const int64_t units = vec ? n / 4 : 0;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += 2 * stride) {
+        const int64_t u2 = u + stride < units ? u + stride : u;
+        const VecT<int32_t, 4> a1 = load_unit<int32_t, 4>(p, u), a2 = load_unit<int32_t, 4>(p, u2);
+        VecT<int32_t, 4> b1{}, b2{};
         if (s) {
-            const unsigned b = bias(s[i]);
-            v[2] = b < v[2] ? b : v[2];
-            v[3] = b > v[3] ? b : v[3];
+            b1 = load_unit<int32_t, 4>(s, u);
+            b2 = load_unit<int32_t, 4>(s, u2);
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            fold_p(a1.v[j]);
+            fold_p(a2.v[j]);
+            if (s) {
+                fold_s(b1.v[j]);
+                fold_s(b2.v[j]);
+            }
+        }
+    }
+    for (int64_t i = units * 4 + int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += stride) {
+        fold_p(p[i]);
+        if (s) fold_s(s[i]);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -199,8 +224,10 @@ int bolap_sort_order_i32(bolap_ctx* ctx, const int32_t* primary, const int32_t* 
     uint64_t* keys = ws.at<uint64_t>(o_keys);
     unsigned* mm = static_cast<unsigned*>(ctx->result_dev);
     hipLaunchKernelGGL(so_range_init_k, dim3(1), dim3(kWave), 0, ctx->stream, mm);
-    hipLaunchKernelGGL(so_range_k, dim3(std::min<unsigned>(grid_of(ctx, rows), unsigned(ctx->num_cus))),
-                       dim3(kT), 0, ctx->stream, primary, secondary, rows, mm);
+    hipLaunchKernelGGL(so_range_k, dim3(std::min<unsigned>(grid_of(ctx, rows / 4 + 1),
+                                                            unsigned(ctx->num_cus) * 2)),
+                       dim3(kT), 0, ctx->stream, primary, secondary, rows,
+                       aligned(primary, 16) && (!secondary || aligned(secondary, 16)), mm);
     BOLAP_TRY(launch_check("so_range_k"));
     unsigned h[4];
     BOLAP_HIP_TRY(hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

@@ -65,3 +65,15 @@ def test_gather_rejects_out_of_range_order(engine):
     order = torch.tensor([0, 3, 10], device="cuda", dtype=torch.int32)
     with pytest.raises(N.BolapError):
         engine.gather_rows(src, order)
+
+
+def test_sort_order_unaligned_views(engine):
+    """Columns starting one row in (not 16-byte aligned) and a ragged length:
+    the min/max pass reads them row by row."""
+    rng = np.random.default_rng(21)
+    p = torch.from_numpy(rng.integers(-9, 9, 100_003).astype(np.int32)).cuda()
+    s = torch.from_numpy(rng.integers(0, 3, 100_003).astype(np.int32)).cuda()
+    got = engine.sort_order(p[1:], s[1:]).cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(got, O.sort_order_i32(p[1:].cpu().numpy(), s[1:].cpu().numpy()))
+    got1 = engine.sort_order(p[3:]).cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(got1, O.sort_order_i32(p[3:].cpu().numpy()))
