@@ -316,8 +316,9 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
                                                    const TP* __restrict__ price,
                                                    const TD* __restrict__ disc,
                                                    const int32_t* __restrict__ dprice,
-                                                   int64_t n, Ranges r, GTable t,
-                                                   int64_t* st) {
+                                                   const int16_t* __restrict__ ship,
+                                                   int32_t ship_gt, int64_t n, Ranges r,
+                                                   GTable t, int64_t* st) {
     const int64_t ospan = r.omax - r.omin + 1;
     const int64_t n_units = n / 4;
     uint64_t matched = 0;
@@ -334,6 +335,7 @@ __global__ __launch_bounds__(kT) void fact_probe_k(const int32_t* __restrict__ f
         for (int j = 0; j < cnt; ++j) {
             const int32_t key = kv[j];
             if (!test_bit(t.obits, int64_t(key) - r.omin, ospan)) continue;
+            if (ship && ship[row0 + j] <= ship_gt) continue;  // run_q3's shipdate test
             ++matched;
             const uint64_t v = uint64_t(row_value(row0 + j));
             if (cur_slot != 0xFFFFFFFFu && key == cur_key) {
@@ -413,6 +415,8 @@ struct RangeArgs {
     int64_t n;
     int32_t day_lt;
     int64_t chunk;
+    const int16_t* ship;  // optional: sum only fact rows with ship > ship_gt
+    int32_t ship_gt;
 };
 
 // Appends `cnt` entries staged in LDS to `out` (one atomic), then empties
@@ -519,18 +523,21 @@ __global__ __launch_bounds__(kT) void range_groups_k(RangeArgs a, const TP* __re
         for (int q = 0; q < 4; ++q) {
             if (e[q] <= s[q]) continue;
             uint64_t acc = 0;
+            uint32_t took = 0;
             for (uint32_t r0 = s[q]; r0 < e[q]; r0 += 8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const uint32_t row = r0 + k;
-                    if (row < e[q]) {
+                    if (row < e[q] && (!a.ship || a.ship[row] > a.ship_gt)) {
                         acc += dprice ? uint64_t(int64_t(dprice[row]))
                                       : uint64_t(int64_t(price[row])) *
                                             uint64_t(100 - int64_t(disc[row]));
+                        ++took;
                     }
                 }
             }
-            matched += e[q] - s[q];
+            if (took == 0) continue;  // no row shipped after the date: no group
+            matched += took;
             const int64_t rev = int64_t(acc);
             rl = rev < rl ? rev : rl;
             rh = rev > rh ? rev : rh;
@@ -817,19 +824,19 @@ int chain_groups(bolap_ctx* ctx, const bolap_chain_args* a, int64_t n_expect, bo
         if (p64 && d64)
             hipLaunchKernelGGL((fact_probe_k<int64_t, int64_t>), dim3(g), dim3(kT), 0, ctx->stream,
                                a->fact_order, static_cast<const int64_t*>(a->fact_price),
-                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_ship, a->ship_gt, a->fact_rows, r, t, st);
         else if (p64)
             hipLaunchKernelGGL((fact_probe_k<int64_t, uint8_t>), dim3(g), dim3(kT), 0, ctx->stream,
                                a->fact_order, static_cast<const int64_t*>(a->fact_price),
-                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_ship, a->ship_gt, a->fact_rows, r, t, st);
         else if (d64)
             hipLaunchKernelGGL((fact_probe_k<int32_t, int64_t>), dim3(g), dim3(kT), 0, ctx->stream,
                                a->fact_order, static_cast<const int32_t*>(a->fact_price),
-                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+                               static_cast<const int64_t*>(a->fact_disc), dp, a->fact_ship, a->ship_gt, a->fact_rows, r, t, st);
         else
             hipLaunchKernelGGL((fact_probe_k<int32_t, uint8_t>), dim3(g), dim3(kT), 0, ctx->stream,
                                a->fact_order, static_cast<const int32_t*>(a->fact_price),
-                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_rows, r, t, st);
+                               static_cast<const uint8_t*>(a->fact_disc), dp, a->fact_ship, a->ship_gt, a->fact_rows, r, t, st);
     }
     BOLAP_TRY(launch_check("fact_probe_k"));
     auto* n_rows = ws.at<unsigned long long>(o_cnt);
@@ -914,7 +921,8 @@ int chain_groups_ranges(bolap_ctx* ctx, const bolap_chain_args* a, WsArena& ws,
     BOLAP_TRY(launch_check("cust_bits_k"));
     if (blocks > 0) {
         const RangeArgs ra{a->order_key, a->order_day, a->order_cust, a->order_fact_start,
-                           a->order_fact_end, a->order_rows, a->day_lt, chunk};
+                           a->order_fact_end, a->order_rows, a->day_lt, chunk,
+                           a->fact_ship, a->ship_gt};
         const uint32_t* cb = ws.at<uint32_t>(o_cb);
         uint32_t* pass = ws.at<uint32_t>(o_pass);
         {

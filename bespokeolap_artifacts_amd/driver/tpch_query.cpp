@@ -1,5 +1,5 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q6, Q14, Q17 and Q18, with the kernels of libbolap doing the work.
+// TPC-H Q1, Q3, Q6, Q14, Q17 and Q18, with the kernels of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
 //
@@ -15,7 +15,8 @@
 //        derives discounted_price and the returnflag_linestatus code on the
 //        device (bolap_derive_lineitem).
 // query  reads one request per line until an empty line (query_impl.cpp:142-157):
-//        `1 DELTA`, `6 DATE DISCOUNT QUANTITY`, `14 DATE`, `17 BRAND CONTAINER`,
+//        `1 DELTA`, `3 SEGMENT DATE`, `6 DATE DISCOUNT QUANTITY`, `14 DATE`,
+//        `17 BRAND CONTAINER`,
 //        `18 QUANTITY` with std::quoted tokens (args_parser.hpp), runs them in
 //        order, writes resultN.csv and prints "N | Execution ms: T"
 //        (query_impl.cpp:161-169); other query ids are reported on stderr and
@@ -204,12 +205,15 @@ struct Orders {
     int64_t rows = -1;
     int32_t *orderkey = nullptr, *custkey = nullptr, *totalprice = nullptr;
     int16_t* orderdate = nullptr;
+    int32_t* orderdate32 = nullptr;                       // Q3's order-day column
+    std::unordered_map<int32_t, int32_t> shippriority;   // by order key (Q3's output)
 };
 
 struct Customer {
     int64_t rows = -1;
     int32_t* custkey = nullptr;
-    std::vector<std::string> name;
+    uint8_t* segment = nullptr;  // c_mktsegment codes (null without the column)
+    std::vector<std::string> segments, name;
 };
 
 Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
@@ -270,9 +274,16 @@ Orders build_orders(const arrow::Table& t) {
     o.rows = t.num_rows();
     o.orderkey = upload(int32s(*column(t, "o_orderkey")));
     o.custkey = upload(int32s(*column(t, "o_custkey")));
-    o.orderdate = upload(date_offsets(*column(t, "o_orderdate")));
+    const auto days = date_offsets(*column(t, "o_orderdate"));
+    o.orderdate = upload(days);
+    o.orderdate32 = upload(std::vector<int32_t>(days.begin(), days.end()));
     const auto total = scaled(*column(t, "o_totalprice"), 100);
     o.totalprice = upload(std::vector<int32_t>(total.begin(), total.end()));
+    if (t.GetColumnByName("o_shippriority")) {
+        const auto keys = int32s(*column(t, "o_orderkey"));
+        const auto prio = int32s(*column(t, "o_shippriority"));
+        for (size_t i = 0; i < keys.size(); ++i) o.shippriority[keys[i]] = prio[i];
+    }
     return o;
 }
 
@@ -281,6 +292,11 @@ Customer build_customer(const arrow::Table& t) {
     c.rows = t.num_rows();
     c.custkey = upload(int32s(*column(t, "c_custkey")));
     c.name = strings(*column(t, "c_name"));
+    if (t.GetColumnByName("c_mktsegment")) {
+        Dict seg = dictionary(*column(t, "c_mktsegment"));
+        c.segment = upload(seg.codes);
+        c.segments = seg.values;
+    }
     return c;
 }
 
@@ -364,6 +380,53 @@ std::vector<std::string> quoted_args(const std::string& line, int n, const char*
     for (int i = 0; ok && i < n; ++i) ok = bool(iss >> std::quoted(v[size_t(i)]));
     if (!ok) throw std::runtime_error(std::string(what) + ": failed to parse its arguments");
     return v;
+}
+
+int16_t parse_date_offset(const std::string& v) {  // query_q3.cpp:40-47
+    const int year = std::stoi(v.substr(0, 4));
+    const int month = std::stoi(v.substr(5, 2));
+    const int day = std::stoi(v.substr(8, 2));
+    return int16_t(days_from_civil(year, unsigned(month), unsigned(day)) - kBaseDays);
+}
+
+void run_q3(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+            const std::string& line, const std::string& csv) {
+    if (o.rows < 0 || c.rows < 0 || !c.segment)
+        throw std::runtime_error("Q3 needs orders and customer (with c_mktsegment)");
+    const auto v = quoted_args(line, 2, "Q3");
+    const int16_t date = parse_date_offset(v[1]);
+    std::vector<bolap_group_row> rows;
+    const auto seg = std::find(c.segments.begin(), c.segments.end(), v[0]);
+    if (seg != c.segments.end()) {  // an unknown segment matches no customer
+        bolap_chain_args a{};
+        a.cust_rows = c.rows;
+        a.cust_key = c.custkey;
+        a.cust_segment = c.segment;
+        a.segment = int32_t(seg - c.segments.begin());
+        a.order_rows = o.rows;
+        a.order_key = o.orderkey;
+        a.order_day = o.orderdate32;
+        a.order_cust = o.custkey;
+        a.day_lt = date;             // o_orderdate < DATE
+        a.fact_rows = li.rows;
+        a.fact_order = li.orderkey;
+        a.fact_disc_price = li.disc_price;
+        a.price_dtype = BOLAP_I32;
+        a.disc_dtype = BOLAP_U8;
+        a.fact_ship = li.ship;       // l_shipdate > DATE
+        a.ship_gt = date;
+        rows.resize(size_t(std::max<int64_t>(1, o.rows)));
+        int64_t n = 0;
+        bolap_chain_stats st;
+        check(bolap_chain_groups_sorted(ctx, &a, rows.data(), int64_t(rows.size()), &n, &st));
+        rows.resize(size_t(n));
+    }
+    std::vector<int32_t> prio(rows.size(), 0);
+    for (size_t i = 0; i < rows.size(); ++i) {
+        const auto it = o.shippriority.find(rows[i].key);
+        if (it != o.shippriority.end()) prio[i] = it->second;
+    }
+    check(bolap_q3_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), prio.data()));
 }
 
 void run_q14(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line,
@@ -487,6 +550,8 @@ int main(int argc, char** argv) {
             const auto start = std::chrono::steady_clock::now();
             if (id == "1") {
                 run_q1(ctx, li, text, csv);
+            } else if (id == "3") {
+                run_q3(ctx, li, orders, customer, text, csv);
             } else if (id == "6") {
                 run_q6(ctx, li, text, csv);
             } else if (id == "14") {

@@ -250,7 +250,8 @@ class Engine:
 
     # ------------------------------------------------- chain join + group-by
     def chain_args(self, t: dict, segment: int, day_lt: int, capacity_hint: int = 0,
-                   disc_price=None, ranges=None, key_ranges=None) -> N.ChainArgs:
+                   disc_price=None, ranges=None, key_ranges=None, fact_ship=None,
+                   ship_gt: int = 0) -> N.ChainArgs:
         """`ranges` = (start, end) device uint32/int32 arrays per order row
         (order_ranges): the fact side is then read by ranges.  `key_ranges` =
         (cust_min, cust_max, order_min, order_max) from the tables' statistics:
@@ -273,6 +274,8 @@ class Engine:
         a.capacity_hint = int(capacity_hint)
         if ranges is not None:
             a.order_fact_start, a.order_fact_end = ptr(ranges[0]), ptr(ranges[1])
+        if fact_ship is not None:  # sum only fact rows with fact_ship > ship_gt
+            a.fact_ship, a.ship_gt = ptr(fact_ship), int(ship_gt)
         if key_ranges is not None:
             a.key_ranges_known = 1
             (a.cust_key_min, a.cust_key_max, a.order_key_min,

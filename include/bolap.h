@@ -302,6 +302,12 @@ typedef struct {
     int32_t key_ranges_known;
     int32_t cust_key_min, cust_key_max;
     int32_t order_key_min, order_key_max;
+    /* Optional fact-side filter, run_q3's l_shipdate > DATE (query_q3.cpp:
+       236-241): with fact_ship set (int16 day offsets), only fact rows with
+       fact_ship > ship_gt are summed and counted, and an order forms a group
+       when one of its rows passes. */
+    const int16_t* fact_ship;
+    int32_t ship_gt;
 } bolap_chain_args;
 
 typedef struct {

@@ -73,7 +73,7 @@ def lib() -> C.CDLL:
                                       i32, C.POINTER(JoinSumOut)]
         L.oracle_chain_groups.restype = i64
         L.oracle_chain_groups.argtypes = [i64, vp, vp, i32, i64, vp, vp, vp, i32, i64, vp,
-                                          vp, i32, vp, i32, vp, vp, i64,
+                                          vp, i32, vp, i32, vp, vp, i32, vp, i64,
                                           C.POINTER(ChainStats)]
         L.oracle_sort_groups.argtypes = [vp, i64]
         L.oracle_radix_sort_u64.argtypes = [vp, vp, i64, i32]
@@ -204,16 +204,18 @@ def join_sum(dim_key, preds: list, fact_fk, fact_price, fact_disc) -> dict:
     return {"revenue": out.revenue, "build_rows": out.build_rows, "matches": out.matches}
 
 
-def chain_groups(t: dict, segment: int, day_lt: int, disc_price=None):
+def chain_groups(t: dict, segment: int, day_lt: int, disc_price=None, fact_ship=None,
+                 ship_gt: int = 0):
     n_orders = len(t["order_key"])
     out = np.empty(max(1, n_orders), GROUP_DTYPE)
     st = ChainStats()
+    ship = None if fact_ship is None else np.ascontiguousarray(fact_ship, np.int16)
     n = lib().oracle_chain_groups(
         len(t["cust_key"]), _p(t["cust_key"]), _p(t["cust_segment"]), segment, n_orders,
         _p(t["order_key"]), _p(t["order_day"]), _p(t["order_cust"]), day_lt,
         len(t["fact_order"]), _p(t["fact_order"]), _p(t["fact_price"]),
         dtype_code(t["fact_price"]), _p(t["fact_disc"]), dtype_code(t["fact_disc"]),
-        _p(disc_price), _p(out), len(out), C.byref(st))
+        _p(disc_price), _p(ship), ship_gt, _p(out), len(out), C.byref(st))
     stats = {"cust_pass": st.cust_pass, "order_pass": st.order_pass,
              "fact_matches": st.fact_matches, "groups": st.groups}
     return out[:n].copy(), stats

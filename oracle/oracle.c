@@ -286,8 +286,9 @@ int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
                             const int32_t* order_cust, int32_t day_lt, int64_t fact_rows,
                             const int32_t* fact_order, const void* fact_price,
                             int32_t price_dtype, const void* fact_disc, int32_t disc_dtype,
-                            const int32_t* fact_disc_price, oracle_group_row* out,
-                            int64_t max_out, oracle_chain_stats* stats) {
+                            const int32_t* fact_disc_price, const int16_t* fact_ship,
+                            int32_t ship_gt, oracle_group_row* out, int64_t max_out,
+                            oracle_chain_stats* stats) {
     /* Key-indexed arrays as in the reference, offset by the smallest key. */
     int64_t cmin = 0, cmax = -1;
     for (int64_t i = 0; i < cust_rows; ++i) {
@@ -327,6 +328,8 @@ int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
         if (k < omin || k > omax) continue;
         const int32_t o = row_of[k - omin];
         if (o < 0) continue;
+        /* run_q3 sums only rows shipped after the date (query_q3.cpp:236-241) */
+        if (fact_ship && fact_ship[i] <= ship_gt) continue;
         int64_t v;
         if (fact_disc_price) {
             v = fact_disc_price[i];

@@ -94,7 +94,7 @@ int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
                             const int32_t* order_cust, int32_t day_lt, int64_t fact_rows,
                             const int32_t* fact_order, const void* fact_price,
                             int32_t price_dtype, const void* fact_disc, int32_t disc_dtype,
-                            const int32_t* fact_disc_price, oracle_group_row* out,
+                            const int32_t* fact_disc_price, const int16_t* fact_ship, int32_t ship_gt, oracle_group_row* out,
                             int64_t max_out, oracle_chain_stats* stats);
 void oracle_sort_groups(oracle_group_row* rows, int64_t n);
 

@@ -282,3 +282,30 @@ def test_chain_key_outside_given_range_is_an_error(engine, which):
     want, _ = want_sorted(t, 1, 1155)
     allg, _ = engine.chain_groups_sorted(engine.chain_args(t, 1, 1155), len(want) + 5)
     assert np.array_equal(allg, want)
+
+
+@pytest.mark.parametrize("path", ["direct", "hashed", "ranges"])
+@pytest.mark.parametrize("ship_gt", [1300, 2525])
+def test_chain_ship_filter_matches_oracle(engine, path, ship_gt):
+    """The fact-side l_shipdate > DATE filter of run_q3 on every path."""
+    import torch
+    t = engine.gen_chain(SEED, 20_000, 200_000)
+    n = int(t["fact_order"].numel())
+    ship_h = np.random.default_rng(ship_gt).integers(0, 2526, n).astype(np.int16)
+    ship = torch.from_numpy(ship_h).cuda()
+    rows, st = oracle.chain_groups(host(t), 2, 1500, fact_ship=ship_h, ship_gt=ship_gt)
+    want = oracle.sort_groups(rows)
+    kw = {"fact_ship": ship, "ship_gt": ship_gt}
+    if path == "ranges":
+        _, start, end = engine.order_ranges(t["order_key"], t["fact_order"],
+                                            int(t["order_key"].max().item()) + 1)
+        kw["ranges"] = (start, end)
+    engine.tune_set("chain.direct", 2 if path == "hashed" else 1)
+    try:
+        allg, gs = engine.chain_groups_sorted(engine.chain_args(t, 2, 1500, **kw), len(want) + 5)
+        top, _ = engine.chain_topk(engine.chain_args(t, 2, 1500, **kw), 10)
+    finally:
+        engine.tune_set("chain.direct", 0)
+    assert np.array_equal(allg, want)
+    assert np.array_equal(as_array(top), want[:10])
+    check_stats(gs, st)

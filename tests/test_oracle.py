@@ -354,3 +354,23 @@ def test_mutation_radix_digit_shift_is_caught():
     oracle.set_mutation(5)
     k, v = oracle.radix_sort_u64(keys, vals, 64)
     assert not np.array_equal(v, vals[np.argsort(keys, kind="stable")])
+
+
+@pytest.mark.parametrize("ship_gt", [0, 1300, 2525])
+def test_chain_groups_ship_filter_matches_pandas(ship_gt):
+    """run_q3's l_shipdate > DATE on the fact side: only rows shipped after
+    the date are summed, and an order without such a row forms no group."""
+    t = oracle.gen_chain(SEED, 3000, 30_000)
+    ship = np.random.default_rng(ship_gt).integers(0, 2526, len(t["fact_order"])).astype(np.int16)
+    got, stats = oracle.chain_groups(t, 1, 1155, fact_ship=ship, ship_gt=ship_gt)
+    keep = ship > ship_gt
+    tf = dict(t)
+    for k in ("fact_order", "fact_price", "fact_disc"):
+        tf[k] = t[k][keep]
+    want = chain_pandas(tf, 1, 1155)
+    assert np.array_equal(got["key"], want["key"].to_numpy())
+    assert np.array_equal(got["revenue"], want["v"].to_numpy())
+    assert stats["groups"] == len(want)
+    assert stats["fact_matches"] == int(np.isin(tf["fact_order"], want["key"].to_numpy()).sum())
+    if ship_gt == 2525:
+        assert len(got) == 0

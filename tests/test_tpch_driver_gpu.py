@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 BASE = datetime.date(1992, 1, 1)
 
 
+SEGMENTS = ["BUILDING", "AUTOMOBILE", "MACHINERY", "HOUSEHOLD", "FURNITURE"]
 BRANDS = [f"Brand#{i}{j}" for i in range(1, 6) for j in range(1, 6)]
 CONTAINERS = [f"{a} {b}" for a in ("SM", "MED", "LG", "JUMBO", "WRAP")
               for b in ("CASE", "BOX", "BAG", "JAR", "PKG", "PACK", "CAN", "DRUM")]
@@ -66,16 +67,21 @@ def tables(seed: int, orders: int, parts: int = 2000, custs: int = 300):
     ocust = rng.integers(1, custs + 1, orders)
     oday = rng.integers(0, 2406, orders)
     ototal = rng.integers(100_000, 50_000_000, orders)            # cents
+    oprio = rng.integers(0, 2, orders)
     ords = pa.table({"o_orderkey": pa.array(np.arange(1, orders + 1, dtype=np.int32)),
                      "o_custkey": pa.array(ocust.astype(np.int32)),
-                     "o_orderdate": day(oday), "o_totalprice": dec(ototal)})
+                     "o_orderdate": day(oday), "o_totalprice": dec(ototal),
+                     "o_shippriority": pa.array(oprio.astype(np.int32))})
     names = [f"Customer#{i:09d}" for i in range(1, custs + 1)]
     names[0], names[1] = 'He said "hi"', "plain,comma"
+    cseg = rng.integers(0, len(SEGMENTS), custs)
     cust = pa.table({"c_custkey": pa.array(np.arange(1, custs + 1, dtype=np.int32)),
-                     "c_name": pa.array(names)})
+                     "c_name": pa.array(names),
+                     "c_mktsegment": pa.array([SEGMENTS[i] for i in cseg])})
     cols = {"qty": qty * 100, "price": price, "disc": disc, "tax": tax, "ship": ship,
             "rf": rf, "ls": ls, "okey": okey, "pkey": pkey, "brand": brand, "cont": cont,
-            "promo": promo, "ocust": ocust, "oday": oday, "ototal": ototal, "names": names}
+            "promo": promo, "ocust": ocust, "oday": oday, "ototal": ototal, "names": names,
+            "oprio": oprio, "cseg": cseg}
     return {"lineitem": li, "part": part, "orders": ords, "customer": cust}, cols
 
 
@@ -154,6 +160,24 @@ def want_q18(c, quantity: str) -> str:
     return "".join(out)
 
 
+def want_q3(c, segment: str, date: str) -> str:
+    out = ["l_orderkey,revenue,o_orderdate,o_shippriority\n"]
+    if segment not in SEGMENTS:
+        return "".join(out)
+    d = (datetime.date.fromisoformat(date) - BASE).days
+    cust_ok = c["cseg"] == SEGMENTS.index(segment)
+    order_ok = (c["oday"] < d) & cust_ok[c["ocust"] - 1]
+    row_ok = (c["ship"] > d) & order_ok[c["okey"] - 1]
+    dp = c["price"] * (100 - c["disc"]) // 100
+    rev = np.bincount(c["okey"][row_ok], weights=dp[row_ok], minlength=len(c["oday"]) + 1)
+    has = np.bincount(c["okey"][row_ok], minlength=len(c["oday"]) + 1) > 0
+    rows = sorted((-int(rev[k]), int(c["oday"][k - 1]), int(k)) for k in np.nonzero(has)[0])
+    for neg, day, k in rows:
+        out.append(f"{k},{-neg / 100.0:.2f},{(BASE + datetime.timedelta(days=day)).isoformat()},"
+                   f"{int(c['oprio'][k - 1])}\n")
+    return "".join(out)
+
+
 def run_driver(data: pathlib.Path, run: pathlib.Path, requests):
     run.mkdir()
     return subprocess.run([str(B.DRIVER), str(data) + "/"], input="\n".join(requests) + "\n\n",
@@ -169,7 +193,8 @@ def test_driver_answers_like_the_reference(tmp_path, orders):
     b0, c0 = BRANDS[int(cols["brand"][0])], CONTAINERS[int(cols["cont"][0])]
     requests = ['1 90', '6 "1994-01-01" "0.06" "24"', '9 "GREEN"', '1 "60"',
                 '6 1995-01-01 0.05 25', '14 "1995-09-01"', f'17 "{b0}" "{c0}"',
-                '17 "Brand#99" "SM BOX"', '18 250', '18 "<<NULL>>"', '14 1996-02-29']
+                '17 "Brand#99" "SM BOX"', '18 250', '18 "<<NULL>>"', '14 1996-02-29',
+                '3 "BUILDING" "1995-03-15"', '3 MACHINERY 1993-06-01', '3 "NOSUCH" "1995-03-15"']
     r = run_driver(tmp_path / "data", tmp_path / "run", requests)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
@@ -183,12 +208,15 @@ def test_driver_answers_like_the_reference(tmp_path, orders):
             want_q1(cols, 60), want_q6(cols, "1995-01-01", "0.05", "25"),
             want_q14(cols, "1995-09-01"), want_q17(cols, b0, c0),
             want_q17(cols, "Brand#99", "SM BOX"), want_q18(cols, "250"),
-            want_q18(cols, "<<NULL>>"), want_q14(cols, "1996-02-29")]
+            want_q18(cols, "<<NULL>>"), want_q14(cols, "1996-02-29"),
+            want_q3(cols, "BUILDING", "1995-03-15"), want_q3(cols, "MACHINERY", "1993-06-01"),
+            want_q3(cols, "NOSUCH", "1995-03-15")]
     for i, w in enumerate(want):
         if w is not None:
             assert (run / f"result{i + 1}.csv").read_text() == w, requests[i]
     if orders > 1:
         assert want[8].count("\n") > 2  # some orders pass the Q18 threshold
+        assert want[11].count("\n") > 100  # Q3 rows
 
 
 def test_driver_lineitem_only_serves_q1_q6_and_rejects_q14(tmp_path):
