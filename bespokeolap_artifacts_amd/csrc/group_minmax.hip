@@ -15,9 +15,14 @@
 //   count   one lane per join row: validates its group, item and range and
 //           sums the expanded rows (sizes the hashed table exactly: it can
 //           never hold more groups than expanded rows).
-//   dense   slots = group * keys + key: int32 min / max side by side under
-//           atomics and a used byte; a bitmap of used slots, a popcount scan
-//           and a rank scatter give the rows in slot order.
+//   expand  a wave takes 64 join rows and spreads their concatenated value
+//           ranges over its lanes; each expanded row reads its slot and
+//           issues an atomic min / max only when its value lies outside
+//           the (min, max) read.
+//   dense   slots = group * keys + key: int32 min / max side by side, and a
+//           used byte set only by rows without a value (a slot is used when
+//           that byte is set or min <= max); a bitmap of used slots, a
+//           popcount scan and a rank scatter give the rows in slot order.
 //   hashed  open addressing on the 64-bit slot number group * keys + key,
 //           16-byte slots {key, min, max} so that an expanded row touches one
 //           cache line (CAS insertion, linear probing); occupied slots are
@@ -129,33 +134,80 @@ __global__ __launch_bounds__(kT) void mm_init_hash_k(HSlot* __restrict__ t, int6
         t[i] = HSlot{kEmpty, INT32_MAX, INT32_MIN};
 }
 
-BOLAP_DEV void fold(int32_t* mn, int32_t* mx, bool has, int32_t v) {
-    if (!has) return;
-    atomicMin(mn, v);
-    atomicMax(mx, v);
+// Folds v into a slot's (min, max) after a plain read `cur` of the slot:
+// the minimum only falls and the maximum only rises, so a value inside a
+// (possibly stale) [cur.x, cur.y] is inside the current range too and needs
+// no atomic.
+BOLAP_DEV void fold_seen(int32_t* mn, int32_t* mx, int2 cur, int32_t v) {
+    if (v < cur.x) atomicMin(mn, v);
+    if (v > cur.y) atomicMax(mx, v);
 }
+
+// The item ranges of join rows p0 .. p0 + 63 expanded across one wave: lane
+// L takes expanded rows L, L + 64, ... of the concatenated ranges and calls
+// body(group, value, has, i) for value index i, so the lanes stay busy
+// whatever the range lengths (a lane per join row idles on short ranges
+// while one long range runs).  p0 is uniform across the wave; every lane
+// reaches each shuffle.  Malformed join rows are skipped here (mm_count_k
+// counts them).
+template <typename F>
+BOLAP_DEV void expand_wave(const Args& a, int64_t p0, F&& body) {
+    const int lane = int(threadIdx.x & (kWave - 1));
+    const int64_t p = p0 + lane;
+    int32_t g = 0, v = 0;
+    int64_t lo = 0, hi = 0;
+    bool has = false, bad = false;
+    const bool ok = p < a.pairs && pair_range(a, p, g, lo, hi, has, v, bad);
+    const int64_t c = ok ? hi - lo : 0;
+    int64_t inc = c;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int64_t t = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += t;
+    }
+    const int64_t total = __shfl(inc, kWave - 1, kWave);
+    const int64_t exc = inc - c;
+    const int hv = has ? 1 : 0;
+    for (int64_t e0 = 0; e0 < total; e0 += kWave) {
+        const int64_t e = e0 + lane;
+        // owner: the last lane whose range starts at or before e; it has a
+        // non-empty range holding e (exclusive starts never decrease)
+        int o = 0;
+        for (int step = kWave / 2; step >= 1; step >>= 1) {
+            const int cand = o + step;
+            if (__shfl(exc, cand, kWave) <= e) o = cand;
+        }
+        const int64_t o_exc = __shfl(exc, o, kWave);
+        const int64_t o_lo = __shfl(lo, o, kWave);
+        const int32_t o_g = __shfl(g, o, kWave);
+        const int32_t o_v = __shfl(v, o, kWave);
+        const int o_has = __shfl(hv, o, kWave);
+        if (e < total) body(o_g, o_v, o_has != 0, o_lo + (e - o_exc));
+    }
+}
+
+// Wave-uniform start of the wave's first batch of join rows, and the stride.
+BOLAP_DEV int64_t wave_first() { return (int64_t(blockIdx.x) * kT + threadIdx.x) & ~int64_t(kWave - 1); }
+BOLAP_DEV int64_t wave_stride() { return int64_t(gridDim.x) * kT; }
 
 __global__ __launch_bounds__(kT) void mm_dense_k(Args a, int2* __restrict__ mm,
                                                  uint8_t* __restrict__ used,
                                                  int64_t* __restrict__ st) {
     uint64_t errs = 0;
-    for (int64_t p = int64_t(blockIdx.x) * kT + threadIdx.x; p < a.pairs;
-         p += int64_t(gridDim.x) * kT) {
-        int32_t g, v;
-        int64_t lo, hi;
-        bool has, bad;
-        if (!pair_range(a, p, g, lo, hi, has, v, bad)) continue;
-        const int64_t base = int64_t(g) * a.keys;
-        for (int64_t i = lo; i < hi; ++i) {
+    for (int64_t p0 = wave_first(); p0 < a.pairs; p0 += wave_stride()) {
+        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i) {
             const int32_t k = a.rv[i];
             if (k < 0 || k >= a.keys) {
                 ++errs;
-                continue;
+                return;
             }
-            const int64_t s = base + k;
-            used[s] = 1;
-            fold(&mm[s].x, &mm[s].y, has, v);
-        }
+            const int64_t s = int64_t(g) * a.keys + k;
+            // a slot with a value is marked by min <= max; only value-less
+            // rows mark `used`
+            if (!has)
+                used[s] = 1;
+            else
+                fold_seen(&mm[s].x, &mm[s].y, mm[s], v);
+        });
     }
     errs = wave_sum(errs);
     if ((threadIdx.x & (kWave - 1)) == 0 && errs)
@@ -163,7 +215,8 @@ __global__ __launch_bounds__(kT) void mm_dense_k(Args a, int2* __restrict__ mm,
 }
 
 // Used bytes -> bitmap words and per-word popcounts; the used-slot total.
-__global__ __launch_bounds__(kT) void mm_bits_k(const uint8_t* __restrict__ used, int64_t slots,
+__global__ __launch_bounds__(kT) void mm_bits_k(const uint8_t* __restrict__ used,
+                                                const int2* __restrict__ mm, int64_t slots,
                                                 uint32_t* __restrict__ bits,
                                                 int64_t* __restrict__ cnt,
                                                 int64_t* __restrict__ st) {
@@ -174,7 +227,7 @@ __global__ __launch_bounds__(kT) void mm_bits_k(const uint8_t* __restrict__ used
         uint32_t b = 0;
         for (int i = 0; i < 32; ++i) {
             const int64_t s = w * 32 + i;
-            if (s < slots && used[s]) b |= 1u << i;
+            if (s < slots && (used[s] || mm[s].x <= mm[s].y)) b |= 1u << i;
         }
         bits[w] = b;
         cnt[w] = __popc(b);
@@ -205,34 +258,33 @@ __global__ __launch_bounds__(kT) void mm_dense_emit_k(const uint32_t* __restrict
 __global__ __launch_bounds__(kT) void mm_hash_k(Args a, HSlot* __restrict__ t, uint64_t mask,
                                                 int64_t* __restrict__ st) {
     uint64_t errs = 0;
-    for (int64_t p = int64_t(blockIdx.x) * kT + threadIdx.x; p < a.pairs;
-         p += int64_t(gridDim.x) * kT) {
-        int32_t g, v;
-        int64_t lo, hi;
-        bool has, bad;
-        if (!pair_range(a, p, g, lo, hi, has, v, bad)) continue;
-        for (int64_t i = lo; i < hi; ++i) {
+    for (int64_t p0 = wave_first(); p0 < a.pairs; p0 += wave_stride()) {
+        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i) {
             const int32_t k = a.rv[i];
             if (k < 0 || k >= a.keys) {
                 ++errs;
-                continue;
+                return;
             }
             const uint64_t key = uint64_t(g) * uint64_t(a.keys) + uint64_t(k);
             // the table holds at most half as many keys as slots: a free
             // slot is always reached within mask + 1 probes
             uint64_t s = mix64(key) & mask;
+            int2 seen = make_int2(INT32_MAX, INT32_MIN);
             for (uint64_t step = 0; step <= mask; ++step) {
-                uint64_t cur = t[s].key;
+                const HSlot cur_slot = t[s];
+                uint64_t cur = cur_slot.key;
                 if (cur == kEmpty) {
                     cur = atomicCAS(reinterpret_cast<unsigned long long*>(&t[s].key),
                                     (unsigned long long)kEmpty, (unsigned long long)key);
                     if (cur == kEmpty) cur = key;
+                } else if (cur == key) {
+                    seen = make_int2(cur_slot.mn, cur_slot.mx);
                 }
                 if (cur == key) break;
                 s = (s + 1) & mask;
             }
-            fold(&t[s].mn, &t[s].mx, has, v);
-        }
+            if (has) fold_seen(&t[s].mn, &t[s].mx, seen, v);
+        });
     }
     errs = wave_sum(errs);
     if ((threadIdx.x & (kWave - 1)) == 0 && errs)
@@ -385,7 +437,7 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
             hipLaunchKernelGGL(mm_dense_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
                                k, mm, used, st);
             hipLaunchKernelGGL(mm_bits_k, dim3(grid_of(ctx, words)), dim3(kT), 0, ctx->stream,
-                               used, slots, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), st);
+                               used, mm, slots, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), st);
             BOLAP_TRY(launch_check("mm_dense_k"));
         }
         BOLAP_TRY(read_st(ctx, h));
