@@ -557,7 +557,7 @@ def bench_c3(eng, torch, args):
     # The reference's own layout: each order's fact rows as a contiguous range
     # (lineitem_ranges, made on the device at build time, outside the timing).
     _, start, end = eng.order_ranges(t["order_key"], t["fact_order"], order_rows + 1)
-    a_rng = eng.chain_args(t, 1, 1155, ranges=(start, end), key_ranges=kr)
+    a_rng = eng.chain_args(t, 1, 1155, ranges=(start, end), key_ranges=kr, fact_order_sorted=True)
     ms_rng, tr_r, r_rng = timed(a_rng)
     _, st_r = eng.chain_topk(a_rng, 10)
     k_ms = stage_ms(tr_r, "chain_ranges")

@@ -417,6 +417,9 @@ struct RangeArgs {
     int64_t chunk;
     const int16_t* ship;  // optional: sum only fact rows with ship > ship_gt
     int32_t ship_gt;
+    // Optional: the fact rows' order keys when ranges are min/max spans over
+    // unsorted rows; a row counts for an order only when its key matches.
+    const int32_t* forder;
 };
 
 // Appends `cnt` entries staged in LDS to `out` (one atomic), then empties
@@ -528,7 +531,8 @@ __global__ __launch_bounds__(kT) void range_groups_k(RangeArgs a, const TP* __re
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const uint32_t row = r0 + k;
-                    if (row < e[q] && (!a.ship || a.ship[row] > a.ship_gt)) {
+                    if (row < e[q] && (!a.forder || a.forder[row] == key[q]) &&
+                        (!a.ship || a.ship[row] > a.ship_gt)) {
                         acc += dprice ? uint64_t(int64_t(dprice[row]))
                                       : uint64_t(int64_t(price[row])) *
                                             uint64_t(100 - int64_t(disc[row]));
@@ -922,7 +926,8 @@ int chain_groups_ranges(bolap_ctx* ctx, const bolap_chain_args* a, WsArena& ws,
     if (blocks > 0) {
         const RangeArgs ra{a->order_key, a->order_day, a->order_cust, a->order_fact_start,
                            a->order_fact_end, a->order_rows, a->day_lt, chunk,
-                           a->fact_ship, a->ship_gt};
+                           a->fact_ship, a->ship_gt,
+                           a->fact_order_sorted ? nullptr : a->fact_order};
         const uint32_t* cb = ws.at<uint32_t>(o_cb);
         uint32_t* pass = ws.at<uint32_t>(o_pass);
         {
@@ -1058,6 +1063,9 @@ int chain_validate(const bolap_chain_args* a) {
                     "bolap_chain: fact_order must be 16-byte aligned");
     BOLAP_ARG_CHECK(!a->order_fact_start || a->fact_rows < (int64_t(1) << 32),
                     "bolap_chain: fact ranges are 32-bit row numbers");
+    BOLAP_ARG_CHECK(!a->order_fact_start || a->fact_order_sorted || a->fact_rows == 0 ||
+                        a->fact_order,
+                    "bolap_chain: order ranges over unsorted fact rows need fact_order");
     BOLAP_ARG_CHECK(a->order_rows == 0 || (aligned(a->order_key, 16) && aligned(a->order_day, 16) &&
                                            aligned(a->order_cust, 16)),
                     "bolap_chain: order columns must be 16-byte aligned");

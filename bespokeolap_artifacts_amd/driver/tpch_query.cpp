@@ -32,6 +32,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <functional>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -153,24 +155,36 @@ std::vector<int16_t> date_offsets(const arrow::ChunkedArray& col) {
     return out;
 }
 
+// A dictionary column as the reference builds it (builder_impl.cpp:204-224):
+// uint16 codes in first-appearance order, at most 65,536 values.
 struct Dict {
-    std::vector<uint8_t> codes;
+    std::vector<uint16_t> codes;
     std::vector<std::string> values;
 };
 
 Dict dictionary(const arrow::ChunkedArray& col) {
     Dict d;
-    std::unordered_map<std::string, uint8_t> index;
+    std::unordered_map<std::string, uint16_t> index;
     for (const auto& v : strings(col)) {
         auto it = index.find(v);
         if (it == index.end()) {
-            if (d.values.size() >= 256) throw std::runtime_error("dictionary beyond 256 values");
-            it = index.emplace(v, uint8_t(d.values.size())).first;
+            if (d.values.size() > 0xffff)
+                throw std::runtime_error("Dictionary column exceeds uint16_t capacity");
+            it = index.emplace(v, uint16_t(d.values.size())).first;
             d.values.push_back(v);
         }
         d.codes.push_back(it->second);
     }
     return d;
+}
+
+// The codes as the byte column the kernels read; `what` names the column in
+// the error for a dictionary that does not fit a byte.
+std::vector<uint8_t> narrow_codes(const Dict& d, const char* what) {
+    if (d.values.size() > 256)
+        throw std::runtime_error(std::string(what) + ": " + std::to_string(d.values.size()) +
+                                 " dictionary values; the kernels reading it take byte codes");
+    return std::vector<uint8_t>(d.codes.begin(), d.codes.end());
 }
 
 template <typename T>
@@ -182,6 +196,20 @@ T* upload(const std::vector<T>& v) {
         hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice),
                   "hipMemcpy");
     return p;
+}
+
+// A dictionary-coded column on the device as predicates read it: byte codes
+// when the dictionary has at most 256 values, else int16 codes.
+struct CodeCol {
+    const void* dev = nullptr;
+    int32_t dtype = BOLAP_U8;
+};
+
+CodeCol upload_codes(const Dict& d, const char* what) {
+    if (d.values.size() <= 256) return {upload(narrow_codes(d, what)), BOLAP_U8};
+    if (d.values.size() > 32768)
+        throw std::runtime_error(std::string(what) + ": more than 32768 dictionary values");
+    return {upload(std::vector<int16_t>(d.codes.begin(), d.codes.end())), BOLAP_I16};
 }
 
 // The device mirror of the columns the served queries read.
@@ -197,7 +225,8 @@ struct Lineitem {
 struct Part {
     int64_t rows = -1;  // -1: no part table
     int32_t* partkey = nullptr;
-    uint8_t *brand = nullptr, *container = nullptr, *promo = nullptr;
+    CodeCol brand, container;
+    uint8_t* promo = nullptr;
     std::vector<std::string> brands, containers;
 };
 
@@ -237,8 +266,8 @@ Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
     li.tax = upload(h_tax);
     li.orderkey = upload(int32s(*column(t, "l_orderkey")));
     li.partkey = upload(int32s(*column(t, "l_partkey")));
-    uint8_t* rf_dev = upload(rf.codes);
-    uint8_t* ls_dev = upload(ls.codes);
+    uint8_t* rf_dev = upload(narrow_codes(rf, "l_returnflag"));
+    uint8_t* ls_dev = upload(narrow_codes(ls, "l_linestatus"));
     li.disc_price = upload(std::vector<int32_t>(size_t(li.rows)));
     li.code = upload(std::vector<uint8_t>(size_t(li.rows)));
     check(bolap_derive_lineitem(ctx, li.rows, li.price, li.disc, rf_dev, ls_dev,
@@ -261,8 +290,8 @@ Part build_part(const arrow::Table& t) {
     std::vector<uint8_t> promo;
     for (const auto& type : strings(*column(t, "p_type")))
         promo.push_back(type.compare(0, 5, "PROMO") == 0 ? 1 : 0);  // query_q14.cpp:96-100
-    p.brand = upload(brand.codes);
-    p.container = upload(container.codes);
+    p.brand = upload_codes(brand, "p_brand");
+    p.container = upload_codes(container, "p_container");
     p.promo = upload(promo);
     p.brands = brand.values;
     p.containers = container.values;
@@ -294,7 +323,7 @@ Customer build_customer(const arrow::Table& t) {
     c.name = strings(*column(t, "c_name"));
     if (t.GetColumnByName("c_mktsegment")) {
         Dict seg = dictionary(*column(t, "c_mktsegment"));
-        c.segment = upload(seg.codes);
+        c.segment = upload(narrow_codes(seg, "c_mktsegment"));
         c.segments = seg.values;
     }
     return c;
@@ -314,7 +343,12 @@ int64_t days_from_civil(int64_t y, unsigned m, unsigned d) {  // Hinnant's algor
     return era * 146097 + int64_t(doe) - 719468;
 }
 
-void run_q1(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const std::string& csv) {
+// Every run_qN computes its rows and returns the writer of its CSV file, so
+// that the caller stops the timer before the file is written, as the
+// reference's query() does (query_impl.cpp:161-169).
+using Writer = std::function<void(const std::string&)>;
+
+Writer run_q1(bolap_ctx* ctx, const Lineitem& li, const std::string& line) {
     std::istringstream iss(line);
     std::string qid, delta;
     if (!(iss >> qid) || !(iss >> std::quoted(delta)))
@@ -344,10 +378,13 @@ void run_q1(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const s
     int32_t n = 0;
     check(bolap_q1_finalize(g.data(), BOLAP_Q1_LAYOUT_REF, a.n_a, a.n_b, rf.data(), ls.data(),
                             rows.data(), &n));
-    check(bolap_q1_write_csv(csv.c_str(), rows.data(), n));
+    rows.resize(size_t(n));
+    return [rows](const std::string& csv) {
+        check(bolap_q1_write_csv(csv.c_str(), rows.data(), int32_t(rows.size())));
+    };
 }
 
-void run_q6(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const std::string& csv) {
+Writer run_q6(bolap_ctx* ctx, const Lineitem& li, const std::string& line) {
     std::istringstream iss(line);
     std::string qid, date, discount, quantity;
     if (!(iss >> qid) || !(iss >> std::quoted(date)) || !(iss >> std::quoted(discount)) ||
@@ -369,7 +406,7 @@ void run_q6(bolap_ctx* ctx, const Lineitem& li, const std::string& line, const s
     a.zones = nullptr;
     bolap_q6_result r;
     check(bolap_q6(ctx, &a, &r));
-    check(bolap_q6_write_csv(csv.c_str(), &r));
+    return [r](const std::string& csv) { check(bolap_q6_write_csv(csv.c_str(), &r)); };
 }
 
 std::vector<std::string> quoted_args(const std::string& line, int n, const char* what) {
@@ -389,8 +426,8 @@ int16_t parse_date_offset(const std::string& v) {  // query_q3.cpp:40-47
     return int16_t(days_from_civil(year, unsigned(month), unsigned(day)) - kBaseDays);
 }
 
-void run_q3(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
-            const std::string& line, const std::string& csv) {
+Writer run_q3(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+              const std::string& line) {
     if (o.rows < 0 || c.rows < 0 || !c.segment)
         throw std::runtime_error("Q3 needs orders and customer (with c_mktsegment)");
     const auto v = quoted_args(line, 2, "Q3");
@@ -426,11 +463,12 @@ void run_q3(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer&
         const auto it = o.shippriority.find(rows[i].key);
         if (it != o.shippriority.end()) prio[i] = it->second;
     }
-    check(bolap_q3_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), prio.data()));
+    return [rows, prio](const std::string& csv) {
+        check(bolap_q3_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), prio.data()));
+    };
 }
 
-void run_q14(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line,
-             const std::string& csv) {
+Writer run_q14(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line) {
     if (part.rows < 0) throw std::runtime_error("Q14 needs part.parquet");
     const auto v = quoted_args(line, 1, "Q14");
     int32_t lo = 0, hi = 0;
@@ -448,11 +486,10 @@ void run_q14(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::st
     a.p_promo = part.promo;
     bolap_q14_result r;
     check(bolap_q14(ctx, &a, &r));
-    check(bolap_q14_write_csv(csv.c_str(), &r));
+    return [r](const std::string& csv) { check(bolap_q14_write_csv(csv.c_str(), &r)); };
 }
 
-void run_q17(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line,
-             const std::string& csv) {
+Writer run_q17(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line) {
     if (part.rows < 0) throw std::runtime_error("Q17 needs part.parquet");
     const auto v = quoted_args(line, 2, "Q17");
     bolap_q17_result r{};
@@ -467,20 +504,21 @@ void run_q17(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::st
         a.p_partkey = part.partkey;
         a.n_part_preds = 2;
         const int64_t bc = b - part.brands.begin(), cc = c - part.containers.begin();
-        a.part_preds[0] = bolap_pred{part.brand, BOLAP_U8, BOLAP_OP_RANGE, bc, bc, {}, nullptr, 0};
-        a.part_preds[1] =
-            bolap_pred{part.container, BOLAP_U8, BOLAP_OP_RANGE, cc, cc, {}, nullptr, 0};
+        a.part_preds[0] =
+            bolap_pred{part.brand.dev, part.brand.dtype, BOLAP_OP_RANGE, bc, bc, {}, nullptr, 0};
+        a.part_preds[1] = bolap_pred{part.container.dev, part.container.dtype, BOLAP_OP_RANGE,
+                                     cc, cc, {}, nullptr, 0};
         a.li_rows = li.rows;
         a.l_partkey = li.partkey;
         a.l_quantity = li.qty;
         a.l_price = li.price;
         check(bolap_q17(ctx, &a, &r));
     }
-    check(bolap_q17_write_csv(csv.c_str(), &r));
+    return [r](const std::string& csv) { check(bolap_q17_write_csv(csv.c_str(), &r)); };
 }
 
-void run_q18(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
-             const std::string& line, const std::string& csv) {
+Writer run_q18(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+               const std::string& line) {
     if (o.rows < 0 || c.rows < 0) throw std::runtime_error("Q18 needs orders and customer");
     const auto v = quoted_args(line, 1, "Q18");
     std::vector<bolap_q18_row> rows;
@@ -498,20 +536,47 @@ void run_q18(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer
         a.cust_rows = c.rows;
         a.c_custkey = c.custkey;
         a.quantity_threshold = int32_t(std::llround(std::stod(v[0]) * 100.0));  // :44-57
-        bolap_q18_stats st;
+        bolap_q18_stats st{};
         rows.resize(4096);
         int s = bolap_q18(ctx, &a, rows.data(), int64_t(rows.size()), &st);
-        if (s != BOLAP_OK && st.rows > int64_t(rows.size())) {
+        if (s == BOLAP_ERR_CAPACITY && st.rows > int64_t(rows.size())) {
             rows.resize(size_t(st.rows));
             s = bolap_q18(ctx, &a, rows.data(), int64_t(rows.size()), &st);
         }
         check(s);
         rows.resize(size_t(st.rows));
     }
-    std::vector<const char*> names;
-    for (const auto& n : c.name) names.push_back(n.c_str());
-    check(bolap_q18_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), names.data(),
-                              int64_t(names.size())));
+    return [rows, &c](const std::string& csv) {
+        std::vector<const char*> names;
+        for (const auto& n : c.name) names.push_back(n.c_str());
+        check(bolap_q18_write_csv(csv.c_str(), rows.data(), int64_t(rows.size()), names.data(),
+                                  int64_t(names.size())));
+    };
+}
+
+// The reference's CSV quoting (query_impl.cpp:82-96): the value in double
+// quotes, an embedded double quote written as backslash + quote.
+std::string escape_csv_field(const std::string& value) {
+    std::string out = "\"";
+    for (char ch : value) {
+        if (ch == '"') out += "\\\"";
+        else out.push_back(ch);
+    }
+    out.push_back('"');
+    return out;
+}
+
+// The stub result of a request without a pipeline (query_impl.cpp:98-103).
+void write_stub_csv(const std::string& path, const std::string& id, const std::string& line) {
+    std::ofstream out(path);
+    out << "query_id,params\n";
+    out << escape_csv_field(id) << "," << escape_csv_field(line) << "\n";
+}
+
+bool is_tpch_id(const std::string& id) {
+    for (int q = 1; q <= 22; ++q)
+        if (id == std::to_string(q)) return true;
+    return false;
 }
 
 }  // namespace
@@ -548,25 +613,33 @@ int main(int argc, char** argv) {
             const auto& [id, text] = requests[i];
             const std::string csv = "result" + std::to_string(i + 1) + ".csv";
             const auto start = std::chrono::steady_clock::now();
+            Writer write;
             if (id == "1") {
-                run_q1(ctx, li, text, csv);
+                write = run_q1(ctx, li, text);
             } else if (id == "3") {
-                run_q3(ctx, li, orders, customer, text, csv);
+                write = run_q3(ctx, li, orders, customer, text);
             } else if (id == "6") {
-                run_q6(ctx, li, text, csv);
+                write = run_q6(ctx, li, text);
             } else if (id == "14") {
-                run_q14(ctx, li, part, text, csv);
+                write = run_q14(ctx, li, part, text);
             } else if (id == "17") {
-                run_q17(ctx, li, part, text, csv);
+                write = run_q17(ctx, li, part, text);
             } else if (id == "18") {
-                run_q18(ctx, li, orders, customer, text, csv);
+                write = run_q18(ctx, li, orders, customer, text);
             } else {
-                std::cerr << "query " << id << " is not served by this driver\n";
-                continue;
+                // The reference's answer to an id it has no pipeline for
+                // (query_impl.cpp:382-388): a stub CSV with the request.
+                if (is_tpch_id(id))
+                    std::cerr << "query " << id << " is not served by this driver\n";
+                write = [id = id, text = text](const std::string& path) {
+                    write_stub_csv(path, id, text);
+                };
             }
+            // The timer stops before the CSV is written (query_impl.cpp:161-169).
             const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
                                 std::chrono::steady_clock::now() - start)
                                 .count();
+            write(csv);
             std::cout << (i + 1) << " | Execution ms: " << ms << "\n";
         }
         check(bolap_ctx_destroy(ctx));

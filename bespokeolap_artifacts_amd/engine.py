@@ -251,9 +251,11 @@ class Engine:
     # ------------------------------------------------- chain join + group-by
     def chain_args(self, t: dict, segment: int, day_lt: int, capacity_hint: int = 0,
                    disc_price=None, ranges=None, key_ranges=None, fact_ship=None,
-                   ship_gt: int = 0) -> N.ChainArgs:
+                   ship_gt: int = 0, fact_order_sorted: bool = False) -> N.ChainArgs:
         """`ranges` = (start, end) device uint32/int32 arrays per order row
-        (order_ranges): the fact side is then read by ranges.  `key_ranges` =
+        (order_ranges): the fact side is then read by ranges;
+        `fact_order_sorted` says the fact rows are sorted by order key (else
+        each row's key is checked against its order's).  `key_ranges` =
         (cust_min, cust_max, order_min, order_max) from the tables' statistics:
         the call then skips its own min/max pass."""
         a = N.ChainArgs()
@@ -274,6 +276,7 @@ class Engine:
         a.capacity_hint = int(capacity_hint)
         if ranges is not None:
             a.order_fact_start, a.order_fact_end = ptr(ranges[0]), ptr(ranges[1])
+            a.fact_order_sorted = int(bool(fact_order_sorted))
         if fact_ship is not None:  # sum only fact rows with fact_ship > ship_gt
             a.fact_ship, a.ship_gt = ptr(fact_ship), int(ship_gt)
         if key_ranges is not None:

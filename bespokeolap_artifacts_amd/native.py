@@ -164,7 +164,8 @@ class ChainArgs(C.Structure):
                 ("order_fact_start", vp), ("order_fact_end", vp),
                 ("key_ranges_known", C.c_int32), ("cust_key_min", C.c_int32),
                 ("cust_key_max", C.c_int32), ("order_key_min", C.c_int32),
-                ("order_key_max", C.c_int32), ("fact_ship", vp), ("ship_gt", C.c_int32)]
+                ("order_key_max", C.c_int32), ("fact_ship", vp), ("ship_gt", C.c_int32),
+                ("fact_order_sorted", C.c_int32)]
 
 
 class GroupRow(C.Structure):

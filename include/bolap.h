@@ -308,6 +308,14 @@ typedef struct {
        when one of its rows passes. */
     const int16_t* fact_ship;
     int32_t ship_gt;
+    /* With order ranges only.  1: the fact rows are sorted by order key (the
+       reference's lineitem.orderkey_sorted, which bolap_derive_lineitem
+       reports), so every row of an order's range belongs to it.  0: the
+       ranges may be min/max spans over unsorted rows (builder_impl.cpp:
+       963-988), and rows whose fact_order differs from the order's key are
+       skipped, as run_q3's unsorted branch does (query_q3.cpp:262-281);
+       fact_order is then required. */
+    int32_t fact_order_sorted;
 } bolap_chain_args;
 
 typedef struct {

@@ -152,14 +152,45 @@ def test_chain_by_ranges_matches_oracle(engine, cust, orders):
     want, st = want_sorted(t, 1, 1155)
     _, start, end = engine.order_ranges(t["order_key"], t["fact_order"],
                                         int(t["order_key"].max().item()) + 1)
-    a = engine.chain_args(t, 1, 1155, ranges=(start, end))
-    top, gs = engine.chain_topk(a, 10)
-    assert np.array_equal(as_array(top), want[:10])
+    # the generated fact rows are clustered by order key: the ranges are
+    # exact, and checking each row's key changes nothing
+    for sorted_ in (True, False):
+        a = engine.chain_args(t, 1, 1155, ranges=(start, end), fact_order_sorted=sorted_)
+        top, gs = engine.chain_topk(a, 10)
+        assert np.array_equal(as_array(top), want[:10])
+        check_stats(gs, st)
+        if orders <= 1_500_000:
+            allg, gs2 = engine.chain_groups_sorted(a, len(want) + 5)
+            assert np.array_equal(allg, want)
+            check_stats(gs2, st)
+
+
+@pytest.mark.parametrize("orders", [40, 60_000, 1_500_000])
+def test_chain_by_ranges_unsorted_fact_rows(engine, orders):
+    """Fact rows out of order (shuffled within blocks of 256 rows, or wholly
+    for the smallest case): bolap_order_ranges gives min/max spans that cover
+    other orders' rows (the reference builder's unsorted branch), and the
+    ranged sum must skip rows of other orders, as run_q3's unsorted branch
+    does."""
+    import torch
+    t = engine.gen_chain(SEED, max(10, orders // 10), orders)
+    n = t["fact_order"].numel()
+    g = torch.Generator().manual_seed(orders)
+    block = (torch.arange(n, dtype=torch.float64) // 256) if orders > 40 else torch.zeros(n, dtype=torch.float64)
+    perm = torch.argsort(block + torch.rand(n, generator=g, dtype=torch.float64)).cuda()
+    t2 = dict(t)
+    for k in ("fact_order", "fact_price", "fact_disc"):
+        t2[k] = t[k][perm].contiguous()
+    want, st = want_sorted(t2, 1, 1155)
+    _, start, end = engine.order_ranges(t2["order_key"], t2["fact_order"],
+                                        int(t2["order_key"].max().item()) + 1)
+    assert int((end - start).max().item()) > 7  # spans wider than any order's rows
+    a = engine.chain_args(t2, 1, 1155, ranges=(start, end))
+    allg, gs = engine.chain_groups_sorted(a, len(want) + 5)
+    assert np.array_equal(allg, want)
     check_stats(gs, st)
-    if orders <= 1_500_000:
-        allg, gs2 = engine.chain_groups_sorted(a, len(want) + 5)
-        assert np.array_equal(allg, want)
-        check_stats(gs2, st)
+    top, _ = engine.chain_topk(a, 10)
+    assert np.array_equal(as_array(top), want[:10])
 
 
 def test_chain_by_ranges_shuffled_orders_and_orders_without_facts(engine):

@@ -105,3 +105,27 @@ def test_out_of_bounds_raise(engine, field):
     d[field][0] = {"range_value": 10, "pair_group": 10, "pair_item": 100}[field]
     with pytest.raises(N.BolapError):
         gpu(engine, d)
+
+
+@pytest.mark.parametrize("dense_max", [0, 1])
+def test_long_ranges_next_to_empty_and_dropped_rows(engine, dense_max):
+    """A few items with very long value ranges (hundreds to thousands of
+    values, spanning many 64-value batches of a wave) among empty ranges and
+    dropped join rows (group -1) in the same waves, as real q10a movie_info
+    ranges are."""
+    d = gen(17, 20_000, 2000, 300, 4000, max_range=3, drop=0.3)
+    rng = np.random.default_rng(17)
+    count = d["item_count"].copy()
+    long_items = rng.choice(2000, 12, replace=False)
+    count[long_items] = rng.integers(200, 5000, 12)
+    count[rng.choice(2000, 400, replace=False)] = 0  # empty ranges
+    count[long_items] = np.maximum(count[long_items], 200)
+    d["item_count"] = count.astype(np.int32)
+    d["item_offset"] = np.concatenate([[0], np.cumsum(count[:-1])]).astype(np.int32)
+    d["range_value"] = rng.integers(0, 4000, int(count.sum())).astype(np.int32)
+    # every tenth join row points at a long item, next to dropped rows
+    d["pair_item"][::10] = rng.choice(long_items, len(d["pair_item"][::10]))
+    d["pair_group"][1::10] = -1
+    st, got = gpu(engine, d, dense_max)
+    assert st["expanded_rows"] > 1_000_000
+    np.testing.assert_array_equal(got, want(d, dense_max))

@@ -178,6 +178,14 @@ def want_q3(c, segment: str, date: str) -> str:
     return "".join(out)
 
 
+def stub_csv(qid: str, line: str) -> str:
+    """The reference's stub result of an id without a pipeline
+    (bespoke_tpch/query_impl.cpp:82-103): both fields quoted, an embedded
+    double quote written as backslash + quote."""
+    esc = lambda v: '"' + v.replace('"', '\\"') + '"'
+    return f"query_id,params\n{esc(qid)},{esc(line)}\n"
+
+
 def run_driver(data: pathlib.Path, run: pathlib.Path, requests):
     run.mkdir()
     return subprocess.run([str(B.DRIVER), str(data) + "/"], input="\n".join(requests) + "\n\n",
@@ -191,20 +199,20 @@ def test_driver_answers_like_the_reference(tmp_path, orders):
     tabs, cols = tables(11 + orders, orders)
     write(tabs, tmp_path / "data")
     b0, c0 = BRANDS[int(cols["brand"][0])], CONTAINERS[int(cols["cont"][0])]
-    requests = ['1 90', '6 "1994-01-01" "0.06" "24"', '9 "GREEN"', '1 "60"',
+    requests = ['1 90', '6 "1994-01-01" "0.06" "24"', '99 "GREEN" x', '1 "60"',
                 '6 1995-01-01 0.05 25', '14 "1995-09-01"', f'17 "{b0}" "{c0}"',
                 '17 "Brand#99" "SM BOX"', '18 250', '18 "<<NULL>>"', '14 1996-02-29',
                 '3 "BUILDING" "1995-03-15"', '3 MACHINERY 1993-06-01', '3 "NOSUCH" "1995-03-15"']
     r = run_driver(tmp_path / "data", tmp_path / "run", requests)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
-    served = [str(i + 1) for i, q in enumerate(requests) if not q.startswith("9 ")]
-    assert [ln.split(" | ")[0] for ln in lines] == served
+    # every request gets a timing line and a result file, in request order;
+    # an id without a pipeline gets the reference's stub file
+    assert [ln.split(" | ")[0] for ln in lines] == [str(i + 1) for i in range(len(requests))]
     assert all(ln.split(" | ")[1].startswith("Execution ms: ") for ln in lines)
-    assert "query 9 is not served" in r.stderr
     run = tmp_path / "run"
-    assert not (run / "result3.csv").exists()
-    want = [want_q1(cols, 90), want_q6(cols, "1994-01-01", "0.06", "24"), None,
+    want = [want_q1(cols, 90), want_q6(cols, "1994-01-01", "0.06", "24"),
+            stub_csv("99", '99 "GREEN" x'),
             want_q1(cols, 60), want_q6(cols, "1995-01-01", "0.05", "25"),
             want_q14(cols, "1995-09-01"), want_q17(cols, b0, c0),
             want_q17(cols, "Brand#99", "SM BOX"), want_q18(cols, "250"),
