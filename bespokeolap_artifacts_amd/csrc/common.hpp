@@ -126,6 +126,9 @@ int pinned_reserve(bolap_ctx* ctx, size_t bytes);
 int wait_pinned_seq(bolap_ctx* ctx, size_t slot, int64_t seq);
 // Checks the last kernel launch.
 int launch_check(const char* what);
+// Looks up this device's copy of the allocation-free join's scratch array
+// (join_sum.hip); called by bolap_ctx_create with `device` current.
+int noalloc_setup(int device);
 
 // Records a pair of events around a stage when tracing is on.
 struct TraceScope {

@@ -32,6 +32,7 @@
 // a build side larger than half its slots raises the overflow flag and the
 // host reruns with device sizing.
 #include <cstring>
+#include <mutex>
 
 #include "preds.hpp"
 
@@ -1121,11 +1122,18 @@ int enqueue_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     return dispatch_probe(ctx, a, p);
 }
 
-// One run of the pipeline with table geometry `g`; host_res receives the
-// result slot.
-int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps, const Geo& g,
-                 size_t lds, int64_t* host_res) {
-    Plan p{};
+// Device regions of one run (byte offsets from the start of the region the
+// run is carved from) and the plan fields that do not depend on where the
+// regions live.
+struct JoinLayout {
+    size_t o_keys, o_slots, o_bits, o_part;
+    size_t bytes;  // total
+    int64_t seg_cap;
+};
+
+JoinLayout plan_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
+                     const Geo& g, size_t lds, Plan& p, WsArena& ws) {
+    p = Plan{};
     p.g = g;
     p.lds = lds;
     // HBM table: the largest the build side can need under `g`.
@@ -1151,25 +1159,41 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
         (units + int64_t(p.compact_grid) * kCompactThreads - 1) /
         (int64_t(p.compact_grid) * kCompactThreads) * kCompactThreads * p.compact_r;
     const int64_t blocks_per_seg = (int64_t(p.compact_grid) + kSegs - 1) / kSegs;
-    const int64_t seg_cap = blocks_per_seg * per_block + p.compact_r;
+    JoinLayout L{};
+    L.seg_cap = blocks_per_seg * per_block + p.compact_r;
+    L.o_keys = ws.plan(size_t(L.seg_cap) * kSegs * 4);
+    L.o_slots = ws.plan(size_t(p.hbm_cap) * 4);
+    L.o_bits = ws.plan(size_t(words) * 4);
+    L.o_part = ws.plan(size_t(p.max_blocks) * 2 * sizeof(int64_t));
+    L.bytes = ws.off;
+    return L;
+}
 
-    WsArena ws;
-    const size_t o_keys = ws.plan(size_t(seg_cap) * kSegs * 4);
-    const size_t o_slots = ws.plan(size_t(p.hbm_cap) * 4);
-    const size_t o_bits = ws.plan(size_t(words) * 4);
-    const size_t o_part = ws.plan(size_t(p.max_blocks) * 2 * sizeof(int64_t));
-    BOLAP_TRY(ws_reserve(ctx, ws.off));
-    BOLAP_TRY(pinned_reserve(ctx, (kResN + 1) * sizeof(int64_t)));
-    ws.base = static_cast<char*>(ctx->ws);
-    // Segment counters (64 bytes apart) and result slots in the context's
-    // join state, which the merge leaves zeroed for the next call.
-    char* js = static_cast<char*>(ctx->join_state);
+// Points the plan's regions into `base` (laid out by plan_join) and the
+// segment counters and result slots into `state` (kJoinStateBytes).
+void carve_join(Plan& p, const JoinLayout& L, char* base, char* state) {
     static_assert(size_t(kSegs) * kSegStride * 4 + kResN * 8 <= bolap_ctx::kJoinStateBytes,
                   "join state too small");
-    p.res = reinterpret_cast<int64_t*>(js + size_t(kSegs) * kSegStride * 4);
-    p.kl = KList{ws.at<int32_t>(o_keys), reinterpret_cast<uint32_t*>(js), seg_cap};
-    p.t = Table{ws.at<int32_t>(o_slots), ws.at<uint32_t>(o_bits)};
-    p.partials = ws.at<int64_t>(o_part);
+    WsArena ws;
+    ws.base = base;
+    p.res = reinterpret_cast<int64_t*>(state + size_t(kSegs) * kSegStride * 4);
+    p.kl = KList{ws.at<int32_t>(L.o_keys), reinterpret_cast<uint32_t*>(state), L.seg_cap};
+    p.t = Table{ws.at<int32_t>(L.o_slots), ws.at<uint32_t>(L.o_bits)};
+    p.partials = ws.at<int64_t>(L.o_part);
+}
+
+// One run of the pipeline with table geometry `g`; host_res receives the
+// result slot.
+int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps, const Geo& g,
+                 size_t lds, int64_t* host_res) {
+    Plan p;
+    WsArena ws;
+    const JoinLayout L = plan_join(ctx, a, ps, g, lds, p, ws);
+    BOLAP_TRY(ws_reserve(ctx, L.bytes));
+    BOLAP_TRY(pinned_reserve(ctx, (kResN + 1) * sizeof(int64_t)));
+    // Segment counters (64 bytes apart) and result slots in the context's
+    // join state, which the merge leaves zeroed for the next call.
+    carve_join(p, L, static_cast<char*>(ctx->ws), static_cast<char*>(ctx->join_state));
     p.host = static_cast<int64_t*>(ctx->pinned_dev);
     p.seq = ++ctx->pinned_seq;
     p.state_clean = ctx->join_state_clean;
@@ -1184,6 +1208,135 @@ int join_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps
     return BOLAP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant (bolap_join_sum_noalloc).  Its key list, HBM table,
+// filter, partials, counters and result row live in `scratch`, a file-scope
+// device array that every GPU the library runs on holds until the process
+// ends.  Host code never passes the array's name to a kernel: it is a
+// host-side stand-in, not a device address.  The device address of each
+// GPU's copy is looked up once per device (noalloc_setup, called by
+// bolap_ctx_create with that device current) and kept in g_scratch.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Scratch layout: join state (counters, result slots), then the published
+// result row (kResN slots + sequence), then the run's regions.
+constexpr size_t kScratchState = 0;
+constexpr size_t kScratchRow = bolap_ctx::kJoinStateBytes;
+constexpr size_t kScratchRuns = kScratchRow + 256;
+
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
+
+int join_noalloc_attempt(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet& ps,
+                         const Geo& g, size_t lds, int64_t* host_res) {
+    char* base = static_cast<char*>(scratch_of(ctx->device));
+    if (!base) {
+        set_error("bolap_join_sum_noalloc: no scratch address for device %d", ctx->device);
+        return BOLAP_ERR_HIP;
+    }
+    Plan p;
+    WsArena ws;
+    const JoinLayout L = plan_join(ctx, a, ps, g, lds, p, ws);
+    if (L.bytes > kScratchBytes - kScratchRuns) {
+        set_error("bolap_join_sum_noalloc: %lld dimension rows need %zu scratch bytes; the "
+                  "maximum is %zu (use bolap_join_sum)",
+                  (long long)a->dim_rows, L.bytes, kScratchBytes - kScratchRuns);
+        return BOLAP_ERR_CAPACITY;
+    }
+    carve_join(p, L, base + kScratchRuns, base + kScratchState);
+    int64_t* row = reinterpret_cast<int64_t*>(base + kScratchRow);
+    p.host = row;
+    p.seq = 1;
+    // The array keeps the previous call's contents: counters and result
+    // slots are cleared by the first kernel of every call.
+    p.state_clean = false;
+    BOLAP_TRY(enqueue_join(ctx, a, ps, p));
+    BOLAP_HIP_TRY(hipMemcpyAsync(host_res, row, kResN * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BOLAP_OK;
+}
+
+}  // namespace
+
+// Looks up the device address of this device's `scratch` (the current
+// device must be `device`).  Called once per device by bolap_ctx_create.
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("noalloc_setup: device %d beyond %d", device, kMaxDevices);
+        return BOLAP_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return BOLAP_OK;
+    void* ptr = nullptr;
+    BOLAP_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = ptr;
+    return BOLAP_OK;
+}
+
+namespace {
+
+// Validates the arguments shared by bolap_join_sum and its allocation-free
+// variant and builds the predicate set.
+int join_check_args(const bolap_join_sum_args* a, PredSet& ps, const char* who) {
+    BOLAP_ARG_CHECK(a->dim_rows >= 0 && a->fact_rows >= 0, "%s: negative rows", who);
+    BOLAP_ARG_CHECK(a->dim_rows <= (int64_t(1) << 30), "%s: %lld dimension rows "
+                    "exceed 2^30", who, (long long)a->dim_rows);
+    BOLAP_ARG_CHECK(a->dim_rows == 0 || a->dim_key, "%s: null dim_key", who);
+    BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_fk && a->fact_price && a->fact_disc),
+                    "%s: null fact column", who);
+    BOLAP_ARG_CHECK(a->price_dtype == BOLAP_I64 || a->price_dtype == BOLAP_I32,
+                    "%s: price dtype must be i64 or i32", who);
+    BOLAP_ARG_CHECK(a->disc_dtype == BOLAP_I64 || a->disc_dtype == BOLAP_U8,
+                    "%s: disc dtype must be i64 or u8", who);
+    BOLAP_ARG_CHECK(a->placement >= BOLAP_PLACE_AUTO && a->placement <= BOLAP_PLACE_HBM,
+                    "%s: placement %d", who, a->placement);
+    BOLAP_ARG_CHECK(aligned(a->fact_fk, 16) || a->fact_rows == 0,
+                    "%s: fact_fk must be 16-byte aligned", who);
+    return make_predset(ps, a->n_dim_preds, a->dim_preds, a->dim_rows, who);
+}
+
+// Device-sized geometry (the exact path) and the probe's dynamic LDS.
+Geo exact_geo(bolap_ctx* ctx, int32_t placement, size_t& lds) {
+    const int64_t lds_keys = placement == BOLAP_PLACE_LDS ? kForcedLdsKeys : kAutoLdsKeys;
+    const Geo g{0, placement, 0, 0, lds_keys,
+                int32_t(clampi(int(ctx->knob("join.lds_bloom_bits", 7)), 5, 8))};
+    lds = 0;
+    if (placement != BOLAP_PLACE_HBM) {
+        const Shape big = shape_of(lds_keys, g);
+        lds = lds_bytes_of(big.cap_log2, big.bb_log2);
+    }
+    if (placement == BOLAP_PLACE_AUTO) {
+        // Room for the LDS filter of a mid-size build as well.
+        const Shape mid = shape_of(kHybridKeys, g);
+        const size_t fb = (size_t(1) << mid.bb_log2) / 8;
+        if (lds < fb) lds = fb;
+    }
+    return g;
+}
+
+void fill_result(bolap_ctx* ctx, const bolap_join_sum_args* a, const int64_t* r, const Geo& g,
+                 int fallbacks, bolap_join_sum_result* out) {
+    const Shape s = shape_of(r[kBuild], g);
+    trace_count(ctx, "join_build_rows", r[kBuild]);
+    trace_count(ctx, "join_probe_rows_in", a->fact_rows);
+    trace_count(ctx, "join_rows_emitted", r[kMatch]);
+    out->revenue = r[kRev];
+    out->matches = r[kMatch];
+    out->build_rows = r[kBuild];
+    out->capacity = int64_t(1) << s.cap_log2;
+    out->placement = s.lds ? BOLAP_PLACE_LDS : BOLAP_PLACE_HBM;
+    out->fallbacks = fallbacks;
+}
+
 }  // namespace
 }  // namespace bolap
 
@@ -1192,22 +1345,8 @@ using namespace bolap;
 extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
                               bolap_join_sum_result* out) {
     BOLAP_ARG_CHECK(ctx && a && out, "bolap_join_sum: null argument");
-    BOLAP_ARG_CHECK(a->dim_rows >= 0 && a->fact_rows >= 0, "bolap_join_sum: negative rows");
-    BOLAP_ARG_CHECK(a->dim_rows <= (int64_t(1) << 30), "bolap_join_sum: %lld dimension rows "
-                    "exceed 2^30", (long long)a->dim_rows);
-    BOLAP_ARG_CHECK(a->dim_rows == 0 || a->dim_key, "bolap_join_sum: null dim_key");
-    BOLAP_ARG_CHECK(a->fact_rows == 0 || (a->fact_fk && a->fact_price && a->fact_disc),
-                    "bolap_join_sum: null fact column");
-    BOLAP_ARG_CHECK(a->price_dtype == BOLAP_I64 || a->price_dtype == BOLAP_I32,
-                    "bolap_join_sum: price dtype must be i64 or i32");
-    BOLAP_ARG_CHECK(a->disc_dtype == BOLAP_I64 || a->disc_dtype == BOLAP_U8,
-                    "bolap_join_sum: disc dtype must be i64 or u8");
-    BOLAP_ARG_CHECK(a->placement >= BOLAP_PLACE_AUTO && a->placement <= BOLAP_PLACE_HBM,
-                    "bolap_join_sum: placement %d", a->placement);
-    BOLAP_ARG_CHECK(aligned(a->fact_fk, 16) || a->fact_rows == 0,
-                    "bolap_join_sum: fact_fk must be 16-byte aligned");
     PredSet ps;
-    BOLAP_TRY(make_predset(ps, a->n_dim_preds, a->dim_preds, a->dim_rows, "bolap_join_sum"));
+    BOLAP_TRY(join_check_args(a, ps, "bolap_join_sum"));
 
     int64_t r[kResN];
     Geo g{};
@@ -1224,37 +1363,35 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
     }
     if (!done) {
         // Exact sizing on the device from the compacted build size.
-        const int64_t lds_keys =
-            a->placement == BOLAP_PLACE_LDS ? kForcedLdsKeys : kAutoLdsKeys;
-        g = Geo{0, a->placement, 0, 0, lds_keys,
-                int32_t(clampi(int(ctx->knob("join.lds_bloom_bits", 7)), 5, 8))};
         size_t lds = 0;
-        if (a->placement != BOLAP_PLACE_HBM) {
-            const Shape big = shape_of(lds_keys, g);
-            lds = lds_bytes_of(big.cap_log2, big.bb_log2);
-        }
-        if (a->placement == BOLAP_PLACE_AUTO) {
-            // Room for the LDS filter of a mid-size build as well.
-            const Shape mid = shape_of(kHybridKeys, g);
-            const size_t fb = (size_t(1) << mid.bb_log2) / 8;
-            if (lds < fb) lds = fb;
-        }
+        g = exact_geo(ctx, a->placement, lds);
         BOLAP_TRY(join_attempt(ctx, a, ps, g, lds, r));
         if (shape_of(r[kBuild], g).flag & 4) {
             set_error("bolap_join_sum: %lld keys do not fit the LDS budget", (long long)r[kBuild]);
             return BOLAP_ERR_CAPACITY;
         }
     }
-    const Shape s = shape_of(r[kBuild], g);
-    trace_count(ctx, "join_build_rows", r[kBuild]);
-    trace_count(ctx, "join_probe_rows_in", a->fact_rows);
-    trace_count(ctx, "join_rows_emitted", r[kMatch]);
-    out->revenue = r[kRev];
-    out->matches = r[kMatch];
-    out->build_rows = r[kBuild];
-    out->capacity = int64_t(1) << s.cap_log2;
-    out->placement = s.lds ? BOLAP_PLACE_LDS : BOLAP_PLACE_HBM;
-    out->fallbacks = fallbacks;
+    fill_result(ctx, a, r, g, fallbacks, out);
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_join_sum_noalloc(bolap_ctx* ctx, const bolap_join_sum_args* a,
+                                      bolap_join_sum_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_join_sum_noalloc: null argument");
+    PredSet ps;
+    BOLAP_TRY(join_check_args(a, ps, "bolap_join_sum_noalloc"));
+    // Exact device sizing only: an estimate could not change the scratch
+    // bound, which is the worst case of the dimension size.
+    size_t lds = 0;
+    const Geo g = exact_geo(ctx, a->placement, lds);
+    int64_t r[kResN];
+    BOLAP_TRY(join_noalloc_attempt(ctx, a, ps, g, lds, r));
+    if (shape_of(r[kBuild], g).flag & 4) {
+        set_error("bolap_join_sum_noalloc: %lld keys do not fit the LDS budget",
+                  (long long)r[kBuild]);
+        return BOLAP_ERR_CAPACITY;
+    }
+    fill_result(ctx, a, r, g, 0, out);
     return BOLAP_OK;
 }
 

@@ -149,13 +149,15 @@ int bolap_ctx_create(int device, void* stream, bolap_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->result_dev, bolap_ctx::kResultBytes);
     if (e == hipSuccess) e = hipMalloc(&ctx->join_state, bolap_ctx::kJoinStateBytes);
     if (e == hipSuccess) e = hipMemset(ctx->join_state, 0, bolap_ctx::kJoinStateBytes);
-    if (e != hipSuccess) {
-        set_error("bolap_ctx_create: %s", hipGetErrorString(e));
+    int setup = BOLAP_OK;
+    if (e == hipSuccess) setup = noalloc_setup(device);
+    if (e != hipSuccess || setup != BOLAP_OK) {
+        if (e != hipSuccess) set_error("bolap_ctx_create: %s", hipGetErrorString(e));
         if (ctx->counters) (void)hipFree(ctx->counters);
         if (ctx->result_dev) (void)hipFree(ctx->result_dev);
         if (ctx->join_state) (void)hipFree(ctx->join_state);
         delete ctx;
-        return BOLAP_ERR_HIP;
+        return e != hipSuccess ? BOLAP_ERR_HIP : setup;
     }
     *out = ctx;
     return BOLAP_OK;

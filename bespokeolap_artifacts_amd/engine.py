@@ -242,9 +242,11 @@ class Engine:
         a = self.join_sum_args(*args, **kw)
         return self.join_sum_run(a)
 
-    def join_sum_run(self, a: N.JoinSumArgs) -> dict:
+    def join_sum_run(self, a: N.JoinSumArgs, noalloc: bool = False) -> dict:
+        """noalloc: the allocation-free entry point (bolap_join_sum_noalloc)."""
         r = N.JoinSumResult()
-        N.check(self.lib.bolap_join_sum(self.ctx, C.byref(a), C.byref(r)))
+        fn = self.lib.bolap_join_sum_noalloc if noalloc else self.lib.bolap_join_sum
+        N.check(fn(self.ctx, C.byref(a), C.byref(r)))
         return {"revenue": r.revenue, "build_rows": r.build_rows, "matches": r.matches,
                 "capacity": r.capacity, "placement": r.placement, "fallbacks": r.fallbacks}
 

@@ -268,6 +268,7 @@ PROTOTYPES = {
     "bolap_q1_scan_agg": (C.c_int, [vp, P(Q1Args), P(Q1Group)]),
     "bolap_q1_scan_agg_async": (C.c_int, [vp, P(Q1Args), vp]),
     "bolap_join_sum": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),
+    "bolap_join_sum_noalloc": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),
     "bolap_chain_topk": (C.c_int, [vp, P(ChainArgs), i32, P(GroupRow), P(i32),
                                    P(ChainStats)]),
     "bolap_chain_groups_sorted": (C.c_int, [vp, P(ChainArgs), P(GroupRow), i64, P(i64),

@@ -233,6 +233,24 @@ typedef struct {
 int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* args,
                    bolap_join_sum_result* out);
 
+/* Allocation-free variant of bolap_join_sum.  No counterpart in the
+   reference: it is for callers that cannot allocate device or pinned memory
+   while they run (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  It runs the same kernels with the same
+   exact, device-side sizing (capacity_hint is ignored) and returns the same
+   result, but calls no hipMalloc, hipFree, hipHostMalloc or allocating
+   library routine: its key list, hash table, filter, partials, counters and
+   result row live in a fixed 64 MiB device array of the library (one copy
+   per GPU, held until the process ends; its address is looked up by
+   bolap_ctx_create).  A dimension whose worst-case build needs more than
+   that (about 4 M rows) is rejected with BOLAP_ERR_CAPACITY.  The array
+   keeps the last call's contents and every call on a GPU shares it: calls
+   that overlap on one GPU (from several contexts or streams) race.  The
+   result is copied back with a stream-ordered copy and a synchronisation
+   of the context's stream. */
+int bolap_join_sum_noalloc(bolap_ctx* ctx, const bolap_join_sum_args* args,
+                           bolap_join_sum_result* out);
+
 /* Q17: parts passing the predicates (brand / container codes) -> per-part
    SUM(quantity) and COUNT over their lineitem rows -> SUM(extendedprice) of
    the rows with quantity * 5 * count < SUM(quantity), i.e. below 0.2 times
