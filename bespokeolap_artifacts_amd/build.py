@@ -109,18 +109,24 @@ def build_driver() -> pathlib.Path | None:
     return DRIVER
 
 
-def build_oracle() -> pathlib.Path:
-    """Compiles the C oracle (test infrastructure) into oracle/liboracle.so."""
+def oracle_sources() -> list[pathlib.Path]:
+    return sorted(ORACLE_DIR.glob("*.c"))
+
+
+def build_oracle(out: pathlib.Path | None = None, march: str = "x86-64-v2") -> pathlib.Path:
+    """Compiles the C oracle (test infrastructure) into oracle/liboracle.so, or
+    into `out` (the CPU baseline builds its own copy with -march=native on
+    the host that times it)."""
+    out = out or ORACLE_LIB
     cc = shutil.which("gcc") or "cc"
-    tmp = ORACLE_LIB.with_suffix(".so.tmp")
-    cmd = [cc, "-O3", "-march=x86-64-v2", "-std=c11", "-fPIC", "-shared",
-           str(ORACLE_DIR / "oracle.c"), str(ORACLE_DIR / "star.c"), str(ORACLE_DIR / "ceb.c"),
-           str(ORACLE_DIR / "tpch.c"), "-o", str(tmp)]
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [cc, "-O3", f"-march={march}", "-std=c11", "-fPIC", "-shared",
+           *map(str, oracle_sources()), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, ORACLE_LIB)
-    return ORACLE_LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_all() -> None:

@@ -7,12 +7,15 @@ product package never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 import numpy as np
 
 _DIR = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _DIR / "liboracle.so"
+# BOLAP_ORACLE_LIB names another build of the same sources (the CPU baseline's
+# -march=native copy, compiled on the host that times it).
+LIB_PATH = pathlib.Path(os.environ.get("BOLAP_ORACLE_LIB", _DIR / "liboracle.so"))
 _lib = None
 
 vp = C.c_void_p
@@ -71,6 +74,9 @@ def lib() -> C.CDLL:
         L.oracle_pred_row.argtypes = [C.POINTER(Pred), i64]
         L.oracle_join_sum.argtypes = [i64, vp, i32, C.POINTER(Pred), i64, vp, vp, i32, vp,
                                       i32, C.POINTER(JoinSumOut)]
+        L.oracle_q3_ranges.restype = i64
+        L.oracle_q3_ranges.argtypes = [i64, vp, vp, i32, i64, vp, vp, vp, vp, vp, i32, vp, vp,
+                                       vp, i64]
         L.oracle_chain_groups.restype = i64
         L.oracle_chain_groups.argtypes = [i64, vp, vp, i32, i64, vp, vp, vp, i32, i64, vp,
                                           vp, i32, vp, i32, vp, vp, i32, vp, i64,
@@ -225,6 +231,22 @@ def sort_groups(rows: np.ndarray) -> np.ndarray:
     r = np.ascontiguousarray(rows.copy())
     lib().oracle_sort_groups(_p(r), len(r))
     return r
+
+
+def q3_ranges(t: dict, segment: int, day_lt: int, order_sorted: dict, start, end):
+    """run_q3's own algorithm (oracle_q3_ranges): `order_sorted` holds the
+    order columns sorted by day, `start`/`end` each sorted order's fact
+    range.  Returns every group row, sorted."""
+    n_orders = len(order_sorted["order_key"])
+    out = np.empty(max(1, n_orders), GROUP_DTYPE)
+    s = np.ascontiguousarray(start, np.uint32)
+    e = np.ascontiguousarray(end, np.uint32)
+    n = lib().oracle_q3_ranges(
+        len(t["cust_key"]), _p(t["cust_key"]), _p(t["cust_segment"]), segment, n_orders,
+        _p(order_sorted["order_key"]), _p(order_sorted["order_day"]),
+        _p(order_sorted["order_cust"]), _p(s), _p(e), day_lt, _p(t["fact_price"]),
+        _p(t["fact_disc"]), _p(out), len(out))
+    return out[:n].copy()
 
 
 def radix_sort_u64(keys: np.ndarray, vals: np.ndarray, key_bits: int = 64):

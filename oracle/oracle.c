@@ -383,6 +383,64 @@ void oracle_sort_groups(oracle_group_row* rows, int64_t n) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* run_q3's own algorithm over the reference layout (query_q3.cpp:182-356):   */
+/* a byte mask over customer keys for the segment (:182-205); orders sorted   */
+/* by date at build (builder_impl.cpp:597-608), so the passing orders are the */
+/* prefix day < day_lt found by lower_bound (:207-210); for every prefix      */
+/* order whose customer is marked, the sum of its fact range                  */
+/* [range_start, range_end) (orders.lineitem_ranges, an empty range has end   */
+/* 0; :219-260, :306-318); then the full sort of the group rows (:348-356).   */
+/* Revenue is price * (100 - disc) of i64 columns.  Used as the CPU baseline  */
+/* of the chain join; returns the number of groups (rows sorted in out).      */
+/* ------------------------------------------------------------------------ */
+int64_t oracle_q3_ranges(int64_t cust_rows, const int32_t* cust_key, const uint8_t* cust_segment,
+                         int32_t segment, int64_t order_rows, const int32_t* order_key,
+                         const int32_t* order_day, const int32_t* order_cust,
+                         const uint32_t* range_start, const uint32_t* range_end, int32_t day_lt,
+                         const int64_t* fact_price, const int64_t* fact_disc,
+                         oracle_group_row* out, int64_t max_out) {
+    int32_t max_cust = -1;
+    for (int64_t i = 0; i < cust_rows; ++i)
+        if (cust_key[i] > max_cust) max_cust = cust_key[i];
+    const size_t mask_n = (size_t)(max_cust + 1);
+    uint8_t* in_seg = (uint8_t*)calloc(mask_n ? mask_n : 1, 1);
+    for (int64_t i = 0; i < cust_rows; ++i)
+        if (cust_key[i] >= 0 && cust_segment[i] == segment) in_seg[cust_key[i]] = 1;
+    int64_t lo = 0, hi = order_rows;  /* lower_bound(order_day, day_lt) */
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (order_day[mid] < day_lt) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t limit = lo;
+    int64_t groups = 0;
+    for (int64_t o = 0; o < limit; ++o) {
+        const int32_t c = order_cust[o];
+        if (c < 0 || (size_t)c >= mask_n || !in_seg[c]) continue;
+        const uint32_t s = range_start[o], e = range_end[o];
+        if (e == 0) continue;
+        int64_t rev = 0;
+        int has = 0;
+        for (uint32_t r = s; r < e; ++r) {
+            rev += fact_price[r] * (100 - fact_disc[r]);
+            has = 1;
+        }
+        if (!has) continue;
+        if (groups < max_out) {
+            out[groups].key = order_key[o];
+            out[groups].day = order_day[o];
+            out[groups].revenue = rev;
+        }
+        ++groups;
+    }
+    free(in_seg);
+    const int64_t kept = groups < max_out ? groups : max_out;
+    oracle_sort_groups(out, kept);
+    return groups;
+}
+
+
+/* ------------------------------------------------------------------------ */
 /* LSD radix sort with 16-bit digits, stable counting-sort passes, as         */
 /* radix_sort_rows_by_orderkey / radix_sort_groups                            */
 /* (bespoke_tpch/query_q10.cpp:134-166, 318-373).  Only the low key_bits bits */

@@ -97,6 +97,12 @@ int64_t oracle_chain_groups(int64_t cust_rows, const int32_t* cust_key,
                             const int32_t* fact_disc_price, const int16_t* fact_ship, int32_t ship_gt, oracle_group_row* out,
                             int64_t max_out, oracle_chain_stats* stats);
 void oracle_sort_groups(oracle_group_row* rows, int64_t n);
+int64_t oracle_q3_ranges(int64_t cust_rows, const int32_t* cust_key, const uint8_t* cust_segment,
+                         int32_t segment, int64_t order_rows, const int32_t* order_key,
+                         const int32_t* order_day, const int32_t* order_cust,
+                         const uint32_t* range_start, const uint32_t* range_end, int32_t day_lt,
+                         const int64_t* fact_price, const int64_t* fact_disc,
+                         oracle_group_row* out, int64_t max_out);
 
 /* --- radix sort / top-k --------------------------------------------------- */
 void oracle_radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int32_t key_bits);

@@ -253,6 +253,33 @@ def test_chain_groups_match_pandas(cust, orders):
         np.isin(t["fact_order"], want["key"].to_numpy()).sum())
 
 
+def date_sorted_with_ranges(t):
+    """The reference build's layout of the chain tables: orders sorted by day
+    (stable) and each order's contiguous fact range (the generated fact rows
+    are clustered by order key, order rows by key)."""
+    perm = np.argsort(t["order_day"], kind="stable")
+    o = {k: np.ascontiguousarray(t[k][perm]) for k in ("order_key", "order_day", "order_cust")}
+    off = t["fact_offsets"]
+    return o, off[:-1][perm], off[1:][perm]
+
+
+@pytest.mark.parametrize("cust,orders", [(10, 50), (1500, 15_000), (20_000, 150_000)])
+def test_q3_range_walk_matches_pandas(cust, orders):
+    """run_q3's own algorithm (date-prefix of the sorted orders, range sums,
+    full sort) against the pandas merge/group-by, ordered by the full key."""
+    t = oracle.gen_chain(SEED, cust, orders)
+    o, start, end = date_sorted_with_ranges(t)
+    got = oracle.q3_ranges(t, 1, 1155, o, start, end)
+    want = chain_pandas(t, 1, 1155).sort_values(["v", "day", "key"],
+                                                ascending=[False, True, True])
+    assert np.array_equal(got["key"], want["key"].to_numpy())
+    assert np.array_equal(got["day"], want["day"].to_numpy())
+    assert np.array_equal(got["revenue"], want["v"].to_numpy())
+    # an empty prefix and an unknown segment give no rows
+    assert len(oracle.q3_ranges(t, 1, 0, o, start, end)) == 0
+    assert len(oracle.q3_ranges(t, 9, 1155, o, start, end)) == 0
+
+
 def test_chain_groups_shuffled_fact_and_disc_price():
     t = oracle.gen_chain(SEED, 2000, 20_000)
     rng = np.random.default_rng(0)
