@@ -1,0 +1,760 @@
+// tpch_orders.hip — TPC-H Q4, Q5, Q10 (orders by date -> lineitem ranges),
+// Q12 (lineitem filter -> order priority) and Q19 (part mask -> lineitem
+// probe) over the reference Database layout.
+//
+// Orders are sorted by date at build (bespoke_tpch/builder_impl.cpp:597-608),
+// so a date window is a contiguous run of order rows.  Every workgroup finds
+// that run itself with a workgroup-wide lower_bound (each round probes one
+// position per thread, three rounds for 15 M orders), then walks its share
+// of the window one order per lane through lineitem_ranges.  Group counters
+// with few keys (priorities, nations, ship modes) are summed in LDS and
+// added to the device result once per workgroup; Q10's per-customer revenue
+// goes to a dense array over the customer keys, compacted in key order and
+// sorted by the stable LSD radix sort.
+//
+// Q12 and Q19 stream the lineitem columns four rows per lane (one 4-byte
+// load of a u8 column, 8 of an i16 column, 16 of an i32 column) and gather
+// the order / part side only for rows that pass the row predicates.
+#include <cstring>
+
+#include "radix.hpp"
+
+namespace bolap {
+namespace {
+
+constexpr int kT = 256;
+
+// First index i in [0, n) with v[i] >= x (n when none) of a non-decreasing
+// column, computed by the whole workgroup: each round probes kT evenly
+// spaced positions of the remaining interval.  Every thread must call it.
+BOLAP_DEV int64_t wg_lower_bound(const int16_t* __restrict__ v, int64_t n, int32_t x) {
+    int64_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+    while (lo < hi) {
+        const int64_t step = (hi - lo + kT - 1) / kT;
+        const int64_t p = lo + int64_t(threadIdx.x) * step;
+        const bool below = p < hi && int32_t(v[p]) < x;
+        const int k = __syncthreads_count(below);
+        // positions lo + t * step, t < k, hold values below x
+        const int64_t nlo = k > 0 ? lo + int64_t(k - 1) * step + 1 : lo;
+        const int64_t top = lo + int64_t(k) * step;
+        hi = top < hi ? top : hi;
+        lo = nlo;
+    }
+    return lo;
+}
+
+struct Window {
+    int64_t s, e;  // order rows [s, e)
+};
+
+// The order rows of [date_lo, date_hi) and this workgroup's contiguous part.
+BOLAP_DEV Window wg_window(const int16_t* __restrict__ date, int64_t n, int32_t lo, int32_t hi) {
+    const int64_t s = wg_lower_bound(date, n, lo);
+    const int64_t e = hi > lo ? wg_lower_bound(date, n, hi) : s;
+    const int64_t len = e > s ? e - s : 0;
+    const int64_t per = (len + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = s + per * blockIdx.x;
+    const int64_t b1 = b0 + per < e ? b0 + per : e;
+    return Window{b0, b1 > b0 ? b1 : b0};
+}
+
+// The lineitem side of an order-range query.
+struct LiSide {
+    const uint32_t* start;
+    const uint32_t* end;
+    const int32_t* okey;   // null when lineitem is sorted by order key
+};
+
+// ---------------------------------------------------------------------------
+// Q4: orders with a late lineitem, counted per priority code.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void q4_k(const int16_t* __restrict__ odate, int64_t o_rows,
+                                           const int32_t* __restrict__ okey,
+                                           const uint8_t* __restrict__ prio, int32_t n_prio,
+                                           LiSide li, const uint32_t* __restrict__ cr,
+                                           int32_t date_lo, int32_t date_hi,
+                                           unsigned long long* __restrict__ counts) {
+    __shared__ unsigned s_cnt[256];
+    s_cnt[threadIdx.x] = 0;
+    const Window w = wg_window(odate, o_rows, date_lo, date_hi);
+    __syncthreads();
+    for (int64_t o = w.s + threadIdx.x; o < w.e; o += kT) {
+        const uint32_t e = li.end[o];
+        if (e == 0) continue;
+        const int32_t key = li.okey ? okey[o] : 0;
+        bool late = false;
+        for (uint32_t r = li.start[o]; r < e && !late; ++r) {
+            if (li.okey && li.okey[r] != key) continue;
+            const uint32_t p = cr[r];
+            late = uint16_t(p) < uint16_t(p >> 16);  // q4_is_late, query_q4.cpp:70-72
+        }
+        if (late) {
+            const uint32_t c = prio[o];
+            if (c < uint32_t(n_prio)) atomicAdd(&s_cnt[c], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < n_prio && s_cnt[threadIdx.x])
+        atomicAdd(counts + threadIdx.x, (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Q5: revenue of lineitems whose supplier nation is the customer's, per nation.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void q5_k(const int16_t* __restrict__ odate, int64_t o_rows,
+                                           const int32_t* __restrict__ okey,
+                                           const int16_t* __restrict__ cnation, LiSide li,
+                                           const int16_t* __restrict__ snation,
+                                           const int32_t* __restrict__ dprice,
+                                           const uint8_t* __restrict__ in_region,
+                                           int32_t n_nation, int32_t date_lo, int32_t date_hi,
+                                           unsigned long long* __restrict__ revenue) {
+    __shared__ unsigned long long s_rev[256];
+    __shared__ uint8_t s_in[256];
+    s_rev[threadIdx.x] = 0;
+    s_in[threadIdx.x] = threadIdx.x < n_nation ? in_region[threadIdx.x] : 0;
+    const Window w = wg_window(odate, o_rows, date_lo, date_hi);
+    __syncthreads();
+    for (int64_t o = w.s + threadIdx.x; o < w.e; o += kT) {
+        const int32_t cn = cnation[o];
+        if (cn < 0 || cn >= n_nation || !s_in[cn]) continue;
+        const uint32_t e = li.end[o];
+        if (e == 0) continue;
+        const int32_t key = li.okey ? okey[o] : 0;
+        int64_t local = 0;
+        for (uint32_t r = li.start[o]; r < e; ++r) {
+            if (li.okey && li.okey[r] != key) continue;
+            if (snation[r] == cn) local += dprice[r];
+        }
+        if (local) atomicAdd(&s_rev[cn], (unsigned long long)local);
+    }
+    __syncthreads();
+    if (threadIdx.x < n_nation && s_rev[threadIdx.x])
+        atomicAdd(revenue + threadIdx.x, s_rev[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Q10: per-order revenue of returned lineitems, summed per customer key.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void q10_orders_k(
+    const int16_t* __restrict__ odate, int64_t o_rows, const int32_t* __restrict__ okey,
+    const int32_t* __restrict__ ocust, LiSide li, const uint8_t* __restrict__ code,
+    int32_t code_lo, int32_t code_span, const int32_t* __restrict__ dprice, int32_t date_lo,
+    int32_t date_hi, int64_t cust_size, unsigned long long* __restrict__ rev,
+    uint8_t* __restrict__ has) {
+    const Window w = wg_window(odate, o_rows, date_lo, date_hi);
+    for (int64_t o = w.s + threadIdx.x; o < w.e; o += kT) {
+        const int32_t ck = ocust[o];
+        if (ck < 0 || ck >= cust_size) continue;
+        const uint32_t e = li.end[o];
+        if (e == 0) continue;
+        const int32_t key = li.okey ? okey[o] : 0;
+        int64_t order_rev = 0;
+        for (uint32_t r = li.start[o]; r < e; ++r) {
+            if (li.okey && li.okey[r] != key) continue;
+            if (uint32_t(int32_t(code[r]) - code_lo) >= uint32_t(code_span)) continue;
+            order_rev += dprice[r];
+        }
+        if (order_rev != 0) {
+            atomicAdd(rev + ck, (unsigned long long)order_rev);
+            has[ck] = 1;
+        }
+    }
+}
+
+// custkey -> (row + 1) of the last customer row with that key (atomic max).
+__global__ __launch_bounds__(kT) void cust_map_k(const int32_t* __restrict__ ck, int64_t rows,
+                                                 int64_t size, int32_t* __restrict__ map) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = ck[i];
+        if (k >= 0 && k < size) atomicMax(map + k, int32_t(i + 1));
+    }
+}
+
+// A kept Q10 group: has revenue, a customer row, a nation key in range.
+BOLAP_DEV bool q10_keep(int64_t k, const uint8_t* has, const int32_t* map,
+                        const int32_t* nation, int32_t n_names) {
+    if (!has[k]) return false;
+    const int32_t row = map[k] - 1;
+    if (row < 0) return false;
+    const int32_t nk = nation[row];
+    return nk >= 0 && nk < n_names;
+}
+
+// Ordered compaction of the kept keys, pass 1: kept keys per workgroup tile.
+constexpr int kTile = kT * 16;
+__global__ __launch_bounds__(kT) void q10_count_k(int64_t size, const uint8_t* __restrict__ has,
+                                                  const int32_t* __restrict__ map,
+                                                  const int32_t* __restrict__ nation,
+                                                  int32_t n_names, uint32_t* __restrict__ tiles) {
+    const int64_t t0 = int64_t(blockIdx.x) * kTile;
+    uint32_t n = 0;
+    for (int j = 0; j < 16; ++j) {
+        const int64_t k = t0 + int64_t(j) * kT + threadIdx.x;
+        if (k < size && q10_keep(k, has, map, nation, n_names)) ++n;
+    }
+    __shared__ uint32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    atomicAdd(&s, n);
+    __syncthreads();
+    if (threadIdx.x == 0) tiles[blockIdx.x] = s;
+}
+
+// Exclusive scan of the tile counts (one workgroup); total in tiles[n].
+__global__ __launch_bounds__(1024) void scan_tiles_k(uint32_t* __restrict__ tiles, int64_t n) {
+    __shared__ uint32_t s_carry;
+    __shared__ uint32_t s_w[1024 / kWave];
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t b = 0; b < n; b += 1024) {
+        const int64_t i = b + threadIdx.x;
+        const uint32_t v = i < n ? tiles[i] : 0;
+        // wave inclusive scan through LDS by wave totals
+        uint32_t x = v;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, kWave);
+            if ((threadIdx.x & (kWave - 1)) >= d) x += y;
+        }
+        if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_w[threadIdx.x / kWave] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int w = 0; w < int(threadIdx.x / kWave); ++w) before += s_w[w];
+        const uint32_t carry = s_carry;
+        if (i < n) tiles[i] = carry + before + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = carry + before + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tiles[n] = s_carry;
+}
+
+// Pass 2: each tile writes its kept keys in key order at its offset, as
+// (~revenue, custkey) pairs for the stable sort.
+__global__ __launch_bounds__(kT) void q10_write_k(int64_t size, const uint8_t* __restrict__ has,
+                                                  const int32_t* __restrict__ map,
+                                                  const int32_t* __restrict__ nation,
+                                                  int32_t n_names,
+                                                  const unsigned long long* __restrict__ rev,
+                                                  const uint32_t* __restrict__ tiles,
+                                                  uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+    __shared__ uint32_t s_w[kT / kWave];
+    const int64_t t0 = int64_t(blockIdx.x) * kTile;
+    uint32_t base = tiles[blockIdx.x];
+    for (int j = 0; j < 16; ++j) {
+        const int64_t k = t0 + int64_t(j) * kT + threadIdx.x;
+        const bool keep = k < size && q10_keep(k, has, map, nation, n_names);
+        const uint64_t bal = __ballot(keep);
+        const uint32_t lane = threadIdx.x & (kWave - 1);
+        const uint32_t before_lane = __popcll(bal & ((uint64_t(1) << lane) - 1));
+        if (lane == 0) s_w[threadIdx.x / kWave] = __popcll(bal);
+        __syncthreads();
+        uint32_t before = 0;
+        for (int w = 0; w < int(threadIdx.x / kWave); ++w) before += s_w[w];
+        uint32_t total = 0;
+        for (int w = 0; w < kT / kWave; ++w) total += s_w[w];
+        if (keep) {
+            const uint32_t at = base + before + before_lane;
+            keys[at] = ~uint64_t(rev[k]);
+            vals[at] = uint32_t(k);
+        }
+        base += total;
+        __syncthreads();
+    }
+}
+
+// Sorted pairs -> output rows {custkey, cust_row, revenue}.
+__global__ __launch_bounds__(kT) void q10_rows_k(int64_t n, const uint64_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ vals,
+                                                 const int32_t* __restrict__ map,
+                                                 bolap_q10_row* __restrict__ out) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const uint32_t k = vals[i];
+        out[i] = bolap_q10_row{int32_t(k), map[k] - 1, int64_t(~keys[i])};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Q12: lineitems by ship mode, split by the priority of their order.
+// ---------------------------------------------------------------------------
+struct Q12K {
+    int64_t rows;
+    const uint8_t* mode;
+    const uint32_t* cr;
+    const int16_t* ship;
+    const int32_t* okey;
+    int32_t mode1, mode2;
+    int32_t date_lo;
+    uint32_t span;
+    const int32_t* map;
+    int64_t map_size;
+    const uint8_t* prio;
+    int32_t high1, high2;
+};
+
+// Adds one passing row to the lane's four counters {high, low} x {mode1, mode2}.
+BOLAP_DEV void q12_row(const Q12K& a, int64_t i, int32_t m, uint32_t p, int32_t ship,
+                       uint32_t (&c)[4]) {
+    if (m != a.mode1 && m != a.mode2) return;
+    const int32_t commit = int16_t(p & 0xffff), receipt = int16_t(p >> 16);
+    if (commit >= receipt || ship >= commit) return;
+    if (uint32_t(receipt - a.date_lo) >= a.span) return;
+    const int32_t k = a.okey[i];
+    if (uint32_t(k) >= uint64_t(a.map_size)) return;
+    const int32_t row = a.map[k];
+    if (row < 0) return;
+    const int32_t pr = a.prio[row];
+    const int slot = m == a.mode1 ? 0 : 2;
+    c[slot + ((pr == a.high1 || pr == a.high2) ? 0 : 1)] += 1;
+}
+
+__global__ __launch_bounds__(kT) void q12_k(Q12K a, unsigned long long* __restrict__ out) {
+    uint32_t c[4] = {0, 0, 0, 0};
+    const int64_t units = a.rows / 4;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += stride) {
+        const uint32_t m4 = reinterpret_cast<const uint32_t*>(a.mode)[u];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int32_t m = (m4 >> (8 * j)) & 0xff;
+            any |= m == a.mode1 || m == a.mode2;
+        }
+        if (!any) continue;
+        const uint4 p4 = reinterpret_cast<const uint4*>(a.cr)[u];
+        const VecT<int16_t, 4> s4 = load_unit<int16_t, 4>(a.ship, u);
+        const uint32_t pv[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            q12_row(a, u * 4 + j, int32_t((m4 >> (8 * j)) & 0xff), pv[j], s4.v[j], c);
+    }
+    // tail rows
+    for (int64_t i = units * 4 + int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows; i += stride)
+        q12_row(a, i, a.mode[i], a.cr[i], a.ship[i], c);
+    __shared__ unsigned s[4];
+    if (threadIdx.x < 4) s[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t t = wave_sum_u64(c[j]);
+        if ((threadIdx.x & (kWave - 1)) == 0 && t) atomicAdd(&s[j], unsigned(t));
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s[threadIdx.x])
+        atomicAdd(out + threadIdx.x, (unsigned long long)s[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Q19: part mask (last part row of a key wins), then the lineitem probe.
+// ---------------------------------------------------------------------------
+struct Q19Part {
+    int64_t rows;
+    const int32_t* key;
+    const void* brand;
+    int32_t brand_dtype;
+    const void* container;
+    int32_t container_dtype;
+    const int32_t* size;
+    bolap_q19_group g[3];
+};
+
+BOLAP_DEV int32_t code_at(const void* col, int32_t dtype, int64_t i) {
+    return dtype == BOLAP_U8 ? int32_t(static_cast<const uint8_t*>(col)[i])
+                             : int32_t(static_cast<const int16_t*>(col)[i]);
+}
+
+__global__ __launch_bounds__(kT) void q19_part_k(Q19Part a, int64_t size,
+                                                 int32_t* __restrict__ owner) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = a.key[i];
+        if (k < 0 || k >= size) continue;
+        const int32_t b = code_at(a.brand, a.brand_dtype, i);
+        const int32_t c = code_at(a.container, a.container_dtype, i);
+        const int32_t sz = a.size[i];
+        uint32_t m = 0;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            const bolap_q19_group& G = a.g[g];
+            const bool in = c == G.containers[0] || c == G.containers[1] ||
+                            c == G.containers[2] || c == G.containers[3];
+            if (G.enabled && b == G.brand && c >= 0 && in && sz >= 1 && sz <= G.size_hi)
+                m |= 1u << g;
+        }
+        // rows numbered from 1 so that 0 means "no part row"
+        atomicMax(owner + k, int32_t((uint32_t(i + 1) << 3) | m));
+    }
+}
+
+__global__ __launch_bounds__(kT) void q19_narrow_k(const int32_t* __restrict__ owner,
+                                                   int64_t size, uint8_t* __restrict__ mask) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < size;
+         i += int64_t(gridDim.x) * kT)
+        mask[i] = uint8_t(owner[i] & 7);
+}
+
+struct Q19Li {
+    int64_t rows;
+    const uint8_t* mode;
+    const uint8_t* instruct;
+    const int32_t* pkey;
+    const int16_t* qty;
+    const int32_t* price;
+    const uint8_t* disc;
+    int32_t mode1, mode2, instr;
+    int32_t qlo[3];
+    const uint8_t* mask;
+    int64_t size;
+};
+
+BOLAP_DEV int64_t q19_row(const Q19Li& a, int64_t i, int32_t pk) {
+    if (pk < 0 || pk >= a.size) return 0;
+    const uint32_t m = a.mask[pk];
+    if (!m) return 0;
+    const int32_t q = a.qty[i];
+    bool ok = false;
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+        ok |= ((m >> g) & 1) && q >= a.qlo[g] && q <= a.qlo[g] + 1000;
+    if (!ok) return 0;
+    return int64_t(a.price[i]) * int64_t(100 - int32_t(a.disc[i]));
+}
+
+BOLAP_DEV bool q19_codes(const Q19Li& a, int32_t m, int32_t ins) {
+    return (m == a.mode1 || m == a.mode2) && ins == a.instr;
+}
+
+__global__ __launch_bounds__(kT) void q19_li_k(Q19Li a, unsigned long long* __restrict__ out) {
+    int64_t acc = 0;
+    const int64_t units = a.rows / 4;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += stride) {
+        const uint32_t m4 = reinterpret_cast<const uint32_t*>(a.mode)[u];
+        const uint32_t i4 = reinterpret_cast<const uint32_t*>(a.instruct)[u];
+        uint32_t pass = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            pass |= uint32_t(q19_codes(a, (m4 >> (8 * j)) & 0xff, (i4 >> (8 * j)) & 0xff)) << j;
+        if (!pass) continue;
+        const VecT<int32_t, 4> k4 = load_unit<int32_t, 4>(a.pkey, u);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if ((pass >> j) & 1) acc += q19_row(a, u * 4 + j, k4.v[j]);
+    }
+    for (int64_t i = units * 4 + int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows; i += stride)
+        if (q19_codes(a, a.mode[i], a.instruct[i])) acc += q19_row(a, i, a.pkey[i]);
+    __shared__ uint64_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const uint64_t t = wave_sum_u64(uint64_t(acc));
+    if ((threadIdx.x & (kWave - 1)) == 0 && t) atomicAdd((unsigned long long*)&s, t);
+    __syncthreads();
+    if (threadIdx.x == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------------------
+// Small helpers.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void fill_bytes_k(uint32_t* __restrict__ p, int64_t words) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < words;
+         i += int64_t(gridDim.x) * kT)
+        p[i] = 0;
+}
+
+// Maximum of an i32 column into *out (which starts at INT32_MIN).
+__global__ __launch_bounds__(kT) void max_i32_k(const int32_t* __restrict__ v, int64_t n,
+                                                int32_t* __restrict__ out) {
+    int32_t m = INT32_MIN;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        m = v[i] > m ? v[i] : m;
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const int32_t o = __shfl_xor(m, d, kWave);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && m != INT32_MIN) atomicMax(out, m);
+}
+
+inline unsigned grid_of(bolap_ctx* ctx, int64_t items, int per_cu = 4) {
+    int64_t b = (items + kT - 1) / kT;
+    const int64_t cap = int64_t(ctx->num_cus) * per_cu;
+    if (b > cap) b = cap;
+    return unsigned(b < 1 ? 1 : b);
+}
+
+// Zeroes `bytes` (a multiple of 4) of device memory with a kernel (so the
+// calls stay free of memset nodes when a caller captures them).
+int zero_dev(bolap_ctx* ctx, void* p, size_t bytes) {
+    const int64_t words = int64_t((bytes + 3) / 4);
+    hipLaunchKernelGGL(fill_bytes_k, dim3(grid_of(ctx, words)), dim3(kT), 0, ctx->stream,
+                       static_cast<uint32_t*>(p), words);
+    return launch_check("fill_bytes_k");
+}
+
+// Copies `bytes` of device results to the host and waits for them.
+int fetch(bolap_ctx* ctx, void* host, const void* dev, size_t bytes) {
+    BOLAP_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BOLAP_OK;
+}
+
+// Maximum of an i32 column (INT32_MIN when empty), read back to the host.
+int max_i32(bolap_ctx* ctx, const int32_t* v, int64_t n, int32_t* slot, int32_t* out) {
+    const int32_t init = INT32_MIN;
+    BOLAP_HIP_TRY(hipMemcpyAsync(slot, &init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+    if (n > 0) {
+        hipLaunchKernelGGL(max_i32_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream, v, n, slot);
+        BOLAP_TRY(launch_check("max_i32_k"));
+    }
+    return fetch(ctx, out, slot, sizeof(int32_t));
+}
+
+int check_ranges(const bolap_order_side* o, const char* who) {
+    BOLAP_ARG_CHECK(o, "%s: null order side", who);
+    BOLAP_ARG_CHECK(o->order_rows >= 0 && o->order_rows < (int64_t(1) << 31),
+                    "%s: order rows %lld", who, (long long)o->order_rows);
+    BOLAP_ARG_CHECK(o->order_rows == 0 || (o->o_orderdate && o->range_start && o->range_end),
+                    "%s: null order column", who);
+    BOLAP_ARG_CHECK(o->l_orderkey_sorted || o->order_rows == 0 ||
+                        (o->l_orderkey && o->o_orderkey),
+                    "%s: unsorted lineitem needs l_orderkey and o_orderkey", who);
+    return BOLAP_OK;
+}
+
+LiSide li_side(const bolap_order_side* o) {
+    return LiSide{o->range_start, o->range_end, o->l_orderkey_sorted ? nullptr : o->l_orderkey};
+}
+
+unsigned window_grid(bolap_ctx* ctx, int64_t rows) {
+    // enough workgroups for a window of any size; each searches its window
+    return grid_of(ctx, rows, 2);
+}
+
+}  // namespace
+}  // namespace bolap
+
+using namespace bolap;
+
+extern "C" int bolap_q4(bolap_ctx* ctx, const bolap_q4_args* a, int64_t* counts) {
+    BOLAP_ARG_CHECK(ctx && a && counts, "bolap_q4: null argument");
+    BOLAP_TRY(check_ranges(&a->orders, "bolap_q4"));
+    BOLAP_ARG_CHECK(a->n_priority >= 0 && a->n_priority <= 256, "bolap_q4: %d priority codes",
+                    a->n_priority);
+    const bolap_order_side& o = a->orders;
+    BOLAP_ARG_CHECK(o.order_rows == 0 || (a->o_orderpriority && a->l_commit_receipt),
+                    "bolap_q4: null column");
+    auto* res = static_cast<unsigned long long*>(ctx->result_dev);
+    BOLAP_TRY(zero_dev(ctx, res, 256 * 8));
+    if (o.order_rows > 0 && a->n_priority > 0) {
+        TraceScope ts(ctx, "q4_orders");
+        hipLaunchKernelGGL(q4_k, dim3(window_grid(ctx, o.order_rows)), dim3(kT), 0, ctx->stream,
+                           o.o_orderdate, o.order_rows, o.o_orderkey, a->o_orderpriority,
+                           a->n_priority, li_side(&o), a->l_commit_receipt, a->date_lo,
+                           a->date_hi, res);
+        BOLAP_TRY(launch_check("q4_k"));
+    }
+    if (a->n_priority > 0) BOLAP_TRY(fetch(ctx, counts, res, size_t(a->n_priority) * 8));
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_q5(bolap_ctx* ctx, const bolap_q5_args* a, int64_t* revenue) {
+    BOLAP_ARG_CHECK(ctx && a && revenue, "bolap_q5: null argument");
+    BOLAP_TRY(check_ranges(&a->orders, "bolap_q5"));
+    BOLAP_ARG_CHECK(a->n_nation >= 0 && a->n_nation <= 256, "bolap_q5: %d nations (at most 256)",
+                    a->n_nation);
+    const bolap_order_side& o = a->orders;
+    BOLAP_ARG_CHECK(o.order_rows == 0 || (a->o_cust_nation && a->l_supp_nation &&
+                                          a->l_disc_price && a->nation_in_region),
+                    "bolap_q5: null column");
+    auto* res = static_cast<unsigned long long*>(ctx->result_dev);
+    uint8_t* inreg = reinterpret_cast<uint8_t*>(res + 256);
+    BOLAP_TRY(zero_dev(ctx, res, 256 * 8));
+    if (o.order_rows > 0 && a->n_nation > 0) {
+        BOLAP_HIP_TRY(hipMemcpyAsync(inreg, a->nation_in_region, size_t(a->n_nation),
+                                     hipMemcpyHostToDevice, ctx->stream));
+        TraceScope ts(ctx, "q5_orders");
+        hipLaunchKernelGGL(q5_k, dim3(window_grid(ctx, o.order_rows)), dim3(kT), 0, ctx->stream,
+                           o.o_orderdate, o.order_rows, o.o_orderkey, a->o_cust_nation,
+                           li_side(&o), a->l_supp_nation, a->l_disc_price, inreg, a->n_nation,
+                           a->date_lo, a->date_hi, res);
+        BOLAP_TRY(launch_check("q5_k"));
+    }
+    if (a->n_nation > 0) BOLAP_TRY(fetch(ctx, revenue, res, size_t(a->n_nation) * 8));
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_q10(bolap_ctx* ctx, const bolap_q10_args* a, bolap_q10_row* rows,
+                         int64_t max_rows, int64_t* n_rows) {
+    BOLAP_ARG_CHECK(ctx && a && n_rows && (rows || max_rows == 0), "bolap_q10: null argument");
+    BOLAP_TRY(check_ranges(&a->orders, "bolap_q10"));
+    const bolap_order_side& o = a->orders;
+    BOLAP_ARG_CHECK(o.order_rows == 0 || (a->o_custkey && a->l_code && a->l_disc_price),
+                    "bolap_q10: null column");
+    BOLAP_ARG_CHECK(a->cust_rows >= 0 && (a->cust_rows == 0 || (a->c_custkey && a->c_nationkey)),
+                    "bolap_q10: null customer column");
+    *n_rows = 0;
+    if (a->code_lo < 0 || a->code_span <= 0) return BOLAP_OK;  // no 'R' in the dictionary
+    // customer key domain [0, max key] (q10::max_value; 0 for no customers)
+    int32_t* slot = static_cast<int32_t*>(ctx->result_dev);
+    int32_t kmax = 0;
+    if (a->cust_rows > 0) BOLAP_TRY(max_i32(ctx, a->c_custkey, a->cust_rows, slot, &kmax));
+    if (kmax < 0) return BOLAP_OK;
+    const int64_t size = int64_t(kmax) + 1;
+    const int64_t tiles = (size + kTile - 1) / kTile;
+    WsArena ws;
+    const size_t o_rev = ws.plan(size_t(size) * 8);
+    const size_t o_has = ws.plan(size_t(size));
+    const size_t o_map = ws.plan(size_t(size) * 4);
+    const size_t o_tiles = ws.plan(size_t(tiles + 1) * 4);
+    const size_t o_keys = ws.plan(size_t(size) * 8);
+    const size_t o_vals = ws.plan(size_t(size) * 4);
+    const size_t o_out = ws.plan(size_t(size) * sizeof(bolap_q10_row));
+    const size_t o_sort = ws.plan(radix_sort_scratch_bytes(ctx, size));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* rev = ws.at<unsigned long long>(o_rev);
+    auto* has = ws.at<uint8_t>(o_has);
+    auto* map = ws.at<int32_t>(o_map);
+    auto* tl = ws.at<uint32_t>(o_tiles);
+    // rev, has and map are contiguous regions: one clearing launch
+    BOLAP_TRY(zero_dev(ctx, rev, o_tiles - o_rev));
+    {
+        TraceScope ts(ctx, "q10_orders");
+        hipLaunchKernelGGL(cust_map_k, dim3(grid_of(ctx, a->cust_rows)), dim3(kT), 0,
+                           ctx->stream, a->c_custkey, a->cust_rows, size, map);
+        BOLAP_TRY(launch_check("cust_map_k"));
+        if (o.order_rows > 0) {
+            hipLaunchKernelGGL(q10_orders_k, dim3(window_grid(ctx, o.order_rows)), dim3(kT), 0,
+                               ctx->stream, o.o_orderdate, o.order_rows, o.o_orderkey, a->o_custkey,
+                               li_side(&o), a->l_code, a->code_lo, a->code_span, a->l_disc_price,
+                               a->date_lo, a->date_hi, size, rev, has);
+            BOLAP_TRY(launch_check("q10_orders_k"));
+        }
+    }
+    {
+        TraceScope ts(ctx, "q10_compact");
+        hipLaunchKernelGGL(q10_count_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, size, has,
+                           map, a->c_nationkey, a->n_nation_names, tl);
+        hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(1024), 0, ctx->stream, tl, tiles);
+        hipLaunchKernelGGL(q10_write_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, size, has,
+                           map, a->c_nationkey, a->n_nation_names, rev, tl,
+                           ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals));
+        BOLAP_TRY(launch_check("q10_write_k"));
+    }
+    uint32_t n = 0;
+    BOLAP_TRY(fetch(ctx, &n, tl + tiles, sizeof(n)));
+    *n_rows = n;
+    trace_count(ctx, "q10_agg_groups_created", n);
+    if (n > max_rows) {
+        set_error("bolap_q10: %u result rows do not fit %lld", n, (long long)max_rows);
+        return BOLAP_ERR_CAPACITY;
+    }
+    if (n == 0) return BOLAP_OK;
+    {
+        TraceScope ts(ctx, "q10_sort");
+        int passes = 0;
+        BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals), n, 64,
+                                   ws.at<void>(o_sort), &passes));
+        hipLaunchKernelGGL(q10_rows_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream, int64_t(n),
+                           ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals), map,
+                           ws.at<bolap_q10_row>(o_out));
+        BOLAP_TRY(launch_check("q10_rows_k"));
+    }
+    return fetch(ctx, rows, ws.at<bolap_q10_row>(o_out), size_t(n) * sizeof(bolap_q10_row));
+}
+
+extern "C" int bolap_q12(bolap_ctx* ctx, const bolap_q12_args* a, bolap_q12_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_q12: null argument");
+    BOLAP_ARG_CHECK(a->li_rows >= 0, "bolap_q12: negative rows");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (a->l_shipmode && a->l_commit_receipt && a->l_shipdate &&
+                                        a->l_orderkey && a->order_map && a->o_orderpriority),
+                    "bolap_q12: null column");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (aligned(a->l_shipmode, 4) &&
+                                        aligned(a->l_commit_receipt, 16) &&
+                                        aligned(a->l_shipdate, 8)),
+                    "bolap_q12: l_shipmode / l_commit_receipt / l_shipdate must be 4 / 16 / 8-"
+                    "byte aligned");
+    std::memset(out, 0, sizeof(*out));
+    if (a->mode1 < 0 && a->mode2 < 0) return BOLAP_OK;
+    auto* res = static_cast<unsigned long long*>(ctx->result_dev);
+    BOLAP_TRY(zero_dev(ctx, res, 4 * 8));
+    if (a->li_rows > 0) {
+        const Q12K k{a->li_rows, a->l_shipmode, a->l_commit_receipt, a->l_shipdate, a->l_orderkey,
+                     a->mode1, a->mode2, a->date_lo, uint32_t(a->date_hi - a->date_lo),
+                     a->order_map, a->map_size, a->o_orderpriority, a->high1, a->high2};
+        TraceScope ts(ctx, "q12_lineitem");
+        hipLaunchKernelGGL(q12_k, dim3(grid_of(ctx, (a->li_rows + 3) / 4, 8)), dim3(kT), 0,
+                           ctx->stream, k, res);
+        BOLAP_TRY(launch_check("q12_k"));
+    }
+    unsigned long long c[4];
+    BOLAP_TRY(fetch(ctx, c, res, sizeof(c)));
+    out->high[0] = int64_t(c[0]);
+    out->low[0] = int64_t(c[1]);
+    out->high[1] = int64_t(c[2]);
+    out->low[1] = int64_t(c[3]);
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_q19(bolap_ctx* ctx, const bolap_q19_args* a, int64_t* revenue) {
+    BOLAP_ARG_CHECK(ctx && a && revenue, "bolap_q19: null argument");
+    BOLAP_ARG_CHECK(a->part_rows >= 0 && a->li_rows >= 0, "bolap_q19: negative rows");
+    BOLAP_ARG_CHECK(a->part_rows == 0 || (a->p_partkey && a->p_brand && a->p_container &&
+                                          a->p_size),
+                    "bolap_q19: null part column");
+    BOLAP_ARG_CHECK((a->p_brand_dtype == BOLAP_U8 || a->p_brand_dtype == BOLAP_I16) &&
+                        (a->p_container_dtype == BOLAP_U8 || a->p_container_dtype == BOLAP_I16),
+                    "bolap_q19: brand / container codes must be u8 or i16");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (a->l_partkey && a->l_quantity && a->l_price &&
+                                        a->l_disc && a->l_shipmode && a->l_shipinstruct),
+                    "bolap_q19: null lineitem column");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (aligned(a->l_shipmode, 4) &&
+                                        aligned(a->l_shipinstruct, 4) &&
+                                        aligned(a->l_partkey, 16)),
+                    "bolap_q19: l_shipmode / l_shipinstruct / l_partkey must be 4 / 4 / 16-byte "
+                    "aligned");
+    BOLAP_ARG_CHECK(a->part_rows < (int64_t(1) << 28) - 1, "bolap_q19: %lld part rows (at most "
+                    "2^28 - 2)", (long long)a->part_rows);
+    *revenue = 0;
+    const bool any = a->groups[0].enabled || a->groups[1].enabled || a->groups[2].enabled;
+    if (!any || a->shipinstruct < 0 || (a->mode1 < 0 && a->mode2 < 0) || a->part_rows == 0 ||
+        a->li_rows == 0)
+        return BOLAP_OK;
+    int32_t* slot = static_cast<int32_t*>(ctx->result_dev);
+    int32_t kmax = 0;
+    BOLAP_TRY(max_i32(ctx, a->p_partkey, a->part_rows, slot, &kmax));
+    if (kmax < 0) return BOLAP_OK;
+    const int64_t size = int64_t(kmax) + 1;
+    WsArena ws;
+    const size_t o_owner = ws.plan(size_t(size) * 4);
+    const size_t o_mask = ws.plan(size_t(size));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* res = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->result_dev) + 256);
+    BOLAP_TRY(zero_dev(ctx, res, 8));
+    BOLAP_TRY(zero_dev(ctx, ws.at<int32_t>(o_owner), size_t(size) * 4));
+    Q19Part p{a->part_rows, a->p_partkey, a->p_brand, a->p_brand_dtype, a->p_container,
+              a->p_container_dtype, a->p_size, {a->groups[0], a->groups[1], a->groups[2]}};
+    {
+        TraceScope ts(ctx, "q19_part_scan");
+        hipLaunchKernelGGL(q19_part_k, dim3(grid_of(ctx, a->part_rows)), dim3(kT), 0, ctx->stream,
+                           p, size, ws.at<int32_t>(o_owner));
+        hipLaunchKernelGGL(q19_narrow_k, dim3(grid_of(ctx, size)), dim3(kT), 0, ctx->stream,
+                           ws.at<int32_t>(o_owner), size, ws.at<uint8_t>(o_mask));
+        BOLAP_TRY(launch_check("q19_narrow_k"));
+    }
+    Q19Li l{a->li_rows, a->l_shipmode, a->l_shipinstruct, a->l_partkey, a->l_quantity, a->l_price,
+            a->l_disc, a->mode1, a->mode2, a->shipinstruct,
+            {a->groups[0].qty_lo, a->groups[1].qty_lo, a->groups[2].qty_lo},
+            ws.at<uint8_t>(o_mask), size};
+    {
+        TraceScope ts(ctx, "q19_lineitem_scan");
+        hipLaunchKernelGGL(q19_li_k, dim3(grid_of(ctx, (a->li_rows + 3) / 4, 8)), dim3(kT), 0,
+                           ctx->stream, l, res);
+        BOLAP_TRY(launch_check("q19_li_k"));
+    }
+    return fetch(ctx, revenue, res, 8);
+}

@@ -498,6 +498,92 @@ class Engine:
         return {"promo_sum": r.promo_sum, "total_sum": r.total_sum,
                 "rows_emitted": r.rows_emitted, "promo_revenue": r.promo_revenue}
 
+    # ------------------------------------- Q4, Q5, Q10, Q12, Q19 (tpch_orders)
+    @staticmethod
+    def order_side(t: dict) -> N.OrderSide:
+        """t: reference-layout device columns o_orderdate (sorted), o_orderkey,
+        r_start, r_end, l_orderkey and the flag l_sorted."""
+        o = N.OrderSide()
+        o.order_rows = int(t["o_orderdate"].numel())
+        o.o_orderdate, o.o_orderkey = ptr(t["o_orderdate"]), ptr(t["o_orderkey"])
+        o.range_start, o.range_end = ptr(t["r_start"]), ptr(t["r_end"])
+        o.l_orderkey, o.l_orderkey_sorted = ptr(t["l_orderkey"]), int(t["l_sorted"])
+        return o
+
+    def q4(self, t: dict, n_priority: int, date_lo: int, date_hi: int) -> list:
+        a = N.Q4Args()
+        a.orders = self.order_side(t)
+        a.o_orderpriority, a.n_priority = ptr(t["o_orderpriority"]), int(n_priority)
+        a.l_commit_receipt = ptr(t["l_commit_receipt"])
+        a.date_lo, a.date_hi = int(date_lo), int(date_hi)
+        out = (C.c_int64 * max(1, n_priority))()
+        N.check(self.lib.bolap_q4(self.ctx, C.byref(a), out))
+        return list(out)[:n_priority]
+
+    def q5(self, t: dict, nation_in_region, date_lo: int, date_hi: int) -> list:
+        import numpy as np
+        nr = np.ascontiguousarray(nation_in_region, np.uint8)
+        a = N.Q5Args()
+        a.orders = self.order_side(t)
+        a.o_cust_nation, a.l_supp_nation = ptr(t["o_cust_nation"]), ptr(t["l_supp_nation"])
+        a.l_disc_price = ptr(t["l_disc_price"])
+        a.nation_in_region, a.n_nation = nr.ctypes.data, len(nr)
+        a.date_lo, a.date_hi = int(date_lo), int(date_hi)
+        out = (C.c_int64 * max(1, len(nr)))()
+        N.check(self.lib.bolap_q5(self.ctx, C.byref(a), out))
+        return list(out)[:len(nr)]
+
+    def q10(self, t: dict, code_lo: int, code_span: int, date_lo: int, date_hi: int,
+            n_nation_names: int, max_rows: int | None = None):
+        a = N.Q10Args()
+        a.orders = self.order_side(t)
+        a.o_custkey, a.l_code = ptr(t["o_custkey"]), ptr(t["l_code"])
+        a.code_lo, a.code_span = int(code_lo), int(code_span)
+        a.l_disc_price = ptr(t["l_disc_price"])
+        a.date_lo, a.date_hi = int(date_lo), int(date_hi)
+        a.cust_rows = int(t["c_custkey"].numel())
+        a.c_custkey, a.c_nationkey = ptr(t["c_custkey"]), ptr(t["c_nationkey"])
+        a.n_nation_names = int(n_nation_names)
+        cap = max(1, a.cust_rows if max_rows is None else max_rows)
+        rows = (N.Q10Row * cap)()
+        n = C.c_int64(0)
+        N.check(self.lib.bolap_q10(self.ctx, C.byref(a), rows, cap, C.byref(n)))
+        return [(rows[i].custkey, rows[i].cust_row, rows[i].revenue) for i in range(n.value)]
+
+    def q12(self, t: dict, mode1: int, mode2: int, date_lo: int, date_hi: int, high1: int,
+            high2: int):
+        a = N.Q12Args()
+        a.li_rows = int(t["l_shipmode"].numel())
+        a.l_shipmode, a.l_commit_receipt = ptr(t["l_shipmode"]), ptr(t["l_commit_receipt"])
+        a.l_shipdate, a.l_orderkey = ptr(t["l_shipdate"]), ptr(t["l_orderkey"])
+        a.mode1, a.mode2, a.date_lo, a.date_hi = int(mode1), int(mode2), int(date_lo), int(date_hi)
+        a.order_map, a.map_size = ptr(t["order_map"]), int(t["order_map"].numel())
+        a.o_orderpriority, a.high1, a.high2 = ptr(t["o_orderpriority"]), int(high1), int(high2)
+        r = N.Q12Result()
+        N.check(self.lib.bolap_q12(self.ctx, C.byref(a), C.byref(r)))
+        return (r.high[0], r.low[0]), (r.high[1], r.low[1])
+
+    def q19(self, t: dict, groups, mode1: int, mode2: int, instruct: int) -> int:
+        """groups: 3 tuples (enabled, brand, [4 container codes], size_hi, qty_lo)."""
+        a = N.Q19Args()
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_size = ptr(t["p_partkey"]), ptr(t["p_size"])
+        a.p_brand, a.p_brand_dtype = ptr(t["p_brand"]), dtype_code(t["p_brand"])
+        a.p_container, a.p_container_dtype = ptr(t["p_container"]), dtype_code(t["p_container"])
+        a.li_rows = int(t["l_partkey"].numel())
+        a.l_partkey, a.l_quantity = ptr(t["l_partkey"]), ptr(t["l_quantity"])
+        a.l_price, a.l_disc = ptr(t["l_extendedprice"]), ptr(t["l_discount"])
+        a.l_shipmode, a.l_shipinstruct = ptr(t["l_shipmode"]), ptr(t["l_shipinstruct"])
+        for g, (en, br, cont, sh, ql) in enumerate(groups):
+            a.groups[g].enabled, a.groups[g].brand = int(en), int(br)
+            a.groups[g].size_hi, a.groups[g].qty_lo = int(sh), int(ql)
+            for c in range(4):
+                a.groups[g].containers[c] = int(cont[c])
+        a.mode1, a.mode2, a.shipinstruct = int(mode1), int(mode2), int(instruct)
+        r = C.c_int64(0)
+        N.check(self.lib.bolap_q19(self.ctx, C.byref(a), C.byref(r)))
+        return r.value
+
     def q17_args(self, t: dict, preds: list) -> N.Q17Args:
         a = N.Q17Args()
         a.part_rows = int(t["p_partkey"].numel())

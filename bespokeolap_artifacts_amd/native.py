@@ -250,6 +250,60 @@ class Q17Result(C.Structure):
                 ("matches", C.c_int64), ("rows_emitted", C.c_int64)]
 
 
+class OrderSide(C.Structure):
+    _fields_ = [("order_rows", C.c_int64), ("o_orderdate", vp), ("o_orderkey", vp),
+                ("range_start", vp), ("range_end", vp), ("l_orderkey", vp),
+                ("l_orderkey_sorted", C.c_int32)]
+
+
+class Q4Args(C.Structure):
+    _fields_ = [("orders", OrderSide), ("o_orderpriority", vp), ("n_priority", C.c_int32),
+                ("l_commit_receipt", vp), ("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q5Args(C.Structure):
+    _fields_ = [("orders", OrderSide), ("o_cust_nation", vp), ("l_supp_nation", vp),
+                ("l_disc_price", vp), ("nation_in_region", vp), ("n_nation", C.c_int32),
+                ("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q10Args(C.Structure):
+    _fields_ = [("orders", OrderSide), ("o_custkey", vp), ("l_code", vp),
+                ("code_lo", C.c_int32), ("code_span", C.c_int32), ("l_disc_price", vp),
+                ("date_lo", C.c_int32), ("date_hi", C.c_int32), ("cust_rows", C.c_int64),
+                ("c_custkey", vp), ("c_nationkey", vp), ("n_nation_names", C.c_int32)]
+
+
+class Q10Row(C.Structure):
+    _fields_ = [("custkey", C.c_int32), ("cust_row", C.c_int32), ("revenue", C.c_int64)]
+
+
+class Q12Args(C.Structure):
+    _fields_ = [("li_rows", C.c_int64), ("l_shipmode", vp), ("l_commit_receipt", vp),
+                ("l_shipdate", vp), ("l_orderkey", vp), ("mode1", C.c_int32),
+                ("mode2", C.c_int32), ("date_lo", C.c_int32), ("date_hi", C.c_int32),
+                ("order_map", vp), ("map_size", C.c_int64), ("o_orderpriority", vp),
+                ("high1", C.c_int32), ("high2", C.c_int32)]
+
+
+class Q12Result(C.Structure):
+    _fields_ = [("high", C.c_int64 * 2), ("low", C.c_int64 * 2)]
+
+
+class Q19Group(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("brand", C.c_int32), ("containers", C.c_int32 * 4),
+                ("size_hi", C.c_int32), ("qty_lo", C.c_int32)]
+
+
+class Q19Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_partkey", vp), ("p_brand", vp),
+                ("p_brand_dtype", C.c_int32), ("p_container", vp),
+                ("p_container_dtype", C.c_int32), ("p_size", vp), ("li_rows", C.c_int64),
+                ("l_partkey", vp), ("l_quantity", vp), ("l_price", vp), ("l_disc", vp),
+                ("l_shipmode", vp), ("l_shipinstruct", vp), ("groups", Q19Group * 3),
+                ("mode1", C.c_int32), ("mode2", C.c_int32), ("shipinstruct", C.c_int32)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -313,6 +367,11 @@ PROTOTYPES = {
     "bolap_q18_write_csv": (C.c_int, [C.c_char_p, P(Q18Row), i64, P(C.c_char_p), i64]),
     "bolap_q14_parse": (C.c_int, [C.c_char_p, P(i32), P(i32)]),
     "bolap_q14": (C.c_int, [vp, P(Q14Args), P(Q14Result)]),
+    "bolap_q4": (C.c_int, [vp, P(Q4Args), P(i64)]),
+    "bolap_q5": (C.c_int, [vp, P(Q5Args), P(i64)]),
+    "bolap_q10": (C.c_int, [vp, P(Q10Args), P(Q10Row), i64, P(i64)]),
+    "bolap_q12": (C.c_int, [vp, P(Q12Args), P(Q12Result)]),
+    "bolap_q19": (C.c_int, [vp, P(Q19Args), P(i64)]),
     "bolap_q14_write_csv": (C.c_int, [C.c_char_p, P(Q14Result)]),
     "bolap_q17": (C.c_int, [vp, P(Q17Args), P(Q17Result)]),
     "bolap_q17_write_csv": (C.c_int, [C.c_char_p, P(Q17Result)]),

@@ -801,6 +801,153 @@ int bolap_q14_write_csv(const char* path, const bolap_q14_result* result);
 int bolap_q17_write_csv(const char* path, const bolap_q17_result* result);
 
 /* ------------------------------------------------------------------------ */
+/* Order-range queries Q4, Q5, Q10 over the reference layout: orders sorted  */
+/* by orderdate (builder_impl.cpp:597-608) and each order's lineitem rows as */
+/* lineitem_ranges [start, end), {0, 0} for none (:929-991, as               */
+/* bolap_order_ranges makes them).  A date window [date_lo, date_hi) is found */
+/* on the device with a lower_bound over o_orderdate, as the reference's     */
+/* std::lower_bound does.  With l_orderkey_sorted = 0 the ranges may be      */
+/* min/max spans and each lineitem's key is checked against its order's, as  */
+/* the queries' unsorted branches do.                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t order_rows;
+    const int16_t* o_orderdate;      /* non-decreasing day offsets */
+    const int32_t* o_orderkey;       /* needed when l_orderkey_sorted == 0 */
+    const uint32_t* range_start;
+    const uint32_t* range_end;
+    const int32_t* l_orderkey;       /* needed when l_orderkey_sorted == 0 */
+    int32_t l_orderkey_sorted;       /* lineitem.orderkey_sorted */
+} bolap_order_side;
+
+/* Q4 (bespoke_tpch/query_q4.cpp:140-268): orders of [date_lo, date_hi) with
+   a lineitem whose commitdate < receiptdate (the packed commit_receipt
+   halves compared as uint16, :70-72), counted per priority code;
+   counts[n_priority] (host).  Codes >= n_priority are not counted. */
+typedef struct {
+    bolap_order_side orders;
+    const uint8_t* o_orderpriority;  /* codes */
+    int32_t n_priority;              /* <= 256 */
+    const uint32_t* l_commit_receipt; /* receipt << 16 | commit (builder_impl.cpp:536-544) */
+    int32_t date_lo, date_hi;
+} bolap_q4_args;
+int bolap_q4(bolap_ctx* ctx, const bolap_q4_args* args, int64_t* counts);
+
+/* Q5 (bespoke_tpch/query_q5.cpp:190-440): orders of [date_lo, date_hi) whose
+   customer nation (orders.cust_nationkey, builder_impl.cpp:913-928) is
+   marked in nation_in_region (a HOST array of n_nation bytes, n_nation =
+   max nation key + 1 <= 256); per order the sum of discounted_price over its
+   lineitems whose supplier nation (lineitem.supp_nationkey, :838-853) is the
+   customer's.  revenue[n_nation] (host), exact int64. */
+typedef struct {
+    bolap_order_side orders;
+    const int16_t* o_cust_nation;
+    const int16_t* l_supp_nation;
+    const int32_t* l_disc_price;
+    const uint8_t* nation_in_region;
+    int32_t n_nation;
+    int32_t date_lo, date_hi;
+} bolap_q5_args;
+int bolap_q5(bolap_ctx* ctx, const bolap_q5_args* args, int64_t* revenue);
+
+/* Q10 (bespoke_tpch/query_q10.cpp:375-560): orders of [date_lo, date_hi)
+   with a customer key in [0, max c_custkey]; per order the sum of
+   discounted_price over lineitems whose returnflag_linestatus code c has
+   (u32)(c - code_lo) < code_span (returnflag 'R': code_lo = R * n_linestatus,
+   code_span = n_linestatus; code_lo < 0 gives no rows); orders with a
+   nonzero sum grouped by customer; a group is kept when its key has a
+   customer row (the last row of a key) whose c_nationkey is in
+   [0, n_nation_names).  Rows in the reference's order: revenue descending as
+   an unsigned 64-bit value, ties by customer key ascending (its stable LSD
+   passes, :318-373).  *n_rows = number of groups; BOLAP_ERR_CAPACITY when
+   it exceeds max_rows. */
+typedef struct {
+    bolap_order_side orders;
+    const int32_t* o_custkey;
+    const uint8_t* l_code;           /* returnflag_linestatus */
+    int32_t code_lo, code_span;
+    const int32_t* l_disc_price;
+    int32_t date_lo, date_hi;
+    int64_t cust_rows;
+    const int32_t* c_custkey;
+    const int32_t* c_nationkey;
+    int32_t n_nation_names;          /* max nation key + 1 */
+} bolap_q10_args;
+typedef struct {
+    int32_t custkey;
+    int32_t cust_row;                /* customer row (name, address, ... on the host) */
+    int64_t revenue;                 /* discounted_price units (cents) */
+} bolap_q10_row;
+int bolap_q10(bolap_ctx* ctx, const bolap_q10_args* args, bolap_q10_row* rows,
+              int64_t max_rows, int64_t* n_rows);
+
+/* Q12 (bespoke_tpch/query_q12.cpp:150-250): lineitems with shipmode code
+   mode1 or mode2 (-1 = absent from the dictionary), commitdate <
+   receiptdate and shipdate < commitdate (packed halves as int16), and
+   (u32)(receiptdate - date_lo) < (u32)(date_hi - date_lo); their order row
+   is order_map[orderkey] (map_size entries, -1 = none, as
+   bolap_order_ranges makes it) and they count as high when its priority
+   code is high1 or high2.  The reference's shard loop prunes shards whose
+   shipdates all precede date_lo - 30, which counts the same rows when
+   receiptdate <= shipdate + 30 (TPC-H).  Columns l_shipmode, l_commit_receipt
+   and l_shipdate must be 4-, 16- and 8-byte aligned. */
+typedef struct {
+    int64_t li_rows;
+    const uint8_t* l_shipmode;
+    const uint32_t* l_commit_receipt;
+    const int16_t* l_shipdate;
+    const int32_t* l_orderkey;
+    int32_t mode1, mode2;
+    int32_t date_lo, date_hi;
+    const int32_t* order_map;
+    int64_t map_size;
+    const uint8_t* o_orderpriority;
+    int32_t high1, high2;            /* codes of 1-URGENT and 2-HIGH (-1 = absent) */
+} bolap_q12_args;
+typedef struct {
+    int64_t high[2];                 /* [0]: mode1, [1]: mode2 */
+    int64_t low[2];
+} bolap_q12_result;
+int bolap_q12(bolap_ctx* ctx, const bolap_q12_args* args, bolap_q12_result* out);
+
+/* Q19 (bespoke_tpch/query_q19.cpp:130-365).  A part row sets its key's mask
+   (the last row of a key wins, also with mask 0): bit g when group g is
+   enabled, brand == groups[g].brand, container is one of its four codes and
+   1 <= size <= size_hi.  A lineitem adds price * (100 - discount) when its
+   shipmode is mode1 or mode2, its shipinstruct is `shipinstruct`, and its
+   quantity lies in [qty_lo, qty_lo + 1000] for a group of its part's mask.
+   Codes are -1 when absent from their dictionary.  Part rows < 2^28 - 1.
+   l_shipmode, l_shipinstruct and l_partkey must be 4-, 4- and 16-byte
+   aligned. */
+typedef struct {
+    int32_t enabled;                 /* QUANTITYg and BRANDg given (not <<NULL>>) */
+    int32_t brand;
+    int32_t containers[4];
+    int32_t size_hi;                 /* 5, 10, 15 */
+    int32_t qty_lo;                  /* llround(QUANTITYg * 100) */
+} bolap_q19_group;
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const void* p_brand;             /* u8 or i16 codes */
+    int32_t p_brand_dtype;
+    const void* p_container;
+    int32_t p_container_dtype;
+    const int32_t* p_size;
+    int64_t li_rows;
+    const int32_t* l_partkey;
+    const int16_t* l_quantity;       /* x100 */
+    const int32_t* l_price;          /* extendedprice, cents */
+    const uint8_t* l_disc;
+    const uint8_t* l_shipmode;
+    const uint8_t* l_shipinstruct;
+    bolap_q19_group groups[3];
+    int32_t mode1, mode2;            /* "AIR", "AIR REG" */
+    int32_t shipinstruct;            /* "DELIVER IN PERSON" */
+} bolap_q19_args;
+int bolap_q19(bolap_ctx* ctx, const bolap_q19_args* args, int64_t* revenue);
+
+/* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */
 /* ------------------------------------------------------------------------ */
 /* discounted_price = extendedprice * (100 - discount) / 100 truncated, and

@@ -662,3 +662,113 @@ def sort_order_i32(primary, secondary=None) -> np.ndarray:
     out = np.zeros(len(p), np.uint32)
     L.oracle_sort_order_i32(_p(p), None if s is None else _p(s), len(p), _p(out))
     return out
+
+
+# ------------------------------------------- Q4, Q5, Q10, Q12, Q19 (tpch2.c)
+Q10_DTYPE = np.dtype([("custkey", "<i4"), ("cust_row", "<i4"), ("revenue", "<i8")])
+
+
+class Q19Group(C.Structure):
+    _fields_ = [("enabled", i32), ("brand", i32), ("containers", i32 * 4), ("size_hi", i32),
+                ("qty_lo", i32)]
+
+
+def _tpch2():
+    L = lib()
+    if not getattr(L, "_tpch2_ready", False):
+        L.oracle_q4.restype = None
+        L.oracle_q4.argtypes = [i64, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, i32, vp]
+        L.oracle_q5.restype = None
+        L.oracle_q5.argtypes = [i64, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, i32, i32, vp]
+        L.oracle_q10.restype = i64
+        L.oracle_q10.argtypes = [i64, vp, vp, vp, vp, vp, vp, i32, vp, i32, i32, vp, i32, i32,
+                                 i64, vp, vp, i32, vp, i64]
+        L.oracle_q12.restype = None
+        L.oracle_q12.argtypes = [i64, vp, vp, vp, vp, i32, i32, i32, i32, vp, i64, vp, i32, i32,
+                                 vp, vp]
+        L.oracle_q19.restype = i64
+        L.oracle_q19.argtypes = [i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp,
+                                 C.POINTER(Q19Group), i32, i32, i32]
+        L._tpch2_ready = True
+    return L
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dt)
+
+
+def q4(db: dict, date_lo: int, date_hi: int, n_priority: int) -> np.ndarray:
+    """Per-priority-code counts of run_q4 (oracle_q4)."""
+    out = np.zeros(max(1, n_priority), np.int64)
+    a = [_c(db["o_orderkey"], np.int32), _c(db["o_orderdate"], np.int16),
+         _c(db["o_orderpriority"], np.uint16)]
+    r = [_c(db["r_start"], np.uint32), _c(db["r_end"], np.uint32)]
+    lk, cr = _c(db["l_orderkey"], np.int32), _c(db["l_commit_receipt"], np.uint32)
+    _tpch2().oracle_q4(len(a[0]), _p(a[0]), _p(a[1]), _p(a[2]), n_priority, _p(r[0]), _p(r[1]),
+                       _p(lk), int(db["l_sorted"]), _p(cr), date_lo, date_hi, _p(out))
+    return out[:n_priority]
+
+
+def q5(db: dict, nation_in_region: np.ndarray, date_lo: int, date_hi: int) -> np.ndarray:
+    """Revenue per nation key of run_q5 (oracle_q5)."""
+    nr = _c(nation_in_region, np.uint8)
+    out = np.zeros(max(1, len(nr)), np.int64)
+    a = [_c(db["o_orderkey"], np.int32), _c(db["o_orderdate"], np.int16),
+         _c(db["o_cust_nation"], np.int16), _c(db["r_start"], np.uint32),
+         _c(db["r_end"], np.uint32), _c(db["l_orderkey"], np.int32)]
+    sn, dp = _c(db["l_supp_nation"], np.int16), _c(db["l_disc_price"], np.int32)
+    _tpch2().oracle_q5(len(a[0]), *[_p(x) for x in a], int(db["l_sorted"]), _p(sn), _p(dp),
+                       _p(nr), len(nr), date_lo, date_hi, _p(out))
+    return out[:len(nr)]
+
+
+def q10(db: dict, code_lo: int, code_span: int, date_lo: int, date_hi: int,
+        n_nation_names: int) -> np.ndarray:
+    """(custkey, cust_row, revenue) rows of run_q10 in its order (oracle_q10)."""
+    a = [_c(db["o_orderkey"], np.int32), _c(db["o_orderdate"], np.int16),
+         _c(db["o_custkey"], np.int32), _c(db["r_start"], np.uint32),
+         _c(db["r_end"], np.uint32), _c(db["l_orderkey"], np.int32)]
+    code, dp = _c(db["l_code"], np.uint8), _c(db["l_disc_price"], np.int32)
+    ck, nk = _c(db["c_custkey"], np.int32), _c(db["c_nationkey"], np.int32)
+    cap = max(1, len(ck))
+    out = np.zeros(cap, Q10_DTYPE)
+    n = _tpch2().oracle_q10(len(a[0]), *[_p(x) for x in a], int(db["l_sorted"]), _p(code),
+                            code_lo, code_span, _p(dp), date_lo, date_hi, len(ck), _p(ck),
+                            _p(nk), n_nation_names, _p(out), cap)
+    assert n <= cap
+    return out[:n]
+
+
+def q12(db: dict, mode1: int, mode2: int, date_lo: int, date_hi: int, high1: int,
+        high2: int):
+    """((high, low) for mode1, (high, low) for mode2) of run_q12 (oracle_q12)."""
+    hi = np.zeros(2, np.int64)
+    lo = np.zeros(2, np.int64)
+    a = [_c(db["l_shipmode"], np.uint16), _c(db["l_commit_receipt"], np.uint32),
+         _c(db["l_shipdate"], np.int16), _c(db["l_orderkey"], np.int32)]
+    m, pr = _c(db["order_map"], np.int32), _c(db["o_orderpriority"], np.uint16)
+    _tpch2().oracle_q12(len(a[0]), *[_p(x) for x in a], mode1, mode2, date_lo, date_hi, _p(m),
+                        len(m), _p(pr), high1, high2, _p(hi), _p(lo))
+    return (int(hi[0]), int(lo[0])), (int(hi[1]), int(lo[1]))
+
+
+def q19_groups(groups) -> "C.Array":
+    """groups: 3 tuples (enabled, brand, [4 container codes], size_hi, qty_lo)."""
+    arr = (Q19Group * 3)()
+    for g, (en, br, cont, sh, ql) in enumerate(groups):
+        arr[g].enabled, arr[g].brand, arr[g].size_hi, arr[g].qty_lo = int(en), br, sh, ql
+        for c in range(4):
+            arr[g].containers[c] = int(cont[c])
+    return arr
+
+
+def q19(db: dict, groups, mode1: int, mode2: int, instruct: int) -> int:
+    """Revenue of run_q19 (oracle_q19)."""
+    p = [_c(db["p_partkey"], np.int32), _c(db["p_brand"], np.uint16),
+         _c(db["p_container"], np.uint16), _c(db["p_size"], np.int32)]
+    li = [_c(db["l_partkey"], np.int32), _c(db["l_quantity"], np.int16),
+          _c(db["l_extendedprice"], np.int32), _c(db["l_discount"], np.uint8),
+          _c(db["l_shipmode"], np.uint16), _c(db["l_shipinstruct"], np.uint16)]
+    return int(_tpch2().oracle_q19(len(p[0]), *[_p(x) for x in p], len(li[0]),
+                                   *[_p(x) for x in li], q19_groups(groups), mode1, mode2,
+                                   instruct))

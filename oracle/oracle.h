@@ -243,4 +243,48 @@ int32_t oracle_mutation(void);
 #ifdef __cplusplus
 }
 #endif
+/* tpch2.c: Q4, Q5, Q10, Q12, Q19 over the reference Database layout. */
+typedef struct {
+    int32_t custkey;
+    int32_t cust_row;
+    int64_t revenue;
+} oracle_q10_row;
+
+typedef struct {
+    int32_t enabled;        /* QUANTITYg and BRANDg both given */
+    int32_t brand;          /* brand code, -1 when not in the dictionary */
+    int32_t containers[4];  /* container codes, -1 when not in the dictionary */
+    int32_t size_hi;        /* 5, 10, 15 */
+    int32_t qty_lo;         /* llround(QUANTITYg * 100) */
+} oracle_q19_group;
+
+void oracle_q4(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const uint16_t* o_priority, int32_t n_priority, const uint32_t* r_start,
+               const uint32_t* r_end, const int32_t* l_orderkey, int32_t l_sorted,
+               const uint32_t* l_commit_receipt, int32_t date_lo, int32_t date_hi,
+               int64_t* counts);
+void oracle_q5(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const int16_t* o_cust_nation, const uint32_t* r_start, const uint32_t* r_end,
+               const int32_t* l_orderkey, int32_t l_sorted, const int16_t* l_supp_nation,
+               const int32_t* l_disc_price, const uint8_t* nation_in_region, int32_t n_nation,
+               int32_t date_lo, int32_t date_hi, int64_t* revenue);
+int64_t oracle_q10(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+                   const int32_t* o_custkey, const uint32_t* r_start, const uint32_t* r_end,
+                   const int32_t* l_orderkey, int32_t l_sorted, const uint8_t* l_code,
+                   int32_t code_lo, int32_t code_span, const int32_t* l_disc_price,
+                   int32_t date_lo, int32_t date_hi, int64_t c_rows, const int32_t* c_custkey,
+                   const int32_t* c_nationkey, int32_t n_nation_names, oracle_q10_row* out,
+                   int64_t max_out);
+void oracle_q12(int64_t li_rows, const uint16_t* l_shipmode, const uint32_t* l_commit_receipt,
+                const int16_t* l_shipdate, const int32_t* l_orderkey, int32_t mode1,
+                int32_t mode2, int32_t date_lo, int32_t date_hi, const int32_t* order_map,
+                int64_t map_size, const uint16_t* o_priority, int32_t high1, int32_t high2,
+                int64_t* high, int64_t* low);
+int64_t oracle_q19(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_brand,
+                   const uint16_t* p_container, const int32_t* p_size, int64_t li_rows,
+                   const int32_t* l_partkey, const int16_t* l_quantity, const int32_t* l_price,
+                   const uint8_t* l_disc, const uint16_t* l_shipmode,
+                   const uint16_t* l_shipinstruct, const oracle_q19_group* groups, int32_t mode1,
+                   int32_t mode2, int32_t instruct);
+
 #endif

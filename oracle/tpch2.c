@@ -1,0 +1,276 @@
+/*
+ * tpch2.c — CPU restatement of the reference's Q4, Q5, Q10, Q12 and Q19
+ * pipelines over the reference Database layout (see oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/ and bench.py's cpu_baseline leg
+ * as the checker / timed CPU baseline.  The product library never links or
+ * calls anything here.  Reference paths are relative to the
+ * DataManagementLab/BespokeOLAP_Artifacts root; every function follows the
+ * reference loop it cites, including its orderkey_sorted / unsorted split.
+ *
+ * Layout assumed (bespoke_tpch/builder_impl.hpp:39-84): orders sorted by
+ * orderdate (builder_impl.cpp:597-608); lineitem_ranges[o] = [start, end) of
+ * order row o's lineitem rows, {0, 0} for none (:929-991); with unsorted
+ * lineitem the ranges are min/max spans and the queries check each row's
+ * orderkey.  Dictionary codes are passed as uint16 like DictionaryColumn.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* lower_bound over a non-decreasing int16 column (std::lower_bound). */
+static int64_t lower_bound_i16(const int16_t* v, int64_t n, int32_t x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if ((int32_t)v[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q4 (bespoke_tpch/query_q4.cpp:140-268): orders with orderdate in          */
+/* [date_lo, date_hi) (lower_bound on the sorted dates) that have a lineitem  */
+/* with commitdate < receiptdate, compared as the packed uint16 halves of     */
+/* commit_receipt (q4_is_late, :70-72), counted per orderpriority code.       */
+/* ------------------------------------------------------------------------ */
+void oracle_q4(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const uint16_t* o_priority, int32_t n_priority, const uint32_t* r_start,
+               const uint32_t* r_end, const int32_t* l_orderkey, int32_t l_sorted,
+               const uint32_t* l_commit_receipt, int32_t date_lo, int32_t date_hi,
+               int64_t* counts) {
+    for (int32_t c = 0; c < n_priority; ++c) counts[c] = 0;
+    const int64_t s = lower_bound_i16(o_orderdate, o_rows, date_lo);
+    const int64_t e = lower_bound_i16(o_orderdate, o_rows, date_hi);
+    for (int64_t o = s; o < e; ++o) {
+        if (r_end[o] == 0) continue;
+        int late = 0;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const uint32_t p = l_commit_receipt[r];
+            const int is_late = oracle_mutation() == 15 ? (uint16_t)p <= (uint16_t)(p >> 16)
+                                                        : (uint16_t)p < (uint16_t)(p >> 16);
+            if (is_late) {
+                late = 1;
+                break;
+            }
+        }
+        if (late && o_priority[o] < n_priority) counts[o_priority[o]] += 1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q5 (bespoke_tpch/query_q5.cpp:190-440): orders with orderdate in           */
+/* [date_lo, date_hi) whose customer nation (orders.cust_nationkey,           */
+/* builder_impl.cpp:913-928) is in the region; per such order the sum of      */
+/* discounted_price over its lineitems whose supplier nation                   */
+/* (lineitem.supp_nationkey, :838-853) equals the customer's, added to that   */
+/* nation.  revenue[n] for n < n_nation (max nation key + 1); a customer      */
+/* nation outside [0, n_nation) counts as not in the region.                  */
+/* ------------------------------------------------------------------------ */
+void oracle_q5(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const int16_t* o_cust_nation, const uint32_t* r_start, const uint32_t* r_end,
+               const int32_t* l_orderkey, int32_t l_sorted, const int16_t* l_supp_nation,
+               const int32_t* l_disc_price, const uint8_t* nation_in_region, int32_t n_nation,
+               int32_t date_lo, int32_t date_hi, int64_t* revenue) {
+    for (int32_t n = 0; n < n_nation; ++n) revenue[n] = 0;
+    const int64_t s = lower_bound_i16(o_orderdate, o_rows, date_lo);
+    const int64_t e = lower_bound_i16(o_orderdate, o_rows, date_hi);
+    for (int64_t o = s; o < e; ++o) {
+        const int32_t cn = o_cust_nation[o];
+        if (cn < 0 || cn >= n_nation || !nation_in_region[cn]) continue;
+        if (r_end[o] == 0) continue;
+        int64_t local = 0;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const int same = oracle_mutation() == 16 ? 1 : l_supp_nation[r] == cn;
+            if (!same) continue;
+            local += l_disc_price[r];
+        }
+        revenue[cn] += local;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q10 (bespoke_tpch/query_q10.cpp:375-560): orders with orderdate in         */
+/* [date_lo, date_hi) and a customer key in [0, max customer key]; per order  */
+/* the sum of discounted_price over its lineitems whose returnflag_linestatus */
+/* code c has (u32)(c - code_lo) < code_span (returnflag 'R'); orders with a  */
+/* nonzero sum are grouped by customer; a group is kept when its customer     */
+/* has a row (custkey_to_row, last row of a key wins) whose nation key is in  */
+/* [0, n_nation_names).  Rows ordered by the reference's LSD passes: revenue */
+/* descending as an unsigned 64-bit value (~revenue ascending), ties by       */
+/* customer key.  Returns the number of groups.                               */
+/* ------------------------------------------------------------------------ */
+static int cmp_q10(const void* pa, const void* pb) {
+    const oracle_q10_row* a = (const oracle_q10_row*)pa;
+    const oracle_q10_row* b = (const oracle_q10_row*)pb;
+    const uint64_t ra = (uint64_t)a->revenue, rb = (uint64_t)b->revenue;
+    if (ra != rb) return ra > rb ? -1 : 1;
+    if (a->custkey != b->custkey) return a->custkey < b->custkey ? -1 : 1;
+    return 0;
+}
+
+int64_t oracle_q10(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+                   const int32_t* o_custkey, const uint32_t* r_start, const uint32_t* r_end,
+                   const int32_t* l_orderkey, int32_t l_sorted, const uint8_t* l_code,
+                   int32_t code_lo, int32_t code_span, const int32_t* l_disc_price,
+                   int32_t date_lo, int32_t date_hi, int64_t c_rows, const int32_t* c_custkey,
+                   const int32_t* c_nationkey, int32_t n_nation_names, oracle_q10_row* out,
+                   int64_t max_out) {
+    if (code_lo < 0) return 0;  /* no 'R' in the returnflag dictionary */
+    int32_t max_ck = 0;         /* q10::max_value: 0 for an empty table */
+    for (int64_t i = 0; i < c_rows; ++i)
+        if (i == 0 || c_custkey[i] > max_ck) max_ck = c_custkey[i];
+    const int64_t size = (int64_t)max_ck + 1;
+    int32_t* row_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)(size > 0 ? size : 1));
+    for (int64_t k = 0; k < size; ++k) row_of[k] = -1;
+    for (int64_t i = 0; i < c_rows; ++i)
+        if (c_custkey[i] >= 0) row_of[c_custkey[i]] = (int32_t)i;
+    uint64_t* rev = (uint64_t*)calloc((size_t)(size > 0 ? size : 1), sizeof(uint64_t));
+    uint8_t* has = (uint8_t*)calloc((size_t)(size > 0 ? size : 1), 1);
+    const int64_t s = lower_bound_i16(o_orderdate, o_rows, date_lo);
+    const int64_t e = lower_bound_i16(o_orderdate, o_rows, date_hi);
+    for (int64_t o = s; o < e; ++o) {
+        const int32_t ck = o_custkey[o];
+        if (ck < 0 || ck >= size) continue;
+        if (r_end[o] == 0) continue;
+        int64_t order_rev = 0;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const uint32_t off = (uint32_t)((int32_t)l_code[r] - code_lo);
+            if (oracle_mutation() == 17 ? off > (uint32_t)code_span : off >= (uint32_t)code_span)
+                continue;
+            order_rev += l_disc_price[r];
+        }
+        if (order_rev != 0) {
+            rev[ck] += (uint64_t)order_rev;
+            has[ck] = 1;
+        }
+    }
+    int64_t n = 0;
+    for (int64_t k = 0; k < size; ++k) {
+        if (!has[k]) continue;
+        const int32_t row = row_of[k];
+        if (row < 0) continue;
+        const int32_t nk = c_nationkey[row];
+        if (nk < 0 || nk >= n_nation_names) continue;
+        if (n < max_out) {
+            out[n].custkey = (int32_t)k;
+            out[n].cust_row = row;
+            out[n].revenue = (int64_t)rev[k];
+        }
+        ++n;
+    }
+    free(row_of);
+    free(rev);
+    free(has);
+    const int64_t kept = n < max_out ? n : max_out;
+    qsort(out, (size_t)kept, sizeof(oracle_q10_row), cmp_q10);
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q12 (bespoke_tpch/query_q12.cpp:150-250, the loop without shards): a       */
+/* lineitem counts when its shipmode code is mode1 or mode2, commitdate <     */
+/* receiptdate and shipdate < commitdate (the packed halves as int16), and    */
+/* (u32)(receiptdate - date_lo) < (u32)(date_hi - date_lo); its order row is  */
+/* orderkey_to_row[orderkey] (keys outside the map or without a row are       */
+/* skipped), and it adds to high[slot] when that order's priority code is     */
+/* high1 or high2, else to low[slot] (slot 0 for mode1, 1 for mode2).         */
+/* The reference's shard loop prunes shards whose shipdates all precede       */
+/* date_lo - 30; for data with receiptdate <= shipdate + 30 (TPC-H) both      */
+/* loops count the same rows.                                                  */
+/* ------------------------------------------------------------------------ */
+void oracle_q12(int64_t li_rows, const uint16_t* l_shipmode, const uint32_t* l_commit_receipt,
+                const int16_t* l_shipdate, const int32_t* l_orderkey, int32_t mode1,
+                int32_t mode2, int32_t date_lo, int32_t date_hi, const int32_t* order_map,
+                int64_t map_size, const uint16_t* o_priority, int32_t high1, int32_t high2,
+                int64_t* high, int64_t* low) {
+    high[0] = high[1] = low[0] = low[1] = 0;
+    const uint32_t span = (uint32_t)(date_hi - date_lo);
+    for (int64_t i = 0; i < li_rows; ++i) {
+        const int32_t m = l_shipmode[i];
+        if (m != mode1 && m != mode2) continue;
+        const uint32_t p = l_commit_receipt[i];
+        const int32_t commit = (int16_t)(p & 0xffff);
+        const int32_t receipt = (int16_t)(p >> 16);
+        if (commit >= receipt) continue;
+        if ((int32_t)l_shipdate[i] >= commit) continue;
+        if ((uint32_t)(receipt - date_lo) >= span) continue;
+        const int32_t k = l_orderkey[i];
+        if ((uint32_t)k >= (uint64_t)map_size) continue;
+        const int32_t row = order_map[k];
+        if (row < 0) continue;
+        const int32_t pr = o_priority[row];
+        const int slot = m == mode1 ? 0 : 1;
+        const int is_high = pr == high1 || pr == high2;
+        if (oracle_mutation() == 18 ? !is_high : is_high) high[slot] += 1;
+        else low[slot] += 1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q19 (bespoke_tpch/query_q19.cpp:130-365).  Part scan: for every part row   */
+/* in order, mask[partkey] = OR over the enabled groups g of (brand ==        */
+/* brand[g] and container in containers[g] and 1 <= size <= size_hi[g]) << g  */
+/* (a plain assignment, so the last row of a duplicated key wins, even with  */
+/* mask 0).  Lineitem scan: shipmode in {mode1, mode2} (the codes >= 0),     */
+/* shipinstruct == instruct, mask[partkey] != 0 and the quantity in          */
+/* [qty_lo[g], qty_lo[g] + 1000] for one of the mask's groups; revenue sums  */
+/* extendedprice * (100 - discount) exactly.  Returns the revenue.            */
+/* ------------------------------------------------------------------------ */
+int64_t oracle_q19(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_brand,
+                   const uint16_t* p_container, const int32_t* p_size, int64_t li_rows,
+                   const int32_t* l_partkey, const int16_t* l_quantity, const int32_t* l_price,
+                   const uint8_t* l_disc, const uint16_t* l_shipmode,
+                   const uint16_t* l_shipinstruct, const oracle_q19_group* groups, int32_t mode1,
+                   int32_t mode2, int32_t instruct) {
+    const int any = groups[0].enabled || groups[1].enabled || groups[2].enabled;
+    if (!any) return 0;
+    int32_t max_key = -1;
+    for (int64_t i = 0; i < p_rows; ++i)
+        if (p_partkey[i] > max_key) max_key = p_partkey[i];
+    for (int64_t i = 0; i < li_rows; ++i)
+        if (l_partkey[i] > max_key) max_key = l_partkey[i];
+    if (max_key < 0) return 0;
+    uint8_t* mask = (uint8_t*)calloc((size_t)max_key + 1, 1);
+    for (int64_t i = 0; i < p_rows; ++i) {
+        uint8_t m = 0;
+        for (int g = 0; g < 3; ++g) {
+            const oracle_q19_group* G = &groups[g];
+            if (!G->enabled || (int32_t)p_brand[i] != G->brand) continue;
+            int in = 0;
+            for (int c = 0; c < 4; ++c) in |= G->containers[c] >= 0 &&
+                                             (int32_t)p_container[i] == G->containers[c];
+            if (!in) continue;
+            if (p_size[i] < 1 || p_size[i] > G->size_hi) continue;
+            m |= (uint8_t)(1u << g);
+        }
+        if (p_partkey[i] >= 0) mask[p_partkey[i]] = m;
+    }
+    int64_t revenue = 0;
+    if (instruct >= 0 && (mode1 >= 0 || mode2 >= 0)) {
+        for (int64_t i = 0; i < li_rows; ++i) {
+            const int32_t sm = l_shipmode[i];
+            if (sm != mode1 && sm != mode2) continue;
+            if ((int32_t)l_shipinstruct[i] != instruct) continue;
+            const int32_t pk = l_partkey[i];
+            if (pk < 0) continue;
+            const uint8_t m = mask[pk];
+            if (!m) continue;
+            const int32_t q = l_quantity[i];
+            int ok = 0;
+            for (int g = 0; g < 3; ++g) {
+                const int32_t hi = groups[g].qty_lo + (oracle_mutation() == 19 ? 999 : 1000);
+                if ((m >> g) & 1) ok |= q >= groups[g].qty_lo && q <= hi;
+            }
+            if (!ok) continue;
+            revenue += (int64_t)l_price[i] * (int64_t)(100 - (int32_t)l_disc[i]);
+        }
+    }
+    free(mask);
+    return revenue;
+}

@@ -1,0 +1,206 @@
+"""Q4, Q5, Q10, Q12 and Q19 on the GPU (tpch_orders.hip) bit-exact against
+the oracle (oracle/tpch2.c) over the same reference-layout columns: small,
+medium and SF10-sized databases (15 M orders, ~60 M lineitems), lineitem in
+order-key order and shuffled (the unsorted branches), and the edge cases the
+reference handles (empty windows, absent dictionary codes, duplicate part
+keys, lineitem keys outside the part and order maps)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests import tpch_data as T
+from tests.test_tpch2_oracle import q19_groups
+
+pytestmark = pytest.mark.gpu
+
+U8_CODES = ("l_shipmode", "l_shipinstruct", "o_orderpriority")
+
+
+def upload(db: dict) -> dict:
+    t = {}
+    for k, v in db.items():
+        if not isinstance(v, np.ndarray):
+            t[k] = v
+            continue
+        a = v.astype(np.uint8) if k in U8_CODES or k in ("p_brand", "p_container") else v
+        if a.dtype == np.uint32:
+            a = a.view(np.int32)
+        elif a.dtype == np.uint16:
+            a = a.view(np.int16)
+        t[k] = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t
+
+
+def shuffled(r: dict, seed: int) -> dict:
+    """The same database with lineitem rows shuffled within blocks of 64."""
+    r2 = dict(r)
+    n = len(r["l_orderkey"])
+    rng = np.random.default_rng(seed)
+    perm = np.argsort(np.arange(n) // 64 + rng.random(n), kind="stable")
+    for k in list(r):
+        if k.startswith("l_") and isinstance(r[k], np.ndarray) and len(r[k]) == n:
+            r2[k] = r[k][perm]
+    return r2
+
+
+CASES = {"small": (5, 4000, 800, 300, 40), "medium": (6, 300_000, 20_000, 30_000, 2000)}
+
+
+@pytest.fixture(scope="module", params=["small", "medium", "small-unsorted"])
+def db(request):
+    name = request.param.split("-")[0]
+    seed, orders, parts, custs, supps = CASES[name]
+    r = T.raw(seed, orders, parts=parts, custs=custs, supps=supps, strings=False)
+    if request.param.endswith("unsorted"):
+        r = shuffled(r, seed)
+    h = T.build(r)
+    if request.param.endswith("unsorted"):
+        assert h["l_sorted"] == 0
+    return r, h, upload(h)
+
+
+WINDOWS = [("1993-07-01", 3), ("1995-01-01", 12), ("1998-07-01", 3), ("2010-01-01", 3),
+           ("1990-01-01", 3)]
+
+
+@pytest.mark.parametrize("date,months", WINDOWS)
+def test_q4(engine, db, date, months):
+    r, h, t = db
+    lo, hi = T.date_offset(date), T.add_months(date, months)
+    n = len(h["priority_dict"])
+    assert engine.q4(t, n, lo, hi) == oracle.q4(h, lo, hi, n).tolist()
+
+
+@pytest.mark.parametrize("date,months", WINDOWS)
+@pytest.mark.parametrize("region", [2, 3])
+def test_q5(engine, db, date, months, region):
+    r, h, t = db
+    lo, hi = T.date_offset(date), T.add_months(date, months)
+    inreg = (np.array(h["n_regionkey"]) == region).astype(np.uint8)
+    assert engine.q5(t, inreg, lo, hi) == oracle.q5(h, inreg, lo, hi).tolist()
+
+
+@pytest.mark.parametrize("date,months", WINDOWS)
+def test_q10(engine, db, date, months):
+    r, h, t = db
+    lo, hi = T.date_offset(date), T.add_months(date, months)
+    nls = len(h["ls_dict"])
+    rc = h["rf_dict"].index("R") * nls
+    got = engine.q10(t, rc, nls, lo, hi, 25)
+    want = oracle.q10(h, rc, nls, lo, hi, 25)
+    assert got == [tuple(x) for x in want.tolist()]
+    # no 'R' in the dictionary: no rows
+    assert engine.q10(t, -1, nls, lo, hi, 25) == []
+
+
+def test_q10_customers_without_rows_or_nation(engine, db):
+    r, h, t = db
+    h2 = dict(h)
+    # drop every tenth customer row and give some a nation key out of range
+    keep = np.arange(len(h["c_custkey"])) % 10 != 3
+    h2["c_custkey"] = np.ascontiguousarray(h["c_custkey"][keep])
+    nk = h["c_nationkey"][keep].copy()
+    nk[::7] = 30
+    h2["c_nationkey"] = nk
+    t2 = dict(t, c_custkey=torch.from_numpy(h2["c_custkey"]).cuda(),
+              c_nationkey=torch.from_numpy(nk).cuda())
+    nls = len(h["ls_dict"])
+    rc = h["rf_dict"].index("R") * nls
+    lo, hi = T.date_offset("1994-01-01"), T.add_months("1994-01-01", 12)
+    got = engine.q10(t2, rc, nls, lo, hi, 25)
+    want = oracle.q10(h2, rc, nls, lo, hi, 25)
+    assert got == [tuple(x) for x in want.tolist()] and len(got) > 0
+
+
+@pytest.mark.parametrize("modes,date", [(("MAIL", "SHIP"), "1994-01-01"),
+                                        (("AIR", "TRUCK"), "1997-01-01"),
+                                        (("FOB", "NOSUCH"), "1995-01-01"),
+                                        (("NOSUCH", "NOPE"), "1995-01-01")])
+def test_q12(engine, db, modes, date):
+    r, h, t = db
+    lo, hi = T.date_offset(date), T.add_months(date, 12)
+    codes = [h["shipmode_dict"].index(m) if m in h["shipmode_dict"] else -1 for m in modes]
+    hc = [h["priority_dict"].index(p) for p in ("1-URGENT", "2-HIGH")]
+    got = engine.q12(t, codes[0], codes[1], lo, hi, *hc)
+    assert got == oracle.q12(h, codes[0], codes[1], lo, hi, *hc)
+
+
+@pytest.mark.parametrize("brands,qtys", [(("Brand#12", "Brand#23", "Brand#34"), (1, 10, 20)),
+                                         (("Brand#11", None, "Brand#55"), (5, 3, 25)),
+                                         (("Brand#99", "Brand#21", "Brand#21"), (1, 12, 14)),
+                                         ((None, None, None), (1, 2, 3))])
+def test_q19(engine, db, brands, qtys):
+    r, h, t = db
+    g = q19_groups(h, brands, qtys)
+    m1, m2 = h["shipmode_dict"].index("AIR"), h["shipmode_dict"].index("REG AIR")
+    ins = h["instruct_dict"].index("DELIVER IN PERSON")
+    want = oracle.q19(h, g, m1, m2, ins)
+    assert engine.q19(t, g, m1, m2, ins) == want
+    assert engine.q19(t, g, m1, -1, ins) == oracle.q19(h, g, m1, -1, ins)
+    assert engine.q19(t, g, -1, -1, ins) == 0
+
+
+def test_q19_duplicate_part_keys_and_keys_outside_the_parts(engine, db):
+    r, h, t = db
+    h2 = dict(h)
+    n = len(h["p_partkey"])
+    # duplicate the first 100 part keys with other brands (last row wins),
+    # and point some lineitems past the largest part key
+    h2["p_partkey"] = np.concatenate([h["p_partkey"], h["p_partkey"][:100]])
+    h2["p_brand"] = np.concatenate([h["p_brand"], (h["p_brand"][:100] + 1) % 25]).astype(
+        np.uint16)
+    h2["p_container"] = np.concatenate([h["p_container"], h["p_container"][:100]])
+    h2["p_size"] = np.concatenate([h["p_size"], h["p_size"][:100]])
+    lk = h["l_partkey"].copy()
+    lk[::97] = n + 5
+    h2["l_partkey"] = lk
+    t2 = upload(h2)
+    g = q19_groups(h, ("Brand#12", "Brand#23", "Brand#34"), (1, 10, 20))
+    m1, m2 = h["shipmode_dict"].index("AIR"), h["shipmode_dict"].index("REG AIR")
+    ins = h["instruct_dict"].index("DELIVER IN PERSON")
+    assert engine.q19(t2, g, m1, m2, ins) == oracle.q19(h2, g, m1, m2, ins)
+    # int16 brand / container codes give the same answer
+    t3 = dict(t2, p_brand=t2["p_brand"].to(torch.int16), p_container=t2["p_container"].to(torch.int16))
+    assert engine.q19(t3, g, m1, m2, ins) == oracle.q19(h2, g, m1, m2, ins)
+
+
+@pytest.fixture(scope="module")
+def sf10():
+    """15,000,000 orders, about 60,000,000 lineitems, 2,000,000 parts,
+    1,500,000 customers, 100,000 suppliers."""
+    r = T.raw(42, 15_000_000, parts=2_000_000, custs=1_500_000, supps=100_000, strings=False)
+    h = T.build(r)
+    del r
+    return h, upload(h)
+
+
+def test_sf10_all_five(engine, sf10):
+    h, t = sf10
+    assert len(h["l_orderkey"]) > 55_000_000
+    lo, hi = T.date_offset("1994-01-01"), T.add_months("1994-01-01", 3)
+    print("q4", flush=True)
+    assert engine.q4(t, 5, lo, hi) == oracle.q4(h, lo, hi, 5).tolist()
+    print("q5", flush=True)
+    inreg = (np.array(h["n_regionkey"]) == 2).astype(np.uint8)
+    assert engine.q5(t, inreg, lo, T.add_months("1994-01-01", 12)) == \
+        oracle.q5(h, inreg, lo, T.add_months("1994-01-01", 12)).tolist()
+    print("q10", flush=True)
+    nls = len(h["ls_dict"])
+    rc = h["rf_dict"].index("R") * nls
+    got = engine.q10(t, rc, nls, lo, hi, 25)
+    want = oracle.q10(h, rc, nls, lo, hi, 25)
+    assert len(got) > 100_000
+    assert got == [tuple(x) for x in want.tolist()]
+    print("q12", flush=True)
+    m = [h["shipmode_dict"].index(x) for x in ("MAIL", "SHIP")]
+    hc = [h["priority_dict"].index(p) for p in ("1-URGENT", "2-HIGH")]
+    yr = T.add_months("1994-01-01", 12)
+    assert engine.q12(t, m[0], m[1], lo, yr, *hc) == oracle.q12(h, m[0], m[1], lo, yr, *hc)
+    print("q19", flush=True)
+    g = q19_groups(h, ("Brand#12", "Brand#23", "Brand#34"), (1, 10, 20))
+    m1, m2 = h["shipmode_dict"].index("AIR"), h["shipmode_dict"].index("REG AIR")
+    ins = h["instruct_dict"].index("DELIVER IN PERSON")
+    want19 = oracle.q19(h, g, m1, m2, ins)
+    assert want19 > 0
+    assert engine.q19(t, g, m1, m2, ins) == want19
