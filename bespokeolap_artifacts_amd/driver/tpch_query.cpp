@@ -1,11 +1,12 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q3, Q6, Q14, Q17 and Q18, with the kernels of libbolap doing the work.
+// TPC-H Q1, Q3, Q4, Q5, Q6, Q10, Q12, Q14, Q17, Q18 and Q19, with the kernels
+// of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
 //
-// load   reads <dir>lineitem.parquet and, when present, part, orders and
-//        customer .parquet with Arrow (bespoke_tpch/loader_impl.cpp:9-22; the
-//        path is a directory prefix, :12).
+// load   reads <dir>lineitem.parquet and, when present, part, orders,
+//        customer, supplier, nation and region .parquet with Arrow
+//        (bespoke_tpch/loader_impl.cpp:9-22; the path is a directory prefix, :12).
 // build  converts the columns these queries read into the reference Database
 //        layout (bespoke_tpch/builder_impl.cpp:447-999): decimals through
 //        FormatValue + stod + llround(x * scale) (:287-338), dates as int16 day
@@ -13,14 +14,18 @@
 //        first-appearance dictionary codes (:204-224), part's promo flag from
 //        its type strings (query_q14.cpp:96-100, 200-208); uploads them and
 //        derives discounted_price and the returnflag_linestatus code on the
-//        device (bolap_derive_lineitem).
+//        device (bolap_derive_lineitem); orders sorted by date with their
+//        lineitem ranges (bolap_order_ranges), commit/receipt packed, the
+//        supplier and customer nations denormalised (:536-544, 838-928).
 // query  reads one request per line until an empty line (query_impl.cpp:142-157):
-//        `1 DELTA`, `3 SEGMENT DATE`, `6 DATE DISCOUNT QUANTITY`, `14 DATE`,
-//        `17 BRAND CONTAINER`,
-//        `18 QUANTITY` with std::quoted tokens (args_parser.hpp), runs them in
-//        order, writes resultN.csv and prints "N | Execution ms: T"
-//        (query_impl.cpp:161-169); other query ids are reported on stderr and
-//        skipped.
+//        `1 DELTA`, `3 SEGMENT DATE`, `4 DATE`, `5 REGION DATE`,
+//        `6 DATE DISCOUNT QUANTITY`, `10 DATE`, `12 MODE1 MODE2 DATE`,
+//        `14 DATE`, `17 BRAND CONTAINER`, `18 QUANTITY`,
+//        `19 Q1 Q2 Q3 BRAND1 BRAND2 BRAND3` with std::quoted tokens
+//        (args_parser.hpp), runs them in order, writes resultN.csv and prints
+//        "N | Execution ms: T" with the timer stopped before the file is
+//        written (query_impl.cpp:161-169); an id without a pipeline gets the
+//        reference's stub CSV (:82-103, 382-388).
 #include <arrow/api.h>
 #include <arrow/io/file.h>
 #include <parquet/arrow/reader.h>
@@ -212,7 +217,8 @@ CodeCol upload_codes(const Dict& d, const char* what) {
     return {upload(std::vector<int16_t>(d.codes.begin(), d.codes.end())), BOLAP_I16};
 }
 
-// The device mirror of the columns the served queries read.
+// The device mirror of the columns the served queries read, in the
+// reference Database layout (bespoke_tpch/builder_impl.hpp:39-158).
 struct Lineitem {
     int64_t rows = 0;
     int16_t *ship = nullptr, *qty = nullptr;
@@ -220,32 +226,77 @@ struct Lineitem {
     uint8_t *disc = nullptr, *tax = nullptr, *code = nullptr;
     int32_t orderkey_sorted = 0;
     std::vector<std::string> rf, ls;
+    // present when the file has the columns
+    uint32_t* commit_receipt = nullptr;       // receipt << 16 | commit (:536-544)
+    uint8_t *shipmode = nullptr, *shipinstruct = nullptr;
+    std::vector<std::string> shipmodes, instructs;
+    int16_t* supp_nation = nullptr;           // supplier nation per row (:838-853)
 };
 
 struct Part {
     int64_t rows = -1;  // -1: no part table
     int32_t* partkey = nullptr;
+    int32_t* size = nullptr;
     CodeCol brand, container;
     uint8_t* promo = nullptr;
     std::vector<std::string> brands, containers;
 };
 
+// Orders sorted by orderdate (stable), as build() keeps them (:597-608).
 struct Orders {
     int64_t rows = -1;
     int32_t *orderkey = nullptr, *custkey = nullptr, *totalprice = nullptr;
     int16_t* orderdate = nullptr;
     int32_t* orderdate32 = nullptr;                       // Q3's order-day column
     std::unordered_map<int32_t, int32_t> shippriority;   // by order key (Q3's output)
+    uint8_t* priority = nullptr;                          // o_orderpriority codes
+    std::vector<std::string> priorities;
+    int16_t* cust_nation = nullptr;                       // customer nation (:913-928)
+    int32_t* order_map = nullptr;                         // orderkey -> row (:610-630)
+    int64_t map_size = 0;
+    uint32_t *range_start = nullptr, *range_end = nullptr;  // lineitem_ranges (:929-991)
 };
 
 struct Customer {
     int64_t rows = -1;
     int32_t* custkey = nullptr;
+    int32_t* nationkey = nullptr;
     uint8_t* segment = nullptr;  // c_mktsegment codes (null without the column)
-    std::vector<std::string> segments, name;
+    std::vector<std::string> segments, name, address, phone, comment;
+    std::vector<int32_t> acctbal;          // cents
+    std::vector<int32_t> h_nationkey;      // c_nationkey by row
+    std::vector<int16_t> nation_by_key;    // nationkey_by_custkey (:675-692)
 };
 
-Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
+struct Nation {
+    std::vector<int32_t> key, region;
+    std::vector<std::string> name;
+    std::unordered_map<std::string, int32_t> region_by_name;  // region.name_to_key
+    int32_t max_key = 0;                                       // at least 0, as the queries
+};
+
+std::vector<std::string> strings_or_empty(const arrow::Table& t, const char* name) {
+    if (!t.GetColumnByName(name)) return std::vector<std::string>(size_t(t.num_rows()));
+    return strings(*column(t, name));
+}
+
+// key -> nation of the table's rows, -1 for keys without a row (last row
+// of a key wins), as nationkey_by_custkey / nationkey_by_suppkey.
+std::vector<int16_t> nation_by_key(const std::vector<int32_t>& keys,
+                                   const std::vector<int32_t>& nations) {
+    int32_t mx = -1;
+    for (int32_t k : keys) mx = std::max(mx, k);
+    std::vector<int16_t> m(size_t(mx + 1), int16_t(-1));
+    for (size_t i = 0; i < keys.size(); ++i)
+        if (keys[i] >= 0) m[size_t(keys[i])] = int16_t(nations[i]);
+    return m;
+}
+
+int16_t nation_of(const std::vector<int16_t>& m, int32_t key) {
+    return key >= 0 && size_t(key) < m.size() ? m[size_t(key)] : int16_t(-1);
+}
+
+Lineitem build(bolap_ctx* ctx, const arrow::Table& t, const std::vector<int16_t>& supp_nation) {
     Lineitem li;
     li.rows = t.num_rows();
     const auto qty = scaled(*column(t, "l_quantity"), 100);
@@ -278,6 +329,30 @@ Lineitem build(bolap_ctx* ctx, const arrow::Table& t) {
     hip_check(hipFree(ls_dev), "hipFree");
     li.rf = rf.values;
     li.ls = ls.values;
+    if (t.GetColumnByName("l_commitdate") && t.GetColumnByName("l_receiptdate")) {
+        const auto commit = date_offsets(*column(t, "l_commitdate"));
+        const auto receipt = date_offsets(*column(t, "l_receiptdate"));
+        std::vector<uint32_t> packed(commit.size());
+        for (size_t i = 0; i < packed.size(); ++i)
+            packed[i] = (uint32_t(uint16_t(receipt[i])) << 16) | uint32_t(uint16_t(commit[i]));
+        li.commit_receipt = upload(packed);
+    }
+    if (t.GetColumnByName("l_shipmode")) {
+        Dict d = dictionary(*column(t, "l_shipmode"));
+        li.shipmode = upload(narrow_codes(d, "l_shipmode"));
+        li.shipmodes = d.values;
+    }
+    if (t.GetColumnByName("l_shipinstruct")) {
+        Dict d = dictionary(*column(t, "l_shipinstruct"));
+        li.shipinstruct = upload(narrow_codes(d, "l_shipinstruct"));
+        li.instructs = d.values;
+    }
+    if (t.GetColumnByName("l_suppkey")) {
+        const auto sk = int32s(*column(t, "l_suppkey"));
+        std::vector<int16_t> sn(sk.size());
+        for (size_t i = 0; i < sk.size(); ++i) sn[i] = nation_of(supp_nation, sk[i]);
+        li.supp_nation = upload(sn);
+    }
     return li;
 }
 
@@ -295,38 +370,104 @@ Part build_part(const arrow::Table& t) {
     p.promo = upload(promo);
     p.brands = brand.values;
     p.containers = container.values;
+    if (t.GetColumnByName("p_size")) p.size = upload(int32s(*column(t, "p_size")));
     return p;
 }
 
-Orders build_orders(const arrow::Table& t) {
+template <typename T>
+std::vector<T> permuted(const std::vector<T>& v, const std::vector<uint32_t>& order) {
+    std::vector<T> out(v.size());
+    for (size_t i = 0; i < order.size(); ++i) out[i] = v[order[i]];
+    return out;
+}
+
+Orders build_orders(bolap_ctx* ctx, const arrow::Table& t, const Lineitem& li,
+                    const std::vector<int16_t>& cust_nation) {
     Orders o;
     o.rows = t.num_rows();
-    o.orderkey = upload(int32s(*column(t, "o_orderkey")));
-    o.custkey = upload(int32s(*column(t, "o_custkey")));
-    const auto days = date_offsets(*column(t, "o_orderdate"));
+    const auto keys0 = int32s(*column(t, "o_orderkey"));
+    const auto days0 = date_offsets(*column(t, "o_orderdate"));
+    // stable order by date (build_date_sorted_indices, builder_impl.cpp:374-405)
+    std::vector<uint32_t> order(keys0.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = uint32_t(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return days0[a] < days0[b]; });
+    const auto keys = permuted(keys0, order);
+    const auto cust = permuted(int32s(*column(t, "o_custkey")), order);
+    const auto days = permuted(days0, order);
+    o.orderkey = upload(keys);
+    o.custkey = upload(cust);
     o.orderdate = upload(days);
     o.orderdate32 = upload(std::vector<int32_t>(days.begin(), days.end()));
-    const auto total = scaled(*column(t, "o_totalprice"), 100);
+    const auto total = permuted(scaled(*column(t, "o_totalprice"), 100), order);
     o.totalprice = upload(std::vector<int32_t>(total.begin(), total.end()));
     if (t.GetColumnByName("o_shippriority")) {
-        const auto keys = int32s(*column(t, "o_orderkey"));
         const auto prio = int32s(*column(t, "o_shippriority"));
-        for (size_t i = 0; i < keys.size(); ++i) o.shippriority[keys[i]] = prio[i];
+        for (size_t i = 0; i < keys0.size(); ++i) o.shippriority[keys0[i]] = prio[i];
     }
+    if (t.GetColumnByName("o_orderpriority")) {
+        Dict d = dictionary(*column(t, "o_orderpriority"));
+        o.priority = upload(permuted(narrow_codes(d, "o_orderpriority"), order));
+        o.priorities = d.values;
+    }
+    std::vector<int16_t> cn(cust.size());
+    for (size_t i = 0; i < cust.size(); ++i) cn[i] = nation_of(cust_nation, cust[i]);
+    o.cust_nation = upload(cn);
+    // orderkey -> row and each order's lineitem rows, made on the device
+    int32_t kmax = -1;
+    for (int32_t k : keys) kmax = std::max(kmax, k);
+    o.map_size = int64_t(kmax) + 1;
+    o.order_map = upload(std::vector<int32_t>(size_t(std::max<int64_t>(1, o.map_size))));
+    o.range_start = upload(std::vector<uint32_t>(size_t(std::max<int64_t>(1, o.rows))));
+    o.range_end = upload(std::vector<uint32_t>(size_t(std::max<int64_t>(1, o.rows))));
+    if (o.rows > 0 && o.map_size > 0)
+        check(bolap_order_ranges(ctx, o.rows, o.orderkey, o.map_size, o.order_map, li.rows,
+                                 li.orderkey, o.range_start, o.range_end));
+    hip_check(hipDeviceSynchronize(), "order ranges");
     return o;
 }
 
 Customer build_customer(const arrow::Table& t) {
     Customer c;
     c.rows = t.num_rows();
-    c.custkey = upload(int32s(*column(t, "c_custkey")));
+    const auto keys = int32s(*column(t, "c_custkey"));
+    c.custkey = upload(keys);
     c.name = strings(*column(t, "c_name"));
     if (t.GetColumnByName("c_mktsegment")) {
         Dict seg = dictionary(*column(t, "c_mktsegment"));
         c.segment = upload(narrow_codes(seg, "c_mktsegment"));
         c.segments = seg.values;
     }
+    if (t.GetColumnByName("c_nationkey")) {
+        const auto nk = int32s(*column(t, "c_nationkey"));
+        c.nationkey = upload(nk);
+        c.nation_by_key = nation_by_key(keys, nk);
+        c.h_nationkey = nk;
+    }
+    if (t.GetColumnByName("c_acctbal")) {
+        const auto bal = scaled(*column(t, "c_acctbal"), 100);
+        c.acctbal.assign(bal.begin(), bal.end());
+    } else {
+        c.acctbal.assign(size_t(c.rows), 0);
+    }
+    c.address = strings_or_empty(t, "c_address");
+    c.phone = strings_or_empty(t, "c_phone");
+    c.comment = strings_or_empty(t, "c_comment");
     return c;
+}
+
+Nation build_nation(const arrow::Table& n, const arrow::Table* r) {
+    Nation x;
+    x.key = int32s(*column(n, "n_nationkey"));
+    x.region = int32s(*column(n, "n_regionkey"));
+    x.name = strings(*column(n, "n_name"));
+    for (int32_t k : x.key) x.max_key = std::max(x.max_key, k);
+    if (r) {
+        const auto rk = int32s(*column(*r, "r_regionkey"));
+        const auto rn = strings(*column(*r, "r_name"));
+        for (size_t i = 0; i < rk.size(); ++i) x.region_by_name.emplace(rn[i], rk[i]);
+    }
+    return x;
 }
 
 bool exists(const std::string& path) {
@@ -554,9 +695,299 @@ Writer run_q18(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Custom
     };
 }
 
+// The day offset of DATE plus `months` months, the day clamped to the
+// month's end (parse_date_offset_add_months, query_q4.cpp:40-60; a year is
+// twelve months, query_q5.cpp:54-67).
+int16_t add_months_offset(const std::string& v, int months) {
+    int year = std::stoi(v.substr(0, 4));
+    int month = std::stoi(v.substr(5, 2));
+    int day = std::stoi(v.substr(8, 2));
+    int mi = month - 1 + months;
+    year += mi / 12;
+    mi %= 12;
+    if (mi < 0) {
+        mi += 12;
+        year -= 1;
+    }
+    static const int kDays[] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+    const bool leap = (year % 4 == 0 && year % 100 != 0) || year % 400 == 0;
+    const int dim = (mi == 1 && leap) ? 29 : kDays[mi];
+    if (day > dim) day = dim;
+    return int16_t(days_from_civil(year, unsigned(mi + 1), unsigned(day)) - kBaseDays);
+}
+
+std::string quoted_field(const std::string& value);  // the reference's CSV quoting
+
+// "i.dd" of a value in hundredths (append_fixed_2, query_q10.cpp:108-122).
+std::string fixed_2(int64_t value) {
+    std::string out;
+    if (value < 0) {
+        out.push_back('-');
+        value = -value;
+    }
+    out += std::to_string(value / 100);
+    out.push_back('.');
+    out.push_back(char('0' + (value % 100) / 10));
+    out.push_back(char('0' + value % 10));
+    return out;
+}
+
+// The fixed-notation, two-decimal iostream output of a double.
+std::string fixed_double(double v) {
+    std::ostringstream os;
+    os.setf(std::ios::fixed);
+    os << std::setprecision(2) << v;
+    return os.str();
+}
+
+int32_t code_of(const std::vector<std::string>& dict, const std::string& v) {
+    const auto it = std::find(dict.begin(), dict.end(), v);
+    return it == dict.end() ? -1 : int32_t(it - dict.begin());
+}
+
+bolap_order_side order_side(const Lineitem& li, const Orders& o) {
+    bolap_order_side s{};
+    s.order_rows = o.rows;
+    s.o_orderdate = o.orderdate;
+    s.o_orderkey = o.orderkey;
+    s.range_start = o.range_start;
+    s.range_end = o.range_end;
+    s.l_orderkey = li.orderkey;
+    s.l_orderkey_sorted = li.orderkey_sorted;
+    return s;
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) throw std::runtime_error(what);
+}
+
+Writer run_q4(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const std::string& line) {
+    need(o.rows >= 0 && o.priority && li.commit_receipt,
+         "Q4 needs orders (with o_orderpriority) and l_commitdate / l_receiptdate");
+    const auto v = quoted_args(line, 1, "Q4");
+    bolap_q4_args a{};
+    a.orders = order_side(li, o);
+    a.o_orderpriority = o.priority;
+    a.n_priority = int32_t(o.priorities.size());
+    a.l_commit_receipt = li.commit_receipt;
+    a.date_lo = parse_date_offset(v[0]);
+    a.date_hi = add_months_offset(v[0], 3);
+    std::vector<int64_t> counts(std::max<size_t>(1, o.priorities.size()));
+    check(bolap_q4(ctx, &a, counts.data()));
+    std::vector<std::pair<std::string, int64_t>> rows;  // query_q4.cpp:236-258
+    for (size_t c = 0; c < o.priorities.size(); ++c)
+        if (counts[c] != 0) rows.emplace_back(o.priorities[c], counts[c]);
+    std::sort(rows.begin(), rows.end(),
+              [](const auto& x, const auto& y) { return x.first < y.first; });
+    return [rows](const std::string& csv) {  // write_q4_csv, query_q4.cpp:270-276
+        std::ofstream out(csv);
+        out << "o_orderpriority,order_count\n";
+        for (const auto& [p, n] : rows) out << quoted_field(p) << ',' << n << "\n";
+    };
+}
+
+Writer run_q5(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Nation& nat,
+              const std::string& line) {
+    need(o.rows >= 0 && li.supp_nation, "Q5 needs orders, customer, supplier and l_suppkey");
+    const auto v = quoted_args(line, 2, "Q5");
+    std::vector<std::pair<std::string, int64_t>> rows;
+    const auto reg = nat.region_by_name.find(v[0]);
+    if (reg != nat.region_by_name.end()) {  // an unknown region gives no rows (:196-201)
+        const int32_t n = nat.max_key + 1;
+        std::vector<uint8_t> in_region(static_cast<size_t>(n), 0);
+        std::vector<std::string> names(static_cast<size_t>(n));
+        for (size_t i = 0; i < nat.key.size(); ++i) {
+            if (nat.key[i] < 0) continue;
+            names[size_t(nat.key[i])] = nat.name[i];
+            if (nat.region[i] == reg->second) in_region[size_t(nat.key[i])] = 1;
+        }
+        bolap_q5_args a{};
+        a.orders = order_side(li, o);
+        a.o_cust_nation = o.cust_nation;
+        a.l_supp_nation = li.supp_nation;
+        a.l_disc_price = li.disc_price;
+        a.nation_in_region = in_region.data();
+        a.n_nation = n;
+        a.date_lo = parse_date_offset(v[1]);
+        a.date_hi = add_months_offset(v[1], 12);
+        std::vector<int64_t> rev(static_cast<size_t>(n));
+        check(bolap_q5(ctx, &a, rev.data()));
+        for (int32_t k = 0; k < n; ++k)  // :400-420
+            if (in_region[size_t(k)] && rev[size_t(k)] != 0)
+                rows.emplace_back(names[size_t(k)], rev[size_t(k)]);
+        std::sort(rows.begin(), rows.end(), [](const auto& x, const auto& y) {
+            return x.second != y.second ? x.second > y.second : x.first < y.first;
+        });
+    }
+    return [rows](const std::string& csv) {  // write_q5_csv, :443-452
+        std::ofstream out(csv);
+        out << "n_name,revenue\n";
+        for (const auto& [n, r] : rows)
+            out << quoted_field(n) << ',' << fixed_double(double(r) / 100.0) << "\n";
+    };
+}
+
+Writer run_q10(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+               const Nation& nat, const std::string& line) {
+    need(o.rows >= 0 && c.rows >= 0 && c.nationkey,
+         "Q10 needs orders, customer (with c_nationkey) and nation");
+    const auto v = quoted_args(line, 1, "Q10");
+    std::vector<bolap_q10_row> rows;
+    const int32_t rcode = code_of(li.rf, "R");  // query_q10.cpp:386-396
+    std::vector<std::string> names(static_cast<size_t>(nat.max_key + 1));
+    for (size_t i = 0; i < nat.key.size(); ++i)
+        if (nat.key[i] >= 0) names[size_t(nat.key[i])] = nat.name[i];
+    if (rcode >= 0) {
+        bolap_q10_args a{};
+        a.orders = order_side(li, o);
+        a.o_custkey = o.custkey;
+        a.l_code = li.code;
+        a.code_lo = rcode * int32_t(li.ls.size());
+        a.code_span = int32_t(li.ls.size());
+        a.l_disc_price = li.disc_price;
+        a.date_lo = parse_date_offset(v[0]);
+        a.date_hi = add_months_offset(v[0], 3);
+        a.cust_rows = c.rows;
+        a.c_custkey = c.custkey;
+        a.c_nationkey = c.nationkey;
+        a.n_nation_names = int32_t(names.size());
+        rows.resize(size_t(std::max<int64_t>(1, c.rows)));
+        int64_t n = 0;
+        check(bolap_q10(ctx, &a, rows.data(), int64_t(rows.size()), &n));
+        rows.resize(size_t(n));
+    }
+    return [rows, &c, names](const std::string& csv) {  // write_q10_csv, :588-640
+        std::ofstream out(csv, std::ios::binary);
+        out << "c_custkey,c_name,revenue,c_acctbal,n_name,c_address,c_phone,c_comment\n";
+        for (const auto& r : rows) {
+            const size_t row = size_t(r.cust_row);
+            out << r.custkey << ',' << quoted_field(c.name[row]) << ',' << fixed_2(r.revenue)
+                << ',' << fixed_2(c.acctbal[row]) << ',' << quoted_field(names[size_t(c.h_nationkey[row])])
+                << ',' << quoted_field(c.address[row]) << ',' << quoted_field(c.phone[row])
+                << ',' << quoted_field(c.comment[row]) << "\n";
+        }
+    };
+}
+
+Writer run_q12(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const std::string& line) {
+    need(o.rows >= 0 && o.priority && li.commit_receipt && li.shipmode,
+         "Q12 needs orders (with o_orderpriority), l_shipmode, l_commitdate, l_receiptdate");
+    const auto v = quoted_args(line, 3, "Q12");
+    int32_t m1 = -1, m2 = -1;  // query_q12.cpp:176-186
+    for (size_t i = 0; i < li.shipmodes.size(); ++i) {
+        if (li.shipmodes[i] == v[0]) m1 = int32_t(i);
+        else if (li.shipmodes[i] == v[1]) m2 = int32_t(i);
+    }
+    struct Row {
+        std::string mode;
+        int64_t high, low;
+    };
+    std::vector<Row> rows;
+    if (m1 >= 0 || m2 >= 0) {
+        int32_t h1 = -1, h2 = -1;
+        for (size_t i = 0; i < o.priorities.size(); ++i) {
+            if (o.priorities[i] == "1-URGENT") h1 = int32_t(i);
+            else if (o.priorities[i] == "2-HIGH") h2 = int32_t(i);
+        }
+        bolap_q12_args a{};
+        a.li_rows = li.rows;
+        a.l_shipmode = li.shipmode;
+        a.l_commit_receipt = li.commit_receipt;
+        a.l_shipdate = li.ship;
+        a.l_orderkey = li.orderkey;
+        a.mode1 = m1;
+        a.mode2 = m2;
+        a.date_lo = parse_date_offset(v[2]);
+        a.date_hi = add_months_offset(v[2], 12);
+        a.order_map = o.order_map;
+        a.map_size = o.map_size;
+        a.o_orderpriority = o.priority;
+        a.high1 = h1;
+        a.high2 = h2;
+        bolap_q12_result r{};
+        check(bolap_q12(ctx, &a, &r));
+        for (size_t i = 0; i < li.shipmodes.size(); ++i) {  // :298-316
+            const int slot = int32_t(i) == m1 ? 0 : int32_t(i) == m2 ? 1 : -1;
+            if (slot < 0 || (r.high[slot] == 0 && r.low[slot] == 0)) continue;
+            rows.push_back({li.shipmodes[i], r.high[slot], r.low[slot]});
+        }
+        std::sort(rows.begin(), rows.end(),
+                  [](const Row& x, const Row& y) { return x.mode < y.mode; });
+    }
+    return [rows](const std::string& csv) {  // write_q12_csv, :358-366
+        std::ofstream out(csv);
+        out << "l_shipmode,high_line_count,low_line_count\n";
+        for (const auto& r : rows)
+            out << quoted_field(r.mode) << ',' << r.high << ',' << r.low << "\n";
+    };
+}
+
+std::string trim_right(const std::string& v) {
+    size_t e = v.size();
+    while (e > 0 && v[e - 1] == ' ') --e;
+    return v.substr(0, e);
+}
+
+Writer run_q19(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::string& line) {
+    need(part.rows >= 0 && part.size && li.shipmode && li.shipinstruct,
+         "Q19 needs part (with p_size), l_shipmode and l_shipinstruct");
+    const auto v = quoted_args(line, 6, "Q19");
+    int64_t revenue = 0;
+    bolap_q19_args a{};
+    static const char* kCont[3][4] = {{"SM CASE", "SM BOX", "SM PACK", "SM PKG"},
+                                      {"MED BAG", "MED BOX", "MED PKG", "MED PACK"},
+                                      {"LG CASE", "LG BOX", "LG PACK", "LG PKG"}};
+    bool any = false;
+    for (int g = 0; g < 3; ++g) {  // query_q19.cpp:136-141, 186-206
+        bolap_q19_group& G = a.groups[g];
+        G.enabled = v[size_t(g)] != "<<NULL>>" && v[size_t(3 + g)] != "<<NULL>>";
+        any = any || G.enabled;
+        G.brand = code_of(part.brands, v[size_t(3 + g)]);
+        for (int c = 0; c < 4; ++c) G.containers[c] = code_of(part.containers, kCont[g][c]);
+        G.size_hi = 5 * (g + 1);
+        G.qty_lo = G.enabled ? int32_t(std::llround(std::stod(v[size_t(g)]) * 100.0)) : 0;
+    }
+    if (any) {
+        a.mode1 = a.mode2 = a.shipinstruct = -1;  // :160-183
+        for (size_t i = 0; i < li.shipmodes.size(); ++i) {
+            const auto t = trim_right(li.shipmodes[i]);
+            if (t == "AIR") a.mode1 = int32_t(i);
+            else if (t == "AIR REG") a.mode2 = int32_t(i);
+        }
+        for (size_t i = 0; i < li.instructs.size(); ++i)
+            if (trim_right(li.instructs[i]) == "DELIVER IN PERSON") {
+                a.shipinstruct = int32_t(i);
+                break;
+            }
+        a.part_rows = part.rows;
+        a.p_partkey = part.partkey;
+        a.p_brand = part.brand.dev;
+        a.p_brand_dtype = part.brand.dtype;
+        a.p_container = part.container.dev;
+        a.p_container_dtype = part.container.dtype;
+        a.p_size = part.size;
+        a.li_rows = li.rows;
+        a.l_partkey = li.partkey;
+        a.l_quantity = li.qty;
+        a.l_price = li.price;
+        a.l_disc = li.disc;
+        a.l_shipmode = li.shipmode;
+        a.l_shipinstruct = li.shipinstruct;
+        check(bolap_q19(ctx, &a, &revenue));
+    }
+    return [revenue](const std::string& csv) {  // write_q19_csv, :367-377
+        std::ofstream out(csv);
+        out << "revenue\n" << fixed_double(double(revenue) / 10000.0) << "\n";
+    };
+}
+
 // The reference's CSV quoting (query_impl.cpp:82-96): the value in double
 // quotes, an embedded double quote written as backslash + quote.
 std::string escape_csv_field(const std::string& value) {
+    return quoted_field(value);
+}
+
+std::string quoted_field(const std::string& value) {
     std::string out = "\"";
     for (char ch : value) {
         if (ch == '"') out += "\\\"";
@@ -590,16 +1021,30 @@ int main(int argc, char** argv) {
         bolap_ctx* ctx = nullptr;
         check(bolap_ctx_create(0, nullptr, &ctx));
         const std::string dir = argv[1];
-        // load + build: lineitem, and the other tables when their files exist
-        const Lineitem li = build(ctx, *read_parquet(dir + "lineitem.parquet"));
-        Part part;
-        Orders orders;
+        // load + build (builder_impl.cpp:447-999): the tables whose files
+        // exist; lineitem is required.  Host-side maps first (supplier and
+        // customer nations), then lineitem, part, and the date-sorted orders
+        // with their lineitem ranges.
+        auto load_if = [&](const char* name) -> std::shared_ptr<arrow::Table> {
+            const std::string path = dir + name + ".parquet";
+            return exists(path) ? read_parquet(path) : nullptr;
+        };
+        const auto t_nation = load_if("nation"), t_region = load_if("region");
+        const auto t_supplier = load_if("supplier"), t_customer = load_if("customer");
+        const auto t_part = load_if("part"), t_orders = load_if("orders");
+        Nation nation;
+        if (t_nation) nation = build_nation(*t_nation, t_region.get());
+        std::vector<int16_t> supp_nation;
+        if (t_supplier)
+            supp_nation = nation_by_key(int32s(*column(*t_supplier, "s_suppkey")),
+                                        int32s(*column(*t_supplier, "s_nationkey")));
         Customer customer;
-        if (exists(dir + "part.parquet")) part = build_part(*read_parquet(dir + "part.parquet"));
-        if (exists(dir + "orders.parquet"))
-            orders = build_orders(*read_parquet(dir + "orders.parquet"));
-        if (exists(dir + "customer.parquet"))
-            customer = build_customer(*read_parquet(dir + "customer.parquet"));
+        if (t_customer) customer = build_customer(*t_customer);
+        const Lineitem li = build(ctx, *read_parquet(dir + "lineitem.parquet"), supp_nation);
+        Part part;
+        if (t_part) part = build_part(*t_part);
+        Orders orders;
+        if (t_orders) orders = build_orders(ctx, *t_orders, li, customer.nation_by_key);
         std::vector<std::pair<std::string, std::string>> requests;   // query
         std::string line;
         while (std::getline(std::cin, line)) {
@@ -618,6 +1063,16 @@ int main(int argc, char** argv) {
                 write = run_q1(ctx, li, text);
             } else if (id == "3") {
                 write = run_q3(ctx, li, orders, customer, text);
+            } else if (id == "4") {
+                write = run_q4(ctx, li, orders, text);
+            } else if (id == "5") {
+                write = run_q5(ctx, li, orders, nation, text);
+            } else if (id == "10") {
+                write = run_q10(ctx, li, orders, customer, nation, text);
+            } else if (id == "12") {
+                write = run_q12(ctx, li, orders, text);
+            } else if (id == "19") {
+                write = run_q19(ctx, li, part, text);
             } else if (id == "6") {
                 write = run_q6(ctx, li, text);
             } else if (id == "14") {

@@ -245,3 +245,134 @@ def test_driver_reports_a_missing_file(tmp_path):
     r = subprocess.run([str(B.DRIVER), str(tmp_path) + "/"], input="1 90\n\n",
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "lineitem.parquet" in r.stderr
+
+
+# ------------------------------------------------- Q4, Q5, Q10, Q12, Q19
+def idx(values, v) -> int:
+    return values.index(v) if v in values else -1
+
+
+def esc(v: str) -> str:
+    return '"' + v.replace('"', '\\"') + '"'
+
+
+def fixed_2(v: int) -> str:
+    """append_fixed_2 (bespoke_tpch/query_q10.cpp:108-122)."""
+    s = "-" if v < 0 else ""
+    v = abs(v)
+    return f"{s}{v // 100}.{v % 100:02d}"
+
+
+def csv_q4(db, date):
+    import oracle
+    from tests import tpch_data as T
+    c = oracle.q4(db, T.date_offset(date), T.add_months(date, 3), len(db["priority_dict"]))
+    rows = sorted((db["priority_dict"][i], int(n)) for i, n in enumerate(c) if n)
+    return "o_orderpriority,order_count\n" + "".join(f"{esc(p)},{n}\n" for p, n in rows)
+
+
+def csv_q5(db, region, date):
+    import oracle
+    from tests import tpch_data as T
+    out = "n_name,revenue\n"
+    if region not in db["r_name"]:
+        return out
+    rk = db["r_name"].index(region)
+    inreg = (np.array(db["n_regionkey"]) == rk).astype(np.uint8)
+    rev = oracle.q5(db, inreg, T.date_offset(date), T.add_months(date, 12))
+    rows = sorted(((-int(v), db["n_name"][k]) for k, v in enumerate(rev) if inreg[k] and v))
+    return out + "".join(f"{esc(n)},{-r / 100.0:.2f}\n" for r, n in rows)
+
+
+def csv_q10(db, r, date):
+    import oracle
+    from tests import tpch_data as T
+    nls = len(db["ls_dict"])
+    rc = idx(db["rf_dict"], "R") * nls
+    if rc < 0:
+        return "c_custkey,c_name,revenue,c_acctbal,n_name,c_address,c_phone,c_comment\n"
+    rows = oracle.q10(db, rc, nls, T.date_offset(date), T.add_months(date, 3), 25)
+    out = ["c_custkey,c_name,revenue,c_acctbal,n_name,c_address,c_phone,c_comment\n"]
+    for ck, row, rev in rows.tolist():
+        out.append(f"{ck},{esc(r['c_name'][row])},{fixed_2(rev)},{fixed_2(int(r['c_acctbal'][row]))},"
+                   f"{esc(r['n_name'][int(r['c_nationkey'][row])])},{esc(r['c_address'][row])},"
+                   f"{esc(r['c_phone'][row])},{esc(r['c_comment'][row])}\n")
+    return "".join(out)
+
+
+def csv_q12(db, m1, m2, date):
+    import oracle
+    from tests import tpch_data as T
+    d = db["shipmode_dict"]
+    c1, c2 = -1, -1
+    for i, v in enumerate(d):
+        if v == m1:
+            c1 = i
+        elif v == m2:
+            c2 = i
+    out = "l_shipmode,high_line_count,low_line_count\n"
+    if c1 < 0 and c2 < 0:
+        return out
+    hc = [idx(db["priority_dict"], p) for p in ("1-URGENT", "2-HIGH")]
+    got = oracle.q12(db, c1, c2, T.date_offset(date), T.add_months(date, 12), *hc)
+    rows = []
+    for i, v in enumerate(d):
+        slot = 0 if i == c1 else 1 if i == c2 else -1
+        if slot >= 0 and any(got[slot]):
+            rows.append((v, *got[slot]))
+    return out + "".join(f"{esc(m)},{h},{lo}\n" for m, h, lo in sorted(rows))
+
+
+def csv_q19(db, qtys, brands):
+    import oracle
+    from tests.test_tpch2_oracle import q19_groups
+    q = [None if x == "<<NULL>>" else float(x) for x in qtys]
+    b = [None if x == "<<NULL>>" else x for x in brands]
+    g = q19_groups(db, b, q)
+    rev = 0
+    if any(x[0] for x in g):
+        rev = oracle.q19(db, g, idx(db["shipmode_dict"], "AIR"), -1,
+                         idx(db["instruct_dict"], "DELIVER IN PERSON"))
+    return f"revenue\n{rev / 10000.0:.2f}\n"
+
+
+@pytest.mark.parametrize("orders", [3, 40_001])
+def test_driver_order_range_queries_q12_q19(tmp_path, orders):
+    """Q4, Q5, Q10, Q12 and Q19 through load / build / query over all eight
+    Parquet tables, byte for byte against the oracle over the reference
+    layout and the reference's CSV writers restated in Python."""
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    from tests import tpch_data as T
+    r = T.raw(17 + orders, orders, parts=800, custs=500, supps=50)
+    write(T.parquet(r), tmp_path / "data")
+    db = T.build(r)
+    requests = ['4 "1995-01-01"', '4 1993-07-31', '5 "ASIA" "1994-01-01"', '5 EUROPE 1996-02-29',
+                '5 "ATLANTIS" "1994-01-01"', '10 "1993-10-01"', '10 1994-11-30', '10 2010-01-01',
+                '12 "MAIL" "SHIP" "1994-01-01"', '12 AIR NOSUCH 1995-01-01',
+                '12 NOSUCH NOPE 1995-01-01', '19 "1" "10" "20" "Brand#12" "Brand#23" "Brand#34"',
+                '19 "5" "<<NULL>>" "25" "Brand#11" "Brand#21" "Brand#55"',
+                '19 "<<NULL>>" "<<NULL>>" "<<NULL>>" "Brand#11" "Brand#21" "Brand#55"',
+                '2 "15" "BRASS" "EUROPE"']
+    res = run_driver(tmp_path / "data", tmp_path / "run", requests)
+    assert res.returncode == 0, res.stderr
+    assert [ln.split(" | ")[0] for ln in res.stdout.splitlines()] == \
+        [str(i + 1) for i in range(len(requests))]
+    want = [csv_q4(db, "1995-01-01"), csv_q4(db, "1993-07-31"),
+            csv_q5(db, "ASIA", "1994-01-01"), csv_q5(db, "EUROPE", "1996-02-29"),
+            csv_q5(db, "ATLANTIS", "1994-01-01"), csv_q10(db, r, "1993-10-01"),
+            csv_q10(db, r, "1994-11-30"), csv_q10(db, r, "2010-01-01"),
+            csv_q12(db, "MAIL", "SHIP", "1994-01-01"), csv_q12(db, "AIR", "NOSUCH", "1995-01-01"),
+            csv_q12(db, "NOSUCH", "NOPE", "1995-01-01"),
+            csv_q19(db, ("1", "10", "20"), ("Brand#12", "Brand#23", "Brand#34")),
+            csv_q19(db, ("5", "<<NULL>>", "25"), ("Brand#11", "Brand#21", "Brand#55")),
+            csv_q19(db, ("<<NULL>>",) * 3, ("Brand#11", "Brand#21", "Brand#55")),
+            stub_csv("2", '2 "15" "BRASS" "EUROPE"')]
+    run = tmp_path / "run"
+    for i, w in enumerate(want):
+        assert (run / f"result{i + 1}.csv").read_text() == w, requests[i]
+    assert "query 2 is not served" in res.stderr
+    if orders > 3:
+        assert want[0].count("\n") == 6 and want[5].count("\n") > 50
+        assert want[2].count("\n") > 2 and want[8].count("\n") == 3
+        assert want[11] != "revenue\n0.00\n"
