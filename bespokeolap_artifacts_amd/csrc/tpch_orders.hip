@@ -758,3 +758,239 @@ extern "C" int bolap_q19(bolap_ctx* ctx, const bolap_q19_args* a, int64_t* reven
     }
     return fetch(ctx, revenue, res, 8);
 }
+
+// ---------------------------------------------------------------------------
+// Q21 and Q22: the anti-join queries.
+// ---------------------------------------------------------------------------
+namespace bolap {
+namespace {
+
+// suppkey -> (row + 1) << 1 | in target nation, of the last supplier row of
+// the key (atomic max over row numbers).
+__global__ __launch_bounds__(kT) void q21_owner_k(const int32_t* __restrict__ sk,
+                                                  const int32_t* __restrict__ sn, int64_t rows,
+                                                  int64_t n, int32_t nation,
+                                                  int32_t* __restrict__ owner) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = sk[i];
+        if (k >= 0 && k < n)
+            atomicMax(owner + k, int32_t((uint32_t(i + 1) << 1) | uint32_t(sn[i] == nation)));
+    }
+}
+
+// One order per lane over every order: a final-status order with more than
+// one lineitem, several suppliers and exactly one late supplier adds its
+// late-line count to that supplier when the supplier is a target.
+__global__ __launch_bounds__(kT) void q21_k(int64_t o_rows, const uint8_t* __restrict__ status,
+                                            int32_t final_code, const int32_t* __restrict__ okey,
+                                            LiSide li, const int32_t* __restrict__ sk,
+                                            const uint32_t* __restrict__ cr,
+                                            const int32_t* __restrict__ owner, int64_t n,
+                                            unsigned long long* __restrict__ wait) {
+    for (int64_t o = int64_t(blockIdx.x) * kT + threadIdx.x; o < o_rows;
+         o += int64_t(gridDim.x) * kT) {
+        if (int32_t(status[o]) != final_code) continue;
+        const uint32_t e = li.end[o];
+        if (e == 0) continue;
+        const uint32_t s = li.start[o];
+        if (e - s <= 1) continue;
+        const int32_t key = li.okey ? okey[o] : 0;
+        int32_t first = -1, late_supp = -1, late_n = 0;
+        bool multi = false, multi_late = false;
+        for (uint32_t r = s; r < e; ++r) {
+            if (li.okey && li.okey[r] != key) continue;
+            const int32_t k = sk[r];
+            if (uint32_t(k) >= uint64_t(n)) continue;
+            if (first < 0) first = k;
+            else if (k != first) multi = true;
+            const uint32_t p = cr[r];
+            if ((p >> 16) > (p & 0xffffu)) {
+                if (late_supp == -1 || late_supp == k) {
+                    late_supp = k;
+                    ++late_n;
+                } else {
+                    multi_late = true;
+                    break;
+                }
+            }
+        }
+        if (multi_late || !multi || late_supp < 0) continue;
+        if (owner[late_supp] & 1) atomicAdd(wait + late_supp, (unsigned long long)late_n);
+    }
+}
+
+struct Q22F {
+    uint64_t allowed[2];  // two-digit prefix codes 0..99
+    int32_t n_extra;
+    int32_t extra[7];
+};
+
+// Prefix slot of a customer: the code (0..99), 100 + extra index, or -1.
+BOLAP_DEV int32_t q22_slot(const Q22F& f, uint8_t code, int32_t pair) {
+    if (code != 255) return code < 100 && ((f.allowed[code >> 6] >> (code & 63)) & 1) ? code : -1;
+    if (pair < 0) return -1;
+    for (int e = 0; e < f.n_extra; ++e)
+        if (f.extra[e] == pair) return 100 + e;
+    return -1;
+}
+
+__global__ __launch_bounds__(kT) void q22_mark_k(const int32_t* __restrict__ ock, int64_t rows,
+                                                 int64_t size, uint8_t* __restrict__ has) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = ock[i];
+        if (k >= 0 && k < size) has[k] = 1;
+    }
+}
+
+// Sum and count of the candidates' acctbal (acctbal > 0) into tot[0], tot[1].
+__global__ __launch_bounds__(kT) void q22_avg_k(Q22F f, int64_t rows,
+                                                const uint8_t* __restrict__ code,
+                                                const int32_t* __restrict__ pair,
+                                                const int32_t* __restrict__ bal,
+                                                unsigned long long* __restrict__ tot) {
+    uint64_t s = 0, c = 0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        if (q22_slot(f, code[i], pair[i]) >= 0 && bal[i] > 0) {
+            s += uint64_t(int64_t(bal[i]));
+            ++c;
+        }
+    }
+    s = wave_sum_u64(s);
+    c = wave_sum_u64(c);
+    if ((threadIdx.x & (kWave - 1)) == 0 && c) {
+        atomicAdd(tot, (unsigned long long)s);
+        atomicAdd(tot + 1, (unsigned long long)c);
+    }
+}
+
+// Candidates above the floor of the average and without orders, per slot.
+__global__ __launch_bounds__(kT) void q22_agg_k(Q22F f, int64_t rows,
+                                                const int32_t* __restrict__ ck,
+                                                const uint8_t* __restrict__ code,
+                                                const int32_t* __restrict__ pair,
+                                                const int32_t* __restrict__ bal,
+                                                const uint8_t* __restrict__ has, int64_t size,
+                                                unsigned long long* __restrict__ tot) {
+    __shared__ unsigned long long s_agg[107 * 2];
+    for (int i = threadIdx.x; i < 107 * 2; i += kT) s_agg[i] = 0;
+    __syncthreads();
+    const uint64_t cnt = tot[1];
+    if (cnt > 0) {
+        const int64_t avg = int64_t(tot[0]) / int64_t(cnt);
+        for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+             i += int64_t(gridDim.x) * kT) {
+            const int32_t slot = q22_slot(f, code[i], pair[i]);
+            if (slot < 0 || bal[i] <= 0 || int64_t(bal[i]) <= avg) continue;
+            const int32_t k = ck[i];
+            if (k >= 0 && k < size && has[k]) continue;
+            atomicAdd(&s_agg[2 * slot], 1ull);
+            atomicAdd(&s_agg[2 * slot + 1], (unsigned long long)int64_t(bal[i]));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 107 * 2; i += kT)
+        if (s_agg[i]) atomicAdd(tot + 2 + i, s_agg[i]);
+}
+
+}  // namespace
+}  // namespace bolap
+
+extern "C" int bolap_q21(bolap_ctx* ctx, const bolap_q21_args* a, int64_t* wait,
+                         int64_t n_suppkeys) {
+    BOLAP_ARG_CHECK(ctx && a && (wait || n_suppkeys == 0), "bolap_q21: null argument");
+    BOLAP_ARG_CHECK(n_suppkeys >= 0 && n_suppkeys < (int64_t(1) << 31),
+                    "bolap_q21: %lld supplier keys", (long long)n_suppkeys);
+    BOLAP_ARG_CHECK(a->order_rows >= 0 && a->supp_rows >= 0 && a->supp_rows < (int64_t(1) << 30),
+                    "bolap_q21: rows out of range");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || (a->o_orderstatus && a->range_start && a->range_end &&
+                                           a->l_suppkey && a->l_commit_receipt),
+                    "bolap_q21: null column");
+    BOLAP_ARG_CHECK(a->l_orderkey_sorted || a->order_rows == 0 || (a->o_orderkey && a->l_orderkey),
+                    "bolap_q21: unsorted lineitem needs l_orderkey and o_orderkey");
+    BOLAP_ARG_CHECK(a->supp_rows == 0 || (a->s_suppkey && a->s_nationkey),
+                    "bolap_q21: null supplier column");
+    if (n_suppkeys == 0) return BOLAP_OK;
+    WsArena ws;
+    const size_t o_owner = ws.plan(size_t(n_suppkeys) * 4);
+    const size_t o_wait = ws.plan(size_t(n_suppkeys) * 8);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* owner = ws.at<int32_t>(o_owner);
+    auto* w = ws.at<unsigned long long>(o_wait);
+    BOLAP_TRY(zero_dev(ctx, owner, ws.off));
+    TraceScope ts(ctx, "q21_orders");
+    if (a->supp_rows > 0) {
+        hipLaunchKernelGGL(q21_owner_k, dim3(grid_of(ctx, a->supp_rows)), dim3(kT), 0, ctx->stream,
+                           a->s_suppkey, a->s_nationkey, a->supp_rows, n_suppkeys, a->nation,
+                           owner);
+        BOLAP_TRY(launch_check("q21_owner_k"));
+    }
+    if (a->order_rows > 0 && a->final_code >= 0) {
+        const LiSide li{a->range_start, a->range_end,
+                        a->l_orderkey_sorted ? nullptr : a->l_orderkey};
+        hipLaunchKernelGGL(q21_k, dim3(grid_of(ctx, a->order_rows, 8)), dim3(kT), 0, ctx->stream,
+                           a->order_rows, a->o_orderstatus, a->final_code, a->o_orderkey, li,
+                           a->l_suppkey, a->l_commit_receipt, owner, n_suppkeys, w);
+        BOLAP_TRY(launch_check("q21_k"));
+    }
+    return fetch(ctx, wait, w, size_t(n_suppkeys) * 8);
+}
+
+extern "C" int bolap_q22(bolap_ctx* ctx, const bolap_q22_args* a, bolap_q22_result* out) {
+    BOLAP_ARG_CHECK(ctx && a && out, "bolap_q22: null argument");
+    BOLAP_ARG_CHECK(a->cust_rows >= 0 && a->order_rows >= 0, "bolap_q22: negative rows");
+    BOLAP_ARG_CHECK(a->n_extra >= 0 && a->n_extra <= 7, "bolap_q22: %d extra prefixes",
+                    a->n_extra);
+    BOLAP_ARG_CHECK(a->cust_rows == 0 || (a->c_custkey && a->c_phone_code && a->c_phone_pair &&
+                                          a->c_acctbal),
+                    "bolap_q22: null customer column");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || a->o_custkey, "bolap_q22: null o_custkey");
+    std::memset(out, 0, sizeof(*out));
+    if (a->cust_rows == 0) return BOLAP_OK;
+    Q22F f{};
+    for (int c = 0; c < 100; ++c)
+        if (a->numeric_allowed[c]) f.allowed[c >> 6] |= uint64_t(1) << (c & 63);
+    f.n_extra = a->n_extra;
+    for (int e = 0; e < a->n_extra; ++e) f.extra[e] = a->extra_pairs[e];
+    int32_t* slot = static_cast<int32_t*>(ctx->result_dev);
+    int32_t kmax = 0;
+    BOLAP_TRY(max_i32(ctx, a->c_custkey, a->cust_rows, slot, &kmax));
+    const int64_t size = kmax >= 0 ? int64_t(kmax) + 1 : 0;
+    WsArena ws;
+    const size_t o_has = ws.plan(size_t(size > 0 ? size : 1));
+    const size_t o_tot = ws.plan((2 + 107 * 2) * 8);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* has = ws.at<uint8_t>(o_has);
+    auto* tot = ws.at<unsigned long long>(o_tot);
+    BOLAP_TRY(zero_dev(ctx, has, ws.off));
+    {
+        TraceScope ts(ctx, "q22_customers");
+        if (a->order_rows > 0 && size > 0) {
+            hipLaunchKernelGGL(q22_mark_k, dim3(grid_of(ctx, a->order_rows, 8)), dim3(kT), 0,
+                               ctx->stream, a->o_custkey, a->order_rows, size, has);
+            BOLAP_TRY(launch_check("q22_mark_k"));
+        }
+        const unsigned g = grid_of(ctx, a->cust_rows, 4);
+        hipLaunchKernelGGL(q22_avg_k, dim3(g), dim3(kT), 0, ctx->stream, f, a->cust_rows,
+                           a->c_phone_code, a->c_phone_pair, a->c_acctbal, tot);
+        hipLaunchKernelGGL(q22_agg_k, dim3(g), dim3(kT), 0, ctx->stream, f, a->cust_rows,
+                           a->c_custkey, a->c_phone_code, a->c_phone_pair, a->c_acctbal, has, size,
+                           tot);
+        BOLAP_TRY(launch_check("q22_agg_k"));
+    }
+    uint64_t h[2 + 107 * 2];
+    BOLAP_TRY(fetch(ctx, h, tot, sizeof(h)));
+    for (int c = 0; c < 100; ++c) {
+        out->numcust[c] = int64_t(h[2 + 2 * c]);
+        out->totacctbal[c] = int64_t(h[3 + 2 * c]);
+    }
+    for (int e = 0; e < 7; ++e) {
+        out->extra_numcust[e] = int64_t(h[2 + 2 * (100 + e)]);
+        out->extra_totacctbal[e] = int64_t(h[3 + 2 * (100 + e)]);
+    }
+    return BOLAP_OK;
+}

@@ -1,5 +1,5 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q3, Q4, Q5, Q6, Q10, Q12, Q14, Q17, Q18 and Q19, with the kernels
+// TPC-H Q1, Q3, Q4, Q5, Q6, Q10, Q12, Q14, Q17, Q18, Q19, Q21 and Q22, with the kernels
 // of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
@@ -21,7 +21,8 @@
 //        `1 DELTA`, `3 SEGMENT DATE`, `4 DATE`, `5 REGION DATE`,
 //        `6 DATE DISCOUNT QUANTITY`, `10 DATE`, `12 MODE1 MODE2 DATE`,
 //        `14 DATE`, `17 BRAND CONTAINER`, `18 QUANTITY`,
-//        `19 Q1 Q2 Q3 BRAND1 BRAND2 BRAND3` with std::quoted tokens
+//        `19 Q1 Q2 Q3 BRAND1 BRAND2 BRAND3`, `21 NATION`, `22 I1 .. I7`
+//        with std::quoted tokens
 //        (args_parser.hpp), runs them in order, writes resultN.csv and prints
 //        "N | Execution ms: T" with the timer stopped before the file is
 //        written (query_impl.cpp:161-169); an id without a pipeline gets the
@@ -147,6 +148,11 @@ std::vector<std::string> strings(const arrow::ChunkedArray& col) {
     return out;
 }
 
+std::vector<std::string> strings_or_empty(const arrow::Table& t, const char* name) {
+    if (!t.GetColumnByName(name)) return std::vector<std::string>(size_t(t.num_rows()));
+    return strings(*column(t, name));
+}
+
 std::vector<int16_t> date_offsets(const arrow::ChunkedArray& col) {
     std::vector<int16_t> out;
     out.reserve(size_t(col.length()));
@@ -231,6 +237,7 @@ struct Lineitem {
     uint8_t *shipmode = nullptr, *shipinstruct = nullptr;
     std::vector<std::string> shipmodes, instructs;
     int16_t* supp_nation = nullptr;           // supplier nation per row (:838-853)
+    int32_t* suppkey = nullptr;
 };
 
 struct Part {
@@ -251,6 +258,8 @@ struct Orders {
     std::unordered_map<int32_t, int32_t> shippriority;   // by order key (Q3's output)
     uint8_t* priority = nullptr;                          // o_orderpriority codes
     std::vector<std::string> priorities;
+    uint8_t* status = nullptr;                            // o_orderstatus codes
+    std::vector<std::string> statuses;
     int16_t* cust_nation = nullptr;                       // customer nation (:913-928)
     int32_t* order_map = nullptr;                         // orderkey -> row (:610-630)
     int64_t map_size = 0;
@@ -265,20 +274,41 @@ struct Customer {
     std::vector<std::string> segments, name, address, phone, comment;
     std::vector<int32_t> acctbal;          // cents
     std::vector<int32_t> h_nationkey;      // c_nationkey by row
+    uint8_t* phone_code = nullptr;         // phone_prefix_code (builder_impl.cpp:152-178)
+    int32_t* phone_pair = nullptr;         // first two characters, -1 when shorter
+    int32_t* acctbal_dev = nullptr;
     std::vector<int16_t> nation_by_key;    // nationkey_by_custkey (:675-692)
 };
+
+struct Supplier {
+    int64_t rows = -1;
+    int32_t *suppkey = nullptr, *nationkey = nullptr;
+    std::vector<std::string> name;
+    std::vector<int32_t> row_by_key;  // suppkey -> last row, -1 absent
+    int32_t max_key = -1;
+};
+
+Supplier build_supplier(const arrow::Table& t) {
+    Supplier s;
+    s.rows = t.num_rows();
+    const auto keys = int32s(*column(t, "s_suppkey"));
+    s.suppkey = upload(keys);
+    s.nationkey = upload(int32s(*column(t, "s_nationkey")));
+    s.name = strings_or_empty(t, "s_name");
+    for (int32_t k : keys) s.max_key = std::max(s.max_key, k);
+    s.row_by_key.assign(size_t(s.max_key + 1), -1);
+    for (size_t i = 0; i < keys.size(); ++i)
+        if (keys[i] >= 0) s.row_by_key[size_t(keys[i])] = int32_t(i);
+    return s;
+}
 
 struct Nation {
     std::vector<int32_t> key, region;
     std::vector<std::string> name;
     std::unordered_map<std::string, int32_t> region_by_name;  // region.name_to_key
+    std::unordered_map<std::string, int32_t> key_by_name;     // nation.name_to_key
     int32_t max_key = 0;                                       // at least 0, as the queries
 };
-
-std::vector<std::string> strings_or_empty(const arrow::Table& t, const char* name) {
-    if (!t.GetColumnByName(name)) return std::vector<std::string>(size_t(t.num_rows()));
-    return strings(*column(t, name));
-}
 
 // key -> nation of the table's rows, -1 for keys without a row (last row
 // of a key wins), as nationkey_by_custkey / nationkey_by_suppkey.
@@ -349,6 +379,7 @@ Lineitem build(bolap_ctx* ctx, const arrow::Table& t, const std::vector<int16_t>
     }
     if (t.GetColumnByName("l_suppkey")) {
         const auto sk = int32s(*column(t, "l_suppkey"));
+        li.suppkey = upload(sk);
         std::vector<int16_t> sn(sk.size());
         for (size_t i = 0; i < sk.size(); ++i) sn[i] = nation_of(supp_nation, sk[i]);
         li.supp_nation = upload(sn);
@@ -410,6 +441,11 @@ Orders build_orders(bolap_ctx* ctx, const arrow::Table& t, const Lineitem& li,
         o.priority = upload(permuted(narrow_codes(d, "o_orderpriority"), order));
         o.priorities = d.values;
     }
+    if (t.GetColumnByName("o_orderstatus")) {
+        Dict d = dictionary(*column(t, "o_orderstatus"));
+        o.status = upload(permuted(narrow_codes(d, "o_orderstatus"), order));
+        o.statuses = d.values;
+    }
     std::vector<int16_t> cn(cust.size());
     for (size_t i = 0; i < cust.size(); ++i) cn[i] = nation_of(cust_nation, cust[i]);
     o.cust_nation = upload(cn);
@@ -450,8 +486,24 @@ Customer build_customer(const arrow::Table& t) {
     } else {
         c.acctbal.assign(size_t(c.rows), 0);
     }
+    c.acctbal_dev = upload(c.acctbal);
     c.address = strings_or_empty(t, "c_address");
     c.phone = strings_or_empty(t, "c_phone");
+    std::vector<uint8_t> code(c.phone.size());
+    std::vector<int32_t> pair(c.phone.size());
+    for (size_t i = 0; i < c.phone.size(); ++i) {
+        const std::string& ph = c.phone[i];
+        if (ph.size() < 2) {
+            code[i] = 255;
+            pair[i] = -1;
+            continue;
+        }
+        const bool digits = ph[0] >= '0' && ph[0] <= '9' && ph[1] >= '0' && ph[1] <= '9';
+        code[i] = digits ? uint8_t((ph[0] - '0') * 10 + (ph[1] - '0')) : uint8_t(255);
+        pair[i] = int32_t(uint32_t(uint8_t(ph[0])) << 8 | uint8_t(ph[1]));
+    }
+    c.phone_code = upload(code);
+    c.phone_pair = upload(pair);
     c.comment = strings_or_empty(t, "c_comment");
     return c;
 }
@@ -462,6 +514,7 @@ Nation build_nation(const arrow::Table& n, const arrow::Table* r) {
     x.region = int32s(*column(n, "n_regionkey"));
     x.name = strings(*column(n, "n_name"));
     for (int32_t k : x.key) x.max_key = std::max(x.max_key, k);
+    for (size_t i = 0; i < x.key.size(); ++i) x.key_by_name.emplace(x.name[i], x.key[i]);
     if (r) {
         const auto rk = int32s(*column(*r, "r_regionkey"));
         const auto rn = strings(*column(*r, "r_name"));
@@ -981,6 +1034,107 @@ Writer run_q19(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::
     };
 }
 
+Writer run_q21(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Supplier& sup,
+               const Nation& nat, const std::string& line) {
+    need(o.rows >= 0 && o.status && sup.rows >= 0 && li.suppkey && li.commit_receipt,
+         "Q21 needs orders (with o_orderstatus), supplier, l_suppkey and commit/receipt dates");
+    const auto v = quoted_args(line, 1, "Q21");
+    std::vector<std::pair<std::string, int64_t>> rows;
+    const auto nk = nat.key_by_name.find(v[0]);
+    const int32_t f = code_of(o.statuses, "F");  // query_q21.cpp:160-175
+    if (v[0] != "<<NULL>>" && nk != nat.key_by_name.end() && f >= 0 && sup.max_key >= 0) {
+        bolap_q21_args a{};
+        a.order_rows = o.rows;
+        a.o_orderstatus = o.status;
+        a.final_code = f;
+        a.o_orderkey = o.orderkey;
+        a.range_start = o.range_start;
+        a.range_end = o.range_end;
+        a.l_orderkey = li.orderkey;
+        a.l_orderkey_sorted = li.orderkey_sorted;
+        a.l_suppkey = li.suppkey;
+        a.l_commit_receipt = li.commit_receipt;
+        a.supp_rows = sup.rows;
+        a.s_suppkey = sup.suppkey;
+        a.s_nationkey = sup.nationkey;
+        a.nation = nk->second;
+        const int64_t n = int64_t(sup.max_key) + 1;
+        std::vector<int64_t> wait(static_cast<size_t>(n));
+        check(bolap_q21(ctx, &a, wait.data(), n));
+        for (int64_t k = 0; k < n; ++k) {  // :336-356
+            if (wait[size_t(k)] == 0 || sup.row_by_key[size_t(k)] < 0) continue;
+            rows.emplace_back(sup.name[size_t(sup.row_by_key[size_t(k)])], wait[size_t(k)]);
+        }
+        std::sort(rows.begin(), rows.end(), [](const auto& x, const auto& y) {
+            return x.second != y.second ? x.second > y.second : x.first < y.first;
+        });
+    }
+    return [rows](const std::string& csv) {  // write_q21_csv
+        std::ofstream out(csv);
+        out << "s_name,numwait\n";
+        for (const auto& [n, w] : rows) out << quoted_field(n) << ',' << w << "\n";
+    };
+}
+
+Writer run_q22(bolap_ctx* ctx, const Orders& o, const Customer& c, const std::string& line) {
+    need(o.rows >= 0 && c.rows >= 0, "Q22 needs orders and customer");
+    const auto v = quoted_args(line, 7, "Q22");
+    // the country-code filter (query_q22.cpp:38-70): two leading digits, or
+    // else a two-character prefix; duplicates, "<<NULL>>" and short values
+    // are skipped
+    bolap_q22_args a{};
+    std::vector<int32_t> numeric;
+    std::vector<std::string> extra;
+    for (const auto& s : v) {
+        if (s == "<<NULL>>" || s.size() < 2) continue;
+        if (s[0] >= '0' && s[0] <= '9' && s[1] >= '0' && s[1] <= '9') {
+            const int32_t code = (s[0] - '0') * 10 + (s[1] - '0');
+            if (!a.numeric_allowed[code]) {
+                a.numeric_allowed[code] = 1;
+                numeric.push_back(code);
+            }
+            continue;
+        }
+        const std::string prefix = s.substr(0, 2);
+        if (std::find(extra.begin(), extra.end(), prefix) == extra.end()) extra.push_back(prefix);
+    }
+    struct Row {
+        std::string code;
+        int64_t n, total;
+    };
+    std::vector<Row> rows;
+    if (!numeric.empty() || !extra.empty()) {
+        a.cust_rows = c.rows;
+        a.c_custkey = c.custkey;
+        a.c_phone_code = c.phone_code;
+        a.c_phone_pair = c.phone_pair;
+        a.c_acctbal = c.acctbal_dev;
+        a.n_extra = int32_t(extra.size());
+        for (size_t i = 0; i < extra.size(); ++i)
+            a.extra_pairs[i] = int32_t(uint32_t(uint8_t(extra[i][0])) << 8 | uint8_t(extra[i][1]));
+        a.order_rows = o.rows;
+        a.o_custkey = o.custkey;
+        bolap_q22_result r{};
+        check(bolap_q22(ctx, &a, &r));
+        for (int32_t code : numeric)
+            if (r.numcust[code] != 0)
+                rows.push_back({std::string{char('0' + code / 10), char('0' + code % 10)},
+                                r.numcust[code], r.totacctbal[code]});
+        for (size_t i = 0; i < extra.size(); ++i)
+            if (r.extra_numcust[i] != 0)
+                rows.push_back({extra[i], r.extra_numcust[i], r.extra_totacctbal[i]});
+        std::sort(rows.begin(), rows.end(),
+                  [](const Row& x, const Row& y) { return x.code < y.code; });
+    }
+    return [rows](const std::string& csv) {  // write_q22_csv
+        std::ofstream out(csv);
+        out << "cntrycode,numcust,totacctbal\n";
+        for (const auto& r : rows)
+            out << quoted_field(r.code) << ',' << r.n << ',' << fixed_double(double(r.total) / 100.0)
+                << "\n";
+    };
+}
+
 // The reference's CSV quoting (query_impl.cpp:82-96): the value in double
 // quotes, an embedded double quote written as backslash + quote.
 std::string escape_csv_field(const std::string& value) {
@@ -1035,9 +1189,12 @@ int main(int argc, char** argv) {
         Nation nation;
         if (t_nation) nation = build_nation(*t_nation, t_region.get());
         std::vector<int16_t> supp_nation;
-        if (t_supplier)
+        Supplier supplier;
+        if (t_supplier) {
             supp_nation = nation_by_key(int32s(*column(*t_supplier, "s_suppkey")),
                                         int32s(*column(*t_supplier, "s_nationkey")));
+            supplier = build_supplier(*t_supplier);
+        }
         Customer customer;
         if (t_customer) customer = build_customer(*t_customer);
         const Lineitem li = build(ctx, *read_parquet(dir + "lineitem.parquet"), supp_nation);
@@ -1073,6 +1230,10 @@ int main(int argc, char** argv) {
                 write = run_q12(ctx, li, orders, text);
             } else if (id == "19") {
                 write = run_q19(ctx, li, part, text);
+            } else if (id == "21") {
+                write = run_q21(ctx, li, orders, supplier, nation, text);
+            } else if (id == "22") {
+                write = run_q22(ctx, orders, customer, text);
             } else if (id == "6") {
                 write = run_q6(ctx, li, text);
             } else if (id == "14") {

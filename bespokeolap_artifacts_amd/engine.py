@@ -584,6 +584,38 @@ class Engine:
         N.check(self.lib.bolap_q19(self.ctx, C.byref(a), C.byref(r)))
         return r.value
 
+    def q21(self, t: dict, final_code: int, nation: int, n_suppkeys: int) -> list:
+        a = N.Q21Args()
+        a.order_rows = int(t["o_orderstatus"].numel())
+        a.o_orderstatus, a.final_code = ptr(t["o_orderstatus"]), int(final_code)
+        a.o_orderkey, a.range_start, a.range_end = (ptr(t["o_orderkey"]), ptr(t["r_start"]),
+                                                    ptr(t["r_end"]))
+        a.l_orderkey, a.l_orderkey_sorted = ptr(t["l_orderkey"]), int(t["l_sorted"])
+        a.l_suppkey, a.l_commit_receipt = ptr(t["l_suppkey"]), ptr(t["l_commit_receipt"])
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.s_nationkey, a.nation = ptr(t["s_suppkey"]), ptr(t["s_nationkey"]), nation
+        out = (C.c_int64 * max(1, n_suppkeys))()
+        N.check(self.lib.bolap_q21(self.ctx, C.byref(a), out, int(n_suppkeys)))
+        return list(out)[:n_suppkeys]
+
+    def q22(self, t: dict, numeric_codes, extra_pairs):
+        a = N.Q22Args()
+        a.cust_rows = int(t["c_custkey"].numel())
+        a.c_custkey, a.c_phone_code = ptr(t["c_custkey"]), ptr(t["c_phone_code"])
+        a.c_phone_pair, a.c_acctbal = ptr(t["c_phone_pair"]), ptr(t["c_acctbal"])
+        for c in numeric_codes:
+            a.numeric_allowed[c] = 1
+        a.n_extra = len(extra_pairs)
+        for i, p in enumerate(extra_pairs):
+            a.extra_pairs[i] = p
+        a.order_rows = int(t["o_custkey"].numel())
+        a.o_custkey = ptr(t["o_custkey"])
+        r = N.Q22Result()
+        N.check(self.lib.bolap_q22(self.ctx, C.byref(a), C.byref(r)))
+        num = [[r.numcust[c], r.totacctbal[c]] for c in range(100)]
+        ext = [[r.extra_numcust[i], r.extra_totacctbal[i]] for i in range(len(extra_pairs))]
+        return num, ext
+
     def q17_args(self, t: dict, preds: list) -> N.Q17Args:
         a = N.Q17Args()
         a.part_rows = int(t["p_partkey"].numel())

@@ -304,6 +304,26 @@ class Q19Args(C.Structure):
                 ("mode1", C.c_int32), ("mode2", C.c_int32), ("shipinstruct", C.c_int32)]
 
 
+class Q21Args(C.Structure):
+    _fields_ = [("order_rows", C.c_int64), ("o_orderstatus", vp), ("final_code", C.c_int32),
+                ("o_orderkey", vp), ("range_start", vp), ("range_end", vp), ("l_orderkey", vp),
+                ("l_orderkey_sorted", C.c_int32), ("l_suppkey", vp), ("l_commit_receipt", vp),
+                ("supp_rows", C.c_int64), ("s_suppkey", vp), ("s_nationkey", vp),
+                ("nation", C.c_int32)]
+
+
+class Q22Args(C.Structure):
+    _fields_ = [("cust_rows", C.c_int64), ("c_custkey", vp), ("c_phone_code", vp),
+                ("c_phone_pair", vp), ("c_acctbal", vp), ("numeric_allowed", C.c_uint8 * 100),
+                ("n_extra", C.c_int32), ("extra_pairs", C.c_int32 * 7), ("order_rows", C.c_int64),
+                ("o_custkey", vp)]
+
+
+class Q22Result(C.Structure):
+    _fields_ = [("numcust", C.c_int64 * 100), ("totacctbal", C.c_int64 * 100),
+                ("extra_numcust", C.c_int64 * 7), ("extra_totacctbal", C.c_int64 * 7)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -372,6 +392,8 @@ PROTOTYPES = {
     "bolap_q10": (C.c_int, [vp, P(Q10Args), P(Q10Row), i64, P(i64)]),
     "bolap_q12": (C.c_int, [vp, P(Q12Args), P(Q12Result)]),
     "bolap_q19": (C.c_int, [vp, P(Q19Args), P(i64)]),
+    "bolap_q21": (C.c_int, [vp, P(Q21Args), P(i64), i64]),
+    "bolap_q22": (C.c_int, [vp, P(Q22Args), P(Q22Result)]),
     "bolap_q14_write_csv": (C.c_int, [C.c_char_p, P(Q14Result)]),
     "bolap_q17": (C.c_int, [vp, P(Q17Args), P(Q17Result)]),
     "bolap_q17_write_csv": (C.c_int, [C.c_char_p, P(Q17Result)]),

@@ -947,6 +947,58 @@ typedef struct {
 } bolap_q19_args;
 int bolap_q19(bolap_ctx* ctx, const bolap_q19_args* args, int64_t* revenue);
 
+/* Q21 (bespoke_tpch/query_q21.cpp:140-380): over every order whose status
+   code is final_code ("F"; -1 = absent) and whose lineitem range has more
+   than one row, the lineitems with suppkey in [0, n_suppkeys) (key-checked
+   when unsorted): the order counts when its lines have several suppliers
+   and exactly one supplier with late lines (receiptdate > commitdate, the
+   packed halves unsigned; a second late supplier ends the walk); that
+   supplier's late-line count is added when the last supplier row of its
+   key has s_nationkey == nation.  wait[n_suppkeys] (host) per supplier key;
+   n_suppkeys = max s_suppkey + 1. */
+typedef struct {
+    int64_t order_rows;
+    const uint8_t* o_orderstatus;    /* codes */
+    int32_t final_code;
+    const int32_t* o_orderkey;       /* needed when l_orderkey_sorted == 0 */
+    const uint32_t* range_start;
+    const uint32_t* range_end;
+    const int32_t* l_orderkey;       /* needed when l_orderkey_sorted == 0 */
+    int32_t l_orderkey_sorted;
+    const int32_t* l_suppkey;
+    const uint32_t* l_commit_receipt;
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    const int32_t* s_nationkey;
+    int32_t nation;
+} bolap_q21_args;
+int bolap_q21(bolap_ctx* ctx, const bolap_q21_args* args, int64_t* wait, int64_t n_suppkeys);
+
+/* Q22 (bespoke_tpch/query_q22.cpp:190-520): a customer is a candidate when
+   its phone prefix is in the filter — the two-digit prefix code c_phone_code
+   (255 when the phone does not start with two digits, builder_impl.cpp:
+   152-178) with numeric_allowed[code], or else its first two characters
+   c_phone_pair (c0 << 8 | c1, -1 when shorter) equal to one of extra_pairs —
+   and c_acctbal > 0.  Candidates above floor(sum / count) of the candidates'
+   acctbal whose key no order carries count per prefix. */
+typedef struct {
+    int64_t cust_rows;
+    const int32_t* c_custkey;
+    const uint8_t* c_phone_code;
+    const int32_t* c_phone_pair;
+    const int32_t* c_acctbal;        /* cents */
+    uint8_t numeric_allowed[100];
+    int32_t n_extra;                 /* <= 7 */
+    int32_t extra_pairs[7];
+    int64_t order_rows;
+    const int32_t* o_custkey;
+} bolap_q22_args;
+typedef struct {
+    int64_t numcust[100], totacctbal[100];          /* by two-digit code */
+    int64_t extra_numcust[7], extra_totacctbal[7];  /* by extra prefix */
+} bolap_q22_result;
+int bolap_q22(bolap_ctx* ctx, const bolap_q22_args* args, bolap_q22_result* out);
+
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */
 /* ------------------------------------------------------------------------ */

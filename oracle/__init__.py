@@ -772,3 +772,52 @@ def q19(db: dict, groups, mode1: int, mode2: int, instruct: int) -> int:
     return int(_tpch2().oracle_q19(len(p[0]), *[_p(x) for x in p], len(li[0]),
                                    *[_p(x) for x in li], q19_groups(groups), mode1, mode2,
                                    instruct))
+
+
+def q21(db: dict, final_code: int, target_nation: int, n_suppkeys: int) -> np.ndarray:
+    """Late-line counts per supplier key of run_q21 (oracle_q21)."""
+    L = lib()
+    if not getattr(L, "_q21_ready", False):
+        L.oracle_q21.restype = None
+        L.oracle_q21.argtypes = [i64, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, i64, vp]
+        L.oracle_q22.restype = None
+        L.oracle_q22.argtypes = [i64, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp]
+        L._q21_ready = True
+    tgt = np.zeros(max(1, n_suppkeys), np.uint8)
+    sk, sn = db["s_suppkey"], db["s_nationkey"]
+    for k, nk in zip(sk.tolist(), sn.tolist()):
+        if 0 <= k < n_suppkeys:
+            tgt[k] = 1 if nk == target_nation else 0   # last row of a key wins
+    out = np.zeros(max(1, n_suppkeys), np.int64)
+    a = [_c(db["o_orderkey"], np.int32), _c(db["o_orderstatus"], np.uint16)]
+    r = [_c(db["r_start"], np.uint32), _c(db["r_end"], np.uint32), _c(db["l_orderkey"], np.int32)]
+    li = [_c(db["l_suppkey"], np.int32), _c(db["l_commit_receipt"], np.uint32)]
+    L.oracle_q21(len(a[0]), _p(a[0]), _p(a[1]), final_code, _p(r[0]), _p(r[1]), _p(r[2]),
+                 int(db["l_sorted"]), _p(li[0]), _p(li[1]), _p(tgt), n_suppkeys, _p(out))
+    return out[:n_suppkeys]
+
+
+def q22(db: dict, numeric_codes, extra_pairs):
+    """(numeric[100, 2], extra[n, 2]) = (customers, acctbal sum) per prefix of
+    run_q22 (oracle_q22)."""
+    q21  # noqa: B018 (argtypes are registered together)
+    L = lib()
+    if not getattr(L, "_q21_ready", False):
+        q21({"s_suppkey": np.zeros(0, np.int32), "s_nationkey": np.zeros(0, np.int32),
+             "o_orderkey": np.zeros(0, np.int32), "o_orderstatus": np.zeros(0, np.uint16),
+             "r_start": np.zeros(0, np.uint32), "r_end": np.zeros(0, np.uint32),
+             "l_orderkey": np.zeros(0, np.int32), "l_sorted": 1,
+             "l_suppkey": np.zeros(0, np.int32), "l_commit_receipt": np.zeros(0, np.uint32)},
+            0, 0, 0)
+    allowed = np.zeros(100, np.uint8)
+    for c in numeric_codes:
+        allowed[c] = 1
+    ex = _c(np.asarray(list(extra_pairs) or [0], np.int32), np.int32)
+    num = np.zeros(200, np.int64)
+    ext = np.zeros(max(2, 2 * len(extra_pairs)), np.int64)
+    c = [_c(db["c_custkey"], np.int32), _c(db["c_phone_code"], np.uint8),
+         _c(db["c_phone_pair"], np.int32), _c(db["c_acctbal"], np.int32)]
+    ok = _c(db["o_custkey"], np.int32)
+    L.oracle_q22(len(c[0]), *[_p(x) for x in c], _p(allowed), len(extra_pairs), _p(ex), len(ok),
+                 _p(ok), _p(num), _p(ext))
+    return num.reshape(100, 2), ext[:2 * len(extra_pairs)].reshape(-1, 2)

@@ -287,4 +287,14 @@ int64_t oracle_q19(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_b
                    const uint16_t* l_shipinstruct, const oracle_q19_group* groups, int32_t mode1,
                    int32_t mode2, int32_t instruct);
 
+void oracle_q21(int64_t o_rows, const int32_t* o_orderkey, const uint16_t* o_status,
+                int32_t final_code, const uint32_t* r_start, const uint32_t* r_end,
+                const int32_t* l_orderkey, int32_t l_sorted, const int32_t* l_suppkey,
+                const uint32_t* l_commit_receipt, const uint8_t* is_target, int64_t n_suppkeys,
+                int64_t* wait);
+void oracle_q22(int64_t c_rows, const int32_t* c_custkey, const uint8_t* c_phone_code,
+                const int32_t* c_phone_pair, const int32_t* c_acctbal,
+                const uint8_t* numeric_allowed, int32_t n_extra, const int32_t* extra_pairs,
+                int64_t o_rows, const int32_t* o_custkey, int64_t* numeric, int64_t* extra);
+
 #endif

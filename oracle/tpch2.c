@@ -274,3 +274,114 @@ int64_t oracle_q19(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_b
     free(mask);
     return revenue;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Q21 (bespoke_tpch/query_q21.cpp:140-380): every order with status code    */
+/* final_code and a range of more than one row; over its lineitems (with    */
+/* the key check when unsorted) whose suppkey is in [0, n_suppkeys): the    */
+/* order has several suppliers when a suppkey differs from the first, and   */
+/* a row is late when receiptdate > commitdate (packed halves, unsigned);   */
+/* when late rows come from two suppliers the order is dropped (the walk    */
+/* stops there).  An order with several suppliers and exactly one late      */
+/* supplier adds its late-row count to that supplier when the supplier is   */
+/* in the target nation (is_target[suppkey]).  wait[n_suppkeys] out.        */
+/* ------------------------------------------------------------------------ */
+void oracle_q21(int64_t o_rows, const int32_t* o_orderkey, const uint16_t* o_status,
+                int32_t final_code, const uint32_t* r_start, const uint32_t* r_end,
+                const int32_t* l_orderkey, int32_t l_sorted, const int32_t* l_suppkey,
+                const uint32_t* l_commit_receipt, const uint8_t* is_target, int64_t n_suppkeys,
+                int64_t* wait) {
+    for (int64_t k = 0; k < n_suppkeys; ++k) wait[k] = 0;
+    for (int64_t o = 0; o < o_rows; ++o) {
+        if ((int32_t)o_status[o] != final_code) continue;
+        if (r_end[o] == 0) continue;
+        if (r_end[o] - r_start[o] <= 1) continue;
+        int32_t first = -1, late_supp = -1, late_n = 0;
+        int multi = 0, multi_late = 0;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const int32_t s = l_suppkey[r];
+            if ((uint32_t)s >= (uint64_t)n_suppkeys) continue;
+            if (first < 0) first = s;
+            else if (s != first) multi = 1;
+            const uint32_t p = l_commit_receipt[r];
+            const int late = oracle_mutation() == 20 ? (p >> 16) >= (p & 0xffffu)
+                                                     : (p >> 16) > (p & 0xffffu);
+            if (late) {
+                if (late_supp == -1 || late_supp == s) {
+                    late_supp = s;
+                    late_n += 1;
+                } else {
+                    multi_late = 1;
+                    break;
+                }
+            }
+        }
+        if (multi_late || !multi || late_supp < 0) continue;
+        if (!is_target[late_supp]) continue;
+        wait[late_supp] += late_n;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q22 (bespoke_tpch/query_q22.cpp:190-520): a customer is a candidate when */
+/* its phone prefix is in the filter — a two-digit prefix code (phone_code, */
+/* 255 when the phone does not start with two digits, builder_impl.cpp:     */
+/* 152-178) with numeric_allowed[code], or else its first two characters    */
+/* (phone_pair = c0 << 8 | c1, -1 for phones shorter than two) equal one of */
+/* the extra prefixes — and acctbal > 0.  avg = floor(sum / count) of the   */
+/* candidates' acctbal; candidates with acctbal > avg and no orders count   */
+/* per prefix: numeric[2 * code] (customers), numeric[2 * code + 1]          */
+/* (acctbal sum), extra[2 * i], extra[2 * i + 1].  A customer has orders    */
+/* when an order carries its key (the reference's orders_by_customer         */
+/* offsets, builder_impl.cpp:625-668; a key no order carries has none).      */
+/* ------------------------------------------------------------------------ */
+void oracle_q22(int64_t c_rows, const int32_t* c_custkey, const uint8_t* c_phone_code,
+                const int32_t* c_phone_pair, const int32_t* c_acctbal,
+                const uint8_t* numeric_allowed, int32_t n_extra, const int32_t* extra_pairs,
+                int64_t o_rows, const int32_t* o_custkey, int64_t* numeric, int64_t* extra) {
+    for (int i = 0; i < 200; ++i) numeric[i] = 0;
+    for (int i = 0; i < 2 * n_extra; ++i) extra[i] = 0;
+    int32_t omax = -1;
+    for (int64_t i = 0; i < o_rows; ++i)
+        if (o_custkey[i] > omax) omax = o_custkey[i];
+    uint8_t* has = (uint8_t*)calloc((size_t)(omax + 2), 1);
+    for (int64_t i = 0; i < o_rows; ++i)
+        if (o_custkey[i] >= 0) has[o_custkey[i]] = 1;
+    int64_t sum = 0, cnt = 0;
+    int32_t* slot = (int32_t*)malloc(sizeof(int32_t) * (size_t)(c_rows > 0 ? c_rows : 1));
+    for (int64_t i = 0; i < c_rows; ++i) {
+        slot[i] = -1;
+        const uint8_t code = c_phone_code[i];
+        if (code != 255) {
+            if (code < 100 && numeric_allowed[code]) slot[i] = code;
+        } else if (c_phone_pair[i] >= 0) {
+            for (int32_t e = 0; e < n_extra; ++e)
+                if (extra_pairs[e] == c_phone_pair[i]) {
+                    slot[i] = 100 + e;
+                    break;
+                }
+        }
+        if (slot[i] < 0) continue;
+        if (c_acctbal[i] <= 0) {
+            slot[i] = -1;
+            continue;
+        }
+        sum += c_acctbal[i];
+        cnt += 1;
+    }
+    if (cnt > 0) {
+        const int64_t avg = sum / cnt;
+        for (int64_t i = 0; i < c_rows; ++i) {
+            if (slot[i] < 0) continue;
+            if (c_acctbal[i] <= avg) continue;
+            const int32_t k = c_custkey[i];
+            if (oracle_mutation() != 21 && k >= 0 && k <= omax && has[k]) continue;
+            int64_t* agg = slot[i] < 100 ? numeric + 2 * slot[i] : extra + 2 * (slot[i] - 100);
+            agg[0] += 1;
+            agg[1] += c_acctbal[i];
+        }
+    }
+    free(has);
+    free(slot);
+}

@@ -14,7 +14,7 @@ from tests.test_tpch2_oracle import q19_groups
 
 pytestmark = pytest.mark.gpu
 
-U8_CODES = ("l_shipmode", "l_shipinstruct", "o_orderpriority")
+U8_CODES = ("l_shipmode", "l_shipinstruct", "o_orderpriority", "o_orderstatus")
 
 
 def upload(db: dict) -> dict:
@@ -165,6 +165,40 @@ def test_q19_duplicate_part_keys_and_keys_outside_the_parts(engine, db):
     assert engine.q19(t3, g, m1, m2, ins) == oracle.q19(h2, g, m1, m2, ins)
 
 
+@pytest.mark.parametrize("nation", [3, 12, 24, 99])
+def test_q21(engine, db, nation):
+    r, h, t = db
+    n = int(h["s_suppkey"].max()) + 1
+    f = h["status_dict"].index("F") if "F" in h["status_dict"] else -1
+    assert engine.q21(t, f, nation, n) == oracle.q21(h, f, nation, n).tolist()
+
+
+def test_q21_duplicate_supplier_rows_and_absent_status(engine, db):
+    r, h, t = db
+    h2 = dict(h)
+    # the last row of a duplicated key decides its nation
+    h2["s_suppkey"] = np.concatenate([h["s_suppkey"], h["s_suppkey"][:5]])
+    h2["s_nationkey"] = np.concatenate([h["s_nationkey"], (h["s_nationkey"][:5] + 1) % 25])
+    t2 = dict(t, s_suppkey=torch.from_numpy(h2["s_suppkey"]).cuda(),
+              s_nationkey=torch.from_numpy(h2["s_nationkey"].astype(np.int32)).cuda())
+    n = int(h["s_suppkey"].max()) + 1
+    f = h["status_dict"].index("F")
+    for nation in range(25):
+        assert engine.q21(t2, f, nation, n) == oracle.q21(h2, f, nation, n).tolist()
+    assert engine.q21(t, -1, 3, n) == [0] * n          # no "F" status
+    assert engine.q21(t, f, 3, n // 2) == oracle.q21(h, f, 3, n // 2).tolist()
+
+
+@pytest.mark.parametrize("codes", [["13", "31", "23", "29", "30", "18", "17"], ["AB", "20", "7"],
+                                   ["AB"], ["99"]])
+def test_q22(engine, db, codes):
+    r, h, t = db
+    numeric = [int(c) for c in codes if len(c) >= 2 and c[:2].isdigit()]
+    extra = [(ord(c[0]) << 8) | ord(c[1]) for c in codes if len(c) >= 2 and not c[:2].isdigit()]
+    num, ext = oracle.q22(h, numeric, extra)
+    assert engine.q22(t, numeric, extra) == (num.tolist(), ext.tolist())
+
+
 @pytest.fixture(scope="module")
 def sf10():
     """15,000,000 orders, about 60,000,000 lineitems, 2,000,000 parts,
@@ -204,3 +238,13 @@ def test_sf10_all_five(engine, sf10):
     want19 = oracle.q19(h, g, m1, m2, ins)
     assert want19 > 0
     assert engine.q19(t, g, m1, m2, ins) == want19
+    print("q21", flush=True)
+    n = int(h["s_suppkey"].max()) + 1
+    w21 = oracle.q21(h, h["status_dict"].index("F"), 7, n)
+    assert w21.sum() > 0
+    assert engine.q21(t, h["status_dict"].index("F"), 7, n) == w21.tolist()
+    print("q22", flush=True)
+    num, ext = oracle.q22(h, [13, 31, 23, 29, 30, 18, 17], [(ord("A") << 8) | ord("B")])
+    assert num[:, 0].sum() > 0
+    assert engine.q22(t, [13, 31, 23, 29, 30, 18, 17], [(ord("A") << 8) | ord("B")]) == \
+        (num.tolist(), ext.tolist())

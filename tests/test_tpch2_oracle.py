@@ -232,3 +232,71 @@ def test_planted_mistakes_are_caught(mutation):
     finally:
         oracle.lib().oracle_set_mutation(0)
     assert bad[mutation - 15] != good[mutation - 15]
+
+
+def q21_pandas(r, nation):
+    li = lineitem_df(r)
+    li["late"] = li.l_receiptdate > li.l_commitdate
+    o = orders_df(r).assign(st=r["o_orderstatus"])
+    li = li.merge(o[o.st == 0][["o_orderkey"]], left_on="l_orderkey", right_on="o_orderkey")
+    g = li.groupby("l_orderkey").agg(n_supp=("l_suppkey", "nunique"))
+    lg = li[li.late].groupby("l_orderkey").agg(n_late=("l_suppkey", "nunique"))
+    ok = g.join(lg, how="inner")
+    ok = ok[(ok.n_supp >= 2) & (ok.n_late == 1)].index
+    j = li[li.late & li.l_orderkey.isin(ok)]
+    s = pd.DataFrame({"l_suppkey": r["s_suppkey"], "s_nation": r["s_nationkey"]})
+    j = j.merge(s, on="l_suppkey")
+    j = j[j.s_nation == nation]
+    return j.groupby("l_suppkey").size()
+
+
+@pytest.mark.parametrize("nation", [3, 12, 24])
+def test_q21_matches_pandas(data, nation):
+    r, db = data
+    n = int(db["s_suppkey"].max()) + 1
+    got = oracle.q21(db, db["status_dict"].index("F"), nation, n)
+    want = q21_pandas(r, nation)
+    assert {k: int(v) for k, v in enumerate(got) if v} == {int(k): int(v) for k, v in want.items()}
+    assert len(want) > 0
+
+
+def q22_pandas(r, codes):
+    pr = [("" if p < 0 else chr(p >> 8) + chr(p & 255)) for p in r["c_phone_pair"].tolist()]
+    c = pd.DataFrame({"k": r["c_custkey"], "pre": pr, "bal": r["c_acctbal"]})
+    c = c[c.pre.isin(codes)]
+    avg = c[c.bal > 0].bal.mean()
+    c = c[(c.bal > avg) & ~c.k.isin(set(r["o_custkey"].tolist()))]
+    return c.groupby("pre").agg(n=("k", "size"), s=("bal", "sum"))
+
+
+@pytest.mark.parametrize("codes", [["13", "31", "23", "29", "30", "18", "17"], ["AB", "20", "7"],
+                                   ["AB"]])
+def test_q22_matches_pandas(data, codes):
+    r, db = data
+    numeric = [int(c) for c in codes if len(c) >= 2 and c[:2].isdigit()]
+    extra = [(ord(c[0]) << 8) | ord(c[1]) for c in codes if len(c) >= 2 and not c[:2].isdigit()]
+    num, ext = oracle.q22(db, numeric, extra)
+    got = {f"{k:02d}": (int(num[k, 0]), int(num[k, 1])) for k in range(100) if num[k, 0]}
+    for e, (n, s) in zip(extra, ext.tolist()):
+        if n:
+            got[chr(e >> 8) + chr(e & 255)] = (n, s)
+    want = q22_pandas(r, [c[:2] for c in codes if len(c) >= 2])
+    assert got == {k: (int(v.n), int(v.s)) for k, v in want.iterrows()}
+    assert len(got) > 0
+
+
+@pytest.mark.parametrize("mutation", [20, 21])
+def test_planted_mistakes_q21_q22_are_caught(mutation):
+    r = T.raw(13, 40_000, parts=400, custs=3000, supps=40, strings=False)
+    db = T.build(r)
+
+    def run_all():
+        return (oracle.q21(db, db["status_dict"].index("F"), 3, 41).tolist(),
+                [x.tolist() for x in oracle.q22(db, [13, 31, 23, 29, 30, 18, 17], [])])
+    good = run_all()
+    oracle.lib().oracle_set_mutation(mutation)
+    try:
+        bad = run_all()
+    finally:
+        oracle.lib().oracle_set_mutation(0)
+    assert bad[mutation - 20] != good[mutation - 20]

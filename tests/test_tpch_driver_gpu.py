@@ -376,3 +376,58 @@ def test_driver_order_range_queries_q12_q19(tmp_path, orders):
         assert want[0].count("\n") == 6 and want[5].count("\n") > 50
         assert want[2].count("\n") > 2 and want[8].count("\n") == 3
         assert want[11] != "revenue\n0.00\n"
+
+
+def csv_q21(db, r, nation):
+    import oracle
+    out = "s_name,numwait\n"
+    if nation == "<<NULL>>" or nation not in r["n_name"] or "F" not in db["status_dict"]:
+        return out
+    n = int(db["s_suppkey"].max()) + 1
+    w = oracle.q21(db, db["status_dict"].index("F"), r["n_name"].index(nation), n)
+    rows = sorted((-int(c), r["s_name"][k - 1]) for k, c in enumerate(w.tolist()) if c)
+    return out + "".join(f"{esc(name)},{-c}\n" for c, name in rows)
+
+
+def csv_q22(db, codes):
+    import oracle
+    numeric, extra = [], []
+    for c in codes:
+        if c == "<<NULL>>" or len(c) < 2:
+            continue
+        if c[:2].isdigit():
+            if int(c[:2]) not in numeric:
+                numeric.append(int(c[:2]))
+        elif c[:2] not in extra:
+            extra.append(c[:2])
+    out = "cntrycode,numcust,totacctbal\n"
+    if not numeric and not extra:
+        return out
+    num, ext = oracle.q22(db, numeric, [(ord(e[0]) << 8) | ord(e[1]) for e in extra])
+    rows = [(f"{c:02d}", int(num[c, 0]), int(num[c, 1])) for c in numeric if num[c, 0]]
+    rows += [(e, int(ext[i, 0]), int(ext[i, 1])) for i, e in enumerate(extra) if ext[i, 0]]
+    return out + "".join(f"{esc(c)},{n},{t / 100.0:.2f}\n" for c, n, t in sorted(rows))
+
+
+@pytest.mark.parametrize("orders", [3, 40_001])
+def test_driver_q21_q22(tmp_path, orders):
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    from tests import tpch_data as T
+    r = T.raw(23 + orders, orders, parts=800, custs=3000, supps=60)
+    write(T.parquet(r), tmp_path / "data")
+    db = T.build(r)
+    q22a = ["13", "31", "23", "29", "30", "18", "17"]
+    q22b = ["AB", "20", "7", "<<NULL>>", "20", "AB", "34"]
+    q = lambda v: " ".join(f'"{x}"' for x in v)
+    requests = ['21 "SAUDI ARABIA"', '21 GERMANY', '21 "ATLANTIS"', '21 "<<NULL>>"',
+                f"22 {q(q22a)}", f"22 {q(q22b)}", f"22 {q(['<<NULL>>'] * 7)}"]
+    res = run_driver(tmp_path / "data", tmp_path / "run", requests)
+    assert res.returncode == 0, res.stderr
+    want = [csv_q21(db, r, "SAUDI ARABIA"), csv_q21(db, r, "GERMANY"), csv_q21(db, r, "ATLANTIS"),
+            csv_q21(db, r, "<<NULL>>"), csv_q22(db, q22a), csv_q22(db, q22b),
+            csv_q22(db, ["<<NULL>>"] * 7)]
+    for i, w in enumerate(want):
+        assert (tmp_path / "run" / f"result{i + 1}.csv").read_text() == w, requests[i]
+    if orders > 3:
+        assert want[0].count("\n") > 1 and want[4].count("\n") > 3 and want[5].count("\n") > 2

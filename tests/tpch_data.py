@@ -64,9 +64,20 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["c_nationkey"] = rng.integers(0, 25, custs).astype(np.int32)
     r["c_acctbal"] = rng.integers(-99_999, 999_999, custs).astype(np.int64)
     r["c_mktsegment"] = rng.integers(0, 5, custs)
+    # phone prefixes: TPC-H's country code 10 + nation, except every 17th
+    # customer ("AB-...", no digits) and every 31st (a one-character phone)
+    code = (10 + r["c_nationkey"]).astype(np.uint8)
+    pair = ((ord("0") + code // 10).astype(np.int32) << 8) | (ord("0") + code % 10)
+    i = np.arange(custs)
+    odd, short = i % 17 == 5, i % 31 == 7
+    code[odd | short] = 255
+    pair[odd] = (ord("A") << 8) | ord("B")
+    pair[short] = -1
+    r["c_phone_code"], r["c_phone_pair"] = code, pair.astype(np.int32)
     # supplier
     r["s_suppkey"] = np.arange(1, supps + 1, dtype=np.int32)
     r["s_nationkey"] = rng.integers(0, 25, supps).astype(np.int32)
+    r["s_name"] = [f"Supplier#{k:09d}" for k in r["s_suppkey"]]
     # part
     r["p_partkey"] = np.arange(1, parts + 1, dtype=np.int32)
     r["p_brand"] = rng.integers(0, len(BRANDS), parts)
@@ -75,7 +86,10 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["p_type"] = rng.integers(0, len(TYPES), parts)
     # orders (created in key order)
     r["o_orderkey"] = okeys(orders)
-    r["o_custkey"] = rng.integers(1, custs + 1, orders).astype(np.int32)
+    # as in TPC-H, no order has a customer key divisible by 3
+    ck = rng.integers(1, custs + 1, orders)
+    ck = np.where(ck % 3 == 0, np.maximum(ck - 1, 1), ck)
+    r["o_custkey"] = ck.astype(np.int32)
     r["o_orderdate"] = rng.integers(0, 2406, orders).astype(np.int32)
     r["o_orderpriority"] = rng.integers(0, 5, orders)
     r["o_totalprice"] = rng.integers(100_000, 50_000_000, orders).astype(np.int64)
@@ -101,6 +115,10 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     rflag = np.where(receipt <= CURRENT, np.where(rng.random(n) < 0.5, 0, 1), 2)  # R, A, N
     r["l_returnflag"] = rflag                      # values index ["R", "A", "N"]
     r["l_linestatus"] = np.where(ship > CURRENT, 0, 1)   # ["O", "F"]
+    # orderstatus from the lines' status: F all F, O all O, else P
+    ls = r["l_linestatus"]
+    n_f = np.bincount(oi, weights=(ls == 1), minlength=orders)
+    r["o_orderstatus"] = np.where(n_f == per, 0, np.where(n_f == 0, 1, 2))  # F, O, P
     r["l_shipinstruct"] = rng.integers(0, 4, n)
     r["l_shipmode"] = rng.integers(0, 7, n)
     r["rf_values"], r["ls_values"] = ["R", "A", "N"], ["O", "F"]
@@ -109,8 +127,9 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
         if custs > 1:
             r["c_name"][0], r["c_name"][1] = 'He said "hi"', "plain,comma"
         r["c_address"] = [f"addr {k * 7919 % 100003}" for k in r["c_custkey"]]
-        r["c_phone"] = [f"{10 + nk}-{k % 900 + 100}-{k % 9000 + 1000}"
-                        for k, nk in zip(r["c_custkey"], r["c_nationkey"])]
+        r["c_phone"] = [("7" if pr < 0 else chr(pr >> 8) + chr(pr & 255)) +
+                        ("" if pr < 0 else f"-{k % 900 + 100}-{k % 9000 + 1000}")
+                        for k, pr in zip(r["c_custkey"], r["c_phone_pair"])]
         r["c_comment"] = [f'note "{k}" about, things' for k in r["c_custkey"]]
     return r
 
@@ -164,6 +183,8 @@ def build(r: dict) -> dict:
     db["o_orderdate"] = r["o_orderdate"][perm].astype(np.int16)
     pr, db["priority_dict"] = first_appearance(r["o_orderpriority"], PRIORITIES)
     db["o_orderpriority"] = pr[perm]
+    st, db["status_dict"] = first_appearance(r["o_orderstatus"], ["F", "O", "P"])
+    db["o_orderstatus"] = st[perm]
     cmax = int(r["c_custkey"].max()) if len(r["c_custkey"]) else -1
     c_by_key = np.full(cmax + 1, -1, np.int16)
     c_by_key[r["c_custkey"]] = r["c_nationkey"]
@@ -203,6 +224,12 @@ def build(r: dict) -> dict:
     # customer, part, nation
     db["c_custkey"] = r["c_custkey"].astype(np.int32)
     db["c_nationkey"] = r["c_nationkey"].astype(np.int32)
+    db["c_acctbal"] = r["c_acctbal"].astype(np.int32)
+    db["c_phone_code"] = r["c_phone_code"].astype(np.uint8)
+    db["c_phone_pair"] = r["c_phone_pair"].astype(np.int32)
+    db["s_suppkey"] = r["s_suppkey"].astype(np.int32)
+    db["s_nationkey"] = r["s_nationkey"].astype(np.int32)
+    db["o_custkey_raw"] = r["o_custkey"].astype(np.int32)
     db["p_partkey"] = r["p_partkey"].astype(np.int32)
     db["p_brand"], db["brand_dict"] = first_appearance(r["p_brand"], BRANDS)
     db["p_container"], db["container_dict"] = first_appearance(r["p_container"], CONTAINERS)
@@ -239,7 +266,8 @@ def parquet(r: dict) -> dict:
         "o_orderkey": i32(r["o_orderkey"]), "o_custkey": i32(r["o_custkey"]),
         "o_orderdate": day(r["o_orderdate"]), "o_totalprice": dec(r["o_totalprice"]),
         "o_shippriority": i32(r["o_shippriority"]),
-        "o_orderpriority": strs(r["o_orderpriority"], PRIORITIES)})
+        "o_orderpriority": strs(r["o_orderpriority"], PRIORITIES),
+        "o_orderstatus": strs(r["o_orderstatus"], ["F", "O", "P"])})
     cust = pa.table({
         "c_custkey": i32(r["c_custkey"]), "c_name": pa.array(r["c_name"]),
         "c_nationkey": i32(r["c_nationkey"]), "c_acctbal": dec(r["c_acctbal"]),
@@ -250,7 +278,8 @@ def parquet(r: dict) -> dict:
         "p_partkey": i32(r["p_partkey"]), "p_brand": strs(r["p_brand"], BRANDS),
         "p_container": strs(r["p_container"], CONTAINERS), "p_size": i32(r["p_size"]),
         "p_type": strs(r["p_type"], TYPES)})
-    supp = pa.table({"s_suppkey": i32(r["s_suppkey"]), "s_nationkey": i32(r["s_nationkey"])})
+    supp = pa.table({"s_suppkey": i32(r["s_suppkey"]), "s_nationkey": i32(r["s_nationkey"]),
+                     "s_name": pa.array(r["s_name"])})
     nation = pa.table({"n_nationkey": i32(r["n_nationkey"]), "n_name": pa.array(r["n_name"]),
                        "n_regionkey": i32(r["n_regionkey"])})
     region = pa.table({"r_regionkey": i32(r["r_regionkey"]), "r_name": pa.array(r["r_name"])})
