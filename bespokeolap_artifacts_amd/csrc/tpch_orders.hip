@@ -994,3 +994,500 @@ extern "C" int bolap_q22(bolap_ctx* ctx, const bolap_q22_args* a, bolap_q22_resu
     }
     return BOLAP_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Q7, Q8, Q9: the multi-way joins.
+// ---------------------------------------------------------------------------
+namespace bolap {
+namespace {
+
+// Q7 supplier tags: the last supplier row of a key with nation n1 (tag 0)
+// or, when n1 != n2, n2 (tag 1): owner = (row + 1) << 1 | tag, 0 untagged.
+__global__ __launch_bounds__(kT) void q7_tag_k(const int32_t* __restrict__ sk,
+                                               const int32_t* __restrict__ sn, int64_t rows,
+                                               int64_t n, int32_t n1, int32_t n2,
+                                               int32_t* __restrict__ owner) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = sk[i];
+        if (k < 0 || k >= n) continue;
+        const int32_t nat = sn[i];
+        int tag = -1;
+        if (nat == n1) tag = 0;
+        else if (n1 != n2 && nat == n2) tag = 1;
+        if (tag >= 0) atomicMax(owner + k, int32_t((uint32_t(i + 1) << 1) | uint32_t(tag)));
+    }
+}
+
+struct Q7K {
+    int64_t o_rows;
+    const int16_t* cnation;
+    const int32_t* okey;
+    LiSide li;
+    const int16_t* ship;
+    const int32_t* sk;
+    const int32_t* dprice;
+    const int32_t* owner;
+    int64_t n;
+    int32_t n1, n2, ship_lo;
+    uint32_t ship_span;
+    int32_t split;
+};
+
+__global__ __launch_bounds__(kT) void q7_k(Q7K a, unsigned long long* __restrict__ out) {
+    int64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bool same = a.n1 == a.n2;
+    for (int64_t o = int64_t(blockIdx.x) * kT + threadIdx.x; o < a.o_rows;
+         o += int64_t(gridDim.x) * kT) {
+        const int32_t cn = a.cnation[o];
+        int ci;
+        if (cn == a.n1) ci = 0;
+        else if (!same && cn == a.n2) ci = 1;
+        else continue;
+        const uint32_t s = a.li.start[o], e = a.li.end[o];
+        if (e <= s) continue;
+        const int32_t key = a.li.okey ? a.okey[o] : 0;
+        for (uint32_t r = s; r < e; ++r) {
+            if (a.li.okey && a.li.okey[r] != key) continue;
+            const int32_t ship = a.ship[r];
+            if (uint32_t(ship - a.ship_lo) > a.ship_span) continue;
+            const int32_t k = a.sk[r];
+            if (k < 0 || k >= a.n) continue;
+            const int32_t w = a.owner[k];
+            if (w == 0) continue;
+            const int si = w & 1;
+            if (same ? (si != 0 || ci != 0) : si == ci) continue;
+            const int idx = si * 4 + ci * 2 + (ship >= a.split ? 1 : 0);
+            const int64_t v = a.dprice[r];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += j == idx ? v : 0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint64_t t = wave_sum_u64(uint64_t(acc[j]));
+        if ((threadIdx.x & (kWave - 1)) == 0 && t) atomicAdd(out + j, (unsigned long long)t);
+    }
+}
+
+// Marks keys of rows that pass (a byte per key; marks are never cleared).
+__global__ __launch_bounds__(kT) void mark_keys_k(const int32_t* __restrict__ keys,
+                                                  const void* __restrict__ col, int32_t dtype,
+                                                  int64_t rows, int64_t n, int32_t lo,
+                                                  int32_t hi, const uint8_t* __restrict__ lut,
+                                                  int32_t lut_n, uint8_t* __restrict__ mark) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = keys[i];
+        if (k < 0 || k >= n) continue;
+        int32_t v;
+        switch (dtype) {
+            case BOLAP_U8: v = static_cast<const uint8_t*>(col)[i]; break;
+            case BOLAP_I16: v = static_cast<const int16_t*>(col)[i]; break;
+            default: v = static_cast<const int32_t*>(col)[i]; break;
+        }
+        // lut: value passes when lut[v] (v in [0, lut_n)); else lo <= v <= hi
+        const bool pass = lut ? (v >= 0 && v < lut_n && lut[v]) : (v >= lo && v <= hi);
+        if (pass) mark[k] = 1;
+    }
+}
+
+struct Q8K {
+    const int16_t* odate;
+    int64_t o_rows;
+    const int32_t* okey;
+    const int32_t* ocust;
+    LiSide li;
+    const int32_t* pk;
+    const int32_t* sk;
+    const int32_t* price;
+    const uint8_t* disc;
+    const uint8_t *cmark, *smark, *pmark;
+    int64_t nc, ns, np;
+    int32_t date_lo, date_hi, split;
+};
+
+__global__ __launch_bounds__(kT) void q8_k(Q8K a, unsigned long long* __restrict__ out) {
+    int64_t tot[2] = {0, 0}, nat[2] = {0, 0};
+    const Window w = wg_window(a.odate, a.o_rows, a.date_lo, a.date_hi + 1);
+    for (int64_t o = w.s + threadIdx.x; o < w.e; o += kT) {
+        const int32_t ck = a.ocust[o];
+        if (ck < 0 || ck >= a.nc || !a.cmark[ck]) continue;
+        const uint32_t s = a.li.start[o], e = a.li.end[o];
+        if (e <= s) continue;
+        const int y = a.odate[o] >= a.split ? 1 : 0;
+        const int32_t key = a.li.okey ? a.okey[o] : 0;
+        for (uint32_t r = s; r < e; ++r) {
+            if (a.li.okey && a.li.okey[r] != key) continue;
+            const int32_t p = a.pk[r];
+            if (p < 0 || p >= a.np || !a.pmark[p]) continue;
+            const int64_t vol = int64_t(a.price[r]) * int64_t(100 - int32_t(a.disc[r]));
+            const int32_t k = a.sk[r];
+            const bool hit = k >= 0 && k < a.ns && a.smark[k];
+            if (y) {
+                tot[1] += vol;
+                nat[1] += hit ? vol : 0;
+            } else {
+                tot[0] += vol;
+                nat[0] += hit ? vol : 0;
+            }
+        }
+    }
+    const int64_t v[4] = {tot[0], tot[1], nat[0], nat[1]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t t = wave_sum_u64(uint64_t(v[j]));
+        if ((threadIdx.x & (kWave - 1)) == 0 && t) atomicAdd(out + j, (unsigned long long)t);
+    }
+}
+
+// Q9: partsupp rows grouped by partkey (CSR in three steps: counts, scan,
+// fill; a key's rows land in any order and the lookup takes the smallest
+// row of a match, which is the reference's first match in row order).
+__global__ __launch_bounds__(kT) void ps_count_k(const int32_t* __restrict__ pk, int64_t rows,
+                                                 int64_t n, uint32_t* __restrict__ cnt) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = pk[i];
+        if (k >= 0 && k < n) atomicAdd(cnt + k, 1u);
+    }
+}
+
+// Exclusive scan of n u32 counts in place (tiles of kTile per workgroup:
+// tile sums, scan_tiles_k over them, then each tile's local scan).
+__global__ __launch_bounds__(kT) void tile_sum_k(const uint32_t* __restrict__ v, int64_t n,
+                                                 uint32_t* __restrict__ tiles) {
+    const int64_t t0 = int64_t(blockIdx.x) * kTile;
+    uint32_t s = 0;
+    for (int j = 0; j < 16; ++j) {
+        const int64_t i = t0 + int64_t(j) * kT + threadIdx.x;
+        if (i < n) s += v[i];
+    }
+    __shared__ uint32_t sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    atomicAdd(&sh, s);
+    __syncthreads();
+    if (threadIdx.x == 0) tiles[blockIdx.x] = sh;
+}
+
+__global__ __launch_bounds__(kT) void tile_scan_k(uint32_t* __restrict__ v, int64_t n,
+                                                  const uint32_t* __restrict__ tiles) {
+    __shared__ uint32_t s_w[kT / kWave];
+    const int64_t t0 = int64_t(blockIdx.x) * kTile;
+    uint32_t base = tiles[blockIdx.x];
+    for (int j = 0; j < 16; ++j) {
+        const int64_t i = t0 + int64_t(j) * kT + threadIdx.x;
+        const uint32_t x = i < n ? v[i] : 0;
+        uint32_t inc = x;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d, kWave);
+            if ((threadIdx.x & (kWave - 1)) >= d) inc += y;
+        }
+        if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_w[threadIdx.x / kWave] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < kT / kWave; ++w) {
+            if (w < int(threadIdx.x / kWave)) before += s_w[w];
+            total += s_w[w];
+        }
+        if (i < n) v[i] = base + before + inc - x;
+        base += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kT) void ps_fill_k(const int32_t* __restrict__ pk, int64_t rows,
+                                                int64_t n, uint32_t* __restrict__ cursor,
+                                                uint32_t* __restrict__ idx) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = pk[i];
+        if (k >= 0 && k < n) idx[atomicAdd(cursor + k, 1u)] = uint32_t(i);
+    }
+}
+
+// Year of a day offset from 1992-01-01 (Hinnant's civil_from_days).
+BOLAP_DEV int32_t year_of_offset(int32_t off) {
+    const int32_t z = off + 8035 + 719468;  // 1992-01-01 is day 8035 since 1970-01-01
+    const int32_t era = (z >= 0 ? z : z - 146096) / 146097;
+    const uint32_t doe = uint32_t(z - era * 146097);
+    const uint32_t yoe = (doe - doe / 1460u + doe / 36524u - doe / 146096u) / 365u;
+    const uint32_t doy = doe - (365u * yoe + yoe / 4u - yoe / 100u);
+    const uint32_t mp = (5u * doy + 2u) / 153u;
+    const uint32_t m = mp < 10u ? mp + 3u : mp - 9u;
+    return int32_t(yoe) + era * 400 + (m <= 2u ? 1 : 0);
+}
+
+constexpr int kQ9Groups = 2048;
+
+struct Q9K {
+    int64_t rows;
+    const int32_t* pk;
+    const int32_t* sk;
+    const int32_t* ok;
+    const int16_t* qty;
+    const int32_t* dprice;
+    const int16_t* snation;
+    const uint8_t* pmark;
+    int64_t np;
+    const uint32_t* ps_off;  // np + 1 offsets
+    const uint32_t* ps_idx;
+    const int32_t* ps_sk;
+    const int32_t* ps_cost;
+    const int32_t* omap;
+    int64_t map_size;
+    const int16_t* odate;
+    int32_t n_nation;
+    int32_t min_year, span;
+};
+
+BOLAP_DEV void q9_row(const Q9K& a, int64_t i, int32_t p, unsigned long long* s_tot,
+                      uint8_t* s_used) {
+    if (p < 0 || p >= a.np || !a.pmark[p]) return;
+    const int32_t k = a.sk[i];
+    uint32_t best = 0xffffffffu;
+    for (uint32_t j = a.ps_off[p]; j < a.ps_off[p + 1]; ++j) {
+        const uint32_t row = a.ps_idx[j];
+        if (a.ps_sk[row] == k && row < best) best = row;
+    }
+    if (best == 0xffffffffu) return;
+    const int32_t nk = a.snation[i];
+    if (nk < 0 || nk >= a.n_nation) return;
+    const int32_t okey = a.ok[i];
+    if (okey < 0 || okey >= a.map_size) return;
+    const int32_t orow = a.omap[okey];
+    if (orow < 0) return;
+    const int32_t yi = year_of_offset(a.odate[orow]) - a.min_year;
+    const int64_t profit =
+        int64_t(a.dprice[i]) - int64_t(a.ps_cost[best]) * int64_t(a.qty[i]) / 100;
+    const int g = nk * a.span + yi;
+    atomicAdd(s_tot + g, (unsigned long long)profit);
+    s_used[g] = 1;
+}
+
+__global__ __launch_bounds__(kT) void q9_k(Q9K a, unsigned long long* __restrict__ tot,
+                                           uint8_t* __restrict__ used) {
+    __shared__ unsigned long long s_tot[kQ9Groups];
+    __shared__ uint8_t s_used[kQ9Groups];
+    const int groups = a.n_nation * a.span;
+    for (int g = threadIdx.x; g < groups; g += kT) {
+        s_tot[g] = 0;
+        s_used[g] = 0;
+    }
+    __syncthreads();
+    const int64_t units = a.rows / 4;
+    const int64_t stride = int64_t(gridDim.x) * kT;
+    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += stride) {
+        const VecT<int32_t, 4> p4 = load_unit<int32_t, 4>(a.pk, u);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q9_row(a, u * 4 + j, p4.v[j], s_tot, s_used);
+    }
+    for (int64_t i = units * 4 + int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows; i += stride)
+        q9_row(a, i, a.pk[i], s_tot, s_used);
+    __syncthreads();
+    for (int g = threadIdx.x; g < groups; g += kT) {
+        if (s_tot[g]) atomicAdd(tot + g, s_tot[g]);
+        if (s_used[g]) used[g] = 1;
+    }
+}
+
+}  // namespace
+}  // namespace bolap
+
+extern "C" int bolap_q7(bolap_ctx* ctx, const bolap_q7_args* a, int64_t* rev) {
+    BOLAP_ARG_CHECK(ctx && a && rev, "bolap_q7: null argument");
+    const bolap_order_side& o = a->orders;
+    BOLAP_ARG_CHECK(o.order_rows >= 0 && o.order_rows < (int64_t(1) << 31),
+                    "bolap_q7: order rows");
+    BOLAP_ARG_CHECK(o.order_rows == 0 || (a->o_cust_nation && o.range_start && o.range_end &&
+                                          a->l_shipdate && a->l_suppkey && a->l_disc_price),
+                    "bolap_q7: null column");
+    BOLAP_ARG_CHECK(o.l_orderkey_sorted || o.order_rows == 0 || (o.l_orderkey && o.o_orderkey),
+                    "bolap_q7: unsorted lineitem needs l_orderkey and o_orderkey");
+    BOLAP_ARG_CHECK(a->supp_rows >= 0 && a->supp_rows < (int64_t(1) << 30) &&
+                        (a->supp_rows == 0 || (a->s_suppkey && a->s_nationkey)),
+                    "bolap_q7: supplier columns");
+    BOLAP_ARG_CHECK(a->n_suppkeys >= 0 && a->n_suppkeys < (int64_t(1) << 31),
+                    "bolap_q7: n_suppkeys");
+    for (int i = 0; i < 8; ++i) rev[i] = 0;
+    if (o.order_rows == 0 || a->n_suppkeys == 0) return BOLAP_OK;
+    WsArena ws;
+    const size_t o_owner = ws.plan(size_t(a->n_suppkeys) * 4);
+    const size_t o_out = ws.plan(8 * 8);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* owner = ws.at<int32_t>(o_owner);
+    auto* out = ws.at<unsigned long long>(o_out);
+    BOLAP_TRY(zero_dev(ctx, owner, ws.off));
+    TraceScope ts(ctx, "q7_orders");
+    if (a->supp_rows > 0) {
+        hipLaunchKernelGGL(q7_tag_k, dim3(grid_of(ctx, a->supp_rows)), dim3(kT), 0, ctx->stream,
+                           a->s_suppkey, a->s_nationkey, a->supp_rows, a->n_suppkeys, a->nation1,
+                           a->nation2, owner);
+        BOLAP_TRY(launch_check("q7_tag_k"));
+    }
+    const Q7K k{o.order_rows, a->o_cust_nation, o.o_orderkey, li_side(&o), a->l_shipdate,
+                a->l_suppkey, a->l_disc_price, owner, a->n_suppkeys, a->nation1, a->nation2,
+                a->ship_lo, uint32_t(a->ship_hi - a->ship_lo), a->split};
+    hipLaunchKernelGGL(q7_k, dim3(grid_of(ctx, o.order_rows, 8)), dim3(kT), 0, ctx->stream, k, out);
+    BOLAP_TRY(launch_check("q7_k"));
+    return fetch(ctx, rev, out, 8 * 8);
+}
+
+extern "C" int bolap_q8(bolap_ctx* ctx, const bolap_q8_args* a, int64_t* total, int64_t* nation) {
+    BOLAP_ARG_CHECK(ctx && a && total && nation, "bolap_q8: null argument");
+    BOLAP_TRY(check_ranges(&a->orders, "bolap_q8"));
+    const bolap_order_side& o = a->orders;
+    BOLAP_ARG_CHECK(o.order_rows == 0 || (a->o_custkey && a->l_partkey && a->l_suppkey &&
+                                          a->l_price && a->l_disc),
+                    "bolap_q8: null column");
+    BOLAP_ARG_CHECK(a->n_nation >= 0 && a->n_nation <= 256, "bolap_q8: n_nation %d", a->n_nation);
+    BOLAP_ARG_CHECK(a->n_cust_keys >= 0 && a->n_supp_keys >= 0 && a->n_part_keys >= 0,
+                    "bolap_q8: key domains");
+    BOLAP_ARG_CHECK((a->cust_rows == 0 || (a->c_custkey && a->c_nationkey)) &&
+                        (a->supp_rows == 0 || (a->s_suppkey && a->s_nationkey)) &&
+                        (a->part_rows == 0 || (a->p_partkey && a->p_type)),
+                    "bolap_q8: null dimension column");
+    total[0] = total[1] = nation[0] = nation[1] = 0;
+    if (o.order_rows == 0) return BOLAP_OK;
+    WsArena ws;
+    const size_t o_c = ws.plan(size_t(a->n_cust_keys) + 1);
+    const size_t o_s = ws.plan(size_t(a->n_supp_keys) + 1);
+    const size_t o_p = ws.plan(size_t(a->n_part_keys) + 1);
+    const size_t o_out = ws.plan(4 * 8);
+    const size_t o_lut = ws.plan(256);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    BOLAP_TRY(zero_dev(ctx, ws.base, o_lut));
+    auto* lut = ws.at<uint8_t>(o_lut);
+    BOLAP_HIP_TRY(hipMemcpyAsync(lut, a->nation_in_region, size_t(a->n_nation),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    TraceScope ts(ctx, "q8_orders");
+    if (a->cust_rows > 0)
+        hipLaunchKernelGGL(mark_keys_k, dim3(grid_of(ctx, a->cust_rows)), dim3(kT), 0, ctx->stream,
+                           a->c_custkey, static_cast<const void*>(a->c_nationkey), int32_t(BOLAP_I32),
+                           a->cust_rows, a->n_cust_keys, 0, -1, lut, a->n_nation,
+                           ws.at<uint8_t>(o_c));
+    if (a->supp_rows > 0)
+        hipLaunchKernelGGL(mark_keys_k, dim3(grid_of(ctx, a->supp_rows)), dim3(kT), 0, ctx->stream,
+                           a->s_suppkey, static_cast<const void*>(a->s_nationkey), int32_t(BOLAP_I32),
+                           a->supp_rows, a->n_supp_keys, a->target_nation, a->target_nation,
+                           static_cast<const uint8_t*>(nullptr), 0, ws.at<uint8_t>(o_s));
+    if (a->part_rows > 0 && a->type_code >= 0)
+        hipLaunchKernelGGL(mark_keys_k, dim3(grid_of(ctx, a->part_rows)), dim3(kT), 0, ctx->stream,
+                           a->p_partkey, static_cast<const void*>(a->p_type), int32_t(BOLAP_U8),
+                           a->part_rows, a->n_part_keys, a->type_code, a->type_code,
+                           static_cast<const uint8_t*>(nullptr), 0, ws.at<uint8_t>(o_p));
+    BOLAP_TRY(launch_check("mark_keys_k"));
+    const Q8K k{o.o_orderdate, o.order_rows, o.o_orderkey, a->o_custkey, li_side(&o), a->l_partkey,
+                a->l_suppkey, a->l_price, a->l_disc, ws.at<uint8_t>(o_c), ws.at<uint8_t>(o_s),
+                ws.at<uint8_t>(o_p), a->n_cust_keys, a->n_supp_keys, a->n_part_keys, a->date_lo,
+                a->date_hi, a->split};
+    auto* out = ws.at<unsigned long long>(o_out);
+    hipLaunchKernelGGL(q8_k, dim3(window_grid(ctx, o.order_rows)), dim3(kT), 0, ctx->stream, k, out);
+    BOLAP_TRY(launch_check("q8_k"));
+    int64_t h[4];
+    BOLAP_TRY(fetch(ctx, h, out, sizeof(h)));
+    total[0] = h[0];
+    total[1] = h[1];
+    nation[0] = h[2];
+    nation[1] = h[3];
+    return BOLAP_OK;
+}
+
+extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals, uint8_t* used,
+                        int32_t* min_year, int32_t* year_span) {
+    BOLAP_ARG_CHECK(ctx && a && totals && used && min_year && year_span,
+                    "bolap_q9: null argument");
+    BOLAP_ARG_CHECK(a->part_rows >= 0 && a->ps_rows >= 0 && a->order_rows >= 0 && a->li_rows >= 0 &&
+                        a->ps_rows < (int64_t(1) << 32),
+                    "bolap_q9: rows out of range");
+    BOLAP_ARG_CHECK(a->part_rows == 0 || (a->p_partkey && a->p_ok), "bolap_q9: null part column");
+    BOLAP_ARG_CHECK(a->ps_rows == 0 || (a->ps_partkey && a->ps_suppkey && a->ps_supplycost),
+                    "bolap_q9: null partsupp column");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || (a->o_orderdate && a->order_map),
+                    "bolap_q9: null order column");
+    BOLAP_ARG_CHECK(a->li_rows == 0 || (a->l_partkey && a->l_suppkey && a->l_orderkey &&
+                                        a->l_quantity && a->l_disc_price && a->l_supp_nation &&
+                                        aligned(a->l_partkey, 16)),
+                    "bolap_q9: null lineitem column (l_partkey 16-byte aligned)");
+    BOLAP_ARG_CHECK(a->n_nation >= 1 && a->n_part_keys >= 0, "bolap_q9: n_nation / n_part_keys");
+    *min_year = 0;
+    *year_span = 0;
+    if (a->order_rows == 0) return BOLAP_OK;
+    // earliest and latest orderdate: the first and last rows (sorted by date)
+    int16_t ends[2];
+    BOLAP_TRY(fetch(ctx, &ends[0], a->o_orderdate, 2));
+    BOLAP_TRY(fetch(ctx, &ends[1], a->o_orderdate + a->order_rows - 1, 2));
+    auto year_of = [](int32_t off) {
+        int64_t z = int64_t(off) + 8035 + 719468;
+        const int64_t era = (z >= 0 ? z : z - 146096) / 146097;
+        const int64_t doe = z - era * 146097;
+        const int64_t yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+        const int64_t doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+        const int64_t mp = (5 * doy + 2) / 153;
+        const int64_t m = mp < 10 ? mp + 3 : mp - 9;
+        return int32_t(yoe + era * 400 + (m <= 2 ? 1 : 0));
+    };
+    const int32_t y0 = year_of(ends[0]), span = year_of(ends[1]) - y0 + 1;
+    BOLAP_ARG_CHECK(span >= 1 && int64_t(span) * a->n_nation <= kQ9Groups,
+                    "bolap_q9: %d nations x %d years exceed %d groups", a->n_nation, span,
+                    kQ9Groups);
+    *min_year = y0;
+    *year_span = span;
+    const int64_t np = a->n_part_keys;
+    const int64_t tiles = (np + 1 + kTile - 1) / kTile;  // np + 1 offsets
+    WsArena ws;
+    const size_t o_mark = ws.plan(size_t(np) + 1);
+    const size_t o_off = ws.plan(size_t(np + 1) * 4);
+    const size_t o_tot = ws.plan(size_t(kQ9Groups) * 8);
+    const size_t o_used = ws.plan(size_t(kQ9Groups));
+    const size_t o_zero_end = ws.off;
+    const size_t o_cur = ws.plan(size_t(np + 1) * 4);
+    const size_t o_tiles = ws.plan(size_t(tiles + 1) * 4);
+    const size_t o_idx = ws.plan(size_t(a->ps_rows > 0 ? a->ps_rows : 1) * 4);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    BOLAP_TRY(zero_dev(ctx, ws.base, o_zero_end));
+    auto* off = ws.at<uint32_t>(o_off);
+    auto* cur = ws.at<uint32_t>(o_cur);
+    auto* tl = ws.at<uint32_t>(o_tiles);
+    {
+        TraceScope ts(ctx, "q9_build");
+        if (a->part_rows > 0 && np > 0)
+            hipLaunchKernelGGL(mark_keys_k, dim3(grid_of(ctx, a->part_rows)), dim3(kT), 0,
+                               ctx->stream, a->p_partkey, static_cast<const void*>(a->p_ok),
+                               int32_t(BOLAP_U8), a->part_rows, np, 1, 1,
+                               static_cast<const uint8_t*>(nullptr), 0, ws.at<uint8_t>(o_mark));
+        if (a->ps_rows > 0 && np > 0) {
+            hipLaunchKernelGGL(ps_count_k, dim3(grid_of(ctx, a->ps_rows)), dim3(kT), 0, ctx->stream,
+                               a->ps_partkey, a->ps_rows, np, off);
+            hipLaunchKernelGGL(tile_sum_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, off,
+                               np + 1, tl);
+            hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(1024), 0, ctx->stream, tl, tiles);
+            hipLaunchKernelGGL(tile_scan_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, off,
+                               np + 1, tl);
+            BOLAP_HIP_TRY(hipMemcpyAsync(cur, off, size_t(np + 1) * 4, hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            hipLaunchKernelGGL(ps_fill_k, dim3(grid_of(ctx, a->ps_rows)), dim3(kT), 0, ctx->stream,
+                               a->ps_partkey, a->ps_rows, np, cur, ws.at<uint32_t>(o_idx));
+        }
+        BOLAP_TRY(launch_check("ps_fill_k"));
+    }
+    auto* tot = ws.at<unsigned long long>(o_tot);
+    auto* usd = ws.at<uint8_t>(o_used);
+    if (a->li_rows > 0 && np > 0) {
+        TraceScope ts(ctx, "q9_lineitem");
+        const Q9K k{a->li_rows, a->l_partkey, a->l_suppkey, a->l_orderkey, a->l_quantity,
+                    a->l_disc_price, a->l_supp_nation, ws.at<uint8_t>(o_mark), np, off,
+                    ws.at<uint32_t>(o_idx), a->ps_suppkey, a->ps_supplycost, a->order_map,
+                    a->map_size, a->o_orderdate, a->n_nation, y0, span};
+        hipLaunchKernelGGL(q9_k, dim3(grid_of(ctx, (a->li_rows + 3) / 4, 8)), dim3(kT), 0,
+                           ctx->stream, k, tot, usd);
+        BOLAP_TRY(launch_check("q9_k"));
+    }
+    const size_t g = size_t(a->n_nation) * size_t(span);
+    BOLAP_TRY(fetch(ctx, totals, tot, g * 8));
+    return fetch(ctx, used, usd, g);
+}

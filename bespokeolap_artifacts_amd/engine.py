@@ -616,6 +616,71 @@ class Engine:
         ext = [[r.extra_numcust[i], r.extra_totacctbal[i]] for i in range(len(extra_pairs))]
         return num, ext
 
+    def q7(self, t: dict, n1: int, n2: int, ship_lo: int, ship_hi: int, split: int) -> list:
+        a = N.Q7Args()
+        a.orders = self.order_side(t)
+        a.o_cust_nation, a.l_shipdate = ptr(t["o_cust_nation"]), ptr(t["l_shipdate"])
+        a.l_suppkey, a.l_disc_price = ptr(t["l_suppkey"]), ptr(t["l_disc_price"])
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.s_nationkey = ptr(t["s_suppkey"]), ptr(t["s_nationkey"])
+        a.n_suppkeys = int(t["s_suppkey"].max().item()) + 1 if a.supp_rows else 0
+        a.nation1, a.nation2, a.ship_lo, a.ship_hi, a.split = n1, n2, ship_lo, ship_hi, split
+        out = (C.c_int64 * 8)()
+        N.check(self.lib.bolap_q7(self.ctx, C.byref(a), out))
+        return list(out)
+
+    def q8(self, t: dict, nation_in_region, target: int, type_code: int, date_lo: int,
+           date_hi: int, split: int):
+        import numpy as np
+        nr = np.ascontiguousarray(nation_in_region, np.uint8)
+        a = N.Q8Args()
+        a.orders = self.order_side(t)
+        a.o_custkey, a.l_partkey, a.l_suppkey = (ptr(t["o_custkey"]), ptr(t["l_partkey"]),
+                                                 ptr(t["l_suppkey"]))
+        a.l_price, a.l_disc = ptr(t["l_extendedprice"]), ptr(t["l_discount"])
+        a.cust_rows = int(t["c_custkey"].numel())
+        a.c_custkey, a.c_nationkey = ptr(t["c_custkey"]), ptr(t["c_nationkey"])
+        a.n_cust_keys = int(t["c_custkey"].max().item()) + 1
+        a.nation_in_region, a.n_nation = nr.ctypes.data, len(nr)
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.s_nationkey = ptr(t["s_suppkey"]), ptr(t["s_nationkey"])
+        a.n_supp_keys = int(t["s_suppkey"].max().item()) + 1
+        a.target_nation = target
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_type = ptr(t["p_partkey"]), ptr(t["p_type"])
+        a.n_part_keys = int(t["p_partkey"].max().item()) + 1
+        a.type_code, a.date_lo, a.date_hi, a.split = type_code, date_lo, date_hi, split
+        tot, nat = (C.c_int64 * 2)(), (C.c_int64 * 2)()
+        N.check(self.lib.bolap_q8(self.ctx, C.byref(a), tot, nat))
+        return list(tot), list(nat)
+
+    def q9(self, t: dict, p_ok, n_nation: int):
+        import numpy as np
+        a = N.Q9Args()
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_ok = ptr(t["p_partkey"]), ptr(p_ok)
+        a.n_part_keys = int(t["p_partkey"].max().item()) + 1
+        a.ps_rows = int(t["ps_partkey"].numel())
+        a.ps_partkey, a.ps_suppkey = ptr(t["ps_partkey"]), ptr(t["ps_suppkey"])
+        a.ps_supplycost = ptr(t["ps_supplycost"])
+        a.order_rows = int(t["o_orderdate"].numel())
+        a.o_orderdate, a.order_map = ptr(t["o_orderdate"]), ptr(t["order_map"])
+        a.map_size = int(t["order_map"].numel())
+        a.li_rows = int(t["l_partkey"].numel())
+        a.l_partkey, a.l_suppkey, a.l_orderkey = (ptr(t["l_partkey"]), ptr(t["l_suppkey"]),
+                                                  ptr(t["l_orderkey"]))
+        a.l_quantity, a.l_disc_price = ptr(t["l_quantity"]), ptr(t["l_disc_price"])
+        a.l_supp_nation, a.n_nation = ptr(t["l_supp_nation"]), n_nation
+        tot = np.zeros(2048, np.int64)
+        used = np.zeros(2048, np.uint8)
+        y0, span = C.c_int32(0), C.c_int32(0)
+        N.check(self.lib.bolap_q9(self.ctx, C.byref(a), tot.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  used.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(y0),
+                                  C.byref(span)))
+        g = n_nation * span.value
+        return y0.value, tot[:g].reshape(n_nation, span.value), used[:g].reshape(n_nation,
+                                                                                 span.value)
+
     def q17_args(self, t: dict, preds: list) -> N.Q17Args:
         a = N.Q17Args()
         a.part_rows = int(t["p_partkey"].numel())

@@ -324,6 +324,35 @@ class Q22Result(C.Structure):
                 ("extra_numcust", C.c_int64 * 7), ("extra_totacctbal", C.c_int64 * 7)]
 
 
+class Q7Args(C.Structure):
+    _fields_ = [("orders", OrderSide), ("o_cust_nation", vp), ("l_shipdate", vp),
+                ("l_suppkey", vp), ("l_disc_price", vp), ("supp_rows", C.c_int64),
+                ("s_suppkey", vp), ("s_nationkey", vp), ("n_suppkeys", C.c_int64),
+                ("nation1", C.c_int32), ("nation2", C.c_int32), ("ship_lo", C.c_int32),
+                ("ship_hi", C.c_int32), ("split", C.c_int32)]
+
+
+class Q8Args(C.Structure):
+    _fields_ = [("orders", OrderSide), ("o_custkey", vp), ("l_partkey", vp), ("l_suppkey", vp),
+                ("l_price", vp), ("l_disc", vp), ("cust_rows", C.c_int64), ("c_custkey", vp),
+                ("c_nationkey", vp), ("n_cust_keys", C.c_int64), ("nation_in_region", vp),
+                ("n_nation", C.c_int32), ("supp_rows", C.c_int64), ("s_suppkey", vp),
+                ("s_nationkey", vp), ("n_supp_keys", C.c_int64), ("target_nation", C.c_int32),
+                ("part_rows", C.c_int64), ("p_partkey", vp), ("p_type", vp),
+                ("n_part_keys", C.c_int64), ("type_code", C.c_int32), ("date_lo", C.c_int32),
+                ("date_hi", C.c_int32), ("split", C.c_int32)]
+
+
+class Q9Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_partkey", vp), ("p_ok", vp),
+                ("n_part_keys", C.c_int64), ("ps_rows", C.c_int64), ("ps_partkey", vp),
+                ("ps_suppkey", vp), ("ps_supplycost", vp), ("order_rows", C.c_int64),
+                ("o_orderdate", vp), ("order_map", vp), ("map_size", C.c_int64),
+                ("li_rows", C.c_int64), ("l_partkey", vp), ("l_suppkey", vp), ("l_orderkey", vp),
+                ("l_quantity", vp), ("l_disc_price", vp), ("l_supp_nation", vp),
+                ("n_nation", C.c_int32)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -393,6 +422,9 @@ PROTOTYPES = {
     "bolap_q12": (C.c_int, [vp, P(Q12Args), P(Q12Result)]),
     "bolap_q19": (C.c_int, [vp, P(Q19Args), P(i64)]),
     "bolap_q21": (C.c_int, [vp, P(Q21Args), P(i64), i64]),
+    "bolap_q7": (C.c_int, [vp, P(Q7Args), P(i64)]),
+    "bolap_q8": (C.c_int, [vp, P(Q8Args), P(i64), P(i64)]),
+    "bolap_q9": (C.c_int, [vp, P(Q9Args), P(i64), P(C.c_uint8), P(i32), P(i32)]),
     "bolap_q22": (C.c_int, [vp, P(Q22Args), P(Q22Result)]),
     "bolap_q14_write_csv": (C.c_int, [C.c_char_p, P(Q14Result)]),
     "bolap_q17": (C.c_int, [vp, P(Q17Args), P(Q17Result)]),

@@ -999,6 +999,100 @@ typedef struct {
 } bolap_q22_result;
 int bolap_q22(bolap_ctx* ctx, const bolap_q22_args* args, bolap_q22_result* out);
 
+/* Q7 (bespoke_tpch/query_q7.cpp:150-330): suppliers tagged 0 by nation1 or
+   (nation1 != nation2) 1 by nation2 — the last row of a key with one of the
+   two nations decides.  Over every order whose customer nation
+   (o_cust_nation) is nation1 (index 0) or nation2 (index 1), its lineitems
+   with shipdate in [ship_lo, ship_hi] (inclusive) and a tagged supplier of
+   the other index (both 0 when the nations are equal) add discounted_price
+   to rev[supp * 4 + cust * 2 + (shipdate >= split)].  n_suppkeys = max
+   s_suppkey + 1; lineitem suppkeys outside it are untagged.  rev[8] host. */
+typedef struct {
+    bolap_order_side orders;         /* o_orderdate unused */
+    const int16_t* o_cust_nation;
+    const int16_t* l_shipdate;
+    const int32_t* l_suppkey;
+    const int32_t* l_disc_price;
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    const int32_t* s_nationkey;
+    int64_t n_suppkeys;
+    int32_t nation1, nation2;
+    int32_t ship_lo, ship_hi, split;
+} bolap_q7_args;
+int bolap_q7(bolap_ctx* ctx, const bolap_q7_args* args, int64_t* rev);
+
+/* Q8 (bespoke_tpch/query_q8.cpp:150-330): customer keys marked by rows whose
+   nation is marked in nation_in_region (a HOST array of n_nation bytes),
+   supplier keys by rows of target_nation, part keys by rows whose type code
+   is type_code (-1: none); marks are never cleared.  Orders of
+   [date_lo, date_hi] (inclusive) with a marked customer: every lineitem of a
+   marked part adds its volume price * (100 - discount) to total[year] and,
+   with a marked supplier, to nation[year]; year = orderdate >= split.
+   n_*_keys = max key + 1 of each table (keys outside are unmarked). */
+typedef struct {
+    bolap_order_side orders;
+    const int32_t* o_custkey;
+    const int32_t* l_partkey;
+    const int32_t* l_suppkey;
+    const int32_t* l_price;
+    const uint8_t* l_disc;
+    int64_t cust_rows;
+    const int32_t* c_custkey;
+    const int32_t* c_nationkey;
+    int64_t n_cust_keys;
+    const uint8_t* nation_in_region;
+    int32_t n_nation;
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    const int32_t* s_nationkey;
+    int64_t n_supp_keys;
+    int32_t target_nation;
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const uint8_t* p_type;           /* codes */
+    int64_t n_part_keys;
+    int32_t type_code;
+    int32_t date_lo, date_hi, split;
+} bolap_q8_args;
+int bolap_q8(bolap_ctx* ctx, const bolap_q8_args* args, int64_t* total, int64_t* nation);
+
+/* Q9 (bespoke_tpch/query_q9.cpp:150-400): part keys marked by the rows with
+   p_ok (the name contains the colour, decided on the host per name as the
+   reference does).  Every lineitem of a marked part takes the supply cost of
+   the first partsupp row (in row order) with its partkey and suppkey, its
+   supplier nation (l_supp_nation in [0, n_nation)) and its order's year
+   (order_map[orderkey], map_size entries; rows without an order are
+   skipped) and adds discounted_price - supplycost * quantity / 100 (C
+   division) to totals[nation * year_span + year - min_year], setting used[]
+   too.  min_year / year_span come from the first and last o_orderdate (the
+   orders are sorted by date); n_nation * year_span <= 2048.  totals and used
+   (host) hold n_nation * year_span entries. */
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const uint8_t* p_ok;
+    int64_t n_part_keys;             /* max p_partkey + 1 */
+    int64_t ps_rows;
+    const int32_t* ps_partkey;
+    const int32_t* ps_suppkey;
+    const int32_t* ps_supplycost;    /* cents */
+    int64_t order_rows;
+    const int16_t* o_orderdate;
+    const int32_t* order_map;
+    int64_t map_size;
+    int64_t li_rows;
+    const int32_t* l_partkey;
+    const int32_t* l_suppkey;
+    const int32_t* l_orderkey;
+    const int16_t* l_quantity;
+    const int32_t* l_disc_price;
+    const int16_t* l_supp_nation;
+    int32_t n_nation;
+} bolap_q9_args;
+int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* args, int64_t* totals, uint8_t* used,
+             int32_t* min_year, int32_t* year_span);
+
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */
 /* ------------------------------------------------------------------------ */

@@ -821,3 +821,78 @@ def q22(db: dict, numeric_codes, extra_pairs):
     L.oracle_q22(len(c[0]), *[_p(x) for x in c], _p(allowed), len(extra_pairs), _p(ex), len(ok),
                  _p(ok), _p(num), _p(ext))
     return num.reshape(100, 2), ext[:2 * len(extra_pairs)].reshape(-1, 2)
+
+
+def _tpch3():
+    L = lib()
+    if not getattr(L, "_tpch3_ready", False):
+        L.oracle_q7.restype = None
+        L.oracle_q7.argtypes = [i64, vp, vp, vp, vp, vp, i32, vp, vp, vp, i64, vp, vp, i32, i32,
+                                i32, i32, i32, vp]
+        L.oracle_q8.restype = None
+        L.oracle_q8.argtypes = [i64, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp,
+                                i32, i64, vp, vp, i32, i64, vp, vp, i32, i32, i32, i32, vp, vp]
+        L.oracle_q9.restype = i32
+        L.oracle_q9.argtypes = [i64, vp, vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, vp, vp,
+                                vp, vp, i32, vp, vp, vp]
+        L._tpch3_ready = True
+    return L
+
+
+def _orders_side(db):
+    return [_c(db["o_orderkey"], np.int32), _c(db["r_start"], np.uint32),
+            _c(db["r_end"], np.uint32), _c(db["l_orderkey"], np.int32)]
+
+
+def q7(db: dict, n1: int, n2: int, ship_lo: int, ship_hi: int, split: int) -> np.ndarray:
+    """rev[supp, cust, year] of run_q7 (oracle_q7), shape (2, 2, 2)."""
+    ok, rs, re_, lk = _orders_side(db)
+    cn = _c(db["o_cust_nation"], np.int16)
+    li = [_c(db["l_shipdate"], np.int16), _c(db["l_suppkey"], np.int32),
+          _c(db["l_disc_price"], np.int32)]
+    s = [_c(db["s_suppkey"], np.int32), _c(db["s_nationkey"], np.int32)]
+    out = np.zeros(8, np.int64)
+    _tpch3().oracle_q7(len(ok), _p(ok), _p(cn), _p(rs), _p(re_), _p(lk), int(db["l_sorted"]),
+                       *[_p(x) for x in li], len(s[0]), _p(s[0]), _p(s[1]), n1, n2, ship_lo,
+                       ship_hi, split, _p(out))
+    return out.reshape(2, 2, 2)
+
+
+def q8(db: dict, nation_in_region, target: int, type_code: int, date_lo: int, date_hi: int,
+       split: int):
+    """(total[2], nation[2]) volumes of run_q8 (oracle_q8)."""
+    ok, rs, re_, lk = _orders_side(db)
+    od, oc = _c(db["o_orderdate"], np.int16), _c(db["o_custkey"], np.int32)
+    li = [_c(db["l_partkey"], np.int32), _c(db["l_suppkey"], np.int32),
+          _c(db["l_extendedprice"], np.int32), _c(db["l_discount"], np.uint8)]
+    c = [_c(db["c_custkey"], np.int32), _c(db["c_nationkey"], np.int32)]
+    nr = _c(nation_in_region, np.uint8)
+    s = [_c(db["s_suppkey"], np.int32), _c(db["s_nationkey"], np.int32)]
+    p = [_c(db["p_partkey"], np.int32), _c(db["p_type"], np.uint16)]
+    tot, nat = np.zeros(2, np.int64), np.zeros(2, np.int64)
+    _tpch3().oracle_q8(len(ok), _p(ok), _p(od), _p(oc), _p(rs), _p(re_), _p(lk),
+                       int(db["l_sorted"]), *[_p(x) for x in li], len(c[0]), _p(c[0]), _p(c[1]),
+                       _p(nr), len(nr), len(s[0]), _p(s[0]), _p(s[1]), target, len(p[0]),
+                       _p(p[0]), _p(p[1]), type_code, date_lo, date_hi, split, _p(tot), _p(nat))
+    return tot, nat
+
+
+def q9(db: dict, p_ok_row, n_nation: int):
+    """(min_year, totals[n_nation, span], used[n_nation, span]) of run_q9."""
+    po = _c(p_ok_row, np.uint8)
+    p = _c(db["p_partkey"], np.int32)
+    ps = [_c(db["ps_partkey"], np.int32), _c(db["ps_suppkey"], np.int32),
+          _c(db["ps_supplycost"], np.int32)]
+    o = [_c(db["o_orderkey"], np.int32), _c(db["o_orderdate"], np.int16)]
+    li = [_c(db["l_partkey"], np.int32), _c(db["l_suppkey"], np.int32),
+          _c(db["l_orderkey"], np.int32), _c(db["l_quantity"], np.int16),
+          _c(db["l_disc_price"], np.int32), _c(db["l_supp_nation"], np.int16)]
+    cap = n_nation * 64
+    tot, used = np.zeros(cap, np.int64), np.zeros(cap, np.uint8)
+    y0 = C.c_int32(0)
+    span = _tpch3().oracle_q9(len(p), _p(p), _p(po), len(ps[0]), *[_p(x) for x in ps], len(o[0]),
+                              _p(o[0]), _p(o[1]), len(li[0]), *[_p(x) for x in li], n_nation,
+                              _p(tot), _p(used), C.byref(y0))
+    assert span <= 64
+    return y0.value, tot[:n_nation * span].reshape(n_nation, span), \
+        used[:n_nation * span].reshape(n_nation, span)

@@ -297,4 +297,28 @@ void oracle_q22(int64_t c_rows, const int32_t* c_custkey, const uint8_t* c_phone
                 const uint8_t* numeric_allowed, int32_t n_extra, const int32_t* extra_pairs,
                 int64_t o_rows, const int32_t* o_custkey, int64_t* numeric, int64_t* extra);
 
+void oracle_q7(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_cust_nation,
+               const uint32_t* r_start, const uint32_t* r_end, const int32_t* l_orderkey,
+               int32_t l_sorted, const int16_t* l_shipdate, const int32_t* l_suppkey,
+               const int32_t* l_disc_price, int64_t s_rows, const int32_t* s_suppkey,
+               const int32_t* s_nationkey, int32_t n1, int32_t n2, int32_t ship_lo,
+               int32_t ship_hi, int32_t split, int64_t* rev);
+void oracle_q8(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const int32_t* o_custkey, const uint32_t* r_start, const uint32_t* r_end,
+               const int32_t* l_orderkey, int32_t l_sorted, const int32_t* l_partkey,
+               const int32_t* l_suppkey, const int32_t* l_price, const uint8_t* l_disc,
+               int64_t c_rows, const int32_t* c_custkey, const int32_t* c_nationkey,
+               const uint8_t* nation_in_region, int32_t n_nation, int64_t s_rows,
+               const int32_t* s_suppkey, const int32_t* s_nationkey, int32_t target,
+               int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_type,
+               int32_t type_code, int32_t date_lo, int32_t date_hi, int32_t split,
+               int64_t* total, int64_t* nation);
+int32_t oracle_q9(int64_t p_rows, const int32_t* p_partkey, const uint8_t* p_ok_row,
+                  int64_t ps_rows, const int32_t* ps_partkey, const int32_t* ps_suppkey,
+                  const int32_t* ps_supplycost, int64_t o_rows, const int32_t* o_orderkey,
+                  const int16_t* o_orderdate, int64_t li_rows, const int32_t* l_partkey,
+                  const int32_t* l_suppkey, const int32_t* l_orderkey, const int16_t* l_quantity,
+                  const int32_t* l_disc_price, const int16_t* l_supp_nation, int32_t n_nation,
+                  int64_t* totals, uint8_t* used, int32_t* min_year_out);
+
 #endif

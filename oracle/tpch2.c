@@ -17,6 +17,7 @@
 #include "oracle.h"
 
 #include <stdlib.h>
+
 #include <string.h>
 
 /* lower_bound over a non-decreasing int16 column (std::lower_bound). */
@@ -384,4 +385,213 @@ void oracle_q22(int64_t c_rows, const int32_t* c_custkey, const uint8_t* c_phone
     }
     free(has);
     free(slot);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q7 (bespoke_tpch/query_q7.cpp:150-330).  Supplier tags: a supplier row   */
+/* with nation n1 tags its key 0, else (n1 != n2) with nation n2 tags it 1; */
+/* other rows leave the tag as it is (-1 at first).  Over every order whose */
+/* customer nation is n1 (index 0) or, when n1 != n2, n2 (index 1) and whose */
+/* range is not empty: lineitems with shipdate in [ship_lo, ship_hi]        */
+/* (inclusive) and a tagged supplier of the other index (both 0 when        */
+/* n1 == n2) add discounted_price to rev[supp * 4 + cust * 2 + year], year  */
+/* = shipdate >= split.  Lineitem suppkeys outside the tag array count as   */
+/* untagged.                                                                 */
+/* ------------------------------------------------------------------------ */
+void oracle_q7(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_cust_nation,
+               const uint32_t* r_start, const uint32_t* r_end, const int32_t* l_orderkey,
+               int32_t l_sorted, const int16_t* l_shipdate, const int32_t* l_suppkey,
+               const int32_t* l_disc_price, int64_t s_rows, const int32_t* s_suppkey,
+               const int32_t* s_nationkey, int32_t n1, int32_t n2, int32_t ship_lo,
+               int32_t ship_hi, int32_t split, int64_t* rev) {
+    for (int i = 0; i < 8; ++i) rev[i] = 0;
+    const int same = n1 == n2;
+    int32_t smax = -1;
+    for (int64_t i = 0; i < s_rows; ++i)
+        if (s_suppkey[i] > smax) smax = s_suppkey[i];
+    int8_t* tag = (int8_t*)malloc((size_t)(smax + 1 > 0 ? smax + 1 : 1));
+    for (int32_t k = 0; k <= smax; ++k) tag[k] = -1;
+    for (int64_t i = 0; i < s_rows; ++i) {
+        if (s_suppkey[i] < 0) continue;
+        if (s_nationkey[i] == n1) tag[s_suppkey[i]] = 0;
+        else if (!same && s_nationkey[i] == n2) tag[s_suppkey[i]] = 1;
+    }
+    for (int64_t o = 0; o < o_rows; ++o) {
+        const int32_t cn = o_cust_nation[o];
+        int ci;
+        if (cn == n1) ci = 0;
+        else if (!same && cn == n2) ci = 1;
+        else continue;
+        if (r_end[o] <= r_start[o]) continue;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const int32_t ship = l_shipdate[r];
+            if ((uint32_t)(ship - ship_lo) > (uint32_t)(ship_hi - ship_lo)) continue;
+            const int32_t sk = l_suppkey[r];
+            const int si = (sk >= 0 && sk <= smax) ? tag[sk] : -1;
+            if (si < 0) continue;
+            if (oracle_mutation() != 22 && (same ? (si != 0 || ci != 0) : si == ci)) continue;
+            const int y = ship >= split;
+            rev[si * 4 + ci * 2 + y] += l_disc_price[r];
+        }
+    }
+    free(tag);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q8 (bespoke_tpch/query_q8.cpp:150-330): customers whose nation is in the */
+/* region mark their key, suppliers of the target nation mark theirs, parts */
+/* of type code type_code mark theirs (marks are never cleared).  Orders of */
+/* [date_lo, date_hi] (inclusive; lower_bound / upper_bound on the sorted   */
+/* dates) with a marked customer: every lineitem of a marked part adds its  */
+/* volume price * (100 - discount) to total[year] and, for a marked         */
+/* supplier, to nation[year]; year = orderdate >= split.                     */
+/* ------------------------------------------------------------------------ */
+static uint8_t* key_marks(int64_t rows, const int32_t* keys, int32_t* size_out) {
+    int32_t mx = -1;
+    for (int64_t i = 0; i < rows; ++i)
+        if (keys[i] > mx) mx = keys[i];
+    *size_out = mx + 1;
+    return (uint8_t*)calloc((size_t)(mx + 1 > 0 ? mx + 1 : 1), 1);
+}
+
+void oracle_q8(int64_t o_rows, const int32_t* o_orderkey, const int16_t* o_orderdate,
+               const int32_t* o_custkey, const uint32_t* r_start, const uint32_t* r_end,
+               const int32_t* l_orderkey, int32_t l_sorted, const int32_t* l_partkey,
+               const int32_t* l_suppkey, const int32_t* l_price, const uint8_t* l_disc,
+               int64_t c_rows, const int32_t* c_custkey, const int32_t* c_nationkey,
+               const uint8_t* nation_in_region, int32_t n_nation, int64_t s_rows,
+               const int32_t* s_suppkey, const int32_t* s_nationkey, int32_t target,
+               int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_type,
+               int32_t type_code, int32_t date_lo, int32_t date_hi, int32_t split,
+               int64_t* total, int64_t* nation) {
+    total[0] = total[1] = nation[0] = nation[1] = 0;
+    int32_t cn, sn, pn;
+    uint8_t* cmark = key_marks(c_rows, c_custkey, &cn);
+    uint8_t* smark = key_marks(s_rows, s_suppkey, &sn);
+    uint8_t* pmark = key_marks(p_rows, p_partkey, &pn);
+    for (int64_t i = 0; i < c_rows; ++i) {
+        const int32_t nk = c_nationkey[i];
+        if (c_custkey[i] >= 0 && nk >= 0 && nk < n_nation && nation_in_region[nk])
+            cmark[c_custkey[i]] = 1;
+    }
+    for (int64_t i = 0; i < s_rows; ++i)
+        if (s_suppkey[i] >= 0 && s_nationkey[i] == target) smark[s_suppkey[i]] = 1;
+    for (int64_t i = 0; i < p_rows; ++i)
+        if (p_partkey[i] >= 0 && type_code >= 0 && (int32_t)p_type[i] == type_code)
+            pmark[p_partkey[i]] = 1;
+    const int64_t s = lower_bound_i16(o_orderdate, o_rows, date_lo);
+    const int64_t e = lower_bound_i16(o_orderdate, o_rows, date_hi + 1);
+    for (int64_t o = s; o < e; ++o) {
+        const int32_t ck = o_custkey[o];
+        if (ck < 0 || ck >= cn || !cmark[ck]) continue;
+        if (r_end[o] <= r_start[o]) continue;
+        const int y = o_orderdate[o] >= split;
+        for (uint32_t r = r_start[o]; r < r_end[o]; ++r) {
+            if (!l_sorted && l_orderkey[r] != o_orderkey[o]) continue;
+            const int32_t pk = l_partkey[r];
+            if (pk < 0 || pk >= pn || !pmark[pk]) continue;
+            const int64_t vol = (int64_t)l_price[r] * (100 - (int32_t)l_disc[r]);
+            total[y] += vol;
+            const int32_t sk = l_suppkey[r];
+            const int hit = sk >= 0 && sk < sn && smark[sk];
+            if (oracle_mutation() == 23 ? !hit : hit) nation[y] += vol;
+        }
+    }
+    free(cmark);
+    free(smark);
+    free(pmark);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q9 (bespoke_tpch/query_q9.cpp:150-400): parts whose name contains the    */
+/* colour (p_ok_row, decided on the host as the reference does per name)    */
+/* mark their key.  Every lineitem of a marked part finds its supply cost   */
+/* in the first partsupp row (row order) with its partkey and suppkey, its  */
+/* supplier nation (lineitem.supp_nationkey, >= 0) and its order's year     */
+/* (the last order row of its key; rows without an order are skipped), and  */
+/* adds discounted_price - supplycost * quantity / 100 (C division) to      */
+/* totals[nation * year_span + year - min_year], marking used[] too.         */
+/* min_year / year_span come from the earliest and latest orderdate.        */
+/* ------------------------------------------------------------------------ */
+static int32_t base_days(void) { return (int32_t)oracle_days_from_civil(1992, 1, 1); }
+
+static int32_t year_of(int32_t offset) {
+    int32_t y, m, d;
+    oracle_civil_from_days(base_days() + offset, &y, &m, &d);
+    return y;
+}
+
+int32_t oracle_q9(int64_t p_rows, const int32_t* p_partkey, const uint8_t* p_ok_row,
+                  int64_t ps_rows, const int32_t* ps_partkey, const int32_t* ps_suppkey,
+                  const int32_t* ps_supplycost, int64_t o_rows, const int32_t* o_orderkey,
+                  const int16_t* o_orderdate, int64_t li_rows, const int32_t* l_partkey,
+                  const int32_t* l_suppkey, const int32_t* l_orderkey, const int16_t* l_quantity,
+                  const int32_t* l_disc_price, const int16_t* l_supp_nation, int32_t n_nation,
+                  int64_t* totals, uint8_t* used, int32_t* min_year_out) {
+    if (o_rows == 0) return 0;
+    int32_t dmin = o_orderdate[0], dmax = o_orderdate[0];
+    for (int64_t i = 0; i < o_rows; ++i) {
+        if (o_orderdate[i] < dmin) dmin = o_orderdate[i];
+        if (o_orderdate[i] > dmax) dmax = o_orderdate[i];
+    }
+    const int32_t y0 = year_of(dmin), span = year_of(dmax) - y0 + 1;
+    *min_year_out = y0;
+    for (int64_t i = 0; i < (int64_t)n_nation * span; ++i) {
+        totals[i] = 0;
+        used[i] = 0;
+    }
+    int32_t pn = 0;
+    uint8_t* pok = key_marks(p_rows, p_partkey, &pn);
+    for (int64_t i = 0; i < p_rows; ++i)
+        if (p_partkey[i] >= 0 && p_ok_row[i]) pok[p_partkey[i]] = 1;
+    /* partsupp rows grouped by partkey, in row order within a key */
+    int32_t psn = 0;
+    free(key_marks(ps_rows, ps_partkey, &psn));
+    int64_t* ps_off = (int64_t*)calloc((size_t)psn + 2, sizeof(int64_t));
+    int64_t* ps_idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ps_rows > 0 ? ps_rows : 1));
+    for (int64_t j = 0; j < ps_rows; ++j)
+        if (ps_partkey[j] >= 0) ps_off[ps_partkey[j] + 1] += 1;
+    for (int32_t k = 0; k < psn; ++k) ps_off[k + 1] += ps_off[k];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)psn + 1));
+    for (int32_t k = 0; k <= psn; ++k) fill[k] = ps_off[k];
+    for (int64_t j = 0; j < ps_rows; ++j)
+        if (ps_partkey[j] >= 0) ps_idx[fill[ps_partkey[j]]++] = j;
+    free(fill);
+    int32_t on = 0;
+    uint8_t* dummy = key_marks(o_rows, o_orderkey, &on);
+    free(dummy);
+    int32_t* orow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(on > 0 ? on : 1));
+    for (int32_t k = 0; k < on; ++k) orow[k] = -1;
+    for (int64_t i = 0; i < o_rows; ++i)
+        if (o_orderkey[i] >= 0) orow[o_orderkey[i]] = (int32_t)i;
+    for (int64_t i = 0; i < li_rows; ++i) {
+        const int32_t pk = l_partkey[i];
+        if (pk < 0 || pk >= pn || !pok[pk]) continue;
+        const int32_t sk = l_suppkey[i];
+        int64_t cost = 0;
+        int found = 0;
+        if (pk < psn)  /* first partsupp row of (pk, sk), in row order */
+            for (int64_t j = ps_off[pk]; j < ps_off[pk + 1]; ++j)
+                if (ps_suppkey[ps_idx[j]] == sk) {
+                    cost = ps_supplycost[ps_idx[j]];
+                    found = 1;
+                    break;
+                }
+        if (!found) continue;
+        const int32_t nk = l_supp_nation[i];
+        if (nk < 0 || nk >= n_nation) continue;
+        const int32_t ok = l_orderkey[i];
+        if (ok < 0 || ok >= on || orow[ok] < 0) continue;
+        const int32_t yi = year_of(o_orderdate[orow[ok]]) - y0;
+        const int64_t div = oracle_mutation() == 24 ? 10 : 100;
+        const int64_t profit = (int64_t)l_disc_price[i] - cost * (int64_t)l_quantity[i] / div;
+        totals[(int64_t)nk * span + yi] += profit;
+        used[(int64_t)nk * span + yi] = 1;
+    }
+    free(pok);
+    free(orow);
+    free(ps_off);
+    free(ps_idx);
+    return span;
 }

@@ -14,7 +14,7 @@ from tests.test_tpch2_oracle import q19_groups
 
 pytestmark = pytest.mark.gpu
 
-U8_CODES = ("l_shipmode", "l_shipinstruct", "o_orderpriority", "o_orderstatus")
+U8_CODES = ("l_shipmode", "l_shipinstruct", "o_orderpriority", "o_orderstatus", "p_type")
 
 
 def upload(db: dict) -> dict:
@@ -199,6 +199,34 @@ def test_q22(engine, db, codes):
     assert engine.q22(t, numeric, extra) == (num.tolist(), ext.tolist())
 
 
+Q7_DATES = (T.date_offset("1995-01-01"), T.date_offset("1996-12-31"), T.date_offset("1996-01-01"))
+
+
+@pytest.mark.parametrize("n1,n2", [(6, 7), (7, 6), (12, 18), (3, 3), (3, 40)])
+def test_q7(engine, db, n1, n2):
+    r, h, t = db
+    assert engine.q7(t, n1, n2, *Q7_DATES) == oracle.q7(h, n1, n2, *Q7_DATES).reshape(-1).tolist()
+
+
+@pytest.mark.parametrize("nation,region,ptype", [(8, 2, "ECONOMY ANODIZED STEEL"),
+                                                 (3, 1, "PROMO BRUSHED TIN"), (3, 1, "NOSUCH")])
+def test_q8(engine, db, nation, region, ptype):
+    r, h, t = db
+    inreg = (np.array(h["n_regionkey"]) == region).astype(np.uint8)
+    tc = h["type_dict"].index(ptype) if ptype in h["type_dict"] else -1
+    tot, nat = oracle.q8(h, inreg, nation, tc, *Q7_DATES)
+    assert engine.q8(t, inreg, nation, tc, *Q7_DATES) == (tot.tolist(), nat.tolist())
+
+
+@pytest.mark.parametrize("color", ["green", "puff", "tan"])
+def test_q9(engine, db, color):
+    r, h, t = db
+    p_ok = (h["p_name_words"] == T.COLORS.index(color)).any(axis=1).astype(np.uint8)
+    y0, tot, used = oracle.q9(h, p_ok, 25)
+    gy, gtot, gused = engine.q9(t, torch.from_numpy(p_ok).cuda(), 25)
+    assert gy == y0 and np.array_equal(gtot, tot) and np.array_equal(gused, used)
+
+
 @pytest.fixture(scope="module")
 def sf10():
     """15,000,000 orders, about 60,000,000 lineitems, 2,000,000 parts,
@@ -238,6 +266,17 @@ def test_sf10_all_five(engine, sf10):
     want19 = oracle.q19(h, g, m1, m2, ins)
     assert want19 > 0
     assert engine.q19(t, g, m1, m2, ins) == want19
+    print("q7 q8 q9", flush=True)
+    assert engine.q7(t, 6, 7, *Q7_DATES) == oracle.q7(h, 6, 7, *Q7_DATES).reshape(-1).tolist()
+    inreg = (np.array(h["n_regionkey"]) == 1).astype(np.uint8)
+    tc = h["type_dict"].index("ECONOMY ANODIZED STEEL")
+    tot, nat = oracle.q8(h, inreg, 2, tc, *Q7_DATES)
+    assert tot.sum() > 0
+    assert engine.q8(t, inreg, 2, tc, *Q7_DATES) == (tot.tolist(), nat.tolist())
+    p_ok = (h["p_name_words"] == T.COLORS.index("green")).any(axis=1).astype(np.uint8)
+    y0, tot9, used9 = oracle.q9(h, p_ok, 25)
+    gy, gtot, gused = engine.q9(t, torch.from_numpy(p_ok).cuda(), 25)
+    assert gy == y0 and np.array_equal(gtot, tot9) and np.array_equal(gused, used9)
     print("q21", flush=True)
     n = int(h["s_suppkey"].max()) + 1
     w21 = oracle.q21(h, h["status_dict"].index("F"), 7, n)

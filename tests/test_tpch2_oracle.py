@@ -214,11 +214,12 @@ def test_planted_mistakes_are_caught(mutation):
     lo, hi = T.date_offset("1994-01-01"), T.add_months("1994-01-01", 12)
 
     def run_all():
-        rc = db["rf_dict"].index("R") * len(db["ls_dict"])
+        # R-O alone (span 1): the populated R-F code follows the window
+        rc = db["rf_dict"].index("R") * len(db["ls_dict"]) + db["ls_dict"].index("O")
         inreg = (np.array(r["n_regionkey"]) == 2).astype(np.uint8)
         m = [db["shipmode_dict"].index(x) for x in ("MAIL", "SHIP")]
         return (oracle.q4(db, lo, hi, 5).tolist(), oracle.q5(db, inreg, lo, hi).tolist(),
-                oracle.q10(db, rc, len(db["ls_dict"]), lo, hi, 25).tolist(),
+                oracle.q10(db, rc, 1, lo, hi, 25).tolist(),
                 oracle.q12(db, m[0], m[1], lo, hi, db["priority_dict"].index("1-URGENT"),
                            db["priority_dict"].index("2-HIGH")),
                 oracle.q19(db, q19_groups(db, ("Brand#12", "Brand#23", "Brand#34"),
@@ -300,3 +301,103 @@ def test_planted_mistakes_q21_q22_are_caught(mutation):
     finally:
         oracle.lib().oracle_set_mutation(0)
     assert bad[mutation - 20] != good[mutation - 20]
+
+
+def year_of(off):
+    import datetime
+    return (T.BASE + datetime.timedelta(days=int(off))).year
+
+
+@pytest.mark.parametrize("n1,n2", [(6, 7), (7, 6), (12, 18), (3, 3)])
+def test_q7_matches_pandas(data, n1, n2):
+    r, db = data
+    lo, hi, split = T.date_offset("1995-01-01"), T.date_offset("1996-12-31"), \
+        T.date_offset("1996-01-01")
+    got = oracle.q7(db, n1, n2, lo, hi, split)
+    li = lineitem_df(r)
+    li["rev"] = r["l_extendedprice"] * (100 - r["l_discount"]) // 100
+    li = li[(li.l_shipdate >= lo) & (li.l_shipdate <= hi)]
+    s = pd.DataFrame({"l_suppkey": r["s_suppkey"], "sn": r["s_nationkey"]})
+    o = orders_df(r).merge(pd.DataFrame({"o_custkey": r["c_custkey"], "cn": r["c_nationkey"]}),
+                           on="o_custkey")
+    j = li.merge(s, on="l_suppkey").merge(o, left_on="l_orderkey", right_on="o_orderkey")
+    j = j[((j.sn == n1) & (j.cn == n2)) | ((j.sn == n2) & (j.cn == n1))]
+    j["y"] = (j.l_shipdate >= split).astype(int)
+    want = j.groupby(["sn", "cn", "y"]).rev.sum()
+    idx = {n1: 0, n2: 1} if n1 != n2 else {n1: 0}
+    gotd = {(a, b, y): int(got[idx[a], idx[b], y]) for a in idx for b in idx for y in (0, 1)
+            if got[idx[a], idx[b], y]}
+    assert gotd == {k: int(v) for k, v in want.items()}
+    assert len(want) > 0
+
+
+@pytest.mark.parametrize("nation,region,ptype", [(8, 2, "ECONOMY ANODIZED STEEL"),
+                                                 (3, 1, "PROMO BRUSHED TIN")])
+def test_q8_matches_pandas(data, nation, region, ptype):
+    r, db = data
+    lo, hi, split = T.date_offset("1995-01-01"), T.date_offset("1996-12-31"), \
+        T.date_offset("1996-01-01")
+    inreg = (np.array(r["n_regionkey"]) == region).astype(np.uint8)
+    tc = db["type_dict"].index(ptype)
+    tot, nat = oracle.q8(db, inreg, nation, tc, lo, hi, split)
+    li = lineitem_df(r)
+    li["vol"] = r["l_extendedprice"] * (100 - r["l_discount"])
+    o = orders_df(r)
+    o = o[(o.o_orderdate >= lo) & (o.o_orderdate <= hi)]
+    c = pd.DataFrame({"o_custkey": r["c_custkey"], "cn": r["c_nationkey"]})
+    c = c[inreg[c.cn.to_numpy()] == 1]
+    p = pd.DataFrame({"l_partkey": r["p_partkey"], "t": r["p_type"]})
+    p = p[p.t == T.TYPES.index(ptype)]
+    s = pd.DataFrame({"l_suppkey": r["s_suppkey"], "sn": r["s_nationkey"]})
+    j = li.merge(p, on="l_partkey").merge(o.merge(c, on="o_custkey"), left_on="l_orderkey",
+                                          right_on="o_orderkey").merge(s, on="l_suppkey")
+    j["y"] = (j.o_orderdate >= split).astype(int)
+    assert tot.tolist() == [int(j[j.y == y].vol.sum()) for y in (0, 1)]
+    assert nat.tolist() == [int(j[(j.y == y) & (j.sn == nation)].vol.sum()) for y in (0, 1)]
+    assert tot.sum() > 0
+
+
+@pytest.mark.parametrize("color", ["green", "puff", "tan"])
+def test_q9_matches_pandas(data, color):
+    r, db = data
+    ci = T.COLORS.index(color)
+    p_ok = (r["p_name_words"] == ci).any(axis=1)
+    y0, tot, used = oracle.q9(db, p_ok, 25)
+    li = lineitem_df(r)
+    li["rev"] = r["l_extendedprice"] * (100 - r["l_discount"]) // 100
+    li = li[p_ok[li.l_partkey.to_numpy() - 1]]
+    ps = pd.DataFrame({"l_partkey": r["ps_partkey"], "l_suppkey": r["ps_suppkey"],
+                       "cost": r["ps_supplycost"]})
+    s = pd.DataFrame({"l_suppkey": r["s_suppkey"], "sn": r["s_nationkey"]})
+    o = orders_df(r)
+    j = li.merge(ps, on=["l_partkey", "l_suppkey"]).merge(s, on="l_suppkey").merge(
+        o, left_on="l_orderkey", right_on="o_orderkey")
+    j["y"] = [year_of(d) for d in j.o_orderdate]
+    j["profit"] = j.rev - j.cost * j.l_quantity // 100
+    want = j.groupby(["sn", "y"]).profit.sum()
+    gotd = {(n, y0 + k): int(tot[n, k]) for n in range(25) for k in range(tot.shape[1])
+            if used[n, k]}
+    assert gotd == {k: int(v) for k, v in want.items()}
+    assert len(want) > 10
+
+
+@pytest.mark.parametrize("mutation", [22, 23, 24])
+def test_planted_mistakes_q7_q8_q9_are_caught(mutation):
+    r = T.raw(19, 40_000, parts=400, custs=300, supps=40, strings=False)
+    db = T.build(r)
+    lo, hi, split = T.date_offset("1995-01-01"), T.date_offset("1996-12-31"), \
+        T.date_offset("1996-01-01")
+
+    def run_all():
+        inreg = (np.array(r["n_regionkey"]) == 2).astype(np.uint8)
+        p_ok = (r["p_name_words"] == 3).any(axis=1)
+        return (oracle.q7(db, 6, 7, lo, hi, split).tolist(),
+                [x.tolist() for x in oracle.q8(db, inreg, 8, 0, lo, hi, split)],
+                [x.tolist() if hasattr(x, "tolist") else x for x in oracle.q9(db, p_ok, 25)])
+    good = run_all()
+    oracle.lib().oracle_set_mutation(mutation)
+    try:
+        bad = run_all()
+    finally:
+        oracle.lib().oracle_set_mutation(0)
+    assert bad[mutation - 22] != good[mutation - 22]

@@ -41,6 +41,11 @@ TYPES = [f"{a} {b} {c}" for a in ("STANDARD", "SMALL", "MEDIUM", "LARGE", "ECONO
          for b in ("ANODIZED", "BURNISHED", "PLATED", "POLISHED", "BRUSHED")
          for c in ("TIN", "NICKEL", "BRASS", "STEEL", "COPPER")]
 CURRENT = 1263  # 1995-06-17: returnflag / linestatus cut-over
+# part-name words; no word contains another, so "name contains colour" is
+# "colour is one of the name's words"
+COLORS = ["almond", "azure", "blush", "chiffon", "coral", "cyan", "forest", "ghost", "green",
+          "honeydew", "khaki", "lavender", "magenta", "navy", "olive", "orchid", "peru", "plum",
+          "puff", "sienna", "tan", "thistle", "violet", "wheat"]
 
 
 def okeys(n: int) -> np.ndarray:
@@ -84,6 +89,13 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["p_container"] = rng.integers(0, len(CONTAINERS), parts)
     r["p_size"] = rng.integers(1, 51, parts).astype(np.int32)
     r["p_type"] = rng.integers(0, len(TYPES), parts)
+    r["p_name_words"] = rng.integers(0, len(COLORS), (parts, 5))
+    # partsupp: four distinct suppliers per part (spread over the suppliers)
+    pk = np.repeat(r["p_partkey"].astype(np.int64), 4)
+    j = np.tile(np.arange(4), parts)
+    r["ps_partkey"] = pk.astype(np.int32)
+    r["ps_suppkey"] = ((pk + j * max(1, supps // 4)) % supps + 1).astype(np.int32)
+    r["ps_supplycost"] = rng.integers(100, 100_001, 4 * parts).astype(np.int32)
     # orders (created in key order)
     r["o_orderkey"] = okeys(orders)
     # as in TPC-H, no order has a customer key divisible by 3
@@ -100,7 +112,9 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     oi = np.repeat(np.arange(orders), per)
     r["l_orderkey"] = r["o_orderkey"][oi]
     r["l_partkey"] = rng.integers(1, parts + 1, n).astype(np.int32)
-    r["l_suppkey"] = rng.integers(1, supps + 1, n).astype(np.int32)
+    # each lineitem's supplier is one of its part's four (partsupp)
+    r["l_suppkey"] = r["ps_suppkey"][(r["l_partkey"].astype(np.int64) - 1) * 4 +
+                                     rng.integers(0, 4, n)]
     qty = rng.integers(1, 51, n)
     r["l_quantity"] = (qty * 100).astype(np.int64)                       # hundredths
     r["l_extendedprice"] = (qty * rng.integers(90_000, 200_001, n) // 100).astype(np.int64)
@@ -123,6 +137,7 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["l_shipmode"] = rng.integers(0, 7, n)
     r["rf_values"], r["ls_values"] = ["R", "A", "N"], ["O", "F"]
     if strings:
+        r["p_name"] = [" ".join(COLORS[w] for w in row) for row in r["p_name_words"].tolist()]
         r["c_name"] = [f"Customer#{k:09d}" for k in r["c_custkey"]]
         if custs > 1:
             r["c_name"][0], r["c_name"][1] = 'He said "hi"', "plain,comma"
@@ -234,6 +249,11 @@ def build(r: dict) -> dict:
     db["p_brand"], db["brand_dict"] = first_appearance(r["p_brand"], BRANDS)
     db["p_container"], db["container_dict"] = first_appearance(r["p_container"], CONTAINERS)
     db["p_size"] = r["p_size"].astype(np.int32)
+    db["p_type"], db["type_dict"] = first_appearance(r["p_type"], TYPES)
+    db["p_name_words"] = r["p_name_words"]
+    db["ps_partkey"] = r["ps_partkey"]
+    db["ps_suppkey"] = r["ps_suppkey"]
+    db["ps_supplycost"] = r["ps_supplycost"]
     db["n_nationkey"] = r["n_nationkey"]
     db["n_name"] = r["n_name"]
     db["n_regionkey"] = r["n_regionkey"]
@@ -277,13 +297,15 @@ def parquet(r: dict) -> dict:
     part = pa.table({
         "p_partkey": i32(r["p_partkey"]), "p_brand": strs(r["p_brand"], BRANDS),
         "p_container": strs(r["p_container"], CONTAINERS), "p_size": i32(r["p_size"]),
-        "p_type": strs(r["p_type"], TYPES)})
+        "p_type": strs(r["p_type"], TYPES), "p_name": pa.array(r["p_name"])})
+    partsupp = pa.table({"ps_partkey": i32(r["ps_partkey"]), "ps_suppkey": i32(r["ps_suppkey"]),
+                         "ps_supplycost": dec(r["ps_supplycost"])})
     supp = pa.table({"s_suppkey": i32(r["s_suppkey"]), "s_nationkey": i32(r["s_nationkey"]),
                      "s_name": pa.array(r["s_name"])})
     nation = pa.table({"n_nationkey": i32(r["n_nationkey"]), "n_name": pa.array(r["n_name"]),
                        "n_regionkey": i32(r["n_regionkey"])})
     region = pa.table({"r_regionkey": i32(r["r_regionkey"]), "r_name": pa.array(r["r_name"])})
-    return {"lineitem": li, "orders": orders, "customer": cust, "part": part,
+    return {"lineitem": li, "orders": orders, "customer": cust, "part": part, "partsupp": partsupp,
             "supplier": supp, "nation": nation, "region": region}
 
 
