@@ -1,5 +1,5 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q3, Q4, Q5, Q6, Q10, Q12, Q14, Q17, Q18, Q19, Q21 and Q22, with the kernels
+// TPC-H Q1, Q3-Q10, Q12, Q14, Q17-Q19, Q21 and Q22, with the kernels
 // of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
@@ -19,7 +19,8 @@
 //        supplier and customer nations denormalised (:536-544, 838-928).
 // query  reads one request per line until an empty line (query_impl.cpp:142-157):
 //        `1 DELTA`, `3 SEGMENT DATE`, `4 DATE`, `5 REGION DATE`,
-//        `6 DATE DISCOUNT QUANTITY`, `10 DATE`, `12 MODE1 MODE2 DATE`,
+//        `6 DATE DISCOUNT QUANTITY`, `7 NATION1 NATION2`, `8 NATION REGION TYPE`,
+//        `9 COLOR`, `10 DATE`, `12 MODE1 MODE2 DATE`,
 //        `14 DATE`, `17 BRAND CONTAINER`, `18 QUANTITY`,
 //        `19 Q1 Q2 Q3 BRAND1 BRAND2 BRAND3`, `21 NATION`, `22 I1 .. I7`
 //        with std::quoted tokens
@@ -247,7 +248,29 @@ struct Part {
     CodeCol brand, container;
     uint8_t* promo = nullptr;
     std::vector<std::string> brands, containers;
+    uint8_t* type = nullptr;                  // p_type codes
+    std::vector<std::string> types, name;
+    int32_t max_key = 0;                      // max p_partkey (0 for none, as the queries)
 };
+
+// partsupp rows (builder_impl.cpp:753-777 keeps them for Q2/Q9/Q11/Q16/Q20).
+struct Partsupp {
+    int64_t rows = -1;
+    int32_t *partkey = nullptr, *suppkey = nullptr, *supplycost = nullptr;
+    int32_t max_partkey = 0;
+};
+
+Partsupp build_partsupp(const arrow::Table& t) {
+    Partsupp ps;
+    ps.rows = t.num_rows();
+    const auto pk = int32s(*column(t, "ps_partkey"));
+    for (int32_t k : pk) ps.max_partkey = std::max(ps.max_partkey, k);
+    ps.partkey = upload(pk);
+    ps.suppkey = upload(int32s(*column(t, "ps_suppkey")));
+    const auto cost = scaled(*column(t, "ps_supplycost"), 100);
+    ps.supplycost = upload(std::vector<int32_t>(cost.begin(), cost.end()));
+    return ps;
+}
 
 // Orders sorted by orderdate (stable), as build() keeps them (:597-608).
 struct Orders {
@@ -307,6 +330,7 @@ struct Nation {
     std::vector<std::string> name;
     std::unordered_map<std::string, int32_t> region_by_name;  // region.name_to_key
     std::unordered_map<std::string, int32_t> key_by_name;     // nation.name_to_key
+    std::unordered_map<int32_t, size_t> row_by_key;           // nation.nationkey_to_row
     int32_t max_key = 0;                                       // at least 0, as the queries
 };
 
@@ -402,6 +426,13 @@ Part build_part(const arrow::Table& t) {
     p.brands = brand.values;
     p.containers = container.values;
     if (t.GetColumnByName("p_size")) p.size = upload(int32s(*column(t, "p_size")));
+    if (t.GetColumnByName("p_type")) {
+        Dict d = dictionary(*column(t, "p_type"));
+        p.type = upload(narrow_codes(d, "p_type"));
+        p.types = d.values;
+    }
+    p.name = strings_or_empty(t, "p_name");
+    for (int32_t k : int32s(*column(t, "p_partkey"))) p.max_key = std::max(p.max_key, k);
     return p;
 }
 
@@ -514,7 +545,10 @@ Nation build_nation(const arrow::Table& n, const arrow::Table* r) {
     x.region = int32s(*column(n, "n_regionkey"));
     x.name = strings(*column(n, "n_name"));
     for (int32_t k : x.key) x.max_key = std::max(x.max_key, k);
-    for (size_t i = 0; i < x.key.size(); ++i) x.key_by_name.emplace(x.name[i], x.key[i]);
+    for (size_t i = 0; i < x.key.size(); ++i) {
+        x.key_by_name.emplace(x.name[i], x.key[i]);
+        x.row_by_key.emplace(x.key[i], i);
+    }
     if (r) {
         const auto rk = int32s(*column(*r, "r_regionkey"));
         const auto rn = strings(*column(*r, "r_name"));
@@ -1034,6 +1068,184 @@ Writer run_q19(bolap_ctx* ctx, const Lineitem& li, const Part& part, const std::
     };
 }
 
+// Nation names by key (the queries' nation_name vectors: max key + 1
+// entries, empty for keys without a row).
+std::vector<std::string> nation_names(const Nation& nat) {
+    std::vector<std::string> names(static_cast<size_t>(nat.max_key + 1));
+    for (size_t i = 0; i < nat.key.size(); ++i)
+        if (nat.key[i] >= 0) names[size_t(nat.key[i])] = nat.name[i];
+    return names;
+}
+
+Writer run_q7(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Supplier& sup,
+              const Nation& nat, const std::string& line) {
+    need(o.rows >= 0 && sup.rows >= 0 && li.suppkey, "Q7 needs orders, customer and supplier");
+    const auto v = quoted_args(line, 2, "Q7");
+    struct Row {
+        std::string supp, cust;
+        int32_t year;
+        int64_t rev;
+    };
+    std::vector<Row> rows;
+    const auto a1 = nat.key_by_name.find(v[0]), a2 = nat.key_by_name.find(v[1]);
+    if (a1 != nat.key_by_name.end() && a2 != nat.key_by_name.end()) {  // query_q7.cpp:148-155
+        bolap_q7_args a{};
+        a.orders = order_side(li, o);
+        a.o_cust_nation = o.cust_nation;
+        a.l_shipdate = li.ship;
+        a.l_suppkey = li.suppkey;
+        a.l_disc_price = li.disc_price;
+        a.supp_rows = sup.rows;
+        a.s_suppkey = sup.suppkey;
+        a.s_nationkey = sup.nationkey;
+        a.n_suppkeys = int64_t(std::max(sup.max_key, 0)) + 1;
+        a.nation1 = a1->second;
+        a.nation2 = a2->second;
+        a.ship_lo = parse_date_offset("1995-01-01");
+        a.ship_hi = parse_date_offset("1996-12-31");
+        a.split = parse_date_offset("1996-01-01");
+        int64_t rev[8];
+        check(bolap_q7(ctx, &a, rev));
+        const int32_t keys[2] = {a.nation1, a.nation2};
+        const int n = a.nation1 == a.nation2 ? 1 : 2;
+        for (int si = 0; si < n; ++si)  // :290-320
+            for (int ci = 0; ci < n; ++ci) {
+                if (n == 2 && si == ci) continue;
+                const auto sr = nat.row_by_key.find(keys[si]), cr = nat.row_by_key.find(keys[ci]);
+                if (sr == nat.row_by_key.end() || cr == nat.row_by_key.end()) continue;
+                for (int y = 0; y < 2; ++y)
+                    if (rev[si * 4 + ci * 2 + y] != 0)
+                        rows.push_back({nat.name[sr->second], nat.name[cr->second], 1995 + y,
+                                        rev[si * 4 + ci * 2 + y]});
+            }
+        std::sort(rows.begin(), rows.end(), [](const Row& x, const Row& y) {
+            if (x.supp != y.supp) return x.supp < y.supp;
+            if (x.cust != y.cust) return x.cust < y.cust;
+            return x.year < y.year;
+        });
+    }
+    return [rows](const std::string& csv) {  // write_q7_csv
+        std::ofstream out(csv);
+        out << "supp_nation,cust_nation,l_year,revenue\n";
+        for (const auto& r : rows)
+            out << quoted_field(r.supp) << ',' << quoted_field(r.cust) << ',' << r.year << ','
+                << fixed_double(double(r.rev) / 100.0) << "\n";
+    };
+}
+
+Writer run_q8(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Customer& c,
+              const Supplier& sup, const Part& part, const Nation& nat, const std::string& line) {
+    need(o.rows >= 0 && c.rows >= 0 && c.nationkey && sup.rows >= 0 && part.rows >= 0 &&
+             part.type && li.suppkey,
+         "Q8 needs orders, customer, supplier, part (with p_type) and nation");
+    const auto v = quoted_args(line, 3, "Q8");  // NATION REGION TYPE
+    std::vector<std::pair<int32_t, double>> rows;
+    const auto reg = nat.region_by_name.find(v[1]);
+    const auto tn = nat.key_by_name.find(v[0]);
+    if (reg != nat.region_by_name.end() && tn != nat.key_by_name.end()) {  // query_q8.cpp:135-146
+        const int32_t n = nat.max_key + 1;
+        std::vector<uint8_t> in_region(static_cast<size_t>(n), 0);
+        for (size_t i = 0; i < nat.key.size(); ++i)
+            if (nat.key[i] >= 0 && nat.region[i] == reg->second) in_region[size_t(nat.key[i])] = 1;
+        bolap_q8_args a{};
+        a.orders = order_side(li, o);
+        a.o_custkey = o.custkey;
+        a.l_partkey = li.partkey;
+        a.l_suppkey = li.suppkey;
+        a.l_price = li.price;
+        a.l_disc = li.disc;
+        a.cust_rows = c.rows;
+        a.c_custkey = c.custkey;
+        a.c_nationkey = c.nationkey;
+        a.n_cust_keys = int64_t(c.nation_by_key.size());  // max custkey + 1
+        a.nation_in_region = in_region.data();
+        a.n_nation = n;
+        a.supp_rows = sup.rows;
+        a.s_suppkey = sup.suppkey;
+        a.s_nationkey = sup.nationkey;
+        a.n_supp_keys = int64_t(std::max(sup.max_key, 0)) + 1;
+        a.target_nation = tn->second;
+        a.part_rows = part.rows;
+        a.p_partkey = part.partkey;
+        a.p_type = part.type;
+        a.n_part_keys = int64_t(part.max_key) + 1;
+        a.type_code = code_of(part.types, v[2]);
+        a.date_lo = parse_date_offset("1995-01-01");
+        a.date_hi = parse_date_offset("1996-12-31");
+        a.split = parse_date_offset("1996-01-01");
+        int64_t total[2], share[2];
+        check(bolap_q8(ctx, &a, total, share));
+        for (int y = 0; y < 2; ++y)  // :330-345
+            if (total[y] != 0) rows.emplace_back(1995 + y, double(share[y]) / double(total[y]));
+    }
+    return [rows](const std::string& csv) {  // write_q8_csv
+        std::ofstream out(csv);
+        out << "o_year,mkt_share\n";
+        for (const auto& [y, m] : rows) out << y << ',' << fixed_double(m) << "\n";
+    };
+}
+
+Writer run_q9(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Part& part,
+              const Partsupp& ps, const Nation& nat, const std::string& line) {
+    need(o.rows >= 0 && part.rows >= 0 && ps.rows >= 0 && li.suppkey && li.supp_nation,
+         "Q9 needs orders, part, partsupp, supplier and nation");
+    const auto v = quoted_args(line, 1, "Q9");
+    // parts whose name contains the colour (query_q9.cpp:160-172)
+    std::vector<uint8_t> ok(part.name.size());
+    for (size_t i = 0; i < ok.size(); ++i)
+        ok[i] = part.name[i].find(v[0]) != std::string::npos ? 1 : 0;
+    uint8_t* ok_dev = upload(ok);
+    const auto names = nation_names(nat);
+    bolap_q9_args a{};
+    a.part_rows = part.rows;
+    a.p_partkey = part.partkey;
+    a.p_ok = ok_dev;
+    a.n_part_keys = int64_t(std::max(part.max_key, ps.max_partkey)) + 1;
+    a.ps_rows = ps.rows;
+    a.ps_partkey = ps.partkey;
+    a.ps_suppkey = ps.suppkey;
+    a.ps_supplycost = ps.supplycost;
+    a.order_rows = o.rows;
+    a.o_orderdate = o.orderdate;
+    a.order_map = o.order_map;
+    a.map_size = o.map_size;
+    a.li_rows = li.rows;
+    a.l_partkey = li.partkey;
+    a.l_suppkey = li.suppkey;
+    a.l_orderkey = li.orderkey;
+    a.l_quantity = li.qty;
+    a.l_disc_price = li.disc_price;
+    a.l_supp_nation = li.supp_nation;
+    a.n_nation = int32_t(names.size());
+    std::vector<int64_t> totals(2048);
+    std::vector<uint8_t> used(2048);
+    int32_t y0 = 0, span = 0;
+    check(bolap_q9(ctx, &a, totals.data(), used.data(), &y0, &span));
+    hip_check(hipFree(ok_dev), "hipFree");
+    struct Row {
+        std::string nation;
+        int32_t year;
+        int64_t profit;
+    };
+    std::vector<Row> rows;
+    for (size_t k = 0; k < names.size(); ++k) {  // :368-388
+        if (names[k].empty()) continue;
+        for (int32_t y = 0; y < span; ++y)
+            if (used[k * size_t(span) + size_t(y)])
+                rows.push_back({names[k], y0 + y, totals[k * size_t(span) + size_t(y)]});
+    }
+    std::sort(rows.begin(), rows.end(), [](const Row& x, const Row& y) {
+        return x.nation != y.nation ? x.nation < y.nation : x.year > y.year;
+    });
+    return [rows](const std::string& csv) {  // write_q9_csv
+        std::ofstream out(csv);
+        out << "nation,o_year,sum_profit\n";
+        for (const auto& r : rows)
+            out << quoted_field(r.nation) << ',' << r.year << ','
+                << fixed_double(double(r.profit) / 100.0) << "\n";
+    };
+}
+
 Writer run_q21(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Supplier& sup,
                const Nation& nat, const std::string& line) {
     need(o.rows >= 0 && o.status && sup.rows >= 0 && li.suppkey && li.commit_receipt,
@@ -1186,6 +1398,7 @@ int main(int argc, char** argv) {
         const auto t_nation = load_if("nation"), t_region = load_if("region");
         const auto t_supplier = load_if("supplier"), t_customer = load_if("customer");
         const auto t_part = load_if("part"), t_orders = load_if("orders");
+        const auto t_partsupp = load_if("partsupp");
         Nation nation;
         if (t_nation) nation = build_nation(*t_nation, t_region.get());
         std::vector<int16_t> supp_nation;
@@ -1200,6 +1413,8 @@ int main(int argc, char** argv) {
         const Lineitem li = build(ctx, *read_parquet(dir + "lineitem.parquet"), supp_nation);
         Part part;
         if (t_part) part = build_part(*t_part);
+        Partsupp partsupp;
+        if (t_partsupp) partsupp = build_partsupp(*t_partsupp);
         Orders orders;
         if (t_orders) orders = build_orders(ctx, *t_orders, li, customer.nation_by_key);
         std::vector<std::pair<std::string, std::string>> requests;   // query
@@ -1230,6 +1445,12 @@ int main(int argc, char** argv) {
                 write = run_q12(ctx, li, orders, text);
             } else if (id == "19") {
                 write = run_q19(ctx, li, part, text);
+            } else if (id == "7") {
+                write = run_q7(ctx, li, orders, supplier, nation, text);
+            } else if (id == "8") {
+                write = run_q8(ctx, li, orders, customer, supplier, part, nation, text);
+            } else if (id == "9") {
+                write = run_q9(ctx, li, orders, part, partsupp, nation, text);
             } else if (id == "21") {
                 write = run_q21(ctx, li, orders, supplier, nation, text);
             } else if (id == "22") {

@@ -431,3 +431,80 @@ def test_driver_q21_q22(tmp_path, orders):
         assert (tmp_path / "run" / f"result{i + 1}.csv").read_text() == w, requests[i]
     if orders > 3:
         assert want[0].count("\n") > 1 and want[4].count("\n") > 3 and want[5].count("\n") > 2
+
+
+Q7_DATES = None
+
+
+def q7_dates():
+    from tests import tpch_data as T
+    return (T.date_offset("1995-01-01"), T.date_offset("1996-12-31"),
+            T.date_offset("1996-01-01"))
+
+
+def csv_q7(db, r, a, b):
+    import oracle
+    out = "supp_nation,cust_nation,l_year,revenue\n"
+    if a not in r["n_name"] or b not in r["n_name"]:
+        return out
+    n1, n2 = r["n_name"].index(a), r["n_name"].index(b)
+    rev = oracle.q7(db, n1, n2, *q7_dates())
+    keys = [n1, n2] if n1 != n2 else [n1]
+    rows = []
+    for si, sk in enumerate(keys):
+        for ci, ck in enumerate(keys):
+            if len(keys) == 2 and si == ci:
+                continue
+            for y in (0, 1):
+                if rev[si, ci, y]:
+                    rows.append((r["n_name"][sk], r["n_name"][ck], 1995 + y, int(rev[si, ci, y])))
+    return out + "".join(f"{esc(s)},{esc(c)},{y},{v / 100.0:.2f}\n" for s, c, y, v in sorted(rows))
+
+
+def csv_q8(db, r, nation, region, ptype):
+    import oracle
+    out = "o_year,mkt_share\n"
+    if nation not in r["n_name"] or region not in r["r_name"]:
+        return out
+    inreg = (np.array(r["n_regionkey"]) == r["r_name"].index(region)).astype(np.uint8)
+    tc = idx(db["type_dict"], ptype)
+    tot, nat = oracle.q8(db, inreg, r["n_name"].index(nation), tc, *q7_dates())
+    return out + "".join(f"{1995 + y},{nat[y] / tot[y]:.2f}\n" for y in (0, 1) if tot[y])
+
+
+def csv_q9(db, r, color):
+    import oracle
+    p_ok = np.array([color in name for name in r["p_name"]], np.uint8)
+    y0, tot, used = oracle.q9(db, p_ok, 25)
+    rows = [(r["n_name"][k], -(y0 + y), int(tot[k, y])) for k in range(25)
+            for y in range(tot.shape[1]) if used[k, y]]
+    return "nation,o_year,sum_profit\n" + "".join(
+        f"{esc(n)},{-y},{v / 100.0:.2f}\n" for n, y, v in sorted(rows))
+
+
+@pytest.mark.parametrize("orders", [3, 40_001])
+def test_driver_q7_q8_q9(tmp_path, orders):
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    from tests import tpch_data as T
+    r = T.raw(29 + orders, orders, parts=900, custs=600, supps=60)
+    write(T.parquet(r), tmp_path / "data")
+    db = T.build(r)
+    requests = ['7 "FRANCE" "GERMANY"', '7 GERMANY FRANCE', '7 "JAPAN" "JAPAN"',
+                '7 "ATLANTIS" "FRANCE"', '8 "BRAZIL" "AMERICA" "ECONOMY ANODIZED STEEL"',
+                '8 "CHINA" "ASIA" "PROMO BRUSHED TIN"', '8 "CHINA" "ASIA" "NO SUCH TYPE"',
+                '8 "CHINA" "NOWHERE" "PROMO BRUSHED TIN"', '9 "green"', '9 "puff"', '9 "zzz"']
+    res = run_driver(tmp_path / "data", tmp_path / "run", requests)
+    assert res.returncode == 0, res.stderr
+    want = [csv_q7(db, r, "FRANCE", "GERMANY"), csv_q7(db, r, "GERMANY", "FRANCE"),
+            csv_q7(db, r, "JAPAN", "JAPAN"), csv_q7(db, r, "ATLANTIS", "FRANCE"),
+            csv_q8(db, r, "BRAZIL", "AMERICA", "ECONOMY ANODIZED STEEL"),
+            csv_q8(db, r, "CHINA", "ASIA", "PROMO BRUSHED TIN"),
+            csv_q8(db, r, "CHINA", "ASIA", "NO SUCH TYPE"),
+            csv_q8(db, r, "CHINA", "NOWHERE", "PROMO BRUSHED TIN"),
+            csv_q9(db, r, "green"), csv_q9(db, r, "puff"), csv_q9(db, r, "zzz")]
+    for i, w in enumerate(want):
+        assert (tmp_path / "run" / f"result{i + 1}.csv").read_text() == w, requests[i]
+    if orders > 3:
+        assert want[0].count("\n") == 5 and want[4].count("\n") == 3
+        assert want[8].count("\n") > 100
