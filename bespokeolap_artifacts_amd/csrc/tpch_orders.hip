@@ -1491,3 +1491,330 @@ extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals,
     BOLAP_TRY(fetch(ctx, totals, tot, g * 8));
     return fetch(ctx, used, usd, g);
 }
+
+// ---------------------------------------------------------------------------
+// Q13 (string prefilters and substring search) and Q16 (count distinct).
+// ---------------------------------------------------------------------------
+namespace bolap {
+namespace {
+
+// The builder's per-string prefilter masks (builder_impl.cpp:124-150): a bit
+// per letter a-z (either case) and 64 bits of (c0 * 131 + c1) & 63 over the
+// byte pairs.
+BOLAP_DEV uint32_t alpha_bits(uint8_t c) {
+    if (c >= 'a' && c <= 'z') return 1u << (c - 'a');
+    if (c >= 'A' && c <= 'Z') return 1u << (c - 'A');
+    return 0;
+}
+
+__global__ __launch_bounds__(kT) void str_masks_k(const uint8_t* __restrict__ chars,
+                                                  const uint32_t* __restrict__ off, int64_t n,
+                                                  uint32_t* __restrict__ alpha,
+                                                  uint64_t* __restrict__ bigram) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        const uint32_t s = off[i], e = off[i + 1];
+        uint32_t a = 0;
+        uint64_t b = 0;
+        uint32_t prev = 0;
+        for (uint32_t p = s; p < e; ++p) {
+            const uint32_t c = chars[p];
+            a |= alpha_bits(uint8_t(c));
+            if (p > s) b |= uint64_t(1) << ((prev * 131u + c) & 63u);
+            prev = c;
+        }
+        alpha[i] = a;
+        bigram[i] = b;
+    }
+}
+
+struct Q13W {
+    uint8_t w1[64], w2[64];
+    int32_t n1, n2;
+    uint32_t req_alpha;
+    uint64_t req_bigram;
+};
+
+// First position p in [s, e) where w occurs (s itself for an empty word),
+// e when none (find_substring, query_q13.cpp:27-49).
+BOLAP_DEV uint32_t find_word(const uint8_t* __restrict__ chars, uint32_t s, uint32_t e,
+                             const uint8_t* w, int32_t n) {
+    if (n == 0) return s;
+    for (uint32_t p = s; p + uint32_t(n) <= e; ++p) {
+        if (chars[p] != w[0]) continue;
+        int32_t j = 1;
+        while (j < n && chars[p + j] == w[j]) ++j;
+        if (j == n) return p;
+    }
+    return e;
+}
+
+__global__ __launch_bounds__(kT) void q13_k(Q13W w, int64_t rows, const int32_t* __restrict__ ock,
+                                            const uint8_t* __restrict__ chars,
+                                            const uint32_t* __restrict__ off,
+                                            const uint32_t* __restrict__ alpha,
+                                            const uint64_t* __restrict__ bigram, int64_t size,
+                                            int32_t* __restrict__ cnt) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        bool match = false;
+        const bool cand = (!alpha || (alpha[i] & w.req_alpha) == w.req_alpha) &&
+                          (!bigram || (bigram[i] & w.req_bigram) == w.req_bigram);
+        if (cand) {
+            const uint32_t s = off[i], e = off[i + 1];
+            const uint32_t p1 = find_word(chars, s, e, w.w1, w.n1);
+            match = p1 != e && find_word(chars, p1 + uint32_t(w.n1), e, w.w2, w.n2) != e;
+        }
+        if (match) continue;
+        const int32_t k = ock[i];
+        if (k >= 0 && k < size) atomicAdd(cnt + k, 1);
+    }
+}
+
+__global__ __launch_bounds__(kT) void max_i32_arr_k(const int32_t* __restrict__ v, int64_t n,
+                                                    int32_t* __restrict__ out) {
+    int32_t m = 0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        m = v[i] > m ? v[i] : m;
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const int32_t o = __shfl_xor(m, d, kWave);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out, m);
+}
+
+constexpr int kDistLds = 4096;
+
+__global__ __launch_bounds__(kT) void q13_dist_k(const int32_t* __restrict__ ck, int64_t rows,
+                                                 const int32_t* __restrict__ cnt, int64_t size,
+                                                 int32_t n_dist,
+                                                 unsigned long long* __restrict__ dist) {
+    __shared__ unsigned s_d[kDistLds];
+    const bool lds = n_dist <= kDistLds;
+    if (lds)
+        for (int i = threadIdx.x; i < n_dist; i += kT) s_d[i] = 0;
+    __syncthreads();
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = ck[i];
+        const int32_t c = (k >= 0 && k < size) ? cnt[k] : 0;
+        if (c >= n_dist) continue;
+        if (lds) atomicAdd(&s_d[c], 1u);
+        else atomicAdd(dist + c, 1ull);
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < n_dist; i += kT)
+            if (s_d[i]) atomicAdd(dist + i, (unsigned long long)s_d[i]);
+}
+
+// Q16: part key -> (row + 1) << 32 | group of the last eligible part row.
+struct Q16P {
+    int64_t rows;
+    const int32_t* pk;
+    const uint8_t* brand;
+    const uint8_t* type;
+    const int32_t* size;
+    int32_t n_types;
+    int32_t banned_brand;
+    uint64_t size_mask;
+    const uint8_t* type_banned;  // device, n_types bytes
+    int64_t n;
+};
+
+__global__ __launch_bounds__(kT) void q16_part_k(Q16P a, unsigned long long* __restrict__ owner) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t b = a.brand[i];
+        if (b == a.banned_brand) continue;
+        const int32_t sz = a.size[i];
+        if (sz < 0 || sz > 50 || !((a.size_mask >> sz) & 1)) continue;
+        const int32_t t = a.type[i];
+        if (t >= a.n_types || a.type_banned[t]) continue;
+        const int32_t k = a.pk[i];
+        if (k < 0 || k >= a.n) continue;
+        const uint64_t g = (uint64_t(b) * uint64_t(a.n_types) + uint64_t(t)) * 51u + uint64_t(sz);
+        atomicMax(owner + k, (unsigned long long)((uint64_t(i + 1) << 32) | g));
+    }
+}
+
+__global__ __launch_bounds__(kT) void q16_ps_k(int64_t rows, const int32_t* __restrict__ pk,
+                                               const int32_t* __restrict__ sk, int64_t np,
+                                               const unsigned long long* __restrict__ owner,
+                                               const uint8_t* __restrict__ banned, int64_t ns,
+                                               uint64_t* __restrict__ keys,
+                                               unsigned long long* __restrict__ n_out) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int32_t k = pk[i];
+        if (k < 0 || k >= np) continue;
+        const unsigned long long w = owner[k];
+        if (w == 0) continue;
+        const int32_t s = sk[i];
+        if (s >= 0 && s < ns && banned[s]) continue;
+        const uint64_t g = w & 0xffffffffull;
+        keys[atomicAdd(n_out, 1ull)] = (g << 32) | uint32_t(s);
+    }
+}
+
+__global__ __launch_bounds__(kT) void q16_distinct_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                     unsigned long long* __restrict__ counts) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        if (i == 0 || keys[i] != keys[i - 1]) atomicAdd(counts + (keys[i] >> 32), 1ull);
+}
+
+}  // namespace
+}  // namespace bolap
+
+extern "C" int bolap_string_masks(bolap_ctx* ctx, const uint8_t* chars, const uint32_t* offsets,
+                                  int64_t n, uint32_t* alpha_out, uint64_t* bigram_out) {
+    BOLAP_ARG_CHECK(ctx && n >= 0 && (n == 0 || (chars && offsets && alpha_out && bigram_out)),
+                    "bolap_string_masks: null argument");
+    if (n == 0) return BOLAP_OK;
+    hipLaunchKernelGGL(str_masks_k, dim3(grid_of(ctx, n, 8)), dim3(kT), 0, ctx->stream, chars,
+                       offsets, n, alpha_out, bigram_out);
+    return launch_check("str_masks_k");
+}
+
+extern "C" int bolap_q13(bolap_ctx* ctx, const bolap_q13_args* a, int64_t* dist, int32_t max_dist,
+                         int32_t* max_count) {
+    BOLAP_ARG_CHECK(ctx && a && max_count && (dist || max_dist == 0), "bolap_q13: null argument");
+    BOLAP_ARG_CHECK(a->order_rows >= 0 && a->cust_rows >= 0, "bolap_q13: negative rows");
+    BOLAP_ARG_CHECK(a->order_rows == 0 || (a->o_custkey && a->chars && a->offsets),
+                    "bolap_q13: null order column");
+    BOLAP_ARG_CHECK(a->cust_rows == 0 || a->c_custkey, "bolap_q13: null c_custkey");
+    BOLAP_ARG_CHECK(a->word1 && a->word2 && std::strlen(a->word1) < 64 && std::strlen(a->word2) < 64,
+                    "bolap_q13: words must be shorter than 64 bytes");
+    Q13W w{};
+    w.n1 = int32_t(std::strlen(a->word1));
+    w.n2 = int32_t(std::strlen(a->word2));
+    std::memcpy(w.w1, a->word1, size_t(w.n1));
+    std::memcpy(w.w2, a->word2, size_t(w.n2));
+    // required prefilter bits of the two words (query_q13.cpp:166-176)
+    auto host_masks = [](const uint8_t* s, int32_t n, uint32_t& al, uint64_t& bg) {
+        for (int32_t i = 0; i < n; ++i) {
+            const uint8_t c = s[i];
+            if (c >= 'a' && c <= 'z') al |= 1u << (c - 'a');
+            else if (c >= 'A' && c <= 'Z') al |= 1u << (c - 'A');
+            if (i > 0) bg |= uint64_t(1) << ((uint32_t(s[i - 1]) * 131u + c) & 63u);
+        }
+    };
+    host_masks(w.w1, w.n1, w.req_alpha, w.req_bigram);
+    host_masks(w.w2, w.n2, w.req_alpha, w.req_bigram);
+    *max_count = 0;
+    for (int32_t i = 0; i < max_dist; ++i) dist[i] = 0;
+    int32_t* slot = static_cast<int32_t*>(ctx->result_dev);
+    int32_t kmax = 0;
+    if (a->cust_rows > 0) BOLAP_TRY(max_i32(ctx, a->c_custkey, a->cust_rows, slot, &kmax));
+    const int64_t size = kmax >= 0 ? int64_t(kmax) + 1 : 0;
+    WsArena ws;
+    const size_t o_cnt = ws.plan(size_t(size > 0 ? size : 1) * 4);
+    const size_t o_max = ws.plan(8);
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    auto* cnt = ws.at<int32_t>(o_cnt);
+    auto* mx = ws.at<int32_t>(o_max);
+    BOLAP_TRY(zero_dev(ctx, cnt, ws.off));
+    {
+        TraceScope ts(ctx, "q13_orders");
+        if (a->order_rows > 0 && size > 0) {
+            hipLaunchKernelGGL(q13_k, dim3(grid_of(ctx, a->order_rows, 8)), dim3(kT), 0,
+                               ctx->stream, w, a->order_rows, a->o_custkey, a->chars, a->offsets,
+                               a->alpha, a->bigram, size, cnt);
+            hipLaunchKernelGGL(max_i32_arr_k, dim3(grid_of(ctx, size)), dim3(kT), 0, ctx->stream,
+                               cnt, size, mx);
+            BOLAP_TRY(launch_check("q13_k"));
+        }
+    }
+    int32_t m = 0;
+    BOLAP_TRY(fetch(ctx, &m, mx, 4));
+    *max_count = m;
+    if (m + 1 > max_dist) {
+        set_error("bolap_q13: counts up to %d need %d distribution slots", m, m + 1);
+        return BOLAP_ERR_CAPACITY;
+    }
+    WsArena ws2 = ws;
+    const size_t o_dist = ws2.plan(size_t(m + 1) * 8);
+    BOLAP_TRY(ws_reserve(ctx, ws2.off, ws.off));
+    ws2.base = static_cast<char*>(ctx->ws);
+    auto* dd = ws2.at<unsigned long long>(o_dist);
+    BOLAP_TRY(zero_dev(ctx, dd, size_t(m + 1) * 8));
+    if (a->cust_rows > 0) {
+        TraceScope ts(ctx, "q13_dist");
+        hipLaunchKernelGGL(q13_dist_k, dim3(grid_of(ctx, a->cust_rows)), dim3(kT), 0, ctx->stream,
+                           a->c_custkey, a->cust_rows, ws2.at<int32_t>(o_cnt), size, m + 1, dd);
+        BOLAP_TRY(launch_check("q13_dist_k"));
+    }
+    return fetch(ctx, dist, dd, size_t(m + 1) * 8);
+}
+
+extern "C" int bolap_q16(bolap_ctx* ctx, const bolap_q16_args* a, int64_t* counts,
+                         int64_t n_counts) {
+    BOLAP_ARG_CHECK(ctx && a && (counts || n_counts == 0), "bolap_q16: null argument");
+    BOLAP_ARG_CHECK(a->part_rows >= 0 && a->part_rows < (int64_t(1) << 31) && a->ps_rows >= 0 &&
+                        a->supp_rows >= 0,
+                    "bolap_q16: rows out of range");
+    BOLAP_ARG_CHECK(a->n_types > 0 && a->n_types <= 256 && a->n_brands > 0 && a->n_brands <= 256,
+                    "bolap_q16: %d brands / %d types", a->n_brands, a->n_types);
+    BOLAP_ARG_CHECK(n_counts == int64_t(a->n_brands) * a->n_types * 51,
+                    "bolap_q16: counts must hold n_brands * n_types * 51 groups");
+    BOLAP_ARG_CHECK(a->part_rows == 0 || (a->p_partkey && a->p_brand && a->p_type && a->p_size),
+                    "bolap_q16: null part column");
+    BOLAP_ARG_CHECK(a->ps_rows == 0 || (a->ps_partkey && a->ps_suppkey),
+                    "bolap_q16: null partsupp column");
+    BOLAP_ARG_CHECK(a->supp_rows == 0 || (a->s_suppkey && a->s_banned),
+                    "bolap_q16: null supplier column");
+    for (int64_t i = 0; i < n_counts; ++i) counts[i] = 0;
+    if (a->part_rows == 0 || a->ps_rows == 0) return BOLAP_OK;
+    const int64_t np = a->n_part_keys, ns = a->n_supp_keys;
+    WsArena ws;
+    const size_t o_owner = ws.plan(size_t(np > 0 ? np : 1) * 8);
+    const size_t o_ban = ws.plan(size_t(ns > 0 ? ns : 1));
+    const size_t o_cnt = ws.plan(size_t(n_counts) * 8);
+    const size_t o_n = ws.plan(8);
+    const size_t o_zero_end = ws.off;
+    const size_t o_tb = ws.plan(256);
+    const size_t o_keys = ws.plan(size_t(a->ps_rows) * 8);
+    const size_t o_vals = ws.plan(size_t(a->ps_rows) * 4);
+    const size_t o_sort = ws.plan(radix_sort_scratch_bytes(ctx, a->ps_rows));
+    BOLAP_TRY(ws_reserve(ctx, ws.off));
+    ws.base = static_cast<char*>(ctx->ws);
+    BOLAP_TRY(zero_dev(ctx, ws.base, o_zero_end));
+    BOLAP_HIP_TRY(hipMemcpyAsync(ws.at<uint8_t>(o_tb), a->type_banned, size_t(a->n_types),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    auto* owner = ws.at<unsigned long long>(o_owner);
+    auto* nout = ws.at<unsigned long long>(o_n);
+    {
+        TraceScope ts(ctx, "q16_part_supp");
+        const Q16P p{a->part_rows, a->p_partkey, a->p_brand, a->p_type, a->p_size, a->n_types,
+                     a->banned_brand, a->size_mask, ws.at<uint8_t>(o_tb), np};
+        hipLaunchKernelGGL(q16_part_k, dim3(grid_of(ctx, a->part_rows)), dim3(kT), 0, ctx->stream,
+                           p, owner);
+        if (a->supp_rows > 0 && ns > 0)
+            hipLaunchKernelGGL(mark_keys_k, dim3(grid_of(ctx, a->supp_rows)), dim3(kT), 0,
+                               ctx->stream, a->s_suppkey, static_cast<const void*>(a->s_banned),
+                               int32_t(BOLAP_U8), a->supp_rows, ns, 1, 1,
+                               static_cast<const uint8_t*>(nullptr), 0, ws.at<uint8_t>(o_ban));
+        hipLaunchKernelGGL(q16_ps_k, dim3(grid_of(ctx, a->ps_rows, 8)), dim3(kT), 0, ctx->stream,
+                           a->ps_rows, a->ps_partkey, a->ps_suppkey, np, owner,
+                           ws.at<uint8_t>(o_ban), ns, ws.at<uint64_t>(o_keys), nout);
+        BOLAP_TRY(launch_check("q16_ps_k"));
+    }
+    unsigned long long n = 0;
+    BOLAP_TRY(fetch(ctx, &n, nout, 8));
+    auto* cnt = ws.at<unsigned long long>(o_cnt);
+    if (n > 0) {
+        TraceScope ts(ctx, "q16_distinct");
+        int bits = 33;
+        while ((int64_t(1) << (bits - 32)) < n_counts) ++bits;
+        int passes = 0;
+        BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals),
+                                   int64_t(n), bits, ws.at<void>(o_sort), &passes));
+        hipLaunchKernelGGL(q16_distinct_k, dim3(grid_of(ctx, int64_t(n))), dim3(kT), 0,
+                           ctx->stream, ws.at<uint64_t>(o_keys), int64_t(n), cnt);
+        BOLAP_TRY(launch_check("q16_distinct_k"));
+    }
+    return fetch(ctx, counts, cnt, size_t(n_counts) * 8);
+}

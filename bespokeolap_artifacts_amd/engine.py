@@ -681,6 +681,49 @@ class Engine:
         return y0.value, tot[:g].reshape(n_nation, span.value), used[:g].reshape(n_nation,
                                                                                  span.value)
 
+    def string_masks(self, chars, offsets):
+        torch = _torch()
+        n = int(offsets.numel()) - 1
+        alpha = torch.empty(max(1, n), dtype=torch.int32, device=chars.device)
+        bigram = torch.empty(max(1, n), dtype=torch.int64, device=chars.device)
+        N.check(self.lib.bolap_string_masks(self.ctx, ptr(chars), ptr(offsets), n, ptr(alpha),
+                                            ptr(bigram)))
+        return alpha[:n], bigram[:n]
+
+    def q13(self, t: dict, w1: str, w2: str, masks=None, max_dist: int = 4096):
+        a = N.Q13Args()
+        a.order_rows = int(t["o_custkey_raw"].numel())
+        a.o_custkey, a.chars = ptr(t["o_custkey_raw"]), ptr(t["o_comment_chars"])
+        a.offsets = ptr(t["o_comment_offsets"])
+        if masks is not None:
+            a.alpha, a.bigram = ptr(masks[0]), ptr(masks[1])
+        a.word1, a.word2 = w1.encode(), w2.encode()
+        a.cust_rows = int(t["c_custkey"].numel())
+        a.c_custkey = ptr(t["c_custkey"])
+        dist = (C.c_int64 * max_dist)()
+        mx = C.c_int32(0)
+        N.check(self.lib.bolap_q13(self.ctx, C.byref(a), dist, max_dist, C.byref(mx)))
+        return mx.value, list(dist)[:mx.value + 1]
+
+    def q16(self, t: dict, n_brands: int, banned_brand: int, type_banned, size_mask: int,
+            n_part_keys: int, n_supp_keys: int):
+        import numpy as np
+        tb = np.ascontiguousarray(type_banned, np.uint8)
+        a = N.Q16Args()
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_brand, a.p_type = ptr(t["p_partkey"]), ptr(t["p_brand"]), ptr(t["p_type"])
+        a.p_size, a.n_brands, a.n_types = ptr(t["p_size"]), n_brands, len(tb)
+        a.banned_brand, a.type_banned, a.size_mask = banned_brand, tb.ctypes.data, size_mask
+        a.n_part_keys = n_part_keys
+        a.ps_rows = int(t["ps_partkey"].numel())
+        a.ps_partkey, a.ps_suppkey = ptr(t["ps_partkey"]), ptr(t["ps_suppkey"])
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.s_banned, a.n_supp_keys = ptr(t["s_suppkey"]), ptr(t["s_banned"]), n_supp_keys
+        n = n_brands * len(tb) * 51
+        out = (C.c_int64 * n)()
+        N.check(self.lib.bolap_q16(self.ctx, C.byref(a), out, n))
+        return list(out)
+
     def q17_args(self, t: dict, preds: list) -> N.Q17Args:
         a = N.Q17Args()
         a.part_rows = int(t["p_partkey"].numel())

@@ -353,6 +353,21 @@ class Q9Args(C.Structure):
                 ("n_nation", C.c_int32)]
 
 
+class Q13Args(C.Structure):
+    _fields_ = [("order_rows", C.c_int64), ("o_custkey", vp), ("chars", vp), ("offsets", vp),
+                ("alpha", vp), ("bigram", vp), ("word1", C.c_char_p), ("word2", C.c_char_p),
+                ("cust_rows", C.c_int64), ("c_custkey", vp)]
+
+
+class Q16Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_partkey", vp), ("p_brand", vp), ("p_type", vp),
+                ("p_size", vp), ("n_brands", C.c_int32), ("n_types", C.c_int32),
+                ("banned_brand", C.c_int32), ("type_banned", vp), ("size_mask", C.c_uint64),
+                ("n_part_keys", C.c_int64), ("ps_rows", C.c_int64), ("ps_partkey", vp),
+                ("ps_suppkey", vp), ("supp_rows", C.c_int64), ("s_suppkey", vp), ("s_banned", vp),
+                ("n_supp_keys", C.c_int64)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -423,6 +438,9 @@ PROTOTYPES = {
     "bolap_q19": (C.c_int, [vp, P(Q19Args), P(i64)]),
     "bolap_q21": (C.c_int, [vp, P(Q21Args), P(i64), i64]),
     "bolap_q7": (C.c_int, [vp, P(Q7Args), P(i64)]),
+    "bolap_string_masks": (C.c_int, [vp, vp, vp, i64, vp, vp]),
+    "bolap_q13": (C.c_int, [vp, P(Q13Args), P(i64), i32, P(i32)]),
+    "bolap_q16": (C.c_int, [vp, P(Q16Args), P(i64), i64]),
     "bolap_q8": (C.c_int, [vp, P(Q8Args), P(i64), P(i64)]),
     "bolap_q9": (C.c_int, [vp, P(Q9Args), P(i64), P(C.c_uint8), P(i32), P(i32)]),
     "bolap_q22": (C.c_int, [vp, P(Q22Args), P(Q22Result)]),

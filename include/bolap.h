@@ -1093,6 +1093,67 @@ typedef struct {
 int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* args, int64_t* totals, uint8_t* used,
              int32_t* min_year, int32_t* year_span);
 
+/* The builder's per-string prefilter masks of a string column (chars +
+   n + 1 uint32 offsets, builder_impl.cpp:124-150): alpha_out[i] has bit
+   c - 'a' for every letter (either case), bigram_out[i] bit
+   (c0 * 131 + c1) & 63 for every pair of adjacent bytes.  Stream-ordered. */
+int bolap_string_masks(bolap_ctx* ctx, const uint8_t* chars, const uint32_t* offsets, int64_t n,
+                       uint32_t* alpha_out, uint64_t* bigram_out);
+
+/* Q13 (bespoke_tpch/query_q13.cpp:150-250): an order matches when word1
+   occurs in its comment and word2 after that occurrence's end (an empty word
+   is found at the search start, which must not be the comment's end;
+   :27-82); alpha / bigram (optional, bolap_string_masks) skip comments that
+   cannot match.  Every order that does not match adds one to its customer
+   key's count (keys in [0, max c_custkey]); dist[c] = customer rows whose
+   key has count c.  *max_count = the largest count; BOLAP_ERR_CAPACITY when
+   max_dist <= it.  Words are host strings shorter than 64 bytes. */
+typedef struct {
+    int64_t order_rows;
+    const int32_t* o_custkey;
+    const uint8_t* chars;            /* o_comment bytes */
+    const uint32_t* offsets;         /* order_rows + 1 */
+    const uint32_t* alpha;           /* optional prefilter masks */
+    const uint64_t* bigram;
+    const char* word1;
+    const char* word2;
+    int64_t cust_rows;
+    const int32_t* c_custkey;
+} bolap_q13_args;
+int bolap_q13(bolap_ctx* ctx, const bolap_q13_args* args, int64_t* dist, int32_t max_dist,
+              int32_t* max_count);
+
+/* Q16 (bespoke_tpch/query_q16.cpp:180-480): a part row is eligible when its
+   brand code is not banned_brand, its size (0..50) is in size_mask and its
+   type code is not marked in type_banned (a HOST array of n_types bytes:
+   types starting with TYPE); its key (in [0, n_part_keys)) joins group
+   (brand * n_types + type) * 51 + size — the last eligible row of a key
+   decides.  Every partsupp row of a grouped key whose supplier is not
+   banned (s_banned per supplier row: the comment has "Customer" followed by
+   "Complaints", decided on the host; keys in [0, n_supp_keys)) adds its
+   supplier to the group's distinct set.  counts (host) has
+   n_brands * n_types * 51 entries: distinct suppliers per group. */
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const uint8_t* p_brand;          /* codes */
+    const uint8_t* p_type;
+    const int32_t* p_size;
+    int32_t n_brands, n_types;
+    int32_t banned_brand;            /* -1: none */
+    const uint8_t* type_banned;
+    uint64_t size_mask;
+    int64_t n_part_keys;
+    int64_t ps_rows;
+    const int32_t* ps_partkey;
+    const int32_t* ps_suppkey;
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    const uint8_t* s_banned;
+    int64_t n_supp_keys;
+} bolap_q16_args;
+int bolap_q16(bolap_ctx* ctx, const bolap_q16_args* args, int64_t* counts, int64_t n_counts);
+
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */
 /* ------------------------------------------------------------------------ */

@@ -896,3 +896,51 @@ def q9(db: dict, p_ok_row, n_nation: int):
     assert span <= 64
     return y0.value, tot[:n_nation * span].reshape(n_nation, span), \
         used[:n_nation * span].reshape(n_nation, span)
+
+
+def q13(db: dict, w1: str, w2: str):
+    """(max count, dist[max count + 1]) of run_q13 (oracle_q13) over the raw
+    order rows (o_custkey_raw with their comments)."""
+    L = lib()
+    if not getattr(L, "_q13_ready", False):
+        L.oracle_q13.restype = i32
+        L.oracle_q13.argtypes = [i64, vp, vp, vp, C.c_char_p, C.c_char_p, i64, vp, vp, i64]
+        L.oracle_q16.restype = i64
+        L.oracle_q16.argtypes = [i64, vp, vp, vp, vp, i32, i32, vp, u64, i64, i64, vp, vp, vp,
+                                 i64, vp, i64]
+        L._q13_ready = True
+    ok = _c(db["o_custkey_raw"], np.int32)
+    ch, off = _c(db["o_comment_chars"], np.uint8), _c(db["o_comment_offsets"], np.uint32)
+    ck = _c(db["c_custkey"], np.int32)
+    cap = 4096
+    dist = np.zeros(cap, np.int64)
+    mx = L.oracle_q13(len(ok), _p(ok), _p(ch), _p(off), w1.encode(), w2.encode(), len(ck), _p(ck),
+                      _p(dist), cap)
+    assert mx < cap
+    return mx, dist[:mx + 1]
+
+
+def q16(db: dict, banned_brand: int, type_banned, size_mask: int, n_part_keys: int,
+        n_supp_keys: int):
+    """counts[group] (group = (brand * n_types + type) * 51 + size) of run_q16."""
+    q13  # noqa: B018
+    L = lib()
+    if not getattr(L, "_q13_ready", False):
+        q13({"o_custkey_raw": np.zeros(0, np.int32), "o_comment_chars": np.zeros(1, np.uint8),
+             "o_comment_offsets": np.zeros(1, np.uint32), "c_custkey": np.zeros(0, np.int32)},
+            "a", "b")
+    tb = _c(type_banned, np.uint8)
+    n_types = len(tb)
+    n_counts = len(db["brand_dict"]) * n_types * 51
+    counts = np.zeros(max(1, n_counts), np.int64)
+    p = [_c(db["p_partkey"], np.int32), _c(db["p_brand"], np.uint16), _c(db["p_type"], np.uint16),
+         _c(db["p_size"], np.int32)]
+    ps = [_c(db["ps_partkey"], np.int32), _c(db["ps_suppkey"], np.int32)]
+    sb = np.zeros(max(1, n_supp_keys), np.uint8)
+    for k, b in zip(db["s_suppkey"].tolist(), db["s_banned"].tolist()):
+        if 0 <= k < n_supp_keys and b:
+            sb[k] = 1
+    L.oracle_q16(len(p[0]), *[_p(x) for x in p], n_types, banned_brand, _p(tb), size_mask,
+                 n_part_keys, len(ps[0]), _p(ps[0]), _p(ps[1]), _p(sb), n_supp_keys, _p(counts),
+                 n_counts)
+    return counts[:n_counts]

@@ -321,4 +321,14 @@ int32_t oracle_q9(int64_t p_rows, const int32_t* p_partkey, const uint8_t* p_ok_
                   const int32_t* l_disc_price, const int16_t* l_supp_nation, int32_t n_nation,
                   int64_t* totals, uint8_t* used, int32_t* min_year_out);
 
+int32_t oracle_q13(int64_t o_rows, const int32_t* o_custkey, const uint8_t* chars,
+                   const uint32_t* offsets, const char* w1, const char* w2, int64_t c_rows,
+                   const int32_t* c_custkey, int64_t* dist, int64_t max_dist);
+int64_t oracle_q16(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_brand,
+                   const uint16_t* p_type, const int32_t* p_size, int32_t n_types,
+                   int32_t banned_brand, const uint8_t* type_banned, uint64_t size_mask,
+                   int64_t n_part_keys, int64_t ps_rows, const int32_t* ps_partkey,
+                   const int32_t* ps_suppkey, const uint8_t* supp_banned, int64_t n_supp_keys,
+                   int64_t* counts, int64_t n_counts);
+
 #endif

@@ -595,3 +595,114 @@ int32_t oracle_q9(int64_t p_rows, const int32_t* p_partkey, const uint8_t* p_ok_
     free(ps_idx);
     return span;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Q13 (bespoke_tpch/query_q13.cpp:150-250): an order matches when WORD1    */
+/* occurs in its comment and WORD2 occurs after that occurrence's end       */
+/* (matches_pattern, :50-82: an empty word is found at the search start,   */
+/* which must not be the comment's end).  Every order that does not match  */
+/* adds one to its customer's count (keys in [0, max customer key]); the    */
+/* result is dist[c] = customer rows with count c, c <= max count.  Returns */
+/* the largest count.                                                        */
+/* ------------------------------------------------------------------------ */
+static const char* find_sub(const char* s, const char* e, const char* w, size_t n) {
+    if (n == 0) return s;
+    for (const char* p = s; p + n <= e; ++p)
+        if (memcmp(p, w, n) == 0) return p;
+    return e;
+}
+
+int32_t oracle_q13(int64_t o_rows, const int32_t* o_custkey, const uint8_t* chars,
+                   const uint32_t* offsets, const char* w1, const char* w2, int64_t c_rows,
+                   const int32_t* c_custkey, int64_t* dist, int64_t max_dist) {
+    int32_t cmax = 0;
+    for (int64_t i = 0; i < c_rows; ++i)
+        if (i == 0 || c_custkey[i] > cmax) cmax = c_custkey[i];
+    const int64_t size = (int64_t)cmax + 1;
+    int32_t* cnt = (int32_t*)calloc((size_t)(size > 0 ? size : 1), sizeof(int32_t));
+    const size_t n1 = strlen(w1), n2 = strlen(w2);
+    for (int64_t o = 0; o < o_rows; ++o) {
+        const char* s = (const char*)chars + offsets[o];
+        const char* e = (const char*)chars + offsets[o + 1];
+        const char* p1 = find_sub(s, e, w1, n1);
+        int match = 0;
+        if (p1 != e) {
+            const char* p2 = find_sub(p1 + (oracle_mutation() == 25 ? 0 : n1), e, w2, n2);
+            match = p2 != e;
+        }
+        if (match) continue;
+        const int32_t k = o_custkey[o];
+        if (k >= 0 && k < size) cnt[k] += 1;
+    }
+    int32_t mx = 0;
+    for (int64_t k = 0; k < size; ++k)
+        if (cnt[k] > mx) mx = cnt[k];
+    for (int64_t c = 0; c < max_dist; ++c) dist[c] = 0;
+    for (int64_t i = 0; i < c_rows; ++i) {
+        const int32_t k = c_custkey[i];
+        const int32_t c = (k >= 0 && k < size) ? cnt[k] : 0;
+        if (c < max_dist) dist[c] += 1;
+    }
+    free(cnt);
+    return mx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Q16 (bespoke_tpch/query_q16.cpp:180-480): a part row is eligible when    */
+/* its brand code is not banned_brand, its size is one of the sizes         */
+/* (size_mask, sizes 0..50) and its type code is not banned (type_banned[], */
+/* types starting with TYPE); its key (in [0, n_part_keys)) then belongs to */
+/* group (brand, type, size) - the last eligible row of a key decides.      */
+/* Every partsupp row of a grouped key whose supplier (keys in             */
+/* [0, n_supp_keys) can be banned, supp_banned[]) is not banned adds its   */
+/* supplier to the group's distinct set.  counts[(brand * n_types + type)   */
+/* * 51 + size] = distinct suppliers; returns the number of groups with    */
+/* suppliers.                                                                */
+/* ------------------------------------------------------------------------ */
+static int cmp_u64(const void* a, const void* b) {
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+int64_t oracle_q16(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_brand,
+                   const uint16_t* p_type, const int32_t* p_size, int32_t n_types,
+                   int32_t banned_brand, const uint8_t* type_banned, uint64_t size_mask,
+                   int64_t n_part_keys, int64_t ps_rows, const int32_t* ps_partkey,
+                   const int32_t* ps_suppkey, const uint8_t* supp_banned, int64_t n_supp_keys,
+                   int64_t* counts, int64_t n_counts) {
+    int64_t* grp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_part_keys > 0 ? n_part_keys : 1));
+    for (int64_t k = 0; k < n_part_keys; ++k) grp[k] = -1;
+    for (int64_t i = 0; i < p_rows; ++i) {
+        const int32_t b = p_brand[i];
+        if (b == banned_brand) continue;
+        const int32_t sz = p_size[i];
+        if (sz < 0 || sz > 50 || !((size_mask >> sz) & 1)) continue;
+        if (type_banned[p_type[i]]) continue;
+        const int32_t k = p_partkey[i];
+        if (k < 0 || k >= n_part_keys) continue;
+        grp[k] = ((int64_t)b * n_types + p_type[i]) * 51 + sz;
+    }
+    uint64_t* pairs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(ps_rows > 0 ? ps_rows : 1));
+    int64_t n = 0;
+    for (int64_t j = 0; j < ps_rows; ++j) {
+        const int32_t k = ps_partkey[j];
+        if (k < 0 || k >= n_part_keys || grp[k] < 0) continue;
+        const int32_t s = ps_suppkey[j];
+        if (s >= 0 && s < n_supp_keys && supp_banned[s] && oracle_mutation() != 26) continue;
+        pairs[n++] = ((uint64_t)grp[k] << 32) | (uint32_t)s;
+    }
+    qsort(pairs, (size_t)n, sizeof(uint64_t), cmp_u64);
+    for (int64_t g = 0; g < n_counts; ++g) counts[g] = 0;
+    int64_t groups = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && pairs[i] == pairs[i - 1]) continue;
+        const int64_t g = (int64_t)(pairs[i] >> 32);
+        if (g < n_counts) {
+            if (counts[g] == 0) ++groups;
+            counts[g] += 1;
+        }
+    }
+    free(grp);
+    free(pairs);
+    return groups;
+}

@@ -227,6 +227,49 @@ def test_q9(engine, db, color):
     assert gy == y0 and np.array_equal(gtot, tot) and np.array_equal(gused, used)
 
 
+@pytest.mark.parametrize("w1,w2", [("special", "requests"), ("pending", "deposits"),
+                                   ("express", ""), ("x", "x"), ("", "")])
+def test_q13(engine, db, w1, w2):
+    r, h, t = db
+    mx, dist = oracle.q13(h, w1, w2)
+    masks = engine.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
+    assert engine.q13(t, w1, w2) == (mx, dist.tolist())
+    assert engine.q13(t, w1, w2, masks=masks) == (mx, dist.tolist())
+
+
+def test_string_masks_match_the_builder(engine, db):
+    r, h, t = db
+    a, b = engine.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
+    a, b = a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint64)
+    ch, off = h["o_comment_chars"].tobytes(), h["o_comment_offsets"]
+    for i in range(0, len(off) - 1, max(1, (len(off) - 1) // 500)):
+        s_ = ch[off[i]:off[i + 1]]
+        al = 0
+        bg = 0
+        for j, c in enumerate(s_):
+            if 97 <= c <= 122:
+                al |= 1 << (c - 97)
+            elif 65 <= c <= 90:
+                al |= 1 << (c - 65)
+            if j:
+                bg |= 1 << ((s_[j - 1] * 131 + c) & 63)
+        assert int(a[i]) == al and int(b[i]) == bg, i
+
+
+@pytest.mark.parametrize("brand,tprefix,sizes", [("Brand#45", "MEDIUM POLISHED", [49, 14, 23, 45,
+                                                                                  19, 3, 36, 9]),
+                                                 ("Brand#11", "PROMO", [1, 2, 3]),
+                                                 ("Brand#99", "ZZZ", [50, 0])])
+def test_q16(engine, db, brand, tprefix, sizes):
+    r, h, t = db
+    bb = h["brand_dict"].index(brand) if brand in h["brand_dict"] else -1
+    tb = np.array([x.startswith(tprefix) for x in h["type_dict"]], np.uint8)
+    mask = sum(1 << s_ for s_ in sizes)
+    npk, nsk = int(h["p_partkey"].max()) + 1, int(h["s_suppkey"].max()) + 1
+    want = oracle.q16(h, bb, tb, mask, npk, nsk)
+    assert engine.q16(t, len(h["brand_dict"]), bb, tb, mask, npk, nsk) == want.tolist()
+
+
 @pytest.fixture(scope="module")
 def sf10():
     """15,000,000 orders, about 60,000,000 lineitems, 2,000,000 parts,
@@ -277,6 +320,17 @@ def test_sf10_all_five(engine, sf10):
     y0, tot9, used9 = oracle.q9(h, p_ok, 25)
     gy, gtot, gused = engine.q9(t, torch.from_numpy(p_ok).cuda(), 25)
     assert gy == y0 and np.array_equal(gtot, tot9) and np.array_equal(gused, used9)
+    print("q13 q16", flush=True)
+    mx, dist = oracle.q13(h, "special", "requests")
+    masks = engine.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
+    assert engine.q13(t, "special", "requests", masks=masks) == (mx, dist.tolist())
+    tb = np.array([x.startswith("MEDIUM POLISHED") for x in h["type_dict"]], np.uint8)
+    mask = sum(1 << s_ for s_ in [49, 14, 23, 45, 19, 3, 36, 9])
+    npk, nsk = int(h["p_partkey"].max()) + 1, int(h["s_suppkey"].max()) + 1
+    want16 = oracle.q16(h, h["brand_dict"].index("Brand#45"), tb, mask, npk, nsk)
+    assert want16.sum() > 0
+    assert engine.q16(t, len(h["brand_dict"]), h["brand_dict"].index("Brand#45"), tb, mask, npk,
+                      nsk) == want16.tolist()
     print("q21", flush=True)
     n = int(h["s_suppkey"].max()) + 1
     w21 = oracle.q21(h, h["status_dict"].index("F"), 7, n)

@@ -401,3 +401,78 @@ def test_planted_mistakes_q7_q8_q9_are_caught(mutation):
     finally:
         oracle.lib().oracle_set_mutation(0)
     assert bad[mutation - 22] != good[mutation - 22]
+
+
+@pytest.mark.parametrize("w1,w2", [("special", "requests"), ("pending", "deposits"),
+                                   ("express", ""), ("x", "x"), ("", "")])
+def test_q13_matches_pandas(data, w1, w2):
+    import re
+    r, db = data
+    mx, dist = oracle.q13(db, w1, w2)
+    ch, off = r["o_comment_chars"].tobytes(), r["o_comment_offsets"]
+    com = [ch[off[i]:off[i + 1]].decode() for i in range(len(r["o_custkey"]))]
+    # LIKE '%w1%w2%' as the reference matches it: w2 after the first w1 ends,
+    # an empty word found at the search start, which must not be the end
+    pat = re.compile(re.escape(w1) + ".*" + re.escape(w2) + ("" if w2 else "."), re.S)
+    keep = np.array([pat.search(c) is None for c in com])
+    cnt = pd.Series(r["o_custkey"][keep]).value_counts()
+    per_cust = np.array([cnt.get(k, 0) for k in r["c_custkey"]])
+    want = np.bincount(per_cust)
+    assert mx == len(want) - 1 and dist.tolist() == want.tolist()
+
+
+def q16_pandas(r, brand, tprefix, sizes):
+    p = pd.DataFrame({"pk": r["p_partkey"], "b": r["p_brand"], "t": r["p_type"],
+                      "s": r["p_size"]})
+    tnames = np.array(T.TYPES)[p.t]
+    p = p[(np.array(T.BRANDS)[p.b] != brand) & ~pd.Series(tnames).str.startswith(tprefix).to_numpy()
+          & p.s.isin(sizes)]
+    ps = pd.DataFrame({"pk": r["ps_partkey"], "sk": r["ps_suppkey"]})
+    banned = set(np.array(r["s_suppkey"])[
+        [("Customer" in c and "Complaints" in c[c.index("Customer") + 8:]) for c in
+         r["s_comment"]]].tolist())
+    j = ps[~ps.sk.isin(banned)].merge(p, on="pk")
+    return j.groupby(["b", "t", "s"]).sk.nunique()
+
+
+@pytest.mark.parametrize("brand,tprefix,sizes", [("Brand#45", "MEDIUM POLISHED", [49, 14, 23, 45,
+                                                                                  19, 3, 36, 9]),
+                                                 ("Brand#11", "PROMO", [1, 2, 3]),
+                                                 ("Brand#99", "ZZZ", [50, 0])])
+def test_q16_matches_pandas(data, brand, tprefix, sizes):
+    r, db = data
+    bb = db["brand_dict"].index(brand) if brand in db["brand_dict"] else -1
+    tb = np.array([t.startswith(tprefix) for t in db["type_dict"]], np.uint8)
+    mask = 0
+    for s_ in sizes:
+        mask |= 1 << s_
+    counts = oracle.q16(db, bb, tb, mask, int(db["p_partkey"].max()) + 1,
+                        int(db["s_suppkey"].max()) + 1)
+    nt = len(db["type_dict"])
+    got = {}
+    for g in np.nonzero(counts)[0].tolist():
+        b, rest = divmod(g, nt * 51)
+        t, sz = divmod(rest, 51)
+        got[(T.BRANDS.index(db["brand_dict"][b]), T.TYPES.index(db["type_dict"][t]), sz)] = \
+            int(counts[g])
+    want = q16_pandas(r, brand, tprefix, sizes)
+    assert got == {k: int(v) for k, v in want.items()}
+    assert len(got) > 0
+
+
+@pytest.mark.parametrize("mutation", [25, 26])
+def test_planted_mistakes_q13_q16_are_caught(mutation):
+    r = T.raw(31, 20_000, parts=400, custs=300, supps=40, strings=False)
+    db = T.build(r)
+
+    def run_all():
+        tb = np.array([t.startswith("PROMO") for t in db["type_dict"]], np.uint8)
+        return (oracle.q13(db, "x", "x")[1].tolist(),
+                oracle.q16(db, 0, tb, (1 << 51) - 1, 401, 41).tolist())
+    good = run_all()
+    oracle.lib().oracle_set_mutation(mutation)
+    try:
+        bad = run_all()
+    finally:
+        oracle.lib().oracle_set_mutation(0)
+    assert bad[mutation - 25] != good[mutation - 25]

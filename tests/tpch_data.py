@@ -48,6 +48,34 @@ COLORS = ["almond", "azure", "blush", "chiffon", "coral", "cyan", "forest", "gho
           "puff", "sienna", "tan", "thistle", "violet", "wheat"]
 
 
+# comment words (Q13 searches them, Q16 bans "Customer ... Complaints")
+WORDS = ["special", "requests", "packages", "pending", "unusual", "express", "accounts",
+         "deposits", "furiously", "quickly", "ironic", "final", "regular", "blithely", "carefully",
+         "slyly", "bold", "even", "sleep", "haggle", "nag", "wake", "use", "boost", "x"]
+
+
+def comments(rng, n: int, words_per: int = 6):
+    """n comments of `words_per` words each (space separated, one trailing
+    space) as (chars uint8, offsets uint32 of n + 1)."""
+    wb = [w.encode() + b" " for w in WORDS]
+    width = max(len(w) for w in wb)
+    table = np.zeros((len(wb), width), np.uint8)
+    wlen = np.array([len(w) for w in wb], np.int64)
+    for i, w in enumerate(wb):
+        table[i, :len(w)] = np.frombuffer(w, np.uint8)
+    ids = rng.integers(0, len(WORDS), (n, words_per)).astype(np.int32)
+    flat = ids.reshape(-1)
+    L = wlen[flat]
+    clen = L.reshape(n, words_per).sum(axis=1)
+    offsets = np.zeros(n + 1, np.int64)
+    np.cumsum(clen, out=offsets[1:])
+    rep_ids = np.repeat(flat, L)
+    starts = np.repeat(np.cumsum(L) - L, L)
+    within = (np.arange(len(rep_ids), dtype=np.int64) - starts).astype(np.int32)
+    chars = table[rep_ids, within]
+    return chars, offsets.astype(np.uint32)
+
+
 def okeys(n: int) -> np.ndarray:
     """TPC-H's sparse order keys: 8 of every 32."""
     i = np.arange(n, dtype=np.int64)
@@ -83,6 +111,11 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["s_suppkey"] = np.arange(1, supps + 1, dtype=np.int32)
     r["s_nationkey"] = rng.integers(0, 25, supps).astype(np.int32)
     r["s_name"] = [f"Supplier#{k:09d}" for k in r["s_suppkey"]]
+    # every seventh supplier has "Customer ... Complaints" in its comment,
+    # every eleventh only "Complaints ... Customer" (not a ban)
+    r["s_comment"] = ["blithely Customer slyly Complaints even" if k % 7 == 3 else
+                      "Complaints about Customer" if k % 11 == 4 else "carefully final deposits"
+                      for k in r["s_suppkey"].tolist()]
     # part
     r["p_partkey"] = np.arange(1, parts + 1, dtype=np.int32)
     r["p_brand"] = rng.integers(0, len(BRANDS), parts)
@@ -106,6 +139,7 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["o_orderpriority"] = rng.integers(0, 5, orders)
     r["o_totalprice"] = rng.integers(100_000, 50_000_000, orders).astype(np.int64)
     r["o_shippriority"] = np.zeros(orders, np.int32)
+    r["o_comment_chars"], r["o_comment_offsets"] = comments(rng, orders)
     # lineitem, in order-key order
     per = rng.integers(1, 8, orders)
     n = int(per.sum())
@@ -137,6 +171,8 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["l_shipmode"] = rng.integers(0, 7, n)
     r["rf_values"], r["ls_values"] = ["R", "A", "N"], ["O", "F"]
     if strings:
+        ch, off = r["o_comment_chars"].tobytes(), r["o_comment_offsets"]
+        r["o_comment"] = [ch[off[i]:off[i + 1]].decode() for i in range(orders)]
         r["p_name"] = [" ".join(COLORS[w] for w in row) for row in r["p_name_words"].tolist()]
         r["c_name"] = [f"Customer#{k:09d}" for k in r["c_custkey"]]
         if custs > 1:
@@ -245,6 +281,11 @@ def build(r: dict) -> dict:
     db["s_suppkey"] = r["s_suppkey"].astype(np.int32)
     db["s_nationkey"] = r["s_nationkey"].astype(np.int32)
     db["o_custkey_raw"] = r["o_custkey"].astype(np.int32)
+    # order comments stay in the table's row order (Q13 reads them with the
+    # raw order custkeys; the reference reorders both with the date sort)
+    db["o_comment_chars"], db["o_comment_offsets"] = r["o_comment_chars"], r["o_comment_offsets"]
+    db["s_banned"] = np.array(["Customer" in c and "Complaints" in c[c.index("Customer") + 8:]
+                               for c in r["s_comment"]], np.uint8)
     db["p_partkey"] = r["p_partkey"].astype(np.int32)
     db["p_brand"], db["brand_dict"] = first_appearance(r["p_brand"], BRANDS)
     db["p_container"], db["container_dict"] = first_appearance(r["p_container"], CONTAINERS)
@@ -287,7 +328,8 @@ def parquet(r: dict) -> dict:
         "o_orderdate": day(r["o_orderdate"]), "o_totalprice": dec(r["o_totalprice"]),
         "o_shippriority": i32(r["o_shippriority"]),
         "o_orderpriority": strs(r["o_orderpriority"], PRIORITIES),
-        "o_orderstatus": strs(r["o_orderstatus"], ["F", "O", "P"])})
+        "o_orderstatus": strs(r["o_orderstatus"], ["F", "O", "P"]),
+        "o_comment": pa.array(r["o_comment"])})
     cust = pa.table({
         "c_custkey": i32(r["c_custkey"]), "c_name": pa.array(r["c_name"]),
         "c_nationkey": i32(r["c_nationkey"]), "c_acctbal": dec(r["c_acctbal"]),
@@ -301,7 +343,7 @@ def parquet(r: dict) -> dict:
     partsupp = pa.table({"ps_partkey": i32(r["ps_partkey"]), "ps_suppkey": i32(r["ps_suppkey"]),
                          "ps_supplycost": dec(r["ps_supplycost"])})
     supp = pa.table({"s_suppkey": i32(r["s_suppkey"]), "s_nationkey": i32(r["s_nationkey"]),
-                     "s_name": pa.array(r["s_name"])})
+                     "s_name": pa.array(r["s_name"]), "s_comment": pa.array(r["s_comment"])})
     nation = pa.table({"n_nationkey": i32(r["n_nationkey"]), "n_name": pa.array(r["n_name"]),
                        "n_regionkey": i32(r["n_regionkey"])})
     region = pa.table({"r_regionkey": i32(r["r_regionkey"]), "r_name": pa.array(r["r_name"])})
