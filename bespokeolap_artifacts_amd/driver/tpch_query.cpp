@@ -1,5 +1,5 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q3-Q10, Q12, Q14, Q17-Q19, Q21 and Q22, with the kernels
+// TPC-H Q1, Q3-Q10, Q12-Q14, Q16-Q19, Q21 and Q22, with the kernels
 // of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
@@ -20,7 +20,8 @@
 // query  reads one request per line until an empty line (query_impl.cpp:142-157):
 //        `1 DELTA`, `3 SEGMENT DATE`, `4 DATE`, `5 REGION DATE`,
 //        `6 DATE DISCOUNT QUANTITY`, `7 NATION1 NATION2`, `8 NATION REGION TYPE`,
-//        `9 COLOR`, `10 DATE`, `12 MODE1 MODE2 DATE`,
+//        `9 COLOR`, `10 DATE`, `12 MODE1 MODE2 DATE`, `13 WORD1 WORD2`,
+//        `16 BRAND TYPE SIZE1 .. SIZE8`,
 //        `14 DATE`, `17 BRAND CONTAINER`, `18 QUANTITY`,
 //        `19 Q1 Q2 Q3 BRAND1 BRAND2 BRAND3`, `21 NATION`, `22 I1 .. I7`
 //        with std::quoted tokens
@@ -287,6 +288,11 @@ struct Orders {
     int32_t* order_map = nullptr;                         // orderkey -> row (:610-630)
     int64_t map_size = 0;
     uint32_t *range_start = nullptr, *range_end = nullptr;  // lineitem_ranges (:929-991)
+    // o_comment in date order with its prefilter masks (:124-150)
+    uint8_t* comment_chars = nullptr;
+    uint32_t* comment_offsets = nullptr;
+    uint32_t* comment_alpha = nullptr;
+    uint64_t* comment_bigram = nullptr;
 };
 
 struct Customer {
@@ -309,6 +315,7 @@ struct Supplier {
     std::vector<std::string> name;
     std::vector<int32_t> row_by_key;  // suppkey -> last row, -1 absent
     int32_t max_key = -1;
+    uint8_t* banned = nullptr;        // comment has "Customer" then "Complaints" (Q16)
 };
 
 Supplier build_supplier(const arrow::Table& t) {
@@ -318,6 +325,12 @@ Supplier build_supplier(const arrow::Table& t) {
     s.suppkey = upload(keys);
     s.nationkey = upload(int32s(*column(t, "s_nationkey")));
     s.name = strings_or_empty(t, "s_name");
+    std::vector<uint8_t> banned;  // has_customer_complaints, query_q16.cpp:60-68
+    for (const auto& c : strings_or_empty(t, "s_comment")) {
+        const size_t p = c.find("Customer");
+        banned.push_back(p != std::string::npos && c.find("Complaints", p + 8) != std::string::npos);
+    }
+    s.banned = upload(banned);
     for (int32_t k : keys) s.max_key = std::max(s.max_key, k);
     s.row_by_key.assign(size_t(s.max_key + 1), -1);
     for (size_t i = 0; i < keys.size(); ++i)
@@ -476,6 +489,25 @@ Orders build_orders(bolap_ctx* ctx, const arrow::Table& t, const Lineitem& li,
         Dict d = dictionary(*column(t, "o_orderstatus"));
         o.status = upload(permuted(narrow_codes(d, "o_orderstatus"), order));
         o.statuses = d.values;
+    }
+    if (t.GetColumnByName("o_comment")) {
+        const auto text = permuted(strings(*column(t, "o_comment")), order);
+        std::vector<uint32_t> off(text.size() + 1, 0);
+        size_t total = 0;
+        for (size_t i = 0; i < text.size(); ++i) {
+            total += text[i].size();
+            if (total > 0xffffffffull) throw std::runtime_error("o_comment beyond 4 GB");
+            off[i + 1] = uint32_t(total);
+        }
+        std::vector<uint8_t> chars;
+        chars.reserve(total);
+        for (const auto& c : text) chars.insert(chars.end(), c.begin(), c.end());
+        o.comment_chars = upload(chars);
+        o.comment_offsets = upload(off);
+        o.comment_alpha = upload(std::vector<uint32_t>(std::max<size_t>(1, text.size())));
+        o.comment_bigram = upload(std::vector<uint64_t>(std::max<size_t>(1, text.size())));
+        check(bolap_string_masks(ctx, o.comment_chars, o.comment_offsets, int64_t(text.size()),
+                                 o.comment_alpha, o.comment_bigram));
     }
     std::vector<int16_t> cn(cust.size());
     for (size_t i = 0; i < cust.size(); ++i) cn[i] = nation_of(cust_nation, cust[i]);
@@ -1246,6 +1278,120 @@ Writer run_q9(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Part& p
     };
 }
 
+Writer run_q13(bolap_ctx* ctx, const Orders& o, const Customer& c, const std::string& line) {
+    need(o.rows >= 0 && c.rows >= 0 && o.comment_chars, "Q13 needs orders (with o_comment) and customer");
+    const auto v = quoted_args(line, 2, "Q13");
+    bolap_q13_args a{};
+    a.order_rows = o.rows;
+    a.o_custkey = o.custkey;
+    a.chars = o.comment_chars;
+    a.offsets = o.comment_offsets;
+    a.alpha = o.comment_alpha;
+    a.bigram = o.comment_bigram;
+    a.word1 = v[0].c_str();
+    a.word2 = v[1].c_str();
+    a.cust_rows = c.rows;
+    a.c_custkey = c.custkey;
+    std::vector<int64_t> dist(4096);
+    int32_t mx = 0;
+    int s = bolap_q13(ctx, &a, dist.data(), int32_t(dist.size()), &mx);
+    if (s == BOLAP_ERR_CAPACITY) {
+        dist.resize(size_t(mx) + 1);
+        s = bolap_q13(ctx, &a, dist.data(), int32_t(dist.size()), &mx);
+    }
+    check(s);
+    std::vector<std::pair<int64_t, int64_t>> rows;  // (c_count, custdist), query_q13.cpp:222-244
+    for (int32_t k = 0; k <= mx; ++k)
+        if (dist[size_t(k)] != 0) rows.emplace_back(k, dist[size_t(k)]);
+    std::sort(rows.begin(), rows.end(), [](const auto& x, const auto& y) {
+        return x.second != y.second ? x.second > y.second : x.first > y.first;
+    });
+    return [rows](const std::string& csv) {  // write_q13_csv
+        std::ofstream out(csv);
+        out << "c_count,custdist\n";
+        for (const auto& [k, d] : rows) out << k << ',' << d << "\n";
+    };
+}
+
+Writer run_q16(bolap_ctx* ctx, const Part& part, const Partsupp& ps, const Supplier& sup,
+               const std::string& line) {
+    need(part.rows >= 0 && part.type && part.size && ps.rows >= 0 && sup.rows >= 0,
+         "Q16 needs part (with p_type, p_size), partsupp and supplier");
+    need(part.brand.dtype == BOLAP_U8, "Q16 takes byte brand codes");
+    const auto v = quoted_args(line, 10, "Q16");
+    struct Row {
+        std::string brand, type;
+        int32_t size;
+        int64_t cnt;
+    };
+    std::vector<Row> rows;
+    // sizes: digits only; "<<NULL>>" and other values are skipped (:190-215)
+    uint64_t mask = 0;
+    bool any = false;
+    for (size_t i = 2; i < 10; ++i) {
+        if (v[i] == "<<NULL>>") continue;
+        bool digits = true;
+        int64_t n = 0;
+        for (char ch : v[i]) {
+            if (ch < '0' || ch > '9') {
+                digits = false;
+                break;
+            }
+            n = n * 10 + (ch - '0');
+        }
+        if (!digits) continue;
+        any = true;
+        if (n >= 0 && n <= 50) mask |= uint64_t(1) << n;
+    }
+    if (any && part.rows > 0) {
+        std::vector<uint8_t> type_banned(part.types.size());
+        for (size_t i = 0; i < part.types.size(); ++i)
+            type_banned[i] = part.types[i].compare(0, v[1].size(), v[1]) == 0 &&
+                             part.types[i].size() >= v[1].size();
+        bolap_q16_args a{};
+        a.part_rows = part.rows;
+        a.p_partkey = part.partkey;
+        a.p_brand = static_cast<const uint8_t*>(part.brand.dev);
+        a.p_type = part.type;
+        a.p_size = part.size;
+        a.n_brands = int32_t(part.brands.size());
+        a.n_types = int32_t(part.types.size());
+        a.banned_brand = code_of(part.brands, v[0]);
+        a.type_banned = type_banned.data();
+        a.size_mask = mask;
+        a.n_part_keys = int64_t(part.max_key) + 1;
+        a.ps_rows = ps.rows;
+        a.ps_partkey = ps.partkey;
+        a.ps_suppkey = ps.suppkey;
+        a.supp_rows = sup.rows;
+        a.s_suppkey = sup.suppkey;
+        a.s_banned = sup.banned;
+        a.n_supp_keys = int64_t(std::max(sup.max_key, 0)) + 1;
+        const int64_t n = int64_t(a.n_brands) * a.n_types * 51;
+        std::vector<int64_t> counts(static_cast<size_t>(n));
+        check(bolap_q16(ctx, &a, counts.data(), n));
+        for (int64_t g = 0; g < n; ++g) {
+            if (counts[size_t(g)] == 0) continue;
+            const int64_t b = g / (int64_t(a.n_types) * 51), rest = g % (int64_t(a.n_types) * 51);
+            rows.push_back({part.brands[size_t(b)], part.types[size_t(rest / 51)],
+                            int32_t(rest % 51), counts[size_t(g)]});
+        }
+        std::sort(rows.begin(), rows.end(), [](const Row& x, const Row& y) {  // :440-470
+            if (x.cnt != y.cnt) return x.cnt > y.cnt;
+            if (x.brand != y.brand) return x.brand < y.brand;
+            if (x.type != y.type) return x.type < y.type;
+            return x.size < y.size;
+        });
+    }
+    return [rows](const std::string& csv) {  // write_q16_csv
+        std::ofstream out(csv);
+        out << "p_brand,p_type,p_size,supplier_cnt\n";
+        for (const auto& r : rows)
+            out << quoted_field(r.brand) << ',' << quoted_field(r.type) << ',' << r.size << ','
+                << r.cnt << "\n";
+    };
+}
+
 Writer run_q21(bolap_ctx* ctx, const Lineitem& li, const Orders& o, const Supplier& sup,
                const Nation& nat, const std::string& line) {
     need(o.rows >= 0 && o.status && sup.rows >= 0 && li.suppkey && li.commit_receipt,
@@ -1451,6 +1597,10 @@ int main(int argc, char** argv) {
                 write = run_q8(ctx, li, orders, customer, supplier, part, nation, text);
             } else if (id == "9") {
                 write = run_q9(ctx, li, orders, part, partsupp, nation, text);
+            } else if (id == "13") {
+                write = run_q13(ctx, orders, customer, text);
+            } else if (id == "16") {
+                write = run_q16(ctx, part, partsupp, supplier, text);
             } else if (id == "21") {
                 write = run_q21(ctx, li, orders, supplier, nation, text);
             } else if (id == "22") {

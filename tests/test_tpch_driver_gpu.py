@@ -508,3 +508,60 @@ def test_driver_q7_q8_q9(tmp_path, orders):
     if orders > 3:
         assert want[0].count("\n") == 5 and want[4].count("\n") == 3
         assert want[8].count("\n") > 100
+
+
+def csv_q13(db, w1, w2):
+    import oracle
+    mx, dist = oracle.q13(db, w1, w2)
+    rows = sorted(((int(d), k) for k, d in enumerate(dist.tolist()) if d), reverse=True)
+    return "c_count,custdist\n" + "".join(f"{k},{d}\n" for d, k in rows)
+
+
+def csv_q16(db, brand, tprefix, sizes):
+    import oracle
+    out = "p_brand,p_type,p_size,supplier_cnt\n"
+    mask, any_ = 0, False
+    for s_ in sizes:
+        if s_ == "<<NULL>>" or not s_.isdigit() and s_ != "":
+            continue
+        any_ = True
+        n = int(s_) if s_ else 0
+        if n <= 50:
+            mask |= 1 << n
+    if not any_:
+        return out
+    bb = idx(db["brand_dict"], brand)
+    tb = np.array([t.startswith(tprefix) for t in db["type_dict"]], np.uint8)
+    counts = oracle.q16(db, bb, tb, mask, int(db["p_partkey"].max()) + 1,
+                        int(db["s_suppkey"].max()) + 1)
+    nt = len(db["type_dict"])
+    rows = []
+    for g in np.nonzero(counts)[0].tolist():
+        b, rest = divmod(g, nt * 51)
+        rows.append((-int(counts[g]), db["brand_dict"][b], db["type_dict"][rest // 51], rest % 51))
+    return out + "".join(f"{esc(b)},{esc(t)},{s_},{-c}\n" for c, b, t, s_ in sorted(rows))
+
+
+@pytest.mark.parametrize("orders", [3, 40_001])
+def test_driver_q13_q16(tmp_path, orders):
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    from tests import tpch_data as T
+    r = T.raw(37 + orders, orders, parts=900, custs=600, supps=60)
+    write(T.parquet(r), tmp_path / "data")
+    db = T.build(r)
+    s1 = ["49", "14", "23", "45", "19", "3", "36", "9"]
+    s2 = ["1", "2", "<<NULL>>", "x7", "3", "4", "5", "6"]
+    q = lambda v: " ".join(f'"{x}"' for x in v)
+    requests = ['13 "special" "requests"', '13 pending deposits', '13 "x" ""',
+                f'16 "Brand#45" "MEDIUM POLISHED" {q(s1)}', f'16 "Brand#11" "PROMO" {q(s2)}',
+                f'16 "Brand#11" "PROMO" {q(["<<NULL>>"] * 8)}']
+    res = run_driver(tmp_path / "data", tmp_path / "run", requests)
+    assert res.returncode == 0, res.stderr
+    want = [csv_q13(db, "special", "requests"), csv_q13(db, "pending", "deposits"),
+            csv_q13(db, "x", ""), csv_q16(db, "Brand#45", "MEDIUM POLISHED", s1),
+            csv_q16(db, "Brand#11", "PROMO", s2), csv_q16(db, "Brand#11", "PROMO", ["<<NULL>>"] * 8)]
+    for i, w in enumerate(want):
+        assert (tmp_path / "run" / f"result{i + 1}.csv").read_text() == w, requests[i]
+    if orders > 3:
+        assert want[0].count("\n") > 5 and want[3].count("\n") > 20
