@@ -129,8 +129,37 @@ def build_oracle(out: pathlib.Path | None = None, march: str = "x86-64-v2") -> p
     return out
 
 
+REFERENCE = pathlib.Path(os.environ.get("BOLAP_REFERENCE", "/root/reference"))
+REF_DIR = ORACLE_DIR / "_ref"
+REF_TPCH = REF_DIR / "libref_tpch.so"
+
+
+def build_ref() -> pathlib.Path | None:
+    """The reference's own TPC-H run_qN / write_qN_csv (bespoke_tpch/query_q*.cpp,
+    compiled unchanged from where they lie) behind the C shim oracle/ref_tpch.cpp,
+    into oracle/_ref/.  A checker for tests only.  Skipped where the reference
+    sources or pyarrow's Arrow headers (which loader_impl.hpp includes; no Arrow
+    library is linked) are absent."""
+    src = REFERENCE / "bespoke_tpch"
+    arrow = _arrow_dir()
+    if not (src / "query_q1.cpp").exists() or arrow is None:
+        print("oracle/_ref skipped: reference sources or Arrow headers absent")
+        return None
+    REF_DIR.mkdir(exist_ok=True)
+    tmp = REF_TPCH.with_suffix(".so.tmp")
+    cmd = [shutil.which("g++") or "c++", "-std=c++20", "-O2", "-fPIC", "-shared", "-pthread",
+           "-w", f"-I{src}", f"-I{arrow / 'include'}", str(ORACLE_DIR / "ref_tpch.cpp"),
+           "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle/_ref build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, REF_TPCH)
+    return REF_TPCH
+
+
 def build_all() -> None:
     build_oracle()
+    build_ref()
     build_lib()
     build_driver()
 

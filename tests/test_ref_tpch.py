@@ -1,0 +1,98 @@
+"""The oracle against the reference's own TPC-H pipelines.
+
+oracle/_ref/libref_tpch.so is the reference's query_q*.cpp compiled unchanged
+(with a shim that fills its Database; tests/tpch_ref.py).  Every served
+pipeline's CSV, as the oracle plus Python restatements of the reference
+writers produce it, must equal the CSV that the reference's run_qN and
+write_qN_csv write for the same database and request line, byte for byte.
+Skipped where the reference could not be built (no sources / no Arrow headers).
+"""
+from __future__ import annotations
+
+import pytest
+
+from tests import tpch_data as T
+from tests import tpch_ref as R
+from tests import test_tpch_driver_gpu as D
+
+pytestmark = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+
+Q16_S1 = ["49", "14", "23", "45", "19", "3", "36", "9"]
+Q16_S2 = ["1", "2", "<<NULL>>", "x7", "3", "4", "5", "6"]
+Q22_A = ["13", "31", "23", "29", "30", "18", "17"]
+Q22_B = ["AB", "20", "7", "<<NULL>>", "20", "AB", "34"]
+
+
+def quoted(v):
+    return " ".join(f'"{x}"' for x in v)
+
+
+def cases(db, r):
+    """(request line, oracle-side CSV) for every pipeline from Q4 on."""
+    return [
+        ('4 "1995-01-01"', lambda: D.csv_q4(db, "1995-01-01")),
+        ("4 1993-07-31", lambda: D.csv_q4(db, "1993-07-31")),
+        ('5 "ASIA" "1994-01-01"', lambda: D.csv_q5(db, "ASIA", "1994-01-01")),
+        ("5 EUROPE 1996-02-29", lambda: D.csv_q5(db, "EUROPE", "1996-02-29")),
+        ('5 "ATLANTIS" "1994-01-01"', lambda: D.csv_q5(db, "ATLANTIS", "1994-01-01")),
+        ('7 "FRANCE" "GERMANY"', lambda: D.csv_q7(db, r, "FRANCE", "GERMANY")),
+        ('7 "JAPAN" "JAPAN"', lambda: D.csv_q7(db, r, "JAPAN", "JAPAN")),
+        ('7 "ATLANTIS" "FRANCE"', lambda: D.csv_q7(db, r, "ATLANTIS", "FRANCE")),
+        ('8 "BRAZIL" "AMERICA" "ECONOMY ANODIZED STEEL"',
+         lambda: D.csv_q8(db, r, "BRAZIL", "AMERICA", "ECONOMY ANODIZED STEEL")),
+        ('8 "CHINA" "ASIA" "NO SUCH TYPE"', lambda: D.csv_q8(db, r, "CHINA", "ASIA", "NO SUCH TYPE")),
+        ('9 "green"', lambda: D.csv_q9(db, r, "green")),
+        ('9 "zzz"', lambda: D.csv_q9(db, r, "zzz")),
+        ('10 "1993-10-01"', lambda: D.csv_q10(db, r, "1993-10-01")),
+        ("10 2010-01-01", lambda: D.csv_q10(db, r, "2010-01-01")),
+        ('12 "MAIL" "SHIP" "1994-01-01"', lambda: D.csv_q12(db, "MAIL", "SHIP", "1994-01-01")),
+        ("12 AIR NOSUCH 1995-01-01", lambda: D.csv_q12(db, "AIR", "NOSUCH", "1995-01-01")),
+        ('13 "special" "requests"', lambda: D.csv_q13(db, "special", "requests")),
+        ('13 "x" ""', lambda: D.csv_q13(db, "x", "")),
+        (f'16 "Brand#45" "MEDIUM POLISHED" {quoted(Q16_S1)}',
+         lambda: D.csv_q16(db, "Brand#45", "MEDIUM POLISHED", Q16_S1)),
+        (f'16 "Brand#11" "PROMO" {quoted(Q16_S2)}', lambda: D.csv_q16(db, "Brand#11", "PROMO", Q16_S2)),
+        ('19 "1" "10" "20" "Brand#12" "Brand#23" "Brand#34"',
+         lambda: D.csv_q19(db, ("1", "10", "20"), ("Brand#12", "Brand#23", "Brand#34"))),
+        ('19 "5" "<<NULL>>" "25" "Brand#11" "Brand#21" "Brand#55"',
+         lambda: D.csv_q19(db, ("5", "<<NULL>>", "25"), ("Brand#11", "Brand#21", "Brand#55"))),
+        ('21 "SAUDI ARABIA"', lambda: D.csv_q21(db, r, "SAUDI ARABIA")),
+        ('21 "ATLANTIS"', lambda: D.csv_q21(db, r, "ATLANTIS")),
+    ] + ([
+        (f"22 {quoted(Q22_A)}", lambda: D.csv_q22(db, Q22_A)),
+        (f"22 {quoted(Q22_B)}", lambda: D.csv_q22(db, Q22_B)),
+    ] if q22_defined(r) else [])
+
+
+def q22_defined(r) -> bool:
+    """The reference's Q22 reads orders_by_customer_offsets[custkey + 1] for
+    every candidate (query_q22.cpp:351-355), one past the array for a customer
+    key above the largest order custkey (builder_impl.cpp:631-633), so its
+    answer is defined only when no customer key exceeds that bound."""
+    return int(r["c_custkey"].max()) <= int(r["o_custkey"].max())
+
+
+@pytest.mark.parametrize("orders", [3, 12_001])
+def test_oracle_csv_equals_the_reference(orders):
+    r = T.raw(53 + orders, orders, parts=700, custs=901, supps=60)
+    db = T.build(r)
+    ref = R.RefTpch(r)
+    todo = cases(db, r)
+    got = ref.run([line for line, _ in todo])
+    for (line, want), out in zip(todo, got):
+        assert out is not None, line
+        assert out == want(), line
+    if orders > 3:
+        # the data reaches every pipeline's non-empty branch
+        assert q22_defined(r)
+        nonempty = sum(out.count("\n") > 1 for out in got)
+        assert nonempty >= len(todo) - 8
+
+
+def test_shim_reports_reference_errors():
+    """A request the reference's parser rejects comes back as its message."""
+    r = T.raw(5, 3, parts=20, custs=10, supps=8)
+    ref = R.RefTpch(r)
+    with pytest.raises(RuntimeError, match="Q4"):
+        ref.run(["4"])
+    assert ref.run(["99 x"]) == [None]

@@ -565,3 +565,45 @@ def test_driver_q13_q16(tmp_path, orders):
         assert (tmp_path / "run" / f"result{i + 1}.csv").read_text() == w, requests[i]
     if orders > 3:
         assert want[0].count("\n") > 5 and want[3].count("\n") > 20
+
+
+@pytest.mark.parametrize("orders", [3, 20_001])
+def test_driver_matches_the_reference_pipelines(tmp_path, orders):
+    """Every served id through bolap_tpch (Parquet -> device mirror -> GPU
+    pipelines -> CSV) against the reference's own run_qN + write_qN_csv
+    (oracle/_ref, compiled from its sources) over the same tables, byte for
+    byte.  Ids the driver does not serve get the reference's stub file."""
+    if not B.DRIVER.exists():
+        pytest.skip("bolap_tpch not built (no Arrow C++)")
+    from tests import tpch_data as T
+    from tests import tpch_ref as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built")
+    from tests.test_ref_tpch import Q16_S1, Q22_A, q22_defined, quoted
+    r = T.raw(61 + orders, orders, parts=700, custs=901, supps=60)
+    write(T.parquet(r), tmp_path / "data")
+    requests = ['1 90', '1 "3"', '3 "BUILDING" "1995-03-15"', '3 "MACHINERY" "1993-01-01"',
+                '4 "1995-01-01"', '5 "ASIA" "1994-01-01"', '6 "1994-01-01" "0.06" "24"',
+                '6 "1997-01-01" "0.01" "40"', '7 "FRANCE" "GERMANY"',
+                '8 "BRAZIL" "AMERICA" "ECONOMY ANODIZED STEEL"', '9 "green"', '10 "1993-10-01"',
+                '12 "MAIL" "SHIP" "1994-01-01"', '13 "special" "requests"', '14 "1995-09-01"',
+                f'16 "Brand#45" "MEDIUM POLISHED" {quoted(Q16_S1)}', '17 "Brand#23" "MED BOX"',
+                '17 "Brand#11" "SM CASE"', '18 "200"', '18 "300"',
+                '19 "1" "10" "20" "Brand#12" "Brand#23" "Brand#34"', '21 "SAUDI ARABIA"',
+                f"22 {quoted(Q22_A)}", '2 "15" "BRASS" "EUROPE"', '11 "GERMANY" "0.0001"',
+                '15 "1996-01-01"', '20 "forest" "1994-01-01" "CANADA"']
+    served = [q for q in requests if q.split()[0] not in ("2", "11", "15", "20")]
+    if not q22_defined(r):
+        served = [q for q in served if not q.startswith("22 ")]
+    res = run_driver(tmp_path / "data", tmp_path / "run", requests)
+    assert res.returncode == 0, res.stderr
+    got = {q: (tmp_path / "run" / f"result{i + 1}.csv").read_text()
+           for i, q in enumerate(requests)}
+    want = R.RefTpch(r).run(served)
+    for q, w in zip(served, want):
+        assert got[q] == w, q
+    for q in requests:
+        if q not in served and not q.startswith("22 "):
+            assert got[q] == stub_csv(q.split()[0], q), q
+    if orders > 3:
+        assert sum(w.count("\n") > 1 for w in want) >= len(served) - 3
