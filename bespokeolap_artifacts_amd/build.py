@@ -132,36 +132,50 @@ def build_oracle(out: pathlib.Path | None = None, march: str = "x86-64-v2") -> p
 REFERENCE = pathlib.Path(os.environ.get("BOLAP_REFERENCE", "/root/reference"))
 REF_DIR = ORACLE_DIR / "_ref"
 REF_TPCH = REF_DIR / "libref_tpch.so"
+DROPIN_TPCH = REF_DIR / "libdropin_tpch.so"
+DROPIN_SRC = PKG / "dropin" / "tpch_run_q.cpp"
 
 
-def build_ref() -> pathlib.Path | None:
-    """The reference's own TPC-H run_qN / write_qN_csv (bespoke_tpch/query_q*.cpp,
-    compiled unchanged from where they lie) behind the C shim oracle/ref_tpch.cpp,
-    into oracle/_ref/.  A checker for tests only.  Skipped where the reference
-    sources or pyarrow's Arrow headers (which loader_impl.hpp includes; no Arrow
-    library is linked) are absent."""
+def _ref_compile(out: pathlib.Path, extra: list[str]) -> pathlib.Path:
     src = REFERENCE / "bespoke_tpch"
-    arrow = _arrow_dir()
-    if not (src / "query_q1.cpp").exists() or arrow is None:
-        print("oracle/_ref skipped: reference sources or Arrow headers absent")
-        return None
-    REF_DIR.mkdir(exist_ok=True)
-    tmp = REF_TPCH.with_suffix(".so.tmp")
+    tmp = out.with_suffix(".so.tmp")
     cmd = [shutil.which("g++") or "c++", "-std=c++20", "-O2", "-fPIC", "-shared", "-pthread",
-           "-w", f"-I{src}", f"-I{arrow / 'include'}", str(ORACLE_DIR / "ref_tpch.cpp"),
-           "-o", str(tmp)]
+           "-w", "-Wl,-Bsymbolic", f"-I{src}", f"-I{_arrow_dir() / 'include'}",
+           str(ORACLE_DIR / "ref_tpch.cpp"), "-o", str(tmp), *extra]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"oracle/_ref build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, REF_TPCH)
-    return REF_TPCH
+        raise RuntimeError(f"{out.name} build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+    return out
+
+
+def build_ref() -> list[pathlib.Path]:
+    """oracle/_ref/: the reference's own TPC-H run_qN / write_qN_csv
+    (bespoke_tpch/query_q*.cpp, compiled unchanged from where they lie) behind
+    the C shim oracle/ref_tpch.cpp -- a checker for tests only -- and the same
+    unit with the drop-in GPU bodies (dropin/tpch_run_q.cpp) in place of six
+    pipelines, linked against libbolap.  Skipped where the reference sources or
+    pyarrow's Arrow headers (which loader_impl.hpp includes; no Arrow library
+    is linked) are absent."""
+    if not (REFERENCE / "bespoke_tpch" / "query_q1.cpp").exists() or _arrow_dir() is None:
+        print("oracle/_ref skipped: reference sources or Arrow headers absent")
+        return []
+    REF_DIR.mkdir(exist_ok=True)
+    out = [_ref_compile(REF_TPCH, [])]
+    if LIB.exists():
+        out.append(_ref_compile(DROPIN_TPCH, [
+            "-DREF_TPCH_DROPIN", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+            f"-I{ROOT / 'include'}", f"-I{DROPIN_SRC.parent}", f"-L{PKG}", "-lbolap",
+            "-Wl,-rpath,$ORIGIN/../../bespokeolap_artifacts_amd", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
+            "-Wl,--no-undefined"]))
+    return out
 
 
 def build_all() -> None:
     build_oracle()
-    build_ref()
     build_lib()
     build_driver()
+    build_ref()
 
 
 if __name__ == "__main__":

@@ -17,13 +17,25 @@
 //
 // Built only where the reference sources exist, into oracle/_ref/ (see
 // bespokeolap_artifacts_amd/build.py:build_ref).  Only tests load it, as a
-// checker; the product never links it.
+// checker; the product never links it.  Built a second time with
+// REF_TPCH_DROPIN, it is the reference's query unit with the GPU drop-in
+// bodies in place of six pipelines, which tests compare against the first.
+#ifdef REF_TPCH_DROPIN
+// The drop-in variant: run_qN / write_qN_csv of Q1, Q3, Q6, Q14, Q17 and Q18
+// come from bespokeolap_artifacts_amd/dropin/tpch_run_q.cpp (the GPU bodies)
+// instead of the reference's files; everything else is unchanged.
+#include "tpch_run_q.cpp"
+#else
 #include "query_q1.cpp"
-#include "query_q2.cpp"
 #include "query_q3.cpp"
+#include "query_q6.cpp"
+#include "query_q14.cpp"
+#include "query_q17.cpp"
+#include "query_q18.cpp"
+#endif
+#include "query_q2.cpp"
 #include "query_q4.cpp"
 #include "query_q5.cpp"
-#include "query_q6.cpp"
 #include "query_q7.cpp"
 #include "query_q8.cpp"
 #include "query_q9.cpp"
@@ -31,11 +43,8 @@
 #include "query_q11.cpp"
 #include "query_q12.cpp"
 #include "query_q13.cpp"
-#include "query_q14.cpp"
 #include "query_q15.cpp"
 #include "query_q16.cpp"
-#include "query_q17.cpp"
-#include "query_q18.cpp"
 #include "query_q19.cpp"
 #include "query_q20.cpp"
 #include "query_q21.cpp"
@@ -250,7 +259,12 @@ void* ref_tpch_db_new(int32_t base_date_days) {
     return d;
 }
 
-void ref_tpch_db_free(void* db) { delete static_cast<Database*>(db); }
+void ref_tpch_db_free(void* db) {
+#ifdef REF_TPCH_DROPIN
+    bolap_dropin::release_mirror(*static_cast<Database*>(db));
+#endif
+    delete static_cast<Database*>(db);
+}
 
 // Numeric column `name` from n elements of `width` bytes (checked against the
 // field's own element size).  Returns 0, or 1 for an unknown name / width.

@@ -96,3 +96,32 @@ def test_shim_reports_reference_errors():
     with pytest.raises(RuntimeError, match="Q4"):
         ref.run(["4"])
     assert ref.run(["99 x"]) == [None]
+
+
+def test_dropin_compiles_and_links_against_the_reference_headers(tmp_path):
+    """bespokeolap_artifacts_amd/dropin/tpch_run_q.cpp -- run_qN / write_qN_csv
+    of Q1, Q3, Q6, Q14, Q17, Q18 over libbolap -- compiles against the
+    reference's own query_qN.hpp / builder_impl.hpp / args_parser.hpp and links
+    against libbolap.so and the HIP runtime with no undefined symbol."""
+    import shutil
+    import subprocess
+
+    from bespokeolap_artifacts_amd import build as B
+    src = B.REFERENCE / "bespoke_tpch"
+    if not (src / "query_q1.hpp").exists() or B._arrow_dir() is None:
+        pytest.skip("reference headers or Arrow headers absent")
+    if not B.LIB.exists():
+        pytest.skip("libbolap.so not built")
+    out = tmp_path / "libdropin_only.so"
+    cmd = [shutil.which("g++") or "c++", "-std=c++20", "-O1", "-fPIC", "-shared", "-Wall",
+           "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           f"-I{B.ROOT / 'include'}", f"-I{src}", f"-I{B._arrow_dir() / 'include'}",
+           str(B.DROPIN_SRC), "-o", str(out), f"-L{B.PKG}", "-lbolap", "-L/opt/rocm/lib",
+           "-lamdhip64", "-Wl,--no-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-4000:]
+    syms = subprocess.run(["nm", "-DC", "--defined-only", str(out)], capture_output=True,
+                          text=True).stdout
+    for q in (1, 3, 6, 14, 17, 18):
+        assert f"run_q{q}(Database const&, Q{q}Args const&)" in syms
+        assert f"write_q{q}_csv(" in syms

@@ -26,6 +26,8 @@ from tests import tpch_data as T
 
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 LIB = ROOT / "oracle" / "_ref" / "libref_tpch.so"
+# the same query unit with the GPU drop-in bodies of Q1, Q3, Q6, Q14, Q17, Q18
+DROPIN = ROOT / "oracle" / "_ref" / "libdropin_tpch.so"
 BASE_DAYS = (T.BASE - datetime.date(1970, 1, 1)).days
 
 
@@ -33,8 +35,8 @@ def available() -> bool:
     return LIB.exists()
 
 
-def _load():
-    lib = C.CDLL(str(LIB))
+def _load(path: pathlib.Path):
+    lib = C.CDLL(str(path))
     vp, cp, u64 = C.c_void_p, C.c_char_p, C.c_uint64
     lib.ref_tpch_db_new.restype = vp
     lib.ref_tpch_db_new.argtypes = [C.c_int32]
@@ -58,8 +60,8 @@ def _strings(values: list[str]) -> tuple[bytes, np.ndarray]:
 
 
 class RefTpch:
-    def __init__(self, r: dict):
-        self.lib = _load()
+    def __init__(self, r: dict, lib: pathlib.Path = LIB):
+        self.lib = _load(lib)
         self.db = self.lib.ref_tpch_db_new(BASE_DAYS)
         self._keep = []
         self._fill(r)
