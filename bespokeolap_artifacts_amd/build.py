@@ -69,6 +69,8 @@ def build_lib(jobs: int | None = None) -> pathlib.Path:
 
 DRIVER_SRC = PKG / "driver" / "tpch_query.cpp"
 DRIVER = PKG / "bolap_tpch"
+CEB_DRIVER_SRC = PKG / "driver" / "ceb_query.cpp"
+CEB_DRIVER = PKG / "bolap_ceb"
 
 
 def _arrow_dir() -> pathlib.Path | None:
@@ -85,28 +87,34 @@ def _arrow_dir() -> pathlib.Path | None:
 
 
 def build_driver() -> pathlib.Path | None:
-    """The TPC-H load/build/query driver (host C++ over libbolap and Arrow);
-    skipped when Arrow C++ is not installed."""
+    """The TPC-H and CEB load/build/query drivers (host C++ over libbolap and
+    Arrow); skipped when Arrow C++ is not installed."""
     arrow = _arrow_dir()
     if arrow is None:
-        print("bolap_tpch skipped: no Arrow C++ (pyarrow) in this environment")
+        print("bolap_tpch / bolap_ceb skipped: no Arrow C++ (pyarrow) in this environment")
         return None
+    _build_driver(arrow, DRIVER_SRC, DRIVER)
+    _build_driver(arrow, CEB_DRIVER_SRC, CEB_DRIVER)
+    return DRIVER
+
+
+def _build_driver(arrow: pathlib.Path, src: pathlib.Path, out: pathlib.Path) -> pathlib.Path:
     libs = sorted(arrow.glob("libarrow.so.*")) + sorted(arrow.glob("libparquet.so.*"))
     arrow_so = next(x.name for x in libs if x.name.startswith("libarrow.so.") and
                     x.name.count(".") == 2)
     parquet_so = next(x.name for x in libs if x.name.startswith("libparquet.so.") and
                       x.name.count(".") == 2)
-    tmp = DRIVER.with_suffix(".tmp")
+    tmp = out.with_suffix(".tmp")
     cmd = [HIPCC, "-x", "c++", "-std=c++20", "-O2", "-D__HIP_PLATFORM_AMD__",
            "-I/opt/rocm/include", f"-I{ROOT / 'include'}", f"-I{arrow / 'include'}",
-           str(DRIVER_SRC), "-o", str(tmp), f"-L{PKG}", "-lbolap", "-Wl,-rpath,$ORIGIN",
+           str(src), "-o", str(tmp), f"-L{PKG}", "-lbolap", "-Wl,-rpath,$ORIGIN",
            "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", f"-L{arrow}",
            f"-l:{arrow_so}", f"-l:{parquet_so}", f"-Wl,-rpath,{arrow}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"bolap_tpch build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, DRIVER)
-    return DRIVER
+        raise RuntimeError(f"{out.name} build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+    return out
 
 
 def oracle_sources() -> list[pathlib.Path]:
@@ -136,12 +144,15 @@ DROPIN_TPCH = REF_DIR / "libdropin_tpch.so"
 DROPIN_SRC = PKG / "dropin" / "tpch_run_q.cpp"
 
 
-def _ref_compile(out: pathlib.Path, extra: list[str]) -> pathlib.Path:
-    src = REFERENCE / "bespoke_tpch"
+REF_CEB = REF_DIR / "libref_ceb.so"
+
+
+def _ref_compile(out: pathlib.Path, extra: list[str], suite: str = "tpch") -> pathlib.Path:
+    src = REFERENCE / f"bespoke_{suite}"
     tmp = out.with_suffix(".so.tmp")
     cmd = [shutil.which("g++") or "c++", "-std=c++20", "-O2", "-fPIC", "-shared", "-pthread",
            "-w", "-Wl,-Bsymbolic", f"-I{src}", f"-I{_arrow_dir() / 'include'}",
-           str(ORACLE_DIR / "ref_tpch.cpp"), "-o", str(tmp), *extra]
+           str(ORACLE_DIR / f"ref_{suite}.cpp"), "-o", str(tmp), *extra]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"{out.name} build failed:\n{r.stdout}\n{r.stderr}")
@@ -152,7 +163,8 @@ def _ref_compile(out: pathlib.Path, extra: list[str]) -> pathlib.Path:
 def build_ref() -> list[pathlib.Path]:
     """oracle/_ref/: the reference's own TPC-H run_qN / write_qN_csv
     (bespoke_tpch/query_q*.cpp, compiled unchanged from where they lie) behind
-    the C shim oracle/ref_tpch.cpp -- a checker for tests only -- and the same
+    the C shim oracle/ref_tpch.cpp -- a checker for tests only --, its CEB
+    run_qN (bespoke_ceb/query_q*.cpp) behind oracle/ref_ceb.cpp, and the same
     unit with the drop-in GPU bodies (dropin/tpch_run_q.cpp) in place of six
     pipelines, linked against libbolap.  Skipped where the reference sources or
     pyarrow's Arrow headers (which loader_impl.hpp includes; no Arrow library
@@ -162,6 +174,8 @@ def build_ref() -> list[pathlib.Path]:
         return []
     REF_DIR.mkdir(exist_ok=True)
     out = [_ref_compile(REF_TPCH, [])]
+    if (REFERENCE / "bespoke_ceb" / "query_q1a.cpp").exists():
+        out.append(_ref_compile(REF_CEB, [], suite="ceb"))
     if LIB.exists():
         out.append(_ref_compile(DROPIN_TPCH, [
             "-DREF_TPCH_DROPIN", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",

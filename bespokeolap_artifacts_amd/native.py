@@ -98,6 +98,31 @@ class CebResult(C.Structure):
     _fields_ = [("count_lo", C.c_uint64), ("count_hi", C.c_uint64), ("min_year", C.c_int64),
                 ("title_pass", C.c_int64), ("joined_movies", C.c_int64)]
 
+CEB_MAX_TERM_PREDS, CEB_MAX_TITLE_PREDS, CEB_MAX_TERMS = 4, 2, 6
+
+
+class CebPred(C.Structure):
+    _fields_ = [("col", vp), ("is_null", vp), ("mask", vp), ("mask_n", C.c_int64)]
+
+
+class CebTerm(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("movie", vp), ("movie_null", vp), ("n_preds", C.c_int32),
+                ("weight", C.c_uint32), ("preds", CebPred * CEB_MAX_TERM_PREDS)]
+
+
+class CebCountArgs(C.Structure):
+    _fields_ = [("title_rows", C.c_int64), ("t_id", vp), ("t_id_null", vp), ("t_year", vp),
+                ("t_year_null", vp), ("year_lo", C.c_int32), ("year_hi", C.c_int32),
+                ("n_title_preds", C.c_int32), ("movie_cap", C.c_int32),
+                ("title_preds", CebPred * CEB_MAX_TITLE_PREDS), ("n_terms", C.c_int32),
+                ("reserved", C.c_int32), ("terms", CebTerm * CEB_MAX_TERMS)]
+
+
+class CebCountResult(C.Structure):
+    _fields_ = [("count", C.c_int64), ("title_pass", C.c_int64), ("joined", C.c_int64),
+                ("term_rows", C.c_int64 * CEB_MAX_TERMS)]
+
+
 MINMAX_DENSE, MINMAX_HASH = 0, 1
 
 
@@ -403,6 +428,7 @@ PROTOTYPES = {
     "bolap_gen_zipf_i32": (C.c_int, [vp, u64, C.c_uint32, i64, vp, i64, u64, vp]),
     "bolap_gen_price_disc": (C.c_int, [vp, u64, C.c_uint32, i64, vp, vp]),
     "bolap_ceb_run": (C.c_int, [vp, P(CebDb), P(CebQuery), P(CebResult)]),
+    "bolap_ceb_count": (C.c_int, [vp, P(CebCountArgs), P(CebCountResult)]),
     "bolap_ceb_run_batch": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(CebResult)]),
     "bolap_q3_write_csv": (C.c_int, [C.c_char_p, P(GroupRow), i64, vp]),
     "bolap_group_minmax": (C.c_int, [vp, P(MinmaxArgs), vp, i64, P(C.c_int64),

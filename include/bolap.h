@@ -489,6 +489,66 @@ int bolap_ceb_run_batch(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_
 int bolap_ceb_autotune(bolap_ctx* ctx, const bolap_ceb_db* db, const bolap_ceb_query* qs,
                        int32_t n, double* best_ms);
 
+/* ------------------------------------------------------------------------ */
+/* CEB factorised COUNT over the reference's nullable columns.  Replaces the */
+/* scans of run_q1a (bespoke_ceb/query_q1a.cpp:241-621), run_q2a            */
+/* (query_q2a.cpp:240-503), run_q2b, run_q2c and run_q8a: a title row passes */
+/* when its id is non-null and in [0, movie_cap), its production year is    */
+/* non-null and in [year_lo, year_hi] (when t_year is set) and every title   */
+/* predicate holds; a term row passes when its movie passed the title scan,  */
+/* its movie column is non-null and every predicate holds, and adds `weight` */
+/* to its movie's u32 counter.  count = sum over passing title rows (in row  */
+/* order, duplicates included, as the reference loops over its title list)   */
+/* of the product of that movie's counters, as wrapping 64-bit integers.     */
+/* A predicate holds when the column value is non-null (is_null NULL: the    */
+/* column has no nulls) and 0 <= v < mask_n with mask[v] != 0.  All columns  */
+/* and masks are device arrays; columns 4-byte aligned.                      */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_CEB_MAX_TERM_PREDS 4
+#define BOLAP_CEB_MAX_TITLE_PREDS 2
+#define BOLAP_CEB_MAX_TERMS 6
+
+typedef struct {
+    const int32_t* col;
+    const uint8_t* is_null;
+    const uint8_t* mask;
+    int64_t mask_n;
+} bolap_ceb_pred;
+
+typedef struct {
+    int64_t rows;
+    const int32_t* movie;
+    const uint8_t* movie_null;
+    int32_t n_preds;
+    uint32_t weight;            /* added per passing row (1 for COUNT(*)) */
+    bolap_ceb_pred preds[BOLAP_CEB_MAX_TERM_PREDS];
+} bolap_ceb_term;
+
+typedef struct {
+    int64_t title_rows;
+    const int32_t* t_id;
+    const uint8_t* t_id_null;
+    const int32_t* t_year;      /* NULL: no year window */
+    const uint8_t* t_year_null;
+    int32_t year_lo, year_hi;
+    int32_t n_title_preds;
+    int32_t movie_cap;          /* above every title id (the id column's max + 1) */
+    bolap_ceb_pred title_preds[BOLAP_CEB_MAX_TITLE_PREDS];
+    int32_t n_terms;
+    int32_t reserved;
+    bolap_ceb_term terms[BOLAP_CEB_MAX_TERMS];
+} bolap_ceb_count_args;
+
+typedef struct {
+    int64_t count;
+    int64_t title_pass;         /* title rows that passed */
+    int64_t joined;             /* of those, rows whose every counter is nonzero */
+    int64_t term_rows[BOLAP_CEB_MAX_TERMS];   /* rows that passed, per term */
+} bolap_ceb_count_result;
+
+int bolap_ceb_count(bolap_ctx* ctx, const bolap_ceb_count_args* args,
+                    bolap_ceb_count_result* out);
+
 /* Host-side ILIKE over a string dictionary: mask[i] = 1 when string i
    (chars[offsets[i] .. offsets[i+1])) matches pattern, '%' matching any
    run of characters and '_' any one character, ASCII case-insensitive

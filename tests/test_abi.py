@@ -58,7 +58,15 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "Q6Params": "bolap_q6_params", "Q18Args": "bolap_q18_args", "Q18Row": "bolap_q18_row",
     "Q18Stats": "bolap_q18_stats", "Q1Row": "bolap_q1_row", "Q14Args": "bolap_q14_args",
     "Q14Result": "bolap_q14_result", "Q17Args": "bolap_q17_args",
-    "Q17Result": "bolap_q17_result",
+    "Q17Result": "bolap_q17_result", "OrderSide": "bolap_order_side", "Q4Args": "bolap_q4_args",
+    "Q5Args": "bolap_q5_args", "Q10Args": "bolap_q10_args", "Q10Row": "bolap_q10_row",
+    "Q12Args": "bolap_q12_args", "Q12Result": "bolap_q12_result", "Q19Group": "bolap_q19_group",
+    "Q19Args": "bolap_q19_args", "Q21Args": "bolap_q21_args", "Q22Args": "bolap_q22_args",
+    "Q22Result": "bolap_q22_result", "Q7Args": "bolap_q7_args", "Q8Args": "bolap_q8_args",
+    "Q9Args": "bolap_q9_args", "Q13Args": "bolap_q13_args", "Q16Args": "bolap_q16_args",
+    "MinmaxArgs": "bolap_minmax_args", "MinmaxRow": "bolap_minmax_row",
+    "MinmaxStats": "bolap_minmax_stats", "CebPred": "bolap_ceb_pred", "CebTerm": "bolap_ceb_term",
+    "CebCountArgs": "bolap_ceb_count_args", "CebCountResult": "bolap_ceb_count_result",
 }
 
 
