@@ -31,13 +31,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workloads", default="c2,c1,c3,q6,q14,q17,q18,c4,c5,minmax,sort,copy")
+    ap.add_argument("--workloads",
+                    default="c2,c1,c3,q6,q14,q17,q18,c4,c5,minmax,sort,copy,tpch,ceb_count")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c4-train", type=int, default=20_000, help="C4 training plans")
     ap.add_argument("--c4-eval", type=int, default=500, help="C4 held-out plans")
     ap.add_argument("--c5-queries", type=int, default=512, help="C5 batch size")
+    ap.add_argument("--tpch-orders", type=int, default=15_000_000,
+                    help="orders of the TPC-H database (15,000,000 = SF10)")
     return ap.parse_args()
 
 
@@ -383,6 +386,299 @@ def bench_q18(eng, torch, args):
     return res
 
 
+# ------------------------------------- TPC-H order-range and join pipelines
+def _tpch_upload(torch, h):
+    """The reference-layout columns of tests/tpch_data.build() on the device
+    (dictionary codes as bytes, unsigned columns as their signed views)."""
+    u8 = ("l_shipmode", "l_shipinstruct", "o_orderpriority", "o_orderstatus", "p_type",
+          "p_brand", "p_container")
+    t = {}
+    for k, v in h.items():
+        if not isinstance(v, np.ndarray):
+            t[k] = v
+            continue
+        a = v.astype(np.uint8) if k in u8 else v
+        if a.dtype == np.uint32:
+            a = a.view(np.int32)
+        elif a.dtype == np.uint16:
+            a = a.view(np.int16)
+        t[k] = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t
+
+
+def bench_tpch(eng, torch, args):
+    """TPC-H Q4, Q5, Q7, Q8, Q9, Q10, Q12, Q13, Q16, Q19, Q21, Q22 over one
+    seeded SF10-sized database in the reference layout (15,000,000 orders
+    sorted by date with lineitem ranges, ~60,000,000 lineitem rows in order-key
+    order, 2,000,000 parts, 8,000,000 partsupp rows, 1,500,000 customers,
+    100,000 suppliers).  Per query: the whole call (HIP events, results copied
+    back), the traced kernel stages, the algorithmic bytes (the reference
+    layout's bytes of the columns the query reads over the rows it must
+    visit: the date window's orders and their lineitem rows for the window
+    queries, whole columns for the scans), and the oracle on one pinned core
+    over the same columns."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from tests import tpch_data as T
+    import oracle
+    orders = args.tpch_orders
+    log(f"tpch: generating {orders:,} orders")
+    r = T.raw(SEED, orders, parts=max(orders // 15 * 2, 1000), custs=max(orders // 10, 1000),
+              supps=max(orders // 150, 100), strings=False)
+    h = T.build(r)
+    del r
+    t = _tpch_upload(torch, h)
+    n_li, n_o = len(h["l_orderkey"]), len(h["o_orderkey"])
+    n_p, n_ps, n_c, n_s = (len(h["p_partkey"]), len(h["ps_partkey"]), len(h["c_custkey"]),
+                           len(h["s_suppkey"]))
+    od = h["o_orderdate"]
+    li_per_order = (h["r_end"].astype(np.int64) - h["r_start"]).astype(np.int64)
+
+    def window(lo, hi):
+        a, b = np.searchsorted(od, lo), np.searchsorted(od, hi)
+        return int(b - a), int(li_per_order[a:b].sum())
+
+    d = T.date_offset
+    am = T.add_months
+    nls = len(h["ls_dict"])
+    rc = h["rf_dict"].index("R") * nls
+    q7d = (d("1995-01-01"), d("1996-12-31"), d("1996-01-01"))
+    inreg2 = (np.array(h["n_regionkey"]) == 2).astype(np.uint8)
+    inreg1 = (np.array(h["n_regionkey"]) == 1).astype(np.uint8)
+    tc8 = h["type_dict"].index("ECONOMY ANODIZED STEEL")
+    p_ok = (h["p_name_words"] == T.COLORS.index("green")).any(axis=1).astype(np.uint8)
+    p_ok_dev = torch.from_numpy(p_ok).cuda()
+    masks = eng.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
+    tb16 = np.array([x.startswith("MEDIUM POLISHED") for x in h["type_dict"]], np.uint8)
+    size16 = sum(1 << s_ for s_ in [49, 14, 23, 45, 19, 3, 36, 9])
+    npk, nsk = int(h["p_partkey"].max()) + 1, int(h["s_suppkey"].max()) + 1
+    b45 = h["brand_dict"].index("Brand#45")
+    m12 = [h["shipmode_dict"].index(x) for x in ("MAIL", "SHIP")]
+    hc = [h["priority_dict"].index(x) for x in ("1-URGENT", "2-HIGH")]
+    from tests.test_tpch2_oracle import q19_groups
+    g19 = q19_groups(h, ("Brand#12", "Brand#23", "Brand#34"), (1, 10, 20))
+    m19 = (h["shipmode_dict"].index("AIR"), h["shipmode_dict"].index("REG AIR"),
+           h["instruct_dict"].index("DELIVER IN PERSON"))
+    fcode = h["status_dict"].index("F")
+    codes22 = [13, 31, 23, 29, 30, 18, 17]
+    lo94, hi94q, hi94y = d("1994-01-01"), am("1994-01-01", 3), am("1994-01-01", 12)
+    w4, w10 = window(lo94, hi94q), window(lo94, hi94q)
+    w5, w8 = window(lo94, hi94y), window(q7d[0], q7d[1] + 1)
+    # (name, gpu call, oracle call, algorithmic bytes, parity check of the results)
+    Q = [
+        ("q4", lambda: eng.q4(t, 5, lo94, hi94q), lambda: oracle.q4(h, lo94, hi94q, 5),
+         w4[0] * 11 + w4[1] * 4, lambda g, o: g == o.tolist()),
+        ("q5", lambda: eng.q5(t, inreg2, lo94, hi94y), lambda: oracle.q5(h, inreg2, lo94, hi94y),
+         w5[0] * 12 + w5[1] * 6, lambda g, o: g == o.tolist()),
+        ("q10", lambda: eng.q10(t, rc, nls, lo94, hi94q, 25, raw=True),
+         lambda: oracle.q10(h, rc, nls, lo94, hi94q, 25), w10[0] * 14 + w10[1] * 5,
+         lambda g, o: [tuple(x) for x in g.tolist()] == [tuple(x) for x in o.tolist()]),
+        ("q12", lambda: eng.q12(t, m12[0], m12[1], lo94, hi94y, *hc),
+         lambda: oracle.q12(h, m12[0], m12[1], lo94, hi94y, *hc), n_li * 7,
+         lambda g, o: g == o),
+        ("q19", lambda: eng.q19(t, g19, *m19), lambda: oracle.q19(h, g19, *m19),
+         n_li * 8 + n_p * 10, lambda g, o: g == o),
+        ("q7", lambda: eng.q7(t, 6, 7, *q7d), lambda: oracle.q7(h, 6, 7, *q7d),
+         n_o * 10 + n_li * 10, lambda g, o: g == o.reshape(-1).tolist()),
+        ("q8", lambda: eng.q8(t, inreg1, 2, tc8, *q7d),
+         lambda: oracle.q8(h, inreg1, 2, tc8, *q7d), w8[0] * 14 + w8[1] * 13,
+         lambda g, o: g == (o[0].tolist(), o[1].tolist())),
+        ("q9", lambda: eng.q9(t, p_ok_dev, 25), lambda: oracle.q9(h, p_ok, 25),
+         n_li * 20 + n_ps * 12 + n_o * 2 + n_p * 5,
+         lambda g, o: g[0] == o[0] and np.array_equal(g[1], o[1]) and np.array_equal(g[2], o[2])),
+        ("q13", lambda: eng.q13(t, "special", "requests", masks=masks),
+         lambda: oracle.q13(h, "special", "requests"), n_o * 20,
+         lambda g, o: g == (o[0], o[1].tolist())),
+        ("q16", lambda: eng.q16(t, len(h["brand_dict"]), b45, tb16, size16, npk, nsk),
+         lambda: oracle.q16(h, b45, tb16, size16, npk, nsk), n_p * 10 + n_ps * 8 + n_s * 5,
+         lambda g, o: g == o.tolist()),
+        ("q21", lambda: eng.q21(t, fcode, 7, nsk), lambda: oracle.q21(h, fcode, 7, nsk),
+         n_o * 13 + n_li * 8, lambda g, o: g == o.tolist()),
+        ("q22", lambda: eng.q22(t, codes22, [(ord("A") << 8) | ord("B")]),
+         lambda: oracle.q22(h, codes22, [(ord("A") << 8) | ord("B")]), n_c * 13 + n_o * 4,
+         lambda g, o: g == (o[0].tolist(), o[1].tolist())),
+    ]
+    out = {"workload": f"TPC-H SF10 reference layout: {n_o:,} orders, {n_li:,} lineitem rows, "
+                       f"{n_p:,} parts, {n_ps:,} partsupp, {n_c:,} customers, {n_s:,} suppliers",
+           "queries": {}}
+    for name, gpu, cpu, alg, same in Q:
+        log("tpch", name)
+        first = gpu()
+        if isinstance(first, np.ndarray):
+            first = first.copy()
+        ms = time_steps(torch, gpu, args.steps, args.warmup)
+        eng.set_trace(True)
+        for _ in range(3):
+            gpu()
+        torch.cuda.synchronize()
+        tr = eng.read_trace()
+        eng.set_trace(False)
+        stages = sorted({n for n, m, _ in tr if m >= 0})
+        kernel_ms = sum(stage_ms(tr, s_) for s_ in stages)
+        res = {"ms_per_step": ms, "algorithmic_bytes": int(alg), "GBps": alg / (ms * 1e-3) / 1e9,
+               "kernel_ms": kernel_ms, "kernel_GBps": alg / (kernel_ms * 1e-3) / 1e9
+               if kernel_ms else None,
+               "stages_ms": {s_: stage_ms(tr, s_) for s_ in stages}}
+        if not args.no_cpu:
+            want = cpu()
+            res["parity"] = bool(same(first, want))
+            med, reps = cpu_time(cpu, min(args.cpu_seconds, 2.0))
+            res["cpu_baseline"] = cpu_bl({"value": alg / med / 1e9, "unit": "GB/s",
+                                          "kind": "port", "ms": med * 1e3,
+                                          "sample": f"full SF10 columns, median of {reps} runs"})
+            res["speedup_vs_cpu"] = med * 1e3 / ms
+        out["queries"][name] = res
+        log(name, json.dumps(res))
+    del t
+    torch.cuda.empty_cache()
+    return out
+
+
+# ---------------------------------------------- CEB factorised COUNT (2a)
+def bench_ceb_count(eng, torch, args):
+    """CEB 2a at the IMDB sizes of the C5 configuration: title 2,500,000 rows
+    (kind 3% null, production_year 10% null), movie_info 15,000,000 (2 terms:
+    (info_type, info) pairs), cast_info 36,000,000 (role 3% null, persons past
+    the name table), movie_keyword 4,500,000, name 4,000,000 (gender 20%
+    null), in the reference builder's row orders.  The whole bolap_ceb_count call over device-resident columns and
+    masks; the CPU baseline is the reference's own run_q2a (oracle/_ref,
+    compiled from its sources) on one pinned core over the same tables."""
+    rng = np.random.default_rng(SEED)
+    T_, P, K = 2_500_000, 4_000_000, 130_000
+    MI, CI, MK = 15_000_000, 36_000_000, 4_500_000
+    kinds = ["movie", "tv series", "tv movie", "video movie", "tv mini series", "video game",
+             "episode"]
+    roles = ["actor", "actress", "producer", "writer", "cinematographer", "composer",
+             "costume designer", "director", "editor", "miscellaneous crew",
+             "production designer", "guest"]
+    vocab = [f"value {i}" for i in range(2000)]     # movie_info.info strings
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+
+    def nulls(n, p):
+        return rng.random(n) < p
+
+    def movies(n):
+        return i32(1 + np.floor(T_ * rng.random(n) ** 2))
+
+    title = {"id": i32(np.arange(1, T_ + 1)), "kind": i32(rng.integers(1, 8, T_)),
+             "kind_null": nulls(T_, 0.03), "year": i32(rng.integers(1900, 2020, T_)),
+             "year_null": nulls(T_, 0.10)}
+    gender = rng.integers(0, 5, P)                      # 0 m, 1 f, 2.. (20% null at 4)
+    mi = {"movie": movies(MI), "type": i32(rng.integers(1, 9, MI)),
+          "info": i32(rng.integers(0, len(vocab), MI)), "info_null": nulls(MI, 0.02)}
+    ci = {"movie": movies(CI), "person": i32(rng.integers(1, P + 100_000, CI)),
+          "role": i32(rng.integers(1, 13, CI)), "role_null": nulls(CI, 0.03)}
+    mk = {"movie": movies(MK), "kw": i32(rng.integers(1, K + 1, MK))}
+    # the reference builder's row orders (bespoke_ceb/builder_impl.cpp:432-452):
+    # movie_info by (movie_id, info_type_id), cast_info by (movie_id, role_id),
+    # movie_keyword by keyword_id
+    o = np.lexsort((mi["type"], mi["movie"]))
+    mi = {k: np.ascontiguousarray(v[o]) for k, v in mi.items()}
+    o = np.lexsort((np.where(ci["role_null"], 0, ci["role"]), ci["movie"]))
+    ci = {k: np.ascontiguousarray(v[o]) for k, v in ci.items()}
+    o = np.argsort(mk["kw"], kind="stable")
+    mk = {k: np.ascontiguousarray(v[o]) for k, v in mk.items()}
+    # the query: ID1 (3), ID2 (8), INFO1 (10 values), INFO2 (25 values), KIND, ROLE, GENDER
+    info1, info2 = list(range(0, 20, 2)), list(range(100, 125))
+    kind_ok = np.zeros(9, np.uint8); kind_ok[[1, 7]] = 1                  # movie, episode
+    role_ok = np.zeros(14, np.uint8); role_ok[[1, 2]] = 1                 # actor, actress
+    person_ok = np.zeros(P + 1, np.uint8); person_ok[1:][gender == 1] = 1  # f
+    type_ok1 = np.zeros(114, np.uint8); type_ok1[3] = 1
+    type_ok2 = np.zeros(114, np.uint8); type_ok2[8] = 1
+    v1 = np.zeros(len(vocab), np.uint8); v1[info1] = 1
+    v2 = np.zeros(len(vocab), np.uint8); v2[info2] = 1
+    kw_ok = np.ones(K + 1, np.uint8); kw_ok[0] = 0
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    dn = lambda m: dev(m.astype(np.uint8))
+    P_ = eng.ceb_pred
+    g = {k: dev(v) for k, v in {"tid": title["id"], "tk": title["kind"], "ty": title["year"],
+                                 "mim": mi["movie"], "mit": mi["type"], "mii": mi["info"],
+                                 "cim": ci["movie"], "cip": ci["person"], "cir": ci["role"],
+                                 "mkm": mk["movie"], "mkk": mk["kw"]}.items()}
+    gn = {"tk": dn(title["kind_null"]), "ty": dn(title["year_null"]),
+          "mii": dn(mi["info_null"]), "cir": dn(ci["role_null"])}
+    gm = {k: dev(v) for k, v in {"kind": kind_ok, "role": role_ok, "person": person_ok,
+                                 "t1": type_ok1, "t2": type_ok2, "v1": v1, "v2": v2,
+                                 "kw": kw_ok}.items()}
+    a = eng.ceb_count_args(
+        {"id": g["tid"], "year": g["ty"], "year_null": gn["ty"],
+         "preds": [P_(g["tk"], gm["kind"], gn["tk"])]},
+        [{"movie": g["mim"], "preds": [P_(g["mit"], gm["t1"]), P_(g["mii"], gm["v1"], gn["mii"])]},
+         {"movie": g["mim"], "preds": [P_(g["mit"], gm["t2"]), P_(g["mii"], gm["v2"], gn["mii"])]},
+         {"movie": g["cim"], "preds": [P_(g["cir"], gm["role"], gn["cir"]),
+                                       P_(g["cip"], gm["person"])]},
+         {"movie": g["mkm"], "preds": [P_(g["mkk"], gm["kw"])]}],
+        1980, 2015, T_ + 1)
+    first = eng.ceb_count(a)
+    ms = time_steps(torch, lambda: eng.ceb_count(a), args.steps, args.warmup)
+    eng.set_trace(True)
+    for _ in range(3):
+        eng.ceb_count(a)
+    torch.cuda.synchronize()
+    tr = eng.read_trace()
+    eng.set_trace(False)
+    term_ms = [m for n, m, _ in tr if n == "ceb_term" and m >= 0]
+    # columns read once each: title id/kind/year (+2 null bytes), movie_info
+    # movie/type/info (+1), cast_info movie/person/role (+1), movie_keyword 2
+    alg = T_ * 14 + MI * 13 + CI * 13 + MK * 8
+    res = {"workload": f"CEB 2a factorised COUNT: title {T_:,}, movie_info {MI:,} (x2 terms), "
+                       f"cast_info {CI:,}, movie_keyword {MK:,}, name {P:,}",
+           "ms_per_step": ms, "algorithmic_bytes": alg, "GBps": alg / (ms * 1e-3) / 1e9,
+           "kernel": "term_k", "count": first["count"], "joined": first["joined"],
+           "stages_ms": {n: stage_ms(tr, n) for n in ("ceb_title", "ceb_term", "ceb_product")},
+           "term_ms_per_call": float(np.sum(term_ms) / 3) if term_ms else None}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from tests import ceb_ref as CR
+        if CR.available():
+            def strs(ids, table):
+                enc = [s.encode() for s in table]
+                lens = np.array([len(e) for e in enc], np.int64)
+                blob = np.frombuffer(b"".join(enc), np.uint8)
+                start = np.concatenate([[0], np.cumsum(lens)[:-1]])
+                ln = lens[ids]
+                off = np.zeros(len(ids) + 1, np.int64)
+                np.cumsum(ln, out=off[1:])
+                idx = np.repeat(start[ids] - off[:-1], ln) + np.arange(off[-1])
+                return {"chars": blob[idx], "offsets": off}
+            gid = np.where(gender == 4, 0, gender)
+            gs = strs(gid, ["m", "f", "x", "y"])
+            gs["null"] = gender == 4
+            mis = strs(mi["info"], vocab)
+            mis["null"] = mi["info_null"]
+            tabs = {
+                "kind_type": {"id": (i32(np.arange(1, 8)), None), "kind": kinds},
+                "role_type": {"id": (i32(np.arange(1, 13)), None), "role": roles},
+                "info_type": {"id": (i32(np.arange(1, 114)), None)},
+                "keyword": {"id": (i32(np.arange(1, K + 1)), None)},
+                "name": {"id": (i32(np.arange(1, P + 1)), None), "gender": gs},
+                "title": {"id": (title["id"], None), "kind_id": (title["kind"], title["kind_null"]),
+                          "production_year": (title["year"], title["year_null"])},
+                "movie_info": {"movie_id": (mi["movie"], None), "info_type_id": (mi["type"], None),
+                               "info": mis},
+                "cast_info": {"movie_id": (ci["movie"], None), "person_id": (ci["person"], None),
+                              "role_id": (ci["role"], ci["role_null"])},
+                "movie_keyword": {"movie_id": (mk["movie"], None),
+                                  "keyword_id": (mk["kw"], None)}}
+            ref = CR.RefCeb(tabs)
+            lst = lambda v: "(" + ", ".join(f"'{x}'" for x in v) + ")"
+            line = (f"2a ('3') ('8') {lst(vocab[i] for i in info1)} {lst(vocab[i] for i in info2)} "
+                    f"('movie', 'episode') ('actor', 'actress') ('f') \"2015\" \"1980\"")
+            n_ref = []
+
+            def run_ref():
+                n, ms_ = ref.count_timed(line)
+                n_ref.append(n)
+            med, reps = cpu_time(run_ref, min(args.cpu_seconds, 4.0))
+            res["parity"] = all(n == first["count"] for n in n_ref)
+            res["cpu_baseline"] = cpu_bl({"value": alg / med / 1e9, "unit": "GB/s",
+                                          "kind": "reference", "ms": med * 1e3,
+                                          "sample": f"the same tables, run_q2a, median of "
+                                                    f"{reps} runs"})
+            res["speedup_vs_cpu"] = med * 1e3 / ms
+            ref.close()
+    return res
+
+
 # ------------------------------------------- CEB q10a grouped MIN/MAX (a13)
 def bench_minmax(eng, torch, args):
     """Grouped MIN/MAX(year) over 4,000,000 (person, movie) join rows expanded
@@ -703,7 +999,8 @@ def main():
                       "q14": lambda e, t, a: bench_q14_q17(e, t, a, "q14"),
                       "q17": lambda e, t, a: bench_q14_q17(e, t, a, "q17"),
                       "q18": bench_q18, "c4": bench_c4, "c5": bench_c5,
-                      "minmax": bench_minmax, "sort": bench_sort,
+                      "minmax": bench_minmax, "sort": bench_sort, "tpch": bench_tpch,
+                      "ceb_count": bench_ceb_count,
                       "copy": bench_copy}[w](eng, torch, args)
         log(json.dumps(results[w]))
     for r in results.values():

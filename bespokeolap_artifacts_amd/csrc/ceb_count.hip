@@ -9,16 +9,17 @@
 // masks; COUNT(*) is the sum, over the passing title rows, of the product of
 // their movie's term counts.  Three kinds of launch:
 //
-//   title_k   one pass over title: a pass flag per title row and a bit per
-//             passing movie id in a dense bitmap (the reference's
-//             title_match / info_both_mask arrays, here one bit per movie so
-//             that the fact scans' random probes hit a 2.5 M-movie set that
-//             stays in L2: 312 KB);
-//   term_k    one pass per term over its fact columns, four rows per lane
+//   title_k   one pass over title: a pass flag per title row and a pass byte
+//             per movie id (plain byte stores), then pack_k folds the bytes
+//             into one bit per movie (the reference's title_match /
+//             info_both_mask arrays, here a bitmap so that the fact scans'
+//             random probes hit a 2.5 M-movie set that stays in L2: 312 KB);
+//   term_k    one pass per fact table over its columns, four rows per lane
 //             with 16-byte loads of the movie column; a row probes the movie
 //             bit first and reads its predicate columns only when the movie
-//             passed; a passing row adds the term's weight to the movie's
-//             u32 counter with a device atomic;
+//             passed; for each of the table's terms (up to two share one
+//             pass: 1a / 2a read movie_info twice) a passing row adds the
+//             term's weight to the movie's u32 counter with a device atomic;
 //   product_k one pass over the title flags: the product of the movie's
 //             counters (u64, wrapping as the reference's int64 products do)
 //             summed by DPP wave sums and one atomic per wave.
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(kT) void title_k(int64_t rows, const int32_t* __res
                                               const uint8_t* __restrict__ year_null,
                                               int32_t year_lo, int32_t year_hi, PredSet preds,
                                               int32_t cap, uint8_t* __restrict__ flags,
-                                              uint32_t* __restrict__ bits,
+                                              uint8_t* __restrict__ movie_pass,
                                               unsigned long long* __restrict__ n_pass) {
     uint64_t mine = 0;
     for (int64_t r = int64_t(blockIdx.x) * kT + threadIdx.x; r < rows;
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(kT) void title_k(int64_t rows, const int32_t* __res
         pass = pass && preds_pass(preds, r);
         flags[r] = pass ? 1 : 0;
         if (pass) {
-            atomicOr(bits + (uint32_t(m) >> 5), 1u << (uint32_t(m) & 31));
+            movie_pass[m] = 1;   // plain byte store: every writer stores the same value
             ++mine;
         }
     }
@@ -89,14 +90,39 @@ __global__ __launch_bounds__(kT) void title_k(int64_t rows, const int32_t* __res
     if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(n_pass, (unsigned long long)s);
 }
 
-template <int NP>
+// One bit per movie from the pass bytes: word w covers movies 32w .. 32w+31.
+__global__ __launch_bounds__(kT) void pack_k(const uint8_t* __restrict__ movie_pass,
+                                             int64_t words, uint32_t* __restrict__ bits) {
+    for (int64_t w = int64_t(blockIdx.x) * kT + threadIdx.x; w < words;
+         w += int64_t(gridDim.x) * kT) {
+        const auto q = load_unit<uint32_t, 8>(reinterpret_cast<const uint32_t*>(movie_pass), w);
+        uint32_t b = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t v = q.v[k];
+            b |= ((v & 0xffu) ? 1u : 0u) << (4 * k) | ((v & 0xff00u) ? 1u : 0u) << (4 * k + 1) |
+                 ((v & 0xff0000u) ? 1u : 0u) << (4 * k + 2) |
+                 ((v & 0xff000000u) ? 1u : 0u) << (4 * k + 3);
+        }
+        bits[w] = b;
+    }
+}
+
+// Up to two terms over the same fact rows (the same movie column): the
+// movie column is read and its bit probed once for both.
+struct TermGroup {
+    int32_t n;
+    PredSet ps[2];
+    uint32_t weight[2];
+    uint32_t* counts[2];
+    unsigned long long* n_rows[2];
+};
+
 __global__ __launch_bounds__(kT) void term_k(int64_t rows, const int32_t* __restrict__ movie,
                                              const uint8_t* __restrict__ movie_null,
-                                             PredSet preds, uint32_t weight, int32_t cap,
-                                             const uint32_t* __restrict__ bits,
-                                             uint32_t* __restrict__ counts,
-                                             unsigned long long* __restrict__ n_rows) {
-    uint64_t mine = 0;
+                                             TermGroup g, int32_t cap,
+                                             const uint32_t* __restrict__ bits) {
+    uint64_t mine0 = 0, mine1 = 0;
     const int64_t units = (rows + 3) / 4;
     const bool vec = (reinterpret_cast<uintptr_t>(movie) & 15) == 0;
     for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units;
@@ -116,16 +142,21 @@ __global__ __launch_bounds__(kT) void term_k(int64_t rows, const int32_t* __rest
             const int32_t m = mv[k];
             if (r >= rows || m < 0 || m >= cap || !movie_bit(bits, m)) continue;
             if (movie_null && movie_null[r]) continue;
-            bool pass = true;
-#pragma unroll
-            for (int i = 0; i < NP; ++i) pass = pass && pred_pass(preds.p[i], r);
-            if (!pass) continue;
-            atomicAdd(counts + m, weight);
-            ++mine;
+            if (preds_pass(g.ps[0], r)) {
+                atomicAdd(g.counts[0] + m, g.weight[0]);
+                ++mine0;
+            }
+            if (g.n > 1 && preds_pass(g.ps[1], r)) {
+                atomicAdd(g.counts[1] + m, g.weight[1]);
+                ++mine1;
+            }
         }
     }
-    const uint64_t s = wave_sum_u64(mine);
-    if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(n_rows, (unsigned long long)s);
+    const uint64_t s0 = wave_sum_u64(mine0), s1 = wave_sum_u64(mine1);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (s0) atomicAdd(g.n_rows[0], (unsigned long long)s0);
+        if (s1) atomicAdd(g.n_rows[1], (unsigned long long)s1);
+    }
 }
 
 struct Terms {
@@ -189,14 +220,6 @@ PredSet pred_set(const bolap_ceb_pred* p, int32_t n) {
     return s;
 }
 
-template <int NP>
-void launch_term(bolap_ctx* ctx, const bolap_ceb_term& t, const PredSet& ps, int32_t cap,
-                 const uint32_t* bits, uint32_t* counts, unsigned long long* n_rows) {
-    hipLaunchKernelGGL(term_k<NP>, dim3(grid_for(ctx, (t.rows + 3) / 4)), dim3(kT), 0,
-                       ctx->stream, t.rows, t.movie, t.movie_null, ps, t.weight, cap, bits,
-                       counts, n_rows);
-}
-
 }  // namespace
 }  // namespace bolap
 
@@ -231,44 +254,57 @@ extern "C" int bolap_ceb_count(bolap_ctx* ctx, const bolap_ceb_count_args* a,
     const int64_t words = (int64_t(cap) + 31) / 32 + 1;
     WsArena ws;
     const size_t o_bits = ws.plan(size_t(words) * 4);
+    const size_t o_pass = ws.plan(size_t(words) * 32);
     const size_t o_flags = ws.plan(size_t(a->title_rows) + 4);
     const size_t o_counts = ws.plan(size_t(a->n_terms) * size_t(cap + 1) * 4);
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     uint32_t* bits = ws.at<uint32_t>(o_bits);
+    uint8_t* movie_pass = ws.at<uint8_t>(o_pass);
     uint8_t* flags = ws.at<uint8_t>(o_flags);
     uint32_t* counts = ws.at<uint32_t>(o_counts);
     // result slots: [0] title passes, [1] sum, [2] joined, [3..] term rows
     auto* res = static_cast<unsigned long long*>(ctx->result_dev);
     const int n_res = 3 + BOLAP_CEB_MAX_TERMS;
     BOLAP_TRY(zero_words(ctx, res, n_res * 2));
-    BOLAP_TRY(zero_words(ctx, bits, words));
+    BOLAP_TRY(zero_words(ctx, movie_pass, words * 8));
     BOLAP_TRY(zero_words(ctx, counts, int64_t(a->n_terms) * (cap + 1)));
     if (a->title_rows > 0) {
         TraceScope ts(ctx, "ceb_title");
         hipLaunchKernelGGL(title_k, dim3(grid_for(ctx, a->title_rows)), dim3(kT), 0, ctx->stream,
                            a->title_rows, a->t_id, a->t_id_null, a->t_year, a->t_year_null,
                            a->year_lo, a->year_hi, pred_set(a->title_preds, a->n_title_preds),
-                           cap, flags, bits, res);
+                           cap, flags, movie_pass, res);
         BOLAP_TRY(launch_check("title_k"));
     }
+    hipLaunchKernelGGL(pack_k, dim3(grid_for(ctx, words)), dim3(kT), 0, ctx->stream, movie_pass,
+                       words, bits);
+    BOLAP_TRY(launch_check("pack_k"));
     Terms terms{};
     terms.n = a->n_terms;
-    for (int t = 0; t < a->n_terms; ++t) {
+    for (int t = 0; t < a->n_terms; ++t) terms.counts[t] = counts + size_t(t) * size_t(cap + 1);
+    // terms over the same movie column share one scan, two at a time
+    for (int t = 0; t < a->n_terms;) {
         const bolap_ceb_term& tm = a->terms[t];
-        uint32_t* c = counts + size_t(t) * size_t(cap + 1);
-        terms.counts[t] = c;
-        if (tm.rows == 0 || a->title_rows == 0) continue;
-        TraceScope ts(ctx, "ceb_term");
-        const PredSet ps = pred_set(tm.preds, tm.n_preds);
-        switch (tm.n_preds) {
-            case 0: launch_term<0>(ctx, tm, ps, cap, bits, c, res + 3 + t); break;
-            case 1: launch_term<1>(ctx, tm, ps, cap, bits, c, res + 3 + t); break;
-            case 2: launch_term<2>(ctx, tm, ps, cap, bits, c, res + 3 + t); break;
-            case 3: launch_term<3>(ctx, tm, ps, cap, bits, c, res + 3 + t); break;
-            default: launch_term<4>(ctx, tm, ps, cap, bits, c, res + 3 + t); break;
+        TermGroup g{};
+        g.n = 1;
+        if (t + 1 < a->n_terms && a->terms[t + 1].movie == tm.movie &&
+            a->terms[t + 1].rows == tm.rows && a->terms[t + 1].movie_null == tm.movie_null)
+            g.n = 2;
+        for (int k = 0; k < g.n; ++k) {
+            const bolap_ceb_term& x = a->terms[t + k];
+            g.ps[k] = pred_set(x.preds, x.n_preds);
+            g.weight[k] = x.weight;
+            g.counts[k] = const_cast<uint32_t*>(terms.counts[t + k]);
+            g.n_rows[k] = res + 3 + t + k;
         }
-        BOLAP_TRY(launch_check("term_k"));
+        if (tm.rows > 0 && a->title_rows > 0) {
+            TraceScope ts(ctx, "ceb_term");
+            hipLaunchKernelGGL(term_k, dim3(grid_for(ctx, (tm.rows + 3) / 4)), dim3(kT), 0,
+                               ctx->stream, tm.rows, tm.movie, tm.movie_null, g, cap, bits);
+            BOLAP_TRY(launch_check("term_k"));
+        }
+        t += g.n;
     }
     if (a->title_rows > 0) {
         TraceScope ts(ctx, "ceb_product");

@@ -534,7 +534,9 @@ class Engine:
         return list(out)[:len(nr)]
 
     def q10(self, t: dict, code_lo: int, code_span: int, date_lo: int, date_hi: int,
-            n_nation_names: int, max_rows: int | None = None):
+            n_nation_names: int, max_rows: int | None = None, raw: bool = False):
+        """Rows (custkey, cust_row, revenue) in the reference's order; raw=True
+        returns them as a numpy record array (a view of a reused buffer)."""
         a = N.Q10Args()
         a.orders = self.order_side(t)
         a.o_custkey, a.l_code = ptr(t["o_custkey"]), ptr(t["l_code"])
@@ -544,11 +546,20 @@ class Engine:
         a.cust_rows = int(t["c_custkey"].numel())
         a.c_custkey, a.c_nationkey = ptr(t["c_custkey"]), ptr(t["c_nationkey"])
         a.n_nation_names = int(n_nation_names)
+        import numpy as np
         cap = max(1, a.cust_rows if max_rows is None else max_rows)
-        rows = (N.Q10Row * cap)()
+        if getattr(self, "_q10_rows_cap", 0) < cap:   # reused host buffer
+            self._q10_rows = (N.Q10Row * cap)()
+            self._q10_rows_cap = cap
+        rows = self._q10_rows
         n = C.c_int64(0)
         N.check(self.lib.bolap_q10(self.ctx, C.byref(a), rows, cap, C.byref(n)))
-        return [(rows[i].custkey, rows[i].cust_row, rows[i].revenue) for i in range(n.value)]
+        rec = np.frombuffer(rows, dtype=np.dtype([("custkey", "<i4"), ("cust_row", "<i4"),
+                                                  ("revenue", "<i8")]), count=n.value)
+        if raw:
+            return rec
+        return list(zip(rec["custkey"].tolist(), rec["cust_row"].tolist(),
+                        rec["revenue"].tolist()))
 
     def q12(self, t: dict, mode1: int, mode2: int, date_lo: int, date_hi: int, high1: int,
             high2: int):
@@ -594,9 +605,11 @@ class Engine:
         a.l_suppkey, a.l_commit_receipt = ptr(t["l_suppkey"]), ptr(t["l_commit_receipt"])
         a.supp_rows = int(t["s_suppkey"].numel())
         a.s_suppkey, a.s_nationkey, a.nation = ptr(t["s_suppkey"]), ptr(t["s_nationkey"]), nation
-        out = (C.c_int64 * max(1, n_suppkeys))()
-        N.check(self.lib.bolap_q21(self.ctx, C.byref(a), out, int(n_suppkeys)))
-        return list(out)[:n_suppkeys]
+        import numpy as np
+        out = np.zeros(max(1, n_suppkeys), np.int64)
+        N.check(self.lib.bolap_q21(self.ctx, C.byref(a), out.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   int(n_suppkeys)))
+        return out[:n_suppkeys].tolist()
 
     def q22(self, t: dict, numeric_codes, extra_pairs):
         a = N.Q22Args()
@@ -720,9 +733,45 @@ class Engine:
         a.supp_rows = int(t["s_suppkey"].numel())
         a.s_suppkey, a.s_banned, a.n_supp_keys = ptr(t["s_suppkey"]), ptr(t["s_banned"]), n_supp_keys
         n = n_brands * len(tb) * 51
-        out = (C.c_int64 * n)()
-        N.check(self.lib.bolap_q16(self.ctx, C.byref(a), out, n))
-        return list(out)
+        out = np.zeros(n, np.int64)
+        N.check(self.lib.bolap_q16(self.ctx, C.byref(a), out.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   n))
+        return out.tolist()
+
+    @staticmethod
+    def ceb_pred(col, mask, is_null=None) -> "N.CebPred":
+        """A predicate over a device int32 column: value v passes when it is
+        non-null and mask[v] != 0 (mask: a device uint8 tensor)."""
+        return N.CebPred(ptr(col), ptr(is_null), ptr(mask), int(mask.numel()))
+
+    def ceb_count_args(self, title: dict, terms: list, year_lo: int, year_hi: int,
+                       movie_cap: int) -> "N.CebCountArgs":
+        """title: {"id", "id_null"?, "year"?, "year_null"?, "preds": [CebPred]};
+        terms: [{"movie", "movie_null"?, "preds": [CebPred], "weight"?}]."""
+        a = N.CebCountArgs()
+        a.title_rows = int(title["id"].numel())
+        a.t_id, a.t_id_null = ptr(title["id"]), ptr(title.get("id_null"))
+        a.t_year, a.t_year_null = ptr(title.get("year")), ptr(title.get("year_null"))
+        a.year_lo, a.year_hi, a.movie_cap = int(year_lo), int(year_hi), int(movie_cap)
+        a.n_title_preds = len(title.get("preds", []))
+        for i, pr in enumerate(title.get("preds", [])):
+            a.title_preds[i] = pr
+        a.n_terms = len(terms)
+        for k, tm in enumerate(terms):
+            dst = a.terms[k]
+            dst.rows = int(tm["movie"].numel())
+            dst.movie, dst.movie_null = ptr(tm["movie"]), ptr(tm.get("movie_null"))
+            dst.weight = int(tm.get("weight", 1))
+            dst.n_preds = len(tm.get("preds", []))
+            for i, pr in enumerate(tm.get("preds", [])):
+                dst.preds[i] = pr
+        return a
+
+    def ceb_count(self, a: "N.CebCountArgs") -> dict:
+        r = N.CebCountResult()
+        N.check(self.lib.bolap_ceb_count(self.ctx, C.byref(a), C.byref(r)))
+        return {"count": r.count, "title_pass": r.title_pass, "joined": r.joined,
+                "term_rows": list(r.term_rows)[:a.n_terms]}
 
     def q17_args(self, t: dict, preds: list) -> N.Q17Args:
         a = N.Q17Args()

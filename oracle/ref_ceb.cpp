@@ -40,6 +40,7 @@
 #include "query_q11b.cpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -285,6 +286,40 @@ void ref_ceb_finish(void* db) {
     }
     sort("name", "id", nullptr);
     sort("title", "id", nullptr);
+}
+
+// Runs one COUNT request (1a, 2a, 2b, 2c, 3a, 4a, 6a, 7a, 8a) and returns its
+// count and the time of run_qN alone (steady_clock, parse outside), as the
+// reference's query() times it (query_impl.cpp:262-268).  0 = done, 2 = not a
+// COUNT id, 1 = the reference threw (message in err).
+int ref_ceb_count_timed(void* db, const char* line, int64_t* count, double* ms, char* err,
+                        uint64_t err_cap) {
+    try {
+        const auto& d = *static_cast<Database*>(db);
+        std::istringstream iss(line);
+        std::string id = "0";
+        iss >> id;
+        const QueryRequest q{id, line};
+        auto timed = [&](auto&& run) {
+            const auto t0 = std::chrono::steady_clock::now();
+            *count = run();
+            *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                      .count();
+            return 0;
+        };
+#define REF_COUNT(N)                                                                   \
+    if (id == #N) {                                                                    \
+        const auto args = parse_q##N(q);                                               \
+        return timed([&] { return run_q##N(d, args); });                              \
+    }
+        REF_COUNT(1a) REF_COUNT(2a) REF_COUNT(2b) REF_COUNT(2c) REF_COUNT(3a) REF_COUNT(4a)
+        REF_COUNT(6a) REF_COUNT(7a) REF_COUNT(8a)
+#undef REF_COUNT
+        return 2;
+    } catch (const std::exception& e) {
+        refceb::set_error(err, static_cast<size_t>(err_cap), e.what());
+        return 1;
+    }
 }
 
 // Runs one request line and writes its CSV to csv_path.  0 = done, 2 = an id

@@ -33,6 +33,8 @@ def _load():
     lib.ref_ceb_set_str.argtypes = [vp, cp, cp, cp, vp, vp, i64]
     lib.ref_ceb_finish.argtypes = [vp]
     lib.ref_ceb_run.argtypes = [vp, cp, cp, cp, C.c_uint64]
+    lib.ref_ceb_count_timed.argtypes = [vp, cp, C.POINTER(C.c_int64), C.POINTER(C.c_double), cp,
+                                        C.c_uint64]
     return lib
 
 
@@ -48,6 +50,15 @@ class RefCeb:
                     self.lib.ref_ceb_set_i32(self.db, tname.encode(), cname.encode(),
                                              vals.ctypes.data,
                                              None if nul is None else nul.ctypes.data, len(vals))
+                elif isinstance(v, dict):   # {"chars": uint8, "offsets": int64, "null"}
+                    off = np.ascontiguousarray(v["offsets"], np.int64)
+                    nul = v.get("null")
+                    nul = None if nul is None else np.ascontiguousarray(nul, np.uint8)
+                    chars = np.ascontiguousarray(v["chars"], np.uint8)
+                    self.lib.ref_ceb_set_str(self.db, tname.encode(), cname.encode(),
+                                             C.c_char_p(chars.ctypes.data), off.ctypes.data,
+                                             None if nul is None else nul.ctypes.data,
+                                             len(off) - 1)
                 else:
                     enc = [b"" if s is None else s.encode() for s in v]
                     off = np.zeros(len(enc) + 1, np.int64)
@@ -64,6 +75,16 @@ class RefCeb:
             self.db = None
 
     __del__ = close
+
+    def count_timed(self, line: str) -> tuple[int, float]:
+        """(COUNT, ms of the reference's run_qN alone) of one COUNT request."""
+        n, ms = C.c_int64(), C.c_double()
+        err = C.create_string_buffer(4096)
+        rc = self.lib.ref_ceb_count_timed(self.db, line.encode(), C.byref(n), C.byref(ms), err,
+                                          len(err))
+        if rc != 0:
+            raise RuntimeError(f"reference could not time {line!r}: {err.value.decode()}")
+        return n.value, ms.value
 
     def run(self, lines: list[str]) -> list[str | None]:
         """CSV text per request line; None for an id the reference does not serve."""
