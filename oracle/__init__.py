@@ -944,3 +944,91 @@ def q16(db: dict, banned_brand: int, type_banned, size_mask: int, n_part_keys: i
                  n_part_keys, len(ps[0]), _p(ps[0]), _p(ps[1]), _p(sb), n_supp_keys, _p(counts),
                  n_counts)
     return counts[:n_counts]
+
+
+# --------------------------------------------------- Q2, Q11, Q15, Q20 (tpch3.c)
+def _tpch4():
+    L = lib()
+    if not getattr(L, "_tpch4_ready", False):
+        f64 = C.c_double
+        L.oracle_q2.restype = i64
+        L.oracle_q2.argtypes = [i64, vp, vp, vp, i32, i64, vp, vp, vp, i64, vp, vp, vp, i64]
+        L.oracle_q11.restype = i64
+        L.oracle_q11.argtypes = [i64, vp, vp, i32, i64, vp, vp, vp, f64, vp, vp, i64, vp, vp]
+        L.oracle_q15.restype = i64
+        L.oracle_q15.argtypes = [i64, vp, vp, vp, i32, i32, i64, vp, vp, vp, vp, i64, vp]
+        L.oracle_q20.restype = i64
+        L.oracle_q20.argtypes = [i64, vp, vp, i64, vp, vp, vp, vp, vp, i32, i32, i32, i64, vp, vp,
+                                 vp, vp, i64, i32, vp, vp]
+        L._tpch4_ready = True
+    return L
+
+
+def q2(db: dict, type_ok, size: int, s_ok) -> np.ndarray:
+    """(part row, supplier row) pairs of run_q2 before its sort, in the
+    reference's emission order (oracle_q2)."""
+    to = _c(type_ok, np.uint8) if len(type_ok) else np.zeros(1, np.uint8)
+    so = _c(s_ok, np.uint8) if len(s_ok) else np.zeros(1, np.uint8)
+    p = [_c(db["p_size"], np.int32), _c(db["p_type"], np.uint16)]
+    ps = [_c(db["ps_partkey"], np.int32), _c(db["ps_suppkey"], np.int32),
+          _c(db["ps_supplycost"], np.int32)]
+    cap = max(1, len(ps[0]))
+    prow, srow = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    n = _tpch4().oracle_q2(len(p[0]), _p(p[0]), _p(p[1]), _p(to), size, len(ps[0]),
+                           *[_p(x) for x in ps], len(s_ok), _p(so), _p(prow), _p(srow), cap)
+    assert n <= cap
+    return np.stack([prow[:n], srow[:n]], axis=1)
+
+
+def q11(db: dict, nation: int, fraction: float):
+    """(partkeys, values, total, groups) of run_q11 (oracle_q11), sorted by
+    value descending then partkey."""
+    s = [_c(db["s_suppkey"], np.int32), _c(db["s_nationkey"], np.int32)]
+    ps = [_c(db["ps_partkey"], np.int32), _c(db["ps_suppkey"], np.int32),
+          _c(db["ps_value"], np.int32)]
+    cap = max(1, len(ps[0]))
+    pk, val = np.zeros(cap, np.int32), np.zeros(cap, np.int64)
+    tot, grp = C.c_int64(0), C.c_int64(0)
+    n = _tpch4().oracle_q11(len(s[0]), _p(s[0]), _p(s[1]), nation, len(ps[0]),
+                            *[_p(x) for x in ps], fraction, _p(pk), _p(val), cap, C.byref(tot),
+                            C.byref(grp))
+    return pk[:n], val[:n], tot.value, grp.value
+
+
+def q15(db: dict, date_lo: int, date_hi: int):
+    """[(suppkey, supplier row, revenue)] of run_q15 (oracle_q15) and the
+    number of lineitems in the window."""
+    li = [_c(db["l_shipdate"], np.int16), _c(db["l_suppkey"], np.int32),
+          _c(db["l_disc_price"], np.int32)]
+    sk = _c(db["s_suppkey"], np.int32)
+    cap = max(1, len(sk))
+    a, b, c = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int64)
+    em = C.c_int64(0)
+    n = _tpch4().oracle_q15(len(li[0]), *[_p(x) for x in li], date_lo, date_hi, len(sk), _p(sk),
+                            _p(a), _p(b), _p(c), cap, C.byref(em))
+    assert n <= cap
+    return [(int(a[i]), int(b[i]), int(c[i])) for i in range(n)], em.value
+
+
+def q20(db: dict, p_ok_row, date_lo: int, date_hi: int, nation: int):
+    """(qualified[max suppkey + 1], count, pairs) of run_q20 (oracle_q20)."""
+    po = _c(p_ok_row, np.uint8)
+    p = _c(db["p_partkey"], np.int32)
+    li = [_c(db["l_shipdate"], np.int16), _c(db["l_partkey"], np.int32),
+          _c(db["l_suppkey"], np.int32), _c(db["l_supp_nation"], np.int16),
+          _c(db["l_quantity"], np.int16)]
+    ps = [_c(db["ps_partkey"], np.int32), _c(db["ps_suppkey"], np.int32),
+          _c(db["ps_availqty"], np.int32)]
+    sk = db["s_suppkey"]
+    smax = int(sk.max()) if len(sk) else -1
+    m = np.full(smax + 1 if smax >= 0 else 0, -1, np.int16)
+    for k, nk in zip(sk.tolist(), db["s_nationkey"].tolist()):
+        if k >= 0:
+            m[k] = nk
+    m = _c(m, np.int16) if len(m) else np.zeros(1, np.int16)
+    q = np.zeros(max(1, smax + 1), np.uint8)
+    pairs = C.c_int64(0)
+    n = _tpch4().oracle_q20(len(p), _p(p), _p(po), len(li[0]), *[_p(x) for x in li], date_lo,
+                            date_hi, nation, len(ps[0]), *[_p(x) for x in ps], _p(m),
+                            smax + 1 if smax >= 0 else 0, smax, _p(q), C.byref(pairs))
+    return q[:smax + 1] if smax >= 0 else q[:0], n, pairs.value

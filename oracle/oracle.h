@@ -331,4 +331,28 @@ int64_t oracle_q16(int64_t p_rows, const int32_t* p_partkey, const uint16_t* p_b
                    const int32_t* ps_suppkey, const uint8_t* supp_banned, int64_t n_supp_keys,
                    int64_t* counts, int64_t n_counts);
 
+/* tpch3.c: Q2, Q11, Q15, Q20 over the reference Database layout. */
+int64_t oracle_q2(int64_t p_rows, const int32_t* p_size, const uint16_t* p_type,
+                  const uint8_t* type_ok, int32_t size, int64_t ps_rows, const int32_t* ps_partkey,
+                  const int32_t* ps_suppkey, const int32_t* ps_supplycost, int64_t s_rows,
+                  const uint8_t* s_ok, int32_t* out_prow, int32_t* out_srow, int64_t max_out);
+int64_t oracle_q11(int64_t s_rows, const int32_t* s_suppkey, const int32_t* s_nationkey,
+                   int32_t nation, int64_t ps_rows, const int32_t* ps_partkey,
+                   const int32_t* ps_suppkey, const int32_t* ps_value, double fraction,
+                   int32_t* out_partkey, int64_t* out_value, int64_t max_out, int64_t* total_out,
+                   int64_t* groups_out);
+int64_t oracle_q15(int64_t li_rows, const int16_t* l_shipdate, const int32_t* l_suppkey,
+                   const int32_t* l_disc_price, int32_t date_lo, int32_t date_hi,
+                   int64_t s_rows, const int32_t* s_suppkey, int32_t* out_suppkey,
+                   int32_t* out_row, int64_t* out_revenue, int64_t max_out,
+                   int64_t* emitted_out);
+int64_t oracle_q20(int64_t p_rows, const int32_t* p_partkey, const uint8_t* p_ok_row,
+                   int64_t li_rows, const int16_t* l_shipdate, const int32_t* l_partkey,
+                   const int32_t* l_suppkey, const int16_t* l_supp_nation,
+                   const int16_t* l_quantity, int32_t date_lo, int32_t date_hi, int32_t nation,
+                   int64_t ps_rows, const int32_t* ps_partkey, const int32_t* ps_suppkey,
+                   const int32_t* ps_availqty, const int16_t* supp_nation_by_key,
+                   int64_t n_supp_map, int32_t max_suppkey, uint8_t* qualified,
+                   int64_t* pairs_out);
+
 #endif

@@ -542,6 +542,72 @@ def csv_q16(db, brand, tprefix, sizes):
     return out + "".join(f"{esc(b)},{esc(t)},{s_},{-c}\n" for c, b, t, s_ in sorted(rows))
 
 
+# ------------------------------------------------- Q2, Q11, Q15, Q20
+def _nation_of_supplier(db, r):
+    """n_name of every supplier row ("" when its nation key is outside the
+    nation rows, as query_q2.cpp:201-209 leaves the view empty)."""
+    names = [""] * len(r["n_name"])
+    for k, n in zip(r["n_nationkey"].tolist(), r["n_name"]):
+        if 0 <= k < len(names):
+            names[k] = n
+    return [names[k] if 0 <= k < len(names) else "" for k in db["s_nationkey"].tolist()]
+
+
+def csv_q2(db, r, size, suffix, region):
+    import oracle
+    out = "s_acctbal,s_name,n_name,p_partkey,p_mfgr,s_address,s_phone,s_comment\n"
+    if region not in r["r_name"]:
+        return out
+    reg = r["r_name"].index(region)
+    in_region = {k for k, rk in zip(r["n_nationkey"].tolist(), r["n_regionkey"].tolist())
+                 if rk == reg and 0 <= k < len(r["n_name"])}
+    s_ok = np.array([k in in_region for k in db["s_nationkey"].tolist()], np.uint8)
+    t_ok = np.array([t.endswith(suffix) for t in db["type_dict"]], np.uint8)
+    pairs = oracle.q2(db, t_ok, int(size), s_ok)
+    nname = _nation_of_supplier(db, r)
+    rows = []
+    for prow, srow in pairs.tolist():
+        rows.append((-int(db["s_acctbal"][srow]), nname[srow], r["s_name"][srow], prow + 1,
+                     db["mfgr_dict"][db["p_mfgr"][prow]], r["s_address"][srow], r["s_phone"][srow],
+                     r["s_comment"][srow]))
+    rows.sort(key=lambda x: x[:4])
+    return out + "".join(f"{-a / 100.0:.2f},{esc(sn)},{esc(nn)},{pk},{esc(mf)},{esc(ad)},"
+                         f"{esc(ph)},{esc(cm)}\n" for a, nn, sn, pk, mf, ad, ph, cm in rows)
+
+
+def csv_q11(db, r, nation, fraction):
+    import oracle
+    out = "ps_partkey,value\n"
+    if nation not in r["n_name"]:
+        return out
+    pk, val, _, _ = oracle.q11(db, r["n_nationkey"][r["n_name"].index(nation)], float(fraction))
+    return out + "".join(f"{k},{v / 100.0:.2f}\n" for k, v in zip(pk.tolist(), val.tolist()))
+
+
+def csv_q15(db, r, date):
+    import oracle
+    from tests import tpch_data as T
+    rows, _ = oracle.q15(db, T.date_offset(date), T.add_months(date, 3))
+    return "s_suppkey,s_name,s_address,s_phone,total_revenue\n" + "".join(
+        f"{k},{esc(r['s_name'][row])},{esc(r['s_address'][row])},{esc(r['s_phone'][row])},"
+        f"{v / 100.0:.2f}\n" for k, row, v in rows)
+
+
+def csv_q20(db, r, color, date, nation):
+    import oracle
+    from tests import tpch_data as T
+    out = "s_name,s_address\n"
+    if "<<NULL>>" in (color, date, nation) or nation not in r["n_name"]:
+        return out
+    nk = int(r["n_nationkey"][r["n_name"].index(nation)])
+    p_ok = np.array([n.startswith(color) for n in r["p_name"]], np.uint8)
+    q, _, _ = oracle.q20(db, p_ok, T.date_offset(date), T.add_months(date, 12), nk)
+    rows = sorted((r["s_name"][i], r["s_address"][i]) for i, (k, n) in
+                  enumerate(zip(db["s_suppkey"].tolist(), db["s_nationkey"].tolist()))
+                  if 0 <= k < len(q) and q[k] and n == nk)
+    return out + "".join(f"{esc(a)},{esc(b)}\n" for a, b in rows)
+
+
 @pytest.mark.parametrize("orders", [3, 40_001])
 def test_driver_q13_q16(tmp_path, orders):
     if not B.DRIVER.exists():

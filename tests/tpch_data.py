@@ -40,6 +40,7 @@ CONTAINERS = [f"{a} {b}" for a in ("SM", "LG", "MED", "JUMBO", "WRAP")
 TYPES = [f"{a} {b} {c}" for a in ("STANDARD", "SMALL", "MEDIUM", "LARGE", "ECONOMY", "PROMO")
          for b in ("ANODIZED", "BURNISHED", "PLATED", "POLISHED", "BRUSHED")
          for c in ("TIN", "NICKEL", "BRASS", "STEEL", "COPPER")]
+MFGRS = [f"Manufacturer#{i}" for i in range(1, 6)]
 CURRENT = 1263  # 1995-06-17: returnflag / linestatus cut-over
 # part-name words; no word contains another, so "name contains colour" is
 # "colour is one of the name's words"
@@ -170,7 +171,23 @@ def raw(seed: int, orders: int, parts: int = 2000, custs: int = 300, supps: int 
     r["l_shipinstruct"] = rng.integers(0, 4, n)
     r["l_shipmode"] = rng.integers(0, 7, n)
     r["rf_values"], r["ls_values"] = ["R", "A", "N"], ["O", "F"]
+    # columns of Q2, Q11, Q15 and Q20, drawn from their own stream so the
+    # columns above do not move: available quantities (small ones too, so
+    # Q20's availqty * 200 > sum sits on its boundary now and then), supplier
+    # balances with repeats (Q2's sort falls through to nation and name),
+    # addresses with a quote, manufacturers
+    r2 = np.random.default_rng(seed + 7919)
+    r["ps_availqty"] = np.where(r2.random(4 * parts) < 0.3, r2.integers(1, 40, 4 * parts),
+                                r2.integers(1, 10_000, 4 * parts)).astype(np.int32)
+    r["s_acctbal"] = (r2.integers(-999, 9_999, supps) * 100 +
+                      np.where(r2.random(supps) < 0.5, 0, r2.integers(0, 100, supps))).astype(
+        np.int64)
+    r["p_mfgr"] = r2.integers(0, 5, parts)
     if strings:
+        r["s_address"] = [f'{k * 37 % 1000} "Elm" St, {k}' if k % 5 == 0 else f"road {k * 911 % 997}"
+                          for k in r["s_suppkey"].tolist()]
+        r["s_phone"] = [f"{10 + nk}-{k % 900 + 100}-{k * 7 % 9000 + 1000}"
+                        for k, nk in zip(r["s_suppkey"].tolist(), r["s_nationkey"].tolist())]
         ch, off = r["o_comment_chars"].tobytes(), r["o_comment_offsets"]
         r["o_comment"] = [ch[off[i]:off[i + 1]].decode() for i in range(orders)]
         r["p_name"] = [" ".join(COLORS[w] for w in row) for row in r["p_name_words"].tolist()]
@@ -295,6 +312,10 @@ def build(r: dict) -> dict:
     db["ps_partkey"] = r["ps_partkey"]
     db["ps_suppkey"] = r["ps_suppkey"]
     db["ps_supplycost"] = r["ps_supplycost"]
+    db["ps_availqty"] = r["ps_availqty"].astype(np.int32)
+    db["ps_value"] = (r["ps_supplycost"].astype(np.int64) * r["ps_availqty"]).astype(np.int32)
+    db["s_acctbal"] = r["s_acctbal"].astype(np.int32)
+    db["p_mfgr"], db["mfgr_dict"] = first_appearance(r["p_mfgr"], MFGRS)
     db["n_nationkey"] = r["n_nationkey"]
     db["n_name"] = r["n_name"]
     db["n_regionkey"] = r["n_regionkey"]
@@ -339,11 +360,15 @@ def parquet(r: dict) -> dict:
     part = pa.table({
         "p_partkey": i32(r["p_partkey"]), "p_brand": strs(r["p_brand"], BRANDS),
         "p_container": strs(r["p_container"], CONTAINERS), "p_size": i32(r["p_size"]),
-        "p_type": strs(r["p_type"], TYPES), "p_name": pa.array(r["p_name"])})
+        "p_type": strs(r["p_type"], TYPES), "p_name": pa.array(r["p_name"]),
+        "p_mfgr": strs(r["p_mfgr"], MFGRS)})
     partsupp = pa.table({"ps_partkey": i32(r["ps_partkey"]), "ps_suppkey": i32(r["ps_suppkey"]),
-                         "ps_supplycost": dec(r["ps_supplycost"])})
+                         "ps_supplycost": dec(r["ps_supplycost"]),
+                         "ps_availqty": i32(r["ps_availqty"])})
     supp = pa.table({"s_suppkey": i32(r["s_suppkey"]), "s_nationkey": i32(r["s_nationkey"]),
-                     "s_name": pa.array(r["s_name"]), "s_comment": pa.array(r["s_comment"])})
+                     "s_name": pa.array(r["s_name"]), "s_comment": pa.array(r["s_comment"]),
+                     "s_acctbal": dec(r["s_acctbal"]), "s_address": pa.array(r["s_address"]),
+                     "s_phone": pa.array(r["s_phone"])})
     nation = pa.table({"n_nationkey": i32(r["n_nationkey"]), "n_name": pa.array(r["n_name"]),
                        "n_regionkey": i32(r["n_regionkey"])})
     region = pa.table({"r_regionkey": i32(r["r_regionkey"]), "r_name": pa.array(r["r_name"])})

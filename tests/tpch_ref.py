@@ -165,7 +165,7 @@ class RefTpch:
         self._set("part.partkey", db["p_partkey"])
         self._set("part.size", db["p_size"])
         self._set("part.retailprice", np.zeros(n_p, np.int32))
-        self._dict("part.mfgr", np.zeros(n_p), [""], keep_codes=True)
+        self._dict("part.mfgr", db["p_mfgr"], db["mfgr_dict"])
         self._dict("part.brand", db["p_brand"], db["brand_dict"])
         self._dict("part.container", db["p_container"], db["container_dict"])
         self._dict("part.type", db["p_type"], db["type_dict"])
@@ -179,11 +179,11 @@ class RefTpch:
         self._set("supplier.suppkey", skeys)
         self._set("supplier.nationkey", db["s_nationkey"])
         self._set("supplier.nationkey_by_suppkey", sb)
-        self._set("supplier.acctbal", np.zeros(n_s, np.int32))
+        self._set("supplier.acctbal", db["s_acctbal"])
         self._str("supplier.name", r["s_name"])
         self._str("supplier.comment", r["s_comment"])
-        self._str("supplier.address", [""] * n_s)
-        self._str("supplier.phone", [""] * n_s)
+        self._str("supplier.address", r["s_address"])
+        self._str("supplier.phone", r["s_phone"])
         # partsupp, stably sorted by part key (:750-778)
         order = np.argsort(r["ps_partkey"], kind="stable")
         n_ps = len(order)
@@ -191,8 +191,9 @@ class RefTpch:
         self._set("partsupp.partkey", r["ps_partkey"][order].astype(np.int32))
         self._set("partsupp.suppkey", r["ps_suppkey"][order].astype(np.int32))
         self._set("partsupp.supplycost", cost)
-        self._set("partsupp.availqty", np.zeros(n_ps, np.int32))
-        self._set("partsupp.supplycost_availqty", np.zeros(n_ps, np.int32))
+        avail = r["ps_availqty"][order].astype(np.int32)
+        self._set("partsupp.availqty", avail)
+        self._set("partsupp.supplycost_availqty", (cost.astype(np.int64) * avail).astype(np.int32))
         self._str("partsupp.comment", [""] * n_ps)
         # nation / region rows and maps (:780-822)
         for k, reg, name in zip(r["n_nationkey"].tolist(), r["n_regionkey"].tolist(),
