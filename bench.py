@@ -407,7 +407,7 @@ def _tpch_upload(torch, h):
 
 
 def bench_tpch(eng, torch, args):
-    """TPC-H Q4, Q5, Q7, Q8, Q9, Q10, Q12, Q13, Q16, Q19, Q21, Q22 over one
+    """TPC-H Q2, Q4, Q5, Q7-Q13, Q15, Q16, Q19-Q22 over one
     seeded SF10-sized database in the reference layout (15,000,000 orders
     sorted by date with lineitem ranges, ~60,000,000 lineitem rows in order-key
     order, 2,000,000 parts, 8,000,000 partsupp rows, 1,500,000 customers,
@@ -463,6 +463,22 @@ def bench_tpch(eng, torch, args):
     lo94, hi94q, hi94y = d("1994-01-01"), am("1994-01-01", 3), am("1994-01-01", 12)
     w4, w10 = window(lo94, hi94q), window(lo94, hi94q)
     w5, w8 = window(lo94, hi94y), window(q7d[0], q7d[1] + 1)
+    # Q2: BRASS parts of size 15 from EUROPE's suppliers; Q15: the first
+    # quarter of 1996; Q20: forest parts, CANADA, 1994, the group table sized
+    # by the reference's own estimate (rows x 1/7 x colour selectivity)
+    tok2 = np.array([x.endswith("BRASS") for x in h["type_dict"]], np.uint8)
+    nok2 = (np.array(h["n_regionkey"]) == 3).astype(np.uint8)
+    lo96, hi96q = d("1996-01-01"), am("1996-01-01", 3)
+    ship = h["l_shipdate"].astype(np.int32)
+    w15 = int(((ship >= lo96) & (ship < hi96q)).sum())
+    w20 = int(((ship >= lo94) & (ship < hi94y)).sum())
+    del ship
+    p_ok20 = (h["p_name_words"][:, 0] == T.COLORS.index("forest")).astype(np.uint8)
+    p_ok20_dev = torch.from_numpy(p_ok20).cuda()
+    hint20 = int(max(16.0, n_li / 7.0 * p_ok20.mean())) + 1
+    smap = np.full(nsk, -1, np.int16)
+    smap[h["s_suppkey"]] = h["s_nationkey"]
+    smap_dev = torch.from_numpy(smap).cuda()
     # (name, gpu call, oracle call, algorithmic bytes, parity check of the results)
     Q = [
         ("q4", lambda: eng.q4(t, 5, lo94, hi94q), lambda: oracle.q4(h, lo94, hi94q, 5),
@@ -496,6 +512,17 @@ def bench_tpch(eng, torch, args):
         ("q22", lambda: eng.q22(t, codes22, [(ord("A") << 8) | ord("B")]),
          lambda: oracle.q22(h, codes22, [(ord("A") << 8) | ord("B")]), n_c * 13 + n_o * 4,
          lambda g, o: g == (o[0].tolist(), o[1].tolist())),
+        ("q2", lambda: eng.q2(t, tok2, 15, nok2), lambda: oracle.q2(h, tok2, 15, nok2[h["s_nationkey"]]),
+         n_p * 5, lambda g, o: np.array_equal(g, o)),
+        ("q11", lambda: eng.q11(t, 7, nsk, 0.0001 / 10), lambda: oracle.q11(h, 7, 0.0001 / 10),
+         n_ps * 12 + n_s * 8,
+         lambda g, o: np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1]) and g[2] == o[2]),
+        ("q15", lambda: eng.q15(t, lo96, hi96q, nsk), lambda: oracle.q15(h, lo96, hi96q),
+         n_li * 2 + w15 * 8 + n_s * 4, lambda g, o: g == o),
+        ("q20", lambda: eng.q20(t, p_ok20_dev, npk - 1, lo94, hi94y, 3, smap_dev, nsk - 1,
+                                capacity_hint=hint20),
+         lambda: oracle.q20(h, p_ok20, lo94, hi94y, 3), n_li * 2 + w20 * 2 + n_ps * 8 + n_p * 5,
+         lambda g, o: np.array_equal(g[0], o[0]) and g[1].qualified == o[1]),
     ]
     out = {"workload": f"TPC-H SF10 reference layout: {n_o:,} orders, {n_li:,} lineitem rows, "
                        f"{n_p:,} parts, {n_ps:,} partsupp, {n_c:,} customers, {n_s:,} suppliers",

@@ -1,5 +1,5 @@
 // tpch_query.cpp — the reference's load() / build() / query() sequence for
-// TPC-H Q1, Q3-Q10, Q12-Q14, Q16-Q19, Q21 and Q22, with the kernels
+// TPC-H Q1-Q22, with the kernels
 // of libbolap doing the work.
 //
 //   bolap_tpch <dir/>  <  requests
@@ -251,25 +251,60 @@ struct Part {
     std::vector<std::string> brands, containers;
     uint8_t* type = nullptr;                  // p_type codes
     std::vector<std::string> types, name;
+    std::vector<uint16_t> mfgr;               // p_mfgr codes (host) and values
+    std::vector<std::string> mfgrs;
     int32_t max_key = 0;                      // max p_partkey (0 for none, as the queries)
 };
 
-// partsupp rows (builder_impl.cpp:753-777 keeps them for Q2/Q9/Q11/Q16/Q20).
+// partsupp rows (builder_impl.cpp:751-777 keeps them for Q2/Q9/Q11/Q16/Q20),
+// stably sorted by partkey as the builder leaves them.
 struct Partsupp {
     int64_t rows = -1;
     int32_t *partkey = nullptr, *suppkey = nullptr, *supplycost = nullptr;
+    int32_t *availqty = nullptr, *value = nullptr;  // value = supplycost * availqty
     int32_t max_partkey = 0;
 };
+
+// Row order of a stable counting sort by key (build_key_sorted_indices,
+// builder_impl.cpp:410-445).
+std::vector<uint32_t> key_sorted_order(const std::vector<int32_t>& keys) {
+    std::vector<uint32_t> order(keys.size());
+    if (keys.empty()) return order;
+    const auto [mn, mx] = std::minmax_element(keys.begin(), keys.end());
+    const int64_t lo = *mn, range = int64_t(*mx) - lo + 1;
+    std::vector<uint32_t> off(size_t(range) + 1, 0);
+    for (int32_t k : keys) off[size_t(k - lo) + 1] += 1;
+    for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
+    for (size_t i = 0; i < keys.size(); ++i) order[off[size_t(keys[i] - lo)]++] = uint32_t(i);
+    return order;
+}
+
+template <typename T>
+std::vector<T> permuted(const std::vector<T>& v, const std::vector<uint32_t>& order) {
+    std::vector<T> out(v.size());
+    for (size_t i = 0; i < order.size(); ++i) out[i] = v[order[i]];
+    return out;
+}
 
 Partsupp build_partsupp(const arrow::Table& t) {
     Partsupp ps;
     ps.rows = t.num_rows();
     const auto pk = int32s(*column(t, "ps_partkey"));
+    const auto order = key_sorted_order(pk);
     for (int32_t k : pk) ps.max_partkey = std::max(ps.max_partkey, k);
-    ps.partkey = upload(pk);
-    ps.suppkey = upload(int32s(*column(t, "ps_suppkey")));
-    const auto cost = scaled(*column(t, "ps_supplycost"), 100);
-    ps.supplycost = upload(std::vector<int32_t>(cost.begin(), cost.end()));
+    ps.partkey = upload(permuted(pk, order));
+    ps.suppkey = upload(permuted(int32s(*column(t, "ps_suppkey")), order));
+    const auto cost64 = scaled(*column(t, "ps_supplycost"), 100);
+    const std::vector<int32_t> cost(cost64.begin(), cost64.end());
+    ps.supplycost = upload(permuted(cost, order));
+    if (t.GetColumnByName("ps_availqty")) {
+        const auto avail = int32s(*column(t, "ps_availqty"));
+        std::vector<int32_t> value(avail.size());  // int32 product (:760-766)
+        for (size_t i = 0; i < value.size(); ++i)
+            value[i] = int32_t(uint32_t(int64_t(cost[i]) * avail[i]));
+        ps.availqty = upload(permuted(avail, order));
+        ps.value = upload(permuted(value, order));
+    }
     return ps;
 }
 
@@ -312,10 +347,14 @@ struct Customer {
 struct Supplier {
     int64_t rows = -1;
     int32_t *suppkey = nullptr, *nationkey = nullptr;
-    std::vector<std::string> name;
+    std::vector<std::string> name, address, phone, comment;
+    std::vector<int32_t> key_h;       // s_suppkey by row
+    std::vector<int32_t> nation_h;    // s_nationkey by row
+    std::vector<int32_t> acctbal;     // cents
     std::vector<int32_t> row_by_key;  // suppkey -> last row, -1 absent
     int32_t max_key = -1;
     uint8_t* banned = nullptr;        // comment has "Customer" then "Complaints" (Q16)
+    int16_t* nation_by_key = nullptr; // nationkey_by_suppkey (builder_impl.cpp:731-742)
 };
 
 Supplier build_supplier(const arrow::Table& t) {
@@ -325,8 +364,19 @@ Supplier build_supplier(const arrow::Table& t) {
     s.suppkey = upload(keys);
     s.nationkey = upload(int32s(*column(t, "s_nationkey")));
     s.name = strings_or_empty(t, "s_name");
+    s.address = strings_or_empty(t, "s_address");
+    s.phone = strings_or_empty(t, "s_phone");
+    s.comment = strings_or_empty(t, "s_comment");
+    s.nation_h = int32s(*column(t, "s_nationkey"));
+    s.key_h = keys;
+    if (t.GetColumnByName("s_acctbal")) {
+        const auto bal = scaled(*column(t, "s_acctbal"), 100);
+        s.acctbal.assign(bal.begin(), bal.end());
+    } else {
+        s.acctbal.assign(size_t(s.rows), 0);
+    }
     std::vector<uint8_t> banned;  // has_customer_complaints, query_q16.cpp:60-68
-    for (const auto& c : strings_or_empty(t, "s_comment")) {
+    for (const auto& c : s.comment) {
         const size_t p = c.find("Customer");
         banned.push_back(p != std::string::npos && c.find("Complaints", p + 8) != std::string::npos);
     }
@@ -335,6 +385,10 @@ Supplier build_supplier(const arrow::Table& t) {
     s.row_by_key.assign(size_t(s.max_key + 1), -1);
     for (size_t i = 0; i < keys.size(); ++i)
         if (keys[i] >= 0) s.row_by_key[size_t(keys[i])] = int32_t(i);
+    std::vector<int16_t> nb(size_t(s.max_key + 1), int16_t(-1));
+    for (size_t i = 0; i < keys.size(); ++i)
+        if (keys[i] >= 0) nb[size_t(keys[i])] = int16_t(s.nation_h[i]);
+    s.nation_by_key = upload(nb);
     return s;
 }
 
@@ -445,15 +499,13 @@ Part build_part(const arrow::Table& t) {
         p.types = d.values;
     }
     p.name = strings_or_empty(t, "p_name");
+    if (t.GetColumnByName("p_mfgr")) {
+        Dict d = dictionary(*column(t, "p_mfgr"));
+        p.mfgr = d.codes;
+        p.mfgrs = d.values;
+    }
     for (int32_t k : int32s(*column(t, "p_partkey"))) p.max_key = std::max(p.max_key, k);
     return p;
-}
-
-template <typename T>
-std::vector<T> permuted(const std::vector<T>& v, const std::vector<uint32_t>& order) {
-    std::vector<T> out(v.size());
-    for (size_t i = 0; i < order.size(); ++i) out[i] = v[order[i]];
-    return out;
 }
 
 Orders build_orders(bolap_ctx* ctx, const arrow::Table& t, const Lineitem& li,
@@ -1495,6 +1547,241 @@ Writer run_q22(bolap_ctx* ctx, const Orders& o, const Customer& c, const std::st
 
 // The reference's CSV quoting (query_impl.cpp:82-96): the value in double
 // quotes, an embedded double quote written as backslash + quote.
+// Q2 (query_q2.cpp:146-455): the part / partsupp scans on the GPU, the
+// string-keyed sort and the output rows on the host.
+Writer run_q2(bolap_ctx* ctx, const Part& part, const Partsupp& ps, const Supplier& sup,
+              const Nation& nat, const std::string& line) {
+    need(part.rows >= 0 && part.type && part.size && ps.rows >= 0 && sup.rows >= 0,
+         "Q2 needs part (with p_type, p_size), partsupp, supplier and nation");
+    const auto v = quoted_args(line, 3, "Q2");
+    const int32_t size = std::stoi(v[0]);
+    struct Row {
+        int32_t acctbal;
+        std::string s_name, n_name;
+        int32_t partkey;
+        std::string mfgr, address, phone, comment;
+    };
+    std::vector<Row> rows;
+    const auto reg = nat.region_by_name.find(v[2]);
+    if (reg != nat.region_by_name.end() && part.rows > 0) {
+        // nations indexed by key below the nation row count (:170-189)
+        const size_t n_rows = nat.key.size();
+        std::vector<uint8_t> nation_ok(n_rows, 0);
+        std::vector<std::string> nation_name(n_rows);
+        for (size_t i = 0; i < n_rows; ++i) {
+            const uint32_t k = uint32_t(nat.key[i]);
+            if (k >= n_rows) continue;
+            nation_name[k] = nat.name[i];
+            if (nat.region[i] == reg->second) nation_ok[k] = 1;
+        }
+        std::vector<uint8_t> type_ok(part.types.size(), 0);  // TYPE suffix (:219-231)
+        for (size_t i = 0; i < type_ok.size(); ++i) {
+            const auto& t = part.types[i];
+            type_ok[i] = t.size() >= v[1].size() &&
+                         t.compare(t.size() - v[1].size(), v[1].size(), v[1]) == 0;
+        }
+        bolap_q2_args a{};
+        a.part_rows = part.rows;
+        a.p_size = part.size;
+        a.p_type = part.type;
+        a.type_ok = type_ok.data();
+        a.n_types = int32_t(type_ok.size());
+        a.size = size;
+        a.ps_rows = ps.rows;
+        a.ps_partkey = ps.partkey;
+        a.ps_suppkey = ps.suppkey;
+        a.ps_supplycost = ps.supplycost;
+        a.supp_rows = sup.rows;
+        a.s_nationkey = sup.nationkey;
+        a.nation_ok = nation_ok.data();
+        a.n_nation = int32_t(n_rows);
+        std::vector<int32_t> pairs(2 * 65536);
+        int64_t n = 0;
+        int st = bolap_q2(ctx, &a, pairs.data(), 65536, &n);
+        if (st == BOLAP_ERR_CAPACITY) {
+            pairs.resize(size_t(2 * n));
+            st = bolap_q2(ctx, &a, pairs.data(), n, &n);
+        }
+        check(st);
+        rows.reserve(size_t(n));
+        for (int64_t i = 0; i < n; ++i) {
+            const size_t prow = size_t(pairs[size_t(2 * i)]);
+            const size_t srow = size_t(pairs[size_t(2 * i + 1)]);
+            const uint32_t nk = uint32_t(sup.nation_h[srow]);
+            rows.push_back({sup.acctbal[srow], sup.name[srow],
+                            nk < n_rows ? nation_name[nk] : std::string(), int32_t(prow + 1),
+                            part.mfgr.empty() ? std::string() : part.mfgrs[part.mfgr[prow]],
+                            sup.address[srow], sup.phone[srow], sup.comment[srow]});
+        }
+        std::sort(rows.begin(), rows.end(), [](const Row& x, const Row& y) {  // :408-423
+            if (x.acctbal != y.acctbal) return x.acctbal > y.acctbal;
+            if (x.n_name != y.n_name) return x.n_name < y.n_name;
+            if (x.s_name != y.s_name) return x.s_name < y.s_name;
+            return x.partkey < y.partkey;
+        });
+    }
+    return [rows](const std::string& csv) {  // write_q2_csv, :434-455
+        std::ofstream out(csv);
+        out << "s_acctbal,s_name,n_name,p_partkey,p_mfgr,s_address,s_phone,s_comment\n";
+        for (const auto& r : rows)
+            out << fixed_double(double(r.acctbal) / 100.0) << ',' << quoted_field(r.s_name) << ','
+                << quoted_field(r.n_name) << ',' << r.partkey << ',' << quoted_field(r.mfgr) << ','
+                << quoted_field(r.address) << ',' << quoted_field(r.phone) << ','
+                << quoted_field(r.comment) << "\n";
+    };
+}
+
+// Q11 (query_q11.cpp:119-234).
+Writer run_q11(bolap_ctx* ctx, const Partsupp& ps, const Supplier& sup, const Nation& nat,
+               const std::string& line) {
+    need(ps.rows >= 0 && ps.value && sup.rows >= 0,
+         "Q11 needs partsupp (with ps_availqty), supplier and nation");
+    const auto v = quoted_args(line, 2, "Q11");
+    std::vector<std::pair<int32_t, int64_t>> rows;
+    const auto it = nat.key_by_name.find(v[0]);
+    if (it != nat.key_by_name.end()) {
+        const double fraction = std::stod(v[1]);
+        bolap_q11_args a{};
+        a.supp_rows = sup.rows;
+        a.s_suppkey = sup.suppkey;
+        a.s_nationkey = sup.nationkey;
+        a.nation = it->second;
+        a.n_supp_keys = int64_t(std::max(sup.max_key, 0)) + 1;
+        a.ps_rows = ps.rows;
+        a.ps_partkey = ps.partkey;
+        a.ps_suppkey = ps.suppkey;
+        a.ps_value = ps.value;
+        a.fraction = fraction;
+        std::vector<int32_t> pk(65536);
+        std::vector<int64_t> val(65536);
+        int64_t n = 0, total = 0;
+        int st = bolap_q11(ctx, &a, pk.data(), val.data(), 65536, &n, &total);
+        if (st == BOLAP_ERR_CAPACITY) {
+            pk.resize(size_t(n));
+            val.resize(size_t(n));
+            st = bolap_q11(ctx, &a, pk.data(), val.data(), n, &n, &total);
+        }
+        check(st);
+        for (int64_t i = 0; i < n; ++i) rows.emplace_back(pk[size_t(i)], val[size_t(i)]);
+    }
+    return [rows](const std::string& csv) {  // write_q11_csv, :219-234
+        std::ofstream out(csv);
+        out << "ps_partkey,value\n";
+        for (const auto& [k, val] : rows)
+            out << k << ',' << fixed_double(double(val) / 100.0) << "\n";
+    };
+}
+
+// Q15 (query_q15.cpp:215-549).
+Writer run_q15(bolap_ctx* ctx, const Lineitem& li, const Supplier& sup, const std::string& line) {
+    need(li.suppkey && sup.rows >= 0, "Q15 needs l_suppkey and supplier");
+    const auto v = quoted_args(line, 2, "Q15");
+    struct Row {
+        int32_t key, row;
+        int64_t rev;
+    };
+    std::vector<Row> rows;
+    bolap_q15_args a{};
+    a.li_rows = li.rows;
+    a.l_shipdate = li.ship;
+    a.l_suppkey = li.suppkey;
+    a.l_disc_price = li.disc_price;
+    a.date_lo = parse_date_offset(v[0]);
+    a.date_hi = add_months_offset(v[0], 3);
+    a.supp_rows = sup.rows;
+    a.s_suppkey = sup.suppkey;
+    a.n_supp_keys = std::max<int64_t>(int64_t(sup.max_key) + 1, 1);
+    std::vector<int32_t> key(1024), row(1024);
+    std::vector<int64_t> rev(1024);
+    int64_t n = 0, emitted = 0;
+    int st = bolap_q15(ctx, &a, key.data(), row.data(), rev.data(), 1024, &n, &emitted);
+    if (st == BOLAP_ERR_CAPACITY) {
+        key.resize(size_t(n));
+        row.resize(size_t(n));
+        rev.resize(size_t(n));
+        st = bolap_q15(ctx, &a, key.data(), row.data(), rev.data(), n, &n, &emitted);
+    }
+    check(st);
+    for (int64_t i = 0; i < n; ++i) rows.push_back({key[size_t(i)], row[size_t(i)], rev[size_t(i)]});
+    return [rows, &sup](const std::string& csv) {  // write_q15_csv, :536-549
+        std::ofstream out(csv);
+        out << "s_suppkey,s_name,s_address,s_phone,total_revenue\n";
+        for (const auto& r : rows)
+            out << r.key << ',' << quoted_field(sup.name[size_t(r.row)]) << ','
+                << quoted_field(sup.address[size_t(r.row)]) << ','
+                << quoted_field(sup.phone[size_t(r.row)]) << ','
+                << fixed_double(double(r.rev) / 100.0) << "\n";
+    };
+}
+
+// Q20 (query_q20.cpp:331-688): the lineitem group-by and the partsupp check
+// on the GPU, with the reference's own estimate sizing the group table.
+Writer run_q20(bolap_ctx* ctx, const Lineitem& li, const Part& part, const Partsupp& ps,
+               const Supplier& sup, const Nation& nat, const std::string& line) {
+    need(part.rows >= 0 && ps.rows >= 0 && ps.availqty && sup.rows >= 0 && li.suppkey &&
+             li.supp_nation,
+         "Q20 needs part, partsupp (with ps_availqty), supplier, nation and l_suppkey");
+    const auto v = quoted_args(line, 3, "Q20");
+    std::vector<std::pair<std::string, std::string>> rows;
+    const auto it = nat.key_by_name.find(v[2]);
+    const bool null_arg = v[0] == "<<NULL>>" || v[1] == "<<NULL>>" || v[2] == "<<NULL>>";
+    if (!null_arg && it != nat.key_by_name.end() && part.rows > 0) {
+        const int32_t target = it->second;
+        const int32_t max_partkey = part.max_key;
+        std::vector<uint8_t> ok(part.name.size());
+        int64_t colored = 0;  // :366-381
+        for (size_t i = 0; i < ok.size(); ++i) {
+            ok[i] = part.name[i].compare(0, v[0].size(), v[0]) == 0 &&
+                    part.name[i].size() >= v[0].size();
+            colored += ok[i];
+        }
+        uint8_t* ok_dev = upload(ok);
+        // the reference's estimate of the groups: rows x 1/7 x colour
+        // selectivity, at least 16 (:383-391)
+        const double sel = double(colored) / double(part.rows);
+        const double expected = double(li.rows) * (1.0 / 7.0) * sel;
+        bolap_q20_args a{};
+        a.part_rows = part.rows;
+        a.p_partkey = part.partkey;
+        a.p_ok_row = ok_dev;
+        a.max_partkey = max_partkey;
+        a.li_rows = li.rows;
+        a.l_shipdate = li.ship;
+        a.l_partkey = li.partkey;
+        a.l_suppkey = li.suppkey;
+        a.l_supp_nation = li.supp_nation;
+        a.l_quantity = li.qty;
+        a.date_lo = parse_date_offset(v[1]);
+        a.date_hi = add_months_offset(v[1], 12);
+        a.nation = target;
+        a.ps_rows = ps.rows;
+        a.ps_partkey = ps.partkey;
+        a.ps_suppkey = ps.suppkey;
+        a.ps_availqty = ps.availqty;
+        a.supp_nation_by_key = sup.nation_by_key;
+        a.n_supp_map = int64_t(sup.max_key) + 1;
+        a.max_suppkey = sup.max_key;
+        a.capacity_hint = int64_t(std::max(16.0, expected)) + 1;
+        std::vector<uint8_t> qualified(size_t(std::max(sup.max_key + 1, 1)));
+        bolap_q20_stats st{};
+        check(bolap_q20(ctx, &a, qualified.data(), &st));
+        hip_check(hipFree(ok_dev), "hipFree");
+        for (size_t r = 0; r < size_t(sup.rows); ++r) {  // :594-612
+            const int32_t k = sup.key_h[r];
+            if (k < 0 || k > sup.max_key || !qualified[size_t(k)]) continue;
+            if (sup.nation_h[r] != target) continue;
+            rows.emplace_back(sup.name[r], sup.address[r]);
+        }
+        std::sort(rows.begin(), rows.end(),
+                  [](const auto& x, const auto& y) { return x.first < y.first; });
+    }
+    return [rows](const std::string& csv) {  // write_q20_csv, :681-688
+        std::ofstream out(csv);
+        out << "s_name,s_address\n";
+        for (const auto& [n, addr] : rows) out << quoted_field(n) << ',' << quoted_field(addr) << "\n";
+    };
+}
+
 std::string escape_csv_field(const std::string& value) {
     return quoted_field(value);
 }
@@ -1605,6 +1892,14 @@ int main(int argc, char** argv) {
                 write = run_q21(ctx, li, orders, supplier, nation, text);
             } else if (id == "22") {
                 write = run_q22(ctx, orders, customer, text);
+            } else if (id == "2") {
+                write = run_q2(ctx, part, partsupp, supplier, nation, text);
+            } else if (id == "11") {
+                write = run_q11(ctx, partsupp, supplier, nation, text);
+            } else if (id == "15") {
+                write = run_q15(ctx, li, supplier, text);
+            } else if (id == "20") {
+                write = run_q20(ctx, li, part, partsupp, supplier, nation, text);
             } else if (id == "6") {
                 write = run_q6(ctx, li, text);
             } else if (id == "14") {

@@ -738,6 +738,88 @@ class Engine:
                                    n))
         return out.tolist()
 
+    # ----------------------------------------------- Q2, Q11, Q15, Q20 (tpch_supp)
+    def q2(self, t: dict, type_ok, size: int, nation_ok, max_rows: int | None = None):
+        """(part row, supplier row) pairs of run_q2 in its emission order."""
+        import numpy as np
+        to = np.ascontiguousarray(type_ok, np.uint8)
+        no = np.ascontiguousarray(nation_ok, np.uint8)
+        a = N.Q2Args()
+        a.part_rows = int(t["p_size"].numel())
+        a.p_size, a.p_type = ptr(t["p_size"]), ptr(t["p_type"])
+        a.type_ok, a.n_types = to.ctypes.data if len(to) else None, len(to)
+        a.size = size
+        a.ps_rows = int(t["ps_partkey"].numel())
+        a.ps_partkey, a.ps_suppkey = ptr(t["ps_partkey"]), ptr(t["ps_suppkey"])
+        a.ps_supplycost = ptr(t["ps_supplycost"])
+        a.supp_rows = int(t["s_nationkey"].numel())
+        a.s_nationkey = ptr(t["s_nationkey"])
+        a.nation_ok, a.n_nation = no.ctypes.data if len(no) else None, len(no)
+        cap = max_rows if max_rows is not None else max(1, a.ps_rows)
+        out = np.zeros((max(cap, 1), 2), np.int32)
+        n = C.c_int64(0)
+        N.check(self.lib.bolap_q2(self.ctx, C.byref(a), out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value]
+
+    def q11(self, t: dict, nation: int, n_supp_keys: int, fraction: float,
+            max_rows: int | None = None):
+        """(partkeys, values, total) of run_q11, sorted as the reference."""
+        import numpy as np
+        a = N.Q11Args()
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.s_nationkey = ptr(t["s_suppkey"]), ptr(t["s_nationkey"])
+        a.nation, a.n_supp_keys = nation, n_supp_keys
+        a.ps_rows = int(t["ps_partkey"].numel())
+        a.ps_partkey, a.ps_suppkey, a.ps_value = (ptr(t["ps_partkey"]), ptr(t["ps_suppkey"]),
+                                                  ptr(t["ps_value"]))
+        a.fraction = fraction
+        cap = max_rows if max_rows is not None else max(1, a.ps_rows)
+        pk, val = np.zeros(max(cap, 1), np.int32), np.zeros(max(cap, 1), np.int64)
+        n, tot = C.c_int64(0), C.c_int64(0)
+        N.check(self.lib.bolap_q11(self.ctx, C.byref(a), pk.ctypes.data, val.ctypes.data, cap,
+                                   C.byref(n), C.byref(tot)))
+        return pk[:n.value], val[:n.value], tot.value
+
+    def q15(self, t: dict, date_lo: int, date_hi: int, n_supp_keys: int):
+        """([(suppkey, supplier row, revenue)], lineitems in the window) of run_q15."""
+        import numpy as np
+        a = N.Q15Args()
+        a.li_rows = int(t["l_shipdate"].numel())
+        a.l_shipdate, a.l_suppkey = ptr(t["l_shipdate"]), ptr(t["l_suppkey"])
+        a.l_disc_price = ptr(t["l_disc_price"])
+        a.date_lo, a.date_hi = date_lo, date_hi
+        a.supp_rows = int(t["s_suppkey"].numel())
+        a.s_suppkey, a.n_supp_keys = ptr(t["s_suppkey"]), n_supp_keys
+        cap = max(1, n_supp_keys)
+        k, r, v = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int64)
+        n, em = C.c_int64(0), C.c_int64(0)
+        N.check(self.lib.bolap_q15(self.ctx, C.byref(a), k.ctypes.data, r.ctypes.data,
+                                   v.ctypes.data, cap, C.byref(n), C.byref(em)))
+        return [(int(k[i]), int(r[i]), int(v[i])) for i in range(n.value)], em.value
+
+    def q20(self, t: dict, p_ok_row, max_partkey: int, date_lo: int, date_hi: int, nation: int,
+            supp_nation_by_key, max_suppkey: int, capacity_hint: int = 0):
+        """(qualified[max_suppkey + 1], stats) of run_q20; p_ok_row and
+        supp_nation_by_key are device tensors."""
+        import numpy as np
+        a = N.Q20Args()
+        a.part_rows = int(t["p_partkey"].numel())
+        a.p_partkey, a.p_ok_row, a.max_partkey = ptr(t["p_partkey"]), ptr(p_ok_row), max_partkey
+        a.li_rows = int(t["l_shipdate"].numel())
+        a.l_shipdate, a.l_partkey, a.l_suppkey = (ptr(t["l_shipdate"]), ptr(t["l_partkey"]),
+                                                  ptr(t["l_suppkey"]))
+        a.l_supp_nation, a.l_quantity = ptr(t["l_supp_nation"]), ptr(t["l_quantity"])
+        a.date_lo, a.date_hi, a.nation = date_lo, date_hi, nation
+        a.ps_rows = int(t["ps_partkey"].numel())
+        a.ps_partkey, a.ps_suppkey, a.ps_availqty = (ptr(t["ps_partkey"]), ptr(t["ps_suppkey"]),
+                                                     ptr(t["ps_availqty"]))
+        a.supp_nation_by_key, a.n_supp_map = ptr(supp_nation_by_key), int(supp_nation_by_key.numel())
+        a.max_suppkey, a.capacity_hint = max_suppkey, capacity_hint
+        q = np.zeros(max(1, max_suppkey + 1), np.uint8)
+        st = N.Q20Stats()
+        N.check(self.lib.bolap_q20(self.ctx, C.byref(a), q.ctypes.data, C.byref(st)))
+        return q[:max(0, max_suppkey + 1)], st
+
     @staticmethod
     def ceb_pred(col, mask, is_null=None) -> "N.CebPred":
         """A predicate over a device int32 column: value v passes when it is

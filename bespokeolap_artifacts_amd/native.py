@@ -393,6 +393,42 @@ class Q16Args(C.Structure):
                 ("n_supp_keys", C.c_int64)]
 
 
+class Q2Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_size", vp), ("p_type", vp), ("type_ok", vp),
+                ("n_types", C.c_int32), ("size", C.c_int32), ("ps_rows", C.c_int64),
+                ("ps_partkey", vp), ("ps_suppkey", vp), ("ps_supplycost", vp),
+                ("supp_rows", C.c_int64), ("s_nationkey", vp), ("nation_ok", vp),
+                ("n_nation", C.c_int32)]
+
+
+class Q11Args(C.Structure):
+    _fields_ = [("supp_rows", C.c_int64), ("s_suppkey", vp), ("s_nationkey", vp),
+                ("nation", C.c_int32), ("n_supp_keys", C.c_int64), ("ps_rows", C.c_int64),
+                ("ps_partkey", vp), ("ps_suppkey", vp), ("ps_value", vp),
+                ("fraction", C.c_double)]
+
+
+class Q15Args(C.Structure):
+    _fields_ = [("li_rows", C.c_int64), ("l_shipdate", vp), ("l_suppkey", vp),
+                ("l_disc_price", vp), ("date_lo", C.c_int32), ("date_hi", C.c_int32),
+                ("supp_rows", C.c_int64), ("s_suppkey", vp), ("n_supp_keys", C.c_int64)]
+
+
+class Q20Args(C.Structure):
+    _fields_ = [("part_rows", C.c_int64), ("p_partkey", vp), ("p_ok_row", vp),
+                ("max_partkey", C.c_int32), ("li_rows", C.c_int64), ("l_shipdate", vp),
+                ("l_partkey", vp), ("l_suppkey", vp), ("l_supp_nation", vp), ("l_quantity", vp),
+                ("date_lo", C.c_int32), ("date_hi", C.c_int32), ("nation", C.c_int32),
+                ("ps_rows", C.c_int64), ("ps_partkey", vp), ("ps_suppkey", vp),
+                ("ps_availqty", vp), ("supp_nation_by_key", vp), ("n_supp_map", C.c_int64),
+                ("max_suppkey", C.c_int32), ("capacity_hint", C.c_int64)]
+
+
+class Q20Stats(C.Structure):
+    _fields_ = [("pairs", C.c_int64), ("capacity", C.c_int64), ("fallbacks", C.c_int32),
+                ("qualified", C.c_int64)]
+
+
 P = C.POINTER
 i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
 
@@ -467,6 +503,10 @@ PROTOTYPES = {
     "bolap_string_masks": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "bolap_q13": (C.c_int, [vp, P(Q13Args), P(i64), i32, P(i32)]),
     "bolap_q16": (C.c_int, [vp, P(Q16Args), P(i64), i64]),
+    "bolap_q2": (C.c_int, [vp, P(Q2Args), vp, i64, P(i64)]),
+    "bolap_q11": (C.c_int, [vp, P(Q11Args), vp, vp, i64, P(i64), P(i64)]),
+    "bolap_q15": (C.c_int, [vp, P(Q15Args), vp, vp, vp, i64, P(i64), P(i64)]),
+    "bolap_q20": (C.c_int, [vp, P(Q20Args), vp, P(Q20Stats)]),
     "bolap_q8": (C.c_int, [vp, P(Q8Args), P(i64), P(i64)]),
     "bolap_q9": (C.c_int, [vp, P(Q9Args), P(i64), P(C.c_uint8), P(i32), P(i32)]),
     "bolap_q22": (C.c_int, [vp, P(Q22Args), P(Q22Result)]),

@@ -1214,6 +1214,124 @@ typedef struct {
 } bolap_q16_args;
 int bolap_q16(bolap_ctx* ctx, const bolap_q16_args* args, int64_t* counts, int64_t n_counts);
 
+/* Q2 (bespoke_tpch/query_q2.cpp:146-405), its scans: a part row qualifies
+   when p_size == size and type_ok[p_type] (host array of n_types flags: the
+   type ends with TYPE, :219-231).  Its key is taken as row + 1 (:295); its
+   partsupp run (partsupp sorted by partkey, builder_impl.cpp:769-777) comes
+   from the row's stride block when that block is exactly the key's run, else
+   from a lower_bound (:288-324).  Rows whose supplier row suppkey - 1 lies in
+   the supplier table and whose nation (s_nationkey, keys below n_nation)
+   has nation_ok set (host array: the nation is in REGION) keep the minimum
+   supplycost; every such row at the minimum is a result (:331-392).
+   pairs_out receives (part row, supplier row) int32 pairs in the reference's
+   emission order (part rows ascending, then partsupp rows); *rows = their
+   number, BOLAP_ERR_CAPACITY when it exceeds max_rows.  The caller sorts by
+   (s_acctbal desc, n_name, s_name, p_partkey) with the strings (:408-423). */
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_size;
+    const uint8_t* p_type;         /* dictionary codes (at most 256 types) */
+    const uint8_t* type_ok;        /* host, n_types entries */
+    int32_t n_types;
+    int32_t size;                  /* stoi(SIZE) */
+    int64_t ps_rows;
+    const int32_t* ps_partkey;     /* sorted */
+    const int32_t* ps_suppkey;
+    const int32_t* ps_supplycost;  /* cents */
+    int64_t supp_rows;
+    const int32_t* s_nationkey;
+    const uint8_t* nation_ok;      /* host, n_nation entries, by nation key */
+    int32_t n_nation;
+} bolap_q2_args;
+int bolap_q2(bolap_ctx* ctx, const bolap_q2_args* args, int32_t* pairs_out, int64_t max_rows,
+             int64_t* rows);
+
+/* Q11 (bespoke_tpch/query_q11.cpp:119-217): suppliers with s_nationkey ==
+   nation mark their key (keys below n_supp_keys = max s_suppkey + 1, or 1
+   without suppliers).  Each run of equal ps_partkey in row order sums
+   ps_value (supplycost_availqty, builder_impl.cpp:760-766) over its rows of
+   marked suppliers; a non-zero sum is a group.  *total = the sum over all
+   marked rows; groups with (double)sum > (double)total * fraction are
+   written, sorted by value descending then partkey (:194-214).  *rows =
+   their number, BOLAP_ERR_CAPACITY beyond max_rows. */
+typedef struct {
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    const int32_t* s_nationkey;
+    int32_t nation;
+    int64_t n_supp_keys;
+    int64_t ps_rows;
+    const int32_t* ps_partkey;     /* sorted */
+    const int32_t* ps_suppkey;
+    const int32_t* ps_value;
+    double fraction;               /* stod(FRACTION) */
+} bolap_q11_args;
+int bolap_q11(bolap_ctx* ctx, const bolap_q11_args* args, int32_t* partkey_out,
+              int64_t* value_out, int64_t max_rows, int64_t* rows, int64_t* total);
+
+/* Q15 (bespoke_tpch/query_q15.cpp:215-534): lineitems with
+   (uint32)(l_shipdate - date_lo) < (uint32)(date_hi - date_lo) add
+   discounted_price to revenue[l_suppkey] (keys below n_supp_keys = max
+   s_suppkey + 1, at least 1; other keys are skipped).  *emitted = lineitems
+   in the window.  Without one, or without a positive revenue, there is no
+   result; otherwise the keys whose revenue equals the largest positive one
+   and that have a supplier row are written ascending with their last
+   supplier row and the revenue (:461-525).  *rows = their number. */
+typedef struct {
+    int64_t li_rows;
+    const int16_t* l_shipdate;     /* 16-byte aligned */
+    const int32_t* l_suppkey;
+    const int32_t* l_disc_price;
+    int32_t date_lo, date_hi;      /* int16 day offsets, as the reference parses them */
+    int64_t supp_rows;
+    const int32_t* s_suppkey;
+    int64_t n_supp_keys;
+} bolap_q15_args;
+int bolap_q15(bolap_ctx* ctx, const bolap_q15_args* args, int32_t* suppkey_out, int32_t* row_out,
+              int64_t* revenue_out, int64_t max_rows, int64_t* rows, int64_t* emitted);
+
+/* Q20 (bespoke_tpch/query_q20.cpp:331-620): part rows with p_ok_row (device
+   flags: p_name starts with COLOR) and a key in [0, max_partkey] colour
+   their key.  Lineitems in [date_lo, date_hi) (the unsigned window test)
+   whose supp_nationkey == nation and whose partkey is coloured sum
+   l_quantity per (partkey, suppkey) in an open-addressing table of a power
+   of two >= 2 x capacity_hint slots (the reference's FlatHashMap reserve,
+   :145-152, with its estimate rows x 1/7 x colour selectivity, :399-409).
+   capacity_hint = 0 counts the rows first and sizes exactly; a table that
+   fills beyond 0.7 is rerun exactly sized (stats.fallbacks).  Partsupp rows
+   with a coloured key, a supplier of the nation (supp_nation_by_key, keys
+   below n_supp_map) and a sum set qualified[suppkey] (host array of
+   max_suppkey + 1 bytes) when availqty * 2 * 100 > sum (:548-580). */
+typedef struct {
+    int64_t part_rows;
+    const int32_t* p_partkey;
+    const uint8_t* p_ok_row;       /* device */
+    int32_t max_partkey;           /* -1 without parts */
+    int64_t li_rows;
+    const int16_t* l_shipdate;     /* 16-byte aligned */
+    const int32_t* l_partkey;
+    const int32_t* l_suppkey;
+    const int16_t* l_supp_nation;  /* 16-byte aligned */
+    const int16_t* l_quantity;     /* x 100 */
+    int32_t date_lo, date_hi, nation;
+    int64_t ps_rows;
+    const int32_t* ps_partkey;
+    const int32_t* ps_suppkey;
+    const int32_t* ps_availqty;
+    const int16_t* supp_nation_by_key;
+    int64_t n_supp_map;
+    int32_t max_suppkey;           /* -1 without suppliers */
+    int64_t capacity_hint;
+} bolap_q20_args;
+typedef struct {
+    int64_t pairs;      /* (partkey, suppkey) groups */
+    int64_t capacity;   /* table slots of the final run */
+    int32_t fallbacks;  /* 1 when the estimate was short */
+    int64_t qualified;  /* suppkeys set */
+} bolap_q20_stats;
+int bolap_q20(bolap_ctx* ctx, const bolap_q20_args* args, uint8_t* qualified,
+              bolap_q20_stats* stats);
+
 /* ------------------------------------------------------------------------ */
 /* Build-time derivations of the reference Database layout on the device.    */
 /* ------------------------------------------------------------------------ */

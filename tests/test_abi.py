@@ -64,6 +64,8 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "Q19Args": "bolap_q19_args", "Q21Args": "bolap_q21_args", "Q22Args": "bolap_q22_args",
     "Q22Result": "bolap_q22_result", "Q7Args": "bolap_q7_args", "Q8Args": "bolap_q8_args",
     "Q9Args": "bolap_q9_args", "Q13Args": "bolap_q13_args", "Q16Args": "bolap_q16_args",
+    "Q2Args": "bolap_q2_args", "Q11Args": "bolap_q11_args", "Q15Args": "bolap_q15_args",
+    "Q20Args": "bolap_q20_args", "Q20Stats": "bolap_q20_stats",
     "MinmaxArgs": "bolap_minmax_args", "MinmaxRow": "bolap_minmax_row",
     "MinmaxStats": "bolap_minmax_stats", "CebPred": "bolap_ceb_pred", "CebTerm": "bolap_ceb_term",
     "CebCountArgs": "bolap_ceb_count_args", "CebCountResult": "bolap_ceb_count_result",

@@ -341,3 +341,28 @@ def test_sf10_all_five(engine, sf10):
     assert num[:, 0].sum() > 0
     assert engine.q22(t, [13, 31, 23, 29, 30, 18, 17], [(ord("A") << 8) | ord("B")]) == \
         (num.tolist(), ext.tolist())
+    print("q2 q11 q15 q20", flush=True)
+    nation_ok = (np.array(h["n_regionkey"]) == 3).astype(np.uint8)
+    t_ok = np.array([x.endswith("BRASS") for x in h["type_dict"]], np.uint8)
+    want2 = oracle.q2(h, t_ok, 15, nation_ok[h["s_nationkey"]])
+    assert len(want2) > 1000
+    np.testing.assert_array_equal(engine.q2(t, t_ok, 15, nation_ok), want2)
+    nk = int(h["s_suppkey"].max()) + 1
+    pk11, val11, tot11 = engine.q11(t, 7, nk, 0.0001 / 10)
+    w11 = oracle.q11(h, 7, 0.0001 / 10)
+    assert len(w11[0]) > 100 and tot11 == w11[2]
+    assert np.array_equal(pk11, w11[0]) and np.array_equal(val11, w11[1])
+    lo15, hi15 = T.date_offset("1996-01-01"), T.add_months("1996-01-01", 3)
+    assert engine.q15(t, lo15, hi15, nk) == oracle.q15(h, lo15, hi15)
+    p_ok = np.isin(h["p_name_words"][:, 0], [T.COLORS.index("forest")]).astype(np.uint8)
+    smap = np.full(nk, -1, np.int16)
+    smap[h["s_suppkey"]] = h["s_nationkey"]
+    lo20, hi20 = T.date_offset("1994-01-01"), T.add_months("1994-01-01", 12)
+    want20, n20, pairs20 = oracle.q20(h, p_ok, lo20, hi20, 3)
+    assert n20 > 10
+    for hint in (0, 64, 1 << 16):   # exact, short (rerun), generous
+        q20, st20 = engine.q20(t, torch.from_numpy(p_ok).cuda(), int(h["p_partkey"].max()),
+                               lo20, hi20, 3, torch.from_numpy(smap).cuda(), nk - 1,
+                               capacity_hint=hint)
+        assert np.array_equal(q20, want20) and (st20.qualified, st20.pairs) == (n20, pairs20)
+        assert st20.fallbacks == (1 if hint == 64 else 0)
