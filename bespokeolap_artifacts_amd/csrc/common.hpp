@@ -323,6 +323,23 @@ struct WaveCount {
     }
 };
 
+// Slot of this lane's item in a list whose length is *counter, for lanes
+// with `p`; one atomic per wave.  Every lane of the wave must call it;
+// lanes without `p` get ~0ull.
+BOLAP_DEV unsigned long long wave_append(bool p, unsigned long long* counter) {
+    const uint64_t m = __ballot(p);
+    if (m == 0) return ~0ull;
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t rank =
+        __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    unsigned long long base = 0;
+    if (int(__lane_id()) == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(base)), leader));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(base >> 32)), leader));
+    const unsigned long long b = (uint64_t(hi) << 32) | lo;
+    return p ? b + rank : ~0ull;
+}
+
 // 64-bit mixing hash for open-addressing tables (SplitMix64 output function).
 BOLAP_DEV uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;

@@ -1,16 +1,24 @@
 // fold.hpp — workgroup folds into replicated device statistics.
 //
 // A statistic is an int64 with kRep replicas: workgroup b folds into replica
-// b % kRep, so the workgroups of one XCD share a replica and no address sees
-// more than (#workgroups / kRep) atomics (same-address atomics serialise at
-// roughly 10 ns each).  The host folds the replicas after the copy back.
+// b % kRep, so a replica's workgroups sit on one XCD (workgroups are dealt to
+// the XCDs in turn) and no address sees more than (#workgroups / kRep)
+// atomics.  Each replica holds its kStatLine statistics in one 128-byte line
+// of its own: atomics on one line serialise (roughly 5-10 ns each), so the
+// replicas must not share lines.  The host folds the replicas after the copy
+// back; the area is kStatWords int64 slots.
 #pragma once
 
 #include "common.hpp"
 
 namespace bolap {
 
-constexpr int kRep = 8;
+constexpr int kRep = 16;
+constexpr int kStatLine = 16;                   // statistics per replica (one line)
+constexpr int kStatWords = kRep * kStatLine;    // int64 slots of the statistics area
+
+// Slot of statistic idx (< kStatLine) in replica rep.
+__host__ __device__ constexpr int stat_slot(int idx, int rep) { return rep * kStatLine + idx; }
 
 BOLAP_DEV int64_t wave_min(int64_t v) {
 #pragma unroll
@@ -45,7 +53,7 @@ BOLAP_DEV void block_fold(int64_t* st, int idx, int64_t wave_value, int op) {
             v = op == 0 ? int64_t(uint64_t(v) + uint64_t(x)) : op == 1 ? (x < v ? x : v)
                                                                        : (x > v ? x : v);
         }
-        long long* dst = reinterpret_cast<long long*>(st + idx * kRep + (blockIdx.x % kRep));
+        long long* dst = reinterpret_cast<long long*>(st + stat_slot(idx, blockIdx.x % kRep));
         if (op == 0) {
             if (v) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
         } else if (op == 1) {

@@ -85,8 +85,10 @@ struct GTab {
 
 BOLAP_DEV uint32_t slot_of(uint32_t key1, int log2) { return (key1 * kMul) >> (32 - log2); }
 
+static_assert(sN <= kStatLine, "statistics fit a replica line");
+
 __global__ void st_init_k(int64_t* st) {
-    if (threadIdx.x < sN * kRep) st[threadIdx.x] = 0;
+    if (threadIdx.x < kStatWords) st[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(kT) void fill32_k(uint32_t* p, int64_t n) {
@@ -398,7 +400,7 @@ int ceil_log2(int64_t v) {
 }
 
 int read_stats(bolap_ctx* ctx, int64_t* dst) {
-    const size_t bytes = sN * kRep * 8;
+    const size_t bytes = kStatWords * 8;
     BOLAP_TRY(pinned_reserve(ctx, bytes));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->result_dev, bytes, hipMemcpyDeviceToHost,
                                  ctx->stream));
@@ -406,7 +408,7 @@ int read_stats(bolap_ctx* ctx, int64_t* dst) {
     const int64_t* rep = static_cast<const int64_t*>(ctx->pinned);
     for (int i = 0; i < sN; ++i) {
         uint64_t v = 0;
-        for (int r = 0; r < kRep; ++r) v += uint64_t(rep[i * kRep + r]);
+        for (int r = 0; r < kRep; ++r) v += uint64_t(rep[stat_slot(i, r)]);
         dst[i] = int64_t(v);
     }
     return BOLAP_OK;
@@ -490,7 +492,7 @@ unsigned probe_grid(bolap_ctx* ctx, int64_t rows, size_t lds) {
 int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, bool want_rows, WsArena& ws,
              StarRun& run) {
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
-    hipLaunchKernelGGL(st_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+    hipLaunchKernelGGL(st_init_k, dim3(1), dim3(kStatWords), 0, ctx->stream, st);
     BOLAP_TRY(launch_check("st_init_k"));
     std::memset(run.st, 0, sizeof(run.st));
     StarK& k = run.k;
@@ -525,7 +527,7 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, bool want_row
         ++run.count_passes;
         for (int j = 0; j < a->n_dims; ++j)
             if (exact || est[j] == 0) est[j] = run.st[sPass0 + j];
-        hipLaunchKernelGGL(st_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+        hipLaunchKernelGGL(st_init_k, dim3(1), dim3(kStatWords), 0, ctx->stream, st);
     }
     size_t o_set[kMaxDims];
     int64_t words[kMaxDims];
@@ -606,7 +608,7 @@ int star_run(bolap_ctx* ctx, const bolap_star_args* a, bool exact, bool want_row
     const bool had_read = group_read;
     if (had_read) {
         std::memcpy(pre, run.st, sizeof(pre));
-        hipLaunchKernelGGL(st_init_k, dim3(1), dim3(sN * kRep), 0, ctx->stream, st);
+        hipLaunchKernelGGL(st_init_k, dim3(1), dim3(kStatWords), 0, ctx->stream, st);
     }
     {
         TraceScope ts(ctx, "star_init");

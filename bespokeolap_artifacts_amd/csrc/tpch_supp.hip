@@ -53,22 +53,6 @@ int fetch(bolap_ctx* ctx, void* host, const void* dev, size_t bytes) {
     return BOLAP_OK;
 }
 
-// Slot of this lane's item in a list whose length is *counter, for lanes
-// with `p`; one atomic per wave.  Lanes without `p` get ~0ull.
-BOLAP_DEV unsigned long long wave_append(bool p, unsigned long long* counter) {
-    const uint64_t m = __ballot(p);
-    if (m == 0) return ~0ull;
-    const int leader = __ffsll((long long)m) - 1;
-    const uint32_t rank =
-        __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    unsigned long long base = 0;
-    if (int(__lane_id()) == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(base)), leader));
-    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(base >> 32)), leader));
-    const unsigned long long b = (uint64_t(hi) << 32) | lo;
-    return p ? b + rank : ~0ull;
-}
-
 // ---------------------------------------------------------------------------
 // Q2: per qualifying part, the partsupp rows of in-region suppliers at the
 // minimum supply cost.
