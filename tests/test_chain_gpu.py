@@ -340,3 +340,38 @@ def test_chain_ship_filter_matches_oracle(engine, path, ship_gt):
     assert np.array_equal(allg, want)
     assert np.array_equal(as_array(top), want[:10])
     check_stats(gs, st)
+
+
+@pytest.mark.parametrize("k", [1, 10, 64])
+def test_chain_topk_when_revenues_tie(engine, k):
+    """Every group has the same revenue, so one revenue code holds every row:
+    the top-k cut cannot narrow the candidates and the call orders the rows
+    by the radix path instead (day, then key, break the ties).  A second
+    table with a few distinct revenues on top checks the cut itself."""
+    import torch
+    rng = np.random.default_rng(5)
+    orders, cust = 30_000, 3_000
+    t = {"cust_key": torch.arange(1, cust + 1, dtype=torch.int32),
+         "cust_segment": torch.from_numpy(rng.integers(0, 2, cust).astype(np.uint8)),
+         "order_key": torch.arange(1, orders + 1, dtype=torch.int32),
+         "order_day": torch.from_numpy(rng.integers(0, 50, orders).astype(np.int32)),
+         "order_cust": torch.from_numpy(rng.integers(1, cust + 1, orders).astype(np.int32)),
+         "fact_order": torch.arange(1, orders + 1, dtype=torch.int32),
+         "fact_price": torch.full((orders,), 1000, dtype=torch.int64),
+         "fact_disc": torch.full((orders,), 5, dtype=torch.int64)}
+    for bump in (False, True):
+        if bump:  # a few orders stand out: the cut is narrow again
+            p = t["fact_price"].clone()
+            p[rng.integers(0, orders, 100)] += torch.from_numpy(rng.integers(1, 10**6, 100))
+            t["fact_price"] = p
+        d = {key: v.cuda() for key, v in t.items()}
+        want, _ = want_sorted(d, 1, 25)
+        assert len(want) > 2048
+        for ranges in (None, "ranges"):
+            r = None
+            if ranges:
+                ar = torch.arange(orders, dtype=torch.int32)
+                r = (ar.cuda(), (ar + 1).cuda())
+            top, _ = engine.chain_topk(engine.chain_args(d, 1, 25, ranges=r,
+                                                         fact_order_sorted=True), k)
+            assert np.array_equal(as_array(top), want[:k])
