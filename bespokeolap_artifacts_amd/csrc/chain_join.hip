@@ -51,21 +51,6 @@ struct alignas(16) RowT {
     int32_t key;
 };
 
-// Appends `cnt` entries staged in LDS to `out` (one atomic), then empties
-// the stage.  Every thread of the workgroup calls it with the same cnt.
-template <typename T>
-BOLAP_DEV void flush_stage(T* stage, uint32_t* s_n, unsigned long long* s_base, uint32_t cnt,
-                           T* __restrict__ out, unsigned long long* __restrict__ n_out) {
-    if (threadIdx.x == 0) *s_base = cnt ? atomicAdd(n_out, (unsigned long long)cnt) : 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += kT) out[*s_base + i] = stage[i];
-    __syncthreads();
-    if (threadIdx.x == 0) *s_n = 0;
-    __syncthreads();
-}
-
-
-
 static_assert(sN <= kStatLine, "statistics fit a replica line");
 
 __global__ void stats_init_k(int64_t* st) {
@@ -791,13 +776,13 @@ __global__ __launch_bounds__(kT) void compact_list_k(GTable t, Ranges r,
             const uint32_t cnt = s_n;
             __syncthreads();
             if (cnt > uint32_t(kRowStage - 2 * kT))
-                flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+                flush_stage<kT>(s_rows, &s_n, &s_base, cnt, out, n_out);
         }
     }
     __syncthreads();
     const uint32_t cnt = s_n;
     __syncthreads();
-    flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+    flush_stage<kT>(s_rows, &s_n, &s_base, cnt, out, n_out);
     fold_minmax<kT>(rl, rh, st, sRMin, sRMax);
 }
 
@@ -967,12 +952,12 @@ __global__ __launch_bounds__(kT) void range_filter_k(RangeArgs a, const uint32_t
         const uint32_t cnt = s_n;
         __syncthreads();
         if (cnt > uint32_t(kPassCap - kPassBuf))
-            flush_stage(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
+            flush_stage<kT>(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
     }
     __syncthreads();
     const uint32_t cnt = s_n;
     __syncthreads();
-    flush_stage(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
+    flush_stage<kT>(s_idx, &s_n, &s_base, cnt, pass_idx, n_pass);
     block_fold<kT>(st, sOrderPass, int64_t(passed.n), 0);
 }
 
@@ -1051,12 +1036,13 @@ __global__ __launch_bounds__(kT) void range_groups_k(RangeArgs a, const TP* __re
         __syncthreads();
         const uint32_t cnt = s_n;
         __syncthreads();
-        if (cnt > uint32_t(kRangeBuf - kT * 4)) flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+        if (cnt > uint32_t(kRangeBuf - kT * 4))
+            flush_stage<kT>(s_rows, &s_n, &s_base, cnt, out, n_out);
     }
     __syncthreads();
     const uint32_t cnt = s_n;
     __syncthreads();
-    flush_stage(s_rows, &s_n, &s_base, cnt, out, n_out);
+    flush_stage<kT>(s_rows, &s_n, &s_base, cnt, out, n_out);
     block_fold<kT>(st, sMatch, int64_t(wave_sum(matched)), 0);
     fold_minmax<kT>(dl, dh, st, sDMin, sDMax);
     fold_minmax<kT>(kl, kh, st, sOMin, sOMax);

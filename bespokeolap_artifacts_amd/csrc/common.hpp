@@ -340,6 +340,20 @@ BOLAP_DEV unsigned long long wave_append(bool p, unsigned long long* counter) {
     return p ? b + rank : ~0ull;
 }
 
+// Appends `cnt` entries staged in LDS to `out` (one atomic), then empties
+// the stage.  Every thread of the BS-thread workgroup calls it with the same
+// cnt.
+template <int BS, typename T, typename C>
+BOLAP_DEV void flush_stage(T* stage, C* s_n, unsigned long long* s_base, uint32_t cnt,
+                           T* __restrict__ out, unsigned long long* __restrict__ n_out) {
+    if (threadIdx.x == 0) *s_base = cnt ? atomicAdd(n_out, (unsigned long long)cnt) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += BS) out[*s_base + i] = stage[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_n = 0;
+    __syncthreads();
+}
+
 // 64-bit mixing hash for open-addressing tables (SplitMix64 output function).
 BOLAP_DEV uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;

@@ -465,17 +465,29 @@ __global__ __launch_bounds__(kT) void fill_bytes_k(uint32_t* __restrict__ p, int
 }
 
 // Maximum of an i32 column into *out (which starts at INT32_MIN).
+// Workgroup maximum of m; every thread gets it.
+BOLAP_DEV int32_t block_max_i32(int32_t m) {
+    __shared__ int32_t s_m[kT / kWave];
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const int32_t o = __shfl_xor(m, d, kWave);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_m[threadIdx.x / kWave] = m;
+    __syncthreads();
+    m = s_m[0];
+    for (int w = 1; w < kT / kWave; ++w) m = s_m[w] > m ? s_m[w] : m;
+    return m;
+}
+
+// One atomic per workgroup (same-address atomics serialise).
 __global__ __launch_bounds__(kT) void max_i32_k(const int32_t* __restrict__ v, int64_t n,
                                                 int32_t* __restrict__ out) {
     int32_t m = INT32_MIN;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT)
         m = v[i] > m ? v[i] : m;
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        const int32_t o = __shfl_xor(m, d, kWave);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0 && m != INT32_MIN) atomicMax(out, m);
+    m = block_max_i32(m);
+    if (threadIdx.x == 0 && m != INT32_MIN) atomicMax(out, m);
 }
 
 inline unsigned grid_of(bolap_ctx* ctx, int64_t items, int per_cu = 4) {
@@ -506,7 +518,8 @@ int max_i32(bolap_ctx* ctx, const int32_t* v, int64_t n, int32_t* slot, int32_t*
     const int32_t init = INT32_MIN;
     BOLAP_HIP_TRY(hipMemcpyAsync(slot, &init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
     if (n > 0) {
-        hipLaunchKernelGGL(max_i32_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream, v, n, slot);
+        hipLaunchKernelGGL(max_i32_k, dim3(grid_of(ctx, n, 1)), dim3(kT), 0, ctx->stream, v, n,
+                           slot);
         BOLAP_TRY(launch_check("max_i32_k"));
     }
     return fetch(ctx, out, slot, sizeof(int32_t));
@@ -1141,9 +1154,28 @@ __global__ __launch_bounds__(kT) void q8_k(Q8K a, unsigned long long* __restrict
     }
 }
 
-// Q9: partsupp rows grouped by partkey (CSR in three steps: counts, scan,
-// fill; a key's rows land in any order and the lookup takes the smallest
-// row of a match, which is the reference's first match in row order).
+// Q9: partsupp rows grouped by partkey.  The builder's layout keeps
+// partsupp sorted by partkey; then the offsets come straight from the run
+// boundaries and a key's rows are the contiguous range off[k]..off[k + 1].
+// Row i writes off[k] = i for the keys k in (pk[i - 1], pk[i]] within
+// [0, n], so each offset has one writer; a descent anywhere sets *unsorted.
+__global__ __launch_bounds__(kT) void ps_offsets_k(const int32_t* __restrict__ pk, int64_t rows,
+                                                   int64_t n, uint32_t* __restrict__ off,
+                                                   uint32_t* __restrict__ unsorted) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i <= rows;
+         i += int64_t(gridDim.x) * kT) {
+        const int64_t prev = i == 0 ? INT64_MIN / 2 : int64_t(pk[i - 1]);
+        const int64_t cur = i == rows ? INT64_MAX / 2 : int64_t(pk[i]);
+        if (cur < prev) *unsorted = 1u;
+        const int64_t lo = prev + 1 > 0 ? prev + 1 : 0;
+        const int64_t hi = cur < n ? cur : n;
+        for (int64_t k = lo; k <= hi; ++k) off[k] = uint32_t(i);
+    }
+}
+
+// Otherwise a CSR in three steps (counts, scan, fill); a key's rows land in
+// any order and the lookup takes the smallest row of a match, which is the
+// reference's first match in row order.
 __global__ __launch_bounds__(kT) void ps_count_k(const int32_t* __restrict__ pk, int64_t rows,
                                                  int64_t n, uint32_t* __restrict__ cnt) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
@@ -1244,12 +1276,20 @@ struct Q9K {
 
 BOLAP_DEV void q9_row(const Q9K& a, int64_t i, int32_t p, unsigned long long* s_tot,
                       uint8_t* s_used) {
-    if (p < 0 || p >= a.np || !a.pmark[p]) return;
     const int32_t k = a.sk[i];
     uint32_t best = 0xffffffffu;
-    for (uint32_t j = a.ps_off[p]; j < a.ps_off[p + 1]; ++j) {
-        const uint32_t row = a.ps_idx[j];
-        if (a.ps_sk[row] == k && row < best) best = row;
+    if (a.ps_idx) {
+        for (uint32_t j = a.ps_off[p]; j < a.ps_off[p + 1]; ++j) {
+            const uint32_t row = a.ps_idx[j];
+            if (a.ps_sk[row] == k && row < best) best = row;
+        }
+    } else {  // sorted partsupp: the first match in the key's range
+        for (uint32_t j = a.ps_off[p]; j < a.ps_off[p + 1]; ++j) {
+            if (a.ps_sk[j] == k) {
+                best = j;
+                break;
+            }
+        }
     }
     if (best == 0xffffffffu) return;
     const int32_t nk = a.snation[i];
@@ -1266,26 +1306,49 @@ BOLAP_DEV void q9_row(const Q9K& a, int64_t i, int32_t p, unsigned long long* s_
     s_used[g] = 1;
 }
 
+// A workgroup takes tiles of kQ9Tile lineitem rows: the rows of marked parts
+// are gathered into LDS first (about one in twenty for one colour), then
+// every lane of the workgroup takes one of them through the partsupp,
+// order and date lookups, so the dependent loads run with full waves.
+constexpr int kQ9Tile = kT * 16;
+
 __global__ __launch_bounds__(kT) void q9_k(Q9K a, unsigned long long* __restrict__ tot,
                                            uint8_t* __restrict__ used) {
     __shared__ unsigned long long s_tot[kQ9Groups];
     __shared__ uint8_t s_used[kQ9Groups];
+    __shared__ uint32_t s_rows[kQ9Tile];
+    __shared__ unsigned long long s_n;
     const int groups = a.n_nation * a.span;
     for (int g = threadIdx.x; g < groups; g += kT) {
         s_tot[g] = 0;
         s_used[g] = 0;
     }
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const int64_t units = a.rows / 4;
-    const int64_t stride = int64_t(gridDim.x) * kT;
-    for (int64_t u = int64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += stride) {
-        const VecT<int32_t, 4> p4 = load_unit<int32_t, 4>(a.pk, u);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) q9_row(a, u * 4 + j, p4.v[j], s_tot, s_used);
+    const int64_t tiles = (a.rows + kQ9Tile - 1) / kQ9Tile;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base = t * kQ9Tile;
+#pragma unroll 4
+        for (int j = 0; j < kQ9Tile / kT; ++j) {
+            const int64_t i = base + int64_t(j) * kT + threadIdx.x;
+            bool hit = false;
+            if (i < a.rows) {
+                const int32_t p = a.pk[i];
+                hit = p >= 0 && p < a.np && a.pmark[p];
+            }
+            const unsigned long long slot = wave_append(hit, &s_n);
+            if (hit) s_rows[slot] = uint32_t(i - base);
+        }
+        __syncthreads();
+        const uint32_t n = uint32_t(s_n);
+        for (uint32_t x = threadIdx.x; x < n; x += kT) {
+            const int64_t i = base + s_rows[x];
+            q9_row(a, i, a.pk[i], s_tot, s_used);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
     }
-    for (int64_t i = units * 4 + int64_t(blockIdx.x) * kT + threadIdx.x; i < a.rows; i += stride)
-        q9_row(a, i, a.pk[i], s_tot, s_used);
-    __syncthreads();
     for (int g = threadIdx.x; g < groups; g += kT) {
         if (s_tot[g]) atomicAdd(tot + g, s_tot[g]);
         if (s_used[g]) used[g] = 1;
@@ -1443,6 +1506,7 @@ extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals,
     const size_t o_off = ws.plan(size_t(np + 1) * 4);
     const size_t o_tot = ws.plan(size_t(kQ9Groups) * 8);
     const size_t o_used = ws.plan(size_t(kQ9Groups));
+    const size_t o_flag = ws.plan(4);
     const size_t o_zero_end = ws.off;
     const size_t o_cur = ws.plan(size_t(np + 1) * 4);
     const size_t o_tiles = ws.plan(size_t(tiles + 1) * 4);
@@ -1453,6 +1517,7 @@ extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals,
     auto* off = ws.at<uint32_t>(o_off);
     auto* cur = ws.at<uint32_t>(o_cur);
     auto* tl = ws.at<uint32_t>(o_tiles);
+    uint32_t unsorted = 0;
     {
         TraceScope ts(ctx, "q9_build");
         if (a->part_rows > 0 && np > 0)
@@ -1461,6 +1526,14 @@ extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals,
                                int32_t(BOLAP_U8), a->part_rows, np, 1, 1,
                                static_cast<const uint8_t*>(nullptr), 0, ws.at<uint8_t>(o_mark));
         if (a->ps_rows > 0 && np > 0) {
+            hipLaunchKernelGGL(ps_offsets_k, dim3(grid_of(ctx, a->ps_rows + 1)), dim3(kT), 0,
+                               ctx->stream, a->ps_partkey, a->ps_rows, np, off,
+                               ws.at<uint32_t>(o_flag));
+            BOLAP_TRY(launch_check("ps_offsets_k"));
+            BOLAP_TRY(fetch(ctx, &unsorted, ws.at<uint32_t>(o_flag), 4));
+        }
+        if (a->ps_rows > 0 && np > 0 && unsorted) {
+            BOLAP_TRY(zero_dev(ctx, off, size_t(np + 1) * 4));
             hipLaunchKernelGGL(ps_count_k, dim3(grid_of(ctx, a->ps_rows)), dim3(kT), 0, ctx->stream,
                                a->ps_partkey, a->ps_rows, np, off);
             hipLaunchKernelGGL(tile_sum_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, off,
@@ -1481,9 +1554,9 @@ extern "C" int bolap_q9(bolap_ctx* ctx, const bolap_q9_args* a, int64_t* totals,
         TraceScope ts(ctx, "q9_lineitem");
         const Q9K k{a->li_rows, a->l_partkey, a->l_suppkey, a->l_orderkey, a->l_quantity,
                     a->l_disc_price, a->l_supp_nation, ws.at<uint8_t>(o_mark), np, off,
-                    ws.at<uint32_t>(o_idx), a->ps_suppkey, a->ps_supplycost, a->order_map,
+                    unsorted ? ws.at<uint32_t>(o_idx) : nullptr, a->ps_suppkey, a->ps_supplycost, a->order_map,
                     a->map_size, a->o_orderdate, a->n_nation, y0, span};
-        hipLaunchKernelGGL(q9_k, dim3(grid_of(ctx, (a->li_rows + 3) / 4, 8)), dim3(kT), 0,
+        hipLaunchKernelGGL(q9_k, dim3(grid_of(ctx, (a->li_rows + 15) / 16, 4)), dim3(kT), 0,
                            ctx->stream, k, tot, usd);
         BOLAP_TRY(launch_check("q9_k"));
     }
@@ -1533,6 +1606,10 @@ struct Q13W {
     int32_t n1, n2;
     uint32_t req_alpha;
     uint64_t req_bigram;
+    // each word's first min(4, n) bytes packed little-endian (pre), their
+    // count (m) and the right shift that aligns a 4-byte window to them (sh)
+    uint32_t pre1, pre2;
+    int32_t m1, m2, sh1, sh2;
 };
 
 // First position p in [s, e) where w occurs (s itself for an empty word),
@@ -1549,25 +1626,112 @@ BOLAP_DEV uint32_t find_word(const uint8_t* __restrict__ chars, uint32_t s, uint
     return e;
 }
 
+BOLAP_DEV bool tail_eq(const uint8_t* __restrict__ c, const uint8_t* w, int32_t from, int32_t n) {
+    for (int32_t j = from; j < n; ++j)
+        if (c[j] != w[j]) return false;
+    return true;
+}
+
+// The two searches of the reference (query_q13.cpp:178-186): p1 = first
+// occurrence of word1 in [s, e) (s for an empty word), then an occurrence of
+// word2 in [p1 + n1, e); a match needs p1 != e and the second result != e.
+// Strings of up to kRegBytes bytes (with their alignment lead) are loaded
+// into registers once and scanned with a rolling 4-byte window against each
+// word's first bytes, so no byte load waits in the loop; a window hit checks
+// the word's remaining bytes in memory.  Longer strings take find_word.
+constexpr int kRegWords = 24;
+constexpr int kRegBytes = kRegWords * 4;
+
+BOLAP_DEV bool comment_match(const Q13W& w, const uint8_t* __restrict__ chars, uint32_t s,
+                             uint32_t e) {
+    const int32_t len = int32_t(e - s);
+    const uintptr_t at = reinterpret_cast<uintptr_t>(chars + s);
+    const int32_t lead = int32_t(at & 3u);
+    if (len + lead > kRegBytes) {
+        const uint32_t p1 = find_word(chars, s, e, w.w1, w.n1);
+        return p1 != e && find_word(chars, p1 + uint32_t(w.n1), e, w.w2, w.n2) != e;
+    }
+    if (len == 0) return false;  // p1 == e whatever the words
+    // aligned words holding at least one byte of the string (never past its page)
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(at - uintptr_t(lead));
+    const int32_t nw = (lead + len + 3) >> 2;
+    uint32_t W[kRegWords];
+#pragma unroll
+    for (int k = 0; k < kRegWords; ++k) W[k] = k < nw ? wp[k] : 0u;
+    const uint8_t* str = chars + s;
+    int32_t first1 = w.n1 == 0 ? 0 : -1;
+    bool found = false;
+    uint32_t win = 0;
+#pragma unroll
+    for (int k = 0; k < kRegWords; ++k) {
+        if (k >= nw || found) break;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            win = (win >> 8) | ((W[k] >> (8 * b)) << 24);
+            const int32_t r = 4 * k + b - lead;  // window ends at str[r]
+            if (r < 0 || r >= len) continue;
+            if (first1 < 0 && (win >> w.sh1) == w.pre1) {
+                const int32_t st = r - w.m1 + 1;
+                if (st >= 0 && st + w.n1 <= len && tail_eq(str + st, w.w1, w.m1, w.n1))
+                    first1 = st;
+            }
+            if (w.n2 > 0 && first1 >= 0 && (win >> w.sh2) == w.pre2) {
+                const int32_t st = r - w.m2 + 1;
+                if (st >= first1 + w.n1 && st + w.n2 <= len &&
+                    tail_eq(str + st, w.w2, w.m2, w.n2))
+                    found = true;
+            }
+        }
+    }
+    if (w.n2 == 0) return first1 >= 0 && first1 + w.n1 < len;
+    return found;
+}
+
+// Per-customer order counts of the orders whose comment does not match.  A
+// workgroup takes tiles of kQ13Tile orders: orders the prefilter masks rule
+// out are counted at once; the candidates are gathered into LDS and then
+// searched one per lane, so the string scans run with full waves.
+constexpr int kQ13Tile = kT * 8;
+
 __global__ __launch_bounds__(kT) void q13_k(Q13W w, int64_t rows, const int32_t* __restrict__ ock,
                                             const uint8_t* __restrict__ chars,
                                             const uint32_t* __restrict__ off,
                                             const uint32_t* __restrict__ alpha,
                                             const uint64_t* __restrict__ bigram, int64_t size,
                                             int32_t* __restrict__ cnt) {
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
-         i += int64_t(gridDim.x) * kT) {
-        bool match = false;
-        const bool cand = (!alpha || (alpha[i] & w.req_alpha) == w.req_alpha) &&
-                          (!bigram || (bigram[i] & w.req_bigram) == w.req_bigram);
-        if (cand) {
-            const uint32_t s = off[i], e = off[i + 1];
-            const uint32_t p1 = find_word(chars, s, e, w.w1, w.n1);
-            match = p1 != e && find_word(chars, p1 + uint32_t(w.n1), e, w.w2, w.n2) != e;
+    __shared__ uint32_t s_c[kQ13Tile];
+    __shared__ unsigned long long s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t tiles = (rows + kQ13Tile - 1) / kQ13Tile;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base = t * kQ13Tile;
+#pragma unroll 2
+        for (int j = 0; j < kQ13Tile / kT; ++j) {
+            const int64_t i = base + int64_t(j) * kT + threadIdx.x;
+            bool cand = false;
+            if (i < rows) {
+                cand = (!alpha || (alpha[i] & w.req_alpha) == w.req_alpha) &&
+                       (!bigram || (bigram[i] & w.req_bigram) == w.req_bigram);
+                if (!cand) {
+                    const int32_t k = ock[i];
+                    if (k >= 0 && k < size) atomicAdd(cnt + k, 1);
+                }
+            }
+            const unsigned long long slot = wave_append(cand, &s_n);
+            if (cand) s_c[slot] = uint32_t(i - base);
         }
-        if (match) continue;
-        const int32_t k = ock[i];
-        if (k >= 0 && k < size) atomicAdd(cnt + k, 1);
+        __syncthreads();
+        const uint32_t n = uint32_t(s_n);
+        for (uint32_t x = threadIdx.x; x < n; x += kT) {
+            const int64_t i = base + s_c[x];
+            if (comment_match(w, chars, off[i], off[i + 1])) continue;
+            const int32_t k = ock[i];
+            if (k >= 0 && k < size) atomicAdd(cnt + k, 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
     }
 }
 
@@ -1577,11 +1741,8 @@ __global__ __launch_bounds__(kT) void max_i32_arr_k(const int32_t* __restrict__ 
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT)
         m = v[i] > m ? v[i] : m;
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        const int32_t o = __shfl_xor(m, d, kWave);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out, m);
+    m = block_max_i32(m);
+    if (threadIdx.x == 0 && m) atomicMax(out, m);
 }
 
 constexpr int kDistLds = 4096;
@@ -1639,30 +1800,71 @@ __global__ __launch_bounds__(kT) void q16_part_k(Q16P a, unsigned long long* __r
     }
 }
 
+// The (group, suppkey) keys of the qualifying partsupp rows, in any order
+// (they are sorted next).  Keys are staged in LDS and appended a stage at a
+// time, so the output counter takes one atomic per flush.
+constexpr int kKeyStage = 2048;
+
 __global__ __launch_bounds__(kT) void q16_ps_k(int64_t rows, const int32_t* __restrict__ pk,
                                                const int32_t* __restrict__ sk, int64_t np,
                                                const unsigned long long* __restrict__ owner,
                                                const uint8_t* __restrict__ banned, int64_t ns,
                                                uint64_t* __restrict__ keys,
                                                unsigned long long* __restrict__ n_out) {
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < rows;
-         i += int64_t(gridDim.x) * kT) {
-        const int32_t k = pk[i];
-        if (k < 0 || k >= np) continue;
-        const unsigned long long w = owner[k];
-        if (w == 0) continue;
-        const int32_t s = sk[i];
-        if (s >= 0 && s < ns && banned[s]) continue;
-        const uint64_t g = w & 0xffffffffull;
-        keys[atomicAdd(n_out, 1ull)] = (g << 32) | uint32_t(s);
+    __shared__ uint64_t s_keys[kKeyStage];
+    __shared__ unsigned long long s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t stride_g = int64_t(gridDim.x) * kT;
+    const int64_t rounds = (rows + stride_g - 1) / stride_g;
+    for (int64_t r = 0; r < rounds; ++r) {
+        const int64_t i = r * stride_g + int64_t(blockIdx.x) * kT + threadIdx.x;
+        bool hit = false;
+        uint64_t key = 0;
+        if (i < rows) {
+            const int32_t k = pk[i];
+            const unsigned long long w = (k >= 0 && k < np) ? owner[k] : 0ull;
+            if (w != 0) {
+                const int32_t s = sk[i];
+                hit = !(s >= 0 && s < ns && banned[s]);
+                key = ((w & 0xffffffffull) << 32) | uint32_t(s);
+            }
+        }
+        const unsigned long long slot = wave_append(hit, &s_n);
+        if (hit) s_keys[slot] = key;
+        __syncthreads();
+        const uint32_t cnt = uint32_t(s_n);
+        __syncthreads();
+        if (cnt > uint32_t(kKeyStage - kT))
+            flush_stage<kT>(s_keys, &s_n, &s_base, cnt, keys, n_out);
     }
+    const uint32_t cnt = uint32_t(s_n);
+    __syncthreads();
+    flush_stage<kT>(s_keys, &s_n, &s_base, cnt, keys, n_out);
 }
 
+// Distinct (group, suppkey) keys counted per group: each key is inserted
+// into an open-addressing set (slots hold key + 1, zero is empty; at most
+// half full, so every probe ends) and the insert that places it counts it.
 __global__ __launch_bounds__(kT) void q16_distinct_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                     unsigned long long* __restrict__ set,
+                                                     uint64_t mask,
                                                      unsigned long long* __restrict__ counts) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kT)
-        if (i == 0 || keys[i] != keys[i - 1]) atomicAdd(counts + (keys[i] >> 32), 1ull);
+         i += int64_t(gridDim.x) * kT) {
+        const uint64_t key = keys[i];
+        const unsigned long long tag = key + 1;
+        uint64_t slot = mix64(key) & mask;
+        for (;;) {
+            const unsigned long long prev = atomicCAS(set + slot, 0ull, tag);
+            if (prev == 0ull) {
+                atomicAdd(counts + (key >> 32), 1ull);
+                break;
+            }
+            if (prev == tag) break;
+            slot = (slot + 1) & mask;
+        }
+    }
 }
 
 }  // namespace
@@ -1703,6 +1905,14 @@ extern "C" int bolap_q13(bolap_ctx* ctx, const bolap_q13_args* a, int64_t* dist,
     };
     host_masks(w.w1, w.n1, w.req_alpha, w.req_bigram);
     host_masks(w.w2, w.n2, w.req_alpha, w.req_bigram);
+    auto prefix = [](const uint8_t* s, int32_t n, uint32_t& pre, int32_t& m, int32_t& sh) {
+        m = n < 4 ? n : 4;
+        pre = 0;
+        for (int32_t i = 0; i < m; ++i) pre |= uint32_t(s[i]) << (8 * i);
+        sh = m == 0 ? 0 : 8 * (4 - m);  // an empty word is never looked for
+    };
+    prefix(w.w1, w.n1, w.pre1, w.m1, w.sh1);
+    prefix(w.w2, w.n2, w.pre2, w.m2, w.sh2);
     *max_count = 0;
     for (int32_t i = 0; i < max_dist; ++i) dist[i] = 0;
     int32_t* slot = static_cast<int32_t*>(ctx->result_dev);
@@ -1720,11 +1930,11 @@ extern "C" int bolap_q13(bolap_ctx* ctx, const bolap_q13_args* a, int64_t* dist,
     {
         TraceScope ts(ctx, "q13_orders");
         if (a->order_rows > 0 && size > 0) {
-            hipLaunchKernelGGL(q13_k, dim3(grid_of(ctx, a->order_rows, 8)), dim3(kT), 0,
+            hipLaunchKernelGGL(q13_k, dim3(grid_of(ctx, (a->order_rows + 7) / 8, 8)), dim3(kT), 0,
                                ctx->stream, w, a->order_rows, a->o_custkey, a->chars, a->offsets,
                                a->alpha, a->bigram, size, cnt);
-            hipLaunchKernelGGL(max_i32_arr_k, dim3(grid_of(ctx, size)), dim3(kT), 0, ctx->stream,
-                               cnt, size, mx);
+            hipLaunchKernelGGL(max_i32_arr_k, dim3(grid_of(ctx, size, 1)), dim3(kT), 0,
+                               ctx->stream, cnt, size, mx);
             BOLAP_TRY(launch_check("q13_k"));
         }
     }
@@ -1777,8 +1987,6 @@ extern "C" int bolap_q16(bolap_ctx* ctx, const bolap_q16_args* a, int64_t* count
     const size_t o_zero_end = ws.off;
     const size_t o_tb = ws.plan(256);
     const size_t o_keys = ws.plan(size_t(a->ps_rows) * 8);
-    const size_t o_vals = ws.plan(size_t(a->ps_rows) * 4);
-    const size_t o_sort = ws.plan(radix_sort_scratch_bytes(ctx, a->ps_rows));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     BOLAP_TRY(zero_dev(ctx, ws.base, o_zero_end));
@@ -1807,13 +2015,18 @@ extern "C" int bolap_q16(bolap_ctx* ctx, const bolap_q16_args* a, int64_t* count
     auto* cnt = ws.at<unsigned long long>(o_cnt);
     if (n > 0) {
         TraceScope ts(ctx, "q16_distinct");
-        int bits = 33;
-        while ((int64_t(1) << (bits - 32)) < n_counts) ++bits;
-        int passes = 0;
-        BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals),
-                                   int64_t(n), bits, ws.at<void>(o_sort), &passes));
+        int cap_log2 = 10;
+        while ((uint64_t(1) << cap_log2) < 2 * n) ++cap_log2;
+        WsArena ws2 = ws;
+        const size_t o_set = ws2.plan(size_t(8) << cap_log2);
+        BOLAP_TRY(ws_reserve(ctx, ws2.off, ws.off));
+        ws2.base = static_cast<char*>(ctx->ws);
+        cnt = ws2.at<unsigned long long>(o_cnt);
+        auto* set = ws2.at<unsigned long long>(o_set);
+        BOLAP_TRY(zero_dev(ctx, set, size_t(8) << cap_log2));
         hipLaunchKernelGGL(q16_distinct_k, dim3(grid_of(ctx, int64_t(n))), dim3(kT), 0,
-                           ctx->stream, ws.at<uint64_t>(o_keys), int64_t(n), cnt);
+                           ctx->stream, ws2.at<uint64_t>(o_keys), int64_t(n), set,
+                           (uint64_t(1) << cap_log2) - 1, cnt);
         BOLAP_TRY(launch_check("q16_distinct_k"));
     }
     return fetch(ctx, counts, cnt, size_t(n_counts) * 8);

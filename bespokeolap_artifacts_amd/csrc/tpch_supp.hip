@@ -182,23 +182,36 @@ struct Q11K {
     int64_t n_mark;
 };
 
-__global__ __launch_bounds__(kT) void q11_runs_k(Q11K a, int32_t* __restrict__ g_pk,
+// Each workgroup owns the contiguous row range [b * chunk, (b + 1) * chunk)
+// and writes the groups whose first row lies there into its own region of
+// the group arrays (capacity `chunk`: a range holds at most that many run
+// heads), so the appends meet in LDS and never on one global counter.  The
+// lane holding a run's first row sums the run (:163-186), reading past the
+// range end when the run continues there.
+__global__ __launch_bounds__(kT) void q11_runs_k(Q11K a, int64_t chunk, int32_t* __restrict__ g_pk,
                                                  int64_t* __restrict__ g_sum,
+                                                 uint32_t* __restrict__ blk_n,
                                                  unsigned long long* __restrict__ n_groups,
                                                  unsigned long long* __restrict__ total) {
-    const int64_t stride_g = int64_t(gridDim.x) * kT;
-    const int64_t rounds = (a.rows + stride_g - 1) / stride_g;
+    __shared__ unsigned long long s_n;
+    __shared__ uint64_t s_tot[kT / kWave];
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t hi = lo + chunk < a.rows ? lo + chunk : a.rows;
+    int32_t* r_pk = g_pk + lo;
+    int64_t* r_sum = g_sum + lo;
     int64_t my_total = 0;
-    for (int64_t r = 0; r < rounds; ++r) {
-        const int64_t i = r * stride_g + int64_t(blockIdx.x) * kT + threadIdx.x;
+    for (int64_t t = lo; t < hi; t += kT) {
+        const int64_t i = t + threadIdx.x;
         bool head = false;
         int32_t key = 0;
-        if (i < a.rows) {
+        if (i < hi) {
             key = a.pk[i];
             head = i == 0 || a.pk[i - 1] != key;
         }
         int64_t sum = 0;
-        if (head) {  // the lane holding a run's first row sums the run (:163-186)
+        if (head) {
             for (int64_t j = i; j < a.rows && a.pk[j] == key; ++j) {
                 const int32_t s = a.sk[j];
                 if (s >= 0 && s < a.n_mark && a.mark[s]) sum += a.value[j];
@@ -206,37 +219,134 @@ __global__ __launch_bounds__(kT) void q11_runs_k(Q11K a, int32_t* __restrict__ g
         }
         my_total += sum;
         const bool emit = head && sum != 0;
-        const unsigned long long slot = wave_append(emit, n_groups);
+        const unsigned long long slot = wave_append(emit, &s_n);
         if (emit) {
-            g_pk[slot] = key;
-            g_sum[slot] = sum;
+            r_pk[slot] = key;
+            r_sum[slot] = sum;
         }
     }
-    const uint64_t t = wave_sum_u64(uint64_t(my_total));
-    if ((threadIdx.x & (kWave - 1)) == 0 && t) atomicAdd(total, (unsigned long long)t);
+    const uint64_t wt = wave_sum_u64(uint64_t(my_total));
+    if ((threadIdx.x & (kWave - 1)) == 0) s_tot[threadIdx.x / kWave] = wt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t bt = 0;
+        for (int w = 0; w < kT / kWave; ++w) bt += s_tot[w];
+        blk_n[blockIdx.x] = uint32_t(s_n);
+        if (s_n) atomicAdd(n_groups, s_n);
+        if (bt) atomicAdd(total, (unsigned long long)bt);
+    }
 }
 
-// Groups with (double)sum > (double)total * fraction (:194-203).
+// Order-preserving unsigned images of the signed values and keys.
+BOLAP_DEV uint64_t ob64(int64_t v) { return uint64_t(v) ^ (uint64_t(1) << 63); }
+BOLAP_DEV uint32_t ob32(int32_t v) { return uint32_t(v) ^ 0x80000000u; }
+
+BOLAP_DEV uint64_t wave_max_u64(uint64_t x) {
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const uint64_t o = __shfl_xor(x, d, kWave);
+        x = o > x ? o : x;
+    }
+    return x;
+}
+
+// Groups with (double)sum > (double)total * fraction (:194-203).  Workgroup b
+// filters region b: it counts its kept groups, takes their output range with
+// one atomic, then writes them there.  The kept groups' value and key ranges
+// go to range[4b..4b+3] (max of ~value, max of value, max of ~key, max of
+// key, over the order-preserving unsigned images; 0 for none).
 __global__ __launch_bounds__(kT) void q11_filter_k(const int32_t* __restrict__ g_pk,
                                                    const int64_t* __restrict__ g_sum,
-                                                   const unsigned long long* __restrict__ n_groups,
+                                                   const uint32_t* __restrict__ blk_n,
+                                                   int64_t chunk,
                                                    const unsigned long long* __restrict__ total,
                                                    double fraction, int32_t* __restrict__ o_pk,
                                                    int64_t* __restrict__ o_sum,
-                                                   unsigned long long* __restrict__ n_out) {
-    const int64_t n = int64_t(*n_groups);
+                                                   unsigned long long* __restrict__ n_out,
+                                                   uint64_t* __restrict__ range) {
+    __shared__ uint32_t s_cnt[kT / kWave];
+    __shared__ uint64_t s_r[4][kT / kWave];
+    __shared__ unsigned long long s_base, s_pos;
+    const int64_t n = blk_n[blockIdx.x];
+    const int32_t* r_pk = g_pk + int64_t(blockIdx.x) * chunk;
+    const int64_t* r_sum = g_sum + int64_t(blockIdx.x) * chunk;
     const double threshold = double(int64_t(*total)) * fraction;
-    const int64_t stride_g = int64_t(gridDim.x) * kT;
-    const int64_t rounds = (n + stride_g - 1) / stride_g;
-    for (int64_t r = 0; r < rounds; ++r) {
-        const int64_t i = r * stride_g + int64_t(blockIdx.x) * kT + threadIdx.x;
-        const bool keep = i < n && double(g_sum[i]) > threshold;
-        const unsigned long long slot = wave_append(keep, n_out);
+    uint32_t kept = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kT) kept += double(r_sum[i]) > threshold ? 1u : 0u;
+    const uint32_t wk = uint32_t(wave_sum_u64(kept));
+    if ((threadIdx.x & (kWave - 1)) == 0) s_cnt[threadIdx.x / kWave] = wk;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long bk = 0;
+        for (int w = 0; w < kT / kWave; ++w) bk += s_cnt[w];
+        s_base = bk ? atomicAdd(n_out, bk) : 0;
+        s_pos = 0;
+    }
+    __syncthreads();
+    uint64_t r[4] = {0, 0, 0, 0};
+    for (int64_t t = 0; t < n; t += kT) {
+        const int64_t i = t + threadIdx.x;
+        const bool keep = i < n && double(r_sum[i]) > threshold;
+        const unsigned long long slot = wave_append(keep, &s_pos);
         if (keep) {
-            o_pk[slot] = g_pk[i];
-            o_sum[slot] = g_sum[i];
+            const int32_t pk = r_pk[i];
+            const int64_t v = r_sum[i];
+            o_pk[s_base + slot] = pk;
+            o_sum[s_base + slot] = v;
+            const uint64_t vi = ob64(v), ki = ob32(pk);
+            r[0] = ~vi > r[0] ? ~vi : r[0];
+            r[1] = vi > r[1] ? vi : r[1];
+            r[2] = (~ki & 0xffffffffu) > r[2] ? (~ki & 0xffffffffu) : r[2];
+            r[3] = ki > r[3] ? ki : r[3];
         }
     }
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t m = wave_max_u64(r[j]);
+        if ((threadIdx.x & (kWave - 1)) == 0) s_r[j][threadIdx.x / kWave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        uint64_t m = 0;
+        for (int w = 0; w < kT / kWave; ++w) m = s_r[threadIdx.x][w] > m ? s_r[threadIdx.x][w] : m;
+        range[int64_t(blockIdx.x) * 4 + threadIdx.x] = m;
+    }
+}
+
+// Folds the workgroups' range words into out[0..3] (one workgroup).
+__global__ __launch_bounds__(kT) void q11_range_k(const uint64_t* __restrict__ range,
+                                                  int64_t blocks, uint64_t* __restrict__ out) {
+    __shared__ uint64_t s_r[4][kT / kWave];
+    uint64_t r[4] = {0, 0, 0, 0};
+    for (int64_t b = threadIdx.x; b < blocks; b += kT)
+        for (int j = 0; j < 4; ++j) r[j] = range[b * 4 + j] > r[j] ? range[b * 4 + j] : r[j];
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t m = wave_max_u64(r[j]);
+        if ((threadIdx.x & (kWave - 1)) == 0) s_r[j][threadIdx.x / kWave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        uint64_t m = 0;
+        for (int w = 0; w < kT / kWave; ++w) m = s_r[threadIdx.x][w] > m ? s_r[threadIdx.x][w] : m;
+        out[threadIdx.x] = m;
+    }
+}
+
+// Sort key of a kept group: (vmax - value) above (key - kmin), so ascending
+// keys give value descending, then key ascending.
+__global__ __launch_bounds__(kT) void q11_keys_k(const int32_t* __restrict__ o_pk,
+                                                 const int64_t* __restrict__ o_sum, int64_t n,
+                                                 uint64_t vmax, uint32_t kmin, int kbits,
+                                                 uint64_t* __restrict__ keys) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        keys[i] = ((vmax - ob64(o_sum[i])) << kbits) | uint64_t(ob32(o_pk[i]) - kmin);
+}
+
+__global__ __launch_bounds__(kT) void q11_values_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                   uint64_t vmax, int kbits,
+                                                   int64_t* __restrict__ o_sum) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        o_sum[i] = int64_t((vmax - (keys[i] >> kbits)) ^ (uint64_t(1) << 63));
 }
 
 // ---------------------------------------------------------------------------
@@ -569,13 +679,21 @@ extern "C" int bolap_q11(bolap_ctx* ctx, const bolap_q11_args* a, int32_t* partk
     if (a->ps_rows == 0) return BOLAP_OK;
     const int64_t nk = a->n_supp_keys > 0 ? a->n_supp_keys : 1;
     WsArena ws;
-    const size_t o_cnt = ws.plan(32);  // groups, total, kept
+    const size_t o_cnt = ws.plan(64);  // groups, total, kept, four range words
     const size_t o_mark = ws.plan(size_t(nk));
     const size_t o_zero_end = ws.off;
+    // contiguous row ranges of whole tiles, one per workgroup
+    const int64_t g0 = grid_of(ctx, a->ps_rows);
+    const int64_t chunk = ((a->ps_rows + g0 - 1) / g0 + kT - 1) / kT * kT;
+    const unsigned blocks = unsigned((a->ps_rows + chunk - 1) / chunk);
+    const size_t o_blk = ws.plan(size_t(blocks) * 4);
+    const size_t o_rng = ws.plan(size_t(blocks) * 32);
     const size_t o_gpk = ws.plan(size_t(a->ps_rows) * 4);
     const size_t o_gsum = ws.plan(size_t(a->ps_rows) * 8);
     const size_t o_opk = ws.plan(size_t(a->ps_rows) * 4);
     const size_t o_osum = ws.plan(size_t(a->ps_rows) * 8);
+    const size_t o_keys = ws.plan(size_t(a->ps_rows) * 8);
+    const size_t o_sort = ws.plan(radix_sort_scratch_bytes(ctx, a->ps_rows));
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     BOLAP_TRY(zero_dev(ctx, ws.base, o_zero_end));
@@ -588,14 +706,19 @@ extern "C" int bolap_q11(bolap_ctx* ctx, const bolap_q11_args* a, int32_t* partk
                                nk, ws.at<uint8_t>(o_mark));
         const Q11K k{a->ps_rows, a->ps_partkey, a->ps_suppkey, a->ps_value, ws.at<uint8_t>(o_mark),
                      nk};
-        hipLaunchKernelGGL(q11_runs_k, dim3(grid_of(ctx, a->ps_rows)), dim3(kT), 0, ctx->stream,
-                           k, ws.at<int32_t>(o_gpk), ws.at<int64_t>(o_gsum), cnt, cnt + 1);
-        hipLaunchKernelGGL(q11_filter_k, dim3(grid_of(ctx, a->ps_rows / 4 + 1)), dim3(kT), 0,
-                           ctx->stream, ws.at<int32_t>(o_gpk), ws.at<int64_t>(o_gsum), cnt, cnt + 1,
-                           a->fraction, ws.at<int32_t>(o_opk), ws.at<int64_t>(o_osum), cnt + 2);
-        BOLAP_TRY(launch_check("q11_filter_k"));
+        hipLaunchKernelGGL(q11_runs_k, dim3(blocks), dim3(kT), 0, ctx->stream, k, chunk,
+                           ws.at<int32_t>(o_gpk), ws.at<int64_t>(o_gsum), ws.at<uint32_t>(o_blk),
+                           cnt, cnt + 1);
+        hipLaunchKernelGGL(q11_filter_k, dim3(blocks), dim3(kT), 0, ctx->stream,
+                           ws.at<int32_t>(o_gpk), ws.at<int64_t>(o_gsum), ws.at<uint32_t>(o_blk),
+                           chunk, cnt + 1, a->fraction, ws.at<int32_t>(o_opk),
+                           ws.at<int64_t>(o_osum), cnt + 2, ws.at<uint64_t>(o_rng));
+        hipLaunchKernelGGL(q11_range_k, dim3(1), dim3(kT), 0, ctx->stream,
+                           ws.at<uint64_t>(o_rng), int64_t(blocks),
+                           reinterpret_cast<uint64_t*>(cnt + 3));
+        BOLAP_TRY(launch_check("q11_range_k"));
     }
-    unsigned long long c[3];
+    unsigned long long c[7];
     BOLAP_TRY(fetch(ctx, c, cnt, sizeof(c)));
     *total = int64_t(c[1]);
     *rows = int64_t(c[2]);
@@ -607,12 +730,37 @@ extern "C" int bolap_q11(bolap_ctx* ctx, const bolap_q11_args* a, int32_t* partk
     }
     const int64_t n = int64_t(c[2]);
     if (n == 0) return BOLAP_OK;
+    // the kept groups in the reference's order: value descending, then key.
+    // On the device when (vmax - value, key - kmin) packs into 64 bits.
+    const uint64_t vmin = ~c[3], vmax = c[4];
+    const uint32_t kmin = uint32_t(~c[5]), kmax = uint32_t(c[6]);
+    auto bit_len = [](uint64_t x) {
+        int b = 0;
+        while (b < 64 && (x >> b) != 0) ++b;
+        return b;
+    };
+    const int kbits = bit_len(uint64_t(kmax - kmin)), vbits = bit_len(vmax - vmin);
+    if (kbits + vbits <= 64) {
+        TraceScope ts(ctx, "q11_order");
+        auto* keys = ws.at<uint64_t>(o_keys);
+        hipLaunchKernelGGL(q11_keys_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream,
+                           ws.at<int32_t>(o_opk), ws.at<int64_t>(o_osum), n, vmax, kmin, kbits,
+                           keys);
+        int passes = 0;
+        BOLAP_TRY(radix_sort_pairs(ctx, keys, ws.at<uint32_t>(o_opk), n, kbits + vbits,
+                                   ws.at<void>(o_sort), &passes));
+        hipLaunchKernelGGL(q11_values_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream, keys, n,
+                           vmax, kbits, ws.at<int64_t>(o_osum));
+        BOLAP_TRY(launch_check("q11_values_k"));
+        BOLAP_HIP_TRY(hipMemcpyAsync(partkey_out, ws.at<int32_t>(o_opk), size_t(n) * 4,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        return fetch(ctx, value_out, ws.at<int64_t>(o_osum), size_t(n) * 8);
+    }
     std::vector<int32_t> pk(static_cast<size_t>(n));
     std::vector<int64_t> v(static_cast<size_t>(n));
     BOLAP_HIP_TRY(hipMemcpyAsync(pk.data(), ws.at<int32_t>(o_opk), size_t(n) * 4,
                                  hipMemcpyDeviceToHost, ctx->stream));
     BOLAP_TRY(fetch(ctx, v.data(), ws.at<int64_t>(o_osum), size_t(n) * 8));
-    // the kept groups in the reference's order: value descending, then key
     std::vector<int64_t> idx(static_cast<size_t>(n));
     for (int64_t i = 0; i < n; ++i) idx[size_t(i)] = i;
     std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {

@@ -736,7 +736,7 @@ class Engine:
         out = np.zeros(n, np.int64)
         N.check(self.lib.bolap_q16(self.ctx, C.byref(a), out.ctypes.data_as(C.POINTER(C.c_int64)),
                                    n))
-        return out.tolist()
+        return out
 
     # ----------------------------------------------- Q2, Q11, Q15, Q20 (tpch_supp)
     def q2(self, t: dict, type_ok, size: int, nation_ok, max_rows: int | None = None):
@@ -756,7 +756,7 @@ class Engine:
         a.s_nationkey = ptr(t["s_nationkey"])
         a.nation_ok, a.n_nation = no.ctypes.data if len(no) else None, len(no)
         cap = max_rows if max_rows is not None else max(1, a.ps_rows)
-        out = np.zeros((max(cap, 1), 2), np.int32)
+        out = np.empty((max(cap, 1), 2), np.int32)
         n = C.c_int64(0)
         N.check(self.lib.bolap_q2(self.ctx, C.byref(a), out.ctypes.data, cap, C.byref(n)))
         return out[:n.value]
@@ -774,7 +774,7 @@ class Engine:
                                                   ptr(t["ps_value"]))
         a.fraction = fraction
         cap = max_rows if max_rows is not None else max(1, a.ps_rows)
-        pk, val = np.zeros(max(cap, 1), np.int32), np.zeros(max(cap, 1), np.int64)
+        pk, val = np.empty(max(cap, 1), np.int32), np.empty(max(cap, 1), np.int64)
         n, tot = C.c_int64(0), C.c_int64(0)
         N.check(self.lib.bolap_q11(self.ctx, C.byref(a), pk.ctypes.data, val.ctypes.data, cap,
                                    C.byref(n), C.byref(tot)))

@@ -227,6 +227,33 @@ def test_q9(engine, db, color):
     assert gy == y0 and np.array_equal(gtot, tot) and np.array_equal(gused, used)
 
 
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_q9_partsupp_duplicates_and_order(engine, db, order):
+    """partsupp with duplicate (part, supplier) pairs of different cost (the
+    first row in row order wins), keys past the part range and negative keys,
+    kept sorted by partkey (contiguous key ranges) or shuffled (the CSR)."""
+    r, h, t = db
+    rng = np.random.default_rng(9)
+    n = len(h["ps_partkey"])
+    dup = rng.choice(n, n // 5, replace=False)
+    past = int(h["p_partkey"].max()) + 7
+    pk = np.concatenate([h["ps_partkey"], h["ps_partkey"][dup], [-4, past]]).astype(np.int32)
+    sk = np.concatenate([h["ps_suppkey"], h["ps_suppkey"][dup], [1, 1]]).astype(np.int32)
+    cost = np.concatenate([h["ps_supplycost"], rng.integers(1, 10**5, len(dup)),
+                           [5, 5]]).astype(np.int32)
+    perm = (np.argsort(pk, kind="stable") if order == "sorted"
+            else rng.permutation(len(pk)))
+    h2 = dict(h, ps_partkey=pk[perm], ps_suppkey=sk[perm], ps_supplycost=cost[perm])
+    t2 = dict(t)
+    for k in ("ps_partkey", "ps_suppkey", "ps_supplycost"):
+        t2[k] = torch.from_numpy(np.ascontiguousarray(h2[k])).cuda()
+    p_ok = (h["p_name_words"] == T.COLORS.index("green")).any(axis=1).astype(np.uint8)
+    y0, tot, used = oracle.q9(h2, p_ok, 25)
+    gy, gtot, gused = engine.q9(t2, torch.from_numpy(p_ok).cuda(), 25)
+    assert used.any()
+    assert gy == y0 and np.array_equal(gtot, tot) and np.array_equal(gused, used)
+
+
 @pytest.mark.parametrize("w1,w2", [("special", "requests"), ("pending", "deposits"),
                                    ("express", ""), ("x", "x"), ("", "")])
 def test_q13(engine, db, w1, w2):
@@ -235,6 +262,42 @@ def test_q13(engine, db, w1, w2):
     masks = engine.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
     assert engine.q13(t, w1, w2) == (mx, dist.tolist())
     assert engine.q13(t, w1, w2, masks=masks) == (mx, dist.tolist())
+
+
+def test_q13_comment_edges(engine):
+    """Comments built to hit the search's edges: words at the first and last
+    byte, word2 overlapping word1 or only before it, repeated prefixes
+    ("spespecial"), empty comments, and comments of 80..300 bytes (past the
+    register window, searched in memory), at every alignment."""
+    rng = np.random.default_rng(13)
+    parts = ["special", "requests", "spe", "spespecial", "requestsrequests", "x", "", " ",
+             "specia", "equests", "specialrequests", "requests special", "pending", "deposits"]
+    comments = []
+    for i in range(6000):
+        k = int(rng.integers(0, 6))
+        s = "".join(parts[int(j)] + (" " if rng.random() < 0.5 else "")
+                    for j in rng.integers(0, len(parts), k))
+        if i % 7 == 0:
+            s = "z" * int(rng.integers(60, 290)) + s
+        elif i % 11 == 0:
+            s = s + "q" * int(rng.integers(60, 290))
+        comments.append(s.encode())
+    lens = np.array([len(c) for c in comments])
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    chars = np.frombuffer(b"".join(comments) + b"\0", np.uint8)[:-1].copy()
+    ck = rng.integers(1, 400, len(comments)).astype(np.int32)
+    h = {"o_custkey_raw": ck, "o_comment_chars": chars, "o_comment_offsets": off,
+         "c_custkey": np.arange(1, 401, dtype=np.int32)}
+    assert (lens > 100).sum() > 500 and (lens == 0).sum() > 10
+    t = {k: torch.from_numpy(v.view(np.int32) if v.dtype == np.uint32 else v).cuda()
+         for k, v in h.items()}
+    masks = engine.string_masks(t["o_comment_chars"], t["o_comment_offsets"])
+    for w1, w2 in [("special", "requests"), ("spe", "special"), ("requests", "special"),
+                   ("x", ""), ("", "x"), ("", ""), ("special", ""), ("s", "s"),
+                   ("specialrequests", "requests"), ("z" * 60, "special")]:
+        want = oracle.q13(h, w1, w2)
+        assert engine.q13(t, w1, w2) == (want[0], want[1].tolist()), (w1, w2)
+        assert engine.q13(t, w1, w2, masks=masks) == (want[0], want[1].tolist()), (w1, w2)
 
 
 def test_string_masks_match_the_builder(engine, db):
@@ -267,7 +330,7 @@ def test_q16(engine, db, brand, tprefix, sizes):
     mask = sum(1 << s_ for s_ in sizes)
     npk, nsk = int(h["p_partkey"].max()) + 1, int(h["s_suppkey"].max()) + 1
     want = oracle.q16(h, bb, tb, mask, npk, nsk)
-    assert engine.q16(t, len(h["brand_dict"]), bb, tb, mask, npk, nsk) == want.tolist()
+    assert np.array_equal(engine.q16(t, len(h["brand_dict"]), bb, tb, mask, npk, nsk), want)
 
 
 @pytest.fixture(scope="module")
@@ -329,8 +392,8 @@ def test_sf10_all_five(engine, sf10):
     npk, nsk = int(h["p_partkey"].max()) + 1, int(h["s_suppkey"].max()) + 1
     want16 = oracle.q16(h, h["brand_dict"].index("Brand#45"), tb, mask, npk, nsk)
     assert want16.sum() > 0
-    assert engine.q16(t, len(h["brand_dict"]), h["brand_dict"].index("Brand#45"), tb, mask, npk,
-                      nsk) == want16.tolist()
+    assert np.array_equal(engine.q16(t, len(h["brand_dict"]), h["brand_dict"].index("Brand#45"),
+                                     tb, mask, npk, nsk), want16)
     print("q21", flush=True)
     n = int(h["s_suppkey"].max()) + 1
     w21 = oracle.q21(h, h["status_dict"].index("F"), 7, n)
