@@ -130,6 +130,30 @@ int launch_check(const char* what);
 // (join_sum.hip); called by bolap_ctx_create with `device` current.
 int noalloc_setup(int device);
 
+// Grouped MIN/MAX in steps (group_minmax.hip), for pipelines that build the
+// join rows in the workspace themselves: the expanded row count (one
+// synchronisation), the workspace end of a call whose own arrays start at
+// `base`, and the call (rows in (group, key) order, or with first_seen in
+// the order of the first join row and value that reach each group).
+int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t* expanded);
+size_t minmax_ws_end(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded,
+                     bool first_seen, size_t base);
+int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, bool first_seen,
+               size_t base, bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
+               bolap_minmax_stats* stats);
+
+// An ILIKE pattern as the reference classifies it (query_q10a.cpp:135-173):
+// lower-cased, and for patterns without '_' whose '%' signs sit only at the
+// ends, the literal token of the whole-string / prefix / suffix / substring
+// test (like.hip).
+enum class LikeKind { kAll, kExact, kPrefix, kSuffix, kContains, kGeneral };
+struct LikePattern {
+    LikeKind kind = LikeKind::kGeneral;
+    std::string p;      // lower-cased pattern
+    std::string token;  // literal part of the whole-string / prefix / suffix / substring forms
+};
+LikePattern like_classify(const char* raw);
+
 // Records a pair of events around a stage when tracing is on.
 struct TraceScope {
     bolap_ctx* ctx;

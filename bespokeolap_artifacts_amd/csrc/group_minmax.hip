@@ -128,6 +128,13 @@ __global__ __launch_bounds__(kT) void mm_init_dense_k(int2* __restrict__ mm,
     }
 }
 
+__global__ __launch_bounds__(kT) void fill_u64_k(unsigned long long* __restrict__ p, int64_t n,
+                                                 unsigned long long v) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        p[i] = v;
+}
+
 __global__ __launch_bounds__(kT) void mm_init_hash_k(HSlot* __restrict__ t, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kT)
@@ -145,7 +152,8 @@ BOLAP_DEV void fold_seen(int32_t* mn, int32_t* mx, int2 cur, int32_t v) {
 
 // The item ranges of join rows p0 .. p0 + 63 expanded across one wave: lane
 // L takes expanded rows L, L + 64, ... of the concatenated ranges and calls
-// body(group, value, has, i) for value index i, so the lanes stay busy
+// body(group, value, has, i, pos) for value index i, the j-th value of join
+// row p (pos = p << 32 | j: the reference's loop order), so the lanes stay busy
 // whatever the range lengths (a lane per join row idles on short ranges
 // while one long range runs).  p0 is uniform across the wave; every lane
 // reaches each shuffle.  Malformed join rows are skipped here (mm_count_k
@@ -181,7 +189,8 @@ BOLAP_DEV void expand_wave(const Args& a, int64_t p0, F&& body) {
         const int32_t o_g = __shfl(g, o, kWave);
         const int32_t o_v = __shfl(v, o, kWave);
         const int o_has = __shfl(hv, o, kWave);
-        if (e < total) body(o_g, o_v, o_has != 0, o_lo + (e - o_exc));
+        const uint64_t pos = (uint64_t(p0 + o) << 32) | uint64_t(e - o_exc);
+        if (e < total) body(o_g, o_v, o_has != 0, o_lo + (e - o_exc), pos);
     }
 }
 
@@ -189,18 +198,24 @@ BOLAP_DEV void expand_wave(const Args& a, int64_t p0, F&& body) {
 BOLAP_DEV int64_t wave_first() { return (int64_t(blockIdx.x) * kT + threadIdx.x) & ~int64_t(kWave - 1); }
 BOLAP_DEV int64_t wave_stride() { return int64_t(gridDim.x) * kT; }
 
+// With FS, first[s] also takes the smallest loop position that reaches slot
+// s (the reference appends a dense slot to its output when first used,
+// query_q10a.cpp:866-872), read first so that most rows skip the atomic.
+template <bool FS>
 __global__ __launch_bounds__(kT) void mm_dense_k(Args a, int2* __restrict__ mm,
                                                  uint8_t* __restrict__ used,
+                                                 unsigned long long* __restrict__ first,
                                                  int64_t* __restrict__ st) {
     uint64_t errs = 0;
     for (int64_t p0 = wave_first(); p0 < a.pairs; p0 += wave_stride()) {
-        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i) {
+        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i, uint64_t pos) {
             const int32_t k = a.rv[i];
             if (k < 0 || k >= a.keys) {
                 ++errs;
                 return;
             }
             const int64_t s = int64_t(g) * a.keys + k;
+            if (FS && pos < first[s]) atomicMin(first + s, (unsigned long long)pos);
             // a slot with a value is marked by min <= max; only value-less
             // rows mark `used`
             if (!has)
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(kT) void mm_hash_k(Args a, HSlot* __restrict__ t, u
                                                 int64_t* __restrict__ st) {
     uint64_t errs = 0;
     for (int64_t p0 = wave_first(); p0 < a.pairs; p0 += wave_stride()) {
-        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i) {
+        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i, uint64_t) {
             const int32_t k = a.rv[i];
             if (k < 0 || k >= a.keys) {
                 ++errs;
@@ -360,6 +375,27 @@ __global__ __launch_bounds__(kT) void mm_hash_emit_k(const uint64_t* __restrict_
     }
 }
 
+// Sort keys of the dense rows: the first loop position of each row's slot.
+__global__ __launch_bounds__(kT) void mm_first_keys_k(const bolap_minmax_row* __restrict__ rows,
+                                                      int64_t n, int32_t keys,
+                                                      const unsigned long long* __restrict__ first,
+                                                      uint64_t* __restrict__ k,
+                                                      uint32_t* __restrict__ v) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT) {
+        k[i] = first[int64_t(rows[i].group) * keys + rows[i].key];
+        v[i] = uint32_t(i);
+    }
+}
+
+__global__ __launch_bounds__(kT) void mm_permute_k(const bolap_minmax_row* __restrict__ src,
+                                                   const uint32_t* __restrict__ idx, int64_t n,
+                                                   bolap_minmax_row* __restrict__ dst) {
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kT)
+        dst[i] = src[idx[i]];
+}
+
 int read_st(bolap_ctx* ctx, int64_t* dst) {
     BOLAP_TRY(pinned_reserve(ctx, mN * 8));
     BOLAP_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->result_dev, mN * 8, hipMemcpyDeviceToHost,
@@ -369,15 +405,60 @@ int read_st(bolap_ctx* ctx, int64_t* dst) {
     return BOLAP_OK;
 }
 
-}  // namespace
-}  // namespace bolap
+// Workspace plan of one call (offsets from `base`), shared by the size query
+// and the call so that they agree.
+struct Plan {
+    bool dense = false;
+    int64_t slots = 0, words = 0, bound = 0, cap = 0, nb = 0, chunk = 0, rows_cap = 0;
+    size_t o_mm = 0, o_used = 0, o_bits = 0, o_cnt = 0, o_scr = 0, o_first = 0, o_tmp = 0,
+           o_k = 0, o_v = 0, o_sort = 0;
+    size_t o_t = 0, o_ck = 0, o_cs = 0, o_occ = 0, o_oscr = 0;
+    size_t end = 0;
+};
 
-using namespace bolap;
+Plan make_plan(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, bool first_seen,
+               size_t base) {
+    Plan p;
+    WsArena ws;
+    ws.off = WsArena::align(base);
+    p.slots = int64_t(a->groups) * a->keys;
+    const int64_t dense_max = a->dense_max_slots > 0 ? a->dense_max_slots : kDenseDefault;
+    p.dense = p.slots <= dense_max;
+    if (p.dense) {
+        p.words = (p.slots + 31) / 32;
+        p.o_mm = ws.plan(size_t(p.slots) * 8);
+        p.o_used = ws.plan(size_t(p.slots));
+        p.o_bits = ws.plan(size_t(p.words) * 4);
+        p.o_cnt = ws.plan(size_t(p.words) * 8);
+        p.o_scr = ws.plan(scan_scratch_bytes(p.words));
+        if (first_seen) {
+            // rows: at most one per slot and per expanded row
+            p.rows_cap = std::max<int64_t>(1, std::min(p.slots, expanded));
+            p.o_first = ws.plan(size_t(p.slots) * 8);
+            p.o_tmp = ws.plan(size_t(p.rows_cap) * sizeof(bolap_minmax_row));
+            p.o_k = ws.plan(size_t(p.rows_cap) * 8);
+            p.o_v = ws.plan(size_t(p.rows_cap) * 4);
+            p.o_sort = ws.plan(radix_sort_scratch_bytes(ctx, p.rows_cap));
+        }
+    } else {
+        // exact bound on the distinct keys: expanded rows (and slots)
+        p.bound = std::max<int64_t>(1, std::min(expanded, p.slots));
+        const int lg = std::max(10, log2_ceil(p.bound * 2));
+        p.cap = int64_t(1) << std::min(lg, 40);
+        p.o_t = ws.plan(size_t(p.cap) * sizeof(HSlot));
+        p.o_ck = ws.plan(size_t(p.bound) * 8);
+        p.o_cs = ws.plan(size_t(p.bound) * 4);
+        p.nb = int64_t(grid_of(ctx, p.cap));
+        p.chunk = (p.cap + p.nb - 1) / p.nb;
+        p.o_occ = ws.plan(size_t(p.nb) * 8);
+        p.o_oscr = ws.plan(scan_scratch_bytes(p.nb));
+        p.o_sort = ws.plan(radix_sort_scratch_bytes(ctx, p.bound));
+    }
+    p.end = ws.off;
+    return p;
+}
 
-extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
-                                  bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
-                                  bolap_minmax_stats* stats) {
-    BOLAP_ARG_CHECK(ctx && a, "bolap_group_minmax: null argument");
+int check_args(const bolap_minmax_args* a, int64_t max_rows) {
     BOLAP_ARG_CHECK(a->pairs >= 0 && a->items >= 0 && a->values >= 0,
                     "bolap_group_minmax: negative size");
     BOLAP_ARG_CHECK(a->groups >= 1 && a->keys >= 1, "bolap_group_minmax: groups %d, keys %d",
@@ -388,7 +469,10 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
                     "bolap_group_minmax: null item column");
     BOLAP_ARG_CHECK(a->values == 0 || a->range_value, "bolap_group_minmax: null range values");
     BOLAP_ARG_CHECK(max_rows >= 0, "bolap_group_minmax: max_rows < 0");
-    TraceScope ts_all(ctx, "minmax");
+    return BOLAP_OK;
+}
+
+Args args_of(const bolap_minmax_args* a) {
     Args k;
     k.pairs = a->pairs;
     k.items = a->items;
@@ -402,6 +486,14 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
     k.has = a->item_has;
     k.groups = a->groups;
     k.keys = a->keys;
+    return k;
+}
+
+}  // namespace
+
+int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t* expanded) {
+    BOLAP_TRY(check_args(a, 0));
+    const Args k = args_of(a);
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
     int64_t h[mN];
     hipLaunchKernelGGL(mm_zero_k, dim3(1), dim3(kT), 0, ctx->stream, st);
@@ -415,29 +507,53 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
     BOLAP_ARG_CHECK(h[mErrors] == 0,
                     "bolap_group_minmax: %lld join rows with a group, item or range out of bounds",
                     (long long)h[mErrors]);
-    const int64_t expanded = h[mExpanded];
-    const int64_t slots = int64_t(a->groups) * a->keys;
-    const int64_t dense_max = a->dense_max_slots > 0 ? a->dense_max_slots : kDenseDefault;
-    const bool dense = slots <= dense_max;
-    int64_t groups_n = 0;
+    *expanded = h[mExpanded];
+    return BOLAP_OK;
+}
+
+size_t minmax_ws_end(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded,
+                     bool first_seen, size_t base) {
+    return make_plan(ctx, a, expanded, first_seen, base).end;
+}
+
+int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, bool first_seen,
+               size_t base, bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
+               bolap_minmax_stats* stats) {
+    BOLAP_TRY(check_args(a, max_rows));
+    const Args k = args_of(a);
+    int64_t* st = static_cast<int64_t*>(ctx->result_dev);
+    int64_t h[mN];
+    const Plan p = make_plan(ctx, a, expanded, first_seen, base);
+    BOLAP_ARG_CHECK(p.dense || p.cap <= (int64_t(1) << 32),
+                    "bolap_group_minmax: %lld expanded rows need a table beyond 2^32 slots",
+                    (long long)expanded);
+    // the caller's arrays below `base` stay where they are: a workspace that
+    // must grow keeps them, but only a caller that reserved p.end already may
+    // pass pointers into the workspace
+    BOLAP_TRY(ws_reserve(ctx, p.end, base));
     WsArena ws;
-    if (dense) {
-        const int64_t words = (slots + 31) / 32;
-        const size_t o_mm = ws.plan(size_t(slots) * 8), o_used = ws.plan(size_t(slots));
-        const size_t o_bits = ws.plan(size_t(words) * 4), o_cnt = ws.plan(size_t(words) * 8);
-        const size_t o_scr = ws.plan(scan_scratch_bytes(words));
-        BOLAP_TRY(ws_reserve(ctx, ws.off));
-        ws.base = static_cast<char*>(ctx->ws);
-        int2* mm = ws.at<int2>(o_mm);
-        uint8_t* used = ws.at<uint8_t>(o_used);
+    ws.base = static_cast<char*>(ctx->ws);
+    int64_t groups_n = 0;
+    if (p.dense) {
+        int2* mm = ws.at<int2>(p.o_mm);
+        uint8_t* used = ws.at<uint8_t>(p.o_used);
+        auto* first = first_seen ? ws.at<unsigned long long>(p.o_first) : nullptr;
         {
             TraceScope ts(ctx, "minmax_dense");
-            hipLaunchKernelGGL(mm_init_dense_k, dim3(grid_of(ctx, slots)), dim3(kT), 0,
-                               ctx->stream, mm, used, slots);
-            hipLaunchKernelGGL(mm_dense_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
-                               k, mm, used, st);
-            hipLaunchKernelGGL(mm_bits_k, dim3(grid_of(ctx, words)), dim3(kT), 0, ctx->stream,
-                               used, mm, slots, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), st);
+            hipLaunchKernelGGL(mm_init_dense_k, dim3(grid_of(ctx, p.slots)), dim3(kT), 0,
+                               ctx->stream, mm, used, p.slots);
+            if (first_seen) {
+                hipLaunchKernelGGL(fill_u64_k, dim3(grid_of(ctx, p.slots)), dim3(kT), 0,
+                                   ctx->stream, first, p.slots, ~0ull);
+                hipLaunchKernelGGL(mm_dense_k<true>, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0,
+                                   ctx->stream, k, mm, used, first, st);
+            } else {
+                hipLaunchKernelGGL(mm_dense_k<false>, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0,
+                                   ctx->stream, k, mm, used, first, st);
+            }
+            hipLaunchKernelGGL(mm_bits_k, dim3(grid_of(ctx, p.words)), dim3(kT), 0, ctx->stream,
+                               used, mm, p.slots, ws.at<uint32_t>(p.o_bits),
+                               ws.at<int64_t>(p.o_cnt), st);
             BOLAP_TRY(launch_check("mm_dense_k"));
         }
         BOLAP_TRY(read_st(ctx, h));
@@ -450,40 +566,43 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
                             "bolap_group_minmax: %lld groups exceed max_rows %lld",
                             (long long)groups_n, (long long)max_rows);
             TraceScope ts(ctx, "minmax_emit");
-            BOLAP_TRY(exclusive_scan_i64(ctx, ws.at<int64_t>(o_cnt), words, ws.at<void>(o_scr)));
-            hipLaunchKernelGGL(mm_dense_emit_k, dim3(grid_of(ctx, slots)), dim3(kT), 0,
-                               ctx->stream, ws.at<uint32_t>(o_bits), ws.at<int64_t>(o_cnt), slots,
-                               a->keys, mm, rows_out);
+            BOLAP_TRY(exclusive_scan_i64(ctx, ws.at<int64_t>(p.o_cnt), p.words,
+                                         ws.at<void>(p.o_scr)));
+            bolap_minmax_row* dst = first_seen ? ws.at<bolap_minmax_row>(p.o_tmp) : rows_out;
+            hipLaunchKernelGGL(mm_dense_emit_k, dim3(grid_of(ctx, p.slots)), dim3(kT), 0,
+                               ctx->stream, ws.at<uint32_t>(p.o_bits), ws.at<int64_t>(p.o_cnt),
+                               p.slots, a->keys, mm, dst);
             BOLAP_TRY(launch_check("mm_dense_emit_k"));
+            if (first_seen) {
+                // rows in the order their slots were first reached
+                uint64_t* fk = ws.at<uint64_t>(p.o_k);
+                uint32_t* fv = ws.at<uint32_t>(p.o_v);
+                hipLaunchKernelGGL(mm_first_keys_k, dim3(grid_of(ctx, groups_n)), dim3(kT), 0,
+                                   ctx->stream, dst, groups_n, a->keys, first, fk, fv);
+                int passes = 0;
+                BOLAP_TRY(radix_sort_pairs(ctx, fk, fv, groups_n,
+                                           32 + std::max(1, log2_ceil(a->pairs)),
+                                           ws.at<void>(p.o_sort), &passes));
+                hipLaunchKernelGGL(mm_permute_k, dim3(grid_of(ctx, groups_n)), dim3(kT), 0,
+                                   ctx->stream, dst, fv, groups_n, rows_out);
+                BOLAP_TRY(launch_check("mm_permute_k"));
+            }
         }
     } else {
-        // exact bound on the distinct keys: expanded rows (and slots)
-        const int64_t bound = std::max<int64_t>(1, std::min(expanded, slots));
-        const int lg = std::max(10, log2_ceil(bound * 2));
-        BOLAP_ARG_CHECK(lg <= 32, "bolap_group_minmax: %lld expanded rows need a table beyond "
-                        "2^32 slots", (long long)expanded);
-        const int64_t cap = int64_t(1) << lg;
-        const size_t o_t = ws.plan(size_t(cap) * sizeof(HSlot));
-        const size_t o_ck = ws.plan(size_t(bound) * 8), o_cs = ws.plan(size_t(bound) * 4);
-        const int64_t nb = int64_t(grid_of(ctx, cap));
-        const int64_t chunk = (cap + nb - 1) / nb;
-        const size_t o_occ = ws.plan(size_t(nb) * 8), o_oscr = ws.plan(scan_scratch_bytes(nb));
-        const size_t o_scr = ws.plan(radix_sort_scratch_bytes(ctx, bound));
-        BOLAP_TRY(ws_reserve(ctx, ws.off));
-        ws.base = static_cast<char*>(ctx->ws);
-        HSlot* t = ws.at<HSlot>(o_t);
+        HSlot* t = ws.at<HSlot>(p.o_t);
         {
             TraceScope ts(ctx, "minmax_hash");
-            hipLaunchKernelGGL(mm_init_hash_k, dim3(grid_of(ctx, cap)), dim3(kT), 0, ctx->stream,
-                               t, cap);
+            hipLaunchKernelGGL(mm_init_hash_k, dim3(grid_of(ctx, p.cap)), dim3(kT), 0,
+                               ctx->stream, t, p.cap);
             hipLaunchKernelGGL(mm_hash_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
-                               k, t, uint64_t(cap - 1), st);
-            int64_t* occ = ws.at<int64_t>(o_occ);
-            hipLaunchKernelGGL(mm_occ_count_k, dim3(unsigned(nb)), dim3(kT), 0, ctx->stream, t,
-                               cap, chunk, occ, st);
-            BOLAP_TRY(exclusive_scan_i64(ctx, occ, nb, ws.at<void>(o_oscr)));
-            hipLaunchKernelGGL(mm_compact_k, dim3(unsigned(nb)), dim3(kT), 0, ctx->stream, t, cap,
-                               chunk, occ, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs));
+                               k, t, uint64_t(p.cap - 1), st);
+            int64_t* occ = ws.at<int64_t>(p.o_occ);
+            hipLaunchKernelGGL(mm_occ_count_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, t,
+                               p.cap, p.chunk, occ, st);
+            BOLAP_TRY(exclusive_scan_i64(ctx, occ, p.nb, ws.at<void>(p.o_oscr)));
+            hipLaunchKernelGGL(mm_compact_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, t,
+                               p.cap, p.chunk, occ, ws.at<uint64_t>(p.o_ck),
+                               ws.at<uint32_t>(p.o_cs));
             BOLAP_TRY(launch_check("mm_hash_k"));
         }
         BOLAP_TRY(read_st(ctx, h));
@@ -497,11 +616,11 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
                             (long long)groups_n, (long long)max_rows);
             TraceScope ts(ctx, "minmax_emit");
             int passes = 0;
-            BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs),
-                                       groups_n, std::max(1, log2_ceil(slots)),
-                                       ws.at<void>(o_scr), &passes));
+            BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(p.o_ck), ws.at<uint32_t>(p.o_cs),
+                                       groups_n, std::max(1, log2_ceil(p.slots)),
+                                       ws.at<void>(p.o_sort), &passes));
             hipLaunchKernelGGL(mm_hash_emit_k, dim3(grid_of(ctx, groups_n)), dim3(kT), 0,
-                               ctx->stream, ws.at<uint64_t>(o_ck), ws.at<uint32_t>(o_cs),
+                               ctx->stream, ws.at<uint64_t>(p.o_ck), ws.at<uint32_t>(p.o_cs),
                                groups_n, a->keys, t, rows_out);
             BOLAP_TRY(launch_check("mm_hash_emit_k"));
         }
@@ -511,8 +630,20 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
     if (stats) {
         stats->expanded_rows = expanded;
         stats->groups = groups_n;
-        stats->placement = dense ? BOLAP_MINMAX_DENSE : BOLAP_MINMAX_HASH;
-        stats->slots = dense ? slots : 0;
+        stats->placement = p.dense ? BOLAP_MINMAX_DENSE : BOLAP_MINMAX_HASH;
+        stats->slots = p.dense ? p.slots : 0;
     }
     return BOLAP_OK;
+}
+
+}  // namespace bolap
+
+extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
+                                  bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
+                                  bolap_minmax_stats* stats) {
+    BOLAP_ARG_CHECK(ctx && a, "bolap_group_minmax: null argument");
+    bolap::TraceScope ts_all(ctx, "minmax");
+    int64_t expanded = 0;
+    BOLAP_TRY(bolap::minmax_expanded(ctx, a, &expanded));
+    return bolap::minmax_run(ctx, a, expanded, false, 0, rows_out, max_rows, n_out, stats);
 }

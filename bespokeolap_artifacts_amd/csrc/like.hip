@@ -1,4 +1,5 @@
-// like.hip — host-side ILIKE over a string dictionary (no device code).
+// like.hip — host-side ILIKE over a string dictionary (no device code; the
+// device form over a dictionary in HBM is in ceb_q10a.hip).
 //
 // The reference evaluates ILIKE once per distinct string id and caches the
 // answer (bespoke_ceb/query_q10a.cpp:43-209, 756-808); the dictionary-code
@@ -45,15 +46,12 @@ bool wildcard(const std::string& p, const char* t, size_t tn) {
     return i == p.size();
 }
 
-enum class Kind { kAll, kExact, kPrefix, kSuffix, kContains, kGeneral };
+using Kind = bolap::LikeKind;
+using Pattern = bolap::LikePattern;
 
-struct Pattern {
-    Kind kind = Kind::kGeneral;
-    std::string p;      // lower-cased pattern
-    std::string token;  // literal part for the whole-string/prefix/suffix/substring forms
-};
+}  // namespace
 
-Pattern classify(const char* raw) {
+bolap::LikePattern bolap::like_classify(const char* raw) {
     Pattern r;
     r.p = raw;
     for (char& c : r.p) c = lower(c);
@@ -84,6 +82,8 @@ Pattern classify(const char* raw) {
     return r;
 }
 
+namespace {
+
 bool matches(const Pattern& pt, const char* t, size_t tn) {
     std::string v(t, tn);
     for (char& c : v) c = lower(c);
@@ -106,7 +106,7 @@ extern "C" int bolap_ilike_mask(const char* chars, const int64_t* offsets, int64
                                 const char* pattern, uint8_t* mask) {
     BOLAP_ARG_CHECK(n >= 0 && pattern && (n == 0 || (chars && offsets && mask)),
                     "bolap_ilike_mask: bad arguments");
-    const Pattern pt = classify(pattern);
+    const Pattern pt = bolap::like_classify(pattern);
     for (int64_t i = 0; i < n; ++i) {
         const int64_t a = offsets[i], b = offsets[i + 1];
         BOLAP_ARG_CHECK(0 <= a && a <= b, "bolap_ilike_mask: offsets not increasing at %lld",

@@ -1,6 +1,6 @@
 // ceb_query.cpp — the reference's CEB load() / build() / query() sequence for
-// the factorised COUNT queries 1a and 2a, with libbolap's ceb_count kernels
-// doing the scans.
+// the factorised COUNT queries 1a and 2a (libbolap's ceb_count kernels) and
+// the grouped MIN / MAX query 10a (bolap_ceb_q10a).
 //
 //   bolap_ceb <dir/>  <  requests
 //
@@ -14,13 +14,19 @@
 //        pool interns tables in the builder's table order and columns in file
 //        order, first appearance first; the reference's own ids depend on its
 //        thread timing (builder_impl.cpp:407-431), and no COUNT depends on
-//        them.  The reference also sorts some tables (:432-452); the count
-//        kernels do not depend on row order, so the tables stay as loaded.
-//        Columns are copied to the device when a query first reads them.
+//        them.  The tables whose order a served query can observe are put
+//        in the builder's orders (:432-452): movie_info by (movie_id,
+//        info_type_id), cast_info by (movie_id, role_id), name and title by
+//        id, with ties in load order (the reference's std::sort leaves them
+//        unspecified); 10a emits its rows in the order its loops first reach
+//        them.  Columns are copied to the device when a query first reads
+//        them.
 // query  reads one request per line until an empty line (query_impl.cpp:
 //        242-256), runs them in order, prints "N | Execution ms: T" (the
 //        timer around run_qN only) and writes resultN.csv: count_star() for
-//        1a and 2a (:52-58); an id of the workload without a GPU pipeline here
+//        1a and 2a (:52-58), the name,info,min,max rows of 10a with the
+//        reference's quoting (:68-88, 140-160); an id of the workload
+//        without a GPU pipeline here
 //        gets the reference's stub file "result / 0" (:45-51) and a note on
 //        stderr; an unknown id is an error, as there (:222-224).
 #include <arrow/api.h>
@@ -485,17 +491,12 @@ int64_t run_1a(bolap_ctx* ctx, const Db& db, const Q1a& q) {
 // run_q2a (bespoke_ceb/query_q2a.cpp:205-503).
 struct Q2a : Q1a {};
 
-// INFO values as query_q2a.cpp:75-121 resolves them: an IN-list value split
-// at its commas is joined back with ", " or "," when the joined text is a
-// value of the column; the longest such join starting at a value wins.
-std::vector<int32_t> column_string_filter(const Db& db, const Column& col,
-                                          const std::vector<std::string>& vals) {
-    std::unordered_set<int32_t> present;
-    for (size_t i = 0; i < col.v.size(); ++i)
-        if (!col.null(i)) present.insert(col.v[i]);
-    auto lookup = [&](const std::string& s, int32_t& id) {
-        return db.pool.find(s, id) && present.count(id) != 0;
-    };
+// IN-list strings resolved as query_q2a.cpp:75-121 and query_q10a.cpp:
+// 229-265 do: a value split at its commas is joined back with ", " or ","
+// when the joined text is known (`known`); the longest such join starting at
+// a value wins.
+template <class Known>
+std::vector<int32_t> merged_string_filter(const std::vector<std::string>& vals, Known known) {
     std::vector<int32_t> out;
     for (size_t i = 0; i < vals.size();) {
         if (is_null_token(vals[i])) {
@@ -512,8 +513,8 @@ std::vector<int32_t> column_string_filter(const Db& db, const Column& col,
                 without += "," + vals[j];
             }
             int32_t id;
-            if (lookup(with_space, id)) found = id, last = j;
-            if (lookup(without, id)) found = id, last = j;
+            if (known(with_space, id)) found = id, last = j;
+            if (known(without, id)) found = id, last = j;
         }
         if (found < 0) {
             ++i;
@@ -523,6 +524,17 @@ std::vector<int32_t> column_string_filter(const Db& db, const Column& col,
         i = last + 1;
     }
     return out;
+}
+
+// 2a: a joined text counts when it is a value of the column.
+std::vector<int32_t> column_string_filter(const Db& db, const Column& col,
+                                          const std::vector<std::string>& vals) {
+    std::unordered_set<int32_t> present;
+    for (size_t i = 0; i < col.v.size(); ++i)
+        if (!col.null(i)) present.insert(col.v[i]);
+    return merged_string_filter(vals, [&](const std::string& s, int32_t& id) {
+        return db.pool.find(s, id) && present.count(id) != 0;
+    });
 }
 
 int64_t run_2a(bolap_ctx* ctx, const Db& db, const Q1a& q) {
@@ -587,6 +599,222 @@ int64_t run_2a(bolap_ctx* ctx, const Db& db, const Q1a& q) {
     return count_on_gpu(ctx, a);
 }
 
+// ---------------------------------------------------------------------------
+// 10a: SELECT n.name, mi1.info, MIN(t.production_year), MAX(...)
+// (bespoke_ceb/query_q10a.cpp:334-1120)
+// ---------------------------------------------------------------------------
+struct Q10a {
+    std::vector<std::string> id, info, kind, role;
+    std::string name;
+};
+
+Q10a parse_10a(const std::string& line) {  // args_parser.hpp:623-649
+    std::istringstream in(line);
+    std::string qid;
+    if (!(in >> qid)) throw std::runtime_error("Q10a: failed to parse query id");
+    Q10a q;
+    q.id = in_list(in);
+    q.info = in_list(in);
+    q.name = quoted_arg(in, "Q10a", "NAME");
+    q.kind = in_list(in);
+    q.role = in_list(in);
+    return q;
+}
+
+// id -> 1 for the rows of a dimension table whose string column holds one of
+// `ids` (both non-null), over rows + 1 entries; ids outside are dropped
+// (build_allowed_id_mask, query_q10a.cpp:183-209).
+std::vector<uint8_t> allowed_ids(const Db& db, const char* table, const char* str_col,
+                                 const std::vector<int32_t>& ids, bool& any) {
+    const auto& t = db.at(table);
+    const auto& id = t.at(table, "id");
+    const auto& s = t.at(table, str_col);
+    const std::unordered_set<int32_t> want(ids.begin(), ids.end());
+    std::vector<uint8_t> m(size_t(t.rows) + 1, 0);
+    any = false;
+    for (int64_t i = 0; i < t.rows; ++i) {
+        if (id.null(size_t(i)) || s.null(size_t(i))) continue;
+        if (!want.count(s.v[size_t(i)])) continue;
+        const int32_t k = id.v[size_t(i)];
+        if (k < 0 || size_t(k) >= m.size()) continue;
+        m[size_t(k)] = 1;
+        any = true;
+    }
+    return m;
+}
+
+// The pool's strings on the device, for ILIKE (uploaded on first use).
+struct PoolDev {
+    std::unique_ptr<DevBuf> chars, offsets;
+    int64_t n = -1;
+    void ensure(const Pool& pool) {
+        if (n == int64_t(pool.values.size())) return;
+        std::vector<char> c;
+        std::vector<int64_t> off{0};
+        for (const auto& v : pool.values) {
+            c.insert(c.end(), v.begin(), v.end());
+            off.push_back(int64_t(c.size()));
+        }
+        chars = std::make_unique<DevBuf>();
+        offsets = std::make_unique<DevBuf>();
+        chars->upload(c);
+        offsets->upload(off);
+        n = int64_t(pool.values.size());
+    }
+};
+
+bolap_ceb_col col_of(const Column& c) { return bolap_ceb_col{c.dev(), c.dev_null()}; }
+
+std::vector<bolap_minmax_row> run_10a(bolap_ctx* ctx, const Db& db, PoolDev& pdev,
+                                      const Q10a& q) {
+    const std::vector<bolap_minmax_row> none;
+    const auto types = int_values(q.id);
+    if (types.empty()) return none;  // :352-358
+    auto in_pool = [&](const std::string& s, int32_t& id) { return db.pool.find(s, id); };
+    const auto kind_f = merged_string_filter(q.kind, in_pool);
+    const auto role_f = merged_string_filter(q.role, in_pool);
+    const auto info_f = merged_string_filter(q.info, in_pool);
+    // a dense index per INFO string (:373-400; any numbering gives the same rows)
+    const int64_t pool_n = int64_t(db.pool.values.size());
+    std::vector<int32_t> info_dense(size_t(pool_n), -1), info_ids;
+    for (int32_t id : info_f) {
+        if (id < 0 || id >= pool_n || info_dense[size_t(id)] != -1) continue;
+        info_dense[size_t(id)] = int32_t(info_ids.size());
+        info_ids.push_back(id);
+    }
+    bool kind_any = false, role_any = false;
+    const auto kind_ok = allowed_ids(db, "kind_type", "kind", kind_f, kind_any);
+    if (!kind_any || role_f.empty() || info_f.empty()) return none;  // :402-409
+    const auto role_ok = allowed_ids(db, "role_type", "role", role_f, role_any);
+    if (!role_any) return none;  // :410-417
+    const auto& info_type = db.at("info_type");
+    std::vector<uint8_t> type_ok(size_t(info_type.rows) + 1, 0);
+    bool type_any = false;
+    for (int32_t v : types) {
+        if (v < 0 || size_t(v) >= type_ok.size()) continue;
+        type_ok[size_t(v)] = 1;
+        type_any = true;
+    }
+    if (!type_any) return none;  // :418-433
+    // the NAME pattern (:137-173): quotes stripped, lower-cased; '%' alone
+    // matches every name
+    std::string pat = q.name;
+    if (pat.size() >= 2 && pat.front() == '\'' && pat.back() == '\'')
+        pat = pat.substr(1, pat.size() - 2);
+    std::string low = pat;
+    for (char& c : low) c = char(std::tolower(static_cast<unsigned char>(c)));
+    const auto& title = db.at("title");
+    const auto& mi = db.at("movie_info");
+    const auto& ci = db.at("cast_info");
+    const auto& nm = db.at("name");
+    DevBuf d_kind, d_role, d_type, d_dense, d_ids, d_name;
+    bolap_ceb_q10a_args a{};
+    a.title_rows = title.rows;
+    a.t_id = col_of(title.at("title", "id"));
+    a.t_kind_id = col_of(title.at("title", "kind_id"));
+    a.t_year = col_of(title.at("title", "production_year"));
+    a.mi_rows = mi.rows;
+    a.mi_movie = col_of(mi.at("movie_info", "movie_id"));
+    a.mi_type = col_of(mi.at("movie_info", "info_type_id"));
+    a.mi_info = col_of(mi.at("movie_info", "info"));
+    a.ci_rows = ci.rows;
+    a.ci_movie = col_of(ci.at("cast_info", "movie_id"));
+    a.ci_person = col_of(ci.at("cast_info", "person_id"));
+    a.ci_role = col_of(ci.at("cast_info", "role_id"));
+    a.name_rows = nm.rows;
+    a.n_id = col_of(nm.at("name", "id"));
+    a.n_name = col_of(nm.at("name", "name"));
+    a.kind_ok = d_kind.upload(kind_ok);
+    a.kind_n = int64_t(kind_ok.size());
+    a.role_ok = d_role.upload(role_ok);
+    a.role_n = int64_t(role_ok.size());
+    a.type_ok = d_type.upload(type_ok);
+    a.type_n = int64_t(type_ok.size());
+    a.pool_n = pool_n;
+    a.info_dense = d_dense.upload(info_dense);
+    a.info_ids = d_ids.upload(info_ids);
+    a.n_infos = int32_t(info_ids.size());
+    if (low != "%") {
+        pdev.ensure(db.pool);
+        uint8_t* m = d_name.upload(std::vector<uint8_t>(size_t(std::max<int64_t>(pool_n, 1))));
+        check(bolap_ilike_mask_device(ctx, static_cast<const char*>(pdev.chars->p),
+                                      static_cast<const int64_t*>(pdev.offsets->p), pool_n,
+                                      pat.c_str(), m));
+        a.name_ok = m;
+    }
+    std::vector<bolap_minmax_row> rows(4096);
+    int64_t n = 0;
+    int st = bolap_ceb_q10a(ctx, &a, rows.data(), int64_t(rows.size()), &n, nullptr);
+    if (st == BOLAP_ERR_CAPACITY) {
+        rows.resize(size_t(n));
+        st = bolap_ceb_q10a(ctx, &a, rows.data(), n, &n, nullptr);
+    }
+    check(st);
+    rows.resize(size_t(n));
+    return rows;
+}
+
+// write_csv_field (query_impl.cpp:68-88): quoted when the text holds a
+// comma, quote, newline, carriage return or backslash; quote and backslash
+// then take a backslash.
+void csv_field(std::ostream& out, const std::string& v) {
+    if (v.find_first_of(",\"\n\r\\") == std::string::npos) {
+        out << v;
+        return;
+    }
+    out << '"';
+    for (char c : v) {
+        if (c == '"' || c == '\\') out << '\\';
+        out << c;
+    }
+    out << '"';
+}
+
+void write_10a(const std::string& path, const Db& db, const std::vector<bolap_minmax_row>& rows) {
+    std::ofstream out(path);  // query_impl.cpp:140-160
+    out << "name,info,min(t.production_year),max(t.production_year)\n";
+    for (const auto& r : rows) {
+        csv_field(out, db.pool.values[size_t(r.group)]);
+        out << ",";
+        csv_field(out, db.pool.values[size_t(r.key)]);
+        out << ",";
+        if (r.has) out << r.min;
+        out << ",";
+        if (r.has) out << r.max;
+        out << "\n";
+    }
+}
+
+// The builder's row order of one table by int32 columns (builder_impl.cpp:
+// 265-340), ties in load order; a missing key column leaves it as loaded.
+void sort_table(bolap_ctx* ctx, Table& t, const char* primary, const char* secondary) {
+    auto p = t.cols.find(primary);
+    if (p == t.cols.end() || p->second.str || t.rows < 2) return;
+    const Column* sec = nullptr;
+    if (secondary) {
+        auto it = t.cols.find(secondary);
+        if (it == t.cols.end() || it->second.str) return;
+        sec = &it->second;
+    }
+    DevBuf dp, ds, dord;
+    const int32_t* pp = dp.upload(p->second.v);
+    const int32_t* sp = sec ? ds.upload(sec->v) : nullptr;
+    uint32_t* ord = dord.upload(std::vector<uint32_t>(size_t(t.rows)));
+    check(bolap_sort_order_i32(ctx, pp, sp, t.rows, ord));
+    std::vector<uint32_t> order(size_t(t.rows));
+    hip_check(hipMemcpy(order.data(), ord, order.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (auto& [name, c] : t.cols) {
+        std::vector<int32_t> v(c.v.size());
+        std::vector<uint8_t> nu(c.nul.size());
+        for (size_t i = 0; i < order.size(); ++i) {
+            v[i] = c.v[order[i]];
+            nu[i] = c.nul[order[i]];
+        }
+        c.v.swap(v);
+        c.nul.swap(nu);
+    }
+}
+
 const char* const kTables[] = {
     "aka_name", "aka_title", "cast_info", "char_name", "comp_cast_type", "company_name",
     "company_type", "complete_cast", "info_type", "keyword", "kind_type", "link_type",
@@ -612,6 +840,18 @@ int main(int argc, char** argv) {
             const std::string path = dir + name + ".parquet";
             if (exists(path)) db.tables.emplace(name, build_table(*read_parquet(path), db.pool));
         }
+        // builder_impl.cpp:432-452, for the tables a served query reads in order
+        const struct {
+            const char *table, *primary, *secondary;
+        } orders[] = {{"movie_info", "movie_id", "info_type_id"},
+                      {"cast_info", "movie_id", "role_id"},
+                      {"name", "id", nullptr},
+                      {"title", "id", nullptr}};
+        for (const auto& o : orders) {
+            auto it = db.tables.find(o.table);
+            if (it != db.tables.end()) sort_table(ctx, it->second, o.primary, o.secondary);
+        }
+        PoolDev pdev;
         std::vector<std::pair<std::string, std::string>> requests;
         std::string line;
         while (std::getline(std::cin, line)) {
@@ -633,6 +873,17 @@ int main(int argc, char** argv) {
                                     .count();
                 std::cout << (i + 1) << " | Execution ms: " << ms << "\n";
                 std::ofstream(csv) << "count_star()\n" << n << "\n";
+                continue;
+            }
+            if (id == "10a") {
+                const Q10a q = parse_10a(text);
+                const auto start = std::chrono::steady_clock::now();
+                const auto rows = run_10a(ctx, db, pdev, q);
+                const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                                    std::chrono::steady_clock::now() - start)
+                                    .count();
+                std::cout << (i + 1) << " | Execution ms: " << ms << "\n";
+                write_10a(csv, db, rows);
                 continue;
             }
             if (!kWorkload.count(id)) throw std::runtime_error("Unsupported query id: " + id);

@@ -144,6 +144,22 @@ class MinmaxStats(C.Structure):
                 ("placement", C.c_int32), ("pad", C.c_int32), ("slots", C.c_int64)]
 
 
+class CebCol(C.Structure):
+    _fields_ = [("v", vp), ("null", vp)]
+
+
+class CebQ10aArgs(C.Structure):
+    _fields_ = [("title_rows", C.c_int64), ("t_id", CebCol), ("t_kind_id", CebCol),
+                ("t_year", CebCol), ("mi_rows", C.c_int64), ("mi_movie", CebCol),
+                ("mi_type", CebCol), ("mi_info", CebCol), ("ci_rows", C.c_int64),
+                ("ci_movie", CebCol), ("ci_person", CebCol), ("ci_role", CebCol),
+                ("name_rows", C.c_int64), ("n_id", CebCol), ("n_name", CebCol),
+                ("kind_ok", vp), ("kind_n", C.c_int64), ("role_ok", vp), ("role_n", C.c_int64),
+                ("type_ok", vp), ("type_n", C.c_int64), ("pool_n", C.c_int64),
+                ("info_dense", vp), ("info_ids", vp), ("name_ok", vp), ("n_infos", C.c_int32),
+                ("reserved", C.c_int32), ("dense_max_slots", C.c_int64)]
+
+
 class Q1Args(C.Structure):
     _fields_ = [("layout", C.c_int32), ("rows", C.c_int64), ("ship", vp),
                 ("ship_lo", C.c_int64), ("ship_hi", C.c_int64),
@@ -478,6 +494,9 @@ PROTOTYPES = {
     "bolap_gather_rows": (C.c_int, [vp, vp, i64, C.c_int32, vp, i64, vp]),
     "bolap_ceb_autotune": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(C.c_double)]),
     "bolap_ilike_mask": (C.c_int, [C.c_char_p, P(C.c_int64), i64, C.c_char_p, P(C.c_uint8)]),
+    "bolap_ilike_mask_device": (C.c_int, [vp, vp, vp, i64, C.c_char_p, vp]),
+    "bolap_ceb_q10a": (C.c_int, [vp, P(CebQ10aArgs), P(MinmaxRow), i64, P(C.c_int64),
+                                 P(MinmaxStats)]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),
     "bolap_mlp_destroy": (C.c_int, [vp]),
     "bolap_mlp_num_params": (C.c_int, []),

@@ -610,6 +610,54 @@ int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* args,
                        bolap_minmax_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* CEB query 10a (bespoke_ceb/query_q10a.cpp:334-1120) over device columns: */
+/* the title, movie_info, cast_info and name passes of run_q10a and its      */
+/* MIN/MAX(production_year) per (name, info) through the aggregation above.  */
+/* The caller resolves the IN lists as the reference does (kind / role /     */
+/* info strings through the string pool, :211-265, and the ID list, :211-   */
+/* 225) into masks indexed like the reference's arrays (kind_type, role_type */
+/* and info_type rows + 1 entries, ids outside them fail), a dense info     */
+/* index per string id, and the name ILIKE as a byte per string id.  Rows   */
+/* carry string ids: group = the name's, key = the info's; in the order the  */
+/* reference emits them when the (name, info) matrix is dense (<=           */
+/* dense_max_slots, 0: 5,000,000), else in (name rank, dense info) order.    */
+/* All pointers are device pointers except rows_out (host, max_rows rows).  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const int32_t* v;            /* values (string ids for string columns) */
+    const uint8_t* null;         /* 1 = null; NULL: the column has no nulls */
+} bolap_ceb_col;
+
+typedef struct {
+    int64_t title_rows;
+    bolap_ceb_col t_id, t_kind_id, t_year;
+    int64_t mi_rows;
+    bolap_ceb_col mi_movie, mi_type, mi_info;
+    int64_t ci_rows;
+    bolap_ceb_col ci_movie, ci_person, ci_role;
+    int64_t name_rows;
+    bolap_ceb_col n_id, n_name;
+    const uint8_t* kind_ok;  int64_t kind_n;   /* kind_type rows + 1 entries */
+    const uint8_t* role_ok;  int64_t role_n;   /* role_type rows + 1 entries */
+    const uint8_t* type_ok;  int64_t type_n;   /* info_type rows + 1 entries */
+    int64_t pool_n;                            /* strings in the pool */
+    const int32_t* info_dense;   /* pool_n entries: dense index of an INFO value, else -1 */
+    const int32_t* info_ids;     /* n_infos entries: string id of each dense index */
+    const uint8_t* name_ok;      /* pool_n entries: 1 when the name matches; NULL: all */
+    int32_t n_infos;
+    int32_t reserved;
+    int64_t dense_max_slots;
+} bolap_ceb_q10a_args;
+
+int bolap_ceb_q10a(bolap_ctx* ctx, const bolap_ceb_q10a_args* args, bolap_minmax_row* rows_out,
+                   int64_t max_rows, int64_t* n_out, bolap_minmax_stats* stats);
+
+/* bolap_ilike_mask over a dictionary in device memory (chars, offsets and
+   mask are device pointers; the pattern is a host string), same answers. */
+int bolap_ilike_mask_device(bolap_ctx* ctx, const char* chars, const int64_t* offsets, int64_t n,
+                            const char* pattern, uint8_t* mask);
+
+/* ------------------------------------------------------------------------ */
 /* CEB build-time data (bespoke_ceb/builder_impl.hpp:14-89, builder_impl.cpp  */
 /* :364-466): one string pool mapping every string column to int32 ids, and  */
 /* per-table sort orders.                                                    */

@@ -2,15 +2,19 @@
 
 Seeded IMDB-shaped Parquet tables with nulls go through the driver: Arrow
 load, string pool and nullable int32 columns, device mirror, the ceb_count
-kernels.  Every 1a / 2a answer must equal, byte for byte, the file the
-reference's own run_q1a / run_q2a (oracle/_ref/libref_ceb.so, compiled from
-its sources) writes for the same tables; ids without a GPU pipeline get the
-reference's stub file.
+kernels and bolap_ceb_q10a.  Every 1a / 2a / 10a answer must equal, byte for
+byte, the file the reference's own run_q1a / run_q2a / run_q10a
+(oracle/_ref/libref_ceb.so, compiled from its sources) writes for the same
+tables; where 10a's (name, info) matrix exceeds the reference's dense limit
+the rows are compared as a set (the reference emits them in its
+unordered_map's order).  Ids without a GPU pipeline get the reference's stub
+file.
 """
 from __future__ import annotations
 
 import subprocess
 
+import numpy as np
 import pytest
 
 from bespokeolap_artifacts_amd import build as B
@@ -48,7 +52,7 @@ def test_ceb_driver_equals_the_reference(tmp_path, movies):
     data.mkdir()
     for name, tab in CD.parquet(t).items():
         pq.write_table(tab, data / f"{name}.parquet")
-    lines = requests() + ["10a ('3') ('Drama') ('%a%') ('movie') ('actor')"]
+    lines = requests() + ["11a x"]
     run = tmp_path / "run"
     run.mkdir()
     res = subprocess.run([str(B.CEB_DRIVER), str(data) + "/"], input="\n".join(lines) + "\n\n",
@@ -60,7 +64,7 @@ def test_ceb_driver_equals_the_reference(tmp_path, movies):
     for i, w in enumerate(want):
         assert (run / f"result{i + 1}.csv").read_text() == w, lines[i]
     assert (run / f"result{len(lines)}.csv").read_text() == "result\n0\n"
-    assert "query 10a is not served" in res.stderr
+    assert "query 11a is not served" in res.stderr
     if movies > 40:
         assert sum(w != "count_star()\n0\n" for w in want) >= 6
 
@@ -71,3 +75,95 @@ def test_ceb_driver_rejects_an_unknown_id(tmp_path):
     res = subprocess.run([str(B.CEB_DRIVER), str(tmp_path) + "/"], input="99z x\n\n",
                          capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert res.returncode == 1 and "Unsupported query id: 99z" in res.stderr
+
+
+def q10a_requests() -> list[str]:
+    ids, roles = "('3', '8', '16')", "('actor', 'actress')"
+    kinds = "('movie', 'episode', 'tv series')"
+    infos = "('Drama', 'Comedy', 'USA', 'Germany', 'Austria', '1995', 'Italy', 'Spain', 'x')"
+    out = []
+    for name in ['"%"', '"\'%son 1%\'"', '"person 1_"', '"Person 12"', '"%3"', '"pers%"',
+                 '"%on%1%"', '"\'%\'"', '"%%"', '"nobody"']:
+        out.append(f"10a {ids} {infos} {name} {kinds} {roles}")
+    out += [
+        "10a ('3') ('Drama') ('%a%') ('movie') ('actor')",          # NAME is "('%a%')"
+        "10a ('<<NULL>>', 'abc') ('Drama') \"%\" ('movie') ('actor')",  # no info type
+        "10a ('3') ('Nope') \"%\" ('movie') ('actor')",             # no info value
+        "10a ('3') ('Drama') \"%\" ('movie') ('nobody')",           # no role
+        "10a ('3', '999') ('Drama', 'NULL', 'Horror') \"%\" ('video game') ('guest')",
+        "10a ('8') ('Germany', 'Austria') \"%\" ('movie') ('actor')",  # one joined value
+    ]
+    return out
+
+
+def names_with_duplicates(t: dict, seed: int) -> dict:
+    """Shared names (a group is a name string), a duplicated person id with
+    another name (the last row in id order wins) and null names."""
+    rng = np.random.default_rng(seed)
+    ids, nul = t["name"]["id"]
+    names = [f"Person {int(i) % 700}" if rng.random() < 0.8 else f"Person {int(i)}"
+             for i in ids.tolist()]
+    for i in np.nonzero(rng.random(len(names)) < 0.03)[0].tolist():
+        names[i] = None
+    dup = rng.choice(len(ids), 50, replace=False)
+    t["name"] = dict(t["name"])
+    t["name"]["id"] = (np.concatenate([ids, ids[dup]]).astype(np.int32), None)
+    t["name"]["name"] = names + [f"Alias {int(i)}" for i in ids[dup].tolist()]
+    for c in ("gender", "name_pcode_cf"):
+        t["name"][c] = t["name"][c] + [t["name"][c][int(j)] for j in dup.tolist()]
+    return t
+
+
+def run_driver(tmp_path, t, lines):
+    import pyarrow.parquet as pq
+    data = tmp_path / "data"
+    data.mkdir()
+    for name, tab in CD.parquet(t).items():
+        pq.write_table(tab, data / f"{name}.parquet")
+    run = tmp_path / "run"
+    run.mkdir()
+    res = subprocess.run([str(B.CEB_DRIVER), str(data) + "/"], input="\n".join(lines) + "\n\n",
+                         capture_output=True, text=True, cwd=run, timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert [ln.split(" | ")[0] for ln in res.stdout.splitlines()] == \
+        [str(i + 1) for i in range(len(lines))]
+    return [(run / f"result{i + 1}.csv").read_text() for i in range(len(lines))]
+
+
+@pytest.mark.parametrize("movies", [40, 60_000])
+def test_ceb_driver_10a_equals_the_reference(tmp_path, movies):
+    if not B.CEB_DRIVER.exists():
+        pytest.skip("bolap_ceb not built (no Arrow C++)")
+    if not CR.available():
+        pytest.skip("oracle/_ref not built")
+    print(f"ceb driver 10a vs reference: movies={movies}", flush=True)
+    t = names_with_duplicates(CD.tables(23 + movies, movies), movies)
+    lines = q10a_requests()
+    got = run_driver(tmp_path, t, lines)
+    want = CR.RefCeb(t).run(lines)
+    for line, g, w in zip(lines, got, want):
+        assert g == w, line
+    if movies > 40:
+        assert sum(w.count("\n") > 1 for w in want) >= 8
+        assert max(w.count("\n") for w in want) > 1000
+
+
+def test_ceb_driver_10a_hashed_groups(tmp_path):
+    """Over 5,000,000 (name, info) slots: the hashed placement; the same rows
+    as the reference's, which it emits in hash order."""
+    if not B.CEB_DRIVER.exists():
+        pytest.skip("bolap_ceb not built (no Arrow C++)")
+    if not CR.available():
+        pytest.skip("oracle/_ref not built")
+    print("ceb driver 10a hashed", flush=True)
+    t = CD.tables(5, 60_000, people=250_000, ci_per=12.0)
+    infos = "('" + "', '".join(CD.GENRES + CD.COUNTRIES + CD.LANGS +
+                               [str(y) for y in range(1990, 2000)]) + "')"
+    lines = [f"10a ('3', '8', '4', '16') {infos} \"%\" ('movie', 'episode', 'tv series', "
+             "'tv movie', 'video movie', 'video game', 'tv mini series') ('actor', 'actress', "
+             "'producer', 'writer', 'director', 'editor', 'guest', 'composer')"]
+    got = run_driver(tmp_path, t, lines)[0]
+    want = CR.RefCeb(t).run(lines)[0]
+    g, w = got.splitlines(), want.splitlines()
+    assert g[0] == w[0] and len(g) > 200_000
+    assert sorted(g[1:]) == sorted(w[1:])
