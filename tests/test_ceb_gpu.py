@@ -1,5 +1,7 @@
 """CEB star queries on the GPU: one at a time and batched (hipGraph replay,
 autotuned), bit-exact against the oracle."""
+import re
+
 import numpy as np
 import pytest
 
@@ -19,11 +21,38 @@ def host(t):
     return t.cpu().numpy()
 
 
+def like_regex_mask(chars: bytes, offsets: np.ndarray, pattern: str) -> np.ndarray:
+    """ILIKE as a regular expression, independent of the library: '%' any
+    run, '_' any one character, everything else literal, ASCII case folded,
+    whole string.  It agrees with the reference's matcher (query_q10a.cpp:
+    43-209) on text without '%' or '_', which the dictionaries here hold."""
+    key = (id(chars), pattern)
+    if key not in _LIKE_CACHE:
+        texts = _TEXTS.get(id(chars))
+        if texts is None:
+            texts = [chars[offsets[i]:offsets[i + 1]].decode() for i in range(len(offsets) - 1)]
+            assert not any("%" in x or "_" in x for x in texts)
+            _TEXTS[id(chars)] = texts
+        rx = re.compile("".join(".*" if c == "%" else "." if c == "_" else re.escape(c)
+                                for c in pattern), re.DOTALL | re.IGNORECASE | re.ASCII)
+        _LIKE_CACHE[key] = np.fromiter((rx.fullmatch(x) is not None for x in texts), np.uint8,
+                                       len(texts))
+    return _LIKE_CACHE[key]
+
+
+_TEXTS: dict = {}
+_LIKE_CACHE: dict = {}
+
+
 def oracle_masks(schema, q):
+    """The oracle's masks from like_regex_mask; the library's own (the
+    device bitmaps' source) must agree with them."""
     m = {}
     for key in ("title", "info", "info_type", "keyword"):
         if key in q:
-            mask = schema.bits(key, q[key])[2]
+            chars, offs = schema.dicts[key]
+            mask = like_regex_mask(chars, offs, q[key])
+            np.testing.assert_array_equal(schema.bits(key, q[key])[2], mask)
             if key in ("info_type", "keyword"):  # over ids: id = code + 1
                 mask = np.concatenate([[0], mask]).astype(np.uint8)
             m[key] = mask
