@@ -28,6 +28,8 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
+import tempfile
 import time
 
 import numpy as np
@@ -51,10 +53,16 @@ def hll_hist(regs: np.ndarray) -> np.ndarray:
 
 def hll_estimate(regs: np.ndarray) -> float:
     """HyperLogLog cardinality (Flajolet et al. 2007) with linear counting
-    for small ranges, m = 4096."""
+    for small ranges, m = 4096.  The harmonic sum runs over register values
+    in ascending order (exact terms), as the library's estimator sums it."""
     m = N.HLL_REGS
-    est = ALPHA_M * m * m / float(np.sum(2.0 ** -regs.astype(np.float64)))
-    zeros = int((regs == 0).sum())
+    cnt = np.bincount(regs.astype(np.int64), minlength=256)
+    s = 0.0
+    for v in range(len(cnt)):
+        if cnt[v]:
+            s += math.ldexp(float(cnt[v]), -v)
+    est = ALPHA_M * m * m / s
+    zeros = int(cnt[0])
     if est <= 2.5 * m and zeros:
         est = m * math.log(m / zeros)
     return est
@@ -67,6 +75,7 @@ class SchemaStats:
         self.fact_rows = int(fact_rows)
         self.dim_rows = [int(r) for r in dim_rows]
         self.n_groups = int(n_groups)
+        self.regs = regs
         self.hist = {c: hll_hist(r) for c, r in regs.items()}
         self.distinct = {c: hll_estimate(r) for c, r in regs.items()}
 
@@ -204,7 +213,7 @@ class Record:
         self.y = np.stack(ys) if ys else None
 
 
-SAMPLE_SEED = 0x5EED
+SAMPLE_SEED = N.EST_SAMPLE_SEED  # the library estimator samples with the same seed
 
 
 def record(engine, schema: W.StarSchema, ss: SchemaStats, plans: list[dict],
@@ -309,12 +318,30 @@ class Estimator:
                 passes.append(a)
                 joins.append(b)
                 groups.append(c)
-            out.append({"dim_pass": passes, "matches": float(np.exp(np.mean(np.log1p(joins))) - 1),
-                        "groups": float(np.exp(np.mean(np.log1p(groups))) - 1)})
+            # mean in log1p space, summed in rotation order (as the library)
+            out.append({"dim_pass": passes,
+                        "matches": math.exp(sum(math.log1p(j) for j in joins) / k) - 1,
+                        "groups": math.exp(sum(math.log1p(g) for g in groups) / k) - 1})
         return out
 
     def hints(self, rec: Record, pred: dict) -> tuple[list[int], int]:
         return sizing_hints(rec, pred, self.margin)
+
+    def save(self, path: str) -> None:
+        """The trained estimator as a file bolap_estimator_load reads: MLP
+        parameters, output standardisation and margins."""
+        lib = self.engine.lib
+        params = np.zeros(lib.bolap_mlp_num_params(), np.float32)
+        N.check(lib.bolap_mlp_get_params(self.h, params.ctypes.data_as(C.POINTER(C.c_float))))
+        musd = np.concatenate([self.mu, self.sd]).astype(np.float32)
+        margins = np.array([self.margin["dim"], self.margin["group"]], np.float64)
+        est = C.c_void_p()
+        N.check(lib.bolap_estimator_create(self.engine.ctx, params.ctypes.data,
+                                           musd.ctypes.data, margins.ctypes.data, C.byref(est)))
+        try:
+            N.check(lib.bolap_estimator_save(est, str(path).encode()))
+        finally:
+            lib.bolap_estimator_destroy(est)
 
 
 def sample_estimate(rec: Record) -> dict:
@@ -340,6 +367,52 @@ def sizing_hints(rec: Record, pred: dict, margin: dict) -> tuple[list[int], int]
     g = min(g, ss.span(rec.plan["group"]), max(pred["matches"] * margin["group"], 1.0))
     g = max(g, float(rec.counts[7]), 1.0)
     return dims, int(math.ceil(g))
+
+
+# --------------------------------------------------- the library estimator
+def est_columns(schema: W.StarSchema, ss: SchemaStats):
+    """Column statistics of a star schema as bolap_estimator_hints takes
+    them: each fact key (FK) and the group column (GROUP) with their
+    HyperLogLog registers, each dimension's value (VALUE) and category
+    (CATEGORY) columns with their domains."""
+    cols = []
+
+    def add(t, kind, span, regs=None):
+        c = N.EstColumn()
+        c.col = t.data_ptr()
+        c.kind = kind
+        c.span = int(span)
+        if regs is not None:
+            C.memmove(c.hll, np.ascontiguousarray(regs, np.uint8).ctypes.data, N.HLL_REGS)
+        cols.append(c)
+
+    for j, rows in enumerate(schema.dim_rows):
+        add(schema.fact[f"fk{j}"], N.EST_FK, rows, ss.regs[f"fk{j}"])
+        add(schema.dims[j]["val"], N.EST_VALUE, W.VAL_MAX + 1)
+        add(schema.dims[j]["cat"], N.EST_CATEGORY, W.CAT_MAX + 1)
+    add(schema.fact["grp"], N.EST_GROUP, schema.n_groups, ss.regs["grp"])
+    return (N.EstColumn * len(cols))(*cols)
+
+
+class LibraryEstimator:
+    """A saved estimator loaded into the library (bolap_estimator_load)."""
+
+    def __init__(self, engine, path: str):
+        self.lib = engine.lib
+        self.h = C.c_void_p()
+        N.check(self.lib.bolap_estimator_load(engine.ctx, str(path).encode(), C.byref(self.h)))
+
+    def hints(self, cols, args: list) -> list[tuple[list[int], int]]:
+        """(pass hints in plan order, group hint) per bound plan."""
+        arr = (N.StarArgs * len(args))(*args)
+        N.check(self.lib.bolap_estimator_hints(self.h, cols, len(cols), arr, len(args)))
+        return [([int(a.dims[i].pass_hint) for i in range(a.n_dims)], int(a.group_hint))
+                for a in arr]
+
+    def close(self):
+        if self.h:
+            self.lib.bolap_estimator_destroy(self.h)
+            self.h = None
 
 
 def q_error(est: float, true: float) -> float:
@@ -461,13 +534,25 @@ def evaluate(engine, est: "Estimator", schema: W.StarSchema, recs: list[Record],
     preds = est.predict(recs)
     engine.sync()
     infer_s = time.perf_counter() - t0
+    # the same hints from the library (saved weights, C featurisation)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "estimator.bin")
+        est.save(path)
+        lib_est = LibraryEstimator(engine, path)
+    cols = est_columns(schema, recs[0].ss)
+    t1 = time.perf_counter()
+    lib_hints = lib_est.hints(cols, [W.bind(schema, r.plan) for r in recs])
+    lib_s = time.perf_counter() - t1
+    lib_est.close()
     base = [sample_estimate(r) for r in recs]
     out = {"t_exact": 0.0, "t_learned": 0.0, "t_base": 0.0, "infer_s": infer_s,
+           "library_s": lib_s, "library_same": 0,
            "fallbacks": 0, "base_fallbacks": 0, "identical": True, "n": len(recs),
            "qn": q_errors(recs, preds), "qb": q_errors(recs, base)}
-    for r, p, b in zip(recs, preds, base):
+    for r, p, b, lh in zip(recs, preds, base, lib_hints):
         te, (st_e, ke, re_) = _timed(engine, W.bind(schema, r.plan), reps)
         dims, g = est.hints(r, p)
+        out["library_same"] += (list(dims), g) == lh
         tl, (st_l, kl, rl) = _timed(engine, W.bind(schema, r.plan, dims, g), reps)
         bd, bg = sizing_hints(r, b, est.margin)
         tb, (st_b, kb, rb) = _timed(engine, W.bind(schema, r.plan, bd, bg), reps)
@@ -502,6 +587,8 @@ def summarize(parts: list[dict]) -> dict:
         "learned_ms_per_plan": 1e3 * t_l / n,
         "without_network_ms_per_plan": 1e3 * t_b / n,
         "estimator_ms_per_plan": 1e3 * inf / n,
+        "library_estimator_ms_per_plan": 1e3 * sum(p["library_s"] for p in parts) / n,
+        "library_hints_same": sum(p["library_same"] for p in parts) / n,
         "exact_over_learned": t_e / t_l,
         "exact_over_learned_incl_estimator": t_e / (t_l + inf),
         "exact_over_without_network": t_e / t_b,

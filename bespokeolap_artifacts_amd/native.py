@@ -144,6 +144,15 @@ class MinmaxStats(C.Structure):
                 ("placement", C.c_int32), ("pad", C.c_int32), ("slots", C.c_int64)]
 
 
+EST_FK, EST_GROUP, EST_VALUE, EST_CATEGORY = 0, 1, 2, 3
+EST_SAMPLE_SEED = 0x5EED
+
+
+class EstColumn(C.Structure):
+    _fields_ = [("col", vp), ("kind", C.c_int32), ("pad", C.c_int32), ("span", C.c_int64),
+                ("hll", C.c_uint8 * 4096)]
+
+
 class CebCol(C.Structure):
     _fields_ = [("v", vp), ("null", vp)]
 
@@ -495,6 +504,11 @@ PROTOTYPES = {
     "bolap_ceb_autotune": (C.c_int, [vp, P(CebDb), P(CebQuery), C.c_int32, P(C.c_double)]),
     "bolap_ilike_mask": (C.c_int, [C.c_char_p, P(C.c_int64), i64, C.c_char_p, P(C.c_uint8)]),
     "bolap_ilike_mask_device": (C.c_int, [vp, vp, vp, i64, C.c_char_p, vp]),
+    "bolap_estimator_create": (C.c_int, [vp, vp, vp, vp, P(vp)]),
+    "bolap_estimator_destroy": (C.c_int, [vp]),
+    "bolap_estimator_save": (C.c_int, [vp, C.c_char_p]),
+    "bolap_estimator_load": (C.c_int, [vp, C.c_char_p, P(vp)]),
+    "bolap_estimator_hints": (C.c_int, [vp, P(EstColumn), C.c_int32, P(StarArgs), C.c_int32]),
     "bolap_ceb_q10a": (C.c_int, [vp, P(CebQ10aArgs), P(MinmaxRow), i64, P(C.c_int64),
                                  P(MinmaxStats)]),
     "bolap_mlp_create": (C.c_int, [vp, u64, P(vp)]),

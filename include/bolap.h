@@ -756,6 +756,47 @@ int bolap_mlp_gradients(bolap_mlp* mlp, const float* x_dev, const float* t_dev, 
                         float* grads_host, float* loss_out);
 
 /* ------------------------------------------------------------------------ */
+/* Learned sizing as a library call (no reference counterpart; the reference  */
+/* sizes tables from fixed selectivities, bespoke_tpch/query_q20.cpp:16,      */
+/* 405-417).  An estimator is the MLP's parameters, the standardisation of    */
+/* its three outputs (mu, sd) and the two sizing margins, saved to and loaded */
+/* from a file.  bolap_estimator_hints fills pass_hint of every dimension and */
+/* group_hint of each plan from the plan's sample counts, its predicate      */
+/* constants and per-column statistics registered by device pointer: every   */
+/* dimension's fact key (FK: HyperLogLog registers of bolap_hll_i32, span =  */
+/* dimension rows), the group column (GROUP, span = groups; or one of the FK  */
+/* columns), and every predicate column (VALUE for range-filtered values,     */
+/* CATEGORY for categorical codes; span = the domain size).  Hints only size: */
+/* bolap_star_groupby's results do not depend on them.                        */
+/* ------------------------------------------------------------------------ */
+#define BOLAP_EST_FK 0
+#define BOLAP_EST_GROUP 1
+#define BOLAP_EST_VALUE 2
+#define BOLAP_EST_CATEGORY 3
+#define BOLAP_EST_SAMPLE_SEED 0x5EEDull   /* the seed the sample counts use */
+
+typedef struct {
+    const void* col;             /* device column, as the plans point at it */
+    int32_t kind;                /* BOLAP_EST_* */
+    int32_t pad;
+    int64_t span;                /* see above */
+    uint8_t hll[BOLAP_HLL_REGS]; /* FK / GROUP: HyperLogLog registers */
+} bolap_est_column;
+
+typedef struct bolap_estimator bolap_estimator;
+
+/* params: bolap_mlp_num_params() floats (bolap_mlp_get_params order);
+   mu_sd: mu[3] then sd[3]; margins: dimension margin, group margin. */
+int bolap_estimator_create(bolap_ctx* ctx, const float* params, const float* mu_sd,
+                           const double* margins, bolap_estimator** out);
+int bolap_estimator_destroy(bolap_estimator* est);
+int bolap_estimator_save(const bolap_estimator* est, const char* path);
+int bolap_estimator_load(bolap_ctx* ctx, const char* path, bolap_estimator** out);
+/* plans are host structs (their columns on the device); hints written in place. */
+int bolap_estimator_hints(bolap_estimator* est, const bolap_est_column* cols, int32_t n_cols,
+                          bolap_star_args* plans, int32_t n_plans);
+
+/* ------------------------------------------------------------------------ */
 /* Q6 shape: three-predicate range scan + SUM(extendedprice * discount) over  */
 /* the reference layout.  Replaces run_q6's scan (bespoke_tpch/query_q6.cpp:  */
 /* 155-397): the flat loop (:197-223), or, with a zone map, the shard-pruned  */

@@ -55,6 +55,7 @@ STRUCTS = {  # ctypes mirror -> C type name in bolap.h
     "StarDim": "bolap_star_dim", "StarArgs": "bolap_star_args", "StarStats": "bolap_star_stats",
     "CebDb": "bolap_ceb_db", "CebQuery": "bolap_ceb_query", "CebResult": "bolap_ceb_result",
     "CebCol": "bolap_ceb_col", "CebQ10aArgs": "bolap_ceb_q10a_args",
+    "EstColumn": "bolap_est_column",
     "Zone": "bolap_zone", "Q6Args": "bolap_q6_args", "Q6Result": "bolap_q6_result",
     "Q6Params": "bolap_q6_params", "Q18Args": "bolap_q18_args", "Q18Row": "bolap_q18_row",
     "Q18Stats": "bolap_q18_stats", "Q1Row": "bolap_q1_row", "Q14Args": "bolap_q14_args",

@@ -33,3 +33,10 @@ def test_network_beats_the_estimate_without_it(result):
 def test_learned_sizing_skips_the_counting_passes(result):
     # Hints replace two counting passes and two host round trips per plan.
     assert result["exact_over_learned"] > 1.0
+
+
+def test_library_estimator_gives_the_same_hints(result):
+    """The saved estimator loaded into the library (C featurisation, the same
+    MLP kernels, the same policy) sizes every held-out plan as Python does."""
+    assert result["library_hints_same"] == 1.0
+    assert result["library_estimator_ms_per_plan"] > 0
