@@ -23,11 +23,15 @@
 //           used byte set only by rows without a value (a slot is used when
 //           that byte is set or min <= max); a bitmap of used slots, a
 //           popcount scan and a rank scatter give the rows in slot order.
-//   hashed  open addressing on the 64-bit slot number group * keys + key,
-//           16-byte slots {key, min, max} so that an expanded row touches one
-//           cache line (CAS insertion, linear probing); occupied slots are
-//           compacted (one atomic per wave) and LSD radix sorted on the
-//           slot number's bits.
+//   sorted  beyond the dense limit nearly every expanded row is a group of
+//           its own (q10a: 14.1 M groups from 14.4 M expanded rows), so a
+//           table would only scatter them: the expanded rows are written as
+//           (slot << 1 | has, value) pairs (one cursor reservation per
+//           wave), radix sorted on the slot bits, and each wave reduces the
+//           runs that start in its chunk of the sorted rows (a segmented
+//           scan across the wave, the open run carried from tile to tile and
+//           past the chunk's end); run heads counted per wave and scanned
+//           give each run its output index, so rows come out in slot order.
 // A group has a value exactly when min <= max: both start at the empty
 // extremes (INT32_MAX, INT32_MIN) and every folded value lands between them.
 #include <algorithm>
@@ -64,6 +68,9 @@ struct Args {
     const int32_t *pg, *pi, *off, *cnt, *val, *rv;
     const uint8_t* has;
     int32_t groups, keys;
+    // optional: the item columns packed per item as {offset, count, value,
+    // has}, so that a join row's item costs one 16-byte gather, not four
+    const int4* pk;
 };
 
 __global__ __launch_bounds__(kT) void mm_zero_k(int64_t* __restrict__ st) {
@@ -82,16 +89,31 @@ BOLAP_DEV bool pair_range(const Args& a, int64_t p, int32_t& g, int64_t& lo, int
         bad = true;
         return false;
     }
-    lo = a.off[it];
-    const int64_t c = a.cnt[it];
+    int64_t c;
+    if (a.pk) {
+        const int4 q = a.pk[it];
+        lo = q.x;
+        c = q.y;
+        v = q.z;
+        has = q.w != 0;
+    } else {
+        lo = a.off[it];
+        c = a.cnt[it];
+        has = a.has ? a.has[it] != 0 : true;
+        v = a.val[it];
+    }
     hi = lo + c;
     if (lo < 0 || c < 0 || hi > a.values) {
         bad = true;
         return false;
     }
-    has = a.has ? a.has[it] != 0 : true;
-    v = a.val[it];
     return true;
+}
+
+__global__ __launch_bounds__(kT) void mm_pack_k(Args a, int4* __restrict__ pk) {
+    for (int64_t it = int64_t(blockIdx.x) * kT + threadIdx.x; it < a.items;
+         it += int64_t(gridDim.x) * kT)
+        pk[it] = make_int4(a.off[it], a.cnt[it], a.val[it], a.has ? (a.has[it] != 0) : 1);
 }
 
 __global__ __launch_bounds__(kT) void mm_count_k(Args a, int64_t* __restrict__ st) {
@@ -114,11 +136,6 @@ __global__ __launch_bounds__(kT) void mm_count_k(Args a, int64_t* __restrict__ s
     }
 }
 
-struct HSlot {  // hashed table slot
-    uint64_t key;
-    int32_t mn, mx;
-};
-
 __global__ __launch_bounds__(kT) void mm_init_dense_k(int2* __restrict__ mm,
                                                       uint8_t* __restrict__ used, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
@@ -135,12 +152,6 @@ __global__ __launch_bounds__(kT) void fill_u64_k(unsigned long long* __restrict_
         p[i] = v;
 }
 
-__global__ __launch_bounds__(kT) void mm_init_hash_k(HSlot* __restrict__ t, int64_t n) {
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kT)
-        t[i] = HSlot{kEmpty, INT32_MAX, INT32_MIN};
-}
-
 // Folds v into a slot's (min, max) after a plain read `cur` of the slot:
 // the minimum only falls and the maximum only rises, so a value inside a
 // (possibly stale) [cur.x, cur.y] is inside the current range too and needs
@@ -152,46 +163,75 @@ BOLAP_DEV void fold_seen(int32_t* mn, int32_t* mx, int2 cur, int32_t v) {
 
 // The item ranges of join rows p0 .. p0 + 63 expanded across one wave: lane
 // L takes expanded rows L, L + 64, ... of the concatenated ranges and calls
-// body(group, value, has, i, pos) for value index i, the j-th value of join
-// row p (pos = p << 32 | j: the reference's loop order), so the lanes stay busy
-// whatever the range lengths (a lane per join row idles on short ranges
-// while one long range runs).  p0 is uniform across the wave; every lane
-// reaches each shuffle.  Malformed join rows are skipped here (mm_count_k
-// counts them).
-template <typename F>
-BOLAP_DEV void expand_wave(const Args& a, int64_t p0, F&& body) {
+// body(group, value, has, i, pos, out) for value index i, the j-th value of
+// join row p (pos = p << 32 | j: the reference's loop order), so the lanes
+// stay busy whatever the range lengths (a lane per join row idles on short
+// ranges while one long range runs).  reserve(total) is called by every
+// lane with the wave's total of expanded rows and returns the (uniform)
+// output index of its first one; out = that index + the row's rank in the
+// batch.  p0 is uniform across the wave; every lane reaches each shuffle.
+// Malformed join rows are skipped here (mm_count_k counts them).
+struct WaveRanges {
+    int64_t lo = 0, exc = 0, total = 0;  // total is uniform across the wave
+    int32_t g = 0, v = 0;
+    int hv = 0;
+};
+
+// Lane-local range of join row p0 + lane and the wave's prefix sums.
+BOLAP_DEV WaveRanges wave_ranges(const Args& a, int64_t p0) {
     const int lane = int(threadIdx.x & (kWave - 1));
     const int64_t p = p0 + lane;
-    int32_t g = 0, v = 0;
-    int64_t lo = 0, hi = 0;
+    WaveRanges w;
+    int64_t hi = 0;
     bool has = false, bad = false;
-    const bool ok = p < a.pairs && pair_range(a, p, g, lo, hi, has, v, bad);
-    const int64_t c = ok ? hi - lo : 0;
+    const bool ok = p < a.pairs && pair_range(a, p, w.g, w.lo, hi, has, w.v, bad);
+    const int64_t c = ok ? hi - w.lo : 0;
     int64_t inc = c;
     for (int d = 1; d < kWave; d <<= 1) {
         const int64_t t = __shfl_up(inc, d, kWave);
         if (lane >= d) inc += t;
     }
-    const int64_t total = __shfl(inc, kWave - 1, kWave);
-    const int64_t exc = inc - c;
-    const int hv = has ? 1 : 0;
-    for (int64_t e0 = 0; e0 < total; e0 += kWave) {
+    w.total = __shfl(inc, kWave - 1, kWave);
+    w.exc = inc - c;
+    w.hv = has ? 1 : 0;
+    return w;
+}
+
+// The expanded rows of the wave's batch: lane L takes rows L, L + 64, ...
+template <typename F>
+BOLAP_DEV void wave_emit(const WaveRanges& w, int64_t p0, int64_t base, F&& body) {
+    const int lane = int(threadIdx.x & (kWave - 1));
+    for (int64_t e0 = 0; e0 < w.total; e0 += kWave) {
         const int64_t e = e0 + lane;
         // owner: the last lane whose range starts at or before e; it has a
         // non-empty range holding e (exclusive starts never decrease)
         int o = 0;
         for (int step = kWave / 2; step >= 1; step >>= 1) {
             const int cand = o + step;
-            if (__shfl(exc, cand, kWave) <= e) o = cand;
+            if (__shfl(w.exc, cand, kWave) <= e) o = cand;
         }
-        const int64_t o_exc = __shfl(exc, o, kWave);
-        const int64_t o_lo = __shfl(lo, o, kWave);
-        const int32_t o_g = __shfl(g, o, kWave);
-        const int32_t o_v = __shfl(v, o, kWave);
-        const int o_has = __shfl(hv, o, kWave);
+        const int64_t o_exc = __shfl(w.exc, o, kWave);
+        const int64_t o_lo = __shfl(w.lo, o, kWave);
+        const int32_t o_g = __shfl(w.g, o, kWave);
+        const int32_t o_v = __shfl(w.v, o, kWave);
+        const int o_has = __shfl(w.hv, o, kWave);
         const uint64_t pos = (uint64_t(p0 + o) << 32) | uint64_t(e - o_exc);
-        if (e < total) body(o_g, o_v, o_has != 0, o_lo + (e - o_exc), pos);
+        if (e < w.total) body(o_g, o_v, o_has != 0, o_lo + (e - o_exc), pos, base + e);
     }
+}
+
+template <typename R, typename F>
+BOLAP_DEV void expand_core(const Args& a, int64_t p0, R&& reserve, F&& body) {
+    const WaveRanges w = wave_ranges(a, p0);
+    wave_emit(w, p0, reserve(w.total), body);
+}
+
+template <typename F>
+BOLAP_DEV void expand_wave(const Args& a, int64_t p0, F&& body) {
+    expand_core(a, p0, [](int64_t) { return int64_t(0); },
+                [&](int32_t g, int32_t v, bool has, int64_t i, uint64_t pos, int64_t) {
+                    body(g, v, has, i, pos);
+                });
 }
 
 // Wave-uniform start of the wave's first batch of join rows, and the stride.
@@ -270,108 +310,231 @@ __global__ __launch_bounds__(kT) void mm_dense_emit_k(const uint32_t* __restrict
     }
 }
 
-__global__ __launch_bounds__(kT) void mm_hash_k(Args a, HSlot* __restrict__ t, uint64_t mask,
-                                                int64_t* __restrict__ st) {
+// Expanded rows as sort pairs: key (slot << 1 | has), value the item's
+// value.  A workgroup of kET threads takes kET join rows per round (a
+// 64-row batch per wave) and reserves the round's rows with one atomic on
+// `cursor` (one atomic per wave serialised 62,500 same-address atomics on
+// 4 M join rows).  A range value outside [0, keys) is counted as an error
+// (the call then fails) and its row gets the largest key.
+constexpr int kET = 1024;
+
+__global__ __launch_bounds__(kET) void mm_expand_k(Args a, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals,
+                                                   unsigned long long* __restrict__ cursor,
+                                                   int64_t cap, uint64_t bad_key,
+                                                   int64_t* __restrict__ st) {
+    constexpr int kEW = kET / kWave;
+    __shared__ int64_t s_tot[kEW];
+    __shared__ int64_t s_base;
+    const int lane = int(threadIdx.x & (kWave - 1));
+    const int wave = int(threadIdx.x / kWave);
     uint64_t errs = 0;
-    for (int64_t p0 = wave_first(); p0 < a.pairs; p0 += wave_stride()) {
-        expand_wave(a, p0, [&](int32_t g, int32_t v, bool has, int64_t i, uint64_t) {
-            const int32_t k = a.rv[i];
-            if (k < 0 || k >= a.keys) {
-                ++errs;
-                return;
-            }
-            const uint64_t key = uint64_t(g) * uint64_t(a.keys) + uint64_t(k);
-            // the table holds at most half as many keys as slots: a free
-            // slot is always reached within mask + 1 probes
-            uint64_t s = mix64(key) & mask;
-            int2 seen = make_int2(INT32_MAX, INT32_MIN);
-            for (uint64_t step = 0; step <= mask; ++step) {
-                const HSlot cur_slot = t[s];
-                uint64_t cur = cur_slot.key;
-                if (cur == kEmpty) {
-                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(&t[s].key),
-                                    (unsigned long long)kEmpty, (unsigned long long)key);
-                    if (cur == kEmpty) cur = key;
-                } else if (cur == key) {
-                    seen = make_int2(cur_slot.mn, cur_slot.mx);
-                }
-                if (cur == key) break;
-                s = (s + 1) & mask;
-            }
-            if (has) fold_seen(&t[s].mn, &t[s].mx, seen, v);
-        });
+    // the trip count is uniform across the workgroup (barriers below)
+    for (int64_t r0 = int64_t(blockIdx.x) * kET; r0 < a.pairs; r0 += int64_t(gridDim.x) * kET) {
+        const int64_t p0 = r0 + int64_t(wave) * kWave;
+        const WaveRanges w = wave_ranges(a, p0);
+        if (lane == 0) s_tot[wave] = w.total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t sum = 0;
+            for (int k = 0; k < kEW; ++k) sum += s_tot[k];
+            s_base = sum ? int64_t(atomicAdd(cursor, (unsigned long long)sum)) : 0;
+        }
+        __syncthreads();
+        int64_t base = s_base;
+        for (int k = 0; k < wave; ++k) base += s_tot[k];
+        wave_emit(w, p0, base,
+                  [&](int32_t g, int32_t v, bool has, int64_t i, uint64_t, int64_t out) {
+                      const int32_t k = a.rv[i];
+                      uint64_t key = bad_key;
+                      if (k < 0 || k >= a.keys)
+                          ++errs;
+                      else
+                          key = ((uint64_t(g) * uint64_t(a.keys) + uint64_t(k)) << 1) |
+                                (has ? 1u : 0u);
+                      if (out < cap) {  // cap = the counted rows
+                          keys[out] = key;
+                          vals[out] = uint32_t(v);
+                      }
+                  });
+        __syncthreads();  // s_tot and s_base are rewritten by the next round
     }
     errs = wave_sum(errs);
-    if ((threadIdx.x & (kWave - 1)) == 0 && errs)
-        atomicAdd(reinterpret_cast<unsigned long long*>(st + mErrors), errs);
+    if (lane == 0 && errs) atomicAdd(reinterpret_cast<unsigned long long*>(st + mErrors), errs);
 }
 
-// Occupied slots -> (key, slot) pairs in two passes over per-workgroup
-// chunks of the table: occupied counts per chunk, a device scan, then each
-// workgroup writes its chunk's pairs from its offset (no shared counter).
-__global__ __launch_bounds__(kT) void mm_occ_count_k(const HSlot* __restrict__ t, int64_t cap,
-                                                     int64_t chunk, int64_t* __restrict__ cnt,
-                                                     int64_t* __restrict__ st) {
-    __shared__ uint64_t s_w[kT / kWave];
-    const int64_t c0 = int64_t(blockIdx.x) * chunk;
-    const int64_t c1 = c0 + chunk < cap ? c0 + chunk : cap;
-    uint64_t n = 0;
-    for (int64_t s = c0 + threadIdx.x; s < c1; s += kT) n += t[s].key != kEmpty ? 1u : 0u;
-    n = wave_sum(n);
-    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = n;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t tot = 0;
-        for (int w = 0; w < kT / kWave; ++w) tot += s_w[w];
-        cnt[blockIdx.x] = int64_t(tot);
-        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(st + mGroups), tot);
-    }
+// Rows [c0, c1) of the sorted pairs belong to wave w: chunk is a multiple of
+// the wave width, so tiles of 64 rows never straddle two chunks.
+BOLAP_DEV bool run_head(const uint64_t* __restrict__ keys, int64_t i, int64_t n) {
+    return i < n && (i == 0 || (keys[i] >> 1) != (keys[i - 1] >> 1));
 }
 
-__global__ __launch_bounds__(kT) void mm_compact_k(const HSlot* __restrict__ t, int64_t cap,
-                                                   int64_t chunk, const int64_t* __restrict__ off,
-                                                   uint64_t* __restrict__ ck,
-                                                   uint32_t* __restrict__ cs) {
-    __shared__ uint32_t s_w[kT / kWave];
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int64_t c0 = int64_t(blockIdx.x) * chunk;
-    const int64_t c1 = c0 + chunk < cap ? c0 + chunk : cap;
-    uint64_t base = uint64_t(off[blockIdx.x]);
-    // the trip count is the same for every thread of the workgroup
-    for (int64_t b0 = c0; b0 < c1; b0 += kT) {
-        const int64_t s = b0 + threadIdx.x;
-        const uint64_t k = s < c1 ? t[s].key : kEmpty;
-        const bool occ = k != kEmpty;
-        const uint64_t bal = __ballot(occ);
-        if (lane == 0) s_w[wave] = uint32_t(__popcll(bal));
-        __syncthreads();
-        uint32_t before = 0, tile = 0;
+// Run heads (rows whose slot differs from the previous row's) per wave
+// chunk; cnt[waves] is set to 0, so that after an exclusive scan over
+// waves + 1 entries it holds the total (the group count).
+__global__ __launch_bounds__(kT) void mm_heads_k(const uint64_t* __restrict__ keys, int64_t n,
+                                                 int64_t chunk, int64_t waves,
+                                                 int64_t* __restrict__ cnt) {
+    const int lane = int(threadIdx.x & (kWave - 1));
+    const int64_t w = (int64_t(blockIdx.x) * kT + threadIdx.x) / kWave;
+    const int64_t c0 = w * chunk;
+    const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
+    uint64_t heads = 0;
+    constexpr int kU = 4;  // tiles per round, their loads in flight together
+    for (int64_t t0 = c0; t0 < c1; t0 += kU * kWave) {
+        bool h[kU];
 #pragma unroll
-        for (int w = 0; w < kT / kWave; ++w) {
-            before += w < wave ? s_w[w] : 0u;
-            tile += s_w[w];
-        }
-        __syncthreads();  // s_w is rewritten by the next tile
-        if (occ) {
-            const uint64_t at = base + before + __popcll(bal & ((1ull << lane) - 1ull));
-            ck[at] = k;
-            cs[at] = uint32_t(s);
-        }
-        base += tile;
+        for (int u = 0; u < kU; ++u) h[u] = run_head(keys, t0 + u * kWave + lane, c1);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) heads += uint64_t(__popcll(__ballot(h[u])));
     }
+    if (lane == 0) cnt[w] = int64_t(heads);
+    if (w == 0 && lane == 0) cnt[waves] = 0;
 }
 
-__global__ __launch_bounds__(kT) void mm_hash_emit_k(const uint64_t* __restrict__ ck,
-                                                     const uint32_t* __restrict__ cs, int64_t n,
-                                                     int32_t keys, const HSlot* __restrict__ t,
-                                                     bolap_minmax_row* __restrict__ out) {
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kT) {
-        const HSlot v = t[cs[i]];
-        const uint64_t k = ck[i];
-        const bool h = v.mn <= v.mx;
-        out[i] = bolap_minmax_row{int32_t(k / uint64_t(keys)), int32_t(k % uint64_t(keys)),
-                                  h ? 1 : 0, h ? v.mn : 0, h ? v.mx : 0};
+__global__ void mm_total_k(const int64_t* __restrict__ scanned, int64_t waves,
+                           int64_t* __restrict__ st) {
+    st[mGroups] = scanned[waves];
+}
+
+struct RunAcc {
+    int32_t mn, mx, has;
+};
+
+BOLAP_DEV RunAcc run_join(RunAcc x, RunAcc y) {
+    return RunAcc{x.mn < y.mn ? x.mn : y.mn, x.mx > y.mx ? x.mx : y.mx, x.has | y.has};
+}
+
+BOLAP_DEV void emit_run(bolap_minmax_row* out, int64_t id, int64_t max_rows, uint64_t key,
+                        int32_t keys_n, RunAcc r) {
+    if (id < 0 || id >= max_rows) return;  // only a broken input could reach here
+    const uint64_t slot = key >> 1;
+    out[id] = bolap_minmax_row{int32_t(slot / uint64_t(keys_n)), int32_t(slot % uint64_t(keys_n)),
+                               r.has ? 1 : 0, r.has ? r.mn : 0, r.has ? r.mx : 0};
+}
+
+// Reduces the runs that start in wave w's chunk of the sorted pairs.  Rows
+// of the chunk before its first head continue the previous wave's run and
+// are skipped; the run still open at the chunk's end is followed into the
+// next chunks until it ends.  Inside a tile a segmented scan across the
+// wave folds each run's rows into the run's last lane in the tile; a run
+// reaching the tile's last lane is carried (uniformly, through readlane)
+// into the next tile.  base[w] is the output index of the chunk's first run.
+__global__ __launch_bounds__(kT) void mm_reduce_k(const uint64_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ vals, int64_t n,
+                                                  int64_t chunk, const int64_t* __restrict__ base,
+                                                  int32_t keys_n, int64_t max_rows,
+                                                  bolap_minmax_row* __restrict__ out) {
+    const int lane = int(threadIdx.x & (kWave - 1));
+    const int64_t w = (int64_t(blockIdx.x) * kT + threadIdx.x) / kWave;
+    const int64_t c0 = w * chunk;
+    if (c0 >= n) return;  // uniform across the wave
+    const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
+    const RunAcc none{INT32_MAX, INT32_MIN, 0};
+    int64_t rid = base[w];        // output index of the next head
+    bool started = c0 == 0;       // a head of this chunk has been seen
+    bool open = false;            // a run reaches into the next tile
+    RunAcc carry = none;
+    uint64_t carry_key = 0;
+    int64_t carry_id = 0;
+    // rounds of kR tiles: each tile's key, the previous row's key and the
+    // value are loaded for the whole round before the first is used
+    constexpr int kR = 4;
+    bool done = false;
+    for (int64_t r0 = c0; r0 < n && !done; r0 += kR * kWave) {
+        uint64_t kk[kR], pp[kR];
+        uint32_t vv[kR];
+    #pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int64_t i = r0 + u * kWave + lane;
+            kk[u] = i < n ? keys[i] : 0;
+            pp[u] = i < n && i > 0 ? keys[i - 1] : 0;
+            vv[u] = i < n ? vals[i] : 0u;
+        }
+    #pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int64_t t0 = r0 + u * kWave;
+            const bool ext = t0 >= c1;  // past the chunk: only the open run
+            if (t0 >= n || (ext && !open)) {
+                done = true;
+                break;
+            }
+            const uint64_t key_c = kk[u], prev_c = pp[u];
+            const uint32_t val_c = vv[u];
+            const int64_t i = t0 + lane;
+            const bool valid = i < n;
+            const uint64_t key = key_c;
+            const bool head = valid && (i == 0 || (key_c >> 1) != (prev_c >> 1));
+            const uint64_t hmask = __ballot(head);
+            const int first = hmask ? __ffsll((unsigned long long)hmask) - 1 : kWave;
+            // live rows: in the chunk once started; before `started`, from the
+            // first head on; past the chunk, up to the first head
+            bool live;
+            if (ext)
+                live = valid && lane < first;
+            else if (!started)
+                live = valid && lane >= first;
+            else
+                live = valid;
+            const bool lhead = head && live && !ext;
+            const uint64_t lmask = __ballot(lhead);
+            RunAcc r = none;
+            if (live) {
+                const bool h = key & 1u;
+                const int32_t v = int32_t(val_c);
+                r = RunAcc{h ? v : INT32_MAX, h ? v : INT32_MIN, h ? 1 : 0};
+            }
+            // segmented inclusive scan: f says a head lies in (lane - d, lane]
+            bool f = lhead;
+    #pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const RunAcc o{__shfl_up(r.mn, d, kWave), __shfl_up(r.mx, d, kWave),
+                               __shfl_up(r.has, d, kWave)};
+                const bool of = __shfl_up(f ? 1 : 0, d, kWave) != 0;
+                if (lane >= d) {
+                    if (!f) r = run_join(r, o);
+                    f = f || of;
+                }
+            }
+            // the open run goes on into this tile when its first row is live and
+            // not a head; otherwise it ended with the previous tile
+            const bool cont = open && (__shfl(live && !lhead ? 1 : 0, 0, kWave) != 0);
+            if (open && !cont && lane == 0) emit_run(out, carry_id, max_rows, carry_key, keys_n, carry);
+            // lanes before the tile's first live head continue the open run
+            if (!f && cont) r = run_join(r, carry);
+            // a live lane ends its run when the next row is a live head or is
+            // not live (end of the input, or past the chunk's run)
+            const bool next_live = __shfl_down(live ? 1 : 0, 1, kWave) != 0;
+            const bool next_head = __shfl_down(lhead ? 1 : 0, 1, kWave) != 0;
+            int top = live ? lane : -1;
+    #pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int o = __shfl_xor(top, d, kWave);
+                top = o > top ? o : top;
+            }
+            const int64_t my_id = f ? rid + __popcll(lmask & ((2ull << lane) - 1ull)) - 1 : carry_id;
+            const uint64_t my_key = f ? key : carry_key;
+            const bool ends_here = live && lane < kWave - 1 && (next_head || !next_live);
+            // the run at the top live lane stays open only when every lane is
+            // live and the input goes on
+            const bool tile_open = top == kWave - 1 && t0 + kWave < n;
+            if (live && (f || cont) && (ends_here || (lane == top && !tile_open)))
+                emit_run(out, my_id, max_rows, my_key, keys_n, r);
+            if (tile_open) {
+                const bool top_f = __shfl(f ? 1 : 0, kWave - 1, kWave) != 0;
+                carry = RunAcc{__shfl(r.mn, kWave - 1, kWave), __shfl(r.mx, kWave - 1, kWave),
+                               __shfl(r.has, kWave - 1, kWave)};
+                if (top_f) {
+                    carry_key = uint64_t(__shfl((long long)key, kWave - 1, kWave));
+                    carry_id = int64_t(__shfl((long long)my_id, kWave - 1, kWave));
+                }
+                open = top_f || cont;
+            } else {
+                open = false;
+            }
+            if (lmask) started = true;
+            rid += __popcll(lmask);
+        }
     }
 }
 
@@ -412,7 +575,8 @@ struct Plan {
     int64_t slots = 0, words = 0, bound = 0, cap = 0, nb = 0, chunk = 0, rows_cap = 0;
     size_t o_mm = 0, o_used = 0, o_bits = 0, o_cnt = 0, o_scr = 0, o_first = 0, o_tmp = 0,
            o_k = 0, o_v = 0, o_sort = 0;
-    size_t o_t = 0, o_ck = 0, o_cs = 0, o_occ = 0, o_oscr = 0;
+    size_t o_ck = 0, o_cs = 0, o_occ = 0, o_oscr = 0, o_cur = 0, o_pk = 0;
+    int kbits = 0;
     size_t end = 0;
 };
 
@@ -424,6 +588,7 @@ Plan make_plan(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
     p.slots = int64_t(a->groups) * a->keys;
     const int64_t dense_max = a->dense_max_slots > 0 ? a->dense_max_slots : kDenseDefault;
     p.dense = p.slots <= dense_max;
+    p.o_pk = ws.plan(size_t(std::max<int64_t>(1, a->items)) * sizeof(int4));
     if (p.dense) {
         p.words = (p.slots + 31) / 32;
         p.o_mm = ws.plan(size_t(p.slots) * 8);
@@ -441,18 +606,19 @@ Plan make_plan(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
             p.o_sort = ws.plan(radix_sort_scratch_bytes(ctx, p.rows_cap));
         }
     } else {
-        // exact bound on the distinct keys: expanded rows (and slots)
-        p.bound = std::max<int64_t>(1, std::min(expanded, p.slots));
-        const int lg = std::max(10, log2_ceil(p.bound * 2));
-        p.cap = int64_t(1) << std::min(lg, 40);
-        p.o_t = ws.plan(size_t(p.cap) * sizeof(HSlot));
+        // sorted: one (key, value) pair per expanded row
+        p.bound = std::max<int64_t>(1, expanded);
+        p.kbits = log2_ceil(p.slots) + 1;
         p.o_ck = ws.plan(size_t(p.bound) * 8);
         p.o_cs = ws.plan(size_t(p.bound) * 4);
-        p.nb = int64_t(grid_of(ctx, p.cap));
-        p.chunk = (p.cap + p.nb - 1) / p.nb;
-        p.o_occ = ws.plan(size_t(p.nb) * 8);
-        p.o_oscr = ws.plan(scan_scratch_bytes(p.nb));
         p.o_sort = ws.plan(radix_sort_scratch_bytes(ctx, p.bound));
+        p.nb = int64_t(grid_of(ctx, p.bound));
+        // one chunk of whole 64-row tiles per wave
+        const int64_t waves = p.nb * (kT / kWave);
+        p.chunk = ((p.bound + waves - 1) / waves + kWave - 1) / kWave * kWave;
+        p.o_occ = ws.plan(size_t(waves + 1) * 8);
+        p.o_oscr = ws.plan(scan_scratch_bytes(waves + 1));
+        p.o_cur = ws.plan(8);
     }
     p.end = ws.off;
     return p;
@@ -486,6 +652,7 @@ Args args_of(const bolap_minmax_args* a) {
     k.has = a->item_has;
     k.groups = a->groups;
     k.keys = a->keys;
+    k.pk = nullptr;
     return k;
 }
 
@@ -520,19 +687,24 @@ int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
                size_t base, bolap_minmax_row* rows_out, int64_t max_rows, int64_t* n_out,
                bolap_minmax_stats* stats) {
     BOLAP_TRY(check_args(a, max_rows));
-    const Args k = args_of(a);
+    Args k = args_of(a);
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
     int64_t h[mN];
     const Plan p = make_plan(ctx, a, expanded, first_seen, base);
-    BOLAP_ARG_CHECK(p.dense || p.cap <= (int64_t(1) << 32),
-                    "bolap_group_minmax: %lld expanded rows need a table beyond 2^32 slots",
-                    (long long)expanded);
+    BOLAP_ARG_CHECK(p.dense || p.slots <= (int64_t(1) << 62),
+                    "bolap_group_minmax: %lld slots exceed 2^62", (long long)p.slots);
     // the caller's arrays below `base` stay where they are: a workspace that
     // must grow keeps them, but only a caller that reserved p.end already may
     // pass pointers into the workspace
     BOLAP_TRY(ws_reserve(ctx, p.end, base));
     WsArena ws;
     ws.base = static_cast<char*>(ctx->ws);
+    if (a->items > 0) {
+        k.pk = ws.at<int4>(p.o_pk);
+        hipLaunchKernelGGL(mm_pack_k, dim3(grid_of(ctx, a->items)), dim3(kT), 0, ctx->stream,
+                           args_of(a), ws.at<int4>(p.o_pk));
+        BOLAP_TRY(launch_check("mm_pack_k"));
+    }
     int64_t groups_n = 0;
     if (p.dense) {
         int2* mm = ws.at<int2>(p.o_mm);
@@ -589,21 +761,30 @@ int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
             }
         }
     } else {
-        HSlot* t = ws.at<HSlot>(p.o_t);
+        uint64_t* ck = ws.at<uint64_t>(p.o_ck);
+        uint32_t* cv = ws.at<uint32_t>(p.o_cs);
+        int64_t* wcnt = ws.at<int64_t>(p.o_occ);
+        const int64_t waves = p.nb * (kT / kWave);
         {
-            TraceScope ts(ctx, "minmax_hash");
-            hipLaunchKernelGGL(mm_init_hash_k, dim3(grid_of(ctx, p.cap)), dim3(kT), 0,
-                               ctx->stream, t, p.cap);
-            hipLaunchKernelGGL(mm_hash_k, dim3(grid_of(ctx, a->pairs)), dim3(kT), 0, ctx->stream,
-                               k, t, uint64_t(p.cap - 1), st);
-            int64_t* occ = ws.at<int64_t>(p.o_occ);
-            hipLaunchKernelGGL(mm_occ_count_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, t,
-                               p.cap, p.chunk, occ, st);
-            BOLAP_TRY(exclusive_scan_i64(ctx, occ, p.nb, ws.at<void>(p.o_oscr)));
-            hipLaunchKernelGGL(mm_compact_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, t,
-                               p.cap, p.chunk, occ, ws.at<uint64_t>(p.o_ck),
-                               ws.at<uint32_t>(p.o_cs));
-            BOLAP_TRY(launch_check("mm_hash_k"));
+            TraceScope ts(ctx, "minmax_sorted");
+            auto* cur = ws.at<unsigned long long>(p.o_cur);
+            hipLaunchKernelGGL(fill_u64_k, dim3(1), dim3(kT), 0, ctx->stream, cur, int64_t(1),
+                               0ull);
+            const uint64_t bad_key = p.kbits >= 64 ? ~0ull : ((uint64_t(1) << p.kbits) - 1);
+            const int64_t eb = std::max<int64_t>(
+                1, std::min<int64_t>((a->pairs + kET - 1) / kET, int64_t(ctx->num_cus) * 2));
+            hipLaunchKernelGGL(mm_expand_k, dim3(unsigned(eb)), dim3(kET), 0, ctx->stream,
+                               k, ck, cv, cur, expanded, bad_key, st);
+            BOLAP_TRY(launch_check("mm_expand_k"));
+            int passes = 0;
+            BOLAP_TRY(radix_sort_pairs(ctx, ck, cv, expanded, p.kbits, ws.at<void>(p.o_sort),
+                                       &passes));
+            hipLaunchKernelGGL(mm_heads_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, ck,
+                               expanded, p.chunk, waves, wcnt);
+            BOLAP_TRY(launch_check("mm_heads_k"));
+            BOLAP_TRY(exclusive_scan_i64(ctx, wcnt, waves + 1, ws.at<void>(p.o_oscr)));
+            hipLaunchKernelGGL(mm_total_k, dim3(1), dim3(1), 0, ctx->stream, wcnt, waves, st);
+            BOLAP_TRY(launch_check("mm_total_k"));
         }
         BOLAP_TRY(read_st(ctx, h));
         BOLAP_ARG_CHECK(h[mErrors] == 0,
@@ -615,14 +796,9 @@ int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
                             "bolap_group_minmax: %lld groups exceed max_rows %lld",
                             (long long)groups_n, (long long)max_rows);
             TraceScope ts(ctx, "minmax_emit");
-            int passes = 0;
-            BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(p.o_ck), ws.at<uint32_t>(p.o_cs),
-                                       groups_n, std::max(1, log2_ceil(p.slots)),
-                                       ws.at<void>(p.o_sort), &passes));
-            hipLaunchKernelGGL(mm_hash_emit_k, dim3(grid_of(ctx, groups_n)), dim3(kT), 0,
-                               ctx->stream, ws.at<uint64_t>(p.o_ck), ws.at<uint32_t>(p.o_cs),
-                               groups_n, a->keys, t, rows_out);
-            BOLAP_TRY(launch_check("mm_hash_emit_k"));
+            hipLaunchKernelGGL(mm_reduce_k, dim3(unsigned(p.nb)), dim3(kT), 0, ctx->stream, ck, cv,
+                               expanded, p.chunk, wcnt, a->keys, groups_n, rows_out);
+            BOLAP_TRY(launch_check("mm_reduce_k"));
         }
     }
     if (rows_out) BOLAP_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -24,26 +24,73 @@ namespace {
 
 constexpr int kRT = 256;        // threads per workgroup
 constexpr int kBins = 256;      // 8-bit digits
-constexpr int kTileK = kRT;     // keys per tile in the scatter kernel
+constexpr int kKPT = 16;        // keys per thread per scatter tile
+constexpr int kTileK = kRT * kKPT;  // keys per scatter tile (4096)
+constexpr int kRW = kRT / kWave;    // waves per workgroup
 
 BOLAP_DEV uint32_t digit_of(uint64_t key, uint64_t kmask, int shift) {
     return uint32_t(((key & kmask) >> shift) & 0xFF);
 }
 
-__global__ __launch_bounds__(kRT) void rs_hist_k(const uint64_t* __restrict__ keys, int64_t n,
+// Lanes of the wave holding the same digit as this lane (among `live`),
+// from eight ballots over the digit bits.
+BOLAP_DEV uint64_t digit_peers(uint32_t d, uint64_t live) {
+    uint64_t peers = live;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+// Digit counts of one scatter workgroup's contiguous chunk, counted by a
+// workgroup of kHT threads (more loads in flight than the scatter's grid
+// gives).  Lanes with the same digit are matched by ballots and their
+// lowest lane adds the group's size, so a skewed digit costs one LDS atomic
+// per wave instead of 64 conflicting ones.
+constexpr int kHT = 1024;
+
+__global__ __launch_bounds__(kHT) void rs_hist_k(const uint64_t* __restrict__ keys, int64_t n,
                                                  int64_t chunk, uint64_t kmask, int shift,
                                                  int nblocks, int64_t* __restrict__ hist) {
     __shared__ uint32_t h[kBins];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < kBins) h[threadIdx.x] = 0;
     __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
     const int64_t b0 = int64_t(blockIdx.x) * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kRT)
-        atomicAdd(&h[digit_of(keys[i], kmask, shift)], 1u);
+    constexpr int kU = 4;
+    // the trip count is uniform across the workgroup (ballots below)
+    for (int64_t base = b0; base < b1; base += int64_t(kHT) * kU) {
+        uint64_t k[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int64_t i = base + int64_t(u) * kHT + threadIdx.x;
+            k[u] = i < b1 ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int64_t i = base + int64_t(u) * kHT + threadIdx.x;
+            const uint32_t d = digit_of(k[u], kmask, shift);
+            const uint64_t peers = digit_peers(d, __ballot(i < b1));
+            if (i < b1 && (peers & lt_mask) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+        }
+    }
     __syncthreads();
-    hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < kBins) hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one workgroup's chunk in tiles of kTileK keys.  Wave w
+// of a tile owns keys [w * kKPT * 64, (w + 1) * kKPT * 64) of it, loaded as
+// kKPT coalesced rows of 64, so (wave, row, lane) order is input order.
+// Ranking: per row, ballot-matched digit peers; the wave's running count per
+// digit (LDS) plus the lane's rank among its peers.  The tile is then
+// reordered by digit in LDS (digit start in the tile + the counts of earlier
+// waves + the rank) and written out with consecutive threads on consecutive
+// slots, so each digit's run of the tile leaves as contiguous stores.
 __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__ kin,
                                                     const uint32_t* __restrict__ vin, int64_t n,
                                                     int64_t chunk, uint64_t kmask, int shift,
@@ -51,44 +98,94 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
                                                     const int64_t* __restrict__ offsets,
                                                     uint64_t* __restrict__ kout,
                                                     uint32_t* __restrict__ vout) {
-    constexpr int kW = kRT / kWave;
-    __shared__ int64_t run[kBins];
-    __shared__ uint32_t wcnt[kW][kBins];
-    run[threadIdx.x] = offsets[int64_t(threadIdx.x) * nblocks + blockIdx.x];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    __shared__ uint64_t s_key[kTileK];
+    __shared__ uint32_t s_val[kTileK];
+    __shared__ uint32_t s_wcnt[kRW][kBins];  // per-wave counts, then exclusive wave offsets
+    __shared__ uint32_t s_start[kBins];      // digit start inside the tile
+    __shared__ uint32_t s_wsum[kRW];
+    __shared__ int64_t s_run[kBins];         // global cursor of each digit
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    s_run[t] = offsets[int64_t(t) * nblocks + blockIdx.x];
     const int64_t b0 = int64_t(blockIdx.x) * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
     for (int64_t base = b0; base < b1; base += kTileK) {
-        for (int w = 0; w < kW; ++w) wcnt[w][threadIdx.x] = 0;
-        __syncthreads();
-        const int64_t i = base + threadIdx.x;
-        const bool valid = i < b1;
-        const uint64_t key = valid ? kin[i] : 0;
-        const uint32_t val = valid ? vin[i] : 0;
-        const uint32_t d = digit_of(key, kmask, shift);
-        uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+        for (int w = 0; w < kRW; ++w) s_wcnt[w][t] = 0;
+        const int64_t wbase = base + int64_t(wave) * (kKPT * kWave);
+        uint64_t key[kKPT];
+        uint32_t val[kKPT];
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+            const int64_t i = wbase + j * kWave + lane;
+            key[j] = i < b1 ? kin[i] : 0;
+            val[j] = i < b1 ? vin[i] : 0;
         }
-        const uint32_t wrank = uint32_t(__popcll(peers & lt_mask));
-        if (valid && wrank == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
-        __syncthreads();
-        if (valid) {
-            int64_t pos = run[d] + wrank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-            kout[pos] = key;
-            vout[pos] = val;
+        __syncthreads();  // counts cleared; the previous tile's reads are done
+        uint32_t rank[kKPT];
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+            const bool valid = wbase + j * kWave + lane < b1;
+            const uint32_t d = digit_of(key[j], kmask, shift);
+            const uint64_t peers = digit_peers(d, __ballot(valid));
+            const uint32_t before = s_wcnt[wave][d];
+            rank[j] = before + uint32_t(__popcll(peers & lt_mask));
+            // one lane per digit group bumps the wave's count; the reads
+            // above precede this store in the wave's instruction order
+            if (valid && (peers & lt_mask) == 0) s_wcnt[wave][d] = before + uint32_t(__popcll(peers));
         }
         __syncthreads();
+        // thread t owns digit t: exclusive offsets over the waves, the tile
+        // total, and the digits' exclusive scan across the workgroup
         uint32_t tot = 0;
-        for (int w = 0; w < kW; ++w) tot += wcnt[w][threadIdx.x];
-        run[threadIdx.x] += tot;
+#pragma unroll
+        for (int w = 0; w < kRW; ++w) {
+            const uint32_t c = s_wcnt[w][t];
+            s_wcnt[w][t] = tot;
+            tot += c;
+        }
+        uint32_t inc = tot;
+#pragma unroll
+        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+            const uint32_t y = __shfl_up(inc, dlt, kWave);
+            if (lane >= dlt) inc += y;
+        }
+        if (lane == kWave - 1) s_wsum[wave] = inc;
         __syncthreads();
+        uint32_t wpre = 0;
+#pragma unroll
+        for (int w = 0; w < kRW; ++w) wpre += w < wave ? s_wsum[w] : 0u;
+        const uint32_t my_start = wpre + inc - tot;
+        s_start[t] = my_start;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+            const bool valid = wbase + j * kWave + lane < b1;
+            if (valid) {
+                const uint32_t d = digit_of(key[j], kmask, shift);
+                const uint32_t at = s_start[d] + s_wcnt[wave][d] + rank[j];
+                s_key[at] = key[j];
+                s_val[at] = val[j];
+            }
+        }
+        __syncthreads();
+        const int64_t left = b1 - base;
+        const int cnt = left < kTileK ? int(left) : kTileK;
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+            const int at = j * kRT + t;
+            if (at < cnt) {
+                const uint64_t k = s_key[at];
+                const uint32_t d = digit_of(k, kmask, shift);
+                const int64_t pos = s_run[d] + (at - int64_t(s_start[d]));
+                kout[pos] = k;
+                vout[pos] = s_val[at];
+            }
+        }
+        __syncthreads();  // s_run, s_start and the staged tile are read above
+        s_run[t] += tot;
     }
 }
 
@@ -259,9 +356,12 @@ int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
     if (passes_out) *passes_out = 0;
     if (n < 2 || key_bits <= 0) return BOLAP_OK;
     if (key_bits > 64) key_bits = 64;
-    // Contiguous chunks per workgroup keep the scatter stable.
-    int64_t nblocks = (n + 8 * kTileK - 1) / (8 * kTileK);
-    if (nblocks > 1024) nblocks = 1024;
+    // Contiguous chunks of whole tiles per workgroup keep the scatter
+    // stable; two workgroups per CU fit the scatter's 55 KB of LDS, so the
+    // grid is at most that many (one round of workgroups).
+    int64_t nblocks = (n + kTileK - 1) / kTileK;
+    const int64_t max_blocks = std::min<int64_t>(1024, int64_t(ctx->num_cus) * 2);
+    if (nblocks > max_blocks) nblocks = max_blocks;
     if (nblocks < 1) nblocks = 1;
     int64_t chunk = (n + nblocks - 1) / nblocks;
     chunk = (chunk + kTileK - 1) / kTileK * kTileK;
@@ -282,7 +382,7 @@ int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
     uint32_t* dv = tv;
     int passes = 0;
     for (int shift = 0; shift < key_bits; shift += 8, ++passes) {
-        hipLaunchKernelGGL(rs_hist_k, dim3(unsigned(nblocks)), dim3(kRT), 0, ctx->stream, sk, n,
+        hipLaunchKernelGGL(rs_hist_k, dim3(unsigned(nblocks)), dim3(kHT), 0, ctx->stream, sk, n,
                            chunk, kmask, shift, int(nblocks), hist);
         BOLAP_TRY(launch_check("rs_hist_k"));
         BOLAP_TRY(exclusive_scan_i64(ctx, hist, int64_t(kBins) * nblocks, w.at<void>(o_s)));

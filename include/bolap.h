@@ -565,13 +565,14 @@ int bolap_ilike_mask(const char* chars, const int64_t* offsets, int64_t n,
 /* range_value, :1081-1090); every (group, info) pair is a group holding      */
 /* {has, min, max} of the movie years, created as {0, 0, 0} by a movie        */
 /* without a year (:880-900).  Placement as the reference: a dense matrix     */
-/* when groups x keys <= dense_max_slots (0: 5,000,000, :843-856), else a     */
-/* hashed table.  Rows come out in (group, key) order (the reference's order  */
-/* is first-seen or hash order, :1094-1120).  All pointers are device         */
+/* when groups x keys <= dense_max_slots (0: 5,000,000, :843-856), else (the  */
+/* reference's unordered_map) the expanded rows sorted by (group, key) and    */
+/* reduced run by run.  Rows come out in (group, key) order (the reference's  */
+/* order is first-seen or hash order, :1094-1120).  All pointers are device   */
 /* pointers; a group, item, range or key outside its bounds is an error.     */
 /* ------------------------------------------------------------------------ */
 #define BOLAP_MINMAX_DENSE 0
-#define BOLAP_MINMAX_HASH 1
+#define BOLAP_MINMAX_HASH 1   /* beyond the dense limit (sort + run reduce) */
 
 typedef struct {
     int64_t pairs;               /* join rows */
