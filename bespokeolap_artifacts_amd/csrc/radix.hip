@@ -23,21 +23,26 @@ namespace bolap {
 namespace {
 
 constexpr int kRT = 256;        // threads per workgroup
-constexpr int kBins = 256;      // 8-bit digits
+constexpr int kBins = 256;      // the widest digit (8 bits)
 constexpr int kKPT = 16;        // keys per thread per scatter tile
 constexpr int kTileK = kRT * kKPT;  // keys per scatter tile (4096)
 constexpr int kRW = kRT / kWave;    // waves per workgroup
 
+// Digits are B bits wide (5 <= B <= 8): the host spreads the key bits over
+// the fewest 8-bit passes and then narrows the digit as far as that number
+// of passes allows, so a tile's digit runs are as long as possible.
+template <int B>
 BOLAP_DEV uint32_t digit_of(uint64_t key, uint64_t kmask, int shift) {
-    return uint32_t(((key & kmask) >> shift) & 0xFF);
+    return uint32_t(((key & kmask) >> shift) & ((1u << B) - 1u));
 }
 
 // Lanes of the wave holding the same digit as this lane (among `live`),
-// from eight ballots over the digit bits.
+// from B ballots over the digit bits.
+template <int B>
 BOLAP_DEV uint64_t digit_peers(uint32_t d, uint64_t live) {
     uint64_t peers = live;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < B; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -52,11 +57,13 @@ BOLAP_DEV uint64_t digit_peers(uint32_t d, uint64_t live) {
 // per wave instead of 64 conflicting ones.
 constexpr int kHT = 1024;
 
+template <int B>
 __global__ __launch_bounds__(kHT) void rs_hist_k(const uint64_t* __restrict__ keys, int64_t n,
                                                  int64_t chunk, uint64_t kmask, int shift,
                                                  int nblocks, int64_t* __restrict__ hist) {
-    __shared__ uint32_t h[kBins];
-    if (threadIdx.x < kBins) h[threadIdx.x] = 0;
+    constexpr int NB = 1 << B;
+    __shared__ uint32_t h[NB];
+    if (threadIdx.x < NB) h[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
@@ -74,13 +81,13 @@ __global__ __launch_bounds__(kHT) void rs_hist_k(const uint64_t* __restrict__ ke
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int64_t i = base + int64_t(u) * kHT + threadIdx.x;
-            const uint32_t d = digit_of(k[u], kmask, shift);
-            const uint64_t peers = digit_peers(d, __ballot(i < b1));
+            const uint32_t d = digit_of<B>(k[u], kmask, shift);
+            const uint64_t peers = digit_peers<B>(d, __ballot(i < b1));
             if (i < b1 && (peers & lt_mask) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
         }
     }
     __syncthreads();
-    if (threadIdx.x < kBins) hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < NB) hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
 // Stable scatter of one workgroup's chunk in tiles of kTileK keys.  Wave w
@@ -91,6 +98,7 @@ __global__ __launch_bounds__(kHT) void rs_hist_k(const uint64_t* __restrict__ ke
 // reordered by digit in LDS (digit start in the tile + the counts of earlier
 // waves + the rank) and written out with consecutive threads on consecutive
 // slots, so each digit's run of the tile leaves as contiguous stores.
+template <int B>
 __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__ kin,
                                                     const uint32_t* __restrict__ vin, int64_t n,
                                                     int64_t chunk, uint64_t kmask, int shift,
@@ -98,22 +106,26 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
                                                     const int64_t* __restrict__ offsets,
                                                     uint64_t* __restrict__ kout,
                                                     uint32_t* __restrict__ vout) {
+    constexpr int NB = 1 << B;
     __shared__ uint64_t s_key[kTileK];
     __shared__ uint32_t s_val[kTileK];
-    __shared__ uint32_t s_wcnt[kRW][kBins];  // per-wave counts, then exclusive wave offsets
-    __shared__ uint32_t s_start[kBins];      // digit start inside the tile
+    __shared__ uint32_t s_wcnt[kRW][NB];  // per-wave counts, then exclusive wave offsets
+    __shared__ uint32_t s_start[NB];      // digit start inside the tile
     __shared__ uint32_t s_wsum[kRW];
-    __shared__ int64_t s_run[kBins];         // global cursor of each digit
+    __shared__ int64_t s_run[NB];         // global cursor of each digit
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wave = t / kWave;
+    const bool owner = t < NB;            // thread t owns digit t
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
-    s_run[t] = offsets[int64_t(t) * nblocks + blockIdx.x];
+    if (owner) s_run[t] = offsets[int64_t(t) * nblocks + blockIdx.x];
     const int64_t b0 = int64_t(blockIdx.x) * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
     for (int64_t base = b0; base < b1; base += kTileK) {
+        if (owner) {
 #pragma unroll
-        for (int w = 0; w < kRW; ++w) s_wcnt[w][t] = 0;
+            for (int w = 0; w < kRW; ++w) s_wcnt[w][t] = 0;
+        }
         const int64_t wbase = base + int64_t(wave) * (kKPT * kWave);
         uint64_t key[kKPT];
         uint32_t val[kKPT];
@@ -128,8 +140,8 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
 #pragma unroll
         for (int j = 0; j < kKPT; ++j) {
             const bool valid = wbase + j * kWave + lane < b1;
-            const uint32_t d = digit_of(key[j], kmask, shift);
-            const uint64_t peers = digit_peers(d, __ballot(valid));
+            const uint32_t d = digit_of<B>(key[j], kmask, shift);
+            const uint64_t peers = digit_peers<B>(d, __ballot(valid));
             const uint32_t before = s_wcnt[wave][d];
             rank[j] = before + uint32_t(__popcll(peers & lt_mask));
             // one lane per digit group bumps the wave's count; the reads
@@ -140,11 +152,13 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
         // thread t owns digit t: exclusive offsets over the waves, the tile
         // total, and the digits' exclusive scan across the workgroup
         uint32_t tot = 0;
+        if (owner) {
 #pragma unroll
-        for (int w = 0; w < kRW; ++w) {
-            const uint32_t c = s_wcnt[w][t];
-            s_wcnt[w][t] = tot;
-            tot += c;
+            for (int w = 0; w < kRW; ++w) {
+                const uint32_t c = s_wcnt[w][t];
+                s_wcnt[w][t] = tot;
+                tot += c;
+            }
         }
         uint32_t inc = tot;
 #pragma unroll
@@ -158,13 +172,13 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
 #pragma unroll
         for (int w = 0; w < kRW; ++w) wpre += w < wave ? s_wsum[w] : 0u;
         const uint32_t my_start = wpre + inc - tot;
-        s_start[t] = my_start;
+        if (owner) s_start[t] = my_start;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kKPT; ++j) {
             const bool valid = wbase + j * kWave + lane < b1;
             if (valid) {
-                const uint32_t d = digit_of(key[j], kmask, shift);
+                const uint32_t d = digit_of<B>(key[j], kmask, shift);
                 const uint32_t at = s_start[d] + s_wcnt[wave][d] + rank[j];
                 s_key[at] = key[j];
                 s_val[at] = val[j];
@@ -178,14 +192,14 @@ __global__ __launch_bounds__(kRT) void rs_scatter_k(const uint64_t* __restrict__
             const int at = j * kRT + t;
             if (at < cnt) {
                 const uint64_t k = s_key[at];
-                const uint32_t d = digit_of(k, kmask, shift);
+                const uint32_t d = digit_of<B>(k, kmask, shift);
                 const int64_t pos = s_run[d] + (at - int64_t(s_start[d]));
                 kout[pos] = k;
                 vout[pos] = s_val[at];
             }
         }
         __syncthreads();  // s_run, s_start and the staged tile are read above
-        s_run[t] += tot;
+        if (owner) s_run[t] += tot;
     }
 }
 
@@ -338,6 +352,21 @@ __global__ void zero_u64_k(unsigned long long* p, int n) {
         p[i] = 0;
 }
 
+// One LSD pass on B-bit digits: per-workgroup digit counts, their
+// exclusive scan in (digit, workgroup) order, then the stable scatter.
+template <int B>
+int radix_pass(bolap_ctx* ctx, const uint64_t* sk, const uint32_t* sv, int64_t n, int64_t chunk,
+               uint64_t kmask, int shift, int64_t nblocks, int64_t* hist, uint64_t* dk,
+               uint32_t* dv, void* scan_tmp) {
+    hipLaunchKernelGGL(rs_hist_k<B>, dim3(unsigned(nblocks)), dim3(kHT), 0, ctx->stream, sk, n,
+                       chunk, kmask, shift, int(nblocks), hist);
+    BOLAP_TRY(launch_check("rs_hist_k"));
+    BOLAP_TRY(exclusive_scan_i64(ctx, hist, int64_t(1 << B) * nblocks, scan_tmp));
+    hipLaunchKernelGGL(rs_scatter_k<B>, dim3(unsigned(nblocks)), dim3(kRT), 0, ctx->stream, sk, sv,
+                       n, chunk, kmask, shift, int(nblocks), hist, dk, dv);
+    return launch_check("rs_scatter_k");
+}
+
 }  // namespace
 
 size_t radix_sort_scratch_bytes(bolap_ctx* ctx, int64_t n) {
@@ -380,15 +409,18 @@ int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
     uint32_t* sv = vals;
     uint64_t* dk = tk;
     uint32_t* dv = tv;
+    // the fewest 8-bit passes, then the narrowest digit that still covers
+    // the key in that many passes (5 bits at least)
+    const int npass = (key_bits + 7) / 8;
+    const int dbits = std::max(5, (key_bits + npass - 1) / npass);
     int passes = 0;
-    for (int shift = 0; shift < key_bits; shift += 8, ++passes) {
-        hipLaunchKernelGGL(rs_hist_k, dim3(unsigned(nblocks)), dim3(kHT), 0, ctx->stream, sk, n,
-                           chunk, kmask, shift, int(nblocks), hist);
-        BOLAP_TRY(launch_check("rs_hist_k"));
-        BOLAP_TRY(exclusive_scan_i64(ctx, hist, int64_t(kBins) * nblocks, w.at<void>(o_s)));
-        hipLaunchKernelGGL(rs_scatter_k, dim3(unsigned(nblocks)), dim3(kRT), 0, ctx->stream, sk,
-                           sv, n, chunk, kmask, shift, int(nblocks), hist, dk, dv);
-        BOLAP_TRY(launch_check("rs_scatter_k"));
+    for (int shift = 0; shift < key_bits; shift += dbits, ++passes) {
+        switch (dbits) {
+            case 5: BOLAP_TRY(radix_pass<5>(ctx, sk, sv, n, chunk, kmask, shift, nblocks, hist, dk, dv, w.at<void>(o_s))); break;
+            case 6: BOLAP_TRY(radix_pass<6>(ctx, sk, sv, n, chunk, kmask, shift, nblocks, hist, dk, dv, w.at<void>(o_s))); break;
+            case 7: BOLAP_TRY(radix_pass<7>(ctx, sk, sv, n, chunk, kmask, shift, nblocks, hist, dk, dv, w.at<void>(o_s))); break;
+            default: BOLAP_TRY(radix_pass<8>(ctx, sk, sv, n, chunk, kmask, shift, nblocks, hist, dk, dv, w.at<void>(o_s)));
+        }
         std::swap(sk, dk);
         std::swap(sv, dv);
     }

@@ -14,7 +14,10 @@ def dev(a, dtype):
 
 @pytest.mark.parametrize("n,bits,spread", [(0, 64, 64), (1, 64, 64), (2, 64, 1), (1000, 20, 20),
                                            (65_537, 64, 64), (1_000_003, 33, 33),
-                                           (3_000_000, 64, 12)])
+                                           (3_000_000, 64, 12),
+                                           # 5- and 6-bit digits (10 bits in two passes,
+                                           # 17 bits in three)
+                                           (200_000, 10, 10), (300_001, 17, 17)])
 def test_radix_sort_matches_stable_sort(engine, n, bits, spread):
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 1 << 62, n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
