@@ -119,3 +119,21 @@ def test_q1_repeated_calls_reset_state(engine):
     first = as_dicts(engine.q1_scan_agg(cols, LO, 2435, 3, 2))
     for _ in range(5):
         assert as_dicts(engine.q1_scan_agg(cols, LO, 2435, 3, 2)) == first
+
+
+@pytest.mark.parametrize("finish", [1, 2])
+@pytest.mark.parametrize("rows,per_cu", [(6_000_000, 1), (1_234_567, 3), (70_001, 1)])
+def test_finish_variants_match_oracle(engine, finish, rows, per_cu):
+    """Both finishes of the scan (the last workgroup merges every partial
+    row; a separate merge kernel), at several grid sizes, give the oracle's
+    groups on two calls in a row (the finish counter returns to zero)."""
+    cols = engine.gen_lineitem(SEED + rows, rows)
+    want = oracle.q1_wide(host(cols), LO, 2435, 3, 2)
+    engine.tune_set("q1.finish", finish)
+    engine.tune_set("q1.blocks_per_cu", per_cu)
+    try:
+        for _ in range(2):
+            assert as_dicts(engine.q1_scan_agg(cols, LO, 2435, 3, 2)) == want
+    finally:
+        engine.tune_set("q1.finish", 0)
+        engine.tune_set("q1.blocks_per_cu", 0)
