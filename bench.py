@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--knobs-out", default=None,
+                    help="write the autotuners' picks as a knob table (bolap_tune_save)")
     ap.add_argument("--c4-train", type=int, default=20_000, help="C4 training plans")
     ap.add_argument("--c4-eval", type=int, default=500, help="C4 held-out plans")
     ap.add_argument("--c5-queries", type=int, default=512, help="C5 batch size")
@@ -193,6 +195,15 @@ def bench_c2(eng, torch, args):
     a_hint = eng.join_sum_args(d["key"], preds, f["fk"], f["price"], f["disc"],
                                capacity_hint=first["build_rows"])
     ms_hint, tr_hint, r_hint = timed(a_hint)
+    # The autotuner's pick for these arguments (probe loop, filter bits,
+    # workgroups per CU), timed the same way; the knobs return to the
+    # defaults afterwards so that the other workloads run untuned.
+    tune_ms = eng.join_autotune(a_exact, reps=3)
+    tuned = {k: eng.tune_get(k) for k in
+             ("join.probe_variant", "join.lds_bloom_bits", "join.blocks_per_cu")}
+    ms_tuned, _, r_tuned = timed(a_exact)
+    for k in tuned:
+        eng.tune_set(k, 0)
     probe_ms = stage_ms(tr_exact, "join_probe")
     matches = first["matches"]
     alg_bytes = fact_rows * 4 + dim_rows * 6 + matches * 16
@@ -211,7 +222,9 @@ def bench_c2(eng, torch, args):
            "build_rows": first["build_rows"], "matches": matches,
            "placement": "LDS" if first["placement"] == N.PLACE_LDS else "HBM",
            "kernel": "probe_sum_k", "kernel_bytes_per_launch": probe_bytes,
-           "consistent": r_exact == first and r_hint["revenue"] == first["revenue"]}
+           "autotuned": {"knobs": tuned, "ms_per_step": ms_tuned, "tuner_median_ms": tune_ms},
+           "consistent": (r_exact == first and r_hint["revenue"] == first["revenue"]
+                          and r_tuned == first)}
     if not args.no_cpu:
         import oracle
         hd = {k: v.cpu().numpy() for k, v in d.items()}
@@ -783,9 +796,20 @@ def bench_sort(eng, torch, args):
         eng.gather_rows(movie, order)
         eng.gather_rows(role, order)
     ms = time_steps(torch, step, args.steps, args.warmup)
+    # The autotuner's digit width for keys of this shape (26 bits: movie id
+    # offsets and role ids packed), then the leg again with it.
+    keys = (torch.randint(0, 1 << 26, (n,), device=dev, dtype=torch.int64, generator=g))
+    vals = torch.arange(n, device=dev, dtype=torch.int32)
+    tune_ms = eng.sort_autotune(keys, vals, 26, reps=3)
+    width = eng.tune_get("radix.digit_bits")
+    del keys, vals
+    ms_tuned = time_steps(torch, step, args.steps, args.warmup)
+    eng.tune_set("radix.digit_bits", 0)
     res = {"workload": "CEB build: cast_info 36,000,000 rows sorted by (movie_id, role_id) "
                        "+ gather of both columns", "ms_per_step": ms,
-           "rows_per_s": n / (ms * 1e-3)}
+           "rows_per_s": n / (ms * 1e-3),
+           "autotuned": {"knobs": {"radix.digit_bits": width}, "ms_per_step": ms_tuned,
+                         "tuner_median_sort_ms": tune_ms}}
     if not args.no_cpu:
         import oracle
         m = 2_000_000
@@ -1036,6 +1060,20 @@ def main():
         br = r.get("by_ranges")
         if isinstance(br, dict) and br.get("kernel") in MEASURED_L2_HIT:
             br["l2_hit_rate"] = MEASURED_L2_HIT[br["kernel"]]
+    # The autotuners' picks, as one knob table (bolap_tune_load reads it back).
+    tuned_knobs = {}
+    for r in results.values():
+        at = r.get("autotuned")
+        if isinstance(at, dict):
+            if isinstance(at.get("knobs"), dict):
+                tuned_knobs.update(at["knobs"])
+            for k in ("blocks_per_cu", "qchunk"):
+                if k in at:
+                    tuned_knobs["ceb." + k] = at[k]
+    if args.knobs_out and tuned_knobs:
+        for k, v in tuned_knobs.items():
+            eng.tune_set(k, int(v))
+        eng.tune_save(args.knobs_out)
     # The headline is the first workload with a bandwidth figure (C2 by default).
     head = next((results[w] for w in results if "GBps" in results[w]),
                 next(iter(results.values())))
@@ -1050,6 +1088,7 @@ def main():
         "cpu_baseline": head.get("cpu_baseline"),
         "parity": head.get("parity"),
         "workloads": results,
+        "tuned_knobs": tuned_knobs,
         "device": info,
     }
     print(json.dumps(line), flush=True)

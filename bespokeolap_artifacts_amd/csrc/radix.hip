@@ -410,9 +410,12 @@ int radix_sort_pairs(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
     uint64_t* dk = tk;
     uint32_t* dv = tv;
     // the fewest 8-bit passes, then the narrowest digit that still covers
-    // the key in that many passes (5 bits at least)
+    // the key in that many passes (5 bits at least); knob radix.digit_bits
+    // (5..8, set by the autotuner) fixes the width instead
     const int npass = (key_bits + 7) / 8;
-    const int dbits = std::max(5, (key_bits + npass - 1) / npass);
+    int dbits = std::max(5, (key_bits + npass - 1) / npass);
+    const int64_t forced = ctx->knob("radix.digit_bits", 0);
+    if (forced >= 5 && forced <= 8) dbits = int(forced);
     int passes = 0;
     for (int shift = 0; shift < key_bits; shift += dbits, ++passes) {
         switch (dbits) {

@@ -95,6 +95,27 @@ class Engine:
         N.check(self.lib.bolap_tune_get(self.ctx, knob.encode(), C.byref(v)))
         return v.value
 
+    def tune_save(self, path: str) -> None:
+        N.check(self.lib.bolap_tune_save(self.ctx, str(path).encode()))
+
+    def tune_load(self, path: str) -> None:
+        N.check(self.lib.bolap_tune_load(self.ctx, str(path).encode()))
+
+    def join_autotune(self, a: N.JoinSumArgs, reps: int = 3) -> float:
+        """Keeps the fastest probe configuration for these arguments; returns
+        its median call time in ms."""
+        ms = C.c_double(0.0)
+        N.check(self.lib.bolap_join_autotune(self.ctx, C.byref(a), int(reps), C.byref(ms)))
+        return ms.value
+
+    def sort_autotune(self, keys, vals, key_bits: int, reps: int = 3) -> float:
+        """Keeps the fastest radix digit width for sorting these pairs (left
+        unchanged); returns its median sort time in ms."""
+        ms = C.c_double(0.0)
+        N.check(self.lib.bolap_sort_autotune(self.ctx, ptr(keys), ptr(vals), keys.numel(),
+                                             int(key_bits), int(reps), C.byref(ms)))
+        return ms.value
+
     def set_trace(self, on: bool) -> None:
         N.check(self.lib.bolap_ctx_set_trace(self.ctx, int(bool(on))))
 

@@ -18,6 +18,7 @@ BOLAP_ERR_HIP = 2
 BOLAP_ERR_UNSUPPORTED = 3
 BOLAP_ERR_CAPACITY = 4
 BOLAP_ERR_NO_DEVICE = 5
+BOLAP_ERR_INTERNAL = 6
 
 U8, I16, I32, I64 = 0, 1, 2, 3
 PLACE_AUTO, PLACE_LDS, PLACE_HBM = 0, 1, 2
@@ -469,6 +470,10 @@ PROTOTYPES = {
     "bolap_ctx_read_trace": (C.c_int, [vp, P(TraceEntry), i32, P(i32)]),
     "bolap_tune_set": (C.c_int, [vp, C.c_char_p, i64]),
     "bolap_tune_get": (C.c_int, [vp, C.c_char_p, P(i64)]),
+    "bolap_tune_save": (C.c_int, [vp, C.c_char_p]),
+    "bolap_tune_load": (C.c_int, [vp, C.c_char_p]),
+    "bolap_join_autotune": (C.c_int, [vp, P(JoinSumArgs), i32, P(C.c_double)]),
+    "bolap_sort_autotune": (C.c_int, [vp, vp, vp, i64, i32, i32, P(C.c_double)]),
     "bolap_q1_scan_agg": (C.c_int, [vp, P(Q1Args), P(Q1Group)]),
     "bolap_q1_scan_agg_async": (C.c_int, [vp, P(Q1Args), vp]),
     "bolap_join_sum": (C.c_int, [vp, P(JoinSumArgs), P(JoinSumResult)]),

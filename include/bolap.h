@@ -39,7 +39,8 @@ typedef enum {
     BOLAP_ERR_HIP = 2,          /* HIP runtime error */
     BOLAP_ERR_UNSUPPORTED = 3,  /* valid request this build does not cover */
     BOLAP_ERR_CAPACITY = 4,     /* a fixed maximum was exceeded */
-    BOLAP_ERR_NO_DEVICE = 5
+    BOLAP_ERR_NO_DEVICE = 5,
+    BOLAP_ERR_INTERNAL = 6      /* a self-check failed (e.g. a tuned variant disagreed) */
 } bolap_status;
 
 typedef enum {
@@ -97,6 +98,11 @@ int bolap_ctx_set_trace(bolap_ctx* ctx, int enabled);
    its constants, e.g. the 16-bit radix of bespoke_tpch/query_q10.cpp:139). */
 int bolap_tune_set(bolap_ctx* ctx, const char* knob, int64_t value);
 int bolap_tune_get(bolap_ctx* ctx, const char* knob, int64_t* value);
+/* The knob table persisted as text, one "name value" line per knob ('#'
+   starts a comment); load adds to (and overrides) the context's knobs and
+   leaves them unchanged when a line is malformed. */
+int bolap_tune_save(bolap_ctx* ctx, const char* path);
+int bolap_tune_load(bolap_ctx* ctx, const char* path);
 int bolap_ctx_read_trace(bolap_ctx* ctx, bolap_trace_entry* out, int32_t max_entries,
                          int32_t* n_out);
 
@@ -232,6 +238,16 @@ typedef struct {
 
 int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* args,
                    bolap_join_sum_result* out);
+
+/* Autotuner of the join: times bolap_join_sum on these arguments under each
+   probe loop (knob join.probe_variant 1..5), LDS filter bits per build key
+   (join.lds_bloom_bits 6..8) and probe workgroups per CU (join.blocks_per_cu
+   1..2), median of reps calls each, and keeps the fastest in the context's
+   knobs; a candidate whose result differs from the first call's is an error
+   (BOLAP_ERR_INTERNAL).  No reference counterpart (its probe loop and unroll
+   are fixed, bespoke_tpch/query_q19.cpp:286-345). */
+int bolap_join_autotune(bolap_ctx* ctx, const bolap_join_sum_args* args, int32_t reps,
+                        double* best_ms);
 
 /* Allocation-free variant of bolap_join_sum.  No counterpart in the
    reference: it is for callers that cannot allocate device or pinned memory
@@ -722,6 +738,14 @@ int bolap_star_sample_counts(bolap_ctx* ctx, const bolap_star_args* plans, int32
 /* ------------------------------------------------------------------------ */
 int bolap_radix_sort_u64(bolap_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n,
                          int32_t key_bits);
+/* Autotuner of the radix digit: sorts copies of the pairs (the arrays are not
+   modified) with 8-, 7-, 6- and 5-bit digits (knob radix.digit_bits; unset,
+   the sort takes the fewest 8-bit passes and the narrowest digit that still
+   fits them), median of reps sorts each, checks every width's output against
+   the 8-bit one and keeps the fastest width.  No reference counterpart (the
+   reference's digit is fixed at 16 bits, bespoke_tpch/query_q10.cpp:139). */
+int bolap_sort_autotune(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64_t n,
+                        int32_t key_bits, int32_t reps, double* best_ms);
 /* k smallest keys (ascending) with their payloads, by LDS-histogram radix
    select; keys_out/vals_out are device arrays of k entries. */
 int bolap_topk_u64(bolap_ctx* ctx, const uint64_t* keys, const uint32_t* vals, int64_t n,
