@@ -997,12 +997,12 @@ def bench_c3(eng, torch, args):
 MEASURED_TRAFFIC = {
     "probe_sum_k": 248.3e6 + 0.0e6,
     "q1_scan_agg_kernel": 228.3e6 + 0.1e6,
-    "fact_probe_k": 612.6e6 + 91.5e6,
+    "fact_probe_k": 622.4e6 + 59.0e6,
     "q6_flat_k": 540.6e6 + 0.0e6,
-    "q18_sum_sorted_k": 378.9e6 + 107.7e6,
-    "range_groups_k": 749.9e6 + 23.3e6,
-    "q14_flat_k": 327.5e6 + 0.1e6,
-    "q17_probe_k<1>": 248.1e6 + 4.3e6,
+    "q18_sum_sorted_k": 382.8e6 + 112.9e6,
+    "range_groups_k": 752.1e6 + 23.3e6,
+    "q14_flat_k": 327.4e6 + 0.1e6,
+    "q17_probe_k<1>": 248.0e6 + 4.3e6,
 }
 
 

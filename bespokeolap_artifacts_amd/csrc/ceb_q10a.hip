@@ -26,9 +26,10 @@
 //   result      the join rows expand through their movie's info range into
 //               group_minmax.hip's aggregation: a dense matrix when
 //               groups x infos <= 5,000,000 slots (:843-856), rows in the
-//               order the reference first reaches them (:866-872); else a
-//               hashed table, rows in (name group, info) order (the
-//               reference's is its unordered_map's iteration order).
+//               order the reference first reaches them (:866-872); else
+//               the expanded rows sorted and reduced run by run, rows in
+//               (name group, info) order (the reference's unordered_map
+//               leaves its order open).
 #include <algorithm>
 #include <cstring>
 

@@ -85,6 +85,10 @@ struct Geo {
     int32_t bb_log2;        // fixed only
     int64_t lds_keys_max;   // device sizing: largest build that goes to LDS
     int32_t lds_bits_log2;  // device sizing: log2 filter bits per key in LDS
+    // device sizing, placement AUTO: first attempt that plans no HBM table
+    // (no table clearing and build launches); a build larger than
+    // lds_keys_max is flagged 4 and the call reruns with the HBM table
+    int32_t lds_only;
 };
 
 // Table geometry for n build keys.  flag: 0 ok, 1 estimate too small
@@ -119,7 +123,7 @@ __host__ __device__ inline Shape shape_of(int64_t n, const Geo& g) {
     if (g.placement != BOLAP_PLACE_HBM && n <= g.lds_keys_max)
         return Shape{true, clog2(m * 2 < 64 ? 64 : m * 2),
                      clampi(clog2(m << g.lds_bits_log2), 10, 18), 0};
-    if (g.placement == BOLAP_PLACE_LDS) return Shape{true, 0, 0, 4};
+    if (g.placement == BOLAP_PLACE_LDS || g.lds_only) return Shape{true, 0, 0, 4};
     // Mid-size builds keep the filter in LDS (32 KB) and only the table in
     // HBM: the filter test per fact key stays an LDS read, and only the
     // collected candidates probe the HBM table.
@@ -1137,7 +1141,8 @@ JoinLayout plan_join(bolap_ctx* ctx, const bolap_join_sum_args* a, const PredSet
     p.g = g;
     p.lds = lds;
     // HBM table: the largest the build side can need under `g`.
-    p.may_hbm = g.fixed ? g.placement == BOLAP_PLACE_HBM : g.placement != BOLAP_PLACE_LDS;
+    p.may_hbm = g.fixed ? g.placement == BOLAP_PLACE_HBM
+                        : g.placement != BOLAP_PLACE_LDS && !g.lds_only;
     int cap_log2 = 0, bb_log2 = 0;
     if (p.may_hbm) {
         const Shape worst = g.fixed ? Shape{false, g.cap_log2, g.bb_log2, 0}
@@ -1362,9 +1367,21 @@ extern "C" int bolap_join_sum(bolap_ctx* ctx, const bolap_join_sum_args* a,
             done = true;
     }
     if (!done) {
-        // Exact sizing on the device from the compacted build size.
+        // Exact sizing on the device from the compacted build size.  With
+        // placement AUTO the first attempt assumes the build goes to LDS
+        // (the common case: no HBM table to clear and build); a larger build
+        // is flagged and the call reruns with the HBM table planned.
         size_t lds = 0;
         g = exact_geo(ctx, a->placement, lds);
+        if (a->placement == BOLAP_PLACE_AUTO) {
+            Geo spec = g;
+            spec.lds_only = 1;
+            BOLAP_TRY(join_attempt(ctx, a, ps, spec, lds, r));
+            if (!(shape_of(r[kBuild], spec).flag & 4)) {
+                fill_result(ctx, a, r, spec, fallbacks, out);
+                return BOLAP_OK;
+            }
+        }
         BOLAP_TRY(join_attempt(ctx, a, ps, g, lds, r));
         if (shape_of(r[kBuild], g).flag & 4) {
             set_error("bolap_join_sum: %lld keys do not fit the LDS budget", (long long)r[kBuild]);
