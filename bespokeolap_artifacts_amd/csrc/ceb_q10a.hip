@@ -529,12 +529,16 @@ extern "C" int bolap_ceb_q10a(bolap_ctx* ctx, const bolap_ceb_q10a_args* a,
         BOLAP_TRY(launch_check("q10_items_k"));
     }
     bolap_minmax_args m{};
+    m.items = t1;
+    // room for the packed items at prep_end first (minmax_expanded writes
+    // them there), so that the addresses taken below stay valid
+    BOLAP_TRY(ws_reserve(ctx, minmax_pack_end(&m, prep_end), prep_end));
+    ws.base = static_cast<char*>(ctx->ws);
     m.pairs = pairs;
     m.pair_group = ws.at<int32_t>(o_pg);
     m.pair_item = ws.at<int32_t>(o_pm);
-    m.items = t1;
-    m.item_offset = reinterpret_cast<const int32_t*>(first);
-    m.item_count = reinterpret_cast<const int32_t*>(count);
+    m.item_offset = ws.at<int32_t>(o_first);
+    m.item_count = ws.at<int32_t>(o_cnt);
     m.item_has = ws.at<uint8_t>(o_has);
     m.item_value = ws.at<int32_t>(o_year);
     m.values = values;
@@ -543,15 +547,16 @@ extern "C" int bolap_ceb_q10a(bolap_ctx* ctx, const bolap_ceb_q10a_args* a,
     m.keys = a->n_infos;
     m.dense_max_slots = a->dense_max_slots;
     int64_t expanded = 0;
-    BOLAP_TRY(minmax_expanded(ctx, &m, &expanded));
+    BOLAP_TRY(minmax_expanded(ctx, &m, prep_end, &expanded));
     const int64_t slots = groups * int64_t(a->n_infos);
     const int64_t row_cap = std::max<int64_t>(1, std::min(slots, expanded));
     const size_t mm_end = minmax_ws_end(ctx, &m, expanded, true, prep_end);
     const size_t o_rows = WsArena::align(mm_end);
     const size_t all_end = o_rows + size_t(row_cap) * sizeof(bolap_minmax_row);
     // one reservation for the aggregation and the rows (keeping the join
-    // rows and items above), then the pointers into the moved workspace
-    BOLAP_TRY(ws_reserve(ctx, all_end, prep_end));
+    // rows and items above and the packed items), then the pointers into
+    // the moved workspace
+    BOLAP_TRY(ws_reserve(ctx, all_end, minmax_pack_end(&m, prep_end)));
     ws.base = static_cast<char*>(ctx->ws);
     m.pair_group = ws.at<int32_t>(o_pg);
     m.pair_item = ws.at<int32_t>(o_pm);

@@ -132,10 +132,15 @@ int noalloc_setup(int device);
 
 // Grouped MIN/MAX in steps (group_minmax.hip), for pipelines that build the
 // join rows in the workspace themselves: the expanded row count (one
-// synchronisation), the workspace end of a call whose own arrays start at
-// `base`, and the call (rows in (group, key) order, or with first_seen in
-// the order of the first join row and value that reach each group).
-int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t* expanded);
+// synchronisation; it also packs the item columns into the workspace at
+// `base`, up to minmax_pack_end, which the caller keeps for the call), the
+// workspace end of a call whose own arrays start at `base`, and the call
+// (rows in (group, key) order, or with first_seen in the order of the first
+// join row and value that reach each group).  A caller whose arrays live in
+// the workspace reserves it up to minmax_pack_end before taking their
+// addresses: minmax_expanded may not move it then.
+size_t minmax_pack_end(const bolap_minmax_args* a, size_t base);
+int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, size_t base, int64_t* expanded);
 size_t minmax_ws_end(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded,
                      bool first_seen, size_t base);
 int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, bool first_seen,

@@ -658,9 +658,25 @@ Args args_of(const bolap_minmax_args* a) {
 
 }  // namespace
 
-int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t* expanded) {
+size_t minmax_pack_end(const bolap_minmax_args* a, size_t base) {
+    return WsArena::align(base) +
+           WsArena::align(size_t(std::max<int64_t>(1, a->items)) * sizeof(int4));
+}
+
+int minmax_expanded(bolap_ctx* ctx, const bolap_minmax_args* a, size_t base, int64_t* expanded) {
     BOLAP_TRY(check_args(a, 0));
-    const Args k = args_of(a);
+    Args k = args_of(a);
+    // the item columns packed at align(base), where the call's plan puts
+    // them (its first region): the count and the call read one 16-byte row
+    // per join row
+    BOLAP_TRY(ws_reserve(ctx, minmax_pack_end(a, base), base));
+    if (a->items > 0) {
+        int4* pk = reinterpret_cast<int4*>(static_cast<char*>(ctx->ws) + WsArena::align(base));
+        hipLaunchKernelGGL(mm_pack_k, dim3(grid_of(ctx, a->items)), dim3(kT), 0, ctx->stream, k,
+                           pk);
+        BOLAP_TRY(launch_check("mm_pack_k"));
+        k.pk = pk;
+    }
     int64_t* st = static_cast<int64_t*>(ctx->result_dev);
     int64_t h[mN];
     hipLaunchKernelGGL(mm_zero_k, dim3(1), dim3(kT), 0, ctx->stream, st);
@@ -693,18 +709,13 @@ int minmax_run(bolap_ctx* ctx, const bolap_minmax_args* a, int64_t expanded, boo
     const Plan p = make_plan(ctx, a, expanded, first_seen, base);
     BOLAP_ARG_CHECK(p.dense || p.slots <= (int64_t(1) << 62),
                     "bolap_group_minmax: %lld slots exceed 2^62", (long long)p.slots);
-    // the caller's arrays below `base` stay where they are: a workspace that
-    // must grow keeps them, but only a caller that reserved p.end already may
-    // pass pointers into the workspace
-    BOLAP_TRY(ws_reserve(ctx, p.end, base));
+    // the caller's arrays below `base` and the items minmax_expanded packed
+    // stay where they are: a workspace that must grow keeps them, but only a
+    // caller that reserved p.end already may pass pointers into the workspace
+    BOLAP_TRY(ws_reserve(ctx, p.end, minmax_pack_end(a, base)));
     WsArena ws;
     ws.base = static_cast<char*>(ctx->ws);
-    if (a->items > 0) {
-        k.pk = ws.at<int4>(p.o_pk);
-        hipLaunchKernelGGL(mm_pack_k, dim3(grid_of(ctx, a->items)), dim3(kT), 0, ctx->stream,
-                           args_of(a), ws.at<int4>(p.o_pk));
-        BOLAP_TRY(launch_check("mm_pack_k"));
-    }
+    if (a->items > 0) k.pk = ws.at<int4>(p.o_pk);  // packed by minmax_expanded
     int64_t groups_n = 0;
     if (p.dense) {
         int2* mm = ws.at<int2>(p.o_mm);
@@ -820,6 +831,6 @@ extern "C" int bolap_group_minmax(bolap_ctx* ctx, const bolap_minmax_args* a,
     BOLAP_ARG_CHECK(ctx && a, "bolap_group_minmax: null argument");
     bolap::TraceScope ts_all(ctx, "minmax");
     int64_t expanded = 0;
-    BOLAP_TRY(bolap::minmax_expanded(ctx, a, &expanded));
+    BOLAP_TRY(bolap::minmax_expanded(ctx, a, 0, &expanded));
     return bolap::minmax_run(ctx, a, expanded, false, 0, rows_out, max_rows, n_out, stats);
 }

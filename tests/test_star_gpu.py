@@ -187,3 +187,25 @@ def test_full_size_hinted_equals_exact(engine):
         assert st["matches"] == st2["matches"]
         if st["groups"]:
             assert bool((k1[1:] > k1[:-1]).all().item())
+
+
+def test_full_size_matches_oracle(engine):
+    """60M fact rows against the oracle itself (not only hinted against
+    exact): the groups, their revenues and the plan counts of three plans,
+    exact-sized and estimate-sized (the first plan's group hint is 1, far
+    too small for its groups)."""
+    cfg = dict(seed=43, fact_rows=60_000_000, dim_rows=[4_000_000, 1_000_000, 100_000],
+               fk_s=[1.0, 0.5, 1.5], n_groups=1_000_000, group_s=1.0)
+    s = W.StarSchema(engine, **cfg)
+    t = O.gen_star(**cfg)
+    for i, plan in enumerate(plans(3, seed=11)):
+        a = W.bind(s, plan)
+        st, keys, rev = engine.star_groupby(a, want_rows=True)
+        ost, okeys, orev = O.star_groupby(t, plan)
+        np.testing.assert_array_equal(host(keys), okeys)
+        np.testing.assert_array_equal(host(rev), orev)
+        small_hint = 1 if i == 0 else max(1, 2 * st["groups"])
+        b = W.bind(s, plan, [max(1, 2 * p) for p in st["dim_pass"]], small_hint)
+        st2, k2, r2 = engine.star_groupby(b, want_rows=True)
+        np.testing.assert_array_equal(host(k2), okeys)
+        np.testing.assert_array_equal(host(r2), orev)
