@@ -117,6 +117,12 @@ void set_error(const char* fmt, ...);
 // `keep` bytes are copied into the new buffer (regions planned earlier in
 // the same call stay valid as offsets; raw pointers must be re-derived).
 int ws_reserve(bolap_ctx* ctx, size_t bytes, size_t keep = 0);
+// Resident workgroups per CU of `kern` at `threads` threads and `lds` bytes
+// of dynamic LDS on the context's device, with the kernel's dynamic-LDS
+// limit raised to `lds` first when needed.  Both runtime queries are made
+// once per (device, kernel, threads, LDS) and cached for the process: they
+// cost host time on every call of a synchronous pipeline otherwise.
+int kernel_occupancy(bolap_ctx* ctx, const void* kern, int threads, size_t lds, int* per_cu);
 // Ensures ctx->pinned holds at least `bytes`.
 int pinned_reserve(bolap_ctx* ctx, size_t bytes);
 // Waits until the pinned int64 at `slot` holds `seq` (written last, with a

@@ -1015,13 +1015,9 @@ struct Plan {
 template <typename TP, typename TD, int kProbeU, int kPipe, bool kLast, bool kDefer>
 int launch_probe_d(bolap_ctx* ctx, const bolap_join_sum_args* a, const Plan& p) {
     auto kern = probe_sum_k<TP, TD, kProbeU, kPipe, kLast, kDefer>;
-    if (p.lds > 0) {
-        BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          int(p.lds)));
-    }
     int per_cu = 0;
-    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, p.lds));
+    BOLAP_TRY(kernel_occupancy(ctx, reinterpret_cast<const void*>(kern), kThreads, p.lds,
+                               &per_cu));
     int64_t want = ctx->knob("join.blocks_per_cu", 2);
     if (per_cu > 0 && want > per_cu) want = per_cu;
     int64_t blocks = int64_t(ctx->num_cus) * want;
@@ -1513,8 +1509,9 @@ extern "C" int bolap_q17(bolap_ctx* ctx, const bolap_q17_args* a, bolap_q17_resu
     for (int pass = 1; pass <= 2; ++pass) {
         TraceScope ts(ctx, pass == 1 ? "q17_sum_scan" : "q17_filter_scan");
         auto kern = pass == 1 ? q17_probe_k<1> : q17_probe_k<2>;
-        BOLAP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        int per_cu = 0;  // raises the kernel's dynamic-LDS limit (once)
+        BOLAP_TRY(kernel_occupancy(ctx, reinterpret_cast<const void*>(kern), kThreads, lds,
+                                   &per_cu));
         hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kThreads), lds, ctx->stream,
                            a->l_partkey, a->l_quantity, a->l_price, a->li_rows, g, t, kl,
                            agg_sum, agg_cnt, mlist, mcnt, list_cap,

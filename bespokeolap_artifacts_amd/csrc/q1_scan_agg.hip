@@ -343,7 +343,8 @@ int launch_q1_f(bolap_ctx* ctx, const Q1K& k, bolap_q1_group* out_dev) {
     const int64_t tile = int64_t(kQ1Threads) * U;
     int64_t blocks = (n_units + tile - 1) / tile;
     int per_cu = 0;
-    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kQ1Threads, 0));
+    BOLAP_TRY(kernel_occupancy(ctx, reinterpret_cast<const void*>(kern), kQ1Threads, 0,
+                               &per_cu));
     // One workgroup per CU measured fastest (fewer partial rows to merge).
     int64_t want = ctx->knob("q1.blocks_per_cu", 1);
     if (per_cu > 0 && want > per_cu) want = per_cu;

@@ -3,6 +3,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <tuple>
 
 #include "common.hpp"
 
@@ -31,6 +33,31 @@ int ws_reserve(bolap_ctx* ctx, size_t bytes, size_t keep) {
     if (ctx->ws) BOLAP_HIP_TRY(hipFree(ctx->ws));
     ctx->ws = fresh;
     ctx->ws_size = want;
+    return BOLAP_OK;
+}
+
+int kernel_occupancy(bolap_ctx* ctx, const void* kern, int threads, size_t lds, int* per_cu) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int, size_t>, int> occ;
+    static std::map<std::pair<int, const void*>, size_t> lds_limit;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_tuple(ctx->device, kern, threads, lds);
+    const auto it = occ.find(key);
+    if (it != occ.end()) {
+        *per_cu = it->second;
+        return BOLAP_OK;
+    }
+    // the limit only grows, so a launch with less LDS stays valid
+    size_t& cur = lds_limit[std::make_pair(ctx->device, kern)];
+    if (lds > cur) {
+        BOLAP_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          int(lds)));
+        cur = lds;
+    }
+    int n = 0;
+    BOLAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, lds));
+    occ.emplace(key, n);
+    *per_cu = n;
     return BOLAP_OK;
 }
 

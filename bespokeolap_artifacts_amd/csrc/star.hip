@@ -461,11 +461,14 @@ void launch_probe(bolap_ctx* ctx, const bolap_star_args* a, const StarK& k, cons
                   unsigned grid, size_t lds, int gl_log2, int64_t* st) {
     const int64_t gspan = int64_t(a->group_max) - a->group_min + 1;
     const bool p64 = a->price_dtype == BOLAP_I64, d64 = a->disc_dtype == BOLAP_I64;
-    // More than 64 KB of dynamic LDS needs the per-kernel limit raised.
+    // More than 64 KB of dynamic LDS needs the per-kernel limit raised
+    // (once per kernel and size: kernel_occupancy caches it).
 #define BOLAP_STAR_LAUNCH(TP, TD)                                                            \
     do {                                                                                       \
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(star_probe_k<TP, TD, kMode>),  \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));       \
+        int per_cu_ = 0;                                                                       \
+        (void)kernel_occupancy(ctx,                                                            \
+                               reinterpret_cast<const void*>(star_probe_k<TP, TD, kMode>),     \
+                               kT, lds, &per_cu_);                                             \
         hipLaunchKernelGGL((star_probe_k<TP, TD, kMode>), dim3(grid), dim3(kT), lds,          \
                            ctx->stream, k, g, a->group_key, a->group_min, gspan,              \
                            static_cast<const TP*>(a->fact_price),                             \
