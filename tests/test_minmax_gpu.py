@@ -92,6 +92,12 @@ def test_extreme_values_and_one_hot_group(engine):
     d["pair_group"][:2] = 0
     for dm in (0, 1):
         np.testing.assert_array_equal(gpu(engine, d, dm)[1], want(d, dm))
+    # the same rows beyond the dense limit (2 slots > 1): one run of every
+    # expanded row through the sort, spanning every wave's chunk
+    d2 = dict(d, keys=2)
+    st, got = gpu(engine, d2, 1)
+    assert st["placement"] == N.MINMAX_HASH and len(got) == 1
+    np.testing.assert_array_equal(got, want(d2, 1))
 
 
 @pytest.mark.parametrize("field", ["range_value", "pair_group", "pair_item"])
