@@ -239,10 +239,13 @@ __global__ __launch_bounds__(kT) void q10_write_k(int64_t size, const uint8_t* _
                                                   const unsigned long long* __restrict__ rev,
                                                   const uint32_t* __restrict__ tiles,
                                                   uint64_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals) {
+                                                  uint32_t* __restrict__ vals,
+                                                  unsigned long long* __restrict__ maxrev) {
     __shared__ uint32_t s_w[kT / kWave];
+    __shared__ unsigned long long s_m[kT / kWave];
     const int64_t t0 = int64_t(blockIdx.x) * kTile;
     uint32_t base = tiles[blockIdx.x];
+    unsigned long long mx = 0;  // the largest kept revenue (sort width)
     for (int j = 0; j < 16; ++j) {
         const int64_t k = t0 + int64_t(j) * kT + threadIdx.x;
         const bool keep = k < size && q10_keep(k, has, map, nation, n_names);
@@ -257,15 +260,35 @@ __global__ __launch_bounds__(kT) void q10_write_k(int64_t size, const uint8_t* _
         for (int w = 0; w < kT / kWave; ++w) total += s_w[w];
         if (keep) {
             const uint32_t at = base + before + before_lane;
-            keys[at] = ~uint64_t(rev[k]);
+            const unsigned long long r = rev[k];
+            keys[at] = ~uint64_t(r);
             vals[at] = uint32_t(k);
+            mx = r > mx ? r : mx;
         }
         base += total;
         __syncthreads();
     }
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long o = __shfl_xor(mx, d, kWave);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_m[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / kWave; ++w) mx = s_m[w] > mx ? s_m[w] : mx;
+        if (mx) atomicMax(maxrev, mx);
+    }
 }
 
 // Sorted pairs -> output rows {custkey, cust_row, revenue}.
+// The group count and the largest revenue side by side, for one read-back.
+__global__ void q10_stage_k(const uint32_t* __restrict__ total,
+                            const unsigned long long* __restrict__ maxrev,
+                            unsigned long long* __restrict__ stage) {
+    stage[0] = *total;
+    stage[1] = *maxrev;
+}
+
 __global__ __launch_bounds__(kT) void q10_rows_k(int64_t n, const uint64_t* __restrict__ keys,
                                                  const uint32_t* __restrict__ vals,
                                                  const int32_t* __restrict__ map,
@@ -621,11 +644,13 @@ extern "C" int bolap_q10(bolap_ctx* ctx, const bolap_q10_args* a, bolap_q10_row*
     const size_t o_rev = ws.plan(size_t(size) * 8);
     const size_t o_has = ws.plan(size_t(size));
     const size_t o_map = ws.plan(size_t(size) * 4);
+    const size_t o_max = ws.plan(8);  // cleared with rev, has and map
     const size_t o_tiles = ws.plan(size_t(tiles + 1) * 4);
     const size_t o_keys = ws.plan(size_t(size) * 8);
     const size_t o_vals = ws.plan(size_t(size) * 4);
     const size_t o_out = ws.plan(size_t(size) * sizeof(bolap_q10_row));
     const size_t o_sort = ws.plan(radix_sort_scratch_bytes(ctx, size));
+    const size_t o_stage = ws.plan(16);
     BOLAP_TRY(ws_reserve(ctx, ws.off));
     ws.base = static_cast<char*>(ctx->ws);
     auto* rev = ws.at<unsigned long long>(o_rev);
@@ -654,11 +679,21 @@ extern "C" int bolap_q10(bolap_ctx* ctx, const bolap_q10_args* a, bolap_q10_row*
         hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(1024), 0, ctx->stream, tl, tiles);
         hipLaunchKernelGGL(q10_write_k, dim3(unsigned(tiles)), dim3(kT), 0, ctx->stream, size, has,
                            map, a->c_nationkey, a->n_nation_names, rev, tl,
-                           ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals));
+                           ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals),
+                           ws.at<unsigned long long>(o_max));
+        hipLaunchKernelGGL(q10_stage_k, dim3(1), dim3(1), 0, ctx->stream, tl + tiles,
+                           ws.at<unsigned long long>(o_max),
+                           ws.at<unsigned long long>(o_stage));
         BOLAP_TRY(launch_check("q10_write_k"));
     }
-    uint32_t n = 0;
-    BOLAP_TRY(fetch(ctx, &n, tl + tiles, sizeof(n)));
+    unsigned long long staged[2] = {0, 0};
+    BOLAP_TRY(fetch(ctx, staged, ws.at<unsigned long long>(o_stage), sizeof(staged)));
+    const uint32_t n = uint32_t(staged[0]);
+    // Every kept revenue is at most the largest, mx < 2^b, so the low b bits
+    // of ~revenue are 2^b - 1 - revenue: sorting on them alone gives the
+    // full 64-bit order (ties stay in customer-key order, the passes being
+    // stable) in ceil(b / 8) passes instead of eight.
+    const int sort_bits = staged[1] ? 64 - __builtin_clzll(staged[1]) : 1;
     *n_rows = n;
     trace_count(ctx, "q10_agg_groups_created", n);
     if (n > max_rows) {
@@ -669,8 +704,8 @@ extern "C" int bolap_q10(bolap_ctx* ctx, const bolap_q10_args* a, bolap_q10_row*
     {
         TraceScope ts(ctx, "q10_sort");
         int passes = 0;
-        BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals), n, 64,
-                                   ws.at<void>(o_sort), &passes));
+        BOLAP_TRY(radix_sort_pairs(ctx, ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals), n,
+                                   sort_bits, ws.at<void>(o_sort), &passes));
         hipLaunchKernelGGL(q10_rows_k, dim3(grid_of(ctx, n)), dim3(kT), 0, ctx->stream, int64_t(n),
                            ws.at<uint64_t>(o_keys), ws.at<uint32_t>(o_vals), map,
                            ws.at<bolap_q10_row>(o_out));
